@@ -1,0 +1,44 @@
+"""Shared loaders for the reference-held v1.dat fixture (two cones of
+verification1/cone_m1_p5.msh, 28 triangles + 553 quads each, solved by the
+reference; expected outputs in verification1/example1.log)."""
+import os
+
+import numpy as np
+
+import oracle as orc
+from datfile import parse_example1_log, read_dat
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+_cache = {}
+
+
+def v1_raw():
+    if "raw" not in _cache:
+        _cache["raw"] = read_dat(os.path.join(GOLDEN, "v1.dat.gz"))
+    return _cache["raw"]
+
+
+def v1_log():
+    return parse_example1_log(os.path.join(GOLDEN, "v1_example1.log"))
+
+
+def v1_oracle():
+    """oracle objects loaded with the stored geometry and solved state"""
+    objs = []
+    for o in v1_raw():
+        ob = orc.Obj.from_arrays(o["rn"], o["ed"], o["md"], o["sd"], o["gd"], o["n"])
+        ob.setup(o["beams"])
+        ob.override(ren=o["ren"], wen=o["wen"], kn=o["kn"])
+        for d in range(o["MN"] + 1):
+            for key in ("U", "dU", "E", "dE", "H", "dH"):
+                ob.state(d, key, o["dom"][d][key])
+        objs.append(ob)
+    return objs
+
+
+def active_slots(ed, sid):
+    """mask over padded (element, slot) pairs that carry a node"""
+    tri = ed[np.abs(sid[1:]), 3] == 0
+    act = np.ones((len(sid) - 1, 4), bool)
+    act[tri, 3] = False
+    return act.reshape(-1)

@@ -1,0 +1,120 @@
+"""Oracle vs the reference's own golden outputs (v1.dat + example1.log)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from fixtures import active_slots, v1_log, v1_oracle, v1_raw
+
+REF_PT = [0.3, 0.0, 1.1]  # example1.c:14-16
+
+
+@pytest.fixture(scope="module")
+def v1():
+    return v1_oracle()
+
+
+@pytest.mark.parametrize("typ", [0, 1])
+def test_total_field_matches_example1(v1, typ):
+    ref = v1_log()["field"][typ]
+    out, oid = orc.mo_field(v1, [REF_PT], "EH", "t", typ)
+    want = np.array([ref[k] for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")])
+    assert oid[0] == -1
+    scale = np.abs(want).max()
+    # every component to 1e-12 of the field magnitude; the two large ones
+    # of each vector also to 1e-12 relative
+    assert np.abs(out[0] - want).max() < 1e-12 * scale
+    big = np.abs(want) > 1e-3 * scale
+    assert (np.abs(out[0] - want)[big] / np.abs(want)[big]).max() < 1e-12
+
+
+@pytest.mark.parametrize("typ", [0, 1])
+@pytest.mark.parametrize("oid", [0, 1])
+def test_force_torque_match_example1(v1, typ, oid):
+    ref = v1_log()["force"][typ][oid]
+    ret, F, N = orc.mo_force(v1, oid, typ)
+    assert ret == 0
+    assert np.abs(F - ref["F"]).max() < 1e-12 * np.abs(ref["F"]).max()
+    assert np.abs(N - ref["N"]).max() < 1e-12 * np.abs(ref["N"]).max()
+
+
+@pytest.mark.parametrize("typ", [0, 1])
+@pytest.mark.parametrize("oid", [0, 1])
+def test_absorbed_power_matches_example1(v1, typ, oid):
+    ref = v1_log()["absorb"][typ][oid]
+    ret, P = orc.mo_absorb(v1, oid, typ)
+    assert ret == 0
+    assert abs(P - ref) < 1e-12 * ref
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("did", [0, 1])
+def test_assembled_G_H_satisfy_stored_boundary_state(v1, did):
+    """The reference solved G [dE|dH] = H [E|H] with its own G, H
+    (mo_solve.c:663-789); our assembly must reproduce that identity."""
+    o = v1_raw()[0]
+    M = v1[0].cmatrix(did, 2, want_d=False)
+    dom = o["dom"][did]
+    act = active_slots(o["ed"], dom["sid"])
+    G, H = M["G"][np.ix_(act, act)], M["H"][np.ix_(act, act)]
+    for a, da in (("E", "dE"), ("H", "dH")):
+        X, dX = dom[a][1:].reshape(-1, 3)[act], dom[da][1:].reshape(-1, 3)[act]
+        r = G @ dX - H @ X
+        assert np.linalg.norm(r) < 1e-12 * np.linalg.norm(H @ X)
+    if did == 0:  # open-domain rows carry no incident term: H U = G dU
+        U, dU = dom["U"][1:].reshape(-1, 4)[act], dom["dU"][1:].reshape(-1, 4)[act]
+        assert np.linalg.norm(G @ dU - H @ U) < 1e-12 * np.linalg.norm(H @ U)
+
+
+@pytest.mark.slow
+def test_derivative_rows_reproduce_stored_E_H_on_triangles(v1):
+    """Boundary E, H of the 28 triangles per domain recomputed from the
+    tangential-derivative BIE rows (mo_solve.c:547-661), which use the
+    self-element principal-value integrals (lit_dcoef.c:460-635)."""
+    o = v1_raw()[0]
+    ob = v1[0]
+    sysA = orc.System(ob, 2, factor=False)
+    try:
+        sysA.solve_EH()
+        for did in (0, 1):
+            dom = o["dom"][did]
+            tri = np.where(o["ed"][np.abs(dom["sid"][1:]), 3] == 0)[0] + 1
+            assert len(tri) == 28
+            for key in ("E", "H"):
+                got = ob.state(did, key)[tri, :3]
+                want = dom[key][tri, :3]
+                err = np.abs(got - want).max() / np.abs(want).max()
+                assert err < 1e-10, (did, key, err)
+    finally:
+        sysA.free()
+        for d in (0, 1):  # restore the stored state for other tests
+            for key in ("E", "H"):
+                ob.state(d, key, o["dom"][d][key])
+
+
+@pytest.mark.slow
+def test_coupling_field_matches_converged_incident_field(v1):
+    """At convergence the incident data of object 0 equals the plane wave
+    plus the field scattered by object 1 (mo_solve.c:296-396).  The stored
+    state holds it implicitly: U(domain 1) - U(domain 0) on each node.
+    Agreement is bounded by the fixture's last iteration change (cc=4.9e-9)."""
+    o0, o1 = v1_raw()
+    rng = np.random.default_rng(7)
+    ob0, ob1 = v1
+    g = ob0.geometry()
+    Ui0 = ob0.incident("Ui0")
+    dUi0 = ob0.incident("dUi0")
+    errs, errs_d = [], []
+    for el in rng.choice(np.arange(1, ob0.Ne + 1), 40, replace=False):
+        nv = 3 if o0["ed"][el, 3] == 0 else 4
+        for l in range(nv):
+            r = g["ren"][el, l]
+            w = g["wen"][el, l] / np.linalg.norm(g["wen"][el, l])
+            did, U, dU = ob1.mEMP2_s(r, w)
+            assert did == 0
+            want = o0["dom"][1]["U"][el, l, :3] - o0["dom"][0]["U"][el, l, :3]
+            delt = o0["kn"][0] ** 2 - o0["kn"][1] ** 2
+            want_d = -o0["dom"][1]["dU"][el, l, :3] - o0["dom"][0]["dU"][el, l, :3] - delt * o0["dom"][0]["U"][el, l, 3] * w
+            errs.append(np.abs(Ui0[el, l, :3] + U[:3] - want).max() / np.abs(want).max())
+            errs_d.append(np.abs(dUi0[el, l, :3] + dU[:3] - want_d).max() / np.abs(want_d).max())
+    assert max(errs) < 1e-4 and np.median(errs) < 5e-5
+    assert max(errs_d) < 1e-3 and np.median(errs_d) < 1e-4
