@@ -1,0 +1,244 @@
+"""bem3_emf_b2_amd -- MI355X (gfx950) implementation of the bem3_emf_b2 hot
+path behind a C-ABI (include/bem3_hip.h).
+
+This module is a thin ctypes front end of ``libbem3hip.so`` for tests,
+benchmarks and Python users; the C host library (drop-in for the reference's
+bem3_emf_b2.h) links the same shared object.  There is no CPU fallback: if
+the library or the GPU is missing every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libbem3hip.so")
+_lib = None
+
+D = C.POINTER(C.c_double)
+Ip = C.POINTER(C.c_int)
+VP = C.c_void_p
+
+
+class BemError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BemError("libbem3hip.so is not built (run __graft_entry__.build() or python -m bem3_emf_b2_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "bh_last_error": (C.c_char_p, []),
+            "bh_init": (C.c_int, [C.c_int]),
+            "bh_sync": (C.c_int, []),
+            "bh_object_create": (VP, [C.c_int, D, C.c_int, Ip, Ip, Ip, C.c_int, D, C.c_double]),
+            "bh_object_destroy": (None, [VP]),
+            "bh_object_info": (C.c_int, [VP, Ip]),
+            "bh_object_geometry": (C.c_int, [VP, D, D, D, D, Ip]),
+            "bh_object_assemble": (C.c_int, [VP, C.c_int]),
+            "bh_object_gh_count": (C.c_int, [VP, C.c_int]),
+            "bh_object_matrix": (C.c_int, [VP, C.c_int, C.c_int, D]),
+            "bh_zgesv": (C.c_int, [C.c_int, C.c_int, D, D]),
+            "bh_object_set_beams": (C.c_int, [VP, C.c_int, D]),
+            "bh_object_factor": (C.c_int, [VP, C.c_int]),
+            "bh_object_solve": (C.c_int, [VP]),
+            "bh_object_solve_EH": (C.c_int, [VP]),
+            "bh_object_state": (C.c_int, [VP, C.c_int, C.c_int, D, C.c_int]),
+            "bh_object_incident": (C.c_int, [VP, C.c_int, D, C.c_int]),
+            "bh_mo_solve": (C.c_int, [C.POINTER(VP), C.c_int, C.c_double, C.c_int, C.c_int, D, Ip]),
+            "bh_mo_field": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D, Ip]),
+            "bh_mo_force": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D, D, D]),
+            "bh_mo_absorb": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D]),
+            "bh_zgemm_host": (C.c_int, [C.c_int, C.c_int, C.c_int, D, D, D, D, D]),
+        }
+        optional = {}
+        try:
+            from . import _sigs
+            optional = _sigs.SIGS
+        except ImportError:
+            pass
+        for name, (res, args) in list(sig.items()) + list(optional.items()):
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise BemError("bem3_hip error %d: %s" % (rc, lib().bh_last_error().decode()))
+    return rc
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def ip(a):
+    return a.ctypes.data_as(Ip)
+
+
+def init(device=0):
+    check(lib().bh_init(device))
+
+
+class Object:
+    """Device-resident object (one paired triangle mesh and its media)."""
+
+    def __init__(self, rn, ed, md, sd, n_med, lambda0):
+        rn = np.ascontiguousarray(rn, dtype=np.float64)
+        ed = np.ascontiguousarray(ed, dtype=np.int32)
+        md = np.ascontiguousarray(md, dtype=np.int32)
+        sd = np.ascontiguousarray(sd, dtype=np.int32)
+        n = np.ascontiguousarray(n_med, dtype=np.complex128)
+        h = lib().bh_object_create(rn.shape[0] - 1, dp(rn), ed.shape[0] - 1, ip(ed), ip(md), ip(sd), len(n) - 1,
+                                   dp(n.view(np.float64)), float(lambda0))
+        if not h:
+            raise BemError("bh_object_create: " + lib().bh_last_error().decode())
+        self.h = C.c_void_p(h)
+        info = np.zeros(20, np.int32)
+        lib().bh_object_info(self.h, ip(info))
+        self.Nn, self.Ne, self.NN, self.MN = (int(x) for x in info[:4])
+        self.dNe = [int(x) for x in info[4:4 + self.MN + 1]]
+        self.nd = [int(x) for x in info[12:12 + self.MN + 1]]
+
+    def assemble(self, type_=2):
+        check(lib().bh_object_assemble(self.h, type_))
+
+    def gh_count(self, did):
+        return lib().bh_object_gh_count(self.h, did)
+
+    def matrix(self, did, which):
+        nd = self.nd[did]
+        k = {"G": 0, "H": 1, "dG": 2, "dH": 3, "dF": 4}[which]
+        if k < 2:
+            out = np.zeros(nd * nd, np.complex128)
+        elif k < 4:
+            out = np.zeros(2 * nd * nd, np.complex128)
+        else:
+            out = np.zeros(3 * nd)
+        check(lib().bh_object_matrix(self.h, did, k, dp(out.view(np.float64))))
+        if k < 2:
+            return out.reshape(nd, nd)
+        if k < 4:
+            return out.reshape(nd, 2, nd)
+        return out.reshape(nd, 3)
+
+    def geometry(self):
+        E1 = self.Ne + 1
+        g = dict(cr=np.zeros((E1, 3, 4)), cw=np.zeros((E1, 3, 3)), ren=np.zeros((E1, 4, 3)), wen=np.zeros((E1, 4, 3)),
+                 eni=np.zeros((E1, 4), np.int32))
+        check(lib().bh_object_geometry(self.h, dp(g["cr"]), dp(g["cw"]), dp(g["ren"]), dp(g["wen"]), ip(g["eni"])))
+        return g
+
+    def set_beams(self, beams):
+        b = np.ascontiguousarray(beams, dtype=np.float64).reshape(-1, 12)
+        check(lib().bh_object_set_beams(self.h, b.shape[0], dp(b)))
+
+    def factor(self, mode=0):
+        check(lib().bh_object_factor(self.h, mode))
+
+    def solve(self):
+        check(lib().bh_object_solve(self.h))
+
+    def solve_EH(self):
+        check(lib().bh_object_solve_EH(self.h))
+
+    def state(self, did, which, value=None):
+        k = ["U", "dU", "E", "dE", "H", "dH"].index(which)
+        nc = 4 if k < 2 else 3
+        n = (self.dNe[did] + 1) * 4 * nc
+        if value is None:
+            buf = np.zeros(n, np.complex128)
+            check(lib().bh_object_state(self.h, did, k, dp(buf.view(np.float64)), 0))
+            return buf.reshape(self.dNe[did] + 1, 4, nc)
+        buf = np.ascontiguousarray(value, dtype=np.complex128).reshape(-1)
+        assert buf.size == n
+        check(lib().bh_object_state(self.h, did, k, dp(buf.view(np.float64)), 1))
+
+    def incident(self, which, value=None):
+        k = ["Ui", "dUi", "Ui0", "dUi0"].index(which)
+        n = (self.Ne + 1) * 16
+        if value is None:
+            buf = np.zeros(n, np.complex128)
+            check(lib().bh_object_incident(self.h, k, dp(buf.view(np.float64)), 0))
+            return buf.reshape(self.Ne + 1, 4, 4)
+        buf = np.ascontiguousarray(value, dtype=np.complex128).reshape(-1)
+        check(lib().bh_object_incident(self.h, k, dp(buf.view(np.float64)), 1))
+
+    def close(self):
+        if self.h:
+            lib().bh_object_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def zgesv(A, B):
+    A = np.ascontiguousarray(A, dtype=np.complex128).copy()
+    B = np.ascontiguousarray(B, dtype=np.complex128).copy()
+    nrhs = 1 if B.ndim == 1 else B.shape[1]
+    check(lib().bh_zgesv(A.shape[0], nrhs, dp(A.view(np.float64)), dp(B.view(np.float64))))
+    return B
+
+
+def zgemm(A, B, Cm=None, alpha=1.0, beta=0.0):
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    B = np.ascontiguousarray(B, dtype=np.complex128)
+    m, k = A.shape
+    n = B.shape[1]
+    Cm = np.zeros((m, n), np.complex128) if Cm is None else np.ascontiguousarray(Cm, dtype=np.complex128).copy()
+    a = np.array([complex(alpha).real, complex(alpha).imag])
+    b = np.array([complex(beta).real, complex(beta).imag])
+    check(lib().bh_zgemm_host(m, n, k, dp(a), dp(A.view(np.float64)), dp(B.view(np.float64)), dp(b),
+                              dp(Cm.view(np.float64))))
+    return Cm
+
+
+def _handles(objs):
+    return (C.c_void_p * len(objs))(*[o.h.value for o in objs])
+
+
+def mo_solve(objs, ccd=1e-6, iter_min=10, iter_max=100):
+    trace = np.zeros(iter_max + 1)
+    it = np.zeros(1, np.int32)
+    check(lib().bh_mo_solve(_handles(objs), len(objs), ccd, iter_min, iter_max, dp(trace), ip(it)))
+    return int(it[0]), trace
+
+
+def mo_field(objs, pts, what="EH", mode="t", type_=0):
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    w = {"EH": 0, "mEMP": 1}[what]
+    m = {"s": 0, "t": 1, "i": 2}[mode]
+    nc = 6 if w == 0 else 4
+    out = np.zeros(pts.shape[0] * nc, np.complex128)
+    oid = np.zeros(pts.shape[0], np.int32)
+    check(lib().bh_mo_field(_handles(objs), len(objs), w, m, type_, pts.shape[0], dp(pts), dp(out.view(np.float64)),
+                            ip(oid)))
+    return out.reshape(-1, nc), oid
+
+
+def mo_force(objs, oid, type_, rc=(0.0, 0.0, 0.0)):
+    F, Nq = np.zeros(3), np.zeros(3)
+    ret = lib().bh_mo_force(_handles(objs), len(objs), oid, type_, dp(np.asarray(rc, np.float64)), dp(F), dp(Nq))
+    if ret not in (0, -1):
+        check(ret)
+    return ret, F, Nq
+
+
+def mo_absorb(objs, oid, type_):
+    P = np.zeros(1)
+    ret = lib().bh_mo_absorb(_handles(objs), len(objs), oid, type_, dp(P))
+    if ret not in (0, -1, -2):
+        check(ret)
+    return ret, float(P[0])

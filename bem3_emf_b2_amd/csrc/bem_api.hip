@@ -1,0 +1,1368 @@
+// bem_api.hip -- C-ABI of the MI355X bem3_emf_b2 hot path (include/bem3_hip.h).
+//
+// Objects live in HBM for their whole life: element records, domain lists,
+// the compact coefficient matrices of every domain, the factorised system,
+// incident and boundary values.  Host code only moves inputs in and results
+// out.  Every entry point returns 0 on success and a negative code on error
+// (the message is kept for bh_last_error()).
+#include "bem_dev.h"
+#include "../../include/bem3_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+typedef std::complex<double> zd;
+
+
+extern "C" {
+void bem_launch_asm_gl(const void *, const int *, const void *, const int *, int, int, double, double, const void *, int,
+                       void *, void *, void *, void *, size_t, double *, unsigned long long *, int, hipStream_t);
+void bem_launch_asm_gh(const void *, const int *, const void *, const int *, const void *, int, double, double,
+                       const void *, int, void *, void *, void *, void *, size_t, double *, hipStream_t);
+void bem_launch_asm_self(const void *, const int *, const void *, const int *, int, double, double, const double *,
+                         const double *, const double *, const double *, void *, void *, void *, void *, size_t,
+                         hipStream_t);
+void bem_launch_asm_fin(const double *, int, const double *, const int *, int, int, void *, size_t, double *,
+                        hipStream_t);
+void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
+               hipStream_t);
+void bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
+void bem_zgetrs(int, int, const void *, size_t, const int *, void *, size_t, int, hipStream_t);
+int bem_la_init(void);
+}
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+static int fail(int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+static void dfree(void *p) { (void)hipFree(p); }
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return fail(BH_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" const char *bh_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------------------ context
+struct Ctx {
+  int dev = -1;
+  hipStream_t st = nullptr;
+  QPt *d_gl = nullptr, *d_gh = nullptr;   // 3-part triangle rules
+  int n_gl = 0, n_gh = 0;
+  double *d_xl = nullptr, *d_wl = nullptr, *d_xh = nullptr, *d_wh = nullptr;
+  double xl[12], wl[12], xh[64], wh[64];
+  hipEvent_t ev[2];
+};
+static Ctx g;
+
+// Gauss-Legendre nodes by Newton iteration on P_n (same fixed point as the
+// reference's gauleg.c:3-36: start cos(pi(i-1/4)/(n+1/2)), stop |dz|<=3e-16)
+static void gl_nodes(int n, double *x, double *w)
+{
+  int half = (n + 1) / 2;
+  for (int i = 1; i <= half; i++) {
+    double z = cos(M_PI * (i - 0.25) / (n + 0.5)), pd;
+    for (;;) {
+      double p0 = 1.0, p1 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        double pm = p1;
+        p1 = p0;
+        p0 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * pm) / j;
+      }
+      pd = n * (z * p0 - p1) / (z * z - 1.0);
+      double zo = z;
+      z = zo - p0 / pd;
+      if (fabs(z - zo) <= 3.0e-16) break;
+    }
+    x[i - 1] = -z; x[n - i] = z;
+    w[i - 1] = 2.0 / ((1.0 - z * z) * pd * pd); w[n - i] = w[i - 1];
+  }
+  if (n % 2) x[n / 2] = 0.0;
+}
+
+// 3-part map of the equilateral triangle onto three [-1,1]^2 squares
+static std::vector<QPt> tri_rule(int n, const double *x, const double *w)
+{
+  std::vector<QPt> r;
+  const double S3 = BEM_SQ3;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      double a = x[i], b = x[j], wt = (1.0 + 0.25 * a + 0.25 * b) * w[i] * w[j];
+      QPt p;
+      p = {0.125 * (3.0 + 2.0 * a + 2.0 * b + a * b), 0.125 * S3 * (-a + b), wt}; r.push_back(p);
+      p = {0.0625 * (-3.0 + a - 5.0 * b - a * b), 0.0625 * S3 * (3.0 + 3.0 * a + b + a * b), wt}; r.push_back(p);
+      p = {0.0625 * (-3.0 - 5.0 * a + b - a * b), 0.0625 * S3 * (-3.0 - a - 3.0 * b - a * b), wt}; r.push_back(p);
+    }
+  return r;
+}
+
+template <class T> static int upload(T **d, const T *h, size_t n)
+{
+  HIPCHK(hipMalloc((void **)d, std::max<size_t>(n, 1) * sizeof(T)));
+  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, g.st));
+  return 0;
+}
+
+static int ensure_init()
+{
+  if (g.dev >= 0) return 0;
+  return bh_init(0);
+}
+
+extern "C" int bh_init(int device)
+{
+  if (g.dev >= 0) return 0;
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (n <= device) return fail(BH_ENODEV, "no HIP device %d (found %d)", device, n);
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreate(&g.st));
+  HIPCHK(hipEventCreate(&g.ev[0]));
+  HIPCHK(hipEventCreate(&g.ev[1]));
+  if (bem_la_init()) fprintf(stderr, "bem3_hip: could not raise the dynamic LDS limit (%s)\n", hipGetErrorString(hipGetLastError()));
+  gl_nodes(12, g.xl, g.wl);
+  gl_nodes(64, g.xh, g.wh);
+  std::vector<QPt> a = tri_rule(12, g.xl, g.wl), b = tri_rule(64, g.xh, g.wh);
+  g.n_gl = (int)a.size(); g.n_gh = (int)b.size();
+  if (upload(&g.d_gl, a.data(), a.size()) || upload(&g.d_gh, b.data(), b.size()) || upload(&g.d_xl, g.xl, 12) ||
+      upload(&g.d_wl, g.wl, 12) || upload(&g.d_xh, g.xh, 64) || upload(&g.d_wh, g.wh, 64))
+    return BH_EHIP;
+  HIPCHK(hipStreamSynchronize(g.st));
+  g.dev = device;
+  return 0;
+}
+
+extern "C" int bh_sync(void)
+{
+  if (g.dev < 0) return 0;
+  HIPCHK(hipStreamSynchronize(g.st));
+  return 0;
+}
+
+extern "C" void *bh_stream(void) { return (void *)g.st; }
+
+// ------------------------------------------------------------------ objects
+struct Dom {
+  int dNe = 0, nd = 0;                 // elements seen, compact node count
+  std::vector<int> sid, colOff;        // [dNe+1]
+  std::vector<RowPt> rows;             // [nd]
+  int *d_sid = nullptr, *d_colOff = nullptr;
+  RowPt *d_rows = nullptr;
+  cd *d_G = nullptr, *d_H = nullptr, *d_dG = nullptr, *d_dH = nullptr;
+  double *d_dF = nullptr;
+  int n_gh = 0;
+  // solve-side tables and boundary values ([dNe+1][4][4] U/dU, [dNe+1][4][3] E/H)
+  ColInfo *d_col = nullptr;
+  int *d_colS = nullptr;
+  RowEH *d_rowEH = nullptr;
+  cd *d_U = nullptr, *d_dU = nullptr, *d_E = nullptr, *d_dE = nullptr, *d_Hf = nullptr, *d_dHf = nullptr;
+  cd *d_Gf = nullptr;                  // LU factors of G (normal derivatives of E, H)
+  int *d_Gpiv = nullptr;
+  // face table for the solid-angle domain test (faces whose main side is d)
+};
+
+struct bh_object {
+  int Nn, Ne, MN, NN;
+  std::vector<double> rn;              // [Nn+1][3]
+  std::vector<int> ed, md, sd, eni;    // [Ne+1][4], [Ne+1]
+  std::vector<double> cr, cw, ren, wen;// [Ne+1][3][4], [Ne+1][3][3], [Ne+1][4][3] x2
+  std::vector<zd> n, kn;
+  std::vector<Dom> dom;
+  TriGeo *d_geo = nullptr;
+  bool all_tri = true;
+  // incident data at every element slot [Ne+1][4][4]
+  cd *d_Ui = nullptr, *d_dUi = nullptr, *d_Ui0 = nullptr, *d_dUi0 = nullptr, *d_Uip = nullptr;
+  // open-domain node slots (targets of the incident and coupling fields)
+  int nslot = 0;
+  double *d_slotPt = nullptr, *d_slotN = nullptr;
+  int *d_slotIdx = nullptr;
+  // faces grouped by main domain for the domain test
+  double *d_fv = nullptr;
+  int *d_fsd = nullptr, *d_seg = nullptr;
+  // beams
+  std::vector<Beam> beams;
+  Beam *d_beams = nullptr;
+  double lambda0 = 0.0;
+  // factorisation: 0 none, 1 dense 8NN, 2 Schur (one medium)
+  int fmode = 0;
+  cd *d_A = nullptr, *d_work = nullptr;
+  int *d_piv = nullptr, *d_info = nullptr;
+  cd *d_G0f = nullptr, *d_Z = nullptr, *d_Mf = nullptr, *d_Sf = nullptr;
+  int *d_piv0 = nullptr, *d_pivM = nullptr, *d_pivS = nullptr;
+  cd sch_delt, sch_beps, sch_sigm;
+  cd *d_X = nullptr, *d_B = nullptr, *d_tmp = nullptr;
+};
+
+// element constants and node data (reference setup2.c:666-749, 838-869)
+static void elem_const(bh_object *o)
+{
+  const double P340 = 0.2, P341 = 0.3464101615137754587055;
+  const double zt[4] = {-P340, 2.0 * P340, -P340, 0.0}, et[4] = {-P341, 0.0, P341, 0.0};
+  o->cr.assign((size_t)(o->Ne + 1) * 12, 0.0);
+  o->cw.assign((size_t)(o->Ne + 1) * 9, 0.0);
+  o->ren.assign((size_t)(o->Ne + 1) * 12, 0.0);
+  o->wen.assign((size_t)(o->Ne + 1) * 12, 0.0);
+  for (int i = 1; i <= o->Ne; i++) {
+    double p[4][3];
+    for (int j = 0; j < 4; j++)
+      for (int d = 0; d < 3; d++) p[j][d] = o->rn[3 * (size_t)o->ed[4 * i + j] + d];
+    double *cr = &o->cr[(size_t)i * 12], *cw = &o->cw[(size_t)i * 9];
+    for (int d = 0; d < 3; d++) {
+      cr[d * 4 + 0] = (p[0][d] + p[1][d] + p[2][d]) / 3.0;
+      cr[d * 4 + 1] = (-p[0][d] + 2.0 * p[1][d] - p[2][d]) / 3.0;
+      cr[d * 4 + 2] = (p[2][d] - p[0][d]) / sqrt(3.0);
+      cr[d * 4 + 3] = 0.0;
+    }
+    double u[3], v[3];
+    for (int d = 0; d < 3; d++) { u[d] = p[1][d] - p[0][d]; v[d] = p[2][d] - p[0][d]; }
+    for (int d = 0; d < 3; d++) {
+      int a = (d + 1) % 3, b = (d + 2) % 3;
+      cw[d * 3 + 0] = u[a] * v[b] - v[a] * u[b];
+    }
+    for (int j = 0; j < 4; j++)
+      for (int d = 0; d < 3; d++) {
+        o->ren[(size_t)i * 12 + j * 3 + d] = cr[d * 4] + cr[d * 4 + 1] * zt[j] + cr[d * 4 + 2] * et[j];
+        o->wen[(size_t)i * 12 + j * 3 + d] = cw[d * 3];
+      }
+  }
+}
+
+static void unit3(double *v)
+{
+  double m = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (m != 0.0) { v[0] /= m; v[1] /= m; v[2] /= m; }
+}
+
+static int build_domains(bh_object *o)
+{
+  o->dom.assign(o->MN + 1, Dom());
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    D.sid.assign(1, 0);
+    for (int i = 1; i <= o->Ne; i++) {
+      if (o->md[i] == d) D.sid.push_back(i);
+      else if (o->sd[i] == d) D.sid.push_back(-i);
+    }
+    D.dNe = (int)D.sid.size() - 1;
+    D.colOff.assign(D.dNe + 1, 0);
+    int off = 0;
+    for (int s = 1; s <= D.dNe; s++) {
+      int a = std::abs(D.sid[s]);
+      D.colOff[s] = off;
+      off += o->ed[4 * a + 3] == 0 ? 3 : 4;
+    }
+    D.nd = off;
+    D.rows.clear();
+    for (int t = 1; t <= D.dNe; t++) {
+      int a = std::abs(D.sid[t]);
+      const double *cr = &o->cr[(size_t)a * 12];
+      double vtz[3] = {cr[1], cr[5], cr[9]}, vte[3] = {cr[2], cr[6], cr[10]};
+      unit3(vtz);
+      unit3(vte);
+      for (int tn = 0; tn < 3; tn++) {
+        RowPt r;
+        memset(&r, 0, sizeof r);
+        for (int q = 0; q < 3; q++) { r.rt[q] = o->ren[(size_t)a * 12 + tn * 3 + q]; r.v0[q] = vtz[q]; r.v1[q] = vte[q]; }
+        r.t = t;
+        r.slot = tn;
+        D.rows.push_back(r);
+      }
+    }
+    if (upload(&D.d_sid, D.sid.data(), D.sid.size()) || upload(&D.d_colOff, D.colOff.data(), D.colOff.size()) ||
+        upload(&D.d_rows, D.rows.data(), D.rows.size()))
+      return BH_EHIP;
+  }
+  return 0;
+}
+
+static int upload_geo(bh_object *o)
+{
+  std::vector<TriGeo> geo(o->Ne + 1);
+  memset(geo.data(), 0, geo.size() * sizeof(TriGeo));
+  for (int i = 1; i <= o->Ne; i++) {
+    const double *cr = &o->cr[(size_t)i * 12], *cw = &o->cw[(size_t)i * 9];
+    for (int d = 0; d < 3; d++) {
+      geo[i].c0[d] = cr[d * 4]; geo[i].c1[d] = cr[d * 4 + 1]; geo[i].c2[d] = cr[d * 4 + 2]; geo[i].W[d] = cw[d * 3];
+    }
+    geo[i].aW = sqrt(cw[0] * cw[0] + cw[3] * cw[3] + cw[6] * cw[6]);
+  }
+  return upload(&o->d_geo, geo.data(), geo.size());
+}
+
+extern "C" bh_object *bh_object_create(int Nn, const double *rn, int Ne, const int *ed, const int *md, const int *sd,
+                                       int MN, const double *n_med, double lambda0)
+{
+  if (ensure_init()) return nullptr;
+  if (Ne <= 0 || Nn <= 0 || MN < 1) { fail(BH_EARG, "empty object"); return nullptr; }
+  bh_object *o = new bh_object();
+  o->Nn = Nn; o->Ne = Ne; o->MN = MN;
+  o->rn.assign(rn, rn + 3 * (size_t)(Nn + 1));
+  o->ed.assign(ed, ed + 4 * (size_t)(Ne + 1));
+  o->md.assign(md, md + Ne + 1);
+  o->sd.assign(sd, sd + Ne + 1);
+  for (int i = 1; i <= Ne; i++) {
+    if (o->ed[4 * i + 3] != 0) o->all_tri = false;
+    if (o->md[i] < 0 || o->md[i] > MN || o->sd[i] < 0 || o->sd[i] > MN) {
+      fail(BH_EARG, "element %d: domain id out of range", i);
+      delete o;
+      return nullptr;
+    }
+  }
+  if (!o->all_tri) { fail(BH_EUNSUP, "bi-linear (quad) elements are not supported by the GPU path"); delete o; return nullptr; }
+  o->eni.assign(4 * (size_t)(Ne + 1), -1);
+  int nc = 0;
+  for (int i = 1; i <= Ne; i++) for (int j = 0; j < 3; j++) o->eni[4 * i + j] = nc++;
+  o->NN = nc;
+  o->n.assign(MN + 1, 0.0);
+  o->kn.assign(MN + 1, 0.0);
+  for (int i = 0; i <= MN; i++) {
+    o->n[i] = zd(n_med[2 * i], n_med[2 * i + 1]);
+    o->kn[i] = 2.0 * M_PI / lambda0 * o->n[i];
+  }
+  elem_const(o);
+  if (build_domains(o) || upload_geo(o)) { delete o; return nullptr; }
+  if (hipStreamSynchronize(g.st) != hipSuccess) { fail(BH_EHIP, "upload failed"); delete o; return nullptr; }
+  return o;
+}
+
+extern "C" void bh_object_destroy(bh_object *o)
+{
+  if (!o) return;
+  for (Dom &D : o->dom) {
+    dfree(D.d_sid); dfree(D.d_colOff); dfree(D.d_rows);
+    dfree(D.d_G); dfree(D.d_H); dfree(D.d_dG); dfree(D.d_dH); dfree(D.d_dF);
+  }
+  dfree(o->d_geo);
+  delete o;
+}
+
+extern "C" int bh_object_info(const bh_object *o, int *out)
+{
+  out[0] = o->Nn; out[1] = o->Ne; out[2] = o->NN; out[3] = o->MN;
+  for (int d = 0; d <= o->MN && d < 8; d++) { out[4 + d] = o->dom[d].dNe; out[12 + d] = o->dom[d].nd; }
+  return 0;
+}
+
+extern "C" int bh_object_geometry(const bh_object *o, double *cr, double *cw, double *ren, double *wen, int *eni)
+{
+  if (cr) memcpy(cr, o->cr.data(), o->cr.size() * sizeof(double));
+  if (cw) memcpy(cw, o->cw.data(), o->cw.size() * sizeof(double));
+  if (ren) memcpy(ren, o->ren.data(), o->ren.size() * sizeof(double));
+  if (wen) memcpy(wen, o->wen.data(), o->wen.size() * sizeof(double));
+  if (eni) memcpy(eni, o->eni.data(), o->eni.size() * sizeof(int));
+  return 0;
+}
+
+// ------------------------------------------------------------------ assembly
+static int assemble_domain(bh_object *o, int d)
+{
+  Dom &D = o->dom[d];
+  size_t nd = (size_t)D.nd;
+  if (!D.d_G) {
+    HIPCHK(hipMalloc((void **)&D.d_G, nd * nd * sizeof(cd)));
+    HIPCHK(hipMalloc((void **)&D.d_H, nd * nd * sizeof(cd)));
+    HIPCHK(hipMalloc((void **)&D.d_dG, 2 * nd * nd * sizeof(cd)));
+    HIPCHK(hipMalloc((void **)&D.d_dH, 2 * nd * nd * sizeof(cd)));
+    HIPCHK(hipMalloc((void **)&D.d_dF, 3 * nd * sizeof(double)));
+  }
+  const int nblk = (D.dNe + 255) / 256, nwv = nblk * 4;
+  double *d_part = nullptr;
+  unsigned long long *d_mask = nullptr;
+  HIPCHK(hipMalloc((void **)&d_part, nd * nwv * 3 * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&d_mask, nd * nwv * sizeof(unsigned long long)));
+  double kr = o->kn[d].real(), ki = o->kn[d].imag();
+  bem_launch_asm_gl(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, D.d_G, D.d_H,
+                    D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+  HIPCHK(hipGetLastError());
+  // near pairs in row-major order (deterministic)
+  std::vector<unsigned long long> mask(nd * nwv);
+  HIPCHK(hipMemcpyAsync(mask.data(), d_mask, mask.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  std::vector<GhPair> pairs;
+  std::vector<int> start(nd + 1, 0);
+  for (size_t r = 0; r < nd; r++) {
+    start[r] = (int)pairs.size();
+    for (int w = 0; w < nwv; w++) {
+      unsigned long long m = mask[r * nwv + w];
+      while (m) {
+        int b = __builtin_ctzll(m);
+        m &= m - 1;
+        pairs.push_back({(int)r, w * 64 + b + 1});
+      }
+    }
+  }
+  start[nd] = (int)pairs.size();
+  D.n_gh = (int)pairs.size();
+  GhPair *d_pairs = nullptr;
+  int *d_start = nullptr;
+  double *d_ghdf = nullptr;
+  if (upload(&d_pairs, pairs.data(), pairs.size()) || upload(&d_start, start.data(), start.size())) return BH_EHIP;
+  HIPCHK(hipMalloc((void **)&d_ghdf, std::max<size_t>(pairs.size(), 1) * 3 * sizeof(double)));
+  bem_launch_asm_gh(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh, D.d_G, D.d_H,
+                    D.d_dG, D.d_dH, nd, d_ghdf, g.st);
+  bem_launch_asm_self(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, (int)nd, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_G,
+                      D.d_H, D.d_dG, D.d_dH, nd, g.st);
+  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, (int)nd, d == 0, D.d_H, nd, D.d_dF, g.st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g.st));
+  dfree(d_part); dfree(d_mask); dfree(d_pairs); dfree(d_start); dfree(d_ghdf);
+  return 0;
+}
+
+extern "C" int bh_object_assemble(bh_object *o, int type)
+{
+  if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
+  for (int d = 0; d <= o->MN; d++) {
+    int rc = assemble_domain(o, d);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int bh_object_gh_count(const bh_object *o, int did) { return o->dom[did].n_gh; }
+
+// which: 0 G, 1 H (nd x nd), 2 dG, 3 dH (nd x 2 x nd), 4 dF (nd x 3 doubles)
+extern "C" int bh_object_matrix(const bh_object *o, int did, int which, double *out)
+{
+  if (did < 0 || did > o->MN) return fail(BH_EARG, "bad domain %d", did);
+  const Dom &D = o->dom[did];
+  size_t nd = (size_t)D.nd;
+  const void *src = which == 0 ? (const void *)D.d_G : which == 1 ? (const void *)D.d_H : which == 2 ? (const void *)D.d_dG
+                  : which == 3 ? (const void *)D.d_dH : (const void *)D.d_dF;
+  size_t bytes = which < 2 ? nd * nd * sizeof(cd) : which < 4 ? 2 * nd * nd * sizeof(cd) : 3 * nd * sizeof(double);
+  if (!src) return fail(BH_ESTATE, "matrices not assembled");
+  HIPCHK(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ------------------------------------------------------------------ dense LA entry points
+extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
+{
+  if (ensure_init()) return BH_EHIP;
+  cd *dA, *dB, *work;
+  int *piv, *info;
+  HIPCHK(hipMalloc((void **)&dA, (size_t)n * n * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&dB, (size_t)n * nrhs * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&work, 256 * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&piv, (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&info, sizeof(int)));
+  HIPCHK(hipMemcpy(dA, A, (size_t)n * n * sizeof(cd), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dB, B, (size_t)n * nrhs * sizeof(cd), hipMemcpyHostToDevice));
+  bem_zgetrf(n, dA, n, piv, work, info, 64, g.st);
+  bem_zgetrs(n, nrhs, dA, n, piv, dB, nrhs, 64, g.st);
+  HIPCHK(hipGetLastError());
+  int hinfo = 0;
+  HIPCHK(hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(B, dB, (size_t)n * nrhs * sizeof(cd), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(A, dA, (size_t)n * n * sizeof(cd), hipMemcpyDeviceToHost));
+  dfree(dA); dfree(dB); dfree(work); dfree(piv); dfree(info);
+  return hinfo ? fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo) : 0;
+}
+
+extern "C" int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B,
+                             const double *beta, double *C)
+{
+  if (ensure_init()) return BH_EHIP;
+  cd *dA, *dB, *dC;
+  HIPCHK(hipMalloc((void **)&dA, (size_t)m * k * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&dB, (size_t)k * n * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&dC, (size_t)m * n * sizeof(cd)));
+  HIPCHK(hipMemcpy(dA, A, (size_t)m * k * sizeof(cd), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dB, B, (size_t)k * n * sizeof(cd), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dC, C, (size_t)m * n * sizeof(cd), hipMemcpyHostToDevice));
+  bem_zgemm(m, n, k, alpha[0], alpha[1], dA, k, dB, n, beta[0], beta[1], dC, n, g.st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(C, dC, (size_t)m * n * sizeof(cd), hipMemcpyDeviceToHost));
+  dfree(dA); dfree(dB); dfree(dC);
+  return 0;
+}
+
+// ================================================================== solve side
+extern "C" {
+void bem_launch_build_A(const void *, const void *, int, size_t, const void *, size_t, size_t, void *, size_t,
+                        hipStream_t);
+void bem_launch_build_B(const void *, const void *, int, size_t, const void *, const void *, const void *, size_t, void *,
+                        hipStream_t);
+void bem_launch_store_X(const void *, int, const void *, size_t, const int *, const void *, const void *, void *, void *,
+                        hipStream_t);
+void bem_launch_eh_rows(const void *, const void *, int, size_t, const double *, const void *, const int *, const void *,
+                        const void *, const void *, double, double, void *, void *, hipStream_t);
+void bem_launch_pack_eh(int, const void *, const int *, const void *, const void *, void *, hipStream_t);
+void bem_launch_unpack_eh(int, const void *, const int *, const void *, void *, void *, hipStream_t);
+void bem_launch_hadamard_nn(void *, int, size_t, const void *, hipStream_t);
+void bem_launch_axpby(const void *, double, double, const void *, void *, size_t, hipStream_t);
+void bem_launch_scale_rows(void *, const void *, int, int, double, double, hipStream_t);
+void bem_launch_domain(const double *, int, const double *, const int *, const int *, int, int *, hipStream_t);
+void bem_launch_field(const double *, const int *, int, int, const void *, const int *, int, double, double, const void *,
+                      int, double, int, const void *, const void *, const void *, const void *, void *, int, hipStream_t);
+void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const int *, int, double, double,
+                        const void *, int, const void *, int, double, const void *, const void *, void *, void *,
+                        hipStream_t);
+void bem_launch_field_bd(const int *, const double *, int, int, const void *, const int *, int, double, double,
+                         const void *, int, double, const double *, const double *, const double *, const double *,
+                         const void *, const void *, const void *, const void *, void *, double *, hipStream_t);
+void bem_launch_incident_bd(const double *, const double *, const int *, int, const void *, int, void *, void *, void *,
+                            void *, void *, hipStream_t);
+void bem_launch_add_incident(const double *, const int *, int, const void *, int, void *, int, hipStream_t);
+void bem_launch_stress(const void *, const double *, const double *, const double *, int, double, const double *,
+                       double *, hipStream_t);
+void bem_launch_poynting(const void *, const double *, const double *, int, double *, hipStream_t);
+}
+
+// rule tables for field evaluation (QPtN), built at first use
+static QPtN *g_rule[4] = {nullptr, nullptr, nullptr, nullptr};
+static int g_nrule[4] = {0, 0, 0, 0};
+static double g_rscale[4];
+
+static int ensure_rules()
+{
+  if (g_rule[0]) return 0;
+  const double S3 = BEM_SQ3;
+  auto N = [&](double z, double e, double *n) {
+    n[0] = (1.0 - 2.5 * (z + S3 * e)) / 3.0;
+    n[1] = (1.0 + 5.0 * z) / 3.0;
+    n[2] = (1.0 - 2.5 * (z - S3 * e)) / 3.0;
+  };
+  std::vector<QPtN> r4, r7, rgl, rgh;
+  // 4-point rule (lit_coef_4p, lit_coef.c:26-78): each node carries only its
+  // own shape function, the centroid (weight -27/48) is shared
+  const double P0 = 0.2, P1 = 0.3464101615137754587055, W0 = 0.5208333333333333333333, W1 = 0.5625;
+  r4.push_back({-P0, -P1, W0, 1, 0, 0});
+  r4.push_back({2 * P0, 0.0, W0, 0, 1, 0});
+  r4.push_back({-P0, P1, W0, 0, 0, 1});
+  { double n[3]; N(0, 0, n); r4.push_back({0.0, 0.0, -W1, n[0], n[1], n[2]}); }
+  // 7-point rule (d3b1_const.h:60-67)
+  const double a0 = 0.3480702390148154917985, a1 = 0.2052130961576726346557, a2 = 0.6028753385763033130539,
+               a3 = 0.3554395089236065568357, w0 = 0.1259391805448271525957, w1 = 0.1323941527885061807376,
+               w2 = 0.225;
+  const double z7[7] = {-a0, 2 * a0, -a0, a1, -2 * a1, a1, 0.0}, e7[7] = {-a2, 0.0, a2, a3, 0.0, -a3, 0.0};
+  const double ww7[7] = {w0, w0, w0, w1, w1, w1, w2};
+  for (int i = 0; i < 7; i++) { double n[3]; N(z7[i], e7[i], n); r7.push_back({z7[i], e7[i], ww7[i], n[0], n[1], n[2]}); }
+  for (const QPt &q : tri_rule(12, g.xl, g.wl)) { double n[3]; N(q.z, q.e, n); rgl.push_back({q.z, q.e, q.w, n[0], n[1], n[2]}); }
+  for (const QPt &q : tri_rule(64, g.xh, g.wh)) { double n[3]; N(q.z, q.e, n); rgh.push_back({q.z, q.e, q.w, n[0], n[1], n[2]}); }
+  std::vector<QPtN> *all[4] = {&r4, &r7, &rgl, &rgh};
+  for (int i = 0; i < 4; i++) {
+    if (upload(&g_rule[i], all[i]->data(), all[i]->size())) return BH_EHIP;
+    g_nrule[i] = (int)all[i]->size();
+  }
+  g_rscale[0] = g_rscale[1] = BEM_I_EP;
+  g_rscale[2] = g_rscale[3] = 0.0625 * BEM_SQ3 * BEM_I_SPSQ3;
+  return 0;
+}
+
+template <class T> static int dalloc(T **p, size_t n)
+{
+  HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+  HIPCHK(hipMemsetAsync(*p, 0, std::max<size_t>(n, 1) * sizeof(T), g.st));
+  return 0;
+}
+
+// per-domain column tables, state buffers, slot and face tables
+static int prepare_solve(bh_object *o)
+{
+  if (o->d_Ui) return 0;
+  size_t E1 = (size_t)o->Ne + 1;
+  if (dalloc(&o->d_Ui, E1 * 16) || dalloc(&o->d_dUi, E1 * 16) || dalloc(&o->d_Ui0, E1 * 16) ||
+      dalloc(&o->d_dUi0, E1 * 16) || dalloc(&o->d_Uip, E1 * 16))
+    return BH_EHIP;
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    std::vector<ColInfo> col(D.nd);
+    std::vector<int> colS(D.nd);
+    std::vector<RowEH> rows(D.nd);
+    for (int s = 1; s <= D.dNe; s++) {
+      int sd = D.sid[s], a = std::abs(sd);
+      int mdid = o->md[a], sdid = o->sd[a];
+      zd k1 = o->kn[mdid], k2 = o->kn[sdid], delt = k1 * k1 - k2 * k2, beps = k1 * k1 / (k2 * k2), sigm = 1.0 - beps;
+      const double *cr = &o->cr[(size_t)a * 12];
+      double vtz[3] = {cr[1], cr[5], cr[9]}, vte[3] = {cr[2], cr[6], cr[10]};
+      unit3(vtz);
+      unit3(vte);
+      for (int l = 0; l < 3; l++) {
+        int j = D.colOff[s] + l;
+        ColInfo &c = col[j];
+        memset(&c, 0, sizeof c);
+        c.node = o->eni[4 * a + l];
+        c.elem = a;
+        c.slot = l;
+        c.main = mdid == d;
+        c.mdid = mdid;
+        for (int q = 0; q < 3; q++) c.n[q] = o->wen[(size_t)a * 12 + l * 3 + q];
+        unit3(c.n);
+        c.delt = mk(delt.real(), delt.imag());
+        c.beps = mk(beps.real(), beps.imag());
+        c.sigm = mk(sigm.real(), sigm.imag());
+        colS[j] = s;
+        RowEH &r = rows[j];
+        double sg = sd < 0 ? -1.0 : 1.0;
+        for (int q = 0; q < 3; q++) { r.vn[q] = sg * c.n[q]; r.vtz[q] = vtz[q]; r.vte[q] = vte[q]; }
+        r.s = s;
+        r.slot = l;
+      }
+    }
+    size_t n1 = (size_t)(D.dNe + 1);
+    if (upload(&D.d_col, col.data(), col.size()) || upload(&D.d_colS, colS.data(), colS.size()) ||
+        upload(&D.d_rowEH, rows.data(), rows.size()) || dalloc(&D.d_U, n1 * 16) || dalloc(&D.d_dU, n1 * 16) ||
+        dalloc(&D.d_E, n1 * 12) || dalloc(&D.d_dE, n1 * 12) || dalloc(&D.d_Hf, n1 * 12) || dalloc(&D.d_dHf, n1 * 12))
+      return BH_EHIP;
+  }
+  // open-domain node slots (4 per element, centroid included)
+  std::vector<double> pt, nm;
+  std::vector<int> idx;
+  for (int i = 1; i <= o->Ne; i++) {
+    if (o->md[i] != 0) continue;
+    for (int j = 0; j < 4; j++) {
+      double w[3];
+      for (int q = 0; q < 3; q++) { pt.push_back(o->ren[(size_t)i * 12 + j * 3 + q]); w[q] = o->wen[(size_t)i * 12 + j * 3 + q]; }
+      unit3(w);
+      nm.insert(nm.end(), w, w + 3);
+      idx.push_back(i * 4 + j);
+    }
+  }
+  o->nslot = (int)idx.size();
+  // faces by main domain for the solid-angle test
+  std::vector<double> fv;
+  std::vector<int> fsd, seg(o->MN + 2, 0);
+  for (int d = 0; d <= o->MN; d++) {
+    seg[d] = (int)fsd.size();
+    for (int i = 1; i <= o->Ne; i++) {
+      if (o->md[i] != d) continue;
+      for (int j = 0; j < 3; j++)
+        for (int q = 0; q < 3; q++) fv.push_back(o->rn[3 * (size_t)o->ed[4 * i + j] + q]);
+      fsd.push_back(o->sd[i]);
+    }
+  }
+  seg[o->MN + 1] = (int)fsd.size();
+  if (upload(&o->d_slotPt, pt.data(), pt.size()) || upload(&o->d_slotN, nm.data(), nm.size()) ||
+      upload(&o->d_slotIdx, idx.data(), idx.size()) || upload(&o->d_fv, fv.data(), fv.size()) ||
+      upload(&o->d_fsd, fsd.data(), fsd.size()) || upload(&o->d_seg, seg.data(), seg.size()))
+    return BH_EHIP;
+  HIPCHK(hipStreamSynchronize(g.st));
+  return 0;
+}
+
+// plane waves: 12 doubles per beam (lambda0, ni, power, re/im e0x, re/im e0y,
+// fx, fy, fz, theta, phi); derived data as setup_ipw (ipw.c:65-89)
+extern "C" int bh_object_set_beams(bh_object *o, int nb, const double *par)
+{
+  if (nb <= 0) return fail(BH_EARG, "no beams defined");
+  std::vector<Beam> bm(nb);
+  double n0 = par[1], lam = par[0];
+  for (int i = 0; i < nb; i++) {
+    const double *p = par + 12 * i;
+    if (p[0] != lam || p[1] != n0) return fail(BH_EARG, "beam %d: wavelength / medium differ from beam 0", i);
+    zd e0x(p[3], p[4]), e0y(p[5], p[6]);
+    double nrm = 1.0 / sqrt(std::norm(e0x) + std::norm(e0y));
+    zd ex = e0x * nrm, ey = e0y * nrm, hx = -ey * p[1], hy = ex * p[1];
+    double st = sin(p[10]), ct = cos(p[10]), sp = sin(p[11]), cp = cos(p[11]);
+    double a11 = sp * sp + cp * cp * ct, a12 = sp * cp * ct - sp * cp, a22 = sp * sp * ct + cp * cp;
+    zd vex = ex * a11 + ey * a12, vey = ex * a12 + ey * a22, vez = -(ex * cp + ey * sp) * st;
+    zd vhx = hx * a11 + hy * a12, vhy = hx * a12 + hy * a22, vhz = -(hx * cp + hy * sp) * st;
+    Beam &b = bm[i];
+    b.ki = 2.0 * M_PI * p[1] / p[0];
+    b.E0 = sqrt(p[2] * 2.0 / p[1]);
+    b.st = st; b.ct = ct; b.sp = sp; b.cp = cp; b.fx = p[7]; b.fy = p[8]; b.fz = p[9]; b.pad = 0;
+    b.ex = mk(vex.real(), vex.imag()); b.ey = mk(vey.real(), vey.imag()); b.ez = mk(vez.real(), vez.imag());
+    b.hx = mk(vhx.real(), vhx.imag()); b.hy = mk(vhy.real(), vhy.imag()); b.hz = mk(vhz.real(), vhz.imag());
+  }
+  o->beams = bm;
+  o->lambda0 = lam;
+  dfree(o->d_beams);
+  o->d_beams = nullptr;
+  if (upload(&o->d_beams, bm.data(), bm.size())) return BH_EHIP;
+  if (prepare_solve(o)) return BH_EHIP;
+  bem_launch_incident_bd(o->d_slotPt, o->d_slotN, o->d_slotIdx, o->nslot, o->d_beams, nb, o->d_Ui, o->d_dUi, o->d_Ui0,
+                         o->d_dUi0, o->d_Uip, g.st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g.st));
+  return 0;
+}
+
+static size_t row_base(const bh_object *o, int d)
+{
+  size_t b = 0;
+  for (int q = 0; q < d; q++) b += 4 * (size_t)o->dom[q].nd;
+  return b;
+}
+
+// mode 0: auto (Schur for one medium, dense otherwise), 1: dense 8NN, 2: Schur
+extern "C" int bh_object_factor(bh_object *o, int mode)
+{
+  if (!o->dom[0].d_G) return fail(BH_ESTATE, "assemble before factor");
+  if (prepare_solve(o)) return BH_EHIP;
+  size_t NN = (size_t)o->NN, na = 8 * NN;
+  if (mode == 0) mode = o->MN == 1 ? 2 : 1;
+  if (mode == 2 && o->MN != 1) return fail(BH_EUNSUP, "Schur form needs exactly one interior medium");
+  if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
+  if (!o->d_X) { if (dalloc(&o->d_X, na) || dalloc(&o->d_B, na) || dalloc(&o->d_tmp, 8 * NN)) return BH_EHIP; }
+  int info = 0;
+  if (mode == 1) {
+    if (!o->d_A) { HIPCHK(hipMalloc((void **)&o->d_A, na * na * sizeof(cd))); HIPCHK(hipMalloc((void **)&o->d_piv, na * sizeof(int))); }
+    HIPCHK(hipMemsetAsync(o->d_A, 0, na * na * sizeof(cd), g.st));
+    for (int d = 0; d <= o->MN; d++) {
+      Dom &D = o->dom[d];
+      bem_launch_build_A(D.d_G, D.d_H, D.nd, D.nd, D.d_col, row_base(o, d), NN, o->d_A, na, g.st);
+    }
+    bem_zgetrf((int)na, o->d_A, na, o->d_piv, o->d_work, o->d_info, 64, g.st);
+  } else {
+    // one medium: domain 0 rows H0 u - G0 q = 0, domain 1 rows couple through phi
+    Dom &D0 = o->dom[0], &D1 = o->dom[1];
+    if (D0.nd != (int)NN || D1.nd != (int)NN) return fail(BH_EUNSUP, "Schur form: unexpected domain sizes");
+    size_t n2 = NN * NN;
+    if (!o->d_G0f) {
+      if (dalloc(&o->d_G0f, n2) || dalloc(&o->d_Z, n2) || dalloc(&o->d_Mf, n2) || dalloc(&o->d_Sf, n2) ||
+          dalloc(&o->d_piv0, NN) || dalloc(&o->d_pivM, NN) || dalloc(&o->d_pivS, NN))
+        return BH_EHIP;
+    }
+    zd k1 = o->kn[0], k2 = o->kn[1], delt = k1 * k1 - k2 * k2, beps = k1 * k1 / (k2 * k2), sigm = 1.0 - beps;
+    o->sch_delt = mk(delt.real(), delt.imag());
+    o->sch_beps = mk(beps.real(), beps.imag());
+    o->sch_sigm = mk(sigm.real(), sigm.imag());
+    int n = (int)NN;
+    // Z = G0^{-1} H0
+    HIPCHK(hipMemcpyAsync(o->d_G0f, D0.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    HIPCHK(hipMemcpyAsync(o->d_Z, D0.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    bem_zgetrf(n, o->d_G0f, NN, o->d_piv0, o->d_work, o->d_info, 64, g.st);
+    bem_zgetrs(n, n, o->d_G0f, NN, o->d_piv0, o->d_Z, NN, 64, g.st);
+    // M = H1 + G1 Z  (Mf), P = G1 Z kept in Sf for the beta term
+    HIPCHK(hipMemcpyAsync(o->d_Mf, D1.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    bem_zgemm(n, n, n, 1.0, 0.0, D1.d_G, NN, o->d_Z, NN, 0.0, 0.0, o->d_Sf, NN, g.st);
+    bem_launch_axpby(o->d_Mf, 1.0, 0.0, o->d_Sf, o->d_Mf, n2, g.st);
+    // S starts as H1 + beta P
+    bem_launch_axpby(D1.d_H, beps.real(), beps.imag(), o->d_Sf, o->d_Sf, n2, g.st);
+    bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, g.st);
+    // W = M^{-1} G1, T = W o (n n^T), S -= sigma delta G1 T
+    cd *W = o->d_A;
+    bool tmpW = false;
+    if (!W) { HIPCHK(hipMalloc((void **)&W, n2 * sizeof(cd))); tmpW = true; }
+    HIPCHK(hipMemcpyAsync(W, D1.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    bem_zgetrs(n, n, o->d_Mf, NN, o->d_pivM, W, NN, 64, g.st);
+    bem_launch_hadamard_nn(W, n, NN, D1.d_col, g.st);
+    zd sd = -(sigm * delt);
+    bem_zgemm(n, n, n, sd.real(), sd.imag(), D1.d_G, NN, W, NN, 1.0, 0.0, o->d_Sf, NN, g.st);
+    if (tmpW) { HIPCHK(hipStreamSynchronize(g.st)); dfree(W); }
+    bem_zgetrf(n, o->d_Sf, NN, o->d_pivS, o->d_work, o->d_info, 64, g.st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  if (info) return fail(BH_ESING, "singular system (zero pivot %d)", info);
+  o->fmode = mode;
+  return 0;
+}
+
+// small strided copy kernels for the Schur solve vectors
+__global__ void k_gather_cols(const cd *__restrict__ src, size_t sstride, int ncol, cd *__restrict__ dst, int n)
+{
+  // dst[r*ncol + c] = src[c*sstride + r]
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ncol) return;
+  int r = i / ncol, c = i % ncol;
+  dst[i] = src[(size_t)c * sstride + r];
+}
+__global__ void k_scatter_cols(const cd *__restrict__ src, int ncol, cd *__restrict__ dst, size_t dstride, int n)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ncol) return;
+  int r = i / ncol, c = i % ncol;
+  dst[(size_t)c * dstride + r] = src[i];
+}
+// v[r] = sum_c n_c(r) Y[r][c]  (3 columns)
+__global__ void k_ndot(const cd *__restrict__ Y, const ColInfo *__restrict__ col, int n, cd *__restrict__ v)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double *nn = col[r].n;
+  v[r] = Y[3 * r] * nn[0] + Y[3 * r + 1] * nn[1] + Y[3 * r + 2] * nn[2];
+}
+// Y[r][c] = n_c(r) u[r]  (3 columns)
+__global__ void k_nscale(const cd *__restrict__ u, const ColInfo *__restrict__ col, int n, cd *__restrict__ Y)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int c = 0; c < 3; c++) Y[3 * r + c] = u[r] * col[r].n[c];
+}
+
+// U4[r][c] = Y[r][c] (c<3), u3[r] (c=3)
+__global__ void k_pack_u4(const cd *__restrict__ Y, const cd *__restrict__ u3, int n, cd *__restrict__ U4)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int c = 0; c < 3; c++) U4[4 * r + c] = Y[3 * r + c];
+  U4[4 * r + 3] = u3[r];
+}
+// X[c n + r] = U4[r][c], X[(c+4) n + r] = Q4[r][c]  (one medium: node = r)
+__global__ void k_fill_x(const cd *__restrict__ U4, const cd *__restrict__ Q4, int n, cd *__restrict__ X)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int c = 0; c < 4; c++) { X[(size_t)c * n + r] = U4[4 * r + c]; X[(size_t)(c + 4) * n + r] = Q4[4 * r + c]; }
+}
+
+static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// B from the incident data, X = A^{-1} B, boundary values (solve_coefficient_mEMP)
+extern "C" int bh_object_solve(bh_object *o)
+{
+  if (!o->fmode) return fail(BH_ESTATE, "factor before solve");
+  if (!o->d_beams) return fail(BH_ESTATE, "set beams before solve");
+  size_t NN = (size_t)o->NN, na = 8 * NN;
+  HIPCHK(hipMemsetAsync(o->d_B, 0, na * sizeof(cd), g.st));
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    bem_launch_build_B(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, row_base(o, d), o->d_B, g.st);
+  }
+  int n = (int)NN;
+  if (o->fmode == 1) {
+    HIPCHK(hipMemcpyAsync(o->d_X, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    bem_zgetrs((int)na, 1, o->d_A, na, o->d_piv, o->d_X, 1, 64, g.st);
+  } else {
+    Dom &D1 = o->dom[1];
+    const cd *b1 = o->d_B + 4 * NN;          // domain-1 rows: b_0..b_3 (domain-0 rows are zero)
+    cd *Y = o->d_tmp, *v = o->d_tmp + 3 * NN, *u3 = o->d_tmp + 4 * NN, *Qn = o->d_tmp + 5 * NN;
+    // Y = M^{-1} [b0 b1 b2]
+    hipLaunchKernelGGL(k_gather_cols, g1(3 * NN), dim3(256), 0, g.st, b1, NN, 3, Y, n);
+    bem_zgetrs(n, 3, o->d_Mf, NN, o->d_pivM, Y, 3, 64, g.st);
+    // u3 = S^{-1} (b3 - sigma G1 sum_c N_c Y_c)
+    hipLaunchKernelGGL(k_ndot, g1(NN), dim3(256), 0, g.st, Y, D1.d_col, n, v);
+    HIPCHK(hipMemcpyAsync(u3, b1 + 3 * NN, NN * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    bem_zgemm(n, 1, n, -o->sch_sigm.x, -o->sch_sigm.y, D1.d_G, NN, v, 1, 1.0, 0.0, u3, 1, g.st);
+    bem_zgetrs(n, 1, o->d_Sf, NN, o->d_pivS, u3, 1, 64, g.st);
+    // u_c = M^{-1} (b_c - delta G1 N_c u3)
+    hipLaunchKernelGGL(k_nscale, g1(NN), dim3(256), 0, g.st, u3, D1.d_col, n, Qn);
+    hipLaunchKernelGGL(k_gather_cols, g1(3 * NN), dim3(256), 0, g.st, b1, NN, 3, Y, n);
+    bem_zgemm(n, 3, n, -o->sch_delt.x, -o->sch_delt.y, D1.d_G, NN, Qn, 3, 1.0, 0.0, Y, 3, g.st);
+    bem_zgetrs(n, 3, o->d_Mf, NN, o->d_pivM, Y, 3, 64, g.st);
+    // U4 = [u0 u1 u2 u3] (in B, no longer needed), Q4 = Z U4, X = [U4^T ; Q4^T]
+    cd *U4 = o->d_B, *Q4 = o->d_tmp + 4 * NN;
+    hipLaunchKernelGGL(k_pack_u4, g1(NN), dim3(256), 0, g.st, Y, u3, n, U4);
+    bem_zgemm(n, 4, n, 1.0, 0.0, o->d_Z, NN, U4, 4, 0.0, 0.0, Q4, 4, g.st);
+    hipLaunchKernelGGL(k_fill_x, g1(NN), dim3(256), 0, g.st, U4, Q4, n, o->d_X);
+  }
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    bem_launch_store_X(o->d_X, D.nd, D.d_col, NN, D.d_colS, o->d_Ui, o->d_dUi, D.d_U, D.d_dU, g.st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g.st));
+  return 0;
+}
+
+// boundary E, H from the derivative rows, then dE/dn, dH/dn from
+// G [dE|dH] = H [E|H] per domain (solve_eh_bv + solve_deh_bv)
+extern "C" int bh_object_solve_EH(bh_object *o)
+{
+  if (!o->dom[0].d_U) return fail(BH_ESTATE, "solve before the boundary fields");
+  zd ch = o->lambda0 / (2.0 * M_PI * zd(0.0, 1.0));
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    size_t nd = (size_t)D.nd;
+    bem_launch_eh_rows(D.d_dG, D.d_dH, D.nd, nd, D.d_dF, D.d_rowEH, D.d_colS, D.d_col, D.d_U, D.d_dU, ch.real(),
+                       ch.imag(), D.d_E, D.d_Hf, g.st);
+    if (!D.d_Gf) {
+      if (dalloc(&D.d_Gf, nd * nd) || dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
+      HIPCHK(hipMemcpyAsync(D.d_Gf, D.d_G, nd * nd * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+      if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
+      bem_zgetrf(D.nd, D.d_Gf, nd, D.d_Gpiv, o->d_work, o->d_info, 64, g.st);
+    }
+    cd *P = nullptr, *R = nullptr;
+    HIPCHK(hipMalloc((void **)&P, nd * 6 * sizeof(cd)));
+    HIPCHK(hipMalloc((void **)&R, nd * 6 * sizeof(cd)));
+    bem_launch_pack_eh(D.nd, D.d_col, D.d_colS, D.d_E, D.d_Hf, P, g.st);
+    bem_zgemm(D.nd, 6, D.nd, 1.0, 0.0, D.d_H, nd, P, 6, 0.0, 0.0, R, 6, g.st);
+    bem_zgetrs(D.nd, 6, D.d_Gf, nd, D.d_Gpiv, R, 6, 64, g.st);
+    bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, g.st);
+    HIPCHK(hipStreamSynchronize(g.st));
+    dfree(P);
+    dfree(R);
+  }
+  HIPCHK(hipGetLastError());
+  int info = 0;
+  HIPCHK(hipMemcpy(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost));
+  if (info) return fail(BH_ESING, "singular G (zero pivot %d)", info);
+  return 0;
+}
+
+// which: 0 U, 1 dU ([dNe+1][4][4]), 2 E, 3 dE, 4 H, 5 dH ([dNe+1][4][3]);
+// set != 0 uploads (e.g. a stored solution), else downloads
+extern "C" int bh_object_state(bh_object *o, int did, int which, double *buf, int set)
+{
+  if (did < 0 || did > o->MN || which < 0 || which > 5) return fail(BH_EARG, "bad state selector");
+  if (prepare_solve(o)) return BH_EHIP;
+  Dom &D = o->dom[did];
+  cd *tab[6] = {D.d_U, D.d_dU, D.d_E, D.d_dE, D.d_Hf, D.d_dHf};
+  size_t n = (size_t)(D.dNe + 1) * (which < 2 ? 16 : 12) * sizeof(cd);
+  if (set) HIPCHK(hipMemcpy(tab[which], buf, n, hipMemcpyHostToDevice));
+  else HIPCHK(hipMemcpy(buf, tab[which], n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// which: 0 Ui, 1 dUi, 2 Ui0, 3 dUi0 ([Ne+1][4][4])
+extern "C" int bh_object_incident(bh_object *o, int which, double *buf, int set)
+{
+  if (prepare_solve(o)) return BH_EHIP;
+  cd *tab[4] = {o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0};
+  if (which < 0 || which > 3) return fail(BH_EARG, "bad incident selector");
+  size_t n = (size_t)(o->Ne + 1) * 16 * sizeof(cd);
+  if (set) HIPCHK(hipMemcpy(tab[which], buf, n, hipMemcpyHostToDevice));
+  else HIPCHK(hipMemcpy(buf, tab[which], n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ------------------------------------------------------------------ multi-object layer
+struct Scratch {
+  void *p = nullptr;
+  size_t n = 0;
+  ~Scratch() { dfree(p); }
+  template <class T> T *get(size_t cnt)
+  {
+    size_t b = cnt * sizeof(T);
+    if (b > n) { dfree(p); p = nullptr; if (hipMalloc(&p, b) != hipSuccess) { n = 0; return nullptr; } n = b; }
+    return (T *)p;
+  }
+};
+
+static int domain_ids(bh_object *o, const double *d_pts, int np, int *d_did)
+{
+  if (prepare_solve(o)) return BH_EHIP;
+  bem_launch_domain(d_pts, np, o->d_fv, o->d_fsd, o->d_seg, o->MN, d_did, g.st);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// fields at points (mo_EH_s/t/i, mo_mEMP_s/t/i; mo_field.c:11-282):
+// what 0 E,H (6 complex per point), 1 U (4); mode 0 s, 1 t, 2 i;
+// type 0/1/2/3 integration rule.  oid[p] = object containing the point or -1.
+extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type, int np, const double *pts, double *out,
+                           int *oid)
+{
+  if (N <= 0 || np < 0 || type < 0 || type > 3 || what < 0 || what > 1) return fail(BH_EARG, "bad field request");
+  if (np == 0) return 0;
+  if (ensure_rules()) return BH_EHIP;
+  for (int m = 0; m < N; m++) {
+    if (!objs[m]->d_beams) return fail(BH_ESTATE, "object %d has no beams", m);
+    if (mode != 2 && !objs[m]->dom[0].d_U) return fail(BH_ESTATE, "object %d is not solved", m);
+  }
+  int nc = what == 0 ? 6 : 4;
+  double *d_pts;
+  cd *d_out;
+  int *d_did, *d_sel, *d_mask;
+  HIPCHK(hipMalloc((void **)&d_pts, 3 * (size_t)np * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&d_out, (size_t)np * nc * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * N * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d_sel, (size_t)np * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d_mask, (size_t)np * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(d_pts, pts, 3 * (size_t)np * sizeof(double), hipMemcpyHostToDevice, g.st));
+  HIPCHK(hipMemsetAsync(d_out, 0, (size_t)np * nc * sizeof(cd), g.st));
+  for (int m = 0; m < N; m++)
+    if (domain_ids(objs[m], d_pts, np, d_did + (size_t)m * np)) return BH_EHIP;
+  std::vector<int> did((size_t)np * N), sel(np), mask(np);
+  HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  // object containing each point: the first one whose domain is not open
+  std::vector<int> pin(np, -1), pdom(np, 0);
+  for (int p = 0; p < np; p++)
+    for (int m = 0; m < N; m++)
+      if (did[(size_t)m * np + p] != 0) { pin[p] = m; pdom[p] = did[(size_t)m * np + p]; break; }
+  if (mode != 2) {
+    for (int m = 0; m < N; m++) {
+      bh_object *o = objs[m];
+      for (int p = 0; p < np; p++) sel[p] = pin[p] < 0 ? 0 : (pin[p] == m ? pdom[p] : -2);
+      HIPCHK(hipMemcpyAsync(d_sel, sel.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+      for (int d = 0; d <= o->MN; d++) {
+        Dom &D = o->dom[d];
+        const void *A0 = what == 0 ? (const void *)D.d_E : (const void *)D.d_U;
+        const void *dA0 = what == 0 ? (const void *)D.d_dE : (const void *)D.d_dU;
+        bem_launch_field(d_pts, d_sel, np, d, o->d_geo, D.d_sid, D.dNe, o->kn[d].real(), o->kn[d].imag(), g_rule[type],
+                         g_nrule[type], g_rscale[type], what, A0, dA0, D.d_Hf, D.d_dHf, d_out, 1, g.st);
+      }
+      HIPCHK(hipStreamSynchronize(g.st));
+    }
+  }
+  // incident field: everywhere for mode i, open-region points for mode t
+  if (mode != 0) {
+    for (int p = 0; p < np; p++) mask[p] = mode == 2 ? 1 : pin[p] < 0;
+    HIPCHK(hipMemcpyAsync(d_mask, mask.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+    bem_launch_add_incident(d_pts, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_out, what, g.st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, d_out, (size_t)np * nc * sizeof(cd), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  if (oid) memcpy(oid, pin.data(), np * sizeof(int));
+  dfree(d_pts); dfree(d_out); dfree(d_did); dfree(d_sel); dfree(d_mask);
+  return 0;
+}
+
+// surface points of the force / power rules of object oid (domain-0 faces):
+// type 0: 3 node slots (stored fields) + centroid (boundary integral);
+// type 1: 7 interior points (boundary integral)
+struct SurfPts {
+  std::vector<int> t, stored;      // element (domain-local), stored slot or -1
+  std::vector<double> ze, r, w, wt;
+};
+
+static void surface_points(const bh_object *o, int type, SurfPts &sp)
+{
+  const Dom &D = o->dom[0];
+  const double P0 = 0.2, P1 = 0.3464101615137754587055, W0 = 0.5208333333333333333333, W1 = 0.5625;
+  const double a0 = 0.3480702390148154917985, a1 = 0.2052130961576726346557, a2 = 0.6028753385763033130539,
+               a3 = 0.3554395089236065568357, w0 = 0.1259391805448271525957, w1 = 0.1323941527885061807376,
+               w2 = 0.225;
+  const double z4[4] = {-P0, 2 * P0, -P0, 0.0}, e4[4] = {-P1, 0.0, P1, 0.0}, ww4[4] = {W0, W0, W0, -W1};
+  const double z7[7] = {-a0, 2 * a0, -a0, a1, -2 * a1, a1, 0.0}, e7[7] = {-a2, 0.0, a2, a3, 0.0, -a3, 0.0};
+  const double ww7[7] = {w0, w0, w0, w1, w1, w1, w2};
+  for (int t = 1; t <= D.dNe; t++) {
+    int a = std::abs(D.sid[t]);
+    const double *cr = &o->cr[(size_t)a * 12], *cw = &o->cw[(size_t)a * 9];
+    int np = type == 0 ? 4 : 7;
+    for (int i = 0; i < np; i++) {
+      double z = type == 0 ? z4[i] : z7[i], e = type == 0 ? e4[i] : e7[i];
+      bool st = type == 0 && i < 3;
+      sp.t.push_back(t);
+      sp.stored.push_back(st ? i : -1);
+      sp.ze.push_back(z); sp.ze.push_back(e);
+      for (int d = 0; d < 3; d++) {
+        sp.r.push_back(st ? o->ren[(size_t)a * 12 + i * 3 + d] : cr[d * 4] + cr[d * 4 + 1] * z + cr[d * 4 + 2] * e);
+        sp.w.push_back(st ? o->wen[(size_t)a * 12 + i * 3 + d] : cw[d * 3]);
+      }
+      sp.wt.push_back(type == 0 ? ww4[i] : ww7[i]);
+    }
+  }
+}
+
+__global__ void k_gather_stored(const int *__restrict__ t, const int *__restrict__ stored, int np,
+                                const cd *__restrict__ E, const cd *__restrict__ H, cd *__restrict__ out)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || stored[p] < 0) return;
+  size_t o = ((size_t)t[p] * 4 + stored[p]) * 3;
+  for (int i = 0; i < 3; i++) { out[6 * (size_t)p + i] = E[o + i]; out[6 * (size_t)p + 3 + i] = H[o + i]; }
+}
+
+// total field on the surface points: self boundary integral (or stored node
+// value) + other objects + incident (mo_EH_t_bd / mo_EH_i_bd)
+static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, cd **d_EH_out, double **d_r_out,
+                          double **d_w_out, double **d_wt_out)
+{
+  bh_object *o = objs[oid];
+  Dom &D = o->dom[0];
+  int np = (int)sp.t.size();
+  int *d_t, *d_st, *d_did, *d_sel, *d_mask;
+  double *d_ze, *d_r, *d_w, *d_wt, *d_rt;
+  cd *d_EH, *d_tmp;
+  if (upload(&d_t, sp.t.data(), np) || upload(&d_st, sp.stored.data(), np) || upload(&d_ze, sp.ze.data(), 2 * (size_t)np) ||
+      upload(&d_r, sp.r.data(), 3 * (size_t)np) || upload(&d_w, sp.w.data(), 3 * (size_t)np) ||
+      upload(&d_wt, sp.wt.data(), np))
+    return BH_EHIP;
+  HIPCHK(hipMalloc((void **)&d_EH, 6 * (size_t)np * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&d_tmp, 6 * (size_t)np * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&d_rt, 3 * (size_t)np * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d_sel, (size_t)np * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d_mask, (size_t)np * sizeof(int)));
+  HIPCHK(hipMemsetAsync(d_EH, 0, 6 * (size_t)np * sizeof(cd), g.st));
+  // self object: integral points through EH_s_bd, node slots from storage
+  bem_launch_field_bd(d_t, d_ze, np, 0, o->d_geo, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(), g_rule[2],
+                      g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_tmp,
+                      d_rt, g.st);
+  // keep only the integral points of d_tmp; copy stored values
+  {
+    std::vector<int> mask(np);
+    for (int p = 0; p < np; p++) mask[p] = sp.stored[p] < 0;
+    HIPCHK(hipMemcpyAsync(d_mask, mask.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+  }
+  // field_bd computed every point; overwrite node points with stored values
+  HIPCHK(hipMemcpyAsync(d_EH, d_tmp, 6 * (size_t)np * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+  hipLaunchKernelGGL(k_gather_stored, g1(np), dim3(256), 0, g.st, d_t, d_st, np, D.d_E, D.d_Hf, d_EH);
+  // other objects at the point r (their open domain, checked by solid angle)
+  for (int m = 0; m < N; m++) {
+    if (m == oid) continue;
+    bh_object *q = objs[m];
+    if (domain_ids(q, d_r, np, d_did)) return BH_EHIP;
+    for (int d = 0; d <= q->MN; d++) {
+      Dom &Q = q->dom[d];
+      bem_launch_field(d_r, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
+                       g_nrule[2], g_rscale[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf, d_EH, 1, g.st);
+    }
+  }
+  {
+    std::vector<int> ones(np, 1);
+    HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+  }
+  bem_launch_add_incident(d_r, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_EH, 0, g.st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g.st));
+  dfree(d_t); dfree(d_st); dfree(d_ze); dfree(d_tmp); dfree(d_rt); dfree(d_did); dfree(d_sel); dfree(d_mask);
+  *d_EH_out = d_EH; *d_r_out = d_r; *d_w_out = d_w; *d_wt_out = d_wt;
+  return 0;
+}
+
+static int check_ready(bh_object **objs, int N, int oid)
+{
+  if (N <= 0 || oid < 0 || oid >= N) return fail(BH_EARG, "bad object id %d", oid);
+  if (ensure_rules()) return BH_EHIP;
+  for (int m = 0; m < N; m++)
+    if (!objs[m]->dom[0].d_U || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d is not solved", m);
+  return 0;
+}
+
+// radiation force and torque about rc (mo_force_FN, mo_force.c:10-240);
+// returns 0, or -1 when the object has no open-domain faces
+extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq)
+{
+  int rc0 = check_ready(objs, N, oid);
+  if (rc0) return rc0;
+  bh_object *o = objs[oid];
+  SurfPts sp;
+  surface_points(o, type, sp);
+  int np = (int)sp.t.size();
+  for (int j = 0; j < 3; j++) { F[j] = 0.0; Nq[j] = 0.0; }
+  if (np == 0) return -1;
+  cd *d_EH;
+  double *d_r, *d_w, *d_wt, *d_part, *d_rc;
+  int rc1 = surface_fields(objs, N, oid, sp, &d_EH, &d_r, &d_w, &d_wt);
+  if (rc1) return rc1;
+  HIPCHK(hipMalloc((void **)&d_part, 6 * (size_t)np * sizeof(double)));
+  if (upload(&d_rc, rc, 3)) return BH_EHIP;
+  double epd = o->n[0].real() * o->n[0].real();
+  bem_launch_stress(d_EH, d_r, d_w, d_wt, np, epd, d_rc, d_part, g.st);
+  std::vector<double> part(6 * (size_t)np);
+  HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  for (int p = 0; p < np; p++)
+    for (int j = 0; j < 3; j++) { F[j] += -0.5 * part[6 * p + j]; Nq[j] += -0.5 * part[6 * p + 3 + j]; }
+  dfree(d_EH); dfree(d_r); dfree(d_w); dfree(d_wt); dfree(d_part); dfree(d_rc);
+  return 0;
+}
+
+// absorbed power (mo_absorb_P, mo_absorb.c:9-44): 0, -1 (no faces) or -2
+// (negative flux, magnitude returned)
+extern "C" int bh_mo_absorb(bh_object **objs, int N, int oid, int type, double *P)
+{
+  int rc0 = check_ready(objs, N, oid);
+  if (rc0) return rc0;
+  SurfPts sp;
+  surface_points(objs[oid], type, sp);
+  int np = (int)sp.t.size();
+  *P = 0.0;
+  if (np == 0) return -1;
+  cd *d_EH;
+  double *d_r, *d_w, *d_wt, *d_part;
+  int rc1 = surface_fields(objs, N, oid, sp, &d_EH, &d_r, &d_w, &d_wt);
+  if (rc1) return rc1;
+  HIPCHK(hipMalloc((void **)&d_part, (size_t)np * sizeof(double)));
+  bem_launch_poynting(d_EH, d_w, d_wt, np, d_part, g.st);
+  std::vector<double> part(np);
+  HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  double p = 0.0;
+  for (int i = 0; i < np; i++) p += -0.25 * part[i];
+  dfree(d_EH); dfree(d_r); dfree(d_w); dfree(d_wt); dfree(d_part);
+  *P = -p;
+  if (*P < 0.0) { *P = -*P; return -2; }
+  return 0;
+}
+
+// ------------------------------------------------------------------ coupling loop
+__global__ void k_reset_incident(cd *__restrict__ Ui, cd *__restrict__ dUi, const cd *__restrict__ Ui0,
+                                 const cd *__restrict__ dUi0, cd *__restrict__ Uip, size_t n)
+{
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Uip[i] = Ui[i];
+  Ui[i] = Ui0[i];
+  dUi[i] = dUi0[i];
+}
+
+// per-slot relative change terms of ccd_f (mo_solve.c:413-455)
+__global__ void k_ccd(const cd *__restrict__ Ui, const cd *__restrict__ Uip, const int *__restrict__ slot, int np,
+                      double *__restrict__ part)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const cd *c = Ui + (size_t)slot[p] * 4, *q = Uip + (size_t)slot[p] * 4;
+  double ac = 0.0, ap = 0.0, dc = 0.0;
+  for (int i = 0; i < 3; i++) { ac += abs2(c[i]); ap += abs2(q[i]); dc += abs2(c[i] - q[i]); }
+  part[2 * p] = 2.0 * dc / (ac + ap);
+  part[2 * p + 1] = 2.0 * abs2(c[3] - q[3]) / (abs2(c[3]) + abs2(q[3]));
+}
+
+static int add_scattered(bh_object *dst, bh_object *src, int *d_did)
+{
+  if (domain_ids(src, dst->d_slotPt, dst->nslot, d_did)) return BH_EHIP;
+  std::vector<int> did(dst->nslot);
+  HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  for (int v : did)
+    if (v != 0) return fail(BH_EDOM, "scattered-field domain error (overlapping objects), domain id %d", v);
+  Dom &S = src->dom[0];
+  bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, S.d_sid, S.dNe,
+                     src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
+                     S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, g.st);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static double ccd_value(bh_object **objs, int N)
+{
+  double us = 0.0, ps = 0.0;
+  long cnt = 0;
+  for (int m = 0; m < N; m++) {
+    bh_object *o = objs[m];
+    double *d_part;
+    if (hipMalloc((void **)&d_part, 2 * (size_t)o->nslot * sizeof(double)) != hipSuccess) return -1.0;
+    hipLaunchKernelGGL(k_ccd, g1(o->nslot), dim3(256), 0, g.st, o->d_Ui, o->d_Uip, o->d_slotIdx, o->nslot, d_part);
+    std::vector<double> part(2 * (size_t)o->nslot);
+    (void)hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st);
+    (void)hipStreamSynchronize(g.st);
+    dfree(d_part);
+    for (int p = 0; p < o->nslot; p++) { us += part[2 * p]; ps += part[2 * p + 1]; }
+    cnt += o->nslot;
+  }
+  return (us > ps ? us : ps) / (double)cnt;
+}
+
+// block-Jacobi fixed point over objects (mo_solve_bv, mo_solve.c:10-88).
+// trace (optional, iter_max+1 doubles) receives cc per iteration; *iters
+// receives the final iteration count.
+extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters)
+{
+  if (N <= 0) return fail(BH_EARG, "no objects");
+  if (ensure_rules()) return BH_EHIP;
+  for (int m = 0; m < N; m++)
+    if (!objs[m]->fmode || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d: factor and set beams first", m);
+  int rc, iter = 0;
+  if (N == 1) {
+    if ((rc = bh_object_solve(objs[0]))) return rc;
+  } else {
+    int maxslot = 0;
+    for (int m = 0; m < N; m++) maxslot = std::max(maxslot, objs[m]->nslot);
+    int *d_did;
+    HIPCHK(hipMalloc((void **)&d_did, (size_t)maxslot * sizeof(int)));
+    auto sweep = [&]() -> int {
+      for (int m = 0; m < N; m++) {
+        bh_object *o = objs[m];
+        size_t n = (size_t)(o->Ne + 1) * 16;
+        hipLaunchKernelGGL(k_reset_incident, g1(n), dim3(256), 0, g.st, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, n);
+      }
+      for (int m = 0; m < N; m++)
+        for (int d = 0; d < N; d++)
+          if (d != m) { int r = add_scattered(objs[d], objs[m], d_did); if (r) return r; }
+      for (int m = 0; m < N; m++) { int r = bh_object_solve(objs[m]); if (r) return r; }
+      return 0;
+    };
+    for (int m = 0; m < N; m++) if ((rc = bh_object_solve(objs[m]))) { dfree(d_did); return rc; }
+    if ((rc = sweep())) { dfree(d_did); return rc; }
+    for (iter = 2; iter < iter_max; iter++) {
+      if ((rc = sweep())) { dfree(d_did); return rc; }
+      double cc = ccd_value(objs, N);
+      if (trace) trace[iter] = cc;
+      if (cc < ccd && iter > iter_min) break;
+    }
+    dfree(d_did);
+  }
+  for (int m = 0; m < N; m++) if ((rc = bh_object_solve_EH(objs[m]))) return rc;
+  if (iters) *iters = iter;
+  return 0;
+}
+
+// domain of np points w.r.t. one object (domain_id, setup2.c:944-978)
+extern "C" int bh_object_domain(bh_object *o, int np, const double *pts, int *did)
+{
+  if (np <= 0) return 0;
+  double *d_pts;
+  int *d_did;
+  if (upload(&d_pts, pts, 3 * (size_t)np)) return BH_EHIP;
+  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * sizeof(int)));
+  if (domain_ids(o, d_pts, np, d_did)) return BH_EHIP;
+  HIPCHK(hipMemcpyAsync(did, d_did, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  dfree(d_pts); dfree(d_did);
+  return 0;
+}
+
+// fields at np points on the surface of object oid (element t of domain
+// did, local coordinates zeta/eta): mode 0 s (own boundary integral),
+// 1 t (+ other objects + incident), 2 i (incident + other objects),
+// 3 a (own + other objects); mo_EH_*_bd, mo_field.c:632-718
+extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, const int *t, const double *ze, int type,
+                           int mode, double *out)
+{
+  if (N <= 0 || oid < 0 || oid >= N || np <= 0) return fail(BH_EARG, "bad boundary-field request");
+  if (type != 2) return fail(BH_EUNSUP, "boundary fields use the GLN rule (type 2)");
+  if (ensure_rules()) return BH_EHIP;
+  bh_object *o = objs[oid];
+  if (did < 0 || did > o->MN) return fail(BH_EARG, "bad domain %d", did);
+  Dom &D = o->dom[did];
+  for (int p = 0; p < np; p++)
+    if (t[p] < 1 || t[p] > D.dNe) return fail(BH_EARG, "element %d out of range", t[p]);
+  if (prepare_solve(o)) return BH_EHIP;
+  if (mode != 2 && !o->fmode && !D.d_E) return fail(BH_ESTATE, "object not solved");
+  for (int m = 0; m < N; m++)
+    if (!objs[m]->d_beams) return fail(BH_ESTATE, "object %d has no beams", m);
+  int *d_t, *d_did, *d_mask;
+  double *d_ze, *d_rt;
+  cd *d_out;
+  if (upload(&d_t, t, np) || upload(&d_ze, ze, 2 * (size_t)np)) return BH_EHIP;
+  HIPCHK(hipMalloc((void **)&d_rt, 3 * (size_t)np * sizeof(double)));
+  HIPCHK(hipMalloc((void **)&d_out, 6 * (size_t)np * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&d_mask, (size_t)np * sizeof(int)));
+  HIPCHK(hipMemsetAsync(d_out, 0, 6 * (size_t)np * sizeof(cd), g.st));
+  // positions on the element (r = c0 + c1 zeta + c2 eta)
+  {
+    std::vector<double> rt(3 * (size_t)np);
+    for (int p = 0; p < np; p++) {
+      const double *cr = &o->cr[(size_t)std::abs(D.sid[t[p]]) * 12];
+      for (int q = 0; q < 3; q++) rt[3 * p + q] = cr[q * 4] + cr[q * 4 + 1] * ze[2 * p] + cr[q * 4 + 2] * ze[2 * p + 1];
+    }
+    HIPCHK(hipMemcpyAsync(d_rt, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice, g.st));
+  }
+  // own boundary integral (needs the solved boundary values)
+  if (mode != 2) {
+    double *d_rt2;
+    HIPCHK(hipMalloc((void **)&d_rt2, 3 * (size_t)np * sizeof(double)));
+    bem_launch_field_bd(d_t, d_ze, np, did, o->d_geo, D.d_sid, D.dNe, o->kn[did].real(), o->kn[did].imag(), g_rule[2],
+                        g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
+                        d_rt2, g.st);
+    HIPCHK(hipStreamSynchronize(g.st));
+    dfree(d_rt2);
+  }
+  if (did == 0 && mode != 0) {
+    for (int m = 0; m < N; m++) {
+      if (m == oid) continue;
+      bh_object *q = objs[m];
+      if (domain_ids(q, d_rt, np, d_did)) return BH_EHIP;
+      for (int d = 0; d <= q->MN; d++) {
+        Dom &Q = q->dom[d];
+        bem_launch_field(d_rt, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
+                         g_nrule[2], g_rscale[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf, d_out, 1, g.st);
+      }
+    }
+    if (mode == 1 || mode == 2) {
+      std::vector<int> ones(np, 1);
+      HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+      bem_launch_add_incident(d_rt, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_out, 0, g.st);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, d_out, 6 * (size_t)np * sizeof(cd), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  dfree(d_t); dfree(d_ze); dfree(d_rt); dfree(d_out); dfree(d_did); dfree(d_mask);
+  return 0;
+}

@@ -1,0 +1,398 @@
+// bem_asm.hip -- coefficient-matrix assembly for linear-triangle meshes.
+//
+// For one domain d (element list sid_d, wave number k_d) and every pair
+// (row node slot, source element) the kernels produce the 18 coefficients of
+// the reference's dcoef_bd_node_t2 (d3b1_elem.c:482-539):
+//   G, H (3 shape functions), dG/dt_z, dG/dt_e, dH/dt_z, dH/dt_e and the
+//   per-row sums F, dF/dt_z, dF/dt_e (solve_bieq2.c:124-166).
+//
+// Pass 1 (k_asm_gl)   : 3-part 12x12 Gauss-Legendre rule (432 points, the
+//                       lit_dcoef_v2_GL rule) for every pair, one lane per
+//                       source element, 64 consecutive elements per
+//                       wavefront, rule table read through scalar loads.
+//                       Pairs with |dF/dt| > 0.5 (the reference's GHN switch)
+//                       are flagged in a ballot bitmap and left for pass 2.
+// Pass 2 (k_asm_gh)   : 3-part 64x64 rule (12288 points) for flagged pairs,
+//                       one wavefront per pair, lanes over points.
+// Pass 3 (k_asm_self) : self element: polar-coordinate G (lit_coef_bd) and
+//                       the principal-value tangential derivatives
+//                       (lit_dcoef_bd_tz0/te0_PV), one wavefront per row.
+// Pass 4 (k_asm_fin)  : per-row F sums in a fixed order, the 1+F / F
+//                       diagonal term, dF rows.
+//
+// Matrices are compact: row = node slot index in the domain's element order,
+// column = shape-function index in the same order (row-major, leading
+// dimension ld, complex128 interleaved).  dG/dH rows are [row][dir][ld].
+#include "bem_dev.h"
+
+struct AsmAcc {
+  cd G[3], H[3], DG0[3], DG1[3], DH0[3], DH1[3];
+  double F, DF0, DF1;
+};
+
+__device__ __forceinline__ void acc_zero(AsmAcc &a)
+{
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    a.G[i] = mk(0, 0); a.H[i] = mk(0, 0); a.DG0[i] = mk(0, 0); a.DG1[i] = mk(0, 0); a.DH0[i] = mk(0, 0); a.DH1[i] = mk(0, 0);
+  }
+  a.F = a.DF0 = a.DF1 = 0.0;
+}
+
+// integrand of lit_sdGL_v2_zeta_eta (lit_dcoef_v2.c:381-413) at one point,
+// all three shape functions at once
+__device__ __forceinline__ void asm_point(AsmAcc &a, const TriGeo &g, const RowPt &rp, double Wdv0, double Wdv1,
+                                          double kr, double ki, double z, double e, double w)
+{
+  double n0, n1, n2;
+  tri_N(z, e, n0, n1, n2);
+  double vx = fma(g.c2[0], e, fma(g.c1[0], z, g.c0[0])) - rp.rt[0];
+  double vy = fma(g.c2[1], e, fma(g.c1[1], z, g.c0[1])) - rp.rt[1];
+  double vz = fma(g.c2[2], e, fma(g.c1[2], z, g.c0[2])) - rp.rt[2];
+  double R = sqrt(vx * vx + vy * vy + vz * vz);
+  double iR = 1.0 / R, iR2 = iR * iR, iR3 = iR2 * iR;
+  double rdW = (vx * g.W[0] + vy * g.W[1] + vz * g.W[2]) * iR;
+  double rd0 = (vx * rp.v0[0] + vy * rp.v0[1] + vz * rp.v0[2]) * iR;
+  double rd1 = (vx * rp.v1[0] + vy * rp.v1[1] + vz * rp.v1[2]) * iR;
+  cd ce = eikr(kr, ki, R);
+  cd cb = mk(-1.0 - ki * R, kr * R);            // i k R - 1
+  cd cbce = cb * ce;
+  cd gk = ce * iR;
+  cd hk = cbce * (rdW * iR2);
+  cd tk = cbce * iR2;
+  cd kR = mk(kr * R, ki * R);
+  cd q = kR * kR + cb * 3.0;
+  cd ce3 = ce * iR3;
+  cd dh0 = ce3 * (q * (rd0 * rdW) - cb * Wdv0);
+  cd dh1 = ce3 * (q * (rd1 * rdW) - cb * Wdv1);
+  double wn[3] = {w * n0, w * n1, w * n2};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    fma_cr(a.G[i], gk, wn[i]);
+    fma_cr(a.H[i], hk, wn[i]);
+    fma_cr(a.DG0[i], tk, wn[i] * rd0);
+    fma_cr(a.DG1[i], tk, wn[i] * rd1);
+    fma_cr(a.DH0[i], dh0, wn[i]);
+    fma_cr(a.DH1[i], dh1, wn[i]);
+  }
+  a.F = fma(w, rdW * iR2, a.F);
+  a.DF0 = fma(w, iR3 * (3.0 * rd0 * rdW - Wdv0), a.DF0);
+  a.DF1 = fma(w, iR3 * (3.0 * rd1 * rdW - Wdv1), a.DF1);
+}
+
+// final scale of the 3-part rule (lit_dcoef_v2.c:252-262) and the sign
+// change of H, F and their derivatives on the sub-domain side
+__device__ __forceinline__ void acc_finish(AsmAcc &a, double aW, bool flip)
+{
+  const double S = 0.0625 * BEM_SQ3 * BEM_I_SPSQ3;
+  double sh = flip ? -S : S;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    a.G[i] = a.G[i] * (S * aW);
+    a.H[i] = a.H[i] * sh;
+    a.DG0[i] = a.DG0[i] * (-S * aW);
+    a.DG1[i] = a.DG1[i] * (-S * aW);
+    a.DH0[i] = a.DH0[i] * sh;
+    a.DH1[i] = a.DH1[i] * sh;
+  }
+  a.F *= sh; a.DF0 *= sh; a.DF1 *= sh;
+}
+
+__device__ __forceinline__ void acc_store(const AsmAcc &a, size_t row, size_t col, size_t ld, cd *G, cd *H, cd *dG, cd *dH)
+{
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    G[row * ld + col + i] = a.G[i];
+    H[row * ld + col + i] = a.H[i];
+    dG[(row * 2 + 0) * ld + col + i] = a.DG0[i];
+    dG[(row * 2 + 1) * ld + col + i] = a.DG1[i];
+    dH[(row * 2 + 0) * ld + col + i] = a.DH0[i];
+    dH[(row * 2 + 1) * ld + col + i] = a.DH1[i];
+  }
+}
+
+// ---- pass 1 ----
+__global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
+                                                const RowPt *__restrict__ rows, const int *__restrict__ colOff, int dNe,
+                                                double kr, double ki, const QPt *__restrict__ rule, int nq,
+                                                cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
+                                                cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
+                                                unsigned long long *__restrict__ ghMask)
+{
+  const int row = blockIdx.y;
+  const RowPt rp = rows[row];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int lane = threadIdx.x & 63;
+  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwv = (gridDim.x * blockDim.x) >> 6;
+  bool active = s <= dNe && s != rp.t;
+  bool flagged = false;
+  double f = 0.0, df0 = 0.0, df1 = 0.0;
+  if (active) {
+    int sd = sid[s];
+    int as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
+    double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
+    AsmAcc a;
+    acc_zero(a);
+    for (int p = 0; p < nq; p++) {
+      const QPt qp = rule[p];
+      asm_point(a, g, rp, Wdv0, Wdv1, kr, ki, qp.z, qp.e, qp.w);
+    }
+    acc_finish(a, g.aW, sd < 0);
+    flagged = fabs(a.DF0) > BEM_CBD_SDF || fabs(a.DF1) > BEM_CBD_SDF;
+    if (!flagged) {
+      acc_store(a, row, colOff[s], ld, G, H, dG, dH);
+      f = a.F; df0 = a.DF0; df1 = a.DF1;
+    }
+  }
+  unsigned long long m = __ballot(flagged);
+  f = wsum(f); df0 = wsum(df0); df1 = wsum(df1);
+  if (lane == 0) {
+    size_t o = (size_t)row * nwv + wv;
+    ghMask[o] = m;
+    dfPart[o * 3 + 0] = f; dfPart[o * 3 + 1] = df0; dfPart[o * 3 + 2] = df1;
+  }
+}
+
+// ---- pass 2: flagged pairs with the 64x64 rule ----
+__global__ void __launch_bounds__(256) k_asm_gh(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
+                                                const RowPt *__restrict__ rows, const int *__restrict__ colOff,
+                                                const GhPair *__restrict__ pairs, int npairs, double kr, double ki,
+                                                const QPt *__restrict__ rule, int nq, cd *__restrict__ G,
+                                                cd *__restrict__ H, cd *__restrict__ dG, cd *__restrict__ dH, size_t ld,
+                                                double *__restrict__ ghDF)
+{
+  const int pid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (pid >= npairs) return;
+  const GhPair pr = pairs[pid];
+  const RowPt rp = rows[pr.row];
+  int sd = sid[pr.s];
+  int as = sd < 0 ? -sd : sd;
+  const TriGeo g = geo[as];
+  double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
+  double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
+  AsmAcc a;
+  acc_zero(a);
+  for (int p = lane; p < nq; p += 64) {
+    const QPt qp = rule[p];
+    asm_point(a, g, rp, Wdv0, Wdv1, kr, ki, qp.z, qp.e, qp.w);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    a.G[i] = wsum(a.G[i]); a.H[i] = wsum(a.H[i]); a.DG0[i] = wsum(a.DG0[i]); a.DG1[i] = wsum(a.DG1[i]);
+    a.DH0[i] = wsum(a.DH0[i]); a.DH1[i] = wsum(a.DH1[i]);
+  }
+  a.F = wsum(a.F); a.DF0 = wsum(a.DF0); a.DF1 = wsum(a.DF1);
+  acc_finish(a, g.aW, sd < 0);
+  if (lane == 0) {
+    acc_store(a, pr.row, colOff[pr.s], ld, G, H, dG, dH);
+    ghDF[pid * 3 + 0] = a.F; ghDF[pid * 3 + 1] = a.DF0; ghDF[pid * 3 + 2] = a.DF1;
+  }
+}
+
+// ---- pass 3: self element ----
+// node slot positions of the triangle rule (setup2.c:735-737)
+__constant__ double c_tri_zt[4] = {-0.2, 0.4, -0.2, 0.0};
+__constant__ double c_tri_et[4] = {-0.3464101615137754587055, 0.0, 0.3464101615137754587055, 0.0};
+
+__device__ __forceinline__ void tri_dN(double c, double s, double &d0, double &d1, double &d2)
+{
+  // derivative of the interpolation functions along (cos, sin), lit_dcoef.c:844-855
+  d0 = -2.0 / 3.0 * 2.5 * (c + BEM_SQ3 * s);
+  d1 = 4.0 / 3.0 * 2.5 * c;
+  d2 = -2.0 / 3.0 * 2.5 * (c - BEM_SQ3 * s);
+}
+
+__device__ __forceinline__ void csincos(cd a, cd &sn, cd &cs)
+{
+  double s, c;
+  sincos(a.x, &s, &c);
+  if (a.y == 0.0) { sn = mk(s, 0.0); cs = mk(c, 0.0); return; }
+  double ch = cosh(a.y), sh = sinh(a.y);
+  sn = mk(s * ch, c * sh);
+  cs = mk(c * ch, -s * sh);
+}
+
+__global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
+                                                  const RowPt *__restrict__ rows, const int *__restrict__ colOff,
+                                                  int nrows, double kr, double ki, const double *__restrict__ xl,
+                                                  const double *__restrict__ wl, const double *__restrict__ xh,
+                                                  const double *__restrict__ wh, cd *__restrict__ G,
+                                                  cd *__restrict__ H, cd *__restrict__ dG, cd *__restrict__ dH, size_t ld)
+{
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const RowPt rp = rows[row];
+  int td = sid[rp.t];
+  int at = td < 0 ? -td : td;
+  const TriGeo g = geo[at];
+  const double zt = c_tri_zt[rp.slot], et = c_tri_et[rp.slot];
+  // polar set-up about (zt, et): vertices (-1/2,-sqrt3/2), (1,0), (-1/2,sqrt3/2)
+  double zp = 1.0 - zt, zm = -0.5 - zt, ep = 0.5 * BEM_SQ3 - et, em = -0.5 * BEM_SQ3 - et;
+  double r[4] = {sqrt(zm * zm + em * em), sqrt(zp * zp + et * et), sqrt(zm * zm + ep * ep), 0.0};
+  double th[4] = {atan2(em, zm), atan2(-et, zp), atan2(ep, zm), 0.0};
+  r[3] = r[0];
+  th[3] = th[0] + 2.0 * M_PI;
+  const double eps = BEM_DEPS;
+  double iaT0 = 1.0 / sqrt(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2]);
+  double iaT1 = 1.0 / sqrt(g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+  cd Ga[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  cd D0[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, D1[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  // complement of D_eps and the plain G integral share the angular points:
+  // 3 edges x 64 GH nodes x 2 halves = 384 angles, 6 per lane
+  for (int ia = lane; ia < 384; ia += 64) {
+    int j = ia / 128, m = (ia % 128) >> 1, half = ia & 1;
+    double dth = th[j + 1] - th[j];
+    double t = 0.25 * dth * xh[m] + (half ? 0.25 * (3.0 * th[j + 1] + th[j]) : 0.25 * (th[j + 1] + 3.0 * th[j]));
+    double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(t - th[j]) - r[j + 1] * sin(t - th[j + 1]));
+    double c, sn;
+    sincos(t, &sn, &c);
+    double K[3] = {g.c1[0] * c + g.c2[0] * sn, g.c1[1] * c + g.c2[1] * sn, g.c1[2] * c + g.c2[2] * sn};
+    double aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]), iaK = 1.0 / aK;
+    double Kd0 = (K[0] * g.c1[0] + K[1] * g.c1[1] + K[2] * g.c1[2]) * iaK;
+    double Kd1 = (K[0] * g.c2[0] + K[1] * g.c2[1] + K[2] * g.c2[2]) * iaK;
+    double wang = wh[m] * 0.25 * dth;
+    // G: radial GLN rule on (0, rm)
+    cd sg[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    for (int i = 0; i < 12; i++) {
+      double rr = 0.5 * rm * (xl[i] + 1.0);
+      double n0, n1, n2;
+      tri_N(rr * c + zt, rr * sn + et, n0, n1, n2);
+      cd v = eikr(kr, ki, aK * rr) * (iaK * wl[i]);
+      fma_cr(sg[0], v, n0); fma_cr(sg[1], v, n1); fma_cr(sg[2], v, n2);
+    }
+    // dG, complement of D_eps: radial GHN rule on (eps, rm)
+    cd sd3[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    for (int i = 0; i < 64; i++) {
+      double rr = 0.5 * (rm - eps) * xh[i] + 0.5 * (rm + eps);
+      double n0, n1, n2;
+      tri_N(rr * c + zt, rr * sn + et, n0, n1, n2);
+      double R = aK * rr;
+      cd cb = mk(-1.0 - ki * R, kr * R);
+      cd v = (cb * eikr(kr, ki, R)) * (iaK * iaK / rr * wh[i]);
+      fma_cr(sd3[0], v, n0); fma_cr(sd3[1], v, n1); fma_cr(sd3[2], v, n2);
+    }
+    double fg = wang * 0.5 * rm, fd = wang * 0.5 * (rm - eps);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      fma_cr(Ga[a], sg[a], fg);
+      fma_cr(D0[a], sd3[a], fd * Kd0);
+      fma_cr(D1[a], sd3[a], fd * Kd1);
+    }
+  }
+  // D_eps: closed-form radial factor, 4 quarters x 64 nodes = 256 angles
+  cd kk = mk(kr, ki);
+  cd pe0[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, pe1[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  cd pp0[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, pp1[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  for (int ia = lane; ia < 256; ia += 64) {
+    int q = ia >> 6, m = ia & 63;
+    double t = 0.25 * M_PI * xh[m] + (-0.75 + 0.5 * q) * M_PI;
+    double c, sn;
+    sincos(t, &sn, &c);
+    double K[3] = {g.c1[0] * c + g.c2[0] * sn, g.c1[1] * c + g.c2[1] * sn, g.c1[2] * c + g.c2[2] * sn};
+    double aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]), iaK = 1.0 / aK;
+    double Kd0 = (K[0] * g.c1[0] + K[1] * g.c1[1] + K[2] * g.c1[2]) * iaK;
+    double Kd1 = (K[0] * g.c2[0] + K[1] * g.c2[1] + K[2] * g.c2[2]) * iaK;
+    double d[3];
+    tri_dN(c, sn, d[0], d[1], d[2]);
+    cd ke = kk * (eps * aK), sk, ck;
+    csincos(ke, sk, ck);
+    cd fe = (ke + mk(0, 1)) * sk + (mk(2, 0) - mk(0, 1) * ke) * ck - mk(2, 0);
+    double ia3 = iaK * iaK * iaK * wh[m];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      fma_cr(pe0[a], fe, d[a] * ia3 * Kd0);
+      fma_cr(pe1[a], fe, d[a] * ia3 * Kd1);
+      if (q >= 2) { fma_cr(pp0[a], sk, d[a] * ia3 * Kd0); fma_cr(pp1[a], sk, d[a] * ia3 * Kd1); }
+    }
+  }
+  const double sG = BEM_I_SPSQ3 * g.aW;
+  size_t col = colOff[rp.t];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    cd ga = wsum(Ga[a]);
+    cd d0 = wsum(D0[a]), d1 = wsum(D1[a]);
+    cd e0 = wsum(pe0[a]), e1 = wsum(pe1[a]), p0 = wsum(pp0[a]), p1 = wsum(pp1[a]);
+    // + i tde (pi/4) / k - tdp (pi/4) / k
+    d0 = d0 + cdiv(mk(0, 1) * e0 * (0.25 * M_PI) - p0 * (0.25 * M_PI), kk);
+    d1 = d1 + cdiv(mk(0, 1) * e1 * (0.25 * M_PI) - p1 * (0.25 * M_PI), kk);
+    if (lane == 0) {
+      G[(size_t)row * ld + col + a] = ga * sG;
+      dG[((size_t)row * 2 + 0) * ld + col + a] = d0 * (-sG * iaT0);
+      dG[((size_t)row * 2 + 1) * ld + col + a] = d1 * (-sG * iaT1);
+      H[(size_t)row * ld + col + a] = mk(0, 0);
+      dH[((size_t)row * 2 + 0) * ld + col + a] = mk(0, 0);
+      dH[((size_t)row * 2 + 1) * ld + col + a] = mk(0, 0);
+    }
+  }
+}
+
+// ---- pass 4: per-row sums and the diagonal term ----
+__global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPart, int nwv,
+                                                 const double *__restrict__ ghDF, const int *__restrict__ ghStart,
+                                                 int nrows, int open_domain, cd *__restrict__ H, size_t ld,
+                                                 double *__restrict__ dF)
+{
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  double f = 0.0, f0 = 0.0, f1 = 0.0;
+  for (int w = lane; w < nwv; w += 64) {
+    const double *p = dfPart + ((size_t)row * nwv + w) * 3;
+    f += p[0]; f0 += p[1]; f1 += p[2];
+  }
+  for (int k = ghStart[row] + lane; k < ghStart[row + 1]; k += 64) {
+    f += ghDF[k * 3]; f0 += ghDF[k * 3 + 1]; f1 += ghDF[k * 3 + 2];
+  }
+  f = wsum(f); f0 = wsum(f0); f1 = wsum(f1);
+  if (lane == 0) {
+    double diag = open_domain ? 1.0 + f : f;
+    cd *h = H + (size_t)row * ld + row;
+    h->x += diag;
+    dF[row * 3 + 0] = diag; dF[row * 3 + 1] = f0; dF[row * 3 + 2] = f1;
+  }
+}
+
+extern "C" {
+// launchers used by the C-ABI layer (bem_api.hip)
+void bem_launch_asm_gl(const void *geo, const int *sid, const void *rows, const int *colOff, int dNe, int nrows,
+                       double kr, double ki, const void *rule, int nq, void *G, void *H, void *dG, void *dH, size_t ld,
+                       double *dfPart, unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
+{
+  dim3 grid(nwv_blocks, nrows);
+  hipLaunchKernelGGL(k_asm_gl, grid, dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff, dNe, kr, ki,
+                     (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
+}
+
+void bem_launch_asm_gh(const void *geo, const int *sid, const void *rows, const int *colOff, const void *pairs,
+                       int npairs, double kr, double ki, const void *rule, int nq, void *G, void *H, void *dG, void *dH,
+                       size_t ld, double *ghDF, hipStream_t st)
+{
+  if (npairs <= 0) return;
+  int nb = (npairs + 3) / 4;
+  hipLaunchKernelGGL(k_asm_gh, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff,
+                     (const GhPair *)pairs, npairs, kr, ki, (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH,
+                     ld, ghDF);
+}
+
+void bem_launch_asm_self(const void *geo, const int *sid, const void *rows, const int *colOff, int nrows, double kr,
+                         double ki, const double *xl, const double *wl, const double *xh, const double *wh, void *G,
+                         void *H, void *dG, void *dH, size_t ld, hipStream_t st)
+{
+  int nb = (nrows + 3) / 4;
+  hipLaunchKernelGGL(k_asm_self, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff,
+                     nrows, kr, ki, xl, wl, xh, wh, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld);
+}
+
+void bem_launch_asm_fin(const double *dfPart, int nwv, const double *ghDF, const int *ghStart, int nrows,
+                        int open_domain, void *H, size_t ld, double *dF, hipStream_t st)
+{
+  int nb = (nrows + 3) / 4;
+  hipLaunchKernelGGL(k_asm_fin, dim3(nb), dim3(256), 0, st, dfPart, nwv, ghDF, ghStart, nrows, open_domain, (cd *)H, ld,
+                     dF);
+}
+}

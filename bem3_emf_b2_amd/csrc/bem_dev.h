@@ -1,0 +1,121 @@
+// bem_dev.h -- device-side helpers shared by the HIP kernels of the
+// bem3_emf_b2 hot path (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BEM_WAVE 64
+
+struct cd {
+  double x, y;
+};
+
+__host__ __device__ __forceinline__ cd mk(double a, double b) { cd r; r.x = a; r.y = b; return r; }
+__host__ __device__ __forceinline__ cd operator+(cd a, cd b) { return mk(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ cd operator-(cd a, cd b) { return mk(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ cd operator-(cd a) { return mk(-a.x, -a.y); }
+__host__ __device__ __forceinline__ cd operator*(cd a, cd b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ __forceinline__ cd operator*(cd a, double s) { return mk(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ cd operator*(double s, cd a) { return mk(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ cd cdiv(cd a, cd b)
+{
+  double d = b.x * b.x + b.y * b.y;
+  return mk((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__host__ __device__ __forceinline__ cd conjd(cd a) { return mk(a.x, -a.y); }
+__host__ __device__ __forceinline__ double abs2(cd a) { return a.x * a.x + a.y * a.y; }
+// a += b * s  (s real)
+__device__ __forceinline__ void fma_cr(cd &a, cd b, double s) { a.x = fma(b.x, s, a.x); a.y = fma(b.y, s, a.y); }
+
+// e^{i k R} for complex k = kr + i ki
+__device__ __forceinline__ cd eikr(double kr, double ki, double R)
+{
+  double s, c;
+  sincos(kr * R, &s, &c);
+  if (ki != 0.0) {
+    double a = exp(-ki * R);
+    s *= a; c *= a;
+  }
+  return mk(c, s);
+}
+
+// ---- wave64 reductions (DPP/shuffle via __shfl_xor) ----
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, BEM_WAVE);
+  return v;
+}
+__device__ __forceinline__ cd wsum(cd v) { return mk(wsum(v.x), wsum(v.y)); }
+
+// triangle element record: r(z,e) = c0 + c1 z + c2 e, W = unnormalised normal
+struct TriGeo {
+  double c0[3], c1[3], c2[3], W[3], aW, pad[3];
+};
+
+// quadrature-point record of a fixed rule: position in the reference
+// element and the full weight (rule weight times mapping factor)
+struct QPt {
+  double z, e, w;
+};
+
+// interpolation functions at the node slots of the triangle rule
+// (reference: d3b1_elem.c:700-711), 1/(2*P34_N0) = 2.5
+__device__ __forceinline__ void tri_N(double z, double e, double &n0, double &n1, double &n2)
+{
+  const double SQ3 = 1.732050807568877293528;
+  n0 = (1.0 - 2.5 * (z + SQ3 * e)) * (1.0 / 3.0);
+  n1 = (1.0 + 5.0 * z) * (1.0 / 3.0);
+  n2 = (1.0 - 2.5 * (z - SQ3 * e)) * (1.0 / 3.0);
+}
+
+#define BEM_SQ3 1.732050807568877293528
+#define BEM_I_EP 3.978873577297383394222e-2    // 1/(8 pi)
+#define BEM_I_FP 7.957747154594766788444e-2    // 1/(4 pi)
+#define BEM_I_SPSQ3 3.062938307898844719507e-2 // 1/(6 pi sqrt 3)
+#define BEM_DEPS 0.1
+#define BEM_CBD_SDF 0.5
+
+// one row of a domain's coefficient matrices: node slot `slot` of the
+// domain's t-th element (1-based), its position and unit tangents
+struct RowPt {
+  double rt[3], v0[3], v1[3];
+  int t, slot;
+  int pad[2];
+};
+
+// a near (row, source element) pair re-integrated with the 64x64 rule
+struct GhPair {
+  int row, s;
+};
+
+// column of a domain matrix in the 8NN unknown numbering and its boundary
+// condition class (main side of its element or sub side with delta, beta,
+// sigma of the element's media)
+struct ColInfo {
+  int node;      // global element-node index (0..NN-1)
+  int elem;      // element id (1-based)
+  int slot;      // node slot 0..2
+  int main;      // 1: the element's main side is this domain
+  int mdid;      // main domain of the element
+  int pad;
+  double n[3];   // unit normal at the node slot
+  cd delt, beps, sigm;
+};
+
+// node row of a domain for the boundary E, H step
+struct RowEH {
+  double vn[3], vtz[3], vte[3];   // unit normal (domain side), unit tangents
+  int s, slot;                    // domain-local element, node slot
+};
+
+// quadrature point with its shape-function weights
+struct QPtN {
+  double z, e, w, n0, n1, n2;
+};
+
+// one incident plane wave (ipw.h:79-99 derived data)
+struct Beam {
+  double ki, E0, st, ct, sp, cp, fx, fy, fz, pad;
+  cd ex, ey, ez, hx, hy, hz;
+};

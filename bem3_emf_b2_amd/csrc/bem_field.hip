@@ -1,0 +1,484 @@
+// bem_field.hip -- surface-integral field evaluation on linear triangles:
+//   * k_domain      : domain of target points by solid angles
+//                     (domain_id + fid_calc_solid_angle, setup2.c:871-978)
+//   * k_field       : E, H (or U) at points from the boundary values of one
+//                     object (EH_s / mEMP_s, d3b1_field.c:12-209) with the
+//                     4-point, 7-point, GLN or GHN rule (coef_rt)
+//   * k_scatter     : potentials and their normal derivative at the node
+//                     slots of another object (mEMP2_s + add_scattered_mEMP,
+//                     mo_solve.c:296-396), GLN with the per-pair GHN switch
+//   * k_field_bd    : E, H at points on the object's own surface (EH_s_bd,
+//                     d3b1_field.c:238-279): GLN for other elements, the polar
+//                     rule (lit_coef_bd) for the element holding the point
+//   * k_incident    : plane-wave boundary data (init_boundary_data_u2,
+//                     mo_setup.c:704-757; calc_ipw_EH_dv, ipw.c:15-33)
+//   * k_stress      : Maxwell-stress quadrature (mo_force.c:145-240)
+// One wavefront per target point, lanes over source elements, wave
+// reductions at the end.
+#include "bem_dev.h"
+
+
+
+__device__ __forceinline__ void tri_pos(const TriGeo &g, double z, double e, double *r)
+{
+#pragma unroll
+  for (int d = 0; d < 3; d++) r[d] = fma(g.c2[d], e, fma(g.c1[d], z, g.c0[d]));
+}
+
+// G[3], H[3], F at rt from one triangle with a tabulated rule; optional
+// derivative along v (dG, dH, dF).  scale = rule scale (aW applied to G).
+template <bool DER>
+__device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, const double *v, double kr, double ki,
+                                         const QPtN *__restrict__ rule, int nq, double scale, bool flip, cd *G, cd *H,
+                                         double &F, cd *dG, cd *dH, double &dF)
+{
+  cd aG[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, aH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  cd aDG[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, aDH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  double f = 0.0, df = 0.0;
+  double Wdv = DER ? g.W[0] * v[0] + g.W[1] * v[1] + g.W[2] * v[2] : 0.0;
+  for (int p = 0; p < nq; p++) {
+    const QPtN q = rule[p];
+    double r[3];
+    tri_pos(g, q.z, q.e, r);
+    double vx = r[0] - rt[0], vy = r[1] - rt[1], vz = r[2] - rt[2];
+    double R = sqrt(vx * vx + vy * vy + vz * vz), iR = 1.0 / R, iR2 = iR * iR;
+    double rdW = (vx * g.W[0] + vy * g.W[1] + vz * g.W[2]) * iR;
+    cd ce = eikr(kr, ki, R);
+    cd cb = mk(-1.0 - ki * R, kr * R);
+    cd cbce = cb * ce;
+    cd gk = ce * iR, hk = cbce * (rdW * iR2);
+    double wn[3] = {q.w * q.n0, q.w * q.n1, q.w * q.n2};
+#pragma unroll
+    for (int i = 0; i < 3; i++) { fma_cr(aG[i], gk, wn[i]); fma_cr(aH[i], hk, wn[i]); }
+    f = fma(q.w, rdW * iR2, f);
+    if (DER) {
+      double rd = (vx * v[0] + vy * v[1] + vz * v[2]) * iR, iR3 = iR2 * iR;
+      cd tk = cbce * iR2;
+      cd kR = mk(kr * R, ki * R);
+      cd qq = kR * kR + cb * 3.0;
+      cd dh = (ce * iR3) * (qq * (rd * rdW) - cb * Wdv);
+#pragma unroll
+      for (int i = 0; i < 3; i++) { fma_cr(aDG[i], tk, wn[i] * rd); fma_cr(aDH[i], dh, wn[i]); }
+      df = fma(q.w, iR3 * (3.0 * rd * rdW - Wdv), df);
+    }
+  }
+  double sh = flip ? -scale : scale;
+#pragma unroll
+  for (int i = 0; i < 3; i++) { G[i] = aG[i] * (scale * g.aW); H[i] = aH[i] * sh; }
+  F = f * sh;
+  if (DER) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) { dG[i] = aDG[i] * (-scale * g.aW); dH[i] = aDH[i] * sh; }
+    dF = df * sh;
+  }
+}
+
+// ---------------------------------------------------------------- domain id
+// Og[d] accumulates the solid angle of faces whose main side is d and
+// subtracts it from their sub side (setup2.c:944-978).  Loops the domains
+// in order; the first face reporting "on the boundary" decides.
+#define BEM_MAXDOM 8
+__device__ __forceinline__ double solid_angle_tri(double a[3], double b[3], double c[3], int &flg)
+{
+  // unit vectors towards the vertices; signed spherical excess
+  double *r[3] = {a, b, c};
+  flg = 0;
+  for (int i = 0; i < 3; i++) {
+    double m = sqrt(r[i][0] * r[i][0] + r[i][1] * r[i][1] + r[i][2] * r[i][2]);
+    if (m == 0.0) { flg = -1; return 0.0; }
+    r[i][0] /= m; r[i][1] /= m; r[i][2] /= m;
+  }
+  double n01[3], n12[3], n20[3], t[3];
+#define CRS(o, u, w) o[0] = u[1] * w[2] - u[2] * w[1]; o[1] = u[2] * w[0] - u[0] * w[2]; o[2] = u[0] * w[1] - u[1] * w[0];
+  CRS(n01, a, b) CRS(n12, b, c) CRS(n20, c, a)
+  CRS(t, n01, n20) double A0 = atan2(sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]), -(n01[0] * n20[0] + n01[1] * n20[1] + n01[2] * n20[2]));
+  CRS(t, n12, n01) double A1 = atan2(sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]), -(n12[0] * n01[0] + n12[1] * n01[1] + n12[2] * n01[2]));
+  CRS(t, n20, n12) double A2 = atan2(sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]), -(n20[0] * n12[0] + n20[1] * n12[1] + n20[2] * n12[2]));
+#undef CRS
+  double D = a[0] * n12[0] + a[1] * n12[1] + a[2] * n12[2];
+  if (D > 0.0) return A0 + A1 + A2 - M_PI;
+  if (D < 0.0) return -(A0 + A1 + A2 - M_PI);
+  flg = -1;
+  return 0.0;
+}
+
+// faces: per face the three vertex positions and its (md, sd); faces are
+// grouped by main domain in segments segStart[d]..segStart[d+1]
+__global__ void __launch_bounds__(256) k_domain(const double *__restrict__ pts, int np, const double *__restrict__ fv,
+                                                const int *__restrict__ fsd, const int *__restrict__ segStart, int MN,
+                                                int *__restrict__ did_out)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  const double rt[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  double og[BEM_MAXDOM];
+  for (int d = 0; d <= MN; d++) og[d] = 0.0;
+  int res = -1;
+  for (int d = 0; d <= MN && res < 0; d++) {
+    int onb = 0;
+    for (int f = segStart[d] + lane; f < segStart[d + 1]; f += 64) {
+      const double *v = fv + 9 * (size_t)f;
+      double a[3] = {v[0] - rt[0], v[1] - rt[1], v[2] - rt[2]};
+      double b[3] = {v[3] - rt[0], v[4] - rt[1], v[5] - rt[2]};
+      double c[3] = {v[6] - rt[0], v[7] - rt[1], v[8] - rt[2]};
+      int flg;
+      double om = solid_angle_tri(a, b, c, flg);
+      if (flg < 0) onb = 1;
+      og[d] += om;
+      og[fsd[f]] -= om;
+    }
+    if (__ballot(onb)) { res = d; break; }
+    double od = wsum(og[d]);
+    // keep the reduced value in lane 0 only so later reductions do not
+    // count it 64 times
+    og[d] = lane == 0 ? od : 0.0;
+    if (d == 0 && fabs(od) < 2.0 * M_PI) res = 0;
+    else if (od > 2.0 * M_PI) res = d;
+    if (res < 0) {
+      // contributions to later domains accumulated so far stay per lane
+    }
+  }
+  if (lane == 0) did_out[p] = res;
+}
+
+// ---------------------------------------------------------------- fields at points
+// what: 0 -> E,H from (E,dE,H,dH); 1 -> U from (U,dU).  Output per point:
+// 6 (E,H) or 4 (U) complex values, scaled by 1/(1+F) (open domain) or 1/F.
+__global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, const int *__restrict__ pdid, int np,
+                                               int want_did, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
+                                               int dNe, double kr, double ki, const QPtN *__restrict__ rule, int nq,
+                                               double scale, int what, const cd *__restrict__ A0, const cd *__restrict__ dA0,
+                                               const cd *__restrict__ A1, const cd *__restrict__ dA1,
+                                               cd *__restrict__ out, int accumulate)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  if (pdid[p] != want_did) return;
+  const double rt[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  cd acc[6] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double F = 0.0;
+  for (int s = 1 + lane; s <= dNe; s += 64) {
+    int sd = sid[s], as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    cd G[3], H[3], dGx[3], dHx[3];
+    double f, dfx;
+    tri_coef<false>(g, rt, nullptr, kr, ki, rule, nq, scale, sd < 0, G, H, f, dGx, dHx, dfx);
+    F += f;
+    if (what == 0) {
+      const cd *E = A0 + (size_t)s * 12, *dE = dA0 + (size_t)s * 12, *Hh = A1 + (size_t)s * 12, *dH = dA1 + (size_t)s * 12;
+#pragma unroll
+      for (int n = 0; n < 3; n++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          acc[l] = acc[l] + G[n] * dE[n * 3 + l] - H[n] * E[n * 3 + l];
+          acc[3 + l] = acc[3 + l] + G[n] * dH[n * 3 + l] - H[n] * Hh[n * 3 + l];
+        }
+    } else {
+      const cd *U = A0 + (size_t)s * 16, *dU = dA0 + (size_t)s * 16;
+#pragma unroll
+      for (int n = 0; n < 3; n++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) acc[l] = acc[l] + G[n] * dU[n * 4 + l] - H[n] * U[n * 4 + l];
+    }
+  }
+  F = wsum(F);
+  int nc = what == 0 ? 6 : 4;
+  for (int l = 0; l < nc; l++) acc[l] = wsum(acc[l]);
+  if (lane == 0) {
+    double den = want_did == 0 ? 1.0 + F : F;
+    for (int l = 0; l < nc; l++) {
+      cd v = acc[l] * (1.0 / den);
+      if (accumulate) out[(size_t)p * nc + l] = out[(size_t)p * nc + l] + v;
+      else out[(size_t)p * nc + l] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- coupling
+// U, dU/dv at node slots of another object from this object's open-domain
+// boundary values; GLN with the GHN switch per pair (dcoef_rt,
+// d3b1_elem.c:277-316), added into Ui / dUi of the destination slots.
+__global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts, const double *__restrict__ nrm,
+                                                 const int *__restrict__ slot, int np, const TriGeo *__restrict__ geo,
+                                                 const int *__restrict__ sid, int dNe, double kr, double ki,
+                                                 const QPtN *__restrict__ gl, int ngl, const QPtN *__restrict__ gh,
+                                                 int ngh, double scale, const cd *__restrict__ U,
+                                                 const cd *__restrict__ dU, cd *__restrict__ Ui, cd *__restrict__ dUi)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  const double rt[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  const double v[3] = {nrm[3 * p], nrm[3 * p + 1], nrm[3 * p + 2]};
+  cd aU[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, aD[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double F = 0.0, dF = 0.0;
+  for (int s = 1 + lane; s <= dNe; s += 64) {
+    int sd = sid[s], as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    cd G[3], H[3], DG[3], DH[3];
+    double f, df;
+    tri_coef<true>(g, rt, v, kr, ki, gl, ngl, scale, sd < 0, G, H, f, DG, DH, df);
+    if (fabs(df) > BEM_CBD_SDF) tri_coef<true>(g, rt, v, kr, ki, gh, ngh, scale, sd < 0, G, H, f, DG, DH, df);
+    F += f; dF += df;
+    const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
+#pragma unroll
+    for (int n = 0; n < 3; n++)
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        aU[l] = aU[l] + G[n] * dUs[n * 4 + l] - H[n] * Us[n * 4 + l];
+        aD[l] = aD[l] + DG[n] * dUs[n * 4 + l] - DH[n] * Us[n * 4 + l];
+      }
+  }
+  F = wsum(F); dF = wsum(dF);
+  for (int l = 0; l < 4; l++) { aU[l] = wsum(aU[l]); aD[l] = wsum(aD[l]); }
+  if (lane == 0) {
+    double den = 1.0 + F;   // the target is in the open domain of this object
+    size_t o = (size_t)slot[p] * 4;
+    for (int l = 0; l < 4; l++) {
+      cd u = aU[l] * (1.0 / den);
+      cd du = (aD[l] - u * dF) * (1.0 / den);
+      Ui[o + l] = Ui[o + l] + u;
+      dUi[o + l] = dUi[o + l] + du;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- boundary points
+// E, H at (t, zeta, eta) on the object's own surface in domain did: GLN for
+// s != t (coef_bd, no GHN switch), polar G for s == t (lit_coef_bd; H and F
+// vanish on a flat element).
+__global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t, const double *__restrict__ tgt_ze,
+                                                  int np, int did, const TriGeo *__restrict__ geo,
+                                                  const int *__restrict__ sid, int dNe, double kr, double ki,
+                                                  const QPtN *__restrict__ gl, int ngl, double scale,
+                                                  const double *__restrict__ xl, const double *__restrict__ wl,
+                                                  const double *__restrict__ xh, const double *__restrict__ wh,
+                                                  const cd *__restrict__ E, const cd *__restrict__ dE,
+                                                  const cd *__restrict__ Hh, const cd *__restrict__ dH,
+                                                  cd *__restrict__ out, double *__restrict__ rt_out)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  const int t = tgt_t[p];
+  const double zt = tgt_ze[2 * p], et = tgt_ze[2 * p + 1];
+  int td = sid[t], at = td < 0 ? -td : td;
+  const TriGeo gt = geo[at];
+  double rt[3];
+  tri_pos(gt, zt, et, rt);
+  cd acc[6] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double F = 0.0;
+  for (int s = 1 + lane; s <= dNe; s += 64) {
+    if (s == t) continue;
+    int sd = sid[s], as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    cd G[3], H[3], x0[3], x1[3];
+    double f, x2;
+    tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
+    F += f;
+    const cd *Es = E + (size_t)s * 12, *dEs = dE + (size_t)s * 12, *Hs = Hh + (size_t)s * 12, *dHs = dH + (size_t)s * 12;
+#pragma unroll
+    for (int n = 0; n < 3; n++)
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        acc[l] = acc[l] + G[n] * dEs[n * 3 + l] - H[n] * Es[n * 3 + l];
+        acc[3 + l] = acc[3 + l] + G[n] * dHs[n * 3 + l] - H[n] * Hs[n * 3 + l];
+      }
+  }
+  // self element: polar rule about (zt, et), 384 angles, 12 radial points
+  double zp = 1.0 - zt, zm = -0.5 - zt, ep = 0.5 * BEM_SQ3 - et, em = -0.5 * BEM_SQ3 - et;
+  double r[4] = {sqrt(zm * zm + em * em), sqrt(zp * zp + et * et), sqrt(zm * zm + ep * ep), 0.0};
+  double th[4] = {atan2(em, zm), atan2(-et, zp), atan2(ep, zm), 0.0};
+  r[3] = r[0];
+  th[3] = th[0] + 2.0 * M_PI;
+  cd sg[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  for (int ia = lane; ia < 384; ia += 64) {
+    int j = ia / 128, m = (ia % 128) >> 1, half = ia & 1;
+    double dth = th[j + 1] - th[j];
+    double tt = 0.25 * dth * xh[m] + (half ? 0.25 * (3.0 * th[j + 1] + th[j]) : 0.25 * (th[j + 1] + 3.0 * th[j]));
+    double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(tt - th[j]) - r[j + 1] * sin(tt - th[j + 1]));
+    double c, sn;
+    sincos(tt, &sn, &c);
+    double K[3] = {gt.c1[0] * c + gt.c2[0] * sn, gt.c1[1] * c + gt.c2[1] * sn, gt.c1[2] * c + gt.c2[2] * sn};
+    double aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]), iaK = 1.0 / aK;
+    double fg = wh[m] * 0.25 * dth * 0.5 * rm;
+    for (int i = 0; i < 12; i++) {
+      double rr = 0.5 * rm * (xl[i] + 1.0);
+      double n0, n1, n2;
+      tri_N(rr * c + zt, rr * sn + et, n0, n1, n2);
+      cd vv = eikr(kr, ki, aK * rr) * (iaK * wl[i] * fg);
+      fma_cr(sg[0], vv, n0); fma_cr(sg[1], vv, n1); fma_cr(sg[2], vv, n2);
+    }
+  }
+  for (int a = 0; a < 3; a++) sg[a] = wsum(sg[a]) * (BEM_I_SPSQ3 * gt.aW);
+  const cd *dEs = dE + (size_t)t * 12, *dHs = dH + (size_t)t * 12;   // the self element's H term vanishes
+  F = wsum(F);
+  for (int l = 0; l < 6; l++) acc[l] = wsum(acc[l]);
+  if (lane == 0) {
+    for (int n = 0; n < 3; n++)
+      for (int l = 0; l < 3; l++) {
+        acc[l] = acc[l] + sg[n] * dEs[n * 3 + l];
+        acc[3 + l] = acc[3 + l] + sg[n] * dHs[n * 3 + l];
+      }
+    double den = did == 0 ? 1.0 + F : F;
+    for (int l = 0; l < 6; l++) out[(size_t)p * 6 + l] = acc[l] * (1.0 / den);
+    rt_out[3 * p] = rt[0]; rt_out[3 * p + 1] = rt[1]; rt_out[3 * p + 2] = rt[2];
+  }
+}
+
+// ---------------------------------------------------------------- incident field
+__device__ __forceinline__ void beam_EH(const Beam *__restrict__ bm, int nb, const double *x, const double *v, cd *e,
+                                        cd *h, cd *de, cd *dh)
+{
+  for (int j = 0; j < 3; j++) { e[j] = mk(0, 0); h[j] = mk(0, 0); if (de) { de[j] = mk(0, 0); dh[j] = mk(0, 0); } }
+  for (int i = 0; i < nb; i++) {
+    const Beam b = bm[i];
+    double ph = b.ki * ((x[0] - b.fx) * b.st * b.cp + (x[1] - b.fy) * b.st * b.sp + (x[2] - b.fz) * b.ct);
+    double s, c;
+    sincos(ph, &s, &c);
+    cd a = mk(c * b.E0, s * b.E0);
+    cd f[6] = {b.ex * a, b.ey * a, b.ez * a, b.hx * a, b.hy * a, b.hz * a};
+    for (int j = 0; j < 3; j++) { e[j] = e[j] + f[j]; h[j] = h[j] + f[3 + j]; }
+    if (de) {
+      cd gcf = mk(0.0, b.ki * (b.st * b.cp * v[0] + b.st * b.sp * v[1] + b.ct * v[2]));
+      for (int j = 0; j < 3; j++) { de[j] = de[j] + f[j] * gcf; dh[j] = dh[j] + f[3 + j] * gcf; }
+    }
+  }
+}
+
+// boundary incident data of the open-domain node slots (4 slots, phi = 0)
+__global__ void k_incident_bd(const double *__restrict__ pts, const double *__restrict__ nrm,
+                              const int *__restrict__ slot, int np, const Beam *__restrict__ bm, int nb,
+                              cd *__restrict__ Ui, cd *__restrict__ dUi, cd *__restrict__ Ui0, cd *__restrict__ dUi0,
+                              cd *__restrict__ Uip)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  cd e[3], h[3], de[3], dh[3];
+  beam_EH(bm, nb, pts + 3 * p, nrm + 3 * p, e, h, de, dh);
+  size_t o = (size_t)slot[p] * 4;
+  for (int c = 0; c < 3; c++) {
+    Ui[o + c] = e[c]; dUi[o + c] = de[c]; Ui0[o + c] = e[c]; dUi0[o + c] = de[c]; Uip[o + c] = e[c];
+  }
+  Ui[o + 3] = mk(0, 0); dUi[o + 3] = mk(0, 0); Ui0[o + 3] = mk(0, 0); dUi0[o + 3] = mk(0, 0); Uip[o + 3] = mk(0, 0);
+}
+
+// add the incident field to open-region points (what 0: E,H rows of 6)
+__global__ void k_add_incident(const double *__restrict__ pts, const int *__restrict__ mask, int np,
+                               const Beam *__restrict__ bm, int nb, cd *__restrict__ out, int what)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || !mask[p]) return;
+  cd e[3], h[3];
+  beam_EH(bm, nb, pts + 3 * p, nullptr, e, h, nullptr, nullptr);
+  if (what == 0)
+    for (int j = 0; j < 3; j++) { out[6 * (size_t)p + j] = out[6 * (size_t)p + j] + e[j]; out[6 * (size_t)p + 3 + j] = out[6 * (size_t)p + 3 + j] + h[j]; }
+  else
+    for (int j = 0; j < 3; j++) out[4 * (size_t)p + j] = out[4 * (size_t)p + j] + e[j];
+}
+
+// ---------------------------------------------------------------- Maxwell stress
+// per surface point: T = 1/2 (eps E E* + H H*) - 1/4 (eps|E|^2 + |H|^2) I;
+// contributions w * (T . W) and w * (r - rc) x (T . W), scaled per element
+// (-1/2 on triangles); per-element partial rows of 6 are summed in order.
+__global__ void k_stress(const cd *__restrict__ EH, const double *__restrict__ rpt, const double *__restrict__ wpt,
+                         const double *__restrict__ wgt, const int *__restrict__ elem, int np, double epd,
+                         const double *__restrict__ rc, double *__restrict__ part)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const cd *e = EH + 6 * (size_t)p, *h = e + 3;
+  double aE2 = abs2(e[0]) + abs2(e[1]) + abs2(e[2]), aH2 = abs2(h[0]) + abs2(h[1]) + abs2(h[2]);
+  double Tc = 0.25 * (epd * aE2 + aH2), T[9];
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) T[3 * j + k] = 0.5 * (epd * (e[j] * conjd(e[k])).x + (h[j] * conjd(h[k])).x);
+  T[0] -= Tc; T[4] -= Tc; T[8] -= Tc;
+  const double *w = wpt + 3 * p, *r = rpt + 3 * p;
+  double d[3] = {r[0] - rc[0], r[1] - rc[1], r[2] - rc[2]}, wt = wgt[p];
+  double *o = part + 6 * (size_t)p;
+  for (int j = 0; j < 3; j++) {
+    o[j] = (T[3 * j] * w[0] + T[3 * j + 1] * w[1] + T[3 * j + 2] * w[2]) * wt;
+    o[3 + j] = ((d[1] * T[3 * j + 2] - d[2] * T[3 * j + 1]) * w[0] + (d[2] * T[3 * j] - d[0] * T[3 * j + 2]) * w[1] +
+                (d[0] * T[3 * j + 1] - d[1] * T[3 * j]) * w[2]) * wt;
+  }
+  (void)elem;
+}
+
+// Poynting flux per surface point: w * Re(E x H*) . W
+__global__ void k_poynting(const cd *__restrict__ EH, const double *__restrict__ wpt, const double *__restrict__ wgt,
+                           int np, double *__restrict__ part)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const cd *e = EH + 6 * (size_t)p, *h = e + 3;
+  double vp[3] = {(e[1] * conjd(h[2]) - e[2] * conjd(h[1])).x, (e[2] * conjd(h[0]) - e[0] * conjd(h[2])).x,
+                  (e[0] * conjd(h[1]) - e[1] * conjd(h[0])).x};
+  const double *w = wpt + 3 * p;
+  part[p] = (vp[0] * w[0] + vp[1] * w[1] + vp[2] * w[2]) * wgt[p];
+}
+
+// ---------------------------------------------------------------- launchers
+extern "C" {
+void bem_launch_domain(const double *pts, int np, const double *fv, const int *fsd, const int *segStart, int MN,
+                       int *did, hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_domain, dim3((np + 3) / 4), dim3(256), 0, st, pts, np, fv, fsd, segStart, MN, did);
+}
+void bem_launch_field(const double *pts, const int *pdid, int np, int want_did, const void *geo, const int *sid,
+                      int dNe, double kr, double ki, const void *rule, int nq, double scale, int what, const void *A0,
+                      const void *dA0, const void *A1, const void *dA1, void *out, int accumulate, hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_field, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo, sid,
+                     dNe, kr, ki, (const QPtN *)rule, nq, scale, what, (const cd *)A0, (const cd *)dA0, (const cd *)A1,
+                     (const cd *)dA1, (cd *)out, accumulate);
+}
+void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const int *sid,
+                        int dNe, double kr, double ki, const void *gl, int ngl, const void *gh, int ngh, double scale,
+                        const void *U, const void *dU, void *Ui, void *dUi, hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo, sid, dNe,
+                     kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale, (const cd *)U, (const cd *)dU, (cd *)Ui,
+                     (cd *)dUi);
+}
+void bem_launch_field_bd(const int *tgt_t, const double *tgt_ze, int np, int did, const void *geo, const int *sid,
+                         int dNe, double kr, double ki, const void *gl, int ngl, double scale, const double *xl,
+                         const double *wl, const double *xh, const double *wh, const void *E, const void *dE,
+                         const void *H, const void *dH, void *out, double *rt_out, hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_field_bd, dim3((np + 3) / 4), dim3(256), 0, st, tgt_t, tgt_ze, np, did, (const TriGeo *)geo, sid,
+                     dNe, kr, ki, (const QPtN *)gl, ngl, scale, xl, wl, xh, wh, (const cd *)E, (const cd *)dE,
+                     (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
+}
+void bem_launch_incident_bd(const double *pts, const double *nrm, const int *slot, int np, const void *bm, int nb,
+                            void *Ui, void *dUi, void *Ui0, void *dUi0, void *Uip, hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_incident_bd, dim3((np + 255) / 256), dim3(256), 0, st, pts, nrm, slot, np, (const Beam *)bm, nb,
+                     (cd *)Ui, (cd *)dUi, (cd *)Ui0, (cd *)dUi0, (cd *)Uip);
+}
+void bem_launch_add_incident(const double *pts, const int *mask, int np, const void *bm, int nb, void *out, int what,
+                             hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_add_incident, dim3((np + 255) / 256), dim3(256), 0, st, pts, mask, np, (const Beam *)bm, nb,
+                     (cd *)out, what);
+}
+void bem_launch_stress(const void *EH, const double *rpt, const double *wpt, const double *wgt, int np, double epd,
+                       const double *rc, double *part, hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_stress, dim3((np + 255) / 256), dim3(256), 0, st, (const cd *)EH, rpt, wpt, wgt, nullptr, np, epd,
+                     rc, part);
+}
+void bem_launch_poynting(const void *EH, const double *wpt, const double *wgt, int np, double *part, hipStream_t st)
+{
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_poynting, dim3((np + 255) / 256), dim3(256), 0, st, (const cd *)EH, wpt, wgt, np, part);
+}
+}

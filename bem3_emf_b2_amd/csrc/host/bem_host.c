@@ -1,0 +1,1011 @@
+/*
+ * bem_host.c -- host side of the drop-in bem3_emf_b2 API (include/bem3_emf_b2.h).
+ *
+ * Reads the reference's input files (medium, Gmsh 2.2 mesh, plane waves,
+ * object settings), keeps the reference's in-memory tables (so callers that
+ * read MOBJ / DOMD keep working) and the reference's binary files (.obj of
+ * create_matrix, .dat of the solver), and runs every computation of the
+ * hot path through the device C-ABI (include/bem3_hip.h).
+ *
+ * Differences from the reference, by design:
+ *  - create_matrix keeps the factorised system in HBM; the .obj file is
+ *    written in the reference layout, the coefficient-matrix files only when
+ *    BEM3_WRITE_CMAT=1 (mo_initialize rebuilds the matrices on the GPU from
+ *    the .obj geometry, which is faster than reading them back);
+ *  - only plane-wave beams (ipw.txt) and linear-triangle meshes are
+ *    accepted; other inputs stop with a message.
+ */
+#include "../../../include/bem3_emf_b2.h"
+#include "../../../include/bem3_hip.h"
+
+#include <time.h>
+
+#define BIPW_SIZE 240   /* sizeof(Ipw) of the reference build */
+
+void *m_alloc2(size_t num, size_t size, char *txt)
+{
+  void *p = calloc(num ? num : 1, size);
+  if (!p) { printf("memory allocation error!\n%s. Exit..\n", txt); exit(1); }
+  return p;
+}
+
+static void *xalloc(size_t n, size_t sz, const char *what) { return m_alloc2(n, sz, (char *)what); }
+
+static void die(const char *msg)
+{
+  printf("%s Exit...\n", msg);
+  exit(1);
+}
+
+static void bh_or_die(int rc, const char *where)
+{
+  if (rc != 0) { printf("%s: device error %d (%s). Exit...\n", where, rc, bh_last_error()); exit(1); }
+}
+
+static double vabs3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+static void vunit3(double *v) { double a = vabs3(v); if (a != 0.0) { v[0] /= a; v[1] /= a; v[2] /= a; } }
+
+/* ------------------------------------------------------------ beams */
+static void beams_init(Bobj *b)
+{
+  memset(b, 0, sizeof *b);
+  strcpy(b->fname_ipw, "null"); strcpy(b->fname_fpw, "null"); strcpy(b->fname_lgb, "null");
+  strcpy(b->fname_bsb, "null"); strcpy(b->fname_blg, "null"); strcpy(b->fname_rab, "null");
+  b->bd.ipw = xalloc(1, sizeof(Ipw), "beams");
+}
+
+/* ipw.txt in the working directory (multi_fbeam.c:145-238); the other beam
+ * files of the reference are rejected when they define beams */
+static void beams_read(Bobj *b)
+{
+  static const char *other[5] = {"fpw.txt", "lgb.txt", "bsb.txt", "blg.txt", "rab.txt"};
+  for (int i = 0; i < 5; i++) {
+    FILE *fo = fopen(other[i], "r");
+    if (!fo) continue;
+    char buf[256];
+    int n = 0;
+    if (fgets(buf, sizeof buf, fo) && fgets(buf, sizeof buf, fo) && fscanf(fo, "%d", &n) == 1 && n > 0) {
+      fclose(fo);
+      printf("beam file %s: only plane waves (ipw.txt) are supported by this build. Exit...\n", other[i]);
+      exit(1);
+    }
+    fclose(fo);
+  }
+  FILE *fp = fopen("ipw.txt", "r");
+  if (!fp) { printf("read_data_mfb() No beam defined, check beam data file : ipw.txt.\n"); exit(1); }
+  char buf[256];
+  int nn;
+  if (!fgets(buf, sizeof buf, fp) || !fgets(buf, sizeof buf, fp) || fscanf(fp, "%d\n", &nn) != 1 || !fgets(buf, sizeof buf, fp))
+    die("ipw.txt: failed to read the header.");
+  if (nn <= 0) { printf("read_data_mfb() No beam defined.\n"); exit(1); }
+  free(b->bd.ipw);
+  b->n_ipw = nn;
+  snprintf(b->fname_ipw, sizeof b->fname_ipw, "%s", "ipw.txt");
+  b->bd.ipw = xalloc(nn, sizeof(Ipw), "ipw");
+  for (int i = 0; i < nn; i++) {
+    double v[12];
+    for (int k = 0; k < 12; k++)
+      if (fscanf(fp, "%lf", &v[k]) != 1) die("ipw.txt: failed to read a beam parameter.");
+    Ipw *w = &b->bd.ipw[i];
+    w->lambda0 = v[0]; w->ni = v[1]; w->power = v[2];
+    w->e0x = v[3] + I * v[4]; w->e0y = v[5] + I * v[6];
+    w->fx = v[7]; w->fy = v[8]; w->fz = v[9]; w->theta = v[10]; w->phi = v[11];
+  }
+  fclose(fp);
+}
+
+/* derived plane-wave data (ipw.c:65-89) and the common medium / wavelength
+ * check (multi_fbeam.c:878-962) */
+static void beams_setup(Bobj *b)
+{
+  if (b->n_ipw <= 0) { printf("no beams defined. check datafile name. Exit...\n"); exit(0); }
+  for (int i = 0; i < b->n_ipw; i++) {
+    Ipw *w = &b->bd.ipw[i];
+    if (i == 0) { b->n_0 = w->ni; b->lambda_0 = w->lambda0; }
+    else if (w->ni != b->n_0 || w->lambda0 != b->lambda_0) die("data error ipw ni or lambda0,");
+    double cs = 1.0 / sqrt(pow(cabs(w->e0x), 2) + pow(cabs(w->e0y), 2));
+    double complex ex = w->e0x * cs, ey = w->e0y * cs, hx = -ey * w->ni, hy = ex * w->ni;
+    w->data.E0 = sqrt(w->power * 2.0 / w->ni);
+    w->data.ki = 2.0 * M_PI * w->ni / w->lambda0;
+    double st = sin(w->theta), ct = cos(w->theta), sp = sin(w->phi), cp = cos(w->phi);
+    w->data.sin_t = st; w->data.cos_t = ct; w->data.sin_p = sp; w->data.cos_p = cp;
+    double a11 = sp * sp + cp * cp * ct, a12 = sp * cp * ct - sp * cp, a22 = sp * sp * ct + cp * cp;
+    w->data.ex = ex * a11 + ey * a12; w->data.ey = ex * a12 + ey * a22; w->data.ez = -(ex * cp + ey * sp) * st;
+    w->data.hx = hx * a11 + hy * a12; w->data.hy = hx * a12 + hy * a22; w->data.hz = -(hx * cp + hy * sp) * st;
+  }
+  b->omega = 2.0 * M_PI / b->lambda_0;
+}
+
+static void beam_params(const Bobj *b, double *par)
+{
+  for (int i = 0; i < b->n_ipw; i++) {
+    const Ipw *w = &b->bd.ipw[i];
+    double v[12] = {w->lambda0, w->ni, w->power, creal(w->e0x), cimag(w->e0x), creal(w->e0y), cimag(w->e0y),
+                    w->fx, w->fy, w->fz, w->theta, w->phi};
+    memcpy(par + 12 * i, v, sizeof v);
+  }
+}
+
+static void print_beams(const Bobj *b, int mksa)
+{
+  for (int i = 0; i < b->n_ipw; i++) {
+    const Ipw *w = &b->bd.ipw[i];
+    printf(" \"%s\" No.%02d -- plane wave%s --\n", b->fname_ipw, i, mksa ? ", MKSA system" : "");
+    printf("wave length of incident beam in vacuum      : %15.14g\n", w->lambda0);
+    printf("refractive index of surrounding             : %15.14g\n", w->ni);
+    printf("incident beam power per unit area           : %15.14g\n", w->power);
+    printf("x-component of polarization coefficient     :%7.6g+%7.6gi\n", creal(w->e0x), cimag(w->e0x));
+    printf("y-component of polarization coefficient     :%7.6g+%7.6gi\n", creal(w->e0y), cimag(w->e0y));
+    printf("x-component of translation vector           : %15.14g\n", w->fx);
+    printf("y-component of translation vector           : %15.14g\n", w->fy);
+    printf("z-component of translation vector           : %15.14g\n", w->fz);
+    printf("rotation parameter theta               [rad]: %15.14g\n", w->theta);
+    printf("rotation parameter phi                 [rad]: %15.14g\n", w->phi);
+  }
+  printf("\n");
+}
+
+/* ------------------------------------------------------------ tables */
+static void alloc_nodes(BOUD *bd)
+{
+  bd->rn = xalloc(bd->Nn + 1, sizeof(double *), "rn");
+  for (int i = 0; i <= bd->Nn; i++) bd->rn[i] = xalloc(3, sizeof(double), "rn[i]");
+}
+
+static double ***alloc3d(int n1, int n2, int n3)
+{
+  double ***p = xalloc(n1, sizeof(double **), "table");
+  for (int i = 0; i < n1; i++) {
+    p[i] = xalloc(n2, sizeof(double *), "table");
+    for (int j = 0; j < n2; j++) p[i][j] = xalloc(n3, sizeof(double), "table");
+  }
+  return p;
+}
+
+static double complex ***alloc3z(int n1, int n2, int n3)
+{
+  double complex ***p = xalloc(n1, sizeof(double complex **), "table");
+  for (int i = 0; i < n1; i++) {
+    p[i] = xalloc(n2, sizeof(double complex *), "table");
+    for (int j = 0; j < n2; j++) p[i][j] = xalloc(n3, sizeof(double complex), "table");
+  }
+  return p;
+}
+
+static void free3(void ***p, int n1, int n2)
+{
+  if (!p) return;
+  for (int i = 0; i < n1; i++) { for (int j = 0; j < n2; j++) free(p[i][j]); free(p[i]); }
+  free(p);
+}
+
+static void alloc_elems(BOUD *bd)
+{
+  int n = bd->Ne + 1;
+  bd->ed = xalloc(n, sizeof(int *), "ed");
+  bd->eni = xalloc(n, sizeof(int *), "eni");
+  for (int i = 0; i < n; i++) { bd->ed[i] = xalloc(4, sizeof(int), "ed[i]"); bd->eni[i] = xalloc(4, sizeof(int), "eni[i]"); }
+  bd->md = xalloc(n, sizeof(int), "md"); bd->sd = xalloc(n, sizeof(int), "sd"); bd->gd = xalloc(n, sizeof(int), "gd");
+  bd->cr = alloc3d(n, 3, 4); bd->cw = alloc3d(n, 3, 3);
+  bd->ren = alloc3d(n, 4, 3); bd->wen = alloc3d(n, 4, 3);
+  bd->Ui = alloc3z(n, 4, 4); bd->dUi = alloc3z(n, 4, 4); bd->Ui0 = alloc3z(n, 4, 4); bd->dUi0 = alloc3z(n, 4, 4);
+  bd->Uip = alloc3z(n, 4, 4);
+}
+
+static void free_elems(BOUD *bd)
+{
+  int n = bd->Ne + 1;
+  for (int i = 0; i < n; i++) { free(bd->ed[i]); free(bd->eni[i]); }
+  free(bd->ed); free(bd->eni); free(bd->md); free(bd->sd); free(bd->gd);
+  free3((void ***)bd->cr, n, 3); free3((void ***)bd->cw, n, 3);
+  free3((void ***)bd->ren, n, 4); free3((void ***)bd->wen, n, 4);
+  free3((void ***)bd->Ui, n, 4); free3((void ***)bd->dUi, n, 4); free3((void ***)bd->Ui0, n, 4);
+  free3((void ***)bd->dUi0, n, 4); free3((void ***)bd->Uip, n, 4);
+}
+
+static void alloc_subdomains(DOMD *md)
+{
+  BOUD *bd = &md->bd;
+  int *cnt = xalloc(md->MN + 1, sizeof(int), "count");
+  for (int i = 1; i <= bd->Ne; i++) { cnt[bd->md[i]]++; cnt[bd->sd[i]]++; }
+  bd->sb = xalloc(md->MN + 1, sizeof(SUBD), "sb");
+  for (int d = 0; d <= md->MN; d++) {
+    SUBD *s = &bd->sb[d];
+    s->Ne = cnt[d];
+    s->sid = xalloc(cnt[d] + 1, sizeof(int), "sid");
+    s->U = alloc3z(cnt[d] + 1, 4, 4); s->dU = alloc3z(cnt[d] + 1, 4, 4);
+    s->E = alloc3z(cnt[d] + 1, 4, 3); s->dE = alloc3z(cnt[d] + 1, 4, 3);
+    s->H = alloc3z(cnt[d] + 1, 4, 3); s->dH = alloc3z(cnt[d] + 1, 4, 3);
+  }
+  free(cnt);
+}
+
+static void free_subdomains(DOMD *md)
+{
+  for (int d = 0; d <= md->MN; d++) {
+    SUBD *s = &md->bd.sb[d];
+    int n = s->Ne + 1;
+    free(s->sid);
+    free3((void ***)s->U, n, 4); free3((void ***)s->dU, n, 4); free3((void ***)s->E, n, 4);
+    free3((void ***)s->dE, n, 4); free3((void ***)s->H, n, 4); free3((void ***)s->dH, n, 4);
+  }
+  free(md->bd.sb);
+}
+
+/* signed element lists per domain (setup2.c:817-835) */
+static void fill_sid(DOMD *md)
+{
+  for (int d = 0; d <= md->MN; d++) {
+    int c = 1;
+    for (int i = 1; i <= md->bd.Ne; i++) {
+      if (md->bd.md[i] == d) md->bd.sb[d].sid[c++] = i;
+      else if (md->bd.sd[i] == d) md->bd.sb[d].sid[c++] = -i;
+    }
+  }
+}
+
+/* rule tables of BOUD (setup2.c:719-748; quadrature nodes from the device
+ * library's tables are not needed on the host, Gauss-Legendre nodes are
+ * kept for callers that read them) */
+static void gl_host(int n, double *x, double *w)
+{
+  for (int i = 1; i <= (n + 1) / 2; i++) {
+    double z = cos(M_PI * (i - 0.25) / (n + 0.5)), pd;
+    for (;;) {
+      double p0 = 1.0, p1 = 0.0;
+      for (int j = 1; j <= n; j++) { double pm = p1; p1 = p0; p0 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * pm) / j; }
+      pd = n * (z * p0 - p1) / (z * z - 1.0);
+      double zo = z;
+      z = zo - p0 / pd;
+      if (fabs(z - zo) <= 3.0e-16) break;
+    }
+    x[i - 1] = -z; x[n - i] = z;
+    w[i - 1] = 2.0 / ((1.0 - z * z) * pd * pd); w[n - i] = w[i - 1];
+  }
+  if (n % 2) x[n / 2] = 0.0;
+}
+
+static void rule_tables(BOUD *bd)
+{
+  const double p44 = 0.5773502691896257645091, p49 = 0.7745966692414833770359;
+  const double z44[4] = {-p44, p44, p44, -p44}, e44[4] = {-p44, -p44, p44, p44};
+  for (int i = 0; i < 4; i++) { bd->zt_44[i] = z44[i]; bd->et_44[i] = e44[i]; bd->wt_44[i] = 1.0; }
+  const double z49[9] = {-p49, p49, p49, -p49, 0.0, p49, 0.0, -p49, 0.0}, e49[9] = {-p49, -p49, p49, p49, -p49, 0.0, p49, 0.0, 0.0};
+  const double w49[9] = {0.3086419753086419753086, 0.3086419753086419753086, 0.3086419753086419753086, 0.3086419753086419753086,
+                         0.4938271604938271604938, 0.4938271604938271604938, 0.4938271604938271604938, 0.4938271604938271604938,
+                         0.7901234567901234567901};
+  memcpy(bd->zt_49, z49, sizeof z49); memcpy(bd->et_49, e49, sizeof e49); memcpy(bd->wt_49, w49, sizeof w49);
+  const double z34[4] = {-0.2, 0.4, -0.2, 0.0}, e34[4] = {-0.3464101615137754587055, 0.0, 0.3464101615137754587055, 0.0};
+  const double w34[4] = {0.5208333333333333333333, 0.5208333333333333333333, 0.5208333333333333333333, -0.5625};
+  memcpy(bd->zt_34, z34, sizeof z34); memcpy(bd->et_34, e34, sizeof e34); memcpy(bd->wt_34, w34, sizeof w34);
+  const double a0 = 0.3480702390148154917985, a1 = 0.2052130961576726346557, a2 = 0.6028753385763033130539, a3 = 0.3554395089236065568357;
+  const double z37[7] = {-a0, 2 * a0, -a0, a1, -2 * a1, a1, 0.0}, e37[7] = {-a2, 0.0, a2, a3, 0.0, -a3, 0.0};
+  const double w37[7] = {0.1259391805448271525957, 0.1259391805448271525957, 0.1259391805448271525957, 0.1323941527885061807376,
+                         0.1323941527885061807376, 0.1323941527885061807376, 0.225};
+  memcpy(bd->zt_37, z37, sizeof z37); memcpy(bd->et_37, e37, sizeof e37); memcpy(bd->wt_37, w37, sizeof w37);
+  gl_host(GLN, bd->xli, bd->wli);
+  gl_host(GHN, bd->xhi, bd->whi);
+}
+
+/* element constants and node data of triangles (setup2.c:666-749, 838-869) */
+static void elem_const(BOUD *bd)
+{
+  for (int i = 1; i <= bd->Ne; i++) {
+    if (bd->ed[i][3] != 0) { printf("element %d is a quadrangle: this build supports linear triangles only. Exit...\n", i); exit(1); }
+    double p[3][3];
+    for (int j = 0; j < 3; j++) for (int d = 0; d < 3; d++) p[j][d] = bd->rn[bd->ed[i][j]][d];
+    for (int d = 0; d < 3; d++) {
+      bd->cr[i][d][0] = (p[0][d] + p[1][d] + p[2][d]) / 3.0;
+      bd->cr[i][d][1] = (-p[0][d] + 2.0 * p[1][d] - p[2][d]) / 3.0;
+      bd->cr[i][d][2] = (p[2][d] - p[0][d]) / sqrt(3.0);
+      bd->cr[i][d][3] = 0.0;
+    }
+    double u[3], v[3];
+    for (int d = 0; d < 3; d++) { u[d] = p[1][d] - p[0][d]; v[d] = p[2][d] - p[0][d]; }
+    for (int d = 0; d < 3; d++) {
+      int a = (d + 1) % 3, b = (d + 2) % 3;
+      bd->cw[i][d][0] = u[a] * v[b] - v[a] * u[b];
+      bd->cw[i][d][1] = bd->cw[i][d][2] = 0.0;
+    }
+    for (int j = 0; j < 4; j++)
+      for (int d = 0; d < 3; d++) {
+        bd->ren[i][j][d] = bd->cr[i][d][0] + bd->cr[i][d][1] * bd->zt_34[j] + bd->cr[i][d][2] * bd->et_34[j];
+        bd->wen[i][j][d] = bd->cw[i][d][0];
+      }
+  }
+}
+
+static void translate_obj(DOMD *md, const double *rv, double th, const double *tv)
+{
+  double nv[3] = {rv[0], rv[1], rv[2]};
+  vunit3(nv);
+  double st = sin(th), ct = cos(th), oc = 1.0 - ct;
+  double M[9] = {ct + nv[0] * nv[0] * oc, nv[0] * nv[1] * oc - nv[2] * st, nv[2] * nv[0] * oc + nv[1] * st,
+                 nv[0] * nv[1] * oc + nv[2] * st, ct + nv[1] * nv[1] * oc, nv[1] * nv[2] * oc - nv[0] * st,
+                 nv[2] * nv[0] * oc - nv[1] * st, nv[1] * nv[2] * oc + nv[0] * st, ct + nv[2] * nv[2] * oc};
+  for (int s = 1; s <= md->bd.Nn; s++) {
+    double *r = md->bd.rn[s], q[3];
+    for (int i = 0; i < 3; i++) q[i] = M[3 * i] * r[0] + M[3 * i + 1] * r[1] + M[3 * i + 2] * r[2] + tv[i];
+    r[0] = q[0]; r[1] = q[1]; r[2] = q[2];
+  }
+}
+
+/* device copy of the object (geometry, media, beams) */
+static bh_object *dev(DOMD *md)
+{
+  if (md->dev) return (bh_object *)md->dev;
+  BOUD *bd = &md->bd;
+  double *rn = xalloc((size_t)(bd->Nn + 1) * 3, sizeof(double), "rn");
+  int *ed = xalloc((size_t)(bd->Ne + 1) * 4, sizeof(int), "ed");
+  for (int i = 0; i <= bd->Nn; i++) for (int d = 0; d < 3; d++) rn[3 * i + d] = bd->rn[i][d];
+  for (int i = 0; i <= bd->Ne; i++) for (int j = 0; j < 4; j++) ed[4 * i + j] = bd->ed[i][j];
+  double *nm = xalloc((size_t)(md->MN + 1) * 2, sizeof(double), "n");
+  for (int i = 0; i <= md->MN; i++) { nm[2 * i] = creal(md->n[i]); nm[2 * i + 1] = cimag(md->n[i]); }
+  bh_object *o = bh_object_create(bd->Nn, rn, bd->Ne, ed, bd->md, bd->sd, md->MN, nm, md->mw.lambda_0);
+  free(rn); free(ed); free(nm);
+  if (!o) { printf("device object: %s. Exit...\n", bh_last_error()); exit(1); }
+  double *par = xalloc((size_t)md->mw.n_ipw * 12, sizeof(double), "beams");
+  beam_params(&md->mw, par);
+  bh_or_die(bh_object_set_beams(o, md->mw.n_ipw, par), "bh_object_set_beams");
+  free(par);
+  md->dev = o;
+  return o;
+}
+
+/* ------------------------------------------------------------ create_matrix side */
+void read_domd2(int argc, char **argv, DOMD *md)
+{
+  if (argc != 4 && argc != 11) {
+    printf("This program needs command line arguments as follows.\n");
+    printf("%s medium_datafile_name mesh_datafile_name output_datafile_name [rv_x rv_y rv_z theta tr_x tr_y tr_z ](optional)\n", argv[0]);
+    printf("rv : vector defining rotation axis, theta : rotation angle ( using Rodrigues' rotation formula ), tr : translation vector\n");
+    printf("Exit...\n");
+    exit(0);
+  }
+  memset(md, 0, sizeof *md);
+  beams_init(&md->mw);
+  beams_read(&md->mw);
+  /* medium file (setup2.c:153-193) */
+  FILE *fp = fopen(argv[1], "r");
+  if (!fp) { printf("Can not open the '%s' file. Exit...\n", argv[1]); exit(1); }
+  snprintf(md->med_fn, sizeof md->med_fn, "%s", argv[1]);
+  char buf[256];
+  int MN;
+  if (!fgets(buf, sizeof buf, fp) || !fgets(buf, sizeof buf, fp) || fscanf(fp, "%d\n", &MN) != 1 || !fgets(buf, sizeof buf, fp))
+    die("read_medium_data(): failed to read the header.");
+  md->MN = MN;
+  md->n = xalloc(MN + 1, sizeof(double complex), "n");
+  md->kn = xalloc(MN + 1, sizeof(double complex), "kn");
+  for (int i = 1; i <= MN; i++) {
+    double a, b;
+    if (fscanf(fp, "%lf %lf", &a, &b) != 2) die("read_medium_data(): failed to read the refractive index.");
+    md->n[i] = a + I * b;
+  }
+  fclose(fp);
+  /* Gmsh 2.2 ASCII mesh, paired faces (setup2.c:205-503) */
+  fp = fopen(argv[2], "r");
+  if (!fp) { printf("Can not open the '%s' file. Exit...\n", argv[2]); exit(1); }
+  snprintf(md->msh_fn, sizeof md->msh_fn, "%s", argv[2]);
+  double ver;
+  int ft, pr, Nn, Nel;
+  if (!fgets(buf, sizeof buf, fp) || fscanf(fp, "%lf %d %d", &ver, &ft, &pr) != 3) die("read_mesh_data(): bad header.");
+  if (ver != 2.2) { printf("This program supports mesh file version 2.2. Reading file version is %g. Exit...\n", ver); exit(1); }
+  if (ft != 0) die("This program supports 'ASCII' data format mesh file.");
+  if (pr != 8) die("This program supports double precision mesh data.");
+  if (fscanf(fp, " %255s %255s %d", buf, buf, &Nn) != 3) die("read_mesh_data(): failed to read the node count.");
+  BOUD *bd = &md->bd;
+  bd->Nn = Nn;
+  alloc_nodes(bd);
+  for (int i = 1; i <= Nn; i++) {
+    int id;
+    if (fscanf(fp, "%d %lf %lf %lf", &id, &bd->rn[i][0], &bd->rn[i][1], &bd->rn[i][2]) != 4) die("read_mesh_data(): failed to read a node.");
+    if (id != i) printf("bad id %d\n", id);
+  }
+  if (fscanf(fp, " %255s %255s %d", buf, buf, &Nel) != 3) die("read_mesh_data(): failed to read the element count.");
+  bd->Ne = Nel / 2;
+  alloc_elems(bd);
+  int nc = 0;
+  for (int i = 1; i <= bd->Ne; i++) {
+    int id, et, nt, tag;
+    if (fscanf(fp, "%d %d %d", &id, &et, &nt) != 3) die("read_mesh_data(): failed to read an element.");
+    if (id != 2 * i - 1) { printf("bad id :%d. Exit...\n", id); exit(1); }
+    if (et != 2 && et != 3) die("bad element type. element type must be 2 or 3.");
+    for (int j = 0; j < nt; j++) {
+      if (fscanf(fp, "%d", &tag) != 1) die("read_mesh_data(): failed to read a tag.");
+      if (j == 0) {
+        if (tag == 99) tag = 0;
+        if (tag > MN) { printf("domain id %d is not defined medium data. check domain and medium data. exit..\n", tag); exit(1); }
+        bd->md[i] = tag;
+      } else if (j == 1) bd->gd[i] = tag;
+    }
+    int nv = et == 2 ? 3 : 4;
+    for (int j = 0; j < nv; j++) if (fscanf(fp, "%d", &bd->ed[i][j]) != 1) die("read_mesh_data(): failed to read a node id.");
+    if (nv == 3) bd->ed[i][3] = 0;
+    for (int j = 0; j < 4; j++) bd->eni[i][j] = j < nv ? nc++ : -1;
+    int id2, et2, nt2;
+    if (fscanf(fp, "%d %d %d", &id2, &et2, &nt2) != 3) die("read_mesh_data(): failed to read an element.");
+    if (id2 != 2 * i) { printf("bad id :%d. Exit...\n", id2); exit(1); }
+    for (int j = 0; j < nt2; j++) {
+      if (fscanf(fp, "%d", &tag) != 1) die("read_mesh_data(): failed to read a tag.");
+      if (j == 0) {
+        if (tag == 99) tag = 0;
+        if (tag > MN) { printf("domain id %d is not defined medium data. check domain and medium data! exit..\n", tag); exit(1); }
+        bd->sd[i] = tag;
+      }
+    }
+    for (int j = 0; j < nv; j++) {
+      int v, want = j == 0 ? bd->ed[i][0] : bd->ed[i][nv - j];
+      if (fscanf(fp, "%d", &v) != 1) die("read_mesh_data(): failed to read a node id.");
+      if (v != want) { printf("node id miss matched. check element id %d. Exit...\n", 2 * i); exit(1); }
+    }
+    if (bd->sd[i] == 0) {
+      bd->sd[i] = bd->md[i];
+      bd->md[i] = 0;
+      if (nv == 3) { int t = bd->ed[i][1]; bd->ed[i][1] = bd->ed[i][2]; bd->ed[i][2] = t; }
+      else { int t = bd->ed[i][3]; bd->ed[i][3] = bd->ed[i][1]; bd->ed[i][1] = t; }
+    }
+  }
+  fclose(fp);
+  bd->NN = nc;
+  if (argc == 11) {
+    md->rv[0] = atof(argv[4]); md->rv[1] = atof(argv[5]); md->rv[2] = atof(argv[6]); md->th = atof(argv[7]);
+    md->tv[0] = atof(argv[8]); md->tv[1] = atof(argv[9]); md->tv[2] = atof(argv[10]);
+  } else {
+    md->rv[0] = 1.0;
+  }
+}
+
+static void print_medium_mesh(DOMD *md)
+{
+  printf("-- medium data --\n");
+  printf("medium data file name                  : %s\n", md->med_fn);
+  for (int i = 1; i <= md->MN; i++) printf("medium (domain) id %2d refractive index :%8.7g + %8.7gI\n", i, creal(md->n[i]), cimag(md->n[i]));
+  printf("\n-- mesh data --\n");
+  printf("mesh data file name    : %s\n", md->msh_fn);
+  printf("node number            : %8d\n", md->bd.Nn);
+  printf("defined element number : %8d\n\n", md->bd.Ne * 2);
+}
+
+void print_domd2(DOMD *md)
+{
+  printf("-- multi_fbeam data --\n");
+  print_beams(&md->mw, 0);
+  print_medium_mesh(md);
+}
+
+void print_domd2_mksa(DOMD *md) { print_domd2(md); }
+
+void initialize_domd2(DOMD *md)
+{
+  beams_setup(&md->mw);
+  md->n[0] = md->mw.n_0;
+  for (int i = 0; i <= md->MN; i++) md->kn[i] = 2.0 * M_PI / md->mw.lambda_0 * md->n[i];
+  if (md->th != 0.0 || vabs3(md->tv) != 0.0) translate_obj(md, md->rv, md->th, md->tv);
+  rule_tables(&md->bd);
+  elem_const(&md->bd);
+  alloc_subdomains(md);
+  fill_sid(md);
+}
+
+static void free_names(CMD *cm)
+{
+  if (!cm->tgfn) return;
+  for (int i = 0; i <= cm->MN; i++) { free(cm->tgfn[i]); free(cm->thfn[i]); free(cm->tdgfn[i]); free(cm->tdhfn[i]); free(cm->tdffn[i]); }
+  free(cm->tgfn); free(cm->thfn); free(cm->tdgfn); free(cm->tdhfn); free(cm->tdffn); free(cm->lupfn);
+  free(cm->aval); free(cm->aptr); free(cm->aidx); free(cm->b);
+  cm->tgfn = NULL;
+}
+
+void finalize_domd2(DOMD *md)
+{
+  if (md->dev) { bh_object_destroy((bh_object *)md->dev); md->dev = NULL; }
+  free_subdomains(md);
+  free_elems(&md->bd);
+  for (int i = 0; i <= md->bd.Nn; i++) free(md->bd.rn[i]);
+  free(md->bd.rn);
+  free(md->n); free(md->kn);
+  free(md->mw.bd.ipw);
+  free_names(&md->cm);
+}
+
+int domain_id(double *rt, DOMD *md)
+{
+  int did = -1;
+  bh_or_die(bh_object_domain(dev(md), 1, rt, &did), "domain_id");
+  return did;
+}
+
+static void init_names(CMD *cm, int MN)
+{
+  cm->MN = MN;
+  cm->tgfn = xalloc(MN + 1, sizeof(char *), "names"); cm->thfn = xalloc(MN + 1, sizeof(char *), "names");
+  cm->tdgfn = xalloc(MN + 1, sizeof(char *), "names"); cm->tdhfn = xalloc(MN + 1, sizeof(char *), "names");
+  cm->tdffn = xalloc(MN + 1, sizeof(char *), "names");
+  for (int i = 0; i <= MN; i++) {
+    cm->tgfn[i] = xalloc(128, 1, "name"); cm->thfn[i] = xalloc(128, 1, "name"); cm->tdgfn[i] = xalloc(128, 1, "name");
+    cm->tdhfn[i] = xalloc(128, 1, "name"); cm->tdffn[i] = xalloc(128, 1, "name");
+    snprintf(cm->tgfn[i], 128, "tmpG_%05d.cmat", i); snprintf(cm->thfn[i], 128, "tmpH_%05d.cmat", i);
+    snprintf(cm->tdgfn[i], 128, "tmpdG_%05d.cmat", i); snprintf(cm->tdhfn[i], 128, "tmpdH_%05d.cmat", i);
+    snprintf(cm->tdffn[i], 128, "tmpdF_%05d.cmat", i);
+  }
+  cm->lupfn = xalloc(128, 1, "name");
+  snprintf(cm->lupfn, 128, "tmpinvA.cmat");
+  cm->aval = xalloc(16, 1, "name"); cm->aptr = xalloc(16, 1, "name"); cm->aidx = xalloc(16, 1, "name"); cm->b = xalloc(16, 1, "name");
+  snprintf(cm->aval, 16, "tmpAval.dat"); snprintf(cm->aptr, 16, "tmpAprt.dat"); snprintf(cm->aidx, 16, "tmpAidx.dat");
+  snprintf(cm->b, 16, "tmpB.dat");
+}
+
+/* coefficient matrices of domain d in the reference's padded layout
+ * (solve_bieq2.c:168-183), written only on request */
+static void write_cmat(DOMD *md, int d, const char *gfn, const char *hfn, const char *dgfn, const char *dhfn, const char *dffn)
+{
+  bh_object *o = dev(md);
+  int info[20];
+  bh_object_info(o, info);
+  size_t nd = (size_t)info[12 + d], Ne = (size_t)md->bd.sb[d].Ne, N = 4 * Ne;
+  double complex *c = xalloc(2 * nd * nd, sizeof(double complex), "matrix");
+  double complex *pad = xalloc(2 * N * N, sizeof(double complex), "matrix");
+  double *f = xalloc(3 * nd, sizeof(double), "dF"), *fp3 = xalloc(3 * N, sizeof(double), "dF");
+  const char *names[4] = {gfn, hfn, dgfn, dhfn};
+  for (int w = 0; w < 4; w++) {
+    bh_or_die(bh_object_matrix(o, d, w, (double *)c), "bh_object_matrix");
+    size_t nr = w < 2 ? 1 : 2;
+    memset(pad, 0, nr * N * N * sizeof(double complex));
+    for (size_t r = 0; r < nd; r++)
+      for (size_t q = 0; q < nr; q++)
+        for (size_t j = 0; j < nd; j++)
+          pad[((r / 3 * 4 + r % 3) * nr + q) * N + j / 3 * 4 + j % 3] = c[(r * nr + q) * nd + j];
+    FILE *fo = fopen(names[w], "wb");
+    if (!fo) { printf("Failed to create %s file.\n", names[w]); exit(1); }
+    fwrite(pad, sizeof(double complex), nr * N * N, fo);
+    fclose(fo);
+  }
+  bh_or_die(bh_object_matrix(o, d, 4, f), "bh_object_matrix");
+  for (size_t r = 0; r < nd; r++) for (int q = 0; q < 3; q++) fp3[(r / 3 * 4 + r % 3) * 3 + q] = f[3 * r + q];
+  FILE *fo = fopen(dffn, "wb");
+  if (!fo) { printf("Failed to create %s file.\n", dffn); exit(1); }
+  fwrite(fp3, sizeof(double), 3 * N, fo);
+  fclose(fo);
+  free(c); free(pad); free(f); free(fp3);
+}
+
+/* .obj file of create_matrix (dat_write2, solve_bieq2.c:439-502) */
+static void write_obj(const char *fname, DOMD *md)
+{
+  FILE *fp = fopen(fname, "wb");
+  if (!fp) { printf("dat_write2(), Failed to create the %s file.\n", fname); exit(1); }
+  BOUD *bd = &md->bd;
+  fwrite(md->med_fn, 1, 128, fp); fwrite(md->msh_fn, 1, 128, fp);
+  fwrite(md->rv, sizeof(double), 3, fp); fwrite(&md->th, sizeof(double), 1, fp); fwrite(md->tv, sizeof(double), 3, fp);
+  fwrite(&md->MN, sizeof(int), 1, fp);
+  fwrite(md->n, sizeof(double complex), md->MN + 1, fp); fwrite(md->kn, sizeof(double complex), md->MN + 1, fp);
+  fwrite(&md->mw, sizeof(Bobj), 1, fp);
+  fwrite(md->mw.bd.ipw, sizeof(Ipw), md->mw.n_ipw, fp);
+  fwrite(&bd->Nn, sizeof(int), 1, fp); fwrite(&bd->Ne, sizeof(int), 1, fp);
+  for (int i = 0; i <= bd->Nn; i++) fwrite(bd->rn[i], sizeof(double), 3, fp);
+  for (int i = 0; i <= bd->Ne; i++) fwrite(bd->ed[i], sizeof(int), 4, fp);
+  fwrite(&bd->NN, sizeof(int), 1, fp);
+  for (int i = 0; i <= bd->Ne; i++) fwrite(bd->eni[i], sizeof(int), 4, fp);
+  fwrite(bd->md, sizeof(int), bd->Ne + 1, fp); fwrite(bd->sd, sizeof(int), bd->Ne + 1, fp); fwrite(bd->gd, sizeof(int), bd->Ne + 1, fp);
+  for (int i = 0; i <= bd->Ne; i++) for (int j = 0; j < 4; j++) fwrite(bd->ren[i][j], sizeof(double), 3, fp);
+  for (int i = 0; i <= bd->Ne; i++) for (int j = 0; j < 4; j++) fwrite(bd->wen[i][j], sizeof(double), 3, fp);
+  for (int d = 0; d <= md->MN; d++) { fwrite(&bd->sb[d].Ne, sizeof(int), 1, fp); fwrite(bd->sb[d].sid, sizeof(int), bd->sb[d].Ne + 1, fp); }
+  /* stored names: "<prefix>_G_00000.cmat" ..., prefix = fname up to its
+   * first '.' (solve_bieq2.c:486-499); fields are 128 bytes, zero padded */
+  char pfn[100] = {0};
+  for (int i = 0; i < 99 && fname[i] && fname[i] != '.'; i++) pfn[i] = fname[i];
+  int wcm = getenv("BEM3_WRITE_CMAT") && atoi(getenv("BEM3_WRITE_CMAT")) == 1;
+  for (int d = 0; d <= md->MN; d++) {
+    char *nm[5] = {md->cm.tgfn[d], md->cm.thfn[d], md->cm.tdgfn[d], md->cm.tdhfn[d], md->cm.tdffn[d]};
+    char full[5][128];
+    for (int k = 0; k < 5; k++) {
+      memset(full[k], 0, 128);
+      snprintf(full[k], 128, "%s_%.24s", pfn, nm[k] + 3);
+    }
+    if (wcm) write_cmat(md, d, full[0], full[1], full[2], full[3], full[4]);
+    for (int k = 0; k < 5; k++) fwrite(full[k], 1, 128, fp);
+  }
+  char lu[128] = {0};
+  snprintf(lu, 128, "%s_%.24s", pfn, md->cm.lupfn + 3);
+  fwrite(lu, 1, 128, fp);
+  fclose(fp);
+}
+
+void create_matrix(DOMD *md, char *ofn)
+{
+  time_t t0, t1, ms, me;
+  printf("\ncreate matrix data \n");
+  time(&t0);
+  printf("  coefficient matrix          "); fflush(stdout);
+  time(&ms);
+  md->cm.type = PREC_DEF;
+  init_names(&md->cm, md->MN);
+  bh_object *o = dev(md);
+  bh_or_die(bh_object_assemble(o, PREC_DEF), "bh_object_assemble");
+  time(&me);
+  printf("finished. Elapsed time : %5g (sec)\n", difftime(me, ms));
+  printf("  inverse matrix              "); fflush(stdout);
+  time(&ms);
+  md->cm.nn = (size_t)md->bd.NN;
+  md->cm.na = md->cm.nn * 8;
+  bh_or_die(bh_object_factor(o, 0), "bh_object_factor");
+  time(&me);
+  printf("finished. Elapsed time : %5g (sec)\n", difftime(me, ms));
+  write_obj(ofn, md);
+  time(&t1);
+  printf("Total elapsed time : %g (sec)\n", difftime(t1, t0));
+}
+
+/* ------------------------------------------------------------ bv_solver side */
+static void mo_alloc(MOBJ *mo)
+{
+  mo->md = xalloc(mo->N, sizeof(DOMD), "md");
+  mo->mod_fn = xalloc(mo->N, sizeof(char *), "mod_fn");
+  mo->rs = xalloc(mo->N, sizeof(double *), "rs");
+  for (int i = 0; i < mo->N; i++) { mo->mod_fn[i] = xalloc(FNLGTH, 1, "mod_fn"); mo->rs[i] = xalloc(3, sizeof(double), "rs"); }
+}
+
+static void rd(void *p, size_t sz, size_t n, FILE *fp, const char *what)
+{
+  if (fread(p, sz, n, fp) != n) { printf("failed to read the %s. exit...\n", what); exit(1); }
+}
+
+/* geometry part shared by .obj and .dat records */
+static void read_geometry(FILE *fp, DOMD *md)
+{
+  rd(md->med_fn, 1, 128, fp, "med_fn"); rd(md->msh_fn, 1, 128, fp, "msh_fn");
+  rd(md->rv, sizeof(double), 3, fp, "rv"); rd(&md->th, sizeof(double), 1, fp, "th"); rd(md->tv, sizeof(double), 3, fp, "tv");
+  rd(&md->MN, sizeof(int), 1, fp, "MN");
+  md->n = xalloc(md->MN + 1, sizeof(double complex), "n"); md->kn = xalloc(md->MN + 1, sizeof(double complex), "kn");
+  rd(md->n, sizeof(double complex), md->MN + 1, fp, "n"); rd(md->kn, sizeof(double complex), md->MN + 1, fp, "kn");
+  rd(&md->mw, sizeof(Bobj), 1, fp, "Bobj");
+  if (md->mw.n_fpw || md->mw.n_lgb || md->mw.n_bsb || md->mw.n_blg || md->mw.n_rab)
+    die("stored beams other than plane waves are not supported by this build.");
+  md->mw.bd.ipw = xalloc(md->mw.n_ipw, sizeof(Ipw), "ipw");
+  rd(md->mw.bd.ipw, sizeof(Ipw), md->mw.n_ipw, fp, "ipw");
+  beams_setup(&md->mw);
+  BOUD *bd = &md->bd;
+  rd(&bd->Nn, sizeof(int), 1, fp, "Nn"); rd(&bd->Ne, sizeof(int), 1, fp, "Ne");
+  alloc_nodes(bd);
+  alloc_elems(bd);
+  for (int i = 0; i <= bd->Nn; i++) rd(bd->rn[i], sizeof(double), 3, fp, "rn");
+  for (int i = 0; i <= bd->Ne; i++) rd(bd->ed[i], sizeof(int), 4, fp, "ed");
+  rd(&bd->NN, sizeof(int), 1, fp, "NN");
+  for (int i = 0; i <= bd->Ne; i++) rd(bd->eni[i], sizeof(int), 4, fp, "eni");
+  rd(bd->md, sizeof(int), bd->Ne + 1, fp, "md"); rd(bd->sd, sizeof(int), bd->Ne + 1, fp, "sd"); rd(bd->gd, sizeof(int), bd->Ne + 1, fp, "gd");
+  for (int i = 0; i <= bd->Ne; i++) for (int j = 0; j < 4; j++) rd(bd->ren[i][j], sizeof(double), 3, fp, "ren");
+  for (int i = 0; i <= bd->Ne; i++) for (int j = 0; j < 4; j++) rd(bd->wen[i][j], sizeof(double), 3, fp, "wen");
+  rule_tables(bd);
+  alloc_subdomains(md);
+}
+
+static void read_obj(const char *fname, DOMD *md)
+{
+  FILE *fp = fopen(fname, "rb");
+  if (!fp) { printf("dat_read2(), Failed to open the %s file.\n", fname); exit(1); }
+  memset(md, 0, sizeof *md);
+  read_geometry(fp, md);
+  for (int d = 0; d <= md->MN; d++) {
+    int t;
+    rd(&t, sizeof(int), 1, fp, "sub-domain Ne");
+    rd(md->bd.sb[d].sid, sizeof(int), md->bd.sb[d].Ne + 1, fp, "sid");
+  }
+  init_names(&md->cm, md->MN);
+  for (int d = 0; d <= md->MN; d++) {
+    rd(md->cm.tgfn[d], 1, 128, fp, "G name"); rd(md->cm.thfn[d], 1, 128, fp, "H name");
+    rd(md->cm.tdgfn[d], 1, 128, fp, "dG name"); rd(md->cm.tdhfn[d], 1, 128, fp, "dH name");
+    rd(md->cm.tdffn[d], 1, 128, fp, "dF name");
+  }
+  rd(md->cm.lupfn, 1, 128, fp, "LU name");
+  md->cm.nn = (size_t)md->bd.NN;
+  md->cm.na = 8 * md->cm.nn;
+  fclose(fp);
+}
+
+void mo_initialize(int argc, char **argv, MOBJ *mo)
+{
+  if (argc != 3) {
+    printf("This program needs command line argument as follows.\n");
+    printf("%s object_setting_file output_data_file_name\n", argv[0]);
+    exit(0);
+  }
+  FILE *fp = fopen(argv[1], "r");
+  if (!fp) { printf("mo_read_data(),Can not open the %s file.\n", argv[1]); exit(1); }
+  char tmp[FNLGTH];
+  int n, flg;
+  if (!fgets(tmp, FNLGTH, fp) || fscanf(fp, "%d %d\n", &n, &flg) != 2 || !fgets(tmp, FNLGTH, fp))
+    die("mo_initialize(): failed to read the object settings.");
+  mo->N = n;
+  mo_alloc(mo);
+  for (int i = 0; i < n; i++) {
+    if (fscanf(fp, "%255s %lf %lf %lf", mo->mod_fn[i], &mo->rs[i][0], &mo->rs[i][1], &mo->rs[i][2]) != 4)
+      die("mo_initialize(): failed to read the mod_fn,rs.");
+    DOMD *md = &mo->md[i];
+    read_obj(mo->mod_fn[i], md);
+    if (flg == 1) {  /* renew beams from the working directory's ipw.txt */
+      double lam = md->mw.lambda_0, n0 = md->mw.n_0;
+      free(md->mw.bd.ipw);
+      beams_init(&md->mw);
+      beams_read(&md->mw);
+      beams_setup(&md->mw);
+      if (lam != md->mw.lambda_0 || n0 != md->mw.n_0) die("renew_beam_data(), wave number check error!");
+    }
+    translate_obj(md, md->rv, 0.0, mo->rs[i]);
+    elem_const(&md->bd);
+    bh_object *o = dev(md);
+    bh_or_die(bh_object_assemble(o, PREC_DEF), "bh_object_assemble");
+    bh_or_die(bh_object_factor(o, 0), "bh_object_factor");
+  }
+  fclose(fp);
+}
+
+void mo_print_data(MOBJ *mo)
+{
+  printf("-- multi_fbeam --\n");
+  print_beams(&mo->md[0].mw, 0);
+  for (int i = 0; i < mo->N; i++) {
+    printf("********** object %d **********\n", i);
+    printf("data file name                : %s\n", mo->mod_fn[i]);
+    printf("additional translation vector : (% 8.7g,% 8.7g,% 8.7g)\n", mo->rs[i][0], mo->rs[i][1], mo->rs[i][2]);
+    print_medium_mesh(&mo->md[i]);
+  }
+  printf("\n");
+}
+
+void mo_print_data_mksa(MOBJ *mo) { mo_print_data(mo); }
+
+void mo_finalize(MOBJ *mo)
+{
+  for (int i = 0; i < mo->N; i++) { finalize_domd2(&mo->md[i]); free(mo->mod_fn[i]); free(mo->rs[i]); }
+  free(mo->md); free(mo->mod_fn); free(mo->rs);
+}
+
+/* device boundary values -> the caller-visible tables */
+static void download_state(DOMD *md)
+{
+  bh_object *o = dev(md);
+  for (int d = 0; d <= md->MN; d++) {
+    SUBD *s = &md->bd.sb[d];
+    size_t n1 = (size_t)s->Ne + 1;
+    double complex *buf = xalloc(n1 * 16, sizeof(double complex), "state");
+    double complex ****tab[6] = {&s->U, &s->dU, &s->E, &s->dE, &s->H, &s->dH};
+    for (int w = 0; w < 6; w++) {
+      int nc = w < 2 ? 4 : 3;
+      bh_or_die(bh_object_state(o, d, w, (double *)buf, 0), "bh_object_state");
+      for (size_t i = 0; i < n1; i++) for (int j = 0; j < 4; j++) for (int c = 0; c < nc; c++) (*tab[w])[i][j][c] = buf[(i * 4 + j) * nc + c];
+    }
+    free(buf);
+  }
+  double complex *ui = xalloc((size_t)(md->bd.Ne + 1) * 16, sizeof(double complex), "incident");
+  double complex ****itab[4] = {&md->bd.Ui, &md->bd.dUi, &md->bd.Ui0, &md->bd.dUi0};
+  for (int w = 0; w < 4; w++) {
+    bh_or_die(bh_object_incident(o, w, (double *)ui, 0), "bh_object_incident");
+    for (int i = 0; i <= md->bd.Ne; i++) for (int j = 0; j < 4; j++) for (int c = 0; c < 4; c++) (*itab[w])[i][j][c] = ui[(i * 4 + j) * 4 + c];
+  }
+  free(ui);
+}
+
+static void upload_state(DOMD *md)
+{
+  bh_object *o = dev(md);
+  for (int d = 0; d <= md->MN; d++) {
+    SUBD *s = &md->bd.sb[d];
+    size_t n1 = (size_t)s->Ne + 1;
+    double complex *buf = xalloc(n1 * 16, sizeof(double complex), "state");
+    double complex ***tab[6] = {s->U, s->dU, s->E, s->dE, s->H, s->dH};
+    for (int w = 0; w < 6; w++) {
+      int nc = w < 2 ? 4 : 3;
+      for (size_t i = 0; i < n1; i++) for (int j = 0; j < 4; j++) for (int c = 0; c < nc; c++) buf[(i * 4 + j) * nc + c] = tab[w][i][j][c];
+      bh_or_die(bh_object_state(o, d, w, (double *)buf, 1), "bh_object_state");
+    }
+    free(buf);
+  }
+}
+
+static bh_object **handles(MOBJ *mo)
+{
+  bh_object **h = xalloc(mo->N, sizeof(bh_object *), "handles");
+  for (int m = 0; m < mo->N; m++) h[m] = dev(&mo->md[m]);
+  return h;
+}
+
+void mo_solve_bv(MOBJ *mo)
+{
+  time_t ms, me;
+  bh_object **h = handles(mo);
+  double *trace = xalloc(ITER_MAX + 1, sizeof(double), "trace");
+  int iters = 0;
+  printf(mo->N == 1 ? "\nsingle object\n" : "\ninitialize boundary value\n");
+  time(&ms);
+  bh_or_die(bh_mo_solve(h, mo->N, CCD, ITER_MIN, ITER_MAX, trace, &iters), "bh_mo_solve");
+  time(&me);
+  if (mo->N > 1) {
+    printf("iterative operation start. convergence criterion cc < %g and count > %d\n", CCD, ITER_MIN);
+    for (int it = 2; it <= iters && it < ITER_MAX; it++) printf("count =% 3d, cc = %g\n", it, trace[it]);
+    if (iters == ITER_MAX) printf("The maximum number of iterations has been reached (The result has not converged).\n");
+  }
+  printf("finished. Elapsed time : %5g (sec)\n", difftime(me, ms));
+  for (int m = 0; m < mo->N; m++) download_state(&mo->md[m]);
+  free(trace);
+  free(h);
+}
+
+void mo_dat_write(char *fname, MOBJ *mo)
+{
+  FILE *fp = fopen(fname, "wb");
+  if (!fp) { printf("mo_dat_write(), Failed to create the %s file.\n", fname); exit(1); }
+  fwrite(&mo->N, sizeof(int), 1, fp);
+  for (int m = 0; m < mo->N; m++) {
+    DOMD *md = &mo->md[m];
+    BOUD *bd = &md->bd;
+    fwrite(mo->mod_fn[m], 1, FNLGTH, fp);
+    fwrite(mo->rs[m], sizeof(double), 3, fp);
+    fwrite(md->med_fn, 1, 128, fp); fwrite(md->msh_fn, 1, 128, fp);
+    fwrite(md->rv, sizeof(double), 3, fp); fwrite(&md->th, sizeof(double), 1, fp); fwrite(md->tv, sizeof(double), 3, fp);
+    fwrite(&md->MN, sizeof(int), 1, fp);
+    fwrite(md->n, sizeof(double complex), md->MN + 1, fp); fwrite(md->kn, sizeof(double complex), md->MN + 1, fp);
+    fwrite(&md->mw, sizeof(Bobj), 1, fp);
+    fwrite(md->mw.bd.ipw, sizeof(Ipw), md->mw.n_ipw, fp);
+    fwrite(&bd->Nn, sizeof(int), 1, fp); fwrite(&bd->Ne, sizeof(int), 1, fp);
+    for (int i = 0; i <= bd->Nn; i++) fwrite(bd->rn[i], sizeof(double), 3, fp);
+    for (int i = 0; i <= bd->Ne; i++) fwrite(bd->ed[i], sizeof(int), 4, fp);
+    fwrite(&bd->NN, sizeof(int), 1, fp);
+    for (int i = 0; i <= bd->Ne; i++) fwrite(bd->eni[i], sizeof(int), 4, fp);
+    fwrite(bd->md, sizeof(int), bd->Ne + 1, fp); fwrite(bd->sd, sizeof(int), bd->Ne + 1, fp); fwrite(bd->gd, sizeof(int), bd->Ne + 1, fp);
+    for (int i = 0; i <= bd->Ne; i++) for (int j = 0; j < 4; j++) fwrite(bd->ren[i][j], sizeof(double), 3, fp);
+    for (int i = 0; i <= bd->Ne; i++) for (int j = 0; j < 4; j++) fwrite(bd->wen[i][j], sizeof(double), 3, fp);
+    for (int d = 0; d <= md->MN; d++) {
+      SUBD *s = &bd->sb[d];
+      fwrite(&s->Ne, sizeof(int), 1, fp);
+      fwrite(s->sid, sizeof(int), s->Ne + 1, fp);
+      double complex ***tab[6] = {s->U, s->dU, s->E, s->dE, s->H, s->dH};
+      for (int w = 0; w < 6; w++)
+        for (int i = 0; i <= s->Ne; i++) for (int j = 0; j < 4; j++) fwrite(tab[w][i][j], sizeof(double complex), w < 2 ? 4 : 3, fp);
+    }
+  }
+  fclose(fp);
+}
+
+void mo_dat_read(char *fname, MOBJ *mo)
+{
+  FILE *fp = fopen(fname, "rb");
+  if (!fp) { printf("mo_dat_read(), Failed to open the %s file.\n", fname); exit(1); }
+  rd(&mo->N, sizeof(int), 1, fp, "N");
+  mo_alloc(mo);
+  for (int m = 0; m < mo->N; m++) {
+    DOMD *md = &mo->md[m];
+    rd(mo->mod_fn[m], 1, FNLGTH, fp, "mod_fn");
+    rd(mo->rs[m], sizeof(double), 3, fp, "rs");
+    memset(md, 0, sizeof *md);
+    read_geometry(fp, md);
+    elem_const(&md->bd);
+    for (int d = 0; d <= md->MN; d++) {
+      SUBD *s = &md->bd.sb[d];
+      int t;
+      rd(&t, sizeof(int), 1, fp, "sub-domain Ne");
+      rd(s->sid, sizeof(int), s->Ne + 1, fp, "sid");
+      double complex ***tab[6] = {s->U, s->dU, s->E, s->dE, s->H, s->dH};
+      for (int w = 0; w < 6; w++)
+        for (int i = 0; i <= s->Ne; i++) for (int j = 0; j < 4; j++) rd(tab[w][i][j], sizeof(double complex), w < 2 ? 4 : 3, fp, "state");
+    }
+    init_names(&md->cm, md->MN);
+    md->cm.nn = (size_t)md->bd.NN;
+    md->cm.na = 8 * md->cm.nn;
+    upload_state(md);
+  }
+  fclose(fp);
+}
+
+void mo_output_node_particles(char *fname, MOBJ *mo)
+{
+  char fo[256] = {0}, tf[200] = {0};
+  size_t s1 = strlen(fname);
+  if (s1 > 200) die("mo_output_node_particles(), file name is too long.");
+  snprintf(fo, sizeof fo, "%s", fname);
+  char *sd = strrchr(fo, '.');
+  if (sd) { strncpy(tf, fname, s1 - strlen(sd)); snprintf(fo, sizeof fo, "%s.particles", tf); }
+  FILE *fp = fopen(fo, "w");
+  if (!fp) { printf("Failed to create the %s file.\n", fo); exit(1); }
+  fprintf(fp, "# x y z object_id\n");
+  for (int o = 0; o < mo->N; o++)
+    for (int i = 1; i <= mo->md[o].bd.Ne; i++)
+      for (int j = 0; j < 4; j++)
+        fprintf(fp, "%15.14e %15.14e %15.14e %d\n", mo->md[o].bd.ren[i][j][0], mo->md[o].bd.ren[i][j][1], mo->md[o].bd.ren[i][j][2], o);
+  fclose(fp);
+}
+
+/* ------------------------------------------------------------ analysis */
+void mo_object_domain_id(int *oid, int *did, double *rt, MOBJ *mo)
+{
+  for (int o = 0; o < mo->N; o++) {
+    bh_or_die(bh_object_domain(dev(&mo->md[o]), 1, rt, did), "domain_id");
+    if (*did != 0) { *oid = o; return; }
+  }
+  *oid = -1;
+}
+
+static int field1(double complex *out, double *rt, int type, MOBJ *mo, int what, int mode)
+{
+  bh_object **h = handles(mo);
+  int oid;
+  bh_or_die(bh_mo_field(h, mo->N, what, mode, type, 1, rt, (double *)out, &oid), "bh_mo_field");
+  free(h);
+  return oid;
+}
+
+int mo_mEMP_s(double complex *U, double *rt, int type, MOBJ *mo) { return field1(U, rt, type, mo, 1, 0); }
+int mo_mEMP_t(double complex *U, double *rt, int type, MOBJ *mo) { return field1(U, rt, type, mo, 1, 1); }
+int mo_mEMP_i(double complex *U, double *rt, int type, MOBJ *mo) { return field1(U, rt, type, mo, 1, 2); }
+
+static int eh1(double complex *E, double complex *H, double *rt, int type, MOBJ *mo, int mode)
+{
+  double complex o[6];
+  int oid = field1(o, rt, type, mo, 0, mode);
+  for (int i = 0; i < 3; i++) { E[i] = o[i]; H[i] = o[3 + i]; }
+  return oid;
+}
+
+int mo_EH_s(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 0); }
+int mo_EH_t(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 1); }
+int mo_EH_i(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 2); }
+
+static int eh_batch(double complex *E, double complex *H, const double *rt, int np, int type, MOBJ *mo, int *oid, int mode)
+{
+  bh_object **h = handles(mo);
+  double complex *o = xalloc((size_t)np * 6, sizeof(double complex), "fields");
+  int *id = oid ? oid : xalloc(np, sizeof(int), "oid");
+  int rc = bh_mo_field(h, mo->N, 0, mode, type, np, rt, (double *)o, id);
+  if (rc == 0)
+    for (int p = 0; p < np; p++) for (int i = 0; i < 3; i++) { E[3 * p + i] = o[6 * p + i]; H[3 * p + i] = o[6 * p + 3 + i]; }
+  free(o); free(h);
+  if (!oid) free(id);
+  return rc;
+}
+
+int mo_EH_t_batch(double complex *E, double complex *H, const double *rt, int np, int type, MOBJ *mo, int *oid) { return eh_batch(E, H, rt, np, type, mo, oid, 1); }
+int mo_EH_s_batch(double complex *E, double complex *H, const double *rt, int np, int type, MOBJ *mo, int *oid) { return eh_batch(E, H, rt, np, type, mo, oid, 0); }
+
+/* fields from the potentials' gradients: not provided by the device build
+ * yet -- stops instead of computing them on the host */
+static int mEMP_unavailable(void)
+{
+  printf("mo_EH_mEMP_*(): the derivative-field formulation is not provided by this build. Exit...\n");
+  exit(1);
+}
+int mo_EH_mEMP_s(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { (void)E; (void)H; (void)rt; (void)type; (void)mo; return mEMP_unavailable(); }
+int mo_EH_mEMP_t(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { (void)E; (void)H; (void)rt; (void)type; (void)mo; return mEMP_unavailable(); }
+int mo_EH_mEMP_i(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return mo_EH_i(E, H, rt, type, mo); }
+
+static void eh_bd(double complex *E, double complex *H, int oid, int did, int t, double zt, double et, int type, MOBJ *mo, int mode)
+{
+  bh_object **h = handles(mo);
+  double ze[2] = {zt, et};
+  double complex o[6];
+  (void)type;
+  bh_or_die(bh_mo_EH_bd(h, mo->N, oid, did, 1, &t, ze, PREC_DEF, mode, (double *)o), "bh_mo_EH_bd");
+  for (int i = 0; i < 3; i++) { E[i] = o[i]; H[i] = o[3 + i]; }
+  free(h);
+}
+
+void mo_EH_s_bd(double complex *E, double complex *H, int oid, int did, int t, double zeta_t, double eta_t, int type, MOBJ *mo) { eh_bd(E, H, oid, did, t, zeta_t, eta_t, type, mo, 0); }
+void mo_EH_t_bd(double complex *E, double complex *H, int oid, int did, int t, double zeta_t, double eta_t, int type, MOBJ *mo) { eh_bd(E, H, oid, did, t, zeta_t, eta_t, type, mo, 1); }
+void mo_EH_i_bd(double complex *E, double complex *H, int oid, int did, int t, double zeta_t, double eta_t, int type, MOBJ *mo) { eh_bd(E, H, oid, did, t, zeta_t, eta_t, type, mo, 2); }
+void mo_EH_a_bd(double complex *E, double complex *H, int oid, int did, int t, double zeta_t, double eta_t, int type, MOBJ *mo) { eh_bd(E, H, oid, did, t, zeta_t, eta_t, type, mo, 3); }
+
+int mo_force_FN(double *F, double *N, double *rc, int oid, int type, MOBJ *mo)
+{
+  bh_object **h = handles(mo);
+  int r = bh_mo_force(h, mo->N, oid, type, rc, F, N);
+  free(h);
+  if (r != 0 && r != -1) bh_or_die(r, "bh_mo_force");
+  return r;
+}
+
+int mo_absorb_P(double *P, int oid, int type, MOBJ *mo)
+{
+  bh_object **h = handles(mo);
+  int r = bh_mo_absorb(h, mo->N, oid, type, P);
+  free(h);
+  if (r != 0 && r != -1 && r != -2) bh_or_die(r, "bh_mo_absorb");
+  return r;
+}

@@ -1,0 +1,122 @@
+/*
+ * bem3_hip.h -- C-ABI of the MI355X (gfx950) implementation of the
+ * bem3_emf_b2 hot path: coefficient-matrix assembly, the dense complex
+ * LU / solve, the multi-object coupling loop, field evaluation and the
+ * Maxwell-stress force.  Plain C types only; complex numbers cross as
+ * interleaved (re, im) doubles.  Tables indexed by element use the
+ * reference's 1-based layout with an unused row 0.
+ *
+ * Every int-returning entry point returns 0 on success or a negative BH_E*
+ * code; bh_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference tree).
+ */
+#ifndef BEM3_HIP_H
+#define BEM3_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_EHIP   -1  /* HIP runtime error                    */
+#define BH_ENODEV -2  /* no GPU                               */
+#define BH_EARG   -3  /* invalid argument                     */
+#define BH_EUNSUP -4  /* unsupported element type / rule      */
+#define BH_ESTATE -5  /* call order (e.g. solve before build) */
+#define BH_ESING  -6  /* singular matrix (zero pivot)         */
+#define BH_EDOM   -7  /* point inside another object (overlap)*/
+
+typedef struct bh_object bh_object;
+
+const char *bh_last_error(void);
+
+/* select the device and build the quadrature tables (called implicitly) */
+int bh_init(int device);
+int bh_sync(void);
+void *bh_stream(void);
+
+/* One object: the paired boundary mesh as the reference holds it after
+ * read_mesh_data (setup2.c:205-503): node table rn[Nn+1][3], element table
+ * ed[Ne+1][4] (ed[i][3] == 0 for triangles), main / sub domain ids md, sd,
+ * MN interior media with refractive indices n_med[MN+1] (n_med[0] = the
+ * surrounding medium) and the vacuum wavelength.  Computes the element
+ * constants and node data (setup2.c:666-749, 838-869) and uploads them.
+ * Replaces initialize_domd2 (setup2.c:99-122) for the device path.       */
+bh_object *bh_object_create(int Nn, const double *rn, int Ne, const int *ed, const int *md, const int *sd, int MN,
+                            const double *n_med, double lambda0);
+void bh_object_destroy(bh_object *o);
+/* out[0..3] = Nn, Ne, NN, MN; out[4+d] = elements seen by domain d;
+ * out[12+d] = node rows of domain d (d < 8)                              */
+int bh_object_info(const bh_object *o, int *out);
+int bh_object_geometry(const bh_object *o, double *cr, double *cw, double *ren, double *wen, int *eni);
+
+/* Coefficient matrices of every domain (create_cmatrix_domain,
+ * solve_bieq2.c:98-184, with dcoef_bd_node_t2, d3b1_elem.c:482-539).
+ * type must be 2 (GLN rule with the GHN near-pair switch, PREC_DEF).     */
+int bh_object_assemble(bh_object *o, int type);
+int bh_object_gh_count(const bh_object *o, int did);
+/* download: which 0 G, 1 H (nd x nd), 2 dG, 3 dH (nd x 2 x nd complex),
+ * 4 dF (nd x 3 doubles: diagonal 1+F or F, dF/dt_zeta, dF/dt_eta).
+ * Rows/columns follow the domain's element order, 3 node slots each.    */
+int bh_object_matrix(const bh_object *o, int did, int which, double *out);
+
+/* Incident plane waves: 12 doubles per beam (lambda0, n_surrounding,
+ * power, re/im e0x, re/im e0y, fx, fy, fz, theta, phi) as read by
+ * read_data_mfb_ipw (multi_fbeam.c:145-238).  Also fills the incident
+ * boundary data of the open-domain node slots (init_boundary_data_u2,
+ * mo_setup.c:704-757).                                                    */
+int bh_object_set_beams(bh_object *o, int nb, const double *par);
+/* Factorise the boundary-integral system (create_Amatrix_csr + lu_dec_A,
+ * solve_bieq2.c:186-408).  mode 1: the reference's dense 8NN matrix;
+ * mode 2: exact block elimination for one interior medium (three NN-sized
+ * LU factorisations); mode 0 picks 2 when possible.                      */
+int bh_object_factor(bh_object *o, int mode);
+/* boundary potentials from the current incident data (create_Bmatrix +
+ * solve_ABmatrix + tmatrix_bd_store, mo_solve.c:137-274, 457-545)         */
+int bh_object_solve(bh_object *o);
+/* boundary E, H and their normal derivatives (solve_eh_bv + solve_deh_bv,
+ * mo_solve.c:547-789)                                                     */
+int bh_object_solve_EH(bh_object *o);
+/* boundary values per domain: which 0 U, 1 dU ([dNe+1][4][4] complex),
+ * 2 E, 3 dE, 4 H, 5 dH ([dNe+1][4][3]); set != 0 uploads                 */
+int bh_object_state(bh_object *o, int did, int which, double *buf, int set);
+/* incident data: which 0 Ui, 1 dUi, 2 Ui0, 3 dUi0 ([Ne+1][4][4])         */
+int bh_object_incident(bh_object *o, int which, double *buf, int set);
+
+/* Coupling loop over N objects (mo_solve_bv, mo_solve.c:10-88): every
+ * object factorised and given the same beams.  Convergence when the
+ * change measure (ccd_f) < ccd after more than iter_min iterations.
+ * trace[iter] (optional) receives the change per iteration.              */
+int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters);
+/* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i; mo_field.c:118-282):
+ * what 0: E,H (6 complex per point), 1: U (4 complex); mode 0 scattered +
+ * internal, 1 total, 2 incident; type 0 4-point, 1 7-point, 2 GLN, 3 GHN.
+ * oid[p] receives the object containing the point or -1.                 */
+int bh_mo_field(bh_object **objs, int N, int what, int mode, int type, int np, const double *pts, double *out, int *oid);
+/* domain ids of np points w.r.t. one object (domain_id, setup2.c:944-978) */
+int bh_object_domain(bh_object *o, int np, const double *pts, int *did);
+/* fields at np points on object oid's surface: element t[p] (1-based, in
+ * domain did's element list), local coordinates ze[2p], ze[2p+1];
+ * mode 0 s, 1 t, 2 i, 3 a (mo_EH_*_bd, mo_field.c:632-718); type 2 only. */
+int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, const int *t, const double *ze, int type, int mode,
+                double *out);
+/* radiation force and torque about rc (mo_force_FN, mo_force.c:10-240)   */
+int bh_mo_force(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq);
+/* absorbed power (mo_absorb_P, mo_absorb.c:9-44); returns -2 when the
+ * flux came out negative (its magnitude is returned)                      */
+int bh_mo_absorb(bh_object **objs, int N, int oid, int type, double *P);
+
+/* dense complex LU solve A X = B (row-major, n x n and n x nrhs), the
+ * device replacement of LAPACKE_zgetrf/zgetri/zgesv (solve_bieq2.c:389-399,
+ * mo_solve.c:751); A is overwritten by its LU factors, B by X            */
+int bh_zgesv(int n, int nrhs, double *A, double *B);
+/* C = alpha A B + beta C (row-major, complex): the MFMA ZGEMM kernel     */
+int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B, const double *beta,
+                  double *C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
