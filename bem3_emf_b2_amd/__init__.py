@@ -53,17 +53,14 @@ def lib():
             "bh_mo_force": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D, D, D]),
             "bh_mo_absorb": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D]),
             "bh_zgemm_host": (C.c_int, [C.c_int, C.c_int, C.c_int, D, D, D, D, D]),
+            "bh_object_domain": (C.c_int, [VP, C.c_int, D, Ip]),
+            "bh_mo_EH_bd": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_int, Ip, D, C.c_int, C.c_int, D]),
+            "bh_prof_enable": (C.c_int, [C.c_int]),
+            "bh_prof_read": (C.c_int, [C.c_int, D, D, Ip]),
+            "bh_stream": (VP, []),
         }
-        optional = {}
-        try:
-            from . import _sigs
-            optional = _sigs.SIGS
-        except ImportError:
-            pass
-        for name, (res, args) in list(sig.items()) + list(optional.items()):
-            f = getattr(L, name, None)
-            if f is None:
-                continue
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
             f.restype = res
             f.argtypes = args
         _lib = L
@@ -86,6 +83,25 @@ def ip(a):
 
 def init(device=0):
     check(lib().bh_init(device))
+
+
+def sync():
+    check(lib().bh_sync())
+
+
+# kernel ids of the timing registry (bem3_hip.h)
+KERNELS = {"asm_gl": 0, "asm_gh": 1, "asm_self": 2, "zgemm": 3, "lu_panel": 4, "trsm": 5, "field": 6, "scatter": 7}
+
+
+def prof_enable(on=True):
+    check(lib().bh_prof_enable(1 if on else 0))
+
+
+def prof_read(name):
+    """(kernel ms summed over launches, algorithmic work summed, launches)"""
+    ms, work, n = np.zeros(1), np.zeros(1), np.zeros(1, np.int32)
+    check(lib().bh_prof_read(KERNELS[name], dp(ms), dp(work), ip(n)))
+    return float(ms[0]), float(work[0]), int(n[0])
 
 
 class Object:
@@ -172,6 +188,12 @@ class Object:
         buf = np.ascontiguousarray(value, dtype=np.complex128).reshape(-1)
         check(lib().bh_object_incident(self.h, k, dp(buf.view(np.float64)), 1))
 
+    def domain(self, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        did = np.zeros(pts.shape[0], np.int32)
+        check(lib().bh_object_domain(self.h, pts.shape[0], dp(pts), ip(did)))
+        return did
+
     def close(self):
         if self.h:
             lib().bh_object_destroy(self.h)
@@ -226,6 +248,17 @@ def mo_field(objs, pts, what="EH", mode="t", type_=0):
     check(lib().bh_mo_field(_handles(objs), len(objs), w, m, type_, pts.shape[0], dp(pts), dp(out.view(np.float64)),
                             ip(oid)))
     return out.reshape(-1, nc), oid
+
+
+def mo_EH_bd(objs, oid, did, t, ze, mode="t", type_=2):
+    """E, H (6 complex per point) at element t[p] of domain did, local
+    coordinates ze[p] = (zeta, eta) on object oid's surface"""
+    t = np.ascontiguousarray(t, dtype=np.int32).reshape(-1)
+    ze = np.ascontiguousarray(ze, dtype=np.float64).reshape(-1, 2)
+    m = {"s": 0, "t": 1, "i": 2, "a": 3}[mode]
+    out = np.zeros(t.size * 6, np.complex128)
+    check(lib().bh_mo_EH_bd(_handles(objs), len(objs), oid, did, t.size, ip(t), dp(ze), type_, m, dp(out.view(np.float64))))
+    return out.reshape(-1, 6)
 
 
 def mo_force(objs, oid, type_, rc=(0.0, 0.0, 0.0)):

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,12 @@ static int fail(int code, const char *fmt, ...)
   return code;
 }
 static void dfree(void *p) { (void)hipFree(p); }
+
+// Entry points are serialised: callers may call the scalar field functions
+// from their own thread pools (e.g. OpenMP loops over grid points); all work
+// goes to one device stream in call order.
+static std::recursive_mutex g_mu;
+#define API_LOCK std::lock_guard<std::recursive_mutex> api_lock_(g_mu)
 #define HIPCHK(x)                                                                  \
   do {                                                                             \
     hipError_t e_ = (x);                                                           \
@@ -126,6 +133,7 @@ static int ensure_init()
 
 extern "C" int bh_init(int device)
 {
+  API_LOCK;
   if (g.dev >= 0) return 0;
   int n = 0;
   HIPCHK(hipGetDeviceCount(&n));
@@ -149,6 +157,7 @@ extern "C" int bh_init(int device)
 
 extern "C" int bh_sync(void)
 {
+  API_LOCK;
   if (g.dev < 0) return 0;
   HIPCHK(hipStreamSynchronize(g.st));
   return 0;
@@ -307,6 +316,7 @@ static int upload_geo(bh_object *o)
 extern "C" bh_object *bh_object_create(int Nn, const double *rn, int Ne, const int *ed, const int *md, const int *sd,
                                        int MN, const double *n_med, double lambda0)
 {
+  API_LOCK;
   if (ensure_init()) return nullptr;
   if (Ne <= 0 || Nn <= 0 || MN < 1) { fail(BH_EARG, "empty object"); return nullptr; }
   bh_object *o = new bh_object();
@@ -342,6 +352,7 @@ extern "C" bh_object *bh_object_create(int Nn, const double *rn, int Ne, const i
 
 extern "C" void bh_object_destroy(bh_object *o)
 {
+  API_LOCK;
   if (!o) return;
   for (Dom &D : o->dom) {
     dfree(D.d_sid); dfree(D.d_colOff); dfree(D.d_rows);
@@ -353,6 +364,7 @@ extern "C" void bh_object_destroy(bh_object *o)
 
 extern "C" int bh_object_info(const bh_object *o, int *out)
 {
+  API_LOCK;
   out[0] = o->Nn; out[1] = o->Ne; out[2] = o->NN; out[3] = o->MN;
   for (int d = 0; d <= o->MN && d < 8; d++) { out[4 + d] = o->dom[d].dNe; out[12 + d] = o->dom[d].nd; }
   return 0;
@@ -360,6 +372,7 @@ extern "C" int bh_object_info(const bh_object *o, int *out)
 
 extern "C" int bh_object_geometry(const bh_object *o, double *cr, double *cw, double *ren, double *wen, int *eni)
 {
+  API_LOCK;
   if (cr) memcpy(cr, o->cr.data(), o->cr.size() * sizeof(double));
   if (cw) memcpy(cw, o->cw.data(), o->cw.size() * sizeof(double));
   if (ren) memcpy(ren, o->ren.data(), o->ren.size() * sizeof(double));
@@ -426,6 +439,7 @@ static int assemble_domain(bh_object *o, int d)
 
 extern "C" int bh_object_assemble(bh_object *o, int type)
 {
+  API_LOCK;
   if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
   for (int d = 0; d <= o->MN; d++) {
     int rc = assemble_domain(o, d);
@@ -439,6 +453,7 @@ extern "C" int bh_object_gh_count(const bh_object *o, int did) { return o->dom[d
 // which: 0 G, 1 H (nd x nd), 2 dG, 3 dH (nd x 2 x nd), 4 dF (nd x 3 doubles)
 extern "C" int bh_object_matrix(const bh_object *o, int did, int which, double *out)
 {
+  API_LOCK;
   if (did < 0 || did > o->MN) return fail(BH_EARG, "bad domain %d", did);
   const Dom &D = o->dom[did];
   size_t nd = (size_t)D.nd;
@@ -453,6 +468,7 @@ extern "C" int bh_object_matrix(const bh_object *o, int did, int which, double *
 // ------------------------------------------------------------------ dense LA entry points
 extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
 {
+  API_LOCK;
   if (ensure_init()) return BH_EHIP;
   cd *dA, *dB, *work;
   int *piv, *info;
@@ -477,6 +493,7 @@ extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
 extern "C" int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B,
                              const double *beta, double *C)
 {
+  API_LOCK;
   if (ensure_init()) return BH_EHIP;
   cd *dA, *dB, *dC;
   HIPCHK(hipMalloc((void **)&dA, (size_t)m * k * sizeof(cd)));
@@ -660,6 +677,7 @@ static int prepare_solve(bh_object *o)
 // fx, fy, fz, theta, phi); derived data as setup_ipw (ipw.c:65-89)
 extern "C" int bh_object_set_beams(bh_object *o, int nb, const double *par)
 {
+  API_LOCK;
   if (nb <= 0) return fail(BH_EARG, "no beams defined");
   std::vector<Beam> bm(nb);
   double n0 = par[1], lam = par[0];
@@ -703,6 +721,7 @@ static size_t row_base(const bh_object *o, int d)
 // mode 0: auto (Schur for one medium, dense otherwise), 1: dense 8NN, 2: Schur
 extern "C" int bh_object_factor(bh_object *o, int mode)
 {
+  API_LOCK;
   if (!o->dom[0].d_G) return fail(BH_ESTATE, "assemble before factor");
   if (prepare_solve(o)) return BH_EHIP;
   size_t NN = (size_t)o->NN, na = 8 * NN;
@@ -819,6 +838,7 @@ static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 // B from the incident data, X = A^{-1} B, boundary values (solve_coefficient_mEMP)
 extern "C" int bh_object_solve(bh_object *o)
 {
+  API_LOCK;
   if (!o->fmode) return fail(BH_ESTATE, "factor before solve");
   if (!o->d_beams) return fail(BH_ESTATE, "set beams before solve");
   size_t NN = (size_t)o->NN, na = 8 * NN;
@@ -867,6 +887,7 @@ extern "C" int bh_object_solve(bh_object *o)
 // G [dE|dH] = H [E|H] per domain (solve_eh_bv + solve_deh_bv)
 extern "C" int bh_object_solve_EH(bh_object *o)
 {
+  API_LOCK;
   if (!o->dom[0].d_U) return fail(BH_ESTATE, "solve before the boundary fields");
   zd ch = o->lambda0 / (2.0 * M_PI * zd(0.0, 1.0));
   for (int d = 0; d <= o->MN; d++) {
@@ -902,6 +923,7 @@ extern "C" int bh_object_solve_EH(bh_object *o)
 // set != 0 uploads (e.g. a stored solution), else downloads
 extern "C" int bh_object_state(bh_object *o, int did, int which, double *buf, int set)
 {
+  API_LOCK;
   if (did < 0 || did > o->MN || which < 0 || which > 5) return fail(BH_EARG, "bad state selector");
   if (prepare_solve(o)) return BH_EHIP;
   Dom &D = o->dom[did];
@@ -915,6 +937,7 @@ extern "C" int bh_object_state(bh_object *o, int did, int which, double *buf, in
 // which: 0 Ui, 1 dUi, 2 Ui0, 3 dUi0 ([Ne+1][4][4])
 extern "C" int bh_object_incident(bh_object *o, int which, double *buf, int set)
 {
+  API_LOCK;
   if (prepare_solve(o)) return BH_EHIP;
   cd *tab[4] = {o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0};
   if (which < 0 || which > 3) return fail(BH_EARG, "bad incident selector");
@@ -951,6 +974,7 @@ static int domain_ids(bh_object *o, const double *d_pts, int np, int *d_did)
 extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type, int np, const double *pts, double *out,
                            int *oid)
 {
+  API_LOCK;
   if (N <= 0 || np < 0 || type < 0 || type > 3 || what < 0 || what > 1) return fail(BH_EARG, "bad field request");
   if (np == 0) return 0;
   if (ensure_rules()) return BH_EHIP;
@@ -1125,6 +1149,7 @@ static int check_ready(bh_object **objs, int N, int oid)
 // returns 0, or -1 when the object has no open-domain faces
 extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq)
 {
+  API_LOCK;
   int rc0 = check_ready(objs, N, oid);
   if (rc0) return rc0;
   bh_object *o = objs[oid];
@@ -1154,6 +1179,7 @@ extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const dou
 // (negative flux, magnitude returned)
 extern "C" int bh_mo_absorb(bh_object **objs, int N, int oid, int type, double *P)
 {
+  API_LOCK;
   int rc0 = check_ready(objs, N, oid);
   if (rc0) return rc0;
   SurfPts sp;
@@ -1242,6 +1268,7 @@ static double ccd_value(bh_object **objs, int N)
 // receives the final iteration count.
 extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters)
 {
+  API_LOCK;
   if (N <= 0) return fail(BH_EARG, "no objects");
   if (ensure_rules()) return BH_EHIP;
   for (int m = 0; m < N; m++)
@@ -1284,6 +1311,7 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
 // domain of np points w.r.t. one object (domain_id, setup2.c:944-978)
 extern "C" int bh_object_domain(bh_object *o, int np, const double *pts, int *did)
 {
+  API_LOCK;
   if (np <= 0) return 0;
   double *d_pts;
   int *d_did;
@@ -1303,6 +1331,7 @@ extern "C" int bh_object_domain(bh_object *o, int np, const double *pts, int *di
 extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, const int *t, const double *ze, int type,
                            int mode, double *out)
 {
+  API_LOCK;
   if (N <= 0 || oid < 0 || oid >= N || np <= 0) return fail(BH_EARG, "bad boundary-field request");
   if (type != 2) return fail(BH_EUNSUP, "boundary fields use the GLN rule (type 2)");
   if (ensure_rules()) return BH_EHIP;

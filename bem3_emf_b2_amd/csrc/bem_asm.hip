@@ -24,6 +24,7 @@
 // column = shape-function index in the same order (row-major, leading
 // dimension ld, complex128 interleaved).  dG/dH rows are [row][dir][ld].
 #include "bem_dev.h"
+#include "bem_prof.h"
 
 struct AsmAcc {
   cd G[3], H[3], DG0[3], DG1[3], DH0[3], DH1[3];
@@ -364,8 +365,10 @@ void bem_launch_asm_gl(const void *geo, const int *sid, const void *rows, const 
                        double *dfPart, unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
 {
   dim3 grid(nwv_blocks, nrows);
+  bem_prof_begin(BH_K_ASM_GL, (double)nrows * dNe, st);
   hipLaunchKernelGGL(k_asm_gl, grid, dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff, dNe, kr, ki,
                      (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
+  bem_prof_end(BH_K_ASM_GL, st);
 }
 
 void bem_launch_asm_gh(const void *geo, const int *sid, const void *rows, const int *colOff, const void *pairs,
@@ -374,9 +377,11 @@ void bem_launch_asm_gh(const void *geo, const int *sid, const void *rows, const 
 {
   if (npairs <= 0) return;
   int nb = (npairs + 3) / 4;
+  bem_prof_begin(BH_K_ASM_GH, (double)npairs, st);
   hipLaunchKernelGGL(k_asm_gh, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff,
                      (const GhPair *)pairs, npairs, kr, ki, (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH,
                      ld, ghDF);
+  bem_prof_end(BH_K_ASM_GH, st);
 }
 
 void bem_launch_asm_self(const void *geo, const int *sid, const void *rows, const int *colOff, int nrows, double kr,
@@ -384,8 +389,10 @@ void bem_launch_asm_self(const void *geo, const int *sid, const void *rows, cons
                          void *H, void *dG, void *dH, size_t ld, hipStream_t st)
 {
   int nb = (nrows + 3) / 4;
+  bem_prof_begin(BH_K_ASM_SELF, (double)nrows, st);
   hipLaunchKernelGGL(k_asm_self, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff,
                      nrows, kr, ki, xl, wl, xh, wh, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld);
+  bem_prof_end(BH_K_ASM_SELF, st);
 }
 
 void bem_launch_asm_fin(const double *dfPart, int nwv, const double *ghDF, const int *ghStart, int nrows,

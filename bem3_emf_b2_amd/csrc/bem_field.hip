@@ -16,6 +16,7 @@
 // One wavefront per target point, lanes over source elements, wave
 // reductions at the end.
 #include "bem_dev.h"
+#include "bem_prof.h"
 
 
 
@@ -432,18 +433,22 @@ void bem_launch_field(const double *pts, const int *pdid, int np, int want_did, 
                       const void *dA0, const void *A1, const void *dA1, void *out, int accumulate, hipStream_t st)
 {
   if (np <= 0) return;
+  bem_prof_begin(BH_K_FIELD, (double)np * dNe, st);
   hipLaunchKernelGGL(k_field, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo, sid,
                      dNe, kr, ki, (const QPtN *)rule, nq, scale, what, (const cd *)A0, (const cd *)dA0, (const cd *)A1,
                      (const cd *)dA1, (cd *)out, accumulate);
+  bem_prof_end(BH_K_FIELD, st);
 }
 void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const int *sid,
                         int dNe, double kr, double ki, const void *gl, int ngl, const void *gh, int ngh, double scale,
                         const void *U, const void *dU, void *Ui, void *dUi, hipStream_t st)
 {
   if (np <= 0) return;
+  bem_prof_begin(BH_K_SCATTER, (double)np * dNe, st);
   hipLaunchKernelGGL(k_scatter, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo, sid, dNe,
                      kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale, (const cd *)U, (const cd *)dU, (cd *)Ui,
                      (cd *)dUi);
+  bem_prof_end(BH_K_SCATTER, st);
 }
 void bem_launch_field_bd(const int *tgt_t, const double *tgt_ze, int np, int did, const void *geo, const int *sid,
                          int dNe, double kr, double ki, const void *gl, int ngl, double scale, const double *xl,

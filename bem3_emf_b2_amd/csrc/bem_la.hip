@@ -11,6 +11,7 @@
 // Replaces LAPACKE_zgetrf/zgetri (solve_bieq2.c:389-399) and zgesv
 // (mo_solve.c:751) of the reference.
 #include "bem_dev.h"
+#include "bem_prof.h"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -216,8 +217,10 @@ void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t 
 {
   if (m <= 0 || n <= 0) return;
   dim3 grid((n + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M);
+  bem_prof_begin(BH_K_ZGEMM, 8.0 * m * n * k, st);
   hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda, (const cd *)B, ldb,
                      mk(br, bi), (cd *)C, ldc);
+  bem_prof_end(BH_K_ZGEMM, st);
 }
 
 // LU factorisation in place.  piv (device, n ints) receives 0-based pivot
@@ -230,19 +233,24 @@ void bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, in
   (void)hipMemsetAsync(info, 0, sizeof(int), st);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int kb = n - k0 < nb ? n - k0 : nb, k1 = k0 + kb;
+    // panel: about 8 (n - k0) kb^2 / 2 real flops of rank-1 updates
+    bem_prof_begin(BH_K_LU_PANEL, 4.0 * (double)(n - k0) * kb * kb, st);
     for (int j = k0; j < k1; j++) {
       hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(1024), 0, st, A, lda, n, j, k0, k1, piv, prow, info);
       int rows = n - j - 1;
       if (rows > 0) hipLaunchKernelGGL(k_lu_update, dim3((rows + 255) / 256), dim3(256), 0, st, A, lda, n, j, k0, k1, prow);
     }
+    bem_prof_end(BH_K_LU_PANEL, st);
     // interchanges on the columns left and right of the panel
     if (k0 > 0) hipLaunchKernelGGL(k_laswp, dim3((k0 + 255) / 256), dim3(256), 0, st, A, lda, 0, k0, k0, k1, piv);
     if (k1 < n) {
       int nr = n - k1;
       hipLaunchKernelGGL(k_laswp, dim3((nr + 255) / 256), dim3(256), 0, st, A, lda, k1, n, k0, k1, piv);
       size_t shm = (size_t)(kb * kb + kb * 64) * sizeof(cd);
+      bem_prof_begin(BH_K_TRSM, 4.0 * kb * kb * (double)nr, st);
       hipLaunchKernelGGL(k_trsm_lunit, dim3((nr + 63) / 64), dim3(64), shm, st, A + (size_t)k0 * lda + k0, lda, kb,
                          A + (size_t)k0 * lda + k1, lda, nr);
+      bem_prof_end(BH_K_TRSM, st);
       bem_zgemm(nr, nr, kb, -1.0, 0.0, A + (size_t)k1 * lda + k0, lda, A + (size_t)k0 * lda + k1, lda, 1.0, 0.0,
                 A + (size_t)k1 * lda + k1, lda, st);
     }

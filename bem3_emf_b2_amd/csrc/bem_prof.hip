@@ -1,0 +1,86 @@
+// bem_prof.hip -- kernel-time registry: when enabled, a pair of HIP events
+// brackets every launch of the tracked kernels on the stream they run on,
+// and each launch carries its algorithmic work (flops, element pairs or
+// points, as the caller states).  Read back through bh_prof_read.
+#include <hip/hip_runtime.h>
+#include <mutex>
+#include <vector>
+
+#include "../../include/bem3_hip.h"
+
+namespace {
+struct Rec {
+  int kid;
+  double work;
+  hipEvent_t a, b;
+};
+struct Prof {
+  bool on = false;
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+};
+Prof P;
+std::mutex M;
+
+hipEvent_t take()
+{
+  if (P.used < P.pool.size()) return P.pool[P.used++];
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  P.pool.push_back(e);
+  P.used++;
+  return e;
+}
+}  // namespace
+
+extern "C" {
+
+void bem_prof_begin(int kid, double work, hipStream_t st)
+{
+  if (!P.on || kid < 0 || kid >= BH_K_COUNT) return;
+  std::lock_guard<std::mutex> lk(M);
+  Rec r{kid, work, take(), take()};
+  if (!r.a || !r.b) return;
+  (void)hipEventRecord(r.a, st);
+  P.recs.push_back(r);
+}
+
+void bem_prof_end(int kid, hipStream_t st)
+{
+  if (!P.on) return;
+  std::lock_guard<std::mutex> lk(M);
+  if (P.recs.empty() || P.recs.back().kid != kid) return;
+  (void)hipEventRecord(P.recs.back().b, st);
+}
+
+int bh_prof_enable(int on)
+{
+  std::lock_guard<std::mutex> lk(M);
+  P.on = on != 0;
+  P.recs.clear();
+  P.used = 0;
+  return 0;
+}
+
+int bh_prof_read(int kid, double *ms, double *work, int *launches)
+{
+  if (kid < 0 || kid >= BH_K_COUNT) return BH_EARG;
+  std::lock_guard<std::mutex> lk(M);
+  double t = 0.0, w = 0.0;
+  int n = 0;
+  for (const Rec &r : P.recs) {
+    if (r.kid != kid) continue;
+    if (hipEventSynchronize(r.b) != hipSuccess) return BH_EHIP;
+    float e = 0.0f;
+    if (hipEventElapsedTime(&e, r.a, r.b) != hipSuccess) return BH_EHIP;
+    t += e;
+    w += r.work;
+    n++;
+  }
+  if (ms) *ms = t;
+  if (work) *work = w;
+  if (launches) *launches = n;
+  return 0;
+}
+}

@@ -116,6 +116,26 @@ int bh_zgesv(int n, int nrhs, double *A, double *B);
 int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B, const double *beta,
                   double *C);
 
+/* Kernel timing (no counterpart in the reference): while enabled, HIP events
+ * on the library's stream bracket every launch of the tracked kernels.
+ * bh_prof_read returns the summed kernel time (ms), the summed algorithmic
+ * work and the launch count of one kernel since the last enable.  Work units:
+ * BH_K_ASM_GL, BH_K_ASM_GH, BH_K_ASM_SELF: (row node, source element) pairs;
+ * BH_K_ZGEMM: real flops (8 m n k); BH_K_LU_PANEL: real flops of the
+ * unblocked panel updates; BH_K_TRSM: real flops; BH_K_FIELD, BH_K_SCATTER:
+ * (point, source element) pairs.                                          */
+#define BH_K_ASM_GL   0
+#define BH_K_ASM_GH   1
+#define BH_K_ASM_SELF 2
+#define BH_K_ZGEMM    3
+#define BH_K_LU_PANEL 4
+#define BH_K_TRSM     5
+#define BH_K_FIELD    6
+#define BH_K_SCATTER  7
+#define BH_K_COUNT    8
+int bh_prof_enable(int on);
+int bh_prof_read(int kid, double *ms, double *work, int *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,7 @@ def lib():
             "orc_api_dcoef_rt": (None, [VP, D, D, C.c_int, D, C.c_int, D, D]),
             "orc_api_dcoef_node_t2": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, D, C.c_int, D, D, D]),
             "orc_api_cmatrix": (C.c_int, [VP, C.c_int, C.c_int, D, D, D, D, D, Ip]),
+            "orc_api_cmatrix_rows": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int, D, D, D, D, D, Ip]),
             "orc_api_create_matrix": (VP, [VP, C.c_int, Ip]),
             "orc_api_sys_free": (None, [VP]),
             "orc_api_assemble": (VP, [VP, C.c_int, Ip]),
@@ -183,6 +184,19 @@ class Obj:
             out["dG"] = dG.reshape(N, 2, N)
             out["dH"] = dH.reshape(N, 2, N)
         return out
+
+    def cmatrix_rows(self, did, t0, t1, type_=2):
+        """rows of elements t0 <= t < t1 (1-based) of the padded matrices"""
+        N = 4 * self.dNe[did]
+        R = 4 * (t1 - t0)
+        G, H = cbuf(R * N), cbuf(R * N)
+        dG, dH = cbuf(2 * R * N), cbuf(2 * R * N)
+        dF = np.zeros(3 * R)
+        ngh = np.zeros(1, np.int32)
+        bad = lib().orc_api_cmatrix_rows(self.h, did, type_, t0, t1, dp(G.view(np.float64)), dp(H.view(np.float64)),
+                                         dp(dG.view(np.float64)), dp(dH.view(np.float64)), dp(dF), ip(ngh))
+        return dict(G=G.reshape(R, N), H=H.reshape(R, N), dG=dG.reshape(R, 2, N), dH=dH.reshape(R, 2, N),
+                    dF=dF.reshape(R, 3), ngh=int(ngh[0]), nbad=bad)
 
     def domain_id(self, rt):
         return lib().orc_api_domain_id(self.h, dp(np.asarray(rt, dtype=np.float64)))

@@ -152,6 +152,8 @@ int orc_dcoef_bd_node_t2(zc *CC, zc *dCz, zc *dCe, int t, int nid, const double 
 
 /* ---- single-object BIE (orc_bie.c) ---- */
 int orc_cmatrix_domain(const orc_obj *o, int did, int type, zc *G, zc *H, zc *dG, zc *dH, double *dF, int *gh_count);
+int orc_cmatrix_rows(const orc_obj *o, int did, int type, int t0, int t1, zc *G, zc *H, zc *dG, zc *dH, double *dF,
+                     int *gh_count);
 int orc_zgetrf(int n, zc *A, int lda, int *piv);
 void orc_zgetrs(int n, int nrhs, const zc *A, int lda, const int *piv, zc *B, int ldb);
 typedef struct {

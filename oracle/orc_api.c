@@ -119,6 +119,13 @@ int orc_api_cmatrix(const orc_obj *o, int did, int type, double *G, double *H, d
   return orc_cmatrix_domain(o, did, type, (zc *)G, (zc *)H, (zc *)dG, (zc *)dH, dF, ngh);
 }
 
+/* rows of elements t0 <= t < t1 of the padded domain matrices */
+int orc_api_cmatrix_rows(const orc_obj *o, int did, int type, int t0, int t1, double *G, double *H, double *dG, double *dH,
+                         double *dF, int *ngh)
+{
+  return orc_cmatrix_rows(o, did, type, t0, t1, (zc *)G, (zc *)H, (zc *)dG, (zc *)dH, dF, ngh);
+}
+
 orc_sys *orc_api_create_matrix(orc_obj *o, int type, int *status)
 {
   orc_sys *sys = calloc(1, sizeof(orc_sys));

@@ -37,18 +37,27 @@ static void *orc_xcalloc(size_t n, size_t sz)
  * open domain and F inside (solid-angle term). */
 int orc_cmatrix_domain(const orc_obj *o, int did, int type, zc *G, zc *H, zc *dG, zc *dH, double *dF, int *gh_count)
 {
+  return orc_cmatrix_rows(o, did, type, 1, o->dNe[did] + 1, G, H, dG, dH, dF, gh_count);
+}
+
+/* The rows of elements t0 <= t < t1 only (row (t - t0) * 4 + tn of the
+ * outputs); the full matrix is t0 = 1, t1 = Ne + 1.  Used to time a bounded
+ * sample of a large assembly. */
+int orc_cmatrix_rows(const orc_obj *o, int did, int type, int t0, int t1, zc *G, zc *H, zc *dG, zc *dH, double *dF,
+                     int *gh_count)
+{
   size_t Ne = (size_t)o->dNe[did], N = 4 * Ne;
   zc kc = o->kn[did];
   int ngh = 0, nbad = 0;
 #pragma omp parallel for schedule(dynamic) reduction(+ : ngh, nbad)
-  for (size_t t = 1; t <= Ne; t++) {
+  for (size_t t = (size_t)t0; t < (size_t)t1; t++) {
     int td = o->sid[did][t];
     int tl = orc_is_tri(o, td) ? 3 : 4;
     for (int tn = 0; tn < tl; tn++) {
       double vtz[3], vte[3];
       orc_tz_te_node(vtz, vte, o, td, tn);
       double F = 0.0, dFz = 0.0, dFe = 0.0;
-      size_t row = (t - 1) * 4 + tn;
+      size_t row = (t - (size_t)t0) * 4 + tn, drow = (t - 1) * 4 + tn;
       for (size_t s = 1; s <= Ne; s++) {
         int sd = o->sid[did][s];
         zc CC[9], dCz[9], dCe[9];
@@ -65,7 +74,7 @@ int orc_cmatrix_domain(const orc_obj *o, int did, int type, zc *G, zc *H, zc *dG
         F += creal(CC[8]); dFz += creal(dCz[8]); dFe += creal(dCe[8]);
       }
       double diag = did == 0 ? 1.0 + F : F;
-      if (H) H[row * N + row] += diag;
+      if (H) H[row * N + drow] += diag;
       if (dF) { dF[row * 3 + 0] = diag; dF[row * 3 + 1] = dFz; dF[row * 3 + 2] = dFe; }
     }
   }
