@@ -206,12 +206,14 @@ class Object:
             pass
 
 
-def zgesv(A, B):
+def zgesv(A, B, return_lu=False):
+    """X = A^{-1} B on the GPU (LU with partial pivoting); with return_lu the
+    packed LU factors (LAPACK zgetrf layout, row-major) come back too."""
     A = np.ascontiguousarray(A, dtype=np.complex128).copy()
     B = np.ascontiguousarray(B, dtype=np.complex128).copy()
     nrhs = 1 if B.ndim == 1 else B.shape[1]
     check(lib().bh_zgesv(A.shape[0], nrhs, dp(A.view(np.float64)), dp(B.view(np.float64))))
-    return B
+    return (B, A) if return_lu else B
 
 
 def zgemm(A, B, Cm=None, alpha=1.0, beta=0.0):

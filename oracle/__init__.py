@@ -243,6 +243,8 @@ class System:
         self.h = C.c_void_p(fn(obj.h, type_, ip(st)))
         self.status = int(st[0])
         self.obj = obj
+        if factor and self.status != 0:
+            raise RuntimeError("oracle create_matrix failed (status %d)" % self.status)
 
     def solve_EH(self):
         lib().orc_api_solve_EH(self.obj.h, self.h)

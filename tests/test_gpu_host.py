@@ -49,7 +49,7 @@ def solved(tmp_path_factory):
     # the oracle on the same input files
     O = []
     for s in SHIFTS:
-        o = orc.Obj.from_msh(os.path.join(d, "sphere.msh"), [0j, N_OBJ])
+        o = orc.Obj.from_msh(os.path.join(d, "sphere.msh"), [N_OBJ])
         o.setup(BEAM, tv=np.array(s))
         O.append(o)
     S = [orc.System(o) for o in O]
@@ -105,5 +105,6 @@ def test_reference_example1_relinked(solved):
             np.testing.assert_allclose(got["force"][typ][m]["F"], F, rtol=1e-7, atol=1e-7 * np.abs(F).max())
             np.testing.assert_allclose(got["force"][typ][m]["N"], N, rtol=1e-7, atol=1e-7 * np.abs(N).max() + 1e-15)
             ret, P = orc.mo_absorb(O, m, typ)
-            # lossless objects: both sides print round-off-level values
-            assert abs(got["absorb"][typ][m]) < 1e-9 and abs(P) < 1e-9
+            # lossless spheres: P is the discretisation error of the flux
+            # balance on this coarse mesh, the same number on both sides
+            assert abs(got["absorb"][typ][m] - P) <= 1e-7 * abs(P) + 1e-14, (typ, m)

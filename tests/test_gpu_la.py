@@ -9,7 +9,9 @@ def _rand(rng, *shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-@pytest.mark.parametrize("m,n,k", [(64, 64, 16), (100, 70, 37), (257, 130, 64), (5, 300, 3)])
+# the last two shapes take the 128x128-tile kernel (m n >= 512 tiles of 128^2)
+@pytest.mark.parametrize("m,n,k", [(64, 64, 16), (100, 70, 37), (257, 130, 64), (5, 300, 3), (3001, 2999, 77),
+                                   (4096, 4096, 64)])
 def test_zgemm_matches_numpy(m, n, k):
     import bem3_emf_b2_amd as bem
     rng = np.random.default_rng(m * 1000 + n + k)
@@ -19,17 +21,38 @@ def test_zgemm_matches_numpy(m, n, k):
     assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (63, 2), (300, 6), (1000, 3)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (63, 2), (300, 6), (1000, 3), (3100, 2)])
 def test_zgesv_matches_numpy(n, nrhs):
     import bem3_emf_b2_amd as bem
     rng = np.random.default_rng(n + 7 * nrhs)
     A = _rand(rng, n, n) + n * 0.1 * np.eye(n)
     B = _rand(rng, n, nrhs)
-    X = bem.zgesv(A, B)
+    X, LU = bem.zgesv(A, B, return_lu=True)
     want = np.linalg.solve(A, B)
     res = np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X))
     assert res < 1e-14
     assert np.abs(X - want).max() < 1e-10 * np.abs(want).max()
+    # same pivot sequence as LAPACK's zgetrf: the packed factors agree
+    import scipy.linalg as sl
+    lu_ref, _ = sl.lu_factor(A)
+    assert np.abs(LU - lu_ref).max() < 1e-9 * np.abs(lu_ref).max()
+
+
+@pytest.mark.parametrize("n", [130, 700])
+def test_lu_pivot_sequence_random(n):
+    """A random matrix needs row interchanges in almost every column; the
+    packed factors then match LAPACK's only if every pivot matches."""
+    import scipy.linalg as sl
+
+    import bem3_emf_b2_amd as bem
+    rng = np.random.default_rng(100 + n)
+    A = _rand(rng, n, n)
+    B = _rand(rng, n, 1)
+    X, LU = bem.zgesv(A, B, return_lu=True)
+    lu_ref, piv_ref = sl.lu_factor(A)
+    assert (piv_ref != np.arange(n)).sum() > n // 2
+    assert np.abs(LU - lu_ref).max() < 1e-10 * np.abs(lu_ref).max()
+    assert np.abs(A @ X - B).max() < 1e-10 * np.abs(B).max() * np.linalg.cond(A)
 
 
 def test_zgesv_needs_pivoting():

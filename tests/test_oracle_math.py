@@ -1,0 +1,136 @@
+"""Pins of the CPU oracle that do not depend on the reference's fixture:
+quadrature tables against numpy's Gauss-Legendre, the triangle element map,
+the element integrals against an independent brute-force quadrature of the
+defining integrals, the tangential derivatives against finite differences,
+and the solid-angle / inside-outside identities of a closed surface.
+
+Conventions fixed here (they are what the reference's coefficient matrices
+mean, solve_bieq2.c:98-184 with lit_coef.c / lit_dcoef_v2.c):
+  G_i = 1/(4 pi) int N_i exp(ikR)/R dS
+  H_i = 1/(4 pi) int N_i (ikR - 1) exp(ikR) ((r' - r).n) / R^3 dS
+  F   = 1/(4 pi) int ((r' - r).n) / R^3 dS
+with n the unit normal (r1 - r0) x (r2 - r0) of the element's stored node
+order and N_i the linear function equal to 1 at node slot i and 0 at the
+other two node slots."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from bem3_emf_b2_amd import meshgen as mg
+
+K_LOSSY = 2 * np.pi * 1.33 / 1.064 + 0.3j
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    V, F = mg.icosphere(2, 0.5)
+    o = orc.Obj.from_arrays(*mg.paired_arrays(V, F), [0j, 1.59])
+    o.setup(mg.xpol_plane_wave())
+    yield o, o.geometry()
+    o.free()
+
+
+def _tri_int(f, P, n=90):
+    """int_T f dS over the flat triangle P[0..2] (collapsed-square Gauss-Legendre)"""
+    x, w = np.polynomial.legendre.leggauss(n)
+    u, wu = 0.5 * (x + 1.0), 0.5 * w
+    a, b, c = P
+    area2 = np.linalg.norm(np.cross(b - a, c - a))
+    S, T = np.meshgrid(u, u, indexing="ij")
+    Wt = np.outer(wu, wu) * (1.0 - S)
+    s_, t_ = S, T * (1.0 - S)
+    R = a[None, None, :] + s_[..., None] * (b - a) + t_[..., None] * (c - a)
+    return (Wt[..., None] * f(R)).sum(axis=(0, 1)) * area2 if f(R).ndim == 3 else (Wt * f(R)).sum() * area2
+
+
+def test_gauss_legendre_tables():
+    xl, wl, xh, wh = orc.quad_tables()
+    for (x, w), n in (((xl, wl), 12), ((xh, wh), 64)):
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        # same nodes and weights to the round-off of the two Newton iterations
+        assert np.abs(x - xr).max() < 2e-15
+        assert (np.abs(w - wr) / wr).max() < 2e-12
+
+
+def test_triangle_element_map(sphere):
+    """r(zeta, eta) = cr0 + cr1 zeta + cr2 eta maps the reference triangle's
+    vertices (-1/2,-sqrt3/2), (1,0), (-1/2,sqrt3/2) to the element's nodes,
+    the node slots to the 4-point abscissae, and W is (r1-r0)x(r2-r0)."""
+    o, g = sphere
+    ref = np.array([[-0.5, -np.sqrt(3) / 2], [1.0, 0.0], [-0.5, np.sqrt(3) / 2]])
+    slots = np.array([[-0.2, -0.3464101615137754587055], [0.4, 0.0], [-0.2, 0.3464101615137754587055], [0.0, 0.0]])
+    for s in (1, 7, 100, 320):
+        P = g["rn"][g["ed"][s][:3]]
+        cr = g["cr"][s]
+        r = lambda z, e: cr[:, 0] + cr[:, 1] * z + cr[:, 2] * e
+        for j in range(3):
+            assert np.abs(r(*ref[j]) - P[j]).max() < 1e-15
+        for j in range(4):
+            assert np.abs(r(*slots[j]) - g["ren"][s][j]).max() < 1e-15
+        W = np.cross(P[1] - P[0], P[2] - P[0])
+        for j in range(4):
+            assert np.abs(g["wen"][s][j] - W).max() < 1e-16
+
+
+@pytest.mark.parametrize("k", [2 * np.pi * 1.33 / 1.064, K_LOSSY])
+@pytest.mark.parametrize("rt", [(0.9, 0.3, -0.2), (0.0, 0.0, 0.0), (-0.47, 0.05, 0.35)])
+def test_element_integrals_brute_force(sphere, k, rt):
+    o, g = sphere
+    rt = np.array(rt)
+    s = 5
+    P = g["rn"][g["ed"][s][:3]]
+    W = np.cross(P[1] - P[0], P[2] - P[0])
+    nv = W / np.linalg.norm(W)
+    nodes = g["ren"][s][:3]
+    # linear interpolation functions through the three node slots
+    M = np.c_[nodes, np.ones(3)]
+    basis = [np.linalg.lstsq(M, np.eye(3)[i], rcond=None)[0] for i in range(3)]
+
+    def Ni(R, i):
+        return R @ basis[i][:3] + basis[i][3]
+
+    def dist(R):
+        return np.linalg.norm(R - rt, axis=-1)
+
+    CC = o.coef_rt(rt, s, k, 3)
+    scale = 1.0 / (4 * np.pi)
+    for i in range(3):
+        G = _tri_int(lambda R: scale * Ni(R, i) * np.exp(1j * k * dist(R)) / dist(R), P)
+        H = _tri_int(lambda R: scale * Ni(R, i) * (1j * k * dist(R) - 1) * np.exp(1j * k * dist(R)) *
+                     ((R - rt) @ nv) / dist(R) ** 3, P)
+        assert abs(CC[i] - G) < 1e-11 * abs(G), (i, CC[i], G)
+        assert abs(CC[4 + i] - H) < 1e-11 * abs(H), (i, CC[4 + i], H)
+    F = _tri_int(lambda R: scale * ((R - rt) @ nv) / dist(R) ** 3, P)
+    assert abs(CC[8].real - F) < 1e-11 * abs(F)
+    assert CC[3] == 0 and CC[7] == 0   # no fourth node on a triangle
+    # the production rule (3-part 12x12) agrees with the 64x64 one this far out
+    CC2 = o.coef_rt(rt, s, k, 2)
+    assert np.abs(CC2 - CC).max() < 1e-8 * np.abs(CC).max()
+
+
+@pytest.mark.parametrize("typ", [2, 3])
+def test_tangential_derivatives_finite_difference(sphere, typ):
+    o, g = sphere
+    rt = np.array([0.6, 0.3, -0.2])
+    v = np.array([0.3, -0.5, 0.8])
+    v /= np.linalg.norm(v)
+    h = 1e-5
+    for s in (5, 77):
+        CC, dCC = o.dcoef_rt(rt, v, s, K_LOSSY, typ)
+        fd = (o.coef_rt(rt + h * v, s, K_LOSSY, typ) - o.coef_rt(rt - h * v, s, K_LOSSY, typ)) / (2 * h)
+        assert np.abs(dCC - fd).max() < 1e-8 * np.abs(fd).max()
+        assert np.abs(CC - o.coef_rt(rt, s, K_LOSSY, typ)).max() == 0.0
+
+
+def test_solid_angle_and_domain_of_closed_surface(sphere):
+    """Over the closed surface seen from the open domain, sum F = -1 for an
+    interior point and 0 outside; domain_id agrees with the analytic sphere."""
+    o, g = sphere
+    sid0 = o.sid(0)
+    rng = np.random.default_rng(3)
+    pts = np.r_[rng.uniform(-0.25, 0.25, (4, 3)), rng.uniform(0.7, 1.5, (4, 3)) * rng.choice([-1, 1], (4, 3))]
+    for p in pts:
+        tot = sum(o.coef_rt(p, int(sid0[t]), K_LOSSY, 2)[8].real for t in range(1, o.dNe[0] + 1))
+        inside = np.linalg.norm(p) < 0.45
+        assert abs(tot - (-1.0 if inside else 0.0)) < 1e-12, (p, tot)
+        assert o.domain_id(p) == (1 if inside else 0)
