@@ -1069,6 +1069,14 @@ static void surface_points(const bh_object *o, int type, SurfPts &sp)
   }
 }
 
+// out[idx[i]] = in[i] (6 complex values per point)
+__global__ void k_scatter6(const cd *__restrict__ in, const int *__restrict__ idx, int n, cd *__restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < 6; c++) out[6 * (size_t)idx[i] + c] = in[6 * (size_t)i + c];
+}
+
 __global__ void k_gather_stored(const int *__restrict__ t, const int *__restrict__ stored, int np,
                                 const cd *__restrict__ E, const cd *__restrict__ H, cd *__restrict__ out)
 {
@@ -1100,18 +1108,32 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
   HIPCHK(hipMalloc((void **)&d_sel, (size_t)np * sizeof(int)));
   HIPCHK(hipMalloc((void **)&d_mask, (size_t)np * sizeof(int)));
   HIPCHK(hipMemsetAsync(d_EH, 0, 6 * (size_t)np * sizeof(cd), g.st));
-  // self object: integral points through EH_s_bd, node slots from storage
-  bem_launch_field_bd(d_t, d_ze, np, 0, o->d_geo, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(), g_rule[2],
-                      g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_tmp,
-                      d_rt, g.st);
-  // keep only the integral points of d_tmp; copy stored values
+  // self object: the boundary integral (EH_s_bd) only at the points that are
+  // not node slots, the node slots from the stored boundary values
   {
-    std::vector<int> mask(np);
-    for (int p = 0; p < np; p++) mask[p] = sp.stored[p] < 0;
-    HIPCHK(hipMemcpyAsync(d_mask, mask.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+    std::vector<int> ci, ct;
+    std::vector<double> cze;
+    for (int p = 0; p < np; p++)
+      if (sp.stored[p] < 0) {
+        ci.push_back(p);
+        ct.push_back(sp.t[p]);
+        cze.push_back(sp.ze[2 * p]);
+        cze.push_back(sp.ze[2 * p + 1]);
+      }
+    int nc = (int)ci.size();
+    if (nc > 0) {
+      int *d_ci, *d_ct;
+      double *d_cze;
+      if (upload(&d_ci, ci.data(), nc) || upload(&d_ct, ct.data(), nc) || upload(&d_cze, cze.data(), cze.size()))
+        return BH_EHIP;
+      bem_launch_field_bd(d_ct, d_cze, nc, 0, o->d_geo, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(), g_rule[2],
+                          g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
+                          d_tmp, d_rt, g.st);
+      hipLaunchKernelGGL(k_scatter6, g1(nc), dim3(256), 0, g.st, d_tmp, d_ci, nc, d_EH);
+      HIPCHK(hipStreamSynchronize(g.st));
+      dfree(d_ci); dfree(d_ct); dfree(d_cze);
+    }
   }
-  // field_bd computed every point; overwrite node points with stored values
-  HIPCHK(hipMemcpyAsync(d_EH, d_tmp, 6 * (size_t)np * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
   hipLaunchKernelGGL(k_gather_stored, g1(np), dim3(256), 0, g.st, d_t, d_st, np, D.d_E, D.d_Hf, d_EH);
   // other objects at the point r (their open domain, checked by solid angle)
   for (int m = 0; m < N; m++) {

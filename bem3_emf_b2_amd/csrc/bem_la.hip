@@ -13,6 +13,8 @@
 #include "bem_dev.h"
 #include "bem_prof.h"
 
+#include <algorithm>
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 #define GB_M 64
@@ -92,22 +94,76 @@ __global__ void __launch_bounds__(256) k_zgemm(int m, int n, int k, cd alpha, co
       }
 }
 
+// ---- skinny ZGEMM (n <= 8 right-hand sides) ----
+// Memory-bound products such as G1 N Y in the block solve: each wavefront
+// streams 4 rows of A along k (lanes over k, coalesced), the matching rows of
+// B (NC contiguous values per k) are reused across the 4 rows, and the
+// 4 x NC partial sums are reduced across the wave at the end.
+template <int NC>
+__global__ void __launch_bounds__(256) k_zgemm_skinny(int m, int n, int k, cd alpha, const cd *__restrict__ A,
+                                                      size_t lda, const cd *__restrict__ B, size_t ldb, cd beta,
+                                                      cd *__restrict__ C, size_t ldc)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row0 = (blockIdx.x * 4 + wave) * 4;
+  if (row0 >= m) return;
+  cd acc[4][NC];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) acc[r][j] = mk(0, 0);
+  const cd *a0 = A + (size_t)min(row0 + 0, m - 1) * lda, *a1 = A + (size_t)min(row0 + 1, m - 1) * lda;
+  const cd *a2 = A + (size_t)min(row0 + 2, m - 1) * lda, *a3 = A + (size_t)min(row0 + 3, m - 1) * lda;
+  for (int kk = lane; kk < k; kk += 64) {
+    cd b[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) b[j] = j < n ? B[(size_t)kk * ldb + j] : mk(0, 0);
+    const cd v[4] = {a0[kk], a1[kk], a2[kk], a3[kk]};
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int j = 0; j < NC; j++) acc[r][j] = acc[r][j] + v[r] * b[j];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int j = 0; j < NC; j++) acc[r][j] = wsum(acc[r][j]);
+  const bool bz = beta.x == 0.0 && beta.y == 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int j = 0; j < NC; j++)
+      if (lane == r * NC + j && row0 + r < m && j < n) {
+        cd *c = C + (size_t)(row0 + r) * ldc + j;
+        cd v = alpha * acc[r][j];
+        if (!bz) v = v + beta * (*c);
+        *c = v;
+      }
+}
+
 // ---- large-tile ZGEMM ----
-// 128x128 tile per 256-thread workgroup (2x2 waves of 64x64, i.e. 4x4
-// MFMA tiles of 16x16 per wave, real and imaginary accumulators), K slices
-// of 8 complex values double-buffered in LDS as split real/imaginary planes
-// (row padded by 4 doubles against bank conflicts), the next slice
-// prefetched into registers while the current one feeds the MFMAs.  Tiles
-// are assigned so that each XCD (workgroup id mod 8) works on a contiguous
-// band of C, grouped 8 tile-rows high, for L2 reuse of the A and B panels.
+// Complex product in the 3M (Gauss) form: T1 = Ar Br, T2 = Ai Bi,
+// T3 = (Ar + Ai)(Br + Bi), Cr = T1 - T2, Ci = T3 - T1 - T2: three real
+// v_mfma_f64_16x16x4f64 per 16x16x4 step instead of four (normwise stable,
+// the error bound of a real product of the summed operands).
+// 128x128 tile per 512-thread workgroup: 8 waves in a 2x4 grid, each wave a
+// 64x32 block = 4x2 MFMA tiles of 16x16 (three accumulators each, all in
+// VGPRs, two waves per SIMD).  K slices of 8 complex values are
+// double-buffered in LDS as six planes (Ar, Ai, Ar+Ai, Br, Bi, Br+Bi; rows
+// padded by 4 doubles against bank conflicts); the next slice is loaded into
+// registers from clamped, always-valid addresses before the current slice's
+// MFMAs and masked to zero outside the matrix only when written to LDS, so
+// the load latency overlaps the MFMAs.  Tiles are assigned so that each XCD
+// (workgroup id mod 8) works on a contiguous band of C, grouped 8 tile-rows
+// high, for L2 reuse of the A and B panels.
 #define ZT 128
 #define ZK 8
 #define ZP (ZT + 4)
-#define ZLDS (2 * 4 * ZK * ZP * (int)sizeof(double))
+#define ZLDS (2 * 6 * ZK * ZP * (int)sizeof(double))
 
-__global__ void __launch_bounds__(256) k_zgemm128(int m, int n, int k, cd alpha, const cd *__restrict__ A, size_t lda,
-                                                  const cd *__restrict__ B, size_t ldb, cd beta, cd *__restrict__ C,
-                                                  size_t ldc, int tiles_x, int tiles_y)
+__global__ void __launch_bounds__(512) k_zgemm128(int m, int n, int k, cd alpha, const cd *__restrict__ A, size_t lda,
+                                                const cd *__restrict__ B, size_t ldb, cd beta, cd *__restrict__ C,
+                                                size_t ldc, int tiles_x, int tiles_y)
 {
   extern __shared__ double zsm[];
   const int T = tiles_x * tiles_y, per = (T + 7) / 8;
@@ -118,31 +174,41 @@ __global__ void __launch_bounds__(256) k_zgemm128(int m, int n, int k, cd alpha,
   const int ty = first + (L % gw) % gsz, tx = (L % gw) / gsz;
   const int i0 = ty * ZT, j0 = tx * ZT;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  const int ra = tid >> 1, ca = (tid & 1) * 4;     // A slice: row, first k column
-  const int rb = tid >> 5, cb = (tid & 31) * 4;    // B slice: k row, first column
-  d4 cr[4][4], ci[4][4];
+  const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
+  const int ra = tid >> 2, ca = (tid & 3) * 2;
+  const int rb = tid >> 6, cb = (tid & 63) * 2;
+  d4 t1[4][2], t2[4][2], t3[4][2];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) { cr[a][b] = (d4){0, 0, 0, 0}; ci[a][b] = (d4){0, 0, 0, 0}; }
-  cd pa[4], pb[4];
+    for (int b = 0; b < 2; b++) { t1[a][b] = (d4){0, 0, 0, 0}; t2[a][b] = (d4){0, 0, 0, 0}; t3[a][b] = (d4){0, 0, 0, 0}; }
+  cd pa[2], pb[2];
+  bool oka[2], okb[2];
+  const int gi = min(i0 + ra, m - 1);
+  const bool rowok = i0 + ra < m;
   auto gload = [&](int k0) {
-    const int gi = i0 + ra, gk = k0 + rb;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      pa[q] = (gi < m && k0 + ca + q < k) ? A[(size_t)gi * lda + k0 + ca + q] : mk(0, 0);
-      pb[q] = (gk < k && j0 + cb + q < n) ? B[(size_t)gk * ldb + j0 + cb + q] : mk(0, 0);
+    for (int q = 0; q < 2; q++) {
+      int ka = k0 + ca + q, kb2 = k0 + rb, jb = j0 + cb + q;
+      pa[q] = A[(size_t)gi * lda + min(ka, k - 1)];
+      pb[q] = B[(size_t)min(kb2, k - 1) * ldb + min(jb, n - 1)];
+      oka[q] = rowok && ka < k;
+      okb[q] = kb2 < k && jb < n;
     }
   };
+  const int PL = ZK * ZP;   // one LDS plane
   auto lstore = [&](int s) {
-    double *Ar = zsm + s * 4 * ZK * ZP, *Ai = Ar + ZK * ZP, *Br = Ai + ZK * ZP, *Bi = Br + ZK * ZP;
+    double *Ar = zsm + s * 6 * PL, *Ai = Ar + PL, *As = Ai + PL, *Br = As + PL, *Bi = Br + PL, *Bs = Bi + PL;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      Ar[(ca + q) * ZP + ra] = pa[q].x;
-      Ai[(ca + q) * ZP + ra] = pa[q].y;
-      Br[rb * ZP + cb + q] = pb[q].x;
-      Bi[rb * ZP + cb + q] = pb[q].y;
+    for (int q = 0; q < 2; q++) {
+      double ax = oka[q] ? pa[q].x : 0.0, ay = oka[q] ? pa[q].y : 0.0;
+      double bx = okb[q] ? pb[q].x : 0.0, by = okb[q] ? pb[q].y : 0.0;
+      Ar[(ca + q) * ZP + ra] = ax;
+      Ai[(ca + q) * ZP + ra] = ay;
+      As[(ca + q) * ZP + ra] = ax + ay;
+      Br[rb * ZP + cb + q] = bx;
+      Bi[rb * ZP + cb + q] = by;
+      Bs[rb * ZP + cb + q] = bx + by;
     }
   };
   gload(0);
@@ -152,46 +218,48 @@ __global__ void __launch_bounds__(256) k_zgemm128(int m, int n, int k, cd alpha,
   for (int k0 = 0; k0 < k; k0 += ZK) {
     const bool more = k0 + ZK < k;
     if (more) gload(k0 + ZK);
-    const double *Ar = zsm + s * 4 * ZK * ZP, *Ai = Ar + ZK * ZP, *Br = Ai + ZK * ZP, *Bi = Br + ZK * ZP;
+    const double *Ar = zsm + s * 6 * PL, *Ai = Ar + PL, *As = Ai + PL, *Br = As + PL, *Bi = Br + PL, *Bs = Bi + PL;
 #pragma unroll
     for (int ks = 0; ks < ZK; ks += 4) {
       const int kr = ks + (lane >> 4), cc = lane & 15;
-      double ar[4], ai[4], nai[4], br[4], bi[4];
+      double ar[4], ai[4], as[4], br[2], bi[2], bs[2];
 #pragma unroll
       for (int a = 0; a < 4; a++) {
         ar[a] = Ar[kr * ZP + wr + 16 * a + cc];
         ai[a] = Ai[kr * ZP + wr + 16 * a + cc];
-        nai[a] = -ai[a];
-        br[a] = Br[kr * ZP + wc + 16 * a + cc];
-        bi[a] = Bi[kr * ZP + wc + 16 * a + cc];
+        as[a] = As[kr * ZP + wr + 16 * a + cc];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        br[b] = Br[kr * ZP + wc + 16 * b + cc];
+        bi[b] = Bi[kr * ZP + wc + 16 * b + cc];
+        bs[b] = Bs[kr * ZP + wc + 16 * b + cc];
       }
 #pragma unroll
       for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
-          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[a], bi[b], cr[a][b], 0, 0, 0);
-          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
-          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+        for (int b = 0; b < 2; b++) {
+          t1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], t1[a][b], 0, 0, 0);
+          t2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], t2[a][b], 0, 0, 0);
+          t3[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[a], bs[b], t3[a][b], 0, 0, 0);
         }
     }
     if (more) lstore(s ^ 1);
     __syncthreads();
     s ^= 1;
   }
-  // D layout of the 16x16 f64 MFMA (measured): lane l, register r -> row l/16 + 4r, col l%16
   const bool bz = beta.x == 0.0 && beta.y == 0.0;
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+    for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gi = i0 + wr + 16 * a + (lane >> 4) + 4 * r;
+        const int gi2 = i0 + wr + 16 * a + (lane >> 4) + 4 * r;
         const int gj = j0 + wc + 16 * b + (lane & 15);
-        if (gi < m && gj < n) {
-          cd *c = C + (size_t)gi * ldc + gj;
-          cd v = alpha * mk(cr[a][b][r], ci[a][b][r]);
+        if (gi2 < m && gj < n) {
+          cd *c = C + (size_t)gi2 * ldc + gj;
+          cd v = alpha * mk(t1[a][b][r] - t2[a][b][r], t3[a][b][r] - t1[a][b][r] - t2[a][b][r]);
           if (!bz) v = v + beta * (*c);
           *c = v;
         }
@@ -199,16 +267,14 @@ __global__ void __launch_bounds__(256) k_zgemm128(int m, int n, int k, cd alpha,
 }
 
 // ---- LU panel ----
-// The nb-column panel (rows k0..n-1) is factorised in a column-major copy P
-// (P[c * ldp + r], r = row - k0) without moving rows: the pivot of column j is
-// searched among the rows not yet used as pivots, and used[r] records the
-// column at which row r became a pivot (INT_MAX while unused).  One launch
-// per column: it applies the elimination of column j-1 to the thread's own
-// row, then finds the per-workgroup maximum of column j; the last workgroup
-// to finish (atomic ticket) reduces the partial maxima and publishes the
-// pivot.  Rows are interchanged afterwards, once per panel, in LAPACK's
-// sequential-swap form.
+// The panel (at most 64 columns, rows k0..n-1) is factorised in a
+// column-major copy P (P[c * ldp + r], r = row - k0) without moving rows: the
+// pivot of column j is searched among the rows not yet used as pivots, and
+// used[r] records the column at which row r became a pivot (INT_MAX while
+// unused).  One launch per column (see k_panel_col).  Rows are interchanged
+// afterwards, once per panel, in LAPACK's sequential-swap form.
 #define LU_NT 256
+#define PERM_LDS_MAX (2 * 16384 * (int)sizeof(int))   // position maps of panels up to 16384 rows
 
 __global__ void __launch_bounds__(256) k_panel_load(const cd *__restrict__ A, size_t lda, int k0, int kb, int m,
                                                     cd *__restrict__ P, size_t ldp, int *__restrict__ used)
@@ -227,14 +293,16 @@ __global__ void __launch_bounds__(256) k_panel_load(const cd *__restrict__ A, si
   if (tid < 32 && r0 + tid < m) used[r0 + tid] = 0x7fffffff;
 }
 
+// panel back into A with the panel's row interchanges applied: position t
+// of the panel receives physical panel row rowAt[t]
 __global__ void __launch_bounds__(256) k_panel_store(cd *__restrict__ A, size_t lda, int k0, int kb, int m,
-                                                     const cd *__restrict__ P, size_t ldp)
+                                                     const cd *__restrict__ P, size_t ldp, const int *__restrict__ rowAt)
 {
   __shared__ cd t[64][33];
   const int r0 = blockIdx.x * 32, tid = threadIdx.x;
   for (int idx = tid; idx < 32 * 64; idx += 256) {
     int c = idx >> 5, rr = idx & 31;
-    if (r0 + rr < m && c < kb) t[c][rr] = P[(size_t)c * ldp + r0 + rr];
+    if (r0 + rr < m && c < kb) t[c][rr] = P[(size_t)c * ldp + rowAt[r0 + rr]];
   }
   __syncthreads();
   for (int idx = tid; idx < 32 * 64; idx += 256) {
@@ -243,103 +311,131 @@ __global__ void __launch_bounds__(256) k_panel_store(cd *__restrict__ A, size_t 
   }
 }
 
+// (value, index) maximum with the smallest index on ties, over one wave
+__device__ __forceinline__ void wave_argmax(double &v, int &i)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o, BEM_WAVE);
+    int oi = __shfl_xor(i, o, BEM_WAVE);
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+  }
+}
+
+// Launch j of the panel: (1) every workgroup reduces the per-workgroup
+// maxima that launch j-1 left for column j-1 (the same deterministic result
+// everywhere: no atomics, no fences; the kernel boundary orders the data),
+// records the pivot and stages the pivot row in LDS; (2) eliminates column
+// j-1 on its own rows; (3) leaves its maximum of column j for launch j+1.
+// Maxima of consecutive columns alternate between two halves of partV/partI.
 __global__ void __launch_bounds__(LU_NT) k_panel_col(cd *__restrict__ P, size_t ldp, int m, int kb, int j,
                                                      int *__restrict__ used, int *__restrict__ pv,
-                                                     unsigned long long *__restrict__ partV, int *__restrict__ partI,
-                                                     unsigned int *__restrict__ ticket, int *__restrict__ info, int k0)
+                                                     double *__restrict__ partV, int *__restrict__ partI, int nbmax,
+                                                     int *__restrict__ info, int k0)
 {
-  __shared__ double sv[LU_NT];
-  __shared__ int si[LU_NT];
-  __shared__ bool last;
-  const int tid = threadIdx.x, r = blockIdx.x * LU_NT + tid;
-  // elimination of column j-1 on this row
-  if (j > 0 && r < m && used[r] > j - 1) {
-    const int p = pv[j - 1];
-    // p is a row of the panel (published by the previous launch); the check
-    // only keeps a corrupted workspace from turning into a wild access
-    const bool ok = p >= 0 && p < m;
-    const cd a = ok ? P[(size_t)(j - 1) * ldp + p] : mk(0, 0);
+  __shared__ cd prow[64];
+  __shared__ double sv[LU_NT / 64];
+  __shared__ int si[LU_NT / 64];
+  __shared__ int sp;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, r = blockIdx.x * LU_NT + tid;
+  int p = -1;
+  if (j > 0) {
+    const double *pV = partV + ((j - 1) & 1) * nbmax;
+    const int *pI = partI + ((j - 1) & 1) * nbmax;
+    double b = -1.0;
+    int bi = 0x7fffffff;
+    for (int q = tid; q < (int)gridDim.x; q += LU_NT)
+      if (pV[q] > b || (pV[q] == b && pI[q] < bi)) { b = pV[q]; bi = pI[q]; }
+    wave_argmax(b, bi);
+    if (lane == 0) { sv[wv] = b; si[wv] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int q = 1; q < LU_NT / 64; q++)
+        if (sv[q] > b || (sv[q] == b && si[q] < bi)) { b = sv[q]; bi = si[q]; }
+      bool bad = bi < 0 || bi >= m;   // never: m - j + 1 >= 1 rows were candidates
+      if (bad) bi = m - 1;
+      sp = bi;
+      if (blockIdx.x == 0) {
+        pv[j - 1] = bi;
+        if ((bad || !(b > 0.0)) && *info == 0) *info = k0 + j;
+      }
+    }
+    __syncthreads();
+    p = sp;
+    if (r == p) used[r] = j - 1;   // the owner of the pivot row marks it
+    if (tid < kb - j + 1) prow[tid] = P[(size_t)(j - 1 + tid) * ldp + p];
+    __syncthreads();
+  }
+  // rows not yet used as pivots: eliminated for column j-1, candidates for column j
+  const bool act = r < m && r != p && used[r] > j - 1;
+  cd xj = mk(0, 0);   // this row's value in column j after the elimination
+  if (act && j < kb) xj = P[(size_t)j * ldp + r];
+  if (act && j > 0) {
+    const cd a = prow[0];
     if (a.x != 0.0 || a.y != 0.0) {
-      cd l = cdiv(P[(size_t)(j - 1) * ldp + r], a);
+      const cd l = cdiv(P[(size_t)(j - 1) * ldp + r], a);
       P[(size_t)(j - 1) * ldp + r] = l;
-      for (int c = j; c < kb; c++) P[(size_t)c * ldp + r] = P[(size_t)c * ldp + r] - l * P[(size_t)c * ldp + p];
+      // columns j .. kb-1 in batches of 16 independent loads
+      for (int c0 = j; c0 < kb; c0 += 16) {
+        cd x[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+          if (c0 + q < kb) x[q] = P[(size_t)(c0 + q) * ldp + r];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+          if (c0 + q < kb) {
+            x[q] = x[q] - l * prow[c0 + q - j + 1];
+            P[(size_t)(c0 + q) * ldp + r] = x[q];
+          }
+        if (c0 == j) xj = x[0];
+      }
     }
   }
   if (j >= kb) return;
-  // candidates for the pivot of column j
+  // this workgroup's pivot candidate for column j
   double best = -1.0;
   int bidx = 0x7fffffff;
-  if (r < m && used[r] > j - 1) {
-    cd v = P[(size_t)j * ldp + r];
-    best = fabs(v.x) + fabs(v.y);   // |re|+|im| as LAPACK izamax
+  if (act) {
+    best = fabs(xj.x) + fabs(xj.y);         // |re|+|im| as LAPACK izamax
     if (!(best >= 0.0)) best = INFINITY;   // a NaN entry still wins over non-candidates
     bidx = r;
   }
-  sv[tid] = best;
-  si[tid] = bidx;
+  wave_argmax(best, bidx);
+  if (lane == 0) { sv[wv] = best; si[wv] = bidx; }
   __syncthreads();
-  for (int s = LU_NT / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      double o = sv[tid + s];
-      int oi = si[tid + s];
-      if (o > sv[tid] || (o == sv[tid] && oi < si[tid])) { sv[tid] = o; si[tid] = oi; }
-    }
-    __syncthreads();
-  }
   if (tid == 0) {
-    __hip_atomic_store(&partV[blockIdx.x], (unsigned long long)__double_as_longlong(sv[0]), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&partI[blockIdx.x], si[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    last = t == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double b = -1.0;
-  int bi = 0x7fffffff;
-  for (int q = tid; q < (int)gridDim.x; q += LU_NT) {
-    double o = __longlong_as_double(
-        (long long)__hip_atomic_load(&partV[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    int oi = __hip_atomic_load(&partI[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (o > b || (o == b && oi < bi)) { b = o; bi = oi; }
-  }
-  sv[tid] = b;
-  si[tid] = bi;
-  __syncthreads();
-  for (int s = LU_NT / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      double o = sv[tid + s];
-      int oi = si[tid + s];
-      if (o > sv[tid] || (o == sv[tid] && oi < si[tid])) { sv[tid] = o; si[tid] = oi; }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    int p = si[0];   // a candidate row: m - j >= 1 rows are unused at column j
-    if (p >= m) { p = m - 1; if (*info == 0) *info = k0 + j + 1; }
-    pv[j] = p;
-    used[p] = j;
-    if (!(sv[0] > 0.0) && *info == 0) *info = k0 + j + 1;
-    *ticket = 0u;
+    for (int q = 1; q < LU_NT / 64; q++)
+      if (sv[q] > best || (sv[q] == best && si[q] < bidx)) { best = sv[q]; bidx = si[q]; }
+    partV[(j & 1) * nbmax + blockIdx.x] = best;
+    partI[(j & 1) * nbmax + blockIdx.x] = bidx;
   }
 }
 
 // pivot rows of the panel -> LAPACK interchanges piv[k0+j] (absolute rows)
-__global__ void k_panel_ipiv(const int *__restrict__ pv, int kb, int k0, int *__restrict__ piv)
+// and the panel's final row order rowAt[t] (physical panel row at position
+// t).  LAPACK swaps position j with the current position of pivot row pv[j];
+// the position map and its inverse live in LDS (dynamic, 2 m ints) when
+// they fit, else in the global scratch arrays.  One thread performs the kb
+// swaps, O(1) each.
+__global__ void __launch_bounds__(256) k_panel_perm(const int *__restrict__ pv, int kb, int k0, int m, int *__restrict__ piv,
+                                                    int *__restrict__ rowAtG, int *__restrict__ posG, int in_lds)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int ip[64];
-  for (int j = 0; j < kb; j++) {
-    int cur = pv[j];
-    for (int q = 0; q < j; q++) {
-      if (cur == q) cur = ip[q];
-      else if (cur == ip[q]) cur = q;
+  extern __shared__ int pm[];
+  int *rowAt = in_lds ? pm : rowAtG;
+  int *pos = in_lds ? pm + m : posG;
+  for (int t = threadIdx.x; t < m; t += blockDim.x) { rowAt[t] = t; pos[t] = t; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < kb; j++) {
+      const int x = pv[j], cur = pos[x], a = rowAt[j];
+      piv[k0 + j] = k0 + cur;
+      rowAt[j] = x; pos[x] = j;
+      rowAt[cur] = a; pos[a] = cur;
     }
-    ip[j] = cur;
-    piv[k0 + j] = k0 + cur;
   }
+  __syncthreads();
+  if (in_lds)
+    for (int t = threadIdx.x; t < m; t += blockDim.x) rowAtG[t] = rowAt[t];
 }
 
 // apply interchanges piv[k0..k1) to columns [c0, c1) of A (or of a RHS block)
@@ -357,45 +453,113 @@ __global__ void __launch_bounds__(256) k_laswp(cd *A, size_t lda, int c0, int c1
   }
 }
 
-// X := L^{-1} X with L the unit-lower nb x nb block at A(k0,k0), X the
-// nb x ncol block at B (row stride ldb); 64 columns per workgroup
-__global__ void __launch_bounds__(64) k_trsm_lunit(const cd *__restrict__ L, size_t lda, int nb, cd *X, size_t ldb,
-                                                   int ncol)
+// Triangular solves with one nb x nb (nb <= 64) diagonal block for ncol
+// right-hand-side columns, 64 columns per 256-thread workgroup.  The block
+// sits in LDS; a column's rows are spread over 4 neighbouring lanes (lane g
+// of the quad owns rows 4i + g, i < 16, in registers), so step q of the
+// substitution broadcasts x_q inside the quad with a shuffle and every lane
+// updates its rows at once: nb dependent steps of 16 independent complex
+// FMAs.  The interleaved ownership makes the four rows a quad reads together
+// consecutive rows of the LDS block (distinct banks with the padded stride).
+#define TRS_LD 65
+__global__ void __launch_bounds__(256) k_trsm_lunit(const cd *__restrict__ L, size_t lda, int nb, cd *X, size_t ldb,
+                                                    int ncol)
 {
-  extern __shared__ double sm[];
-  cd *sL = (cd *)sm;                  // nb x nb
-  cd *sX = sL + nb * nb;              // nb x 64
-  const int c = threadIdx.x, gc = blockIdx.x * 64 + c;
-  for (int idx = c; idx < nb * nb; idx += 64) sL[idx] = L[(size_t)(idx / nb) * lda + idx % nb];
-  for (int r = 0; r < nb; r++) sX[r * 64 + c] = gc < ncol ? X[(size_t)r * ldb + gc] : mk(0, 0);
-  __syncthreads();
-  for (int r = 1; r < nb; r++) {
-    cd acc = sX[r * 64 + c];
-    for (int q = 0; q < r; q++) acc = acc - sL[r * nb + q] * sX[q * 64 + c];
-    sX[r * 64 + c] = acc;
+  extern __shared__ double tsm[];
+  cd *sL = (cd *)tsm;   // nb x TRS_LD
+  const int tid = threadIdx.x, lane = tid & 63, g = lane & 3;
+  const int col = blockIdx.x * 64 + (tid >> 6) * 16 + (lane >> 2);
+  for (int idx = tid; idx < nb * nb; idx += 256) sL[(idx / nb) * TRS_LD + idx % nb] = L[(size_t)(idx / nb) * lda + idx % nb];
+  cd x[16];
+  const bool cok = col < ncol;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = 4 * i + g;
+    x[i] = (cok && r < nb) ? X[(size_t)r * ldb + col] : mk(0, 0);
   }
-  if (gc < ncol)
-    for (int r = 0; r < nb; r++) X[(size_t)r * ldb + gc] = sX[r * 64 + c];
+  __syncthreads();
+  const int qsrc = lane & ~3;
+  // branch-free steps: the 16 factor entries of a step are read together
+  // (rows clamped into the block) and masked by a select
+#pragma unroll
+  for (int q = 0; q < 64; q++) {
+    if (q < nb) {
+      cd lv[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) lv[i] = sL[min(4 * i + g, nb - 1) * TRS_LD + q];
+      const double vx = __shfl(x[q >> 2].x, qsrc | (q & 3), BEM_WAVE);
+      const double vy = __shfl(x[q >> 2].y, qsrc | (q & 3), BEM_WAVE);
+      const cd xq = mk(vx, vy);
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int r = 4 * i + g;
+        const bool on = r > q && r < nb;
+        const cd c = mk(on ? lv[i].x : 0.0, on ? lv[i].y : 0.0);
+        x[i] = x[i] - c * xq;
+      }
+    }
+  }
+  if (cok)
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int r = 4 * i + g;
+      if (r < nb) X[(size_t)r * ldb + col] = x[i];
+    }
+}
+
+// one step q of the backward substitution, then the steps q-1 .. 0
+template <int Q>
+__device__ __forceinline__ void trsm_upper_steps(cd (&x)[16], const cd *sU, const cd *dinv, int nb, int g, int qsrc)
+{
+  if (Q < nb) {
+    cd uv[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) uv[i] = sU[min(4 * i + g, nb - 1) * TRS_LD + Q];
+    const cd xo = x[Q >> 2] * dinv[Q];   // used by the owner lanes only
+    if (g == (Q & 3)) x[Q >> 2] = xo;
+    const double vx = __shfl(x[Q >> 2].x, qsrc | (Q & 3), BEM_WAVE);
+    const double vy = __shfl(x[Q >> 2].y, qsrc | (Q & 3), BEM_WAVE);
+    const cd xq = mk(vx, vy);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int r = 4 * i + g;
+      const bool on = r < Q;
+      const cd c = mk(on ? uv[i].x : 0.0, on ? uv[i].y : 0.0);
+      x[i] = x[i] - c * xq;
+    }
+  }
+  if constexpr (Q > 0) trsm_upper_steps<Q - 1>(x, sU, dinv, nb, g, qsrc);
 }
 
 // X := U^{-1} X with U the upper nb x nb block (non-unit diagonal)
-__global__ void __launch_bounds__(64) k_trsm_upper(const cd *__restrict__ U, size_t lda, int nb, cd *X, size_t ldb,
-                                                   int ncol)
+__global__ void __launch_bounds__(256) k_trsm_upper(const cd *__restrict__ U, size_t lda, int nb, cd *X, size_t ldb,
+                                                    int ncol)
 {
-  extern __shared__ double sm[];
-  cd *sU = (cd *)sm;
-  cd *sX = sU + nb * nb;
-  const int c = threadIdx.x, gc = blockIdx.x * 64 + c;
-  for (int idx = c; idx < nb * nb; idx += 64) sU[idx] = U[(size_t)(idx / nb) * lda + idx % nb];
-  for (int r = 0; r < nb; r++) sX[r * 64 + c] = gc < ncol ? X[(size_t)r * ldb + gc] : mk(0, 0);
-  __syncthreads();
-  for (int r = nb - 1; r >= 0; r--) {
-    cd acc = sX[r * 64 + c];
-    for (int q = r + 1; q < nb; q++) acc = acc - sU[r * nb + q] * sX[q * 64 + c];
-    sX[r * 64 + c] = cdiv(acc, sU[r * nb + r]);
+  extern __shared__ double tsm[];
+  cd *sU = (cd *)tsm;
+  __shared__ cd dinv[64];   // reciprocals of the diagonal
+  const int tid = threadIdx.x, lane = tid & 63, g = lane & 3;
+  const int col = blockIdx.x * 64 + (tid >> 6) * 16 + (lane >> 2);
+  for (int idx = tid; idx < nb * nb; idx += 256) sU[(idx / nb) * TRS_LD + idx % nb] = U[(size_t)(idx / nb) * lda + idx % nb];
+  if (tid < nb) dinv[tid] = cdiv(mk(1.0, 0.0), U[(size_t)tid * lda + tid]);
+  cd x[16];
+  const bool cok = col < ncol;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = 4 * i + g;
+    x[i] = (cok && r < nb) ? X[(size_t)r * ldb + col] : mk(0, 0);
   }
-  if (gc < ncol)
-    for (int r = 0; r < nb; r++) X[(size_t)r * ldb + gc] = sX[r * 64 + c];
+  __syncthreads();
+  const int qsrc = lane & ~3;
+  // the 64 steps are written out through a template so that every x index
+  // is a compile-time constant (a runtime index would put x in scratch)
+  trsm_upper_steps<63>(x, sU, dinv, nb, g, qsrc);
+  if (cok)
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int r = 4 * i + g;
+      if (r < nb) X[(size_t)r * ldb + col] = x[i];
+    }
 }
 
 extern "C" {
@@ -406,9 +570,10 @@ static bool g_big_tiles = false;   // set once the 128x128 kernel's LDS limit is
 
 int bem_la_init(void)
 {
-  const int bytes = 2 * 64 * 64 * (int)sizeof(cd);
+  const int bytes = 64 * TRS_LD * (int)sizeof(cd);
   if (hipFuncSetAttribute((const void *)k_trsm_lunit, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void *)k_trsm_upper, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void *)k_panel_perm, hipFuncAttributeMaxDynamicSharedMemorySize, PERM_LDS_MAX) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void *)k_zgemm128, hipFuncAttributeMaxDynamicSharedMemorySize, ZLDS) != hipSuccess) return -1;
   g_big_tiles = true;
   return 0;
@@ -419,10 +584,25 @@ void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t 
 {
   if (m <= 0 || n <= 0) return;
   bem_prof_begin(BH_K_ZGEMM, 8.0 * m * n * k, st);
-  if (g_big_tiles && (size_t)m * n >= (size_t)512 * ZT * ZT) {
-    // enough 128x128 tiles to fill the chip twice over
-    int tx = (n + ZT - 1) / ZT, ty = (m + ZT - 1) / ZT, T = tx * ty;
-    hipLaunchKernelGGL(k_zgemm128, dim3(8 * ((T + 7) / 8)), dim3(256), ZLDS, st, m, n, k, mk(ar, ai), (const cd *)A,
+  const int tx = (n + ZT - 1) / ZT, ty = (m + ZT - 1) / ZT, T = tx * ty;
+  if (n <= 8 && k >= 256) {
+    // a few right-hand sides against a long k: stream A once
+    dim3 grid((m + 15) / 16);
+    if (n == 1)
+      hipLaunchKernelGGL(k_zgemm_skinny<1>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
+                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+    else if (n == 2)
+      hipLaunchKernelGGL(k_zgemm_skinny<2>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
+                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+    else if (n <= 4)
+      hipLaunchKernelGGL(k_zgemm_skinny<4>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
+                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+    else
+      hipLaunchKernelGGL(k_zgemm_skinny<8>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
+                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+  } else if (g_big_tiles && T >= 256 && k >= 32) {
+    // at least one 128x128 tile per CU
+    hipLaunchKernelGGL(k_zgemm128, dim3(8 * ((T + 7) / 8)), dim3(512), ZLDS, st, m, n, k, mk(ar, ai), (const cd *)A,
                        lda, (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc, tx, ty);
   } else {
     dim3 grid((n + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M);
@@ -437,18 +617,103 @@ void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t 
 static void *g_lu_ws = nullptr;
 static size_t g_lu_ws_bytes = 0;
 
-// LU factorisation in place (nb <= 64).  piv (device, n ints) receives
-// 0-based pivot rows; info (device int) receives 0 or the 1-based index of
-// the first zero pivot.  Returns 0, or -1 if the scratch cannot be allocated.
+// workspace of the panel factorisation for one matrix order
+struct LuWs {
+  cd *P;
+  size_t ldp;
+  int *used, *pv, *partI, *info, *rowAt, *pos;
+  double *partV;
+  int nbmax;
+};
+
+// split point of a recursive step: about half, a multiple of the 64-wide base
+static inline int half64(int n) { return std::max(64, (n / 2) / 64 * 64); }
+
+// columns [k0, k0+kb) (kb <= 64) of rows [k0, n): pivots into piv[k0..k0+kb)
+// (absolute rows), rows interchanged inside these columns only
+static void lu_panel(cd *A, size_t lda, int n, int k0, int kb, int *piv, const LuWs &w, hipStream_t st)
+{
+  const int m = n - k0, gb = (m + LU_NT - 1) / LU_NT;
+  // about 8 m kb^2 / 2 real flops of rank-1 updates
+  bem_prof_begin(BH_K_LU_PANEL, 4.0 * (double)m * kb * kb, st);
+  hipLaunchKernelGGL(k_panel_load, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, w.P, w.ldp, w.used);
+  for (int j = 0; j <= kb; j++)
+    hipLaunchKernelGGL(k_panel_col, dim3(gb), dim3(LU_NT), 0, st, w.P, w.ldp, m, kb, j, w.used, w.pv, w.partV, w.partI,
+                       w.nbmax, w.info, k0);
+  const int in_lds = 2 * (size_t)m * sizeof(int) <= (size_t)PERM_LDS_MAX;
+  hipLaunchKernelGGL(k_panel_perm, dim3(1), dim3(256), in_lds ? 2 * (size_t)m * sizeof(int) : 0, st, w.pv, kb, k0, m, piv,
+                     w.rowAt, w.pos, in_lds);
+  hipLaunchKernelGGL(k_panel_store, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, w.P, w.ldp, w.rowAt);
+  bem_prof_end(BH_K_LU_PANEL, st);
+}
+
+// X := L^{-1} X with L the nr x nr unit-lower block at L (stride lda) and X
+// nr x ncol at B (stride ldb); recursive halving so that the off-diagonal
+// work is large-K ZGEMM, 64x64 diagonal blocks solved in LDS
+static void trsm_lunit_rec(const cd *L, size_t lda, int nr, cd *B, size_t ldb, int ncol, hipStream_t st)
+{
+  if (nr <= 64) {
+    size_t shm = (size_t)nr * TRS_LD * sizeof(cd);
+    bem_prof_begin(BH_K_TRSM, 4.0 * nr * nr * (double)ncol, st);
+    hipLaunchKernelGGL(k_trsm_lunit, dim3((ncol + 63) / 64), dim3(256), shm, st, L, lda, nr, B, ldb, ncol);
+    bem_prof_end(BH_K_TRSM, st);
+    return;
+  }
+  const int h = half64(nr);
+  trsm_lunit_rec(L, lda, h, B, ldb, ncol, st);
+  bem_zgemm(nr - h, ncol, h, -1.0, 0.0, L + (size_t)h * lda, lda, B, ldb, 1.0, 0.0, B + (size_t)h * ldb, ldb, st);
+  trsm_lunit_rec(L + (size_t)h * lda + h, lda, nr - h, B + (size_t)h * ldb, ldb, ncol, st);
+}
+
+// X := U^{-1} X with U the nr x nr upper block (non-unit diagonal)
+static void trsm_upper_rec(const cd *U, size_t lda, int nr, cd *B, size_t ldb, int ncol, hipStream_t st)
+{
+  if (nr <= 64) {
+    size_t shm = (size_t)nr * TRS_LD * sizeof(cd);
+    bem_prof_begin(BH_K_TRSM, 4.0 * nr * nr * (double)ncol, st);
+    hipLaunchKernelGGL(k_trsm_upper, dim3((ncol + 63) / 64), dim3(256), shm, st, U, lda, nr, B, ldb, ncol);
+    bem_prof_end(BH_K_TRSM, st);
+    return;
+  }
+  const int h = half64(nr);
+  trsm_upper_rec(U + (size_t)h * lda + h, lda, nr - h, B + (size_t)h * ldb, ldb, ncol, st);
+  bem_zgemm(h, ncol, nr - h, -1.0, 0.0, U + h, lda, B + (size_t)h * ldb, ldb, 1.0, 0.0, B, ldb, st);
+  trsm_upper_rec(U, lda, h, B, ldb, ncol, st);
+}
+
+// recursive LU (left half, then the right half after its update) of columns
+// [k0, k0+nc) over rows [k0, n); every interchange found below reaches every
+// column of the block through the two laswp calls
+static void lu_rec(cd *A, size_t lda, int n, int k0, int nc, int *piv, const LuWs &w, hipStream_t st)
+{
+  if (nc <= 64) {
+    lu_panel(A, lda, n, k0, nc, piv, w, st);
+    return;
+  }
+  const int h = half64(nc);
+  lu_rec(A, lda, n, k0, h, piv, w, st);
+  hipLaunchKernelGGL(k_laswp, dim3((nc - h + 255) / 256), dim3(256), 0, st, A, lda, k0 + h, k0 + nc, k0, k0 + h, piv);
+  trsm_lunit_rec(A + (size_t)k0 * lda + k0, lda, h, A + (size_t)k0 * lda + k0 + h, lda, nc - h, st);
+  bem_zgemm(n - k0 - h, nc - h, h, -1.0, 0.0, A + (size_t)(k0 + h) * lda + k0, lda, A + (size_t)k0 * lda + k0 + h, lda,
+            1.0, 0.0, A + (size_t)(k0 + h) * lda + k0 + h, lda, st);
+  lu_rec(A, lda, n, k0 + h, nc - h, piv, w, st);
+  hipLaunchKernelGGL(k_laswp, dim3((h + 255) / 256), dim3(256), 0, st, A, lda, k0, k0 + h, k0 + h, k0 + nc, piv);
+}
+
+// LU factorisation in place, LAPACK's pivot choice and interchange vector.
+// piv (device, n ints) receives 0-based pivot rows; info (device int)
+// receives 0 or the 1-based index of the first zero pivot.  Returns 0, or -1
+// if the scratch cannot be allocated.
 int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int nb, hipStream_t st)
 {
   (void)work;
+  (void)nb;
   cd *A = (cd *)Av;
-  if (nb > 64) nb = 64;
+  const int kbmax = 64;
   size_t ldp = ((size_t)n + 63) / 64 * 64;
   int nblk = (n + LU_NT - 1) / LU_NT;
-  size_t need = 256 + (size_t)nb * ldp * sizeof(cd) + ldp * sizeof(int) + 64 * sizeof(int) +
-                (size_t)nblk * (sizeof(unsigned long long) + sizeof(int));
+  size_t need = (size_t)kbmax * ldp * sizeof(cd) + 3 * ldp * sizeof(int) + 64 * sizeof(int) +
+                2 * (size_t)nblk * (sizeof(double) + sizeof(int));
   if (need > g_lu_ws_bytes) {
     if (g_lu_ws) {
       (void)hipStreamSynchronize(st);   // earlier factorisations may still use it
@@ -459,70 +724,35 @@ int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int
     if (hipMalloc(&g_lu_ws, need) != hipSuccess) return -1;
     g_lu_ws_bytes = need;
   }
-  // layout for this n (the buffer may be larger, from an earlier call):
-  // ticket at a fixed offset, cleared on every call
-  char *w = (char *)g_lu_ws;
-  unsigned int *ticket = (unsigned int *)w; w += 256;
-  cd *P = (cd *)w;                       w += (size_t)nb * ldp * sizeof(cd);
-  int *used = (int *)w;                  w += ldp * sizeof(int);
-  int *pv = (int *)w;                    w += 64 * sizeof(int);
-  unsigned long long *partV = (unsigned long long *)w; w += (size_t)nblk * sizeof(unsigned long long);
-  int *partI = (int *)w;
-  (void)hipMemsetAsync(ticket, 0, sizeof(unsigned int), st);
+  // layout for this n (the buffer may be larger, from an earlier call);
+  // nothing in it needs clearing: the panel loader initialises used[], every
+  // launch writes the maxima the next one reads
+  char *p = (char *)g_lu_ws;
+  LuWs w;
+  w.P = (cd *)p;                         p += (size_t)kbmax * ldp * sizeof(cd);
+  w.ldp = ldp;
+  w.used = (int *)p;                     p += ldp * sizeof(int);
+  w.rowAt = (int *)p;                    p += ldp * sizeof(int);
+  w.pos = (int *)p;                      p += ldp * sizeof(int);
+  w.pv = (int *)p;                       p += 64 * sizeof(int);
+  w.partV = (double *)p;                 p += 2 * (size_t)nblk * sizeof(double);
+  w.partI = (int *)p;
+  w.nbmax = nblk;
+  w.info = info;
   (void)hipMemsetAsync(info, 0, sizeof(int), st);
-  for (int k0 = 0; k0 < n; k0 += nb) {
-    int kb = n - k0 < nb ? n - k0 : nb, k1 = k0 + kb, m = n - k0;
-    int gb = (m + LU_NT - 1) / LU_NT;
-    // panel: about 8 m kb^2 / 2 real flops of rank-1 updates
-    bem_prof_begin(BH_K_LU_PANEL, 4.0 * (double)m * kb * kb, st);
-    hipLaunchKernelGGL(k_panel_load, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, P, ldp, used);
-    for (int j = 0; j <= kb; j++)
-      hipLaunchKernelGGL(k_panel_col, dim3(gb), dim3(LU_NT), 0, st, P, ldp, m, kb, j, used, pv, partV, partI, ticket,
-                         info, k0);
-    hipLaunchKernelGGL(k_panel_store, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, P, ldp);
-    hipLaunchKernelGGL(k_panel_ipiv, dim3(1), dim3(64), 0, st, pv, kb, k0, piv);
-    bem_prof_end(BH_K_LU_PANEL, st);
-    // interchanges on every column (the panel was factorised in place order)
-    hipLaunchKernelGGL(k_laswp, dim3((n + 255) / 256), dim3(256), 0, st, A, lda, 0, n, k0, k1, piv);
-    if (k1 < n) {
-      int nr = n - k1;
-      size_t shm = (size_t)(kb * kb + kb * 64) * sizeof(cd);
-      bem_prof_begin(BH_K_TRSM, 4.0 * kb * kb * (double)nr, st);
-      hipLaunchKernelGGL(k_trsm_lunit, dim3((nr + 63) / 64), dim3(64), shm, st, A + (size_t)k0 * lda + k0, lda, kb,
-                         A + (size_t)k0 * lda + k1, lda, nr);
-      bem_prof_end(BH_K_TRSM, st);
-      bem_zgemm(nr, nr, kb, -1.0, 0.0, A + (size_t)k1 * lda + k0, lda, A + (size_t)k0 * lda + k1, lda, 1.0, 0.0,
-                A + (size_t)k1 * lda + k1, lda, st);
-    }
-  }
+  if (n > 0) lu_rec(A, lda, n, 0, n, piv, w, st);
   return 0;
 }
 
 // solve A X = B with the factors of bem_zgetrf; B is n x nrhs (row stride ldb)
 void bem_zgetrs(int n, int nrhs, const void *Av, size_t lda, const int *piv, void *Bv, size_t ldb, int nb, hipStream_t st)
 {
+  (void)nb;
   const cd *A = (const cd *)Av;
   cd *B = (cd *)Bv;
+  if (n <= 0 || nrhs <= 0) return;
   hipLaunchKernelGGL(k_laswp, dim3((nrhs + 255) / 256), dim3(256), 0, st, B, ldb, 0, nrhs, 0, n, piv);
-  // forward substitution with the unit lower factor
-  for (int k0 = 0; k0 < n; k0 += nb) {
-    int kb = n - k0 < nb ? n - k0 : nb, k1 = k0 + kb;
-    size_t shm = (size_t)(kb * kb + kb * 64) * sizeof(cd);
-    hipLaunchKernelGGL(k_trsm_lunit, dim3((nrhs + 63) / 64), dim3(64), shm, st, A + (size_t)k0 * lda + k0, lda, kb,
-                       B + (size_t)k0 * ldb, ldb, nrhs);
-    if (k1 < n)
-      bem_zgemm(n - k1, nrhs, kb, -1.0, 0.0, A + (size_t)k1 * lda + k0, lda, B + (size_t)k0 * ldb, ldb, 1.0, 0.0,
-                B + (size_t)k1 * ldb, ldb, st);
-  }
-  // backward substitution with the upper factor
-  int last = ((n - 1) / nb) * nb;
-  for (int k0 = last; k0 >= 0; k0 -= nb) {
-    int kb = n - k0 < nb ? n - k0 : nb;
-    size_t shm = (size_t)(kb * kb + kb * 64) * sizeof(cd);
-    hipLaunchKernelGGL(k_trsm_upper, dim3((nrhs + 63) / 64), dim3(64), shm, st, A + (size_t)k0 * lda + k0, lda, kb,
-                       B + (size_t)k0 * ldb, ldb, nrhs);
-    if (k0 > 0)
-      bem_zgemm(k0, nrhs, kb, -1.0, 0.0, A + k0, lda, B + (size_t)k0 * ldb, ldb, 1.0, 0.0, B, ldb, st);
-  }
+  trsm_lunit_rec(A, lda, n, B, ldb, nrhs, st);
+  trsm_upper_rec(A, lda, n, B, ldb, nrhs, st);
 }
 }

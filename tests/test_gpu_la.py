@@ -9,9 +9,11 @@ def _rand(rng, *shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-# the last two shapes take the 128x128-tile kernel (m n >= 512 tiles of 128^2)
+# (3001, 2999, 77) and (4096, 4096, 64) take the 128x128-tile kernel (at least
+# 256 tiles), the n <= 8 shapes with k >= 256 the skinny kernel
 @pytest.mark.parametrize("m,n,k", [(64, 64, 16), (100, 70, 37), (257, 130, 64), (5, 300, 3), (3001, 2999, 77),
-                                   (4096, 4096, 64)])
+                                   (4096, 4096, 64), (3001, 1, 500), (1000, 6, 2048), (777, 3, 300), (17, 8, 999),
+                                   (4000, 2, 256)])
 def test_zgemm_matches_numpy(m, n, k):
     import bem3_emf_b2_amd as bem
     rng = np.random.default_rng(m * 1000 + n + k)
