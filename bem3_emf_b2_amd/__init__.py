@@ -90,7 +90,8 @@ def sync():
 
 
 # kernel ids of the timing registry (bem3_hip.h)
-KERNELS = {"asm_gl": 0, "asm_gh": 1, "asm_self": 2, "zgemm": 3, "lu_panel": 4, "trsm": 5, "field": 6, "scatter": 7}
+KERNELS = {"asm_gl": 0, "asm_gh": 1, "asm_self": 2, "zgemm": 3, "lu_panel": 4, "trsm": 5, "field": 6, "scatter": 7,
+           "zgemm_small": 8, "field_bd": 9}
 
 
 def prof_enable(on=True):

@@ -51,6 +51,7 @@ static int fail(int code, const char *fmt, ...)
   return code;
 }
 static void dfree(void *p) { (void)hipFree(p); }
+static int lu_failed(void) { return fail(BH_EHIP, "LU factorisation: scratch allocation failed or a panel barrier timed out"); }
 
 // Entry points are serialised: callers may call the scalar field functions
 // from their own thread pools (e.g. OpenMP loops over grid points); all work
@@ -479,7 +480,7 @@ extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
   HIPCHK(hipMalloc((void **)&info, sizeof(int)));
   HIPCHK(hipMemcpy(dA, A, (size_t)n * n * sizeof(cd), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dB, B, (size_t)n * nrhs * sizeof(cd), hipMemcpyHostToDevice));
-  if (bem_zgetrf(n, dA, n, piv, work, info, 64, g.st)) return fail(BH_EHIP, "LU scratch allocation failed");
+  if (bem_zgetrf(n, dA, n, piv, work, info, 64, g.st)) return lu_failed();
   bem_zgetrs(n, nrhs, dA, n, piv, dB, nrhs, 64, g.st);
   HIPCHK(hipGetLastError());
   int hinfo = 0;
@@ -737,7 +738,7 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
       Dom &D = o->dom[d];
       bem_launch_build_A(D.d_G, D.d_H, D.nd, D.nd, D.d_col, row_base(o, d), NN, o->d_A, na, g.st);
     }
-    if (bem_zgetrf((int)na, o->d_A, na, o->d_piv, o->d_work, o->d_info, 64, g.st)) return fail(BH_EHIP, "LU scratch allocation failed");
+    if (bem_zgetrf((int)na, o->d_A, na, o->d_piv, o->d_work, o->d_info, 64, g.st)) return lu_failed();
   } else {
     // one medium: domain 0 rows H0 u - G0 q = 0, domain 1 rows couple through phi
     Dom &D0 = o->dom[0], &D1 = o->dom[1];
@@ -756,7 +757,7 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
     // Z = G0^{-1} H0
     HIPCHK(hipMemcpyAsync(o->d_G0f, D0.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
     HIPCHK(hipMemcpyAsync(o->d_Z, D0.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    if (bem_zgetrf(n, o->d_G0f, NN, o->d_piv0, o->d_work, o->d_info, 64, g.st)) return fail(BH_EHIP, "LU scratch allocation failed");
+    if (bem_zgetrf(n, o->d_G0f, NN, o->d_piv0, o->d_work, o->d_info, 64, g.st)) return lu_failed();
     bem_zgetrs(n, n, o->d_G0f, NN, o->d_piv0, o->d_Z, NN, 64, g.st);
     // M = H1 + G1 Z  (Mf), P = G1 Z kept in Sf for the beta term
     HIPCHK(hipMemcpyAsync(o->d_Mf, D1.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
@@ -764,7 +765,7 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
     bem_launch_axpby(o->d_Mf, 1.0, 0.0, o->d_Sf, o->d_Mf, n2, g.st);
     // S starts as H1 + beta P
     bem_launch_axpby(D1.d_H, beps.real(), beps.imag(), o->d_Sf, o->d_Sf, n2, g.st);
-    if (bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, g.st)) return fail(BH_EHIP, "LU scratch allocation failed");
+    if (bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, g.st)) return lu_failed();
     // W = M^{-1} G1, T = W o (n n^T), S -= sigma delta G1 T
     cd *W = o->d_A;
     bool tmpW = false;
@@ -775,7 +776,7 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
     zd sd = -(sigm * delt);
     bem_zgemm(n, n, n, sd.real(), sd.imag(), D1.d_G, NN, W, NN, 1.0, 0.0, o->d_Sf, NN, g.st);
     if (tmpW) { HIPCHK(hipStreamSynchronize(g.st)); dfree(W); }
-    if (bem_zgetrf(n, o->d_Sf, NN, o->d_pivS, o->d_work, o->d_info, 64, g.st)) return fail(BH_EHIP, "LU scratch allocation failed");
+    if (bem_zgetrf(n, o->d_Sf, NN, o->d_pivS, o->d_work, o->d_info, 64, g.st)) return lu_failed();
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, g.st));
@@ -899,7 +900,7 @@ extern "C" int bh_object_solve_EH(bh_object *o)
       if (dalloc(&D.d_Gf, nd * nd) || dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
       HIPCHK(hipMemcpyAsync(D.d_Gf, D.d_G, nd * nd * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
       if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
-      if (bem_zgetrf(D.nd, D.d_Gf, nd, D.d_Gpiv, o->d_work, o->d_info, 64, g.st)) return fail(BH_EHIP, "LU scratch allocation failed");
+      if (bem_zgetrf(D.nd, D.d_Gf, nd, D.d_Gpiv, o->d_work, o->d_info, 64, g.st)) return lu_failed();
     }
     cd *P = nullptr, *R = nullptr;
     HIPCHK(hipMalloc((void **)&P, nd * 6 * sizeof(cd)));

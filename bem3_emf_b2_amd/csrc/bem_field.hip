@@ -456,9 +456,11 @@ void bem_launch_field_bd(const int *tgt_t, const double *tgt_ze, int np, int did
                          const void *H, const void *dH, void *out, double *rt_out, hipStream_t st)
 {
   if (np <= 0) return;
+  bem_prof_begin(BH_K_FIELD_BD, (double)np * dNe, st);
   hipLaunchKernelGGL(k_field_bd, dim3((np + 3) / 4), dim3(256), 0, st, tgt_t, tgt_ze, np, did, (const TriGeo *)geo, sid,
                      dNe, kr, ki, (const QPtN *)gl, ngl, scale, xl, wl, xh, wh, (const cd *)E, (const cd *)dE,
                      (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
+  bem_prof_end(BH_K_FIELD_BD, st);
 }
 void bem_launch_incident_bd(const double *pts, const double *nrm, const int *slot, int np, const void *bm, int nb,
                             void *Ui, void *dUi, void *Ui0, void *dUi0, void *Uip, hipStream_t st)

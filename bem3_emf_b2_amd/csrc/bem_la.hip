@@ -411,6 +411,127 @@ __global__ void __launch_bounds__(LU_NT) k_panel_col(cd *__restrict__ P, size_t 
   }
 }
 
+// ---- one-launch panel (cooperative) ----
+// The whole panel in one cooperative launch: one 64-lane workgroup per 64
+// panel rows, the rows held in LDS (column-major, 64 KB), a grid barrier per
+// column.  Column j: every workgroup publishes its best candidate (maximum
+// |re|+|im| among its rows not yet used as pivots, smallest row on ties)
+// together with that row's entries j..kb-1; after the barrier every
+// workgroup reduces the candidates to the same pivot, takes the pivot row
+// from the published copy and eliminates column j on its rows.  Each lane
+// tracks the position of its row under LAPACK's sequential interchanges
+// (interchange j is published by the pivot row's owner and applied by all
+// after the next barrier), and the rows are stored once, at their final
+// positions, after the last column.  The barrier spins with a bounded count:
+// on timeout the error flag is raised and every wave still runs to the end.
+#define COOP_SPIN_LIMIT (1 << 24)
+struct CoopWs {
+  double *candV;         // [2][nbmax]
+  int *candI;            // [2][nbmax]
+  cd *candRow;           // [2][nbmax][64]
+  int *ip;               // [64] interchange of each column (panel positions)
+  unsigned int *cnt;
+  int *err;
+  int nbmax;
+};
+
+// Grid barrier on a monotonically increasing arrival count (cleared before
+// the launch): barrier e of the launch completes when the count reaches
+// (e + 1) * nb.  The count and everything exchanged across workgroups live
+// in uncached device memory, so visibility needs no cache write-back or
+// invalidation: a workgroup waits for its own stores to complete before it
+// arrives, and compiler barriers keep the exchange loads after the poll.
+__device__ __forceinline__ void coop_barrier(unsigned int *cnt, int *err, unsigned int &target, unsigned int nb)
+{
+  target += nb;
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
+  if ((threadIdx.x & 63) == 0) {
+    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
+           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      if (++spins > COOP_SPIN_LIMIT) { atomicExch(err, 1); break; }   // after one timeout nobody waits
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__global__ void __launch_bounds__(64) k_panel_coop(cd *__restrict__ A, size_t lda, int k0, int kb, int m,
+                                                   int *__restrict__ piv, int *__restrict__ info, CoopWs w)
+{
+  __shared__ cd S[64 * 64];   // S[c * 64 + lane]: column c of this workgroup's 64 rows
+  __shared__ cd sp[64];       // pivot row of the current column
+  const int lane = threadIdx.x, blk = blockIdx.x, nb = gridDim.x;
+  const int r = blk * 64 + lane;
+  const bool live = r < m;
+  unsigned int target = 0;
+  for (int c = 0; c < kb; c++) S[c * 64 + lane] = live ? A[(size_t)(k0 + r) * lda + k0 + c] : mk(0, 0);
+  bool used = false;
+  int pos = r;
+  for (int j = 0; j < kb; j++) {
+    // this workgroup's candidate for column j
+    double v = -1.0;
+    int vi = 0x7fffffff;
+    if (live && !used) {
+      const cd x = S[j * 64 + lane];
+      v = fabs(x.x) + fabs(x.y);           // |re|+|im| as LAPACK izamax
+      if (!(v >= 0.0)) v = INFINITY;      // a NaN entry still wins over non-candidates
+      vi = r;
+    }
+    wave_argmax(v, vi);
+    const int buf = j & 1;
+    if (lane == 0) { w.candV[buf * w.nbmax + blk] = v; w.candI[buf * w.nbmax + blk] = vi; }
+    if (v >= 0.0 && lane == vi - blk * 64)
+      for (int c = j; c < kb; c++) w.candRow[((size_t)buf * w.nbmax + blk) * 64 + c] = S[c * 64 + lane];
+    coop_barrier(w.cnt, w.err, target, nb);
+    // the interchange of column j-1 (published before this barrier)
+    if (j > 0) {
+      const int cj = ((volatile int *)w.ip)[j - 1];
+      if (pos == j - 1) pos = cj; else if (pos == cj) pos = j - 1;
+    }
+    // the pivot: maximum over the workgroups' candidates
+    double b = -1.0;
+    int bi = 0x7fffffff, bw = 0;
+    for (int q = lane; q < nb; q += 64) {
+      const double o = ((volatile double *)w.candV)[buf * w.nbmax + q];
+      const int oi = ((volatile int *)w.candI)[buf * w.nbmax + q];
+      if (o > b || (o == b && oi < bi)) { b = o; bi = oi; }
+    }
+    wave_argmax(b, bi);
+    bw = bi / 64;
+    if (bi < 0 || bi >= m) { bi = m - 1; bw = bi / 64; if (blk == 0 && lane == 0 && *info == 0) *info = k0 + j + 1; }
+    if (blk == 0 && lane == 0 && !(b > 0.0) && *info == 0) *info = k0 + j + 1;
+    if (lane >= j && lane < kb) {
+      const volatile double *src = (const volatile double *)(w.candRow + ((size_t)buf * w.nbmax + bw) * 64 + lane);
+      sp[lane] = mk(src[0], src[1]);
+    }
+    __syncthreads();
+    if (r == bi) {
+      used = true;
+      w.ip[j] = pos;   // LAPACK interchange: column j's position <-> the pivot row's current position
+    }
+    const cd a = sp[j];
+    if (live && !used && (a.x != 0.0 || a.y != 0.0)) {
+      const cd l = cdiv(S[j * 64 + lane], a);
+      S[j * 64 + lane] = l;
+      for (int c = j + 1; c < kb; c++) S[c * 64 + lane] = S[c * 64 + lane] - l * sp[c];
+    }
+    __syncthreads();
+  }
+  coop_barrier(w.cnt, w.err, target, nb);
+  {
+    const int cj = ((volatile int *)w.ip)[kb - 1];
+    if (pos == kb - 1) pos = cj; else if (pos == cj) pos = kb - 1;
+  }
+  if (blk == 0 && lane < kb) piv[k0 + lane] = k0 + ((volatile int *)w.ip)[lane];
+  // every row to its final position (a permutation of 0..m-1; all reads of
+  // the panel happened before the first barrier)
+  if (live)
+    for (int c = 0; c < kb; c++) A[(size_t)(k0 + pos) * lda + k0 + c] = S[c * 64 + lane];
+}
+
 // pivot rows of the panel -> LAPACK interchanges piv[k0+j] (absolute rows)
 // and the panel's final row order rowAt[t] (physical panel row at position
 // t).  LAPACK swaps position j with the current position of pivot row pv[j];
@@ -567,6 +688,13 @@ extern "C" {
 // the triangular-solve kernels stage a 64x64 factor block and a 64x64 RHS
 // block (128 KiB) in LDS, above the default dynamic-LDS limit
 static bool g_big_tiles = false;   // set once the 128x128 kernel's LDS limit is raised
+static bool g_coop = false;        // the device supports cooperative launches
+// exchange area of the one-launch panel: up to COOP_NBMAX workgroups
+// (panels of up to 65536 rows)
+#define COOP_NBMAX 1024
+#define COOP_BYTES (256 + 2 * COOP_NBMAX * (sizeof(double) + sizeof(int)) + 64 * sizeof(int) + \
+                    2 * (size_t)COOP_NBMAX * 64 * sizeof(cd))
+static void *g_coop_buf = nullptr;
 
 int bem_la_init(void)
 {
@@ -576,6 +704,9 @@ int bem_la_init(void)
   if (hipFuncSetAttribute((const void *)k_panel_perm, hipFuncAttributeMaxDynamicSharedMemorySize, PERM_LDS_MAX) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void *)k_zgemm128, hipFuncAttributeMaxDynamicSharedMemorySize, ZLDS) != hipSuccess) return -1;
   g_big_tiles = true;
+  int dev = 0, coop = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess)
+    g_coop = coop != 0;
   return 0;
 }
 
@@ -583,8 +714,10 @@ void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t 
                double br, double bi, void *C, size_t ldc, hipStream_t st)
 {
   if (m <= 0 || n <= 0) return;
-  bem_prof_begin(BH_K_ZGEMM, 8.0 * m * n * k, st);
   const int tx = (n + ZT - 1) / ZT, ty = (m + ZT - 1) / ZT, T = tx * ty;
+  const bool big = g_big_tiles && !(n <= 8 && k >= 256) && T >= 256 && k >= 32;
+  const int kid = big ? BH_K_ZGEMM : BH_K_ZGEMM_SMALL;
+  bem_prof_begin(kid, 8.0 * m * n * k, st);
   if (n <= 8 && k >= 256) {
     // a few right-hand sides against a long k: stream A once
     dim3 grid((m + 15) / 16);
@@ -600,7 +733,7 @@ void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t 
     else
       hipLaunchKernelGGL(k_zgemm_skinny<8>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
                          (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
-  } else if (g_big_tiles && T >= 256 && k >= 32) {
+  } else if (big) {
     // at least one 128x128 tile per CU
     hipLaunchKernelGGL(k_zgemm128, dim3(8 * ((T + 7) / 8)), dim3(512), ZLDS, st, m, n, k, mk(ar, ai), (const cd *)A,
                        lda, (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc, tx, ty);
@@ -609,7 +742,7 @@ void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t 
     hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda, (const cd *)B, ldb,
                        mk(br, bi), (cd *)C, ldc);
   }
-  bem_prof_end(BH_K_ZGEMM, st);
+  bem_prof_end(kid, st);
 }
 
 // grow-only scratch of the panel factorisation (device memory kept for the
@@ -624,6 +757,8 @@ struct LuWs {
   int *used, *pv, *partI, *info, *rowAt, *pos;
   double *partV;
   int nbmax;
+  CoopWs coop;
+  bool coop_ok;
 };
 
 // split point of a recursive step: about half, a multiple of the 64-wide base
@@ -636,6 +771,17 @@ static void lu_panel(cd *A, size_t lda, int n, int k0, int kb, int *piv, const L
   const int m = n - k0, gb = (m + LU_NT - 1) / LU_NT;
   // about 8 m kb^2 / 2 real flops of rank-1 updates
   bem_prof_begin(BH_K_LU_PANEL, 4.0 * (double)m * kb * kb, st);
+  const int nbc = (m + 63) / 64;
+  if (w.coop_ok && nbc <= w.coop.nbmax) {
+    CoopWs cw = w.coop;
+    (void)hipMemsetAsync(cw.cnt, 0, sizeof(unsigned int), st);   // arrival count of this launch
+    void *args[] = {&A, &lda, (void *)&k0, (void *)&kb, (void *)&m, &piv, (void *)&w.info, &cw};
+    if (hipLaunchCooperativeKernel((const void *)k_panel_coop, dim3(nbc), dim3(64), args, 0, st) == hipSuccess) {
+      bem_prof_end(BH_K_LU_PANEL, st);
+      return;
+    }
+    (void)hipGetLastError();   // not co-resident: the launch-per-column path below
+  }
   hipLaunchKernelGGL(k_panel_load, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, w.P, w.ldp, w.used);
   for (int j = 0; j <= kb; j++)
     hipLaunchKernelGGL(k_panel_col, dim3(gb), dim3(LU_NT), 0, st, w.P, w.ldp, m, kb, j, w.used, w.pv, w.partV, w.partI,
@@ -712,6 +858,7 @@ int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int
   const int kbmax = 64;
   size_t ldp = ((size_t)n + 63) / 64 * 64;
   int nblk = (n + LU_NT - 1) / LU_NT;
+  const int nbc = (n + 63) / 64;   // workgroups of the one-launch panel
   size_t need = (size_t)kbmax * ldp * sizeof(cd) + 3 * ldp * sizeof(int) + 64 * sizeof(int) +
                 2 * (size_t)nblk * (sizeof(double) + sizeof(int));
   if (need > g_lu_ws_bytes) {
@@ -736,11 +883,39 @@ int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int
   w.pos = (int *)p;                      p += ldp * sizeof(int);
   w.pv = (int *)p;                       p += 64 * sizeof(int);
   w.partV = (double *)p;                 p += 2 * (size_t)nblk * sizeof(double);
-  w.partI = (int *)p;
+  w.partI = (int *)p;                    p += 2 * (size_t)nblk * sizeof(int);
   w.nbmax = nblk;
   w.info = info;
+  // one-launch panel: exchange area in uncached memory (allocated once for
+  // the life of the process, never freed: memory a kernel has read uncached
+  // is not returned to the allocator)
+  w.coop_ok = false;
+  if (g_coop && nbc <= COOP_NBMAX) {
+    if (!g_coop_buf && hipExtMallocWithFlags(&g_coop_buf, COOP_BYTES, hipDeviceMallocUncached) != hipSuccess) {
+      g_coop_buf = nullptr;
+      g_coop = false;
+    }
+    if (g_coop_buf) {
+      char *q = (char *)g_coop_buf;
+      w.coop.cnt = (unsigned int *)q;
+      w.coop.err = (int *)q + 1;             q += 256;
+      w.coop.candV = (double *)q;            q += 2 * COOP_NBMAX * sizeof(double);
+      w.coop.candI = (int *)q;               q += 2 * COOP_NBMAX * sizeof(int);
+      w.coop.ip = (int *)q;                  q += 64 * sizeof(int);
+      w.coop.candRow = (cd *)q;
+      w.coop.nbmax = COOP_NBMAX;
+      w.coop_ok = true;
+    }
+  }
   (void)hipMemsetAsync(info, 0, sizeof(int), st);
+  if (w.coop_ok) (void)hipMemsetAsync(w.coop.cnt, 0, 2 * sizeof(int), st);   // barrier count, error flag
   if (n > 0) lu_rec(A, lda, n, 0, n, piv, w, st);
+  if (w.coop_ok) {
+    int herr = 0;
+    if (hipMemcpyAsync(&herr, w.coop.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess || herr != 0)
+      return -2;   // a panel barrier timed out
+  }
   return 0;
 }
 

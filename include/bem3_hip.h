@@ -121,9 +121,10 @@ int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, con
  * bh_prof_read returns the summed kernel time (ms), the summed algorithmic
  * work and the launch count of one kernel since the last enable.  Work units:
  * BH_K_ASM_GL, BH_K_ASM_GH, BH_K_ASM_SELF: (row node, source element) pairs;
- * BH_K_ZGEMM: real flops (8 m n k); BH_K_LU_PANEL: real flops of the
- * unblocked panel updates; BH_K_TRSM: real flops; BH_K_FIELD, BH_K_SCATTER:
- * (point, source element) pairs.                                          */
+ * BH_K_ZGEMM (the 128x128-tile kernel) and BH_K_ZGEMM_SMALL (64x64-tile and
+ * few-right-hand-side kernels): real flops (8 m n k); BH_K_LU_PANEL: real
+ * flops of the unblocked panel updates; BH_K_TRSM: real flops; BH_K_FIELD,
+ * BH_K_SCATTER, BH_K_FIELD_BD: (point, source element) pairs.              */
 #define BH_K_ASM_GL   0
 #define BH_K_ASM_GH   1
 #define BH_K_ASM_SELF 2
@@ -132,7 +133,9 @@ int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, con
 #define BH_K_TRSM     5
 #define BH_K_FIELD    6
 #define BH_K_SCATTER  7
-#define BH_K_COUNT    8
+#define BH_K_ZGEMM_SMALL 8
+#define BH_K_FIELD_BD 9
+#define BH_K_COUNT    10
 int bh_prof_enable(int on);
 int bh_prof_read(int kid, double *ms, double *work, int *launches);
 
