@@ -243,9 +243,9 @@ def mo_solve(objs, ccd=1e-6, iter_min=10, iter_max=100):
 
 def mo_field(objs, pts, what="EH", mode="t", type_=0):
     pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
-    w = {"EH": 0, "mEMP": 1}[what]
+    w = {"EH": 0, "mEMP": 1, "EH_mEMP": 2}[what]
     m = {"s": 0, "t": 1, "i": 2}[mode]
-    nc = 6 if w == 0 else 4
+    nc = 4 if w == 1 else 6
     out = np.zeros(pts.shape[0] * nc, np.complex128)
     oid = np.zeros(pts.shape[0], np.int32)
     check(lib().bh_mo_field(_handles(objs), len(objs), w, m, type_, pts.shape[0], dp(pts), dp(out.view(np.float64)),

@@ -183,6 +183,7 @@ struct Dom {
   cd *d_U = nullptr, *d_dU = nullptr, *d_E = nullptr, *d_dE = nullptr, *d_Hf = nullptr, *d_dHf = nullptr;
   cd *d_Gf = nullptr;                  // LU factors of G (normal derivatives of E, H)
   int *d_Gpiv = nullptr;
+  bool gf_valid = false;               // d_Gf factors the current G (cleared by assembly)
   // face table for the solid-angle domain test (faces whose main side is d)
 };
 
@@ -209,7 +210,7 @@ struct bh_object {
   Beam *d_beams = nullptr;
   double lambda0 = 0.0;
   // factorisation: 0 none, 1 dense 8NN, 2 Schur (one medium)
-  int fmode = 0;
+  int fmode = 0;                       // factorisation of the current matrices (cleared by assembly)
   cd *d_A = nullptr, *d_work = nullptr;
   int *d_piv = nullptr, *d_info = nullptr;
   cd *d_G0f = nullptr, *d_Z = nullptr, *d_Mf = nullptr, *d_Sf = nullptr;
@@ -442,6 +443,9 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
 {
   API_LOCK;
   if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
+  // every factorisation of earlier matrices is stale from here on
+  o->fmode = 0;
+  for (Dom &D : o->dom) D.gf_valid = false;
   for (int d = 0; d <= o->MN; d++) {
     int rc = assemble_domain(o, d);
     if (rc) return rc;
@@ -528,6 +532,9 @@ void bem_launch_scale_rows(void *, const void *, int, int, double, double, hipSt
 void bem_launch_domain(const double *, int, const double *, const int *, const int *, int, int *, hipStream_t);
 void bem_launch_field(const double *, const int *, int, int, const void *, const int *, int, double, double, const void *,
                       int, double, int, const void *, const void *, const void *, const void *, void *, int, hipStream_t);
+void bem_launch_field_memp(const double *, const int *, int, int, const void *, const int *, int, double, double,
+                           const void *, int, double, int, const void *, int, double, const void *, const void *,
+                           double, void *, hipStream_t);
 void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const int *, int, double, double,
                         const void *, int, const void *, int, double, const void *, const void *, void *, void *,
                         hipStream_t);
@@ -896,18 +903,28 @@ extern "C" int bh_object_solve_EH(bh_object *o)
     size_t nd = (size_t)D.nd;
     bem_launch_eh_rows(D.d_dG, D.d_dH, D.nd, nd, D.d_dF, D.d_rowEH, D.d_colS, D.d_col, D.d_U, D.d_dU, ch.real(),
                        ch.imag(), D.d_E, D.d_Hf, g.st);
-    if (!D.d_Gf) {
-      if (dalloc(&D.d_Gf, nd * nd) || dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
+    // LU of G_d: domain 0 of the block form already factorised G0 for this
+    // assembly; otherwise factorise G_d once per assembly
+    const cd *Gf = D.d_Gf;
+    const int *Gpiv = D.d_Gpiv;
+    if (d == 0 && o->fmode == 2) {
+      Gf = o->d_G0f;
+      Gpiv = o->d_piv0;
+    } else if (!D.gf_valid) {
+      if (!D.d_Gf && (dalloc(&D.d_Gf, nd * nd) || dalloc(&D.d_Gpiv, nd))) return BH_EHIP;
       HIPCHK(hipMemcpyAsync(D.d_Gf, D.d_G, nd * nd * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
       if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
       if (bem_zgetrf(D.nd, D.d_Gf, nd, D.d_Gpiv, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+      D.gf_valid = true;
+      Gf = D.d_Gf;
+      Gpiv = D.d_Gpiv;
     }
     cd *P = nullptr, *R = nullptr;
     HIPCHK(hipMalloc((void **)&P, nd * 6 * sizeof(cd)));
     HIPCHK(hipMalloc((void **)&R, nd * 6 * sizeof(cd)));
     bem_launch_pack_eh(D.nd, D.d_col, D.d_colS, D.d_E, D.d_Hf, P, g.st);
     bem_zgemm(D.nd, 6, D.nd, 1.0, 0.0, D.d_H, nd, P, 6, 0.0, 0.0, R, 6, g.st);
-    bem_zgetrs(D.nd, 6, D.d_Gf, nd, D.d_Gpiv, R, 6, 64, g.st);
+    bem_zgetrs(D.nd, 6, Gf, nd, Gpiv, R, 6, 64, g.st);
     bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, g.st);
     HIPCHK(hipStreamSynchronize(g.st));
     dfree(P);
@@ -976,14 +993,14 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
                            int *oid)
 {
   API_LOCK;
-  if (N <= 0 || np < 0 || type < 0 || type > 3 || what < 0 || what > 1) return fail(BH_EARG, "bad field request");
+  if (N <= 0 || np < 0 || type < 0 || type > 3 || what < 0 || what > 2) return fail(BH_EARG, "bad field request");
   if (np == 0) return 0;
   if (ensure_rules()) return BH_EHIP;
   for (int m = 0; m < N; m++) {
     if (!objs[m]->d_beams) return fail(BH_ESTATE, "object %d has no beams", m);
     if (mode != 2 && !objs[m]->dom[0].d_U) return fail(BH_ESTATE, "object %d is not solved", m);
   }
-  int nc = what == 0 ? 6 : 4;
+  int nc = what == 1 ? 4 : 6;
   double *d_pts;
   cd *d_out;
   int *d_did, *d_sel, *d_mask;
@@ -1011,6 +1028,13 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
       HIPCHK(hipMemcpyAsync(d_sel, sel.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
       for (int d = 0; d <= o->MN; d++) {
         Dom &D = o->dom[d];
+        if (what == 2) {
+          // E, H from the potentials' gradients (mo_EH_mEMP_s/t, mo_field.c:154-207)
+          bem_launch_field_memp(d_pts, d_sel, np, d, o->d_geo, D.d_sid, D.dNe, o->kn[d].real(), o->kn[d].imag(),
+                                g_rule[type], g_nrule[type], g_rscale[type], type < 3, g_rule[3], g_nrule[3],
+                                g_rscale[3], D.d_U, D.d_dU, o->lambda0, d_out, g.st);
+          continue;
+        }
         const void *A0 = what == 0 ? (const void *)D.d_E : (const void *)D.d_U;
         const void *dA0 = what == 0 ? (const void *)D.d_dE : (const void *)D.d_dU;
         bem_launch_field(d_pts, d_sel, np, d, o->d_geo, D.d_sid, D.dNe, o->kn[d].real(), o->kn[d].imag(), g_rule[type],
@@ -1023,7 +1047,7 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
   if (mode != 0) {
     for (int p = 0; p < np; p++) mask[p] = mode == 2 ? 1 : pin[p] < 0;
     HIPCHK(hipMemcpyAsync(d_mask, mask.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
-    bem_launch_add_incident(d_pts, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_out, what, g.st);
+    bem_launch_add_incident(d_pts, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_out, what == 1, g.st);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, d_out, (size_t)np * nc * sizeof(cd), hipMemcpyDeviceToHost, g.st));

@@ -75,6 +75,7 @@ __device__ __forceinline__ void tri_N(double z, double e, double &n0, double &n1
 #define BEM_I_SPSQ3 3.062938307898844719507e-2 // 1/(6 pi sqrt 3)
 #define BEM_DEPS 0.1
 #define BEM_CBD_SDF 0.5
+#define BEM_CBD_CDF 100.0   // field-derivative cut-off of EH_mEMP_s (d3b1_const.h:78)
 
 // one row of a domain's coefficient matrices: node slot `slot` of the
 // domain's t-th element (1-based), its position and unit tangents

@@ -197,6 +197,157 @@ __global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, c
   }
 }
 
+// ---------------------------------------------------------------- fields from potential gradients
+// G[3], H[3], F and their gradients with respect to rt (x, y, z) from one
+// triangle with a tabulated rule: the three dcoef_rt calls of EH_mEMP_s
+// (unit directions) share one evaluation of the kernel per point.
+__device__ __forceinline__ void tri_coef_grad(const TriGeo &g, const double *rt, double kr, double ki,
+                                              const QPtN *__restrict__ rule, int nq, double scale, bool flip, cd *G,
+                                              cd *H, double &F, cd (*dG)[3], cd (*dH)[3], double *dF)
+{
+  cd aG[3], aH[3], aDG[3][3], aDH[3][3];
+  double f = 0.0, df[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    aG[i] = mk(0, 0); aH[i] = mk(0, 0);
+#pragma unroll
+    for (int d = 0; d < 3; d++) { aDG[d][i] = mk(0, 0); aDH[d][i] = mk(0, 0); }
+  }
+  for (int p = 0; p < nq; p++) {
+    const QPtN q = rule[p];
+    double r[3];
+    tri_pos(g, q.z, q.e, r);
+    const double vR[3] = {r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]};
+    const double R = sqrt(vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]), iR = 1.0 / R, iR2 = iR * iR, iR3 = iR2 * iR;
+    const double rdW = (vR[0] * g.W[0] + vR[1] * g.W[1] + vR[2] * g.W[2]) * iR;
+    const cd ce = eikr(kr, ki, R);
+    const cd cb = mk(-1.0 - ki * R, kr * R);
+    const cd cbce = cb * ce;
+    const cd gk = ce * iR, hk = cbce * (rdW * iR2), tk = cbce * iR2;
+    const cd kR = mk(kr * R, ki * R);
+    const cd qq = kR * kR + cb * 3.0;
+    const cd ce3 = ce * iR3;
+    const double wn[3] = {q.w * q.n0, q.w * q.n1, q.w * q.n2};
+#pragma unroll
+    for (int i = 0; i < 3; i++) { fma_cr(aG[i], gk, wn[i]); fma_cr(aH[i], hk, wn[i]); }
+    f = fma(q.w, rdW * iR2, f);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double rd = vR[d] * iR;   // (r' - rt).e_d / R
+      const cd dh = ce3 * (qq * (rd * rdW) - cb * g.W[d]);
+#pragma unroll
+      for (int i = 0; i < 3; i++) { fma_cr(aDG[d][i], tk, wn[i] * rd); fma_cr(aDH[d][i], dh, wn[i]); }
+      df[d] = fma(q.w, iR3 * (3.0 * rd * rdW - g.W[d]), df[d]);
+    }
+  }
+  const double sh = flip ? -scale : scale;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    G[i] = aG[i] * (scale * g.aW);
+    H[i] = aH[i] * sh;
+#pragma unroll
+    for (int d = 0; d < 3; d++) { dG[d][i] = aDG[d][i] * (-scale * g.aW); dH[d][i] = aDH[d][i] * sh; }
+  }
+  F = f * sh;
+#pragma unroll
+  for (int d = 0; d < 3; d++) dF[d] = df[d] * sh;
+}
+
+// E, H at points from the potentials and their gradients (EH_mEMP_s,
+// d3b1_field.c:68-138): E = -grad phi + A, H = lambda0/(2 pi i) curl A.  Per
+// element and direction d the rule switches to GHN when |dF_d| > 0.5
+// (dcoef_rt, d3b1_elem.c:277-316); the potentials use the coefficients of
+// the last direction (z), as the reference's loop leaves them.  Points with
+// any |dF| > 100 contribute nothing (the reference returns -did there).
+// Output added to out (6 complex per point).
+__global__ void __launch_bounds__(256) k_field_memp(const double *__restrict__ pts, const int *__restrict__ pdid, int np,
+                                                    int want_did, const TriGeo *__restrict__ geo,
+                                                    const int *__restrict__ sid, int dNe, double kr, double ki,
+                                                    const QPtN *__restrict__ rule, int nq, double scale, int sw,
+                                                    const QPtN *__restrict__ gh, int ngh, double ghscale,
+                                                    const cd *__restrict__ U, const cd *__restrict__ dU, double lam0,
+                                                    cd *__restrict__ out)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  if (pdid[p] != want_did) return;
+  const double rt[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  cd aU[4], aD[3][4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) { aU[l] = mk(0, 0); aD[0][l] = mk(0, 0); aD[1][l] = mk(0, 0); aD[2][l] = mk(0, 0); }
+  double F = 0.0, dF[3] = {0.0, 0.0, 0.0};
+  for (int s = 1 + lane; s <= dNe; s += 64) {
+    const int sd = sid[s], as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    cd G[3], H[3], DG[3][3], DH[3][3];
+    double f, df[3];
+    tri_coef_grad(g, rt, kr, ki, rule, nq, scale, sd < 0, G, H, f, DG, DH, df);
+    if (sw) {
+      const bool s0 = fabs(df[0]) > BEM_CBD_SDF, s1 = fabs(df[1]) > BEM_CBD_SDF, s2 = fabs(df[2]) > BEM_CBD_SDF;
+      if (s0 || s1 || s2) {
+        cd tG[3], tH[3], tDG[3][3], tDH[3][3];
+        double tf, tdf[3];
+        tri_coef_grad(g, rt, kr, ki, gh, ngh, ghscale, sd < 0, tG, tH, tf, tDG, tDH, tdf);
+        const bool sws[3] = {s0, s1, s2};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          if (sws[d]) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) { DG[d][i] = tDG[d][i]; DH[d][i] = tDH[d][i]; }
+            df[d] = tdf[d];
+          }
+        if (s2) {
+#pragma unroll
+          for (int i = 0; i < 3; i++) { G[i] = tG[i]; H[i] = tH[i]; }
+          f = tf;
+        }
+      }
+    }
+    const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
+#pragma unroll
+    for (int n = 0; n < 3; n++)
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        const cd a = dUs[n * 4 + l], b = Us[n * 4 + l];
+        aU[l] = aU[l] + G[n] * a - H[n] * b;
+#pragma unroll
+        for (int d = 0; d < 3; d++) aD[d][l] = aD[d][l] + DG[d][n] * a - DH[d][n] * b;
+      }
+    F += f;
+#pragma unroll
+    for (int d = 0; d < 3; d++) dF[d] += df[d];
+  }
+  F = wsum(F);
+#pragma unroll
+  for (int d = 0; d < 3; d++) dF[d] = wsum(dF[d]);
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    aU[l] = wsum(aU[l]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) aD[d][l] = wsum(aD[d][l]);
+  }
+  if (lane == 0) {
+    if (fabs(dF[0]) > BEM_CBD_CDF || fabs(dF[1]) > BEM_CBD_CDF || fabs(dF[2]) > BEM_CBD_CDF) return;
+    const double iden = 1.0 / (want_did == 0 ? 1.0 + F : F);
+    cd u[4], du[3][4];
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      u[l] = aU[l] * iden;
+#pragma unroll
+      for (int d = 0; d < 3; d++) du[d][l] = (aD[d][l] - u[l] * dF[d]) * iden;
+    }
+    // lambda0 / (2 pi i) = -i lambda0 / (2 pi)
+    const cd ch = mk(0.0, -lam0 / (2.0 * M_PI));
+    cd *o = out + (size_t)p * 6;
+#pragma unroll
+    for (int i = 0; i < 3; i++) o[i] = o[i] + u[i] - du[i][3];
+    o[3] = o[3] + ch * (du[1][2] - du[2][1]);
+    o[4] = o[4] + ch * (du[2][0] - du[0][2]);
+    o[5] = o[5] + ch * (du[0][1] - du[1][0]);
+  }
+}
+
 // ---------------------------------------------------------------- coupling
 // U, dU/dv at node slots of another object from this object's open-domain
 // boundary values; GLN with the GHN switch per pair (dcoef_rt,
@@ -437,6 +588,18 @@ void bem_launch_field(const double *pts, const int *pdid, int np, int want_did, 
   hipLaunchKernelGGL(k_field, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo, sid,
                      dNe, kr, ki, (const QPtN *)rule, nq, scale, what, (const cd *)A0, (const cd *)dA0, (const cd *)A1,
                      (const cd *)dA1, (cd *)out, accumulate);
+  bem_prof_end(BH_K_FIELD, st);
+}
+void bem_launch_field_memp(const double *pts, const int *pdid, int np, int want_did, const void *geo, const int *sid,
+                           int dNe, double kr, double ki, const void *rule, int nq, double scale, int sw,
+                           const void *gh, int ngh, double ghscale, const void *U, const void *dU, double lam0,
+                           void *out, hipStream_t st)
+{
+  if (np <= 0) return;
+  bem_prof_begin(BH_K_FIELD, (double)np * dNe, st);
+  hipLaunchKernelGGL(k_field_memp, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo,
+                     sid, dNe, kr, ki, (const QPtN *)rule, nq, scale, sw, (const QPtN *)gh, ngh, ghscale, (const cd *)U,
+                     (const cd *)dU, lam0, (cd *)out);
   bem_prof_end(BH_K_FIELD, st);
 }
 void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const int *sid,

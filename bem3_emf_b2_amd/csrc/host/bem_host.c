@@ -937,17 +937,17 @@ int mo_mEMP_s(double complex *U, double *rt, int type, MOBJ *mo) { return field1
 int mo_mEMP_t(double complex *U, double *rt, int type, MOBJ *mo) { return field1(U, rt, type, mo, 1, 1); }
 int mo_mEMP_i(double complex *U, double *rt, int type, MOBJ *mo) { return field1(U, rt, type, mo, 1, 2); }
 
-static int eh1(double complex *E, double complex *H, double *rt, int type, MOBJ *mo, int mode)
+static int eh1(double complex *E, double complex *H, double *rt, int type, MOBJ *mo, int mode, int what)
 {
   double complex o[6];
-  int oid = field1(o, rt, type, mo, 0, mode);
+  int oid = field1(o, rt, type, mo, what, mode);
   for (int i = 0; i < 3; i++) { E[i] = o[i]; H[i] = o[3 + i]; }
   return oid;
 }
 
-int mo_EH_s(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 0); }
-int mo_EH_t(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 1); }
-int mo_EH_i(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 2); }
+int mo_EH_s(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 0, 0); }
+int mo_EH_t(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 1, 0); }
+int mo_EH_i(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 2, 0); }
 
 static int eh_batch(double complex *E, double complex *H, const double *rt, int np, int type, MOBJ *mo, int *oid, int mode)
 {
@@ -965,16 +965,10 @@ static int eh_batch(double complex *E, double complex *H, const double *rt, int 
 int mo_EH_t_batch(double complex *E, double complex *H, const double *rt, int np, int type, MOBJ *mo, int *oid) { return eh_batch(E, H, rt, np, type, mo, oid, 1); }
 int mo_EH_s_batch(double complex *E, double complex *H, const double *rt, int np, int type, MOBJ *mo, int *oid) { return eh_batch(E, H, rt, np, type, mo, oid, 0); }
 
-/* fields from the potentials' gradients: not provided by the device build
- * yet -- stops instead of computing them on the host */
-static int mEMP_unavailable(void)
-{
-  printf("mo_EH_mEMP_*(): the derivative-field formulation is not provided by this build. Exit...\n");
-  exit(1);
-}
-int mo_EH_mEMP_s(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { (void)E; (void)H; (void)rt; (void)type; (void)mo; return mEMP_unavailable(); }
-int mo_EH_mEMP_t(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { (void)E; (void)H; (void)rt; (void)type; (void)mo; return mEMP_unavailable(); }
-int mo_EH_mEMP_i(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return mo_EH_i(E, H, rt, type, mo); }
+/* E, H from the potentials' gradients (the far-field form of the reference) */
+int mo_EH_mEMP_s(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 0, 2); }
+int mo_EH_mEMP_t(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 1, 2); }
+int mo_EH_mEMP_i(double complex *E, double complex *H, double *rt, int type, MOBJ *mo) { return eh1(E, H, rt, type, mo, 2, 2); }
 
 static void eh_bd(double complex *E, double complex *H, int oid, int did, int t, double zt, double et, int type, MOBJ *mo, int mode)
 {

@@ -90,10 +90,12 @@ int bh_object_incident(bh_object *o, int which, double *buf, int set);
  * change measure (ccd_f) < ccd after more than iter_min iterations.
  * trace[iter] (optional) receives the change per iteration.              */
 int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters);
-/* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i; mo_field.c:118-282):
- * what 0: E,H (6 complex per point), 1: U (4 complex); mode 0 scattered +
- * internal, 1 total, 2 incident; type 0 4-point, 1 7-point, 2 GLN, 3 GHN.
- * oid[p] receives the object containing the point or -1.                 */
+/* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i, mo_EH_mEMP_s/t/i;
+ * mo_field.c:11-282): what 0: E,H (6 complex per point), 1: U (4 complex),
+ * 2: E,H from the potentials' gradients (EH_mEMP_s, d3b1_field.c:68-138,
+ * the far-field form); mode 0 scattered + internal, 1 total, 2 incident;
+ * type 0 4-point, 1 7-point, 2 GLN, 3 GHN.  oid[p] receives the object
+ * containing the point or -1.                                             */
 int bh_mo_field(bh_object **objs, int N, int what, int mode, int type, int np, const double *pts, double *out, int *oid);
 /* domain ids of np points w.r.t. one object (domain_id, setup2.c:944-978) */
 int bh_object_domain(bh_object *o, int np, const double *pts, int *did);

@@ -108,3 +108,27 @@ def test_reference_example1_relinked(solved):
             # lossless spheres: P is the discretisation error of the flux
             # balance on this coarse mesh, the same number on both sides
             assert abs(got["absorb"][typ][m] - P) <= 1e-7 * abs(P) + 1e-14, (typ, m)
+
+
+EX4 = os.path.join(ROOT, "oracle", "_ref", "example4")
+
+
+@pytest.mark.skipif(not os.path.exists(EX4), reason="reference example4 not compiled against the drop-in library")
+def test_reference_example4_far_field_relinked(solved):
+    """example4.c (far-field intensity from E, H of the potentials' gradients,
+    called from an OpenMP loop) relinked against the drop-in library: the
+    printed intensities on the x=0 plane at 2000 wavelengths against the
+    oracle's mo_EH_mEMP_s at the same points."""
+    d, O, log = solved
+    sn, mf, ty = 12, 2000.0, 1
+    _run([EX4, "out.dat", str(sn), str(mf), str(ty)], d)
+    rows = [l.split() for l in open(os.path.join(d, "fsIe_yz.txt")) if not l.startswith("#")]
+    assert len(rows) == sn
+    ra = mf * 1.064
+    dth = 2 * np.pi / sn
+    th = 0.5 * dth + dth * np.arange(sn)
+    pts = np.c_[np.zeros(sn), -ra * np.sin(th), ra * np.cos(th)]
+    EH, _ = orc.mo_field(O, pts, "EH_mEMP", "s", ty)
+    ie = (np.abs(EH[:, :3]) ** 2).sum(axis=1)
+    got = np.array([float(r[1]) for r in rows])
+    assert np.abs(got - ie).max() < 1e-8 * ie.max()

@@ -73,6 +73,27 @@ def test_single_object_fields(single, typ):
     assert _rel(got, want) < 1e-9
 
 
+@pytest.mark.parametrize("typ", [0, 1, 2, 3])
+def test_gradient_field_form(single, typ):
+    """E, H from the potentials' gradients (EH_mEMP_s): near points, points
+    inside, and far-field points at 2000 wavelengths as the reference's
+    far-field example uses them."""
+    import bem3_emf_b2_amd as bem
+    O, G = single
+    G[0].factor(2)
+    bem.mo_solve(G)
+    th = np.linspace(0.1, 3.0, 7)
+    far = 2000 * 1.064 * np.c_[np.sin(th) * np.cos(2 * th), np.sin(th) * np.sin(2 * th), np.cos(th)]
+    pts = np.r_[PTS, far]
+    for mode in ("s", "t", "i"):
+        want, oid_w = orc.mo_field(O, pts, "EH_mEMP", mode, typ)
+        got, oid_g = bem.mo_field(G, pts, "EH_mEMP", mode, typ)
+        assert (oid_w == oid_g).all()
+        for p in range(len(pts)):
+            scale = max(np.abs(want[p]).max(), 1e-300)
+            assert np.abs(got[p] - want[p]).max() < 1e-9 * scale, (typ, mode, p)
+
+
 @pytest.mark.parametrize("typ", [0, 1])
 def test_single_object_force_absorb(single, typ):
     import bem3_emf_b2_amd as bem

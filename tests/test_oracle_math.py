@@ -134,3 +134,40 @@ def test_solid_angle_and_domain_of_closed_surface(sphere):
         inside = np.linalg.norm(p) < 0.45
         assert abs(tot - (-1.0 if inside else 0.0)) < 1e-12, (p, tot)
         assert o.domain_id(p) == (1 if inside else 0)
+
+
+@pytest.fixture(scope="module")
+def solved_sphere():
+    V, F = mg.icosphere(1, 0.5)
+    o = orc.Obj.from_arrays(*mg.paired_arrays(V, F), [0j, 1.59])
+    o.setup(mg.xpol_plane_wave())
+    S = orc.System(o)
+    orc.mo_solve([o], [S])
+    yield o
+    S.free()
+    o.free()
+
+
+@pytest.mark.parametrize("typ", [1, 2])
+def test_gradient_field_form_matches_finite_differences(solved_sphere, typ):
+    """EH_mEMP_s: E = -grad(phi) + A, H = lambda0 / (2 pi i) curl A from the
+    potentials (A, phi) = U; against central differences of U itself."""
+    o = solved_sphere
+    lam0 = 1.064
+    for rt in (np.array([1.3, -0.4, 0.9]), np.array([0.1, 0.2, -0.15])):
+        h = 1e-4
+        U0, _ = orc.mo_field([o], rt[None, :], "mEMP", "s", typ)
+        grad = np.zeros((3, 4), complex)
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            up, _ = orc.mo_field([o], (rt + e)[None, :], "mEMP", "s", typ)
+            um, _ = orc.mo_field([o], (rt - e)[None, :], "mEMP", "s", typ)
+            grad[d] = (up[0] - um[0]) / (2 * h)
+        E = -grad[:, 3] + U0[0, :3]
+        ch = lam0 / (2 * np.pi * 1j)
+        H = ch * np.array([grad[1, 2] - grad[2, 1], grad[2, 0] - grad[0, 2], grad[0, 1] - grad[1, 0]])
+        EH, _ = orc.mo_field([o], rt[None, :], "EH_mEMP", "s", typ)
+        scale = np.abs(EH[0]).max()
+        assert np.abs(EH[0, :3] - E).max() < 1e-6 * scale
+        assert np.abs(EH[0, 3:] - H).max() < 1e-6 * scale
