@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_PKG, "libbem3hip.so")
 _lib = None
 
 D = C.POINTER(C.c_double)
+Fp = C.POINTER(C.c_float)
 Ip = C.POINTER(C.c_int)
 VP = C.c_void_p
 
@@ -58,6 +59,10 @@ def lib():
             "bh_prof_enable": (C.c_int, [C.c_int]),
             "bh_prof_read": (C.c_int, [C.c_int, D, D, Ip]),
             "bh_stream": (VP, []),
+            "bh_object_solve_stats": (C.c_int, [VP, Ip, D]),
+            "bh_object_set_precision": (C.c_int, [VP, C.c_int]),
+            "bh_cgesv": (C.c_int, [C.c_int, C.c_int, Fp, Fp]),
+            "bh_cgemm_host": (C.c_int, [C.c_int, C.c_int, C.c_int, Fp, Fp, Fp, Fp, Fp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -91,7 +96,10 @@ def sync():
 
 # kernel ids of the timing registry (bem3_hip.h)
 KERNELS = {"asm_gl": 0, "asm_gh": 1, "asm_self": 2, "zgemm": 3, "lu_panel": 4, "trsm": 5, "field": 6, "scatter": 7,
-           "zgemm_small": 8, "field_bd": 9}
+           "zgemm_small": 8, "field_bd": 9, "cgemm": 10, "cgemm_small": 11, "lu_panel_c": 12, "trsm_c": 13,
+           "asm_gl32": 14, "field_bd32": 15}
+# factorisation forms of bh_object_factor
+FACTOR_MODES = {"auto": 0, "dense": 1, "block": 2, "mixed": 3}
 
 
 def prof_enable(on=True):
@@ -128,6 +136,12 @@ class Object:
     def assemble(self, type_=2):
         check(lib().bh_object_assemble(self.h, type_))
 
+    def set_precision(self, prec):
+        """"fp64" (default) or "mixed": FP32 far-element quadrature and
+        boundary field sums, factor mode 0 -> the FP32 block elimination
+        refined in FP64"""
+        check(lib().bh_object_set_precision(self.h, {"fp64": 0, "mixed": 1}[prec]))
+
     def gh_count(self, did):
         return lib().bh_object_gh_count(self.h, did)
 
@@ -159,10 +173,20 @@ class Object:
         check(lib().bh_object_set_beams(self.h, b.shape[0], dp(b)))
 
     def factor(self, mode=0):
+        """mode: 0/"auto", 1/"dense", 2/"block", 3/"mixed" (complex-FP32
+        block elimination refined in FP64)"""
+        if isinstance(mode, str):
+            mode = FACTOR_MODES[mode]
         check(lib().bh_object_factor(self.h, mode))
 
     def solve(self):
         check(lib().bh_object_solve(self.h))
+
+    def solve_stats(self):
+        """(refinement steps, final relative residual) of the last solve"""
+        it, res = np.zeros(1, np.int32), np.zeros(1)
+        check(lib().bh_object_solve_stats(self.h, ip(it), dp(res)))
+        return int(it[0]), float(res[0])
 
     def solve_EH(self):
         check(lib().bh_object_solve_EH(self.h))
@@ -215,6 +239,29 @@ def zgesv(A, B, return_lu=False):
     nrhs = 1 if B.ndim == 1 else B.shape[1]
     check(lib().bh_zgesv(A.shape[0], nrhs, dp(A.view(np.float64)), dp(B.view(np.float64))))
     return (B, A) if return_lu else B
+
+
+def cgesv(A, B):
+    """complex-FP32 LU solve on the device (the kernels of the mixed form)"""
+    A = np.array(A, dtype=np.complex64, order="C")
+    B = np.array(B, dtype=np.complex64, order="C").reshape(A.shape[0], -1)
+    check(lib().bh_cgesv(A.shape[0], B.shape[1], A.view(np.float32).ctypes.data_as(Fp),
+                         B.view(np.float32).ctypes.data_as(Fp)))
+    return B
+
+
+def cgemm(A, B, Cm=None, alpha=1.0, beta=0.0):
+    """C = alpha A B + beta C in complex FP32 on the device"""
+    A = np.ascontiguousarray(A, dtype=np.complex64)
+    B = np.ascontiguousarray(B, dtype=np.complex64)
+    m, k = A.shape
+    n = B.shape[1]
+    Cm = np.zeros((m, n), np.complex64) if Cm is None else np.array(Cm, dtype=np.complex64, order="C")
+    al = np.array([complex(alpha).real, complex(alpha).imag], np.float32)
+    be = np.array([complex(beta).real, complex(beta).imag], np.float32)
+    f = lambda a: a.ctypes.data_as(Fp)
+    check(lib().bh_cgemm_host(m, n, k, f(al), f(A.view(np.float32)), f(B.view(np.float32)), f(be), f(Cm.view(np.float32))))
+    return Cm
 
 
 def zgemm(A, B, Cm=None, alpha=1.0, beta=0.0):

@@ -6,6 +6,7 @@
 // out.  Every entry point returns 0 on success and a negative code on error
 // (the message is kept for bh_last_error()).
 #include "bem_dev.h"
+#include "bem_schur.h"
 #include "../../include/bem3_hip.h"
 
 #include <algorithm>
@@ -25,7 +26,10 @@ extern "C" {
 void bem_launch_asm_gl(const void *, const int *, const void *, const int *, int, int, double, double, const void *, int,
                        void *, void *, void *, void *, size_t, double *, unsigned long long *, int, hipStream_t);
 void bem_launch_asm_gh(const void *, const int *, const void *, const int *, const void *, int, double, double,
-                       const void *, int, void *, void *, void *, void *, size_t, double *, hipStream_t);
+                       const void *, int, const void *, int, void *, void *, void *, void *, size_t, double *, hipStream_t);
+void bem_launch_asm_gl32(const void *, const int *, const void *, const int *, int, int, double, double, const void *, int,
+                         void *, void *, void *, void *, size_t, double *, unsigned long long *, int, hipStream_t);
+int bem_rule32(const void *, int, void **);
 void bem_launch_asm_self(const void *, const int *, const void *, const int *, int, double, double, const double *,
                          const double *, const double *, const double *, void *, void *, void *, void *, size_t,
                          hipStream_t);
@@ -36,6 +40,10 @@ void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *
 int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
 void bem_zgetrs(int, int, const void *, size_t, const int *, void *, size_t, int, hipStream_t);
 int bem_la_init(void);
+void bem_cgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
+               hipStream_t);
+int bem_cgetrf(int, void *, size_t, int *, int *, hipStream_t);
+void bem_cgetrs(int, int, const void *, size_t, const int *, void *, size_t, hipStream_t);
 }
 
 // ------------------------------------------------------------------ errors
@@ -72,6 +80,7 @@ struct Ctx {
   hipStream_t st = nullptr;
   QPt *d_gl = nullptr, *d_gh = nullptr;   // 3-part triangle rules
   int n_gl = 0, n_gh = 0;
+  void *d_gl32 = nullptr;                 // FP32 copy of the 12x12 rule (QPt32)
   double *d_xl = nullptr, *d_wl = nullptr, *d_xh = nullptr, *d_wh = nullptr;
   double xl[12], wl[12], xh[64], wh[64];
   hipEvent_t ev[2];
@@ -152,6 +161,7 @@ extern "C" int bh_init(int device)
       upload(&g.d_wl, g.wl, 12) || upload(&g.d_xh, g.xh, 64) || upload(&g.d_wh, g.wh, 64))
     return BH_EHIP;
   HIPCHK(hipStreamSynchronize(g.st));
+  if (bem_rule32(a.data(), g.n_gl, &g.d_gl32)) return fail(BH_EHIP, "FP32 rule table");
   g.dev = device;
   return 0;
 }
@@ -184,6 +194,8 @@ struct Dom {
   cd *d_Gf = nullptr;                  // LU factors of G (normal derivatives of E, H)
   int *d_Gpiv = nullptr;
   bool gf_valid = false;               // d_Gf factors the current G (cleared by assembly)
+  cf *c_Gf = nullptr;                  // complex-FP32 LU factors of G (mixed-precision form)
+  bool gfc_valid = false;
   // face table for the solid-angle domain test (faces whose main side is d)
 };
 
@@ -209,7 +221,8 @@ struct bh_object {
   std::vector<Beam> beams;
   Beam *d_beams = nullptr;
   double lambda0 = 0.0;
-  // factorisation: 0 none, 1 dense 8NN, 2 Schur (one medium)
+  int prec = BH_PREC_FP64;             // arithmetic of assembly, field sums and factor mode 0
+  // factorisation: 0 none, 1 dense 8NN, 2 Schur (one medium), 3 Schur in FP32 refined in FP64
   int fmode = 0;                       // factorisation of the current matrices (cleared by assembly)
   cd *d_A = nullptr, *d_work = nullptr;
   int *d_piv = nullptr, *d_info = nullptr;
@@ -217,6 +230,15 @@ struct bh_object {
   int *d_piv0 = nullptr, *d_pivM = nullptr, *d_pivS = nullptr;
   cd sch_delt, sch_beps, sch_sigm;
   cd *d_X = nullptr, *d_B = nullptr, *d_tmp = nullptr;
+  cd *d_w = nullptr;                   // 27 NN work of the block solve
+  // mixed-precision form (fmode 3): complex-FP32 factors of the block
+  // elimination, refined against the FP64 matrices
+  cf *c_G0f = nullptr, *c_Z = nullptr, *c_Mf = nullptr, *c_Sf = nullptr, *c_G1 = nullptr;
+  cf *c_w = nullptr;                   // 27 NN work, 8 NN right-hand side, 8 NN correction
+  cd *d_R = nullptr;                   // 8 NN residual
+  double *d_nrm = nullptr;             // 256 partial sums of the norm
+  int ir_iters = 0;                    // refinement steps of the last solve
+  double ir_res = 0.0;                 // its final relative residual
 };
 
 // element constants and node data (reference setup2.c:666-749, 838-869)
@@ -356,11 +378,24 @@ extern "C" void bh_object_destroy(bh_object *o)
 {
   API_LOCK;
   if (!o) return;
+  (void)hipStreamSynchronize(g.st);   // no kernel still uses the object's memory
   for (Dom &D : o->dom) {
     dfree(D.d_sid); dfree(D.d_colOff); dfree(D.d_rows);
     dfree(D.d_G); dfree(D.d_H); dfree(D.d_dG); dfree(D.d_dH); dfree(D.d_dF);
+    dfree(D.d_col); dfree(D.d_colS); dfree(D.d_rowEH);
+    dfree(D.d_U); dfree(D.d_dU); dfree(D.d_E); dfree(D.d_dE); dfree(D.d_Hf); dfree(D.d_dHf);
+    dfree(D.d_Gf); dfree(D.d_Gpiv); dfree(D.c_Gf);
   }
   dfree(o->d_geo);
+  dfree(o->d_Ui); dfree(o->d_dUi); dfree(o->d_Ui0); dfree(o->d_dUi0); dfree(o->d_Uip);
+  dfree(o->d_slotPt); dfree(o->d_slotN); dfree(o->d_slotIdx); dfree(o->d_fv); dfree(o->d_fsd); dfree(o->d_seg);
+  dfree(o->d_beams);
+  dfree(o->d_A); dfree(o->d_work); dfree(o->d_piv); dfree(o->d_info);
+  dfree(o->d_G0f); dfree(o->d_Z); dfree(o->d_Mf); dfree(o->d_Sf);
+  dfree(o->d_piv0); dfree(o->d_pivM); dfree(o->d_pivS);
+  dfree(o->d_X); dfree(o->d_B); dfree(o->d_tmp); dfree(o->d_w);
+  dfree(o->c_G0f); dfree(o->c_Z); dfree(o->c_Mf); dfree(o->c_Sf); dfree(o->c_G1); dfree(o->c_w);
+  dfree(o->d_R); dfree(o->d_nrm);
   delete o;
 }
 
@@ -401,8 +436,13 @@ static int assemble_domain(bh_object *o, int d)
   HIPCHK(hipMalloc((void **)&d_part, nd * nwv * 3 * sizeof(double)));
   HIPCHK(hipMalloc((void **)&d_mask, nd * nwv * sizeof(unsigned long long)));
   double kr = o->kn[d].real(), ki = o->kn[d].imag();
-  bem_launch_asm_gl(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, D.d_G, D.d_H,
-                    D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+  const bool mixed = o->prec == BH_PREC_MIXED;
+  if (mixed)
+    bem_launch_asm_gl32(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl32, g.n_gl, D.d_G, D.d_H,
+                        D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+  else
+    bem_launch_asm_gl(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, D.d_G, D.d_H,
+                      D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
   HIPCHK(hipGetLastError());
   // near pairs in row-major order (deterministic)
   std::vector<unsigned long long> mask(nd * nwv);
@@ -428,8 +468,8 @@ static int assemble_domain(bh_object *o, int d)
   double *d_ghdf = nullptr;
   if (upload(&d_pairs, pairs.data(), pairs.size()) || upload(&d_start, start.data(), start.size())) return BH_EHIP;
   HIPCHK(hipMalloc((void **)&d_ghdf, std::max<size_t>(pairs.size(), 1) * 3 * sizeof(double)));
-  bem_launch_asm_gh(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh, D.d_G, D.d_H,
-                    D.d_dG, D.d_dH, nd, d_ghdf, g.st);
+  bem_launch_asm_gh(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh,
+                    mixed ? (const void *)g.d_gl : nullptr, g.n_gl, D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_ghdf, g.st);
   bem_launch_asm_self(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, (int)nd, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_G,
                       D.d_H, D.d_dG, D.d_dH, nd, g.st);
   bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, (int)nd, d == 0, D.d_H, nd, D.d_dF, g.st);
@@ -445,11 +485,19 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
   if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
   // every factorisation of earlier matrices is stale from here on
   o->fmode = 0;
-  for (Dom &D : o->dom) D.gf_valid = false;
+  for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   for (int d = 0; d <= o->MN; d++) {
     int rc = assemble_domain(o, d);
     if (rc) return rc;
   }
+  return 0;
+}
+
+extern "C" int bh_object_set_precision(bh_object *o, int prec)
+{
+  API_LOCK;
+  if (prec != BH_PREC_FP64 && prec != BH_PREC_MIXED) return fail(BH_EARG, "precision %d", prec);
+  o->prec = prec;
   return 0;
 }
 
@@ -514,6 +562,49 @@ extern "C" int bh_zgemm_host(int m, int n, int k, const double *alpha, const dou
   return 0;
 }
 
+// complex FP32 counterparts (the factorisation of the mixed-precision form)
+extern "C" int bh_cgesv(int n, int nrhs, float *A, float *B)
+{
+  API_LOCK;
+  if (ensure_init()) return BH_EHIP;
+  cf *dA, *dB;
+  int *piv, *info;
+  HIPCHK(hipMalloc((void **)&dA, (size_t)n * n * sizeof(cf)));
+  HIPCHK(hipMalloc((void **)&dB, (size_t)n * nrhs * sizeof(cf)));
+  HIPCHK(hipMalloc((void **)&piv, (size_t)n * sizeof(int)));
+  HIPCHK(hipMalloc((void **)&info, sizeof(int)));
+  HIPCHK(hipMemcpy(dA, A, (size_t)n * n * sizeof(cf), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dB, B, (size_t)n * nrhs * sizeof(cf), hipMemcpyHostToDevice));
+  if (bem_cgetrf(n, dA, n, piv, info, g.st)) return lu_failed();
+  bem_cgetrs(n, nrhs, dA, n, piv, dB, nrhs, g.st);
+  HIPCHK(hipGetLastError());
+  int hinfo = 0;
+  HIPCHK(hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(B, dB, (size_t)n * nrhs * sizeof(cf), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(A, dA, (size_t)n * n * sizeof(cf), hipMemcpyDeviceToHost));
+  dfree(dA); dfree(dB); dfree(piv); dfree(info);
+  return hinfo ? fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo) : 0;
+}
+
+extern "C" int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const float *B, const float *beta,
+                             float *C)
+{
+  API_LOCK;
+  if (ensure_init()) return BH_EHIP;
+  cf *dA, *dB, *dC;
+  HIPCHK(hipMalloc((void **)&dA, (size_t)m * k * sizeof(cf)));
+  HIPCHK(hipMalloc((void **)&dB, (size_t)k * n * sizeof(cf)));
+  HIPCHK(hipMalloc((void **)&dC, (size_t)m * n * sizeof(cf)));
+  HIPCHK(hipMemcpy(dA, A, (size_t)m * k * sizeof(cf), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dB, B, (size_t)k * n * sizeof(cf), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dC, C, (size_t)m * n * sizeof(cf), hipMemcpyHostToDevice));
+  bem_cgemm(m, n, k, alpha[0], alpha[1], dA, k, dB, n, beta[0], beta[1], dC, n, g.st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(C, dC, (size_t)m * n * sizeof(cf), hipMemcpyDeviceToHost));
+  dfree(dA); dfree(dB); dfree(dC);
+  return 0;
+}
+
 // ================================================================== solve side
 extern "C" {
 void bem_launch_build_A(const void *, const void *, int, size_t, const void *, size_t, size_t, void *, size_t,
@@ -539,8 +630,9 @@ void bem_launch_scatter(const double *, const double *, const int *, int, const 
                         const void *, int, const void *, int, double, const void *, const void *, void *, void *,
                         hipStream_t);
 void bem_launch_field_bd(const int *, const double *, int, int, const void *, const int *, int, double, double,
-                         const void *, int, double, const double *, const double *, const double *, const double *,
-                         const void *, const void *, const void *, const void *, void *, double *, hipStream_t);
+                         const void *, const void *, int, double, const double *, const double *, const double *,
+                         const double *, const void *, const void *, const void *, const void *, void *, double *,
+                         hipStream_t);
 void bem_launch_incident_bd(const double *, const double *, const int *, int, const void *, int, void *, void *, void *,
                             void *, void *, hipStream_t);
 void bem_launch_add_incident(const double *, const int *, int, const void *, int, void *, int, hipStream_t);
@@ -726,15 +818,32 @@ static size_t row_base(const bh_object *o, int d)
   return b;
 }
 
-// mode 0: auto (Schur for one medium, dense otherwise), 1: dense 8NN, 2: Schur
+// the FP64 / FP32 factors of the block elimination as plain pointers
+static SchurF<cd> schur64(const bh_object *o)
+{
+  const Dom &D1 = o->dom[1];
+  return SchurF<cd>{o->NN, (size_t)o->NN, o->d_G0f, o->d_Z, o->d_Mf, o->d_Sf, D1.d_G, o->d_piv0, o->d_pivM, o->d_pivS,
+                    D1.d_col, o->sch_beps, o->sch_sigm, o->sch_delt};
+}
+static SchurF<cf> schur32(const bh_object *o)
+{
+  const Dom &D1 = o->dom[1];
+  return SchurF<cf>{o->NN, (size_t)o->NN, o->c_G0f, o->c_Z, o->c_Mf, o->c_Sf, o->c_G1, o->d_piv0, o->d_pivM, o->d_pivS,
+                    D1.d_col, to_cf(o->sch_beps), to_cf(o->sch_sigm), to_cf(o->sch_delt)};
+}
+
+// mode 0: auto (Schur for one medium, dense otherwise), 1: dense 8NN,
+// 2: Schur, 3: Schur in complex FP32 refined in FP64 (mixed precision)
 extern "C" int bh_object_factor(bh_object *o, int mode)
 {
   API_LOCK;
   if (!o->dom[0].d_G) return fail(BH_ESTATE, "assemble before factor");
+  if (mode < 0 || mode > 3) return fail(BH_EARG, "factor mode %d", mode);
   if (prepare_solve(o)) return BH_EHIP;
   size_t NN = (size_t)o->NN, na = 8 * NN;
-  if (mode == 0) mode = o->MN == 1 ? 2 : 1;
-  if (mode == 2 && o->MN != 1) return fail(BH_EUNSUP, "Schur form needs exactly one interior medium");
+  if (mode == 0) mode = o->MN == 1 ? (o->prec == BH_PREC_MIXED ? 3 : 2) : 1;
+  if (mode >= 2 && o->MN != 1) return fail(BH_EUNSUP, "the block elimination needs exactly one interior medium");
+  o->fmode = 0;
   if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
   if (!o->d_X) { if (dalloc(&o->d_X, na) || dalloc(&o->d_B, na) || dalloc(&o->d_tmp, 8 * NN)) return BH_EHIP; }
   int info = 0;
@@ -749,41 +858,55 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
   } else {
     // one medium: domain 0 rows H0 u - G0 q = 0, domain 1 rows couple through phi
     Dom &D0 = o->dom[0], &D1 = o->dom[1];
-    if (D0.nd != (int)NN || D1.nd != (int)NN) return fail(BH_EUNSUP, "Schur form: unexpected domain sizes");
+    if (D0.nd != (int)NN || D1.nd != (int)NN) return fail(BH_EUNSUP, "block elimination: unexpected domain sizes");
     size_t n2 = NN * NN;
-    if (!o->d_G0f) {
-      if (dalloc(&o->d_G0f, n2) || dalloc(&o->d_Z, n2) || dalloc(&o->d_Mf, n2) || dalloc(&o->d_Sf, n2) ||
-          dalloc(&o->d_piv0, NN) || dalloc(&o->d_pivM, NN) || dalloc(&o->d_pivS, NN))
-        return BH_EHIP;
-    }
+    if (!o->d_piv0 && (dalloc(&o->d_piv0, NN) || dalloc(&o->d_pivM, NN) || dalloc(&o->d_pivS, NN) ||
+                       dalloc(&o->d_w, 27 * NN) || dalloc(&o->d_R, na) || dalloc(&o->d_nrm, 256)))
+      return BH_EHIP;
     zd k1 = o->kn[0], k2 = o->kn[1], delt = k1 * k1 - k2 * k2, beps = k1 * k1 / (k2 * k2), sigm = 1.0 - beps;
     o->sch_delt = mk(delt.real(), delt.imag());
     o->sch_beps = mk(beps.real(), beps.imag());
     o->sch_sigm = mk(sigm.real(), sigm.imag());
     int n = (int)NN;
-    // Z = G0^{-1} H0
-    HIPCHK(hipMemcpyAsync(o->d_G0f, D0.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    HIPCHK(hipMemcpyAsync(o->d_Z, D0.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    if (bem_zgetrf(n, o->d_G0f, NN, o->d_piv0, o->d_work, o->d_info, 64, g.st)) return lu_failed();
-    bem_zgetrs(n, n, o->d_G0f, NN, o->d_piv0, o->d_Z, NN, 64, g.st);
-    // M = H1 + G1 Z  (Mf), P = G1 Z kept in Sf for the beta term
-    HIPCHK(hipMemcpyAsync(o->d_Mf, D1.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    bem_zgemm(n, n, n, 1.0, 0.0, D1.d_G, NN, o->d_Z, NN, 0.0, 0.0, o->d_Sf, NN, g.st);
-    bem_launch_axpby(o->d_Mf, 1.0, 0.0, o->d_Sf, o->d_Mf, n2, g.st);
-    // S starts as H1 + beta P
-    bem_launch_axpby(D1.d_H, beps.real(), beps.imag(), o->d_Sf, o->d_Sf, n2, g.st);
-    if (bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, g.st)) return lu_failed();
-    // W = M^{-1} G1, T = W o (n n^T), S -= sigma delta G1 T
-    cd *W = o->d_A;
-    bool tmpW = false;
-    if (!W) { HIPCHK(hipMalloc((void **)&W, n2 * sizeof(cd))); tmpW = true; }
-    HIPCHK(hipMemcpyAsync(W, D1.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    bem_zgetrs(n, n, o->d_Mf, NN, o->d_pivM, W, NN, 64, g.st);
-    bem_launch_hadamard_nn(W, n, NN, D1.d_col, g.st);
-    zd sd = -(sigm * delt);
-    bem_zgemm(n, n, n, sd.real(), sd.imag(), D1.d_G, NN, W, NN, 1.0, 0.0, o->d_Sf, NN, g.st);
-    if (tmpW) { HIPCHK(hipStreamSynchronize(g.st)); dfree(W); }
-    if (bem_zgetrf(n, o->d_Sf, NN, o->d_pivS, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+    if (mode == 3) {
+      if (!o->c_G0f && (dalloc(&o->c_G0f, n2) || dalloc(&o->c_Z, n2) || dalloc(&o->c_Mf, n2) || dalloc(&o->c_Sf, n2) ||
+                        dalloc(&o->c_G1, n2) || dalloc(&o->c_w, 43 * NN)))
+        return BH_EHIP;
+      cf *Wt = nullptr;
+      HIPCHK(hipMalloc((void **)&Wt, n2 * sizeof(cf)));
+      int rc = schur_factor_c(n, NN, D0.d_G, D0.d_H, D1.d_G, D1.d_H, D1.d_col, o->sch_beps, o->sch_sigm, o->sch_delt,
+                              o->c_G0f, o->c_Z, o->c_Mf, o->c_Sf, o->c_G1, Wt, o->d_piv0, o->d_pivM, o->d_pivS, o->d_info,
+                              g.st);
+      (void)hipStreamSynchronize(g.st);
+      dfree(Wt);
+      if (rc) return lu_failed();
+    } else {
+      if (!o->d_G0f && (dalloc(&o->d_G0f, n2) || dalloc(&o->d_Z, n2) || dalloc(&o->d_Mf, n2) || dalloc(&o->d_Sf, n2)))
+        return BH_EHIP;
+      // Z = G0^{-1} H0
+      HIPCHK(hipMemcpyAsync(o->d_G0f, D0.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+      HIPCHK(hipMemcpyAsync(o->d_Z, D0.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+      if (bem_zgetrf(n, o->d_G0f, NN, o->d_piv0, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+      bem_zgetrs(n, n, o->d_G0f, NN, o->d_piv0, o->d_Z, NN, 64, g.st);
+      // M = H1 + G1 Z  (Mf), P = G1 Z kept in Sf for the beta term
+      HIPCHK(hipMemcpyAsync(o->d_Mf, D1.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+      bem_zgemm(n, n, n, 1.0, 0.0, D1.d_G, NN, o->d_Z, NN, 0.0, 0.0, o->d_Sf, NN, g.st);
+      bem_launch_axpby(o->d_Mf, 1.0, 0.0, o->d_Sf, o->d_Mf, n2, g.st);
+      // S starts as H1 + beta P
+      bem_launch_axpby(D1.d_H, beps.real(), beps.imag(), o->d_Sf, o->d_Sf, n2, g.st);
+      if (bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+      // W = M^{-1} G1, T = W o (n n^T), S -= sigma delta G1 T
+      cd *W = o->d_A;
+      bool tmpW = false;
+      if (!W) { HIPCHK(hipMalloc((void **)&W, n2 * sizeof(cd))); tmpW = true; }
+      HIPCHK(hipMemcpyAsync(W, D1.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+      bem_zgetrs(n, n, o->d_Mf, NN, o->d_pivM, W, NN, 64, g.st);
+      bem_launch_hadamard_nn(W, n, NN, D1.d_col, g.st);
+      zd sd = -(sigm * delt);
+      bem_zgemm(n, n, n, sd.real(), sd.imag(), D1.d_G, NN, W, NN, 1.0, 0.0, o->d_Sf, NN, g.st);
+      if (tmpW) { HIPCHK(hipStreamSynchronize(g.st)); dfree(W); }
+      if (bem_zgetrf(n, o->d_Sf, NN, o->d_pivS, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+    }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, g.st));
@@ -793,55 +916,12 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
   return 0;
 }
 
-// small strided copy kernels for the Schur solve vectors
-__global__ void k_gather_cols(const cd *__restrict__ src, size_t sstride, int ncol, cd *__restrict__ dst, int n)
-{
-  // dst[r*ncol + c] = src[c*sstride + r]
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * ncol) return;
-  int r = i / ncol, c = i % ncol;
-  dst[i] = src[(size_t)c * sstride + r];
-}
-__global__ void k_scatter_cols(const cd *__restrict__ src, int ncol, cd *__restrict__ dst, size_t dstride, int n)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * ncol) return;
-  int r = i / ncol, c = i % ncol;
-  dst[(size_t)c * dstride + r] = src[i];
-}
-// v[r] = sum_c n_c(r) Y[r][c]  (3 columns)
-__global__ void k_ndot(const cd *__restrict__ Y, const ColInfo *__restrict__ col, int n, cd *__restrict__ v)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const double *nn = col[r].n;
-  v[r] = Y[3 * r] * nn[0] + Y[3 * r + 1] * nn[1] + Y[3 * r + 2] * nn[2];
-}
-// Y[r][c] = n_c(r) u[r]  (3 columns)
-__global__ void k_nscale(const cd *__restrict__ u, const ColInfo *__restrict__ col, int n, cd *__restrict__ Y)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  for (int c = 0; c < 3; c++) Y[3 * r + c] = u[r] * col[r].n[c];
-}
-
-// U4[r][c] = Y[r][c] (c<3), u3[r] (c=3)
-__global__ void k_pack_u4(const cd *__restrict__ Y, const cd *__restrict__ u3, int n, cd *__restrict__ U4)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  for (int c = 0; c < 3; c++) U4[4 * r + c] = Y[3 * r + c];
-  U4[4 * r + 3] = u3[r];
-}
-// X[c n + r] = U4[r][c], X[(c+4) n + r] = Q4[r][c]  (one medium: node = r)
-__global__ void k_fill_x(const cd *__restrict__ U4, const cd *__restrict__ Q4, int n, cd *__restrict__ X)
-{
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  for (int c = 0; c < 4; c++) { X[(size_t)c * n + r] = U4[4 * r + c]; X[(size_t)(c + 4) * n + r] = Q4[4 * r + c]; }
-}
-
 static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// iterative refinement of the mixed-precision form: relative residual
+// target ||b - A x|| / ||b|| and step limit
+#define IR_TOL 1e-10
+#define IR_MAX 30
 
 // B from the incident data, X = A^{-1} B, boundary values (solve_coefficient_mEMP)
 extern "C" int bh_object_solve(bh_object *o)
@@ -855,32 +935,41 @@ extern "C" int bh_object_solve(bh_object *o)
     Dom &D = o->dom[d];
     bem_launch_build_B(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, row_base(o, d), o->d_B, g.st);
   }
-  int n = (int)NN;
+  o->ir_iters = 0;
+  o->ir_res = 0.0;
   if (o->fmode == 1) {
     HIPCHK(hipMemcpyAsync(o->d_X, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
     bem_zgetrs((int)na, 1, o->d_A, na, o->d_piv, o->d_X, 1, 64, g.st);
+  } else if (o->fmode == 2) {
+    // the incident data enter the domain-1 rows only
+    schur_apply(schur64(o), o->d_B, false, o->d_X, o->d_w, g.st);
   } else {
-    Dom &D1 = o->dom[1];
-    const cd *b1 = o->d_B + 4 * NN;          // domain-1 rows: b_0..b_3 (domain-0 rows are zero)
-    cd *Y = o->d_tmp, *v = o->d_tmp + 3 * NN, *u3 = o->d_tmp + 4 * NN, *Qn = o->d_tmp + 5 * NN;
-    // Y = M^{-1} [b0 b1 b2]
-    hipLaunchKernelGGL(k_gather_cols, g1(3 * NN), dim3(256), 0, g.st, b1, NN, 3, Y, n);
-    bem_zgetrs(n, 3, o->d_Mf, NN, o->d_pivM, Y, 3, 64, g.st);
-    // u3 = S^{-1} (b3 - sigma G1 sum_c N_c Y_c)
-    hipLaunchKernelGGL(k_ndot, g1(NN), dim3(256), 0, g.st, Y, D1.d_col, n, v);
-    HIPCHK(hipMemcpyAsync(u3, b1 + 3 * NN, NN * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    bem_zgemm(n, 1, n, -o->sch_sigm.x, -o->sch_sigm.y, D1.d_G, NN, v, 1, 1.0, 0.0, u3, 1, g.st);
-    bem_zgetrs(n, 1, o->d_Sf, NN, o->d_pivS, u3, 1, 64, g.st);
-    // u_c = M^{-1} (b_c - delta G1 N_c u3)
-    hipLaunchKernelGGL(k_nscale, g1(NN), dim3(256), 0, g.st, u3, D1.d_col, n, Qn);
-    hipLaunchKernelGGL(k_gather_cols, g1(3 * NN), dim3(256), 0, g.st, b1, NN, 3, Y, n);
-    bem_zgemm(n, 3, n, -o->sch_delt.x, -o->sch_delt.y, D1.d_G, NN, Qn, 3, 1.0, 0.0, Y, 3, g.st);
-    bem_zgetrs(n, 3, o->d_Mf, NN, o->d_pivM, Y, 3, 64, g.st);
-    // U4 = [u0 u1 u2 u3] (in B, no longer needed), Q4 = Z U4, X = [U4^T ; Q4^T]
-    cd *U4 = o->d_B, *Q4 = o->d_tmp + 4 * NN;
-    hipLaunchKernelGGL(k_pack_u4, g1(NN), dim3(256), 0, g.st, Y, u3, n, U4);
-    bem_zgemm(n, 4, n, 1.0, 0.0, o->d_Z, NN, U4, 4, 0.0, 0.0, Q4, 4, g.st);
-    hipLaunchKernelGGL(k_fill_x, g1(NN), dim3(256), 0, g.st, U4, Q4, n, o->d_X);
+    // complex-FP32 solves of the residual equation, residuals in FP64
+    const double nb = bem_nrm2(o->d_B, na, o->d_nrm, g.st);
+    if (nb < 0.0) return fail(BH_EHIP, "norm of the right-hand side");
+    HIPCHK(hipMemsetAsync(o->d_X, 0, na * sizeof(cd), g.st));
+    HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    cf *cw = o->c_w, *crhs = cw + 27 * NN, *cdx = crhs + 8 * NN;
+    const SchurF<cf> f = schur32(o);
+    double res = nb > 0.0 ? 1.0 : 0.0;
+    int it = 0;
+    while (res > IR_TOL && it < IR_MAX) {
+      bem_cvt_z2c(o->d_R, crhs, na, g.st);
+      schur_apply(f, crhs, true, cdx, cw, g.st);
+      bem_acc_c2z(o->d_X, cdx, na, g.st);
+      for (int d = 0; d <= o->MN; d++) {
+        Dom &D = o->dom[d];
+        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, g.st);
+      }
+      const double nr = bem_nrm2(o->d_R, na, o->d_nrm, g.st);
+      if (nr < 0.0) return fail(BH_EHIP, "norm of the residual");
+      res = nr / nb;
+      it++;
+    }
+    o->ir_iters = it;
+    o->ir_res = res;
+    if (!(res <= IR_TOL))
+      return fail(BH_ECONV, "mixed-precision refinement: relative residual %.3g after %d steps", res, it);
   }
   for (int d = 0; d <= o->MN; d++) {
     Dom &D = o->dom[d];
@@ -891,6 +980,49 @@ extern "C" int bh_object_solve(bh_object *o)
   return 0;
 }
 
+extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *resid)
+{
+  API_LOCK;
+  if (iters) *iters = o->ir_iters;
+  if (resid) *resid = o->ir_res;
+  return 0;
+}
+
+// G x = b for 6 right-hand sides (nd x 6, b in R, x into R) with the
+// complex-FP32 LU of G, refined against the FP64 G
+static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *piv, cd *R)
+{
+  const size_t nd = (size_t)D.nd, m = 6 * nd;
+  cd *X = nullptr, *Rr = nullptr;
+  cf *C = nullptr;
+  HIPCHK(hipMalloc((void **)&X, m * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&Rr, m * sizeof(cd)));
+  HIPCHK(hipMalloc((void **)&C, m * sizeof(cf)));
+  HIPCHK(hipMemsetAsync(X, 0, m * sizeof(cd), g.st));
+  HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+  const double nb = bem_nrm2(R, m, o->d_nrm, g.st);
+  double res = nb > 0.0 ? 1.0 : 0.0;
+  int it = 0, rc = 0;
+  while (res > IR_TOL && it < IR_MAX) {
+    bem_cvt_z2c(Rr, C, m, g.st);
+    bem_cgetrs(D.nd, 6, Gc, nd, piv, C, 6, g.st);
+    bem_acc_c2z(X, C, m, g.st);
+    HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    bem_zgemm(D.nd, 6, D.nd, -1.0, 0.0, D.d_G, nd, X, 6, 1.0, 0.0, Rr, 6, g.st);
+    const double nr = bem_nrm2(Rr, m, o->d_nrm, g.st);
+    if (nr < 0.0) { rc = fail(BH_EHIP, "norm of the residual"); break; }
+    res = nr / nb;
+    it++;
+  }
+  if (!rc && !(res <= IR_TOL)) rc = fail(BH_ECONV, "mixed-precision refinement (G): relative residual %.3g after %d steps", res, it);
+  if (!rc) HIPCHK(hipMemcpyAsync(R, X, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  dfree(X);
+  dfree(Rr);
+  dfree(C);
+  return rc;
+}
+
 // boundary E, H from the derivative rows, then dE/dn, dH/dn from
 // G [dE|dH] = H [E|H] per domain (solve_eh_bv + solve_deh_bv)
 extern "C" int bh_object_solve_EH(bh_object *o)
@@ -898,24 +1030,43 @@ extern "C" int bh_object_solve_EH(bh_object *o)
   API_LOCK;
   if (!o->dom[0].d_U) return fail(BH_ESTATE, "solve before the boundary fields");
   zd ch = o->lambda0 / (2.0 * M_PI * zd(0.0, 1.0));
+  if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
+  HIPCHK(hipMemsetAsync(o->d_info, 0, sizeof(int), g.st));
+  const bool mixed = o->fmode == 3;
+  if (mixed && !o->d_nrm && dalloc(&o->d_nrm, 256)) return BH_EHIP;
   for (int d = 0; d <= o->MN; d++) {
     Dom &D = o->dom[d];
     size_t nd = (size_t)D.nd;
     bem_launch_eh_rows(D.d_dG, D.d_dH, D.nd, nd, D.d_dF, D.d_rowEH, D.d_colS, D.d_col, D.d_U, D.d_dU, ch.real(),
                        ch.imag(), D.d_E, D.d_Hf, g.st);
     // LU of G_d: domain 0 of the block form already factorised G0 for this
-    // assembly; otherwise factorise G_d once per assembly
+    // assembly (in the precision of the form); otherwise factorise G_d once
+    // per assembly
     const cd *Gf = D.d_Gf;
+    const cf *Gc = D.c_Gf;
     const int *Gpiv = D.d_Gpiv;
-    if (d == 0 && o->fmode == 2) {
+    if (d == 0 && (o->fmode == 2 || mixed)) {
       Gf = o->d_G0f;
+      Gc = o->c_G0f;
       Gpiv = o->d_piv0;
+    } else if (mixed) {
+      if (!D.gfc_valid) {
+        if (!D.c_Gf && dalloc(&D.c_Gf, nd * nd)) return BH_EHIP;
+        if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
+        bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, g.st);
+        if (bem_cgetrf(D.nd, D.c_Gf, nd, D.d_Gpiv, o->d_info, g.st)) return lu_failed();
+        D.gfc_valid = true;
+        D.gf_valid = false;   // the shared pivot vector now belongs to the FP32 factors
+      }
+      Gc = D.c_Gf;
+      Gpiv = D.d_Gpiv;
     } else if (!D.gf_valid) {
-      if (!D.d_Gf && (dalloc(&D.d_Gf, nd * nd) || dalloc(&D.d_Gpiv, nd))) return BH_EHIP;
+      if (!D.d_Gf && dalloc(&D.d_Gf, nd * nd)) return BH_EHIP;
+      if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
       HIPCHK(hipMemcpyAsync(D.d_Gf, D.d_G, nd * nd * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-      if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
       if (bem_zgetrf(D.nd, D.d_Gf, nd, D.d_Gpiv, o->d_work, o->d_info, 64, g.st)) return lu_failed();
       D.gf_valid = true;
+      D.gfc_valid = false;
       Gf = D.d_Gf;
       Gpiv = D.d_Gpiv;
     }
@@ -924,11 +1075,16 @@ extern "C" int bh_object_solve_EH(bh_object *o)
     HIPCHK(hipMalloc((void **)&R, nd * 6 * sizeof(cd)));
     bem_launch_pack_eh(D.nd, D.d_col, D.d_colS, D.d_E, D.d_Hf, P, g.st);
     bem_zgemm(D.nd, 6, D.nd, 1.0, 0.0, D.d_H, nd, P, 6, 0.0, 0.0, R, 6, g.st);
-    bem_zgetrs(D.nd, 6, Gf, nd, Gpiv, R, 6, 64, g.st);
-    bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, g.st);
+    int rc = 0;
+    if (mixed)
+      rc = solve_g_mixed(o, D, Gc, Gpiv, R);
+    else
+      bem_zgetrs(D.nd, 6, Gf, nd, Gpiv, R, 6, 64, g.st);
+    if (!rc) bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, g.st);
     HIPCHK(hipStreamSynchronize(g.st));
     dfree(P);
     dfree(R);
+    if (rc) return rc;
   }
   HIPCHK(hipGetLastError());
   int info = 0;
@@ -1152,7 +1308,7 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
       if (upload(&d_ci, ci.data(), nc) || upload(&d_ct, ct.data(), nc) || upload(&d_cze, cze.data(), cze.size()))
         return BH_EHIP;
       bem_launch_field_bd(d_ct, d_cze, nc, 0, o->d_geo, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(), g_rule[2],
-                          g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
+                          o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
                           d_tmp, d_rt, g.st);
       hipLaunchKernelGGL(k_scatter6, g1(nc), dim3(256), 0, g.st, d_tmp, d_ci, nc, d_EH);
       HIPCHK(hipStreamSynchronize(g.st));
@@ -1414,7 +1570,7 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
     double *d_rt2;
     HIPCHK(hipMalloc((void **)&d_rt2, 3 * (size_t)np * sizeof(double)));
     bem_launch_field_bd(d_t, d_ze, np, did, o->d_geo, D.d_sid, D.dNe, o->kn[did].real(), o->kn[did].imag(), g_rule[2],
-                        g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
+                        o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
                         d_rt2, g.st);
     HIPCHK(hipStreamSynchronize(g.st));
     dfree(d_rt2);

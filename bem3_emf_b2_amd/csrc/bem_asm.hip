@@ -26,6 +26,8 @@
 #include "bem_dev.h"
 #include "bem_prof.h"
 
+#include <cstdlib>
+
 struct AsmAcc {
   cd G[3], H[3], DG0[3], DG1[3], DH0[3], DH1[3];
   double F, DF0, DF1;
@@ -157,13 +159,138 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
   }
 }
 
+// ---- pass 1, mixed-precision form ----
+// The same far-pair rule with the integrand evaluated in FP32 (positions
+// relative to the row point formed in FP64 first, so that no cancellation
+// enters the FP32 part), hardware reciprocal square root, FP32 sin/cos/exp,
+// sums over blocks of 12 points in FP32 added into FP64 accumulators.
+// Pairs whose FP32 |dF/dt| comes within a relative 1e-4 of the GHN switch
+// level are flagged too: pass 2 redoes them in FP64 and takes the
+// reference's decision there.
+struct QPt32 {
+  float z, e, w, wn0, wn1, wn2;   // point, weight, weight x shape functions
+};
+struct Acc32 {
+  cf G[3], H[3], DG0[3], DG1[3], DH0[3], DH1[3];
+  float F, DF0, DF1;
+};
+
+__device__ __forceinline__ void fma_cf(cf &a, cf b, float s) { a.x = fmaf(b.x, s, a.x); a.y = fmaf(b.y, s, a.y); }
+__device__ __forceinline__ void add_cf(cd &a, cf b) { a.x += (double)b.x; a.y += (double)b.y; }
+
+__global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
+                                                  const RowPt *__restrict__ rows, const int *__restrict__ colOff,
+                                                  int dNe, double kr, double ki, const QPt32 *__restrict__ rule, int nq,
+                                                  cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
+                                                  cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
+                                                  unsigned long long *__restrict__ ghMask)
+{
+  const int row = blockIdx.y;
+  const RowPt rp = rows[row];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int lane = threadIdx.x & 63;
+  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwv = (gridDim.x * blockDim.x) >> 6;
+  bool active = s <= dNe && s != rp.t;
+  bool flagged = false;
+  double f = 0.0, df0 = 0.0, df1 = 0.0;
+  if (active) {
+    int sd = sid[s];
+    int as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    const float d0x = (float)(g.c0[0] - rp.rt[0]), d0y = (float)(g.c0[1] - rp.rt[1]), d0z = (float)(g.c0[2] - rp.rt[2]);
+    const float c1x = (float)g.c1[0], c1y = (float)g.c1[1], c1z = (float)g.c1[2];
+    const float c2x = (float)g.c2[0], c2y = (float)g.c2[1], c2z = (float)g.c2[2];
+    const float Wx = (float)g.W[0], Wy = (float)g.W[1], Wz = (float)g.W[2];
+    const float v0x = (float)rp.v0[0], v0y = (float)rp.v0[1], v0z = (float)rp.v0[2];
+    const float v1x = (float)rp.v1[0], v1y = (float)rp.v1[1], v1z = (float)rp.v1[2];
+    const float Wdv0 = (float)(g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2]);
+    const float Wdv1 = (float)(g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2]);
+    const float krf = (float)kr, kif = (float)ki;
+    AsmAcc a;
+    acc_zero(a);
+    for (int p0 = 0; p0 < nq; p0 += 12) {
+      Acc32 b;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        b.G[i] = mkf(0, 0); b.H[i] = mkf(0, 0); b.DG0[i] = mkf(0, 0); b.DG1[i] = mkf(0, 0); b.DH0[i] = mkf(0, 0);
+        b.DH1[i] = mkf(0, 0);
+      }
+      b.F = b.DF0 = b.DF1 = 0.0f;
+      const int p1 = min(p0 + 12, nq);
+      for (int p = p0; p < p1; p++) {
+        const QPt32 q = rule[p];
+        const float vx = fmaf(c2x, q.e, fmaf(c1x, q.z, d0x));
+        const float vy = fmaf(c2y, q.e, fmaf(c1y, q.z, d0y));
+        const float vz = fmaf(c2z, q.e, fmaf(c1z, q.z, d0z));
+        const float R2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
+        const float iR = rsqrtf(R2), R = R2 * iR, iR2 = iR * iR, iR3 = iR2 * iR;
+        const float rdW = fmaf(vx, Wx, fmaf(vy, Wy, vz * Wz)) * iR;
+        const float rd0 = fmaf(vx, v0x, fmaf(vy, v0y, vz * v0z)) * iR;
+        const float rd1 = fmaf(vx, v1x, fmaf(vy, v1y, vz * v1z)) * iR;
+        float sn, cs;
+        sincosf(krf * R, &sn, &cs);
+        if (kif != 0.0f) {
+          const float at = expf(-kif * R);
+          sn *= at;
+          cs *= at;
+        }
+        const cf ce = mkf(cs, sn);
+        const cf cb = mkf(-1.0f - kif * R, krf * R);   // i k R - 1
+        const cf cbce = cb * ce;
+        const cf gk = ce * iR;
+        const cf hk = cbce * (rdW * iR2);
+        const cf tk = cbce * iR2;
+        const cf kR = mkf(krf * R, kif * R);
+        const cf qq = kR * kR + cb * 3.0f;
+        const cf ce3 = ce * iR3;
+        const cf dh0 = ce3 * (qq * (rd0 * rdW) - cb * Wdv0);
+        const cf dh1 = ce3 * (qq * (rd1 * rdW) - cb * Wdv1);
+        const float wn[3] = {q.wn0, q.wn1, q.wn2};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          fma_cf(b.G[i], gk, wn[i]);
+          fma_cf(b.H[i], hk, wn[i]);
+          fma_cf(b.DG0[i], tk, wn[i] * rd0);
+          fma_cf(b.DG1[i], tk, wn[i] * rd1);
+          fma_cf(b.DH0[i], dh0, wn[i]);
+          fma_cf(b.DH1[i], dh1, wn[i]);
+        }
+        b.F = fmaf(q.w, rdW * iR2, b.F);
+        b.DF0 = fmaf(q.w, iR3 * fmaf(3.0f * rd0, rdW, -Wdv0), b.DF0);
+        b.DF1 = fmaf(q.w, iR3 * fmaf(3.0f * rd1, rdW, -Wdv1), b.DF1);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        add_cf(a.G[i], b.G[i]); add_cf(a.H[i], b.H[i]); add_cf(a.DG0[i], b.DG0[i]); add_cf(a.DG1[i], b.DG1[i]);
+        add_cf(a.DH0[i], b.DH0[i]); add_cf(a.DH1[i], b.DH1[i]);
+      }
+      a.F += (double)b.F; a.DF0 += (double)b.DF0; a.DF1 += (double)b.DF1;
+    }
+    acc_finish(a, g.aW, sd < 0);
+    const double lim = BEM_CBD_SDF * (1.0 - 1e-4);
+    flagged = fabs(a.DF0) > lim || fabs(a.DF1) > lim;
+    if (!flagged) {
+      acc_store(a, row, colOff[s], ld, G, H, dG, dH);
+      f = a.F; df0 = a.DF0; df1 = a.DF1;
+    }
+  }
+  unsigned long long m = __ballot(flagged);
+  f = wsum(f); df0 = wsum(df0); df1 = wsum(df1);
+  if (lane == 0) {
+    size_t o = (size_t)row * nwv + wv;
+    ghMask[o] = m;
+    dfPart[o * 3 + 0] = f; dfPart[o * 3 + 1] = df0; dfPart[o * 3 + 2] = df1;
+  }
+}
+
 // ---- pass 2: flagged pairs with the 64x64 rule ----
 __global__ void __launch_bounds__(256) k_asm_gh(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
                                                 const RowPt *__restrict__ rows, const int *__restrict__ colOff,
                                                 const GhPair *__restrict__ pairs, int npairs, double kr, double ki,
-                                                const QPt *__restrict__ rule, int nq, cd *__restrict__ G,
-                                                cd *__restrict__ H, cd *__restrict__ dG, cd *__restrict__ dH, size_t ld,
-                                                double *__restrict__ ghDF)
+                                                const QPt *__restrict__ rule, int nq, const QPt *__restrict__ gl,
+                                                int ngl, cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
+                                                cd *__restrict__ dH, size_t ld, double *__restrict__ ghDF)
 {
   const int pid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -176,6 +303,29 @@ __global__ void __launch_bounds__(256) k_asm_gh(const TriGeo *__restrict__ geo, 
   double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
   double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
   AsmAcc a;
+  if (gl) {
+    // pairs flagged near the switch level by the FP32 pass: the FP64
+    // 12x12 rule first, and the 64x64 rule only where it says so
+    acc_zero(a);
+    for (int p = lane; p < ngl; p += 64) {
+      const QPt qp = gl[p];
+      asm_point(a, g, rp, Wdv0, Wdv1, kr, ki, qp.z, qp.e, qp.w);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      a.G[i] = wsum(a.G[i]); a.H[i] = wsum(a.H[i]); a.DG0[i] = wsum(a.DG0[i]); a.DG1[i] = wsum(a.DG1[i]);
+      a.DH0[i] = wsum(a.DH0[i]); a.DH1[i] = wsum(a.DH1[i]);
+    }
+    a.F = wsum(a.F); a.DF0 = wsum(a.DF0); a.DF1 = wsum(a.DF1);
+    acc_finish(a, g.aW, sd < 0);
+    if (!(fabs(a.DF0) > BEM_CBD_SDF || fabs(a.DF1) > BEM_CBD_SDF)) {   // the same value in every lane
+      if (lane == 0) {
+        acc_store(a, pr.row, colOff[pr.s], ld, G, H, dG, dH);
+        ghDF[pid * 3 + 0] = a.F; ghDF[pid * 3 + 1] = a.DF0; ghDF[pid * 3 + 2] = a.DF1;
+      }
+      return;
+    }
+  }
   acc_zero(a);
   for (int p = lane; p < nq; p += 64) {
     const QPt qp = rule[p];
@@ -371,17 +521,50 @@ void bem_launch_asm_gl(const void *geo, const int *sid, const void *rows, const 
   bem_prof_end(BH_K_ASM_GL, st);
 }
 
+// gl != nullptr: the pairs come from the FP32 pass and are first redone
+// with the FP64 12x12 rule (gl, ngl points)
 void bem_launch_asm_gh(const void *geo, const int *sid, const void *rows, const int *colOff, const void *pairs,
-                       int npairs, double kr, double ki, const void *rule, int nq, void *G, void *H, void *dG, void *dH,
-                       size_t ld, double *ghDF, hipStream_t st)
+                       int npairs, double kr, double ki, const void *rule, int nq, const void *gl, int ngl, void *G,
+                       void *H, void *dG, void *dH, size_t ld, double *ghDF, hipStream_t st)
 {
   if (npairs <= 0) return;
   int nb = (npairs + 3) / 4;
   bem_prof_begin(BH_K_ASM_GH, (double)npairs, st);
   hipLaunchKernelGGL(k_asm_gh, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff,
-                     (const GhPair *)pairs, npairs, kr, ki, (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH,
-                     ld, ghDF);
+                     (const GhPair *)pairs, npairs, kr, ki, (const QPt *)rule, nq, (const QPt *)gl, ngl, (cd *)G,
+                     (cd *)H, (cd *)dG, (cd *)dH, ld, ghDF);
   bem_prof_end(BH_K_ASM_GH, st);
+}
+
+void bem_launch_asm_gl32(const void *geo, const int *sid, const void *rows, const int *colOff, int dNe, int nrows,
+                         double kr, double ki, const void *rule32, int nq, void *G, void *H, void *dG, void *dH,
+                         size_t ld, double *dfPart, unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
+{
+  dim3 grid(nwv_blocks, nrows);
+  bem_prof_begin(BH_K_ASM_GL32, (double)nrows * dNe, st);
+  hipLaunchKernelGGL(k_asm_gl32, grid, dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff, dNe, kr,
+                     ki, (const QPt32 *)rule32, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
+  bem_prof_end(BH_K_ASM_GL32, st);
+}
+
+// FP32 copy (device) of a 3-part rule given on the host: point, weight and
+// weight x shape functions
+int bem_rule32(const void *rule, int nq, void **out)
+{
+  const QPt *r = (const QPt *)rule;
+  QPt32 *h = (QPt32 *)malloc(sizeof(QPt32) * (size_t)nq);
+  if (!h) return -1;
+  for (int p = 0; p < nq; p++) {
+    const double S3 = BEM_SQ3, z = r[p].z, e = r[p].e, w = r[p].w;
+    const double n0 = (1.0 - 2.5 * (z + S3 * e)) / 3.0, n1 = (1.0 + 5.0 * z) / 3.0, n2 = (1.0 - 2.5 * (z - S3 * e)) / 3.0;
+    h[p] = {(float)z, (float)e, (float)w, (float)(w * n0), (float)(w * n1), (float)(w * n2)};
+  }
+  int rc = 0;
+  if (hipMalloc(out, sizeof(QPt32) * (size_t)nq) != hipSuccess ||
+      hipMemcpy(*out, h, sizeof(QPt32) * (size_t)nq, hipMemcpyHostToDevice) != hipSuccess)
+    rc = -1;
+  free(h);
+  return rc;
 }
 
 void bem_launch_asm_self(const void *geo, const int *sid, const void *rows, const int *colOff, int nrows, double kr,

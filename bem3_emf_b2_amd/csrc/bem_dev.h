@@ -27,6 +27,24 @@ __host__ __device__ __forceinline__ double abs2(cd a) { return a.x * a.x + a.y *
 // a += b * s  (s real)
 __device__ __forceinline__ void fma_cr(cd &a, cd b, double s) { a.x = fma(b.x, s, a.x); a.y = fma(b.y, s, a.y); }
 
+// complex single precision (the mixed-precision factorisation)
+struct cf {
+  float x, y;
+};
+__host__ __device__ __forceinline__ cf mkf(float a, float b) { cf r; r.x = a; r.y = b; return r; }
+__host__ __device__ __forceinline__ cf operator+(cf a, cf b) { return mkf(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ cf operator-(cf a, cf b) { return mkf(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ cf operator-(cf a) { return mkf(-a.x, -a.y); }
+__host__ __device__ __forceinline__ cf operator*(cf a, cf b) { return mkf(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ __forceinline__ cf operator*(cf a, float s) { return mkf(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ cf cdiv(cf a, cf b)
+{
+  float d = b.x * b.x + b.y * b.y;
+  return mkf((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__host__ __device__ __forceinline__ cf to_cf(cd a) { return mkf((float)a.x, (float)a.y); }
+__host__ __device__ __forceinline__ cd to_cd(cf a) { return mk((double)a.x, (double)a.y); }
+
 // e^{i k R} for complex k = kr + i ki
 __device__ __forceinline__ cd eikr(double kr, double ki, double R)
 {
@@ -47,6 +65,13 @@ __device__ __forceinline__ double wsum(double v)
   return v;
 }
 __device__ __forceinline__ cd wsum(cd v) { return mk(wsum(v.x), wsum(v.y)); }
+__device__ __forceinline__ float wsum(float v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, BEM_WAVE);
+  return v;
+}
+__device__ __forceinline__ cf wsum(cf v) { return mkf(wsum(v.x), wsum(v.y)); }
 
 // triangle element record: r(z,e) = c0 + c1 z + c2 e, W = unnormalised normal
 struct TriGeo {

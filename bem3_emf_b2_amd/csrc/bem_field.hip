@@ -74,6 +74,60 @@ __device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, cons
   }
 }
 
+// G[3], H[3], F at rt from one triangle with the FP32 copy of a rule
+// (mixed-precision form): position relative to rt formed in FP64, the
+// integrand in FP32, sums over blocks of 12 points added in FP64
+struct QPt32 {
+  float z, e, w, wn0, wn1, wn2;
+};
+__device__ __forceinline__ void fma_cf(cf &a, cf b, float s) { a.x = fmaf(b.x, s, a.x); a.y = fmaf(b.y, s, a.y); }
+__device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, float kr, float ki,
+                                           const QPt32 *__restrict__ rule, int nq, double scale, bool flip, cd *G, cd *H,
+                                           double &F)
+{
+  const float d0x = (float)(g.c0[0] - rt[0]), d0y = (float)(g.c0[1] - rt[1]), d0z = (float)(g.c0[2] - rt[2]);
+  const float c1x = (float)g.c1[0], c1y = (float)g.c1[1], c1z = (float)g.c1[2];
+  const float c2x = (float)g.c2[0], c2y = (float)g.c2[1], c2z = (float)g.c2[2];
+  const float Wx = (float)g.W[0], Wy = (float)g.W[1], Wz = (float)g.W[2];
+  cd aG[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, aH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  double f = 0.0;
+  for (int p0 = 0; p0 < nq; p0 += 12) {
+    cf bG[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)}, bH[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)};
+    float bf = 0.0f;
+    const int p1 = min(p0 + 12, nq);
+    for (int p = p0; p < p1; p++) {
+      const QPt32 q = rule[p];
+      const float vx = fmaf(c2x, q.e, fmaf(c1x, q.z, d0x));
+      const float vy = fmaf(c2y, q.e, fmaf(c1y, q.z, d0y));
+      const float vz = fmaf(c2z, q.e, fmaf(c1z, q.z, d0z));
+      const float R2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
+      const float iR = rsqrtf(R2), R = R2 * iR, iR2 = iR * iR;
+      const float rdW = fmaf(vx, Wx, fmaf(vy, Wy, vz * Wz)) * iR;
+      float sn, cs;
+      sincosf(kr * R, &sn, &cs);
+      if (ki != 0.0f) {
+        const float at = expf(-ki * R);
+        sn *= at;
+        cs *= at;
+      }
+      const cf ce = mkf(cs, sn);
+      const cf cb = mkf(-1.0f - ki * R, kr * R);
+      const cf gk = ce * iR, hk = (cb * ce) * (rdW * iR2);
+      const float wn[3] = {q.wn0, q.wn1, q.wn2};
+#pragma unroll
+      for (int i = 0; i < 3; i++) { fma_cf(bG[i], gk, wn[i]); fma_cf(bH[i], hk, wn[i]); }
+      bf = fmaf(q.w, rdW * iR2, bf);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) { aG[i] = aG[i] + to_cd(bG[i]); aH[i] = aH[i] + to_cd(bH[i]); }
+    f += (double)bf;
+  }
+  double sh = flip ? -scale : scale;
+#pragma unroll
+  for (int i = 0; i < 3; i++) { G[i] = aG[i] * (scale * g.aW); H[i] = aH[i] * sh; }
+  F = f * sh;
+}
+
 // ---------------------------------------------------------------- domain id
 // Og[d] accumulates the solid angle of faces whose main side is d and
 // subtracts it from their sub side (setup2.c:944-978).  Loops the domains
@@ -401,10 +455,14 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
 // E, H at (t, zeta, eta) on the object's own surface in domain did: GLN for
 // s != t (coef_bd, no GHN switch), polar G for s == t (lit_coef_bd; H and F
 // vanish on a flat element).
+// F32: the other elements' sums with the FP32 rule gl32 (mixed-precision
+// form); the self element stays FP64
+template <bool F32>
 __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t, const double *__restrict__ tgt_ze,
                                                   int np, int did, const TriGeo *__restrict__ geo,
                                                   const int *__restrict__ sid, int dNe, double kr, double ki,
-                                                  const QPtN *__restrict__ gl, int ngl, double scale,
+                                                  const QPtN *__restrict__ gl, const QPt32 *__restrict__ gl32, int ngl,
+                                                  double scale,
                                                   const double *__restrict__ xl, const double *__restrict__ wl,
                                                   const double *__restrict__ xh, const double *__restrict__ wh,
                                                   const cd *__restrict__ E, const cd *__restrict__ dE,
@@ -428,7 +486,12 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
     const TriGeo g = geo[as];
     cd G[3], H[3], x0[3], x1[3];
     double f, x2;
-    tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
+    if constexpr (F32) {
+      (void)x0; (void)x1; (void)x2;
+      tri_coef32(g, rt, (float)kr, (float)ki, gl32, ngl, scale, sd < 0, G, H, f);
+    } else {
+      tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
+    }
     F += f;
     const cd *Es = E + (size_t)s * 12, *dEs = dE + (size_t)s * 12, *Hs = Hh + (size_t)s * 12, *dHs = dH + (size_t)s * 12;
 #pragma unroll
@@ -613,17 +676,24 @@ void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, i
                      (cd *)dUi);
   bem_prof_end(BH_K_SCATTER, st);
 }
+// gl32 != nullptr: the mixed-precision form (FP32 copy of the same rule)
 void bem_launch_field_bd(const int *tgt_t, const double *tgt_ze, int np, int did, const void *geo, const int *sid,
-                         int dNe, double kr, double ki, const void *gl, int ngl, double scale, const double *xl,
-                         const double *wl, const double *xh, const double *wh, const void *E, const void *dE,
-                         const void *H, const void *dH, void *out, double *rt_out, hipStream_t st)
+                         int dNe, double kr, double ki, const void *gl, const void *gl32, int ngl, double scale,
+                         const double *xl, const double *wl, const double *xh, const double *wh, const void *E,
+                         const void *dE, const void *H, const void *dH, void *out, double *rt_out, hipStream_t st)
 {
   if (np <= 0) return;
-  bem_prof_begin(BH_K_FIELD_BD, (double)np * dNe, st);
-  hipLaunchKernelGGL(k_field_bd, dim3((np + 3) / 4), dim3(256), 0, st, tgt_t, tgt_ze, np, did, (const TriGeo *)geo, sid,
-                     dNe, kr, ki, (const QPtN *)gl, ngl, scale, xl, wl, xh, wh, (const cd *)E, (const cd *)dE,
-                     (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
-  bem_prof_end(BH_K_FIELD_BD, st);
+  const int kid = gl32 ? BH_K_FIELD_BD32 : BH_K_FIELD_BD;
+  bem_prof_begin(kid, (double)np * dNe, st);
+  if (gl32)
+    hipLaunchKernelGGL(k_field_bd<true>, dim3((np + 3) / 4), dim3(256), 0, st, tgt_t, tgt_ze, np, did, (const TriGeo *)geo,
+                       sid, dNe, kr, ki, (const QPtN *)gl, (const QPt32 *)gl32, ngl, scale, xl, wl, xh, wh,
+                       (const cd *)E, (const cd *)dE, (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
+  else
+    hipLaunchKernelGGL(k_field_bd<false>, dim3((np + 3) / 4), dim3(256), 0, st, tgt_t, tgt_ze, np, did,
+                       (const TriGeo *)geo, sid, dNe, kr, ki, (const QPtN *)gl, (const QPt32 *)nullptr, ngl, scale, xl,
+                       wl, xh, wh, (const cd *)E, (const cd *)dE, (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
+  bem_prof_end(kid, st);
 }
 void bem_launch_incident_bd(const double *pts, const double *nrm, const int *slot, int np, const void *bm, int nb,
                             void *Ui, void *dUi, void *Ui0, void *dUi0, void *Uip, hipStream_t st)

@@ -1,41 +1,85 @@
-// bem_la.hip -- dense complex linear algebra for the BIE solve (FP64):
-//   * ZGEMM C = alpha A B + beta C on v_mfma_f64_16x16x4f64 (4 real MFMAs per
-//     complex product), 64x64 tiles per 256-thread workgroup, 16-deep K
-//     slices staged in LDS as split real/imaginary planes;
-//   * blocked right-looking LU with partial pivoting (row-major): per-column
-//     pivot/swap + panel update kernels, row interchanges on the rest of the
-//     matrix, unit-lower triangular solve for the U12 block row, and the
-//     rank-nb trailing update on the ZGEMM above;
-//   * forward/backward substitution with the factors for any number of
+// bem_la.hip -- dense complex linear algebra for the BIE solve, in complex
+// FP64 (cd) and complex FP32 (cf) from the same templates:
+//   * GEMM C = alpha A B + beta C on the 16x16x4 MFMAs (v_mfma_f64_16x16x4f64,
+//     v_mfma_f32_16x16x4f32): 64x64 tiles (four real products per complex
+//     product) for small shapes, 128x128 tiles in the 3M form for large ones,
+//     a streaming kernel for a few right-hand sides;
+//   * recursive LU with partial pivoting (row-major) whose panels are
+//     factorised in one cooperative launch, LAPACK's pivot choice and
+//     interchange vector;
+//   * recursive triangular solves with the factors for any number of
 //     right-hand sides.
 // Replaces LAPACKE_zgetrf/zgetri (solve_bieq2.c:389-399) and zgesv
-// (mo_solve.c:751) of the reference.
+// (mo_solve.c:751) of the reference.  The FP32 instances serve the
+// mixed-precision form (a complex-FP32 factorisation refined in FP64).
 #include "bem_dev.h"
 #include "bem_prof.h"
 
 #include <algorithm>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// per-precision types, LDS row padding of the 128x128 kernel (double rows
+// are read 16 lanes at a time: 132 keeps them conflict-free; float rows 32
+// lanes at a time: 144 = 16 mod 32 puts the two k rows of a half-wave on
+// disjoint banks) and the kernel-time registry ids
+template <class R> struct Cx;
+template <> struct Cx<double> {
+  typedef cd Z;
+  typedef d4 V;
+  static constexpr int ZP = 132;
+  static constexpr int KG = BH_K_ZGEMM, KGS = BH_K_ZGEMM_SMALL, KP = BH_K_LU_PANEL, KT = BH_K_TRSM;
+};
+template <> struct Cx<float> {
+  typedef cf Z;
+  typedef f4 V;
+  static constexpr int ZP = 144;
+  static constexpr int KG = BH_K_CGEMM, KGS = BH_K_CGEMM_SMALL, KP = BH_K_LU_PANEL_C, KT = BH_K_TRSM_C;
+};
+
+template <class Z> __host__ __device__ __forceinline__ Z zmk(double a, double b)
+{
+  Z r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+// row (within the 16x16 block) of accumulator register r of lane l; the
+// column is l % 16.  f64: l/16 + 4r (measured); f32: 4(l/16) + r.
+__device__ __forceinline__ int drow(double, int l, int r) { return (l >> 4) + 4 * r; }
+__device__ __forceinline__ int drow(float, int l, int r) { return 4 * (l >> 4) + r; }
+
+// |re| + |im| (LAPACK's izamax measure), compared in double
+__device__ __forceinline__ double zabs1(cd x) { return fabs(x.x) + fabs(x.y); }
+__device__ __forceinline__ double zabs1(cf x) { return (double)(fabsf(x.x) + fabsf(x.y)); }
 
 #define GB_M 64
 #define GB_N 64
 #define GB_K 16
 
 // C[m x n] = alpha * A[m x k] B[k x n] + beta * C   (row-major, complex)
-__global__ void __launch_bounds__(256) k_zgemm(int m, int n, int k, cd alpha, const cd *__restrict__ A, size_t lda,
-                                               const cd *__restrict__ B, size_t ldb, cd beta, cd *__restrict__ C,
-                                               size_t ldc)
+template <class R>
+__global__ void __launch_bounds__(256) k_gemm64(int m, int n, int k, typename Cx<R>::Z alpha,
+                                                const typename Cx<R>::Z *__restrict__ A, size_t lda,
+                                                const typename Cx<R>::Z *__restrict__ B, size_t ldb,
+                                                typename Cx<R>::Z beta, typename Cx<R>::Z *__restrict__ C, size_t ldc)
 {
-  __shared__ double sAr[GB_K][GB_M + 1], sAi[GB_K][GB_M + 1];
-  __shared__ double sBr[GB_K][GB_N + 1], sBi[GB_K][GB_N + 1];
+  typedef typename Cx<R>::Z Z;
+  typedef typename Cx<R>::V V;
+  __shared__ R sAr[GB_K][GB_M + 1], sAi[GB_K][GB_M + 1];
+  __shared__ R sBr[GB_K][GB_N + 1], sBi[GB_K][GB_N + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i0 = blockIdx.y * GB_M, j0 = blockIdx.x * GB_N;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;   // wave sub-tile origin
-  d4 cr[2][2], ci[2][2];
+  V cr[2][2], ci[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) { cr[a][b] = (d4){0, 0, 0, 0}; ci[a][b] = (d4){0, 0, 0, 0}; }
+    for (int b = 0; b < 2; b++) { cr[a][b] = (V){0, 0, 0, 0}; ci[a][b] = (V){0, 0, 0, 0}; }
   for (int k0 = 0; k0 < k; k0 += GB_K) {
     // stage A (64 x 16) and B (16 x 64): 4 complex values per thread each
 #pragma unroll
@@ -43,18 +87,18 @@ __global__ void __launch_bounds__(256) k_zgemm(int m, int n, int k, cd alpha, co
       int idx = tid + q * 256;
       int ar = idx >> 4, ac = idx & 15;          // A tile row, k column
       int gi = i0 + ar, gk = k0 + ac;
-      cd va = (gi < m && gk < k) ? A[(size_t)gi * lda + gk] : mk(0, 0);
+      Z va = (gi < m && gk < k) ? A[(size_t)gi * lda + gk] : zmk<Z>(0, 0);
       sAr[ac][ar] = va.x; sAi[ac][ar] = va.y;
       int br = idx >> 6, bc = idx & 63;          // B tile k row, column
       int gk2 = k0 + br, gj = j0 + bc;
-      cd vb = (gk2 < k && gj < n) ? B[(size_t)gk2 * ldb + gj] : mk(0, 0);
+      Z vb = (gk2 < k && gj < n) ? B[(size_t)gk2 * ldb + gj] : zmk<Z>(0, 0);
       sBr[br][bc] = vb.x; sBi[br][bc] = vb.y;
     }
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < GB_K; kk += 4) {
       int kr = kk + (lane >> 4), cc = lane & 15;
-      double ar[2], ai[2], nai[2], br[2], bi[2];
+      R ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
         ar[a] = sAr[kr][wr + 16 * a + cc];
@@ -67,58 +111,60 @@ __global__ void __launch_bounds__(256) k_zgemm(int m, int n, int k, cd alpha, co
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
-          cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[a], bi[b], cr[a][b], 0, 0, 0);
-          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
-          ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+          cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
+          cr[a][b] = mfma4(nai[a], bi[b], cr[a][b]);
+          ci[a][b] = mfma4(ar[a], bi[b], ci[a][b]);
+          ci[a][b] = mfma4(ai[a], br[b], ci[a][b]);
         }
     }
     __syncthreads();
   }
-  // D layout of the 16x16 f64 MFMA (measured): lane l, register r -> row l/16 + 4r, col l%16
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int gi = i0 + wr + 16 * a + (lane >> 4) + 4 * r;
+        int gi = i0 + wr + 16 * a + drow(R(0), lane, r);
         int gj = j0 + wc + 16 * b + (lane & 15);
         if (gi < m && gj < n) {
-          cd ab = mk(cr[a][b][r], ci[a][b][r]);
-          cd *c = C + (size_t)gi * ldc + gj;
-          cd v = alpha * ab;
-          if (beta.x != 0.0 || beta.y != 0.0) v = v + beta * (*c);
+          Z ab = zmk<Z>(cr[a][b][r], ci[a][b][r]);
+          Z *c = C + (size_t)gi * ldc + gj;
+          Z v = alpha * ab;
+          if (beta.x != 0 || beta.y != 0) v = v + beta * (*c);
           *c = v;
         }
       }
 }
 
-// ---- skinny ZGEMM (n <= 8 right-hand sides) ----
+// ---- skinny GEMM (n <= 8 right-hand sides) ----
 // Memory-bound products such as G1 N Y in the block solve: each wavefront
 // streams 4 rows of A along k (lanes over k, coalesced), the matching rows of
 // B (NC contiguous values per k) are reused across the 4 rows, and the
 // 4 x NC partial sums are reduced across the wave at the end.
-template <int NC>
-__global__ void __launch_bounds__(256) k_zgemm_skinny(int m, int n, int k, cd alpha, const cd *__restrict__ A,
-                                                      size_t lda, const cd *__restrict__ B, size_t ldb, cd beta,
-                                                      cd *__restrict__ C, size_t ldc)
+template <class R, int NC>
+__global__ void __launch_bounds__(256) k_gemm_skinny(int m, int n, int k, typename Cx<R>::Z alpha,
+                                                     const typename Cx<R>::Z *__restrict__ A, size_t lda,
+                                                     const typename Cx<R>::Z *__restrict__ B, size_t ldb,
+                                                     typename Cx<R>::Z beta, typename Cx<R>::Z *__restrict__ C,
+                                                     size_t ldc)
 {
+  typedef typename Cx<R>::Z Z;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row0 = (blockIdx.x * 4 + wave) * 4;
   if (row0 >= m) return;
-  cd acc[4][NC];
+  Z acc[4][NC];
 #pragma unroll
   for (int r = 0; r < 4; r++)
 #pragma unroll
-    for (int j = 0; j < NC; j++) acc[r][j] = mk(0, 0);
-  const cd *a0 = A + (size_t)min(row0 + 0, m - 1) * lda, *a1 = A + (size_t)min(row0 + 1, m - 1) * lda;
-  const cd *a2 = A + (size_t)min(row0 + 2, m - 1) * lda, *a3 = A + (size_t)min(row0 + 3, m - 1) * lda;
+    for (int j = 0; j < NC; j++) acc[r][j] = zmk<Z>(0, 0);
+  const Z *a0 = A + (size_t)min(row0 + 0, m - 1) * lda, *a1 = A + (size_t)min(row0 + 1, m - 1) * lda;
+  const Z *a2 = A + (size_t)min(row0 + 2, m - 1) * lda, *a3 = A + (size_t)min(row0 + 3, m - 1) * lda;
   for (int kk = lane; kk < k; kk += 64) {
-    cd b[NC];
+    Z b[NC];
 #pragma unroll
-    for (int j = 0; j < NC; j++) b[j] = j < n ? B[(size_t)kk * ldb + j] : mk(0, 0);
-    const cd v[4] = {a0[kk], a1[kk], a2[kk], a3[kk]};
+    for (int j = 0; j < NC; j++) b[j] = j < n ? B[(size_t)kk * ldb + j] : zmk<Z>(0, 0);
+    const Z v[4] = {a0[kk], a1[kk], a2[kk], a3[kk]};
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -128,44 +174,50 @@ __global__ void __launch_bounds__(256) k_zgemm_skinny(int m, int n, int k, cd al
   for (int r = 0; r < 4; r++)
 #pragma unroll
     for (int j = 0; j < NC; j++) acc[r][j] = wsum(acc[r][j]);
-  const bool bz = beta.x == 0.0 && beta.y == 0.0;
+  const bool bz = beta.x == 0 && beta.y == 0;
 #pragma unroll
   for (int r = 0; r < 4; r++)
 #pragma unroll
     for (int j = 0; j < NC; j++)
       if (lane == r * NC + j && row0 + r < m && j < n) {
-        cd *c = C + (size_t)(row0 + r) * ldc + j;
-        cd v = alpha * acc[r][j];
+        Z *c = C + (size_t)(row0 + r) * ldc + j;
+        Z v = alpha * acc[r][j];
         if (!bz) v = v + beta * (*c);
         *c = v;
       }
 }
 
-// ---- large-tile ZGEMM ----
+// ---- large-tile GEMM ----
 // Complex product in the 3M (Gauss) form: T1 = Ar Br, T2 = Ai Bi,
 // T3 = (Ar + Ai)(Br + Bi), Cr = T1 - T2, Ci = T3 - T1 - T2: three real
-// v_mfma_f64_16x16x4f64 per 16x16x4 step instead of four (normwise stable,
-// the error bound of a real product of the summed operands).
+// 16x16x4 MFMAs per step instead of four (normwise stable, the error bound
+// of a real product of the summed operands).
 // 128x128 tile per 512-thread workgroup: 8 waves in a 2x4 grid, each wave a
 // 64x32 block = 4x2 MFMA tiles of 16x16 (three accumulators each, all in
 // VGPRs, two waves per SIMD).  K slices of 8 complex values are
 // double-buffered in LDS as six planes (Ar, Ai, Ar+Ai, Br, Bi, Br+Bi; rows
-// padded by 4 doubles against bank conflicts); the next slice is loaded into
-// registers from clamped, always-valid addresses before the current slice's
-// MFMAs and masked to zero outside the matrix only when written to LDS, so
-// the load latency overlaps the MFMAs.  Tiles are assigned so that each XCD
+// padded against bank conflicts); the next slice is loaded into registers
+// from clamped, always-valid addresses before the current slice's MFMAs and
+// masked to zero outside the matrix only when written to LDS, so the load
+// latency overlaps the MFMAs.  Tiles are assigned so that each XCD
 // (workgroup id mod 8) works on a contiguous band of C, grouped 8 tile-rows
 // high, for L2 reuse of the A and B panels.
 #define ZT 128
 #define ZK 8
-#define ZP (ZT + 4)
-#define ZLDS (2 * 6 * ZK * ZP * (int)sizeof(double))
+template <class R> constexpr int zlds_bytes() { return 2 * 6 * ZK * Cx<R>::ZP * (int)sizeof(R); }
 
-__global__ void __launch_bounds__(512) k_zgemm128(int m, int n, int k, cd alpha, const cd *__restrict__ A, size_t lda,
-                                                const cd *__restrict__ B, size_t ldb, cd beta, cd *__restrict__ C,
-                                                size_t ldc, int tiles_x, int tiles_y)
+template <class R>
+__global__ void __launch_bounds__(512) k_gemm128(int m, int n, int k, typename Cx<R>::Z alpha,
+                                                 const typename Cx<R>::Z *__restrict__ A, size_t lda,
+                                                 const typename Cx<R>::Z *__restrict__ B, size_t ldb,
+                                                 typename Cx<R>::Z beta, typename Cx<R>::Z *__restrict__ C, size_t ldc,
+                                                 int tiles_x, int tiles_y)
 {
-  extern __shared__ double zsm[];
+  typedef typename Cx<R>::Z Z;
+  typedef typename Cx<R>::V V;
+  constexpr int ZP = Cx<R>::ZP;
+  extern __shared__ double dyn_lds[];
+  R *zsm = (R *)dyn_lds;
   const int T = tiles_x * tiles_y, per = (T + 7) / 8;
   const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (L >= T) return;
@@ -177,12 +229,12 @@ __global__ void __launch_bounds__(512) k_zgemm128(int m, int n, int k, cd alpha,
   const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
   const int ra = tid >> 2, ca = (tid & 3) * 2;
   const int rb = tid >> 6, cb = (tid & 63) * 2;
-  d4 t1[4][2], t2[4][2], t3[4][2];
+  V t1[4][2], t2[4][2], t3[4][2];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) { t1[a][b] = (d4){0, 0, 0, 0}; t2[a][b] = (d4){0, 0, 0, 0}; t3[a][b] = (d4){0, 0, 0, 0}; }
-  cd pa[2], pb[2];
+    for (int b = 0; b < 2; b++) { t1[a][b] = (V){0, 0, 0, 0}; t2[a][b] = (V){0, 0, 0, 0}; t3[a][b] = (V){0, 0, 0, 0}; }
+  Z pa[2], pb[2];
   bool oka[2], okb[2];
   const int gi = min(i0 + ra, m - 1);
   const bool rowok = i0 + ra < m;
@@ -198,11 +250,11 @@ __global__ void __launch_bounds__(512) k_zgemm128(int m, int n, int k, cd alpha,
   };
   const int PL = ZK * ZP;   // one LDS plane
   auto lstore = [&](int s) {
-    double *Ar = zsm + s * 6 * PL, *Ai = Ar + PL, *As = Ai + PL, *Br = As + PL, *Bi = Br + PL, *Bs = Bi + PL;
+    R *Ar = zsm + s * 6 * PL, *Ai = Ar + PL, *As = Ai + PL, *Br = As + PL, *Bi = Br + PL, *Bs = Bi + PL;
 #pragma unroll
     for (int q = 0; q < 2; q++) {
-      double ax = oka[q] ? pa[q].x : 0.0, ay = oka[q] ? pa[q].y : 0.0;
-      double bx = okb[q] ? pb[q].x : 0.0, by = okb[q] ? pb[q].y : 0.0;
+      R ax = oka[q] ? pa[q].x : (R)0, ay = oka[q] ? pa[q].y : (R)0;
+      R bx = okb[q] ? pb[q].x : (R)0, by = okb[q] ? pb[q].y : (R)0;
       Ar[(ca + q) * ZP + ra] = ax;
       Ai[(ca + q) * ZP + ra] = ay;
       As[(ca + q) * ZP + ra] = ax + ay;
@@ -218,11 +270,11 @@ __global__ void __launch_bounds__(512) k_zgemm128(int m, int n, int k, cd alpha,
   for (int k0 = 0; k0 < k; k0 += ZK) {
     const bool more = k0 + ZK < k;
     if (more) gload(k0 + ZK);
-    const double *Ar = zsm + s * 6 * PL, *Ai = Ar + PL, *As = Ai + PL, *Br = As + PL, *Bi = Br + PL, *Bs = Bi + PL;
+    const R *Ar = zsm + s * 6 * PL, *Ai = Ar + PL, *As = Ai + PL, *Br = As + PL, *Bi = Br + PL, *Bs = Bi + PL;
 #pragma unroll
     for (int ks = 0; ks < ZK; ks += 4) {
       const int kr = ks + (lane >> 4), cc = lane & 15;
-      double ar[4], ai[4], as[4], br[2], bi[2], bs[2];
+      R ar[4], ai[4], as[4], br[2], bi[2], bs[2];
 #pragma unroll
       for (int a = 0; a < 4; a++) {
         ar[a] = Ar[kr * ZP + wr + 16 * a + cc];
@@ -239,47 +291,51 @@ __global__ void __launch_bounds__(512) k_zgemm128(int m, int n, int k, cd alpha,
       for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) {
-          t1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], t1[a][b], 0, 0, 0);
-          t2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bi[b], t2[a][b], 0, 0, 0);
-          t3[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[a], bs[b], t3[a][b], 0, 0, 0);
+          t1[a][b] = mfma4(ar[a], br[b], t1[a][b]);
+          t2[a][b] = mfma4(ai[a], bi[b], t2[a][b]);
+          t3[a][b] = mfma4(as[a], bs[b], t3[a][b]);
         }
     }
     if (more) lstore(s ^ 1);
     __syncthreads();
     s ^= 1;
   }
-  const bool bz = beta.x == 0.0 && beta.y == 0.0;
+  const bool bz = beta.x == 0 && beta.y == 0;
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gi2 = i0 + wr + 16 * a + (lane >> 4) + 4 * r;
+        const int gi2 = i0 + wr + 16 * a + drow(R(0), lane, r);
         const int gj = j0 + wc + 16 * b + (lane & 15);
         if (gi2 < m && gj < n) {
-          cd *c = C + (size_t)gi2 * ldc + gj;
-          cd v = alpha * mk(t1[a][b][r] - t2[a][b][r], t3[a][b][r] - t1[a][b][r] - t2[a][b][r]);
+          Z *c = C + (size_t)gi2 * ldc + gj;
+          Z v = alpha * zmk<Z>(t1[a][b][r] - t2[a][b][r], t3[a][b][r] - t1[a][b][r] - t2[a][b][r]);
           if (!bz) v = v + beta * (*c);
           *c = v;
         }
       }
 }
 
-// ---- LU panel ----
-// The panel (at most 64 columns, rows k0..n-1) is factorised in a
-// column-major copy P (P[c * ldp + r], r = row - k0) without moving rows: the
-// pivot of column j is searched among the rows not yet used as pivots, and
-// used[r] records the column at which row r became a pivot (INT_MAX while
-// unused).  One launch per column (see k_panel_col).  Rows are interchanged
-// afterwards, once per panel, in LAPACK's sequential-swap form.
+// ---- LU panel, launch-per-column form ----
+// Used where a cooperative launch is not available.  The panel (at most 64
+// columns, rows k0..n-1) is factorised in a column-major copy P
+// (P[c * ldp + r], r = row - k0) without moving rows: the pivot of column j
+// is searched among the rows not yet used as pivots, and used[r] records
+// the column at which row r became a pivot (INT_MAX while unused).  One
+// launch per column (see k_panel_col).  Rows are interchanged afterwards,
+// once per panel, in LAPACK's sequential-swap form.
 #define LU_NT 256
 #define PERM_LDS_MAX (2 * 16384 * (int)sizeof(int))   // position maps of panels up to 16384 rows
 
-__global__ void __launch_bounds__(256) k_panel_load(const cd *__restrict__ A, size_t lda, int k0, int kb, int m,
-                                                    cd *__restrict__ P, size_t ldp, int *__restrict__ used)
+template <class R>
+__global__ void __launch_bounds__(256) k_panel_load(const typename Cx<R>::Z *__restrict__ A, size_t lda, int k0, int kb,
+                                                    int m, typename Cx<R>::Z *__restrict__ P, size_t ldp,
+                                                    int *__restrict__ used)
 {
-  __shared__ cd t[64][33];
+  typedef typename Cx<R>::Z Z;
+  __shared__ Z t[64][33];
   const int r0 = blockIdx.x * 32, tid = threadIdx.x;
   for (int idx = tid; idx < 32 * 64; idx += 256) {
     int rr = idx >> 6, c = idx & 63;
@@ -295,10 +351,13 @@ __global__ void __launch_bounds__(256) k_panel_load(const cd *__restrict__ A, si
 
 // panel back into A with the panel's row interchanges applied: position t
 // of the panel receives physical panel row rowAt[t]
-__global__ void __launch_bounds__(256) k_panel_store(cd *__restrict__ A, size_t lda, int k0, int kb, int m,
-                                                     const cd *__restrict__ P, size_t ldp, const int *__restrict__ rowAt)
+template <class R>
+__global__ void __launch_bounds__(256) k_panel_store(typename Cx<R>::Z *__restrict__ A, size_t lda, int k0, int kb, int m,
+                                                     const typename Cx<R>::Z *__restrict__ P, size_t ldp,
+                                                     const int *__restrict__ rowAt)
 {
-  __shared__ cd t[64][33];
+  typedef typename Cx<R>::Z Z;
+  __shared__ Z t[64][33];
   const int r0 = blockIdx.x * 32, tid = threadIdx.x;
   for (int idx = tid; idx < 32 * 64; idx += 256) {
     int c = idx >> 5, rr = idx & 31;
@@ -328,12 +387,14 @@ __device__ __forceinline__ void wave_argmax(double &v, int &i)
 // records the pivot and stages the pivot row in LDS; (2) eliminates column
 // j-1 on its own rows; (3) leaves its maximum of column j for launch j+1.
 // Maxima of consecutive columns alternate between two halves of partV/partI.
-__global__ void __launch_bounds__(LU_NT) k_panel_col(cd *__restrict__ P, size_t ldp, int m, int kb, int j,
+template <class R>
+__global__ void __launch_bounds__(LU_NT) k_panel_col(typename Cx<R>::Z *__restrict__ P, size_t ldp, int m, int kb, int j,
                                                      int *__restrict__ used, int *__restrict__ pv,
                                                      double *__restrict__ partV, int *__restrict__ partI, int nbmax,
                                                      int *__restrict__ info, int k0)
 {
-  __shared__ cd prow[64];
+  typedef typename Cx<R>::Z Z;
+  __shared__ Z prow[64];
   __shared__ double sv[LU_NT / 64];
   __shared__ int si[LU_NT / 64];
   __shared__ int sp;
@@ -368,16 +429,16 @@ __global__ void __launch_bounds__(LU_NT) k_panel_col(cd *__restrict__ P, size_t 
   }
   // rows not yet used as pivots: eliminated for column j-1, candidates for column j
   const bool act = r < m && r != p && used[r] > j - 1;
-  cd xj = mk(0, 0);   // this row's value in column j after the elimination
+  Z xj = zmk<Z>(0, 0);   // this row's value in column j after the elimination
   if (act && j < kb) xj = P[(size_t)j * ldp + r];
   if (act && j > 0) {
-    const cd a = prow[0];
-    if (a.x != 0.0 || a.y != 0.0) {
-      const cd l = cdiv(P[(size_t)(j - 1) * ldp + r], a);
+    const Z a = prow[0];
+    if (a.x != 0 || a.y != 0) {
+      const Z l = cdiv(P[(size_t)(j - 1) * ldp + r], a);
       P[(size_t)(j - 1) * ldp + r] = l;
       // columns j .. kb-1 in batches of 16 independent loads
       for (int c0 = j; c0 < kb; c0 += 16) {
-        cd x[16];
+        Z x[16];
 #pragma unroll
         for (int q = 0; q < 16; q++)
           if (c0 + q < kb) x[q] = P[(size_t)(c0 + q) * ldp + r];
@@ -396,7 +457,7 @@ __global__ void __launch_bounds__(LU_NT) k_panel_col(cd *__restrict__ P, size_t 
   double best = -1.0;
   int bidx = 0x7fffffff;
   if (act) {
-    best = fabs(xj.x) + fabs(xj.y);         // |re|+|im| as LAPACK izamax
+    best = zabs1(xj);                       // |re|+|im| as LAPACK izamax
     if (!(best >= 0.0)) best = INFINITY;   // a NaN entry still wins over non-candidates
     bidx = r;
   }
@@ -413,7 +474,7 @@ __global__ void __launch_bounds__(LU_NT) k_panel_col(cd *__restrict__ P, size_t 
 
 // ---- one-launch panel (cooperative) ----
 // The whole panel in one cooperative launch: one 64-lane workgroup per 64
-// panel rows, the rows held in LDS (column-major, 64 KB), a grid barrier per
+// panel rows, the rows held in LDS (column-major), a grid barrier per
 // column.  Column j: every workgroup publishes its best candidate (maximum
 // |re|+|im| among its rows not yet used as pivots, smallest row on ties)
 // together with that row's entries j..kb-1; after the barrier every
@@ -428,7 +489,7 @@ __global__ void __launch_bounds__(LU_NT) k_panel_col(cd *__restrict__ P, size_t 
 struct CoopWs {
   double *candV;         // [2][nbmax]
   int *candI;            // [2][nbmax]
-  cd *candRow;           // [2][nbmax][64]
+  void *candRow;         // [2][nbmax][64] complex values of the precision in use
   int *ip;               // [64] interchange of each column (panel positions)
   unsigned int *cnt;
   int *err;
@@ -458,16 +519,19 @@ __device__ __forceinline__ void coop_barrier(unsigned int *cnt, int *err, unsign
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-__global__ void __launch_bounds__(64) k_panel_coop(cd *__restrict__ A, size_t lda, int k0, int kb, int m,
+template <class R>
+__global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict__ A, size_t lda, int k0, int kb, int m,
                                                    int *__restrict__ piv, int *__restrict__ info, CoopWs w)
 {
-  __shared__ cd S[64 * 64];   // S[c * 64 + lane]: column c of this workgroup's 64 rows
-  __shared__ cd sp[64];       // pivot row of the current column
+  typedef typename Cx<R>::Z Z;
+  __shared__ Z S[64 * 64];   // S[c * 64 + lane]: column c of this workgroup's 64 rows
+  __shared__ Z sp[64];       // pivot row of the current column
+  Z *candRow = (Z *)w.candRow;
   const int lane = threadIdx.x, blk = blockIdx.x, nb = gridDim.x;
   const int r = blk * 64 + lane;
   const bool live = r < m;
   unsigned int target = 0;
-  for (int c = 0; c < kb; c++) S[c * 64 + lane] = live ? A[(size_t)(k0 + r) * lda + k0 + c] : mk(0, 0);
+  for (int c = 0; c < kb; c++) S[c * 64 + lane] = live ? A[(size_t)(k0 + r) * lda + k0 + c] : zmk<Z>(0, 0);
   bool used = false;
   int pos = r;
   for (int j = 0; j < kb; j++) {
@@ -475,8 +539,7 @@ __global__ void __launch_bounds__(64) k_panel_coop(cd *__restrict__ A, size_t ld
     double v = -1.0;
     int vi = 0x7fffffff;
     if (live && !used) {
-      const cd x = S[j * 64 + lane];
-      v = fabs(x.x) + fabs(x.y);           // |re|+|im| as LAPACK izamax
+      v = zabs1(S[j * 64 + lane]);        // |re|+|im| as LAPACK izamax
       if (!(v >= 0.0)) v = INFINITY;      // a NaN entry still wins over non-candidates
       vi = r;
     }
@@ -484,7 +547,7 @@ __global__ void __launch_bounds__(64) k_panel_coop(cd *__restrict__ A, size_t ld
     const int buf = j & 1;
     if (lane == 0) { w.candV[buf * w.nbmax + blk] = v; w.candI[buf * w.nbmax + blk] = vi; }
     if (v >= 0.0 && lane == vi - blk * 64)
-      for (int c = j; c < kb; c++) w.candRow[((size_t)buf * w.nbmax + blk) * 64 + c] = S[c * 64 + lane];
+      for (int c = j; c < kb; c++) candRow[((size_t)buf * w.nbmax + blk) * 64 + c] = S[c * 64 + lane];
     coop_barrier(w.cnt, w.err, target, nb);
     // the interchange of column j-1 (published before this barrier)
     if (j > 0) {
@@ -504,17 +567,17 @@ __global__ void __launch_bounds__(64) k_panel_coop(cd *__restrict__ A, size_t ld
     if (bi < 0 || bi >= m) { bi = m - 1; bw = bi / 64; if (blk == 0 && lane == 0 && *info == 0) *info = k0 + j + 1; }
     if (blk == 0 && lane == 0 && !(b > 0.0) && *info == 0) *info = k0 + j + 1;
     if (lane >= j && lane < kb) {
-      const volatile double *src = (const volatile double *)(w.candRow + ((size_t)buf * w.nbmax + bw) * 64 + lane);
-      sp[lane] = mk(src[0], src[1]);
+      const volatile R *src = (const volatile R *)(candRow + ((size_t)buf * w.nbmax + bw) * 64 + lane);
+      sp[lane] = zmk<Z>(src[0], src[1]);
     }
     __syncthreads();
     if (r == bi) {
       used = true;
       w.ip[j] = pos;   // LAPACK interchange: column j's position <-> the pivot row's current position
     }
-    const cd a = sp[j];
-    if (live && !used && (a.x != 0.0 || a.y != 0.0)) {
-      const cd l = cdiv(S[j * 64 + lane], a);
+    const Z a = sp[j];
+    if (live && !used && (a.x != 0 || a.y != 0)) {
+      const Z l = cdiv(S[j * 64 + lane], a);
       S[j * 64 + lane] = l;
       for (int c = j + 1; c < kb; c++) S[c * 64 + lane] = S[c * 64 + lane] - l * sp[c];
     }
@@ -560,14 +623,15 @@ __global__ void __launch_bounds__(256) k_panel_perm(const int *__restrict__ pv, 
 }
 
 // apply interchanges piv[k0..k1) to columns [c0, c1) of A (or of a RHS block)
-__global__ void __launch_bounds__(256) k_laswp(cd *A, size_t lda, int c0, int c1, int k0, int k1, const int *piv)
+template <class Z>
+__global__ void __launch_bounds__(256) k_laswp(Z *A, size_t lda, int c0, int c1, int k0, int k1, const int *piv)
 {
   int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= c1) return;
   for (int j = k0; j < k1; j++) {
     int p = piv[j];
     if (p != j) {
-      cd a = A[(size_t)j * lda + c];
+      Z a = A[(size_t)j * lda + c];
       A[(size_t)j * lda + c] = A[(size_t)p * lda + c];
       A[(size_t)p * lda + c] = a;
     }
@@ -583,20 +647,22 @@ __global__ void __launch_bounds__(256) k_laswp(cd *A, size_t lda, int c0, int c1
 // FMAs.  The interleaved ownership makes the four rows a quad reads together
 // consecutive rows of the LDS block (distinct banks with the padded stride).
 #define TRS_LD 65
-__global__ void __launch_bounds__(256) k_trsm_lunit(const cd *__restrict__ L, size_t lda, int nb, cd *X, size_t ldb,
-                                                    int ncol)
+template <class R>
+__global__ void __launch_bounds__(256) k_trsm_lunit(const typename Cx<R>::Z *__restrict__ L, size_t lda, int nb,
+                                                    typename Cx<R>::Z *X, size_t ldb, int ncol)
 {
-  extern __shared__ double tsm[];
-  cd *sL = (cd *)tsm;   // nb x TRS_LD
+  typedef typename Cx<R>::Z Z;
+  extern __shared__ double dyn_lds[];
+  Z *sL = (Z *)dyn_lds;   // nb x TRS_LD
   const int tid = threadIdx.x, lane = tid & 63, g = lane & 3;
   const int col = blockIdx.x * 64 + (tid >> 6) * 16 + (lane >> 2);
   for (int idx = tid; idx < nb * nb; idx += 256) sL[(idx / nb) * TRS_LD + idx % nb] = L[(size_t)(idx / nb) * lda + idx % nb];
-  cd x[16];
+  Z x[16];
   const bool cok = col < ncol;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const int r = 4 * i + g;
-    x[i] = (cok && r < nb) ? X[(size_t)r * ldb + col] : mk(0, 0);
+    x[i] = (cok && r < nb) ? X[(size_t)r * ldb + col] : zmk<Z>(0, 0);
   }
   __syncthreads();
   const int qsrc = lane & ~3;
@@ -605,17 +671,17 @@ __global__ void __launch_bounds__(256) k_trsm_lunit(const cd *__restrict__ L, si
 #pragma unroll
   for (int q = 0; q < 64; q++) {
     if (q < nb) {
-      cd lv[16];
+      Z lv[16];
 #pragma unroll
       for (int i = 0; i < 16; i++) lv[i] = sL[min(4 * i + g, nb - 1) * TRS_LD + q];
-      const double vx = __shfl(x[q >> 2].x, qsrc | (q & 3), BEM_WAVE);
-      const double vy = __shfl(x[q >> 2].y, qsrc | (q & 3), BEM_WAVE);
-      const cd xq = mk(vx, vy);
+      const R vx = __shfl(x[q >> 2].x, qsrc | (q & 3), BEM_WAVE);
+      const R vy = __shfl(x[q >> 2].y, qsrc | (q & 3), BEM_WAVE);
+      const Z xq = zmk<Z>(vx, vy);
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         const int r = 4 * i + g;
         const bool on = r > q && r < nb;
-        const cd c = mk(on ? lv[i].x : 0.0, on ? lv[i].y : 0.0);
+        const Z c = zmk<Z>(on ? lv[i].x : (R)0, on ? lv[i].y : (R)0);
         x[i] = x[i] - c * xq;
       }
     }
@@ -629,52 +695,56 @@ __global__ void __launch_bounds__(256) k_trsm_lunit(const cd *__restrict__ L, si
 }
 
 // one step q of the backward substitution, then the steps q-1 .. 0
-template <int Q>
-__device__ __forceinline__ void trsm_upper_steps(cd (&x)[16], const cd *sU, const cd *dinv, int nb, int g, int qsrc)
+template <int Q, class R>
+__device__ __forceinline__ void trsm_upper_steps(typename Cx<R>::Z (&x)[16], const typename Cx<R>::Z *sU,
+                                                 const typename Cx<R>::Z *dinv, int nb, int g, int qsrc)
 {
+  typedef typename Cx<R>::Z Z;
   if (Q < nb) {
-    cd uv[16];
+    Z uv[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) uv[i] = sU[min(4 * i + g, nb - 1) * TRS_LD + Q];
-    const cd xo = x[Q >> 2] * dinv[Q];   // used by the owner lanes only
+    const Z xo = x[Q >> 2] * dinv[Q];   // used by the owner lanes only
     if (g == (Q & 3)) x[Q >> 2] = xo;
-    const double vx = __shfl(x[Q >> 2].x, qsrc | (Q & 3), BEM_WAVE);
-    const double vy = __shfl(x[Q >> 2].y, qsrc | (Q & 3), BEM_WAVE);
-    const cd xq = mk(vx, vy);
+    const R vx = __shfl(x[Q >> 2].x, qsrc | (Q & 3), BEM_WAVE);
+    const R vy = __shfl(x[Q >> 2].y, qsrc | (Q & 3), BEM_WAVE);
+    const Z xq = zmk<Z>(vx, vy);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int r = 4 * i + g;
       const bool on = r < Q;
-      const cd c = mk(on ? uv[i].x : 0.0, on ? uv[i].y : 0.0);
+      const Z c = zmk<Z>(on ? uv[i].x : (R)0, on ? uv[i].y : (R)0);
       x[i] = x[i] - c * xq;
     }
   }
-  if constexpr (Q > 0) trsm_upper_steps<Q - 1>(x, sU, dinv, nb, g, qsrc);
+  if constexpr (Q > 0) trsm_upper_steps<Q - 1, R>(x, sU, dinv, nb, g, qsrc);
 }
 
 // X := U^{-1} X with U the upper nb x nb block (non-unit diagonal)
-__global__ void __launch_bounds__(256) k_trsm_upper(const cd *__restrict__ U, size_t lda, int nb, cd *X, size_t ldb,
-                                                    int ncol)
+template <class R>
+__global__ void __launch_bounds__(256) k_trsm_upper(const typename Cx<R>::Z *__restrict__ U, size_t lda, int nb,
+                                                    typename Cx<R>::Z *X, size_t ldb, int ncol)
 {
-  extern __shared__ double tsm[];
-  cd *sU = (cd *)tsm;
-  __shared__ cd dinv[64];   // reciprocals of the diagonal
+  typedef typename Cx<R>::Z Z;
+  extern __shared__ double dyn_lds[];
+  Z *sU = (Z *)dyn_lds;
+  __shared__ Z dinv[64];   // reciprocals of the diagonal
   const int tid = threadIdx.x, lane = tid & 63, g = lane & 3;
   const int col = blockIdx.x * 64 + (tid >> 6) * 16 + (lane >> 2);
   for (int idx = tid; idx < nb * nb; idx += 256) sU[(idx / nb) * TRS_LD + idx % nb] = U[(size_t)(idx / nb) * lda + idx % nb];
-  if (tid < nb) dinv[tid] = cdiv(mk(1.0, 0.0), U[(size_t)tid * lda + tid]);
-  cd x[16];
+  if (tid < nb) dinv[tid] = cdiv(zmk<Z>(1, 0), U[(size_t)tid * lda + tid]);
+  Z x[16];
   const bool cok = col < ncol;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const int r = 4 * i + g;
-    x[i] = (cok && r < nb) ? X[(size_t)r * ldb + col] : mk(0, 0);
+    x[i] = (cok && r < nb) ? X[(size_t)r * ldb + col] : zmk<Z>(0, 0);
   }
   __syncthreads();
   const int qsrc = lane & ~3;
   // the 64 steps are written out through a template so that every x index
   // is a compile-time constant (a runtime index would put x in scratch)
-  trsm_upper_steps<63>(x, sU, dinv, nb, g, qsrc);
+  trsm_upper_steps<63, R>(x, sU, dinv, nb, g, qsrc);
   if (cok)
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -683,11 +753,8 @@ __global__ void __launch_bounds__(256) k_trsm_upper(const cd *__restrict__ U, si
     }
 }
 
-extern "C" {
-
-// the triangular-solve kernels stage a 64x64 factor block and a 64x64 RHS
-// block (128 KiB) in LDS, above the default dynamic-LDS limit
-static bool g_big_tiles = false;   // set once the 128x128 kernel's LDS limit is raised
+// ---- host side ----
+static bool g_big_tiles = false;   // set once the 128x128 kernels' LDS limits are raised
 static bool g_coop = false;        // the device supports cooperative launches
 // exchange area of the one-launch panel: up to COOP_NBMAX workgroups
 // (panels of up to 65536 rows)
@@ -696,51 +763,47 @@ static bool g_coop = false;        // the device supports cooperative launches
                     2 * (size_t)COOP_NBMAX * 64 * sizeof(cd))
 static void *g_coop_buf = nullptr;
 
-int bem_la_init(void)
+template <class R> static int la_init_t()
 {
-  const int bytes = 64 * TRS_LD * (int)sizeof(cd);
-  if (hipFuncSetAttribute((const void *)k_trsm_lunit, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
-  if (hipFuncSetAttribute((const void *)k_trsm_upper, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
-  if (hipFuncSetAttribute((const void *)k_panel_perm, hipFuncAttributeMaxDynamicSharedMemorySize, PERM_LDS_MAX) != hipSuccess) return -1;
-  if (hipFuncSetAttribute((const void *)k_zgemm128, hipFuncAttributeMaxDynamicSharedMemorySize, ZLDS) != hipSuccess) return -1;
-  g_big_tiles = true;
-  int dev = 0, coop = 0;
-  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess)
-    g_coop = coop != 0;
+  // the triangular-solve kernels stage a 64x64 factor block in LDS, the
+  // 128x128 GEMM its double-buffered planes: above the default dynamic limit
+  const int bytes = 64 * TRS_LD * (int)sizeof(typename Cx<R>::Z);
+  if (hipFuncSetAttribute((const void *)k_trsm_lunit<R>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void *)k_trsm_upper<R>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
+  if (hipFuncSetAttribute((const void *)k_gemm128<R>, hipFuncAttributeMaxDynamicSharedMemorySize, zlds_bytes<R>()) != hipSuccess)
+    return -1;
   return 0;
 }
 
-void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t lda, const void *B, size_t ldb,
-               double br, double bi, void *C, size_t ldc, hipStream_t st)
+template <class R>
+static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename Cx<R>::Z *A, size_t lda,
+                   const typename Cx<R>::Z *B, size_t ldb, typename Cx<R>::Z beta, typename Cx<R>::Z *C, size_t ldc,
+                   hipStream_t st)
 {
   if (m <= 0 || n <= 0) return;
   const int tx = (n + ZT - 1) / ZT, ty = (m + ZT - 1) / ZT, T = tx * ty;
-  const bool big = g_big_tiles && !(n <= 8 && k >= 256) && T >= 256 && k >= 32;
-  const int kid = big ? BH_K_ZGEMM : BH_K_ZGEMM_SMALL;
+  const bool skinny = n <= 8 && k >= 256;
+  const bool big = g_big_tiles && !skinny && T >= 256 && k >= 32;
+  const int kid = big ? Cx<R>::KG : Cx<R>::KGS;
   bem_prof_begin(kid, 8.0 * m * n * k, st);
-  if (n <= 8 && k >= 256) {
+  if (skinny) {
     // a few right-hand sides against a long k: stream A once
     dim3 grid((m + 15) / 16);
     if (n == 1)
-      hipLaunchKernelGGL(k_zgemm_skinny<1>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
-                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+      hipLaunchKernelGGL((k_gemm_skinny<R, 1>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     else if (n == 2)
-      hipLaunchKernelGGL(k_zgemm_skinny<2>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
-                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+      hipLaunchKernelGGL((k_gemm_skinny<R, 2>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     else if (n <= 4)
-      hipLaunchKernelGGL(k_zgemm_skinny<4>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
-                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+      hipLaunchKernelGGL((k_gemm_skinny<R, 4>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     else
-      hipLaunchKernelGGL(k_zgemm_skinny<8>, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda,
-                         (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc);
+      hipLaunchKernelGGL((k_gemm_skinny<R, 8>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   } else if (big) {
     // at least one 128x128 tile per CU
-    hipLaunchKernelGGL(k_zgemm128, dim3(8 * ((T + 7) / 8)), dim3(512), ZLDS, st, m, n, k, mk(ar, ai), (const cd *)A,
-                       lda, (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc, tx, ty);
+    hipLaunchKernelGGL(k_gemm128<R>, dim3(8 * ((T + 7) / 8)), dim3(512), zlds_bytes<R>(), st, m, n, k, alpha, A, lda, B,
+                       ldb, beta, C, ldc, tx, ty);
   } else {
     dim3 grid((n + GB_N - 1) / GB_N, (m + GB_M - 1) / GB_M);
-    hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, st, m, n, k, mk(ar, ai), (const cd *)A, lda, (const cd *)B, ldb,
-                       mk(br, bi), (cd *)C, ldc);
+    hipLaunchKernelGGL(k_gemm64<R>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
   }
   bem_prof_end(kid, st);
 }
@@ -752,7 +815,7 @@ static size_t g_lu_ws_bytes = 0;
 
 // workspace of the panel factorisation for one matrix order
 struct LuWs {
-  cd *P;
+  void *P;   // column-major panel copy (complex values of the precision in use)
   size_t ldp;
   int *used, *pv, *partI, *info, *rowAt, *pos;
   double *partV;
@@ -766,95 +829,103 @@ static inline int half64(int n) { return std::max(64, (n / 2) / 64 * 64); }
 
 // columns [k0, k0+kb) (kb <= 64) of rows [k0, n): pivots into piv[k0..k0+kb)
 // (absolute rows), rows interchanged inside these columns only
-static void lu_panel(cd *A, size_t lda, int n, int k0, int kb, int *piv, const LuWs &w, hipStream_t st)
+template <class R>
+static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, int *piv, const LuWs &w, hipStream_t st)
 {
+  typedef typename Cx<R>::Z Z;
   const int m = n - k0, gb = (m + LU_NT - 1) / LU_NT;
   // about 8 m kb^2 / 2 real flops of rank-1 updates
-  bem_prof_begin(BH_K_LU_PANEL, 4.0 * (double)m * kb * kb, st);
+  bem_prof_begin(Cx<R>::KP, 4.0 * (double)m * kb * kb, st);
   const int nbc = (m + 63) / 64;
   if (w.coop_ok && nbc <= w.coop.nbmax) {
     CoopWs cw = w.coop;
     (void)hipMemsetAsync(cw.cnt, 0, sizeof(unsigned int), st);   // arrival count of this launch
     void *args[] = {&A, &lda, (void *)&k0, (void *)&kb, (void *)&m, &piv, (void *)&w.info, &cw};
-    if (hipLaunchCooperativeKernel((const void *)k_panel_coop, dim3(nbc), dim3(64), args, 0, st) == hipSuccess) {
-      bem_prof_end(BH_K_LU_PANEL, st);
+    if (hipLaunchCooperativeKernel((const void *)k_panel_coop<R>, dim3(nbc), dim3(64), args, 0, st) == hipSuccess) {
+      bem_prof_end(Cx<R>::KP, st);
       return;
     }
     (void)hipGetLastError();   // not co-resident: the launch-per-column path below
   }
-  hipLaunchKernelGGL(k_panel_load, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, w.P, w.ldp, w.used);
+  Z *P = (Z *)w.P;
+  hipLaunchKernelGGL(k_panel_load<R>, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, P, w.ldp, w.used);
   for (int j = 0; j <= kb; j++)
-    hipLaunchKernelGGL(k_panel_col, dim3(gb), dim3(LU_NT), 0, st, w.P, w.ldp, m, kb, j, w.used, w.pv, w.partV, w.partI,
+    hipLaunchKernelGGL(k_panel_col<R>, dim3(gb), dim3(LU_NT), 0, st, P, w.ldp, m, kb, j, w.used, w.pv, w.partV, w.partI,
                        w.nbmax, w.info, k0);
   const int in_lds = 2 * (size_t)m * sizeof(int) <= (size_t)PERM_LDS_MAX;
   hipLaunchKernelGGL(k_panel_perm, dim3(1), dim3(256), in_lds ? 2 * (size_t)m * sizeof(int) : 0, st, w.pv, kb, k0, m, piv,
                      w.rowAt, w.pos, in_lds);
-  hipLaunchKernelGGL(k_panel_store, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, w.P, w.ldp, w.rowAt);
-  bem_prof_end(BH_K_LU_PANEL, st);
+  hipLaunchKernelGGL(k_panel_store<R>, dim3((m + 31) / 32), dim3(256), 0, st, A, lda, k0, kb, m, P, w.ldp, w.rowAt);
+  bem_prof_end(Cx<R>::KP, st);
 }
 
 // X := L^{-1} X with L the nr x nr unit-lower block at L (stride lda) and X
 // nr x ncol at B (stride ldb); recursive halving so that the off-diagonal
-// work is large-K ZGEMM, 64x64 diagonal blocks solved in LDS
-static void trsm_lunit_rec(const cd *L, size_t lda, int nr, cd *B, size_t ldb, int ncol, hipStream_t st)
+// work is large-K GEMM, 64x64 diagonal blocks solved in LDS
+template <class R>
+static void trsm_lunit_rec(const typename Cx<R>::Z *L, size_t lda, int nr, typename Cx<R>::Z *B, size_t ldb, int ncol,
+                           hipStream_t st)
 {
+  typedef typename Cx<R>::Z Z;
   if (nr <= 64) {
-    size_t shm = (size_t)nr * TRS_LD * sizeof(cd);
-    bem_prof_begin(BH_K_TRSM, 4.0 * nr * nr * (double)ncol, st);
-    hipLaunchKernelGGL(k_trsm_lunit, dim3((ncol + 63) / 64), dim3(256), shm, st, L, lda, nr, B, ldb, ncol);
-    bem_prof_end(BH_K_TRSM, st);
+    size_t shm = (size_t)nr * TRS_LD * sizeof(Z);
+    bem_prof_begin(Cx<R>::KT, 4.0 * nr * nr * (double)ncol, st);
+    hipLaunchKernelGGL(k_trsm_lunit<R>, dim3((ncol + 63) / 64), dim3(256), shm, st, L, lda, nr, B, ldb, ncol);
+    bem_prof_end(Cx<R>::KT, st);
     return;
   }
   const int h = half64(nr);
-  trsm_lunit_rec(L, lda, h, B, ldb, ncol, st);
-  bem_zgemm(nr - h, ncol, h, -1.0, 0.0, L + (size_t)h * lda, lda, B, ldb, 1.0, 0.0, B + (size_t)h * ldb, ldb, st);
-  trsm_lunit_rec(L + (size_t)h * lda + h, lda, nr - h, B + (size_t)h * ldb, ldb, ncol, st);
+  trsm_lunit_rec<R>(L, lda, h, B, ldb, ncol, st);
+  gemm_t<R>(nr - h, ncol, h, zmk<Z>(-1, 0), L + (size_t)h * lda, lda, B, ldb, zmk<Z>(1, 0), B + (size_t)h * ldb, ldb, st);
+  trsm_lunit_rec<R>(L + (size_t)h * lda + h, lda, nr - h, B + (size_t)h * ldb, ldb, ncol, st);
 }
 
 // X := U^{-1} X with U the nr x nr upper block (non-unit diagonal)
-static void trsm_upper_rec(const cd *U, size_t lda, int nr, cd *B, size_t ldb, int ncol, hipStream_t st)
+template <class R>
+static void trsm_upper_rec(const typename Cx<R>::Z *U, size_t lda, int nr, typename Cx<R>::Z *B, size_t ldb, int ncol,
+                           hipStream_t st)
 {
+  typedef typename Cx<R>::Z Z;
   if (nr <= 64) {
-    size_t shm = (size_t)nr * TRS_LD * sizeof(cd);
-    bem_prof_begin(BH_K_TRSM, 4.0 * nr * nr * (double)ncol, st);
-    hipLaunchKernelGGL(k_trsm_upper, dim3((ncol + 63) / 64), dim3(256), shm, st, U, lda, nr, B, ldb, ncol);
-    bem_prof_end(BH_K_TRSM, st);
+    size_t shm = (size_t)nr * TRS_LD * sizeof(Z);
+    bem_prof_begin(Cx<R>::KT, 4.0 * nr * nr * (double)ncol, st);
+    hipLaunchKernelGGL(k_trsm_upper<R>, dim3((ncol + 63) / 64), dim3(256), shm, st, U, lda, nr, B, ldb, ncol);
+    bem_prof_end(Cx<R>::KT, st);
     return;
   }
   const int h = half64(nr);
-  trsm_upper_rec(U + (size_t)h * lda + h, lda, nr - h, B + (size_t)h * ldb, ldb, ncol, st);
-  bem_zgemm(h, ncol, nr - h, -1.0, 0.0, U + h, lda, B + (size_t)h * ldb, ldb, 1.0, 0.0, B, ldb, st);
-  trsm_upper_rec(U, lda, h, B, ldb, ncol, st);
+  trsm_upper_rec<R>(U + (size_t)h * lda + h, lda, nr - h, B + (size_t)h * ldb, ldb, ncol, st);
+  gemm_t<R>(h, ncol, nr - h, zmk<Z>(-1, 0), U + h, lda, B + (size_t)h * ldb, ldb, zmk<Z>(1, 0), B, ldb, st);
+  trsm_upper_rec<R>(U, lda, h, B, ldb, ncol, st);
 }
 
 // recursive LU (left half, then the right half after its update) of columns
 // [k0, k0+nc) over rows [k0, n); every interchange found below reaches every
 // column of the block through the two laswp calls
-static void lu_rec(cd *A, size_t lda, int n, int k0, int nc, int *piv, const LuWs &w, hipStream_t st)
+template <class R>
+static void lu_rec(typename Cx<R>::Z *A, size_t lda, int n, int k0, int nc, int *piv, const LuWs &w, hipStream_t st)
 {
+  typedef typename Cx<R>::Z Z;
   if (nc <= 64) {
-    lu_panel(A, lda, n, k0, nc, piv, w, st);
+    lu_panel<R>(A, lda, n, k0, nc, piv, w, st);
     return;
   }
   const int h = half64(nc);
-  lu_rec(A, lda, n, k0, h, piv, w, st);
-  hipLaunchKernelGGL(k_laswp, dim3((nc - h + 255) / 256), dim3(256), 0, st, A, lda, k0 + h, k0 + nc, k0, k0 + h, piv);
-  trsm_lunit_rec(A + (size_t)k0 * lda + k0, lda, h, A + (size_t)k0 * lda + k0 + h, lda, nc - h, st);
-  bem_zgemm(n - k0 - h, nc - h, h, -1.0, 0.0, A + (size_t)(k0 + h) * lda + k0, lda, A + (size_t)k0 * lda + k0 + h, lda,
-            1.0, 0.0, A + (size_t)(k0 + h) * lda + k0 + h, lda, st);
-  lu_rec(A, lda, n, k0 + h, nc - h, piv, w, st);
-  hipLaunchKernelGGL(k_laswp, dim3((h + 255) / 256), dim3(256), 0, st, A, lda, k0, k0 + h, k0 + h, k0 + nc, piv);
+  lu_rec<R>(A, lda, n, k0, h, piv, w, st);
+  hipLaunchKernelGGL(k_laswp<Z>, dim3((nc - h + 255) / 256), dim3(256), 0, st, A, lda, k0 + h, k0 + nc, k0, k0 + h, piv);
+  trsm_lunit_rec<R>(A + (size_t)k0 * lda + k0, lda, h, A + (size_t)k0 * lda + k0 + h, lda, nc - h, st);
+  gemm_t<R>(n - k0 - h, nc - h, h, zmk<Z>(-1, 0), A + (size_t)(k0 + h) * lda + k0, lda, A + (size_t)k0 * lda + k0 + h, lda,
+            zmk<Z>(1, 0), A + (size_t)(k0 + h) * lda + k0 + h, lda, st);
+  lu_rec<R>(A, lda, n, k0 + h, nc - h, piv, w, st);
+  hipLaunchKernelGGL(k_laswp<Z>, dim3((h + 255) / 256), dim3(256), 0, st, A, lda, k0, k0 + h, k0 + h, k0 + nc, piv);
 }
 
 // LU factorisation in place, LAPACK's pivot choice and interchange vector.
 // piv (device, n ints) receives 0-based pivot rows; info (device int)
-// receives 0 or the 1-based index of the first zero pivot.  Returns 0, or -1
-// if the scratch cannot be allocated.
-int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int nb, hipStream_t st)
+// receives 0 or the 1-based index of the first zero pivot.  Returns 0, -1 if
+// the scratch cannot be allocated, -2 if a panel barrier timed out.
+template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, int *piv, int *info, hipStream_t st)
 {
-  (void)work;
-  (void)nb;
-  cd *A = (cd *)Av;
   const int kbmax = 64;
   size_t ldp = ((size_t)n + 63) / 64 * 64;
   int nblk = (n + LU_NT - 1) / LU_NT;
@@ -876,7 +947,7 @@ int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int
   // launch writes the maxima the next one reads
   char *p = (char *)g_lu_ws;
   LuWs w;
-  w.P = (cd *)p;                         p += (size_t)kbmax * ldp * sizeof(cd);
+  w.P = (void *)p;                       p += (size_t)kbmax * ldp * sizeof(cd);
   w.ldp = ldp;
   w.used = (int *)p;                     p += ldp * sizeof(int);
   w.rowAt = (int *)p;                    p += ldp * sizeof(int);
@@ -902,14 +973,14 @@ int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int
       w.coop.candV = (double *)q;            q += 2 * COOP_NBMAX * sizeof(double);
       w.coop.candI = (int *)q;               q += 2 * COOP_NBMAX * sizeof(int);
       w.coop.ip = (int *)q;                  q += 64 * sizeof(int);
-      w.coop.candRow = (cd *)q;
+      w.coop.candRow = (void *)q;
       w.coop.nbmax = COOP_NBMAX;
       w.coop_ok = true;
     }
   }
   (void)hipMemsetAsync(info, 0, sizeof(int), st);
   if (w.coop_ok) (void)hipMemsetAsync(w.coop.cnt, 0, 2 * sizeof(int), st);   // barrier count, error flag
-  if (n > 0) lu_rec(A, lda, n, 0, n, piv, w, st);
+  if (n > 0) lu_rec<R>(A, lda, n, 0, n, piv, w, st);
   if (w.coop_ok) {
     int herr = 0;
     if (hipMemcpyAsync(&herr, w.coop.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -919,15 +990,67 @@ int bem_zgetrf(int n, void *Av, size_t lda, int *piv, void *work, int *info, int
   return 0;
 }
 
-// solve A X = B with the factors of bem_zgetrf; B is n x nrhs (row stride ldb)
-void bem_zgetrs(int n, int nrhs, const void *Av, size_t lda, const int *piv, void *Bv, size_t ldb, int nb, hipStream_t st)
+// solve A X = B with the factors of getrf_t; B is n x nrhs (row stride ldb)
+template <class R>
+static void getrs_t(int n, int nrhs, const typename Cx<R>::Z *A, size_t lda, const int *piv, typename Cx<R>::Z *B,
+                    size_t ldb, hipStream_t st)
+{
+  typedef typename Cx<R>::Z Z;
+  if (n <= 0 || nrhs <= 0) return;
+  hipLaunchKernelGGL(k_laswp<Z>, dim3((nrhs + 255) / 256), dim3(256), 0, st, B, ldb, 0, nrhs, 0, n, piv);
+  trsm_lunit_rec<R>(A, lda, n, B, ldb, nrhs, st);
+  trsm_upper_rec<R>(A, lda, n, B, ldb, nrhs, st);
+}
+
+extern "C" {
+
+int bem_la_init(void)
+{
+  if (la_init_t<double>() || la_init_t<float>()) return -1;
+  if (hipFuncSetAttribute((const void *)k_panel_perm, hipFuncAttributeMaxDynamicSharedMemorySize, PERM_LDS_MAX) != hipSuccess)
+    return -1;
+  g_big_tiles = true;
+  int dev = 0, coop = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess)
+    g_coop = coop != 0;
+  return 0;
+}
+
+// complex FP64 (interleaved double pairs)
+void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t lda, const void *B, size_t ldb,
+               double br, double bi, void *C, size_t ldc, hipStream_t st)
+{
+  gemm_t<double>(m, n, k, mk(ar, ai), (const cd *)A, lda, (const cd *)B, ldb, mk(br, bi), (cd *)C, ldc, st);
+}
+
+int bem_zgetrf(int n, void *A, size_t lda, int *piv, void *work, int *info, int nb, hipStream_t st)
+{
+  (void)work;
+  (void)nb;
+  return getrf_t<double>(n, (cd *)A, lda, piv, info, st);
+}
+
+void bem_zgetrs(int n, int nrhs, const void *A, size_t lda, const int *piv, void *B, size_t ldb, int nb, hipStream_t st)
 {
   (void)nb;
-  const cd *A = (const cd *)Av;
-  cd *B = (cd *)Bv;
-  if (n <= 0 || nrhs <= 0) return;
-  hipLaunchKernelGGL(k_laswp, dim3((nrhs + 255) / 256), dim3(256), 0, st, B, ldb, 0, nrhs, 0, n, piv);
-  trsm_lunit_rec(A, lda, n, B, ldb, nrhs, st);
-  trsm_upper_rec(A, lda, n, B, ldb, nrhs, st);
+  getrs_t<double>(n, nrhs, (const cd *)A, lda, piv, (cd *)B, ldb, st);
+}
+
+// complex FP32 (interleaved float pairs)
+void bem_cgemm(int m, int n, int k, double ar, double ai, const void *A, size_t lda, const void *B, size_t ldb,
+               double br, double bi, void *C, size_t ldc, hipStream_t st)
+{
+  gemm_t<float>(m, n, k, mkf((float)ar, (float)ai), (const cf *)A, lda, (const cf *)B, ldb, mkf((float)br, (float)bi),
+                (cf *)C, ldc, st);
+}
+
+int bem_cgetrf(int n, void *A, size_t lda, int *piv, int *info, hipStream_t st)
+{
+  return getrf_t<float>(n, (cf *)A, lda, piv, info, st);
+}
+
+void bem_cgetrs(int n, int nrhs, const void *A, size_t lda, const int *piv, void *B, size_t ldb, hipStream_t st)
+{
+  getrs_t<float>(n, nrhs, (const cf *)A, lda, piv, (cf *)B, ldb, st);
 }
 }

@@ -1,0 +1,47 @@
+// bem_schur.h -- the one-medium block elimination of the boundary-integral
+// system in either precision, and the FP64 residual of the full system
+// (bem_schur.hip).
+#pragma once
+#include "bem_dev.h"
+
+// Factors of the block elimination of one object with one interior medium:
+// domain-0 rows H0 U_c - G0 Q_c = r0_c, domain-1 rows (sub side of every
+// element) H1 U_c + delta G1 N_c U_3 + G1 Q_c = r1_c (c < 3) and
+// sigma G1 sum_c N_c U_c + H1 U_3 + beta G1 Q_3 = r1_3 (N_c: the normal's
+// component c at each column).  With Z = G0^{-1} H0, M = H1 + G1 Z,
+// W = M^{-1} G1 and S = H1 + beta G1 Z - sigma delta G1 (W o n n^T).
+template <class Zt> struct SchurF {
+  int n;                                   // NN (nodes = columns of each domain)
+  size_t ld;                               // row stride of the matrices
+  const Zt *G0f, *Zm, *Mf, *Sf, *G1;       // LU(G0), Z, LU(M), LU(S), G1
+  const int *piv0, *pivM, *pivS;
+  const ColInfo *col1;                     // domain-1 columns (unit normals)
+  Zt beps, sigm, delt;
+};
+
+// X = A^{-1} rhs through the factors.  rhs and X use the 8NN layout of the
+// system (rows: domain 0 components 0..3, then domain 1; unknowns U_0..U_3,
+// Q_0..Q_3, NN each).  r0nz = false asserts that the domain-0 rows of rhs
+// are zero (the case of every incident-field right-hand side).  work holds
+// 27 NN values.
+void schur_apply(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, hipStream_t st);
+void schur_apply(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, hipStream_t st);
+
+// complex-FP32 factors from the FP64 matrices of both domains (Wt: NN^2
+// scratch).  Returns 0, or the code of the LU (-1 scratch, -2 barrier).
+int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, const cd *H1, const ColInfo *col1,
+                   cd beps, cd sigm, cd delt, cf *G0f, cf *Zm, cf *Mf, cf *Sf, cf *G1c, cf *Wt, int *piv0, int *pivM,
+                   int *pivS, int *info, hipStream_t st);
+
+// R = B - A X over the rows of one domain (rows rowBase + c nd + r), from
+// the domain's FP64 G, H and its column classes; work holds 12 nd values
+void bem_resid_domain(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
+                      size_t rowBase, const cd *B, cd *R, cd *work, hipStream_t st);
+
+// Euclidean norm of n complex values (synchronises the stream); tmp holds
+// 256 doubles of device memory
+double bem_nrm2(const cd *x, size_t n, double *tmp, hipStream_t st);
+
+// elementwise conversions: dst = src rounded to FP32; X += d widened to FP64
+void bem_cvt_z2c(const cd *src, cf *dst, size_t n, hipStream_t st);
+void bem_acc_c2z(cd *X, const cf *d, size_t n, hipStream_t st);

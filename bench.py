@@ -48,14 +48,30 @@ def points(n, seed=20240601):
     return rng.uniform(-2.0, 2.0, size=(n, 3))
 
 
-def gpu_step(bem, obj, pts):
+def gpu_step(bem, obj, pts, form="fp64"):
+    """one configuration end to end; form "fp64" is the plain FP64 form,
+    "mixed" the mixed-precision form (FP32 far-element quadrature and
+    boundary field sums, complex-FP32 factorisation refined in FP64)"""
+    obj.set_precision("mixed" if form == "mixed" else "fp64")
     obj.assemble(2)
     obj.factor(0)
     bem.mo_solve([obj])
-    bem.mo_field([obj], pts, "EH", "t", 1)
-    for typ in (0, 1):
-        bem.mo_force([obj], 0, typ, (0.0, 0.0, 0.0))
+    fld, _ = bem.mo_field([obj], pts, "EH", "t", 1)
+    FN = [bem.mo_force([obj], 0, typ, (0.0, 0.0, 0.0))[1:] for typ in (0, 1)]
     bem.sync()
+    return fld, FN
+
+
+def kernel_table(bem, steps):
+    kern = {}
+    for name in bem.KERNELS:
+        ms, work, n = bem.prof_read(name)
+        kern[name] = (ms, work, n)
+    tot = sum(v[0] for v in kern.values())
+    for name, (ms, work, n) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
+        if n:
+            log("  %-11s %9.1f ms total  %6d launches  %5.1f %% of tracked" % (name, ms, n, 100.0 * ms / max(tot, 1e-9)))
+    return kern
 
 
 def cpu_baseline(arrays, beams, npts, target_s):
@@ -135,6 +151,8 @@ def main():
     ap.add_argument("--points", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=2,
+                    help="timed steps of the mixed-precision form after the FP64 ones (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -177,7 +195,7 @@ def main():
     bem.sync()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        gpu_step(bem, obj, pts)
+        ref = gpu_step(bem, obj, pts)
         log("step %d done (%.3f s)" % (s, time.perf_counter() - t0))
     bem.sync()
     if torch is not None:
@@ -186,36 +204,39 @@ def main():
     ms_step = 1000.0 * dt / args.steps
 
     # kernel table from the HIP-event registry
-    kern = {}
-    for name in bem.KERNELS:
-        ms, work, n = bem.prof_read(name)
-        kern[name] = (ms, work, n)
+    kern = kernel_table(bem, args.steps)
     bem.prof_enable(False)
-    tot = sum(v[0] for v in kern.values())
-    for name, (ms, work, n) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
-        if n:
-            log("  %-9s %9.1f ms total  %6d launches  %5.1f %% of tracked" % (name, ms, n, 100.0 * ms / max(tot, 1e-9)))
     # dominant kernel: the tracked kernel with the largest summed time
     dom = max(kern, key=lambda k: kern[k][0])
-    ms, work, n = kern[dom]
-    avg_s = ms / 1000.0 / max(n, 1)
-    roof = None
-    if dom == "zgemm":
-        ach = work / n / avg_s / 1e12
-        roof = {"kernel": "k_zgemm", "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "avg_launch_ms": 1000.0 * avg_s,
-                "work_per_launch": work / n, "work_unit": "real FP64 flops (8 m n k)"}
-    elif dom in ("asm_gl", "field", "scatter"):
-        # far-pair quadrature: FP64 VALU; flops per (row node, element) pair
-        # = 432 points x FLOP_PER_POINT (ISA count of the kernel's point loop)
-        fpp = 432 * FLOP_PER_POINT[dom]
-        ach = work / n * fpp / avg_s / 1e12
-        roof = {"kernel": "k_" + dom, "bound": "valu-fp64", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
-                "avg_launch_ms": 1000.0 * avg_s, "work_per_launch": work / n, "work_unit": "pairs"}
-    else:
-        roof = {"kernel": dom, "bound": "mfma", "achieved": None, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": None, "traffic": None, "avg_launch_ms": 1000.0 * avg_s}
+    roof = roofline(dom, kern[dom])
+
+    # the mixed-precision form on the same workload, its error against the
+    # FP64 form's fields and force, and its own dominant kernel
+    mixed = None
+    if args.mixed_steps > 0:
+        gpu_step(bem, obj, pts, "mixed")
+        bem.prof_enable(True)
+        bem.sync()
+        t0 = time.perf_counter()
+        for s in range(args.mixed_steps):
+            got = gpu_step(bem, obj, pts, "mixed")
+            log("mixed step %d done (%.3f s)" % (s, time.perf_counter() - t0))
+        bem.sync()
+        dtm = (time.perf_counter() - t0) / args.mixed_steps
+        kmix = kernel_table(bem, args.mixed_steps)
+        bem.prof_enable(False)
+        it, res = obj.solve_stats()
+        fe = float(np.abs(got[0] - ref[0]).max() / np.abs(ref[0]).max())
+        Fe = max(float(np.abs(got[1][t][0] - ref[1][t][0]).max() / np.abs(ref[1][t][0]).max()) for t in (0, 1))
+        Ne_ = max(float(np.abs(got[1][t][1] - ref[1][t][1]).max() / max(np.abs(ref[1][t][1]).max(), np.abs(ref[1][t][0]).max()))
+                  for t in (0, 1))
+        dmx = max(kmix, key=lambda k: kmix[k][0])
+        mixed = {"value": dtm, "unit": "s/config", "speedup_vs_fp64": (ms_step / 1000.0) / dtm,
+                 "refinement_steps": it, "relative_residual": res,
+                 "rel_err_vs_fp64": {"fields": fe, "force": Fe, "torque": Ne_},
+                 "kernels_ms_per_step": {k: v[0] / args.mixed_steps for k, v in kmix.items() if v[2]},
+                 "roofline": roofline(dmx, kmix[dmx])}
+        log("mixed form: %.3f s/config (x%.2f), %d refinement steps, field error %.2e" % (dtm, mixed["speedup_vs_fp64"], it, fe))
 
     cpu = None
     if not args.no_cpu:
@@ -244,6 +265,7 @@ def main():
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items() if v[2]},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed_form": mixed,
     }
     print(json.dumps(out), flush=True)
 
@@ -251,7 +273,31 @@ def main():
 # FP64 operations per quadrature point of the far-pair kernels (counted on
 # the gfx950 ISA of the point loop: v_fma/mul/add_f64 count 2/1/1, the
 # sqrt/rcp/sin/cos sequences at their instruction counts)
-FLOP_PER_POINT = {"asm_gl": 160.0, "field": 80.0, "scatter": 160.0}
+FLOP_PER_POINT = {"asm_gl": 325.0, "field": 80.0, "scatter": 160.0}
+PEAK_FP32_TFLOPS = 157.3    # FP32 vector and FP32 matrix (dense)
+
+
+def roofline(dom, rec):
+    """roofline entry of one tracked kernel from its (ms, work, launches)"""
+    ms, work, n = rec
+    avg_s = ms / 1000.0 / max(n, 1)
+    if dom in ("zgemm", "cgemm"):
+        peak = PEAK_FP64_TFLOPS if dom == "zgemm" else PEAK_FP32_TFLOPS
+        ach = work / n / avg_s / 1e12
+        return {"kernel": "k_gemm128<%s>" % ("double" if dom == "zgemm" else "float"), "bound": "mfma",
+                "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                "avg_launch_ms": 1000.0 * avg_s, "work_per_launch": work / n,
+                "work_unit": "real %s flops (8 m n k)" % ("FP64" if dom == "zgemm" else "FP32")}
+    if dom in FLOP_PER_POINT:
+        # far-pair quadrature: FP64 VALU; flops per (row node, element) pair
+        # = 432 points x FLOP_PER_POINT (ISA count of the kernel's point loop)
+        fpp = 432 * FLOP_PER_POINT[dom]
+        ach = work / n * fpp / avg_s / 1e12
+        return {"kernel": "k_" + dom, "bound": "valu-fp64", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+                "avg_launch_ms": 1000.0 * avg_s, "work_per_launch": work / n, "work_unit": "pairs"}
+    return {"kernel": dom, "bound": "mfma", "achieved": None, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": None, "traffic": None, "avg_launch_ms": 1000.0 * avg_s}
 
 if __name__ == "__main__":
     main()

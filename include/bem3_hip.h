@@ -27,6 +27,7 @@ extern "C" {
 #define BH_ESTATE -5  /* call order (e.g. solve before build) */
 #define BH_ESING  -6  /* singular matrix (zero pivot)         */
 #define BH_EDOM   -7  /* point inside another object (overlap)*/
+#define BH_ECONV  -8  /* iterative refinement did not converge */
 
 typedef struct bh_object bh_object;
 
@@ -56,6 +57,15 @@ int bh_object_geometry(const bh_object *o, double *cr, double *cw, double *ren, 
  * solve_bieq2.c:98-184, with dcoef_bd_node_t2, d3b1_elem.c:482-539).
  * type must be 2 (GLN rule with the GHN near-pair switch, PREC_DEF).     */
 int bh_object_assemble(bh_object *o, int type);
+/* Arithmetic of the object's later calls (no counterpart in the reference,
+ * which is FP64 throughout): BH_PREC_FP64 (default) or BH_PREC_MIXED --
+ * far-element quadrature of the assembly and the boundary field sums in
+ * FP32 (near-singular and self-element integrals, the force accumulation
+ * and every residual stay FP64), factor mode 0 picks the FP32 block
+ * elimination refined in FP64 (mode 3).                                   */
+#define BH_PREC_FP64  0
+#define BH_PREC_MIXED 1
+int bh_object_set_precision(bh_object *o, int prec);
 int bh_object_gh_count(const bh_object *o, int did);
 /* download: which 0 G, 1 H (nd x nd), 2 dG, 3 dH (nd x 2 x nd complex),
  * 4 dF (nd x 3 doubles: diagonal 1+F or F, dF/dt_zeta, dF/dt_eta).
@@ -71,8 +81,14 @@ int bh_object_set_beams(bh_object *o, int nb, const double *par);
 /* Factorise the boundary-integral system (create_Amatrix_csr + lu_dec_A,
  * solve_bieq2.c:186-408).  mode 1: the reference's dense 8NN matrix;
  * mode 2: exact block elimination for one interior medium (three NN-sized
- * LU factorisations); mode 0 picks 2 when possible.                      */
+ * LU factorisations); mode 0 picks 2 when possible; mode 3: the block
+ * elimination in complex FP32 (mixed-precision form), every later solve
+ * refined in FP64 against the FP64 matrices to a relative residual of
+ * 1e-10 (BH_ECONV if 30 steps do not reach it).                          */
 int bh_object_factor(bh_object *o, int mode);
+/* refinement steps and final relative residual ||b - A x|| / ||b|| of the
+ * last bh_object_solve (0, 0 outside the mixed-precision form)            */
+int bh_object_solve_stats(const bh_object *o, int *iters, double *resid);
 /* boundary potentials from the current incident data (create_Bmatrix +
  * solve_ABmatrix + tmatrix_bd_store, mo_solve.c:137-274, 457-545)         */
 int bh_object_solve(bh_object *o);
@@ -117,6 +133,11 @@ int bh_zgesv(int n, int nrhs, double *A, double *B);
 /* C = alpha A B + beta C (row-major, complex): the MFMA ZGEMM kernel     */
 int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B, const double *beta,
                   double *C);
+/* the same in complex FP32 (interleaved float pairs): the factorisation
+ * and product kernels of the mixed-precision form (no counterpart in the
+ * reference, whose arithmetic is FP64 throughout)                        */
+int bh_cgesv(int n, int nrhs, float *A, float *B);
+int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const float *B, const float *beta, float *C);
 
 /* Kernel timing (no counterpart in the reference): while enabled, HIP events
  * on the library's stream bracket every launch of the tracked kernels.
@@ -126,7 +147,10 @@ int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, con
  * BH_K_ZGEMM (the 128x128-tile kernel) and BH_K_ZGEMM_SMALL (64x64-tile and
  * few-right-hand-side kernels): real flops (8 m n k); BH_K_LU_PANEL: real
  * flops of the unblocked panel updates; BH_K_TRSM: real flops; BH_K_FIELD,
- * BH_K_SCATTER, BH_K_FIELD_BD: (point, source element) pairs.              */
+ * BH_K_SCATTER, BH_K_FIELD_BD: (point, source element) pairs.
+ * BH_K_CGEMM, BH_K_CGEMM_SMALL, BH_K_LU_PANEL_C, BH_K_TRSM_C,
+ * BH_K_ASM_GL32, BH_K_FIELD_BD32: the FP32 kernels of the mixed-precision
+ * form (same work units).                                                 */
 #define BH_K_ASM_GL   0
 #define BH_K_ASM_GH   1
 #define BH_K_ASM_SELF 2
@@ -137,7 +161,13 @@ int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, con
 #define BH_K_SCATTER  7
 #define BH_K_ZGEMM_SMALL 8
 #define BH_K_FIELD_BD 9
-#define BH_K_COUNT    10
+#define BH_K_CGEMM    10
+#define BH_K_CGEMM_SMALL 11
+#define BH_K_LU_PANEL_C 12
+#define BH_K_TRSM_C   13
+#define BH_K_ASM_GL32 14
+#define BH_K_FIELD_BD32 15
+#define BH_K_COUNT    16
 int bh_prof_enable(int on);
 int bh_prof_read(int kid, double *ms, double *work, int *launches);
 
