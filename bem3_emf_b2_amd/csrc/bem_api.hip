@@ -27,8 +27,9 @@ void bem_launch_asm_gl(const void *, const int *, const void *, const int *, int
                        void *, void *, void *, void *, size_t, double *, unsigned long long *, int, hipStream_t);
 void bem_launch_asm_gh(const void *, const int *, const void *, const int *, const void *, int, double, double,
                        const void *, int, const void *, int, void *, void *, void *, void *, size_t, double *, hipStream_t);
-void bem_launch_asm_gl32(const void *, const int *, const void *, const int *, int, int, double, double, const void *, int,
-                         void *, void *, void *, void *, size_t, double *, unsigned long long *, int, hipStream_t);
+void bem_launch_asm_gl32(const void *, const int *, const void *, const int *, int, int, double, double, const void *,
+                         const void *, int, void *, void *, void *, void *, size_t, double *, unsigned long long *, int,
+                         hipStream_t);
 int bem_rule32(const void *, int, void **);
 void bem_launch_asm_self(const void *, const int *, const void *, const int *, int, double, double, const double *,
                          const double *, const double *, const double *, void *, void *, void *, void *, size_t,
@@ -207,6 +208,7 @@ struct bh_object {
   std::vector<zd> n, kn;
   std::vector<Dom> dom;
   TriGeo *d_geo = nullptr;
+  QuadExt *d_gq = nullptr;             // bi-linear terms of quadrangles (zero rows for triangles)
   bool all_tri = true;
   // incident data at every element slot [Ne+1][4][4]
   cd *d_Ui = nullptr, *d_dUi = nullptr, *d_Ui0 = nullptr, *d_dUi0 = nullptr, *d_Uip = nullptr;
@@ -216,7 +218,7 @@ struct bh_object {
   int *d_slotIdx = nullptr;
   // faces grouped by main domain for the domain test
   double *d_fv = nullptr;
-  int *d_fsd = nullptr, *d_seg = nullptr;
+  int *d_fsd = nullptr, *d_fq = nullptr, *d_seg = nullptr;
   // beams
   std::vector<Beam> beams;
   Beam *d_beams = nullptr;
@@ -241,7 +243,14 @@ struct bh_object {
   double ir_res = 0.0;                 // its final relative residual
 };
 
-// element constants and node data (reference setup2.c:666-749, 838-869)
+// element constants and node data (reference setup2.c:666-749, 838-869):
+// triangles r = c0 + c1 z + c2 e on the equilateral reference triangle,
+// node slots at the 4-point rule's abscissae (centroid in slot 3); quads
+// r = c0 + c1 z + c2 e + c3 z e on [-1,1]^2 with W(z,e) = W0 + W1 z + W2 e,
+// node slots at the 2x2 Gauss points
+static const double Q44 = 0.5773502691896257645091;   // 1/sqrt(3)
+static const double ZT44[4] = {-Q44, Q44, Q44, -Q44}, ET44[4] = {-Q44, -Q44, Q44, Q44};
+
 static void elem_const(bh_object *o)
 {
   const double P340 = 0.2, P341 = 0.3464101615137754587055;
@@ -252,9 +261,32 @@ static void elem_const(bh_object *o)
   o->wen.assign((size_t)(o->Ne + 1) * 12, 0.0);
   for (int i = 1; i <= o->Ne; i++) {
     double p[4][3];
-    for (int j = 0; j < 4; j++)
+    const bool quad = o->ed[4 * i + 3] != 0;
+    for (int j = 0; j < (quad ? 4 : 3); j++)
       for (int d = 0; d < 3; d++) p[j][d] = o->rn[3 * (size_t)o->ed[4 * i + j] + d];
     double *cr = &o->cr[(size_t)i * 12], *cw = &o->cw[(size_t)i * 9];
+    if (quad) {
+      for (int d = 0; d < 3; d++) {
+        cr[d * 4 + 0] = 0.25 * (p[0][d] + p[1][d] + p[2][d] + p[3][d]);
+        cr[d * 4 + 1] = 0.25 * (p[1][d] + p[2][d] - p[0][d] - p[3][d]);
+        cr[d * 4 + 2] = 0.25 * (p[2][d] + p[3][d] - p[0][d] - p[1][d]);
+        cr[d * 4 + 3] = 0.25 * (p[0][d] + p[2][d] - p[1][d] - p[3][d]);
+      }
+      // dr/dz x dr/de = (c1 + c3 e) x (c2 + c3 z) = c1 x c2 + z (c1 x c3) + e (c3 x c2)
+      for (int d = 0; d < 3; d++) {
+        int a = (d + 1) % 3, b = (d + 2) % 3;
+        cw[d * 3 + 0] = cr[a * 4 + 1] * cr[b * 4 + 2] - cr[b * 4 + 1] * cr[a * 4 + 2];
+        cw[d * 3 + 1] = cr[a * 4 + 1] * cr[b * 4 + 3] - cr[b * 4 + 1] * cr[a * 4 + 3];
+        cw[d * 3 + 2] = cr[a * 4 + 3] * cr[b * 4 + 2] - cr[b * 4 + 3] * cr[a * 4 + 2];
+      }
+      for (int j = 0; j < 4; j++)
+        for (int d = 0; d < 3; d++) {
+          const double z = ZT44[j], e = ET44[j];
+          o->ren[(size_t)i * 12 + j * 3 + d] = cr[d * 4] + cr[d * 4 + 1] * z + cr[d * 4 + 2] * e + cr[d * 4 + 3] * z * e;
+          o->wen[(size_t)i * 12 + j * 3 + d] = cw[d * 3] + cw[d * 3 + 1] * z + cw[d * 3 + 2] * e;
+        }
+      continue;
+    }
     for (int d = 0; d < 3; d++) {
       cr[d * 4 + 0] = (p[0][d] + p[1][d] + p[2][d]) / 3.0;
       cr[d * 4 + 1] = (-p[0][d] + 2.0 * p[1][d] - p[2][d]) / 3.0;
@@ -272,6 +304,17 @@ static void elem_const(bh_object *o)
         o->ren[(size_t)i * 12 + j * 3 + d] = cr[d * 4] + cr[d * 4 + 1] * zt[j] + cr[d * 4 + 2] * et[j];
         o->wen[(size_t)i * 12 + j * 3 + d] = cw[d * 3];
       }
+  }
+}
+
+// unit tangents along z and e at node slot j of element a
+static void node_tangents(const bh_object *o, int a, int j, double *vtz, double *vte)
+{
+  const double *cr = &o->cr[(size_t)a * 12];
+  const bool quad = o->ed[4 * a + 3] != 0;
+  for (int d = 0; d < 3; d++) {
+    vtz[d] = cr[d * 4 + 1] + (quad ? cr[d * 4 + 3] * ET44[j] : 0.0);
+    vte[d] = cr[d * 4 + 2] + (quad ? cr[d * 4 + 3] * ZT44[j] : 0.0);
   }
 }
 
@@ -303,11 +346,12 @@ static int build_domains(bh_object *o)
     D.rows.clear();
     for (int t = 1; t <= D.dNe; t++) {
       int a = std::abs(D.sid[t]);
-      const double *cr = &o->cr[(size_t)a * 12];
-      double vtz[3] = {cr[1], cr[5], cr[9]}, vte[3] = {cr[2], cr[6], cr[10]};
-      unit3(vtz);
-      unit3(vte);
-      for (int tn = 0; tn < 3; tn++) {
+      const int ns = o->ed[4 * a + 3] == 0 ? 3 : 4;
+      for (int tn = 0; tn < ns; tn++) {
+        double vtz[3], vte[3];
+        node_tangents(o, a, tn, vtz, vte);
+        unit3(vtz);
+        unit3(vte);
         RowPt r;
         memset(&r, 0, sizeof r);
         for (int q = 0; q < 3; q++) { r.rt[q] = o->ren[(size_t)a * 12 + tn * 3 + q]; r.v0[q] = vtz[q]; r.v1[q] = vte[q]; }
@@ -333,8 +377,16 @@ static int upload_geo(bh_object *o)
       geo[i].c0[d] = cr[d * 4]; geo[i].c1[d] = cr[d * 4 + 1]; geo[i].c2[d] = cr[d * 4 + 2]; geo[i].W[d] = cw[d * 3];
     }
     geo[i].aW = sqrt(cw[0] * cw[0] + cw[3] * cw[3] + cw[6] * cw[6]);
+    geo[i].quad = o->ed[4 * i + 3] != 0 ? 1.0 : 0.0;
   }
-  return upload(&o->d_geo, geo.data(), geo.size());
+  std::vector<QuadExt> gq(o->Ne + 1);
+  memset(gq.data(), 0, gq.size() * sizeof(QuadExt));
+  for (int i = 1; i <= o->Ne; i++) {
+    if (o->ed[4 * i + 3] == 0) continue;
+    const double *cr = &o->cr[(size_t)i * 12], *cw = &o->cw[(size_t)i * 9];
+    for (int d = 0; d < 3; d++) { gq[i].c3[d] = cr[d * 4 + 3]; gq[i].W1[d] = cw[d * 3 + 1]; gq[i].W2[d] = cw[d * 3 + 2]; }
+  }
+  return upload(&o->d_geo, geo.data(), geo.size()) || upload(&o->d_gq, gq.data(), gq.size());
 }
 
 extern "C" bh_object *bh_object_create(int Nn, const double *rn, int Ne, const int *ed, const int *md, const int *sd,
@@ -357,7 +409,7 @@ extern "C" bh_object *bh_object_create(int Nn, const double *rn, int Ne, const i
       return nullptr;
     }
   }
-  if (!o->all_tri) { fail(BH_EUNSUP, "bi-linear (quad) elements are not supported by the GPU path"); delete o; return nullptr; }
+
   o->eni.assign(4 * (size_t)(Ne + 1), -1);
   int nc = 0;
   for (int i = 1; i <= Ne; i++) for (int j = 0; j < 3; j++) o->eni[4 * i + j] = nc++;
@@ -386,9 +438,9 @@ extern "C" void bh_object_destroy(bh_object *o)
     dfree(D.d_U); dfree(D.d_dU); dfree(D.d_E); dfree(D.d_dE); dfree(D.d_Hf); dfree(D.d_dHf);
     dfree(D.d_Gf); dfree(D.d_Gpiv); dfree(D.c_Gf);
   }
-  dfree(o->d_geo);
+  dfree(o->d_geo); dfree(o->d_gq);
   dfree(o->d_Ui); dfree(o->d_dUi); dfree(o->d_Ui0); dfree(o->d_dUi0); dfree(o->d_Uip);
-  dfree(o->d_slotPt); dfree(o->d_slotN); dfree(o->d_slotIdx); dfree(o->d_fv); dfree(o->d_fsd); dfree(o->d_seg);
+  dfree(o->d_slotPt); dfree(o->d_slotN); dfree(o->d_slotIdx); dfree(o->d_fv); dfree(o->d_fsd); dfree(o->d_fq); dfree(o->d_seg);
   dfree(o->d_beams);
   dfree(o->d_A); dfree(o->d_work); dfree(o->d_piv); dfree(o->d_info);
   dfree(o->d_G0f); dfree(o->d_Z); dfree(o->d_Mf); dfree(o->d_Sf);
@@ -438,7 +490,7 @@ static int assemble_domain(bh_object *o, int d)
   double kr = o->kn[d].real(), ki = o->kn[d].imag();
   const bool mixed = o->prec == BH_PREC_MIXED;
   if (mixed)
-    bem_launch_asm_gl32(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl32, g.n_gl, D.d_G, D.d_H,
+    bem_launch_asm_gl32(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl32, g.d_gl, g.n_gl, D.d_G, D.d_H,
                         D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
   else
     bem_launch_asm_gl(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, D.d_G, D.d_H,
@@ -483,6 +535,9 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
 {
   API_LOCK;
   if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
+  if (!o->all_tri)
+    return fail(BH_EUNSUP, "coefficient matrices of bi-linear (quad) elements are not implemented on the GPU "
+                           "(fields, force and absorbed power of a stored quad-mesh state are)");
   // every factorisation of earlier matrices is stale from here on
   o->fmode = 0;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
@@ -620,7 +675,8 @@ void bem_launch_unpack_eh(int, const void *, const int *, const void *, void *, 
 void bem_launch_hadamard_nn(void *, int, size_t, const void *, hipStream_t);
 void bem_launch_axpby(const void *, double, double, const void *, void *, size_t, hipStream_t);
 void bem_launch_scale_rows(void *, const void *, int, int, double, double, hipStream_t);
-void bem_launch_domain(const double *, int, const double *, const int *, const int *, int, int *, hipStream_t);
+void bem_launch_domain(const double *, int, const double *, const int *, const int *, const int *, int, int *,
+                       hipStream_t);
 void bem_launch_field(const double *, const int *, int, int, const void *, const int *, int, double, double, const void *,
                       int, double, int, const void *, const void *, const void *, const void *, void *, int, hipStream_t);
 void bem_launch_field_memp(const double *, const int *, int, int, const void *, const int *, int, double, double,
@@ -706,11 +762,12 @@ static int prepare_solve(bh_object *o)
       int sd = D.sid[s], a = std::abs(sd);
       int mdid = o->md[a], sdid = o->sd[a];
       zd k1 = o->kn[mdid], k2 = o->kn[sdid], delt = k1 * k1 - k2 * k2, beps = k1 * k1 / (k2 * k2), sigm = 1.0 - beps;
-      const double *cr = &o->cr[(size_t)a * 12];
-      double vtz[3] = {cr[1], cr[5], cr[9]}, vte[3] = {cr[2], cr[6], cr[10]};
-      unit3(vtz);
-      unit3(vte);
-      for (int l = 0; l < 3; l++) {
+      const int ns = o->ed[4 * a + 3] == 0 ? 3 : 4;
+      for (int l = 0; l < ns; l++) {
+        double vtz[3], vte[3];
+        node_tangents(o, a, l, vtz, vte);
+        unit3(vtz);
+        unit3(vte);
         int j = D.colOff[s] + l;
         ColInfo &c = col[j];
         memset(&c, 0, sizeof c);
@@ -753,21 +810,26 @@ static int prepare_solve(bh_object *o)
   }
   o->nslot = (int)idx.size();
   // faces by main domain for the solid-angle test
+  // (4 vertices per face; a triangle repeats its first vertex and is
+  // flagged by fq = 0)
   std::vector<double> fv;
-  std::vector<int> fsd, seg(o->MN + 2, 0);
+  std::vector<int> fsd, fq, seg(o->MN + 2, 0);
   for (int d = 0; d <= o->MN; d++) {
     seg[d] = (int)fsd.size();
     for (int i = 1; i <= o->Ne; i++) {
       if (o->md[i] != d) continue;
-      for (int j = 0; j < 3; j++)
-        for (int q = 0; q < 3; q++) fv.push_back(o->rn[3 * (size_t)o->ed[4 * i + j] + q]);
+      const bool quad = o->ed[4 * i + 3] != 0;
+      for (int j = 0; j < 4; j++)
+        for (int q = 0; q < 3; q++) fv.push_back(o->rn[3 * (size_t)o->ed[4 * i + (j < 3 || quad ? j : 0)] + q]);
       fsd.push_back(o->sd[i]);
+      fq.push_back(quad ? 1 : 0);
     }
   }
   seg[o->MN + 1] = (int)fsd.size();
   if (upload(&o->d_slotPt, pt.data(), pt.size()) || upload(&o->d_slotN, nm.data(), nm.size()) ||
       upload(&o->d_slotIdx, idx.data(), idx.size()) || upload(&o->d_fv, fv.data(), fv.size()) ||
-      upload(&o->d_fsd, fsd.data(), fsd.size()) || upload(&o->d_seg, seg.data(), seg.size()))
+      upload(&o->d_fsd, fsd.data(), fsd.size()) || upload(&o->d_fq, fq.data(), fq.size()) ||
+      upload(&o->d_seg, seg.data(), seg.size()))
     return BH_EHIP;
   HIPCHK(hipStreamSynchronize(g.st));
   return 0;
@@ -1137,7 +1199,7 @@ struct Scratch {
 static int domain_ids(bh_object *o, const double *d_pts, int np, int *d_did)
 {
   if (prepare_solve(o)) return BH_EHIP;
-  bem_launch_domain(d_pts, np, o->d_fv, o->d_fsd, o->d_seg, o->MN, d_did, g.st);
+  bem_launch_domain(d_pts, np, o->d_fv, o->d_fsd, o->d_fq, o->d_seg, o->MN, d_did, g.st);
   HIPCHK(hipGetLastError());
   return 0;
 }

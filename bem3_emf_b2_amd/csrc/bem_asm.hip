@@ -160,13 +160,17 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
 }
 
 // ---- pass 1, mixed-precision form ----
-// The same far-pair rule with the integrand evaluated in FP32 (positions
-// relative to the row point formed in FP64 first, so that no cancellation
-// enters the FP32 part), hardware reciprocal square root, FP32 sin/cos/exp,
-// sums over blocks of 12 points in FP32 added into FP64 accumulators.
-// Pairs whose FP32 |dF/dt| comes within a relative 1e-4 of the GHN switch
-// level are flagged too: pass 2 redoes them in FP64 and takes the
-// reference's decision there.
+// The same 12x12 rule; for far pairs (row point farther than 4 element
+// radii from the element's centre) the integrand is evaluated in FP32
+// (positions relative to the row point formed in FP64 first, so that no
+// cancellation enters the FP32 part), with the hardware reciprocal square
+// root, FP32 sin/cos/exp, and sums over blocks of 12 points in FP32 added
+// into FP64 accumulators.  Nearer pairs, whose large and rapidly varying
+// terms would carry FP32 rounding into the cancelling F sums, take the
+// FP64 path of k_asm_gl.  Pairs whose |dF/dt| comes within a relative 1e-4
+// of the GHN switch level are flagged too: pass 2 redoes them in FP64 and
+// takes the reference's decision there.
+#define BEM_FAR_RADII 4.0
 struct QPt32 {
   float z, e, w, wn0, wn1, wn2;   // point, weight, weight x shape functions
 };
@@ -180,7 +184,8 @@ __device__ __forceinline__ void add_cf(cd &a, cf b) { a.x += (double)b.x; a.y +=
 
 __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
                                                   const RowPt *__restrict__ rows, const int *__restrict__ colOff,
-                                                  int dNe, double kr, double ki, const QPt32 *__restrict__ rule, int nq,
+                                                  int dNe, double kr, double ki, const QPt32 *__restrict__ rule,
+                                                  const QPt *__restrict__ rule64, int nq,
                                                   cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
                                                   cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
                                                   unsigned long long *__restrict__ ghMask)
@@ -204,12 +209,22 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
     const float Wx = (float)g.W[0], Wy = (float)g.W[1], Wz = (float)g.W[2];
     const float v0x = (float)rp.v0[0], v0y = (float)rp.v0[1], v0z = (float)rp.v0[2];
     const float v1x = (float)rp.v1[0], v1y = (float)rp.v1[1], v1z = (float)rp.v1[2];
-    const float Wdv0 = (float)(g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2]);
-    const float Wdv1 = (float)(g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2]);
+    const double Wdv0d = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
+    const double Wdv1d = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
+    const float Wdv0 = (float)Wdv0d, Wdv1 = (float)Wdv1d;
     const float krf = (float)kr, kif = (float)ki;
+    const double dx = g.c0[0] - rp.rt[0], dy = g.c0[1] - rp.rt[1], dz = g.c0[2] - rp.rt[2];
+    const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
+                             g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+    const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
     AsmAcc a;
     acc_zero(a);
-    for (int p0 = 0; p0 < nq; p0 += 12) {
+    if (!far)
+      for (int p = 0; p < nq; p++) {
+        const QPt qp = rule64[p];
+        asm_point(a, g, rp, (double)Wdv0d, (double)Wdv1d, kr, ki, qp.z, qp.e, qp.w);
+      }
+    for (int p0 = 0; p0 < (far ? nq : 0); p0 += 12) {
       Acc32 b;
 #pragma unroll
       for (int i = 0; i < 3; i++) {
@@ -536,14 +551,16 @@ void bem_launch_asm_gh(const void *geo, const int *sid, const void *rows, const 
   bem_prof_end(BH_K_ASM_GH, st);
 }
 
+// rule32: FP32 copy of the 12x12 rule (far pairs), rule: the FP64 one (near pairs)
 void bem_launch_asm_gl32(const void *geo, const int *sid, const void *rows, const int *colOff, int dNe, int nrows,
-                         double kr, double ki, const void *rule32, int nq, void *G, void *H, void *dG, void *dH,
-                         size_t ld, double *dfPart, unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
+                         double kr, double ki, const void *rule32, const void *rule, int nq, void *G, void *H, void *dG,
+                         void *dH, size_t ld, double *dfPart, unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
 {
   dim3 grid(nwv_blocks, nrows);
   bem_prof_begin(BH_K_ASM_GL32, (double)nrows * dNe, st);
   hipLaunchKernelGGL(k_asm_gl32, grid, dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff, dNe, kr,
-                     ki, (const QPt32 *)rule32, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
+                     ki, (const QPt32 *)rule32, (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart,
+                     ghMask);
   bem_prof_end(BH_K_ASM_GL32, st);
 }
 

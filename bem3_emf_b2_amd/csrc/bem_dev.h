@@ -73,9 +73,15 @@ __device__ __forceinline__ float wsum(float v)
 }
 __device__ __forceinline__ cf wsum(cf v) { return mkf(wsum(v.x), wsum(v.y)); }
 
-// triangle element record: r(z,e) = c0 + c1 z + c2 e, W = unnormalised normal
+// element record: r(z,e) = c0 + c1 z + c2 e (+ c3 z e on a quad), W =
+// unnormalised normal (constant on a triangle; W0 of W(z,e) = W0 + W1 z +
+// W2 e on a quad), aW = |W|, quad = 1 for a bi-linear quadrangle
 struct TriGeo {
-  double c0[3], c1[3], c2[3], W[3], aW, pad[3];
+  double c0[3], c1[3], c2[3], W[3], aW, quad, pad[2];
+};
+// the bi-linear terms of a quadrangle (zero on triangles)
+struct QuadExt {
+  double c3[3], W1[3], W2[3], pad[3];
 };
 
 // quadrature-point record of a fixed rule: position in the reference

@@ -157,11 +157,13 @@ __device__ __forceinline__ double solid_angle_tri(double a[3], double b[3], doub
   return 0.0;
 }
 
-// faces: per face the three vertex positions and its (md, sd); faces are
-// grouped by main domain in segments segStart[d]..segStart[d+1]
+// faces: per face four vertex positions (fq = 1: a quadrangle, counted as
+// the triangles (0,1,2) and (0,2,3); fq = 0: a triangle, the fourth vertex
+// unused) and its (md, sd); faces are grouped by main domain in segments
+// segStart[d]..segStart[d+1]
 __global__ void __launch_bounds__(256) k_domain(const double *__restrict__ pts, int np, const double *__restrict__ fv,
-                                                const int *__restrict__ fsd, const int *__restrict__ segStart, int MN,
-                                                int *__restrict__ did_out)
+                                                const int *__restrict__ fsd, const int *__restrict__ fq,
+                                                const int *__restrict__ segStart, int MN, int *__restrict__ did_out)
 {
   const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -173,13 +175,22 @@ __global__ void __launch_bounds__(256) k_domain(const double *__restrict__ pts, 
   for (int d = 0; d <= MN && res < 0; d++) {
     int onb = 0;
     for (int f = segStart[d] + lane; f < segStart[d + 1]; f += 64) {
-      const double *v = fv + 9 * (size_t)f;
+      const double *v = fv + 12 * (size_t)f;
       double a[3] = {v[0] - rt[0], v[1] - rt[1], v[2] - rt[2]};
       double b[3] = {v[3] - rt[0], v[4] - rt[1], v[5] - rt[2]};
       double c[3] = {v[6] - rt[0], v[7] - rt[1], v[8] - rt[2]};
       int flg;
       double om = solid_angle_tri(a, b, c, flg);
       if (flg < 0) onb = 1;
+      if (fq[f]) {
+        double a2[3] = {v[0] - rt[0], v[1] - rt[1], v[2] - rt[2]};
+        double c2[3] = {v[6] - rt[0], v[7] - rt[1], v[8] - rt[2]};
+        double d2[3] = {v[9] - rt[0], v[10] - rt[1], v[11] - rt[2]};
+        int flg2;
+        double om2 = solid_angle_tri(a2, c2, d2, flg2);
+        if (flg2 < 0) onb = 1;
+        om += om2;
+      }
       og[d] += om;
       og[fsd[f]] -= om;
     }
@@ -455,8 +466,9 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
 // E, H at (t, zeta, eta) on the object's own surface in domain did: GLN for
 // s != t (coef_bd, no GHN switch), polar G for s == t (lit_coef_bd; H and F
 // vanish on a flat element).
-// F32: the other elements' sums with the FP32 rule gl32 (mixed-precision
-// form); the self element stays FP64
+// F32: the sums over far elements (farther than 4 element radii) with the
+// FP32 rule gl32 (mixed-precision form); nearer elements and the self
+// element stay FP64
 template <bool F32>
 __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t, const double *__restrict__ tgt_ze,
                                                   int np, int did, const TriGeo *__restrict__ geo,
@@ -486,12 +498,18 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
     const TriGeo g = geo[as];
     cd G[3], H[3], x0[3], x1[3];
     double f, x2;
+    bool far = false;
     if constexpr (F32) {
-      (void)x0; (void)x1; (void)x2;
-      tri_coef32(g, rt, (float)kr, (float)ki, gl32, ngl, scale, sd < 0, G, H, f);
-    } else {
-      tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
+      // FP32 only farther than 4 element radii from the element's centre
+      const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+      const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
+                               g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+      far = dx * dx + dy * dy + dz * dz > 16.0 * rad2;
     }
+    if (far)
+      tri_coef32(g, rt, (float)kr, (float)ki, gl32, ngl, scale, sd < 0, G, H, f);
+    else
+      tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
     F += f;
     const cd *Es = E + (size_t)s * 12, *dEs = dE + (size_t)s * 12, *Hs = Hh + (size_t)s * 12, *dHs = dH + (size_t)s * 12;
 #pragma unroll
@@ -636,11 +654,11 @@ __global__ void k_poynting(const cd *__restrict__ EH, const double *__restrict__
 
 // ---------------------------------------------------------------- launchers
 extern "C" {
-void bem_launch_domain(const double *pts, int np, const double *fv, const int *fsd, const int *segStart, int MN,
-                       int *did, hipStream_t st)
+void bem_launch_domain(const double *pts, int np, const double *fv, const int *fsd, const int *fq, const int *segStart,
+                       int MN, int *did, hipStream_t st)
 {
   if (np <= 0) return;
-  hipLaunchKernelGGL(k_domain, dim3((np + 3) / 4), dim3(256), 0, st, pts, np, fv, fsd, segStart, MN, did);
+  hipLaunchKernelGGL(k_domain, dim3((np + 3) / 4), dim3(256), 0, st, pts, np, fv, fsd, fq, segStart, MN, did);
 }
 void bem_launch_field(const double *pts, const int *pdid, int np, int want_did, const void *geo, const int *sid,
                       int dNe, double kr, double ki, const void *rule, int nq, double scale, int what, const void *A0,

@@ -29,13 +29,14 @@ template <> struct Cx<double> {
   typedef cd Z;
   typedef d4 V;
   static constexpr int ZP = 132;
-  static constexpr int KG = BH_K_ZGEMM, KGS = BH_K_ZGEMM_SMALL, KP = BH_K_LU_PANEL, KT = BH_K_TRSM;
+  static constexpr int KG = BH_K_ZGEMM, KGS = BH_K_ZGEMM_SMALL, KGV = BH_K_ZGEMV, KP = BH_K_LU_PANEL, KT = BH_K_TRSM;
 };
 template <> struct Cx<float> {
   typedef cf Z;
   typedef f4 V;
   static constexpr int ZP = 144;
-  static constexpr int KG = BH_K_CGEMM, KGS = BH_K_CGEMM_SMALL, KP = BH_K_LU_PANEL_C, KT = BH_K_TRSM_C;
+  static constexpr int KG = BH_K_CGEMM, KGS = BH_K_CGEMM_SMALL, KGV = BH_K_CGEMV, KP = BH_K_LU_PANEL_C,
+                       KT = BH_K_TRSM_C;
 };
 
 template <class Z> __host__ __device__ __forceinline__ Z zmk(double a, double b)
@@ -782,9 +783,11 @@ static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename 
 {
   if (m <= 0 || n <= 0) return;
   const int tx = (n + ZT - 1) / ZT, ty = (m + ZT - 1) / ZT, T = tx * ty;
+  // measured on gfx950: the 128x128 kernel overtakes the 64x64 one from
+  // about 160 tiles (k >= 128) in both precisions
   const bool skinny = n <= 8 && k >= 256;
-  const bool big = g_big_tiles && !skinny && T >= 256 && k >= 32;
-  const int kid = big ? Cx<R>::KG : Cx<R>::KGS;
+  const bool big = g_big_tiles && !skinny && T >= 160 && k >= 32;
+  const int kid = big ? Cx<R>::KG : skinny ? Cx<R>::KGV : Cx<R>::KGS;
   bem_prof_begin(kid, 8.0 * m * n * k, st);
   if (skinny) {
     // a few right-hand sides against a long k: stream A once

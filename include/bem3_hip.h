@@ -144,13 +144,14 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
  * bh_prof_read returns the summed kernel time (ms), the summed algorithmic
  * work and the launch count of one kernel since the last enable.  Work units:
  * BH_K_ASM_GL, BH_K_ASM_GH, BH_K_ASM_SELF: (row node, source element) pairs;
- * BH_K_ZGEMM (the 128x128-tile kernel) and BH_K_ZGEMM_SMALL (64x64-tile and
- * few-right-hand-side kernels): real flops (8 m n k); BH_K_LU_PANEL: real
+ * BH_K_ZGEMM (the 128x128-tile kernel) and BH_K_ZGEMM_SMALL (64x64-tile
+ * kernel): real flops (8 m n k); BH_K_LU_PANEL: real
  * flops of the unblocked panel updates; BH_K_TRSM: real flops; BH_K_FIELD,
  * BH_K_SCATTER, BH_K_FIELD_BD: (point, source element) pairs.
  * BH_K_CGEMM, BH_K_CGEMM_SMALL, BH_K_LU_PANEL_C, BH_K_TRSM_C,
  * BH_K_ASM_GL32, BH_K_FIELD_BD32: the FP32 kernels of the mixed-precision
- * form (same work units).                                                 */
+ * form (same work units).  BH_K_ZGEMV, BH_K_CGEMV: the few-right-hand-side
+ * (n <= 8, k >= 256) product kernel, real flops (8 m n k).               */
 #define BH_K_ASM_GL   0
 #define BH_K_ASM_GH   1
 #define BH_K_ASM_SELF 2
@@ -167,7 +168,9 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
 #define BH_K_TRSM_C   13
 #define BH_K_ASM_GL32 14
 #define BH_K_FIELD_BD32 15
-#define BH_K_COUNT    16
+#define BH_K_ZGEMV    16
+#define BH_K_CGEMV    17
+#define BH_K_COUNT    18
 int bh_prof_enable(int on);
 int bh_prof_read(int kid, double *ms, double *work, int *launches);
 
