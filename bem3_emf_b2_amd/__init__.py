@@ -59,7 +59,7 @@ def lib():
             "bh_prof_enable": (C.c_int, [C.c_int]),
             "bh_prof_read": (C.c_int, [C.c_int, D, D, Ip]),
             "bh_stream": (VP, []),
-            "bh_object_solve_stats": (C.c_int, [VP, Ip, D]),
+            "bh_object_solve_stats": (C.c_int, [VP, Ip, D, Ip]),
             "bh_object_set_precision": (C.c_int, [VP, C.c_int]),
             "bh_cgesv": (C.c_int, [C.c_int, C.c_int, Fp, Fp]),
             "bh_cgemm_host": (C.c_int, [C.c_int, C.c_int, C.c_int, Fp, Fp, Fp, Fp, Fp]),
@@ -182,10 +182,13 @@ class Object:
     def solve(self):
         check(lib().bh_object_solve(self.h))
 
-    def solve_stats(self):
-        """(refinement steps, final relative residual) of the last solve"""
-        it, res = np.zeros(1, np.int32), np.zeros(1)
-        check(lib().bh_object_solve_stats(self.h, ip(it), dp(res)))
+    def solve_stats(self, gmres=False):
+        """(refinement steps, final relative residual) of the last solve,
+        with gmres=True also the GMRES steps spent on the Schur complement"""
+        it, res, gm = np.zeros(1, np.int32), np.zeros(1), np.zeros(1, np.int32)
+        check(lib().bh_object_solve_stats(self.h, ip(it), dp(res), ip(gm)))
+        if gmres:
+            return int(it[0]), float(res[0]), int(gm[0])
         return int(it[0]), float(res[0])
 
     def solve_EH(self):

@@ -231,12 +231,14 @@ struct bh_object {
   cd *d_G0f = nullptr, *d_Z = nullptr, *d_Mf = nullptr, *d_Sf = nullptr;
   int *d_piv0 = nullptr, *d_pivM = nullptr, *d_pivS = nullptr;
   cd sch_delt, sch_beps, sch_sigm;
+  bool s_formed = false;               // Sf holds LU(S) (else Sp = H1 + beta G1 Z, S solved by GMRES)
+  int gm_steps = 0;                    // GMRES steps of the last S solve(s)
   cd *d_X = nullptr, *d_B = nullptr, *d_tmp = nullptr;
-  cd *d_w = nullptr;                   // 27 NN work of the block solve
+  cd *d_w = nullptr;                   // work of the block solve (schur_work(NN))
   // mixed-precision form (fmode 3): complex-FP32 factors of the block
   // elimination, refined against the FP64 matrices
   cf *c_G0f = nullptr, *c_Z = nullptr, *c_Mf = nullptr, *c_Sf = nullptr, *c_G1 = nullptr;
-  cf *c_w = nullptr;                   // 27 NN work, 8 NN right-hand side, 8 NN correction
+  cf *c_w = nullptr;                   // schur_work(NN) work, 8 NN right-hand side, 8 NN correction
   cd *d_R = nullptr;                   // 8 NN residual
   double *d_nrm = nullptr;             // 256 partial sums of the norm
   int ir_iters = 0;                    // refinement steps of the last solve
@@ -678,17 +680,18 @@ void bem_launch_scale_rows(void *, const void *, int, int, double, double, hipSt
 void bem_launch_domain(const double *, int, const double *, const int *, const int *, const int *, int, int *,
                        hipStream_t);
 void bem_launch_field(const double *, const int *, int, int, const void *, const int *, int, double, double, const void *,
-                      int, double, int, const void *, const void *, const void *, const void *, void *, int, hipStream_t);
+                      int, double, const void *, const void *, int, int, const void *, const void *, const void *,
+                      const void *, void *, int, hipStream_t);
 void bem_launch_field_memp(const double *, const int *, int, int, const void *, const int *, int, double, double,
                            const void *, int, double, int, const void *, int, double, const void *, const void *,
                            double, void *, hipStream_t);
 void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const int *, int, double, double,
                         const void *, int, const void *, int, double, const void *, const void *, void *, void *,
                         hipStream_t);
-void bem_launch_field_bd(const int *, const double *, int, int, const void *, const int *, int, double, double,
-                         const void *, const void *, int, double, const double *, const double *, const double *,
-                         const double *, const void *, const void *, const void *, const void *, void *, double *,
-                         hipStream_t);
+void bem_launch_field_bd(const int *, const double *, int, int, const void *, const void *, const int *, int, double,
+                         double, const void *, const void *, int, double, const void *, int, const double *,
+                         const double *, const double *, const double *, const void *, const void *, const void *,
+                         const void *, void *, double *, hipStream_t);
 void bem_launch_incident_bd(const double *, const double *, const int *, int, const void *, int, void *, void *, void *,
                             void *, void *, hipStream_t);
 void bem_launch_add_incident(const double *, const int *, int, const void *, int, void *, int, hipStream_t);
@@ -701,6 +704,8 @@ void bem_launch_poynting(const void *, const double *, const double *, int, doub
 static QPtN *g_rule[4] = {nullptr, nullptr, nullptr, nullptr};
 static int g_nrule[4] = {0, 0, 0, 0};
 static double g_rscale[4];
+static QPtQ *g_qrule[4] = {nullptr, nullptr, nullptr, nullptr};   // the same rules on the square (quads)
+static int g_nqrule[4] = {0, 0, 0, 0};
 
 static int ensure_rules()
 {
@@ -735,6 +740,36 @@ static int ensure_rules()
   }
   g_rscale[0] = g_rscale[1] = BEM_I_EP;
   g_rscale[2] = g_rscale[3] = 0.0625 * BEM_SQ3 * BEM_I_SPSQ3;
+  // the square's rules (bil_coef_4p/9p/GL/GH; constants d3b1_const.h:50-59):
+  // node points with N = delta and weight 1, the 3x3 Gauss rule, and the
+  // GLN x GLN and GHN x GHN tensor rules
+  auto NQ = [](double z, double e, double *n) {
+    const double ia = 1.0 / 0.5773502691896257645091;
+    n[0] = 0.25 * (1.0 - z * ia) * (1.0 - e * ia);
+    n[1] = 0.25 * (1.0 + z * ia) * (1.0 - e * ia);
+    n[2] = 0.25 * (1.0 + z * ia) * (1.0 + e * ia);
+    n[3] = 0.25 * (1.0 - z * ia) * (1.0 + e * ia);
+  };
+  std::vector<QPtQ> q4, q9, qgl, qgh;
+  for (int i = 0; i < 4; i++) {
+    QPtQ q = {ZT44[i], ET44[i], 1.0, {0.0, 0.0, 0.0, 0.0}};
+    q.n[i] = 1.0;
+    q4.push_back(q);
+  }
+  const double p9 = 0.7745966692414833770359, v0 = 0.3086419753086419753086, v1 = 0.4938271604938271604938,
+               v2 = 0.7901234567901234567901;
+  const double z9[9] = {-p9, p9, p9, -p9, 0.0, p9, 0.0, -p9, 0.0}, e9[9] = {-p9, -p9, p9, p9, -p9, 0.0, p9, 0.0, 0.0};
+  const double w9[9] = {v0, v0, v0, v0, v1, v1, v1, v1, v2};
+  for (int i = 0; i < 9; i++) { QPtQ q = {z9[i], e9[i], w9[i], {}}; NQ(q.z, q.e, q.n); q9.push_back(q); }
+  for (int i = 0; i < 12; i++)
+    for (int j = 0; j < 12; j++) { QPtQ q = {g.xl[i], g.xl[j], g.wl[i] * g.wl[j], {}}; NQ(q.z, q.e, q.n); qgl.push_back(q); }
+  for (int i = 0; i < 64; i++)
+    for (int j = 0; j < 64; j++) { QPtQ q = {g.xh[i], g.xh[j], g.wh[i] * g.wh[j], {}}; NQ(q.z, q.e, q.n); qgh.push_back(q); }
+  std::vector<QPtQ> *qall[4] = {&q4, &q9, &qgl, &qgh};
+  for (int i = 0; i < 4; i++) {
+    if (upload(&g_qrule[i], qall[i]->data(), qall[i]->size())) return BH_EHIP;
+    g_nqrule[i] = (int)qall[i]->size();
+  }
   return 0;
 }
 
@@ -885,13 +920,41 @@ static SchurF<cd> schur64(const bh_object *o)
 {
   const Dom &D1 = o->dom[1];
   return SchurF<cd>{o->NN, (size_t)o->NN, o->d_G0f, o->d_Z, o->d_Mf, o->d_Sf, D1.d_G, o->d_piv0, o->d_pivM, o->d_pivS,
-                    D1.d_col, o->sch_beps, o->sch_sigm, o->sch_delt};
+                    D1.d_col, o->sch_beps, o->sch_sigm, o->sch_delt, o->s_formed, 1e-13};
 }
 static SchurF<cf> schur32(const bh_object *o)
 {
   const Dom &D1 = o->dom[1];
   return SchurF<cf>{o->NN, (size_t)o->NN, o->c_G0f, o->c_Z, o->c_Mf, o->c_Sf, o->c_G1, o->d_piv0, o->d_pivM, o->d_pivS,
-                    D1.d_col, to_cf(o->sch_beps), to_cf(o->sch_sigm), to_cf(o->sch_delt)};
+                    D1.d_col, to_cf(o->sch_beps), to_cf(o->sch_sigm), to_cf(o->sch_delt), o->s_formed, 1e-6};
+}
+
+// S = Sp - sigma delta G1 (M^{-1} G1 o n n^T) and its LU, for objects
+// solved many times (the coupling loop); a single solve uses GMRES instead
+static int form_s(bh_object *o)
+{
+  if (o->s_formed || (o->fmode != 2 && o->fmode != 3)) return 0;
+  size_t n2 = (size_t)o->NN * o->NN;
+  int rc;
+  if (o->fmode == 2) {
+    cd *Wt = nullptr;
+    HIPCHK(hipMalloc((void **)&Wt, n2 * sizeof(cd)));
+    rc = schur_form_s(schur64(o), o->d_Sf, o->d_pivS, Wt, o->d_info, g.st);
+    (void)hipStreamSynchronize(g.st);
+    dfree(Wt);
+  } else {
+    cf *Wt = nullptr;
+    HIPCHK(hipMalloc((void **)&Wt, n2 * sizeof(cf)));
+    rc = schur_form_s(schur32(o), o->c_Sf, o->d_pivS, Wt, o->d_info, g.st);
+    (void)hipStreamSynchronize(g.st);
+    dfree(Wt);
+  }
+  if (rc) return lu_failed();
+  int info = 0;
+  HIPCHK(hipMemcpy(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost));
+  if (info) return fail(BH_ESING, "singular Schur complement (zero pivot %d)", info);
+  o->s_formed = true;
+  return 0;
 }
 
 // mode 0: auto (Schur for one medium, dense otherwise), 1: dense 8NN,
@@ -923,8 +986,9 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
     if (D0.nd != (int)NN || D1.nd != (int)NN) return fail(BH_EUNSUP, "block elimination: unexpected domain sizes");
     size_t n2 = NN * NN;
     if (!o->d_piv0 && (dalloc(&o->d_piv0, NN) || dalloc(&o->d_pivM, NN) || dalloc(&o->d_pivS, NN) ||
-                       dalloc(&o->d_w, 27 * NN) || dalloc(&o->d_R, na) || dalloc(&o->d_nrm, 256)))
+                       dalloc(&o->d_w, schur_work(NN)) || dalloc(&o->d_R, na) || dalloc(&o->d_nrm, 256)))
       return BH_EHIP;
+    o->s_formed = false;
     zd k1 = o->kn[0], k2 = o->kn[1], delt = k1 * k1 - k2 * k2, beps = k1 * k1 / (k2 * k2), sigm = 1.0 - beps;
     o->sch_delt = mk(delt.real(), delt.imag());
     o->sch_beps = mk(beps.real(), beps.imag());
@@ -932,16 +996,11 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
     int n = (int)NN;
     if (mode == 3) {
       if (!o->c_G0f && (dalloc(&o->c_G0f, n2) || dalloc(&o->c_Z, n2) || dalloc(&o->c_Mf, n2) || dalloc(&o->c_Sf, n2) ||
-                        dalloc(&o->c_G1, n2) || dalloc(&o->c_w, 43 * NN)))
+                        dalloc(&o->c_G1, n2) || dalloc(&o->c_w, schur_work(NN) + 16 * NN)))
         return BH_EHIP;
-      cf *Wt = nullptr;
-      HIPCHK(hipMalloc((void **)&Wt, n2 * sizeof(cf)));
-      int rc = schur_factor_c(n, NN, D0.d_G, D0.d_H, D1.d_G, D1.d_H, D1.d_col, o->sch_beps, o->sch_sigm, o->sch_delt,
-                              o->c_G0f, o->c_Z, o->c_Mf, o->c_Sf, o->c_G1, Wt, o->d_piv0, o->d_pivM, o->d_pivS, o->d_info,
-                              g.st);
-      (void)hipStreamSynchronize(g.st);
-      dfree(Wt);
-      if (rc) return lu_failed();
+      if (schur_factor_c(n, NN, D0.d_G, D0.d_H, D1.d_G, D1.d_H, o->sch_beps, o->c_G0f, o->c_Z, o->c_Mf, o->c_Sf,
+                         o->c_G1, o->d_piv0, o->d_pivM, o->d_info, g.st))
+        return lu_failed();
     } else {
       if (!o->d_G0f && (dalloc(&o->d_G0f, n2) || dalloc(&o->d_Z, n2) || dalloc(&o->d_Mf, n2) || dalloc(&o->d_Sf, n2)))
         return BH_EHIP;
@@ -957,17 +1016,7 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
       // S starts as H1 + beta P
       bem_launch_axpby(D1.d_H, beps.real(), beps.imag(), o->d_Sf, o->d_Sf, n2, g.st);
       if (bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, g.st)) return lu_failed();
-      // W = M^{-1} G1, T = W o (n n^T), S -= sigma delta G1 T
-      cd *W = o->d_A;
-      bool tmpW = false;
-      if (!W) { HIPCHK(hipMalloc((void **)&W, n2 * sizeof(cd))); tmpW = true; }
-      HIPCHK(hipMemcpyAsync(W, D1.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-      bem_zgetrs(n, n, o->d_Mf, NN, o->d_pivM, W, NN, 64, g.st);
-      bem_launch_hadamard_nn(W, n, NN, D1.d_col, g.st);
-      zd sd = -(sigm * delt);
-      bem_zgemm(n, n, n, sd.real(), sd.imag(), D1.d_G, NN, W, NN, 1.0, 0.0, o->d_Sf, NN, g.st);
-      if (tmpW) { HIPCHK(hipStreamSynchronize(g.st)); dfree(W); }
-      if (bem_zgetrf(n, o->d_Sf, NN, o->d_pivS, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+      // S itself is formed on demand (form_s): a single solve takes GMRES on Sp
     }
   }
   HIPCHK(hipGetLastError());
@@ -1004,20 +1053,24 @@ extern "C" int bh_object_solve(bh_object *o)
     bem_zgetrs((int)na, 1, o->d_A, na, o->d_piv, o->d_X, 1, 64, g.st);
   } else if (o->fmode == 2) {
     // the incident data enter the domain-1 rows only
-    schur_apply(schur64(o), o->d_B, false, o->d_X, o->d_w, g.st);
+    int rc = schur_apply(schur64(o), o->d_B, false, o->d_X, o->d_w, g.st, &o->gm_steps);
+    if (rc) return fail(rc, "GMRES on the Schur complement did not converge");
   } else {
     // complex-FP32 solves of the residual equation, residuals in FP64
     const double nb = bem_nrm2(o->d_B, na, o->d_nrm, g.st);
     if (nb < 0.0) return fail(BH_EHIP, "norm of the right-hand side");
     HIPCHK(hipMemsetAsync(o->d_X, 0, na * sizeof(cd), g.st));
     HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    cf *cw = o->c_w, *crhs = cw + 27 * NN, *cdx = crhs + 8 * NN;
+    cf *cw = o->c_w, *crhs = cw + schur_work(NN), *cdx = crhs + 8 * NN;
+    o->gm_steps = 0;
     const SchurF<cf> f = schur32(o);
     double res = nb > 0.0 ? 1.0 : 0.0;
     int it = 0;
     while (res > IR_TOL && it < IR_MAX) {
       bem_cvt_z2c(o->d_R, crhs, na, g.st);
-      schur_apply(f, crhs, true, cdx, cw, g.st);
+      int gs = 0;
+      if (schur_apply(f, crhs, true, cdx, cw, g.st, &gs)) return fail(BH_ECONV, "GMRES on the FP32 Schur complement did not converge");
+      o->gm_steps += gs;
       bem_acc_c2z(o->d_X, cdx, na, g.st);
       for (int d = 0; d <= o->MN; d++) {
         Dom &D = o->dom[d];
@@ -1042,11 +1095,12 @@ extern "C" int bh_object_solve(bh_object *o)
   return 0;
 }
 
-extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *resid)
+extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *resid, int *gmres_steps)
 {
   API_LOCK;
   if (iters) *iters = o->ir_iters;
   if (resid) *resid = o->ir_res;
+  if (gmres_steps) *gmres_steps = o->gm_steps;
   return 0;
 }
 
@@ -1217,6 +1271,8 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
   for (int m = 0; m < N; m++) {
     if (!objs[m]->d_beams) return fail(BH_ESTATE, "object %d has no beams", m);
     if (mode != 2 && !objs[m]->dom[0].d_U) return fail(BH_ESTATE, "object %d is not solved", m);
+    if (what == 2 && mode != 2 && !objs[m]->all_tri)
+      return fail(BH_EUNSUP, "E, H from potential gradients: bi-linear (quad) elements are not implemented");
   }
   int nc = what == 1 ? 4 : 6;
   double *d_pts;
@@ -1256,7 +1312,8 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
         const void *A0 = what == 0 ? (const void *)D.d_E : (const void *)D.d_U;
         const void *dA0 = what == 0 ? (const void *)D.d_dE : (const void *)D.d_dU;
         bem_launch_field(d_pts, d_sel, np, d, o->d_geo, D.d_sid, D.dNe, o->kn[d].real(), o->kn[d].imag(), g_rule[type],
-                         g_nrule[type], g_rscale[type], what, A0, dA0, D.d_Hf, D.d_dHf, d_out, 1, g.st);
+                         g_nrule[type], g_rscale[type], o->d_gq, g_qrule[type], g_nqrule[type], what, A0, dA0, D.d_Hf,
+                         D.d_dHf, d_out, 1, g.st);
       }
       HIPCHK(hipStreamSynchronize(g.st));
     }
@@ -1276,8 +1333,12 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
 }
 
 // surface points of the force / power rules of object oid (domain-0 faces):
-// type 0: 3 node slots (stored fields) + centroid (boundary integral);
-// type 1: 7 interior points (boundary integral)
+// triangles type 0: 3 node slots (stored fields) + centroid (boundary
+// integral), type 1: 7 interior points (boundary integral); quadrangles
+// type 0: the 4 node slots (stored), type 1: the 3x3 Gauss points
+// (mo_force.c:145-240, mo_absorb.c:9-162).  A quadrangle's per-element
+// factor is twice the triangle's (force -1 vs -1/2, power -1/2 vs -1/4):
+// its weights carry the 2.
 struct SurfPts {
   std::vector<int> t, stored;      // element (domain-local), stored slot or -1
   std::vector<double> ze, r, w, wt;
@@ -1293,9 +1354,30 @@ static void surface_points(const bh_object *o, int type, SurfPts &sp)
   const double z4[4] = {-P0, 2 * P0, -P0, 0.0}, e4[4] = {-P1, 0.0, P1, 0.0}, ww4[4] = {W0, W0, W0, -W1};
   const double z7[7] = {-a0, 2 * a0, -a0, a1, -2 * a1, a1, 0.0}, e7[7] = {-a2, 0.0, a2, a3, 0.0, -a3, 0.0};
   const double ww7[7] = {w0, w0, w0, w1, w1, w1, w2};
+  const double p9 = 0.7745966692414833770359, v0 = 0.3086419753086419753086, v1 = 0.4938271604938271604938,
+               v2 = 0.7901234567901234567901;
+  const double z9[9] = {-p9, p9, p9, -p9, 0.0, p9, 0.0, -p9, 0.0}, e9[9] = {-p9, -p9, p9, p9, -p9, 0.0, p9, 0.0, 0.0};
+  const double ww9[9] = {v0, v0, v0, v0, v1, v1, v1, v1, v2};
   for (int t = 1; t <= D.dNe; t++) {
     int a = std::abs(D.sid[t]);
     const double *cr = &o->cr[(size_t)a * 12], *cw = &o->cw[(size_t)a * 9];
+    if (o->ed[4 * a + 3] != 0) {
+      const int np = type == 0 ? 4 : 9;
+      for (int i = 0; i < np; i++) {
+        const double z = type == 0 ? ZT44[i] : z9[i], e = type == 0 ? ET44[i] : e9[i];
+        const bool st = type == 0;
+        sp.t.push_back(t);
+        sp.stored.push_back(st ? i : -1);
+        sp.ze.push_back(z); sp.ze.push_back(e);
+        for (int d = 0; d < 3; d++) {
+          sp.r.push_back(st ? o->ren[(size_t)a * 12 + i * 3 + d]
+                            : cr[d * 4] + cr[d * 4 + 1] * z + cr[d * 4 + 2] * e + cr[d * 4 + 3] * z * e);
+          sp.w.push_back(st ? o->wen[(size_t)a * 12 + i * 3 + d] : cw[d * 3] + cw[d * 3 + 1] * z + cw[d * 3 + 2] * e);
+        }
+        sp.wt.push_back(2.0 * (type == 0 ? 1.0 : ww9[i]));
+      }
+      continue;
+    }
     int np = type == 0 ? 4 : 7;
     for (int i = 0; i < np; i++) {
       double z = type == 0 ? z4[i] : z7[i], e = type == 0 ? e4[i] : e7[i];
@@ -1369,8 +1451,9 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
       double *d_cze;
       if (upload(&d_ci, ci.data(), nc) || upload(&d_ct, ct.data(), nc) || upload(&d_cze, cze.data(), cze.size()))
         return BH_EHIP;
-      bem_launch_field_bd(d_ct, d_cze, nc, 0, o->d_geo, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(), g_rule[2],
-                          o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
+      bem_launch_field_bd(d_ct, d_cze, nc, 0, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(),
+                          g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
+                          g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
                           d_tmp, d_rt, g.st);
       hipLaunchKernelGGL(k_scatter6, g1(nc), dim3(256), 0, g.st, d_tmp, d_ci, nc, d_EH);
       HIPCHK(hipStreamSynchronize(g.st));
@@ -1386,7 +1469,8 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
     for (int d = 0; d <= q->MN; d++) {
       Dom &Q = q->dom[d];
       bem_launch_field(d_r, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
-                       g_nrule[2], g_rscale[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf, d_EH, 1, g.st);
+                       g_nrule[2], g_rscale[2], q->d_gq, g_qrule[2], g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf,
+                       d_EH, 1, g.st);
     }
   }
   {
@@ -1542,6 +1626,8 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
   if (N == 1) {
     if ((rc = bh_object_solve(objs[0]))) return rc;
   } else {
+    // many solves per object: form the Schur complements once
+    for (int m = 0; m < N; m++) if ((rc = form_s(objs[m]))) return rc;
     int maxslot = 0;
     for (int m = 0; m < N; m++) maxslot = std::max(maxslot, objs[m]->nslot);
     int *d_did;
@@ -1631,8 +1717,9 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
   if (mode != 2) {
     double *d_rt2;
     HIPCHK(hipMalloc((void **)&d_rt2, 3 * (size_t)np * sizeof(double)));
-    bem_launch_field_bd(d_t, d_ze, np, did, o->d_geo, D.d_sid, D.dNe, o->kn[did].real(), o->kn[did].imag(), g_rule[2],
-                        o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
+    bem_launch_field_bd(d_t, d_ze, np, did, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[did].real(), o->kn[did].imag(),
+                        g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
+                        g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
                         d_rt2, g.st);
     HIPCHK(hipStreamSynchronize(g.st));
     dfree(d_rt2);
@@ -1645,7 +1732,8 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
       for (int d = 0; d <= q->MN; d++) {
         Dom &Q = q->dom[d];
         bem_launch_field(d_rt, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
-                         g_nrule[2], g_rscale[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf, d_out, 1, g.st);
+                         g_nrule[2], g_rscale[2], q->d_gq, g_qrule[2], g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf,
+                         d_out, 1, g.st);
       }
     }
     if (mode == 1 || mode == 2) {

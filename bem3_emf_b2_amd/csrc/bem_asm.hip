@@ -171,9 +171,6 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
 // of the GHN switch level are flagged too: pass 2 redoes them in FP64 and
 // takes the reference's decision there.
 #define BEM_FAR_RADII 4.0
-struct QPt32 {
-  float z, e, w, wn0, wn1, wn2;   // point, weight, weight x shape functions
-};
 struct Acc32 {
   cf G[3], H[3], DG0[3], DG1[3], DH0[3], DH1[3];
   float F, DF0, DF1;

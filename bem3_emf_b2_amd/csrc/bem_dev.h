@@ -146,6 +146,17 @@ struct QPtN {
   double z, e, w, n0, n1, n2;
 };
 
+// point of a rule on the square [-1,1]^2 with its four shape functions
+// (the 4-point node rule carries N = delta and weight 1)
+struct QPtQ {
+  double z, e, w, n[4];
+};
+
+// FP32 copy of a triangle rule point: position, weight, weight x shape functions
+struct QPt32 {
+  float z, e, w, wn0, wn1, wn2;
+};
+
 // one incident plane wave (ipw.h:79-99 derived data)
 struct Beam {
   double ki, E0, st, ct, sp, cp, fx, fy, fz, pad;

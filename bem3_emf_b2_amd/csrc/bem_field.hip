@@ -77,9 +77,6 @@ __device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, cons
 // G[3], H[3], F at rt from one triangle with the FP32 copy of a rule
 // (mixed-precision form): position relative to rt formed in FP64, the
 // integrand in FP32, sums over blocks of 12 points added in FP64
-struct QPt32 {
-  float z, e, w, wn0, wn1, wn2;
-};
 __device__ __forceinline__ void fma_cf(cf &a, cf b, float s) { a.x = fmaf(b.x, s, a.x); a.y = fmaf(b.y, s, a.y); }
 __device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, float kr, float ki,
                                            const QPt32 *__restrict__ rule, int nq, double scale, bool flip, cd *G, cd *H,
@@ -126,6 +123,109 @@ __device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, fl
 #pragma unroll
   for (int i = 0; i < 3; i++) { G[i] = aG[i] * (scale * g.aW); H[i] = aH[i] * sh; }
   F = f * sh;
+}
+
+// ---------------------------------------------------------------- quadrangles
+__device__ __forceinline__ void quad_geom(const TriGeo &g, const QuadExt &q, double z, double e, double *r, double *W)
+{
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    r[d] = g.c0[d] + g.c1[d] * z + g.c2[d] * e + q.c3[d] * z * e;
+    W[d] = g.W[d] + q.W1[d] * z + q.W2[d] * e;
+  }
+}
+
+// G[4], H[4], F at rt from one bi-linear quadrangle with a tabulated rule
+// (bil_coef_4p / bil_coef_9p / bil_coef_GL / bil_coef_GH, bil_coef.c:25-202):
+// G_a = 1/(4 pi) sum w N_a e^{ikR}/R |W|, H_a = 1/(4 pi) sum w N_a (ikR-1)
+// e^{ikR} (R.W)/R^3, F = 1/(4 pi) sum w (R.W)/R^3, H and F negated on the
+// sub-domain side
+__device__ __forceinline__ void quad_coef(const TriGeo &g, const QuadExt &qx, const double *rt, double kr, double ki,
+                                          const QPtQ *__restrict__ rule, int nq, bool flip, cd *G, cd *H, double &F)
+{
+  cd aG[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, aH[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double f = 0.0;
+  for (int p = 0; p < nq; p++) {
+    const QPtQ q = rule[p];
+    double r[3], W[3];
+    quad_geom(g, qx, q.z, q.e, r, W);
+    const double vx = r[0] - rt[0], vy = r[1] - rt[1], vz = r[2] - rt[2];
+    const double R = sqrt(vx * vx + vy * vy + vz * vz), iR = 1.0 / R;
+    const double aW = sqrt(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
+    const double tD = (vx * W[0] + vy * W[1] + vz * W[2]) * iR * iR * iR;
+    const cd ce = eikr(kr, ki, R);
+    const cd cb = mk(-1.0 - ki * R, kr * R);
+    const cd gk = ce * (iR * aW), hk = (cb * ce) * tD;
+#pragma unroll
+    for (int a = 0; a < 4; a++) { fma_cr(aG[a], gk, q.w * q.n[a]); fma_cr(aH[a], hk, q.w * q.n[a]); }
+    f = fma(q.w, tD, f);
+  }
+  const double sh = flip ? -BEM_I_FP : BEM_I_FP;
+#pragma unroll
+  for (int a = 0; a < 4; a++) { G[a] = aG[a] * BEM_I_FP; H[a] = aH[a] * sh; }
+  F = f * sh;
+}
+
+// bi-linear shape functions on the square (node slots at the 2x2 Gauss
+// points, d3b1_elem.c:556-568)
+__device__ __forceinline__ void quad_N(double z, double e, double *n)
+{
+  const double ia = 1.7320508075688772935;   // 1 / (1/sqrt(3))
+  n[0] = 0.25 * (1.0 - z * ia) * (1.0 - e * ia);
+  n[1] = 0.25 * (1.0 + z * ia) * (1.0 - e * ia);
+  n[2] = 0.25 * (1.0 + z * ia) * (1.0 + e * ia);
+  n[3] = 0.25 * (1.0 - z * ia) * (1.0 + e * ia);
+}
+
+// Self quadrangle at (zt, et) in polar coordinates about the point
+// (bil_coef_bd, bil_coef.c:244-342, 426-536): 4 edges x 2 x 64 angles (this
+// lane's share), 12 radial points; G always, H and F only on a warped
+// quadrangle (their integrands carry the factor c1 . W2).  Partial sums of
+// the calling lane, unscaled (the caller reduces and applies 1/(4 pi)).
+__device__ __forceinline__ void quad_self_part(const TriGeo &g, const QuadExt &qx, double zt, double et, double kr,
+                                               double ki, const double *__restrict__ xl, const double *__restrict__ wl,
+                                               const double *__restrict__ xh, const double *__restrict__ wh, int lane,
+                                               bool warped, cd *sg, cd *sh, double &sf)
+{
+  double zp = 1.0 - zt, zm = -1.0 - zt, ep = 1.0 - et, em = -1.0 - et;
+  double r[5] = {sqrt(zm * zm + em * em), sqrt(zp * zp + em * em), sqrt(zp * zp + ep * ep), sqrt(zm * zm + ep * ep), 0.0};
+  double th[5] = {atan2(em, zm), atan2(em, zp), atan2(ep, zp), atan2(ep, zm), 0.0};
+  r[4] = r[0];
+  th[4] = th[0] + 2.0 * M_PI;
+  for (int a = 0; a < 4; a++) { sg[a] = mk(0, 0); sh[a] = mk(0, 0); }
+  sf = 0.0;
+  for (int ia = lane; ia < 512; ia += 64) {
+    const int j = ia >> 7, m = (ia & 127) >> 1, half = ia & 1;
+    const double dth = th[j + 1] - th[j];
+    const double t = 0.25 * dth * xh[m] + (half ? 0.25 * (3.0 * th[j + 1] + th[j]) : 0.25 * (th[j + 1] + 3.0 * th[j]));
+    const double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(t - th[j]) - r[j + 1] * sin(t - th[j + 1]));
+    double c, sn;
+    sincos(t, &sn, &c);
+    const double fang = wh[m] * 0.25 * dth * 0.5 * rm;
+    for (int i = 0; i < 12; i++) {
+      const double rr = 0.5 * rm * (xl[i] + 1.0), z = rr * c + zt, e = rr * sn + et;
+      double K[3], W[3], n[4];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        K[d] = g.c1[d] * c + g.c2[d] * sn + qx.c3[d] * (rr * c * sn + zt * sn + et * c);
+        W[d] = g.W[d] + qx.W1[d] * z + qx.W2[d] * e;
+      }
+      const double aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]), iaK = 1.0 / aK;
+      const double aW = sqrt(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
+      quad_N(z, e, n);
+      const cd ce = eikr(kr, ki, aK * rr);
+      const cd vg = ce * (iaK * aW * wl[i] * fang);
+#pragma unroll
+      for (int a = 0; a < 4; a++) sg[a] = sg[a] + vg * n[a];
+      if (warped) {
+        const double R = aK * rr, w3 = iaK * iaK * iaK * c * sn * wl[i] * fang;
+        const cd vh = (mk(-1.0 - ki * R, kr * R) * ce) * w3;
+#pragma unroll
+        for (int a = 0; a < 4; a++) sh[a] = sh[a] + vh * n[a];
+        sf += w3;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- domain id
@@ -211,10 +311,13 @@ __global__ void __launch_bounds__(256) k_domain(const double *__restrict__ pts, 
 // ---------------------------------------------------------------- fields at points
 // what: 0 -> E,H from (E,dE,H,dH); 1 -> U from (U,dU).  Output per point:
 // 6 (E,H) or 4 (U) complex values, scaled by 1/(1+F) (open domain) or 1/F.
+// Quadrangles (geo[].quad != 0) take the square's rule qrule (nqq points).
 __global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, const int *__restrict__ pdid, int np,
                                                int want_did, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
                                                int dNe, double kr, double ki, const QPtN *__restrict__ rule, int nq,
-                                               double scale, int what, const cd *__restrict__ A0, const cd *__restrict__ dA0,
+                                               double scale, const QuadExt *__restrict__ gq,
+                                               const QPtQ *__restrict__ qrule, int nqq, int what,
+                                               const cd *__restrict__ A0, const cd *__restrict__ dA0,
                                                const cd *__restrict__ A1, const cd *__restrict__ dA1,
                                                cd *__restrict__ out, int accumulate)
 {
@@ -228,14 +331,22 @@ __global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, c
   for (int s = 1 + lane; s <= dNe; s += 64) {
     int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
-    cd G[3], H[3], dGx[3], dHx[3];
-    double f, dfx;
-    tri_coef<false>(g, rt, nullptr, kr, ki, rule, nq, scale, sd < 0, G, H, f, dGx, dHx, dfx);
+    cd G[4], H[4];
+    double f;
+    int ns = 3;
+    if (g.quad != 0.0) {
+      quad_coef(g, gq[as], rt, kr, ki, qrule, nqq, sd < 0, G, H, f);
+      ns = 4;
+    } else {
+      cd dGx[3], dHx[3];
+      double dfx;
+      tri_coef<false>(g, rt, nullptr, kr, ki, rule, nq, scale, sd < 0, G, H, f, dGx, dHx, dfx);
+      G[3] = H[3] = mk(0, 0);
+    }
     F += f;
     if (what == 0) {
       const cd *E = A0 + (size_t)s * 12, *dE = dA0 + (size_t)s * 12, *Hh = A1 + (size_t)s * 12, *dH = dA1 + (size_t)s * 12;
-#pragma unroll
-      for (int n = 0; n < 3; n++)
+      for (int n = 0; n < ns; n++)
 #pragma unroll
         for (int l = 0; l < 3; l++) {
           acc[l] = acc[l] + G[n] * dE[n * 3 + l] - H[n] * E[n * 3 + l];
@@ -243,8 +354,7 @@ __global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, c
         }
     } else {
       const cd *U = A0 + (size_t)s * 16, *dU = dA0 + (size_t)s * 16;
-#pragma unroll
-      for (int n = 0; n < 3; n++)
+      for (int n = 0; n < ns; n++)
 #pragma unroll
         for (int l = 0; l < 4; l++) acc[l] = acc[l] + G[n] * dU[n * 4 + l] - H[n] * U[n * 4 + l];
     }
@@ -466,15 +576,17 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
 // E, H at (t, zeta, eta) on the object's own surface in domain did: GLN for
 // s != t (coef_bd, no GHN switch), polar G for s == t (lit_coef_bd; H and F
 // vanish on a flat element).
-// F32: the sums over far elements (farther than 4 element radii) with the
-// FP32 rule gl32 (mixed-precision form); nearer elements and the self
-// element stay FP64
+// F32: the sums over far triangles (farther than 4 element radii) with the
+// FP32 rule gl32 (mixed-precision form); nearer elements, quadrangles and
+// the self element stay FP64.  Quadrangles take the square's GLN rule qgl
+// and, as the self element, the polar rule of bil_coef_bd.
 template <bool F32>
 __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t, const double *__restrict__ tgt_ze,
                                                   int np, int did, const TriGeo *__restrict__ geo,
-                                                  const int *__restrict__ sid, int dNe, double kr, double ki,
-                                                  const QPtN *__restrict__ gl, const QPt32 *__restrict__ gl32, int ngl,
-                                                  double scale,
+                                                  const QuadExt *__restrict__ gq, const int *__restrict__ sid, int dNe,
+                                                  double kr, double ki, const QPtN *__restrict__ gl,
+                                                  const QPt32 *__restrict__ gl32, int ngl, double scale,
+                                                  const QPtQ *__restrict__ qgl, int nqgl,
                                                   const double *__restrict__ xl, const double *__restrict__ wl,
                                                   const double *__restrict__ xh, const double *__restrict__ wh,
                                                   const cd *__restrict__ E, const cd *__restrict__ dE,
@@ -488,73 +600,107 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
   const double zt = tgt_ze[2 * p], et = tgt_ze[2 * p + 1];
   int td = sid[t], at = td < 0 ? -td : td;
   const TriGeo gt = geo[at];
+  const bool tquad = gt.quad != 0.0;
   double rt[3];
-  tri_pos(gt, zt, et, rt);
+  if (tquad) {
+    double Wt[3];
+    quad_geom(gt, gq[at], zt, et, rt, Wt);
+  } else {
+    tri_pos(gt, zt, et, rt);
+  }
   cd acc[6] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   double F = 0.0;
   for (int s = 1 + lane; s <= dNe; s += 64) {
     if (s == t) continue;
     int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
-    cd G[3], H[3], x0[3], x1[3];
-    double f, x2;
-    bool far = false;
-    if constexpr (F32) {
-      // FP32 only farther than 4 element radii from the element's centre
-      const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
-      const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
-                               g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
-      far = dx * dx + dy * dy + dz * dz > 16.0 * rad2;
+    cd G[4], H[4];
+    double f;
+    int ns = 3;
+    if (g.quad != 0.0) {
+      quad_coef(g, gq[as], rt, kr, ki, qgl, nqgl, sd < 0, G, H, f);
+      ns = 4;
+    } else {
+      cd x0[3], x1[3];
+      double x2;
+      bool far = false;
+      if constexpr (F32) {
+        // FP32 only farther than 4 element radii from the element's centre
+        const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+        const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
+                                 g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+        far = dx * dx + dy * dy + dz * dz > 16.0 * rad2;
+      }
+      if (far)
+        tri_coef32(g, rt, (float)kr, (float)ki, gl32, ngl, scale, sd < 0, G, H, f);
+      else
+        tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
     }
-    if (far)
-      tri_coef32(g, rt, (float)kr, (float)ki, gl32, ngl, scale, sd < 0, G, H, f);
-    else
-      tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
     F += f;
     const cd *Es = E + (size_t)s * 12, *dEs = dE + (size_t)s * 12, *Hs = Hh + (size_t)s * 12, *dHs = dH + (size_t)s * 12;
-#pragma unroll
-    for (int n = 0; n < 3; n++)
+    for (int n = 0; n < ns; n++)
 #pragma unroll
       for (int l = 0; l < 3; l++) {
         acc[l] = acc[l] + G[n] * dEs[n * 3 + l] - H[n] * Es[n * 3 + l];
         acc[3 + l] = acc[3 + l] + G[n] * dHs[n * 3 + l] - H[n] * Hs[n * 3 + l];
       }
   }
-  // self element: polar rule about (zt, et), 384 angles, 12 radial points
-  double zp = 1.0 - zt, zm = -0.5 - zt, ep = 0.5 * BEM_SQ3 - et, em = -0.5 * BEM_SQ3 - et;
-  double r[4] = {sqrt(zm * zm + em * em), sqrt(zp * zp + et * et), sqrt(zm * zm + ep * ep), 0.0};
-  double th[4] = {atan2(em, zm), atan2(-et, zp), atan2(ep, zm), 0.0};
-  r[3] = r[0];
-  th[3] = th[0] + 2.0 * M_PI;
-  cd sg[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
-  for (int ia = lane; ia < 384; ia += 64) {
-    int j = ia / 128, m = (ia % 128) >> 1, half = ia & 1;
-    double dth = th[j + 1] - th[j];
-    double tt = 0.25 * dth * xh[m] + (half ? 0.25 * (3.0 * th[j + 1] + th[j]) : 0.25 * (th[j + 1] + 3.0 * th[j]));
-    double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(tt - th[j]) - r[j + 1] * sin(tt - th[j + 1]));
-    double c, sn;
-    sincos(tt, &sn, &c);
-    double K[3] = {gt.c1[0] * c + gt.c2[0] * sn, gt.c1[1] * c + gt.c2[1] * sn, gt.c1[2] * c + gt.c2[2] * sn};
-    double aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]), iaK = 1.0 / aK;
-    double fg = wh[m] * 0.25 * dth * 0.5 * rm;
-    for (int i = 0; i < 12; i++) {
-      double rr = 0.5 * rm * (xl[i] + 1.0);
-      double n0, n1, n2;
-      tri_N(rr * c + zt, rr * sn + et, n0, n1, n2);
-      cd vv = eikr(kr, ki, aK * rr) * (iaK * wl[i] * fg);
-      fma_cr(sg[0], vv, n0); fma_cr(sg[1], vv, n1); fma_cr(sg[2], vv, n2);
+  // self element: polar rule about (zt, et), 12 radial points per angle
+  cd sg[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, shq[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double sfq = 0.0, bdC = 0.0;
+  bool warped = false;
+  if (tquad) {
+    // H and F of the self element need a warped quadrangle (bil_check_plane,
+    // d3b1_elem.c:640-649: planar when |c1.W2| / (|c1| |W2|) < 1e-15)
+    const QuadExt qt = gq[at];
+    bdC = gt.c1[0] * qt.W2[0] + gt.c1[1] * qt.W2[1] + gt.c1[2] * qt.W2[2];
+    const double n1 = sqrt(gt.c1[0] * gt.c1[0] + gt.c1[1] * gt.c1[1] + gt.c1[2] * gt.c1[2]);
+    const double n2 = sqrt(qt.W2[0] * qt.W2[0] + qt.W2[1] * qt.W2[1] + qt.W2[2] * qt.W2[2]);
+    warped = !(fabs(bdC / (n1 * n2)) < 1e-15) && bdC != 0.0;
+    quad_self_part(gt, qt, zt, et, kr, ki, xl, wl, xh, wh, lane, warped, sg, shq, sfq);
+  } else {
+    double zp = 1.0 - zt, zm = -0.5 - zt, ep = 0.5 * BEM_SQ3 - et, em = -0.5 * BEM_SQ3 - et;
+    double r[4] = {sqrt(zm * zm + em * em), sqrt(zp * zp + et * et), sqrt(zm * zm + ep * ep), 0.0};
+    double th[4] = {atan2(em, zm), atan2(-et, zp), atan2(ep, zm), 0.0};
+    r[3] = r[0];
+    th[3] = th[0] + 2.0 * M_PI;
+    for (int ia = lane; ia < 384; ia += 64) {
+      int j = ia / 128, m = (ia % 128) >> 1, half = ia & 1;
+      double dth = th[j + 1] - th[j];
+      double tt = 0.25 * dth * xh[m] + (half ? 0.25 * (3.0 * th[j + 1] + th[j]) : 0.25 * (th[j + 1] + 3.0 * th[j]));
+      double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(tt - th[j]) - r[j + 1] * sin(tt - th[j + 1]));
+      double c, sn;
+      sincos(tt, &sn, &c);
+      double K[3] = {gt.c1[0] * c + gt.c2[0] * sn, gt.c1[1] * c + gt.c2[1] * sn, gt.c1[2] * c + gt.c2[2] * sn};
+      double aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]), iaK = 1.0 / aK;
+      double fg = wh[m] * 0.25 * dth * 0.5 * rm;
+      for (int i = 0; i < 12; i++) {
+        double rr = 0.5 * rm * (xl[i] + 1.0);
+        double n0, n1, n2;
+        tri_N(rr * c + zt, rr * sn + et, n0, n1, n2);
+        cd vv = eikr(kr, ki, aK * rr) * (iaK * wl[i] * fg);
+        fma_cr(sg[0], vv, n0); fma_cr(sg[1], vv, n1); fma_cr(sg[2], vv, n2);
+      }
     }
   }
-  for (int a = 0; a < 3; a++) sg[a] = wsum(sg[a]) * (BEM_I_SPSQ3 * gt.aW);
-  const cd *dEs = dE + (size_t)t * 12, *dHs = dH + (size_t)t * 12;   // the self element's H term vanishes
+  const int nst = tquad ? 4 : 3;
+  const double sG = tquad ? BEM_I_FP : BEM_I_SPSQ3 * gt.aW;
+  for (int a = 0; a < nst; a++) sg[a] = wsum(sg[a]) * sG;
+  if (warped) {   // wave-uniform
+    const double sH = (td < 0 ? -BEM_I_FP : BEM_I_FP) * bdC;
+    for (int a = 0; a < 4; a++) shq[a] = wsum(shq[a]) * sH;
+    sfq = wsum(sfq) * sH;
+  }
+  const cd *Et = E + (size_t)t * 12, *dEt = dE + (size_t)t * 12, *Ht = Hh + (size_t)t * 12, *dHt = dH + (size_t)t * 12;
   F = wsum(F);
   for (int l = 0; l < 6; l++) acc[l] = wsum(acc[l]);
   if (lane == 0) {
-    for (int n = 0; n < 3; n++)
+    for (int n = 0; n < nst; n++)
       for (int l = 0; l < 3; l++) {
-        acc[l] = acc[l] + sg[n] * dEs[n * 3 + l];
-        acc[3 + l] = acc[3 + l] + sg[n] * dHs[n * 3 + l];
+        acc[l] = acc[l] + sg[n] * dEt[n * 3 + l] - shq[n] * Et[n * 3 + l];
+        acc[3 + l] = acc[3 + l] + sg[n] * dHt[n * 3 + l] - shq[n] * Ht[n * 3 + l];
       }
+    F += sfq;
     double den = did == 0 ? 1.0 + F : F;
     for (int l = 0; l < 6; l++) out[(size_t)p * 6 + l] = acc[l] * (1.0 / den);
     rt_out[3 * p] = rt[0]; rt_out[3 * p + 1] = rt[1]; rt_out[3 * p + 2] = rt[2];
@@ -660,15 +806,17 @@ void bem_launch_domain(const double *pts, int np, const double *fv, const int *f
   if (np <= 0) return;
   hipLaunchKernelGGL(k_domain, dim3((np + 3) / 4), dim3(256), 0, st, pts, np, fv, fsd, fq, segStart, MN, did);
 }
+// gq, qrule, nqq: quadrangle terms and the square's rule of the same type
 void bem_launch_field(const double *pts, const int *pdid, int np, int want_did, const void *geo, const int *sid,
-                      int dNe, double kr, double ki, const void *rule, int nq, double scale, int what, const void *A0,
-                      const void *dA0, const void *A1, const void *dA1, void *out, int accumulate, hipStream_t st)
+                      int dNe, double kr, double ki, const void *rule, int nq, double scale, const void *gq,
+                      const void *qrule, int nqq, int what, const void *A0, const void *dA0, const void *A1,
+                      const void *dA1, void *out, int accumulate, hipStream_t st)
 {
   if (np <= 0) return;
   bem_prof_begin(BH_K_FIELD, (double)np * dNe, st);
   hipLaunchKernelGGL(k_field, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo, sid,
-                     dNe, kr, ki, (const QPtN *)rule, nq, scale, what, (const cd *)A0, (const cd *)dA0, (const cd *)A1,
-                     (const cd *)dA1, (cd *)out, accumulate);
+                     dNe, kr, ki, (const QPtN *)rule, nq, scale, (const QuadExt *)gq, (const QPtQ *)qrule, nqq, what,
+                     (const cd *)A0, (const cd *)dA0, (const cd *)A1, (const cd *)dA1, (cd *)out, accumulate);
   bem_prof_end(BH_K_FIELD, st);
 }
 void bem_launch_field_memp(const double *pts, const int *pdid, int np, int want_did, const void *geo, const int *sid,
@@ -694,23 +842,27 @@ void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, i
                      (cd *)dUi);
   bem_prof_end(BH_K_SCATTER, st);
 }
-// gl32 != nullptr: the mixed-precision form (FP32 copy of the same rule)
-void bem_launch_field_bd(const int *tgt_t, const double *tgt_ze, int np, int did, const void *geo, const int *sid,
-                         int dNe, double kr, double ki, const void *gl, const void *gl32, int ngl, double scale,
-                         const double *xl, const double *wl, const double *xh, const double *wh, const void *E,
-                         const void *dE, const void *H, const void *dH, void *out, double *rt_out, hipStream_t st)
+// gl32 != nullptr: the mixed-precision form (FP32 copy of the same rule);
+// gq, qgl: quadrangle terms and the square's GLN rule
+void bem_launch_field_bd(const int *tgt_t, const double *tgt_ze, int np, int did, const void *geo, const void *gq,
+                         const int *sid, int dNe, double kr, double ki, const void *gl, const void *gl32, int ngl,
+                         double scale, const void *qgl, int nqgl, const double *xl, const double *wl, const double *xh,
+                         const double *wh, const void *E, const void *dE, const void *H, const void *dH, void *out,
+                         double *rt_out, hipStream_t st)
 {
   if (np <= 0) return;
   const int kid = gl32 ? BH_K_FIELD_BD32 : BH_K_FIELD_BD;
   bem_prof_begin(kid, (double)np * dNe, st);
   if (gl32)
     hipLaunchKernelGGL(k_field_bd<true>, dim3((np + 3) / 4), dim3(256), 0, st, tgt_t, tgt_ze, np, did, (const TriGeo *)geo,
-                       sid, dNe, kr, ki, (const QPtN *)gl, (const QPt32 *)gl32, ngl, scale, xl, wl, xh, wh,
-                       (const cd *)E, (const cd *)dE, (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
+                       (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, (const QPt32 *)gl32, ngl, scale,
+                       (const QPtQ *)qgl, nqgl, xl, wl, xh, wh, (const cd *)E, (const cd *)dE, (const cd *)H,
+                       (const cd *)dH, (cd *)out, rt_out);
   else
     hipLaunchKernelGGL(k_field_bd<false>, dim3((np + 3) / 4), dim3(256), 0, st, tgt_t, tgt_ze, np, did,
-                       (const TriGeo *)geo, sid, dNe, kr, ki, (const QPtN *)gl, (const QPt32 *)nullptr, ngl, scale, xl,
-                       wl, xh, wh, (const cd *)E, (const cd *)dE, (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
+                       (const TriGeo *)geo, (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl,
+                       (const QPt32 *)nullptr, ngl, scale, (const QPtQ *)qgl, nqgl, xl, wl, xh, wh, (const cd *)E,
+                       (const cd *)dE, (const cd *)H, (const cd *)dH, (cd *)out, rt_out);
   bem_prof_end(kid, st);
 }
 void bem_launch_incident_bd(const double *pts, const double *nrm, const int *slot, int np, const void *bm, int nb,

@@ -487,29 +487,40 @@ __global__ void __launch_bounds__(LU_NT) k_panel_col(typename Cx<R>::Z *__restri
 // positions, after the last column.  The barrier spins with a bounded count:
 // on timeout the error flag is raised and every wave still runs to the end.
 #define COOP_SPIN_LIMIT (1 << 24)
+#define COOP_GS 16         // workgroups per arrival group of the grid barrier
+#define COOP_CSTRIDE 64    // spacing of the arrival counters (unsigned ints: 256 B)
 struct CoopWs {
   double *candV;         // [2][nbmax]
   int *candI;            // [2][nbmax]
   void *candRow;         // [2][nbmax][64] complex values of the precision in use
   int *ip;               // [64] interchange of each column (panel positions)
-  unsigned int *cnt;
+  unsigned int *cnt;     // top counter, then one counter per arrival group
   int *err;
   int nbmax;
 };
 
-// Grid barrier on a monotonically increasing arrival count (cleared before
-// the launch): barrier e of the launch completes when the count reaches
-// (e + 1) * nb.  The count and everything exchanged across workgroups live
-// in uncached device memory, so visibility needs no cache write-back or
-// invalidation: a workgroup waits for its own stores to complete before it
-// arrives, and compiler barriers keep the exchange loads after the poll.
-__device__ __forceinline__ void coop_barrier(unsigned int *cnt, int *err, unsigned int &target, unsigned int nb)
+// Two-level grid barrier on monotonically increasing arrival counts (all
+// cleared before the launch).  Barrier e (1, 2, ...) of the launch: each
+// workgroup adds 1 to its group's counter; the group's last arriver (the
+// one that reads e * groupsize - 1 back) adds 1 to the top counter; the
+// barrier completes when the top counter reaches e * ngroups.  Groups of 16
+// keep the serialised atomics on any one counter short.  The counters and
+// everything exchanged across workgroups live in uncached device memory, so
+// visibility needs no cache write-back or invalidation: a workgroup waits
+// for its own stores to complete before it arrives, and compiler barriers
+// keep the exchange loads after the poll.
+__device__ __forceinline__ void coop_barrier(unsigned int *cnt, int *err, unsigned int &epoch, int blk, unsigned int nb)
 {
-  target += nb;
+  epoch += 1;
+  const unsigned int ng = (nb + COOP_GS - 1) / COOP_GS, g = (unsigned int)blk / COOP_GS;
+  const unsigned int gsz = min((unsigned int)COOP_GS, nb - g * COOP_GS);
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
   if ((threadIdx.x & 63) == 0) {
-    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int old = __hip_atomic_fetch_add(cnt + COOP_CSTRIDE * (1 + g), 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    if (old == epoch * gsz - 1) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int target = epoch * ng;
     int spins = 0;
     while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -531,7 +542,7 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
   const int lane = threadIdx.x, blk = blockIdx.x, nb = gridDim.x;
   const int r = blk * 64 + lane;
   const bool live = r < m;
-  unsigned int target = 0;
+  unsigned int epoch = 0;
   for (int c = 0; c < kb; c++) S[c * 64 + lane] = live ? A[(size_t)(k0 + r) * lda + k0 + c] : zmk<Z>(0, 0);
   bool used = false;
   int pos = r;
@@ -549,7 +560,7 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
     if (lane == 0) { w.candV[buf * w.nbmax + blk] = v; w.candI[buf * w.nbmax + blk] = vi; }
     if (v >= 0.0 && lane == vi - blk * 64)
       for (int c = j; c < kb; c++) candRow[((size_t)buf * w.nbmax + blk) * 64 + c] = S[c * 64 + lane];
-    coop_barrier(w.cnt, w.err, target, nb);
+    coop_barrier(w.cnt, w.err, epoch, blk, nb);
     // the interchange of column j-1 (published before this barrier)
     if (j > 0) {
       const int cj = ((volatile int *)w.ip)[j - 1];
@@ -584,7 +595,7 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
     }
     __syncthreads();
   }
-  coop_barrier(w.cnt, w.err, target, nb);
+  coop_barrier(w.cnt, w.err, epoch, blk, nb);
   {
     const int cj = ((volatile int *)w.ip)[kb - 1];
     if (pos == kb - 1) pos = cj; else if (pos == cj) pos = kb - 1;
@@ -760,7 +771,8 @@ static bool g_coop = false;        // the device supports cooperative launches
 // exchange area of the one-launch panel: up to COOP_NBMAX workgroups
 // (panels of up to 65536 rows)
 #define COOP_NBMAX 1024
-#define COOP_BYTES (256 + 2 * COOP_NBMAX * (sizeof(double) + sizeof(int)) + 64 * sizeof(int) + \
+#define COOP_CNT_BYTES ((size_t)COOP_CSTRIDE * sizeof(unsigned int) * (1 + COOP_NBMAX / COOP_GS))
+#define COOP_BYTES (COOP_CNT_BYTES + 256 + 2 * COOP_NBMAX * (sizeof(double) + sizeof(int)) + 64 * sizeof(int) + \
                     2 * (size_t)COOP_NBMAX * 64 * sizeof(cd))
 static void *g_coop_buf = nullptr;
 
@@ -842,7 +854,8 @@ static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, in
   const int nbc = (m + 63) / 64;
   if (w.coop_ok && nbc <= w.coop.nbmax) {
     CoopWs cw = w.coop;
-    (void)hipMemsetAsync(cw.cnt, 0, sizeof(unsigned int), st);   // arrival count of this launch
+    // arrival counters of this launch: the top one and one per group
+    (void)hipMemsetAsync(cw.cnt, 0, (size_t)COOP_CSTRIDE * sizeof(unsigned int) * (1 + (nbc + COOP_GS - 1) / COOP_GS), st);
     void *args[] = {&A, &lda, (void *)&k0, (void *)&kb, (void *)&m, &piv, (void *)&w.info, &cw};
     if (hipLaunchCooperativeKernel((const void *)k_panel_coop<R>, dim3(nbc), dim3(64), args, 0, st) == hipSuccess) {
       bem_prof_end(Cx<R>::KP, st);
@@ -971,8 +984,8 @@ template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, i
     }
     if (g_coop_buf) {
       char *q = (char *)g_coop_buf;
-      w.coop.cnt = (unsigned int *)q;
-      w.coop.err = (int *)q + 1;             q += 256;
+      w.coop.cnt = (unsigned int *)q;        q += COOP_CNT_BYTES;
+      w.coop.err = (int *)q;                 q += 256;
       w.coop.candV = (double *)q;            q += 2 * COOP_NBMAX * sizeof(double);
       w.coop.candI = (int *)q;               q += 2 * COOP_NBMAX * sizeof(int);
       w.coop.ip = (int *)q;                  q += 64 * sizeof(int);
@@ -982,7 +995,7 @@ template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, i
     }
   }
   (void)hipMemsetAsync(info, 0, sizeof(int), st);
-  if (w.coop_ok) (void)hipMemsetAsync(w.coop.cnt, 0, 2 * sizeof(int), st);   // barrier count, error flag
+  if (w.coop_ok) (void)hipMemsetAsync(w.coop.err, 0, sizeof(int), st);   // error flag of all panels
   if (n > 0) lu_rec<R>(A, lda, n, 0, n, piv, w, st);
   if (w.coop_ok) {
     int herr = 0;

@@ -10,28 +10,47 @@
 // sigma G1 sum_c N_c U_c + H1 U_3 + beta G1 Q_3 = r1_3 (N_c: the normal's
 // component c at each column).  With Z = G0^{-1} H0, M = H1 + G1 Z,
 // W = M^{-1} G1 and S = H1 + beta G1 Z - sigma delta G1 (W o n n^T).
+// S is formed only on demand (s_lu): until then Sf holds Sp = H1 + beta G1 Z
+// and the S systems are solved by GMRES with the operator
+// S x = Sp x - sigma delta G1 sum_c N_c M^{-1} G1 N_c x (well conditioned:
+// cond(S) is about 2 on sphere meshes of 960 and 3840 nodes, where GMRES
+// converges in 15 steps at either size).
 template <class Zt> struct SchurF {
   int n;                                   // NN (nodes = columns of each domain)
   size_t ld;                               // row stride of the matrices
-  const Zt *G0f, *Zm, *Mf, *Sf, *G1;       // LU(G0), Z, LU(M), LU(S), G1
+  const Zt *G0f, *Zm, *Mf, *Sf, *G1;       // LU(G0), Z, LU(M), LU(S) or Sp, G1
   const int *piv0, *pivM, *pivS;
   const ColInfo *col1;                     // domain-1 columns (unit normals)
   Zt beps, sigm, delt;
+  bool s_lu;                               // Sf holds LU(S) (else Sp, GMRES)
+  double gm_tol;                           // GMRES relative residual target
 };
+
+// Krylov dimension of one GMRES cycle and the work (in complex values of
+// the precision in use) of schur_apply for n = NN
+#define SCHUR_GM_M 60
+static inline size_t schur_work(size_t n) { return (27 + SCHUR_GM_M + 1 + 9) * n + 512; }
 
 // X = A^{-1} rhs through the factors.  rhs and X use the 8NN layout of the
 // system (rows: domain 0 components 0..3, then domain 1; unknowns U_0..U_3,
 // Q_0..Q_3, NN each).  r0nz = false asserts that the domain-0 rows of rhs
 // are zero (the case of every incident-field right-hand side).  work holds
-// 27 NN values.
-void schur_apply(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, hipStream_t st);
-void schur_apply(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, hipStream_t st);
+// schur_work(NN) values.  Returns 0, or BH_ECONV if GMRES on S does not
+// reach gm_tol within 20 cycles.  gm_steps (optional) receives the GMRES
+// steps taken.
+int schur_apply(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, hipStream_t st, int *gm_steps);
+int schur_apply(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, hipStream_t st, int *gm_steps);
 
-// complex-FP32 factors from the FP64 matrices of both domains (Wt: NN^2
-// scratch).  Returns 0, or the code of the LU (-1 scratch, -2 barrier).
-int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, const cd *H1, const ColInfo *col1,
-                   cd beps, cd sigm, cd delt, cf *G0f, cf *Zm, cf *Mf, cf *Sf, cf *G1c, cf *Wt, int *piv0, int *pivM,
-                   int *pivS, int *info, hipStream_t st);
+// complex-FP32 factors from the FP64 matrices of both domains: LU(G0), Z,
+// LU(M), G1 and Sp = H1 + beta G1 Z (S itself by schur_form_s).  Returns 0,
+// or the code of the LU (-1 scratch, -2 barrier).
+int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, const cd *H1, cd beps, cf *G0f, cf *Zm,
+                   cf *Mf, cf *Sf, cf *G1c, int *piv0, int *pivM, int *info, hipStream_t st);
+
+// S from Sp in place and its LU: W = M^{-1} G1 (Wt: NN^2 scratch),
+// S = Sp - sigma delta G1 (W o n n^T), then LU(S) into Sf / pivS
+int schur_form_s(const SchurF<cd> &f, cd *Sf, int *pivS, cd *Wt, int *info, hipStream_t st);
+int schur_form_s(const SchurF<cf> &f, cf *Sf, int *pivS, cf *Wt, int *info, hipStream_t st);
 
 // R = B - A X over the rows of one domain (rows rowBase + c nd + r), from
 // the domain's FP64 G, H and its column classes; work holds 12 nd values

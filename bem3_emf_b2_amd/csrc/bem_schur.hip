@@ -4,6 +4,12 @@
 // with one interior medium) in complex FP64 or FP32, and the FP64 residual
 // of the full system that the mixed-precision form refines against.
 #include "bem_schur.h"
+#include "../../include/bem3_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
 
 extern "C" {
 void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
@@ -11,6 +17,7 @@ void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *
 void bem_cgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
                hipStream_t);
 int bem_cgetrf(int, void *, size_t, int *, int *, hipStream_t);
+int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
 void bem_zgetrs(int, int, const void *, size_t, const int *, void *, size_t, int, hipStream_t);
 void bem_cgetrs(int, int, const void *, size_t, const int *, void *, size_t, hipStream_t);
 }
@@ -35,6 +42,11 @@ static void xgetrs(int n, int nrhs, const cf *A, size_t lda, const int *piv, cf 
 {
   bem_cgetrs(n, nrhs, A, lda, piv, B, ldb, st);
 }
+static int xgetrf(int n, cd *A, size_t lda, int *piv, int *info, hipStream_t st)
+{
+  return bem_zgetrf(n, A, lda, piv, nullptr, info, 64, st);
+}
+static int xgetrf(int n, cf *A, size_t lda, int *piv, int *info, hipStream_t st) { return bem_cgetrf(n, A, lda, piv, info, st); }
 template <class Zt> static Zt one() { Zt r; r.x = 1; r.y = 0; return r; }
 template <class Zt> static Zt zero() { Zt r; r.x = 0; r.y = 0; return r; }
 
@@ -92,8 +104,189 @@ __global__ void k_fill_x(const Zt *__restrict__ U4, const Zt *__restrict__ Q4, c
   }
 }
 
+// ---- S systems without S: GMRES on the operator ----
+__device__ __forceinline__ void zget(const cd &a, double &x, double &y) { x = a.x; y = a.y; }
+__device__ __forceinline__ void zget(const cf &a, double &x, double &y) { x = a.x; y = a.y; }
+
+// out[2i], out[2i+1] = sum_r conj(V_i[r]) w[r] for i < k (one workgroup per
+// i, double accumulation in a fixed order)
 template <class Zt>
-static void schur_apply_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, Zt *w, hipStream_t st)
+__global__ void __launch_bounds__(256) k_mdot(const Zt *__restrict__ V, size_t vstride, int k, const Zt *__restrict__ w,
+                                              int n, double *__restrict__ out)
+{
+  __shared__ double sx[4], sy[4];
+  const int i = blockIdx.x;
+  if (i >= k) return;
+  const Zt *v = V + (size_t)i * vstride;
+  double ax = 0.0, ay = 0.0;
+  for (int r = threadIdx.x; r < n; r += 256) {
+    double vx, vy, wx, wy;
+    zget(v[r], vx, vy);
+    zget(w[r], wx, wy);
+    ax += vx * wx + vy * wy;
+    ay += vx * wy - vy * wx;
+  }
+  ax = wsum(ax);
+  ay = wsum(ay);
+  if ((threadIdx.x & 63) == 0) { sx[threadIdx.x >> 6] = ax; sy[threadIdx.x >> 6] = ay; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[2 * i] = (sx[0] + sx[1]) + (sx[2] + sx[3]);
+    out[2 * i + 1] = (sy[0] + sy[1]) + (sy[2] + sy[3]);
+  }
+}
+// w[r] += sgn * sum_{i<k} h_i V_i[r]   (h: k complex doubles)
+template <class Zt>
+__global__ void k_maxpy(Zt *__restrict__ w, const Zt *__restrict__ V, size_t vstride, int k, const double *__restrict__ h,
+                        double sgn, int n)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double ax, ay;
+  zget(w[r], ax, ay);
+  for (int i = 0; i < k; i++) {
+    double vx, vy;
+    zget(V[(size_t)i * vstride + r], vx, vy);
+    const double hx = sgn * h[2 * i], hy = sgn * h[2 * i + 1];
+    ax += hx * vx - hy * vy;
+    ay += hx * vy + hy * vx;
+  }
+  w[r].x = ax;
+  w[r].y = ay;
+}
+template <class Zt> __global__ void k_vscale(Zt *__restrict__ w, double s, int n)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) { w[r].x = w[r].x * s; w[r].y = w[r].y * s; }
+}
+// w = b - w
+template <class Zt> __global__ void k_bminus(const Zt *__restrict__ b, Zt *__restrict__ w, int n)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) w[r] = b[r] - w[r];
+}
+
+// y = S x = Sp x - sigma delta G1 sum_c N_c M^{-1} G1 N_c x   (t: 7 n)
+template <class Zt> static void s_apply(const SchurF<Zt> &f, const Zt *x, Zt *y, Zt *t, hipStream_t st)
+{
+  const int n = f.n;
+  const size_t N = (size_t)n;
+  Zt *U3 = t, *T3 = t + 3 * N, *v = t + 6 * N;
+  hipLaunchKernelGGL(k_nscale<Zt>, grid1(N), dim3(256), 0, st, x, f.col1, n, U3);
+  xgemm(n, 3, n, one<Zt>(), f.G1, f.ld, U3, 3, zero<Zt>(), T3, 3, st);
+  xgetrs(n, 3, f.Mf, f.ld, f.pivM, T3, 3, st);
+  hipLaunchKernelGGL(k_ndot<Zt>, grid1(N), dim3(256), 0, st, T3, f.col1, n, v);
+  xgemm(n, 1, n, one<Zt>(), f.Sf, f.ld, x, 1, zero<Zt>(), y, 1, st);
+  Zt sd = f.sigm * f.delt;
+  xgemm(n, 1, n, -sd, f.G1, f.ld, v, 1, one<Zt>(), y, 1, st);
+}
+
+static int copy_dots(const double *d, double *h, int k, hipStream_t st)
+{
+  if (hipMemcpyAsync(h, d, 2 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : -1;
+}
+
+// S x = b by restarted GMRES (classical Gram-Schmidt with one
+// re-orthogonalisation, Givens rotations on the host); bx: in b, out x.
+// gw: (SCHUR_GM_M + 1) n Krylov vectors, 7 n operator scratch, 2 n (b, x),
+// 512 values for the dot products.  Returns 0 or BH_ECONV.
+template <class Zt> static int gmres_s(const SchurF<Zt> &f, Zt *bx, Zt *gw, hipStream_t st, int *steps)
+{
+  const int n = f.n, m = SCHUR_GM_M;
+  const size_t N = (size_t)n;
+  Zt *V = gw, *t = gw + (m + 1) * N, *b = t + 7 * N, *x = b + N;
+  double *dd = (double *)(x + N);
+  (void)hipMemcpyAsync(b, bx, N * sizeof(Zt), hipMemcpyDeviceToDevice, st);
+  (void)hipMemsetAsync(x, 0, N * sizeof(Zt), st);
+  std::vector<double> H((size_t)(m + 1) * m * 2, 0.0), h1(2 * (m + 1)), h2(2 * (m + 1));
+  std::vector<std::complex<double>> g(m + 1), cs(m), sn(m);
+  double bnorm = -1.0;
+  int total = 0;
+  for (int cycle = 0; cycle < 20; cycle++) {
+    // r = b - S x into V_0
+    if (cycle == 0) {
+      (void)hipMemcpyAsync(V, b, N * sizeof(Zt), hipMemcpyDeviceToDevice, st);
+    } else {
+      s_apply(f, x, V, t, st);
+      hipLaunchKernelGGL(k_bminus<Zt>, grid1(N), dim3(256), 0, st, b, V, n);
+    }
+    hipLaunchKernelGGL(k_mdot<Zt>, dim3(1), dim3(256), 0, st, V, N, 1, V, n, dd);
+    if (copy_dots(dd, h1.data(), 1, st)) return BH_EHIP;
+    const double beta = sqrt(h1[0]);
+    if (bnorm < 0.0) bnorm = beta;
+    if (bnorm == 0.0 || beta <= f.gm_tol * bnorm) {
+      if (steps) *steps = total;
+      (void)hipMemcpyAsync(bx, x, N * sizeof(Zt), hipMemcpyDeviceToDevice, st);
+      return 0;
+    }
+    hipLaunchKernelGGL(k_vscale<Zt>, grid1(N), dim3(256), 0, st, V, 1.0 / beta, n);
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    bool conv = false;
+    for (int j = 0; j < m; j++) {
+      Zt *w = V + (size_t)(j + 1) * N;
+      s_apply(f, V + (size_t)j * N, w, t, st);
+      // two passes of classical Gram-Schmidt against V_0..V_j
+      std::complex<double> hc[SCHUR_GM_M + 1];
+      for (int i = 0; i <= j; i++) hc[i] = 0.0;
+      for (int pass = 0; pass < 2; pass++) {
+        hipLaunchKernelGGL(k_mdot<Zt>, dim3(j + 1), dim3(256), 0, st, V, N, j + 1, w, n, dd);
+        hipLaunchKernelGGL(k_maxpy<Zt>, grid1(N), dim3(256), 0, st, w, V, N, j + 1, dd, -1.0, n);
+        if (copy_dots(dd, h1.data(), j + 1, st)) return BH_EHIP;
+        for (int i = 0; i <= j; i++) hc[i] += std::complex<double>(h1[2 * i], h1[2 * i + 1]);
+      }
+      hipLaunchKernelGGL(k_mdot<Zt>, dim3(1), dim3(256), 0, st, w, N, 1, w, n, dd);
+      if (copy_dots(dd, h2.data(), 1, st)) return BH_EHIP;
+      const double hn = sqrt(h2[0]);
+      if (hn > 0.0) hipLaunchKernelGGL(k_vscale<Zt>, grid1(N), dim3(256), 0, st, w, 1.0 / hn, n);
+      // previous rotations on the new column, then this step's rotation
+      std::complex<double> col[SCHUR_GM_M + 2];
+      for (int i = 0; i <= j; i++) col[i] = hc[i];
+      col[j + 1] = hn;
+      for (int i = 0; i < j; i++) {
+        const std::complex<double> a = col[i], c2 = col[i + 1];
+        col[i] = std::conj(cs[i]) * a + std::conj(sn[i]) * c2;
+        col[i + 1] = -sn[i] * a + cs[i] * c2;
+      }
+      const double na = std::abs(col[j]), nb = std::abs(col[j + 1]), den = std::sqrt(na * na + nb * nb);
+      if (den == 0.0) { cs[j] = 1.0; sn[j] = 0.0; }
+      else { cs[j] = col[j] / den; sn[j] = col[j + 1] / den; }
+      col[j] = std::conj(cs[j]) * col[j] + std::conj(sn[j]) * col[j + 1];
+      col[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = std::conj(cs[j]) * g[j];
+      for (int i = 0; i <= j; i++) { H[2 * ((size_t)i * m + j)] = col[i].real(); H[2 * ((size_t)i * m + j) + 1] = col[i].imag(); }
+      k = j + 1;
+      total++;
+      if (std::abs(g[j + 1]) <= f.gm_tol * bnorm || hn == 0.0) { conv = true; break; }
+    }
+    // y = R^{-1} g, x += V y
+    std::vector<std::complex<double>> y(k);
+    for (int i = k - 1; i >= 0; i--) {
+      std::complex<double> a = g[i];
+      for (int l = i + 1; l < k; l++)
+        a -= std::complex<double>(H[2 * ((size_t)i * m + l)], H[2 * ((size_t)i * m + l) + 1]) * y[l];
+      y[i] = a / std::complex<double>(H[2 * ((size_t)i * m + i)], H[2 * ((size_t)i * m + i) + 1]);
+    }
+    for (int i = 0; i < k; i++) { h2[2 * i] = y[i].real(); h2[2 * i + 1] = y[i].imag(); }
+    if (hipMemcpyAsync(dd, h2.data(), 2 * (size_t)k * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+      return BH_EHIP;
+    hipLaunchKernelGGL(k_maxpy<Zt>, grid1(N), dim3(256), 0, st, x, V, N, k, dd, 1.0, n);
+    if (hipStreamSynchronize(st) != hipSuccess) return BH_EHIP;   // dd is rewritten by the next cycle
+    if (conv) {
+      if (steps) *steps = total;
+      (void)hipMemcpyAsync(bx, x, N * sizeof(Zt), hipMemcpyDeviceToDevice, st);
+      return 0;
+    }
+  }
+  return BH_ECONV;
+}
+
+template <class Zt>
+static int schur_apply_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, Zt *w, hipStream_t st, int *gm_steps)
 {
   const int n = f.n;
   const size_t N = (size_t)n;
@@ -115,7 +308,13 @@ static void schur_apply_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, 
   // u3 = S^{-1} (b3 - sigma G1 sum_c N_c Y_c)
   hipLaunchKernelGGL(k_ndot<Zt>, grid1(N), dim3(256), 0, st, Y, f.col1, n, v);
   xgemm(n, 1, n, -f.sigm, f.G1, f.ld, v, 1, one<Zt>(), u3, 1, st);
-  xgetrs(n, 1, f.Sf, f.ld, f.pivS, u3, 1, st);
+  if (gm_steps) *gm_steps = 0;
+  if (f.s_lu) {
+    xgetrs(n, 1, f.Sf, f.ld, f.pivS, u3, 1, st);
+  } else {
+    int rc = gmres_s(f, u3, w + 27 * N, st, gm_steps);
+    if (rc) return rc;
+  }
   // u_c = M^{-1} (b_c - delta G1 N_c u3)
   hipLaunchKernelGGL(k_nscale<Zt>, grid1(N), dim3(256), 0, st, u3, f.col1, n, Qn);
   xgemm(n, 3, n, -f.delt, f.G1, f.ld, Qn, 3, one<Zt>(), Bc, 3, st);
@@ -124,15 +323,16 @@ static void schur_apply_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, 
   hipLaunchKernelGGL(k_pack_u4<Zt>, grid1(N), dim3(256), 0, st, Bc, u3, n, U4);
   xgemm(n, 4, n, one<Zt>(), f.Zm, f.ld, U4, 4, zero<Zt>(), Q4, 4, st);
   hipLaunchKernelGGL(k_fill_x<Zt>, grid1(N), dim3(256), 0, st, U4, Q4, r0nz ? (const Zt *)q0 : nullptr, n, X);
+  return 0;
 }
 
-void schur_apply(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, hipStream_t st)
+int schur_apply(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, hipStream_t st, int *gm_steps)
 {
-  schur_apply_t<cd>(f, rhs, r0nz, X, work, st);
+  return schur_apply_t<cd>(f, rhs, r0nz, X, work, st, gm_steps);
 }
-void schur_apply(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, hipStream_t st)
+int schur_apply(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, hipStream_t st, int *gm_steps)
 {
-  schur_apply_t<cf>(f, rhs, r0nz, X, work, st);
+  return schur_apply_t<cf>(f, rhs, r0nz, X, work, st, gm_steps);
 }
 
 // ---- complex-FP32 factors ----
@@ -153,13 +353,16 @@ __global__ void k_caxpby(const cd *__restrict__ x, cf a, const cf *__restrict__ 
   if (i < n) out[i] = to_cf(x[i]) + a * y[i];
 }
 // W[i][j] *= n(i) . n(j)
-__global__ void k_hadamard_nn_c(cf *__restrict__ W, int nd, size_t ld, const ColInfo *__restrict__ col)
+template <class Zt> __global__ void k_hadamard_nn(Zt *__restrict__ W, int nd, size_t ld, const ColInfo *__restrict__ col)
 {
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)nd * nd) return;
   int i = (int)(idx / nd), j = (int)(idx % nd);
   const double *a = col[i].n, *b = col[j].n;
-  W[(size_t)i * ld + j] = W[(size_t)i * ld + j] * (float)(a[0] * b[0] + a[1] * b[1] + a[2] * b[2]);
+  Zt &w = W[(size_t)i * ld + j];
+  const double s = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  w.x = w.x * s;
+  w.y = w.y * s;
 }
 
 void bem_cvt_z2c(const cd *src, cf *dst, size_t n, hipStream_t st)
@@ -171,9 +374,8 @@ void bem_acc_c2z(cd *X, const cf *d, size_t n, hipStream_t st)
   hipLaunchKernelGGL(k_acc_c2z, grid1(n), dim3(256), 0, st, X, d, n);
 }
 
-int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, const cd *H1, const ColInfo *col1,
-                   cd beps, cd sigm, cd delt, cf *G0f, cf *Zm, cf *Mf, cf *Sf, cf *G1c, cf *Wt, int *piv0, int *pivM,
-                   int *pivS, int *info, hipStream_t st)
+int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, const cd *H1, cd beps, cf *G0f, cf *Zm,
+                   cf *Mf, cf *Sf, cf *G1c, int *piv0, int *pivM, int *info, hipStream_t st)
 {
   const size_t n2 = (size_t)n * ld;
   int rc;
@@ -182,19 +384,33 @@ int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, c
   bem_cvt_z2c(H0, Zm, n2, st);
   if ((rc = bem_cgetrf(n, G0f, ld, piv0, info, st))) return rc;
   bem_cgetrs(n, n, G0f, ld, piv0, Zm, ld, st);
-  // P = G1 Z (in Sf), M = H1 + P, S starts as H1 + beta P
+  // P = G1 Z (in Sf), M = H1 + P, Sp = H1 + beta P
   bem_cvt_z2c(G1, G1c, n2, st);
   bem_cgemm(n, n, n, 1.0, 0.0, G1c, ld, Zm, ld, 0.0, 0.0, Sf, ld, st);
   hipLaunchKernelGGL(k_caxpby, grid1(n2), dim3(256), 0, st, H1, mkf(1.0f, 0.0f), (const cf *)Sf, Mf, n2);
   hipLaunchKernelGGL(k_caxpby, grid1(n2), dim3(256), 0, st, H1, to_cf(beps), (const cf *)Sf, Sf, n2);
-  if ((rc = bem_cgetrf(n, Mf, ld, pivM, info, st))) return rc;
-  // W = M^{-1} G1, S -= sigma delta G1 (W o n n^T)
-  (void)hipMemcpyAsync(Wt, G1c, n2 * sizeof(cf), hipMemcpyDeviceToDevice, st);
-  bem_cgetrs(n, n, Mf, ld, pivM, Wt, ld, st);
-  hipLaunchKernelGGL(k_hadamard_nn_c, grid1((size_t)n * n), dim3(256), 0, st, Wt, n, ld, col1);
-  const cd sd = -(sigm * delt);
-  bem_cgemm(n, n, n, sd.x, sd.y, G1c, ld, Wt, ld, 1.0, 0.0, Sf, ld, st);
-  return bem_cgetrf(n, Sf, ld, pivS, info, st);
+  return bem_cgetrf(n, Mf, ld, pivM, info, st);
+}
+
+template <class Zt> static int form_s_t(const SchurF<Zt> &f, Zt *Sf, int *pivS, Zt *Wt, int *info, hipStream_t st)
+{
+  const int n = f.n;
+  const size_t n2 = (size_t)n * f.ld;
+  // W = M^{-1} G1, T = W o (n n^T), S = Sp - sigma delta G1 T
+  (void)hipMemcpyAsync(Wt, f.G1, n2 * sizeof(Zt), hipMemcpyDeviceToDevice, st);
+  xgetrs(n, n, f.Mf, f.ld, f.pivM, Wt, f.ld, st);
+  hipLaunchKernelGGL(k_hadamard_nn<Zt>, grid1((size_t)n * n), dim3(256), 0, st, Wt, n, f.ld, f.col1);
+  const Zt sd = -(f.sigm * f.delt);
+  xgemm(n, n, n, sd, f.G1, f.ld, Wt, f.ld, one<Zt>(), Sf, f.ld, st);
+  return xgetrf(n, Sf, f.ld, pivS, info, st);
+}
+int schur_form_s(const SchurF<cd> &f, cd *Sf, int *pivS, cd *Wt, int *info, hipStream_t st)
+{
+  return form_s_t<cd>(f, Sf, pivS, Wt, info, st);
+}
+int schur_form_s(const SchurF<cf> &f, cf *Sf, int *pivS, cf *Wt, int *info, hipStream_t st)
+{
+  return form_s_t<cf>(f, Sf, pivS, Wt, info, st);
 }
 
 // ---- FP64 residual of the full system ----

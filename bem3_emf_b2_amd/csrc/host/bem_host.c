@@ -288,10 +288,39 @@ static void rule_tables(BOUD *bd)
 }
 
 /* element constants and node data of triangles (setup2.c:666-749, 838-869) */
+/* element constants (setup2.c:666-749, 838-869): triangles on the
+ * equilateral reference triangle, quadrangles r = c0 + c1 z + c2 e + c3 z e
+ * on [-1,1]^2 with W(z,e) = dr/dz x dr/de = W0 + W1 z + W2 e and node slots
+ * at the 2x2 Gauss points */
+static void elem_const_quad(BOUD *bd, int i)
+{
+  double p[4][3];
+  for (int j = 0; j < 4; j++) for (int d = 0; d < 3; d++) p[j][d] = bd->rn[bd->ed[i][j]][d];
+  for (int d = 0; d < 3; d++) {
+    bd->cr[i][d][0] = 0.25 * (p[0][d] + p[1][d] + p[2][d] + p[3][d]);
+    bd->cr[i][d][1] = 0.25 * (p[1][d] + p[2][d] - p[0][d] - p[3][d]);
+    bd->cr[i][d][2] = 0.25 * (p[2][d] + p[3][d] - p[0][d] - p[1][d]);
+    bd->cr[i][d][3] = 0.25 * (p[0][d] + p[2][d] - p[1][d] - p[3][d]);
+  }
+  for (int d = 0; d < 3; d++) {
+    int a = (d + 1) % 3, b = (d + 2) % 3;
+    bd->cw[i][d][0] = bd->cr[i][a][1] * bd->cr[i][b][2] - bd->cr[i][b][1] * bd->cr[i][a][2];
+    bd->cw[i][d][1] = bd->cr[i][a][1] * bd->cr[i][b][3] - bd->cr[i][b][1] * bd->cr[i][a][3];
+    bd->cw[i][d][2] = bd->cr[i][a][3] * bd->cr[i][b][2] - bd->cr[i][b][3] * bd->cr[i][a][2];
+  }
+  for (int j = 0; j < 4; j++) {
+    double z = bd->zt_44[j], e = bd->et_44[j];
+    for (int d = 0; d < 3; d++) {
+      bd->ren[i][j][d] = bd->cr[i][d][0] + bd->cr[i][d][1] * z + bd->cr[i][d][2] * e + bd->cr[i][d][3] * z * e;
+      bd->wen[i][j][d] = bd->cw[i][d][0] + bd->cw[i][d][1] * z + bd->cw[i][d][2] * e;
+    }
+  }
+}
+
 static void elem_const(BOUD *bd)
 {
   for (int i = 1; i <= bd->Ne; i++) {
-    if (bd->ed[i][3] != 0) { printf("element %d is a quadrangle: this build supports linear triangles only. Exit...\n", i); exit(1); }
+    if (bd->ed[i][3] != 0) { elem_const_quad(bd, i); continue; }
     double p[3][3];
     for (int j = 0; j < 3; j++) for (int d = 0; d < 3; d++) p[j][d] = bd->rn[bd->ed[i][j]][d];
     for (int d = 0; d < 3; d++) {

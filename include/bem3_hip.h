@@ -80,15 +80,20 @@ int bh_object_matrix(const bh_object *o, int did, int which, double *out);
 int bh_object_set_beams(bh_object *o, int nb, const double *par);
 /* Factorise the boundary-integral system (create_Amatrix_csr + lu_dec_A,
  * solve_bieq2.c:186-408).  mode 1: the reference's dense 8NN matrix;
- * mode 2: exact block elimination for one interior medium (three NN-sized
- * LU factorisations); mode 0 picks 2 when possible; mode 3: the block
+ * mode 2: exact block elimination for one interior medium (LU of G0 and
+ * of M; the Schur complement S of the potential is factorised when a
+ * coupling loop of several objects first needs it, a single solve takes
+ * GMRES on it to a 1e-13 relative residual); mode 0 picks 2 when
+ * possible (3 in the mixed precision of bh_object_set_precision); mode 3: the block
  * elimination in complex FP32 (mixed-precision form), every later solve
  * refined in FP64 against the FP64 matrices to a relative residual of
  * 1e-10 (BH_ECONV if 30 steps do not reach it).                          */
 int bh_object_factor(bh_object *o, int mode);
 /* refinement steps and final relative residual ||b - A x|| / ||b|| of the
- * last bh_object_solve (0, 0 outside the mixed-precision form)            */
-int bh_object_solve_stats(const bh_object *o, int *iters, double *resid);
+ * last bh_object_solve (0, 0 outside the mixed-precision form), and the
+ * GMRES steps it spent on the Schur complement (0 once S is formed: the
+ * coupling loop of several objects forms it, a single solve does not)     */
+int bh_object_solve_stats(const bh_object *o, int *iters, double *resid, int *gmres_steps);
 /* boundary potentials from the current incident data (create_Bmatrix +
  * solve_ABmatrix + tmatrix_bd_store, mo_solve.c:137-274, 457-545)         */
 int bh_object_solve(bh_object *o);

@@ -57,6 +57,22 @@ def test_single_object_boundary_values(single, mode):
             assert err < tol, (mode, d, key, err)
 
 
+def test_single_solve_uses_gmres_on_schur_complement(single):
+    """one object, one solve: S is not formed, GMRES converges in a few
+    dozen steps (cond(S) is about 2); the result equals the dense form's"""
+    import bem3_emf_b2_amd as bem
+    O, G = single
+    G[0].factor(1)
+    bem.mo_solve(G)
+    ref = {k: G[0].state(0, k) for k in ("U", "dU")}
+    G[0].factor(2)
+    bem.mo_solve(G)
+    _, _, gm = G[0].solve_stats(gmres=True)
+    assert 3 <= gm <= 60, gm
+    for k, v in ref.items():
+        assert _rel(G[0].state(0, k), v) < 1e-11, k
+
+
 @pytest.mark.parametrize("typ", [0, 1, 2, 3])
 def test_single_object_fields(single, typ):
     import bem3_emf_b2_amd as bem
@@ -122,6 +138,7 @@ def test_two_object_coupling_loop():
         g.factor(2)
     it_g, tr_g = bem.mo_solve(G, ccd=1e-8)
     assert it_g == it_o
+    assert all(g.solve_stats(gmres=True)[2] == 0 for g in G)   # the loop formed S
     assert np.abs(tr_g[2:it_o + 1] - tr_o[2:it_o + 1]).max() / tr_o[2] < 1e-6
     for m in range(2):
         for key in ("U", "E", "H"):
