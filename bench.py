@@ -275,6 +275,9 @@ def main():
 # sqrt/rcp/sin/cos sequences at their instruction counts)
 FLOP_PER_POINT = {"asm_gl": 325.0, "field": 80.0, "scatter": 160.0}
 PEAK_FP32_TFLOPS = 157.3    # FP32 vector and FP32 matrix (dense)
+# k_gemm128<double> at 15360^3: FETCH_SIZE 323,492,022 KiB x 2.0 + WRITE_SIZE 3,830,400 KiB x 1.0
+GEMM128_FP64_TRAFFIC = {"bytes": 323492022 * 1024 * 2.0 + 3830400 * 1024 * 1.0, "shape": "15360x15360x15360",
+                        "algorithmic_bytes": 3 * 15360 ** 2 * 16}
 
 
 def roofline(dom, rec):
@@ -284,8 +287,15 @@ def roofline(dom, rec):
     if dom in ("zgemm", "cgemm"):
         peak = PEAK_FP64_TFLOPS if dom == "zgemm" else PEAK_FP32_TFLOPS
         ach = work / n / avg_s / 1e12
+        # HBM bytes per launch from FETCH_SIZE / WRITE_SIZE counter runs of the
+        # kernel alone at 15360^3 (the step's largest product), scaled by the
+        # copy-kernel calibration (profiles/gemm128_fp64_15360_counters.txt);
+        # measured for the FP64 kernel only
+        t = GEMM128_FP64_TRAFFIC if dom == "zgemm" else None
         return {"kernel": "k_gemm128<%s>" % ("double" if dom == "zgemm" else "float"), "bound": "mfma",
-                "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                "traffic": t["bytes"] if t else None, "traffic_shape": t["shape"] if t else None,
+                "traffic_algorithmic_bytes": t["algorithmic_bytes"] if t else None,
                 "avg_launch_ms": 1000.0 * avg_s, "work_per_launch": work / n,
                 "work_unit": "real %s flops (8 m n k)" % ("FP64" if dom == "zgemm" else "FP32")}
     if dom in FLOP_PER_POINT:
