@@ -41,6 +41,7 @@ void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *
 int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
 void bem_zgetrs(int, int, const void *, size_t, const int *, void *, size_t, int, hipStream_t);
 int bem_la_init(void);
+int bem_trsv_error(hipStream_t);
 void bem_cgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
                hipStream_t);
 int bem_cgetrf(int, void *, size_t, int *, int *, hipStream_t);
@@ -61,6 +62,12 @@ static int fail(int code, const char *fmt, ...)
 }
 static void dfree(void *p) { (void)hipFree(p); }
 static int lu_failed(void) { return fail(BH_EHIP, "LU factorisation: scratch allocation failed or a panel barrier timed out"); }
+// a one-launch triangular solve whose wait timed out since the last check
+static int trsv_failed(hipStream_t st)
+{
+  int e = bem_trsv_error(st);
+  return e ? fail(BH_EHIP, "triangular solve: a block wait timed out") : 0;
+}
 
 // Entry points are serialised: callers may call the scalar field functions
 // from their own thread pools (e.g. OpenMP loops over grid points); all work
@@ -597,7 +604,8 @@ extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
   HIPCHK(hipMemcpy(B, dB, (size_t)n * nrhs * sizeof(cd), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(A, dA, (size_t)n * n * sizeof(cd), hipMemcpyDeviceToHost));
   dfree(dA); dfree(dB); dfree(work); dfree(piv); dfree(info);
-  return hinfo ? fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo) : 0;
+  if (hinfo) return fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo);
+  return trsv_failed(g.st);
 }
 
 extern "C" int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B,
@@ -640,7 +648,8 @@ extern "C" int bh_cgesv(int n, int nrhs, float *A, float *B)
   HIPCHK(hipMemcpy(B, dB, (size_t)n * nrhs * sizeof(cf), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(A, dA, (size_t)n * n * sizeof(cf), hipMemcpyDeviceToHost));
   dfree(dA); dfree(dB); dfree(piv); dfree(info);
-  return hinfo ? fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo) : 0;
+  if (hinfo) return fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo);
+  return trsv_failed(g.st);
 }
 
 extern "C" int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const float *B, const float *beta,
@@ -1092,7 +1101,7 @@ extern "C" int bh_object_solve(bh_object *o)
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g.st));
-  return 0;
+  return trsv_failed(g.st);
 }
 
 extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *resid, int *gmres_steps)
@@ -1206,7 +1215,7 @@ extern "C" int bh_object_solve_EH(bh_object *o)
   int info = 0;
   HIPCHK(hipMemcpy(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost));
   if (info) return fail(BH_ESING, "singular G (zero pivot %d)", info);
-  return 0;
+  return trsv_failed(g.st);
 }
 
 // which: 0 U, 1 dU ([dNe+1][4][4]), 2 E, 3 dE, 4 H, 5 dH ([dNe+1][4][3]);

@@ -558,8 +558,10 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
     wave_argmax(v, vi);
     const int buf = j & 1;
     if (lane == 0) { w.candV[buf * w.nbmax + blk] = v; w.candI[buf * w.nbmax + blk] = vi; }
-    if (v >= 0.0 && lane == vi - blk * 64)
-      for (int c = j; c < kb; c++) candRow[((size_t)buf * w.nbmax + blk) * 64 + c] = S[c * 64 + lane];
+    // the candidate row's entries j..kb-1, one per lane (vi is the same in
+    // every lane after the reduction)
+    if (v >= 0.0 && lane >= j && lane < kb)
+      candRow[((size_t)buf * w.nbmax + blk) * 64 + lane] = S[lane * 64 + (vi - blk * 64)];
     coop_barrier(w.cnt, w.err, epoch, blk, nb);
     // the interchange of column j-1 (published before this barrier)
     if (j > 0) {
@@ -765,6 +767,213 @@ __global__ void __launch_bounds__(256) k_trsm_upper(const typename Cx<R>::Z *__r
     }
 }
 
+// ---- triangular solves with a few right-hand sides, one launch ----
+// X := L^{-1} X (unit lower) or U^{-1} X (upper) for nrhs <= TRSV_MAXR
+// columns.  Workgroup b owns one 64-row block (block i = b for L, nb-1-b for
+// U, so a block depends only on workgroups launched before it).  Its four
+// waves fold in the finished blocks j (acc_r -= A_rj x_j) as their flags
+// appear, wave w taking every fourth j; the block that finishes last (the
+// neighbour, on the critical path) and the diagonal block are staged in LDS
+// before any wait, and wave 0 folds in the neighbour and solves the diagonal
+// block in 64 readlane-broadcast steps, then publishes x and raises the
+// block's flag.  x and the flags live in uncached memory and are read with
+// relaxed agent-scope atomic loads (no wait between them; a volatile load
+// would wait for each one); flag = launch epoch, so flags never need
+// clearing; a block waits for its own stores to complete before it raises
+// its flag.  Every wait is bounded: on timeout the error flag is raised and
+// the launch runs to its end.
+#define TRSV_NBMAX 4096    // 64-row blocks (n <= 262144)
+#define TRSV_XMAX 65536    // rows of the exchange copy of x
+#define TRSV_MAXR 4        // right-hand sides of the one-launch path
+struct TrsvWs {
+  void *x;                 // [TRSV_XMAX][TRSV_MAXR] complex values of the precision in use
+  int *flag;               // [TRSV_NBMAX]
+  int *err;
+};
+// exchange area of the one-launch triangular solves (uncached, allocated
+// once for the life of the process and never freed); false: the recursive
+// path runs instead
+static void *g_trsv_buf = nullptr;
+static bool g_trsv_ok = true;
+static int g_trsv_epoch = 0;
+
+// LDS of one block: the neighbour block, the diagonal block (rows padded
+// to 65), an x block per wave and the inverted diagonal
+template <class R> constexpr int trsv_lds_bytes()
+{
+  return (2 * 64 * 65 + 4 * 64 * TRSV_MAXR + 64) * (int)sizeof(typename Cx<R>::Z);
+}
+
+// value of lane k (uniform k)
+__device__ inline double lane_bcast(double v, int k)
+{
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)u, k), hi = __builtin_amdgcn_readlane((int)(u >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float lane_bcast(float v, int k)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// relaxed agent-scope loads of one exchanged complex value
+__device__ inline cd ld_x(const cd *p)
+{
+  const unsigned long long *q = (const unsigned long long *)p;
+  const unsigned long long re = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long im = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return zmk<cd>(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
+}
+__device__ inline cf ld_x(const cf *p)
+{
+  const unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return zmk<cf>(__int_as_float((int)(unsigned int)u), __int_as_float((int)(unsigned int)(u >> 32)));
+}
+
+// lane 0 waits for flag >= epoch (bounded)
+__device__ inline void trsv_wait(const TrsvWs &w, int j, int epoch, int lane)
+{
+  if (lane == 0) {
+    int spins = 0;
+    while (__hip_atomic_load(w.flag + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch &&
+           __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      if (++spins > COOP_SPIN_LIMIT) { atomicExch(w.err, 1); break; }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// the x block of a finished block into LDS: all TRSV_MAXR slots of a row
+// are in the exchange area, so every load is issued before the one wait
+template <class Z>
+__device__ inline void trsv_fetch_x(Z *xw, const Z *xs, int c0, int cols, int lane)
+{
+  Z xv[TRSV_MAXR];
+  if (lane < cols) {
+    const Z *src = xs + (size_t)(c0 + lane) * TRSV_MAXR;
+#pragma unroll
+    for (int c = 0; c < TRSV_MAXR; c++) xv[c] = ld_x(src + c);
+  } else {
+#pragma unroll
+    for (int c = 0; c < TRSV_MAXR; c++) xv[c] = zmk<Z>(0, 0);
+  }
+#pragma unroll
+  for (int c = 0; c < TRSV_MAXR; c++) xw[lane * TRSV_MAXR + c] = xv[c];
+}
+
+// acc_c += sum_k a_k x_kc over cols <= 64 columns, the row of A read 16
+// values at a time (the loads of a chunk are issued before the first use)
+template <class Z, int NR>
+__device__ inline void trsv_rowdot(Z *acc, const Z *arow, const Z *xw, int cols)
+{
+  for (int k0 = 0; k0 < cols; k0 += 16) {
+    Z a[16];
+    if (k0 + 16 <= cols) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) a[u] = arow[k0 + u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; u++) a[u] = k0 + u < cols ? arow[k0 + u] : zmk<Z>(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++)
+#pragma unroll
+      for (int c = 0; c < NR; c++) acc[c] = acc[c] + a[u] * xw[(k0 + u) * TRSV_MAXR + c];
+  }
+}
+
+template <class R, bool UPPER, int NR>
+__global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__restrict__ A, size_t lda, int n,
+                                                 typename Cx<R>::Z *B, size_t ldb, TrsvWs w, int epoch)
+{
+  typedef typename Cx<R>::Z Z;
+  extern __shared__ double dyn_lds[];
+  Z *Ap = (Z *)dyn_lds;                 // [64][65] the neighbour block
+  Z *Ad = Ap + 64 * 65;                 // [64][65] the diagonal block
+  Z *Xw = Ad + 64 * 65;                 // [4][64][TRSV_MAXR] x block, then partial sums, per wave
+  Z *Di = Xw + 4 * 64 * TRSV_MAXR;      // [64] inverted diagonal (U)
+  const int nb = (n + 63) / 64, b = blockIdx.x;
+  const int i = UPPER ? nb - 1 - b : b;
+  const int r0 = 64 * i, rows = min(64, n - r0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tid = threadIdx.x;
+  const bool live = lane < rows;
+  Z *xs = (Z *)w.x;
+  const int ndep = UPPER ? nb - 1 - i : i;
+  const int jlast = UPPER ? i + 1 : i - 1;
+  // staging (coalesced rows), before any wait
+  {
+    const int c0 = 64 * jlast, cols = ndep > 0 ? min(64, n - c0) : 0;
+    for (int idx = tid; idx < 64 * 64; idx += 256) {
+      const int rr = idx >> 6, kk = idx & 63;
+      const bool rok = rr < rows;
+      Ad[rr * 65 + kk] = (rok && kk < rows) ? A[(size_t)(r0 + rr) * lda + r0 + kk] : zmk<Z>(0, 0);
+      if (ndep > 0) Ap[rr * 65 + kk] = (rok && kk < cols) ? A[(size_t)(r0 + rr) * lda + c0 + kk] : zmk<Z>(0, 0);
+    }
+    if (UPPER && tid < rows) Di[tid] = cdiv(zmk<Z>(1, 0), A[(size_t)(r0 + tid) * lda + r0 + tid]);
+  }
+  Z acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++) acc[c] = zmk<Z>(0, 0);
+  const Z *arow = A + (size_t)(r0 + (live ? lane : 0)) * lda;
+  Z *xw = Xw + wv * 64 * TRSV_MAXR;
+  // the other dependencies in the order they finish, wave w every fourth
+  for (int q = wv; q < ndep - 1; q += 4) {
+    const int j = UPPER ? nb - 1 - q : q;
+    trsv_wait(w, j, epoch, lane);
+    const int c0 = 64 * j, cols = min(64, n - c0);   // U: the first block j = nb-1 may be partial
+    trsv_fetch_x(xw, xs, c0, cols, lane);
+    trsv_rowdot<Z, NR>(acc, arow + c0, xw, cols);
+  }
+  if (wv > 0)
+#pragma unroll
+    for (int c = 0; c < NR; c++) xw[lane * TRSV_MAXR + c] = acc[c];
+  __syncthreads();   // also completes the staging
+  if (wv != 0) return;
+  Z v[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++)
+    v[c] = acc[c] + Xw[(64 + lane) * TRSV_MAXR + c] + Xw[(128 + lane) * TRSV_MAXR + c] +
+           Xw[(192 + lane) * TRSV_MAXR + c];
+  // the neighbour: the wait on the critical path
+  if (ndep > 0) {
+    trsv_wait(w, jlast, epoch, lane);
+    const int c0 = 64 * jlast, cols = min(64, n - c0);
+    trsv_fetch_x(xw, xs, c0, cols, lane);
+#pragma unroll 8
+    for (int k = 0; k < 64; k++) {
+      const Z a = Ap[lane * 65 + k];
+#pragma unroll
+      for (int c = 0; c < NR; c++) v[c] = v[c] + a * xw[k * TRSV_MAXR + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NR; c++) v[c] = live ? B[(size_t)(r0 + lane) * ldb + c] - v[c] : zmk<Z>(0, 0);
+  // the diagonal block: x_k broadcast from lane k
+  for (int t = 0; t < rows; t++) {
+    const int k = UPPER ? rows - 1 - t : t;
+    const bool below = UPPER ? lane < k : (lane > k && live);
+    const Z l = below ? Ad[lane * 65 + k] : zmk<Z>(0, 0);
+    const Z dk = UPPER ? Di[k] : zmk<Z>(1, 0);
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      Z xk = zmk<Z>(lane_bcast(v[c].x, k), lane_bcast(v[c].y, k));
+      if (UPPER) xk = xk * dk;
+      v[c] = lane == k ? xk : v[c] - l * xk;
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < NR; c++) {
+      B[(size_t)(r0 + lane) * ldb + c] = v[c];
+      xs[(size_t)(r0 + lane) * TRSV_MAXR + c] = v[c];
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  if (lane == 0) __hip_atomic_store(w.flag + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---- host side ----
 static bool g_big_tiles = false;   // set once the 128x128 kernels' LDS limits are raised
 static bool g_coop = false;        // the device supports cooperative launches
@@ -785,6 +994,15 @@ template <class R> static int la_init_t()
   if (hipFuncSetAttribute((const void *)k_trsm_upper<R>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) return -1;
   if (hipFuncSetAttribute((const void *)k_gemm128<R>, hipFuncAttributeMaxDynamicSharedMemorySize, zlds_bytes<R>()) != hipSuccess)
     return -1;
+  const void *sf[8] = {(const void *)k_trsv_sf<R, false, 1>, (const void *)k_trsv_sf<R, false, 2>,
+                       (const void *)k_trsv_sf<R, false, 3>, (const void *)k_trsv_sf<R, false, 4>,
+                       (const void *)k_trsv_sf<R, true, 1>,  (const void *)k_trsv_sf<R, true, 2>,
+                       (const void *)k_trsv_sf<R, true, 3>,  (const void *)k_trsv_sf<R, true, 4>};
+  for (int q = 0; q < 8; q++)
+    if (hipFuncSetAttribute(sf[q], hipFuncAttributeMaxDynamicSharedMemorySize, trsv_lds_bytes<R>()) != hipSuccess) {
+      (void)hipGetLastError();
+      g_trsv_ok = false;
+    }
   return 0;
 }
 
@@ -1007,6 +1225,46 @@ template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, i
 }
 
 // solve A X = B with the factors of getrf_t; B is n x nrhs (row stride ldb)
+#define TRSV_XBYTES ((size_t)TRSV_XMAX * TRSV_MAXR * sizeof(cd))
+#define TRSV_BYTES (TRSV_XBYTES + TRSV_NBMAX * sizeof(int) + 256)
+
+// one-launch solve with the factors for nrhs <= TRSV_MAXR; false if not possible
+// (then the recursive path runs)
+template <class R, bool UPPER>
+static bool trsv_sf(const typename Cx<R>::Z *A, size_t lda, int n, typename Cx<R>::Z *B, size_t ldb, int nrhs,
+                    hipStream_t st)
+{
+  const int nb = (n + 63) / 64;
+  if (!g_coop || !g_trsv_ok || nrhs > TRSV_MAXR || n > TRSV_XMAX || nb > TRSV_NBMAX) return false;
+  if (!g_trsv_buf) {
+    if (hipExtMallocWithFlags(&g_trsv_buf, TRSV_BYTES, hipDeviceMallocUncached) != hipSuccess ||
+        hipMemset(g_trsv_buf, 0, TRSV_BYTES) != hipSuccess) {
+      g_trsv_buf = nullptr;
+      g_trsv_ok = false;
+      return false;
+    }
+  }
+  TrsvWs w;
+  char *q = (char *)g_trsv_buf;
+  w.x = (void *)q;              q += TRSV_XBYTES;
+  w.flag = (int *)q;            q += TRSV_NBMAX * sizeof(int);
+  w.err = (int *)q;
+  const int epoch = ++g_trsv_epoch;
+  bem_prof_begin(Cx<R>::KT, 4.0 * (double)n * n * nrhs, st);
+  void *args[] = {(void *)&A, &lda, (void *)&n, &B, &ldb, &w, (void *)&epoch};
+  const void *fn = nrhs == 1 ? (const void *)k_trsv_sf<R, UPPER, 1>
+                 : nrhs == 2 ? (const void *)k_trsv_sf<R, UPPER, 2>
+                 : nrhs == 3 ? (const void *)k_trsv_sf<R, UPPER, 3>
+                             : (const void *)k_trsv_sf<R, UPPER, 4>;
+  const bool ok = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(256), args, trsv_lds_bytes<R>(), st) == hipSuccess;
+  bem_prof_end(Cx<R>::KT, st);
+  if (!ok) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
+}
+
 template <class R>
 static void getrs_t(int n, int nrhs, const typename Cx<R>::Z *A, size_t lda, const int *piv, typename Cx<R>::Z *B,
                     size_t ldb, hipStream_t st)
@@ -1014,8 +1272,21 @@ static void getrs_t(int n, int nrhs, const typename Cx<R>::Z *A, size_t lda, con
   typedef typename Cx<R>::Z Z;
   if (n <= 0 || nrhs <= 0) return;
   hipLaunchKernelGGL(k_laswp<Z>, dim3((nrhs + 255) / 256), dim3(256), 0, st, B, ldb, 0, nrhs, 0, n, piv);
-  trsm_lunit_rec<R>(A, lda, n, B, ldb, nrhs, st);
-  trsm_upper_rec<R>(A, lda, n, B, ldb, nrhs, st);
+  if (!trsv_sf<R, false>(A, lda, n, B, ldb, nrhs, st)) trsm_lunit_rec<R>(A, lda, n, B, ldb, nrhs, st);
+  if (!trsv_sf<R, true>(A, lda, n, B, ldb, nrhs, st)) trsm_upper_rec<R>(A, lda, n, B, ldb, nrhs, st);
+}
+
+// the error flag of the one-launch triangular solves (a wait timed out);
+// clears it
+static int trsv_error(hipStream_t st)
+{
+  if (!g_trsv_buf) return 0;
+  int *err = (int *)((char *)g_trsv_buf + TRSV_XBYTES + TRSV_NBMAX * sizeof(int));
+  int h = 0;
+  if (hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    return -1;
+  if (h) (void)hipMemsetAsync(err, 0, sizeof(int), st);
+  return h;
 }
 
 extern "C" {
@@ -1031,6 +1302,9 @@ int bem_la_init(void)
     g_coop = coop != 0;
   return 0;
 }
+
+// nonzero if a one-launch triangular solve timed out since the last call
+int bem_trsv_error(hipStream_t st) { return trsv_error(st); }
 
 // complex FP64 (interleaved double pairs)
 void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t lda, const void *B, size_t ldb,

@@ -232,6 +232,7 @@ def main():
                   for t in (0, 1))
         dmx = max(kmix, key=lambda k: kmix[k][0])
         mixed = {"value": dtm, "unit": "s/config", "speedup_vs_fp64": (ms_step / 1000.0) / dtm,
+                 "triangles_per_s": Ne / dtm,
                  "refinement_steps": it, "relative_residual": res,
                  "rel_err_vs_fp64": {"fields": fe, "force": Fe, "torque": Ne_},
                  "kernels_ms_per_step": {k: v[0] / args.mixed_steps for k, v in kmix.items() if v[2]},
@@ -262,6 +263,7 @@ def main():
                                "1.064um; plain FP64 form: assembly + factor + solve + boundary E/H + E/H at %d "
                                "points + force/torque" % (args.level, Ne, args.points),
                    "triangles": Ne, "element_nodes": NN, "points": args.points, "precision": "fp64"},
+        "triangles_per_s": Ne / (ms_step / 1000.0),
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items() if v[2]},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -40,6 +40,32 @@ def test_zgesv_matches_numpy(n, nrhs):
     assert np.abs(LU - lu_ref).max() < 1e-9 * np.abs(lu_ref).max()
 
 
+# nrhs <= 4 takes the one-launch triangular solves (64-row blocks; 65, 200
+# and 4100 end on a partial block), nrhs = 5 the recursive ones
+@pytest.mark.parametrize("n", [64, 65, 200, 1000, 4100])
+@pytest.mark.parametrize("nrhs", [1, 3, 4, 5])
+def test_few_rhs_solve(n, nrhs):
+    import bem3_emf_b2_amd as bem
+    rng = np.random.default_rng(3 * n + nrhs)
+    A = _rand(rng, n, n) + n * 0.05 * np.eye(n)
+    B = _rand(rng, n, nrhs)
+    X = bem.zgesv(A, B)
+    want = np.linalg.solve(A, B)
+    assert np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X)) < 1e-14
+    assert np.abs(X - want).max() < 1e-10 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("n", [65, 1000, 3000])
+@pytest.mark.parametrize("nrhs", [1, 2, 4])
+def test_few_rhs_solve_fp32(n, nrhs):
+    import bem3_emf_b2_amd as bem
+    rng = np.random.default_rng(5 * n + nrhs)
+    A = _rand(rng, n, n) + n * 0.05 * np.eye(n)
+    B = _rand(rng, n, nrhs)
+    X = bem.cgesv(A, B).astype(np.complex128)
+    assert np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X)) < 1e-5
+
+
 @pytest.mark.parametrize("n", [130, 700])
 def test_lu_pivot_sequence_random(n):
     """A random matrix needs row interchanges in almost every column; the
