@@ -473,6 +473,29 @@ __global__ void __launch_bounds__(LU_NT) k_panel_col(typename Cx<R>::Z *__restri
   }
 }
 
+// Relaxed agent-scope atomic loads of values exchanged through uncached
+// memory between workgroups of one launch: no wait is inserted between them
+// (a volatile load waits for each one), so a batch costs one round trip.
+__device__ inline double ld_rel(const double *p)
+{
+  return __longlong_as_double((long long)__hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ inline int ld_rel(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// one exchanged complex value
+__device__ inline cd ld_x(const cd *p)
+{
+  const unsigned long long *q = (const unsigned long long *)p;
+  const unsigned long long re = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long im = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return zmk<cd>(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
+}
+__device__ inline cf ld_x(const cf *p)
+{
+  const unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return zmk<cf>(__int_as_float((int)(unsigned int)u), __int_as_float((int)(unsigned int)(u >> 32)));
+}
+
 // ---- one-launch panel (cooperative) ----
 // The whole panel in one cooperative launch: one 64-lane workgroup per 64
 // panel rows, the rows held in LDS (column-major), a grid barrier per
@@ -522,8 +545,10 @@ __device__ __forceinline__ void coop_barrier(unsigned int *cnt, int *err, unsign
     if (old == epoch * gsz - 1) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int target = epoch * ng;
     int spins = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
-           __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    for (;;) {   // both loads of a poll issued together
+      const unsigned int c = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (c >= target || e != 0) break;
       if (++spins > COOP_SPIN_LIMIT) { atomicExch(err, 1); break; }   // after one timeout nobody waits
       __builtin_amdgcn_s_sleep(1);
     }
@@ -563,27 +588,32 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
     if (v >= 0.0 && lane >= j && lane < kb)
       candRow[((size_t)buf * w.nbmax + blk) * 64 + lane] = S[lane * 64 + (vi - blk * 64)];
     coop_barrier(w.cnt, w.err, epoch, blk, nb);
-    // the interchange of column j-1 (published before this barrier)
-    if (j > 0) {
-      const int cj = ((volatile int *)w.ip)[j - 1];
-      if (pos == j - 1) pos = cj; else if (pos == cj) pos = j - 1;
-    }
-    // the pivot: maximum over the workgroups' candidates
+    // the interchange of column j-1 (published before this barrier) and the
+    // pivot: maximum over the workgroups' candidates, loads issued together
+    const int cj = j > 0 ? ld_rel(w.ip + j - 1) : 0;
     double b = -1.0;
     int bi = 0x7fffffff, bw = 0;
-    for (int q = lane; q < nb; q += 64) {
-      const double o = ((volatile double *)w.candV)[buf * w.nbmax + q];
-      const int oi = ((volatile int *)w.candI)[buf * w.nbmax + q];
-      if (o > b || (o == b && oi < bi)) { b = o; bi = oi; }
+    for (int q0 = 0; q0 < nb; q0 += 256) {
+      double o[4];
+      int oi[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int q = q0 + 64 * t + lane;
+        o[t] = q < nb ? ld_rel(w.candV + buf * w.nbmax + q) : -1.0;
+        oi[t] = q < nb ? ld_rel(w.candI + buf * w.nbmax + q) : 0x7fffffff;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        if (o[t] > b || (o[t] == b && oi[t] < bi)) { b = o[t]; bi = oi[t]; }
+    }
+    if (j > 0) {
+      if (pos == j - 1) pos = cj; else if (pos == cj) pos = j - 1;
     }
     wave_argmax(b, bi);
     bw = bi / 64;
     if (bi < 0 || bi >= m) { bi = m - 1; bw = bi / 64; if (blk == 0 && lane == 0 && *info == 0) *info = k0 + j + 1; }
     if (blk == 0 && lane == 0 && !(b > 0.0) && *info == 0) *info = k0 + j + 1;
-    if (lane >= j && lane < kb) {
-      const volatile R *src = (const volatile R *)(candRow + ((size_t)buf * w.nbmax + bw) * 64 + lane);
-      sp[lane] = zmk<Z>(src[0], src[1]);
-    }
+    if (lane >= j && lane < kb) sp[lane] = ld_x(candRow + ((size_t)buf * w.nbmax + bw) * 64 + lane);
     __syncthreads();
     if (r == bi) {
       used = true;
@@ -599,10 +629,10 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
   }
   coop_barrier(w.cnt, w.err, epoch, blk, nb);
   {
-    const int cj = ((volatile int *)w.ip)[kb - 1];
+    const int cj = ld_rel(w.ip + kb - 1);
     if (pos == kb - 1) pos = cj; else if (pos == cj) pos = kb - 1;
   }
-  if (blk == 0 && lane < kb) piv[k0 + lane] = k0 + ((volatile int *)w.ip)[lane];
+  if (blk == 0 && lane < kb) piv[k0 + lane] = k0 + ld_rel(w.ip + lane);
   // every row to its final position (a permutation of 0..m-1; all reads of
   // the panel happened before the first barrier)
   if (live)
@@ -816,27 +846,15 @@ __device__ inline float lane_bcast(float v, int k)
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// relaxed agent-scope loads of one exchanged complex value
-__device__ inline cd ld_x(const cd *p)
-{
-  const unsigned long long *q = (const unsigned long long *)p;
-  const unsigned long long re = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned long long im = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return zmk<cd>(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
-}
-__device__ inline cf ld_x(const cf *p)
-{
-  const unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return zmk<cf>(__int_as_float((int)(unsigned int)u), __int_as_float((int)(unsigned int)(u >> 32)));
-}
-
 // lane 0 waits for flag >= epoch (bounded)
 __device__ inline void trsv_wait(const TrsvWs &w, int j, int epoch, int lane)
 {
   if (lane == 0) {
     int spins = 0;
-    while (__hip_atomic_load(w.flag + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch &&
-           __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    for (;;) {   // both loads of a poll issued together
+      const int f = __hip_atomic_load(w.flag + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int e = __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (f >= epoch || e != 0) break;
       if (++spins > COOP_SPIN_LIMIT) { atomicExch(w.err, 1); break; }
       __builtin_amdgcn_s_sleep(1);
     }
@@ -1272,8 +1290,23 @@ static void getrs_t(int n, int nrhs, const typename Cx<R>::Z *A, size_t lda, con
   typedef typename Cx<R>::Z Z;
   if (n <= 0 || nrhs <= 0) return;
   hipLaunchKernelGGL(k_laswp<Z>, dim3((nrhs + 255) / 256), dim3(256), 0, st, B, ldb, 0, nrhs, 0, n, piv);
-  if (!trsv_sf<R, false>(A, lda, n, B, ldb, nrhs, st)) trsm_lunit_rec<R>(A, lda, n, B, ldb, nrhs, st);
-  if (!trsv_sf<R, true>(A, lda, n, B, ldb, nrhs, st)) trsm_upper_rec<R>(A, lda, n, B, ldb, nrhs, st);
+  // up to two one-launch solves of TRSV_MAXR columns each, else (and for
+  // any group the one-launch path cannot take) the recursive solves
+  const int ng = nrhs <= 2 * TRSV_MAXR ? (nrhs + TRSV_MAXR - 1) / TRSV_MAXR : 0;
+  for (int q = 0; q < 2; q++) {
+    const bool upper = q == 1;
+    int c0 = 0;
+    for (int g = 0; g < ng; g++, c0 += TRSV_MAXR) {
+      const int nc = min(TRSV_MAXR, nrhs - c0);
+      const bool ok = upper ? trsv_sf<R, true>(A, lda, n, B + c0, ldb, nc, st)
+                            : trsv_sf<R, false>(A, lda, n, B + c0, ldb, nc, st);
+      if (!ok) break;
+    }
+    if (c0 < nrhs) {
+      if (upper) trsm_upper_rec<R>(A, lda, n, B + c0, ldb, nrhs - c0, st);
+      else trsm_lunit_rec<R>(A, lda, n, B + c0, ldb, nrhs - c0, st);
+    }
+  }
 }
 
 // the error flag of the one-launch triangular solves (a wait timed out);

@@ -138,6 +138,7 @@ void orc_bil_coef_4p(zc *CC, const double *rt, int s, zc k, const orc_obj *o);
 void orc_bil_coef_9p(zc *CC, const double *rt, int s, zc k, const orc_obj *o);
 void orc_bil_coef_GLH(zc *CC, const double *rt, int s, zc k, const orc_obj *o, int high);
 void orc_bil_coef_bd(zc *CC, double zt, double et, int s, zc k, const orc_obj *o);
+void orc_bil_dcoef_bd_PV(zc *dCC, double zt, double et, int s, zc k, const orc_obj *o, int dir);
 void orc_bil_dcoef_4p(zc *CC, zc *dCC, const double *rt, const double *v, int s, zc k, const orc_obj *o);
 void orc_bil_dcoef_9p(zc *CC, zc *dCC, const double *rt, const double *v, int s, zc k, const orc_obj *o);
 void orc_bil_dcoef_GLH(zc *CC, zc *dCC, const double *rt, const double *v, int s, zc k, const orc_obj *o, int high);

@@ -3,14 +3,13 @@
  * TEST INFRASTRUCTURE ONLY (see orc.h).
  *
  * Reference element [-1,1]^2, node slots at the 2x2 Gauss points
- * (d3b1_elem.c:543-568).  The self-element principal-value derivative
- * terms of quads (bil_dcoef.c:369-740) are NOT restated: the test meshes
- * of this project are triangular, and the quad fixtures are replayed only
- * through quantities that do not need them (parity of those reference
- * functions: unpinned, not implemented).
+ * (d3b1_elem.c:543-568).
  *
  * Parity: pinned by the v1.dat fixture replays (553 quads per object):
- * fields and forces of example1.log and the stored G dE = H E identity.
+ * fields and forces of example1.log, the stored G dE = H E identity, and
+ * the stored boundary E, H of every quad node, which the reference computed
+ * from the tangential-derivative rows, i.e. from the self-element
+ * principal-value integrals of orc_bil_dcoef_bd_PV.
  */
 #include "orc.h"
 #include <math.h>
@@ -193,4 +192,149 @@ void orc_bil_coef_bd(zc *CC, double zt, double et, int seid, zc k, const orc_obj
     CC[8] = I_FP * bdC * orc_bil_bd_int(o, s, k, zt, et, 0, 2);
   }
   if (seid < 0) orc_bil_flip(CC);
+}
+
+/* ---- self-element principal-value tangential derivatives ----
+ * bil_dcoef_bd_tz0_PV / bil_dcoef_bd_te0_PV, bil_dcoef.c:369-740, with the
+ * integrands of bil_dcoef.c:916-1631.  The element is split about the pole
+ * (zt,et) into the disk D_eps (radius DEPS) and its complement.  Complement:
+ * per edge two GHN-point angular halves, GHN radial points on (DEPS, rm).
+ * D_eps: four GHN-point quarters of (-pi, pi), GLN radial points on
+ * (0, DEPS); the principal-value part: two quarters of (0, pi).  dir 0 is
+ * the t_zeta derivative, 1 the t_eta one; t is the unnormalised tangent at
+ * the pole, the result is divided by |t|. */
+typedef struct {
+  double cr[3][4], cw[3][3], vt[3], zt, et;
+  zc k;
+  int dir;
+} orc_bpv;
+
+/* K(r, theta) = dr/dr_polar of the bilinear map, and W at the point */
+static void orc_bpv_KW(const orc_bpv *b, double rr, double c, double sn, double *K, double *W, double *z, double *e)
+{
+  *z = rr * c + b->zt;
+  *e = rr * sn + b->et;
+  for (int d = 0; d < 3; d++) {
+    K[d] = b->cr[d][1] * c + b->cr[d][2] * sn + b->cr[d][3] * (rr * c * sn + b->zt * sn + b->et * c);
+    if (W) W[d] = b->cw[d][0] + b->cw[d][1] * *z + b->cw[d][2] * *e;
+  }
+}
+
+/* which 0: G slot a, 1: H slot a, 2: F.  part 0: complement of D_eps
+ * (radial integrand at rr), 1: D_eps, 2: principal-value pair at +rr / -rr */
+static zc orc_bpv_f(const orc_bpv *b, int which, int a, int part, double rr, double c, double sn)
+{
+  const double ang = b->dir == 0 ? c : sn;   /* K.t enters with cos (zeta) or sin (eta) */
+  double K[3], W[3], z, e;
+  if (part == 2) {
+    zc v[2];
+    for (int q = 0; q < 2; q++) {
+      double sr = q == 0 ? rr : -rr;
+      orc_bpv_KW(b, sr, c, sn, K, W, &z, &e);
+      double aK = orc_norm(K), KdT = orc_dot(K, b->vt);
+      zc cc = cimag(b->k) == 0.0 ? cos(creal(b->k) * aK * sr) : ccos(b->k * aK * sr);
+      if (which == 0) v[q] = cc / (aK * aK) * (KdT / aK) * orc_bil_Nn(a, z, e) * orc_norm(W);
+      else if (which == 1) v[q] = cc / pow(aK, 5) * (3.0 * KdT * ang - aK * aK) * orc_bil_Nn(a, z, e);
+      else v[q] = 1.0 / pow(aK, 5) * (3.0 * KdT * ang - aK * aK);
+    }
+    return (v[0] - v[1]) / rr;
+  }
+  orc_bpv_KW(b, rr, c, sn, K, W, &z, &e);
+  double aK = orc_norm(K), KdT = orc_dot(K, b->vt);
+  zc kR = b->k * aK * rr;
+  if (part == 0) {
+    zc ce = orc_eikr(b->k, aK * rr), cb = I * kR - 1.0;
+    if (which == 0) return cb * ce / (aK * aK) / rr * (KdT / aK) * orc_bil_Nn(a, z, e) * orc_norm(W);
+    if (which == 1) return ce / (rr * pow(aK, 5)) * ((kR * kR + 3.0 * cb) * KdT * ang - cb * aK * aK) * orc_bil_Nn(a, z, e);
+    return 1.0 / (rr * pow(aK, 5)) * (3.0 * KdT * ang - aK * aK);
+  }
+  /* part 1 */
+  zc cs = cimag(b->k) == 0.0 ? sin(creal(kR)) : csin(kR);
+  if (which == 0) {
+    zc cc = cimag(b->k) == 0.0 ? cos(creal(kR)) : ccos(kR), cb = I * kR - 1.0;
+    return (cb * cs + kR * cc) / (aK * aK) / rr * (KdT / aK) * orc_norm(W) * orc_bil_Nn(a, z, e);
+  }
+  /* H: (De0 - i De1) of bil_sdHz_De_theta */
+  zc ce = orc_eikr(b->k, aK * rr), ik = I * b->k;
+  zc d0 = ce / pow(aK, 4) * ((b->k * kR + 3.0 * ik) * KdT * ang - ik * aK * aK) * orc_bil_Nn(a, z, e);
+  zc d1 = cs / (pow(aK, 5) * rr) * (3.0 * KdT * ang - aK * aK) * orc_bil_Nn(a, z, e);
+  return d0 - I * d1;
+}
+
+/* the three integrals of one quantity: complement, D_eps, PV half disk,
+ * each including the angular factor (1 for G, sin/cos theta for H and F) */
+static void orc_bpv_int(const orc_bpv *b, int which, int a, zc *comp, zc *disk, zc *pv)
+{
+  double r[5], th[5];
+  const double eps = ORC_DEPS;
+  orc_bil_polar(r, th, b->zt, b->et);
+  *comp = *disk = *pv = 0.0;
+  for (int j = 0; j < 4; j++) {
+    double dth = th[j + 1] - th[j];
+    zc tc = 0.0;
+    for (int m = 0; m < ORC_GHN; m++) {
+      double tq[2] = {0.25 * dth * ORCQ.xhi[m] + 0.25 * (th[j + 1] + 3.0 * th[j]),
+                      0.25 * dth * ORCQ.xhi[m] + 0.25 * (3.0 * th[j + 1] + th[j])};
+      zc part = 0.0;
+      for (int q = 0; q < 2; q++) {
+        double t = tq[q];
+        double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(t - th[j]) - r[j + 1] * sin(t - th[j + 1]));
+        if (rm < 0.0) { fprintf(stderr, "oracle: integral region error\n"); exit(1); }
+        double c = cos(t), sn = sin(t), fa = which == 0 ? 1.0 : (b->dir == 0 ? sn : c);
+        zc rad = 0.0;
+        for (int ii = 0; ii < ORC_GHN; ii++)
+          rad += orc_bpv_f(b, which, a, 0, 0.5 * (rm - eps) * ORCQ.xhi[ii] + 0.5 * (rm + eps), c, sn) * ORCQ.whi[ii];
+        part += fa * rad * 0.5 * (rm - eps);
+      }
+      tc += part * ORCQ.whi[m];
+    }
+    *comp += tc * 0.25 * dth;
+  }
+  zc td = 0.0, tp = 0.0;
+  for (int m = 0; m < ORC_GHN; m++) {
+    for (int q = 0; q < 4; q++) {
+      double t = 0.25 * M_PI * ORCQ.xhi[m] + (-0.75 + 0.5 * q) * M_PI;
+      double c = cos(t), sn = sin(t), fa = which == 0 ? 1.0 : (b->dir == 0 ? sn : c);
+      zc rd = 0.0, rp = 0.0;
+      for (int ii = 0; ii < ORC_GLN; ii++) {
+        double rr = 0.5 * eps * (ORCQ.xli[ii] + 1.0);
+        if (which != 2) rd += orc_bpv_f(b, which, a, 1, rr, c, sn) * ORCQ.wli[ii];
+        if (q >= 2) rp += orc_bpv_f(b, which, a, 2, rr, c, sn) * ORCQ.wli[ii];
+      }
+      td += fa * rd * 0.5 * eps * ORCQ.whi[m];
+      tp += fa * rp * 0.5 * eps * ORCQ.whi[m];
+    }
+  }
+  *disk = td * 0.25 * M_PI;
+  *pv = tp * 0.25 * M_PI;
+}
+
+void orc_bil_dcoef_bd_PV(zc *dCC, double zt, double et, int seid, zc k, const orc_obj *o, int dir)
+{
+  int s = seid < 0 ? -seid : seid;
+  orc_bpv b;
+  for (int d = 0; d < 3; d++) {
+    for (int j = 0; j < 4; j++) b.cr[d][j] = ORC_CR(o, s, d, j);
+    for (int j = 0; j < 3; j++) b.cw[d][j] = ORC_CW(o, s, d, j);
+    b.vt[d] = dir == 0 ? b.cr[d][1] + b.cr[d][3] * et : b.cr[d][2] + b.cr[d][3] * zt;   /* bil_t_zeta / bil_t_eta */
+  }
+  b.zt = zt; b.et = et; b.k = k; b.dir = dir;
+  double iaT = 1.0 / orc_norm(b.vt);
+  zc comp, disk, pv;
+  for (int i = 0; i < 9; i++) dCC[i] = 0.0;
+  for (int a = 0; a < 4; a++) {
+    orc_bpv_int(&b, 0, a, &comp, &disk, &pv);
+    dCC[a] = (comp + I * disk - pv) * (-I_FP * iaT);
+  }
+  if (!orc_bil_plane(o, s)) {
+    double bdC = 0.0;
+    for (int d = 0; d < 3; d++) bdC += b.cr[d][1] * b.cw[d][2];
+    for (int a = 0; a < 4; a++) {
+      orc_bpv_int(&b, 1, a, &comp, &disk, &pv);
+      dCC[a + 4] = (comp + disk - pv) * (bdC * I_FP * iaT);
+    }
+    orc_bpv_int(&b, 2, 0, &comp, &disk, &pv);
+    dCC[8] = (comp + pv) * (bdC * I_FP * iaT);
+  }
+  if (seid < 0) orc_bil_flip(dCC);
 }

@@ -68,9 +68,7 @@ void orc_dcoef_rt(zc *CC, zc *dCC, const double *rt, const double *v, int s, zc 
 
 /* Coefficients of one (row node, source element) pair with two tangential
  * derivatives, d3b1_elem.c:482-539.  Returns 1 when the GHN fallback was
- * taken, 2 for the self element, 0 otherwise, and -1 when the self element
- * is a quad (its principal-value derivatives are not restated: CC is
- * filled, dCz/dCe are NaN). */
+ * taken, 2 for the self element, 0 otherwise. */
 int orc_dcoef_bd_node_t2(zc *CC, zc *dCz, zc *dCe, int t, int nid, const double *vtz, const double *vte,
                          int s, zc k, int type, const orc_obj *o)
 {
@@ -103,6 +101,7 @@ int orc_dcoef_bd_node_t2(zc *CC, zc *dCz, zc *dCe, int t, int nid, const double 
     return 2;
   }
   orc_bil_coef_bd(CC, ORCQ.zt44[nid], ORCQ.et44[nid], s, k, o);
-  for (int i = 0; i < 9; i++) { dCz[i] = NAN; dCe[i] = NAN; }
-  return -1;
+  orc_bil_dcoef_bd_PV(dCz, ORCQ.zt44[nid], ORCQ.et44[nid], s, k, o, 0);
+  orc_bil_dcoef_bd_PV(dCe, ORCQ.zt44[nid], ORCQ.et44[nid], s, k, o, 1);
+  return 2;
 }

@@ -40,10 +40,11 @@ def test_zgesv_matches_numpy(n, nrhs):
     assert np.abs(LU - lu_ref).max() < 1e-9 * np.abs(lu_ref).max()
 
 
-# nrhs <= 4 takes the one-launch triangular solves (64-row blocks; 65, 200
-# and 4100 end on a partial block), nrhs = 5 the recursive ones
+# nrhs <= 4 takes one one-launch triangular solve per factor (64-row blocks;
+# 65, 200 and 4100 end on a partial block), 5 <= nrhs <= 8 two (4 columns,
+# then the rest), nrhs = 9 the recursive solves
 @pytest.mark.parametrize("n", [64, 65, 200, 1000, 4100])
-@pytest.mark.parametrize("nrhs", [1, 3, 4, 5])
+@pytest.mark.parametrize("nrhs", [1, 3, 4, 5, 8, 9])
 def test_few_rhs_solve(n, nrhs):
     import bem3_emf_b2_amd as bem
     rng = np.random.default_rng(3 * n + nrhs)

@@ -66,10 +66,12 @@ def test_assembled_G_H_satisfy_stored_boundary_state(v1, did):
 
 
 @pytest.mark.slow
-def test_derivative_rows_reproduce_stored_E_H_on_triangles(v1):
-    """Boundary E, H of the 28 triangles per domain recomputed from the
-    tangential-derivative BIE rows (mo_solve.c:547-661), which use the
-    self-element principal-value integrals (lit_dcoef.c:460-635)."""
+def test_derivative_rows_reproduce_stored_E_H(v1):
+    """Boundary E, H of every node (28 triangles and 553 quadrangles per
+    domain) recomputed from the tangential-derivative BIE rows
+    (mo_solve.c:547-661), which use the self-element principal-value
+    integrals of triangles (lit_dcoef.c:460-635) and of quadrangles
+    (bil_dcoef.c:369-740, including the dH/dF terms of warped quads)."""
     o = v1_raw()[0]
     ob = v1[0]
     sysA = orc.System(ob, 2, factor=False)
@@ -77,13 +79,14 @@ def test_derivative_rows_reproduce_stored_E_H_on_triangles(v1):
         sysA.solve_EH()
         for did in (0, 1):
             dom = o["dom"][did]
-            tri = np.where(o["ed"][np.abs(dom["sid"][1:]), 3] == 0)[0] + 1
-            assert len(tri) == 28
-            for key in ("E", "H"):
-                got = ob.state(did, key)[tri, :3]
-                want = dom[key][tri, :3]
-                err = np.abs(got - want).max() / np.abs(want).max()
-                assert err < 1e-10, (did, key, err)
+            quad = o["ed"][np.abs(dom["sid"][1:]), 3] != 0
+            assert (~quad).sum() == 28 and quad.sum() == 553
+            for sel, nv in ((np.where(~quad)[0] + 1, 3), (np.where(quad)[0] + 1, 4)):
+                for key in ("E", "H"):
+                    got = ob.state(did, key)[sel, :nv]
+                    want = dom[key][sel, :nv]
+                    err = np.abs(got - want).max() / np.abs(want).max()
+                    assert err < 1e-12, (did, nv, key, err)
     finally:
         sysA.free()
         for d in (0, 1):  # restore the stored state for other tests
