@@ -23,19 +23,21 @@ typedef std::complex<double> zd;
 
 
 extern "C" {
-void bem_launch_asm_gl(const void *, const int *, const void *, const int *, int, int, double, double, const void *, int,
-                       void *, void *, void *, void *, size_t, double *, unsigned long long *, int, hipStream_t);
-void bem_launch_asm_gh(const void *, const int *, const void *, const int *, const void *, int, double, double,
-                       const void *, int, const void *, int, void *, void *, void *, void *, size_t, double *, hipStream_t);
-void bem_launch_asm_gl32(const void *, const int *, const void *, const int *, int, int, double, double, const void *,
-                         const void *, int, void *, void *, void *, void *, size_t, double *, unsigned long long *, int,
-                         hipStream_t);
+void bem_launch_asm_gl(const void *, const void *, const int *, const void *, const int *, int, int, double, double,
+                       const void *, int, const void *, int, void *, void *, void *, void *, size_t, double *,
+                       unsigned long long *, int, hipStream_t);
+void bem_launch_asm_gh(const void *, const void *, const int *, const void *, const int *, const void *, int, double,
+                       double, const void *, int, const void *, int, const void *, int, const void *, int, void *, void *,
+                       void *, void *, size_t, double *, hipStream_t);
+void bem_launch_asm_gl32(const void *, const void *, const int *, const void *, const int *, int, int, double, double,
+                         const void *, const void *, int, const void *, int, void *, void *, void *, void *, size_t,
+                         double *, unsigned long long *, int, hipStream_t);
 int bem_rule32(const void *, int, void **);
-void bem_launch_asm_self(const void *, const int *, const void *, const int *, int, double, double, const double *,
-                         const double *, const double *, const double *, void *, void *, void *, void *, size_t,
-                         hipStream_t);
-void bem_launch_asm_fin(const double *, int, const double *, const int *, int, int, void *, size_t, double *,
-                        hipStream_t);
+void bem_launch_asm_self(const void *, const void *, const int *, const void *, const int *, int, double, double,
+                         const double *, const double *, const double *, const double *, void *, void *, void *, void *,
+                         size_t, double *, hipStream_t);
+void bem_launch_asm_fin(const double *, int, const double *, const int *, const double *, int, int, void *, size_t,
+                        double *, hipStream_t);
 void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
                hipStream_t);
 int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
@@ -419,9 +421,12 @@ extern "C" bh_object *bh_object_create(int Nn, const double *rn, int Ne, const i
     }
   }
 
+  // node numbers sequential per element: 3 for a triangle, 4 for a
+  // quadrangle (setup2.c:372-383)
   o->eni.assign(4 * (size_t)(Ne + 1), -1);
   int nc = 0;
-  for (int i = 1; i <= Ne; i++) for (int j = 0; j < 3; j++) o->eni[4 * i + j] = nc++;
+  for (int i = 1; i <= Ne; i++)
+    for (int j = 0; j < (o->ed[4 * i + 3] != 0 ? 4 : 3); j++) o->eni[4 * i + j] = nc++;
   o->NN = nc;
   o->n.assign(MN + 1, 0.0);
   o->kn.assign(MN + 1, 0.0);
@@ -479,6 +484,14 @@ extern "C" int bh_object_geometry(const bh_object *o, double *cr, double *cw, do
   return 0;
 }
 
+// rule tables (QPtN on the triangle, QPtQ on the square), built at first use
+static QPtN *g_rule[4] = {nullptr, nullptr, nullptr, nullptr};
+static int g_nrule[4] = {0, 0, 0, 0};
+static double g_rscale[4];
+static QPtQ *g_qrule[4] = {nullptr, nullptr, nullptr, nullptr};   // the same rules on the square (quads)
+static int g_nqrule[4] = {0, 0, 0, 0};
+static int ensure_rules();
+
 // ------------------------------------------------------------------ assembly
 static int assemble_domain(bh_object *o, int d)
 {
@@ -499,11 +512,11 @@ static int assemble_domain(bh_object *o, int d)
   double kr = o->kn[d].real(), ki = o->kn[d].imag();
   const bool mixed = o->prec == BH_PREC_MIXED;
   if (mixed)
-    bem_launch_asm_gl32(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl32, g.d_gl, g.n_gl, D.d_G, D.d_H,
-                        D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+    bem_launch_asm_gl32(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl32, g.d_gl, g.n_gl,
+                        g_qrule[2], g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
   else
-    bem_launch_asm_gl(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, D.d_G, D.d_H,
-                      D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+    bem_launch_asm_gl(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, g_qrule[2],
+                      g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
   HIPCHK(hipGetLastError());
   // near pairs in row-major order (deterministic)
   std::vector<unsigned long long> mask(nd * nwv);
@@ -529,14 +542,17 @@ static int assemble_domain(bh_object *o, int d)
   double *d_ghdf = nullptr;
   if (upload(&d_pairs, pairs.data(), pairs.size()) || upload(&d_start, start.data(), start.size())) return BH_EHIP;
   HIPCHK(hipMalloc((void **)&d_ghdf, std::max<size_t>(pairs.size(), 1) * 3 * sizeof(double)));
-  bem_launch_asm_gh(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh,
-                    mixed ? (const void *)g.d_gl : nullptr, g.n_gl, D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_ghdf, g.st);
-  bem_launch_asm_self(o->d_geo, D.d_sid, D.d_rows, D.d_colOff, (int)nd, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_G,
-                      D.d_H, D.d_dG, D.d_dH, nd, g.st);
-  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, (int)nd, d == 0, D.d_H, nd, D.d_dF, g.st);
+  double *d_self = nullptr;   // the self element's F terms per row
+  HIPCHK(hipMalloc((void **)&d_self, nd * 3 * sizeof(double)));
+  bem_launch_asm_gh(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh,
+                    mixed ? (const void *)g.d_gl : nullptr, g.n_gl, g_qrule[3], g_nqrule[3], g_qrule[2], g_nqrule[2],
+                    D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_ghdf, g.st);
+  bem_launch_asm_self(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, (int)nd, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh,
+                      D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_self, g.st);
+  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, d_self, (int)nd, d == 0, D.d_H, nd, D.d_dF, g.st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g.st));
-  dfree(d_part); dfree(d_mask); dfree(d_pairs); dfree(d_start); dfree(d_ghdf);
+  dfree(d_part); dfree(d_mask); dfree(d_pairs); dfree(d_start); dfree(d_ghdf); dfree(d_self);
   return 0;
 }
 
@@ -544,9 +560,10 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
 {
   API_LOCK;
   if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
-  if (!o->all_tri)
-    return fail(BH_EUNSUP, "coefficient matrices of bi-linear (quad) elements are not implemented on the GPU "
-                           "(fields, force and absorbed power of a stored quad-mesh state are)");
+  {
+    int rc = ensure_rules();   // the square's rules for quadrangles
+    if (rc) return fail(rc, "rule tables");
+  }
   // every factorisation of earlier matrices is stale from here on
   o->fmode = 0;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
@@ -691,12 +708,12 @@ void bem_launch_domain(const double *, int, const double *, const int *, const i
 void bem_launch_field(const double *, const int *, int, int, const void *, const int *, int, double, double, const void *,
                       int, double, const void *, const void *, int, int, const void *, const void *, const void *,
                       const void *, void *, int, hipStream_t);
-void bem_launch_field_memp(const double *, const int *, int, int, const void *, const int *, int, double, double,
-                           const void *, int, double, int, const void *, int, double, const void *, const void *,
-                           double, void *, hipStream_t);
-void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const int *, int, double, double,
-                        const void *, int, const void *, int, double, const void *, const void *, void *, void *,
-                        hipStream_t);
+void bem_launch_field_memp(const double *, const int *, int, int, const void *, const void *, const int *, int, double,
+                           double, const void *, int, double, int, const void *, int, double, const void *, int,
+                           const void *, int, const void *, const void *, double, void *, hipStream_t);
+void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const void *, const int *, int,
+                        double, double, const void *, int, const void *, int, double, const void *, int, const void *,
+                        int, const void *, const void *, void *, void *, hipStream_t);
 void bem_launch_field_bd(const int *, const double *, int, int, const void *, const void *, const int *, int, double,
                          double, const void *, const void *, int, double, const void *, int, const double *,
                          const double *, const double *, const double *, const void *, const void *, const void *,
@@ -709,12 +726,6 @@ void bem_launch_stress(const void *, const double *, const double *, const doubl
 void bem_launch_poynting(const void *, const double *, const double *, int, double *, hipStream_t);
 }
 
-// rule tables for field evaluation (QPtN), built at first use
-static QPtN *g_rule[4] = {nullptr, nullptr, nullptr, nullptr};
-static int g_nrule[4] = {0, 0, 0, 0};
-static double g_rscale[4];
-static QPtQ *g_qrule[4] = {nullptr, nullptr, nullptr, nullptr};   // the same rules on the square (quads)
-static int g_nqrule[4] = {0, 0, 0, 0};
 
 static int ensure_rules()
 {
@@ -1280,8 +1291,6 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
   for (int m = 0; m < N; m++) {
     if (!objs[m]->d_beams) return fail(BH_ESTATE, "object %d has no beams", m);
     if (mode != 2 && !objs[m]->dom[0].d_U) return fail(BH_ESTATE, "object %d is not solved", m);
-    if (what == 2 && mode != 2 && !objs[m]->all_tri)
-      return fail(BH_EUNSUP, "E, H from potential gradients: bi-linear (quad) elements are not implemented");
   }
   int nc = what == 1 ? 4 : 6;
   double *d_pts;
@@ -1313,9 +1322,10 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
         Dom &D = o->dom[d];
         if (what == 2) {
           // E, H from the potentials' gradients (mo_EH_mEMP_s/t, mo_field.c:154-207)
-          bem_launch_field_memp(d_pts, d_sel, np, d, o->d_geo, D.d_sid, D.dNe, o->kn[d].real(), o->kn[d].imag(),
-                                g_rule[type], g_nrule[type], g_rscale[type], type < 3, g_rule[3], g_nrule[3],
-                                g_rscale[3], D.d_U, D.d_dU, o->lambda0, d_out, g.st);
+          bem_launch_field_memp(d_pts, d_sel, np, d, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[d].real(),
+                                o->kn[d].imag(), g_rule[type], g_nrule[type], g_rscale[type], type < 3, g_rule[3],
+                                g_nrule[3], g_rscale[3], g_qrule[type], g_nqrule[type], g_qrule[3], g_nqrule[3], D.d_U,
+                                D.d_dU, o->lambda0, d_out, g.st);
           continue;
         }
         const void *A0 = what == 0 ? (const void *)D.d_E : (const void *)D.d_U;
@@ -1595,9 +1605,9 @@ static int add_scattered(bh_object *dst, bh_object *src, int *d_did)
   for (int v : did)
     if (v != 0) return fail(BH_EDOM, "scattered-field domain error (overlapping objects), domain id %d", v);
   Dom &S = src->dom[0];
-  bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, S.d_sid, S.dNe,
+  bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, src->d_gq, S.d_sid, S.dNe,
                      src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
-                     S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, g.st);
+                     g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, g.st);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -1,4 +1,5 @@
-// bem_asm.hip -- coefficient-matrix assembly for linear-triangle meshes.
+// bem_asm.hip -- coefficient-matrix assembly for meshes of linear triangles
+// and bi-linear quadrangles.
 //
 // For one domain d (element list sid_d, wave number k_d) and every pair
 // (row node slot, source element) the kernels produce the 18 coefficients of
@@ -19,6 +20,11 @@
 //                       (lit_dcoef_bd_tz0/te0_PV), one wavefront per row.
 // Pass 4 (k_asm_fin)  : per-row F sums in a fixed order, the 1+F / F
 //                       diagonal term, dF rows.
+//
+// A quadrangle source element takes the square's rules instead (12x12 and
+// 64x64 tensor Gauss-Legendre, bil_dcoef_v2_GL/GH, bil_dcoef_v2.c:142-326)
+// with four shape functions, and a quadrangle self element the polar rule
+// about the node with the principal-value derivatives of bil_dcoef.c:369-740.
 //
 // Matrices are compact: row = node slot index in the domain's element order,
 // column = shape-function index in the same order (row-major, leading
@@ -114,11 +120,133 @@ __device__ __forceinline__ void acc_store(const AsmAcc &a, size_t row, size_t co
   }
 }
 
+// ---- quadrangle source elements ----
+struct AsmAccQ {
+  cd G[4], H[4], DG0[4], DG1[4], DH0[4], DH1[4];
+  double F, DF0, DF1;
+};
+
+__device__ __forceinline__ void accq_zero(AsmAccQ &a)
+{
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    a.G[i] = mk(0, 0); a.H[i] = mk(0, 0); a.DG0[i] = mk(0, 0); a.DG1[i] = mk(0, 0); a.DH0[i] = mk(0, 0); a.DH1[i] = mk(0, 0);
+  }
+  a.F = a.DF0 = a.DF1 = 0.0;
+}
+
+// integrand of bil_dcoef_v2_GL/GH (bil_dcoef_v2.c:142-326) at one point of
+// the square: r and the unnormalised normal W vary over the element, |W| is
+// the area factor of G and dG
+__device__ __forceinline__ void asmq_point(AsmAccQ &a, const TriGeo &g, const QuadExt &qx, const RowPt &rp, double kr,
+                                           double ki, const QPtQ &q)
+{
+  double vx, vy, vz, W[3];
+  {
+    const double z = q.z, e = q.e, ze = z * e;
+    vx = g.c0[0] + g.c1[0] * z + g.c2[0] * e + qx.c3[0] * ze - rp.rt[0];
+    vy = g.c0[1] + g.c1[1] * z + g.c2[1] * e + qx.c3[1] * ze - rp.rt[1];
+    vz = g.c0[2] + g.c1[2] * z + g.c2[2] * e + qx.c3[2] * ze - rp.rt[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) W[d] = g.W[d] + qx.W1[d] * z + qx.W2[d] * e;
+  }
+  const double R = sqrt(vx * vx + vy * vy + vz * vz);
+  const double iR = 1.0 / R, iR2 = iR * iR, iR3 = iR2 * iR;
+  const double aW = sqrt(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
+  const double rdW = (vx * W[0] + vy * W[1] + vz * W[2]) * iR;
+  const double rd0 = (vx * rp.v0[0] + vy * rp.v0[1] + vz * rp.v0[2]) * iR;
+  const double rd1 = (vx * rp.v1[0] + vy * rp.v1[1] + vz * rp.v1[2]) * iR;
+  const double Wdv0 = W[0] * rp.v0[0] + W[1] * rp.v0[1] + W[2] * rp.v0[2];
+  const double Wdv1 = W[0] * rp.v1[0] + W[1] * rp.v1[1] + W[2] * rp.v1[2];
+  const cd ce = eikr(kr, ki, R);
+  const cd cb = mk(-1.0 - ki * R, kr * R);   // i k R - 1
+  const cd cbce = cb * ce;
+  const cd gk = ce * (iR * aW);
+  const cd hk = cbce * (rdW * iR2);
+  const cd tk = cbce * (iR2 * aW);
+  const cd kR = mk(kr * R, ki * R);
+  const cd qq = kR * kR + cb * 3.0;
+  const cd ce3 = ce * iR3;
+  const cd dh0 = ce3 * (qq * (rd0 * rdW) - cb * Wdv0);
+  const cd dh1 = ce3 * (qq * (rd1 * rdW) - cb * Wdv1);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double wn = q.w * q.n[i];
+    fma_cr(a.G[i], gk, wn);
+    fma_cr(a.H[i], hk, wn);
+    fma_cr(a.DG0[i], tk, wn * rd0);
+    fma_cr(a.DG1[i], tk, wn * rd1);
+    fma_cr(a.DH0[i], dh0, wn);
+    fma_cr(a.DH1[i], dh1, wn);
+  }
+  a.F = fma(q.w, rdW * iR2, a.F);
+  a.DF0 = fma(q.w, iR3 * (3.0 * rd0 * rdW - Wdv0), a.DF0);
+  a.DF1 = fma(q.w, iR3 * (3.0 * rd1 * rdW - Wdv1), a.DF1);
+}
+
+// scale 1/(4 pi), -1/(4 pi) for dG, and the sign change of H, F and their
+// derivatives on the sub-domain side (bil_convert_CC / bil_convert_dCC)
+__device__ __forceinline__ void accq_finish(AsmAccQ &a, bool flip)
+{
+  const double sh = flip ? -BEM_I_FP : BEM_I_FP;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    a.G[i] = a.G[i] * BEM_I_FP;
+    a.H[i] = a.H[i] * sh;
+    a.DG0[i] = a.DG0[i] * (-BEM_I_FP);
+    a.DG1[i] = a.DG1[i] * (-BEM_I_FP);
+    a.DH0[i] = a.DH0[i] * sh;
+    a.DH1[i] = a.DH1[i] * sh;
+  }
+  a.F *= sh; a.DF0 *= sh; a.DF1 *= sh;
+}
+
+__device__ __forceinline__ void accq_store(const AsmAccQ &a, size_t row, size_t col, size_t ld, cd *G, cd *H, cd *dG, cd *dH)
+{
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    G[row * ld + col + i] = a.G[i];
+    H[row * ld + col + i] = a.H[i];
+    dG[(row * 2 + 0) * ld + col + i] = a.DG0[i];
+    dG[(row * 2 + 1) * ld + col + i] = a.DG1[i];
+    dH[(row * 2 + 0) * ld + col + i] = a.DH0[i];
+    dH[(row * 2 + 1) * ld + col + i] = a.DH1[i];
+  }
+}
+
+__device__ __forceinline__ void accq_wsum(AsmAccQ &a)
+{
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    a.G[i] = wsum(a.G[i]); a.H[i] = wsum(a.H[i]); a.DG0[i] = wsum(a.DG0[i]); a.DG1[i] = wsum(a.DG1[i]);
+    a.DH0[i] = wsum(a.DH0[i]); a.DH1[i] = wsum(a.DH1[i]);
+  }
+  a.F = wsum(a.F); a.DF0 = wsum(a.DF0); a.DF1 = wsum(a.DF1);
+}
+
+// one quadrangle pair of pass 1 (the lane's whole rule): the GHN test and,
+// when the pair stays, its store and its F sums
+__device__ __forceinline__ bool asm_gl_quad(const TriGeo &g, const QuadExt &qx, const RowPt &rp, int sd, double kr,
+                                            double ki, const QPtQ *__restrict__ qrule, int nqq, double lim, size_t row,
+                                            size_t col, size_t ld, cd *G, cd *H, cd *dG, cd *dH, double &f, double &df0,
+                                            double &df1)
+{
+  AsmAccQ a;
+  accq_zero(a);
+  for (int p = 0; p < nqq; p++) asmq_point(a, g, qx, rp, kr, ki, qrule[p]);
+  accq_finish(a, sd < 0);
+  if (fabs(a.DF0) > lim || fabs(a.DF1) > lim) return true;
+  accq_store(a, row, col, ld, G, H, dG, dH);
+  f = a.F; df0 = a.DF0; df1 = a.DF1;
+  return false;
+}
+
 // ---- pass 1 ----
-__global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
-                                                const RowPt *__restrict__ rows, const int *__restrict__ colOff, int dNe,
-                                                double kr, double ki, const QPt *__restrict__ rule, int nq,
-                                                cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
+__global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                const int *__restrict__ sid, const RowPt *__restrict__ rows,
+                                                const int *__restrict__ colOff, int dNe, double kr, double ki,
+                                                const QPt *__restrict__ rule, int nq, const QPtQ *__restrict__ qrule,
+                                                int nqq, cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
                                                 cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
                                                 unsigned long long *__restrict__ ghMask)
 {
@@ -135,19 +263,24 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
     int sd = sid[s];
     int as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
-    double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
-    double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
-    AsmAcc a;
-    acc_zero(a);
-    for (int p = 0; p < nq; p++) {
-      const QPt qp = rule[p];
-      asm_point(a, g, rp, Wdv0, Wdv1, kr, ki, qp.z, qp.e, qp.w);
-    }
-    acc_finish(a, g.aW, sd < 0);
-    flagged = fabs(a.DF0) > BEM_CBD_SDF || fabs(a.DF1) > BEM_CBD_SDF;
-    if (!flagged) {
-      acc_store(a, row, colOff[s], ld, G, H, dG, dH);
-      f = a.F; df0 = a.DF0; df1 = a.DF1;
+    if (g.quad != 0.0) {
+      flagged = asm_gl_quad(g, gq[as], rp, sd, kr, ki, qrule, nqq, BEM_CBD_SDF, row, colOff[s], ld, G, H, dG, dH, f,
+                            df0, df1);
+    } else {
+      double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
+      double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
+      AsmAcc a;
+      acc_zero(a);
+      for (int p = 0; p < nq; p++) {
+        const QPt qp = rule[p];
+        asm_point(a, g, rp, Wdv0, Wdv1, kr, ki, qp.z, qp.e, qp.w);
+      }
+      acc_finish(a, g.aW, sd < 0);
+      flagged = fabs(a.DF0) > BEM_CBD_SDF || fabs(a.DF1) > BEM_CBD_SDF;
+      if (!flagged) {
+        acc_store(a, row, colOff[s], ld, G, H, dG, dH);
+        f = a.F; df0 = a.DF0; df1 = a.DF1;
+      }
     }
   }
   unsigned long long m = __ballot(flagged);
@@ -179,10 +312,11 @@ struct Acc32 {
 __device__ __forceinline__ void fma_cf(cf &a, cf b, float s) { a.x = fmaf(b.x, s, a.x); a.y = fmaf(b.y, s, a.y); }
 __device__ __forceinline__ void add_cf(cd &a, cf b) { a.x += (double)b.x; a.y += (double)b.y; }
 
-__global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
-                                                  const RowPt *__restrict__ rows, const int *__restrict__ colOff,
-                                                  int dNe, double kr, double ki, const QPt32 *__restrict__ rule,
-                                                  const QPt *__restrict__ rule64, int nq,
+__global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                  const int *__restrict__ sid, const RowPt *__restrict__ rows,
+                                                  const int *__restrict__ colOff, int dNe, double kr, double ki,
+                                                  const QPt32 *__restrict__ rule, const QPt *__restrict__ rule64, int nq,
+                                                  const QPtQ *__restrict__ qrule, int nqq,
                                                   cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
                                                   cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
                                                   unsigned long long *__restrict__ ghMask)
@@ -196,7 +330,12 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
   bool active = s <= dNe && s != rp.t;
   bool flagged = false;
   double f = 0.0, df0 = 0.0, df1 = 0.0;
-  if (active) {
+  if (active && geo[sid[s] < 0 ? -sid[s] : sid[s]].quad != 0.0) {
+    // quadrangles take the FP64 path (no FP32 form of the square's rule)
+    const int sd = sid[s], as = sd < 0 ? -sd : sd;
+    flagged = asm_gl_quad(geo[as], gq[as], rp, sd, kr, ki, qrule, nqq, BEM_CBD_SDF * (1.0 - 1e-4), row, colOff[s], ld, G,
+                          H, dG, dH, f, df0, df1);
+  } else if (active) {
     int sd = sid[s];
     int as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
@@ -297,12 +436,45 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
 }
 
 // ---- pass 2: flagged pairs with the 64x64 rule ----
-__global__ void __launch_bounds__(256) k_asm_gh(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
-                                                const RowPt *__restrict__ rows, const int *__restrict__ colOff,
-                                                const GhPair *__restrict__ pairs, int npairs, double kr, double ki,
-                                                const QPt *__restrict__ rule, int nq, const QPt *__restrict__ gl,
-                                                int ngl, cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
-                                                cd *__restrict__ dH, size_t ld, double *__restrict__ ghDF)
+// a flagged quadrangle pair: the FP64 12x12 rule first when the pair comes
+// from the FP32 pass (qgl != nullptr), then the 64x64 rule
+__device__ __forceinline__ void asm_gh_quad(const TriGeo &g, const QuadExt &qx, const RowPt &rp, int sd, double kr,
+                                            double ki, const QPtQ *__restrict__ qgh, int nqgh,
+                                            const QPtQ *__restrict__ qgl, int nqgl, int lane, size_t row, size_t col,
+                                            size_t ld, cd *G, cd *H, cd *dG, cd *dH, double *df)
+{
+  AsmAccQ a;
+  if (qgl) {
+    accq_zero(a);
+    for (int p = lane; p < nqgl; p += 64) asmq_point(a, g, qx, rp, kr, ki, qgl[p]);
+    accq_wsum(a);
+    accq_finish(a, sd < 0);
+    if (!(fabs(a.DF0) > BEM_CBD_SDF || fabs(a.DF1) > BEM_CBD_SDF)) {
+      if (lane == 0) {
+        accq_store(a, row, col, ld, G, H, dG, dH);
+        df[0] = a.F; df[1] = a.DF0; df[2] = a.DF1;
+      }
+      return;
+    }
+  }
+  accq_zero(a);
+  for (int p = lane; p < nqgh; p += 64) asmq_point(a, g, qx, rp, kr, ki, qgh[p]);
+  accq_wsum(a);
+  accq_finish(a, sd < 0);
+  if (lane == 0) {
+    accq_store(a, row, col, ld, G, H, dG, dH);
+    df[0] = a.F; df[1] = a.DF0; df[2] = a.DF1;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_asm_gh(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                const int *__restrict__ sid, const RowPt *__restrict__ rows,
+                                                const int *__restrict__ colOff, const GhPair *__restrict__ pairs,
+                                                int npairs, double kr, double ki, const QPt *__restrict__ rule, int nq,
+                                                const QPt *__restrict__ gl, int ngl, const QPtQ *__restrict__ qgh,
+                                                int nqgh, const QPtQ *__restrict__ qgl, int nqgl, cd *__restrict__ G,
+                                                cd *__restrict__ H, cd *__restrict__ dG, cd *__restrict__ dH, size_t ld,
+                                                double *__restrict__ ghDF)
 {
   const int pid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -312,6 +484,11 @@ __global__ void __launch_bounds__(256) k_asm_gh(const TriGeo *__restrict__ geo, 
   int sd = sid[pr.s];
   int as = sd < 0 ? -sd : sd;
   const TriGeo g = geo[as];
+  if (g.quad != 0.0) {
+    asm_gh_quad(g, gq[as], rp, sd, kr, ki, qgh, nqgh, gl ? qgl : nullptr, nqgl, lane, pr.row, colOff[pr.s], ld, G, H, dG,
+                dH, ghDF + (size_t)pid * 3);
+    return;
+  }
   double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
   double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
   AsmAcc a;
@@ -379,12 +556,215 @@ __device__ __forceinline__ void csincos(cd a, cd &sn, cd &cs)
   cs = mk(c * ch, -s * sh);
 }
 
-__global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo, const int *__restrict__ sid,
-                                                  const RowPt *__restrict__ rows, const int *__restrict__ colOff,
-                                                  int nrows, double kr, double ki, const double *__restrict__ xl,
-                                                  const double *__restrict__ wl, const double *__restrict__ xh,
-                                                  const double *__restrict__ wh, cd *__restrict__ G,
-                                                  cd *__restrict__ H, cd *__restrict__ dG, cd *__restrict__ dH, size_t ld)
+// ---- self quadrangle ----
+// node slot positions on the square (d3b1_const.h:50, the 2x2 Gauss points)
+__constant__ double c_quad_zt[4] = {-0.5773502691896257645091, 0.5773502691896257645091, 0.5773502691896257645091,
+                                    -0.5773502691896257645091};
+__constant__ double c_quad_et[4] = {-0.5773502691896257645091, -0.5773502691896257645091, 0.5773502691896257645091,
+                                    0.5773502691896257645091};
+
+// the map at polar (rr, c, sn) about (zt, et): K = dr/drr, W, |K|, |W|,
+// K.t0, K.t1 and the four shape functions
+struct QPol {
+  double aK, aW, Kt0, Kt1, n[4];
+};
+__device__ __forceinline__ QPol qpol(const TriGeo &g, const QuadExt &qx, const double *t0, const double *t1, double zt,
+                                     double et, double rr, double c, double sn)
+{
+  QPol p;
+  const double z = rr * c + zt, e = rr * sn + et, cs = rr * c * sn + zt * sn + et * c;
+  double K[3], W[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    K[d] = g.c1[d] * c + g.c2[d] * sn + qx.c3[d] * cs;
+    W[d] = g.W[d] + qx.W1[d] * z + qx.W2[d] * e;
+  }
+  p.aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]);
+  p.aW = sqrt(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
+  p.Kt0 = K[0] * t0[0] + K[1] * t0[1] + K[2] * t0[2];
+  p.Kt1 = K[0] * t1[0] + K[1] * t1[1] + K[2] * t1[2];
+  const double ia = 1.7320508075688772935;   // 1 / (1/sqrt(3))
+  p.n[0] = 0.25 * (1.0 - z * ia) * (1.0 - e * ia);
+  p.n[1] = 0.25 * (1.0 + z * ia) * (1.0 - e * ia);
+  p.n[2] = 0.25 * (1.0 + z * ia) * (1.0 + e * ia);
+  p.n[3] = 0.25 * (1.0 - z * ia) * (1.0 + e * ia);
+  return p;
+}
+
+// Self quadrangle of one row, one wavefront: G (and H, F on a warped
+// quadrangle) by the polar rule of bil_coef_bd (bil_coef.c:244-342), and the
+// tangential derivatives of bil_dcoef_bd_tz0_PV / te0_PV (bil_dcoef.c:369-740
+// with the integrands of :916-1631): the complement of the disk D_eps (4
+// edges x 2 x 64 angles, 64 radial points from DEPS to the edge), the disk
+// (4 quarters x 64 angles, 12 radial points) and its principal-value half
+// (pairs at +r and -r over the upper two quarters).  Direction 0 is t_zeta,
+// 1 is t_eta (unnormalised tangents at the node; results divided by their
+// lengths).  H, F and their derivatives carry the factor c1 . W2 and vanish
+// on a plane quadrangle (bil_check_plane).
+__device__ void asm_self_quad(const TriGeo &g, const QuadExt &qx, const RowPt &rp, int td, int row, size_t col,
+                              double kr, double ki, const double *__restrict__ xl, const double *__restrict__ wl,
+                              const double *__restrict__ xh, const double *__restrict__ wh, int lane, cd *G, cd *H,
+                              cd *dG, cd *dH, size_t ld, double *selfDF)
+{
+  const double zt = c_quad_zt[rp.slot], et = c_quad_et[rp.slot];
+  double zp = 1.0 - zt, zm = -1.0 - zt, ep = 1.0 - et, em = -1.0 - et;
+  double r[5] = {sqrt(zm * zm + em * em), sqrt(zp * zp + em * em), sqrt(zp * zp + ep * ep), sqrt(zm * zm + ep * ep), 0.0};
+  double th[5] = {atan2(em, zm), atan2(em, zp), atan2(ep, zp), atan2(ep, zm), 0.0};
+  r[4] = r[0];
+  th[4] = th[0] + 2.0 * M_PI;
+  double t0[3], t1[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) { t0[d] = g.c1[d] + qx.c3[d] * et; t1[d] = g.c2[d] + qx.c3[d] * zt; }
+  const double iaT0 = 1.0 / sqrt(t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2]);
+  const double iaT1 = 1.0 / sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+  const double bdC = g.c1[0] * qx.W2[0] + g.c1[1] * qx.W2[1] + g.c1[2] * qx.W2[2];
+  const double ab = sqrt(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2]);
+  const double aC = sqrt(qx.W2[0] * qx.W2[0] + qx.W2[1] * qx.W2[1] + qx.W2[2] * qx.W2[2]);
+  const bool warped = !(fabs(bdC / (ab * aC)) < BEM_GEPS);   // bil_check_plane (0/0 counts as warped)
+  const double eps = BEM_DEPS;
+  const cd kk = mk(kr, ki), ik = mk(-ki, kr);
+  cd sg[4], sh[4], g0[4], g1[4], h0[4], h1[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) { sg[a] = sh[a] = g0[a] = g1[a] = h0[a] = h1[a] = mk(0, 0); }
+  double sf = 0.0, f0 = 0.0, f1 = 0.0;
+  // complement of D_eps, and the G (H, F) integrals over the whole element
+  for (int ia = lane; ia < 512; ia += 64) {
+    const int j = ia >> 7, m = (ia & 127) >> 1, half = ia & 1;
+    const double dth = th[j + 1] - th[j];
+    const double t = 0.25 * dth * xh[m] + (half ? 0.25 * (3.0 * th[j + 1] + th[j]) : 0.25 * (th[j + 1] + 3.0 * th[j]));
+    const double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(t - th[j]) - r[j + 1] * sin(t - th[j + 1]));
+    double c, sn;
+    sincos(t, &sn, &c);
+    const double wa = wh[m] * 0.25 * dth;
+    // coefficients: radial GLN rule on (0, rm)
+    for (int i = 0; i < 12; i++) {
+      const double rr = 0.5 * rm * (xl[i] + 1.0);
+      const QPol p = qpol(g, qx, t0, t1, zt, et, rr, c, sn);
+      const double w = wl[i] * wa * 0.5 * rm, R = p.aK * rr;
+      const cd ce = eikr(kr, ki, R);
+      const cd vg = ce * (p.aW / p.aK * w);
+#pragma unroll
+      for (int a = 0; a < 4; a++) sg[a] = sg[a] + vg * p.n[a];
+      if (warped) {
+        const double w3 = w * c * sn / (p.aK * p.aK * p.aK);
+        const cd vh = (mk(-1.0 - ki * R, kr * R) * ce) * w3;
+#pragma unroll
+        for (int a = 0; a < 4; a++) sh[a] = sh[a] + vh * p.n[a];
+        sf += w3;
+      }
+    }
+    // derivatives: radial GHN rule on (eps, rm)
+    for (int i = 0; i < 64; i++) {
+      const double rr = 0.5 * (rm - eps) * xh[i] + 0.5 * (rm + eps);
+      const QPol p = qpol(g, qx, t0, t1, zt, et, rr, c, sn);
+      const double w = wh[i] * wa * 0.5 * (rm - eps), R = p.aK * rr;
+      const double aK2 = p.aK * p.aK, iK5r = 1.0 / (aK2 * aK2 * p.aK * rr);
+      const cd ce = eikr(kr, ki, R);
+      const cd cb = mk(-1.0 - ki * R, kr * R);
+      const cd vg = (cb * ce) * (p.aW / (aK2 * p.aK * rr) * w);
+#pragma unroll
+      for (int a = 0; a < 4; a++) { g0[a] = g0[a] + vg * (p.Kt0 * p.n[a]); g1[a] = g1[a] + vg * (p.Kt1 * p.n[a]); }
+      if (warped) {
+        const cd kR = mk(kr * R, ki * R);
+        const cd q3 = kR * kR + cb * 3.0;
+        const cd e0 = (ce * (iK5r * w * sn)) * (q3 * (p.Kt0 * c) - cb * aK2);
+        const cd e1 = (ce * (iK5r * w * c)) * (q3 * (p.Kt1 * sn) - cb * aK2);
+#pragma unroll
+        for (int a = 0; a < 4; a++) { h0[a] = h0[a] + e0 * p.n[a]; h1[a] = h1[a] + e1 * p.n[a]; }
+        f0 += iK5r * w * sn * (3.0 * p.Kt0 * c - aK2);
+        f1 += iK5r * w * c * (3.0 * p.Kt1 * sn - aK2);
+      }
+    }
+  }
+  // D_eps (x i for G) and its principal-value half (subtracted from G and
+  // H, added to F)
+  for (int ia = lane; ia < 256; ia += 64) {
+    const int q = ia >> 6, m = ia & 63;
+    const double t = 0.25 * M_PI * xh[m] + (-0.75 + 0.5 * q) * M_PI;
+    double c, sn;
+    sincos(t, &sn, &c);
+    const double wa = wh[m] * 0.25 * M_PI * 0.5 * eps;
+    for (int i = 0; i < 12; i++) {
+      const double rr = 0.5 * eps * (xl[i] + 1.0), w = wl[i] * wa;
+      const QPol p = qpol(g, qx, t0, t1, zt, et, rr, c, sn);
+      const double R = p.aK * rr, aK2 = p.aK * p.aK;
+      const cd kR = mk(kr * R, ki * R);
+      cd skr, ckr;
+      csincos(kR, skr, ckr);
+      const cd cb = mk(-1.0 - ki * R, kr * R);
+      const cd vg = mk(0, 1) * ((cb * skr + kR * ckr) * (p.aW / (aK2 * p.aK * rr) * w));
+#pragma unroll
+      for (int a = 0; a < 4; a++) { g0[a] = g0[a] + vg * (p.Kt0 * p.n[a]); g1[a] = g1[a] + vg * (p.Kt1 * p.n[a]); }
+      if (warped) {
+        // sdH_De: (De0 - i De1) with De0 = e^{ikR}/aK^4 ((k kR + 3ik) K.t ang - ik aK^2),
+        // De1 = sin(kR)/(aK^5 r) (3 K.t ang - aK^2)
+        const cd ce = eikr(kr, ki, R);
+        const cd kkR = kk * kR + ik * 3.0;
+        const double iK4 = 1.0 / (aK2 * aK2), iK5r = iK4 / (p.aK * rr);
+        const cd e0 = (ce * (iK4 * w * sn)) * (kkR * (p.Kt0 * c) - ik * aK2) -
+                      mk(0, 1) * (skr * (iK5r * w * sn * (3.0 * p.Kt0 * c - aK2)));
+        const cd e1 = (ce * (iK4 * w * c)) * (kkR * (p.Kt1 * sn) - ik * aK2) -
+                      mk(0, 1) * (skr * (iK5r * w * c * (3.0 * p.Kt1 * sn - aK2)));
+#pragma unroll
+        for (int a = 0; a < 4; a++) { h0[a] = h0[a] + e0 * p.n[a]; h1[a] = h1[a] + e1 * p.n[a]; }
+      }
+      if (q >= 2) {
+        const QPol pm = qpol(g, qx, t0, t1, zt, et, -rr, c, sn);
+        cd sp, cp, smv, cmv;
+        csincos(mk(kr * p.aK * rr, ki * p.aK * rr), sp, cp);
+        csincos(mk(kr * pm.aK * rr, ki * pm.aK * rr), smv, cmv);
+        const double ir = w / rr;
+        const cd up = cp * (p.aW / (aK2 * p.aK) * ir), um = cmv * (pm.aW / (pm.aK * pm.aK * pm.aK) * ir);
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          g0[a] = g0[a] - (up * (p.Kt0 * p.n[a]) - um * (pm.Kt0 * pm.n[a]));
+          g1[a] = g1[a] - (up * (p.Kt1 * p.n[a]) - um * (pm.Kt1 * pm.n[a]));
+        }
+        if (warped) {
+          const double pk2 = pm.aK * pm.aK, iP5 = 1.0 / (aK2 * aK2 * p.aK), iM5 = 1.0 / (pk2 * pk2 * pm.aK);
+          const double ap0 = iP5 * (3.0 * p.Kt0 * c - aK2), am0 = iM5 * (3.0 * pm.Kt0 * c - pk2);
+          const double ap1 = iP5 * (3.0 * p.Kt1 * sn - aK2), am1 = iM5 * (3.0 * pm.Kt1 * sn - pk2);
+#pragma unroll
+          for (int a = 0; a < 4; a++) {
+            h0[a] = h0[a] - (cp * (ap0 * p.n[a]) - cmv * (am0 * pm.n[a])) * (ir * sn);
+            h1[a] = h1[a] - (cp * (ap1 * p.n[a]) - cmv * (am1 * pm.n[a])) * (ir * c);
+          }
+          f0 += (ap0 - am0) * ir * sn;
+          f1 += (ap1 - am1) * ir * c;
+        }
+      }
+    }
+  }
+  const bool flip = td < 0;
+  const double sH = (flip ? -BEM_I_FP : BEM_I_FP) * bdC;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const cd ga = wsum(sg[a]), d0 = wsum(g0[a]), d1 = wsum(g1[a]);
+    const cd ha = wsum(sh[a]), e0 = wsum(h0[a]), e1 = wsum(h1[a]);
+    if (lane == 0) {
+      G[(size_t)row * ld + col + a] = ga * BEM_I_FP;
+      dG[((size_t)row * 2 + 0) * ld + col + a] = d0 * (-BEM_I_FP * iaT0);
+      dG[((size_t)row * 2 + 1) * ld + col + a] = d1 * (-BEM_I_FP * iaT1);
+      H[(size_t)row * ld + col + a] = warped ? ha * sH : mk(0, 0);
+      dH[((size_t)row * 2 + 0) * ld + col + a] = warped ? e0 * (sH * iaT0) : mk(0, 0);
+      dH[((size_t)row * 2 + 1) * ld + col + a] = warped ? e1 * (sH * iaT1) : mk(0, 0);
+    }
+  }
+  sf = wsum(sf); f0 = wsum(f0); f1 = wsum(f1);
+  if (lane == 0) {
+    selfDF[(size_t)row * 3 + 0] = warped ? sf * sH : 0.0;
+    selfDF[(size_t)row * 3 + 1] = warped ? f0 * sH * iaT0 : 0.0;
+    selfDF[(size_t)row * 3 + 2] = warped ? f1 * sH * iaT1 : 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                  const int *__restrict__ sid, const RowPt *__restrict__ rows,
+                                                  const int *__restrict__ colOff, int nrows, double kr, double ki,
+                                                  const double *__restrict__ xl, const double *__restrict__ wl,
+                                                  const double *__restrict__ xh, const double *__restrict__ wh,
+                                                  cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
+                                                  cd *__restrict__ dH, size_t ld, double *__restrict__ selfDF)
 {
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -393,6 +773,13 @@ __global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo
   int td = sid[rp.t];
   int at = td < 0 ? -td : td;
   const TriGeo g = geo[at];
+  if (g.quad != 0.0) {
+    asm_self_quad(g, gq[at], rp, td, row, colOff[rp.t], kr, ki, xl, wl, xh, wh, lane, G, H, dG, dH, ld, selfDF);
+    return;
+  }
+  if (lane == 0) {   // a flat triangle has no self F terms
+    selfDF[(size_t)row * 3 + 0] = 0.0; selfDF[(size_t)row * 3 + 1] = 0.0; selfDF[(size_t)row * 3 + 2] = 0.0;
+  }
   const double zt = c_tri_zt[rp.slot], et = c_tri_et[rp.slot];
   // polar set-up about (zt, et): vertices (-1/2,-sqrt3/2), (1,0), (-1/2,sqrt3/2)
   double zp = 1.0 - zt, zm = -0.5 - zt, ep = 0.5 * BEM_SQ3 - et, em = -0.5 * BEM_SQ3 - et;
@@ -497,8 +884,8 @@ __global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo
 // ---- pass 4: per-row sums and the diagonal term ----
 __global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPart, int nwv,
                                                  const double *__restrict__ ghDF, const int *__restrict__ ghStart,
-                                                 int nrows, int open_domain, cd *__restrict__ H, size_t ld,
-                                                 double *__restrict__ dF)
+                                                 const double *__restrict__ selfDF, int nrows, int open_domain,
+                                                 cd *__restrict__ H, size_t ld, double *__restrict__ dF)
 {
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -513,6 +900,8 @@ __global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPa
   }
   f = wsum(f); f0 = wsum(f0); f1 = wsum(f1);
   if (lane == 0) {
+    // the self element last (a warped quadrangle's F terms)
+    f += selfDF[(size_t)row * 3]; f0 += selfDF[(size_t)row * 3 + 1]; f1 += selfDF[(size_t)row * 3 + 2];
     double diag = open_domain ? 1.0 + f : f;
     cd *h = H + (size_t)row * ld + row;
     h->x += diag;
@@ -522,42 +911,49 @@ __global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPa
 
 extern "C" {
 // launchers used by the C-ABI layer (bem_api.hip)
-void bem_launch_asm_gl(const void *geo, const int *sid, const void *rows, const int *colOff, int dNe, int nrows,
-                       double kr, double ki, const void *rule, int nq, void *G, void *H, void *dG, void *dH, size_t ld,
-                       double *dfPart, unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
+// qrule: the square's 12x12 rule (nqq points) for quadrangle source elements
+void bem_launch_asm_gl(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int dNe,
+                       int nrows, double kr, double ki, const void *rule, int nq, const void *qrule, int nqq, void *G,
+                       void *H, void *dG, void *dH, size_t ld, double *dfPart, unsigned long long *ghMask,
+                       int nwv_blocks, hipStream_t st)
 {
   dim3 grid(nwv_blocks, nrows);
   bem_prof_begin(BH_K_ASM_GL, (double)nrows * dNe, st);
-  hipLaunchKernelGGL(k_asm_gl, grid, dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff, dNe, kr, ki,
-                     (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
+  hipLaunchKernelGGL(k_asm_gl, grid, dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid, (const RowPt *)rows,
+                     colOff, dNe, kr, ki, (const QPt *)rule, nq, (const QPtQ *)qrule, nqq, (cd *)G, (cd *)H, (cd *)dG,
+                     (cd *)dH, ld, dfPart, ghMask);
   bem_prof_end(BH_K_ASM_GL, st);
 }
 
 // gl != nullptr: the pairs come from the FP32 pass and are first redone
-// with the FP64 12x12 rule (gl, ngl points)
-void bem_launch_asm_gh(const void *geo, const int *sid, const void *rows, const int *colOff, const void *pairs,
-                       int npairs, double kr, double ki, const void *rule, int nq, const void *gl, int ngl, void *G,
-                       void *H, void *dG, void *dH, size_t ld, double *ghDF, hipStream_t st)
+// with the FP64 12x12 rule (gl, ngl points; qgl, nqgl on quadrangles);
+// qgh, nqgh: the square's 64x64 rule
+void bem_launch_asm_gh(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff,
+                       const void *pairs, int npairs, double kr, double ki, const void *rule, int nq, const void *gl,
+                       int ngl, const void *qgh, int nqgh, const void *qgl, int nqgl, void *G, void *H, void *dG,
+                       void *dH, size_t ld, double *ghDF, hipStream_t st)
 {
   if (npairs <= 0) return;
   int nb = (npairs + 3) / 4;
   bem_prof_begin(BH_K_ASM_GH, (double)npairs, st);
-  hipLaunchKernelGGL(k_asm_gh, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff,
-                     (const GhPair *)pairs, npairs, kr, ki, (const QPt *)rule, nq, (const QPt *)gl, ngl, (cd *)G,
-                     (cd *)H, (cd *)dG, (cd *)dH, ld, ghDF);
+  hipLaunchKernelGGL(k_asm_gh, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
+                     (const RowPt *)rows, colOff, (const GhPair *)pairs, npairs, kr, ki, (const QPt *)rule, nq,
+                     (const QPt *)gl, ngl, (const QPtQ *)qgh, nqgh, (const QPtQ *)qgl, nqgl, (cd *)G, (cd *)H, (cd *)dG,
+                     (cd *)dH, ld, ghDF);
   bem_prof_end(BH_K_ASM_GH, st);
 }
 
 // rule32: FP32 copy of the 12x12 rule (far pairs), rule: the FP64 one (near pairs)
-void bem_launch_asm_gl32(const void *geo, const int *sid, const void *rows, const int *colOff, int dNe, int nrows,
-                         double kr, double ki, const void *rule32, const void *rule, int nq, void *G, void *H, void *dG,
-                         void *dH, size_t ld, double *dfPart, unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
+void bem_launch_asm_gl32(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int dNe,
+                         int nrows, double kr, double ki, const void *rule32, const void *rule, int nq, const void *qrule,
+                         int nqq, void *G, void *H, void *dG, void *dH, size_t ld, double *dfPart,
+                         unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
 {
   dim3 grid(nwv_blocks, nrows);
   bem_prof_begin(BH_K_ASM_GL32, (double)nrows * dNe, st);
-  hipLaunchKernelGGL(k_asm_gl32, grid, dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff, dNe, kr,
-                     ki, (const QPt32 *)rule32, (const QPt *)rule, nq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart,
-                     ghMask);
+  hipLaunchKernelGGL(k_asm_gl32, grid, dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
+                     (const RowPt *)rows, colOff, dNe, kr, ki, (const QPt32 *)rule32, (const QPt *)rule, nq,
+                     (const QPtQ *)qrule, nqq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
   bem_prof_end(BH_K_ASM_GL32, st);
 }
 
@@ -581,22 +977,25 @@ int bem_rule32(const void *rule, int nq, void **out)
   return rc;
 }
 
-void bem_launch_asm_self(const void *geo, const int *sid, const void *rows, const int *colOff, int nrows, double kr,
-                         double ki, const double *xl, const double *wl, const double *xh, const double *wh, void *G,
-                         void *H, void *dG, void *dH, size_t ld, hipStream_t st)
+// selfDF: [nrows][3] the self element's F, dF/dt_z, dF/dt_e (non-zero only
+// on warped quadrangles), written here and added by the final pass
+void bem_launch_asm_self(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int nrows,
+                         double kr, double ki, const double *xl, const double *wl, const double *xh, const double *wh,
+                         void *G, void *H, void *dG, void *dH, size_t ld, double *selfDF, hipStream_t st)
 {
   int nb = (nrows + 3) / 4;
   bem_prof_begin(BH_K_ASM_SELF, (double)nrows, st);
-  hipLaunchKernelGGL(k_asm_self, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, sid, (const RowPt *)rows, colOff,
-                     nrows, kr, ki, xl, wl, xh, wh, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld);
+  hipLaunchKernelGGL(k_asm_self, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
+                     (const RowPt *)rows, colOff, nrows, kr, ki, xl, wl, xh, wh, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH,
+                     ld, selfDF);
   bem_prof_end(BH_K_ASM_SELF, st);
 }
 
-void bem_launch_asm_fin(const double *dfPart, int nwv, const double *ghDF, const int *ghStart, int nrows,
-                        int open_domain, void *H, size_t ld, double *dF, hipStream_t st)
+void bem_launch_asm_fin(const double *dfPart, int nwv, const double *ghDF, const int *ghStart, const double *selfDF,
+                        int nrows, int open_domain, void *H, size_t ld, double *dF, hipStream_t st)
 {
   int nb = (nrows + 3) / 4;
-  hipLaunchKernelGGL(k_asm_fin, dim3(nb), dim3(256), 0, st, dfPart, nwv, ghDF, ghStart, nrows, open_domain, (cd *)H, ld,
-                     dF);
+  hipLaunchKernelGGL(k_asm_fin, dim3(nb), dim3(256), 0, st, dfPart, nwv, ghDF, ghStart, selfDF, nrows, open_domain,
+                     (cd *)H, ld, dF);
 }
 }

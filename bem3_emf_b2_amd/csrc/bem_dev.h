@@ -105,6 +105,7 @@ __device__ __forceinline__ void tri_N(double z, double e, double &n0, double &n1
 #define BEM_I_FP 7.957747154594766788444e-2    // 1/(4 pi)
 #define BEM_I_SPSQ3 3.062938307898844719507e-2 // 1/(6 pi sqrt 3)
 #define BEM_DEPS 0.1
+#define BEM_GEPS 1.0e-15   // geometric tolerance of bil_check_plane (d3b1_const.h:28)
 #define BEM_CBD_SDF 0.5
 #define BEM_CBD_CDF 100.0   // field-derivative cut-off of EH_mEMP_s (d3b1_const.h:78)
 

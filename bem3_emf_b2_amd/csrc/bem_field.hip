@@ -177,6 +177,70 @@ __device__ __forceinline__ void quad_N(double z, double e, double *n)
   n[3] = 0.25 * (1.0 - z * ia) * (1.0 + e * ia);
 }
 
+// G[4], H[4], F and their derivatives along ND directions v[d] (unit
+// vectors) at rt from one bi-linear quadrangle with a tabulated rule: the
+// bil_dcoef_4p/9p/GL/GH kernels (bil_dcoef.c:50-320) with the ND dcoef_rt
+// calls of one point sharing one evaluation of the kernel per rule point.
+// dG = -1/(4 pi) sum w N (ikR-1) e^{ikR} (R.v/R) |W| / R^2,
+// dH = 1/(4 pi) sum w N e^{ikR}/R^3 ((kR^2 + 3(ikR-1)) (R.v/R)(R.W/R) - (ikR-1) W.v),
+// dF = 1/(4 pi) sum w (3 (R.v/R)(R.W/R) - W.v)/R^3; H, F and their
+// derivatives negated on the sub-domain side.
+template <int ND>
+__device__ __forceinline__ void quad_coef_dir(const TriGeo &g, const QuadExt &qx, const double *rt, const double (*v)[3],
+                                              double kr, double ki, const QPtQ *__restrict__ rule, int nq, bool flip,
+                                              cd *G, cd *H, double &F, cd (*dG)[4], cd (*dH)[4], double *dF)
+{
+  cd aG[4], aH[4], aDG[ND][4], aDH[ND][4];
+  double f = 0.0, df[ND];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    aG[a] = mk(0, 0); aH[a] = mk(0, 0);
+#pragma unroll
+    for (int d = 0; d < ND; d++) { aDG[d][a] = mk(0, 0); aDH[d][a] = mk(0, 0); }
+  }
+#pragma unroll
+  for (int d = 0; d < ND; d++) df[d] = 0.0;
+  for (int p = 0; p < nq; p++) {
+    const QPtQ q = rule[p];
+    double r[3], W[3];
+    quad_geom(g, qx, q.z, q.e, r, W);
+    const double vR[3] = {r[0] - rt[0], r[1] - rt[1], r[2] - rt[2]};
+    const double R = sqrt(vR[0] * vR[0] + vR[1] * vR[1] + vR[2] * vR[2]), iR = 1.0 / R, iR2 = iR * iR, iR3 = iR2 * iR;
+    const double aW = sqrt(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
+    const double rdW = (vR[0] * W[0] + vR[1] * W[1] + vR[2] * W[2]) * iR;
+    const cd ce = eikr(kr, ki, R);
+    const cd cb = mk(-1.0 - ki * R, kr * R);
+    const cd cbce = cb * ce;
+    const cd gk = ce * (iR * aW), hk = cbce * (rdW * iR2), tk = cbce * (iR2 * aW);
+    const cd kR = mk(kr * R, ki * R);
+    const cd qq = kR * kR + cb * 3.0;
+    const cd ce3 = ce * iR3;
+#pragma unroll
+    for (int a = 0; a < 4; a++) { fma_cr(aG[a], gk, q.w * q.n[a]); fma_cr(aH[a], hk, q.w * q.n[a]); }
+    f = fma(q.w, rdW * iR2, f);
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      const double rd = (vR[0] * v[d][0] + vR[1] * v[d][1] + vR[2] * v[d][2]) * iR;
+      const double Wdv = W[0] * v[d][0] + W[1] * v[d][1] + W[2] * v[d][2];
+      const cd dh = ce3 * (qq * (rd * rdW) - cb * Wdv);
+#pragma unroll
+      for (int a = 0; a < 4; a++) { fma_cr(aDG[d][a], tk, q.w * q.n[a] * rd); fma_cr(aDH[d][a], dh, q.w * q.n[a]); }
+      df[d] = fma(q.w, iR3 * (3.0 * rd * rdW - Wdv), df[d]);
+    }
+  }
+  const double sh = flip ? -BEM_I_FP : BEM_I_FP;
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    G[a] = aG[a] * BEM_I_FP;
+    H[a] = aH[a] * sh;
+#pragma unroll
+    for (int d = 0; d < ND; d++) { dG[d][a] = aDG[d][a] * (-BEM_I_FP); dH[d][a] = aDH[d][a] * sh; }
+  }
+  F = f * sh;
+#pragma unroll
+  for (int d = 0; d < ND; d++) dF[d] = df[d] * sh;
+}
+
 // Self quadrangle at (zt, et) in polar coordinates about the point
 // (bil_coef_bd, bil_coef.c:244-342, 426-536): 4 edges x 2 x 64 angles (this
 // lane's share), 12 radial points; G always, H and F only on a warped
@@ -437,11 +501,12 @@ __device__ __forceinline__ void tri_coef_grad(const TriGeo &g, const double *rt,
 // Output added to out (6 complex per point).
 __global__ void __launch_bounds__(256) k_field_memp(const double *__restrict__ pts, const int *__restrict__ pdid, int np,
                                                     int want_did, const TriGeo *__restrict__ geo,
-                                                    const int *__restrict__ sid, int dNe, double kr, double ki,
-                                                    const QPtN *__restrict__ rule, int nq, double scale, int sw,
-                                                    const QPtN *__restrict__ gh, int ngh, double ghscale,
-                                                    const cd *__restrict__ U, const cd *__restrict__ dU, double lam0,
-                                                    cd *__restrict__ out)
+                                                    const QuadExt *__restrict__ gq, const int *__restrict__ sid, int dNe,
+                                                    double kr, double ki, const QPtN *__restrict__ rule, int nq,
+                                                    double scale, int sw, const QPtN *__restrict__ gh, int ngh,
+                                                    double ghscale, const QPtQ *__restrict__ qrule, int nqq,
+                                                    const QPtQ *__restrict__ qgh, int nqgh, const cd *__restrict__ U,
+                                                    const cd *__restrict__ dU, double lam0, cd *__restrict__ out)
 {
   const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -452,9 +517,53 @@ __global__ void __launch_bounds__(256) k_field_memp(const double *__restrict__ p
 #pragma unroll
   for (int l = 0; l < 4; l++) { aU[l] = mk(0, 0); aD[0][l] = mk(0, 0); aD[1][l] = mk(0, 0); aD[2][l] = mk(0, 0); }
   double F = 0.0, dF[3] = {0.0, 0.0, 0.0};
+  const double ax[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
   for (int s = 1 + lane; s <= dNe; s += 64) {
     const int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
+    if (g.quad != 0.0) {
+      // bi-linear quadrangle: the square's rules, the same per-direction
+      // GHN switch (dcoef_rt, d3b1_elem.c:277-296)
+      const QuadExt qx = gq[as];
+      cd G[4], H[4], DG[3][4], DH[3][4];
+      double f, df[3];
+      quad_coef_dir<3>(g, qx, rt, ax, kr, ki, qrule, nqq, sd < 0, G, H, f, DG, DH, df);
+      if (sw) {
+        const bool s0 = fabs(df[0]) > BEM_CBD_SDF, s1 = fabs(df[1]) > BEM_CBD_SDF, s2 = fabs(df[2]) > BEM_CBD_SDF;
+        if (s0 || s1 || s2) {
+          cd tG[4], tH[4], tDG[3][4], tDH[3][4];
+          double tf, tdf[3];
+          quad_coef_dir<3>(g, qx, rt, ax, kr, ki, qgh, nqgh, sd < 0, tG, tH, tf, tDG, tDH, tdf);
+          const bool sws[3] = {s0, s1, s2};
+#pragma unroll
+          for (int d = 0; d < 3; d++)
+            if (sws[d]) {
+#pragma unroll
+              for (int i = 0; i < 4; i++) { DG[d][i] = tDG[d][i]; DH[d][i] = tDH[d][i]; }
+              df[d] = tdf[d];
+            }
+          if (s2) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) { G[i] = tG[i]; H[i] = tH[i]; }
+            f = tf;
+          }
+        }
+      }
+      const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
+#pragma unroll
+      for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+          const cd a = dUs[n * 4 + l], b = Us[n * 4 + l];
+          aU[l] = aU[l] + G[n] * a - H[n] * b;
+#pragma unroll
+          for (int d = 0; d < 3; d++) aD[d][l] = aD[d][l] + DG[d][n] * a - DH[d][n] * b;
+        }
+      F += f;
+#pragma unroll
+      for (int d = 0; d < 3; d++) dF[d] += df[d];
+      continue;
+    }
     cd G[3], H[3], DG[3][3], DH[3][3];
     double f, df[3];
     tri_coef_grad(g, rt, kr, ki, rule, nq, scale, sd < 0, G, H, f, DG, DH, df);
@@ -529,10 +638,12 @@ __global__ void __launch_bounds__(256) k_field_memp(const double *__restrict__ p
 // d3b1_elem.c:277-316), added into Ui / dUi of the destination slots.
 __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts, const double *__restrict__ nrm,
                                                  const int *__restrict__ slot, int np, const TriGeo *__restrict__ geo,
-                                                 const int *__restrict__ sid, int dNe, double kr, double ki,
-                                                 const QPtN *__restrict__ gl, int ngl, const QPtN *__restrict__ gh,
-                                                 int ngh, double scale, const cd *__restrict__ U,
-                                                 const cd *__restrict__ dU, cd *__restrict__ Ui, cd *__restrict__ dUi)
+                                                 const QuadExt *__restrict__ gq, const int *__restrict__ sid, int dNe,
+                                                 double kr, double ki, const QPtN *__restrict__ gl, int ngl,
+                                                 const QPtN *__restrict__ gh, int ngh, double scale,
+                                                 const QPtQ *__restrict__ qgl, int nqgl, const QPtQ *__restrict__ qgh,
+                                                 int nqgh, const cd *__restrict__ U, const cd *__restrict__ dU,
+                                                 cd *__restrict__ Ui, cd *__restrict__ dUi)
 {
   const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -541,9 +652,27 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
   const double v[3] = {nrm[3 * p], nrm[3 * p + 1], nrm[3 * p + 2]};
   cd aU[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, aD[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   double F = 0.0, dF = 0.0;
+  const double vv[1][3] = {{v[0], v[1], v[2]}};
   for (int s = 1 + lane; s <= dNe; s += 64) {
     int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
+    if (g.quad != 0.0) {
+      const QuadExt qx = gq[as];
+      cd G[4], H[4], DG[1][4], DH[1][4];
+      double f, df[1];
+      quad_coef_dir<1>(g, qx, rt, vv, kr, ki, qgl, nqgl, sd < 0, G, H, f, DG, DH, df);
+      if (fabs(df[0]) > BEM_CBD_SDF) quad_coef_dir<1>(g, qx, rt, vv, kr, ki, qgh, nqgh, sd < 0, G, H, f, DG, DH, df);
+      F += f; dF += df[0];
+      const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
+#pragma unroll
+      for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+          aU[l] = aU[l] + G[n] * dUs[n * 4 + l] - H[n] * Us[n * 4 + l];
+          aD[l] = aD[l] + DG[0][n] * dUs[n * 4 + l] - DH[0][n] * Us[n * 4 + l];
+        }
+      continue;
+    }
     cd G[3], H[3], DG[3], DH[3];
     double f, df;
     tri_coef<true>(g, rt, v, kr, ki, gl, ngl, scale, sd < 0, G, H, f, DG, DH, df);
@@ -819,26 +948,30 @@ void bem_launch_field(const double *pts, const int *pdid, int np, int want_did, 
                      (const cd *)A0, (const cd *)dA0, (const cd *)A1, (const cd *)dA1, (cd *)out, accumulate);
   bem_prof_end(BH_K_FIELD, st);
 }
-void bem_launch_field_memp(const double *pts, const int *pdid, int np, int want_did, const void *geo, const int *sid,
-                           int dNe, double kr, double ki, const void *rule, int nq, double scale, int sw,
-                           const void *gh, int ngh, double ghscale, const void *U, const void *dU, double lam0,
-                           void *out, hipStream_t st)
+// qrule / qgh: the square's rules matching rule / gh, for quadrangles
+void bem_launch_field_memp(const double *pts, const int *pdid, int np, int want_did, const void *geo, const void *gq,
+                           const int *sid, int dNe, double kr, double ki, const void *rule, int nq, double scale, int sw,
+                           const void *gh, int ngh, double ghscale, const void *qrule, int nqq, const void *qgh,
+                           int nqgh, const void *U, const void *dU, double lam0, void *out, hipStream_t st)
 {
   if (np <= 0) return;
   bem_prof_begin(BH_K_FIELD, (double)np * dNe, st);
   hipLaunchKernelGGL(k_field_memp, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo,
-                     sid, dNe, kr, ki, (const QPtN *)rule, nq, scale, sw, (const QPtN *)gh, ngh, ghscale, (const cd *)U,
-                     (const cd *)dU, lam0, (cd *)out);
+                     (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)rule, nq, scale, sw, (const QPtN *)gh, ngh,
+                     ghscale, (const QPtQ *)qrule, nqq, (const QPtQ *)qgh, nqgh, (const cd *)U, (const cd *)dU, lam0,
+                     (cd *)out);
   bem_prof_end(BH_K_FIELD, st);
 }
-void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const int *sid,
-                        int dNe, double kr, double ki, const void *gl, int ngl, const void *gh, int ngh, double scale,
-                        const void *U, const void *dU, void *Ui, void *dUi, hipStream_t st)
+void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const void *gq,
+                        const int *sid, int dNe, double kr, double ki, const void *gl, int ngl, const void *gh, int ngh,
+                        double scale, const void *qgl, int nqgl, const void *qgh, int nqgh, const void *U,
+                        const void *dU, void *Ui, void *dUi, hipStream_t st)
 {
   if (np <= 0) return;
   bem_prof_begin(BH_K_SCATTER, (double)np * dNe, st);
-  hipLaunchKernelGGL(k_scatter, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo, sid, dNe,
-                     kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale, (const cd *)U, (const cd *)dU, (cd *)Ui,
+  hipLaunchKernelGGL(k_scatter, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo,
+                     (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale,
+                     (const QPtQ *)qgl, nqgl, (const QPtQ *)qgh, nqgh, (const cd *)U, (const cd *)dU, (cd *)Ui,
                      (cd *)dUi);
   bem_prof_end(BH_K_SCATTER, st);
 }

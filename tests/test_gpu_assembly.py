@@ -54,3 +54,37 @@ def test_assembly_matches_oracle(name):
             assert err < 1e-12, (did, key, err)
         dF = g.matrix(did, "dF")
         assert np.abs(dF - ref["dF"][idx]).max() < 1e-12 * np.abs(ref["dF"]).max()
+
+
+def test_quad_mesh_assembly_matches_oracle():
+    """The reference's verification1 cone (28 triangles and 553 bi-linear
+    quadrangles, metal n = 0.06+4i): G, H, dG, dH, dF of both domains with
+    the quadrangles' far pairs (12x12 rule), near pairs (64x64 rule) and
+    self element (polar rule, principal-value derivatives of
+    bil_dcoef.c:369-740, H and F terms of warped quadrangles)."""
+    import bem3_emf_b2_amd as bem
+    from fixtures import active_slots, v1_oracle, v1_raw
+    o = v1_raw()[0]
+    ob = v1_oracle()[0]
+    g = bem.Object(o["rn"], o["ed"], o["md"], o["sd"], list(o["n"]), float(o["lambda0"]))
+    g.assemble(2)
+    for did in (0, 1):
+        ref = ob.cmatrix(did, 2, want_d=True)
+        idx = active_slots(o["ed"], o["dom"][did]["sid"])
+        assert g.gh_count(did) == ref["ngh"], did
+        for key in ("G", "H"):
+            want = ref[key][np.ix_(idx, idx)]
+            got = g.matrix(did, key)
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err < 1e-12, (did, key, err)
+        for key in ("dG", "dH"):
+            want = ref[key][idx][:, :, idx]
+            got = g.matrix(did, key)
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err < 1e-12, (did, key, err)
+        # the dF/dt row sums of the rows at the cone's sharp edges cancel
+        # near-pair terms far larger than the sums themselves, so their
+        # summation order shows at 1e-11 of max |dF|
+        dF = g.matrix(did, "dF")
+        assert np.abs(dF - ref["dF"][idx]).max() < 1e-10 * np.abs(ref["dF"]).max()
+    g.close()

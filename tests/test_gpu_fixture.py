@@ -108,3 +108,19 @@ def test_reference_example1_on_v1_dat(tmp_path):
                 a, b = got["force"][typ][m][q], want["force"][typ][m][q]
                 assert np.abs(a - b).max() < TOL * np.abs(b).max(), (typ, m, q)
             assert abs(got["absorb"][typ][m] - want["absorb"][typ][m]) < TOL * want["absorb"][typ][m], (typ, m)
+
+
+@pytest.mark.parametrize("typ", [0, 1, 2])
+def test_EH_mEMP_on_quad_mesh_matches_oracle(v1_device, typ):
+    """E, H from the gradients of the potentials (EH_mEMP_s,
+    d3b1_field.c:68-170) over quadrangle sources (bil_dcoef_4p/9p/GL with
+    the per-direction GHN switch) against the oracle on the stored state"""
+    import bem3_emf_b2_amd as bem
+    import oracle as orc
+    from fixtures import v1_oracle
+    pts = np.array([REF_PT, [0.0, 0.5, 0.1], [0.2, -0.3, 0.6], [-0.4, 0.1, -0.9], [0.1, 0.2, 3.0], [0.0, 0.0, 20.0]])
+    got, gid = bem.mo_field(v1_device, pts, "EH_mEMP", "t", typ)
+    want, wid = orc.mo_field(v1_oracle(), pts, "EH_mEMP", "t", typ)
+    assert (gid == wid).all()
+    for p in range(len(pts)):
+        assert np.abs(got[p] - want[p]).max() < 1e-10 * np.abs(want[p]).max(), (p, typ)
