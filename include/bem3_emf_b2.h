@@ -53,13 +53,134 @@ typedef struct incident_planewave {
   IpwD data;
 } Ipw;
 
-/* the focused / Laguerre-Gauss / Bessel / radial-azimuthal beams of the
- * reference are not provided; only their pointers appear in Bobj */
-typedef struct fpw Fpw;
-typedef struct lgb LGb;
-typedef struct bsb Bsb;
-typedef struct bslgb BsLGb;
-typedef struct rab RAb;
+/* The five other beams of multi_fbeam (fpw.h:14-39, lgb.h:14-41,
+ * bsb.h:13-37, bslgb.h:13-39, rab.h:13-44).  Each record is written raw into
+ * .obj / .dat files, so the member order and types are the reference's; the
+ * pointers of the derived-data part are meaningless in a file and are
+ * rebuilt by the set-up functions. */
+typedef struct fpw_data {
+  double ki, E0;
+  double complex ex, ey;
+  double complex hx, hy;
+  double cos_t, sin_t;
+  double cos_p, sin_p;
+  int nt;
+  double *wt, *ct, *st;
+  double *cp, *sp;
+} FpwD;
+
+typedef struct fpw {              /* focused plane wave */
+  double lambda0;
+  double ni;
+  double NA;
+  double power;
+  double complex e0x;
+  double complex e0y;
+  double fx, fy, fz;
+  double theta, phi;
+  int nn;                         /* Gauss-Legendre points over sin(theta) */
+  FpwD data;
+} Fpw;
+
+typedef struct lgb_data {
+  double ki, E0;
+  double complex ex, ey;
+  double complex hx, hy;
+  double cos_t, sin_t;
+  double cos_p, sin_p;
+  int nt;
+  double *wt, *ct, *st;
+  double *cp, *sp;
+  double complex *ph_p;
+} LGbD;
+
+typedef struct lgb {              /* focused beam with spiral phase modulation */
+  double lambda0;
+  double ni;
+  double NA;
+  double power;
+  double complex e0x;
+  double complex e0y;
+  double fx, fy, fz;
+  double theta, phi;
+  int lg_m;                       /* mode number of the spiral phase */
+  int nn;
+  LGbD data;
+} LGb;
+
+typedef struct bsb_data {
+  double ki, E0;
+  double complex ex, ey;
+  double complex hx, hy;
+  double cos_t, sin_t;
+  double cos_p, sin_p;
+  double ct, st;                  /* cone (deflection) angle */
+  double *cp;
+  double *sp;
+} BsbD;
+
+typedef struct bsb {              /* Bessel beam */
+  double lambda0;
+  double ni;
+  double d_angle;
+  double power;
+  double complex e0x;
+  double complex e0y;
+  double fx, fy, fz;
+  double theta, phi;
+  BsbD data;
+} Bsb;
+
+typedef struct bslgb_data {
+  double ki, E0;
+  double complex ex, ey;
+  double complex hx, hy;
+  double cos_t, sin_t;
+  double cos_p, sin_p;
+  double ct, st;
+  double *cp, *sp;
+  double complex *ph_p;
+} BsLGbD;
+
+typedef struct bslgb {            /* Bessel beam with spiral phase modulation */
+  double lambda0;
+  double ni;
+  double d_angle;
+  double power;
+  double complex e0x;
+  double complex e0y;
+  double fx, fy, fz;
+  double theta, phi;
+  int lg_m;
+  BsLGbD data;
+} BsLGb;
+
+typedef struct rab_data {
+  double ki, E0;
+  double complex er, ea;
+  double complex hr, ha;
+  double cos_t, sin_t;
+  double cos_p, sin_p;
+  int nt;
+  double *wt;
+  double *ct;
+  double *st;
+  double *cp;
+  double *sp;
+} RAbD;
+
+typedef struct rab {              /* focused radially / azimuthally polarised beam */
+  double lambda0;
+  double ni;
+  double NA;
+  double power;
+  double complex e0r;
+  double complex e0a;
+  double fx, fy, fz;
+  double theta, phi;
+  int nn;
+  RAbD data;
+} RAb;
 
 typedef struct beam_data {
   Ipw *ipw;

@@ -35,6 +35,8 @@ def lib():
             "orc_api_from_arrays": (VP, [C.c_int, D, C.c_int, Ip, Ip, Ip, Ip, C.c_int, D]),
             "orc_api_free": (None, [VP]),
             "orc_api_setup": (None, [VP, C.c_int, D, D, C.c_int]),
+            "orc_api_setup_mfb": (None, [VP, C.c_int, D, D, C.c_int]),
+            "orc_api_beam_EH": (None, [VP, C.c_int, D, D, D]),
             "orc_api_override": (None, [VP, D, D, D]),
             "orc_api_refresh_incident": (None, [VP]),
             "orc_api_info": (C.c_int, [VP, Ip]),
@@ -123,9 +125,23 @@ class Obj:
                                              len(n) - 1, dp(n.view(np.float64))))
 
     def setup(self, beams, tv=None):
-        b = np.ascontiguousarray(beams, dtype=np.float64).reshape(-1, 12)
+        """beams: rows of 12 (plane waves, ipw.txt columns) or of 16 (any
+        multi_fbeam type: type, lambda0, ni, NA or deflection angle, power,
+        re/im e0x, re/im e0y, fx, fy, fz, theta, phi, spiral mode, points)"""
+        b = np.ascontiguousarray(beams, dtype=np.float64)
+        b = b.reshape(-1, b.shape[-1] if b.ndim > 1 else 12)
         t = np.zeros(3) if tv is None else np.asarray(tv, dtype=np.float64)
-        lib().orc_api_setup(self.h, b.shape[0], dp(b), dp(t), 0 if tv is None else 1)
+        fn = lib().orc_api_setup_mfb if b.shape[1] == 16 else lib().orc_api_setup
+        fn(self.h, b.shape[0], dp(b), dp(t), 0 if tv is None else 1)
+
+    def beam_EH(self, x, v=None):
+        """incident E, H (rows of 6; with directions v, rows of 12 with
+        dE/dv, dH/dv) of the object's beams at points x"""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        vv = None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((x.shape[0], 6 if v is None else 12), np.complex128)
+        lib().orc_api_beam_EH(self.h, x.shape[0], dp(x), dp(vv) if vv is not None else None, dp(out.view(np.float64)))
+        return out
 
     def override(self, ren=None, wen=None, kn=None):
         a = np.ascontiguousarray(ren, dtype=np.float64) if ren is not None else None

@@ -48,9 +48,27 @@ typedef struct {
   zc ex, ey, ez, hx, hy, hz;
 } orc_ipw;
 
+/* one beam of the other multi_fbeam types (orc_beam.c): user parameters
+ * and derived data; type 1 focused plane wave, 2 with spiral phase, 3
+ * Bessel, 4 Bessel with spiral phase, 5 radially / azimuthally polarised
+ * (e0x, e0y, ex, ey, hx, hy are then the radial / azimuthal components) */
+typedef struct {
+  int type;
+  double lambda0, ni, na, power;   /* na: NA, or the Bessel deflection angle */
+  zc e0x, e0y;
+  double fx, fy, fz, theta, phi;
+  int lg_m, nn;
+  double ki, E0, cos_t, sin_t, cos_p, sin_p, ct, st;
+  zc ex, ey, hx, hy;
+  int nt;
+  double *tc, *ts, *tw;            /* spectral angles: cos, sin, weight x sin */
+} orc_fb;
+
 typedef struct {
   int n;
   orc_ipw *w;
+  int nf;
+  orc_fb *f;
   double n0, lambda0, omega;
 } orc_beams;
 
@@ -115,6 +133,11 @@ void orc_alloc_state(orc_obj *o);
 int orc_domain_id(const double *rt, const orc_obj *o);
 void orc_beam_EH(zc *e, zc *h, const double *x, const orc_beams *bm);
 void orc_beam_EH_dv(zc *e, zc *h, zc *de, zc *dh, const double *x, const double *v, const orc_beams *bm);
+void orc_gauleg_ab(double a, double b, double *x, double *w, int n);
+/* ---- other beams (orc_beam.c) ---- */
+void orc_set_mfb(orc_obj *o, int nb, const double *par);
+void orc_fb_EH(zc *e, zc *h, zc *de, zc *dh, const double *x, const double *v, const orc_fb *b);
+void orc_fb_free(orc_beams *bm);
 void orc_r_bd(double *r, const orc_obj *o, int seid, double zeta, double eta);
 void orc_tz_te_node(double *vtz, double *vte, const orc_obj *o, int seid, int nid);
 

@@ -40,6 +40,29 @@ void orc_api_setup(orc_obj *o, int nb, const double *beams, const double *tv, in
   orc_incident_bd(o);
 }
 
+/* the same with a beam set of any multi_fbeam types, 16 doubles per beam
+ * (orc_set_mfb) */
+void orc_api_setup_mfb(orc_obj *o, int nb, const double *beams, const double *tv, int translate)
+{
+  if (translate) { double rv[3] = {1.0, 0.0, 0.0}; orc_obj_translate(o, rv, 0.0, tv); }
+  orc_elem_const(o);
+  orc_node_data(o);
+  orc_set_mfb(o, nb, beams);
+  orc_set_k(o);
+  orc_incident_bd(o);
+}
+
+/* incident E, H (and their derivatives along v, when v != NULL) of the
+ * object's beams at np points: out rows of 6 (12 with v) */
+void orc_api_beam_EH(const orc_obj *o, int np, const double *x, const double *v, double *out)
+{
+  zc *z = (zc *)out;
+  for (int p = 0; p < np; p++) {
+    if (v) orc_beam_EH_dv(z + 12 * p, z + 12 * p + 3, z + 12 * p + 6, z + 12 * p + 9, x + 3 * p, v + 3 * p, &o->bm);
+    else orc_beam_EH(z + 6 * p, z + 6 * p + 3, x + 3 * p, &o->bm);
+  }
+}
+
 /* replace node points / normals and the wave numbers by stored values */
 void orc_api_override(orc_obj *o, const double *ren, const double *wen, const double *kn)
 {

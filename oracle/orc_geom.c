@@ -255,6 +255,7 @@ void orc_obj_free(orc_obj *o)
   if (o->sid) for (int d = 0; d <= o->MN; d++) free(o->sid[d]);
   if (o->U) for (int d = 0; d <= o->MN; d++) { free(o->U[d]); free(o->dU[d]); free(o->E[d]); free(o->dE[d]); free(o->H[d]); free(o->dH[d]); }
   free(o->sid); free(o->dNe); free(o->U); free(o->dU); free(o->E); free(o->dE); free(o->H); free(o->dH);
+  orc_fb_free(&o->bm);
   free(o->bm.w);
   free(o);
 }
@@ -384,6 +385,8 @@ void orc_set_beams(orc_obj *o, int nb, const double *par)
   o->bm.omega = 2.0 * M_PI / o->bm.lambda0;
 }
 
+void orc_gauleg_ab(double a, double b, double *x, double *w, int n) { orc_gauleg(a, b, x, w, n); }
+
 /* medium wave numbers, setup2.c:111-113 */
 void orc_set_k(orc_obj *o)
 {
@@ -401,6 +404,11 @@ void orc_beam_EH(zc *e, zc *h, const double *x, const orc_beams *bm)
     e[0] += w->ex * a; e[1] += w->ey * a; e[2] += w->ez * a;
     h[0] += w->hx * a; h[1] += w->hy * a; h[2] += w->hz * a;
   }
+  for (int i = 0; i < bm->nf; i++) {   /* the other beam types after the plane waves */
+    zc te[3], th[3];
+    orc_fb_EH(te, th, NULL, NULL, x, NULL, &bm->f[i]);
+    for (int j = 0; j < 3; j++) { e[j] += te[j]; h[j] += th[j]; }
+  }
 }
 
 void orc_beam_EH_dv(zc *e, zc *h, zc *de, zc *dh, const double *x, const double *v, const orc_beams *bm)
@@ -413,6 +421,11 @@ void orc_beam_EH_dv(zc *e, zc *h, zc *de, zc *dh, const double *x, const double 
     zc g = I * w->ki * (w->sin_t * w->cos_p * v[0] + w->sin_t * w->sin_p * v[1] + w->cos_t * v[2]);
     zc f[6] = {w->ex * a, w->ey * a, w->ez * a, w->hx * a, w->hy * a, w->hz * a};
     for (int j = 0; j < 3; j++) { e[j] += f[j]; h[j] += f[3 + j]; de[j] += f[j] * g; dh[j] += f[3 + j] * g; }
+  }
+  for (int i = 0; i < bm->nf; i++) {
+    zc te[3], th[3], tde[3], tdh[3];
+    orc_fb_EH(te, th, tde, tdh, x, v, &bm->f[i]);
+    for (int j = 0; j < 3; j++) { e[j] += te[j]; h[j] += th[j]; de[j] += tde[j]; dh[j] += tdh[j]; }
   }
 }
 

@@ -4,8 +4,10 @@ mo_setup.c:147-203; read back by mo_dat_read, mo_setup.c:205-436).
 Binary layout per file: int N, then per object: mod_fn[256], rs[3],
 med_fn[128], msh_fn[128], rv[3], th, tv[3], int MN, complex n[MN+1],
 complex kn[MN+1], a raw Bobj (264 bytes, multi_fbeam.h:28-41) followed by
-the raw beam records (Ipw = 240 bytes, ipw.h:79-99), the boundary tables
-and, per domain, sid and the U/dU/E/dE/H/dH node values.
+the raw beam records of each type in turn (Ipw 240 bytes, ipw.h:79-99; Fpw
+272, fpw.h:14-39; LGb 280, lgb.h:14-41; Bsb 248, bsb.h:13-37; BsLGb 264,
+bslgb.h:13-39; RAb 272, rab.h:13-44), the boundary tables and, per domain,
+sid and the U/dU/E/dE/H/dH node values.
 """
 import gzip
 
@@ -43,6 +45,22 @@ def _ipw_params(rec):
     return np.array([lam, ni, pw, e0x[0], e0x[1], e0y[0], e0y[1], fx, fy, fz, th, ph])
 
 
+# record size and the byte offset of the trailing int members (spiral mode,
+# quadrature points) of the other beam types; the doubles lambda0, ni,
+# NA / deflection angle, power, e0x, e0y, fx, fy, fz, theta, phi come first
+_FB = {1: (272, None, 104), 2: (280, 104, 108), 3: (248, None, None), 4: (264, 104, None), 5: (272, None, 104)}
+
+
+def _fb_params(typ, rec):
+    """one beam record as the 16-vector (type, lambda0, ni, NA or deflection
+    angle, power, re/im e0x, re/im e0y, fx, fy, fz, theta, phi, mode, points)"""
+    d = np.frombuffer(rec, dtype=np.float64, count=13)
+    _, om, on = _FB[typ]
+    m = int(np.frombuffer(rec, np.int32, 1, om)[0]) if om is not None else 0
+    nn = int(np.frombuffer(rec, np.int32, 1, on)[0]) if on is not None else 0
+    return np.concatenate([[typ], d, [m, nn]])
+
+
 def read_dat(path):
     raw = open(path, "rb").read()
     if raw[:2] == b"\x1f\x8b":
@@ -66,9 +84,11 @@ def read_dat(path):
         bobj = r.raw(BOBJ_SIZE)
         counts = [int(np.frombuffer(bobj, np.int32, 1, 32 * i)[0]) for i in range(6)]
         o["n0"], o["lambda0"], o["omega"] = np.frombuffer(bobj, np.float64, 3, 192)
-        if any(counts[1:]):
-            raise ValueError("only plane-wave beams are supported by this reader")
-        o["beams"] = np.array([_ipw_params(r.raw(IPW_SIZE)) for _ in range(counts[0])])
+        o["beams"] = np.array([_ipw_params(r.raw(IPW_SIZE)) for _ in range(counts[0])]).reshape(-1, 12)
+        mfb = [np.concatenate([[0], b[:2], [0.0], b[2:], [0, 0]]) for b in o["beams"]]
+        for typ in range(1, 6):
+            mfb += [_fb_params(typ, r.raw(_FB[typ][0])) for _ in range(counts[typ])]
+        o["mfb"] = np.array(mfb).reshape(-1, 16)
         Nn = int(r.take(np.int32, 1)[0])
         Ne = int(r.take(np.int32, 1)[0])
         o["rn"] = r.take(np.float64, 3 * (Nn + 1)).reshape(Nn + 1, 3)

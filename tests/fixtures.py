@@ -1,6 +1,8 @@
-"""Shared loaders for the reference-held v1.dat fixture (two cones of
-verification1/cone_m1_p5.msh, 28 triangles + 553 quads each, solved by the
-reference; expected outputs in verification1/example1.log)."""
+"""Shared loaders for the reference-held solved-state fixtures: v1.dat (two
+cones of verification1/cone_m1_p5.msh, 28 triangles + 553 quads each, a
+plane wave; expected outputs in verification1/example1.log) and ex30.dat
+(the glass hemisphere of analysis_sample3/hemisphere_m1.msh, 1008 quads, a
+focused plane wave; expected outputs in analysis_sample3/example1_ex30.log)."""
 import os
 
 import numpy as np
@@ -22,12 +24,30 @@ def v1_log():
     return parse_example1_log(os.path.join(GOLDEN, "v1_example1.log"))
 
 
+def ex30_raw():
+    if "ex30" not in _cache:
+        _cache["ex30"] = read_dat(os.path.join(GOLDEN, "ex30.dat.gz"))
+    return _cache["ex30"]
+
+
+def ex30_log():
+    return parse_example1_log(os.path.join(GOLDEN, "ex30_example1.log"))
+
+
+def ex30_oracle():
+    return stored_oracle(ex30_raw())
+
+
 def v1_oracle():
-    """oracle objects loaded with the stored geometry and solved state"""
+    return stored_oracle(v1_raw())
+
+
+def stored_oracle(raw):
+    """oracle objects loaded with the stored geometry, beams and solved state"""
     objs = []
-    for o in v1_raw():
+    for o in raw:
         ob = orc.Obj.from_arrays(o["rn"], o["ed"], o["md"], o["sd"], o["gd"], o["n"])
-        ob.setup(o["beams"])
+        ob.setup(o["mfb"])
         ob.override(ren=o["ren"], wen=o["wen"], kn=o["kn"])
         for d in range(o["MN"] + 1):
             for key in ("U", "dU", "E", "dE", "H", "dH"):
