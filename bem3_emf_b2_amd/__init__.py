@@ -44,6 +44,7 @@ def lib():
             "bh_object_matrix": (C.c_int, [VP, C.c_int, C.c_int, D]),
             "bh_zgesv": (C.c_int, [C.c_int, C.c_int, D, D]),
             "bh_object_set_beams": (C.c_int, [VP, C.c_int, D]),
+            "bh_object_set_mfb": (C.c_int, [VP, C.c_int, D]),
             "bh_object_factor": (C.c_int, [VP, C.c_int]),
             "bh_object_solve": (C.c_int, [VP]),
             "bh_object_solve_EH": (C.c_int, [VP]),
@@ -169,8 +170,15 @@ class Object:
         return g
 
     def set_beams(self, beams):
-        b = np.ascontiguousarray(beams, dtype=np.float64).reshape(-1, 12)
-        check(lib().bh_object_set_beams(self.h, b.shape[0], dp(b)))
+        """rows of 12 (plane waves: ipw.txt columns) or of 16 (any
+        multi_fbeam type: type, lambda0, ni, NA or deflection angle, power,
+        re/im e0x, re/im e0y, fx, fy, fz, theta, phi, spiral mode, points)"""
+        b = np.ascontiguousarray(beams, dtype=np.float64)
+        b = b.reshape(-1, b.shape[-1] if b.ndim > 1 else 12)
+        if b.shape[1] == 16:
+            check(lib().bh_object_set_mfb(self.h, b.shape[0], dp(b)))
+        else:
+            check(lib().bh_object_set_beams(self.h, b.shape[0], dp(b)))
 
     def factor(self, mode=0):
         """mode: 0/"auto", 1/"dense", 2/"block", 3/"mixed" (complex-FP32

@@ -97,10 +97,12 @@ struct Ctx {
 };
 static Ctx g;
 
-// Gauss-Legendre nodes by Newton iteration on P_n (same fixed point as the
-// reference's gauleg.c:3-36: start cos(pi(i-1/4)/(n+1/2)), stop |dz|<=3e-16)
-static void gl_nodes(int n, double *x, double *w)
+// Gauss-Legendre nodes on (a, b) by Newton iteration on P_n (same fixed
+// point as the reference's gauleg.c:3-36: start cos(pi(i-1/4)/(n+1/2)),
+// stop |dz|<=3e-16)
+static void gl_nodes_ab(double a, double b, int n, double *x, double *w)
 {
+  const double mid = 0.5 * (b + a), hl = 0.5 * (b - a);
   int half = (n + 1) / 2;
   for (int i = 1; i <= half; i++) {
     double z = cos(M_PI * (i - 0.25) / (n + 0.5)), pd;
@@ -116,11 +118,12 @@ static void gl_nodes(int n, double *x, double *w)
       z = zo - p0 / pd;
       if (fabs(z - zo) <= 3.0e-16) break;
     }
-    x[i - 1] = -z; x[n - i] = z;
-    w[i - 1] = 2.0 / ((1.0 - z * z) * pd * pd); w[n - i] = w[i - 1];
+    x[i - 1] = mid - hl * z; x[n - i] = mid + hl * z;
+    w[i - 1] = 2.0 * hl / ((1.0 - z * z) * pd * pd); w[n - i] = w[i - 1];
   }
-  if (n % 2) x[n / 2] = 0.0;
+  if (a == -b && (n % 2)) x[n / 2] = 0.0;
 }
+static void gl_nodes(int n, double *x, double *w) { gl_nodes_ab(-1.0, 1.0, n, x, w); }
 
 // 3-part map of the equilateral triangle onto three [-1,1]^2 squares
 static std::vector<QPt> tri_rule(int n, const double *x, const double *w)
@@ -229,7 +232,10 @@ struct bh_object {
   double *d_fv = nullptr;
   int *d_fsd = nullptr, *d_fq = nullptr, *d_seg = nullptr;
   // beams
-  std::vector<Beam> beams;
+  std::vector<Beam> beams;             // plane waves
+  std::vector<FBeam> fbeams;           // the other multi_fbeam types
+  FBeam *d_fb = nullptr;
+  double *d_fbt = nullptr;             // their spectral angles (cos, sin, weight x sin)
   Beam *d_beams = nullptr;
   double lambda0 = 0.0;
   int prec = BH_PREC_FP64;             // arithmetic of assembly, field sums and factor mode 0
@@ -455,7 +461,7 @@ extern "C" void bh_object_destroy(bh_object *o)
   dfree(o->d_geo); dfree(o->d_gq);
   dfree(o->d_Ui); dfree(o->d_dUi); dfree(o->d_Ui0); dfree(o->d_dUi0); dfree(o->d_Uip);
   dfree(o->d_slotPt); dfree(o->d_slotN); dfree(o->d_slotIdx); dfree(o->d_fv); dfree(o->d_fsd); dfree(o->d_fq); dfree(o->d_seg);
-  dfree(o->d_beams);
+  dfree(o->d_beams); dfree(o->d_fb); dfree(o->d_fbt);
   dfree(o->d_A); dfree(o->d_work); dfree(o->d_piv); dfree(o->d_info);
   dfree(o->d_G0f); dfree(o->d_Z); dfree(o->d_Mf); dfree(o->d_Sf);
   dfree(o->d_piv0); dfree(o->d_pivM); dfree(o->d_pivS);
@@ -721,6 +727,9 @@ void bem_launch_field_bd(const int *, const double *, int, int, const void *, co
 void bem_launch_incident_bd(const double *, const double *, const int *, int, const void *, int, void *, void *, void *,
                             void *, void *, hipStream_t);
 void bem_launch_add_incident(const double *, const int *, int, const void *, int, void *, int, hipStream_t);
+void bem_launch_fbeam(const double *, const double *, const int *, int, const void *, int, const double *,
+                      const double *, int *, int, void *, const int *, void *, void *, void *, void *, void *,
+                      hipStream_t);
 void bem_launch_stress(const void *, const double *, const double *, const double *, int, double, const double *,
                        double *, hipStream_t);
 void bem_launch_poynting(const void *, const double *, const double *, int, double *, hipStream_t);
@@ -890,8 +899,100 @@ static int prepare_solve(bh_object *o)
   return 0;
 }
 
-// plane waves: 12 doubles per beam (lambda0, ni, power, re/im e0x, re/im e0y,
+// one plane wave from 12 doubles (lambda0, ni, power, re/im e0x, re/im e0y,
 // fx, fy, fz, theta, phi); derived data as setup_ipw (ipw.c:65-89)
+static Beam plane_wave(const double *p)
+{
+  zd e0x(p[3], p[4]), e0y(p[5], p[6]);
+  double nrm = 1.0 / sqrt(std::norm(e0x) + std::norm(e0y));
+  zd ex = e0x * nrm, ey = e0y * nrm, hx = -ey * p[1], hy = ex * p[1];
+  double st = sin(p[10]), ct = cos(p[10]), sp = sin(p[11]), cp = cos(p[11]);
+  double a11 = sp * sp + cp * cp * ct, a12 = sp * cp * ct - sp * cp, a22 = sp * sp * ct + cp * cp;
+  zd vex = ex * a11 + ey * a12, vey = ex * a12 + ey * a22, vez = -(ex * cp + ey * sp) * st;
+  zd vhx = hx * a11 + hy * a12, vhy = hx * a12 + hy * a22, vhz = -(hx * cp + hy * sp) * st;
+  Beam b;
+  b.ki = 2.0 * M_PI * p[1] / p[0];
+  b.E0 = sqrt(p[2] * 2.0 / p[1]);
+  b.st = st; b.ct = ct; b.sp = sp; b.cp = cp; b.fx = p[7]; b.fy = p[8]; b.fz = p[9]; b.pad = 0;
+  b.ex = mk(vex.real(), vex.imag()); b.ey = mk(vey.real(), vey.imag()); b.ez = mk(vez.real(), vez.imag());
+  b.hx = mk(vhx.real(), vhx.imag()); b.hy = mk(vhy.real(), vhy.imag()); b.hz = mk(vhz.real(), vhz.imag());
+  return b;
+}
+
+// phi points of the trapezoid in the order they are added (-pi, then the
+// midpoints of each halving, fpw.c:140-156): cos, sin, angle; and the
+// error flag of the phi integrals
+static double *g_phit = nullptr;
+static int *g_fberr = nullptr;
+static int ensure_phi_table()
+{
+  if (g_phit) return 0;
+  std::vector<double> t(3 * (size_t)BEM_NPHI, 0.0);
+  double h = 2.0 * M_PI;
+  t[0] = -1.0; t[1] = 0.0; t[2] = -M_PI;
+  int j = 1, st = 1;
+  for (int i = 0; i <= BEM_TRAP_MNUM; i++) {
+    h *= 0.5;
+    for (int q = 1; q <= st; q++, j++) {
+      const double a = -M_PI + (double)(2 * q - 1) * h;
+      t[3 * j] = cos(a); t[3 * j + 1] = sin(a); t[3 * j + 2] = a;
+    }
+    st *= 2;
+  }
+  if (upload(&g_phit, t.data(), t.size())) return BH_EHIP;
+  HIPCHK(hipMalloc((void **)&g_fberr, sizeof(int)));
+  HIPCHK(hipMemset(g_fberr, 0, sizeof(int)));
+  return 0;
+}
+
+// the phi-integral error flag (read and cleared, synchronises the stream)
+static int fbeam_failed()
+{
+  if (!g_fberr) return 0;
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, g_fberr, sizeof(int), hipMemcpyDeviceToHost, g.st));
+  HIPCHK(hipStreamSynchronize(g.st));
+  if (!h) return 0;
+  HIPCHK(hipMemsetAsync(g_fberr, 0, sizeof(int), g.st));
+  return fail(BH_ECONV, "phi integration limit over (a beam's phi integral did not converge in 2^13 intervals)");
+}
+
+// incident fields of an object's beams added to np points (mask: points
+// that take it; what 0: E, H rows of 6, 1: E into rows of 4)
+static void add_incident(const bh_object *o, const double *d_pts, const int *d_mask, int np, cd *d_out, int what)
+{
+  bem_launch_add_incident(d_pts, d_mask, np, o->d_beams, (int)o->beams.size(), d_out, what, g.st);
+  if (!o->fbeams.empty())
+    bem_launch_fbeam(d_pts, nullptr, d_mask, np, o->d_fb, (int)o->fbeams.size(), o->d_fbt, g_phit, g_fberr, what,
+                     d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, g.st);
+}
+
+// a beam set in place: device copies, then the incident boundary data of the
+// open-domain node slots (plane waves, then the other beams added)
+static int install_beams(bh_object *o, const std::vector<Beam> &bm, const std::vector<FBeam> &fb,
+                         const std::vector<double> &tht, double lam)
+{
+  if (!fb.empty() && ensure_phi_table()) return BH_EHIP;
+  o->beams = bm;
+  o->fbeams = fb;
+  o->lambda0 = lam;
+  dfree(o->d_beams); dfree(o->d_fb); dfree(o->d_fbt);
+  o->d_beams = nullptr; o->d_fb = nullptr; o->d_fbt = nullptr;
+  const Beam none = {};
+  if (bm.empty() ? upload(&o->d_beams, &none, 1) : upload(&o->d_beams, bm.data(), bm.size())) return BH_EHIP;
+  if (!fb.empty() && (upload(&o->d_fb, fb.data(), fb.size()) || upload(&o->d_fbt, tht.data(), tht.size())))
+    return BH_EHIP;
+  if (prepare_solve(o)) return BH_EHIP;
+  bem_launch_incident_bd(o->d_slotPt, o->d_slotN, o->d_slotIdx, o->nslot, o->d_beams, (int)bm.size(), o->d_Ui, o->d_dUi,
+                         o->d_Ui0, o->d_dUi0, o->d_Uip, g.st);
+  if (!fb.empty())
+    bem_launch_fbeam(o->d_slotPt, o->d_slotN, nullptr, o->nslot, o->d_fb, (int)fb.size(), o->d_fbt, g_phit, g_fberr, 2,
+                     nullptr, o->d_slotIdx, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, g.st);
+  HIPCHK(hipGetLastError());
+  return fbeam_failed();
+}
+
+// plane waves only: 12 doubles per beam (ipw.txt columns)
 extern "C" int bh_object_set_beams(bh_object *o, int nb, const double *par)
 {
   API_LOCK;
@@ -901,31 +1002,78 @@ extern "C" int bh_object_set_beams(bh_object *o, int nb, const double *par)
   for (int i = 0; i < nb; i++) {
     const double *p = par + 12 * i;
     if (p[0] != lam || p[1] != n0) return fail(BH_EARG, "beam %d: wavelength / medium differ from beam 0", i);
-    zd e0x(p[3], p[4]), e0y(p[5], p[6]);
-    double nrm = 1.0 / sqrt(std::norm(e0x) + std::norm(e0y));
-    zd ex = e0x * nrm, ey = e0y * nrm, hx = -ey * p[1], hy = ex * p[1];
-    double st = sin(p[10]), ct = cos(p[10]), sp = sin(p[11]), cp = cos(p[11]);
-    double a11 = sp * sp + cp * cp * ct, a12 = sp * cp * ct - sp * cp, a22 = sp * sp * ct + cp * cp;
-    zd vex = ex * a11 + ey * a12, vey = ex * a12 + ey * a22, vez = -(ex * cp + ey * sp) * st;
-    zd vhx = hx * a11 + hy * a12, vhy = hx * a12 + hy * a22, vhz = -(hx * cp + hy * sp) * st;
-    Beam &b = bm[i];
-    b.ki = 2.0 * M_PI * p[1] / p[0];
-    b.E0 = sqrt(p[2] * 2.0 / p[1]);
-    b.st = st; b.ct = ct; b.sp = sp; b.cp = cp; b.fx = p[7]; b.fy = p[8]; b.fz = p[9]; b.pad = 0;
-    b.ex = mk(vex.real(), vex.imag()); b.ey = mk(vey.real(), vey.imag()); b.ez = mk(vez.real(), vez.imag());
-    b.hx = mk(vhx.real(), vhx.imag()); b.hy = mk(vhy.real(), vhy.imag()); b.hz = mk(vhz.real(), vhz.imag());
+    bm[i] = plane_wave(p);
   }
-  o->beams = bm;
-  o->lambda0 = lam;
-  dfree(o->d_beams);
-  o->d_beams = nullptr;
-  if (upload(&o->d_beams, bm.data(), bm.size())) return BH_EHIP;
-  if (prepare_solve(o)) return BH_EHIP;
-  bem_launch_incident_bd(o->d_slotPt, o->d_slotN, o->d_slotIdx, o->nslot, o->d_beams, nb, o->d_Ui, o->d_dUi, o->d_Ui0,
-                         o->d_dUi0, o->d_Uip, g.st);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(g.st));
-  return 0;
+  return install_beams(o, bm, std::vector<FBeam>(), std::vector<double>(), lam);
+}
+
+// a beam set of any multi_fbeam types, 16 doubles per beam: type (0 plane
+// wave, 1 focused plane wave, 2 focused with spiral phase, 3 Bessel, 4
+// Bessel with spiral phase, 5 radially / azimuthally polarised focused),
+// lambda0, ni, NA (types 1, 2, 5) or deflection angle (3, 4), power, re/im
+// e0x (e0r), re/im e0y (e0a), fx, fy, fz, theta, phi, spiral mode (2, 4),
+// Gauss-Legendre points over the aperture (1, 2, 5).  Derived data as
+// setup_ipw, setup_Fpw (fpw.c:115-157), setup_LGb (lgb.c:116-164),
+// setup_Bsb (bsb.c:112-148), setup_BsLGb (bslgb.c:115-156), setup_rab
+// (rab.c:111-158); all beams share the medium index and wavelength
+// (check_data_mfb, multi_fbeam.c:878-962).
+extern "C" int bh_object_set_mfb(bh_object *o, int nb, const double *par)
+{
+  API_LOCK;
+  if (nb <= 0) return fail(BH_EARG, "no beams defined");
+  const double lam = par[1], n0 = par[2];
+  std::vector<Beam> bm;
+  std::vector<FBeam> fb;
+  std::vector<double> tht;
+  for (int i = 0; i < nb; i++) {
+    const double *p = par + 16 * i;
+    const int t = (int)p[0];
+    if (p[1] != lam || p[2] != n0) return fail(BH_EARG, "beam %d: wavelength / medium differ from beam 0", i);
+    if (t == 0) {
+      const double w[12] = {p[1], p[2], p[4], p[5], p[6], p[7], p[8], p[9], p[10], p[11], p[12], p[13]};
+      bm.push_back(plane_wave(w));
+      continue;
+    }
+    if (t < 0 || t > 5) return fail(BH_EARG, "beam %d: unknown beam type %d", i, t);
+    FBeam b;
+    memset(&b, 0, sizeof b);
+    b.type = t;
+    b.ni = p[2];
+    b.ki = 2.0 * M_PI * p[2] / p[1];
+    b.cos_t = cos(p[12]); b.sin_t = sin(p[12]); b.cos_p = cos(p[13]); b.sin_p = sin(p[13]);
+    b.fx = p[9]; b.fy = p[10]; b.fz = p[11];
+    b.lg_m = (t == 2 || t == 4) ? (int)p[14] : 0;
+    const zd e0x(p[5], p[6]), e0y(p[7], p[8]);
+    const double a0 = sqrt(std::norm(e0x) + std::norm(e0y));
+    zd ex, ey;
+    if (t == 3 || t == 4) {
+      b.st = sin(p[3]); b.ct = cos(p[3]);
+      const double ce = 1.0 / a0;
+      ex = e0x * ce; ey = e0y * ce;
+      b.E0 = sqrt(2.0 * p[4] / p[2]);
+    } else {
+      const int nn = (int)p[15];
+      if (nn < 1 || nn > 4096) return fail(BH_EARG, "beam %d: %d Gauss-Legendre points", i, nn);
+      const double kr = p[3] / p[2], ce = sqrt(1.0 / M_PI) / (kr * a0);
+      ex = e0x * ce; ey = e0y * ce;
+      b.E0 = sqrt(2.0 * p[4] * p[2]) / p[1];
+      b.nt = nn;
+      b.toff = (int)(tht.size() / 3);
+      std::vector<double> x(nn), w(nn);
+      gl_nodes_ab(0.0, kr, nn, x.data(), w.data());
+      for (int q = 0; q < nn; q++) {
+        tht.push_back(sqrt(1.0 - x[q] * x[q]));
+        tht.push_back(x[q]);
+        tht.push_back(w[q] * x[q]);
+      }
+    }
+    const zd hx = -ey * p[2], hy = ex * p[2];
+    b.ex = mk(ex.real(), ex.imag()); b.ey = mk(ey.real(), ey.imag());
+    b.hx = mk(hx.real(), hx.imag()); b.hy = mk(hy.real(), hy.imag());
+    fb.push_back(b);
+  }
+  if (tht.empty()) tht.assign(3, 0.0);   // Bessel beams only: no aperture rule
+  return install_beams(o, bm, fb, tht, lam);
 }
 
 static size_t row_base(const bh_object *o, int d)
@@ -1341,9 +1489,10 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
   if (mode != 0) {
     for (int p = 0; p < np; p++) mask[p] = mode == 2 ? 1 : pin[p] < 0;
     HIPCHK(hipMemcpyAsync(d_mask, mask.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
-    bem_launch_add_incident(d_pts, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_out, what == 1, g.st);
+    add_incident(objs[0], d_pts, d_mask, np, d_out, what == 1);
   }
   HIPCHK(hipGetLastError());
+  if (mode != 0 && fbeam_failed()) return BH_ECONV;
   HIPCHK(hipMemcpyAsync(out, d_out, (size_t)np * nc * sizeof(cd), hipMemcpyDeviceToHost, g.st));
   HIPCHK(hipStreamSynchronize(g.st));
   if (oid) memcpy(oid, pin.data(), np * sizeof(int));
@@ -1496,9 +1645,10 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
     std::vector<int> ones(np, 1);
     HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
   }
-  bem_launch_add_incident(d_r, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_EH, 0, g.st);
+  add_incident(objs[0], d_r, d_mask, np, d_EH, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g.st));
+  if (fbeam_failed()) return BH_ECONV;
   dfree(d_t); dfree(d_st); dfree(d_ze); dfree(d_tmp); dfree(d_rt); dfree(d_did); dfree(d_sel); dfree(d_mask);
   *d_EH_out = d_EH; *d_r_out = d_r; *d_w_out = d_w; *d_wt_out = d_wt;
   return 0;
@@ -1758,7 +1908,8 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
     if (mode == 1 || mode == 2) {
       std::vector<int> ones(np, 1);
       HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
-      bem_launch_add_incident(d_rt, d_mask, np, objs[0]->d_beams, (int)objs[0]->beams.size(), d_out, 0, g.st);
+      add_incident(objs[0], d_rt, d_mask, np, d_out, 0);
+      if (fbeam_failed()) return BH_ECONV;
     }
   }
   HIPCHK(hipGetLastError());

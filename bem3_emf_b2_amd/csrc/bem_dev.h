@@ -163,3 +163,19 @@ struct Beam {
   double ki, E0, st, ct, sp, cp, fx, fy, fz, pad;
   cd ex, ey, ez, hx, hy, hz;
 };
+
+// one beam of the other multi_fbeam types: 1 focused plane wave, 2 focused
+// with spiral phase, 3 Bessel, 4 Bessel with spiral phase, 5 radially /
+// azimuthally polarised focused (ex, ey, hx, hy then hold the radial and
+// azimuthal components); derived data as the set-up functions of fpw.c,
+// lgb.c, bsb.c, bslgb.c, rab.c.  toff: offset of the beam's spectral angles
+// (cos, sin, weight x sin per point, nt points) in the angle table.
+struct FBeam {
+  int type, nt, lg_m, toff;
+  double ki, E0, ni, pad;
+  double cos_t, sin_t, cos_p, sin_p, fx, fy, fz;
+  double ct, st;                  // cone angle of a Bessel beam
+  cd ex, ey, hx, hy;
+};
+#define BEM_TRAP_MNUM 12                         // halvings of the phi trapezoid (mfb_const.h:4)
+#define BEM_NPHI ((2 << BEM_TRAP_MNUM) + 1)      // its points

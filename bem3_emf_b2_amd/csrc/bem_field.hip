@@ -887,6 +887,195 @@ __global__ void k_add_incident(const double *__restrict__ pts, const int *__rest
     for (int j = 0; j < 3; j++) out[4 * (size_t)p + j] = out[4 * (size_t)p + j] + e[j];
 }
 
+// ---------------------------------------------------------------- other beams
+// E, H (and their derivatives along v) of the focused, spiral-phase, Bessel
+// and radially / azimuthally polarised beams (fpw.c:3-382 and the same
+// functions of lgb.c, bsb.c, bslgb.c, rab.c): a spectrum of plane waves over
+// the aperture (Gauss-Legendre points in sin(theta), or the one cone angle of
+// a Bessel beam) and, per spectral angle, the reference's adaptive composite
+// trapezoid in phi: halving steps over [-pi, pi) until the change of
+// n^2 |E|^2 + |H|^2 falls below 1e-15 sqrt(points) times its largest value,
+// after at least two halvings; more than 12 halvings is an error.  One
+// wavefront per point, one lane per spectral angle.
+__device__ __forceinline__ void fb_ac(const FBeam &b, double ct, double st, const double *__restrict__ phit, int j,
+                                      const double *x, cd *eh)
+{
+  const double cp = phit[3 * j], sp = phit[3 * j + 1];
+  double sa, ca;
+  sincos(b.ki * (st * cp * x[0] + st * sp * x[1] + ct * x[2]), &sa, &ca);
+  cd ce = mk(ca, sa);
+  if ((b.type == 2 || b.type == 4) && b.lg_m != 0) {
+    // exp(i m phi) with phi taken into [0, 2 pi) (lgb.c:177-187)
+    const double a = phit[3 * j + 2], pp = a < 0.0 ? a + 2.0 * M_PI : a;
+    double sm, cm;
+    sincos(fmod((double)b.lg_m * pp, 2.0 * M_PI), &sm, &cm);
+    ce = ce * mk(cm, sm);
+  }
+  if (b.type == 5) {
+    eh[0] = (b.ex * (ct * cp) - b.ey * sp) * ce;
+    eh[1] = (b.ex * (ct * sp) + b.ey * cp) * ce;
+    eh[2] = (b.ex * (-st)) * ce;
+    eh[3] = (b.hx * (ct * cp) - b.hy * sp) * ce;
+    eh[4] = (b.hx * (ct * sp) + b.hy * cp) * ce;
+    eh[5] = (b.hx * (-st)) * ce;
+  } else {
+    const double m11 = sp * sp + ct * cp * cp, m12 = sp * cp * (ct - 1.0), m22 = ct * sp * sp + cp * cp;
+    eh[0] = (b.ex * m11 + b.ey * m12) * ce;
+    eh[1] = (b.ex * m12 + b.ey * m22) * ce;
+    eh[2] = (b.ex * (-st * cp) - b.ey * (st * sp)) * ce;
+    eh[3] = (b.hx * m11 + b.hy * m12) * ce;
+    eh[4] = (b.hx * m12 + b.hy * m22) * ce;
+    eh[5] = (b.hx * (-st * cp) - b.hy * (st * sp)) * ce;
+  }
+}
+
+// phi integral at one spectral angle; dIt: the derivative along v (each
+// component times i k (direction . v)); false when it does not converge
+template <bool DV>
+__device__ bool fb_phi(const FBeam &b, double ct, double st, const double *__restrict__ phit, const double *x,
+                       const double *v, cd *It, cd *dIt)
+{
+  const double cep = b.ni * b.ni;
+  cd Ia[6];
+  double h = 2.0 * M_PI;
+  fb_ac(b, ct, st, phit, 0, x, Ia);
+  auto gfc = [&](int j) {
+    const double cp = phit[3 * j], sp = phit[3 * j + 1];
+    return mk(0.0, b.ki * (st * cp * v[0] + st * sp * v[1] + ct * v[2]));
+  };
+  double abst = 0.0;
+  {
+    const cd g = DV ? gfc(0) : mk(0, 0);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      It[i] = Ia[i] * h;
+      if (DV) dIt[i] = (Ia[i] * g) * h;
+      abst += (i < 3 ? cep : 1.0) * (It[i].x * It[i].x + It[i].y * It[i].y);
+    }
+  }
+  double f_order = abst;
+  int j = 1, sc = 1, cc = 0;
+  while (cc <= BEM_TRAP_MNUM) {
+    h *= 0.5;
+    cd Is[6], dIs[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { Is[i] = mk(0, 0); dIs[i] = mk(0, 0); }
+    for (int s = 1; s <= sc; s++, j++) {
+      fb_ac(b, ct, st, phit, j, x, Ia);
+      const cd g = DV ? gfc(j) : mk(0, 0);
+#pragma unroll
+      for (int i = 0; i < 6; i++) { Is[i] = Is[i] + Ia[i]; if (DV) dIs[i] = dIs[i] + Ia[i] * g; }
+    }
+    double abss = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      It[i] = It[i] * 0.5 + Is[i] * h;
+      if (DV) dIt[i] = dIt[i] * 0.5 + dIs[i] * h;
+      abss += (i < 3 ? cep : 1.0) * (It[i].x * It[i].x + It[i].y * It[i].y);
+    }
+    const double crc = sqrt((double)sc) * f_order;
+    if ((cc > 1 && fabs(abst - abss) < 1.0e-15 * crc) || (abss == 0.0 && abst == 0.0)) break;
+    abst = abss;
+    if (abst > f_order) f_order = abst;
+    sc *= 2;
+    cc++;
+  }
+  return cc < BEM_TRAP_MNUM;
+}
+
+// rotation into the beam frame (fpw.c:13-17) and back (fpw.c:21-26)
+__device__ __forceinline__ void fb_rot(const FBeam &b, const double *a, double *o)
+{
+  const double ct = b.cos_t, st = b.sin_t, cp = b.cos_p, sp = b.sin_p;
+  o[0] = a[0] * (ct * cp * cp + sp * sp) + a[1] * sp * cp * (ct - 1.0) - a[2] * st * cp;
+  o[1] = a[0] * sp * cp * (ct - 1.0) + a[1] * (ct * sp * sp + cp * cp) - a[2] * st * sp;
+  o[2] = a[0] * st * cp + a[1] * st * sp + a[2] * ct;
+}
+__device__ __forceinline__ void fb_back(const FBeam &b, const cd *a, cd *o)
+{
+  const double ct = b.cos_t, st = b.sin_t, cp = b.cos_p, sp = b.sin_p;
+  o[0] = a[0] * (ct * cp * cp + sp * sp) + a[1] * (sp * cp * (ct - 1.0)) + a[2] * (st * cp);
+  o[1] = a[0] * (sp * cp * (ct - 1.0)) + a[1] * (ct * sp * sp + cp * cp) + a[2] * (st * sp);
+  o[2] = a[0] * (-st * cp) - a[1] * (st * sp) + a[2] * ct;
+}
+
+// mode 0: add E, H to rows of 6 of out; 1: add E to rows of 4 (the
+// potentials' vector part); 2: add E (and dE along nrm) to the incident
+// boundary data of the node slots slot[p] (components 0-2 of Ui, Ui0, Uip
+// and dUi, dUi0)
+template <bool DV>
+__global__ void __launch_bounds__(256) k_fbeam(const double *__restrict__ pts, const double *__restrict__ nrm,
+                                               const int *__restrict__ mask, int np, const FBeam *__restrict__ fb,
+                                               int nf, const double *__restrict__ tht, const double *__restrict__ phit,
+                                               int *__restrict__ err, int mode, cd *__restrict__ out,
+                                               const int *__restrict__ slot, cd *__restrict__ Ui,
+                                               cd *__restrict__ dUi, cd *__restrict__ Ui0, cd *__restrict__ dUi0,
+                                               cd *__restrict__ Uip)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  if (mask && !mask[p]) return;
+  const double x[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  const double v[3] = {DV ? nrm[3 * p] : 0.0, DV ? nrm[3 * p + 1] : 0.0, DV ? nrm[3 * p + 2] : 0.0};
+  cd E[3], H[3], dE[3], dH[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) { E[i] = H[i] = dE[i] = dH[i] = mk(0, 0); }
+  bool ok = true;
+  for (int ib = 0; ib < nf; ib++) {
+    const FBeam b = fb[ib];
+    const double xb[3] = {x[0] - b.fx, x[1] - b.fy, x[2] - b.fz};
+    double xn[3], vn[3] = {0.0, 0.0, 0.0};
+    fb_rot(b, xb, xn);
+    if (DV) fb_rot(b, v, vn);
+    const bool bessel = b.type == 3 || b.type == 4;
+    const int nth = bessel ? 1 : b.nt;
+    cd s[6], ds[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) { s[i] = mk(0, 0); ds[i] = mk(0, 0); }
+    for (int j = lane; j < nth; j += 64) {
+      const double ct = bessel ? b.ct : tht[3 * (b.toff + j)];
+      const double st = bessel ? b.st : tht[3 * (b.toff + j) + 1];
+      const double w = bessel ? b.st : tht[3 * (b.toff + j) + 2];
+      cd It[6], dIt[6];
+      ok = fb_phi<DV>(b, ct, st, phit, xn, vn, It, dIt) && ok;
+#pragma unroll
+      for (int i = 0; i < 6; i++) { s[i] = s[i] + It[i] * w; if (DV) ds[i] = ds[i] + dIt[i] * w; }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) { s[i] = wsum(s[i]); if (DV) ds[i] = wsum(ds[i]); }
+    cd te[3], th[3];
+    fb_back(b, s, te);
+    fb_back(b, s + 3, th);
+#pragma unroll
+    for (int i = 0; i < 3; i++) { E[i] = E[i] + te[i] * b.E0; H[i] = H[i] + th[i] * b.E0; }
+    if (DV) {
+      fb_back(b, ds, te);
+      fb_back(b, ds + 3, th);
+#pragma unroll
+      for (int i = 0; i < 3; i++) { dE[i] = dE[i] + te[i] * b.E0; dH[i] = dH[i] + th[i] * b.E0; }
+    }
+  }
+  if (!ok) atomicOr(err, 1);
+  if (lane != 0) return;
+  if (mode == 0) {
+    cd *o = out + 6 * (size_t)p;
+#pragma unroll
+    for (int i = 0; i < 3; i++) { o[i] = o[i] + E[i]; o[3 + i] = o[3 + i] + H[i]; }
+  } else if (mode == 1) {
+    cd *o = out + 4 * (size_t)p;
+#pragma unroll
+    for (int i = 0; i < 3; i++) o[i] = o[i] + E[i];
+  } else {
+    const size_t o = (size_t)slot[p] * 4;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      Ui[o + i] = Ui[o + i] + E[i]; Ui0[o + i] = Ui0[o + i] + E[i]; Uip[o + i] = Uip[o + i] + E[i];
+      dUi[o + i] = dUi[o + i] + dE[i]; dUi0[o + i] = dUi0[o + i] + dE[i];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- Maxwell stress
 // per surface point: T = 1/2 (eps E E* + H H*) - 1/4 (eps|E|^2 + |H|^2) I;
 // contributions w * (T . W) and w * (r - rc) x (T . W), scaled per element
@@ -1011,6 +1200,22 @@ void bem_launch_add_incident(const double *pts, const int *mask, int np, const v
   if (np <= 0) return;
   hipLaunchKernelGGL(k_add_incident, dim3((np + 255) / 256), dim3(256), 0, st, pts, mask, np, (const Beam *)bm, nb,
                      (cd *)out, what);
+}
+// the other beams' fields at np points (see k_fbeam for mode); err is set
+// when a phi integral does not converge
+void bem_launch_fbeam(const double *pts, const double *nrm, const int *mask, int np, const void *fb, int nf,
+                      const double *tht, const double *phit, int *err, int mode, void *out, const int *slot, void *Ui,
+                      void *dUi, void *Ui0, void *dUi0, void *Uip, hipStream_t st)
+{
+  if (np <= 0 || nf <= 0) return;
+  bem_prof_begin(BH_K_FIELD, (double)np * nf, st);
+  if (nrm)
+    hipLaunchKernelGGL(k_fbeam<true>, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, mask, np, (const FBeam *)fb, nf,
+                       tht, phit, err, mode, (cd *)out, slot, (cd *)Ui, (cd *)dUi, (cd *)Ui0, (cd *)dUi0, (cd *)Uip);
+  else
+    hipLaunchKernelGGL(k_fbeam<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, mask, np, (const FBeam *)fb,
+                       nf, tht, phit, err, mode, (cd *)out, slot, (cd *)Ui, (cd *)dUi, (cd *)Ui0, (cd *)dUi0, (cd *)Uip);
+  bem_prof_end(BH_K_FIELD, st);
 }
 void bem_launch_stress(const void *EH, const double *rpt, const double *wpt, const double *wgt, int np, double epd,
                        const double *rc, double *part, hipStream_t st)

@@ -1,7 +1,7 @@
 /*
  * bem_host.c -- host side of the drop-in bem3_emf_b2 API (include/bem3_emf_b2.h).
  *
- * Reads the reference's input files (medium, Gmsh 2.2 mesh, plane waves,
+ * Reads the reference's input files (medium, Gmsh 2.2 mesh, the six beam files,
  * object settings), keeps the reference's in-memory tables (so callers that
  * read MOBJ / DOMD keep working) and the reference's binary files (.obj of
  * create_matrix, .dat of the solver), and runs every computation of the
@@ -12,15 +12,15 @@
  *    written in the reference layout, the coefficient-matrix files only when
  *    BEM3_WRITE_CMAT=1 (mo_initialize rebuilds the matrices on the GPU from
  *    the .obj geometry, which is faster than reading them back);
- *  - only plane-wave beams (ipw.txt) and linear-triangle meshes are
- *    accepted; other inputs stop with a message.
+ *  - the beam records' derived-data pointers (quadrature tables) are left
+ *    NULL: the device builds its own tables.
  */
 #include "../../../include/bem3_emf_b2.h"
 #include "../../../include/bem3_hip.h"
 
+#include <stddef.h>
 #include <time.h>
 
-#define BIPW_SIZE 240   /* sizeof(Ipw) of the reference build */
 
 void *m_alloc2(size_t num, size_t size, char *txt)
 {
@@ -46,103 +46,262 @@ static double vabs3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v
 static void vunit3(double *v) { double a = vabs3(v); if (a != 0.0) { v[0] /= a; v[1] /= a; v[2] /= a; } }
 
 /* ------------------------------------------------------------ beams */
+/* The six beam files of multi_fbeam (multi_fbeam.h:10-16, read by
+ * read_data_mfb_*, multi_fbeam.c:145-762).  Every record starts with the
+ * same doubles -- lambda0, ni, [NA or deflection angle,] power, e0x, e0y,
+ * fx, fy, fz, theta, phi -- then the type's int parameters; the reader fills
+ * those members in that order, as the reference's readers do. */
+enum { BT_IPW, BT_FPW, BT_LGB, BT_BSB, BT_BLG, BT_RAB, BT_N };
+typedef struct {
+  const char *file, *title;
+  size_t size;
+  int nd;               /* leading doubles (complex values as two) */
+  int nint;             /* trailing int members */
+  size_t oint[2];       /* their offsets */
+} btype_t;
+static const btype_t BT[BT_N] = {
+  {"ipw.txt", "plane wave", sizeof(Ipw), 12, 0, {0, 0}},
+  {"fpw.txt", "focused plane wave", sizeof(Fpw), 13, 1, {offsetof(Fpw, nn), 0}},
+  {"lgb.txt", "focused plane wave with spiral phase modulation", sizeof(LGb), 13, 2, {offsetof(LGb, lg_m), offsetof(LGb, nn)}},
+  {"bsb.txt", "Bessel beam", sizeof(Bsb), 13, 0, {0, 0}},
+  {"blg.txt", "Bessel beam with spiral phase modulation", sizeof(BsLGb), 13, 1, {offsetof(BsLGb, lg_m), 0}},
+  {"rab.txt", "focused radially and azimuthally polarized beam", sizeof(RAb), 13, 1, {offsetof(RAb, nn), 0}},
+};
+_Static_assert(offsetof(Fpw, phi) == 96 && offsetof(LGb, phi) == 96 && offsetof(Bsb, phi) == 96 &&
+               offsetof(BsLGb, phi) == 96 && offsetof(RAb, phi) == 96 && offsetof(Ipw, phi) == 88,
+               "beam records start with their doubles");
+
+static int *bt_count(Bobj *b, int t)
+{
+  int *c[BT_N] = {&b->n_ipw, &b->n_fpw, &b->n_lgb, &b->n_bsb, &b->n_blg, &b->n_rab};
+  return c[t];
+}
+static char *bt_fname(Bobj *b, int t)
+{
+  char *c[BT_N] = {b->fname_ipw, b->fname_fpw, b->fname_lgb, b->fname_bsb, b->fname_blg, b->fname_rab};
+  return c[t];
+}
+static void **bt_data(Bobj *b, int t)
+{
+  void **c[BT_N] = {(void **)&b->bd.ipw, (void **)&b->bd.fpw, (void **)&b->bd.lgb, (void **)&b->bd.bsb,
+                    (void **)&b->bd.blg, (void **)&b->bd.rab};
+  return c[t];
+}
+static char *bt_rec(Bobj *b, int t, int i) { return (char *)*bt_data(b, t) + BT[t].size * (size_t)i; }
+
 static void beams_init(Bobj *b)
 {
   memset(b, 0, sizeof *b);
-  strcpy(b->fname_ipw, "null"); strcpy(b->fname_fpw, "null"); strcpy(b->fname_lgb, "null");
-  strcpy(b->fname_bsb, "null"); strcpy(b->fname_blg, "null"); strcpy(b->fname_rab, "null");
-  b->bd.ipw = xalloc(1, sizeof(Ipw), "beams");
+  for (int t = 0; t < BT_N; t++) {
+    strcpy(bt_fname(b, t), "null");
+    *bt_data(b, t) = xalloc(1, BT[t].size, "beams");
+  }
 }
 
-/* ipw.txt in the working directory (multi_fbeam.c:145-238); the other beam
- * files of the reference are rejected when they define beams */
-static void beams_read(Bobj *b)
+static void beams_free(Bobj *b)
 {
-  static const char *other[5] = {"fpw.txt", "lgb.txt", "bsb.txt", "blg.txt", "rab.txt"};
-  for (int i = 0; i < 5; i++) {
-    FILE *fo = fopen(other[i], "r");
-    if (!fo) continue;
-    char buf[256];
-    int n = 0;
-    if (fgets(buf, sizeof buf, fo) && fgets(buf, sizeof buf, fo) && fscanf(fo, "%d", &n) == 1 && n > 0) {
-      fclose(fo);
-      printf("beam file %s: only plane waves (ipw.txt) are supported by this build. Exit...\n", other[i]);
-      exit(1);
-    }
-    fclose(fo);
-  }
-  FILE *fp = fopen("ipw.txt", "r");
-  if (!fp) { printf("read_data_mfb() No beam defined, check beam data file : ipw.txt.\n"); exit(1); }
-  char buf[256];
+  for (int t = 0; t < BT_N; t++) { free(*bt_data(b, t)); *bt_data(b, t) = NULL; *bt_count(b, t) = 0; }
+}
+
+/* one beam file of the working directory; the number of beams it defines */
+static int beams_read_file(Bobj *b, int t)
+{
+  FILE *fp = fopen(BT[t].file, "r");
+  if (!fp) return 0;
+  char buf[256], msg[160];
   int nn;
+  snprintf(msg, sizeof msg, "multi_fbeam.c, read_data_mfb(), %s: failed to read the header.", BT[t].file);
   if (!fgets(buf, sizeof buf, fp) || !fgets(buf, sizeof buf, fp) || fscanf(fp, "%d\n", &nn) != 1 || !fgets(buf, sizeof buf, fp))
-    die("ipw.txt: failed to read the header.");
-  if (nn <= 0) { printf("read_data_mfb() No beam defined.\n"); exit(1); }
-  free(b->bd.ipw);
-  b->n_ipw = nn;
-  snprintf(b->fname_ipw, sizeof b->fname_ipw, "%s", "ipw.txt");
-  b->bd.ipw = xalloc(nn, sizeof(Ipw), "ipw");
+    die(msg);
+  snprintf(bt_fname(b, t), 28, "%s", BT[t].file);
+  if (nn <= 0) { fclose(fp); return 0; }
+  free(*bt_data(b, t));
+  *bt_data(b, t) = xalloc(nn, BT[t].size, "beams");
+  *bt_count(b, t) = nn;
+  snprintf(msg, sizeof msg, "multi_fbeam.c, read_data_mfb(), %s: failed to read a beam parameter.", BT[t].file);
   for (int i = 0; i < nn; i++) {
-    double v[12];
-    for (int k = 0; k < 12; k++)
-      if (fscanf(fp, "%lf", &v[k]) != 1) die("ipw.txt: failed to read a beam parameter.");
-    Ipw *w = &b->bd.ipw[i];
-    w->lambda0 = v[0]; w->ni = v[1]; w->power = v[2];
-    w->e0x = v[3] + I * v[4]; w->e0y = v[5] + I * v[6];
-    w->fx = v[7]; w->fy = v[8]; w->fz = v[9]; w->theta = v[10]; w->phi = v[11];
+    double *v = (double *)bt_rec(b, t, i);
+    for (int k = 0; k < BT[t].nd; k++)
+      if (fscanf(fp, "%lf", &v[k]) != 1) die(msg);
+    for (int k = 0; k < BT[t].nint; k++)
+      if (fscanf(fp, "%d", (int *)(bt_rec(b, t, i) + BT[t].oint[k])) != 1) die(msg);
   }
   fclose(fp);
+  return nn;
 }
 
-/* derived plane-wave data (ipw.c:65-89) and the common medium / wavelength
- * check (multi_fbeam.c:878-962) */
+/* read_data_mfb (multi_fbeam.c:20-36): every beam file present */
+static void beams_read(Bobj *b)
+{
+  int wn = 0;
+  for (int t = 0; t < BT_N; t++) wn += beams_read_file(b, t);
+  if (wn == 0) {
+    printf("read_data_mfb() No beam defined, check beam data file : %s, %s, %s, %s, %s, %s.\n", BT[0].file, BT[1].file,
+           BT[2].file, BT[3].file, BT[4].file, BT[5].file);
+    exit(1);
+  }
+}
+
+/* the members common to every record */
+typedef struct {
+  double lambda0, ni, na, power;
+  double complex e0x, e0y;
+  double fx, fy, fz, theta, phi;
+  int lg_m, nn;
+} bcommon_t;
+static bcommon_t bt_common(Bobj *b, int t, int i)
+{
+  bcommon_t c;
+  memset(&c, 0, sizeof c);
+  const double *v = (const double *)bt_rec(b, t, i);
+  const int k = t == BT_IPW ? 0 : 1;   /* the plane wave has no third double */
+  c.lambda0 = v[0]; c.ni = v[1]; c.na = k ? v[2] : 0.0; c.power = v[2 + k];
+  c.e0x = v[3 + k] + I * v[4 + k]; c.e0y = v[5 + k] + I * v[6 + k];
+  c.fx = v[7 + k]; c.fy = v[8 + k]; c.fz = v[9 + k]; c.theta = v[10 + k]; c.phi = v[11 + k];
+  if (t == BT_LGB || t == BT_BLG) c.lg_m = *(const int *)(bt_rec(b, t, i) + BT[t].oint[0]);
+  if (t == BT_FPW || t == BT_RAB) c.nn = *(const int *)(bt_rec(b, t, i) + BT[t].oint[0]);
+  if (t == BT_LGB) c.nn = *(const int *)(bt_rec(b, t, i) + BT[t].oint[1]);
+  return c;
+}
+
+/* check_data_mfb (multi_fbeam.c:878-962), then the derived data of the
+ * set-up functions that live in the records (ipw.c:65-89, fpw.c:115-133,
+ * lgb.c:116-134, bsb.c:112-127, bslgb.c:115-130, rab.c:111-130; the
+ * quadrature tables are the device's) */
 static void beams_setup(Bobj *b)
 {
-  if (b->n_ipw <= 0) { printf("no beams defined. check datafile name. Exit...\n"); exit(0); }
+  static const char *tag[BT_N] = {"ipw", "fpw", "lgb", "bsb", "blg", "rab"};
+  int st = 0, tot = 0;
+  b->n_0 = 0.0;
+  b->lambda_0 = 0.0;
+  for (int t = 0; t < BT_N; t++)
+    for (int i = 0; i < *bt_count(b, t); i++, tot++) {
+      bcommon_t c = bt_common(b, t, i);
+      if (st == 0) { b->n_0 = c.ni; b->lambda_0 = c.lambda0; st = 1; }
+      else {
+        if (b->n_0 != c.ni) { printf("data error %s ni, exit\n", tag[t]); exit(1); }
+        if (b->lambda_0 != c.lambda0) { printf("data error %s lambda0, exit\n", tag[t]); exit(1); }
+      }
+    }
+  if (tot == 0) { printf("no beams defined. check datafile name. Exit...\n"); exit(0); }
   for (int i = 0; i < b->n_ipw; i++) {
     Ipw *w = &b->bd.ipw[i];
-    if (i == 0) { b->n_0 = w->ni; b->lambda_0 = w->lambda0; }
-    else if (w->ni != b->n_0 || w->lambda0 != b->lambda_0) die("data error ipw ni or lambda0,");
     double cs = 1.0 / sqrt(pow(cabs(w->e0x), 2) + pow(cabs(w->e0y), 2));
     double complex ex = w->e0x * cs, ey = w->e0y * cs, hx = -ey * w->ni, hy = ex * w->ni;
     w->data.E0 = sqrt(w->power * 2.0 / w->ni);
     w->data.ki = 2.0 * M_PI * w->ni / w->lambda0;
-    double st = sin(w->theta), ct = cos(w->theta), sp = sin(w->phi), cp = cos(w->phi);
-    w->data.sin_t = st; w->data.cos_t = ct; w->data.sin_p = sp; w->data.cos_p = cp;
+    double sn = sin(w->theta), ct = cos(w->theta), sp = sin(w->phi), cp = cos(w->phi);
+    w->data.sin_t = sn; w->data.cos_t = ct; w->data.sin_p = sp; w->data.cos_p = cp;
     double a11 = sp * sp + cp * cp * ct, a12 = sp * cp * ct - sp * cp, a22 = sp * sp * ct + cp * cp;
-    w->data.ex = ex * a11 + ey * a12; w->data.ey = ex * a12 + ey * a22; w->data.ez = -(ex * cp + ey * sp) * st;
-    w->data.hx = hx * a11 + hy * a12; w->data.hy = hx * a12 + hy * a22; w->data.hz = -(hx * cp + hy * sp) * st;
+    w->data.ex = ex * a11 + ey * a12; w->data.ey = ex * a12 + ey * a22; w->data.ez = -(ex * cp + ey * sp) * sn;
+    w->data.hx = hx * a11 + hy * a12; w->data.hy = hx * a12 + hy * a22; w->data.hz = -(hx * cp + hy * sp) * sn;
   }
+#define FOCUSED_SETUP(p)                                                                                   \
+  do {                                                                                                   \
+    double kr_ = (p)->NA / (p)->ni, ce_ = sqrt(1.0 / M_PI) / (kr_ * sqrt(pow(cabs((p)->e0x), 2) + pow(cabs((p)->e0y), 2))); \
+    (p)->data.ki = 2.0 * M_PI * (p)->ni / (p)->lambda0;                                                  \
+    (p)->data.ex = (p)->e0x * ce_; (p)->data.ey = (p)->e0y * ce_;                                        \
+    (p)->data.hx = -(p)->data.ey * (p)->ni; (p)->data.hy = (p)->data.ex * (p)->ni;                       \
+    (p)->data.E0 = sqrt(2.0 * (p)->power * (p)->ni) / (p)->lambda0;                                      \
+    (p)->data.cos_t = cos((p)->theta); (p)->data.sin_t = sin((p)->theta);                                \
+    (p)->data.cos_p = cos((p)->phi); (p)->data.sin_p = sin((p)->phi);                                    \
+    (p)->data.nt = (p)->nn;                                                                              \
+  } while (0)
+#define BESSEL_SETUP(p)                                                                                    \
+  do {                                                                                                   \
+    double ce_ = 1.0 / sqrt(pow(cabs((p)->e0x), 2) + pow(cabs((p)->e0y), 2));                             \
+    (p)->data.st = sin((p)->d_angle); (p)->data.ct = cos((p)->d_angle);                                  \
+    (p)->data.ki = 2.0 * M_PI * (p)->ni / (p)->lambda0;                                                  \
+    (p)->data.ex = (p)->e0x * ce_; (p)->data.ey = (p)->e0y * ce_;                                        \
+    (p)->data.hx = -(p)->data.ey * (p)->ni; (p)->data.hy = (p)->data.ex * (p)->ni;                       \
+    (p)->data.E0 = sqrt(2.0 * (p)->power / (p)->ni);                                                     \
+    (p)->data.cos_t = cos((p)->theta); (p)->data.sin_t = sin((p)->theta);                                \
+    (p)->data.cos_p = cos((p)->phi); (p)->data.sin_p = sin((p)->phi);                                    \
+  } while (0)
+  for (int i = 0; i < b->n_fpw; i++) FOCUSED_SETUP(&b->bd.fpw[i]);
+  for (int i = 0; i < b->n_lgb; i++) FOCUSED_SETUP(&b->bd.lgb[i]);
+  for (int i = 0; i < b->n_bsb; i++) BESSEL_SETUP(&b->bd.bsb[i]);
+  for (int i = 0; i < b->n_blg; i++) BESSEL_SETUP(&b->bd.blg[i]);
+  for (int i = 0; i < b->n_rab; i++) {
+    RAb *r = &b->bd.rab[i];
+    double kr = r->NA / r->ni, ce = sqrt(1.0 / M_PI) / (kr * sqrt(pow(cabs(r->e0r), 2) + pow(cabs(r->e0a), 2)));
+    r->data.ki = 2.0 * M_PI * r->ni / r->lambda0;
+    r->data.er = r->e0r * ce; r->data.ea = r->e0a * ce;
+    r->data.hr = -r->data.ea * r->ni; r->data.ha = r->data.er * r->ni;
+    r->data.E0 = sqrt(2.0 * r->power * r->ni) / r->lambda0;
+    r->data.cos_t = cos(r->theta); r->data.sin_t = sin(r->theta);
+    r->data.cos_p = cos(r->phi); r->data.sin_p = sin(r->phi);
+    r->data.nt = r->nn;
+  }
+#undef FOCUSED_SETUP
+#undef BESSEL_SETUP
+  /* pointers of the derived data are not used on the host (and stale in a file) */
+  for (int i = 0; i < b->n_fpw; i++) { FpwD *d = &b->bd.fpw[i].data; d->wt = d->ct = d->st = d->cp = d->sp = NULL; }
+  for (int i = 0; i < b->n_lgb; i++) { LGbD *d = &b->bd.lgb[i].data; d->wt = d->ct = d->st = d->cp = d->sp = NULL; d->ph_p = NULL; }
+  for (int i = 0; i < b->n_bsb; i++) { BsbD *d = &b->bd.bsb[i].data; d->cp = d->sp = NULL; }
+  for (int i = 0; i < b->n_blg; i++) { BsLGbD *d = &b->bd.blg[i].data; d->cp = d->sp = NULL; d->ph_p = NULL; }
+  for (int i = 0; i < b->n_rab; i++) { RAbD *d = &b->bd.rab[i].data; d->wt = d->ct = d->st = d->cp = d->sp = NULL; }
   b->omega = 2.0 * M_PI / b->lambda_0;
 }
 
-static void beam_params(const Bobj *b, double *par)
+static int beams_total(Bobj *b)
 {
-  for (int i = 0; i < b->n_ipw; i++) {
-    const Ipw *w = &b->bd.ipw[i];
-    double v[12] = {w->lambda0, w->ni, w->power, creal(w->e0x), cimag(w->e0x), creal(w->e0y), cimag(w->e0y),
-                    w->fx, w->fy, w->fz, w->theta, w->phi};
-    memcpy(par + 12 * i, v, sizeof v);
-  }
+  int n = 0;
+  for (int t = 0; t < BT_N; t++) n += *bt_count(b, t);
+  return n;
 }
 
-static void print_beams(const Bobj *b, int mksa)
+/* the device's beam records (16 doubles each, bh_object_set_mfb), in the
+ * type order in which calc_mfb_EH sums them */
+static void beam_params(Bobj *b, double *par)
 {
-  for (int i = 0; i < b->n_ipw; i++) {
-    const Ipw *w = &b->bd.ipw[i];
-    printf(" \"%s\" No.%02d -- plane wave%s --\n", b->fname_ipw, i, mksa ? ", MKSA system" : "");
-    printf("wave length of incident beam in vacuum      : %15.14g\n", w->lambda0);
-    printf("refractive index of surrounding             : %15.14g\n", w->ni);
-    printf("incident beam power per unit area           : %15.14g\n", w->power);
-    printf("x-component of polarization coefficient     :%7.6g+%7.6gi\n", creal(w->e0x), cimag(w->e0x));
-    printf("y-component of polarization coefficient     :%7.6g+%7.6gi\n", creal(w->e0y), cimag(w->e0y));
-    printf("x-component of translation vector           : %15.14g\n", w->fx);
-    printf("y-component of translation vector           : %15.14g\n", w->fy);
-    printf("z-component of translation vector           : %15.14g\n", w->fz);
-    printf("rotation parameter theta               [rad]: %15.14g\n", w->theta);
-    printf("rotation parameter phi                 [rad]: %15.14g\n", w->phi);
-  }
+  int k = 0;
+  for (int t = 0; t < BT_N; t++)
+    for (int i = 0; i < *bt_count(b, t); i++, k++) {
+      bcommon_t c = bt_common(b, t, i);
+      double v[16] = {(double)t, c.lambda0, c.ni, c.na, c.power, creal(c.e0x), cimag(c.e0x), creal(c.e0y), cimag(c.e0y),
+                      c.fx, c.fy, c.fz, c.theta, c.phi, (double)c.lg_m, (double)c.nn};
+      memcpy(par + 16 * k, v, sizeof v);
+    }
+}
+
+/* print_data_mfb (multi_fbeam.c:38-67) with each type's print function */
+static void print_beams(Bobj *b, int mksa)
+{
+  for (int t = 0; t < BT_N; t++)
+    for (int i = 0; i < *bt_count(b, t); i++) {
+      bcommon_t c = bt_common(b, t, i);
+      printf(" \"%s\" No.%02d -- %s%s --\n", bt_fname(b, t), i, BT[t].title, mksa ? ", MKSA system" : "");
+      printf("wave length of incident beam in vacuum      : %15.14g\n", c.lambda0);
+      printf("refractive index of surrounding             : %15.14g\n", c.ni);
+      if (t == BT_FPW || t == BT_LGB || t == BT_RAB) printf("numerical aperture                          : %15.14g\n", c.na);
+      if (t == BT_BSB || t == BT_BLG) printf("deflection angle                       [rad]: %15.14g\n", c.na);
+      if (t == BT_IPW || t == BT_BSB || t == BT_BLG) printf("incident beam power per unit area           : %15.14g\n", c.power);
+      else printf("incident beam power                         : %15.14g\n", c.power);
+      if (t == BT_RAB) {
+        printf("radial-component of polarization coefficient:%7.6g+%7.6gi\n", creal(c.e0x), cimag(c.e0x));
+        printf("azimuthal-component of polarization coef    :%7.6g+%7.6gi\n", creal(c.e0y), cimag(c.e0y));
+      } else {
+        printf("x-component of polarization coefficient     :%7.6g+%7.6gi\n", creal(c.e0x), cimag(c.e0x));
+        printf("y-component of polarization coefficient     :%7.6g+%7.6gi\n", creal(c.e0y), cimag(c.e0y));
+      }
+      printf("x-component of translation vector           : %15.14g\n", c.fx);
+      printf("y-component of translation vector           : %15.14g\n", c.fy);
+      printf("z-component of translation vector           : %15.14g\n", c.fz);
+      printf("rotation parameter theta               [rad]: %15.14g\n", c.theta);
+      printf("rotation parameter phi                 [rad]: %15.14g\n", c.phi);
+      if (t == BT_LGB || t == BT_BLG) printf("mode number for spiral phase modulation     : %15d\n", c.lg_m);
+      if (t == BT_FPW || t == BT_LGB || t == BT_RAB) printf("gauss-legendre integration sampling number  : %15d\n", c.nn);
+    }
   printf("\n");
+}
+
+/* the beam records of a .obj / .dat file after the raw Bobj
+ * (solve_bieq2.c:459-465, mo_setup.c:170-176) */
+static void beams_write(FILE *fp, Bobj *b)
+{
+  fwrite(b, sizeof(Bobj), 1, fp);
+  for (int t = 0; t < BT_N; t++) fwrite(*bt_data(b, t), BT[t].size, *bt_count(b, t), fp);
 }
 
 /* ------------------------------------------------------------ tables */
@@ -373,9 +532,10 @@ static bh_object *dev(DOMD *md)
   bh_object *o = bh_object_create(bd->Nn, rn, bd->Ne, ed, bd->md, bd->sd, md->MN, nm, md->mw.lambda_0);
   free(rn); free(ed); free(nm);
   if (!o) { printf("device object: %s. Exit...\n", bh_last_error()); exit(1); }
-  double *par = xalloc((size_t)md->mw.n_ipw * 12, sizeof(double), "beams");
+  const int nb = beams_total(&md->mw);
+  double *par = xalloc((size_t)nb * 16, sizeof(double), "beams");
   beam_params(&md->mw, par);
-  bh_or_die(bh_object_set_beams(o, md->mw.n_ipw, par), "bh_object_set_beams");
+  bh_or_die(bh_object_set_mfb(o, nb, par), "bh_object_set_mfb");
   free(par);
   md->dev = o;
   return o;
@@ -533,7 +693,7 @@ void finalize_domd2(DOMD *md)
   for (int i = 0; i <= md->bd.Nn; i++) free(md->bd.rn[i]);
   free(md->bd.rn);
   free(md->n); free(md->kn);
-  free(md->mw.bd.ipw);
+  beams_free(&md->mw);
   free_names(&md->cm);
 }
 
@@ -608,8 +768,7 @@ static void write_obj(const char *fname, DOMD *md)
   fwrite(md->rv, sizeof(double), 3, fp); fwrite(&md->th, sizeof(double), 1, fp); fwrite(md->tv, sizeof(double), 3, fp);
   fwrite(&md->MN, sizeof(int), 1, fp);
   fwrite(md->n, sizeof(double complex), md->MN + 1, fp); fwrite(md->kn, sizeof(double complex), md->MN + 1, fp);
-  fwrite(&md->mw, sizeof(Bobj), 1, fp);
-  fwrite(md->mw.bd.ipw, sizeof(Ipw), md->mw.n_ipw, fp);
+  beams_write(fp, &md->mw);
   fwrite(&bd->Nn, sizeof(int), 1, fp); fwrite(&bd->Ne, sizeof(int), 1, fp);
   for (int i = 0; i <= bd->Nn; i++) fwrite(bd->rn[i], sizeof(double), 3, fp);
   for (int i = 0; i <= bd->Ne; i++) fwrite(bd->ed[i], sizeof(int), 4, fp);
@@ -688,10 +847,10 @@ static void read_geometry(FILE *fp, DOMD *md)
   md->n = xalloc(md->MN + 1, sizeof(double complex), "n"); md->kn = xalloc(md->MN + 1, sizeof(double complex), "kn");
   rd(md->n, sizeof(double complex), md->MN + 1, fp, "n"); rd(md->kn, sizeof(double complex), md->MN + 1, fp, "kn");
   rd(&md->mw, sizeof(Bobj), 1, fp, "Bobj");
-  if (md->mw.n_fpw || md->mw.n_lgb || md->mw.n_bsb || md->mw.n_blg || md->mw.n_rab)
-    die("stored beams other than plane waves are not supported by this build.");
-  md->mw.bd.ipw = xalloc(md->mw.n_ipw, sizeof(Ipw), "ipw");
-  rd(md->mw.bd.ipw, sizeof(Ipw), md->mw.n_ipw, fp, "ipw");
+  for (int t = 0; t < BT_N; t++) {   /* the stored pointers are stale */
+    *bt_data(&md->mw, t) = xalloc(*bt_count(&md->mw, t), BT[t].size, "beams");
+    rd(*bt_data(&md->mw, t), BT[t].size, *bt_count(&md->mw, t), fp, BT[t].file);
+  }
   beams_setup(&md->mw);
   BOUD *bd = &md->bd;
   rd(&bd->Nn, sizeof(int), 1, fp, "Nn"); rd(&bd->Ne, sizeof(int), 1, fp, "Ne");
@@ -751,9 +910,9 @@ void mo_initialize(int argc, char **argv, MOBJ *mo)
       die("mo_initialize(): failed to read the mod_fn,rs.");
     DOMD *md = &mo->md[i];
     read_obj(mo->mod_fn[i], md);
-    if (flg == 1) {  /* renew beams from the working directory's ipw.txt */
+    if (flg == 1) {  /* renew beams from the working directory's beam files */
       double lam = md->mw.lambda_0, n0 = md->mw.n_0;
-      free(md->mw.bd.ipw);
+      beams_free(&md->mw);
       beams_init(&md->mw);
       beams_read(&md->mw);
       beams_setup(&md->mw);
@@ -873,8 +1032,7 @@ void mo_dat_write(char *fname, MOBJ *mo)
     fwrite(md->rv, sizeof(double), 3, fp); fwrite(&md->th, sizeof(double), 1, fp); fwrite(md->tv, sizeof(double), 3, fp);
     fwrite(&md->MN, sizeof(int), 1, fp);
     fwrite(md->n, sizeof(double complex), md->MN + 1, fp); fwrite(md->kn, sizeof(double complex), md->MN + 1, fp);
-    fwrite(&md->mw, sizeof(Bobj), 1, fp);
-    fwrite(md->mw.bd.ipw, sizeof(Ipw), md->mw.n_ipw, fp);
+    beams_write(fp, &md->mw);
     fwrite(&bd->Nn, sizeof(int), 1, fp); fwrite(&bd->Ne, sizeof(int), 1, fp);
     for (int i = 0; i <= bd->Nn; i++) fwrite(bd->rn[i], sizeof(double), 3, fp);
     for (int i = 0; i <= bd->Ne; i++) fwrite(bd->ed[i], sizeof(int), 4, fp);

@@ -78,6 +78,19 @@ int bh_object_matrix(const bh_object *o, int did, int which, double *out);
  * boundary data of the open-domain node slots (init_boundary_data_u2,
  * mo_setup.c:704-757).                                                    */
 int bh_object_set_beams(bh_object *o, int nb, const double *par);
+/* A beam set of any multi_fbeam types, 16 doubles per beam: type (0 plane
+ * wave, 1 focused plane wave, 2 focused with spiral phase modulation, 3
+ * Bessel, 4 Bessel with spiral phase, 5 focused radially / azimuthally
+ * polarised), lambda0, ni, NA (1, 2, 5) or deflection angle (3, 4), power,
+ * re/im e0x (e0r for 5), re/im e0y (e0a for 5), fx, fy, fz, theta, phi,
+ * spiral mode (2, 4), Gauss-Legendre points (1, 2, 5).  Replaces
+ * read_data_mfb / setup_mfb (multi_fbeam.c:20-36, 100-120) and the beam
+ * set-up functions of fpw.c:115, lgb.c:116, bsb.c:112, bslgb.c:115,
+ * rab.c:111 for the incident fields of calc_mfb_EH / calc_mfb_EH_dv
+ * (multi_fbeam.c:764-876); computes the incident boundary data like
+ * bh_object_set_beams.  BH_ECONV when a phi integral does not converge
+ * ("phi integration limit over" in the reference). */
+int bh_object_set_mfb(bh_object *o, int nb, const double *par);
 /* Factorise the boundary-integral system (create_Amatrix_csr + lu_dec_A,
  * solve_bieq2.c:186-408).  mode 1: the reference's dense 8NN matrix;
  * mode 2: exact block elimination for one interior medium (LU of G0 and
