@@ -11,7 +11,11 @@ radially / azimuthally polarised; fpw.c, lgb.c, bsb.c, bslgb.c, rab.c):
 * the same case run from its input files (hemisphere_m1.msh,
   medium_data1.txt n = 1.515, fpw.txt, object_settings0.txt) through
   d3b2_create_matrix and d3b2_bv_solver: the solved state against ex30.dat
-  (1e-10) and the relinked example1 against example1_ex30.log.
+  (1e-10) and the relinked example1 against example1_ex30.log;
+* the two-object case of the same sample (the hemisphere and a metal cone
+  turned by pi about x, coupled) from its input files: the convergence
+  trace of d3b2_bv_solver.log and the relinked example1 against
+  example1_ex3.log.
 
 The absorbed powers of example1_ex30.log are round-off of a lossless object
 (the reference prints its "loss of significance" warning with them), so
@@ -164,3 +168,64 @@ def test_ex30_from_input_files_example1(ex30_run):
             assert np.abs(a - b).max() < 1e-9 * np.abs(want["force"][typ][0]["F"]).max(), (typ, q)
         # round-off of a lossless object against an incident power of 1
         assert abs(got["absorb"][typ][0]) < 1e-3
+
+
+# analysis_sample3's two-object case: the hemisphere and a metal cone
+# (cone_m2.msh, n = 0.06+4i) turned by pi about x at create_matrix time and
+# shifted by 0.13 in z, under the same focused plane wave
+MEDIUM2 = "## medium data file for bem3_emf_b2\n# number of medium\n1\n# real(refractive_index) imag(refractive_index)\n" \
+          "0.06 4.0\n"
+SETTINGS = "# number_of_objects reset_flag_of_incident_field(0:no reset, 1:reset)\n2 0\n" \
+           "# object_data_file_name shitf_x shift_y shit_z\nhemisphere_m1_md1.obj +0.0 +0.0 -0.0\n" \
+           "cone_m2_md2_xpi.obj +0.0 +0.0 +0.130\n"
+
+
+@pytest.fixture(scope="module")
+def ex3_run(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("ex3"))
+    for m in ("hemisphere_m1.msh", "cone_m2.msh"):
+        shutil.copy(os.path.join(GOLDEN, m), d)
+    for name, text in (("fpw.txt", FPW), ("medium_data1.txt", MEDIUM1), ("medium_data2.txt", MEDIUM2),
+                       ("object_settings.txt", SETTINGS)):
+        with open(os.path.join(d, name), "w") as fp:
+            fp.write(text)
+    _run([os.path.join(BIN, "d3b2_create_matrix"), "medium_data1.txt", "hemisphere_m1.msh", "hemisphere_m1_md1.obj"], d)
+    _run([os.path.join(BIN, "d3b2_create_matrix"), "medium_data2.txt", "cone_m2.msh", "cone_m2_md2_xpi.obj",
+          "1", "0", "0", "3.14159265", "0", "0", "0"], d)
+    log = _run([os.path.join(BIN, "d3b2_bv_solver"), "object_settings.txt", "ex3.dat"], d)
+    return d, log
+
+
+def test_ex3_convergence_trace(ex3_run):
+    import re
+    _, log = ex3_run
+    tr = lambda t: {int(c): float(v) for c, v in re.findall(r"count =\s*(\d+), cc = ([0-9.eE+-]+)", t)}
+    with open(os.path.join(GOLDEN, "ex3_bv_solver.log")) as fp:
+        want = tr(fp.read())
+    got = tr(log)
+    assert sorted(got) == sorted(want), (sorted(got), sorted(want))
+    for c, w in want.items():
+        print("count %2d: %.6g vs %.6g" % (c, got[c], w), flush=True)
+        assert abs(got[c] - w) <= 2e-5 * w + 1e-10, (c, got[c], w)
+
+
+@pytest.mark.skipif(not os.path.exists(EX1), reason="reference example1 not compiled against the drop-in library")
+def test_ex3_example1(ex3_run):
+    d, _ = ex3_run
+    out = _run([EX1, "ex3.dat"], d)
+    with open(os.path.join(d, "ex1.log"), "w") as fp:
+        fp.write(out)
+    got = parse_example1_log(os.path.join(d, "ex1.log"))
+    want = parse_example1_log(os.path.join(GOLDEN, "ex3_example1.log"))
+    keys = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+    for typ in (0, 1):
+        w = np.array([want["field"][typ][k] for k in keys])
+        g = np.array([got["field"][typ][k] for k in keys])
+        print("field type %d: %.2e" % (typ, np.abs(g - w).max() / np.abs(w).max()), flush=True)
+        assert np.abs(g - w).max() < 1e-8 * np.abs(w).max(), typ
+        for m in (0, 1):
+            fs = np.abs(want["force"][typ][m]["F"]).max()
+            for q in ("F", "N"):
+                a, b = got["force"][typ][m][q], want["force"][typ][m][q]
+                print("type %d object %d %s: %.2e" % (typ, m, q, np.abs(a - b).max() / fs), flush=True)
+                assert np.abs(a - b).max() < 1e-8 * fs, (typ, m, q)
