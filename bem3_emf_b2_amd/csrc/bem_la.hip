@@ -804,14 +804,18 @@ __global__ void __launch_bounds__(256) k_trsm_upper(const typename Cx<R>::Z *__r
 // waves fold in the finished blocks j (acc_r -= A_rj x_j) as their flags
 // appear, wave w taking every fourth j; the block that finishes last (the
 // neighbour, on the critical path) and the diagonal block are staged in LDS
-// before any wait, and wave 0 folds in the neighbour and solves the diagonal
-// block in 64 readlane-broadcast steps, then publishes x and raises the
-// block's flag.  x and the flags live in uncached memory and are read with
-// relaxed agent-scope atomic loads (no wait between them; a volatile load
-// would wait for each one); flag = launch epoch, so flags never need
-// clearing; a block waits for its own stores to complete before it raises
-// its flag.  Every wait is bounded: on timeout the error flag is raised and
-// the launch runs to its end.
+// and the block's right-hand-side rows loaded before any wait, and wave 0
+// folds in the neighbour and solves the diagonal block in 64
+// readlane-broadcast steps, then publishes x, raises the block's flag and
+// only then writes its rows of B.  x and the flags live in uncached memory
+// and are read with relaxed agent-scope atomic loads (no wait between them;
+// a volatile load would wait for each one); flag = launch epoch, so flags
+// never need clearing; a block waits for its own x stores to complete
+// before it raises its flag.  (Polling x itself against a sentinel would
+// save the flag's round trips, but a complex FP64 value then needs two
+// single-copy-atomic 8-byte stores, and the doubled store transactions cost
+// more than the flag.)  Every wait is bounded: on timeout the error flag is
+// raised and the launch runs to its end.
 #define TRSV_NBMAX 4096    // 64-row blocks (n <= 262144)
 #define TRSV_XMAX 65536    // rows of the exchange copy of x
 #define TRSV_MAXR 4        // right-hand sides of the one-launch path
@@ -919,7 +923,11 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
   Z *xs = (Z *)w.x;
   const int ndep = UPPER ? nb - 1 - i : i;
   const int jlast = UPPER ? i + 1 : i - 1;
-  // staging (coalesced rows), before any wait
+  // before any wait: this block's right-hand-side rows (wave 0), and the
+  // neighbour and diagonal blocks staged in LDS (coalesced rows)
+  Z bpre[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++) bpre[c] = (wv == 0 && live) ? B[(size_t)(r0 + lane) * ldb + c] : zmk<Z>(0, 0);
   {
     const int c0 = 64 * jlast, cols = ndep > 0 ? min(64, n - c0) : 0;
     for (int idx = tid; idx < 64 * 64; idx += 256) {
@@ -966,7 +974,7 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
     }
   }
 #pragma unroll
-  for (int c = 0; c < NR; c++) v[c] = live ? B[(size_t)(r0 + lane) * ldb + c] - v[c] : zmk<Z>(0, 0);
+  for (int c = 0; c < NR; c++) v[c] = live ? bpre[c] - v[c] : zmk<Z>(0, 0);
   // the diagonal block: x_k broadcast from lane k
   for (int t = 0; t < rows; t++) {
     const int k = UPPER ? rows - 1 - t : t;
@@ -982,14 +990,16 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
   }
   if (live) {
 #pragma unroll
-    for (int c = 0; c < NR; c++) {
-      B[(size_t)(r0 + lane) * ldb + c] = v[c];
-      xs[(size_t)(r0 + lane) * TRSV_MAXR + c] = v[c];
-    }
+    for (int c = 0; c < NR; c++) xs[(size_t)(r0 + lane) * TRSV_MAXR + c] = v[c];
   }
-  __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have completed
+  __builtin_amdgcn_s_waitcnt(0);   // this wave's x stores have completed
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   if (lane == 0) __hip_atomic_store(w.flag + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // this block's rows of B after the flag: no other block reads them
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < NR; c++) B[(size_t)(r0 + lane) * ldb + c] = v[c];
+  }
 }
 
 // ---- host side ----
