@@ -800,22 +800,25 @@ __global__ void __launch_bounds__(256) k_trsm_upper(const typename Cx<R>::Z *__r
 // ---- triangular solves with a few right-hand sides, one launch ----
 // X := L^{-1} X (unit lower) or U^{-1} X (upper) for nrhs <= TRSV_MAXR
 // columns.  Workgroup b owns one 64-row block (block i = b for L, nb-1-b for
-// U, so a block depends only on workgroups launched before it).  Its four
-// waves fold in the finished blocks j (acc_r -= A_rj x_j) as their flags
-// appear, wave w taking every fourth j; the block that finishes last (the
-// neighbour, on the critical path) and the diagonal block are staged in LDS
-// and the block's right-hand-side rows loaded before any wait, and wave 0
-// folds in the neighbour and solves the diagonal block in 64
-// readlane-broadcast steps, then publishes x, raises the block's flag and
-// only then writes its rows of B.  x and the flags live in uncached memory
-// and are read with relaxed agent-scope atomic loads (no wait between them;
-// a volatile load would wait for each one); flag = launch epoch, so flags
-// never need clearing; a block waits for its own x stores to complete
-// before it raises its flag.  (Polling x itself against a sentinel would
-// save the flag's round trips, but a complex FP64 value then needs two
-// single-copy-atomic 8-byte stores, and the doubled store transactions cost
-// more than the flag.)  Every wait is bounded: on timeout the error flag is
-// raised and the launch runs to its end.
+// U, so a block depends only on workgroups launched before it).  With D the
+// block's diagonal block, N the block that finishes last (the neighbour) and
+// r the right-hand side minus the other finished blocks' terms,
+//   x_i = D^{-1} r - W x_nbr,   W = D^{-1} N.
+// Before any wait the block inverts D in LDS (wave 0, one lane per column)
+// and forms W (four waves, one lane per row).  Its four waves then fold in
+// the other finished blocks j (acc -= A_ij x_j) as their flags appear, wave
+// w taking every fourth j; wave 0 forms y = D^{-1} r while the neighbour is
+// still running, so after the neighbour's flag only y - W x_nbr, one 64-term
+// product, lies on the chain from block to block.  The block then publishes
+// x, raises its flag and only then writes its rows of B.  x and the flags
+// live in uncached memory and are read with relaxed agent-scope atomic loads
+// (no wait between them; a volatile load would wait for each one); flag =
+// launch epoch, so flags never need clearing; a block waits for its own x
+// stores to complete before it raises its flag.  (Polling x itself against
+// a sentinel would save the flag's round trips, but a complex FP64 value
+// then needs two single-copy-atomic 8-byte stores, and the doubled store
+// transactions cost more than the flag.)  Every wait is bounded: on timeout
+// the error flag is raised and the launch runs to its end.
 #define TRSV_NBMAX 4096    // 64-row blocks (n <= 262144)
 #define TRSV_XMAX 65536    // rows of the exchange copy of x
 #define TRSV_MAXR 4        // right-hand sides of the one-launch path
@@ -831,23 +834,10 @@ static void *g_trsv_buf = nullptr;
 static bool g_trsv_ok = true;
 static int g_trsv_epoch = 0;
 
-// LDS of one block: the neighbour block, the diagonal block (rows padded
-// to 65), an x block per wave and the inverted diagonal
+// LDS of one block: D^{-1} and W (rows padded to 65), an x block per wave
 template <class R> constexpr int trsv_lds_bytes()
 {
-  return (2 * 64 * 65 + 4 * 64 * TRSV_MAXR + 64) * (int)sizeof(typename Cx<R>::Z);
-}
-
-// value of lane k (uniform k)
-__device__ inline double lane_bcast(double v, int k)
-{
-  const long long u = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)u, k), hi = __builtin_amdgcn_readlane((int)(u >> 32), k);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ inline float lane_bcast(float v, int k)
-{
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+  return (2 * 64 * 65 + 4 * 64 * TRSV_MAXR) * (int)sizeof(typename Cx<R>::Z);
 }
 
 // lane 0 waits for flag >= epoch (bounded)
@@ -905,16 +895,40 @@ __device__ inline void trsv_rowdot(Z *acc, const Z *arow, const Z *xw, int cols)
   }
 }
 
+// D (64x64, row stride 65, in LDS) := D^{-1}, one wave, lane = column c:
+// row k of X = D^{-1} solves sum_m D_km X_mc = delta_kc from the rows of X
+// already formed (above k for L, unit diagonal; below k for U), then
+// overwrites row k of D, whose entries that row's sum has consumed.  The
+// wave's LDS operations run in program order, so every lane's reads of
+// row k precede the writes.  The sum runs in two halves to shorten the
+// dependent chain of multiply-adds.
+template <class Z, bool UPPER>
+__device__ inline void trsv_invert(Z *D, int lane)
+{
+  for (int t = 0; t < 64; t++) {
+    const int k = UPPER ? 63 - t : t;
+    const int m0 = UPPER ? k + 1 : 0, m1 = UPPER ? 64 : k;
+    Z s0 = zmk<Z>(lane == k ? 1 : 0, 0), s1 = zmk<Z>(0, 0);
+    const Z dk = UPPER ? cdiv(zmk<Z>(1, 0), D[k * 65 + k]) : zmk<Z>(1, 0);
+    int m = m0;
+    for (; m + 1 < m1; m += 2) {
+      s0 = s0 - D[k * 65 + m] * D[m * 65 + lane];
+      s1 = s1 - D[k * 65 + m + 1] * D[(m + 1) * 65 + lane];
+    }
+    if (m < m1) s0 = s0 - D[k * 65 + m] * D[m * 65 + lane];
+    D[k * 65 + lane] = UPPER ? (s0 + s1) * dk : s0 + s1;
+  }
+}
+
 template <class R, bool UPPER, int NR>
 __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__restrict__ A, size_t lda, int n,
                                                  typename Cx<R>::Z *B, size_t ldb, TrsvWs w, int epoch)
 {
   typedef typename Cx<R>::Z Z;
   extern __shared__ double dyn_lds[];
-  Z *Ap = (Z *)dyn_lds;                 // [64][65] the neighbour block
-  Z *Ad = Ap + 64 * 65;                 // [64][65] the diagonal block
-  Z *Xw = Ad + 64 * 65;                 // [4][64][TRSV_MAXR] x block, then partial sums, per wave
-  Z *Di = Xw + 4 * 64 * TRSV_MAXR;      // [64] inverted diagonal (U)
+  Z *Dv = (Z *)dyn_lds;                 // [64][65] the diagonal block, then its inverse
+  Z *Wn = Dv + 64 * 65;                 // [64][65] the neighbour block, then W = D^{-1} N
+  Z *Xw = Wn + 64 * 65;                 // [4][64][TRSV_MAXR] x block, then partial sums, per wave
   const int nb = (n + 63) / 64, b = blockIdx.x;
   const int i = UPPER ? nb - 1 - b : b;
   const int r0 = 64 * i, rows = min(64, n - r0);
@@ -924,7 +938,8 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
   const int ndep = UPPER ? nb - 1 - i : i;
   const int jlast = UPPER ? i + 1 : i - 1;
   // before any wait: this block's right-hand-side rows (wave 0), and the
-  // neighbour and diagonal blocks staged in LDS (coalesced rows)
+  // diagonal and neighbour blocks staged in LDS (coalesced rows; the rows
+  // of a partial block padded with identity rows of D and zero rows of N)
   Z bpre[NR];
 #pragma unroll
   for (int c = 0; c < NR; c++) bpre[c] = (wv == 0 && live) ? B[(size_t)(r0 + lane) * ldb + c] : zmk<Z>(0, 0);
@@ -933,10 +948,27 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
     for (int idx = tid; idx < 64 * 64; idx += 256) {
       const int rr = idx >> 6, kk = idx & 63;
       const bool rok = rr < rows;
-      Ad[rr * 65 + kk] = (rok && kk < rows) ? A[(size_t)(r0 + rr) * lda + r0 + kk] : zmk<Z>(0, 0);
-      if (ndep > 0) Ap[rr * 65 + kk] = (rok && kk < cols) ? A[(size_t)(r0 + rr) * lda + c0 + kk] : zmk<Z>(0, 0);
+      Dv[rr * 65 + kk] = (rok && kk < rows) ? A[(size_t)(r0 + rr) * lda + r0 + kk] : zmk<Z>(rr == kk ? 1 : 0, 0);
+      if (ndep > 0) Wn[rr * 65 + kk] = (rok && kk < cols) ? A[(size_t)(r0 + rr) * lda + c0 + kk] : zmk<Z>(0, 0);
     }
-    if (UPPER && tid < rows) Di[tid] = cdiv(zmk<Z>(1, 0), A[(size_t)(r0 + tid) * lda + r0 + tid]);
+  }
+  __syncthreads();
+  if (wv == 0) trsv_invert<Z, UPPER>(Dv, lane);
+  __syncthreads();
+  // W = D^{-1} N in place: wave w owns columns 16w..16w+15, lane = row; a
+  // wave reads a column group whole before it writes it
+  if (ndep > 0) {
+    for (int cb = 16 * wv; cb < 16 * wv + 16; cb += 4) {
+      Z s[4] = {zmk<Z>(0, 0), zmk<Z>(0, 0), zmk<Z>(0, 0), zmk<Z>(0, 0)};
+#pragma unroll 4
+      for (int m = 0; m < 64; m++) {
+        const Z d = Dv[lane * 65 + m];
+#pragma unroll
+        for (int u = 0; u < 4; u++) s[u] = s[u] + d * Wn[m * 65 + cb + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) Wn[lane * 65 + cb + u] = s[u];
+    }
   }
   Z acc[NR];
 #pragma unroll
@@ -954,13 +986,24 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
   if (wv > 0)
 #pragma unroll
     for (int c = 0; c < NR; c++) xw[lane * TRSV_MAXR + c] = acc[c];
-  __syncthreads();   // also completes the staging
+  __syncthreads();   // also completes W
   if (wv != 0) return;
-  Z v[NR];
+  // r = b - (the other blocks' terms), through LDS for the product with D^{-1}
 #pragma unroll
-  for (int c = 0; c < NR; c++)
-    v[c] = acc[c] + Xw[(64 + lane) * TRSV_MAXR + c] + Xw[(128 + lane) * TRSV_MAXR + c] +
-           Xw[(192 + lane) * TRSV_MAXR + c];
+  for (int c = 0; c < NR; c++) {
+    const Z v = acc[c] + Xw[(64 + lane) * TRSV_MAXR + c] + Xw[(128 + lane) * TRSV_MAXR + c] +
+                Xw[(192 + lane) * TRSV_MAXR + c];
+    xw[lane * TRSV_MAXR + c] = live ? bpre[c] - v : zmk<Z>(0, 0);
+  }
+  Z y[NR];
+#pragma unroll
+  for (int c = 0; c < NR; c++) y[c] = zmk<Z>(0, 0);
+#pragma unroll 8
+  for (int k = 0; k < 64; k++) {
+    const Z d = Dv[lane * 65 + k];
+#pragma unroll
+    for (int c = 0; c < NR; c++) y[c] = y[c] + d * xw[k * TRSV_MAXR + c];
+  }
   // the neighbour: the wait on the critical path
   if (ndep > 0) {
     trsv_wait(w, jlast, epoch, lane);
@@ -968,29 +1011,14 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
     trsv_fetch_x(xw, xs, c0, cols, lane);
 #pragma unroll 8
     for (int k = 0; k < 64; k++) {
-      const Z a = Ap[lane * 65 + k];
+      const Z a = Wn[lane * 65 + k];
 #pragma unroll
-      for (int c = 0; c < NR; c++) v[c] = v[c] + a * xw[k * TRSV_MAXR + c];
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < NR; c++) v[c] = live ? bpre[c] - v[c] : zmk<Z>(0, 0);
-  // the diagonal block: x_k broadcast from lane k
-  for (int t = 0; t < rows; t++) {
-    const int k = UPPER ? rows - 1 - t : t;
-    const bool below = UPPER ? lane < k : (lane > k && live);
-    const Z l = below ? Ad[lane * 65 + k] : zmk<Z>(0, 0);
-    const Z dk = UPPER ? Di[k] : zmk<Z>(1, 0);
-#pragma unroll
-    for (int c = 0; c < NR; c++) {
-      Z xk = zmk<Z>(lane_bcast(v[c].x, k), lane_bcast(v[c].y, k));
-      if (UPPER) xk = xk * dk;
-      v[c] = lane == k ? xk : v[c] - l * xk;
+      for (int c = 0; c < NR; c++) y[c] = y[c] - a * xw[k * TRSV_MAXR + c];
     }
   }
   if (live) {
 #pragma unroll
-    for (int c = 0; c < NR; c++) xs[(size_t)(r0 + lane) * TRSV_MAXR + c] = v[c];
+    for (int c = 0; c < NR; c++) xs[(size_t)(r0 + lane) * TRSV_MAXR + c] = y[c];
   }
   __builtin_amdgcn_s_waitcnt(0);   // this wave's x stores have completed
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -998,7 +1026,7 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
   // this block's rows of B after the flag: no other block reads them
   if (live) {
 #pragma unroll
-    for (int c = 0; c < NR; c++) B[(size_t)(r0 + lane) * ldb + c] = v[c];
+    for (int c = 0; c < NR; c++) B[(size_t)(r0 + lane) * ldb + c] = y[c];
   }
 }
 
