@@ -666,19 +666,52 @@ __global__ void __launch_bounds__(256) k_panel_perm(const int *__restrict__ pv, 
     for (int t = threadIdx.x; t < m; t += blockDim.x) rowAtG[t] = rowAt[t];
 }
 
-// apply interchanges piv[k0..k1) to columns [c0, c1) of A (or of a RHS block)
+// apply interchanges piv[k0..k1) to columns [c0, c1) of A (or of a RHS
+// block), one column per thread.  The interchanges are read into LDS in
+// chunks first, those that change a row compacted to the front of the
+// chunk, so the dependent loop over them holds no global loads of piv and
+// no iterations without a swap.
+#define LASWP_CHUNK 2048
 template <class Z>
 __global__ void __launch_bounds__(256) k_laswp(Z *A, size_t lda, int c0, int c1, int k0, int k1, const int *piv)
 {
-  int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
-  for (int j = k0; j < k1; j++) {
-    int p = piv[j];
-    if (p != j) {
-      Z a = A[(size_t)j * lda + c];
-      A[(size_t)j * lda + c] = A[(size_t)p * lda + c];
-      A[(size_t)p * lda + c] = a;
+  __shared__ int sj[LASWP_CHUNK], sp[LASWP_CHUNK];
+  __shared__ int cnt, wbase[4];
+  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j0 = k0; j0 < k1; j0 += LASWP_CHUNK) {
+    const int len = min(LASWP_CHUNK, k1 - j0);
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    // compaction in order: each wave ballots its swaps, wave results are
+    // placed in wave order by a running count
+    for (int t0 = 0; t0 < len; t0 += 256) {
+      const int t = t0 + threadIdx.x;
+      const int p = t < len ? piv[j0 + t] : 0;
+      const bool sw = t < len && p != j0 + t;
+      const unsigned long long m = __ballot(sw);
+      if (lane == 0) wbase[wv] = __popcll(m);
+      __syncthreads();
+      int base = cnt;
+      for (int u = 0; u < wv; u++) base += wbase[u];
+      if (sw) {
+        const int slot = base + __popcll(m & ((1ull << lane) - 1));
+        sj[slot] = j0 + t;
+        sp[slot] = p;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) cnt += wbase[0] + wbase[1] + wbase[2] + wbase[3];
+      __syncthreads();
     }
+    const int ns = cnt;
+    if (c < c1)
+      for (int q = 0; q < ns; q++) {
+        const int j = sj[q], p = sp[q];
+        const Z a = A[(size_t)j * lda + c];
+        A[(size_t)j * lda + c] = A[(size_t)p * lda + c];
+        A[(size_t)p * lda + c] = a;
+      }
+    __syncthreads();   // the chunk's tables are reused
   }
 }
 
