@@ -90,7 +90,7 @@ struct Ctx {
   hipStream_t st = nullptr;
   QPt *d_gl = nullptr, *d_gh = nullptr;   // 3-part triangle rules
   int n_gl = 0, n_gh = 0;
-  void *d_gl32 = nullptr;                 // FP32 copy of the 12x12 rule (QPt32)
+  void *d_gl32 = nullptr;                 // FP32 far-pair rules, 12x12 / 6x6 / 5x5 (QPt32)
   double *d_xl = nullptr, *d_wl = nullptr, *d_xh = nullptr, *d_wh = nullptr;
   double xl[12], wl[12], xh[64], wh[64];
   hipEvent_t ev[2];
@@ -174,7 +174,17 @@ extern "C" int bh_init(int device)
       upload(&g.d_wl, g.wl, 12) || upload(&g.d_xh, g.xh, 64) || upload(&g.d_wh, g.wh, 64))
     return BH_EHIP;
   HIPCHK(hipStreamSynchronize(g.st));
-  if (bem_rule32(a.data(), g.n_gl, &g.d_gl32)) return fail(BH_EHIP, "FP32 rule table");
+  {
+    // far-pair table of the mixed form: 12x12, 6x6, 5x5
+    double x6[6], w6[6], x5[5], w5[5];
+    gl_nodes(6, x6, w6);
+    gl_nodes(5, x5, w5);
+    std::vector<QPt> r32 = a, r6 = tri_rule(6, x6, w6), r5 = tri_rule(5, x5, w5);
+    r32.insert(r32.end(), r6.begin(), r6.end());
+    r32.insert(r32.end(), r5.begin(), r5.end());
+    if ((int)a.size() != BEM_R32_OFF6 || (int)r32.size() != BEM_R32_NPT || bem_rule32(r32.data(), BEM_R32_NPT, &g.d_gl32))
+      return fail(BH_EHIP, "FP32 rule table");
+  }
   g.dev = device;
   return 0;
 }

@@ -293,9 +293,12 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
 }
 
 // ---- pass 1, mixed-precision form ----
-// The same 12x12 rule; for far pairs (row point farther than 4 element
-// radii from the element's centre) the integrand is evaluated in FP32
-// (positions relative to the row point formed in FP64 first, so that no
+// Near pairs take the same 12x12 rule in FP64.  For far pairs (row point
+// farther than 4 element radii from the element's centre) the integrand is
+// evaluated in FP32 with the far rule the wave's far lanes need (5x5, 6x6 or
+// 12x12 squares of the 3-part map, chosen by |k| x element radius so that
+// the rule's error stays two orders below FP32 rounding), with
+// positions relative to the row point formed in FP64 first, so that no
 // cancellation enters the FP32 part), with the hardware reciprocal square
 // root, FP32 sin/cos/exp, and sums over blocks of 12 points in FP32 added
 // into FP64 accumulators.  Nearer pairs, whose large and rapidly varying
@@ -303,7 +306,6 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
 // FP64 path of k_asm_gl.  Pairs whose |dF/dt| comes within a relative 1e-4
 // of the GHN switch level are flagged too: pass 2 redoes them in FP64 and
 // takes the reference's decision there.
-#define BEM_FAR_RADII 4.0
 struct Acc32 {
   cf G[3], H[3], DG0[3], DG1[3], DH0[3], DH1[3];
   float F, DF0, DF1;
@@ -353,6 +355,8 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
     const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
                              g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
     const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
+    int qoff, nq32;
+    far_rule32(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
     AsmAcc a;
     acc_zero(a);
     if (!far)
@@ -360,7 +364,8 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
         const QPt qp = rule64[p];
         asm_point(a, g, rp, (double)Wdv0d, (double)Wdv1d, kr, ki, qp.z, qp.e, qp.w);
       }
-    for (int p0 = 0; p0 < (far ? nq : 0); p0 += 12) {
+    const QPt32 *__restrict__ r32 = rule + qoff;
+    for (int p0 = 0; p0 < (far ? nq32 : 0); p0 += 12) {
       Acc32 b;
 #pragma unroll
       for (int i = 0; i < 3; i++) {
@@ -368,9 +373,9 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
         b.DH1[i] = mkf(0, 0);
       }
       b.F = b.DF0 = b.DF1 = 0.0f;
-      const int p1 = min(p0 + 12, nq);
+      const int p1 = min(p0 + 12, nq32);
       for (int p = p0; p < p1; p++) {
-        const QPt32 q = rule[p];
+        const QPt32 q = r32[p];
         const float vx = fmaf(c2x, q.e, fmaf(c1x, q.z, d0x));
         const float vy = fmaf(c2y, q.e, fmaf(c1y, q.z, d0y));
         const float vz = fmaf(c2z, q.e, fmaf(c1z, q.z, d0z));
@@ -958,7 +963,8 @@ void bem_launch_asm_gl32(const void *geo, const void *gq, const int *sid, const 
 }
 
 // FP32 copy (device) of a 3-part rule given on the host: point, weight and
-// weight x shape functions
+// weight x shape functions (the far-pair table: the 12x12, 6x6 and 5x5 rules
+// back to back)
 int bem_rule32(const void *rule, int nq, void **out)
 {
   const QPt *r = (const QPt *)rule;

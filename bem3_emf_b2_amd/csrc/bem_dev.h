@@ -158,6 +158,26 @@ struct QPt32 {
   float z, e, w, wn0, wn1, wn2;
 };
 
+// FP32 far-pair rules of the mixed-precision form, one table: the 3-part
+// triangle map with 12x12 (432 points), 6x6 (108) and 5x5 (75) Gauss-Legendre
+// squares.  A pair farther than 4 element radii takes 5x5 while |k| times the
+// element radius is at most 1.5, 6x6 up to 3, else 12x12: against a 20x20
+// rule the relative error of every G, H, dG/dv, dH/dv group stays below
+// 5e-9 there (real and lossy k), more than an order below FP32 rounding.
+#define BEM_R32_OFF6 432
+#define BEM_R32_OFF5 540
+#define BEM_R32_NPT 615
+#define BEM_FAR_RADII 4.0
+// wave-uniform rule of the far lanes: the most points any of them needs
+// (call from every lane that may be far; kR = |k| x element radius)
+__device__ __forceinline__ void far_rule32(bool far, double kR, int &off, int &nq)
+{
+  const bool need12 = far && kR > 3.0, need6 = far && kR > 1.5;
+  if (__ballot(need12)) { off = 0; nq = 432; }
+  else if (__ballot(need6)) { off = BEM_R32_OFF6; nq = 108; }
+  else { off = BEM_R32_OFF5; nq = 75; }
+}
+
 // one incident plane wave (ipw.h:79-99 derived data)
 struct Beam {
   double ki, E0, st, ct, sp, cp, fx, fy, fz, pad;

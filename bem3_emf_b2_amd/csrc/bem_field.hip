@@ -706,7 +706,8 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
 // s != t (coef_bd, no GHN switch), polar G for s == t (lit_coef_bd; H and F
 // vanish on a flat element).
 // F32: the sums over far triangles (farther than 4 element radii) with the
-// FP32 rule gl32 (mixed-precision form); nearer elements, quadrangles and
+// FP32 far rules gl32 (mixed-precision form: 5x5, 6x6 or 12x12 squares by
+// |k| x element radius); nearer elements, quadrangles and
 // the self element stay FP64.  Quadrangles take the square's GLN rule qgl
 // and, as the self element, the polar rule of bil_coef_bd.
 template <bool F32>
@@ -744,7 +745,7 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
     int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
     cd G[4], H[4];
-    double f;
+    double f = 0.0;
     int ns = 3;
     if (g.quad != 0.0) {
       quad_coef(g, gq[as], rt, kr, ki, qgl, nqgl, sd < 0, G, H, f);
@@ -754,15 +755,17 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
       double x2;
       bool far = false;
       if constexpr (F32) {
-        // FP32 only farther than 4 element radii from the element's centre
+        // FP32 only farther than 4 element radii from the element's centre,
+        // with the far rule of the wave's far lanes
         const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
         const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
                                  g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
-        far = dx * dx + dy * dy + dz * dz > 16.0 * rad2;
+        far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
+        int qoff, nq32;
+        far_rule32(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
+        if (far) tri_coef32(g, rt, (float)kr, (float)ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f);
       }
-      if (far)
-        tri_coef32(g, rt, (float)kr, (float)ki, gl32, ngl, scale, sd < 0, G, H, f);
-      else
+      if (!far)
         tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
     }
     F += f;
