@@ -98,7 +98,7 @@ def sync():
 # kernel ids of the timing registry (bem3_hip.h)
 KERNELS = {"asm_gl": 0, "asm_gh": 1, "asm_self": 2, "zgemm": 3, "lu_panel": 4, "trsm": 5, "field": 6, "scatter": 7,
            "zgemm_small": 8, "field_bd": 9, "cgemm": 10, "cgemm_small": 11, "lu_panel_c": 12, "trsm_c": 13,
-           "asm_gl32": 14, "field_bd32": 15, "zgemv": 16, "cgemv": 17}
+           "asm_glr": 14, "field_bd32": 15, "zgemv": 16, "cgemv": 17, "scatter32": 18}
 # factorisation forms of bh_object_factor
 FACTOR_MODES = {"auto": 0, "dense": 1, "block": 2, "mixed": 3}
 

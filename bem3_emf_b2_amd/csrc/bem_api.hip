@@ -14,6 +14,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -29,9 +30,9 @@ void bem_launch_asm_gl(const void *, const void *, const int *, const void *, co
 void bem_launch_asm_gh(const void *, const void *, const int *, const void *, const int *, const void *, int, double,
                        double, const void *, int, const void *, int, const void *, int, const void *, int, void *, void *,
                        void *, void *, size_t, double *, hipStream_t);
-void bem_launch_asm_gl32(const void *, const void *, const int *, const void *, const int *, int, int, double, double,
-                         const void *, const void *, int, const void *, int, void *, void *, void *, void *, size_t,
-                         double *, unsigned long long *, int, hipStream_t);
+void bem_launch_asm_glr(const void *, const void *, const int *, const void *, const int *, int, int, double, double,
+                        const void *, const void *, int, void *, void *, void *, void *, size_t, double *,
+                        unsigned long long *, int, hipStream_t);
 int bem_rule32(const void *, int, void **);
 void bem_launch_asm_self(const void *, const void *, const int *, const void *, const int *, int, double, double,
                          const double *, const double *, const double *, const double *, void *, void *, void *, void *,
@@ -91,6 +92,7 @@ struct Ctx {
   QPt *d_gl = nullptr, *d_gh = nullptr;   // 3-part triangle rules
   int n_gl = 0, n_gh = 0;
   void *d_gl32 = nullptr;                 // FP32 far-pair rules, 12x12 / 6x6 / 5x5 (QPt32)
+  QPt *d_glr = nullptr;                   // the same rules in FP64 (QPt): far pairs of the mixed-form assembly
   double *d_xl = nullptr, *d_wl = nullptr, *d_xh = nullptr, *d_wh = nullptr;
   double xl[12], wl[12], xh[64], wh[64];
   hipEvent_t ev[2];
@@ -182,8 +184,10 @@ extern "C" int bh_init(int device)
     std::vector<QPt> r32 = a, r6 = tri_rule(6, x6, w6), r5 = tri_rule(5, x5, w5);
     r32.insert(r32.end(), r6.begin(), r6.end());
     r32.insert(r32.end(), r5.begin(), r5.end());
-    if ((int)a.size() != BEM_R32_OFF6 || (int)r32.size() != BEM_R32_NPT || bem_rule32(r32.data(), BEM_R32_NPT, &g.d_gl32))
+    if ((int)a.size() != BEM_FAR_OFF6 || (int)r32.size() != BEM_FAR_NPT || bem_rule32(r32.data(), BEM_FAR_NPT, &g.d_gl32))
       return fail(BH_EHIP, "FP32 rule table");
+    if (upload(&g.d_glr, r32.data(), r32.size())) return BH_EHIP;
+    HIPCHK(hipStreamSynchronize(g.st));
   }
   g.dev = device;
   return 0;
@@ -528,8 +532,8 @@ static int assemble_domain(bh_object *o, int d)
   double kr = o->kn[d].real(), ki = o->kn[d].imag();
   const bool mixed = o->prec == BH_PREC_MIXED;
   if (mixed)
-    bem_launch_asm_gl32(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl32, g.d_gl, g.n_gl,
-                        g_qrule[2], g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+    bem_launch_asm_glr(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_glr, g_qrule[2],
+                       g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
   else
     bem_launch_asm_gl(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, g_qrule[2],
                       g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
@@ -729,7 +733,7 @@ void bem_launch_field_memp(const double *, const int *, int, int, const void *, 
                            const void *, int, const void *, const void *, double, void *, hipStream_t);
 void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const void *, const int *, int,
                         double, double, const void *, int, const void *, int, double, const void *, int, const void *,
-                        int, const void *, const void *, void *, void *, hipStream_t);
+                        int, const void *, const void *, const void *, void *, void *, hipStream_t);
 void bem_launch_field_bd(const int *, const double *, int, int, const void *, const void *, const int *, int, double,
                          double, const void *, const void *, int, double, const void *, int, const double *,
                          const double *, const double *, const double *, const void *, const void *, const void *,
@@ -1756,18 +1760,24 @@ __global__ void k_ccd(const cd *__restrict__ Ui, const cd *__restrict__ Uip, con
   part[2 * p + 1] = 2.0 * abs2(c[3] - q[3]) / (abs2(c[3]) + abs2(q[3]));
 }
 
-static int add_scattered(bh_object *dst, bh_object *src, int *d_did)
+// the scattered field of src at dst's node slots, added into dst's incident
+// data; check: first verify that no slot lies inside src (the geometry does
+// not change during a solve, so the coupling loop checks on its first sweep)
+static int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check)
 {
-  if (domain_ids(src, dst->d_slotPt, dst->nslot, d_did)) return BH_EHIP;
-  std::vector<int> did(dst->nslot);
-  HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
-  for (int v : did)
-    if (v != 0) return fail(BH_EDOM, "scattered-field domain error (overlapping objects), domain id %d", v);
+  if (check) {
+    if (domain_ids(src, dst->d_slotPt, dst->nslot, d_did)) return BH_EHIP;
+    std::vector<int> did(dst->nslot);
+    HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
+    HIPCHK(hipStreamSynchronize(g.st));
+    for (int v : did)
+      if (v != 0) return fail(BH_EDOM, "scattered-field domain error (overlapping objects), domain id %d", v);
+  }
   Dom &S = src->dom[0];
   bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, src->d_gq, S.d_sid, S.dNe,
                      src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
-                     g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, g.st);
+                     g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], src->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr,
+                     S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, g.st);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1811,6 +1821,7 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     for (int m = 0; m < N; m++) maxslot = std::max(maxslot, objs[m]->nslot);
     int *d_did;
     HIPCHK(hipMalloc((void **)&d_did, (size_t)maxslot * sizeof(int)));
+    bool first = true;
     auto sweep = [&]() -> int {
       for (int m = 0; m < N; m++) {
         bh_object *o = objs[m];
@@ -1819,16 +1830,21 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
       }
       for (int m = 0; m < N; m++)
         for (int d = 0; d < N; d++)
-          if (d != m) { int r = add_scattered(objs[d], objs[m], d_did); if (r) return r; }
+          if (d != m) { int r = add_scattered(objs[d], objs[m], d_did, first); if (r) return r; }
+      first = false;
       for (int m = 0; m < N; m++) { int r = bh_object_solve(objs[m]); if (r) return r; }
       return 0;
     };
     for (int m = 0; m < N; m++) if ((rc = bh_object_solve(objs[m]))) { dfree(d_did); return rc; }
     if ((rc = sweep())) { dfree(d_did); return rc; }
+    // BEM3_HIP_PROGRESS=1: one line per iteration on stderr (long solves)
+    const char *pe = getenv("BEM3_HIP_PROGRESS");
+    const bool progress = pe && *pe && *pe != '0';
     for (iter = 2; iter < iter_max; iter++) {
       if ((rc = sweep())) { dfree(d_did); return rc; }
       double cc = ccd_value(objs, N);
       if (trace) trace[iter] = cc;
+      if (progress) { fprintf(stderr, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(stderr); }
       if (cc < ccd && iter > iter_min) break;
     }
     dfree(d_did);

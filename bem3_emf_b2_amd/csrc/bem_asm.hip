@@ -293,35 +293,25 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
 }
 
 // ---- pass 1, mixed-precision form ----
-// Near pairs take the same 12x12 rule in FP64.  For far pairs (row point
-// farther than 4 element radii from the element's centre) the integrand is
-// evaluated in FP32 with the far rule the wave's far lanes need (5x5, 6x6 or
-// 12x12 squares of the 3-part map, chosen by |k| x element radius so that
-// the rule's error stays two orders below FP32 rounding), with
-// positions relative to the row point formed in FP64 first, so that no
-// cancellation enters the FP32 part), with the hardware reciprocal square
-// root, FP32 sin/cos/exp, and sums over blocks of 12 points in FP32 added
-// into FP64 accumulators.  Nearer pairs, whose large and rapidly varying
-// terms would carry FP32 rounding into the cancelling F sums, take the
-// FP64 path of k_asm_gl.  Pairs whose |dF/dt| comes within a relative 1e-4
-// of the GHN switch level are flagged too: pass 2 redoes them in FP64 and
-// takes the reference's decision there.
-struct Acc32 {
-  cf G[3], H[3], DG0[3], DG1[3], DH0[3], DH1[3];
-  float F, DF0, DF1;
-};
-
-__device__ __forceinline__ void fma_cf(cf &a, cf b, float s) { a.x = fmaf(b.x, s, a.x); a.y = fmaf(b.y, s, a.y); }
-__device__ __forceinline__ void add_cf(cd &a, cf b) { a.x += (double)b.x; a.y += (double)b.y; }
-
-__global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
-                                                  const int *__restrict__ sid, const RowPt *__restrict__ rows,
-                                                  const int *__restrict__ colOff, int dNe, double kr, double ki,
-                                                  const QPt32 *__restrict__ rule, const QPt *__restrict__ rule64, int nq,
-                                                  const QPtQ *__restrict__ qrule, int nqq,
-                                                  cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
-                                                  cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
-                                                  unsigned long long *__restrict__ ghMask)
+// Far pairs (row point farther than 4 element radii from the element's
+// centre) take the reduced far rule the wave's far lanes need: 5x5, 6x6 or
+// 12x12 squares of the 3-part map, by |k| x element radius, with a rule
+// error below 5e-9 there.  Nearer pairs take the reference's 12x12 rule.
+// The arithmetic stays FP64: FP32 would put 1e-7 relative noise on the
+// matrix entries, which the solved state carries on the total field, and
+// the force on a weakly scattering object (a few per cent index contrast),
+// which comes from the scattered part, then sees it magnified past 1e-6.
+// Non-packed FP32 runs at the FP64 rate on CDNA4, so the reduced rule is
+// where the time goes.  Pairs whose |dF/dt| comes within a relative 1e-4
+// of the GHN switch level are flagged as in the FP32 form of the boundary
+// sums: pass 2 redoes them and takes the reference's decision there.
+__global__ void __launch_bounds__(256) k_asm_glr(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                 const int *__restrict__ sid, const RowPt *__restrict__ rows,
+                                                 const int *__restrict__ colOff, int dNe, double kr, double ki,
+                                                 const QPt *__restrict__ rule, const QPtQ *__restrict__ qrule, int nqq,
+                                                 cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
+                                                 cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
+                                                 unsigned long long *__restrict__ ghMask)
 {
   const int row = blockIdx.y;
   const RowPt rp = rows[row];
@@ -333,7 +323,7 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
   bool flagged = false;
   double f = 0.0, df0 = 0.0, df1 = 0.0;
   if (active && geo[sid[s] < 0 ? -sid[s] : sid[s]].quad != 0.0) {
-    // quadrangles take the FP64 path (no FP32 form of the square's rule)
+    // quadrangles: the square's 12x12 rule
     const int sd = sid[s], as = sd < 0 ? -sd : sd;
     flagged = asm_gl_quad(geo[as], gq[as], rp, sd, kr, ki, qrule, nqq, BEM_CBD_SDF * (1.0 - 1e-4), row, colOff[s], ld, G,
                           H, dG, dH, f, df0, df1);
@@ -341,87 +331,20 @@ __global__ void __launch_bounds__(256) k_asm_gl32(const TriGeo *__restrict__ geo
     int sd = sid[s];
     int as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
-    const float d0x = (float)(g.c0[0] - rp.rt[0]), d0y = (float)(g.c0[1] - rp.rt[1]), d0z = (float)(g.c0[2] - rp.rt[2]);
-    const float c1x = (float)g.c1[0], c1y = (float)g.c1[1], c1z = (float)g.c1[2];
-    const float c2x = (float)g.c2[0], c2y = (float)g.c2[1], c2z = (float)g.c2[2];
-    const float Wx = (float)g.W[0], Wy = (float)g.W[1], Wz = (float)g.W[2];
-    const float v0x = (float)rp.v0[0], v0y = (float)rp.v0[1], v0z = (float)rp.v0[2];
-    const float v1x = (float)rp.v1[0], v1y = (float)rp.v1[1], v1z = (float)rp.v1[2];
-    const double Wdv0d = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
-    const double Wdv1d = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
-    const float Wdv0 = (float)Wdv0d, Wdv1 = (float)Wdv1d;
-    const float krf = (float)kr, kif = (float)ki;
+    const double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
+    const double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
     const double dx = g.c0[0] - rp.rt[0], dy = g.c0[1] - rp.rt[1], dz = g.c0[2] - rp.rt[2];
     const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
                              g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
     const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
-    int qoff, nq32;
-    far_rule32(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
+    int qoff, nqf;
+    far_rule(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nqf);
+    if (!far) { qoff = 0; nqf = BEM_FAR_OFF6; }   // the 12x12 rule heads the table
     AsmAcc a;
     acc_zero(a);
-    if (!far)
-      for (int p = 0; p < nq; p++) {
-        const QPt qp = rule64[p];
-        asm_point(a, g, rp, (double)Wdv0d, (double)Wdv1d, kr, ki, qp.z, qp.e, qp.w);
-      }
-    const QPt32 *__restrict__ r32 = rule + qoff;
-    for (int p0 = 0; p0 < (far ? nq32 : 0); p0 += 12) {
-      Acc32 b;
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        b.G[i] = mkf(0, 0); b.H[i] = mkf(0, 0); b.DG0[i] = mkf(0, 0); b.DG1[i] = mkf(0, 0); b.DH0[i] = mkf(0, 0);
-        b.DH1[i] = mkf(0, 0);
-      }
-      b.F = b.DF0 = b.DF1 = 0.0f;
-      const int p1 = min(p0 + 12, nq32);
-      for (int p = p0; p < p1; p++) {
-        const QPt32 q = r32[p];
-        const float vx = fmaf(c2x, q.e, fmaf(c1x, q.z, d0x));
-        const float vy = fmaf(c2y, q.e, fmaf(c1y, q.z, d0y));
-        const float vz = fmaf(c2z, q.e, fmaf(c1z, q.z, d0z));
-        const float R2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
-        const float iR = rsqrtf(R2), R = R2 * iR, iR2 = iR * iR, iR3 = iR2 * iR;
-        const float rdW = fmaf(vx, Wx, fmaf(vy, Wy, vz * Wz)) * iR;
-        const float rd0 = fmaf(vx, v0x, fmaf(vy, v0y, vz * v0z)) * iR;
-        const float rd1 = fmaf(vx, v1x, fmaf(vy, v1y, vz * v1z)) * iR;
-        float sn, cs;
-        sincosf(krf * R, &sn, &cs);
-        if (kif != 0.0f) {
-          const float at = expf(-kif * R);
-          sn *= at;
-          cs *= at;
-        }
-        const cf ce = mkf(cs, sn);
-        const cf cb = mkf(-1.0f - kif * R, krf * R);   // i k R - 1
-        const cf cbce = cb * ce;
-        const cf gk = ce * iR;
-        const cf hk = cbce * (rdW * iR2);
-        const cf tk = cbce * iR2;
-        const cf kR = mkf(krf * R, kif * R);
-        const cf qq = kR * kR + cb * 3.0f;
-        const cf ce3 = ce * iR3;
-        const cf dh0 = ce3 * (qq * (rd0 * rdW) - cb * Wdv0);
-        const cf dh1 = ce3 * (qq * (rd1 * rdW) - cb * Wdv1);
-        const float wn[3] = {q.wn0, q.wn1, q.wn2};
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          fma_cf(b.G[i], gk, wn[i]);
-          fma_cf(b.H[i], hk, wn[i]);
-          fma_cf(b.DG0[i], tk, wn[i] * rd0);
-          fma_cf(b.DG1[i], tk, wn[i] * rd1);
-          fma_cf(b.DH0[i], dh0, wn[i]);
-          fma_cf(b.DH1[i], dh1, wn[i]);
-        }
-        b.F = fmaf(q.w, rdW * iR2, b.F);
-        b.DF0 = fmaf(q.w, iR3 * fmaf(3.0f * rd0, rdW, -Wdv0), b.DF0);
-        b.DF1 = fmaf(q.w, iR3 * fmaf(3.0f * rd1, rdW, -Wdv1), b.DF1);
-      }
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        add_cf(a.G[i], b.G[i]); add_cf(a.H[i], b.H[i]); add_cf(a.DG0[i], b.DG0[i]); add_cf(a.DG1[i], b.DG1[i]);
-        add_cf(a.DH0[i], b.DH0[i]); add_cf(a.DH1[i], b.DH1[i]);
-      }
-      a.F += (double)b.F; a.DF0 += (double)b.DF0; a.DF1 += (double)b.DF1;
+    for (int p = qoff; p < qoff + nqf; p++) {
+      const QPt qp = rule[p];
+      asm_point(a, g, rp, Wdv0, Wdv1, kr, ki, qp.z, qp.e, qp.w);
     }
     acc_finish(a, g.aW, sd < 0);
     const double lim = BEM_CBD_SDF * (1.0 - 1e-4);
@@ -949,17 +872,17 @@ void bem_launch_asm_gh(const void *geo, const void *gq, const int *sid, const vo
 }
 
 // rule32: FP32 copy of the 12x12 rule (far pairs), rule: the FP64 one (near pairs)
-void bem_launch_asm_gl32(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int dNe,
-                         int nrows, double kr, double ki, const void *rule32, const void *rule, int nq, const void *qrule,
-                         int nqq, void *G, void *H, void *dG, void *dH, size_t ld, double *dfPart,
-                         unsigned long long *ghMask, int nwv_blocks, hipStream_t st)
+void bem_launch_asm_glr(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int dNe,
+                        int nrows, double kr, double ki, const void *rule, const void *qrule, int nqq, void *G, void *H,
+                        void *dG, void *dH, size_t ld, double *dfPart, unsigned long long *ghMask, int nwv_blocks,
+                        hipStream_t st)
 {
   dim3 grid(nwv_blocks, nrows);
-  bem_prof_begin(BH_K_ASM_GL32, (double)nrows * dNe, st);
-  hipLaunchKernelGGL(k_asm_gl32, grid, dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
-                     (const RowPt *)rows, colOff, dNe, kr, ki, (const QPt32 *)rule32, (const QPt *)rule, nq,
-                     (const QPtQ *)qrule, nqq, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
-  bem_prof_end(BH_K_ASM_GL32, st);
+  bem_prof_begin(BH_K_ASM_GLR, (double)nrows * dNe, st);
+  hipLaunchKernelGGL(k_asm_glr, grid, dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
+                     (const RowPt *)rows, colOff, dNe, kr, ki, (const QPt *)rule, (const QPtQ *)qrule, nqq, (cd *)G,
+                     (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
+  bem_prof_end(BH_K_ASM_GLR, st);
 }
 
 // FP32 copy (device) of a 3-part rule given on the host: point, weight and

@@ -158,24 +158,53 @@ struct QPt32 {
   float z, e, w, wn0, wn1, wn2;
 };
 
-// FP32 far-pair rules of the mixed-precision form, one table: the 3-part
-// triangle map with 12x12 (432 points), 6x6 (108) and 5x5 (75) Gauss-Legendre
-// squares.  A pair farther than 4 element radii takes 5x5 while |k| times the
-// element radius is at most 1.5, 6x6 up to 3, else 12x12: against a 20x20
-// rule the relative error of every G, H, dG/dv, dH/dv group stays below
-// 5e-9 there (real and lossy k), more than an order below FP32 rounding.
-#define BEM_R32_OFF6 432
-#define BEM_R32_OFF5 540
-#define BEM_R32_NPT 615
+// Far-pair rules of the mixed-precision form, one table (an FP64 copy for
+// its assembly, an FP32 copy for its boundary-field and coupling sums): the
+// 3-part triangle map with 12x12 (432 points), 6x6 (108) and 5x5 (75)
+// Gauss-Legendre squares.  A pair farther than 4 element radii takes 5x5
+// while |k| times the element radius is at most 1.5, 6x6 up to 3, else
+// 12x12: against a 20x20 rule the relative error of every G, H, dG/dv,
+// dH/dv group stays below 5e-9 there (real and lossy k), more than an
+// order below FP32 rounding.
+#define BEM_FAR_OFF6 432
+#define BEM_FAR_OFF5 540
+#define BEM_FAR_NPT 615
 #define BEM_FAR_RADII 4.0
+// Phase of an FP32 far-pair point.  e^{ikR} is split as e^{ik Rc} e^{ik dR}:
+// Rc, the distance from the row point to the element's centre, and its
+// factor are formed once per pair in FP64 (far32_centre); a point adds only
+// dR = R - Rc = (2 d0.dl + |dl|^2) / (R + Rc), with d0 the centre offset and
+// dl the point's offset inside the element, so the FP32 rounding of d0
+// (relative 6e-8 of R, a phase error of 6e-8 k R) stays out of the phase.
+__device__ __forceinline__ cf far32_centre(double dx, double dy, double dz, double kr, double ki, float &Rc)
+{
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  double s, c;
+  sincos(kr * r, &s, &c);
+  const double a = ki != 0.0 ? exp(-ki * r) : 1.0;
+  Rc = (float)r;
+  return mkf((float)(c * a), (float)(s * a));
+}
+__device__ __forceinline__ cf far32_phase(cf ec, float dR, float kr, float ki)
+{
+  float sn, cs;
+  sincosf(kr * dR, &sn, &cs);
+  if (ki != 0.0f) {
+    const float at = expf(-ki * dR);
+    sn *= at;
+    cs *= at;
+  }
+  return ec * mkf(cs, sn);
+}
+
 // wave-uniform rule of the far lanes: the most points any of them needs
 // (call from every lane that may be far; kR = |k| x element radius)
-__device__ __forceinline__ void far_rule32(bool far, double kR, int &off, int &nq)
+__device__ __forceinline__ void far_rule(bool far, double kR, int &off, int &nq)
 {
   const bool need12 = far && kR > 3.0, need6 = far && kR > 1.5;
   if (__ballot(need12)) { off = 0; nq = 432; }
-  else if (__ballot(need6)) { off = BEM_R32_OFF6; nq = 108; }
-  else { off = BEM_R32_OFF5; nq = 75; }
+  else if (__ballot(need6)) { off = BEM_FAR_OFF6; nq = 108; }
+  else { off = BEM_FAR_OFF5; nq = 75; }
 }
 
 // one incident plane wave (ipw.h:79-99 derived data)

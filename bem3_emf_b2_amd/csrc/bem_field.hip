@@ -76,53 +76,78 @@ __device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, cons
 
 // G[3], H[3], F at rt from one triangle with the FP32 copy of a rule
 // (mixed-precision form): position relative to rt formed in FP64, the
-// integrand in FP32, sums over blocks of 12 points added in FP64
+// integrand in FP32, sums over blocks of 12 points added in FP64; optional
+// derivative along v (dG, dH, dF) as tri_coef
 __device__ __forceinline__ void fma_cf(cf &a, cf b, float s) { a.x = fmaf(b.x, s, a.x); a.y = fmaf(b.y, s, a.y); }
-__device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, float kr, float ki,
+template <bool DER>
+__device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, const double *v, double krd, double kid,
                                            const QPt32 *__restrict__ rule, int nq, double scale, bool flip, cd *G, cd *H,
-                                           double &F)
+                                           double &F, cd *dG, cd *dH, double &dF)
 {
+  const float kr = (float)krd, ki = (float)kid;
   const float d0x = (float)(g.c0[0] - rt[0]), d0y = (float)(g.c0[1] - rt[1]), d0z = (float)(g.c0[2] - rt[2]);
   const float c1x = (float)g.c1[0], c1y = (float)g.c1[1], c1z = (float)g.c1[2];
   const float c2x = (float)g.c2[0], c2y = (float)g.c2[1], c2z = (float)g.c2[2];
-  const float Wx = (float)g.W[0], Wy = (float)g.W[1], Wz = (float)g.W[2];
+  const float vx0 = DER ? (float)v[0] : 0.0f, vy0 = DER ? (float)v[1] : 0.0f, vz0 = DER ? (float)v[2] : 0.0f;
+  const float Wdv = DER ? (float)(g.W[0] * v[0] + g.W[1] * v[1] + g.W[2] * v[2]) : 0.0f;
   cd aG[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, aH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
-  double f = 0.0;
+  cd aDG[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, aDH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+  double f = 0.0, df = 0.0;
+  float Rc;
+  const cf ec = far32_centre(g.c0[0] - rt[0], g.c0[1] - rt[1], g.c0[2] - rt[2], krd, kid, Rc);
+  // (r - rt).W is the same at every point of a flat triangle: formed once in
+  // FP64, so that coplanar pairs keep H = F = 0 exactly
+  const float dW = (float)((g.c0[0] - rt[0]) * g.W[0] + (g.c0[1] - rt[1]) * g.W[1] + (g.c0[2] - rt[2]) * g.W[2]);
   for (int p0 = 0; p0 < nq; p0 += 12) {
     cf bG[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)}, bH[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)};
-    float bf = 0.0f;
+    cf bDG[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)}, bDH[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)};
+    float bf = 0.0f, bdf = 0.0f;
     const int p1 = min(p0 + 12, nq);
     for (int p = p0; p < p1; p++) {
       const QPt32 q = rule[p];
-      const float vx = fmaf(c2x, q.e, fmaf(c1x, q.z, d0x));
-      const float vy = fmaf(c2y, q.e, fmaf(c1y, q.z, d0y));
-      const float vz = fmaf(c2z, q.e, fmaf(c1z, q.z, d0z));
+      const float lx = fmaf(c2x, q.e, c1x * q.z), ly = fmaf(c2y, q.e, c1y * q.z), lz = fmaf(c2z, q.e, c1z * q.z);
+      const float vx = d0x + lx, vy = d0y + ly, vz = d0z + lz;
       const float R2 = fmaf(vx, vx, fmaf(vy, vy, vz * vz));
       const float iR = rsqrtf(R2), R = R2 * iR, iR2 = iR * iR;
-      const float rdW = fmaf(vx, Wx, fmaf(vy, Wy, vz * Wz)) * iR;
-      float sn, cs;
-      sincosf(kr * R, &sn, &cs);
-      if (ki != 0.0f) {
-        const float at = expf(-ki * R);
-        sn *= at;
-        cs *= at;
-      }
-      const cf ce = mkf(cs, sn);
+      const float rdW = dW * iR;
+      const float dR = fmaf(2.0f, fmaf(d0x, lx, fmaf(d0y, ly, d0z * lz)), fmaf(lx, lx, fmaf(ly, ly, lz * lz))) / (R + Rc);
+      const cf ce = far32_phase(ec, dR, kr, ki);
       const cf cb = mkf(-1.0f - ki * R, kr * R);
-      const cf gk = ce * iR, hk = (cb * ce) * (rdW * iR2);
+      const cf cbce = cb * ce;
+      const cf gk = ce * iR, hk = cbce * (rdW * iR2);
       const float wn[3] = {q.wn0, q.wn1, q.wn2};
 #pragma unroll
       for (int i = 0; i < 3; i++) { fma_cf(bG[i], gk, wn[i]); fma_cf(bH[i], hk, wn[i]); }
       bf = fmaf(q.w, rdW * iR2, bf);
+      if (DER) {
+        const float rd = fmaf(vx, vx0, fmaf(vy, vy0, vz * vz0)) * iR, iR3 = iR2 * iR;
+        const cf tk = cbce * iR2;
+        const cf kR = mkf(kr * R, ki * R);
+        const cf qq = kR * kR + cb * 3.0f;
+        const cf dh = (ce * iR3) * (qq * (rd * rdW) - cb * Wdv);
+#pragma unroll
+        for (int i = 0; i < 3; i++) { fma_cf(bDG[i], tk, wn[i] * rd); fma_cf(bDH[i], dh, wn[i]); }
+        bdf = fmaf(q.w, iR3 * fmaf(3.0f * rd, rdW, -Wdv), bdf);
+      }
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) { aG[i] = aG[i] + to_cd(bG[i]); aH[i] = aH[i] + to_cd(bH[i]); }
     f += (double)bf;
+    if (DER) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) { aDG[i] = aDG[i] + to_cd(bDG[i]); aDH[i] = aDH[i] + to_cd(bDH[i]); }
+      df += (double)bdf;
+    }
   }
   double sh = flip ? -scale : scale;
 #pragma unroll
   for (int i = 0; i < 3; i++) { G[i] = aG[i] * (scale * g.aW); H[i] = aH[i] * sh; }
   F = f * sh;
+  if (DER) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) { dG[i] = aDG[i] * (-scale * g.aW); dH[i] = aDH[i] * sh; }
+    dF = df * sh;
+  }
 }
 
 // ---------------------------------------------------------------- quadrangles
@@ -642,8 +667,8 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
                                                  double kr, double ki, const QPtN *__restrict__ gl, int ngl,
                                                  const QPtN *__restrict__ gh, int ngh, double scale,
                                                  const QPtQ *__restrict__ qgl, int nqgl, const QPtQ *__restrict__ qgh,
-                                                 int nqgh, const cd *__restrict__ U, const cd *__restrict__ dU,
-                                                 cd *__restrict__ Ui, cd *__restrict__ dUi)
+                                                 int nqgh, const QPt32 *__restrict__ gl32, const cd *__restrict__ U,
+                                                 const cd *__restrict__ dU, cd *__restrict__ Ui, cd *__restrict__ dUi)
 {
   const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -675,8 +700,26 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
     }
     cd G[3], H[3], DG[3], DH[3];
     double f, df;
-    tri_coef<true>(g, rt, v, kr, ki, gl, ngl, scale, sd < 0, G, H, f, DG, DH, df);
-    if (fabs(df) > BEM_CBD_SDF) tri_coef<true>(g, rt, v, kr, ki, gh, ngh, scale, sd < 0, G, H, f, DG, DH, df);
+    bool done = false;
+    if (gl32) {
+      // mixed-precision form: far triangles in FP32 with the far rule of the
+      // wave's far lanes; a result within 1e-4 of the GHN switch level is
+      // redone in FP64 below (far pairs stay orders of magnitude under it)
+      const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+      const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
+                               g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+      const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
+      int qoff, nq32;
+      far_rule(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
+      if (far) {
+        tri_coef32<true>(g, rt, v, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, DG, DH, df);
+        done = !(fabs(df) > BEM_CBD_SDF * (1.0 - 1e-4));
+      }
+    }
+    if (!done) {
+      tri_coef<true>(g, rt, v, kr, ki, gl, ngl, scale, sd < 0, G, H, f, DG, DH, df);
+      if (fabs(df) > BEM_CBD_SDF) tri_coef<true>(g, rt, v, kr, ki, gh, ngh, scale, sd < 0, G, H, f, DG, DH, df);
+    }
     F += f; dF += df;
     const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
 #pragma unroll
@@ -762,8 +805,8 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
                                  g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
         far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
         int qoff, nq32;
-        far_rule32(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
-        if (far) tri_coef32(g, rt, (float)kr, (float)ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f);
+        far_rule(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
+        if (far) tri_coef32<false>(g, rt, nullptr, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, x0, x1, x2);
       }
       if (!far)
         tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
@@ -1154,18 +1197,20 @@ void bem_launch_field_memp(const double *pts, const int *pdid, int np, int want_
                      (cd *)out);
   bem_prof_end(BH_K_FIELD, st);
 }
+// gl32 != nullptr: the mixed-precision form (FP32 far rules)
 void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const void *gq,
                         const int *sid, int dNe, double kr, double ki, const void *gl, int ngl, const void *gh, int ngh,
-                        double scale, const void *qgl, int nqgl, const void *qgh, int nqgh, const void *U,
-                        const void *dU, void *Ui, void *dUi, hipStream_t st)
+                        double scale, const void *qgl, int nqgl, const void *qgh, int nqgh, const void *gl32,
+                        const void *U, const void *dU, void *Ui, void *dUi, hipStream_t st)
 {
   if (np <= 0) return;
-  bem_prof_begin(BH_K_SCATTER, (double)np * dNe, st);
+  const int kid = gl32 ? BH_K_SCATTER32 : BH_K_SCATTER;
+  bem_prof_begin(kid, (double)np * dNe, st);
   hipLaunchKernelGGL(k_scatter, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo,
                      (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale,
-                     (const QPtQ *)qgl, nqgl, (const QPtQ *)qgh, nqgh, (const cd *)U, (const cd *)dU, (cd *)Ui,
-                     (cd *)dUi);
-  bem_prof_end(BH_K_SCATTER, st);
+                     (const QPtQ *)qgl, nqgl, (const QPtQ *)qgh, nqgh, (const QPt32 *)gl32, (const cd *)U,
+                     (const cd *)dU, (cd *)Ui, (cd *)dUi);
+  bem_prof_end(kid, st);
 }
 // gl32 != nullptr: the mixed-precision form (FP32 copy of the same rule);
 // gq, qgl: quadrangle terms and the square's GLN rule

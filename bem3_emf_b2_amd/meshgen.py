@@ -75,6 +75,15 @@ def box(edge, n, center=(0.0, 0.0, 0.0)):
     return orient_outward(V, np.array(faces, dtype=np.int64))
 
 
+def cubesphere(radius, n, center=(0.0, 0.0, 0.0)):
+    """Sphere from a cube whose faces are cut into n x n squares, vertices
+    projected onto the sphere (12 n^2 triangles: any count of that form, where
+    an icosphere offers only 20 x 4^level)."""
+    V, F = box(2.0, n)
+    V = V / np.linalg.norm(V, axis=1)[:, None] * radius + np.asarray(center, dtype=np.float64)
+    return orient_outward(V, F)
+
+
 def cone(radius, height, n_ring, n_side, center=(0.0, 0.0, 0.0)):
     """Right circular cone (apex up) with a flat base: n_ring azimuthal
     segments, n_side rings along the slant and the base."""
@@ -191,8 +200,10 @@ def paired_arrays(V, F, domain=1):
 
 def scene16(seed=2024, box_um=10.0, n_obj=16):
     """Seeded uneven multi-object scene: spheres, cubes and cones of size
-    U(0.3,1.5) um with 500..5000 triangles each, refractive index
-    U(1.4,1.7), placed without overlap (bounding spheres) in a cube."""
+    U(0.3,1.5) um with 500..5000 triangles each (the nearest count the
+    mesher offers inside that range), refractive index U(1.4,1.7), placed
+    without overlap (bounding spheres) in a cube.  'V' is centred at the
+    origin; 'center' is the object's position."""
     rng = np.random.default_rng(seed)
     objs = []
     centers = []
@@ -209,14 +220,13 @@ def scene16(seed=2024, box_um=10.0, n_obj=16):
         else:
             continue
         centers.append((c, rb))
+        # 12 n^2 triangles in [500, 5000]: n in [7, 20]; cones 2m (4 (m // 2) - 2)
         if kind == "sphere":
-            lvl = max(0, int(round(math.log(ntri / 20.0, 4))))
-            V, F = icosphere(lvl, 0.5 * size)
+            V, F = cubesphere(0.5 * size, min(20, max(7, int(round(math.sqrt(ntri / 12.0))))))
         elif kind == "cube":
-            n = max(1, int(round(math.sqrt(ntri / 12.0))))
-            V, F = box(size, n)
+            V, F = box(size, min(20, max(7, int(round(math.sqrt(ntri / 12.0))))))
         else:
-            m = max(3, int(round(math.sqrt(ntri / 4.0))))
+            m = min(35, max(12, int(round(math.sqrt(ntri / 4.0)))))
             V, F = cone(0.5 * size, size, 2 * m, max(1, m // 2))
         objs.append(dict(kind=str(kind), size=size, n=nidx, center=c, V=V, F=F))
     return objs

@@ -155,6 +155,8 @@ def main():
                     help="timed steps of the mixed-precision form after the FP64 ones (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB (scale of the HBM byte counters)")
+    ap.add_argument("--scene16", action="store_true",
+                    help="only time the 16-object uneven scene (both forms, one configuration each)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -173,6 +175,9 @@ def main():
 
     import bem3_emf_b2_amd as bem
     bem.init(0)
+    if args.scene16:
+        scene16_main(bem)
+        return
     arrays, beams = workload(args.level)
     rn, ed, md, sd, gd = arrays
     obj = bem.Object(rn, ed, md, sd, [beams[0][1], 1.59], beams[0][0])
@@ -268,6 +273,79 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed_form": mixed,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def scene16_run(bem, objs, beams, pts, form):
+    """one configuration of the 16-object scene in one form: assembly and
+    factorisation of every object, then the coupling loop to a 1e-8 change
+    (timed); fields at the given points and force/torque of every object
+    afterwards (untimed, for the comparison of the forms)"""
+    from bem3_emf_b2_amd import meshgen as mg
+    G = []
+    for o in objs:
+        rn, ed, md, sd, gd = mg.paired_arrays(o["V"] + o["center"], o["F"])
+        g = bem.Object(rn, ed, md, sd, [beams[0][1], o["n"]], beams[0][0])
+        g.set_precision(form)
+        g.set_beams(beams)
+        G.append(g)
+    bem.prof_enable(True)
+    bem.sync()
+    t0 = time.perf_counter()
+    for i, g in enumerate(G):
+        g.assemble(2)
+        g.factor(0)
+        bem.sync()
+        log("%s: object %d (%d triangles) assembled and factorised at %.1f s" % (form, i, g.Ne, time.perf_counter() - t0))
+    t1 = time.perf_counter()
+    it, tr = bem.mo_solve(G, ccd=1e-8)
+    bem.sync()
+    t2 = time.perf_counter()
+    kern = kernel_table(bem, 1)
+    bem.prof_enable(False)
+    log("%s: coupling loop %d iterations, %.1f s" % (form, it, t2 - t1))
+    fld, oid = bem.mo_field(G, pts, "EH", "t", 1)
+    FN = [bem.mo_force(G, m, 0, tuple(objs[m]["center"]))[1:] for m in range(len(G))]
+    for g in G:
+        g.close()
+    return {"s": t2 - t0, "objects_s": t1 - t0, "coupling_s": t2 - t1, "iterations": it, "fld": fld, "oid": oid,
+            "FN": FN, "kernels_ms": {k: v[0] for k, v in kern.items() if v[2]}}
+
+
+def scene16_main(bem):
+    """the batching configuration: 16 objects (seeded spheres, cubes and
+    cones, 0.3-1.5 um, 500-5000 triangles, n 1.4-1.7) in a 10 um cube of
+    water, x-polarised 1.064 um plane wave, coupling to a 1e-8 change; the
+    plain FP64 form and the mixed-precision form, one configuration each"""
+    import numpy as np
+
+    from bem3_emf_b2_amd import meshgen as mg
+    os.environ["BEM3_HIP_PROGRESS"] = "1"
+    objs = mg.scene16()
+    S = sum(len(o["F"]) for o in objs)
+    beams = mg.xpol_plane_wave(lambda0=1.064, n0=1.33, E0=1.0)
+    pts = np.random.default_rng(20240602).uniform(-5.0, 5.0, size=(200, 3))
+    log("scene16: %d objects, %d triangles" % (len(objs), S))
+    r64 = scene16_run(bem, objs, beams, pts, "fp64")
+    rmx = scene16_run(bem, objs, beams, pts, "mixed")
+    keep = (r64["oid"] < 0) & (rmx["oid"] < 0)
+    fe = float(np.abs(rmx["fld"][keep] - r64["fld"][keep]).max() / np.abs(r64["fld"][keep]).max())
+    Fe = max(float(np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()) for a, b in zip(rmx["FN"], r64["FN"]))
+    Ne_ = max(float(np.abs(a[1] - b[1]).max() / max(np.abs(b[1]).max(), np.abs(b[0]).max()))
+              for a, b in zip(rmx["FN"], r64["FN"]))
+    out = {
+        "metric": METRIC, "value": r64["s"], "unit": "s/config", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1000.0 * r64["s"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic",
+        "config": {"workload": "16-object uneven scene (seeded spheres, cubes, cones 0.3-1.5 um, 500-5000 triangles, "
+                               "n 1.4-1.7) in a 10 um cube of water, x-pol plane wave 1.064 um, coupling to 1e-8; "
+                               "plain FP64 form: assembly + factor of every object + coupling loop",
+                   "objects": len(objs), "triangles": S, "precision": "fp64"},
+        "fp64": {k: r64[k] for k in ("s", "objects_s", "coupling_s", "iterations", "kernels_ms")},
+        "mixed_form": {"value": rmx["s"], "unit": "s/config", "speedup_vs_fp64": r64["s"] / rmx["s"],
+                       **{k: rmx[k] for k in ("objects_s", "coupling_s", "iterations", "kernels_ms")},
+                       "rel_err_vs_fp64": {"fields": fe, "force": Fe, "torque": Ne_, "field_points": int(keep.sum())}},
     }
     print(json.dumps(out), flush=True)
 

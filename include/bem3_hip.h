@@ -122,7 +122,9 @@ int bh_object_incident(bh_object *o, int which, double *buf, int set);
 /* Coupling loop over N objects (mo_solve_bv, mo_solve.c:10-88): every
  * object factorised and given the same beams.  Convergence when the
  * change measure (ccd_f) < ccd after more than iter_min iterations.
- * trace[iter] (optional) receives the change per iteration.              */
+ * trace[iter] (optional) receives the change per iteration.  With the
+ * environment variable BEM3_HIP_PROGRESS=1 each iteration also prints
+ * its change on stderr.                                                  */
 int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters);
 /* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i, mo_EH_mEMP_s/t/i;
  * mo_field.c:11-282): what 0: E,H (6 complex per point), 1: U (4 complex),
@@ -167,8 +169,9 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
  * flops of the unblocked panel updates; BH_K_TRSM: real flops; BH_K_FIELD,
  * BH_K_SCATTER, BH_K_FIELD_BD: (point, source element) pairs.
  * BH_K_CGEMM, BH_K_CGEMM_SMALL, BH_K_LU_PANEL_C, BH_K_TRSM_C,
- * BH_K_ASM_GL32, BH_K_FIELD_BD32: the FP32 kernels of the mixed-precision
- * form (same work units).  BH_K_ZGEMV, BH_K_CGEMV: the few-right-hand-side
+ * BH_K_FIELD_BD32, BH_K_SCATTER32: the FP32 kernels of the mixed-precision
+ * form (same work units); BH_K_ASM_GLR: its assembly pass (FP64, reduced
+ * far-pair rules).  BH_K_ZGEMV, BH_K_CGEMV: the few-right-hand-side
  * (n <= 8, k >= 256) product kernel, real flops (8 m n k).               */
 #define BH_K_ASM_GL   0
 #define BH_K_ASM_GH   1
@@ -184,11 +187,12 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
 #define BH_K_CGEMM_SMALL 11
 #define BH_K_LU_PANEL_C 12
 #define BH_K_TRSM_C   13
-#define BH_K_ASM_GL32 14
+#define BH_K_ASM_GLR  14
 #define BH_K_FIELD_BD32 15
 #define BH_K_ZGEMV    16
 #define BH_K_CGEMV    17
-#define BH_K_COUNT    18
+#define BH_K_SCATTER32 18
+#define BH_K_COUNT    19
 int bh_prof_enable(int on);
 int bh_prof_read(int kid, double *ms, double *work, int *launches);
 
