@@ -91,7 +91,7 @@ struct Ctx {
   hipStream_t st = nullptr;
   QPt *d_gl = nullptr, *d_gh = nullptr;   // 3-part triangle rules
   int n_gl = 0, n_gh = 0;
-  void *d_gl32 = nullptr;                 // FP32 far-pair rules, 12x12 / 6x6 / 5x5 (QPt32)
+  void *d_gl32 = nullptr;                 // FP32 far-pair rules, 12x12 / 6x6 / 5x5 / 4x4 (QPt32)
   QPt *d_glr = nullptr;                   // the same rules in FP64 (QPt): far pairs of the mixed-form assembly
   double *d_xl = nullptr, *d_wl = nullptr, *d_xh = nullptr, *d_wh = nullptr;
   double xl[12], wl[12], xh[64], wh[64];
@@ -177,13 +177,15 @@ extern "C" int bh_init(int device)
     return BH_EHIP;
   HIPCHK(hipStreamSynchronize(g.st));
   {
-    // far-pair table of the mixed form: 12x12, 6x6, 5x5
-    double x6[6], w6[6], x5[5], w5[5];
+    // far-pair table of the mixed form: 12x12, 6x6, 5x5, 4x4
+    double x6[6], w6[6], x5[5], w5[5], x4[4], w4[4];
     gl_nodes(6, x6, w6);
     gl_nodes(5, x5, w5);
-    std::vector<QPt> r32 = a, r6 = tri_rule(6, x6, w6), r5 = tri_rule(5, x5, w5);
+    gl_nodes(4, x4, w4);
+    std::vector<QPt> r32 = a, r6 = tri_rule(6, x6, w6), r5 = tri_rule(5, x5, w5), r4 = tri_rule(4, x4, w4);
     r32.insert(r32.end(), r6.begin(), r6.end());
     r32.insert(r32.end(), r5.begin(), r5.end());
+    r32.insert(r32.end(), r4.begin(), r4.end());
     if ((int)a.size() != BEM_FAR_OFF6 || (int)r32.size() != BEM_FAR_NPT || bem_rule32(r32.data(), BEM_FAR_NPT, &g.d_gl32))
       return fail(BH_EHIP, "FP32 rule table");
     if (upload(&g.d_glr, r32.data(), r32.size())) return BH_EHIP;

@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(256) k_asm_glr(const TriGeo *__restrict__ geo,
                              g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
     const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
     int qoff, nqf;
-    far_rule(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nqf);
+    far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nqf);
     if (!far) { qoff = 0; nqf = BEM_FAR_OFF6; }   // the 12x12 rule heads the table
     AsmAcc a;
     acc_zero(a);

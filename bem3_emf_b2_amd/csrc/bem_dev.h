@@ -160,15 +160,16 @@ struct QPt32 {
 
 // Far-pair rules of the mixed-precision form, one table (an FP64 copy for
 // its assembly, an FP32 copy for its boundary-field and coupling sums): the
-// 3-part triangle map with 12x12 (432 points), 6x6 (108) and 5x5 (75)
-// Gauss-Legendre squares.  A pair farther than 4 element radii takes 5x5
-// while |k| times the element radius is at most 1.5, 6x6 up to 3, else
-// 12x12: against a 20x20 rule the relative error of every G, H, dG/dv,
-// dH/dv group stays below 5e-9 there (real and lossy k), more than an
-// order below FP32 rounding.
+// 3-part triangle map with 12x12 (432 points), 6x6 (108), 5x5 (75) and 4x4
+// (48) Gauss-Legendre squares.  A pair farther than 4 element radii takes
+// 5x5 while |k| times the element radius is at most 1.5, 6x6 up to 3, else
+// 12x12; farther than 8 radii with |k| x radius at most 0.6 it takes 4x4.
+// Against a 20x20 rule the relative error of every G, H, dG/dv, dH/dv
+// group stays below 1e-8 there (real and lossy k).
 #define BEM_FAR_OFF6 432
 #define BEM_FAR_OFF5 540
-#define BEM_FAR_NPT 615
+#define BEM_FAR_OFF4 615
+#define BEM_FAR_NPT 663
 #define BEM_FAR_RADII 4.0
 // Phase of an FP32 far-pair point.  e^{ikR} is split as e^{ik Rc} e^{ik dR}:
 // Rc, the distance from the row point to the element's centre, and its
@@ -198,13 +199,16 @@ __device__ __forceinline__ cf far32_phase(cf ec, float dR, float kr, float ki)
 }
 
 // wave-uniform rule of the far lanes: the most points any of them needs
-// (call from every lane that may be far; kR = |k| x element radius)
-__device__ __forceinline__ void far_rule(bool far, double kR, int &off, int &nq)
+// (call from every lane that may be far; kR = |k| x element radius, d2 =
+// squared distance to the element's centre over the squared radius)
+__device__ __forceinline__ void far_rule(bool far, double kR, double d2, int &off, int &nq)
 {
   const bool need12 = far && kR > 3.0, need6 = far && kR > 1.5;
+  const bool need5 = far && (kR > 0.6 || d2 <= 64.0);
   if (__ballot(need12)) { off = 0; nq = 432; }
   else if (__ballot(need6)) { off = BEM_FAR_OFF6; nq = 108; }
-  else { off = BEM_FAR_OFF5; nq = 75; }
+  else if (__ballot(need5)) { off = BEM_FAR_OFF5; nq = 75; }
+  else { off = BEM_FAR_OFF4; nq = 48; }
 }
 
 // one incident plane wave (ipw.h:79-99 derived data)

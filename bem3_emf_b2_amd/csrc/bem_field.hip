@@ -710,7 +710,7 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
                                g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
       const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
       int qoff, nq32;
-      far_rule(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
+      far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
       if (far) {
         tri_coef32<true>(g, rt, v, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, DG, DH, df);
         done = !(fabs(df) > BEM_CBD_SDF * (1.0 - 1e-4));
@@ -805,7 +805,7 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
                                  g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
         far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
         int qoff, nq32;
-        far_rule(far, sqrt((kr * kr + ki * ki) * rad2), qoff, nq32);
+        far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
         if (far) tri_coef32<false>(g, rt, nullptr, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, x0, x1, x2);
       }
       if (!far)

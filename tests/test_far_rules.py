@@ -1,12 +1,13 @@
 """The far-pair rules of the mixed-precision form (bem_dev.h far_rule32):
-pairs farther than 4 element radii from the element's centre are summed in
-FP32 with 5x5 Gauss-Legendre squares of the 3-part triangle map while |k| x
-element radius <= 1.5, 6x6 up to 3, 12x12 beyond.  Checked here: over
-random triangles, directions, normals and distances from 4 to 32 radii, the
+pairs farther than 4 element radii from the element's centre take 5x5
+Gauss-Legendre squares of the 3-part triangle map while |k| x element
+radius <= 1.5, 6x6 up to 3, 12x12 beyond; pairs farther than 8 radii with
+|k| x radius <= 0.6 take 4x4.  Checked here: over random triangles,
+directions, normals and distances (4 to 32 radii; 8 to 64 for 4x4), the
 relative error of each group G_i, H_i, dG_i/dv, dH_i/dv (the integrands of
-lit_dcoef_v2.c:404-412) against a 20x20 rule stays below 5e-9, more than an
-order below FP32 rounding, for real and lossy wavenumbers, and the 12x12 rule
-of the FP64 form agrees with the 20x20 rule to round-off there."""
+lit_dcoef_v2.c:404-412) against a 20x20 rule stays below 5e-9 (1e-8 for
+4x4), for real and lossy wavenumbers, and the 12x12 rule of the FP64 form
+agrees with the 20x20 rule to round-off there."""
 import numpy as np
 import pytest
 
@@ -27,7 +28,7 @@ def tri_rule(n):
     return z, e, np.concatenate([wt, wt, wt])
 
 
-RULES = {n: tri_rule(n) for n in (5, 6, 12, 20)}
+RULES = {n: tri_rule(n) for n in (4, 5, 6, 12, 20)}
 
 
 def groups(n, c1, c2, W, x, v, k):
@@ -46,12 +47,17 @@ def groups(n, c1, c2, W, x, v, k):
             (Nw * ce / R ** 3 * (qq * rd * rdW - (ikr - 1) * (W @ v))).sum(1)]
 
 
-@pytest.mark.parametrize("n,kR,bound", [(5, 0.2, 5e-9), (5, 0.8, 5e-9), (5, 1.5, 5e-9), (6, 3.0, 5e-9),
-                                        (12, 3.0, 1e-13)])
-def test_far_rule_error(n, kR, bound):
+NEAR = (4.0, 5.0, 6.0, 8.0, 12.0, 16.0, 32.0)   # element radii from the centre
+FAR = (8.0, 12.0, 16.0, 32.0, 64.0)
+
+
+@pytest.mark.parametrize("n,kR,ratios,bound", [(5, 0.2, NEAR, 5e-9), (5, 0.8, NEAR, 5e-9), (5, 1.5, NEAR, 5e-9),
+                                               (6, 3.0, NEAR, 5e-9), (12, 3.0, NEAR, 1e-13),
+                                               (4, 0.2, FAR, 1e-8), (4, 0.6, FAR, 1e-8)])
+def test_far_rule_error(n, kR, ratios, bound):
     rng = np.random.default_rng(int(100 * kR) + n)
     worst = 0.0
-    for ratio in (4.0, 5.0, 6.0, 8.0, 12.0, 16.0, 32.0):
+    for ratio in ratios:
         for _ in range(40):
             c1 = rng.standard_normal(3)
             c1 /= np.linalg.norm(c1)
