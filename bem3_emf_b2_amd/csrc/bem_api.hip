@@ -230,6 +230,7 @@ struct Dom {
 
 struct bh_object {
   int Nn, Ne, MN, NN;
+  int nquad = 0;                       // quadrangle elements
   std::vector<double> rn;              // [Nn+1][3]
   std::vector<int> ed, md, sd, eni;    // [Ne+1][4], [Ne+1]
   std::vector<double> cr, cw, ren, wen;// [Ne+1][3][4], [Ne+1][3][3], [Ne+1][4][3] x2
@@ -411,6 +412,7 @@ static int upload_geo(bh_object *o)
     }
     geo[i].aW = sqrt(cw[0] * cw[0] + cw[3] * cw[3] + cw[6] * cw[6]);
     geo[i].quad = o->ed[4 * i + 3] != 0 ? 1.0 : 0.0;
+    if (o->ed[4 * i + 3] != 0) o->nquad++;
   }
   std::vector<QuadExt> gq(o->Ne + 1);
   memset(gq.data(), 0, gq.size() * sizeof(QuadExt));
@@ -735,7 +737,7 @@ void bem_launch_field_memp(const double *, const int *, int, int, const void *, 
                            const void *, int, const void *, const void *, double, void *, hipStream_t);
 void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const void *, const int *, int,
                         double, double, const void *, int, const void *, int, double, const void *, int, const void *,
-                        int, const void *, const void *, const void *, void *, void *, hipStream_t);
+                        int, const void *, const void *, const void *, void *, void *, int, double *, hipStream_t);
 void bem_launch_field_bd(const int *, const double *, int, int, const void *, const void *, const int *, int, double,
                          double, const void *, const void *, int, double, const void *, int, const double *,
                          const double *, const double *, const double *, const void *, const void *, const void *,
@@ -1765,7 +1767,7 @@ __global__ void k_ccd(const cd *__restrict__ Ui, const cd *__restrict__ Uip, con
 // the scattered field of src at dst's node slots, added into dst's incident
 // data; check: first verify that no slot lies inside src (the geometry does
 // not change during a solve, so the coupling loop checks on its first sweep)
-static int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check)
+static int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double *d_part)
 {
   if (check) {
     if (domain_ids(src, dst->d_slotPt, dst->nslot, d_did)) return BH_EHIP;
@@ -1779,7 +1781,7 @@ static int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check)
   bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, src->d_gq, S.d_sid, S.dNe,
                      src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
                      g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], src->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr,
-                     S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, g.st);
+                     S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, src->nquad > 0, d_part, g.st);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1822,7 +1824,9 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     int maxslot = 0;
     for (int m = 0; m < N; m++) maxslot = std::max(maxslot, objs[m]->nslot);
     int *d_did;
+    double *d_part;   // far-triangle sums of the mixed form's first coupling pass
     HIPCHK(hipMalloc((void **)&d_did, (size_t)maxslot * sizeof(int)));
+    HIPCHK(hipMalloc((void **)&d_part, (size_t)maxslot * 19 * sizeof(double)));
     bool first = true;
     auto sweep = [&]() -> int {
       for (int m = 0; m < N; m++) {
@@ -1832,24 +1836,25 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
       }
       for (int m = 0; m < N; m++)
         for (int d = 0; d < N; d++)
-          if (d != m) { int r = add_scattered(objs[d], objs[m], d_did, first); if (r) return r; }
+          if (d != m) { int r = add_scattered(objs[d], objs[m], d_did, first, d_part); if (r) return r; }
       first = false;
       for (int m = 0; m < N; m++) { int r = bh_object_solve(objs[m]); if (r) return r; }
       return 0;
     };
-    for (int m = 0; m < N; m++) if ((rc = bh_object_solve(objs[m]))) { dfree(d_did); return rc; }
-    if ((rc = sweep())) { dfree(d_did); return rc; }
+    auto done = [&](int r) { dfree(d_did); dfree(d_part); return r; };
+    for (int m = 0; m < N; m++) if ((rc = bh_object_solve(objs[m]))) return done(rc);
+    if ((rc = sweep())) return done(rc);
     // BEM3_HIP_PROGRESS=1: one line per iteration on stderr (long solves)
     const char *pe = getenv("BEM3_HIP_PROGRESS");
     const bool progress = pe && *pe && *pe != '0';
     for (iter = 2; iter < iter_max; iter++) {
-      if ((rc = sweep())) { dfree(d_did); return rc; }
+      if ((rc = sweep())) return done(rc);
       double cc = ccd_value(objs, N);
       if (trace) trace[iter] = cc;
       if (progress) { fprintf(stderr, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(stderr); }
       if (cc < ccd && iter > iter_min) break;
     }
-    dfree(d_did);
+    done(0);
   }
   for (int m = 0; m < N; m++) if ((rc = bh_object_solve_EH(objs[m]))) return rc;
   if (iters) *iters = iter;

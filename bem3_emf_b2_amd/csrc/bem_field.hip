@@ -76,10 +76,12 @@ __device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, cons
 
 // G[3], H[3], F at rt from one triangle with the FP32 copy of a rule
 // (mixed-precision form): position relative to rt formed in FP64, the
-// integrand in FP32, sums over blocks of 12 points added in FP64; optional
-// derivative along v (dG, dH, dF) as tri_coef
+// integrand in FP32, sums over blocks of 12 points added in FP64 (PAIR32:
+// the pair's sum in FP32 throughout, fewer registers, for the coupling sums
+// whose error enters only through the other objects' scattered field);
+// optional derivative along v (dG, dH, dF) as tri_coef
 __device__ __forceinline__ void fma_cf(cf &a, cf b, float s) { a.x = fmaf(b.x, s, a.x); a.y = fmaf(b.y, s, a.y); }
-template <bool DER>
+template <bool DER, bool PAIR32 = false>
 __device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, const double *v, double krd, double kid,
                                            const QPt32 *__restrict__ rule, int nq, double scale, bool flip, cd *G, cd *H,
                                            double &F, cd *dG, cd *dH, double &dF)
@@ -98,11 +100,13 @@ __device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, co
   // (r - rt).W is the same at every point of a flat triangle: formed once in
   // FP64, so that coplanar pairs keep H = F = 0 exactly
   const float dW = (float)((g.c0[0] - rt[0]) * g.W[0] + (g.c0[1] - rt[1]) * g.W[1] + (g.c0[2] - rt[2]) * g.W[2]);
-  for (int p0 = 0; p0 < nq; p0 += 12) {
+  const int blk = PAIR32 ? nq : 12;
+  for (int p0 = 0; p0 < nq; p0 += blk) {
     cf bG[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)}, bH[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)};
     cf bDG[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)}, bDH[3] = {mkf(0, 0), mkf(0, 0), mkf(0, 0)};
     float bf = 0.0f, bdf = 0.0f;
-    const int p1 = min(p0 + 12, nq);
+    const int p1 = min(p0 + blk, nq);
+#pragma unroll 2
     for (int p = p0; p < p1; p++) {
       const QPt32 q = rule[p];
       const float lx = fmaf(c2x, q.e, c1x * q.z), ly = fmaf(c2y, q.e, c1y * q.z), lz = fmaf(c2z, q.e, c1z * q.z);
@@ -129,6 +133,17 @@ __device__ __forceinline__ void tri_coef32(const TriGeo &g, const double *rt, co
         for (int i = 0; i < 3; i++) { fma_cf(bDG[i], tk, wn[i] * rd); fma_cf(bDH[i], dh, wn[i]); }
         bdf = fmaf(q.w, iR3 * fmaf(3.0f * rd, rdW, -Wdv), bdf);
       }
+    }
+    if constexpr (PAIR32) {   // one block: the pair's FP32 sums, converted once
+#pragma unroll
+      for (int i = 0; i < 3; i++) { aG[i] = to_cd(bG[i]); aH[i] = to_cd(bH[i]); }
+      f = (double)bf;
+      if (DER) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) { aDG[i] = to_cd(bDG[i]); aDH[i] = to_cd(bDH[i]); }
+        df = (double)bdf;
+      }
+      break;
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) { aG[i] = aG[i] + to_cd(bG[i]); aH[i] = aH[i] + to_cd(bH[i]); }
@@ -661,6 +676,78 @@ __global__ void __launch_bounds__(256) k_field_memp(const double *__restrict__ p
 // U, dU/dv at node slots of another object from this object's open-domain
 // boundary values; GLN with the GHN switch per pair (dcoef_rt,
 // d3b1_elem.c:277-316), added into Ui / dUi of the destination slots.
+// far triangle of the coupling sums: centre farther than 4 element radii
+// (the same FP64 test in both coupling kernels)
+__device__ __forceinline__ bool scat_far(const TriGeo &g, const double *rt)
+{
+  const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+  const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
+                           g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+  return g.quad == 0.0 && dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
+}
+
+// Mixed-precision form, first coupling pass: the far triangles only, FP32
+// with the far rule of the wave's far lanes, in a kernel of their own so
+// that none of the FP64 near-pair code's registers weigh on this loop.
+// Per point: sums of U (4 complex), dU/dv (4), F, dF, and a flag (any far
+// pair at the GHN switch level: the second pass then redoes the point).
+#define BEM_SCAT_PART 19
+__global__ void __launch_bounds__(256) k_scatter_far32(const double *__restrict__ pts, const double *__restrict__ nrm,
+                                                       int np, const TriGeo *__restrict__ geo,
+                                                       const int *__restrict__ sid, int dNe, double kr, double ki,
+                                                       double scale, const QPt32 *__restrict__ gl32,
+                                                       const cd *__restrict__ U, const cd *__restrict__ dU,
+                                                       double *__restrict__ part)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  const double rt[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  const double v[3] = {nrm[3 * p], nrm[3 * p + 1], nrm[3 * p + 2]};
+  cd aU[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, aD[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  double F = 0.0, dF = 0.0;
+  bool redo = false;
+  for (int s = 1 + lane; s <= dNe; s += 64) {
+    const int sd = sid[s], as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    const bool far = scat_far(g, rt);
+    const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+    const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
+                             g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+    int qoff, nq32;
+    far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
+    if (!far) continue;
+    cd G[3], H[3], DG[3], DH[3];
+    double f, df;
+    tri_coef32<true, true>(g, rt, v, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, DG, DH, df);
+    if (fabs(df) > BEM_CBD_SDF * (1.0 - 1e-4)) { redo = true; continue; }
+    F += f; dF += df;
+    const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
+#pragma unroll 1
+    for (int n = 0; n < 3; n++)
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        const cd u = Us[n * 4 + l], du = dUs[n * 4 + l];
+        aU[l] = aU[l] + G[n] * du - H[n] * u;
+        aD[l] = aD[l] + DG[n] * du - DH[n] * u;
+      }
+  }
+  const bool any_redo = __ballot(redo) != 0;
+  F = wsum(F); dF = wsum(dF);
+  for (int l = 0; l < 4; l++) { aU[l] = wsum(aU[l]); aD[l] = wsum(aD[l]); }
+  if (lane == 0) {
+    double *q = part + (size_t)p * BEM_SCAT_PART;
+    for (int l = 0; l < 4; l++) {
+      q[2 * l] = aU[l].x; q[2 * l + 1] = aU[l].y;
+      q[8 + 2 * l] = aD[l].x; q[9 + 2 * l] = aD[l].y;
+    }
+    q[16] = F; q[17] = dF; q[18] = any_redo ? 1.0 : 0.0;
+  }
+}
+
+// QUADS false: the source object has triangles only (the quadrangle branch
+// compiled out, which keeps the kernel's registers down)
+template <bool QUADS>
 __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts, const double *__restrict__ nrm,
                                                  const int *__restrict__ slot, int np, const TriGeo *__restrict__ geo,
                                                  const QuadExt *__restrict__ gq, const int *__restrict__ sid, int dNe,
@@ -668,7 +755,8 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
                                                  const QPtN *__restrict__ gh, int ngh, double scale,
                                                  const QPtQ *__restrict__ qgl, int nqgl, const QPtQ *__restrict__ qgh,
                                                  int nqgh, const QPt32 *__restrict__ gl32, const cd *__restrict__ U,
-                                                 const cd *__restrict__ dU, cd *__restrict__ Ui, cd *__restrict__ dUi)
+                                                 const cd *__restrict__ dU, const double *__restrict__ part,
+                                                 cd *__restrict__ Ui, cd *__restrict__ dUi)
 {
   const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -678,10 +766,13 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
   cd aU[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, aD[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   double F = 0.0, dF = 0.0;
   const double vv[1][3] = {{v[0], v[1], v[2]}};
+  // part: the far triangles' sums of k_scatter_far32 for this point, unless
+  // it met a far pair at the GHN switch level (then everything is done here)
+  const bool use_part = part && part[(size_t)p * BEM_SCAT_PART + 18] == 0.0;
   for (int s = 1 + lane; s <= dNe; s += 64) {
     int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
-    if (g.quad != 0.0) {
+    if (QUADS && g.quad != 0.0) {
       const QuadExt qx = gq[as];
       cd G[4], H[4], DG[1][4], DH[1][4];
       double f, df[1];
@@ -701,7 +792,9 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
     cd G[3], H[3], DG[3], DH[3];
     double f, df;
     bool done = false;
-    if (gl32) {
+    if (use_part) {
+      if (scat_far(g, rt)) continue;   // in part
+    } else if (gl32) {
       // mixed-precision form: far triangles in FP32 with the far rule of the
       // wave's far lanes; a result within 1e-4 of the GHN switch level is
       // redone in FP64 below (far pairs stay orders of magnitude under it)
@@ -732,6 +825,14 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
   }
   F = wsum(F); dF = wsum(dF);
   for (int l = 0; l < 4; l++) { aU[l] = wsum(aU[l]); aD[l] = wsum(aD[l]); }
+  if (use_part) {
+    const double *q = part + (size_t)p * BEM_SCAT_PART;
+    for (int l = 0; l < 4; l++) {
+      aU[l] = aU[l] + mk(q[2 * l], q[2 * l + 1]);
+      aD[l] = aD[l] + mk(q[8 + 2 * l], q[9 + 2 * l]);
+    }
+    F += q[16]; dF += q[17];
+  }
   if (lane == 0) {
     double den = 1.0 + F;   // the target is in the open domain of this object
     size_t o = (size_t)slot[p] * 4;
@@ -1201,15 +1302,24 @@ void bem_launch_field_memp(const double *pts, const int *pdid, int np, int want_
 void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const void *gq,
                         const int *sid, int dNe, double kr, double ki, const void *gl, int ngl, const void *gh, int ngh,
                         double scale, const void *qgl, int nqgl, const void *qgh, int nqgh, const void *gl32,
-                        const void *U, const void *dU, void *Ui, void *dUi, hipStream_t st)
+                        const void *U, const void *dU, void *Ui, void *dUi, int quads, double *part, hipStream_t st)
 {
   if (np <= 0) return;
+  if (gl32 && part) {
+    // mixed form: the far triangles first (FP32), then the rest
+    bem_prof_begin(BH_K_SCATTER32, (double)np * dNe, st);
+    hipLaunchKernelGGL(k_scatter_far32, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, np, (const TriGeo *)geo, sid,
+                       dNe, kr, ki, scale, (const QPt32 *)gl32, (const cd *)U, (const cd *)dU, part);
+    bem_prof_end(BH_K_SCATTER32, st);
+  } else {
+    part = nullptr;
+  }
   const int kid = gl32 ? BH_K_SCATTER32 : BH_K_SCATTER;
   bem_prof_begin(kid, (double)np * dNe, st);
-  hipLaunchKernelGGL(k_scatter, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo,
+  hipLaunchKernelGGL(quads ? k_scatter<true> : k_scatter<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo,
                      (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale,
                      (const QPtQ *)qgl, nqgl, (const QPtQ *)qgh, nqgh, (const QPt32 *)gl32, (const cd *)U,
-                     (const cd *)dU, (cd *)Ui, (cd *)dUi);
+                     (const cd *)dU, (const double *)part, (cd *)Ui, (cd *)dUi);
   bem_prof_end(kid, st);
 }
 // gl32 != nullptr: the mixed-precision form (FP32 copy of the same rule);
