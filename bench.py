@@ -157,6 +157,8 @@ def main():
                     help="only run a device copy of 1 GiB (scale of the HBM byte counters)")
     ap.add_argument("--scene16", action="store_true",
                     help="only time the 16-object uneven scene (both forms, one configuration each)")
+    ap.add_argument("--scene16-forms", default="fp64,mixed",
+                    help="forms of the 16-object scene to run (the comparison needs both)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -176,7 +178,7 @@ def main():
     import bem3_emf_b2_amd as bem
     bem.init(0)
     if args.scene16:
-        scene16_main(bem)
+        scene16_main(bem, args.scene16_forms.split(","))
         return
     arrays, beams = workload(args.level)
     rn, ed, md, sd, gd = arrays
@@ -313,7 +315,7 @@ def scene16_run(bem, objs, beams, pts, form):
             "FN": FN, "kernels_ms": {k: v[0] for k, v in kern.items() if v[2]}}
 
 
-def scene16_main(bem):
+def scene16_main(bem, forms=("fp64", "mixed")):
     """the batching configuration: 16 objects (seeded spheres, cubes and
     cones, 0.3-1.5 um, 500-5000 triangles, n 1.4-1.7) in a 10 um cube of
     water, x-polarised 1.064 um plane wave, coupling to a 1e-8 change; the
@@ -327,6 +329,12 @@ def scene16_main(bem):
     beams = mg.xpol_plane_wave(lambda0=1.064, n0=1.33, E0=1.0)
     pts = np.random.default_rng(20240602).uniform(-5.0, 5.0, size=(200, 3))
     log("scene16: %d objects, %d triangles" % (len(objs), S))
+    if set(forms) != {"fp64", "mixed"}:
+        # one form alone (for a profiler run): its timing, no comparison
+        r = scene16_run(bem, objs, beams, pts, forms[0])
+        print(json.dumps({"form": forms[0], "s": r["s"], "objects_s": r["objects_s"], "coupling_s": r["coupling_s"],
+                          "iterations": r["iterations"], "kernels_ms": r["kernels_ms"]}), flush=True)
+        return
     r64 = scene16_run(bem, objs, beams, pts, "fp64")
     rmx = scene16_run(bem, objs, beams, pts, "mixed")
     keep = (r64["oid"] < 0) & (rmx["oid"] < 0)
