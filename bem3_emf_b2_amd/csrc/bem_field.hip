@@ -686,18 +686,21 @@ __device__ __forceinline__ bool scat_far(const TriGeo &g, const double *rt)
   return g.quad == 0.0 && dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
 }
 
-// Mixed-precision form, first coupling pass: the far triangles only, FP32
-// with the far rule of the wave's far lanes, in a kernel of their own so
-// that none of the FP64 near-pair code's registers weigh on this loop.
-// Per point: sums of U (4 complex), dU/dv (4), F, dF, and a flag (any far
-// pair at the GHN switch level: the second pass then redoes the point).
+// First coupling pass: the far triangles only, in a kernel of their own so
+// that none of the near-pair and quadrangle code's registers weigh on this
+// loop.  F32 (mixed-precision form): FP32 with the far rule of the wave's
+// far lanes; otherwise the reference's GLN rule in FP64, exactly as the
+// second pass would sum it.  Per point: sums of U (4 complex), dU/dv (4),
+// F, dF, and a flag (a far pair at the GHN switch level: the second pass
+// then redoes the whole point, with the reference's switch).
 #define BEM_SCAT_PART 19
-__global__ void __launch_bounds__(256) k_scatter_far32(const double *__restrict__ pts, const double *__restrict__ nrm,
-                                                       int np, const TriGeo *__restrict__ geo,
-                                                       const int *__restrict__ sid, int dNe, double kr, double ki,
-                                                       double scale, const QPt32 *__restrict__ gl32,
-                                                       const cd *__restrict__ U, const cd *__restrict__ dU,
-                                                       double *__restrict__ part)
+template <bool F32>
+__global__ void __launch_bounds__(256) k_scatter_far(const double *__restrict__ pts, const double *__restrict__ nrm,
+                                                     int np, const TriGeo *__restrict__ geo,
+                                                     const int *__restrict__ sid, int dNe, double kr, double ki,
+                                                     double scale, const QPt32 *__restrict__ gl32,
+                                                     const QPtN *__restrict__ gl, int ngl, const cd *__restrict__ U,
+                                                     const cd *__restrict__ dU, double *__restrict__ part)
 {
   const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -711,16 +714,22 @@ __global__ void __launch_bounds__(256) k_scatter_far32(const double *__restrict_
     const int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
     const bool far = scat_far(g, rt);
-    const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
-    const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
-                             g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
-    int qoff, nq32;
-    far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
-    if (!far) continue;
     cd G[3], H[3], DG[3], DH[3];
     double f, df;
-    tri_coef32<true, true>(g, rt, v, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, DG, DH, df);
-    if (fabs(df) > BEM_CBD_SDF * (1.0 - 1e-4)) { redo = true; continue; }
+    if constexpr (F32) {
+      const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+      const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
+                               g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+      int qoff, nq32;
+      far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
+      if (!far) continue;
+      tri_coef32<true, true>(g, rt, v, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, DG, DH, df);
+      if (fabs(df) > BEM_CBD_SDF * (1.0 - 1e-4)) { redo = true; continue; }
+    } else {
+      if (!far) continue;
+      tri_coef<true>(g, rt, v, kr, ki, gl, ngl, scale, sd < 0, G, H, f, DG, DH, df);
+      if (fabs(df) > BEM_CBD_SDF) { redo = true; continue; }
+    }
     F += f; dF += df;
     const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
 #pragma unroll 1
@@ -766,7 +775,7 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
   cd aU[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, aD[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   double F = 0.0, dF = 0.0;
   const double vv[1][3] = {{v[0], v[1], v[2]}};
-  // part: the far triangles' sums of k_scatter_far32 for this point, unless
+  // part: the far triangles' sums of k_scatter_far for this point, unless
   // it met a far pair at the GHN switch level (then everything is done here)
   const bool use_part = part && part[(size_t)p * BEM_SCAT_PART + 18] == 0.0;
   for (int s = 1 + lane; s <= dNe; s += 64) {
@@ -1298,23 +1307,23 @@ void bem_launch_field_memp(const double *pts, const int *pdid, int np, int want_
                      (cd *)out);
   bem_prof_end(BH_K_FIELD, st);
 }
-// gl32 != nullptr: the mixed-precision form (FP32 far rules)
+// gl32 != nullptr: the mixed-precision form (FP32 far rules); part
+// (np x 19 doubles) != nullptr: far triangles in a first pass
 void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, int np, const void *geo, const void *gq,
                         const int *sid, int dNe, double kr, double ki, const void *gl, int ngl, const void *gh, int ngh,
                         double scale, const void *qgl, int nqgl, const void *qgh, int nqgh, const void *gl32,
                         const void *U, const void *dU, void *Ui, void *dUi, int quads, double *part, hipStream_t st)
 {
   if (np <= 0) return;
-  if (gl32 && part) {
-    // mixed form: the far triangles first (FP32), then the rest
-    bem_prof_begin(BH_K_SCATTER32, (double)np * dNe, st);
-    hipLaunchKernelGGL(k_scatter_far32, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, np, (const TriGeo *)geo, sid,
-                       dNe, kr, ki, scale, (const QPt32 *)gl32, (const cd *)U, (const cd *)dU, part);
-    bem_prof_end(BH_K_SCATTER32, st);
-  } else {
-    part = nullptr;
-  }
   const int kid = gl32 ? BH_K_SCATTER32 : BH_K_SCATTER;
+  if (part) {
+    // the far triangles first, then the rest
+    bem_prof_begin(kid, (double)np * dNe, st);
+    hipLaunchKernelGGL(gl32 ? k_scatter_far<true> : k_scatter_far<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm,
+                       np, (const TriGeo *)geo, sid, dNe, kr, ki, scale, (const QPt32 *)gl32, (const QPtN *)gl, ngl,
+                       (const cd *)U, (const cd *)dU, part);
+    bem_prof_end(kid, st);
+  }
   bem_prof_begin(kid, (double)np * dNe, st);
   hipLaunchKernelGGL(quads ? k_scatter<true> : k_scatter<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo,
                      (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale,
