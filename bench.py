@@ -159,6 +159,7 @@ def main():
                     help="only time the 16-object uneven scene (both forms, one configuration each)")
     ap.add_argument("--scene16-forms", default="fp64,mixed",
                     help="forms of the 16-object scene to run (the comparison needs both)")
+    ap.add_argument("--scene16-out", default=None, help="one form alone: also save its fields and forces (.npz)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -176,10 +177,15 @@ def main():
     import numpy as np
 
     import bem3_emf_b2_amd as bem
-    bem.init(0)
     if args.scene16:
-        scene16_main(bem, args.scene16_forms.split(","))
+        forms = args.scene16_forms.split(",")
+        if set(forms) == {"fp64", "mixed"}:
+            scene16_compare()      # each form in a fresh child process
+        else:
+            bem.init(0)
+            scene16_main(bem, forms, args.scene16_out)
         return
+    bem.init(0)
     arrays, beams = workload(args.level)
     rn, ed, md, sd, gd = arrays
     obj = bem.Object(rn, ed, md, sd, [beams[0][1], 1.59], beams[0][0])
@@ -315,7 +321,7 @@ def scene16_run(bem, objs, beams, pts, form):
             "FN": FN, "kernels_ms": {k: v[0] for k, v in kern.items() if v[2]}}
 
 
-def scene16_main(bem, forms=("fp64", "mixed")):
+def scene16_main(bem, forms, out_npz=None):
     """the batching configuration: 16 objects (seeded spheres, cubes and
     cones, 0.3-1.5 um, 500-5000 triangles, n 1.4-1.7) in a 10 um cube of
     water, x-polarised 1.064 um plane wave, coupling to a 1e-8 change; the
@@ -329,14 +335,40 @@ def scene16_main(bem, forms=("fp64", "mixed")):
     beams = mg.xpol_plane_wave(lambda0=1.064, n0=1.33, E0=1.0)
     pts = np.random.default_rng(20240602).uniform(-5.0, 5.0, size=(200, 3))
     log("scene16: %d objects, %d triangles" % (len(objs), S))
-    if set(forms) != {"fp64", "mixed"}:
-        # one form alone (for a profiler run): its timing, no comparison
-        r = scene16_run(bem, objs, beams, pts, forms[0])
-        print(json.dumps({"form": forms[0], "s": r["s"], "objects_s": r["objects_s"], "coupling_s": r["coupling_s"],
-                          "iterations": r["iterations"], "kernels_ms": r["kernels_ms"]}), flush=True)
-        return
-    r64 = scene16_run(bem, objs, beams, pts, "fp64")
-    rmx = scene16_run(bem, objs, beams, pts, "mixed")
+    # one form (a child of scene16_compare, or a profiler run): its timing
+    r = scene16_run(bem, objs, beams, pts, forms[0])
+    if out_npz:
+        np.savez(out_npz, fld=r["fld"], oid=r["oid"], F=np.array([f for f, n in r["FN"]]),
+                 N=np.array([n for f, n in r["FN"]]))
+    print(json.dumps({"form": forms[0], "s": r["s"], "objects_s": r["objects_s"], "coupling_s": r["coupling_s"],
+                      "iterations": r["iterations"], "kernels_ms": r["kernels_ms"], "triangles": S,
+                      "objects": len(objs)}), flush=True)
+
+
+def scene16_compare():
+    """both forms of the 16-object scene, each in a fresh child process (the
+    device memory the first form freed does not weigh on the second), then
+    the mixed form's errors against the FP64 form"""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for form in ("fp64", "mixed"):
+            npz = os.path.join(tmp, form + ".npz")
+            r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--scene16", "--scene16-forms", form,
+                                "--scene16-out", npz], stdout=subprocess.PIPE, text=True)
+            if r.returncode != 0:
+                log("scene16: the %s form failed (exit %d)" % (form, r.returncode))
+                sys.exit(r.returncode)
+            line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
+            d = json.loads(line)
+            z = np.load(npz)
+            d.update(fld=z["fld"], oid=z["oid"], FN=list(zip(z["F"], z["N"])))
+            res[form] = d
+    r64, rmx = res["fp64"], res["mixed"]
+    S = r64["triangles"]
     keep = (r64["oid"] < 0) & (rmx["oid"] < 0)
     fe = float(np.abs(rmx["fld"][keep] - r64["fld"][keep]).max() / np.abs(r64["fld"][keep]).max())
     Fe = max(float(np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()) for a, b in zip(rmx["FN"], r64["FN"]))
@@ -349,7 +381,7 @@ def scene16_main(bem, forms=("fp64", "mixed")):
         "config": {"workload": "16-object uneven scene (seeded spheres, cubes, cones 0.3-1.5 um, 500-5000 triangles, "
                                "n 1.4-1.7) in a 10 um cube of water, x-pol plane wave 1.064 um, coupling to 1e-8; "
                                "plain FP64 form: assembly + factor of every object + coupling loop",
-                   "objects": len(objs), "triangles": S, "precision": "fp64"},
+                   "objects": r64["objects"], "triangles": S, "precision": "fp64"},
         "fp64": {k: r64[k] for k in ("s", "objects_s", "coupling_s", "iterations", "kernels_ms")},
         "mixed_form": {"value": rmx["s"], "unit": "s/config", "speedup_vs_fp64": r64["s"] / rmx["s"],
                        **{k: rmx[k] for k in ("objects_s", "coupling_s", "iterations", "kernels_ms")},
