@@ -1786,19 +1786,18 @@ static int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check,
   return 0;
 }
 
-static double ccd_value(bh_object **objs, int N)
+// d_part: device scratch of 2 x (largest slot count) doubles
+static double ccd_value(bh_object **objs, int N, double *d_part)
 {
   double us = 0.0, ps = 0.0;
   long cnt = 0;
   for (int m = 0; m < N; m++) {
     bh_object *o = objs[m];
-    double *d_part;
-    if (hipMalloc((void **)&d_part, 2 * (size_t)o->nslot * sizeof(double)) != hipSuccess) return -1.0;
     hipLaunchKernelGGL(k_ccd, g1(o->nslot), dim3(256), 0, g.st, o->d_Ui, o->d_Uip, o->d_slotIdx, o->nslot, d_part);
     std::vector<double> part(2 * (size_t)o->nslot);
-    (void)hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st);
-    (void)hipStreamSynchronize(g.st);
-    dfree(d_part);
+    if (hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st) != hipSuccess ||
+        hipStreamSynchronize(g.st) != hipSuccess)
+      return -1.0;
     for (int p = 0; p < o->nslot; p++) { us += part[2 * p]; ps += part[2 * p + 1]; }
     cnt += o->nslot;
   }
@@ -1849,7 +1848,8 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     const bool progress = pe && *pe && *pe != '0';
     for (iter = 2; iter < iter_max; iter++) {
       if ((rc = sweep())) return done(rc);
-      double cc = ccd_value(objs, N);
+      double cc = ccd_value(objs, N, d_part);
+      if (cc < 0.0) return done(fail(BH_EHIP, "coupling loop: change measure"));
       if (trace) trace[iter] = cc;
       if (progress) { fprintf(stderr, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(stderr); }
       if (cc < ccd && iter > iter_min) break;
