@@ -264,6 +264,8 @@ struct bh_object {
   int *d_piv0 = nullptr, *d_pivM = nullptr, *d_pivS = nullptr;
   cd sch_delt, sch_beps, sch_sigm;
   bool s_formed = false;               // Sf holds LU(S) (else Sp = H1 + beta G1 Z, S solved by GMRES)
+  bool x_warm = false;                 // d_X solves the current factors' system for an earlier B
+                                       // (the refinement of the mixed form starts from it)
   int gm_steps = 0;                    // GMRES steps of the last S solve(s)
   cd *d_X = nullptr, *d_B = nullptr, *d_tmp = nullptr;
   cd *d_w = nullptr;                   // work of the block solve (schur_work(NN))
@@ -590,6 +592,7 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
   }
   // every factorisation of earlier matrices is stale from here on
   o->fmode = 0;
+  o->x_warm = false;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   for (int d = 0; d <= o->MN; d++) {
     int rc = assemble_domain(o, d);
@@ -1155,6 +1158,7 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
   if (mode == 0) mode = o->MN == 1 ? (o->prec == BH_PREC_MIXED ? 3 : 2) : 1;
   if (mode >= 2 && o->MN != 1) return fail(BH_EUNSUP, "the block elimination needs exactly one interior medium");
   o->fmode = 0;
+  o->x_warm = false;
   if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
   if (!o->d_X) { if (dalloc(&o->d_X, na) || dalloc(&o->d_B, na) || dalloc(&o->d_tmp, 8 * NN)) return BH_EHIP; }
   int info = 0;
@@ -1242,15 +1246,28 @@ extern "C" int bh_object_solve(bh_object *o)
     int rc = schur_apply(schur64(o), o->d_B, false, o->d_X, o->d_w, g.st, &o->gm_steps);
     if (rc) return fail(rc, "GMRES on the Schur complement did not converge");
   } else {
-    // complex-FP32 solves of the residual equation, residuals in FP64
+    // complex-FP32 solves of the residual equation, residuals in FP64; a
+    // repeated solve (the coupling loop) starts from the previous solution,
+    // whose residual for the new B is usually a small fraction of B
     const double nb = bem_nrm2(o->d_B, na, o->d_nrm, g.st);
     if (nb < 0.0) return fail(BH_EHIP, "norm of the right-hand side");
-    HIPCHK(hipMemsetAsync(o->d_X, 0, na * sizeof(cd), g.st));
-    HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    double res = nb > 0.0 ? 1.0 : 0.0;
+    if (o->x_warm && nb > 0.0) {
+      for (int d = 0; d <= o->MN; d++) {
+        Dom &D = o->dom[d];
+        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, g.st);
+      }
+      const double nr = bem_nrm2(o->d_R, na, o->d_nrm, g.st);
+      if (nr < 0.0) return fail(BH_EHIP, "norm of the residual");
+      res = nr / nb;
+    } else {
+      HIPCHK(hipMemsetAsync(o->d_X, 0, na * sizeof(cd), g.st));
+      HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+    }
+    o->x_warm = false;
     cf *cw = o->c_w, *crhs = cw + schur_work(NN), *cdx = crhs + 8 * NN;
     o->gm_steps = 0;
     const SchurF<cf> f = schur32(o);
-    double res = nb > 0.0 ? 1.0 : 0.0;
     int it = 0;
     while (res > IR_TOL && it < IR_MAX) {
       bem_cvt_z2c(o->d_R, crhs, na, g.st);
@@ -1271,6 +1288,7 @@ extern "C" int bh_object_solve(bh_object *o)
     o->ir_res = res;
     if (!(res <= IR_TOL))
       return fail(BH_ECONV, "mixed-precision refinement: relative residual %.3g after %d steps", res, it);
+    o->x_warm = true;
   }
   for (int d = 0; d <= o->MN; d++) {
     Dom &D = o->dom[d];
