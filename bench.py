@@ -398,6 +398,9 @@ PEAK_FP32_TFLOPS = 157.3    # FP32 vector and FP32 matrix (dense)
 # k_gemm128<double> at 15360^3: FETCH_SIZE 323,492,022 KiB x 2.0 + WRITE_SIZE 3,830,400 KiB x 1.0
 GEMM128_FP64_TRAFFIC = {"bytes": 323492022 * 1024 * 2.0 + 3830400 * 1024 * 1.0, "shape": "15360x15360x15360",
                         "algorithmic_bytes": 3 * 15360 ** 2 * 16}
+# k_gemm128<float> at 15360^3: FETCH_SIZE 42,520,650 KiB x 2.0 + WRITE_SIZE 1,843,200 KiB x 1.0
+GEMM128_FP32_TRAFFIC = {"bytes": 42520650 * 1024 * 2.0 + 1843200 * 1024 * 1.0, "shape": "15360x15360x15360",
+                        "algorithmic_bytes": 3 * 15360 ** 2 * 8}
 
 
 def roofline(dom, rec):
@@ -409,9 +412,9 @@ def roofline(dom, rec):
         ach = work / n / avg_s / 1e12
         # HBM bytes per launch from FETCH_SIZE / WRITE_SIZE counter runs of the
         # kernel alone at 15360^3 (the step's largest product), scaled by the
-        # copy-kernel calibration (profiles/gemm128_fp64_15360_counters.txt);
-        # measured for the FP64 kernel only
-        t = GEMM128_FP64_TRAFFIC if dom == "zgemm" else None
+        # copy-kernel calibration (profiles/gemm128_fp64_15360_counters.txt,
+        # profiles/gemm128_fp32_15360_counters.txt)
+        t = GEMM128_FP64_TRAFFIC if dom == "zgemm" else GEMM128_FP32_TRAFFIC
         return {"kernel": "k_gemm128<%s>" % ("double" if dom == "zgemm" else "float"), "bound": "mfma",
                 "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
                 "traffic": t["bytes"] if t else None, "traffic_shape": t["shape"] if t else None,
