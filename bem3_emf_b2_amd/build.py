@@ -22,18 +22,26 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-u
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
 
 
+def _compile(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
 def build(verbose=False):
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
+    objs, cmds = [], []
     for s in SOURCES:
         src = os.path.join(CSRC, s)
         if not os.path.exists(src):
             continue
         obj = os.path.join(CSRC, s.replace(".hip", ".o"))
-        cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        cmds.append([HIPCC] + FLAGS + ["-c", src, "-o", obj])
         objs.append(obj)
+    # the device sources compile independently (a few at a time: hipcc is memory hungry)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, min(8, os.cpu_count() or 1)))) as ex:
+        for f in [ex.submit(_compile, c, verbose) for c in cmds]:
+            f.result()
     for s in HOST_SOURCES:
         src = os.path.join(CSRC, s)
         obj = src[:-2] + ".o"

@@ -294,9 +294,10 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
 
 // ---- pass 1, mixed-precision form ----
 // Far pairs (row point farther than 4 element radii from the element's
-// centre) take the reduced far rule the wave's far lanes need: 5x5, 6x6 or
-// 12x12 squares of the 3-part map, by |k| x element radius, with a rule
-// error below 5e-9 there.  Nearer pairs take the reference's 12x12 rule.
+// centre) take the reduced far rule the wave's far lanes need (far_rule:
+// 4x4, 5x5, 6x6 or 12x12 squares of the 3-part map, by distance and |k| x
+// element radius), with a rule error below 3e-8 there.  Nearer pairs take
+// the reference's 12x12 rule.
 // The arithmetic stays FP64: FP32 would put 1e-7 relative noise on the
 // matrix entries, which the solved state carries on the total field, and
 // the force on a weakly scattering object (a few per cent index contrast),
@@ -334,8 +335,7 @@ __global__ void __launch_bounds__(256) k_asm_glr(const TriGeo *__restrict__ geo,
     const double Wdv0 = g.W[0] * rp.v0[0] + g.W[1] * rp.v0[1] + g.W[2] * rp.v0[2];
     const double Wdv1 = g.W[0] * rp.v1[0] + g.W[1] * rp.v1[1] + g.W[2] * rp.v1[2];
     const double dx = g.c0[0] - rp.rt[0], dy = g.c0[1] - rp.rt[1], dz = g.c0[2] - rp.rt[2];
-    const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
-                             g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+    const double rad2 = elem_rad2(g);
     const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
     int qoff, nqf;
     far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nqf);

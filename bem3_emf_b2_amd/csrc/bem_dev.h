@@ -161,16 +161,33 @@ struct QPt32 {
 // Far-pair rules of the mixed-precision form, one table (an FP64 copy for
 // its assembly, an FP32 copy for its boundary-field and coupling sums): the
 // 3-part triangle map with 12x12 (432 points), 6x6 (108), 5x5 (75) and 4x4
-// (48) Gauss-Legendre squares.  A pair farther than 4 element radii takes
-// 5x5 while |k| times the element radius is at most 1.5, 6x6 up to 3, else
-// 12x12; farther than 8 radii with |k| x radius at most 0.6 it takes 4x4.
-// Against a 20x20 rule the relative error of every G, H, dG/dv, dH/dv
-// group stays below 1e-8 there (real and lossy k).
+// (48) Gauss-Legendre squares.  The element radius is the largest distance
+// from the centre c0 to a vertex (elem_rad2).  A pair farther than 4 radii
+// takes 5x5 while |k| times the radius is at most 1.2, 6x6 up to 3, else
+// 12x12; farther than 8 radii with |k| x radius at most 0.5 it takes 4x4.
+// Against a 20x20 rule the relative error of every G, H, dG/dv, dH/dv group
+// stays below 3e-8 there, for general (also skewed) triangles and real and
+// lossy k (largest value seen over thousands of random pairs per tier: 2e-8).
 #define BEM_FAR_OFF6 432
 #define BEM_FAR_OFF5 540
 #define BEM_FAR_OFF4 615
 #define BEM_FAR_NPT 663
 #define BEM_FAR_RADII 4.0
+// squared element radius: the largest squared distance from the centre c0
+// of a triangle (the reference triangle's origin) to its vertices c0 + c1,
+// c0 - c1/2 -+ (sqrt3/2) c2
+__host__ __device__ __forceinline__ double elem_rad2(const TriGeo &g)
+{
+  double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double h = -0.5 * g.c1[d], q = 0.8660254037844386467637 * g.c2[d];
+    a += g.c1[d] * g.c1[d];
+    b += (h - q) * (h - q);
+    c += (h + q) * (h + q);
+  }
+  return fmax(a, fmax(b, c));
+}
 // Phase of an FP32 far-pair point.  e^{ikR} is split as e^{ik Rc} e^{ik dR}:
 // Rc, the distance from the row point to the element's centre, and its
 // factor are formed once per pair in FP64 (far32_centre); a point adds only
@@ -203,8 +220,8 @@ __device__ __forceinline__ cf far32_phase(cf ec, float dR, float kr, float ki)
 // squared distance to the element's centre over the squared radius)
 __device__ __forceinline__ void far_rule(bool far, double kR, double d2, int &off, int &nq)
 {
-  const bool need12 = far && kR > 3.0, need6 = far && kR > 1.5;
-  const bool need5 = far && (kR > 0.6 || d2 <= 64.0);
+  const bool need12 = far && kR > 3.0, need6 = far && kR > 1.2;
+  const bool need5 = far && (kR > 0.5 || d2 <= 64.0);
   if (__ballot(need12)) { off = 0; nq = 432; }
   else if (__ballot(need6)) { off = BEM_FAR_OFF6; nq = 108; }
   else if (__ballot(need5)) { off = BEM_FAR_OFF5; nq = 75; }

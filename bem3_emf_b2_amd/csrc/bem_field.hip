@@ -681,8 +681,7 @@ __global__ void __launch_bounds__(256) k_field_memp(const double *__restrict__ p
 __device__ __forceinline__ bool scat_far(const TriGeo &g, const double *rt)
 {
   const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
-  const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
-                           g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+  const double rad2 = elem_rad2(g);
   return g.quad == 0.0 && dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
 }
 
@@ -718,8 +717,7 @@ __global__ void __launch_bounds__(256) k_scatter_far(const double *__restrict__ 
     double f, df;
     if constexpr (F32) {
       const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
-      const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
-                               g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+      const double rad2 = elem_rad2(g);
       int qoff, nq32;
       far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
       if (!far) continue;
@@ -808,8 +806,7 @@ __global__ void __launch_bounds__(256) k_scatter(const double *__restrict__ pts,
       // wave's far lanes; a result within 1e-4 of the GHN switch level is
       // redone in FP64 below (far pairs stay orders of magnitude under it)
       const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
-      const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
-                               g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+      const double rad2 = elem_rad2(g);
       const bool far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
       int qoff, nq32;
       far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
@@ -911,8 +908,7 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
         // FP32 only farther than 4 element radii from the element's centre,
         // with the far rule of the wave's far lanes
         const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
-        const double rad2 = fmax(g.c1[0] * g.c1[0] + g.c1[1] * g.c1[1] + g.c1[2] * g.c1[2],
-                                 g.c2[0] * g.c2[0] + g.c2[1] * g.c2[1] + g.c2[2] * g.c2[2]);
+        const double rad2 = elem_rad2(g);
         far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
         int qoff, nq32;
         far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);

@@ -1,13 +1,16 @@
-"""The far-pair rules of the mixed-precision form (bem_dev.h far_rule32):
-pairs farther than 4 element radii from the element's centre take 5x5
-Gauss-Legendre squares of the 3-part triangle map while |k| x element
-radius <= 1.5, 6x6 up to 3, 12x12 beyond; pairs farther than 8 radii with
-|k| x radius <= 0.6 take 4x4.  Checked here: over random triangles,
-directions, normals and distances (4 to 32 radii; 8 to 64 for 4x4), the
-relative error of each group G_i, H_i, dG_i/dv, dH_i/dv (the integrands of
-lit_dcoef_v2.c:404-412) against a 20x20 rule stays below 5e-9 (1e-8 for
-4x4), for real and lossy wavenumbers, and the 12x12 rule of the FP64 form
-agrees with the 20x20 rule to round-off there."""
+"""The far-pair rules of the mixed-precision form (bem_dev.h far_rule): the
+element radius is the largest distance from the triangle's centre to a
+vertex; pairs farther than 4 radii take 5x5 Gauss-Legendre squares of the
+3-part triangle map while |k| x radius <= 1.2, 6x6 up to 3, 12x12 beyond;
+pairs farther than 8 radii with |k| x radius <= 0.5 take 4x4.  Checked
+here: over general triangles from random vertices (a third of them skewed
+slivers), random directions, normals and distances (4 to 32 radii; 8 to 64
+for 4x4), the relative error of each group G_i, H_i, dG_i/dv, dH_i/dv (the
+integrands of lit_dcoef_v2.c:404-412) against a 20x20 rule stays below
+3e-8 at each tier's largest |k| x radius (about 2e-8 is the largest value
+seen over thousands of pairs per tier), for real and lossy wavenumbers,
+and the 12x12 rule of the FP64 form agrees with the 20x20 rule to
+round-off there."""
 import numpy as np
 import pytest
 
@@ -51,21 +54,27 @@ NEAR = (4.0, 5.0, 6.0, 8.0, 12.0, 16.0, 32.0)   # element radii from the centre
 FAR = (8.0, 12.0, 16.0, 32.0, 64.0)
 
 
-@pytest.mark.parametrize("n,kR,ratios,bound", [(5, 0.2, NEAR, 5e-9), (5, 0.8, NEAR, 5e-9), (5, 1.5, NEAR, 5e-9),
-                                               (6, 3.0, NEAR, 5e-9), (12, 3.0, NEAR, 1e-13),
-                                               (4, 0.2, FAR, 1e-8), (4, 0.6, FAR, 1e-8)])
+def triangle(rng):
+    """c1, c2 of r = c0 + c1 z + c2 e for a random triangle (origin at its
+    centre c0) and its radius (largest centre-to-vertex distance)"""
+    V = rng.standard_normal((3, 3))
+    if rng.uniform() < 1.0 / 3.0:   # sliver: third vertex near the first edge
+        V[2] = V[0] + (V[1] - V[0]) * rng.uniform(0.1, 0.9) + rng.standard_normal(3) * rng.uniform(0.02, 0.2)
+    c0 = V.mean(0)
+    c1 = V[1] - c0
+    c2 = (V[2] - V[0]) / S3
+    rad = max(np.linalg.norm(V[i] - c0) for i in range(3))
+    return c1, c2, rad
+
+
+@pytest.mark.parametrize("n,kR,ratios,bound", [(5, 0.2, NEAR, 3e-8), (5, 1.2, NEAR, 3e-8), (6, 3.0, NEAR, 3e-8),
+                                               (12, 3.0, NEAR, 1e-13), (4, 0.2, FAR, 3e-8), (4, 0.5, FAR, 3e-8)])
 def test_far_rule_error(n, kR, ratios, bound):
     rng = np.random.default_rng(int(100 * kR) + n)
     worst = 0.0
     for ratio in ratios:
-        for _ in range(40):
-            c1 = rng.standard_normal(3)
-            c1 /= np.linalg.norm(c1)
-            c2 = rng.standard_normal(3)
-            c2 -= (c2 @ c1) * c1
-            c2 /= np.linalg.norm(c2)
-            c1, c2 = c1 * rng.uniform(0.6, 1.0), c2 * rng.uniform(0.6, 1.0)
-            rad = max(np.linalg.norm(c1), np.linalg.norm(c2))
+        for _ in range(120):
+            c1, c2, rad = triangle(rng)
             W = np.cross(c1, c2)
             u = rng.standard_normal(3)
             x = u / np.linalg.norm(u) * ratio * rad * 1.0001
@@ -77,3 +86,14 @@ def test_far_rule_error(n, kR, ratios, bound):
             for g, r in zip(got, ref):
                 worst = max(worst, np.abs(g - r).max() / np.abs(r).max())
     assert worst < bound, worst
+
+
+def test_element_radius_is_vertex_distance():
+    """c0 + c1, c0 - c1/2 -+ sqrt3/2 c2 are the vertices (the map of the
+    reference triangle's corners (1, 0), (-1/2, -+sqrt3/2)); the radius the
+    rules use is the largest of their distances from c0"""
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        c1, c2, rad = triangle(rng)
+        verts = [c1, -0.5 * c1 - 0.5 * S3 * c2, -0.5 * c1 + 0.5 * S3 * c2]
+        assert abs(max(np.linalg.norm(v) for v in verts) - rad) < 1e-12 * rad
