@@ -6,8 +6,10 @@ benchmarks and Python users; the C host library (drop-in for the reference's
 bem3_emf_b2.h) links the same shared object.  There is no CPU fallback: if
 the library or the GPU is missing every call raises.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -64,6 +66,9 @@ def lib():
             "bh_object_set_precision": (C.c_int, [VP, C.c_int]),
             "bh_cgesv": (C.c_int, [C.c_int, C.c_int, Fp, Fp]),
             "bh_cgemm_host": (C.c_int, [C.c_int, C.c_int, C.c_int, Fp, Fp, Fp, Fp, Fp]),
+            "bh_finalize": (C.c_int, []),
+            "bh_mo_set_coupling": (C.c_int, [C.c_int]),
+            "bh_mo_plan_stats": (C.c_int, [Ip, Ip, Ip]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -95,10 +100,42 @@ def sync():
     check(lib().bh_sync())
 
 
+# live objects: closed, and the library's device state released, before the
+# interpreter shuts down (no device frees from finalizers at exit)
+_live = weakref.WeakSet()
+
+
+def finalize():
+    """destroy every live Object, then release the library's device state"""
+    for o in list(_live):
+        o.close()
+    if _lib is not None:
+        check(_lib.bh_finalize())
+
+
+atexit.register(finalize)
+
+
+def set_coupling(mode):
+    """how the coupling loop sums the field of one object at another's
+    slots: "auto" (a multipole series where the mixed form allows it) or
+    "direct" (the element sums everywhere)"""
+    check(lib().bh_mo_set_coupling({"auto": 0, "direct": 1}[mode]))
+
+
+def plan_stats():
+    """(object pairs by series, by element sums, largest series order) of
+    the last coupling loop"""
+    a = np.zeros(3, np.int32)
+    check(lib().bh_mo_plan_stats(ip(a[0:1]), ip(a[1:2]), ip(a[2:3])))
+    return int(a[0]), int(a[1]), int(a[2])
+
+
 # kernel ids of the timing registry (bem3_hip.h)
 KERNELS = {"asm_gl": 0, "asm_gh": 1, "asm_self": 2, "zgemm": 3, "lu_panel": 4, "trsm": 5, "field": 6, "scatter": 7,
            "zgemm_small": 8, "field_bd": 9, "cgemm": 10, "cgemm_small": 11, "lu_panel_c": 12, "trsm_c": 13,
-           "asm_glr": 14, "field_bd32": 15, "zgemv": 16, "cgemv": 17, "scatter32": 18}
+           "asm_glr": 14, "field_bd32": 15, "zgemv": 16, "cgemv": 17, "scatter32": 18, "scatter_far": 19,
+           "scatter_far32": 20, "mp_basis": 21, "mp_moments": 22, "mp_eval": 23}
 # factorisation forms of bh_object_factor
 FACTOR_MODES = {"auto": 0, "dense": 1, "block": 2, "mixed": 3}
 
@@ -133,6 +170,7 @@ class Object:
         self.Nn, self.Ne, self.NN, self.MN = (int(x) for x in info[:4])
         self.dNe = [int(x) for x in info[4:4 + self.MN + 1]]
         self.nd = [int(x) for x in info[12:12 + self.MN + 1]]
+        _live.add(self)
 
     def assemble(self, type_=2):
         check(lib().bh_object_assemble(self.h, type_))

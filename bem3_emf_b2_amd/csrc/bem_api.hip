@@ -5,23 +5,9 @@
 // incident and boundary values.  Host code only moves inputs in and results
 // out.  Every entry point returns 0 on success and a negative code on error
 // (the message is kept for bh_last_error()).
-#include "bem_dev.h"
-#include "bem_schur.h"
-#include "../../include/bem3_hip.h"
+#include "bem_obj.h"
 
-#include <algorithm>
-#include <cmath>
-#include <complex>
-#include <cstdarg>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <mutex>
-#include <string>
-#include <vector>
-
-typedef std::complex<double> zd;
-
+using namespace bem;
 
 extern "C" {
 void bem_launch_asm_gl(const void *, const void *, const int *, const void *, const int *, int, int, double, double,
@@ -45,15 +31,21 @@ int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
 void bem_zgetrs(int, int, const void *, size_t, const int *, void *, size_t, int, hipStream_t);
 int bem_la_init(void);
 int bem_trsv_error(hipStream_t);
+void bem_la_release(void);
+void bem_prof_release(void);
+void bem_mo_release(void);
 void bem_cgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
                hipStream_t);
 int bem_cgetrf(int, void *, size_t, int *, int *, hipStream_t);
 void bem_cgetrs(int, int, const void *, size_t, const int *, void *, size_t, hipStream_t);
 }
 
-// ------------------------------------------------------------------ errors
+static_assert(sizeof(cd) == 16, "complex layout");
+
+// ------------------------------------------------------------------ errors, context
 static thread_local std::string g_err;
-static int fail(int code, const char *fmt, ...)
+namespace bem {
+int fail(int code, const char *fmt, ...)
 {
   char buf[512];
   va_list ap;
@@ -63,41 +55,19 @@ static int fail(int code, const char *fmt, ...)
   g_err = buf;
   return code;
 }
-static void dfree(void *p) { (void)hipFree(p); }
-static int lu_failed(void) { return fail(BH_EHIP, "LU factorisation: scratch allocation failed or a panel barrier timed out"); }
+std::recursive_mutex g_mu;
+Ctx g;
 // a one-launch triangular solve whose wait timed out since the last check
-static int trsv_failed(hipStream_t st)
+// (reads and clears the flag)
+int trsv_failed(hipStream_t st)
 {
   int e = bem_trsv_error(st);
   return e ? fail(BH_EHIP, "triangular solve: a block wait timed out") : 0;
 }
-
-// Entry points are serialised: callers may call the scalar field functions
-// from their own thread pools (e.g. OpenMP loops over grid points); all work
-// goes to one device stream in call order.
-static std::recursive_mutex g_mu;
-#define API_LOCK std::lock_guard<std::recursive_mutex> api_lock_(g_mu)
-#define HIPCHK(x)                                                                  \
-  do {                                                                             \
-    hipError_t e_ = (x);                                                           \
-    if (e_ != hipSuccess) return fail(BH_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
-  } while (0)
+}  // namespace bem
+static int lu_failed(void) { return fail(BH_EHIP, "LU factorisation: scratch allocation failed or a panel barrier timed out"); }
 
 extern "C" const char *bh_last_error(void) { return g_err.c_str(); }
-
-// ------------------------------------------------------------------ context
-struct Ctx {
-  int dev = -1;
-  hipStream_t st = nullptr;
-  QPt *d_gl = nullptr, *d_gh = nullptr;   // 3-part triangle rules
-  int n_gl = 0, n_gh = 0;
-  void *d_gl32 = nullptr;                 // FP32 far-pair rules, 12x12 / 6x6 / 5x5 / 4x4 (QPt32)
-  QPt *d_glr = nullptr;                   // the same rules in FP64 (QPt): far pairs of the mixed-form assembly
-  double *d_xl = nullptr, *d_wl = nullptr, *d_xh = nullptr, *d_wh = nullptr;
-  double xl[12], wl[12], xh[64], wh[64];
-  hipEvent_t ev[2];
-};
-static Ctx g;
 
 // Gauss-Legendre nodes on (a, b) by Newton iteration on P_n (same fixed
 // point as the reference's gauleg.c:3-36: start cos(pi(i-1/4)/(n+1/2)),
@@ -143,12 +113,6 @@ static std::vector<QPt> tri_rule(int n, const double *x, const double *w)
   return r;
 }
 
-template <class T> static int upload(T **d, const T *h, size_t n)
-{
-  HIPCHK(hipMalloc((void **)d, std::max<size_t>(n, 1) * sizeof(T)));
-  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, g.st));
-  return 0;
-}
 
 static int ensure_init()
 {
@@ -206,78 +170,6 @@ extern "C" int bh_sync(void)
 extern "C" void *bh_stream(void) { return (void *)g.st; }
 
 // ------------------------------------------------------------------ objects
-struct Dom {
-  int dNe = 0, nd = 0;                 // elements seen, compact node count
-  std::vector<int> sid, colOff;        // [dNe+1]
-  std::vector<RowPt> rows;             // [nd]
-  int *d_sid = nullptr, *d_colOff = nullptr;
-  RowPt *d_rows = nullptr;
-  cd *d_G = nullptr, *d_H = nullptr, *d_dG = nullptr, *d_dH = nullptr;
-  double *d_dF = nullptr;
-  int n_gh = 0;
-  // solve-side tables and boundary values ([dNe+1][4][4] U/dU, [dNe+1][4][3] E/H)
-  ColInfo *d_col = nullptr;
-  int *d_colS = nullptr;
-  RowEH *d_rowEH = nullptr;
-  cd *d_U = nullptr, *d_dU = nullptr, *d_E = nullptr, *d_dE = nullptr, *d_Hf = nullptr, *d_dHf = nullptr;
-  cd *d_Gf = nullptr;                  // LU factors of G (normal derivatives of E, H)
-  int *d_Gpiv = nullptr;
-  bool gf_valid = false;               // d_Gf factors the current G (cleared by assembly)
-  cf *c_Gf = nullptr;                  // complex-FP32 LU factors of G (mixed-precision form)
-  bool gfc_valid = false;
-  // face table for the solid-angle domain test (faces whose main side is d)
-};
-
-struct bh_object {
-  int Nn, Ne, MN, NN;
-  int nquad = 0;                       // quadrangle elements
-  std::vector<double> rn;              // [Nn+1][3]
-  std::vector<int> ed, md, sd, eni;    // [Ne+1][4], [Ne+1]
-  std::vector<double> cr, cw, ren, wen;// [Ne+1][3][4], [Ne+1][3][3], [Ne+1][4][3] x2
-  std::vector<zd> n, kn;
-  std::vector<Dom> dom;
-  TriGeo *d_geo = nullptr;
-  QuadExt *d_gq = nullptr;             // bi-linear terms of quadrangles (zero rows for triangles)
-  bool all_tri = true;
-  // incident data at every element slot [Ne+1][4][4]
-  cd *d_Ui = nullptr, *d_dUi = nullptr, *d_Ui0 = nullptr, *d_dUi0 = nullptr, *d_Uip = nullptr;
-  // open-domain node slots (targets of the incident and coupling fields)
-  int nslot = 0;
-  double *d_slotPt = nullptr, *d_slotN = nullptr;
-  int *d_slotIdx = nullptr;
-  // faces grouped by main domain for the domain test
-  double *d_fv = nullptr;
-  int *d_fsd = nullptr, *d_fq = nullptr, *d_seg = nullptr;
-  // beams
-  std::vector<Beam> beams;             // plane waves
-  std::vector<FBeam> fbeams;           // the other multi_fbeam types
-  FBeam *d_fb = nullptr;
-  double *d_fbt = nullptr;             // their spectral angles (cos, sin, weight x sin)
-  Beam *d_beams = nullptr;
-  double lambda0 = 0.0;
-  int prec = BH_PREC_FP64;             // arithmetic of assembly, field sums and factor mode 0
-  // factorisation: 0 none, 1 dense 8NN, 2 Schur (one medium), 3 Schur in FP32 refined in FP64
-  int fmode = 0;                       // factorisation of the current matrices (cleared by assembly)
-  cd *d_A = nullptr, *d_work = nullptr;
-  int *d_piv = nullptr, *d_info = nullptr;
-  cd *d_G0f = nullptr, *d_Z = nullptr, *d_Mf = nullptr, *d_Sf = nullptr;
-  int *d_piv0 = nullptr, *d_pivM = nullptr, *d_pivS = nullptr;
-  cd sch_delt, sch_beps, sch_sigm;
-  bool s_formed = false;               // Sf holds LU(S) (else Sp = H1 + beta G1 Z, S solved by GMRES)
-  bool x_warm = false;                 // d_X solves the current factors' system for an earlier B
-                                       // (the refinement of the mixed form starts from it)
-  int gm_steps = 0;                    // GMRES steps of the last S solve(s)
-  cd *d_X = nullptr, *d_B = nullptr, *d_tmp = nullptr;
-  cd *d_w = nullptr;                   // work of the block solve (schur_work(NN))
-  // mixed-precision form (fmode 3): complex-FP32 factors of the block
-  // elimination, refined against the FP64 matrices
-  cf *c_G0f = nullptr, *c_Z = nullptr, *c_Mf = nullptr, *c_Sf = nullptr, *c_G1 = nullptr;
-  cf *c_w = nullptr;                   // schur_work(NN) work, 8 NN right-hand side, 8 NN correction
-  cd *d_R = nullptr;                   // 8 NN residual
-  double *d_nrm = nullptr;             // 256 partial sums of the norm
-  int ir_iters = 0;                    // refinement steps of the last solve
-  double ir_res = 0.0;                 // its final relative residual
-};
 
 // element constants and node data (reference setup2.c:666-749, 838-869):
 // triangles r = c0 + c1 z + c2 e on the equilateral reference triangle,
@@ -511,12 +403,13 @@ extern "C" int bh_object_geometry(const bh_object *o, double *cr, double *cw, do
 }
 
 // rule tables (QPtN on the triangle, QPtQ on the square), built at first use
-static QPtN *g_rule[4] = {nullptr, nullptr, nullptr, nullptr};
-static int g_nrule[4] = {0, 0, 0, 0};
-static double g_rscale[4];
-static QPtQ *g_qrule[4] = {nullptr, nullptr, nullptr, nullptr};   // the same rules on the square (quads)
-static int g_nqrule[4] = {0, 0, 0, 0};
-static int ensure_rules();
+namespace bem {
+QPtN *g_rule[4] = {nullptr, nullptr, nullptr, nullptr};
+int g_nrule[4] = {0, 0, 0, 0};
+double g_rscale[4];
+QPtQ *g_qrule[4] = {nullptr, nullptr, nullptr, nullptr};   // the same rules on the square (quads)
+int g_nqrule[4] = {0, 0, 0, 0};
+}  // namespace bem
 
 // ------------------------------------------------------------------ assembly
 static int assemble_domain(bh_object *o, int d)
@@ -531,10 +424,10 @@ static int assemble_domain(bh_object *o, int d)
     HIPCHK(hipMalloc((void **)&D.d_dF, 3 * nd * sizeof(double)));
   }
   const int nblk = (D.dNe + 255) / 256, nwv = nblk * 4;
-  double *d_part = nullptr;
-  unsigned long long *d_mask = nullptr;
-  HIPCHK(hipMalloc((void **)&d_part, nd * nwv * 3 * sizeof(double)));
-  HIPCHK(hipMalloc((void **)&d_mask, nd * nwv * sizeof(unsigned long long)));
+  // work areas: grow-only, owned by the object
+  double *d_part = o->ws_part.get<double>(nd * nwv * 3);
+  unsigned long long *d_mask = o->ws_mask.get<unsigned long long>(nd * nwv);
+  if (!d_part || !d_mask) return fail(BH_EHIP, "assembly: work area of %zu rows", nd);
   double kr = o->kn[d].real(), ki = o->kn[d].imag();
   const bool mixed = o->prec == BH_PREC_MIXED;
   if (mixed)
@@ -563,13 +456,13 @@ static int assemble_domain(bh_object *o, int d)
   }
   start[nd] = (int)pairs.size();
   D.n_gh = (int)pairs.size();
-  GhPair *d_pairs = nullptr;
-  int *d_start = nullptr;
-  double *d_ghdf = nullptr;
-  if (upload(&d_pairs, pairs.data(), pairs.size()) || upload(&d_start, start.data(), start.size())) return BH_EHIP;
-  HIPCHK(hipMalloc((void **)&d_ghdf, std::max<size_t>(pairs.size(), 1) * 3 * sizeof(double)));
-  double *d_self = nullptr;   // the self element's F terms per row
-  HIPCHK(hipMalloc((void **)&d_self, nd * 3 * sizeof(double)));
+  GhPair *d_pairs = o->ws_pairs.get<GhPair>(std::max<size_t>(pairs.size(), 1));
+  int *d_start = o->ws_start.get<int>(start.size());
+  double *d_ghdf = o->ws_ghdf.get<double>(std::max<size_t>(pairs.size(), 1) * 3);
+  double *d_self = o->ws_self.get<double>(nd * 3);   // the self element's F terms per row
+  if (!d_pairs || !d_start || !d_ghdf || !d_self) return fail(BH_EHIP, "assembly: work area of the near pairs");
+  if (!pairs.empty()) HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(GhPair), hipMemcpyHostToDevice, g.st));
+  HIPCHK(hipMemcpyAsync(d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, g.st));
   bem_launch_asm_gh(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh,
                     mixed ? (const void *)g.d_gl : nullptr, g.n_gl, g_qrule[3], g_nqrule[3], g_qrule[2], g_nqrule[2],
                     D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_ghdf, g.st);
@@ -577,8 +470,7 @@ static int assemble_domain(bh_object *o, int d)
                       D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_self, g.st);
   bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, d_self, (int)nd, d == 0, D.d_H, nd, D.d_dF, g.st);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(g.st));
-  dfree(d_part); dfree(d_mask); dfree(d_pairs); dfree(d_start); dfree(d_ghdf); dfree(d_self);
+  HIPCHK(hipStreamSynchronize(g.st));   // the host tables above stay alive until here
   return 0;
 }
 
@@ -594,10 +486,12 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
   o->fmode = 0;
   o->x_warm = false;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
+  o->asm_prec = -1;
   for (int d = 0; d <= o->MN; d++) {
     int rc = assemble_domain(o, d);
     if (rc) return rc;
   }
+  o->asm_prec = o->prec;
   return 0;
 }
 
@@ -605,7 +499,14 @@ extern "C" int bh_object_set_precision(bh_object *o, int prec)
 {
   API_LOCK;
   if (prec != BH_PREC_FP64 && prec != BH_PREC_MIXED) return fail(BH_EARG, "precision %d", prec);
+  if (prec == o->prec) return 0;
+  // matrices, factors and warm starts of the other precision are stale:
+  // the object has to be assembled (and factorised) again
   o->prec = prec;
+  o->fmode = 0;
+  o->x_warm = false;
+  o->s_formed = false;
+  for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   return 0;
 }
 
@@ -757,7 +658,7 @@ void bem_launch_poynting(const void *, const double *, const double *, int, doub
 }
 
 
-static int ensure_rules()
+int bem::ensure_rules()
 {
   if (g_rule[0]) return 0;
   const double S3 = BEM_SQ3;
@@ -831,7 +732,7 @@ template <class T> static int dalloc(T **p, size_t n)
 }
 
 // per-domain column tables, state buffers, slot and face tables
-static int prepare_solve(bh_object *o)
+int bem::prepare_solve(bh_object *o)
 {
   if (o->d_Ui) return 0;
   size_t E1 = (size_t)o->Ne + 1;
@@ -1120,23 +1021,21 @@ static SchurF<cf> schur32(const bh_object *o)
 
 // S = Sp - sigma delta G1 (M^{-1} G1 o n n^T) and its LU, for objects
 // solved many times (the coupling loop); a single solve uses GMRES instead
-static int form_s(bh_object *o)
+int bem::form_s(bh_object *o)
 {
   if (o->s_formed || (o->fmode != 2 && o->fmode != 3)) return 0;
   size_t n2 = (size_t)o->NN * o->NN;
   int rc;
   if (o->fmode == 2) {
-    cd *Wt = nullptr;
-    HIPCHK(hipMalloc((void **)&Wt, n2 * sizeof(cd)));
+    cd *Wt = o->ws_wt.get<cd>(n2);
+    if (!Wt) return fail(BH_EHIP, "Schur complement: work area of %zu entries", n2);
     rc = schur_form_s(schur64(o), o->d_Sf, o->d_pivS, Wt, o->d_info, g.st);
     (void)hipStreamSynchronize(g.st);
-    dfree(Wt);
   } else {
-    cf *Wt = nullptr;
-    HIPCHK(hipMalloc((void **)&Wt, n2 * sizeof(cf)));
+    cf *Wt = o->ws_wt.get<cf>(n2);
+    if (!Wt) return fail(BH_EHIP, "Schur complement: work area of %zu entries", n2);
     rc = schur_form_s(schur32(o), o->c_Sf, o->d_pivS, Wt, o->d_info, g.st);
     (void)hipStreamSynchronize(g.st);
-    dfree(Wt);
   }
   if (rc) return lu_failed();
   int info = 0;
@@ -1151,7 +1050,8 @@ static int form_s(bh_object *o)
 extern "C" int bh_object_factor(bh_object *o, int mode)
 {
   API_LOCK;
-  if (!o->dom[0].d_G) return fail(BH_ESTATE, "assemble before factor");
+  if (!o->dom[0].d_G || o->asm_prec < 0) return fail(BH_ESTATE, "assemble before factor");
+  if (o->asm_prec != o->prec) return fail(BH_ESTATE, "the precision changed since the assembly: assemble again");
   if (mode < 0 || mode > 3) return fail(BH_EARG, "factor mode %d", mode);
   if (prepare_solve(o)) return BH_EHIP;
   size_t NN = (size_t)o->NN, na = 8 * NN;
@@ -1217,7 +1117,6 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
   return 0;
 }
 
-static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 // iterative refinement of the mixed-precision form: relative residual
 // target ||b - A x|| / ||b|| and step limit
@@ -1225,9 +1124,8 @@ static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 #define IR_MAX 30
 
 // B from the incident data, X = A^{-1} B, boundary values (solve_coefficient_mEMP)
-extern "C" int bh_object_solve(bh_object *o)
+static int object_solve(bh_object *o)
 {
-  API_LOCK;
   if (!o->fmode) return fail(BH_ESTATE, "factor before solve");
   if (!o->d_beams) return fail(BH_ESTATE, "set beams before solve");
   size_t NN = (size_t)o->NN, na = 8 * NN;
@@ -1296,7 +1194,17 @@ extern "C" int bh_object_solve(bh_object *o)
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g.st));
-  return trsv_failed(g.st);
+  return 0;
+}
+
+extern "C" int bh_object_solve(bh_object *o)
+{
+  API_LOCK;
+  const int rc = object_solve(o);
+  // the one-launch triangular solves' timeout flag is read and cleared on
+  // every exit: a timed-out wait is the cause of whatever failed after it
+  const int tr = trsv_failed(g.st);
+  return tr ? tr : rc;
 }
 
 extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *resid, int *gmres_steps)
@@ -1313,11 +1221,9 @@ extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *res
 static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *piv, cd *R)
 {
   const size_t nd = (size_t)D.nd, m = 6 * nd;
-  cd *X = nullptr, *Rr = nullptr;
-  cf *C = nullptr;
-  HIPCHK(hipMalloc((void **)&X, m * sizeof(cd)));
-  HIPCHK(hipMalloc((void **)&Rr, m * sizeof(cd)));
-  HIPCHK(hipMalloc((void **)&C, m * sizeof(cf)));
+  cd *X = o->ws_eh[2].get<cd>(m), *Rr = o->ws_eh[3].get<cd>(m);
+  cf *C = o->ws_eh[4].get<cf>(m);
+  if (!X || !Rr || !C) return fail(BH_EHIP, "refinement of the boundary fields: work area");
   HIPCHK(hipMemsetAsync(X, 0, m * sizeof(cd), g.st));
   HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
   const double nb = bem_nrm2(R, m, o->d_nrm, g.st);
@@ -1337,9 +1243,6 @@ static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *pi
   if (!rc && !(res <= IR_TOL)) rc = fail(BH_ECONV, "mixed-precision refinement (G): relative residual %.3g after %d steps", res, it);
   if (!rc) HIPCHK(hipMemcpyAsync(R, X, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
   HIPCHK(hipStreamSynchronize(g.st));
-  dfree(X);
-  dfree(Rr);
-  dfree(C);
   return rc;
 }
 
@@ -1390,9 +1293,8 @@ extern "C" int bh_object_solve_EH(bh_object *o)
       Gf = D.d_Gf;
       Gpiv = D.d_Gpiv;
     }
-    cd *P = nullptr, *R = nullptr;
-    HIPCHK(hipMalloc((void **)&P, nd * 6 * sizeof(cd)));
-    HIPCHK(hipMalloc((void **)&R, nd * 6 * sizeof(cd)));
+    cd *P = o->ws_eh[0].get<cd>(nd * 6), *R = o->ws_eh[1].get<cd>(nd * 6);
+    if (!P || !R) return fail(BH_EHIP, "boundary fields: work area");
     bem_launch_pack_eh(D.nd, D.d_col, D.d_colS, D.d_E, D.d_Hf, P, g.st);
     bem_zgemm(D.nd, 6, D.nd, 1.0, 0.0, D.d_H, nd, P, 6, 0.0, 0.0, R, 6, g.st);
     int rc = 0;
@@ -1402,8 +1304,6 @@ extern "C" int bh_object_solve_EH(bh_object *o)
       bem_zgetrs(D.nd, 6, Gf, nd, Gpiv, R, 6, 64, g.st);
     if (!rc) bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, g.st);
     HIPCHK(hipStreamSynchronize(g.st));
-    dfree(P);
-    dfree(R);
     if (rc) return rc;
   }
   HIPCHK(hipGetLastError());
@@ -1442,25 +1342,35 @@ extern "C" int bh_object_incident(bh_object *o, int which, double *buf, int set)
 }
 
 // ------------------------------------------------------------------ multi-object layer
-struct Scratch {
-  void *p = nullptr;
-  size_t n = 0;
-  ~Scratch() { dfree(p); }
-  template <class T> T *get(size_t cnt)
-  {
-    size_t b = cnt * sizeof(T);
-    if (b > n) { dfree(p); p = nullptr; if (hipMalloc(&p, b) != hipSuccess) { n = 0; return nullptr; } n = b; }
-    return (T *)p;
-  }
-};
 
-static int domain_ids(bh_object *o, const double *d_pts, int np, int *d_did)
+int bem::domain_ids(bh_object *o, const double *d_pts, int np, int *d_did)
 {
   if (prepare_solve(o)) return BH_EHIP;
   bem_launch_domain(d_pts, np, o->d_fv, o->d_fsd, o->d_fq, o->d_seg, o->MN, d_did, g.st);
   HIPCHK(hipGetLastError());
   return 0;
 }
+
+// grow-only device work areas of the field entry points (calls are
+// serialised; bh_finalize releases them).  Slots 0-15: bh_mo_field,
+// bh_mo_EH_bd, bh_object_domain and the surface points of the force and
+// power use separate slots where they nest.
+#define BEM_NCWS 20
+static Scratch *g_cws = nullptr;
+template <class T> static T *cws(int i, size_t n)
+{
+  if (!g_cws) g_cws = new Scratch[BEM_NCWS];
+  return g_cws[i].get<T>(std::max<size_t>(n, 1));
+}
+// a grow-only work area holding n host values (copied on the library's stream)
+template <class T> static int cput(int i, T **d, const T *h, size_t n)
+{
+  *d = cws<T>(i, n);
+  if (!*d) return fail(BH_EHIP, "device work area of %zu bytes", n * sizeof(T));
+  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, g.st));
+  return 0;
+}
+#define CWS_CHECK(p) do { if (!(p)) return fail(BH_EHIP, "device work area"); } while (0)
 
 // fields at points (mo_EH_s/t/i, mo_mEMP_s/t/i; mo_field.c:11-282):
 // what 0 E,H (6 complex per point), 1 U (4); mode 0 s, 1 t, 2 i;
@@ -1480,12 +1390,12 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
   double *d_pts;
   cd *d_out;
   int *d_did, *d_sel, *d_mask;
-  HIPCHK(hipMalloc((void **)&d_pts, 3 * (size_t)np * sizeof(double)));
-  HIPCHK(hipMalloc((void **)&d_out, (size_t)np * nc * sizeof(cd)));
-  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * N * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&d_sel, (size_t)np * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&d_mask, (size_t)np * sizeof(int)));
-  HIPCHK(hipMemcpyAsync(d_pts, pts, 3 * (size_t)np * sizeof(double), hipMemcpyHostToDevice, g.st));
+  if (cput(0, &d_pts, pts, 3 * (size_t)np)) return BH_EHIP;
+  d_out = cws<cd>(1, (size_t)np * nc);
+  d_did = cws<int>(2, (size_t)np * N);
+  d_sel = cws<int>(3, np);
+  d_mask = cws<int>(4, np);
+  CWS_CHECK(d_out && d_did && d_sel && d_mask);
   HIPCHK(hipMemsetAsync(d_out, 0, (size_t)np * nc * sizeof(cd), g.st));
   for (int m = 0; m < N; m++)
     if (domain_ids(objs[m], d_pts, np, d_did + (size_t)m * np)) return BH_EHIP;
@@ -1532,7 +1442,6 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
   HIPCHK(hipMemcpyAsync(out, d_out, (size_t)np * nc * sizeof(cd), hipMemcpyDeviceToHost, g.st));
   HIPCHK(hipStreamSynchronize(g.st));
   if (oid) memcpy(oid, pin.data(), np * sizeof(int));
-  dfree(d_pts); dfree(d_out); dfree(d_did); dfree(d_sel); dfree(d_mask);
   return 0;
 }
 
@@ -1626,16 +1535,17 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
   int *d_t, *d_st, *d_did, *d_sel, *d_mask;
   double *d_ze, *d_r, *d_w, *d_wt, *d_rt;
   cd *d_EH, *d_tmp;
-  if (upload(&d_t, sp.t.data(), np) || upload(&d_st, sp.stored.data(), np) || upload(&d_ze, sp.ze.data(), 2 * (size_t)np) ||
-      upload(&d_r, sp.r.data(), 3 * (size_t)np) || upload(&d_w, sp.w.data(), 3 * (size_t)np) ||
-      upload(&d_wt, sp.wt.data(), np))
+  if (cput(0, &d_t, sp.t.data(), np) || cput(1, &d_st, sp.stored.data(), np) || cput(2, &d_ze, sp.ze.data(), 2 * (size_t)np) ||
+      cput(3, &d_r, sp.r.data(), 3 * (size_t)np) || cput(4, &d_w, sp.w.data(), 3 * (size_t)np) ||
+      cput(5, &d_wt, sp.wt.data(), np))
     return BH_EHIP;
-  HIPCHK(hipMalloc((void **)&d_EH, 6 * (size_t)np * sizeof(cd)));
-  HIPCHK(hipMalloc((void **)&d_tmp, 6 * (size_t)np * sizeof(cd)));
-  HIPCHK(hipMalloc((void **)&d_rt, 3 * (size_t)np * sizeof(double)));
-  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&d_sel, (size_t)np * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&d_mask, (size_t)np * sizeof(int)));
+  d_EH = cws<cd>(6, 6 * (size_t)np);
+  d_tmp = cws<cd>(7, 6 * (size_t)np);
+  d_rt = cws<double>(8, 3 * (size_t)np);
+  d_did = cws<int>(9, np);
+  d_sel = cws<int>(10, np);
+  d_mask = cws<int>(11, np);
+  CWS_CHECK(d_EH && d_tmp && d_rt && d_did && d_sel && d_mask);
   HIPCHK(hipMemsetAsync(d_EH, 0, 6 * (size_t)np * sizeof(cd), g.st));
   // self object: the boundary integral (EH_s_bd) only at the points that are
   // not node slots, the node slots from the stored boundary values
@@ -1653,15 +1563,14 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
     if (nc > 0) {
       int *d_ci, *d_ct;
       double *d_cze;
-      if (upload(&d_ci, ci.data(), nc) || upload(&d_ct, ct.data(), nc) || upload(&d_cze, cze.data(), cze.size()))
+      if (cput(12, &d_ci, ci.data(), nc) || cput(13, &d_ct, ct.data(), nc) || cput(14, &d_cze, cze.data(), cze.size()))
         return BH_EHIP;
       bem_launch_field_bd(d_ct, d_cze, nc, 0, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(),
                           g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
                           g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
                           d_tmp, d_rt, g.st);
       hipLaunchKernelGGL(k_scatter6, g1(nc), dim3(256), 0, g.st, d_tmp, d_ci, nc, d_EH);
-      HIPCHK(hipStreamSynchronize(g.st));
-      dfree(d_ci); dfree(d_ct); dfree(d_cze);
+      HIPCHK(hipStreamSynchronize(g.st));   // the host tables above stay alive until here
     }
   }
   hipLaunchKernelGGL(k_gather_stored, g1(np), dim3(256), 0, g.st, d_t, d_st, np, D.d_E, D.d_Hf, d_EH);
@@ -1685,7 +1594,6 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(g.st));
   if (fbeam_failed()) return BH_ECONV;
-  dfree(d_t); dfree(d_st); dfree(d_ze); dfree(d_tmp); dfree(d_rt); dfree(d_did); dfree(d_sel); dfree(d_mask);
   *d_EH_out = d_EH; *d_r_out = d_r; *d_w_out = d_w; *d_wt_out = d_wt;
   return 0;
 }
@@ -1716,8 +1624,9 @@ extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const dou
   double *d_r, *d_w, *d_wt, *d_part, *d_rc;
   int rc1 = surface_fields(objs, N, oid, sp, &d_EH, &d_r, &d_w, &d_wt);
   if (rc1) return rc1;
-  HIPCHK(hipMalloc((void **)&d_part, 6 * (size_t)np * sizeof(double)));
-  if (upload(&d_rc, rc, 3)) return BH_EHIP;
+  d_part = cws<double>(15, 6 * (size_t)np);
+  CWS_CHECK(d_part);
+  if (cput(16, &d_rc, rc, 3)) return BH_EHIP;
   double epd = o->n[0].real() * o->n[0].real();
   bem_launch_stress(d_EH, d_r, d_w, d_wt, np, epd, d_rc, d_part, g.st);
   std::vector<double> part(6 * (size_t)np);
@@ -1725,7 +1634,6 @@ extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const dou
   HIPCHK(hipStreamSynchronize(g.st));
   for (int p = 0; p < np; p++)
     for (int j = 0; j < 3; j++) { F[j] += -0.5 * part[6 * p + j]; Nq[j] += -0.5 * part[6 * p + 3 + j]; }
-  dfree(d_EH); dfree(d_r); dfree(d_w); dfree(d_wt); dfree(d_part); dfree(d_rc);
   return 0;
 }
 
@@ -1745,137 +1653,16 @@ extern "C" int bh_mo_absorb(bh_object **objs, int N, int oid, int type, double *
   double *d_r, *d_w, *d_wt, *d_part;
   int rc1 = surface_fields(objs, N, oid, sp, &d_EH, &d_r, &d_w, &d_wt);
   if (rc1) return rc1;
-  HIPCHK(hipMalloc((void **)&d_part, (size_t)np * sizeof(double)));
+  d_part = cws<double>(15, np);
+  CWS_CHECK(d_part);
   bem_launch_poynting(d_EH, d_w, d_wt, np, d_part, g.st);
   std::vector<double> part(np);
   HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st));
   HIPCHK(hipStreamSynchronize(g.st));
   double p = 0.0;
   for (int i = 0; i < np; i++) p += -0.25 * part[i];
-  dfree(d_EH); dfree(d_r); dfree(d_w); dfree(d_wt); dfree(d_part);
   *P = -p;
   if (*P < 0.0) { *P = -*P; return -2; }
-  return 0;
-}
-
-// ------------------------------------------------------------------ coupling loop
-__global__ void k_reset_incident(cd *__restrict__ Ui, cd *__restrict__ dUi, const cd *__restrict__ Ui0,
-                                 const cd *__restrict__ dUi0, cd *__restrict__ Uip, size_t n)
-{
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Uip[i] = Ui[i];
-  Ui[i] = Ui0[i];
-  dUi[i] = dUi0[i];
-}
-
-// per-slot relative change terms of ccd_f (mo_solve.c:413-455)
-__global__ void k_ccd(const cd *__restrict__ Ui, const cd *__restrict__ Uip, const int *__restrict__ slot, int np,
-                      double *__restrict__ part)
-{
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const cd *c = Ui + (size_t)slot[p] * 4, *q = Uip + (size_t)slot[p] * 4;
-  double ac = 0.0, ap = 0.0, dc = 0.0;
-  for (int i = 0; i < 3; i++) { ac += abs2(c[i]); ap += abs2(q[i]); dc += abs2(c[i] - q[i]); }
-  part[2 * p] = 2.0 * dc / (ac + ap);
-  part[2 * p + 1] = 2.0 * abs2(c[3] - q[3]) / (abs2(c[3]) + abs2(q[3]));
-}
-
-// the scattered field of src at dst's node slots, added into dst's incident
-// data; check: first verify that no slot lies inside src (the geometry does
-// not change during a solve, so the coupling loop checks on its first sweep)
-static int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double *d_part)
-{
-  if (check) {
-    if (domain_ids(src, dst->d_slotPt, dst->nslot, d_did)) return BH_EHIP;
-    std::vector<int> did(dst->nslot);
-    HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
-    HIPCHK(hipStreamSynchronize(g.st));
-    for (int v : did)
-      if (v != 0) return fail(BH_EDOM, "scattered-field domain error (overlapping objects), domain id %d", v);
-  }
-  Dom &S = src->dom[0];
-  bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, src->d_gq, S.d_sid, S.dNe,
-                     src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
-                     g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], src->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr,
-                     S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, src->nquad > 0, d_part, g.st);
-  HIPCHK(hipGetLastError());
-  return 0;
-}
-
-// d_part: device scratch of 2 x (largest slot count) doubles
-static double ccd_value(bh_object **objs, int N, double *d_part)
-{
-  double us = 0.0, ps = 0.0;
-  long cnt = 0;
-  for (int m = 0; m < N; m++) {
-    bh_object *o = objs[m];
-    hipLaunchKernelGGL(k_ccd, g1(o->nslot), dim3(256), 0, g.st, o->d_Ui, o->d_Uip, o->d_slotIdx, o->nslot, d_part);
-    std::vector<double> part(2 * (size_t)o->nslot);
-    if (hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st) != hipSuccess ||
-        hipStreamSynchronize(g.st) != hipSuccess)
-      return -1.0;
-    for (int p = 0; p < o->nslot; p++) { us += part[2 * p]; ps += part[2 * p + 1]; }
-    cnt += o->nslot;
-  }
-  return (us > ps ? us : ps) / (double)cnt;
-}
-
-// block-Jacobi fixed point over objects (mo_solve_bv, mo_solve.c:10-88).
-// trace (optional, iter_max+1 doubles) receives cc per iteration; *iters
-// receives the final iteration count.
-extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters)
-{
-  API_LOCK;
-  if (N <= 0) return fail(BH_EARG, "no objects");
-  if (ensure_rules()) return BH_EHIP;
-  for (int m = 0; m < N; m++)
-    if (!objs[m]->fmode || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d: factor and set beams first", m);
-  int rc, iter = 0;
-  if (N == 1) {
-    if ((rc = bh_object_solve(objs[0]))) return rc;
-  } else {
-    // many solves per object: form the Schur complements once
-    for (int m = 0; m < N; m++) if ((rc = form_s(objs[m]))) return rc;
-    int maxslot = 0;
-    for (int m = 0; m < N; m++) maxslot = std::max(maxslot, objs[m]->nslot);
-    int *d_did;
-    double *d_part;   // far-triangle sums of the mixed form's first coupling pass
-    HIPCHK(hipMalloc((void **)&d_did, (size_t)maxslot * sizeof(int)));
-    HIPCHK(hipMalloc((void **)&d_part, (size_t)maxslot * 19 * sizeof(double)));
-    bool first = true;
-    auto sweep = [&]() -> int {
-      for (int m = 0; m < N; m++) {
-        bh_object *o = objs[m];
-        size_t n = (size_t)(o->Ne + 1) * 16;
-        hipLaunchKernelGGL(k_reset_incident, g1(n), dim3(256), 0, g.st, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, n);
-      }
-      for (int m = 0; m < N; m++)
-        for (int d = 0; d < N; d++)
-          if (d != m) { int r = add_scattered(objs[d], objs[m], d_did, first, d_part); if (r) return r; }
-      first = false;
-      for (int m = 0; m < N; m++) { int r = bh_object_solve(objs[m]); if (r) return r; }
-      return 0;
-    };
-    auto done = [&](int r) { dfree(d_did); dfree(d_part); return r; };
-    for (int m = 0; m < N; m++) if ((rc = bh_object_solve(objs[m]))) return done(rc);
-    if ((rc = sweep())) return done(rc);
-    // BEM3_HIP_PROGRESS=1: one line per iteration on stderr (long solves)
-    const char *pe = getenv("BEM3_HIP_PROGRESS");
-    const bool progress = pe && *pe && *pe != '0';
-    for (iter = 2; iter < iter_max; iter++) {
-      if ((rc = sweep())) return done(rc);
-      double cc = ccd_value(objs, N, d_part);
-      if (cc < 0.0) return done(fail(BH_EHIP, "coupling loop: change measure"));
-      if (trace) trace[iter] = cc;
-      if (progress) { fprintf(stderr, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(stderr); }
-      if (cc < ccd && iter > iter_min) break;
-    }
-    done(0);
-  }
-  for (int m = 0; m < N; m++) if ((rc = bh_object_solve_EH(objs[m]))) return rc;
-  if (iters) *iters = iter;
   return 0;
 }
 
@@ -1885,13 +1672,12 @@ extern "C" int bh_object_domain(bh_object *o, int np, const double *pts, int *di
   API_LOCK;
   if (np <= 0) return 0;
   double *d_pts;
-  int *d_did;
-  if (upload(&d_pts, pts, 3 * (size_t)np)) return BH_EHIP;
-  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * sizeof(int)));
+  if (cput(0, &d_pts, pts, 3 * (size_t)np)) return BH_EHIP;
+  int *d_did = cws<int>(1, np);
+  CWS_CHECK(d_did);
   if (domain_ids(o, d_pts, np, d_did)) return BH_EHIP;
   HIPCHK(hipMemcpyAsync(did, d_did, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, g.st));
   HIPCHK(hipStreamSynchronize(g.st));
-  dfree(d_pts); dfree(d_did);
   return 0;
 }
 
@@ -1918,11 +1704,13 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
   int *d_t, *d_did, *d_mask;
   double *d_ze, *d_rt;
   cd *d_out;
-  if (upload(&d_t, t, np) || upload(&d_ze, ze, 2 * (size_t)np)) return BH_EHIP;
-  HIPCHK(hipMalloc((void **)&d_rt, 3 * (size_t)np * sizeof(double)));
-  HIPCHK(hipMalloc((void **)&d_out, 6 * (size_t)np * sizeof(cd)));
-  HIPCHK(hipMalloc((void **)&d_did, (size_t)np * sizeof(int)));
-  HIPCHK(hipMalloc((void **)&d_mask, (size_t)np * sizeof(int)));
+  if (cput(0, &d_t, t, np) || cput(1, &d_ze, ze, 2 * (size_t)np)) return BH_EHIP;
+  d_rt = cws<double>(2, 3 * (size_t)np);
+  d_out = cws<cd>(3, 6 * (size_t)np);
+  d_did = cws<int>(4, np);
+  d_mask = cws<int>(5, np);
+  double *d_rt2 = cws<double>(6, 3 * (size_t)np);
+  CWS_CHECK(d_rt && d_out && d_did && d_mask && d_rt2);
   HIPCHK(hipMemsetAsync(d_out, 0, 6 * (size_t)np * sizeof(cd), g.st));
   // positions on the element (r = c0 + c1 zeta + c2 eta)
   {
@@ -1935,14 +1723,10 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
   }
   // own boundary integral (needs the solved boundary values)
   if (mode != 2) {
-    double *d_rt2;
-    HIPCHK(hipMalloc((void **)&d_rt2, 3 * (size_t)np * sizeof(double)));
     bem_launch_field_bd(d_t, d_ze, np, did, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[did].real(), o->kn[did].imag(),
                         g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
                         g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
                         d_rt2, g.st);
-    HIPCHK(hipStreamSynchronize(g.st));
-    dfree(d_rt2);
   }
   if (did == 0 && mode != 0) {
     for (int m = 0; m < N; m++) {
@@ -1966,6 +1750,33 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, d_out, 6 * (size_t)np * sizeof(cd), hipMemcpyDeviceToHost, g.st));
   HIPCHK(hipStreamSynchronize(g.st));
-  dfree(d_t); dfree(d_ze); dfree(d_rt); dfree(d_out); dfree(d_did); dfree(d_mask);
+  return 0;
+}
+
+// device state of the library released (every object destroyed first):
+// rule tables, work areas, timing events, the stream.  bh_init (or any entry
+// point) sets the library up again.  The uncached exchange areas of the
+// one-launch LU panel and triangular solves stay until the process ends.
+extern "C" int bh_finalize(void)
+{
+  API_LOCK;
+  if (g.dev < 0) return 0;
+  (void)hipStreamSynchronize(g.st);
+  bem_mo_release();
+  if (g_cws) { delete[] g_cws; g_cws = nullptr; }
+  for (int i = 0; i < 4; i++) {
+    dfree(g_rule[i]); g_rule[i] = nullptr; g_nrule[i] = 0;
+    dfree(g_qrule[i]); g_qrule[i] = nullptr; g_nqrule[i] = 0;
+  }
+  dfree(g_phit); g_phit = nullptr;
+  dfree(g_fberr); g_fberr = nullptr;
+  dfree(g.d_gl); dfree(g.d_gh); dfree(g.d_gl32); dfree(g.d_glr);
+  dfree(g.d_xl); dfree(g.d_wl); dfree(g.d_xh); dfree(g.d_wh);
+  bem_la_release();
+  bem_prof_release();
+  (void)hipEventDestroy(g.ev[0]);
+  (void)hipEventDestroy(g.ev[1]);
+  (void)hipStreamDestroy(g.st);
+  g = Ctx();
   return 0;
 }

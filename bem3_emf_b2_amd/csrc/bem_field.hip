@@ -1313,14 +1313,16 @@ void bem_launch_scatter(const double *pts, const double *nrm, const int *slot, i
   if (np <= 0) return;
   const int kid = gl32 ? BH_K_SCATTER32 : BH_K_SCATTER;
   if (part) {
-    // the far triangles first, then the rest
-    bem_prof_begin(kid, (double)np * dNe, st);
+    // the far triangles first (their own timing record), then the rest
+    const int kf = gl32 ? BH_K_SCATTER_FAR32 : BH_K_SCATTER_FAR;
+    bem_prof_begin(kf, (double)np * dNe, st);
     hipLaunchKernelGGL(gl32 ? k_scatter_far<true> : k_scatter_far<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm,
                        np, (const TriGeo *)geo, sid, dNe, kr, ki, scale, (const QPt32 *)gl32, (const QPtN *)gl, ngl,
                        (const cd *)U, (const cd *)dU, part);
-    bem_prof_end(kid, st);
+    bem_prof_end(kf, st);
   }
-  bem_prof_begin(kid, (double)np * dNe, st);
+  // the second pass carries the pairs only when it sums them all
+  bem_prof_begin(kid, part ? 0.0 : (double)np * dNe, st);
   hipLaunchKernelGGL(quads ? k_scatter<true> : k_scatter<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, nrm, slot, np, (const TriGeo *)geo,
                      (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, ngl, (const QPtN *)gh, ngh, scale,
                      (const QPtQ *)qgl, nqgl, (const QPtQ *)qgh, nqgh, (const QPt32 *)gl32, (const cd *)U,

@@ -1410,6 +1410,17 @@ int bem_la_init(void)
 // nonzero if a one-launch triangular solve timed out since the last call
 int bem_trsv_error(hipStream_t st) { return trsv_error(st); }
 
+// frees the cached work area of the factorisation; the uncached exchange
+// areas of the one-launch panel and triangular solves stay allocated until
+// the process ends (memory freed after a kernel read it uncached must not be
+// handed out again)
+void bem_la_release(void)
+{
+  if (g_lu_ws) (void)hipFree(g_lu_ws);
+  g_lu_ws = nullptr;
+  g_lu_ws_bytes = 0;
+}
+
 // complex FP64 (interleaved double pairs)
 void bem_zgemm(int m, int n, int k, double ar, double ai, const void *A, size_t lda, const void *B, size_t ldb,
                double br, double bi, void *C, size_t ldc, hipStream_t st)

@@ -1,0 +1,810 @@
+// bem_mo.hip -- the coupling loop of several objects (mo_solve_bv,
+// mo_solve.c:10-88) on one MI355X.
+//
+// Each sweep adds to every object's incident data the field scattered by
+// every other object at its open-domain node slots (add_scattered_mEMP and
+// mEMP2_s, mo_solve.c:296-396), then solves every object again.  Two ways to
+// form those coupling sums:
+//
+//   * direct: the reference's element quadrature over every (slot, source
+//     element) pair (k_scatter_far + k_scatter, bem_field.hip);
+//   * multipole (the mixed-precision form, source objects of triangles in a
+//     lossless surrounding medium): the field of source object n outside
+//     its bounding sphere is the series
+//        U(x) = sum_{l<=L} sum_{|m|<=l} A_lm h_l(k|x-c|) Y_l^m(x-c)
+//     whose coefficients collect the same quadrature points the direct sum
+//     uses (the reduced far rules): with the point monopoles a (from dU) and
+//     dipoles b W (from U) of the element integrals,
+//        A_lm = 4 pi i k sum_q [a_q conj(R_l^m(y_q)) + b_q W . grad conj(R_l^m)(y_q)],
+//     R_l^m = j_l Y_l^m.  The addition theorem reproduces the direct
+//     discrete sum up to a truncation error of about rho^L, rho = the source
+//     sphere's radius over the distance of the slot from its centre.  A pair
+//     of objects takes the series when rho <= 0.6 for all its slots and the
+//     gap to the sphere is at least 4 element radii (so every pair is a far
+//     pair of the direct sum, and its near-pair switch never fires there);
+//     L = max(ln 1e-10 / ln rho, kr + 6 (kr)^(1/3) + 4), at most 50.  The
+//     solid-angle terms F, dF of the direct sum (a closed surface seen from
+//     outside: zero up to quadrature error) are taken as zero there.
+//
+// Work per sweep of the series: a basis kernel (lane per source point)
+// writes conj(R_l^m) of every point, a moment kernel contracts them with the
+// 16 point densities (monopole and three dipole components of each of the 4
+// potentials), a coefficient kernel folds the dipoles' gradients into A and
+// forms the gradient coefficient sets (d_z, d_+ = d_x + i d_y, d_-), and one
+// evaluation launch over all destination slots (lane per slot, the source
+// object's coefficients wave-uniform) adds U and dU/dn into the incident
+// data.  All series arithmetic is FP64.
+//
+// Spherical functions: Y_l^m orthonormal with the Condon-Shortley phase,
+// Y_l^{-m} = (-1)^m conj(Y_l^m); Y_l^m = P^_l^m(cos t) w^m with w = (x+iy)/r
+// and P^ = Pbar / sin^m (a polynomial in cos t: no pole).  Gradients:
+//   d_z F_l^m = k [cz1 F_{l-1}^m + cz2 F_{l+1}^m],
+//   d_+ F_l^m = k [cp1 F_{l-1}^{m+1} + cp2 F_{l+1}^{m+1}],
+//   d_- F_l^m = k [cm1 F_{l-1}^{m-1} + cm2 F_{l+1}^{m-1}]
+// for F = j Y or h Y (coefficients below).
+#include "bem_obj.h"
+#include "bem_prof.h"
+
+using namespace bem;
+
+extern "C" {
+void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const void *, const int *, int,
+                        double, double, const void *, int, const void *, int, double, const void *, int, const void *,
+                        int, const void *, const void *, const void *, void *, void *, int, double *, hipStream_t);
+}
+
+#define MP_LMAX 50              // largest series order
+#define MP_LPMAX (MP_LMAX + 1)  // moments and gradient sets run one order further
+#define MP_RHO 0.6              // largest source-sphere radius / slot distance
+#define MP_EPS 1e-10            // truncation target (relative)
+#define MP_GAP_RADII 4.0        // gap to the source sphere in element radii
+#define MP_NDENS 16             // point densities: 4 potentials x (monopole, dipole z, +, -)
+#define MP_QSPLIT 2048          // source points per moment workgroup
+
+// ------------------------------------------------------------------ series helpers
+__host__ __device__ __forceinline__ int mp_idx(int n, int m, int Lp)
+{
+  if (m == 0) return n;
+  const int am = m < 0 ? -m : m;
+  return (Lp + 1) + (am - 1) * (2 * Lp + 2 - am) + 2 * (n - am) + (m < 0 ? 1 : 0);
+}
+__host__ __device__ __forceinline__ int mp_count(int Lp) { return (Lp + 1) * (Lp + 1); }
+
+__device__ __forceinline__ double cz1(int n, int m) { return n > 0 ? sqrt(fmax(0.0, (double)((n + m) * (n - m))) / ((2.0 * n - 1) * (2.0 * n + 1))) : 0.0; }
+__device__ __forceinline__ double cz2(int n, int m) { return -sqrt((double)((n + m + 1) * (n - m + 1)) / ((2.0 * n + 1) * (2.0 * n + 3))); }
+__device__ __forceinline__ double cp1(int n, int m) { return n > 0 ? sqrt(fmax(0.0, (double)((n - m - 1) * (n - m))) / ((2.0 * n - 1) * (2.0 * n + 1))) : 0.0; }
+__device__ __forceinline__ double cp2(int n, int m) { return sqrt((double)((n + m + 1) * (n + m + 2)) / ((2.0 * n + 1) * (2.0 * n + 3))); }
+__device__ __forceinline__ double cm1(int n, int m) { return n > 0 ? -sqrt(fmax(0.0, (double)((n + m - 1) * (n + m))) / ((2.0 * n - 1) * (2.0 * n + 1))) : 0.0; }
+__device__ __forceinline__ double cm2(int n, int m) { return -sqrt((double)((n - m + 1) * (n - m + 2)) / ((2.0 * n + 1) * (2.0 * n + 3))); }
+
+// recurrence constants of P^_n^m: P^_n^m = a (u P^_{n-1}^m - b P^_{n-2}^m)
+__device__ __forceinline__ double pa(int n, int m) { return sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m)); }
+__device__ __forceinline__ double pb(int n, int m) { return sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) / (4.0 * (n - 1.0) * (n - 1.0) - 1.0)); }
+
+// spherical Bessel j_0..j_Lp(x) into t[n * stride]: downward recurrence
+// (Miller), rescaled on the way down against overflow, normalised by the
+// sum rule sum_n (2n+1) j_n^2 = 1 and signed by j_0 or j_1, whichever is the
+// larger (no division by a zero of j_0)
+__device__ void sph_j(double x, int Lp, double *t, int stride)
+{
+  if (x < 1e-12) {
+    t[0] = 1.0;
+    for (int n = 1; n <= Lp; n++) t[n * stride] = 0.0;
+    return;
+  }
+  const int N = Lp + 24 + (int)(1.5 * x);
+  double jp1 = 0.0, j = 1e-100, ix = 1.0 / x, sum = (2.0 * N + 1.0) * j * j, j1 = 0.0;
+  for (int n = N; n >= 1; n--) {
+    const double jm1 = (2.0 * n + 1.0) * ix * j - jp1;
+    jp1 = j;
+    j = jm1;
+    sum += (2.0 * n - 1.0) * j * j;
+    if (n - 1 <= Lp) t[(n - 1) * stride] = j;
+    if (n == 2) j1 = j;
+    if (fabs(j) > 1e100) {   // values stay within 1e+-100, their squares within 1e+-200
+      j *= 1e-100;
+      jp1 *= 1e-100;
+      j1 *= 1e-100;
+      sum *= 1e-200;
+      for (int q = n - 1; q <= Lp; q++) t[q * stride] *= 1e-100;
+    }
+  }
+  double s, c;
+  sincos(x, &s, &c);
+  const double j0t = s * ix, j1t = (s * ix - c) * ix;
+  double f = 1.0 / sqrt(sum);
+  if (fabs(j0t) >= fabs(j1t) ? (j0t * j < 0.0) : (j1t * j1 < 0.0)) f = -f;
+  for (int n = 0; n <= Lp; n++) t[n * stride] *= f;
+}
+
+// spherical Hankel h_0..h_Lp(x) (first kind, upward recurrence)
+__device__ void sph_h(double x, int Lp, cd *t, int stride)
+{
+  double s, c;
+  sincos(x, &s, &c);
+  const double ix = 1.0 / x;
+  cd h0 = mk(s * ix, -c * ix);                                  // -i e^{ix} / x
+  cd h1 = mk((s * ix - c) * ix, -(c * ix + s) * ix);           // -e^{ix} (x + i) / x^2
+  t[0] = h0;
+  if (Lp >= 1) t[stride] = h1;
+  for (int n = 1; n < Lp; n++) {
+    const cd h2 = h1 * ((2.0 * n + 1.0) * ix) - h0;
+    t[(n + 1) * stride] = h2;
+    h0 = h1;
+    h1 = h2;
+  }
+}
+
+// ------------------------------------------------------------------ source side
+// One source object's series data.
+struct MpSrc {
+  double c[3];      // expansion centre
+  double kr;        // wavenumber of the surrounding medium (real)
+  double scale;     // rule scale of the element integrals
+  int Lp;           // series order + 1 (moments, gradient sets)
+  int L;            // series order
+  int qoff, npt;    // far rule: offset in the rule table, points per element
+  int dNe;          // source elements (domain 0)
+  int nq;           // source points = dNe * npt
+  int nsplit;       // moment workgroups over the points
+  int pad;
+};
+
+// lane per source point q (element 1 + q / npt, rule point q % npt): the
+// 16 densities and conj(R_n^m(y - c)) of every (n, m) (layout [q][idx])
+__global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
+                                                 const QPt *__restrict__ rule, const cd *__restrict__ U,
+                                                 const cd *__restrict__ dU, int q0, int nqc, cd *__restrict__ B,
+                                                 size_t ldb, cd *__restrict__ dens)
+{
+  __shared__ double jt[(MP_LPMAX + 1) * 64];
+  const int lane = threadIdx.x;
+  const int ql = blockIdx.x * 64 + lane;
+  if (ql >= nqc) return;
+  const int q = q0 + ql;
+  const int s = 1 + q / S.npt, p = q % S.npt;
+  const int sd = sid[s], as = sd < 0 ? -sd : sd;
+  const TriGeo g = geo[as];
+  const QPt qp = rule[S.qoff + p];
+  double y[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) y[d] = fma(g.c2[d], qp.e, fma(g.c1[d], qp.z, g.c0[d])) - S.c[d];
+  // densities: a_c = w scale |W| du_c (monopole), b_c = -sigma w scale u_c
+  // (dipole b_c W), split as W_z, (W_x - i W_y)/2, (W_x + i W_y)/2
+  double n0, n1, n2;
+  tri_N(qp.z, qp.e, n0, n1, n2);
+  const double wt = qp.w * S.scale;
+  const double sg = sd < 0 ? -1.0 : 1.0;
+  const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
+  cd *dq = dens + (size_t)ql * MP_NDENS;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const cd u = Us[c] * n0 + Us[4 + c] * n1 + Us[8 + c] * n2;
+    const cd du = dUs[c] * n0 + dUs[4 + c] * n1 + dUs[8 + c] * n2;
+    const cd a = du * (wt * g.aW), b = u * (-sg * wt);
+    dq[c] = a;
+    dq[4 + c] = b * g.W[2];
+    dq[8 + c] = b * mk(0.5 * g.W[0], -0.5 * g.W[1]);
+    dq[12 + c] = b * mk(0.5 * g.W[0], 0.5 * g.W[1]);
+  }
+  const double r = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+  const int Lp = S.Lp;
+  sph_j(S.kr * r, Lp, jt + lane, 64);
+  const double ir = r > 0.0 ? 1.0 / r : 0.0;
+  const double u = r > 0.0 ? y[2] * ir : 1.0;
+  const cd w = mk(y[0] * ir, y[1] * ir);
+  cd *Bq = B + (size_t)ql * ldb;
+  // m-outer, n-inner; P^_m^m = -sqrt((2m+1)/2m) P^_{m-1}^{m-1}
+  double pmm = 0.28209479177387814347;   // 1 / sqrt(4 pi)
+  cd wm = mk(1.0, 0.0);
+  for (int m = 0; m <= Lp; m++) {
+    if (m > 0) {
+      pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m));
+      wm = wm * w;
+    }
+    const cd cwm = conjd(wm);
+    const double sgn = (m & 1) ? -1.0 : 1.0;
+    double p2 = 0.0, p1 = pmm;
+    for (int n = m; n <= Lp; n++) {
+      double pn;
+      if (n == m) pn = pmm;
+      else if (n == m + 1) pn = sqrt(2.0 * m + 3.0) * u * pmm;
+      else pn = pa(n, m) * (u * p1 - pb(n, m) * p2);
+      if (n > m) { p2 = p1; p1 = pn; }
+      const double jr = jt[n * 64 + lane] * pn;
+      const int i = mp_idx(n, m, Lp);
+      Bq[i] = cwm * jr;                          // conj(R_n^m)
+      if (m > 0) Bq[i + 1] = wm * (sgn * jr);    // conj(R_n^{-m}) = (-1)^m R_n^m
+    }
+  }
+}
+
+// T[split][idx][16] = sum over the split's points of B[q][idx] dens[q][:]
+// (thread per idx, densities wave-uniform)
+__global__ void __launch_bounds__(256) k_mp_moments(int nidx, int nqc, int qper, const cd *__restrict__ B, size_t ldb,
+                                                    const cd *__restrict__ dens, cd *__restrict__ T)
+{
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  const int split = blockIdx.y;
+  const int qa = split * qper, qb = min(nqc, qa + qper);
+  cd acc[MP_NDENS];
+#pragma unroll
+  for (int d = 0; d < MP_NDENS; d++) acc[d] = mk(0.0, 0.0);
+  if (idx < nidx) {
+    for (int q = qa; q < qb; q++) {
+      const cd b = B[(size_t)q * ldb + idx];
+      const cd *dq = dens + (size_t)q * MP_NDENS;
+#pragma unroll
+      for (int d = 0; d < MP_NDENS; d++) acc[d] = acc[d] + b * dq[d];
+    }
+    cd *t = T + ((size_t)split * nidx + idx) * MP_NDENS;
+#pragma unroll
+    for (int d = 0; d < MP_NDENS; d++) t[d] = acc[d];
+  }
+}
+
+__global__ void k_zero_cd(cd *__restrict__ a, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = mk(0.0, 0.0);
+}
+
+// sum of the splits' partial moments, added to Tsum (chunks of the source
+// points accumulate across launches)
+__global__ void k_mp_reduce(int n, int nsplit, const cd *__restrict__ T, cd *__restrict__ Tsum)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cd s = Tsum[i];
+  for (int k = 0; k < nsplit; k++) s = s + T[(size_t)k * n + i];
+  Tsum[i] = s;
+}
+
+// moment of density set d (0 monopole, 1 z, 2 +, 3 -) and potential c at (n, m)
+__device__ __forceinline__ cd mp_T(const cd *T, int Lp, int n, int m, int d, int c)
+{
+  if (n < 0 || n > Lp || m > n || -m > n) return mk(0.0, 0.0);
+  return T[(size_t)mp_idx(n, m, Lp) * MP_NDENS + d * 4 + c];
+}
+// series coefficient A_nm of potential c (zero beyond the order L)
+__device__ cd mp_A(const cd *T, int L, int Lp, double k, int n, int m, int c)
+{
+  if (n < 0 || n > L || m > n || -m > n) return mk(0.0, 0.0);
+  cd s = mp_T(T, Lp, n, m, 0, c);
+  s = s + (mp_T(T, Lp, n - 1, m, 1, c) * cz1(n, m) + mp_T(T, Lp, n + 1, m, 1, c) * cz2(n, m)) * k;
+  s = s + (mp_T(T, Lp, n - 1, m - 1, 2, c) * cm1(n, m) + mp_T(T, Lp, n + 1, m - 1, 2, c) * cm2(n, m)) * k;
+  s = s + (mp_T(T, Lp, n - 1, m + 1, 3, c) * cp1(n, m) + mp_T(T, Lp, n + 1, m + 1, 3, c) * cp2(n, m)) * k;
+  return s * mk(0.0, 4.0 * M_PI * k);
+}
+
+// coefficient sets of the evaluation, [idx][set][c]: set 0 A (U), 1 the
+// coefficients of d_z U, 2 of d_+ U, 3 of d_- U
+__global__ void k_mp_coef(int L, int Lp, double k, const cd *__restrict__ T, cd *__restrict__ C)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nidx = mp_count(Lp);
+  if (t >= nidx * 4) return;
+  const int i = t >> 2, c = t & 3;
+  // (n, m) of index i
+  int n = 0, m = 0;
+  if (i <= Lp) { n = i; m = 0; }
+  else {
+    int am = 1;
+    while (am < Lp && mp_idx(am + 1, am + 1, Lp) <= i) am++;
+    const int r = i - mp_idx(am, am, Lp);
+    n = am + r / 2;
+    m = (r & 1) ? -am : am;
+  }
+  cd *o = C + (size_t)i * 16;
+  o[c] = mp_A(T, L, Lp, k, n, m, c);
+  cd dz = mk(0.0, 0.0), dp = mk(0.0, 0.0), dm = mk(0.0, 0.0);
+  if (n + 1 <= Lp) {
+    if (m <= n + 1 && -m <= n + 1) dz = dz + mp_A(T, L, Lp, k, n + 1, m, c) * cz1(n + 1, m);
+    if (m - 1 <= n + 1 && 1 - m <= n + 1) dp = dp + mp_A(T, L, Lp, k, n + 1, m - 1, c) * cp1(n + 1, m - 1);
+    if (m + 1 <= n + 1 && -m - 1 <= n + 1) dm = dm + mp_A(T, L, Lp, k, n + 1, m + 1, c) * cm1(n + 1, m + 1);
+  }
+  if (n - 1 >= 0) {
+    if (m <= n - 1 && -m <= n - 1) dz = dz + mp_A(T, L, Lp, k, n - 1, m, c) * cz2(n - 1, m);
+    if (m - 1 <= n - 1 && 1 - m <= n - 1) dp = dp + mp_A(T, L, Lp, k, n - 1, m - 1, c) * cp2(n - 1, m - 1);
+    if (m + 1 <= n - 1 && -m - 1 <= n - 1) dm = dm + mp_A(T, L, Lp, k, n - 1, m + 1, c) * cm2(n - 1, m + 1);
+  }
+  o[4 + c] = dz * k;
+  o[8 + c] = dp * k;
+  o[12 + c] = dm * k;
+}
+
+// ------------------------------------------------------------------ evaluation
+// one (destination object, source object) pair of the series
+struct MpPair {
+  const cd *coef;     // the source's coefficient sets
+  double c[3];        // its centre
+  double kr;
+  int Lp;             // the source's layout order
+  int Le;             // evaluation order of this pair (<= Lp)
+  int pad[2];
+};
+// one tile: 64 destination slots of one object and its list of source pairs
+struct MpTile {
+  const double *pts, *nrm;   // the object's slot positions and normals
+  const int *slot;           // slot -> incident-data index
+  cd *Ui, *dUi;
+  int s0, ns;                // first slot, slots in this tile
+  int p0, np;                // its pairs in the pair table
+};
+
+__global__ void __launch_bounds__(64) k_mp_eval(const MpTile *__restrict__ tiles, const MpPair *__restrict__ pairs)
+{
+  __shared__ cd ht[(MP_LPMAX + 1) * 64];
+  const MpTile T = tiles[blockIdx.x];
+  const int lane = threadIdx.x;
+  const bool act = lane < T.ns;
+  const int sl = T.s0 + (act ? lane : 0);
+  const double x[3] = {T.pts[3 * sl], T.pts[3 * sl + 1], T.pts[3 * sl + 2]};
+  const double v[3] = {T.nrm[3 * sl], T.nrm[3 * sl + 1], T.nrm[3 * sl + 2]};
+  cd aU[4], aZ[4], aP[4], aM[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) aU[c] = aZ[c] = aP[c] = aM[c] = mk(0.0, 0.0);
+  for (int pi = 0; pi < T.np; pi++) {
+    const MpPair P = pairs[T.p0 + pi];
+    const double y[3] = {x[0] - P.c[0], x[1] - P.c[1], x[2] - P.c[2]};
+    const double r = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]), ir = 1.0 / r;
+    const int Le = P.Le, Lp = P.Lp;
+    sph_h(P.kr * r, Le, ht + lane, 64);
+    const double u = y[2] * ir;
+    const cd w = mk(y[0] * ir, y[1] * ir);
+    double pmm = 0.28209479177387814347;
+    cd wm = mk(1.0, 0.0);
+    for (int m = 0; m <= Le; m++) {
+      if (m > 0) {
+        pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m));
+        wm = wm * w;
+      }
+      const cd cwm = conjd(wm);
+      const double sgn = (m & 1) ? -1.0 : 1.0;
+      double p2 = 0.0, p1 = pmm;
+      for (int n = m; n <= Le; n++) {
+        double pn;
+        if (n == m) pn = pmm;
+        else if (n == m + 1) pn = sqrt(2.0 * m + 3.0) * u * pmm;
+        else pn = pa(n, m) * (u * p1 - pb(n, m) * p2);
+        if (n > m) { p2 = p1; p1 = pn; }
+        const cd hp = ht[n * 64 + lane] * pn;
+        const cd Sp = hp * wm;                   // S_n^m
+        const cd *C = P.coef + (size_t)mp_idx(n, m, Lp) * 16;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          aU[c] = aU[c] + C[c] * Sp;
+          aZ[c] = aZ[c] + C[4 + c] * Sp;
+          aP[c] = aP[c] + C[8 + c] * Sp;
+          aM[c] = aM[c] + C[12 + c] * Sp;
+        }
+        if (m > 0) {
+          const cd Sn = hp * cwm * sgn;          // S_n^{-m} = h_n (-1)^m conj(Y_n^m)
+          const cd *Cn = C + 16;
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            aU[c] = aU[c] + Cn[c] * Sn;
+            aZ[c] = aZ[c] + Cn[4 + c] * Sn;
+            aP[c] = aP[c] + Cn[8 + c] * Sn;
+            aM[c] = aM[c] + Cn[12 + c] * Sn;
+          }
+        }
+      }
+    }
+  }
+  if (!act) return;
+  // d/dv = v_z d_z + (v_x - i v_y)/2 d_+ + (v_x + i v_y)/2 d_-
+  const cd vm = mk(0.5 * v[0], -0.5 * v[1]), vp = mk(0.5 * v[0], 0.5 * v[1]);
+  const size_t o = (size_t)T.slot[sl] * 4;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const cd du = aZ[c] * v[2] + aP[c] * vm + aM[c] * vp;
+    T.Ui[o + c] = T.Ui[o + c] + aU[c];
+    T.dUi[o + c] = T.dUi[o + c] + du;
+  }
+}
+
+// ------------------------------------------------------------------ loop kernels
+__global__ void k_reset_incident(cd *__restrict__ Ui, cd *__restrict__ dUi, const cd *__restrict__ Ui0,
+                                 const cd *__restrict__ dUi0, cd *__restrict__ Uip, size_t n)
+{
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Uip[i] = Ui[i];
+  Ui[i] = Ui0[i];
+  dUi[i] = dUi0[i];
+}
+
+// per-slot relative change terms of ccd_f (mo_solve.c:413-455), written at
+// the object's offset of the loop's slot table
+__global__ void k_ccd(const cd *__restrict__ Ui, const cd *__restrict__ Uip, const int *__restrict__ slot, int np,
+                      double *__restrict__ part)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const cd *c = Ui + (size_t)slot[p] * 4, *q = Uip + (size_t)slot[p] * 4;
+  double ac = 0.0, ap = 0.0, dc = 0.0;
+  for (int i = 0; i < 3; i++) { ac += abs2(c[i]); ap += abs2(q[i]); dc += abs2(c[i] - q[i]); }
+  part[2 * p] = 2.0 * dc / (ac + ap);
+  part[2 * p + 1] = 2.0 * abs2(c[3] - q[3]) / (abs2(c[3]) + abs2(q[3]));
+}
+
+// the two sums of the change terms over n slots, in a fixed order (one
+// workgroup; out[0], out[1])
+__global__ void __launch_bounds__(256) k_ccd_sum(const double *__restrict__ part, int n, double *__restrict__ out)
+{
+  __shared__ double su[256], sp[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) { a += part[2 * i]; b += part[2 * i + 1]; }
+  su[threadIdx.x] = a;
+  sp[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { su[threadIdx.x] += su[threadIdx.x + s]; sp[threadIdx.x] += sp[threadIdx.x + s]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = su[0]; out[1] = sp[0]; }
+}
+
+// ------------------------------------------------------------------ host side
+namespace {
+
+int g_coupling_mode = 0;   // 0: series where the pair allows it (mixed form), 1: direct sums only
+int g_last_series = 0, g_last_direct = 0, g_last_lmax = -1;   // the last loop's plan
+
+// grow-only work areas of the loop (bem_mo_release frees them)
+Scratch *g_ws = nullptr;
+enum { W_DID, W_PART, W_CCD, W_CCDSUM, W_B, W_DENS, W_T, W_TSUM, W_COEF, W_TILES, W_PAIRS, W_NWS };
+template <class T> T *ws(int i, size_t n)
+{
+  if (!g_ws) g_ws = new Scratch[W_NWS];
+  return g_ws[i].get<T>(std::max<size_t>(n, 1));
+}
+
+// geometry of one object as a source of the series: centre, radius of the
+// bounding sphere of its open-domain faces, largest element radius
+struct SrcGeo {
+  double c[3], rad, erad;
+  bool ok;       // triangles only
+};
+
+SrcGeo src_geo(const bh_object *o)
+{
+  SrcGeo s{};
+  s.ok = o->nquad == 0;
+  const Dom &D = o->dom[0];
+  double c[3] = {0, 0, 0};
+  int nv = 0;
+  for (int t = 1; t <= D.dNe; t++) {
+    const int a = std::abs(D.sid[t]);
+    for (int j = 0; j < 3; j++) {
+      const int v = o->ed[4 * a + j];
+      for (int d = 0; d < 3; d++) c[d] += o->rn[3 * (size_t)v + d];
+      nv++;
+    }
+  }
+  for (int d = 0; d < 3; d++) s.c[d] = nv ? c[d] / nv : 0.0;
+  double r2 = 0.0, e2 = 0.0;
+  for (int t = 1; t <= D.dNe; t++) {
+    const int a = std::abs(D.sid[t]);
+    double cen[3] = {0, 0, 0};
+    for (int j = 0; j < 3; j++)
+      for (int d = 0; d < 3; d++) cen[d] += o->rn[3 * (size_t)o->ed[4 * a + j] + d] / 3.0;
+    for (int j = 0; j < 3; j++) {
+      const double *p = &o->rn[3 * (size_t)o->ed[4 * a + j]];
+      double q = 0.0, w = 0.0;
+      for (int d = 0; d < 3; d++) { q += (p[d] - s.c[d]) * (p[d] - s.c[d]); w += (p[d] - cen[d]) * (p[d] - cen[d]); }
+      r2 = std::max(r2, q);
+      e2 = std::max(e2, w);
+    }
+  }
+  s.rad = sqrt(r2);
+  s.erad = sqrt(e2);
+  return s;
+}
+
+// open-domain node slots of an object on the host, in the order of its
+// device slot table (4 per element, centroid included)
+void slot_points(const bh_object *o, std::vector<double> &pt)
+{
+  pt.clear();
+  for (int i = 1; i <= o->Ne; i++) {
+    if (o->md[i] != 0) continue;
+    for (int j = 0; j < 4; j++)
+      for (int q = 0; q < 3; q++) pt.push_back(o->ren[(size_t)i * 12 + j * 3 + q]);
+  }
+}
+
+// the plan of one coupling loop: which pairs take the series, with what
+// order, and the evaluation tiles
+struct Plan {
+  std::vector<SrcGeo> geo;
+  std::vector<std::vector<int>> Lmn;   // [dst][src]: series order, -1 direct, -2 same object
+  std::vector<int> Lsrc;               // per source: largest order over its pairs (-1: none)
+  std::vector<int> rule;               // per source: rule-table offset of its far rule
+  std::vector<int> npt;                // and points per element
+  int npairs_mp = 0, npairs_direct = 0;
+};
+
+int make_plan(bh_object **objs, int N, Plan &P)
+{
+  P.geo.resize(N);
+  for (int m = 0; m < N; m++) P.geo[m] = src_geo(objs[m]);
+  P.Lmn.assign(N, std::vector<int>(N, -1));
+  P.Lsrc.assign(N, -1);
+  P.rule.assign(N, 0);
+  P.npt.assign(N, 0);
+  std::vector<double> mingap(N, 1e300);
+  std::vector<double> pt;
+  for (int m = 0; m < N; m++) {
+    slot_points(objs[m], pt);
+    const size_t ns = pt.size() / 3;
+    for (int n = 0; n < N; n++) {
+      if (n == m) { P.Lmn[m][n] = -2; continue; }
+      const bh_object *s = objs[n];
+      const SrcGeo &G = P.geo[n];
+      const zd k = s->kn[0];
+      const bool eligible = g_coupling_mode == 0 && s->prec == BH_PREC_MIXED && objs[m]->prec == BH_PREC_MIXED &&
+                            G.ok && k.imag() == 0.0 && k.real() > 0.0 && G.rad > 0.0;
+      if (!eligible) continue;
+      double dmin = 1e300;
+      for (size_t p = 0; p < ns; p++) {
+        double q = 0.0;
+        for (int d = 0; d < 3; d++) q += (pt[3 * p + d] - G.c[d]) * (pt[3 * p + d] - G.c[d]);
+        dmin = std::min(dmin, q);
+      }
+      dmin = sqrt(dmin);
+      const double rho = G.rad / dmin, gap = dmin - G.rad;
+      if (!(rho <= MP_RHO) || gap < MP_GAP_RADII * G.erad) continue;
+      const double kr = k.real() * G.rad;
+      const int L = std::max((int)ceil(log(MP_EPS) / log(rho)), (int)ceil(kr + 6.0 * cbrt(kr)) + 4);
+      if (L > MP_LMAX) continue;
+      P.Lmn[m][n] = L;
+      P.Lsrc[n] = std::max(P.Lsrc[n], L);
+      mingap[n] = std::min(mingap[n], gap);
+    }
+  }
+  // far rule of each source (the mixed form's rules, keyed as far_rule keys
+  // them, by the closest slot of any of its series pairs)
+  for (int n = 0; n < N; n++) {
+    if (P.Lsrc[n] < 0) continue;
+    const double kR = std::abs(objs[n]->kn[0]) * P.geo[n].erad;
+    const double d = mingap[n] / std::max(P.geo[n].erad, 1e-300);
+    if (d >= 8.0 && kR <= 0.5) { P.rule[n] = BEM_FAR_OFF4; P.npt[n] = 48; }
+    else if (kR <= 1.2) { P.rule[n] = BEM_FAR_OFF5; P.npt[n] = 75; }
+    else if (kR <= 3.0) { P.rule[n] = BEM_FAR_OFF6; P.npt[n] = 108; }
+    else { P.rule[n] = 0; P.npt[n] = 432; }
+  }
+  for (int m = 0; m < N; m++)
+    for (int n = 0; n < N; n++)
+      if (m != n) (P.Lmn[m][n] >= 0 ? P.npairs_mp : P.npairs_direct)++;
+  return 0;
+}
+
+// the series coefficients of source object n from its current boundary values
+int mp_coefficients(bh_object *o, const Plan &P, int n, cd *coef)
+{
+  const Dom &D = o->dom[0];
+  MpSrc S{};
+  for (int d = 0; d < 3; d++) S.c[d] = P.geo[n].c[d];
+  S.kr = o->kn[0].real();
+  S.scale = g_rscale[2];
+  S.L = P.Lsrc[n];
+  S.Lp = S.L + 1;
+  S.qoff = P.rule[n];
+  S.npt = P.npt[n];
+  S.dNe = D.dNe;
+  S.nq = D.dNe * S.npt;
+  const int nidx = mp_count(S.Lp);
+  // the source points in chunks whose basis fits the work area (~1.5 GB)
+  const size_t ldb = (size_t)nidx;
+  const int qc = (int)std::max<size_t>(MP_QSPLIT, std::min<size_t>((size_t)S.nq, ((size_t)96 << 20) / ldb / MP_QSPLIT * MP_QSPLIT));
+  const int nsplit = (qc + MP_QSPLIT - 1) / MP_QSPLIT;
+  cd *B = ws<cd>(W_B, (size_t)qc * ldb);
+  cd *dens = ws<cd>(W_DENS, (size_t)qc * MP_NDENS);
+  cd *T = ws<cd>(W_T, (size_t)nsplit * nidx * MP_NDENS);
+  cd *Ts = ws<cd>(W_TSUM, (size_t)nidx * MP_NDENS);
+  if (!B || !dens || !T || !Ts) return fail(BH_EHIP, "series: work area (%d coefficients)", nidx);
+  // the sum is cleared by a kernel (no memset node if a caller captures this)
+  hipLaunchKernelGGL(k_zero_cd, g1((size_t)nidx * MP_NDENS), dim3(256), 0, g.st, Ts, (size_t)nidx * MP_NDENS);
+  for (int q0 = 0; q0 < S.nq; q0 += qc) {
+    const int nqc = std::min(qc, S.nq - q0);
+    bem_prof_begin(BH_K_MP_BASIS, (double)nqc * nidx, g.st);
+    hipLaunchKernelGGL(k_mp_basis, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr, D.d_U, D.d_dU,
+                       q0, nqc, B, ldb, dens);
+    bem_prof_end(BH_K_MP_BASIS, g.st);
+    const int ns = (nqc + MP_QSPLIT - 1) / MP_QSPLIT;
+    bem_prof_begin(BH_K_MP_MOM, 8.0 * nidx * MP_NDENS * (double)nqc, g.st);
+    hipLaunchKernelGGL(k_mp_moments, dim3((nidx + 255) / 256, ns), dim3(256), 0, g.st, nidx, nqc, MP_QSPLIT, B, ldb,
+                       dens, T);
+    bem_prof_end(BH_K_MP_MOM, g.st);
+    const size_t nt = (size_t)nidx * MP_NDENS;
+    hipLaunchKernelGGL(k_mp_reduce, g1(nt), dim3(256), 0, g.st, (int)nt, ns, T, Ts);
+  }
+  hipLaunchKernelGGL(k_mp_coef, g1((size_t)nidx * 4), dim3(256), 0, g.st, S.L, S.Lp, S.kr, Ts, coef);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the scattered field of src at dst's node slots by direct sums, added into
+// dst's incident data; check: first verify that no slot lies inside src
+// (the geometry does not change during a solve: checked on the first sweep)
+int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double *d_part)
+{
+  if (check) {
+    if (domain_ids(src, dst->d_slotPt, dst->nslot, d_did)) return BH_EHIP;
+    std::vector<int> did(dst->nslot);
+    HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
+    HIPCHK(hipStreamSynchronize(g.st));
+    for (int v : did)
+      if (v != 0) return fail(BH_EDOM, "scattered-field domain error (overlapping objects), domain id %d", v);
+  }
+  Dom &S = src->dom[0];
+  bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, src->d_gq, S.d_sid, S.dNe,
+                     src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
+                     g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], src->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr,
+                     S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, src->nquad > 0, d_part, g.st);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" void bem_mo_release(void)
+{
+  if (g_ws) { delete[] g_ws; g_ws = nullptr; }
+}
+
+// 0: series where a pair allows it (the mixed-precision form), 1: the
+// direct element sums for every pair
+extern "C" int bh_mo_set_coupling(int mode)
+{
+  API_LOCK;
+  if (mode != 0 && mode != 1) return fail(BH_EARG, "coupling mode %d", mode);
+  g_coupling_mode = mode;
+  return 0;
+}
+
+// ordered object pairs of the last coupling loop summed by series and by
+// element sums, and the largest series order (-1: none)
+extern "C" int bh_mo_plan_stats(int *series, int *direct, int *lmax)
+{
+  API_LOCK;
+  if (series) *series = g_last_series;
+  if (direct) *direct = g_last_direct;
+  if (lmax) *lmax = g_last_lmax;
+  return 0;
+}
+
+// block-Jacobi fixed point over objects (mo_solve_bv, mo_solve.c:10-88).
+// trace (optional, iter_max+1 doubles) receives cc per iteration; *iters
+// receives the final iteration count.
+extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters)
+{
+  API_LOCK;
+  if (N <= 0) return fail(BH_EARG, "no objects");
+  if (ensure_rules()) return BH_EHIP;
+  for (int m = 0; m < N; m++)
+    if (!objs[m]->fmode || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d: factor and set beams first", m);
+  int rc, iter = 0;
+  if (N == 1) {
+    if ((rc = bh_object_solve(objs[0]))) return rc;
+  } else {
+    // many solves per object: form the Schur complements once
+    for (int m = 0; m < N; m++) if ((rc = form_s(objs[m]))) return rc;
+    Plan P;
+    make_plan(objs, N, P);
+    g_last_series = P.npairs_mp;
+    g_last_direct = P.npairs_direct;
+    g_last_lmax = -1;
+    for (int n = 0; n < N; n++) g_last_lmax = std::max(g_last_lmax, P.Lsrc[n]);
+    int maxslot = 0, totslot = 0;
+    std::vector<int> slot0(N);
+    for (int m = 0; m < N; m++) { maxslot = std::max(maxslot, objs[m]->nslot); slot0[m] = totslot; totslot += objs[m]->nslot; }
+    int *d_did = ws<int>(W_DID, maxslot);
+    double *d_part = ws<double>(W_PART, (size_t)maxslot * 19);   // far-triangle sums of the direct pass
+    double *d_ccd = ws<double>(W_CCD, 2 * (size_t)totslot);
+    double *d_ccs = ws<double>(W_CCDSUM, 2);
+    if (!d_did || !d_part || !d_ccd || !d_ccs) return fail(BH_EHIP, "coupling loop: work area");
+    // series coefficients of every source object, and the evaluation tiles
+    std::vector<size_t> coff(N, 0);
+    size_t ncoef = 0;
+    for (int n = 0; n < N; n++)
+      if (P.Lsrc[n] >= 0) { coff[n] = ncoef; ncoef += (size_t)mp_count(P.Lsrc[n] + 1) * 16; }
+    cd *d_coef = ncoef ? ws<cd>(W_COEF, ncoef) : nullptr;
+    std::vector<MpTile> tiles;
+    std::vector<MpPair> pairs;
+    for (int m = 0; m < N; m++) {
+      bh_object *o = objs[m];
+      const int p0 = (int)pairs.size();
+      for (int n = 0; n < N; n++) {
+        if (P.Lmn[m][n] < 0) continue;
+        MpPair q{};
+        q.coef = d_coef + coff[n];
+        for (int d = 0; d < 3; d++) q.c[d] = P.geo[n].c[d];
+        q.kr = objs[n]->kn[0].real();
+        q.Lp = P.Lsrc[n] + 1;
+        q.Le = P.Lmn[m][n] + 1;
+        pairs.push_back(q);
+      }
+      const int np = (int)pairs.size() - p0;
+      if (!np) continue;
+      for (int s0 = 0; s0 < o->nslot; s0 += 64) {
+        MpTile t{};
+        t.pts = o->d_slotPt; t.nrm = o->d_slotN; t.slot = o->d_slotIdx; t.Ui = o->d_Ui; t.dUi = o->d_dUi;
+        t.s0 = s0; t.ns = std::min(64, o->nslot - s0);
+        t.p0 = p0; t.np = np;
+        tiles.push_back(t);
+      }
+    }
+    MpTile *d_tiles = nullptr;
+    MpPair *d_pairs = nullptr;
+    if (!tiles.empty()) {
+      d_tiles = ws<MpTile>(W_TILES, tiles.size());
+      d_pairs = ws<MpPair>(W_PAIRS, pairs.size());
+      if (!d_coef || !d_tiles || !d_pairs) return fail(BH_EHIP, "coupling loop: series work area");
+      HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(MpTile), hipMemcpyHostToDevice, g.st));
+      HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(MpPair), hipMemcpyHostToDevice, g.st));
+    }
+    double evwork = 0.0;
+    for (const MpTile &t : tiles)
+      for (int i = 0; i < t.np; i++) evwork += (double)t.ns * mp_count(pairs[t.p0 + i].Le);
+    bool first = true;
+    auto sweep = [&]() -> int {
+      for (int m = 0; m < N; m++) {
+        bh_object *o = objs[m];
+        size_t n = (size_t)(o->Ne + 1) * 16;
+        hipLaunchKernelGGL(k_reset_incident, g1(n), dim3(256), 0, g.st, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, n);
+      }
+      // series pairs: coefficients of every source, then one evaluation launch
+      for (int n = 0; n < N; n++)
+        if (P.Lsrc[n] >= 0) { int r = mp_coefficients(objs[n], P, n, d_coef + coff[n]); if (r) return r; }
+      if (!tiles.empty()) {
+        bem_prof_begin(BH_K_MP_EVAL, evwork, g.st);
+        hipLaunchKernelGGL(k_mp_eval, dim3((unsigned)tiles.size()), dim3(64), 0, g.st, d_tiles, d_pairs);
+        bem_prof_end(BH_K_MP_EVAL, g.st);
+      }
+      // the other pairs: direct sums
+      for (int m = 0; m < N; m++)
+        for (int d = 0; d < N; d++)
+          if (d != m && P.Lmn[d][m] == -1) { int r = add_scattered(objs[d], objs[m], d_did, first, d_part); if (r) return r; }
+      HIPCHK(hipGetLastError());
+      first = false;
+      for (int m = 0; m < N; m++) { int r = bh_object_solve(objs[m]); if (r) return r; }
+      return 0;
+    };
+    auto change = [&]() -> double {
+      for (int m = 0; m < N; m++) {
+        bh_object *o = objs[m];
+        hipLaunchKernelGGL(k_ccd, g1(o->nslot), dim3(256), 0, g.st, o->d_Ui, o->d_Uip, o->d_slotIdx, o->nslot,
+                           d_ccd + 2 * (size_t)slot0[m]);
+      }
+      hipLaunchKernelGGL(k_ccd_sum, dim3(1), dim3(256), 0, g.st, d_ccd, totslot, d_ccs);
+      double h[2];
+      if (hipMemcpyAsync(h, d_ccs, sizeof h, hipMemcpyDeviceToHost, g.st) != hipSuccess ||
+          hipStreamSynchronize(g.st) != hipSuccess)
+        return -1.0;
+      return (h[0] > h[1] ? h[0] : h[1]) / (double)totslot;
+    };
+    const char *pe = getenv("BEM3_HIP_PROGRESS");
+    const bool progress = pe && *pe && *pe != '0';
+    if (progress) {
+      fprintf(stderr, "bh_mo_solve: %d object pairs by series, %d by direct sums\n", P.npairs_mp, P.npairs_direct);
+      fflush(stderr);
+    }
+    for (int m = 0; m < N; m++) if ((rc = bh_object_solve(objs[m]))) return rc;
+    if ((rc = sweep())) return rc;
+    // BEM3_HIP_PROGRESS=1: one line per iteration on stderr (long solves)
+    for (iter = 2; iter < iter_max; iter++) {
+      if ((rc = sweep())) return rc;
+      double cc = change();
+      if (cc < 0.0) return fail(BH_EHIP, "coupling loop: change measure");
+      if (trace) trace[iter] = cc;
+      if (progress) { fprintf(stderr, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(stderr); }
+      if (cc < ccd && iter > iter_min) break;
+    }
+  }
+  for (int m = 0; m < N; m++) if ((rc = bh_object_solve_EH(objs[m]))) return rc;
+  if (iters) *iters = iter;
+  return 0;
+}

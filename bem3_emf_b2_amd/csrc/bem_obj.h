@@ -1,0 +1,180 @@
+// bem_obj.h -- internals shared by the host-side sources of the HIP library:
+// the device context, the object record that lives in HBM for the object's
+// whole life, grow-only work areas and the error / locking conventions of
+// the C-ABI (include/bem3_hip.h).  Everything but the object record sits in
+// namespace bem, so the shared object exports only the bh_* entry points
+// under C names.
+#pragma once
+#include "bem_dev.h"
+#include "bem_schur.h"
+#include "../../include/bem3_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+typedef std::complex<double> zd;
+
+namespace bem {
+
+// the message of the last failure on the calling thread; returns code
+int fail(int code, const char *fmt, ...);
+inline void dfree(void *p) { (void)hipFree(p); }
+
+// Entry points are serialised: callers may call the scalar field functions
+// from their own thread pools (e.g. OpenMP loops over grid points); all work
+// goes to one device stream in call order.
+extern std::recursive_mutex g_mu;
+
+struct Ctx {
+  int dev = -1;
+  hipStream_t st = nullptr;
+  QPt *d_gl = nullptr, *d_gh = nullptr;   // 3-part triangle rules
+  int n_gl = 0, n_gh = 0;
+  void *d_gl32 = nullptr;                 // FP32 far-pair rules, 12x12 / 6x6 / 5x5 / 4x4 (QPt32)
+  QPt *d_glr = nullptr;                   // the same rules in FP64 (QPt): far pairs of the mixed-form assembly
+  double *d_xl = nullptr, *d_wl = nullptr, *d_xh = nullptr, *d_wh = nullptr;
+  double xl[12], wl[12], xh[64], wh[64];
+  hipEvent_t ev[2];
+};
+extern Ctx g;
+
+// a grow-only device buffer
+struct Scratch {
+  void *p = nullptr;
+  size_t n = 0;
+  Scratch() = default;
+  Scratch(const Scratch &) = delete;
+  Scratch &operator=(const Scratch &) = delete;
+  ~Scratch() { dfree(p); }
+  void release() { dfree(p); p = nullptr; n = 0; }
+  template <class T> T *get(size_t cnt)
+  {
+    size_t b = cnt * sizeof(T);
+    if (b > n) { dfree(p); p = nullptr; if (hipMalloc(&p, b) != hipSuccess) { n = 0; return nullptr; } n = b; }
+    return (T *)p;
+  }
+};
+
+
+inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// rules of the field kernels: 0 4-point, 1 7-point, 2 GLN, 3 GHN
+extern QPtN *g_rule[4];
+extern int g_nrule[4];
+extern double g_rscale[4];
+extern QPtQ *g_qrule[4];
+extern int g_nqrule[4];
+int ensure_rules();
+
+struct Dom {
+  int dNe = 0, nd = 0;                 // elements seen, compact node count
+  std::vector<int> sid, colOff;        // [dNe+1]
+  std::vector<RowPt> rows;             // [nd]
+  int *d_sid = nullptr, *d_colOff = nullptr;
+  RowPt *d_rows = nullptr;
+  cd *d_G = nullptr, *d_H = nullptr, *d_dG = nullptr, *d_dH = nullptr;
+  double *d_dF = nullptr;
+  int n_gh = 0;
+  // solve-side tables and boundary values ([dNe+1][4][4] U/dU, [dNe+1][4][3] E/H)
+  ColInfo *d_col = nullptr;
+  int *d_colS = nullptr;
+  RowEH *d_rowEH = nullptr;
+  cd *d_U = nullptr, *d_dU = nullptr, *d_E = nullptr, *d_dE = nullptr, *d_Hf = nullptr, *d_dHf = nullptr;
+  cd *d_Gf = nullptr;                  // LU factors of G (normal derivatives of E, H)
+  int *d_Gpiv = nullptr;
+  bool gf_valid = false;               // d_Gf factors the current G (cleared by assembly)
+  cf *c_Gf = nullptr;                  // complex-FP32 LU factors of G (mixed-precision form)
+  bool gfc_valid = false;
+  // face table for the solid-angle domain test (faces whose main side is d)
+};
+
+
+}  // namespace bem
+
+#define API_LOCK std::lock_guard<std::recursive_mutex> api_lock_(bem::g_mu)
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return bem::fail(BH_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace bem {
+// a device copy of n host values (on the library's stream)
+template <class T> int upload(T **d, const T *h, size_t n)
+{
+  HIPCHK(hipMalloc((void **)d, std::max<size_t>(n, 1) * sizeof(T)));
+  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, g.st));
+  return 0;
+}
+}  // namespace bem
+
+struct bh_object {
+  int Nn, Ne, MN, NN;
+  int nquad = 0;                       // quadrangle elements
+  std::vector<double> rn;              // [Nn+1][3]
+  std::vector<int> ed, md, sd, eni;    // [Ne+1][4], [Ne+1]
+  std::vector<double> cr, cw, ren, wen;// [Ne+1][3][4], [Ne+1][3][3], [Ne+1][4][3] x2
+  std::vector<zd> n, kn;
+  std::vector<bem::Dom> dom;
+  TriGeo *d_geo = nullptr;
+  QuadExt *d_gq = nullptr;             // bi-linear terms of quadrangles (zero rows for triangles)
+  bool all_tri = true;
+  // incident data at every element slot [Ne+1][4][4]
+  cd *d_Ui = nullptr, *d_dUi = nullptr, *d_Ui0 = nullptr, *d_dUi0 = nullptr, *d_Uip = nullptr;
+  // open-domain node slots (targets of the incident and coupling fields)
+  int nslot = 0;
+  double *d_slotPt = nullptr, *d_slotN = nullptr;
+  int *d_slotIdx = nullptr;
+  // faces grouped by main domain for the domain test
+  double *d_fv = nullptr;
+  int *d_fsd = nullptr, *d_fq = nullptr, *d_seg = nullptr;
+  // beams
+  std::vector<Beam> beams;             // plane waves
+  std::vector<FBeam> fbeams;           // the other multi_fbeam types
+  FBeam *d_fb = nullptr;
+  double *d_fbt = nullptr;             // their spectral angles (cos, sin, weight x sin)
+  Beam *d_beams = nullptr;
+  double lambda0 = 0.0;
+  int prec = BH_PREC_FP64;             // arithmetic of assembly, field sums and factor mode 0
+  // factorisation: 0 none, 1 dense 8NN, 2 Schur (one medium), 3 Schur in FP32 refined in FP64
+  int fmode = 0;                       // factorisation of the current matrices (cleared by assembly)
+  cd *d_A = nullptr, *d_work = nullptr;
+  int *d_piv = nullptr, *d_info = nullptr;
+  cd *d_G0f = nullptr, *d_Z = nullptr, *d_Mf = nullptr, *d_Sf = nullptr;
+  int *d_piv0 = nullptr, *d_pivM = nullptr, *d_pivS = nullptr;
+  cd sch_delt, sch_beps, sch_sigm;
+  bool s_formed = false;               // Sf holds LU(S) (else Sp = H1 + beta G1 Z, S solved by GMRES)
+  bool x_warm = false;                 // d_X solves the current factors' system for an earlier B
+                                       // (the refinement of the mixed form starts from it)
+  int gm_steps = 0;                    // GMRES steps of the last S solve(s)
+  cd *d_X = nullptr, *d_B = nullptr, *d_tmp = nullptr;
+  cd *d_w = nullptr;                   // work of the block solve (schur_work(NN))
+  // mixed-precision form (fmode 3): complex-FP32 factors of the block
+  // elimination, refined against the FP64 matrices
+  cf *c_G0f = nullptr, *c_Z = nullptr, *c_Mf = nullptr, *c_Sf = nullptr, *c_G1 = nullptr;
+  cf *c_w = nullptr;                   // schur_work(NN) work, 8 NN right-hand side, 8 NN correction
+  cd *d_R = nullptr;                   // 8 NN residual
+  double *d_nrm = nullptr;             // 256 partial sums of the norm
+  int ir_iters = 0;                    // refinement steps of the last solve
+  double ir_res = 0.0;                 // its final relative residual
+  int asm_prec = -1;                   // precision the current matrices were assembled in (-1: none)
+  // grow-only device work areas of the entry points (freed with the object)
+  bem::Scratch ws_part, ws_mask, ws_pairs, ws_start, ws_ghdf, ws_self;   // assembly
+  bem::Scratch ws_wt;                  // forming S
+  bem::Scratch ws_eh[5];               // boundary E, H: packed rows, products, refinement
+};
+
+namespace bem {
+int prepare_solve(bh_object *o);
+int form_s(bh_object *o);
+int domain_ids(bh_object *o, const double *d_pts, int np, int *d_did);
+int trsv_failed(hipStream_t st);
+}  // namespace bem

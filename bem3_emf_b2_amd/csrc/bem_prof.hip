@@ -36,6 +36,17 @@ hipEvent_t take()
 
 extern "C" {
 
+// destroys the event pool (library shutdown)
+void bem_prof_release(void)
+{
+  std::lock_guard<std::mutex> lk(M);
+  for (hipEvent_t e : P.pool) (void)hipEventDestroy(e);
+  P.pool.clear();
+  P.recs.clear();
+  P.used = 0;
+  P.on = false;
+}
+
 void bem_prof_begin(int kid, double work, hipStream_t st)
 {
   if (!P.on || kid < 0 || kid >= BH_K_COUNT) return;

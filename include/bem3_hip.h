@@ -35,6 +35,11 @@ const char *bh_last_error(void);
 
 /* select the device and build the quadrature tables (called implicitly) */
 int bh_init(int device);
+/* Releases the library's device state (rule tables, work areas, timing
+ * events, its stream) after the caller destroyed every object; the next
+ * entry point sets the library up again.  For programs that must not leave
+ * device frees to process exit (no counterpart in the reference).          */
+int bh_finalize(void);
 int bh_sync(void);
 void *bh_stream(void);
 
@@ -126,6 +131,17 @@ int bh_object_incident(bh_object *o, int which, double *buf, int set);
  * environment variable BEM3_HIP_PROGRESS=1 each iteration also prints
  * its change on stderr.                                                  */
 int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters);
+/* How the coupling loop forms the field of one object at another's node
+ * slots (no counterpart in the reference, which always sums every element
+ * pair): 0 (default) a multipole series of the source object where both
+ * objects are in the mixed-precision form, the source has triangles only,
+ * the surrounding medium is lossless and the slots keep a distance (source
+ * sphere radius / slot distance <= 0.6, truncation about 1e-10 relative),
+ * the reference's element sums elsewhere; 1 the element sums everywhere.  */
+int bh_mo_set_coupling(int mode);
+/* ordered object pairs of the last coupling loop of several objects summed
+ * by the series and by element sums, and the largest series order used   */
+int bh_mo_plan_stats(int *series, int *direct, int *lmax);
 /* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i, mo_EH_mEMP_s/t/i;
  * mo_field.c:11-282): what 0: E,H (6 complex per point), 1: U (4 complex),
  * 2: E,H from the potentials' gradients (EH_mEMP_s, d3b1_field.c:68-138,
@@ -171,7 +187,13 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
  * BH_K_CGEMM, BH_K_CGEMM_SMALL, BH_K_LU_PANEL_C, BH_K_TRSM_C,
  * BH_K_FIELD_BD32, BH_K_SCATTER32: the FP32 kernels of the mixed-precision
  * form (same work units); BH_K_ASM_GLR: its assembly pass (FP64, reduced
- * far-pair rules).  BH_K_ZGEMV, BH_K_CGEMV: the few-right-hand-side
+ * far-pair rules).  BH_K_SCATTER_FAR, BH_K_SCATTER_FAR32: the far-triangle
+ * pass of the direct coupling sums ((slot, source element) pairs; the
+ * second pass counts under BH_K_SCATTER, BH_K_SCATTER32 with zero work).
+ * BH_K_MP_BASIS: the coupling series' basis at the source points
+ * ((point, coefficient) pairs); BH_K_MP_MOM: its moments (real flops);
+ * BH_K_MP_EVAL: its evaluation at the node slots ((slot, coefficient)
+ * pairs).  BH_K_ZGEMV, BH_K_CGEMV: the few-right-hand-side
  * (n <= 8, k >= 256) product kernel, real flops (8 m n k).               */
 #define BH_K_ASM_GL   0
 #define BH_K_ASM_GH   1
@@ -192,7 +214,12 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
 #define BH_K_ZGEMV    16
 #define BH_K_CGEMV    17
 #define BH_K_SCATTER32 18
-#define BH_K_COUNT    19
+#define BH_K_SCATTER_FAR 19
+#define BH_K_SCATTER_FAR32 20
+#define BH_K_MP_BASIS 21
+#define BH_K_MP_MOM   22
+#define BH_K_MP_EVAL  23
+#define BH_K_COUNT    24
 int bh_prof_enable(int on);
 int bh_prof_read(int kid, double *ms, double *work, int *launches);
 
