@@ -69,6 +69,7 @@ def lib():
             "bh_finalize": (C.c_int, []),
             "bh_mo_set_coupling": (C.c_int, [C.c_int]),
             "bh_mo_plan_stats": (C.c_int, [Ip, Ip, Ip]),
+            "bh_mo_prepare": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -328,6 +329,12 @@ def zgemm(A, B, Cm=None, alpha=1.0, beta=0.0):
 
 def _handles(objs):
     return (C.c_void_p * len(objs))(*[o.h.value for o in objs])
+
+
+def mo_prepare(objs, type_=2, mode="auto"):
+    """assembly and factorisation of every object, the objects spread over
+    several device streams"""
+    check(lib().bh_mo_prepare(_handles(objs), len(objs), type_, FACTOR_MODES[mode]))
 
 
 def mo_solve(objs, ccd=1e-6, iter_min=10, iter_max=100):

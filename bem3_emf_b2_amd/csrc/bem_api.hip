@@ -57,6 +57,7 @@ int fail(int code, const char *fmt, ...)
 }
 std::recursive_mutex g_mu;
 Ctx g;
+thread_local hipStream_t t_cur_st = nullptr;
 // a one-launch triangular solve whose wait timed out since the last check
 // (reads and clears the flag)
 int trsv_failed(hipStream_t st)
@@ -139,7 +140,7 @@ extern "C" int bh_init(int device)
   if (upload(&g.d_gl, a.data(), a.size()) || upload(&g.d_gh, b.data(), b.size()) || upload(&g.d_xl, g.xl, 12) ||
       upload(&g.d_wl, g.wl, 12) || upload(&g.d_xh, g.xh, 64) || upload(&g.d_wh, g.wh, 64))
     return BH_EHIP;
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   {
     // far-pair table of the mixed form: 12x12, 6x6, 5x5, 4x4
     double x6[6], w6[6], x5[5], w5[5], x4[4], w4[4];
@@ -153,7 +154,7 @@ extern "C" int bh_init(int device)
     if ((int)a.size() != BEM_FAR_OFF6 || (int)r32.size() != BEM_FAR_NPT || bem_rule32(r32.data(), BEM_FAR_NPT, &g.d_gl32))
       return fail(BH_EHIP, "FP32 rule table");
     if (upload(&g.d_glr, r32.data(), r32.size())) return BH_EHIP;
-    HIPCHK(hipStreamSynchronize(g.st));
+    HIPCHK(hipStreamSynchronize(cur_st()));
   }
   g.dev = device;
   return 0;
@@ -163,7 +164,7 @@ extern "C" int bh_sync(void)
 {
   API_LOCK;
   if (g.dev < 0) return 0;
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   return 0;
 }
 
@@ -354,7 +355,7 @@ extern "C" bh_object *bh_object_create(int Nn, const double *rn, int Ne, const i
   }
   elem_const(o);
   if (build_domains(o) || upload_geo(o)) { delete o; return nullptr; }
-  if (hipStreamSynchronize(g.st) != hipSuccess) { fail(BH_EHIP, "upload failed"); delete o; return nullptr; }
+  if (hipStreamSynchronize(cur_st()) != hipSuccess) { fail(BH_EHIP, "upload failed"); delete o; return nullptr; }
   return o;
 }
 
@@ -362,7 +363,7 @@ extern "C" void bh_object_destroy(bh_object *o)
 {
   API_LOCK;
   if (!o) return;
-  (void)hipStreamSynchronize(g.st);   // no kernel still uses the object's memory
+  (void)hipStreamSynchronize(cur_st());   // no kernel still uses the object's memory
   for (Dom &D : o->dom) {
     dfree(D.d_sid); dfree(D.d_colOff); dfree(D.d_rows);
     dfree(D.d_G); dfree(D.d_H); dfree(D.d_dG); dfree(D.d_dH); dfree(D.d_dF);
@@ -432,15 +433,15 @@ static int assemble_domain(bh_object *o, int d)
   const bool mixed = o->prec == BH_PREC_MIXED;
   if (mixed)
     bem_launch_asm_glr(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_glr, g_qrule[2],
-                       g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+                       g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, cur_st());
   else
     bem_launch_asm_gl(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, g_qrule[2],
-                      g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, g.st);
+                      g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, cur_st());
   HIPCHK(hipGetLastError());
   // near pairs in row-major order (deterministic)
   std::vector<unsigned long long> mask(nd * nwv);
-  HIPCHK(hipMemcpyAsync(mask.data(), d_mask, mask.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(mask.data(), d_mask, mask.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   std::vector<GhPair> pairs;
   std::vector<int> start(nd + 1, 0);
   for (size_t r = 0; r < nd; r++) {
@@ -461,22 +462,21 @@ static int assemble_domain(bh_object *o, int d)
   double *d_ghdf = o->ws_ghdf.get<double>(std::max<size_t>(pairs.size(), 1) * 3);
   double *d_self = o->ws_self.get<double>(nd * 3);   // the self element's F terms per row
   if (!d_pairs || !d_start || !d_ghdf || !d_self) return fail(BH_EHIP, "assembly: work area of the near pairs");
-  if (!pairs.empty()) HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(GhPair), hipMemcpyHostToDevice, g.st));
-  HIPCHK(hipMemcpyAsync(d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, g.st));
+  if (!pairs.empty()) HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(GhPair), hipMemcpyHostToDevice, cur_st()));
+  HIPCHK(hipMemcpyAsync(d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, cur_st()));
   bem_launch_asm_gh(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh,
                     mixed ? (const void *)g.d_gl : nullptr, g.n_gl, g_qrule[3], g_nqrule[3], g_qrule[2], g_nqrule[2],
-                    D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_ghdf, g.st);
+                    D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_ghdf, cur_st());
   bem_launch_asm_self(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, (int)nd, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh,
-                      D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_self, g.st);
-  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, d_self, (int)nd, d == 0, D.d_H, nd, D.d_dF, g.st);
+                      D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_self, cur_st());
+  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, d_self, (int)nd, d == 0, D.d_H, nd, D.d_dF, cur_st());
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(g.st));   // the host tables above stay alive until here
+  HIPCHK(hipStreamSynchronize(cur_st()));   // the host tables above stay alive until here
   return 0;
 }
 
-extern "C" int bh_object_assemble(bh_object *o, int type)
+int bem::object_assemble(bh_object *o, int type)
 {
-  API_LOCK;
   if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
   {
     int rc = ensure_rules();   // the square's rules for quadrangles
@@ -493,6 +493,12 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
   }
   o->asm_prec = o->prec;
   return 0;
+}
+
+extern "C" int bh_object_assemble(bh_object *o, int type)
+{
+  API_LOCK;
+  return object_assemble(o, type);
 }
 
 extern "C" int bh_object_set_precision(bh_object *o, int prec)
@@ -541,8 +547,8 @@ extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
   HIPCHK(hipMalloc((void **)&info, sizeof(int)));
   HIPCHK(hipMemcpy(dA, A, (size_t)n * n * sizeof(cd), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dB, B, (size_t)n * nrhs * sizeof(cd), hipMemcpyHostToDevice));
-  if (bem_zgetrf(n, dA, n, piv, work, info, 64, g.st)) return lu_failed();
-  bem_zgetrs(n, nrhs, dA, n, piv, dB, nrhs, 64, g.st);
+  if (bem_zgetrf(n, dA, n, piv, work, info, 64, cur_st())) return lu_failed();
+  bem_zgetrs(n, nrhs, dA, n, piv, dB, nrhs, 64, cur_st());
   HIPCHK(hipGetLastError());
   int hinfo = 0;
   HIPCHK(hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
@@ -550,7 +556,7 @@ extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
   HIPCHK(hipMemcpy(A, dA, (size_t)n * n * sizeof(cd), hipMemcpyDeviceToHost));
   dfree(dA); dfree(dB); dfree(work); dfree(piv); dfree(info);
   if (hinfo) return fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo);
-  return trsv_failed(g.st);
+  return trsv_failed(cur_st());
 }
 
 extern "C" int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B,
@@ -565,7 +571,7 @@ extern "C" int bh_zgemm_host(int m, int n, int k, const double *alpha, const dou
   HIPCHK(hipMemcpy(dA, A, (size_t)m * k * sizeof(cd), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dB, B, (size_t)k * n * sizeof(cd), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dC, C, (size_t)m * n * sizeof(cd), hipMemcpyHostToDevice));
-  bem_zgemm(m, n, k, alpha[0], alpha[1], dA, k, dB, n, beta[0], beta[1], dC, n, g.st);
+  bem_zgemm(m, n, k, alpha[0], alpha[1], dA, k, dB, n, beta[0], beta[1], dC, n, cur_st());
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(C, dC, (size_t)m * n * sizeof(cd), hipMemcpyDeviceToHost));
   dfree(dA); dfree(dB); dfree(dC);
@@ -585,8 +591,8 @@ extern "C" int bh_cgesv(int n, int nrhs, float *A, float *B)
   HIPCHK(hipMalloc((void **)&info, sizeof(int)));
   HIPCHK(hipMemcpy(dA, A, (size_t)n * n * sizeof(cf), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dB, B, (size_t)n * nrhs * sizeof(cf), hipMemcpyHostToDevice));
-  if (bem_cgetrf(n, dA, n, piv, info, g.st)) return lu_failed();
-  bem_cgetrs(n, nrhs, dA, n, piv, dB, nrhs, g.st);
+  if (bem_cgetrf(n, dA, n, piv, info, cur_st())) return lu_failed();
+  bem_cgetrs(n, nrhs, dA, n, piv, dB, nrhs, cur_st());
   HIPCHK(hipGetLastError());
   int hinfo = 0;
   HIPCHK(hipMemcpy(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost));
@@ -594,7 +600,7 @@ extern "C" int bh_cgesv(int n, int nrhs, float *A, float *B)
   HIPCHK(hipMemcpy(A, dA, (size_t)n * n * sizeof(cf), hipMemcpyDeviceToHost));
   dfree(dA); dfree(dB); dfree(piv); dfree(info);
   if (hinfo) return fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo);
-  return trsv_failed(g.st);
+  return trsv_failed(cur_st());
 }
 
 extern "C" int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const float *B, const float *beta,
@@ -609,7 +615,7 @@ extern "C" int bh_cgemm_host(int m, int n, int k, const float *alpha, const floa
   HIPCHK(hipMemcpy(dA, A, (size_t)m * k * sizeof(cf), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dB, B, (size_t)k * n * sizeof(cf), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dC, C, (size_t)m * n * sizeof(cf), hipMemcpyHostToDevice));
-  bem_cgemm(m, n, k, alpha[0], alpha[1], dA, k, dB, n, beta[0], beta[1], dC, n, g.st);
+  bem_cgemm(m, n, k, alpha[0], alpha[1], dA, k, dB, n, beta[0], beta[1], dC, n, cur_st());
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(C, dC, (size_t)m * n * sizeof(cf), hipMemcpyDeviceToHost));
   dfree(dA); dfree(dB); dfree(dC);
@@ -727,7 +733,7 @@ int bem::ensure_rules()
 template <class T> static int dalloc(T **p, size_t n)
 {
   HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
-  HIPCHK(hipMemsetAsync(*p, 0, std::max<size_t>(n, 1) * sizeof(T), g.st));
+  HIPCHK(hipMemsetAsync(*p, 0, std::max<size_t>(n, 1) * sizeof(T), cur_st()));
   return 0;
 }
 
@@ -817,7 +823,7 @@ int bem::prepare_solve(bh_object *o)
       upload(&o->d_fsd, fsd.data(), fsd.size()) || upload(&o->d_fq, fq.data(), fq.size()) ||
       upload(&o->d_seg, seg.data(), seg.size()))
     return BH_EHIP;
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   return 0;
 }
 
@@ -872,10 +878,10 @@ static int fbeam_failed()
 {
   if (!g_fberr) return 0;
   int h = 0;
-  HIPCHK(hipMemcpyAsync(&h, g_fberr, sizeof(int), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(&h, g_fberr, sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   if (!h) return 0;
-  HIPCHK(hipMemsetAsync(g_fberr, 0, sizeof(int), g.st));
+  HIPCHK(hipMemsetAsync(g_fberr, 0, sizeof(int), cur_st()));
   return fail(BH_ECONV, "phi integration limit over (a beam's phi integral did not converge in 2^13 intervals)");
 }
 
@@ -883,10 +889,10 @@ static int fbeam_failed()
 // that take it; what 0: E, H rows of 6, 1: E into rows of 4)
 static void add_incident(const bh_object *o, const double *d_pts, const int *d_mask, int np, cd *d_out, int what)
 {
-  bem_launch_add_incident(d_pts, d_mask, np, o->d_beams, (int)o->beams.size(), d_out, what, g.st);
+  bem_launch_add_incident(d_pts, d_mask, np, o->d_beams, (int)o->beams.size(), d_out, what, cur_st());
   if (!o->fbeams.empty())
     bem_launch_fbeam(d_pts, nullptr, d_mask, np, o->d_fb, (int)o->fbeams.size(), o->d_fbt, g_phit, g_fberr, what,
-                     d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, g.st);
+                     d_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, cur_st());
 }
 
 // a beam set in place: device copies, then the incident boundary data of the
@@ -906,10 +912,10 @@ static int install_beams(bh_object *o, const std::vector<Beam> &bm, const std::v
     return BH_EHIP;
   if (prepare_solve(o)) return BH_EHIP;
   bem_launch_incident_bd(o->d_slotPt, o->d_slotN, o->d_slotIdx, o->nslot, o->d_beams, (int)bm.size(), o->d_Ui, o->d_dUi,
-                         o->d_Ui0, o->d_dUi0, o->d_Uip, g.st);
+                         o->d_Ui0, o->d_dUi0, o->d_Uip, cur_st());
   if (!fb.empty())
     bem_launch_fbeam(o->d_slotPt, o->d_slotN, nullptr, o->nslot, o->d_fb, (int)fb.size(), o->d_fbt, g_phit, g_fberr, 2,
-                     nullptr, o->d_slotIdx, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, g.st);
+                     nullptr, o->d_slotIdx, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, cur_st());
   HIPCHK(hipGetLastError());
   return fbeam_failed();
 }
@@ -1029,17 +1035,18 @@ int bem::form_s(bh_object *o)
   if (o->fmode == 2) {
     cd *Wt = o->ws_wt.get<cd>(n2);
     if (!Wt) return fail(BH_EHIP, "Schur complement: work area of %zu entries", n2);
-    rc = schur_form_s(schur64(o), o->d_Sf, o->d_pivS, Wt, o->d_info, g.st);
-    (void)hipStreamSynchronize(g.st);
+    rc = schur_form_s(schur64(o), o->d_Sf, o->d_pivS, Wt, o->d_info, cur_st());
+    (void)hipStreamSynchronize(cur_st());
   } else {
     cf *Wt = o->ws_wt.get<cf>(n2);
     if (!Wt) return fail(BH_EHIP, "Schur complement: work area of %zu entries", n2);
-    rc = schur_form_s(schur32(o), o->c_Sf, o->d_pivS, Wt, o->d_info, g.st);
-    (void)hipStreamSynchronize(g.st);
+    rc = schur_form_s(schur32(o), o->c_Sf, o->d_pivS, Wt, o->d_info, cur_st());
+    (void)hipStreamSynchronize(cur_st());
   }
   if (rc) return lu_failed();
   int info = 0;
-  HIPCHK(hipMemcpy(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   if (info) return fail(BH_ESING, "singular Schur complement (zero pivot %d)", info);
   o->s_formed = true;
   return 0;
@@ -1047,9 +1054,8 @@ int bem::form_s(bh_object *o)
 
 // mode 0: auto (Schur for one medium, dense otherwise), 1: dense 8NN,
 // 2: Schur, 3: Schur in complex FP32 refined in FP64 (mixed precision)
-extern "C" int bh_object_factor(bh_object *o, int mode)
+int bem::object_factor(bh_object *o, int mode)
 {
-  API_LOCK;
   if (!o->dom[0].d_G || o->asm_prec < 0) return fail(BH_ESTATE, "assemble before factor");
   if (o->asm_prec != o->prec) return fail(BH_ESTATE, "the precision changed since the assembly: assemble again");
   if (mode < 0 || mode > 3) return fail(BH_EARG, "factor mode %d", mode);
@@ -1064,12 +1070,12 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
   int info = 0;
   if (mode == 1) {
     if (!o->d_A) { HIPCHK(hipMalloc((void **)&o->d_A, na * na * sizeof(cd))); HIPCHK(hipMalloc((void **)&o->d_piv, na * sizeof(int))); }
-    HIPCHK(hipMemsetAsync(o->d_A, 0, na * na * sizeof(cd), g.st));
+    HIPCHK(hipMemsetAsync(o->d_A, 0, na * na * sizeof(cd), cur_st()));
     for (int d = 0; d <= o->MN; d++) {
       Dom &D = o->dom[d];
-      bem_launch_build_A(D.d_G, D.d_H, D.nd, D.nd, D.d_col, row_base(o, d), NN, o->d_A, na, g.st);
+      bem_launch_build_A(D.d_G, D.d_H, D.nd, D.nd, D.d_col, row_base(o, d), NN, o->d_A, na, cur_st());
     }
-    if (bem_zgetrf((int)na, o->d_A, na, o->d_piv, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+    if (bem_zgetrf((int)na, o->d_A, na, o->d_piv, o->d_work, o->d_info, 64, cur_st())) return lu_failed();
   } else {
     // one medium: domain 0 rows H0 u - G0 q = 0, domain 1 rows couple through phi
     Dom &D0 = o->dom[0], &D1 = o->dom[1];
@@ -1089,32 +1095,38 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
                         dalloc(&o->c_G1, n2) || dalloc(&o->c_w, schur_work(NN) + 16 * NN)))
         return BH_EHIP;
       if (schur_factor_c(n, NN, D0.d_G, D0.d_H, D1.d_G, D1.d_H, o->sch_beps, o->c_G0f, o->c_Z, o->c_Mf, o->c_Sf,
-                         o->c_G1, o->d_piv0, o->d_pivM, o->d_info, g.st))
+                         o->c_G1, o->d_piv0, o->d_pivM, o->d_info, cur_st()))
         return lu_failed();
     } else {
       if (!o->d_G0f && (dalloc(&o->d_G0f, n2) || dalloc(&o->d_Z, n2) || dalloc(&o->d_Mf, n2) || dalloc(&o->d_Sf, n2)))
         return BH_EHIP;
       // Z = G0^{-1} H0
-      HIPCHK(hipMemcpyAsync(o->d_G0f, D0.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-      HIPCHK(hipMemcpyAsync(o->d_Z, D0.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-      if (bem_zgetrf(n, o->d_G0f, NN, o->d_piv0, o->d_work, o->d_info, 64, g.st)) return lu_failed();
-      bem_zgetrs(n, n, o->d_G0f, NN, o->d_piv0, o->d_Z, NN, 64, g.st);
+      HIPCHK(hipMemcpyAsync(o->d_G0f, D0.d_G, n2 * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+      HIPCHK(hipMemcpyAsync(o->d_Z, D0.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+      if (bem_zgetrf(n, o->d_G0f, NN, o->d_piv0, o->d_work, o->d_info, 64, cur_st())) return lu_failed();
+      bem_zgetrs(n, n, o->d_G0f, NN, o->d_piv0, o->d_Z, NN, 64, cur_st());
       // M = H1 + G1 Z  (Mf), P = G1 Z kept in Sf for the beta term
-      HIPCHK(hipMemcpyAsync(o->d_Mf, D1.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-      bem_zgemm(n, n, n, 1.0, 0.0, D1.d_G, NN, o->d_Z, NN, 0.0, 0.0, o->d_Sf, NN, g.st);
-      bem_launch_axpby(o->d_Mf, 1.0, 0.0, o->d_Sf, o->d_Mf, n2, g.st);
+      HIPCHK(hipMemcpyAsync(o->d_Mf, D1.d_H, n2 * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+      bem_zgemm(n, n, n, 1.0, 0.0, D1.d_G, NN, o->d_Z, NN, 0.0, 0.0, o->d_Sf, NN, cur_st());
+      bem_launch_axpby(o->d_Mf, 1.0, 0.0, o->d_Sf, o->d_Mf, n2, cur_st());
       // S starts as H1 + beta P
-      bem_launch_axpby(D1.d_H, beps.real(), beps.imag(), o->d_Sf, o->d_Sf, n2, g.st);
-      if (bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+      bem_launch_axpby(D1.d_H, beps.real(), beps.imag(), o->d_Sf, o->d_Sf, n2, cur_st());
+      if (bem_zgetrf(n, o->d_Mf, NN, o->d_pivM, o->d_work, o->d_info, 64, cur_st())) return lu_failed();
       // S itself is formed on demand (form_s): a single solve takes GMRES on Sp
     }
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   if (info) return fail(BH_ESING, "singular system (zero pivot %d)", info);
   o->fmode = mode;
   return 0;
+}
+
+extern "C" int bh_object_factor(bh_object *o, int mode)
+{
+  API_LOCK;
+  return object_factor(o, mode);
 }
 
 
@@ -1124,43 +1136,43 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
 #define IR_MAX 30
 
 // B from the incident data, X = A^{-1} B, boundary values (solve_coefficient_mEMP)
-static int object_solve(bh_object *o)
+static int object_solve_core(bh_object *o)
 {
   if (!o->fmode) return fail(BH_ESTATE, "factor before solve");
   if (!o->d_beams) return fail(BH_ESTATE, "set beams before solve");
   size_t NN = (size_t)o->NN, na = 8 * NN;
-  HIPCHK(hipMemsetAsync(o->d_B, 0, na * sizeof(cd), g.st));
+  HIPCHK(hipMemsetAsync(o->d_B, 0, na * sizeof(cd), cur_st()));
   for (int d = 0; d <= o->MN; d++) {
     Dom &D = o->dom[d];
-    bem_launch_build_B(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, row_base(o, d), o->d_B, g.st);
+    bem_launch_build_B(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, row_base(o, d), o->d_B, cur_st());
   }
   o->ir_iters = 0;
   o->ir_res = 0.0;
   if (o->fmode == 1) {
-    HIPCHK(hipMemcpyAsync(o->d_X, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    bem_zgetrs((int)na, 1, o->d_A, na, o->d_piv, o->d_X, 1, 64, g.st);
+    HIPCHK(hipMemcpyAsync(o->d_X, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+    bem_zgetrs((int)na, 1, o->d_A, na, o->d_piv, o->d_X, 1, 64, cur_st());
   } else if (o->fmode == 2) {
     // the incident data enter the domain-1 rows only
-    int rc = schur_apply(schur64(o), o->d_B, false, o->d_X, o->d_w, g.st, &o->gm_steps);
+    int rc = schur_apply(schur64(o), o->d_B, false, o->d_X, o->d_w, cur_st(), &o->gm_steps);
     if (rc) return fail(rc, "GMRES on the Schur complement did not converge");
   } else {
     // complex-FP32 solves of the residual equation, residuals in FP64; a
     // repeated solve (the coupling loop) starts from the previous solution,
     // whose residual for the new B is usually a small fraction of B
-    const double nb = bem_nrm2(o->d_B, na, o->d_nrm, g.st);
+    const double nb = bem_nrm2(o->d_B, na, o->d_nrm, cur_st());
     if (nb < 0.0) return fail(BH_EHIP, "norm of the right-hand side");
     double res = nb > 0.0 ? 1.0 : 0.0;
     if (o->x_warm && nb > 0.0) {
       for (int d = 0; d <= o->MN; d++) {
         Dom &D = o->dom[d];
-        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, g.st);
+        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, cur_st());
       }
-      const double nr = bem_nrm2(o->d_R, na, o->d_nrm, g.st);
+      const double nr = bem_nrm2(o->d_R, na, o->d_nrm, cur_st());
       if (nr < 0.0) return fail(BH_EHIP, "norm of the residual");
       res = nr / nb;
     } else {
-      HIPCHK(hipMemsetAsync(o->d_X, 0, na * sizeof(cd), g.st));
-      HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
+      HIPCHK(hipMemsetAsync(o->d_X, 0, na * sizeof(cd), cur_st()));
+      HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
     }
     o->x_warm = false;
     cf *cw = o->c_w, *crhs = cw + schur_work(NN), *cdx = crhs + 8 * NN;
@@ -1168,16 +1180,16 @@ static int object_solve(bh_object *o)
     const SchurF<cf> f = schur32(o);
     int it = 0;
     while (res > IR_TOL && it < IR_MAX) {
-      bem_cvt_z2c(o->d_R, crhs, na, g.st);
+      bem_cvt_z2c(o->d_R, crhs, na, cur_st());
       int gs = 0;
-      if (schur_apply(f, crhs, true, cdx, cw, g.st, &gs)) return fail(BH_ECONV, "GMRES on the FP32 Schur complement did not converge");
+      if (schur_apply(f, crhs, true, cdx, cw, cur_st(), &gs)) return fail(BH_ECONV, "GMRES on the FP32 Schur complement did not converge");
       o->gm_steps += gs;
-      bem_acc_c2z(o->d_X, cdx, na, g.st);
+      bem_acc_c2z(o->d_X, cdx, na, cur_st());
       for (int d = 0; d <= o->MN; d++) {
         Dom &D = o->dom[d];
-        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, g.st);
+        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, cur_st());
       }
-      const double nr = bem_nrm2(o->d_R, na, o->d_nrm, g.st);
+      const double nr = bem_nrm2(o->d_R, na, o->d_nrm, cur_st());
       if (nr < 0.0) return fail(BH_EHIP, "norm of the residual");
       res = nr / nb;
       it++;
@@ -1190,21 +1202,26 @@ static int object_solve(bh_object *o)
   }
   for (int d = 0; d <= o->MN; d++) {
     Dom &D = o->dom[d];
-    bem_launch_store_X(o->d_X, D.nd, D.d_col, NN, D.d_colS, o->d_Ui, o->d_dUi, D.d_U, D.d_dU, g.st);
+    bem_launch_store_X(o->d_X, D.nd, D.d_col, NN, D.d_colS, o->d_Ui, o->d_dUi, D.d_U, D.d_dU, cur_st());
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   return 0;
+}
+
+int bem::object_solve(bh_object *o)
+{
+  const int rc = object_solve_core(o);
+  // the one-launch triangular solves' timeout flag is read and cleared on
+  // every exit: a timed-out wait is the cause of whatever failed after it
+  const int tr = trsv_failed(cur_st());
+  return tr ? tr : rc;
 }
 
 extern "C" int bh_object_solve(bh_object *o)
 {
   API_LOCK;
-  const int rc = object_solve(o);
-  // the one-launch triangular solves' timeout flag is read and cleared on
-  // every exit: a timed-out wait is the cause of whatever failed after it
-  const int tr = trsv_failed(g.st);
-  return tr ? tr : rc;
+  return object_solve(o);
 }
 
 extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *resid, int *gmres_steps)
@@ -1224,44 +1241,43 @@ static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *pi
   cd *X = o->ws_eh[2].get<cd>(m), *Rr = o->ws_eh[3].get<cd>(m);
   cf *C = o->ws_eh[4].get<cf>(m);
   if (!X || !Rr || !C) return fail(BH_EHIP, "refinement of the boundary fields: work area");
-  HIPCHK(hipMemsetAsync(X, 0, m * sizeof(cd), g.st));
-  HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-  const double nb = bem_nrm2(R, m, o->d_nrm, g.st);
+  HIPCHK(hipMemsetAsync(X, 0, m * sizeof(cd), cur_st()));
+  HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+  const double nb = bem_nrm2(R, m, o->d_nrm, cur_st());
   double res = nb > 0.0 ? 1.0 : 0.0;
   int it = 0, rc = 0;
   while (res > IR_TOL && it < IR_MAX) {
-    bem_cvt_z2c(Rr, C, m, g.st);
-    bem_cgetrs(D.nd, 6, Gc, nd, piv, C, 6, g.st);
-    bem_acc_c2z(X, C, m, g.st);
-    HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-    bem_zgemm(D.nd, 6, D.nd, -1.0, 0.0, D.d_G, nd, X, 6, 1.0, 0.0, Rr, 6, g.st);
-    const double nr = bem_nrm2(Rr, m, o->d_nrm, g.st);
+    bem_cvt_z2c(Rr, C, m, cur_st());
+    bem_cgetrs(D.nd, 6, Gc, nd, piv, C, 6, cur_st());
+    bem_acc_c2z(X, C, m, cur_st());
+    HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+    bem_zgemm(D.nd, 6, D.nd, -1.0, 0.0, D.d_G, nd, X, 6, 1.0, 0.0, Rr, 6, cur_st());
+    const double nr = bem_nrm2(Rr, m, o->d_nrm, cur_st());
     if (nr < 0.0) { rc = fail(BH_EHIP, "norm of the residual"); break; }
     res = nr / nb;
     it++;
   }
   if (!rc && !(res <= IR_TOL)) rc = fail(BH_ECONV, "mixed-precision refinement (G): relative residual %.3g after %d steps", res, it);
-  if (!rc) HIPCHK(hipMemcpyAsync(R, X, m * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  if (!rc) HIPCHK(hipMemcpyAsync(R, X, m * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   return rc;
 }
 
 // boundary E, H from the derivative rows, then dE/dn, dH/dn from
 // G [dE|dH] = H [E|H] per domain (solve_eh_bv + solve_deh_bv)
-extern "C" int bh_object_solve_EH(bh_object *o)
+int bem::object_solve_EH(bh_object *o)
 {
-  API_LOCK;
   if (!o->dom[0].d_U) return fail(BH_ESTATE, "solve before the boundary fields");
   zd ch = o->lambda0 / (2.0 * M_PI * zd(0.0, 1.0));
   if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
-  HIPCHK(hipMemsetAsync(o->d_info, 0, sizeof(int), g.st));
+  HIPCHK(hipMemsetAsync(o->d_info, 0, sizeof(int), cur_st()));
   const bool mixed = o->fmode == 3;
   if (mixed && !o->d_nrm && dalloc(&o->d_nrm, 256)) return BH_EHIP;
   for (int d = 0; d <= o->MN; d++) {
     Dom &D = o->dom[d];
     size_t nd = (size_t)D.nd;
     bem_launch_eh_rows(D.d_dG, D.d_dH, D.nd, nd, D.d_dF, D.d_rowEH, D.d_colS, D.d_col, D.d_U, D.d_dU, ch.real(),
-                       ch.imag(), D.d_E, D.d_Hf, g.st);
+                       ch.imag(), D.d_E, D.d_Hf, cur_st());
     // LU of G_d: domain 0 of the block form already factorised G0 for this
     // assembly (in the precision of the form); otherwise factorise G_d once
     // per assembly
@@ -1276,8 +1292,8 @@ extern "C" int bh_object_solve_EH(bh_object *o)
       if (!D.gfc_valid) {
         if (!D.c_Gf && dalloc(&D.c_Gf, nd * nd)) return BH_EHIP;
         if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
-        bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, g.st);
-        if (bem_cgetrf(D.nd, D.c_Gf, nd, D.d_Gpiv, o->d_info, g.st)) return lu_failed();
+        bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, cur_st());
+        if (bem_cgetrf(D.nd, D.c_Gf, nd, D.d_Gpiv, o->d_info, cur_st())) return lu_failed();
         D.gfc_valid = true;
         D.gf_valid = false;   // the shared pivot vector now belongs to the FP32 factors
       }
@@ -1286,8 +1302,8 @@ extern "C" int bh_object_solve_EH(bh_object *o)
     } else if (!D.gf_valid) {
       if (!D.d_Gf && dalloc(&D.d_Gf, nd * nd)) return BH_EHIP;
       if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
-      HIPCHK(hipMemcpyAsync(D.d_Gf, D.d_G, nd * nd * sizeof(cd), hipMemcpyDeviceToDevice, g.st));
-      if (bem_zgetrf(D.nd, D.d_Gf, nd, D.d_Gpiv, o->d_work, o->d_info, 64, g.st)) return lu_failed();
+      HIPCHK(hipMemcpyAsync(D.d_Gf, D.d_G, nd * nd * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+      if (bem_zgetrf(D.nd, D.d_Gf, nd, D.d_Gpiv, o->d_work, o->d_info, 64, cur_st())) return lu_failed();
       D.gf_valid = true;
       D.gfc_valid = false;
       Gf = D.d_Gf;
@@ -1295,22 +1311,29 @@ extern "C" int bh_object_solve_EH(bh_object *o)
     }
     cd *P = o->ws_eh[0].get<cd>(nd * 6), *R = o->ws_eh[1].get<cd>(nd * 6);
     if (!P || !R) return fail(BH_EHIP, "boundary fields: work area");
-    bem_launch_pack_eh(D.nd, D.d_col, D.d_colS, D.d_E, D.d_Hf, P, g.st);
-    bem_zgemm(D.nd, 6, D.nd, 1.0, 0.0, D.d_H, nd, P, 6, 0.0, 0.0, R, 6, g.st);
+    bem_launch_pack_eh(D.nd, D.d_col, D.d_colS, D.d_E, D.d_Hf, P, cur_st());
+    bem_zgemm(D.nd, 6, D.nd, 1.0, 0.0, D.d_H, nd, P, 6, 0.0, 0.0, R, 6, cur_st());
     int rc = 0;
     if (mixed)
       rc = solve_g_mixed(o, D, Gc, Gpiv, R);
     else
-      bem_zgetrs(D.nd, 6, Gf, nd, Gpiv, R, 6, 64, g.st);
-    if (!rc) bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, g.st);
-    HIPCHK(hipStreamSynchronize(g.st));
+      bem_zgetrs(D.nd, 6, Gf, nd, Gpiv, R, 6, 64, cur_st());
+    if (!rc) bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, cur_st());
+    HIPCHK(hipStreamSynchronize(cur_st()));
     if (rc) return rc;
   }
   HIPCHK(hipGetLastError());
   int info = 0;
-  HIPCHK(hipMemcpy(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   if (info) return fail(BH_ESING, "singular G (zero pivot %d)", info);
-  return trsv_failed(g.st);
+  return trsv_failed(cur_st());
+}
+
+extern "C" int bh_object_solve_EH(bh_object *o)
+{
+  API_LOCK;
+  return object_solve_EH(o);
 }
 
 // which: 0 U, 1 dU ([dNe+1][4][4]), 2 E, 3 dE, 4 H, 5 dH ([dNe+1][4][3]);
@@ -1346,7 +1369,7 @@ extern "C" int bh_object_incident(bh_object *o, int which, double *buf, int set)
 int bem::domain_ids(bh_object *o, const double *d_pts, int np, int *d_did)
 {
   if (prepare_solve(o)) return BH_EHIP;
-  bem_launch_domain(d_pts, np, o->d_fv, o->d_fsd, o->d_fq, o->d_seg, o->MN, d_did, g.st);
+  bem_launch_domain(d_pts, np, o->d_fv, o->d_fsd, o->d_fq, o->d_seg, o->MN, d_did, cur_st());
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1367,7 +1390,7 @@ template <class T> static int cput(int i, T **d, const T *h, size_t n)
 {
   *d = cws<T>(i, n);
   if (!*d) return fail(BH_EHIP, "device work area of %zu bytes", n * sizeof(T));
-  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, g.st));
+  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, cur_st()));
   return 0;
 }
 #define CWS_CHECK(p) do { if (!(p)) return fail(BH_EHIP, "device work area"); } while (0)
@@ -1396,12 +1419,12 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
   d_sel = cws<int>(3, np);
   d_mask = cws<int>(4, np);
   CWS_CHECK(d_out && d_did && d_sel && d_mask);
-  HIPCHK(hipMemsetAsync(d_out, 0, (size_t)np * nc * sizeof(cd), g.st));
+  HIPCHK(hipMemsetAsync(d_out, 0, (size_t)np * nc * sizeof(cd), cur_st()));
   for (int m = 0; m < N; m++)
     if (domain_ids(objs[m], d_pts, np, d_did + (size_t)m * np)) return BH_EHIP;
   std::vector<int> did((size_t)np * N), sel(np), mask(np);
-  HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   // object containing each point: the first one whose domain is not open
   std::vector<int> pin(np, -1), pdom(np, 0);
   for (int p = 0; p < np; p++)
@@ -1411,7 +1434,7 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
     for (int m = 0; m < N; m++) {
       bh_object *o = objs[m];
       for (int p = 0; p < np; p++) sel[p] = pin[p] < 0 ? 0 : (pin[p] == m ? pdom[p] : -2);
-      HIPCHK(hipMemcpyAsync(d_sel, sel.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+      HIPCHK(hipMemcpyAsync(d_sel, sel.data(), np * sizeof(int), hipMemcpyHostToDevice, cur_st()));
       for (int d = 0; d <= o->MN; d++) {
         Dom &D = o->dom[d];
         if (what == 2) {
@@ -1419,28 +1442,28 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
           bem_launch_field_memp(d_pts, d_sel, np, d, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[d].real(),
                                 o->kn[d].imag(), g_rule[type], g_nrule[type], g_rscale[type], type < 3, g_rule[3],
                                 g_nrule[3], g_rscale[3], g_qrule[type], g_nqrule[type], g_qrule[3], g_nqrule[3], D.d_U,
-                                D.d_dU, o->lambda0, d_out, g.st);
+                                D.d_dU, o->lambda0, d_out, cur_st());
           continue;
         }
         const void *A0 = what == 0 ? (const void *)D.d_E : (const void *)D.d_U;
         const void *dA0 = what == 0 ? (const void *)D.d_dE : (const void *)D.d_dU;
         bem_launch_field(d_pts, d_sel, np, d, o->d_geo, D.d_sid, D.dNe, o->kn[d].real(), o->kn[d].imag(), g_rule[type],
                          g_nrule[type], g_rscale[type], o->d_gq, g_qrule[type], g_nqrule[type], what, A0, dA0, D.d_Hf,
-                         D.d_dHf, d_out, 1, g.st);
+                         D.d_dHf, d_out, 1, cur_st());
       }
-      HIPCHK(hipStreamSynchronize(g.st));
+      HIPCHK(hipStreamSynchronize(cur_st()));
     }
   }
   // incident field: everywhere for mode i, open-region points for mode t
   if (mode != 0) {
     for (int p = 0; p < np; p++) mask[p] = mode == 2 ? 1 : pin[p] < 0;
-    HIPCHK(hipMemcpyAsync(d_mask, mask.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+    HIPCHK(hipMemcpyAsync(d_mask, mask.data(), np * sizeof(int), hipMemcpyHostToDevice, cur_st()));
     add_incident(objs[0], d_pts, d_mask, np, d_out, what == 1);
   }
   HIPCHK(hipGetLastError());
   if (mode != 0 && fbeam_failed()) return BH_ECONV;
-  HIPCHK(hipMemcpyAsync(out, d_out, (size_t)np * nc * sizeof(cd), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(out, d_out, (size_t)np * nc * sizeof(cd), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   if (oid) memcpy(oid, pin.data(), np * sizeof(int));
   return 0;
 }
@@ -1546,7 +1569,7 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
   d_sel = cws<int>(10, np);
   d_mask = cws<int>(11, np);
   CWS_CHECK(d_EH && d_tmp && d_rt && d_did && d_sel && d_mask);
-  HIPCHK(hipMemsetAsync(d_EH, 0, 6 * (size_t)np * sizeof(cd), g.st));
+  HIPCHK(hipMemsetAsync(d_EH, 0, 6 * (size_t)np * sizeof(cd), cur_st()));
   // self object: the boundary integral (EH_s_bd) only at the points that are
   // not node slots, the node slots from the stored boundary values
   {
@@ -1568,12 +1591,12 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
       bem_launch_field_bd(d_ct, d_cze, nc, 0, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(),
                           g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
                           g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
-                          d_tmp, d_rt, g.st);
-      hipLaunchKernelGGL(k_scatter6, g1(nc), dim3(256), 0, g.st, d_tmp, d_ci, nc, d_EH);
-      HIPCHK(hipStreamSynchronize(g.st));   // the host tables above stay alive until here
+                          d_tmp, d_rt, cur_st());
+      hipLaunchKernelGGL(k_scatter6, g1(nc), dim3(256), 0, cur_st(), d_tmp, d_ci, nc, d_EH);
+      HIPCHK(hipStreamSynchronize(cur_st()));   // the host tables above stay alive until here
     }
   }
-  hipLaunchKernelGGL(k_gather_stored, g1(np), dim3(256), 0, g.st, d_t, d_st, np, D.d_E, D.d_Hf, d_EH);
+  hipLaunchKernelGGL(k_gather_stored, g1(np), dim3(256), 0, cur_st(), d_t, d_st, np, D.d_E, D.d_Hf, d_EH);
   // other objects at the point r (their open domain, checked by solid angle)
   for (int m = 0; m < N; m++) {
     if (m == oid) continue;
@@ -1583,16 +1606,16 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
       Dom &Q = q->dom[d];
       bem_launch_field(d_r, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
                        g_nrule[2], g_rscale[2], q->d_gq, g_qrule[2], g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf,
-                       d_EH, 1, g.st);
+                       d_EH, 1, cur_st());
     }
   }
   {
     std::vector<int> ones(np, 1);
-    HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+    HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, cur_st()));
   }
   add_incident(objs[0], d_r, d_mask, np, d_EH, 0);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   if (fbeam_failed()) return BH_ECONV;
   *d_EH_out = d_EH; *d_r_out = d_r; *d_w_out = d_w; *d_wt_out = d_wt;
   return 0;
@@ -1628,10 +1651,10 @@ extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const dou
   CWS_CHECK(d_part);
   if (cput(16, &d_rc, rc, 3)) return BH_EHIP;
   double epd = o->n[0].real() * o->n[0].real();
-  bem_launch_stress(d_EH, d_r, d_w, d_wt, np, epd, d_rc, d_part, g.st);
+  bem_launch_stress(d_EH, d_r, d_w, d_wt, np, epd, d_rc, d_part, cur_st());
   std::vector<double> part(6 * (size_t)np);
-  HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   for (int p = 0; p < np; p++)
     for (int j = 0; j < 3; j++) { F[j] += -0.5 * part[6 * p + j]; Nq[j] += -0.5 * part[6 * p + 3 + j]; }
   return 0;
@@ -1655,10 +1678,10 @@ extern "C" int bh_mo_absorb(bh_object **objs, int N, int oid, int type, double *
   if (rc1) return rc1;
   d_part = cws<double>(15, np);
   CWS_CHECK(d_part);
-  bem_launch_poynting(d_EH, d_w, d_wt, np, d_part, g.st);
+  bem_launch_poynting(d_EH, d_w, d_wt, np, d_part, cur_st());
   std::vector<double> part(np);
-  HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(part.data(), d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   double p = 0.0;
   for (int i = 0; i < np; i++) p += -0.25 * part[i];
   *P = -p;
@@ -1676,8 +1699,8 @@ extern "C" int bh_object_domain(bh_object *o, int np, const double *pts, int *di
   int *d_did = cws<int>(1, np);
   CWS_CHECK(d_did);
   if (domain_ids(o, d_pts, np, d_did)) return BH_EHIP;
-  HIPCHK(hipMemcpyAsync(did, d_did, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(did, d_did, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   return 0;
 }
 
@@ -1711,7 +1734,7 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
   d_mask = cws<int>(5, np);
   double *d_rt2 = cws<double>(6, 3 * (size_t)np);
   CWS_CHECK(d_rt && d_out && d_did && d_mask && d_rt2);
-  HIPCHK(hipMemsetAsync(d_out, 0, 6 * (size_t)np * sizeof(cd), g.st));
+  HIPCHK(hipMemsetAsync(d_out, 0, 6 * (size_t)np * sizeof(cd), cur_st()));
   // positions on the element (r = c0 + c1 zeta + c2 eta)
   {
     std::vector<double> rt(3 * (size_t)np);
@@ -1719,14 +1742,14 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
       const double *cr = &o->cr[(size_t)std::abs(D.sid[t[p]]) * 12];
       for (int q = 0; q < 3; q++) rt[3 * p + q] = cr[q * 4] + cr[q * 4 + 1] * ze[2 * p] + cr[q * 4 + 2] * ze[2 * p + 1];
     }
-    HIPCHK(hipMemcpyAsync(d_rt, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice, g.st));
+    HIPCHK(hipMemcpyAsync(d_rt, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice, cur_st()));
   }
   // own boundary integral (needs the solved boundary values)
   if (mode != 2) {
     bem_launch_field_bd(d_t, d_ze, np, did, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[did].real(), o->kn[did].imag(),
                         g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
                         g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
-                        d_rt2, g.st);
+                        d_rt2, cur_st());
   }
   if (did == 0 && mode != 0) {
     for (int m = 0; m < N; m++) {
@@ -1737,19 +1760,19 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
         Dom &Q = q->dom[d];
         bem_launch_field(d_rt, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
                          g_nrule[2], g_rscale[2], q->d_gq, g_qrule[2], g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf,
-                         d_out, 1, g.st);
+                         d_out, 1, cur_st());
       }
     }
     if (mode == 1 || mode == 2) {
       std::vector<int> ones(np, 1);
-      HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, g.st));
+      HIPCHK(hipMemcpyAsync(d_mask, ones.data(), np * sizeof(int), hipMemcpyHostToDevice, cur_st()));
       add_incident(objs[0], d_rt, d_mask, np, d_out, 0);
       if (fbeam_failed()) return BH_ECONV;
     }
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, d_out, 6 * (size_t)np * sizeof(cd), hipMemcpyDeviceToHost, g.st));
-  HIPCHK(hipStreamSynchronize(g.st));
+  HIPCHK(hipMemcpyAsync(out, d_out, 6 * (size_t)np * sizeof(cd), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   return 0;
 }
 
@@ -1761,7 +1784,7 @@ extern "C" int bh_finalize(void)
 {
   API_LOCK;
   if (g.dev < 0) return 0;
-  (void)hipStreamSynchronize(g.st);
+  (void)hipStreamSynchronize(cur_st());
   bem_mo_release();
   if (g_cws) { delete[] g_cws; g_cws = nullptr; }
   for (int i = 0; i < 4; i++) {

@@ -13,6 +13,7 @@
 // (mo_solve.c:751) of the reference.  The FP32 instances serve the
 // mixed-precision form (a complex-FP32 factorisation refined in FP64).
 #include "bem_dev.h"
+#include <mutex>
 #include "bem_prof.h"
 
 #include <algorithm>
@@ -860,12 +861,41 @@ struct TrsvWs {
   int *flag;               // [TRSV_NBMAX]
   int *err;
 };
-// exchange area of the one-launch triangular solves (uncached, allocated
-// once for the life of the process and never freed); false: the recursive
-// path runs instead
-static void *g_trsv_buf = nullptr;
+// false: the one-launch triangular solves are unavailable (LDS limit), the
+// recursive path runs instead
 static bool g_trsv_ok = true;
-static int g_trsv_epoch = 0;
+
+// Per-stream state of the factorisation and the one-launch solves: work on
+// different streams (the coupling loop factorises and solves objects on
+// several streams at once) must not share exchange areas, flags or epochs.
+// A lane is bound to a stream on first use; its uncached areas are
+// allocated once and kept until the process ends (memory a kernel has read
+// uncached is not returned to the allocator).
+#define LA_NLANES 8
+struct LaLane {
+  hipStream_t st;
+  bool used;
+  void *lu_ws;           // grow-only work area of the panel factorisation
+  size_t lu_bytes;
+  void *coop_buf;        // exchange area of the one-launch panel (uncached)
+  void *trsv_buf;        // exchange area of the one-launch triangular solves (uncached)
+  int trsv_epoch;
+};
+static LaLane g_lanes[LA_NLANES];
+static std::mutex g_lanes_mu;
+// the lane of a stream (null if every lane is bound to another stream)
+static LaLane *la_lane(hipStream_t st)
+{
+  std::lock_guard<std::mutex> lk(g_lanes_mu);
+  for (int i = 0; i < LA_NLANES; i++)
+    if (g_lanes[i].used && g_lanes[i].st == st) return &g_lanes[i];
+  for (int i = 0; i < LA_NLANES; i++)
+    if (!g_lanes[i].used) {
+      g_lanes[i] = LaLane{st, true, nullptr, 0, nullptr, nullptr, 0};
+      return &g_lanes[i];
+    }
+  return nullptr;
+}
 
 // LDS of one block: D^{-1} and W (rows padded to 65), an x block per wave
 template <class R> constexpr int trsv_lds_bytes()
@@ -1063,6 +1093,12 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
   }
 }
 
+__global__ void k_zero_bytes(unsigned int *p, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 // ---- host side ----
 static bool g_big_tiles = false;   // set once the 128x128 kernels' LDS limits are raised
 static bool g_coop = false;        // the device supports cooperative launches
@@ -1072,8 +1108,6 @@ static bool g_coop = false;        // the device supports cooperative launches
 #define COOP_CNT_BYTES ((size_t)COOP_CSTRIDE * sizeof(unsigned int) * (1 + COOP_NBMAX / COOP_GS))
 #define COOP_BYTES (COOP_CNT_BYTES + 256 + 2 * COOP_NBMAX * (sizeof(double) + sizeof(int)) + 64 * sizeof(int) + \
                     2 * (size_t)COOP_NBMAX * 64 * sizeof(cd))
-static void *g_coop_buf = nullptr;
-
 template <class R> static int la_init_t()
 {
   // the triangular-solve kernels stage a 64x64 factor block in LDS, the
@@ -1130,10 +1164,6 @@ static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename 
   bem_prof_end(kid, st);
 }
 
-// grow-only scratch of the panel factorisation (device memory kept for the
-// life of the process; calls are serialised by the C-ABI)
-static void *g_lu_ws = nullptr;
-static size_t g_lu_ws_bytes = 0;
 
 // workspace of the panel factorisation for one matrix order
 struct LuWs {
@@ -1255,20 +1285,22 @@ template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, i
   const int nbc = (n + 63) / 64;   // workgroups of the one-launch panel
   size_t need = (size_t)kbmax * ldp * sizeof(cd) + 3 * ldp * sizeof(int) + 64 * sizeof(int) +
                 2 * (size_t)nblk * (sizeof(double) + sizeof(int));
-  if (need > g_lu_ws_bytes) {
-    if (g_lu_ws) {
-      (void)hipStreamSynchronize(st);   // earlier factorisations may still use it
-      (void)hipFree(g_lu_ws);
+  LaLane *L = la_lane(st);
+  if (!L) return -1;
+  if (need > L->lu_bytes) {
+    if (L->lu_ws) {
+      (void)hipStreamSynchronize(st);   // earlier factorisations on this stream may still use it
+      (void)hipFree(L->lu_ws);
     }
-    g_lu_ws = nullptr;
-    g_lu_ws_bytes = 0;
-    if (hipMalloc(&g_lu_ws, need) != hipSuccess) return -1;
-    g_lu_ws_bytes = need;
+    L->lu_ws = nullptr;
+    L->lu_bytes = 0;
+    if (hipMalloc(&L->lu_ws, need) != hipSuccess) return -1;
+    L->lu_bytes = need;
   }
   // layout for this n (the buffer may be larger, from an earlier call);
   // nothing in it needs clearing: the panel loader initialises used[], every
   // launch writes the maxima the next one reads
-  char *p = (char *)g_lu_ws;
+  char *p = (char *)L->lu_ws;
   LuWs w;
   w.P = (void *)p;                       p += (size_t)kbmax * ldp * sizeof(cd);
   w.ldp = ldp;
@@ -1285,12 +1317,10 @@ template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, i
   // is not returned to the allocator)
   w.coop_ok = false;
   if (g_coop && nbc <= COOP_NBMAX) {
-    if (!g_coop_buf && hipExtMallocWithFlags(&g_coop_buf, COOP_BYTES, hipDeviceMallocUncached) != hipSuccess) {
-      g_coop_buf = nullptr;
-      g_coop = false;
-    }
-    if (g_coop_buf) {
-      char *q = (char *)g_coop_buf;
+    if (!L->coop_buf && hipExtMallocWithFlags(&L->coop_buf, COOP_BYTES, hipDeviceMallocUncached) != hipSuccess)
+      L->coop_buf = nullptr;   // this stream takes the launch-per-column path
+    if (L->coop_buf) {
+      char *q = (char *)L->coop_buf;
       w.coop.cnt = (unsigned int *)q;        q += COOP_CNT_BYTES;
       w.coop.err = (int *)q;                 q += 256;
       w.coop.candV = (double *)q;            q += 2 * COOP_NBMAX * sizeof(double);
@@ -1325,20 +1355,23 @@ static bool trsv_sf(const typename Cx<R>::Z *A, size_t lda, int n, typename Cx<R
 {
   const int nb = (n + 63) / 64;
   if (!g_coop || !g_trsv_ok || nrhs > TRSV_MAXR || n > TRSV_XMAX || nb > TRSV_NBMAX) return false;
-  if (!g_trsv_buf) {
-    if (hipExtMallocWithFlags(&g_trsv_buf, TRSV_BYTES, hipDeviceMallocUncached) != hipSuccess ||
-        hipMemset(g_trsv_buf, 0, TRSV_BYTES) != hipSuccess) {
-      g_trsv_buf = nullptr;
-      g_trsv_ok = false;
+  LaLane *L = la_lane(st);
+  if (!L) return false;
+  if (!L->trsv_buf) {
+    if (hipExtMallocWithFlags(&L->trsv_buf, TRSV_BYTES, hipDeviceMallocUncached) != hipSuccess) {
+      L->trsv_buf = nullptr;
       return false;
     }
+    // flags start below every epoch (a kernel clears them: no memset node)
+    hipLaunchKernelGGL(k_zero_bytes, dim3((TRSV_BYTES / 4 + 255) / 256), dim3(256), 0, st, (unsigned int *)L->trsv_buf,
+                       TRSV_BYTES / 4);
   }
   TrsvWs w;
-  char *q = (char *)g_trsv_buf;
+  char *q = (char *)L->trsv_buf;
   w.x = (void *)q;              q += TRSV_XBYTES;
   w.flag = (int *)q;            q += TRSV_NBMAX * sizeof(int);
   w.err = (int *)q;
-  const int epoch = ++g_trsv_epoch;
+  const int epoch = ++L->trsv_epoch;
   bem_prof_begin(Cx<R>::KT, 4.0 * (double)n * n * nrhs, st);
   void *args[] = {(void *)&A, &lda, (void *)&n, &B, &ldb, &w, (void *)&epoch};
   const void *fn = nrhs == 1 ? (const void *)k_trsv_sf<R, UPPER, 1>
@@ -1384,8 +1417,9 @@ static void getrs_t(int n, int nrhs, const typename Cx<R>::Z *A, size_t lda, con
 // clears it
 static int trsv_error(hipStream_t st)
 {
-  if (!g_trsv_buf) return 0;
-  int *err = (int *)((char *)g_trsv_buf + TRSV_XBYTES + TRSV_NBMAX * sizeof(int));
+  LaLane *L = la_lane(st);
+  if (!L || !L->trsv_buf) return 0;
+  int *err = (int *)((char *)L->trsv_buf + TRSV_XBYTES + TRSV_NBMAX * sizeof(int));
   int h = 0;
   if (hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
     return -1;
@@ -1416,9 +1450,12 @@ int bem_trsv_error(hipStream_t st) { return trsv_error(st); }
 // handed out again)
 void bem_la_release(void)
 {
-  if (g_lu_ws) (void)hipFree(g_lu_ws);
-  g_lu_ws = nullptr;
-  g_lu_ws_bytes = 0;
+  std::lock_guard<std::mutex> lk(g_lanes_mu);
+  for (int i = 0; i < LA_NLANES; i++) {
+    if (g_lanes[i].lu_ws) (void)hipFree(g_lanes[i].lu_ws);
+    g_lanes[i].lu_ws = nullptr;
+    g_lanes[i].lu_bytes = 0;
+  }
 }
 
 // complex FP64 (interleaved double pairs)

@@ -46,6 +46,13 @@ struct Ctx {
 };
 extern Ctx g;
 
+// the stream a thread's entry-point work goes to: the library's stream,
+// unless the thread is a worker of the coupling loop bound to a stream of
+// its own (set_cur_st)
+extern thread_local hipStream_t t_cur_st;
+inline hipStream_t cur_st() { return t_cur_st ? t_cur_st : g.st; }
+inline void set_cur_st(hipStream_t s) { t_cur_st = s; }
+
 // a grow-only device buffer
 struct Scratch {
   void *p = nullptr;
@@ -111,7 +118,7 @@ namespace bem {
 template <class T> int upload(T **d, const T *h, size_t n)
 {
   HIPCHK(hipMalloc((void **)d, std::max<size_t>(n, 1) * sizeof(T)));
-  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, g.st));
+  if (n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, cur_st()));
   return 0;
 }
 }  // namespace bem
@@ -173,6 +180,12 @@ struct bh_object {
 };
 
 namespace bem {
+// the entry points' work without the entry lock, on the thread's current
+// stream (the coupling loop's workers call these)
+int object_assemble(bh_object *o, int type);
+int object_factor(bh_object *o, int mode);
+int object_solve(bh_object *o);
+int object_solve_EH(bh_object *o);
 int prepare_solve(bh_object *o);
 int form_s(bh_object *o);
 int domain_ids(bh_object *o, const double *d_pts, int np, int *d_did);

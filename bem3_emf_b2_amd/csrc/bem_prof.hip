@@ -13,6 +13,7 @@ struct Rec {
   int kid;
   double work;
   hipEvent_t a, b;
+  bool closed;
 };
 struct Prof {
   bool on = false;
@@ -22,6 +23,9 @@ struct Prof {
 };
 Prof P;
 std::mutex M;
+// the records this thread opened and has not closed (threads driving
+// different streams time their kernels independently)
+thread_local std::vector<size_t> t_open;
 
 hipEvent_t take()
 {
@@ -51,9 +55,10 @@ void bem_prof_begin(int kid, double work, hipStream_t st)
 {
   if (!P.on || kid < 0 || kid >= BH_K_COUNT) return;
   std::lock_guard<std::mutex> lk(M);
-  Rec r{kid, work, take(), take()};
+  Rec r{kid, work, take(), take(), false};
   if (!r.a || !r.b) return;
   (void)hipEventRecord(r.a, st);
+  t_open.push_back(P.recs.size());
   P.recs.push_back(r);
 }
 
@@ -61,8 +66,12 @@ void bem_prof_end(int kid, hipStream_t st)
 {
   if (!P.on) return;
   std::lock_guard<std::mutex> lk(M);
-  if (P.recs.empty() || P.recs.back().kid != kid) return;
-  (void)hipEventRecord(P.recs.back().b, st);
+  if (t_open.empty()) return;
+  const size_t i = t_open.back();
+  t_open.pop_back();
+  if (i >= P.recs.size() || P.recs[i].kid != kid) return;
+  (void)hipEventRecord(P.recs[i].b, st);
+  P.recs[i].closed = true;
 }
 
 int bh_prof_enable(int on)
@@ -71,6 +80,7 @@ int bh_prof_enable(int on)
   P.on = on != 0;
   P.recs.clear();
   P.used = 0;
+  t_open.clear();
   return 0;
 }
 
@@ -81,7 +91,7 @@ int bh_prof_read(int kid, double *ms, double *work, int *launches)
   double t = 0.0, w = 0.0;
   int n = 0;
   for (const Rec &r : P.recs) {
-    if (r.kid != kid) continue;
+    if (r.kid != kid || !r.closed) continue;
     if (hipEventSynchronize(r.b) != hipSuccess) return BH_EHIP;
     float e = 0.0f;
     if (hipEventElapsedTime(&e, r.a, r.b) != hipSuccess) return BH_EHIP;

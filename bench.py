@@ -66,14 +66,18 @@ def scene_objects(bem, objs, beams, form):
     return G
 
 
-def scene_step(bem, G):
+def scene_step(bem, G, phases=None):
     """one configuration: assembly and factorisation of every object, the
-    coupling loop to a 1e-8 change (boundary E/H included)"""
-    for g in G:
-        g.assemble(2)
-        g.factor(0)
+    coupling loop to a 1e-8 change (boundary E/H included); phases (a list)
+    receives the seconds of the two parts"""
+    t0 = time.perf_counter()
+    bem.mo_prepare(G, 2, "auto")
+    bem.sync()
+    t1 = time.perf_counter()
     it, _ = bem.mo_solve(G, ccd=1e-8)
     bem.sync()
+    if phases is not None:
+        phases.append((t1 - t0, time.perf_counter() - t1))
     return it
 
 
@@ -196,7 +200,11 @@ def start_cpu_baseline(target_s):
     cores while the FP64 form runs on the GPU"""
     code = ("import json, sys; sys.path.insert(0, %r); import bench; "
             "print(json.dumps(bench.cpu_scene_baseline(%r)))" % (ROOT, target_s))
-    return subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True)
+    # one host core stays with the process driving the GPU
+    env = dict(os.environ)
+    n = int(env.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 2
+    env["OMP_NUM_THREADS"] = str(max(1, n - 1))
+    return subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True, env=env)
 
 
 def finish_cpu_baseline(p):
@@ -334,9 +342,10 @@ def main():
     bem.sync()
     t0 = time.perf_counter()
     it = 0
+    phases = []
     for s in range(args.steps):
-        it = scene_step(bem, G)
-        log("step %d done (%.3f s)" % (s, time.perf_counter() - t0))
+        it = scene_step(bem, G, phases)
+        log("step %d done (%.3f s; objects %.3f s, coupling loop %.3f s)" % (s, time.perf_counter() - t0, *phases[-1]))
     bem.sync()
     dt = time.perf_counter() - t0
     s_step = dt / args.steps
@@ -386,6 +395,8 @@ def main():
                    "objects": len(objs), "triangles": S, "coupling_iterations": it,
                    "coupling_pairs": {"series": series, "element_sums": direct, "largest_order": lmax}},
         "triangles_per_s": S / s_step,
+        "phases_s": {"objects": sum(p[0] for p in phases) / len(phases),
+                     "coupling_loop": sum(p[1] for p in phases) / len(phases)},
         "fp64_form": fp64,
         "speedup_vs_fp64": (fp64["value"] / s_step) if fp64 else None,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items() if v[2]},
