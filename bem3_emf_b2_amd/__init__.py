@@ -44,6 +44,7 @@ def lib():
             "bh_object_assemble": (C.c_int, [VP, C.c_int]),
             "bh_object_gh_count": (C.c_int, [VP, C.c_int]),
             "bh_object_matrix": (C.c_int, [VP, C.c_int, C.c_int, D]),
+            "bh_object_matrix_rows": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, Ip, D]),
             "bh_zgesv": (C.c_int, [C.c_int, C.c_int, D, D]),
             "bh_object_set_beams": (C.c_int, [VP, C.c_int, D]),
             "bh_object_set_mfb": (C.c_int, [VP, C.c_int, D]),
@@ -70,6 +71,7 @@ def lib():
             "bh_mo_set_coupling": (C.c_int, [C.c_int]),
             "bh_mo_plan_stats": (C.c_int, [Ip, Ip, Ip]),
             "bh_mo_prepare": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int]),
+            "bh_mo_coupling_sums": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -200,6 +202,21 @@ class Object:
         if k < 4:
             return out.reshape(nd, 2, nd)
         return out.reshape(nd, 3)
+
+    def matrix_rows(self, did, key, rows):
+        """selected rows of one domain matrix: G, H (nd columns), dG, dH
+        (2 x nd), dF (3)"""
+        which = ["G", "H", "dG", "dH", "dF"].index(key)
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        nd = self.nd[did]
+        if which < 2:
+            out = np.zeros((len(rows), nd), np.complex128)
+        elif which < 4:
+            out = np.zeros((len(rows), 2, nd), np.complex128)
+        else:
+            out = np.zeros((len(rows), 3))
+        check(lib().bh_object_matrix_rows(self.h, did, which, len(rows), ip(rows), dp(out.view(np.float64))))
+        return out
 
     def geometry(self):
         E1 = self.Ne + 1
@@ -335,6 +352,15 @@ def mo_prepare(objs, type_=2, mode="auto"):
     """assembly and factorisation of every object, the objects spread over
     several device streams"""
     check(lib().bh_mo_prepare(_handles(objs), len(objs), type_, FACTOR_MODES[mode]))
+
+
+def mo_coupling_sums(objs, dst):
+    """the loop's coupling sums at object dst's node slots from the current
+    boundary values: (U, dU) arrays of shape (Ne+1, 4, 4)"""
+    n = objs[dst].Ne + 1
+    out = np.zeros(2 * n * 16, np.complex128)
+    check(lib().bh_mo_coupling_sums(_handles(objs), len(objs), dst, dp(out.view(np.float64))))
+    return out[:n * 16].reshape(n, 4, 4), out[n * 16:].reshape(n, 4, 4)
 
 
 def mo_solve(objs, ccd=1e-6, iter_min=10, iter_max=100):

@@ -533,6 +533,27 @@ extern "C" int bh_object_matrix(const bh_object *o, int did, int which, double *
   return 0;
 }
 
+// nrows rows of one matrix (rows[i] in 0..nd-1): out receives them one after
+// another, each as bh_object_matrix lays a row out
+extern "C" int bh_object_matrix_rows(const bh_object *o, int did, int which, int nrows, const int *rows, double *out)
+{
+  API_LOCK;
+  if (did < 0 || did > o->MN || which < 0 || which > 4 || nrows < 0) return fail(BH_EARG, "bad matrix selector");
+  const Dom &D = o->dom[did];
+  const size_t nd = (size_t)D.nd;
+  const char *src = (const char *)(which == 0 ? (const void *)D.d_G : which == 1 ? (const void *)D.d_H
+                                   : which == 2 ? (const void *)D.d_dG : which == 3 ? (const void *)D.d_dH
+                                                                       : (const void *)D.d_dF);
+  if (!src) return fail(BH_ESTATE, "matrices not assembled");
+  const size_t rb = which < 2 ? nd * sizeof(cd) : which < 4 ? 2 * nd * sizeof(cd) : 3 * sizeof(double);
+  for (int i = 0; i < nrows; i++) {
+    if (rows[i] < 0 || (size_t)rows[i] >= nd) return fail(BH_EARG, "row %d out of range", rows[i]);
+    HIPCHK(hipMemcpyAsync((char *)out + i * rb, src + (size_t)rows[i] * rb, rb, hipMemcpyDeviceToHost, cur_st()));
+  }
+  HIPCHK(hipStreamSynchronize(cur_st()));
+  return 0;
+}
+
 // ------------------------------------------------------------------ dense LA entry points
 extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
 {

@@ -768,43 +768,42 @@ extern "C" int bh_mo_plan_stats(int *series, int *direct, int *lmax)
 // block-Jacobi fixed point over objects (mo_solve_bv, mo_solve.c:10-88).
 // trace (optional, iter_max+1 doubles) receives cc per iteration; *iters
 // receives the final iteration count.
-extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters)
-{
-  API_LOCK;
-  if (N <= 0) return fail(BH_EARG, "no objects");
-  if (ensure_rules()) return BH_EHIP;
-  for (int m = 0; m < N; m++)
-    if (!objs[m]->fmode || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d: factor and set beams first", m);
-  int rc, iter = 0;
-  const std::vector<double> c3 = cost_of(objs, N, 3), c2 = cost_of(objs, N, 2);
-  auto solve_all = [&]() { return run_lanes(objs, N, c2, [](bh_object *o) { return object_solve(o); }); };
-  if (N == 1) {
-    if ((rc = object_solve(objs[0]))) return rc;
-  } else {
-    // many solves per object: form the Schur complements once
-    if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return form_s(o); }))) return rc;
-    Plan P;
+// the coupling sums of one loop: its plan, series coefficients, evaluation
+// tiles and work areas (set up once per loop; the geometry is fixed)
+struct Coupling {
+  Plan P;
+  std::vector<int> slot0;
+  int maxslot = 0, totslot = 0;
+  int *d_did = nullptr;
+  double *d_part = nullptr, *d_ccd = nullptr, *d_ccs = nullptr;
+  std::vector<size_t> coff;
+  cd *d_coef = nullptr;
+  std::vector<MpTile> tiles;
+  std::vector<MpPair> pairs;
+  MpTile *d_tiles = nullptr;
+  MpPair *d_pairs = nullptr;
+  double evwork = 0.0;
+
+  int init(bh_object **objs, int N)
+  {
     make_plan(objs, N, P);
     g_last_series = P.npairs_mp;
     g_last_direct = P.npairs_direct;
     g_last_lmax = -1;
     for (int n = 0; n < N; n++) g_last_lmax = std::max(g_last_lmax, P.Lsrc[n]);
-    int maxslot = 0, totslot = 0;
-    std::vector<int> slot0(N);
+    slot0.assign(N, 0);
     for (int m = 0; m < N; m++) { maxslot = std::max(maxslot, objs[m]->nslot); slot0[m] = totslot; totslot += objs[m]->nslot; }
-    int *d_did = ws<int>(W_DID, maxslot);
-    double *d_part = ws<double>(W_PART, (size_t)maxslot * 19);   // far-triangle sums of the direct pass
-    double *d_ccd = ws<double>(W_CCD, 2 * (size_t)totslot);
-    double *d_ccs = ws<double>(W_CCDSUM, 2);
+    d_did = ws<int>(W_DID, maxslot);
+    d_part = ws<double>(W_PART, (size_t)maxslot * 19);   // far-triangle sums of the direct pass
+    d_ccd = ws<double>(W_CCD, 2 * (size_t)totslot);
+    d_ccs = ws<double>(W_CCDSUM, 2);
     if (!d_did || !d_part || !d_ccd || !d_ccs) return fail(BH_EHIP, "coupling loop: work area");
     // series coefficients of every source object, and the evaluation tiles
-    std::vector<size_t> coff(N, 0);
+    coff.assign(N, 0);
     size_t ncoef = 0;
     for (int n = 0; n < N; n++)
       if (P.Lsrc[n] >= 0) { coff[n] = ncoef; ncoef += (size_t)mp_count(P.Lsrc[n] + 1) * 16; }
-    cd *d_coef = ncoef ? ws<cd>(W_COEF, ncoef) : nullptr;
-    std::vector<MpTile> tiles;
-    std::vector<MpPair> pairs;
+    d_coef = ncoef ? ws<cd>(W_COEF, ncoef) : nullptr;
     for (int m = 0; m < N; m++) {
       bh_object *o = objs[m];
       const int p0 = (int)pairs.size();
@@ -828,18 +827,124 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
         tiles.push_back(t);
       }
     }
-    MpTile *d_tiles = nullptr;
-    MpPair *d_pairs = nullptr;
     if (!tiles.empty()) {
       d_tiles = ws<MpTile>(W_TILES, tiles.size());
       d_pairs = ws<MpPair>(W_PAIRS, pairs.size());
       if (!d_coef || !d_tiles || !d_pairs) return fail(BH_EHIP, "coupling loop: series work area");
       HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(MpTile), hipMemcpyHostToDevice, g.st));
       HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(MpPair), hipMemcpyHostToDevice, g.st));
+      HIPCHK(hipStreamSynchronize(g.st));   // the host tables are locals of this record
     }
-    double evwork = 0.0;
     for (const MpTile &t : tiles)
       for (int i = 0; i < t.np; i++) evwork += (double)t.ns * mp_count(pairs[t.p0 + i].Le);
+    return 0;
+  }
+
+  // the field of every object at the other objects' node slots from the
+  // current boundary values, added into their incident data; check: first
+  // verify that no slot of an element-sum pair lies inside its source
+  int sums(bh_object **objs, int N, bool check)
+  {
+    // series pairs: coefficients of every source, then one evaluation launch
+    for (int n = 0; n < N; n++)
+      if (P.Lsrc[n] >= 0) { int r = mp_coefficients(objs[n], P, n, d_coef + coff[n]); if (r) return r; }
+    if (!tiles.empty()) {
+      bem_prof_begin(BH_K_MP_EVAL, evwork, g.st);
+      hipLaunchKernelGGL(k_mp_eval, dim3((unsigned)tiles.size()), dim3(64), 0, g.st, d_tiles, d_pairs);
+      bem_prof_end(BH_K_MP_EVAL, g.st);
+    }
+    // the other pairs: direct sums
+    for (int m = 0; m < N; m++)
+      for (int d = 0; d < N; d++)
+        if (d != m && P.Lmn[d][m] == -1) { int r = add_scattered(objs[d], objs[m], d_did, check, d_part); if (r) return r; }
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+
+  // ccd_f (mo_solve.c:413-455) over every object's slots; -1 on a failure
+  double change(bh_object **objs, int N)
+  {
+    for (int m = 0; m < N; m++) {
+      bh_object *o = objs[m];
+      hipLaunchKernelGGL(k_ccd, g1(o->nslot), dim3(256), 0, g.st, o->d_Ui, o->d_Uip, o->d_slotIdx, o->nslot,
+                         d_ccd + 2 * (size_t)slot0[m]);
+    }
+    hipLaunchKernelGGL(k_ccd_sum, dim3(1), dim3(256), 0, g.st, d_ccd, totslot, d_ccs);
+    double h[2];
+    if (hipMemcpyAsync(h, d_ccs, sizeof h, hipMemcpyDeviceToHost, g.st) != hipSuccess ||
+        hipStreamSynchronize(g.st) != hipSuccess)
+      return -1.0;
+    return (h[0] > h[1] ? h[0] : h[1]) / (double)totslot;
+  }
+};
+
+__global__ void k_copy_cd(cd *__restrict__ dst, const cd *__restrict__ src, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// the coupling sums of the loop (series or element sums, as the loop would
+// take them) from the objects' current boundary values: out receives the
+// scattered U and dU/dn at object dst's node slots in its incident-data
+// layout, 2 x [Ne+1][4][4] complex (U, then dU); the incident data are left
+// as they were
+extern "C" int bh_mo_coupling_sums(bh_object **objs, int N, int dst, double *out)
+{
+  API_LOCK;
+  if (N < 2 || dst < 0 || dst >= N) return fail(BH_EARG, "coupling sums: %d objects, destination %d", N, dst);
+  if (ensure_rules()) return BH_EHIP;
+  for (int m = 0; m < N; m++)
+    if (!objs[m]->dom[0].d_U || !objs[m]->d_Ui) return fail(BH_ESTATE, "object %d is not solved", m);
+  Coupling C;
+  int rc = C.init(objs, N);
+  if (rc) return rc;
+  // keep the incident data, clear them, add the sums, read dst's, restore
+  std::vector<cd *> keep(2 * (size_t)N);
+  for (int m = 0; m < N; m++) {
+    const size_t n = (size_t)(objs[m]->Ne + 1) * 16;
+    for (int q = 0; q < 2; q++) {
+      cd *a = q ? objs[m]->d_dUi : objs[m]->d_Ui;
+      HIPCHK(hipMalloc((void **)&keep[2 * m + q], n * sizeof(cd)));
+      hipLaunchKernelGGL(k_copy_cd, g1(n), dim3(256), 0, g.st, keep[2 * m + q], a, n);
+      hipLaunchKernelGGL(k_zero_cd, g1(n), dim3(256), 0, g.st, a, n);
+    }
+  }
+  rc = C.sums(objs, N, true);
+  const size_t nd = (size_t)(objs[dst]->Ne + 1) * 16;
+  if (!rc && (hipMemcpyAsync(out, objs[dst]->d_Ui, nd * sizeof(cd), hipMemcpyDeviceToHost, g.st) != hipSuccess ||
+              hipMemcpyAsync(out + 2 * nd, objs[dst]->d_dUi, nd * sizeof(cd), hipMemcpyDeviceToHost, g.st) != hipSuccess))
+    rc = fail(BH_EHIP, "coupling sums: read-back");
+  for (int m = 0; m < N; m++) {
+    const size_t n = (size_t)(objs[m]->Ne + 1) * 16;
+    for (int q = 0; q < 2; q++)
+      hipLaunchKernelGGL(k_copy_cd, g1(n), dim3(256), 0, g.st, q ? objs[m]->d_dUi : objs[m]->d_Ui, keep[2 * m + q], n);
+  }
+  HIPCHK(hipStreamSynchronize(g.st));
+  for (cd *p : keep) dfree(p);
+  return rc;
+}
+
+// block-Jacobi fixed point over objects (mo_solve_bv, mo_solve.c:10-88).
+// trace (optional, iter_max+1 doubles) receives cc per iteration; *iters
+// receives the final iteration count.
+extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters)
+{
+  API_LOCK;
+  if (N <= 0) return fail(BH_EARG, "no objects");
+  if (ensure_rules()) return BH_EHIP;
+  for (int m = 0; m < N; m++)
+    if (!objs[m]->fmode || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d: factor and set beams first", m);
+  int rc, iter = 0;
+  const std::vector<double> c3 = cost_of(objs, N, 3), c2 = cost_of(objs, N, 2);
+  auto solve_all = [&]() { return run_lanes(objs, N, c2, [](bh_object *o) { return object_solve(o); }); };
+  if (N == 1) {
+    if ((rc = object_solve(objs[0]))) return rc;
+  } else {
+    // many solves per object: form the Schur complements once
+    if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return form_s(o); }))) return rc;
+    Coupling C;
+    if ((rc = C.init(objs, N))) return rc;
     bool first = true;
     auto sweep = [&]() -> int {
       for (int m = 0; m < N; m++) {
@@ -847,50 +952,30 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
         size_t n = (size_t)(o->Ne + 1) * 16;
         hipLaunchKernelGGL(k_reset_incident, g1(n), dim3(256), 0, g.st, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, n);
       }
-      // series pairs: coefficients of every source, then one evaluation launch
-      for (int n = 0; n < N; n++)
-        if (P.Lsrc[n] >= 0) { int r = mp_coefficients(objs[n], P, n, d_coef + coff[n]); if (r) return r; }
-      if (!tiles.empty()) {
-        bem_prof_begin(BH_K_MP_EVAL, evwork, g.st);
-        hipLaunchKernelGGL(k_mp_eval, dim3((unsigned)tiles.size()), dim3(64), 0, g.st, d_tiles, d_pairs);
-        bem_prof_end(BH_K_MP_EVAL, g.st);
-      }
-      // the other pairs: direct sums
-      for (int m = 0; m < N; m++)
-        for (int d = 0; d < N; d++)
-          if (d != m && P.Lmn[d][m] == -1) { int r = add_scattered(objs[d], objs[m], d_did, first, d_part); if (r) return r; }
-      HIPCHK(hipGetLastError());
+      int r = C.sums(objs, N, first);
+      if (r) return r;
       first = false;
       return solve_all();
     };
-    auto change = [&]() -> double {
-      for (int m = 0; m < N; m++) {
-        bh_object *o = objs[m];
-        hipLaunchKernelGGL(k_ccd, g1(o->nslot), dim3(256), 0, g.st, o->d_Ui, o->d_Uip, o->d_slotIdx, o->nslot,
-                           d_ccd + 2 * (size_t)slot0[m]);
-      }
-      hipLaunchKernelGGL(k_ccd_sum, dim3(1), dim3(256), 0, g.st, d_ccd, totslot, d_ccs);
-      double h[2];
-      if (hipMemcpyAsync(h, d_ccs, sizeof h, hipMemcpyDeviceToHost, g.st) != hipSuccess ||
-          hipStreamSynchronize(g.st) != hipSuccess)
-        return -1.0;
-      return (h[0] > h[1] ? h[0] : h[1]) / (double)totslot;
-    };
+    // BEM3_HIP_PROGRESS=1: one line per iteration on stderr; any other
+    // value but 0 names a file the lines are appended to (long solves)
     const char *pe = getenv("BEM3_HIP_PROGRESS");
-    const bool progress = pe && *pe && *pe != '0';
-    if (progress) {
-      fprintf(stderr, "bh_mo_solve: %d object pairs by series, %d by direct sums\n", P.npairs_mp, P.npairs_direct);
-      fflush(stderr);
+    FILE *pf = nullptr;
+    if (pe && *pe && strcmp(pe, "0") != 0) pf = strcmp(pe, "1") == 0 ? stderr : fopen(pe, "a");
+    struct Closer { FILE *f; ~Closer() { if (f && f != stderr) fclose(f); } } closer{pf};
+    if (pf) {
+      fprintf(pf, "bh_mo_solve: %d objects, %d object pairs by series, %d by direct sums\n", N, C.P.npairs_mp,
+              C.P.npairs_direct);
+      fflush(pf);
     }
     if ((rc = solve_all())) return rc;
     if ((rc = sweep())) return rc;
-    // BEM3_HIP_PROGRESS=1: one line per iteration on stderr (long solves)
     for (iter = 2; iter < iter_max; iter++) {
       if ((rc = sweep())) return rc;
-      double cc = change();
+      double cc = C.change(objs, N);
       if (cc < 0.0) return fail(BH_EHIP, "coupling loop: change measure");
       if (trace) trace[iter] = cc;
-      if (progress) { fprintf(stderr, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(stderr); }
+      if (pf) { fprintf(pf, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(pf); }
       if (cc < ccd && iter > iter_min) break;
     }
   }

@@ -76,6 +76,8 @@ int bh_object_gh_count(const bh_object *o, int did);
  * 4 dF (nd x 3 doubles: diagonal 1+F or F, dF/dt_zeta, dF/dt_eta).
  * Rows/columns follow the domain's element order, 3 node slots each.    */
 int bh_object_matrix(const bh_object *o, int did, int which, double *out);
+/* nrows selected rows (0-based) of the same matrices, one after another   */
+int bh_object_matrix_rows(const bh_object *o, int did, int which, int nrows, const int *rows, double *out);
 
 /* Incident plane waves: 12 doubles per beam (lambda0, n_surrounding,
  * power, re/im e0x, re/im e0y, fx, fy, fz, theta, phi) as read by
@@ -148,6 +150,12 @@ int bh_mo_set_coupling(int mode);
 /* ordered object pairs of the last coupling loop of several objects summed
  * by the series and by element sums, and the largest series order used   */
 int bh_mo_plan_stats(int *series, int *direct, int *lmax);
+/* The coupling sums of the loop (add_scattered_mEMP / mEMP2_s,
+ * mo_solve.c:296-396, summed as bh_mo_solve would sum them) from the
+ * objects' current boundary values, at object dst's node slots: out
+ * receives 2 x [Ne+1][4][4] complex (the scattered U, then dU/dn, in the
+ * incident-data layout); the objects' incident data are left unchanged.  */
+int bh_mo_coupling_sums(bh_object **objs, int N, int dst, double *out);
 /* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i, mo_EH_mEMP_s/t/i;
  * mo_field.c:11-282): what 0: E,H (6 complex per point), 1: U (4 complex),
  * 2: E,H from the potentials' gradients (EH_mEMP_s, d3b1_field.c:68-138,
