@@ -46,6 +46,7 @@ def lib():
             "bh_object_matrix": (C.c_int, [VP, C.c_int, C.c_int, D]),
             "bh_object_matrix_rows": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, Ip, D]),
             "bh_zgesv": (C.c_int, [C.c_int, C.c_int, D, D]),
+            "bh_zgesv_noalloc": (C.c_int, [C.c_int, C.c_int, VP, VP, VP, VP, VP]),
             "bh_object_set_beams": (C.c_int, [VP, C.c_int, D]),
             "bh_object_set_mfb": (C.c_int, [VP, C.c_int, D]),
             "bh_object_factor": (C.c_int, [VP, C.c_int]),

@@ -32,6 +32,9 @@ void bem_zgetrs(int, int, const void *, size_t, const int *, void *, size_t, int
 int bem_la_init(void);
 int bem_trsv_error(hipStream_t);
 void bem_la_release(void);
+int bem_la_noalloc_init(int);
+int bem_zgesv_noalloc(int, int, int, void *, void *, int *, int *, hipStream_t);
+int bem_noalloc_nmax(void);
 void bem_prof_release(void);
 void bem_mo_release(void);
 void bem_cgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
@@ -133,6 +136,8 @@ extern "C" int bh_init(int device)
   HIPCHK(hipEventCreate(&g.ev[0]));
   HIPCHK(hipEventCreate(&g.ev[1]));
   if (bem_la_init()) fprintf(stderr, "bem3_hip: could not raise the dynamic LDS limit (%s)\n", hipGetErrorString(hipGetLastError()));
+  // the allocation-free solve's work area: its device address, once per device
+  if (bem_la_noalloc_init(device)) return fail(BH_EHIP, "device address of the allocation-free work area");
   gl_nodes(12, g.xl, g.wl);
   gl_nodes(64, g.xh, g.wh);
   std::vector<QPt> a = tri_rule(12, g.xl, g.wl), b = tri_rule(64, g.xh, g.wh);
@@ -578,6 +583,21 @@ extern "C" int bh_zgesv(int n, int nrhs, double *A, double *B)
   dfree(dA); dfree(dB); dfree(work); dfree(piv); dfree(info);
   if (hinfo) return fail(BH_ESING, "singular matrix (zero pivot at %d)", hinfo);
   return trsv_failed(cur_st());
+}
+
+// A X = B on device buffers with no allocation (the caller passes A, B, piv
+// and info; the factorisation's work area is the library's fixed device
+// array, set up by bh_init); enqueue only
+extern "C" int bh_zgesv_noalloc(int n, int nrhs, void *A, void *B, int *piv, int *info, void *stream)
+{
+  API_LOCK;
+  if (g.dev < 0) return fail(BH_ESTATE, "bh_zgesv_noalloc: call bh_init first (it sets up the work area)");
+  if (n < 1 || nrhs < 1 || !A || !B || !piv || !info) return fail(BH_EARG, "bh_zgesv_noalloc: bad arguments");
+  if (n > bem_noalloc_nmax())
+    return fail(BH_EARG, "bh_zgesv_noalloc: n = %d exceeds the fixed work area (n <= %d)", n, bem_noalloc_nmax());
+  if (bem_zgesv_noalloc(g.dev, n, nrhs, A, B, piv, info, stream ? (hipStream_t)stream : g.st))
+    return fail(BH_EHIP, "bh_zgesv_noalloc: no work area on device %d", g.dev);
+  return 0;
 }
 
 extern "C" int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B,

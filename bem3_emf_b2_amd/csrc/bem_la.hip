@@ -1413,6 +1413,54 @@ static void getrs_t(int n, int nrhs, const typename Cx<R>::Z *A, size_t lda, con
   }
 }
 
+
+// ---- the allocation-free solve ----
+// The work area of the factorisation of bem_zgesv_noalloc lives in a fixed
+// device array of the library, sized for n <= NOALLOC_NMAX: the column-major
+// panel copy (64 columns), three row tables, the panel's pivots and the
+// per-block pivot candidates of the launch-per-column panel.  Its device
+// address is taken once per device at set-up (bem_la_noalloc_init, with the
+// device current); host code uses that address, never the array's name.
+// Every call on a GPU shares the array: calls that overlap on one GPU race.
+#define NOALLOC_NMAX 8192
+#define NOALLOC_NBLK ((NOALLOC_NMAX + LU_NT - 1) / LU_NT)
+#define NOALLOC_BYTES ((size_t)64 * NOALLOC_NMAX * sizeof(cd) + 3 * (size_t)NOALLOC_NMAX * sizeof(int) + \
+                       64 * sizeof(int) + 2 * (size_t)NOALLOC_NBLK * (sizeof(double) + sizeof(int)))
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+#define NOALLOC_NDEV 16
+static void *g_scratch_ptr[NOALLOC_NDEV];
+
+// the factorisation of getrf_t with its work area at ws and the
+// launch-per-column panel (no exchange areas, no memset: info is cleared by
+// a kernel), then the row interchanges and the recursive triangular solves
+// of B; enqueued on st, nothing waited for
+template <class R>
+static void gesv_noalloc_t(int n, int nrhs, typename Cx<R>::Z *A, typename Cx<R>::Z *B, int *piv, int *info, char *ws,
+                           hipStream_t st)
+{
+  const size_t ldp = ((size_t)n + 63) / 64 * 64;
+  const int nblk = (n + LU_NT - 1) / LU_NT;
+  LuWs w;
+  char *p = ws;
+  w.P = (void *)p;                       p += (size_t)64 * ldp * sizeof(cd);
+  w.ldp = ldp;
+  w.used = (int *)p;                     p += ldp * sizeof(int);
+  w.rowAt = (int *)p;                    p += ldp * sizeof(int);
+  w.pos = (int *)p;                      p += ldp * sizeof(int);
+  w.pv = (int *)p;                       p += 64 * sizeof(int);
+  w.partV = (double *)p;                 p += 2 * (size_t)nblk * sizeof(double);
+  w.partI = (int *)p;
+  w.nbmax = nblk;
+  w.info = info;
+  w.coop_ok = false;
+  hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(64), 0, st, (unsigned int *)info, (size_t)1);
+  lu_rec<R>(A, (size_t)n, n, 0, n, piv, w, st);
+  typedef typename Cx<R>::Z Z;
+  hipLaunchKernelGGL(k_laswp<Z>, dim3((nrhs + 255) / 256), dim3(256), 0, st, B, (size_t)nrhs, 0, nrhs, 0, n, piv);
+  trsm_lunit_rec<R>(A, (size_t)n, n, B, (size_t)nrhs, nrhs, st);
+  trsm_upper_rec<R>(A, (size_t)n, n, B, (size_t)nrhs, nrhs, st);
+}
+
 // the error flag of the one-launch triangular solves (a wait timed out);
 // clears it
 static int trsv_error(hipStream_t st)
@@ -1440,6 +1488,29 @@ int bem_la_init(void)
     g_coop = coop != 0;
   return 0;
 }
+
+// the device address of the allocation-free solve's work area on device dev
+// (the current device); 0 or -1
+int bem_la_noalloc_init(int dev)
+{
+  if (dev < 0 || dev >= NOALLOC_NDEV) return -1;
+  if (g_scratch_ptr[dev]) return 0;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+  g_scratch_ptr[dev] = ptr;
+  return 0;
+}
+
+// A X = B on device-resident row-major A (n x n) and B (n x nrhs) with the
+// fixed work area of device dev: 0, -1 (n beyond the work area, or no work
+// area on dev), enqueued on st only
+int bem_zgesv_noalloc(int dev, int n, int nrhs, void *A, void *B, int *piv, int *info, hipStream_t st)
+{
+  if (dev < 0 || dev >= NOALLOC_NDEV || !g_scratch_ptr[dev] || n < 1 || n > NOALLOC_NMAX || nrhs < 1) return -1;
+  gesv_noalloc_t<double>(n, nrhs, (cd *)A, (cd *)B, piv, info, (char *)g_scratch_ptr[dev], st);
+  return 0;
+}
+int bem_noalloc_nmax(void) { return NOALLOC_NMAX; }
 
 // nonzero if a one-launch triangular solve timed out since the last call
 int bem_trsv_error(hipStream_t st) { return trsv_error(st); }

@@ -180,6 +180,19 @@ int bh_mo_absorb(bh_object **objs, int N, int oid, int type, double *P);
  * device replacement of LAPACKE_zgetrf/zgetri/zgesv (solve_bieq2.c:389-399,
  * mo_solve.c:751); A is overwritten by its LU factors, B by X            */
 int bh_zgesv(int n, int nrhs, double *A, double *B);
+/* The same solve with no device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops); no
+ * counterpart in the reference.  A (n x n, row-major, overwritten by its LU
+ * factors), B (n x nrhs, overwritten by X), piv (n ints: 0-based pivot rows)
+ * and info (one int: 0, or the 1-based index of the first zero pivot) are
+ * device buffers of the caller; stream is the hipStream_t to enqueue on
+ * (NULL: the library's stream).  Enqueues kernels only: no allocation, no
+ * memset, no synchronisation.  The factorisation's work area is a fixed
+ * device array of the library (8.5 MB per GPU, held until the process
+ * ends), set up by bh_init (call it first: BH_ESTATE otherwise); n is at
+ * most 8192 (BH_EARG beyond).  Every call on a GPU shares that array:
+ * calls that overlap on one GPU race.                                     */
+int bh_zgesv_noalloc(int n, int nrhs, void *A, void *B, int *piv, int *info, void *stream);
 /* C = alpha A B + beta C (row-major, complex): the MFMA ZGEMM kernel     */
 int bh_zgemm_host(int m, int n, int k, const double *alpha, const double *A, const double *B, const double *beta,
                   double *C);
