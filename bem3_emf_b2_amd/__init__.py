@@ -73,6 +73,8 @@ def lib():
             "bh_mo_plan_stats": (C.c_int, [Ip, Ip, Ip]),
             "bh_mo_prepare": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int]),
             "bh_mo_coupling_sums": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D]),
+            "bh_mo_solve_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, C.c_double, C.c_int, C.c_int, Ip]),
+            "bh_object_beam_state": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -271,6 +273,18 @@ class Object:
         assert buf.size == n
         check(lib().bh_object_state(self.h, did, k, dp(buf.view(np.float64)), 1))
 
+    def beam_state(self, b, did, which):
+        """beam b's values after mo_solve_beams: U dU E dE H dH of domain
+        did, or Ui dUi Ui0 dUi0"""
+        keys = ["U", "dU", "E", "dE", "H", "dH", "Ui", "dUi", "Ui0", "dUi0"]
+        k = keys.index(which)
+        if k >= 6:
+            out = np.zeros((self.Ne + 1, 4, 4), np.complex128)
+        else:
+            out = np.zeros((self.dNe[did] + 1, 4, 4 if k < 2 else 3), np.complex128)
+        check(lib().bh_object_beam_state(self.h, b, did, k, dp(out.view(np.float64))))
+        return out
+
     def incident(self, which, value=None):
         k = ["Ui", "dUi", "Ui0", "dUi0"].index(which)
         n = (self.Ne + 1) * 16
@@ -362,6 +376,15 @@ def mo_coupling_sums(objs, dst):
     out = np.zeros(2 * n * 16, np.complex128)
     check(lib().bh_mo_coupling_sums(_handles(objs), len(objs), dst, dp(out.view(np.float64))))
     return out[:n * 16].reshape(n, 4, 4), out[n * 16:].reshape(n, 4, 4)
+
+
+def mo_solve_beams(objs, beams, ccd=1e-6, iter_min=10, iter_max=100):
+    """every beam (rows of 12 doubles) as a separate right-hand side of the
+    coupling loop; returns the iteration count"""
+    par = np.ascontiguousarray(beams, dtype=np.float64).reshape(-1, 12)
+    it = np.zeros(1, np.int32)
+    check(lib().bh_mo_solve_beams(_handles(objs), len(objs), par.shape[0], dp(par), ccd, iter_min, iter_max, ip(it)))
+    return int(it[0])
 
 
 def mo_solve(objs, ccd=1e-6, iter_min=10, iter_max=100):

@@ -702,6 +702,12 @@ void bem_launch_fbeam(const double *, const double *, const int *, int, const vo
 void bem_launch_stress(const void *, const double *, const double *, const double *, int, double, const double *,
                        double *, hipStream_t);
 void bem_launch_poynting(const void *, const double *, const double *, int, double *, hipStream_t);
+void bem_launch_meh_gather(int, int, const void *, const int *, const void *, const void *, size_t, void *, void *,
+                           hipStream_t);
+void bem_launch_meh_rows(int, int, const void *, const double *, const void *, const void *, const void *, size_t, double,
+                         double, void *, void *, size_t, hipStream_t);
+void bem_launch_mpack_eh(int, int, const void *, const int *, const void *, const void *, size_t, void *, hipStream_t);
+void bem_launch_munpack_eh(int, int, const void *, const int *, const void *, void *, void *, size_t, hipStream_t);
 }
 
 
@@ -1276,9 +1282,9 @@ extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *res
 
 // G x = b for 6 right-hand sides (nd x 6, b in R, x into R) with the
 // complex-FP32 LU of G, refined against the FP64 G
-static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *piv, cd *R)
+static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *piv, cd *R, int ncol = 6)
 {
-  const size_t nd = (size_t)D.nd, m = 6 * nd;
+  const size_t nd = (size_t)D.nd, m = (size_t)ncol * nd;
   cd *X = o->ws_eh[2].get<cd>(m), *Rr = o->ws_eh[3].get<cd>(m);
   cf *C = o->ws_eh[4].get<cf>(m);
   if (!X || !Rr || !C) return fail(BH_EHIP, "refinement of the boundary fields: work area");
@@ -1289,10 +1295,10 @@ static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *pi
   int it = 0, rc = 0;
   while (res > IR_TOL && it < IR_MAX) {
     bem_cvt_z2c(Rr, C, m, cur_st());
-    bem_cgetrs(D.nd, 6, Gc, nd, piv, C, 6, cur_st());
+    bem_cgetrs(D.nd, ncol, Gc, nd, piv, C, ncol, cur_st());
     bem_acc_c2z(X, C, m, cur_st());
     HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
-    bem_zgemm(D.nd, 6, D.nd, -1.0, 0.0, D.d_G, nd, X, 6, 1.0, 0.0, Rr, 6, cur_st());
+    bem_zgemm(D.nd, ncol, D.nd, -1.0, 0.0, D.d_G, nd, X, ncol, 1.0, 0.0, Rr, ncol, cur_st());
     const double nr = bem_nrm2(Rr, m, o->d_nrm, cur_st());
     if (nr < 0.0) { rc = fail(BH_EHIP, "norm of the residual"); break; }
     res = nr / nb;
@@ -1375,6 +1381,197 @@ extern "C" int bh_object_solve_EH(bh_object *o)
 {
   API_LOCK;
   return object_solve_EH(o);
+}
+
+
+// ------------------------------------------------------------------ many beams as right-hand sides
+__global__ void k_zero_cd_api(cd *__restrict__ a, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = mk(0.0, 0.0);
+}
+static void zero_cd(cd *a, size_t n) { hipLaunchKernelGGL(k_zero_cd_api, g1(n), dim3(256), 0, cur_st(), a, n); }
+
+cd *bem::mb_ptr(bh_object *o, int which) { return (cd *)o->mb_inc[which].p; }
+cd *bem::mb_dom_ptr(bh_object *o, int d, int which) { return (cd *)o->mb_dom[(size_t)d * 6 + which].p; }
+
+// nb plane waves as separate right-hand sides: device records and the
+// incident data of every beam at the open-domain slots (init_boundary_data_u2,
+// mo_setup.c:704-757, one beam at a time)
+int bem::object_beams_set(bh_object *o, int nb, const double *par)
+{
+  if (nb <= 0) return fail(BH_EARG, "no beams");
+  if (prepare_solve(o)) return BH_EHIP;
+  const size_t E1 = (size_t)o->Ne + 1;
+  for (int q = 0; q < 5; q++)
+    if (!o->mb_inc[q].get<cd>(E1 * 16 * nb)) return fail(BH_EHIP, "beams: incident data of %d beams", nb);
+  o->mb_dom.resize((size_t)(o->MN + 1) * 6);
+  for (int d = 0; d <= o->MN; d++)
+    for (int q = 0; q < 6; q++)
+      if (!o->mb_dom[(size_t)d * 6 + q].get<cd>((size_t)(o->dom[d].dNe + 1) * (q < 2 ? 16 : 12) * nb))
+        return fail(BH_EHIP, "beams: boundary values of %d beams", nb);
+  const size_t na = 8 * (size_t)o->NN;
+  for (int q = 0; q < 3; q++)
+    if (!o->mb_sys[q].get<cd>(na * nb)) return fail(BH_EHIP, "beams: systems of %d beams", nb);
+  std::vector<Beam> bm(nb);
+  for (int b = 0; b < nb; b++) bm[b] = plane_wave(par + 12 * (size_t)b);
+  Beam *db = o->mb_beams.get<Beam>(nb);
+  if (!db) return fail(BH_EHIP, "beams: records");
+  HIPCHK(hipMemcpyAsync(db, bm.data(), nb * sizeof(Beam), hipMemcpyHostToDevice, cur_st()));
+  for (int q = 0; q < 5; q++) zero_cd(mb_ptr(o, q), E1 * 16 * nb);
+  for (int b = 0; b < nb; b++) {
+    const size_t off = (size_t)b * E1 * 16;
+    bem_launch_incident_bd(o->d_slotPt, o->d_slotN, o->d_slotIdx, o->nslot, db + b, 1, mb_ptr(o, 2) + off,
+                           mb_ptr(o, 3) + off, mb_ptr(o, 0) + off, mb_ptr(o, 1) + off, mb_ptr(o, 4) + off, cur_st());
+  }
+  o->nbm = nb;
+  o->mb_xwarm = false;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(cur_st()));   // the host records stay alive until here
+  return 0;
+}
+
+// B, X = A^{-1} B and the boundary values of every beam at once: the FP32
+// block solve of the residual equations on all beams (its products and
+// triangular solves with 3 nb to 4 nb columns), the FP64 residuals of all
+// beams as two products per domain with 4 nb columns, until every beam's
+// relative residual is at most IR_TOL
+int bem::object_solve_multi(bh_object *o)
+{
+  if (o->fmode != 3 || !o->s_formed) return fail(BH_EUNSUP, "many beams: the mixed-precision form with S formed");
+  const int nb = o->nbm;
+  if (nb <= 0) return fail(BH_ESTATE, "many beams: no beams set");
+  const size_t NN = (size_t)o->NN, na = 8 * NN, E1 = (size_t)o->Ne + 1;
+  cd *B = (cd *)o->mb_sys[0].p, *X = (cd *)o->mb_sys[1].p, *R = (cd *)o->mb_sys[2].p;
+  cf *crhs = o->mb_ws[0].get<cf>(na * nb), *cdx = o->mb_ws[1].get<cf>(na * nb);
+  cf *cw = o->mb_ws[2].get<cf>(schur_work_multi(NN, nb));
+  size_t ndmax = 0;
+  for (int d = 0; d <= o->MN; d++) ndmax = std::max(ndmax, (size_t)o->dom[d].nd);
+  cd *rw = o->mb_ws[3].get<cd>(12 * ndmax * nb);
+  double *nrm = o->mb_ws[4].get<double>(2 * (size_t)nb);
+  if (!crhs || !cdx || !cw || !rw || !nrm) return fail(BH_EHIP, "many beams: work areas");
+  zero_cd(B, na * nb);
+  for (int b = 0; b < nb; b++)
+    for (int d = 0; d <= o->MN; d++) {
+      Dom &D = o->dom[d];
+      bem_launch_build_B(D.d_G, D.d_H, D.nd, D.nd, D.d_col, mb_ptr(o, 2) + (size_t)b * E1 * 16,
+                         mb_ptr(o, 3) + (size_t)b * E1 * 16, row_base(o, d), B + (size_t)b * na, cur_st());
+    }
+  auto resid = [&]() {
+    for (int d = 0; d <= o->MN; d++) {
+      Dom &D = o->dom[d];
+      bem_resid_domain_multi(D.d_G, D.d_H, D.nd, D.nd, D.d_col, X, NN, row_base(o, d), B, R, rw, nb, cur_st());
+    }
+  };
+  std::vector<double> hb(nb), hr(nb);
+  bem_mnrm2(B, na, nb, nrm, cur_st());
+  HIPCHK(hipMemcpyAsync(hb.data(), nrm, nb * sizeof(double), hipMemcpyDeviceToHost, cur_st()));
+  if (o->mb_xwarm) resid();
+  else {
+    zero_cd(X, na * nb);
+    HIPCHK(hipMemcpyAsync(R, B, na * nb * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+  }
+  o->mb_xwarm = false;
+  const SchurF<cf> f = schur32(o);
+  int it = 0;
+  double worst = 0.0;
+  for (;;) {
+    bem_mnrm2(R, na, nb, nrm + nb, cur_st());
+    HIPCHK(hipMemcpyAsync(hr.data(), nrm + nb, nb * sizeof(double), hipMemcpyDeviceToHost, cur_st()));
+    HIPCHK(hipStreamSynchronize(cur_st()));
+    worst = 0.0;
+    for (int b = 0; b < nb; b++) worst = std::max(worst, hb[b] > 0.0 ? sqrt(hr[b] / hb[b]) : 0.0);
+    if (worst <= IR_TOL || it >= IR_MAX) break;
+    bem_cvt_z2c(R, crhs, na * nb, cur_st());
+    if (schur_apply_multi(f, crhs, true, cdx, cw, nb, cur_st())) return fail(BH_ESTATE, "many beams: S is not formed");
+    bem_acc_c2z(X, cdx, na * nb, cur_st());
+    resid();
+    it++;
+  }
+  o->mb_iters = it;
+  o->mb_res = worst;
+  if (!(worst <= IR_TOL))
+    return fail(BH_ECONV, "many beams: relative residual %.3g after %d refinement steps", worst, it);
+  o->mb_xwarm = true;
+  for (int b = 0; b < nb; b++)
+    for (int d = 0; d <= o->MN; d++) {
+      Dom &D = o->dom[d];
+      const size_t dn = (size_t)(D.dNe + 1) * 16;
+      bem_launch_store_X(X + (size_t)b * na, D.nd, D.d_col, NN, D.d_colS, mb_ptr(o, 2) + (size_t)b * E1 * 16,
+                         mb_ptr(o, 3) + (size_t)b * E1 * 16, mb_dom_ptr(o, d, 0) + b * dn, mb_dom_ptr(o, d, 1) + b * dn,
+                         cur_st());
+    }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(cur_st()));
+  return 0;
+}
+
+// boundary E, H and their normal derivatives of every beam (solve_eh_bv +
+// solve_deh_bv, mo_solve.c:547-789): the tangential derivative rows as two
+// products dG [dU], dH [U] with 4 nb columns, then G [dE|dH] = H [E|H] with
+// 6 nb columns (the FP32 LU of G refined in FP64)
+int bem::object_solve_EH_multi(bh_object *o)
+{
+  if (o->fmode != 3) return fail(BH_EUNSUP, "many beams: the mixed-precision form");
+  const int nb = o->nbm;
+  zd ch = o->lambda0 / (2.0 * M_PI * zd(0.0, 1.0));
+  HIPCHK(hipMemsetAsync(o->d_info, 0, sizeof(int), cur_st()));
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    const size_t nd = (size_t)D.nd, us = (size_t)(D.dNe + 1) * 16, es = (size_t)(D.dNe + 1) * 12;
+    cd *PU = o->mb_ws[5].get<cd>(2 * nd * 6 * nb), *PdU = PU ? PU + 4 * nd * nb : nullptr;
+    cd *Z = o->mb_ws[6].get<cd>(2 * nd * 4 * nb);
+    if (!PU || !Z) return fail(BH_EHIP, "many beams: boundary-field work areas");
+    bem_launch_meh_gather(D.nd, nb, D.d_col, D.d_colS, mb_dom_ptr(o, d, 0), mb_dom_ptr(o, d, 1), us, PU, PdU, cur_st());
+    bem_zgemm(2 * D.nd, 4 * nb, D.nd, 1.0, 0.0, D.d_dG, nd, PdU, 4 * nb, 0.0, 0.0, Z, 4 * nb, cur_st());
+    bem_zgemm(2 * D.nd, 4 * nb, D.nd, -1.0, 0.0, D.d_dH, nd, PU, 4 * nb, 1.0, 0.0, Z, 4 * nb, cur_st());
+    bem_launch_meh_rows(D.nd, nb, Z, D.d_dF, D.d_rowEH, mb_dom_ptr(o, d, 0), mb_dom_ptr(o, d, 1), us, ch.real(),
+                        ch.imag(), mb_dom_ptr(o, d, 2), mb_dom_ptr(o, d, 4), es, cur_st());
+    // G [dE|dH] = H [E|H], 6 nb columns
+    const cf *Gc = D.c_Gf;
+    const int *Gpiv = D.d_Gpiv;
+    if (d == 0) { Gc = o->c_G0f; Gpiv = o->d_piv0; }
+    else if (!D.gfc_valid) {
+      if (!D.c_Gf && dalloc(&D.c_Gf, nd * nd)) return BH_EHIP;
+      if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
+      bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, cur_st());
+      if (bem_cgetrf(D.nd, D.c_Gf, nd, D.d_Gpiv, o->d_info, cur_st())) return lu_failed();
+      D.gfc_valid = true;
+      D.gf_valid = false;
+      Gc = D.c_Gf;
+      Gpiv = D.d_Gpiv;
+    }
+    cd *P = PU, *Rm = Z;   // reused: [nd x 6nb] each
+    bem_launch_mpack_eh(D.nd, nb, D.d_col, D.d_colS, mb_dom_ptr(o, d, 2), mb_dom_ptr(o, d, 4), es, P, cur_st());
+    bem_zgemm(D.nd, 6 * nb, D.nd, 1.0, 0.0, D.d_H, nd, P, 6 * nb, 0.0, 0.0, Rm, 6 * nb, cur_st());
+    int rc = solve_g_mixed(o, D, Gc, Gpiv, Rm, 6 * nb);
+    if (rc) return rc;
+    bem_launch_munpack_eh(D.nd, nb, D.d_col, D.d_colS, Rm, mb_dom_ptr(o, d, 3), mb_dom_ptr(o, d, 5), es, cur_st());
+  }
+  HIPCHK(hipGetLastError());
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, o->d_info, sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
+  if (info) return fail(BH_ESING, "singular G (zero pivot %d)", info);
+  return 0;
+}
+
+// one beam's boundary values of one domain (which as bh_object_state) or
+// incident data (which 6 + 0..3: Ui, dUi, Ui0, dUi0) after bh_mo_solve_beams
+extern "C" int bh_object_beam_state(bh_object *o, int b, int did, int which, double *buf)
+{
+  API_LOCK;
+  if (b < 0 || b >= o->nbm) return fail(BH_EARG, "beam %d of %d", b, o->nbm);
+  if (which >= 6 && which < 10) {
+    const int map[4] = {2, 3, 0, 1};
+    const size_t n = ((size_t)o->Ne + 1) * 16;
+    HIPCHK(hipMemcpy(buf, mb_ptr(o, map[which - 6]) + b * n, n * sizeof(cd), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  if (did < 0 || did > o->MN || which < 0 || which > 5) return fail(BH_EARG, "bad state selector");
+  const size_t n = (size_t)(o->dom[did].dNe + 1) * (which < 2 ? 16 : 12);
+  HIPCHK(hipMemcpy(buf, mb_dom_ptr(o, did, which) + b * n, n * sizeof(cd), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 // which: 0 U, 1 dU ([dNe+1][4][4]), 2 E, 3 dE, 4 H, 5 dH ([dNe+1][4][3]);

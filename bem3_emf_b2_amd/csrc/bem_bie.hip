@@ -196,6 +196,78 @@ __global__ void k_scale_rows(cd *__restrict__ X, const ColInfo *__restrict__ col
   if (j < nd) X[j] = X[j] * (alpha * col[j].n[comp]);
 }
 
+
+// ---- many beams: the boundary E, H steps on all beams at once ----
+// PU[j][4b+i], PdU[j][4b+i]: U, dU of beam b at the node of column j (U_b
+// at U + b ustride)
+__global__ void k_meh_gather(int nd, int nb, const ColInfo *__restrict__ col, const int *__restrict__ colS,
+                             const cd *__restrict__ U, const cd *__restrict__ dU, size_t ustride, cd *__restrict__ PU,
+                             cd *__restrict__ PdU)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nd * nb) return;
+  const int j = (int)(i / nb), b = (int)(i % nb);
+  const size_t o = ((size_t)colS[j] * 4 + col[j].slot) * 4 + (size_t)b * ustride;
+  for (int k = 0; k < 4; k++) { PU[i * 4 + k] = U[o + k]; PdU[i * 4 + k] = dU[o + k]; }
+}
+
+// the finish of k_eh_rows for row r and beam b from the tangential
+// products Z[2r][4b+i] (zeta) and Z[2r+1][4b+i] (eta)
+__global__ void k_meh_rows(int nd, int nb, const cd *__restrict__ Z, const double *__restrict__ dF,
+                           const RowEH *__restrict__ rows, const cd *__restrict__ U, const cd *__restrict__ dU,
+                           size_t ustride, cd ch, cd *__restrict__ E, cd *__restrict__ H, size_t estride)
+{
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nd * nb) return;
+  const int r = (int)(t / nb), b = (int)(t % nb);
+  const RowEH rw = rows[r];
+  const size_t ot = ((size_t)rw.s * 4 + rw.slot) * 4 + (size_t)b * ustride;
+  const double f0 = dF[3 * r], f1 = dF[3 * r + 1], f2 = dF[3 * r + 2];
+  const cd *zz = Z + (size_t)(2 * r) * 4 * nb + 4 * b, *ze = Z + (size_t)(2 * r + 1) * 4 * nb + 4 * b;
+  cd Ut[4], dUt[4], dz[4], de[4];
+  for (int i = 0; i < 4; i++) {
+    Ut[i] = U[ot + i]; dUt[i] = dU[ot + i];
+    dz[i] = (zz[i] - Ut[i] * f1) * (1.0 / f0);
+    de[i] = (ze[i] - Ut[i] * f2) * (1.0 / f0);
+  }
+  const double *n = rw.vn, *a = rw.vtz, *bb = rw.vte;
+  double det = n[0] * a[1] * bb[2] + n[1] * a[2] * bb[0] + n[2] * a[0] * bb[1] -
+               (n[2] * a[1] * bb[0] + n[1] * a[0] * bb[2] + n[0] * a[2] * bb[1]);
+  double id = 1.0 / det, m[9];
+  m[0] = a[1] * bb[2] - a[2] * bb[1]; m[1] = -n[1] * bb[2] + n[2] * bb[1]; m[2] = n[1] * a[2] - n[2] * a[1];
+  m[3] = -a[0] * bb[2] + a[2] * bb[0]; m[4] = n[0] * bb[2] - n[2] * bb[0]; m[5] = -n[0] * a[2] + n[2] * a[0];
+  m[6] = a[0] * bb[1] - a[1] * bb[0]; m[7] = -n[0] * bb[1] + n[1] * bb[0]; m[8] = n[0] * a[1] - n[1] * a[0];
+  for (int i = 0; i < 9; i++) m[i] *= id;
+  cd grd[4][3];
+  for (int c = 0; c < 4; c++)
+    for (int i = 0; i < 3; i++) grd[c][i] = dUt[c] * m[i * 3] + dz[c] * m[i * 3 + 1] + de[c] * m[i * 3 + 2];
+  const size_t oe = ((size_t)rw.s * 4 + rw.slot) * 3 + (size_t)b * estride;
+  for (int i = 0; i < 3; i++) E[oe + i] = Ut[i] - grd[3][i];
+  H[oe + 0] = ch * (grd[2][1] - grd[1][2]);
+  H[oe + 1] = ch * (grd[0][2] - grd[2][0]);
+  H[oe + 2] = ch * (grd[1][0] - grd[0][1]);
+}
+
+// [E|H] of every beam in column order, P[j][6b+i], and back (dE, dH)
+__global__ void k_mpack_eh(int nd, int nb, const ColInfo *__restrict__ col, const int *__restrict__ colS,
+                           const cd *__restrict__ E, const cd *__restrict__ H, size_t estride, cd *__restrict__ P)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nd * nb) return;
+  const int j = (int)(i / nb), b = (int)(i % nb);
+  const size_t o = ((size_t)colS[j] * 4 + col[j].slot) * 3 + (size_t)b * estride;
+  for (int k = 0; k < 3; k++) { P[i * 6 + k] = E[o + k]; P[i * 6 + 3 + k] = H[o + k]; }
+}
+__global__ void k_munpack_eh(int nd, int nb, const ColInfo *__restrict__ col, const int *__restrict__ colS,
+                             const cd *__restrict__ P, cd *__restrict__ dE, cd *__restrict__ dH, size_t estride)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nd * nb) return;
+  const int j = (int)(i / nb), b = (int)(i % nb);
+  const size_t o = ((size_t)colS[j] * 4 + col[j].slot) * 3 + (size_t)b * estride;
+  for (int k = 0; k < 3; k++) { dE[o + k] = P[i * 6 + k]; dH[o + k] = P[i * 6 + 3 + k]; }
+}
+
 extern "C" {
 void bem_launch_build_A(const void *G, const void *H, int nd, size_t ld, const void *col, size_t rowBase, size_t NN,
                         void *A, size_t na, hipStream_t st)
@@ -249,5 +321,33 @@ void bem_launch_scale_rows(void *X, const void *col, int nd, int comp, double ar
 {
   hipLaunchKernelGGL(k_scale_rows, dim3((nd + 255) / 256), dim3(256), 0, st, (cd *)X, (const ColInfo *)col, nd, comp,
                      mk(ar, ai));
+}
+void bem_launch_meh_gather(int nd, int nb, const void *col, const int *colS, const void *U, const void *dU, size_t ustride,
+                           void *PU, void *PdU, hipStream_t st)
+{
+  const size_t n = (size_t)nd * nb;
+  hipLaunchKernelGGL(k_meh_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nd, nb, (const ColInfo *)col, colS,
+                     (const cd *)U, (const cd *)dU, ustride, (cd *)PU, (cd *)PdU);
+}
+void bem_launch_meh_rows(int nd, int nb, const void *Z, const double *dF, const void *rows, const void *U, const void *dU,
+                         size_t ustride, double chr, double chi, void *E, void *H, size_t estride, hipStream_t st)
+{
+  const size_t n = (size_t)nd * nb;
+  hipLaunchKernelGGL(k_meh_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nd, nb, (const cd *)Z, dF,
+                     (const RowEH *)rows, (const cd *)U, (const cd *)dU, ustride, mk(chr, chi), (cd *)E, (cd *)H, estride);
+}
+void bem_launch_mpack_eh(int nd, int nb, const void *col, const int *colS, const void *E, const void *H, size_t estride,
+                         void *P, hipStream_t st)
+{
+  const size_t n = (size_t)nd * nb;
+  hipLaunchKernelGGL(k_mpack_eh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nd, nb, (const ColInfo *)col, colS,
+                     (const cd *)E, (const cd *)H, estride, (cd *)P);
+}
+void bem_launch_munpack_eh(int nd, int nb, const void *col, const int *colS, const void *P, void *dE, void *dH,
+                           size_t estride, hipStream_t st)
+{
+  const size_t n = (size_t)nd * nb;
+  hipLaunchKernelGGL(k_munpack_eh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nd, nb, (const ColInfo *)col, colS,
+                     (const cd *)P, (cd *)dE, (cd *)dH, estride);
 }
 }

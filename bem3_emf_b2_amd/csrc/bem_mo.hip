@@ -51,6 +51,8 @@
 using namespace bem;
 
 extern "C" {
+void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
+               hipStream_t);
 void bem_launch_scatter(const double *, const double *, const int *, int, const void *, const void *, const int *, int,
                         double, double, const void *, int, const void *, int, double, const void *, int, const void *,
                         int, const void *, const void *, const void *, void *, void *, int, double *, hipStream_t);
@@ -155,6 +157,10 @@ struct MpSrc {
 
 // lane per source point q (element 1 + q / npt, rule point q % npt): the
 // 16 densities and conj(R_n^m(y - c)) of every (n, m) (layout [q][idx])
+// ROWQ: B[q][idx] (rows per point, the moment kernel's layout); otherwise
+// B[idx][q] (row-major idx x points: the A operand of the many-beam moment
+// product).  dens null: no densities (the many-beam path forms its own).
+template <bool ROWQ>
 __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
                                                  const QPt *__restrict__ rule, const cd *__restrict__ U,
                                                  const cd *__restrict__ dU, int q0, int nqc, cd *__restrict__ B,
@@ -179,9 +185,10 @@ __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restri
   const double wt = qp.w * S.scale;
   const double sg = sd < 0 ? -1.0 : 1.0;
   const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
-  cd *dq = dens + (size_t)ql * MP_NDENS;
+  cd *dq = dens ? dens + (size_t)ql * MP_NDENS : nullptr;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
+    if (!dq) break;
     const cd u = Us[c] * n0 + Us[4 + c] * n1 + Us[8 + c] * n2;
     const cd du = dUs[c] * n0 + dUs[4 + c] * n1 + dUs[8 + c] * n2;
     const cd a = du * (wt * g.aW), b = u * (-sg * wt);
@@ -196,7 +203,8 @@ __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restri
   const double ir = r > 0.0 ? 1.0 / r : 0.0;
   const double u = r > 0.0 ? y[2] * ir : 1.0;
   const cd w = mk(y[0] * ir, y[1] * ir);
-  cd *Bq = B + (size_t)ql * ldb;
+  cd *Bq = ROWQ ? B + (size_t)ql * ldb : B + ql;
+  const size_t bs = ROWQ ? 1 : ldb;   // stride between coefficients
   // m-outer, n-inner; P^_m^m = -sqrt((2m+1)/2m) P^_{m-1}^{m-1}
   double pmm = 0.28209479177387814347;   // 1 / sqrt(4 pi)
   cd wm = mk(1.0, 0.0);
@@ -216,8 +224,8 @@ __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restri
       if (n > m) { p2 = p1; p1 = pn; }
       const double jr = jt[n * 64 + lane] * pn;
       const int i = mp_idx(n, m, Lp);
-      Bq[i] = cwm * jr;                          // conj(R_n^m)
-      if (m > 0) Bq[i + 1] = wm * (sgn * jr);    // conj(R_n^{-m}) = (-1)^m R_n^m
+      Bq[i * bs] = cwm * jr;                          // conj(R_n^m)
+      if (m > 0) Bq[(i + 1) * bs] = wm * (sgn * jr);  // conj(R_n^{-m}) = (-1)^m R_n^m
     }
   }
 }
@@ -264,30 +272,36 @@ __global__ void k_mp_reduce(int n, int nsplit, const cd *__restrict__ T, cd *__r
 }
 
 // moment of density set d (0 monopole, 1 z, 2 +, 3 -) and potential c at (n, m)
-__device__ __forceinline__ cd mp_T(const cd *T, int Lp, int n, int m, int d, int c)
+// (T rows of ldt values; a beam's 16 moments start at column boff)
+__device__ __forceinline__ cd mp_T(const cd *T, int Lp, int n, int m, int d, int c, size_t ldt = MP_NDENS, int boff = 0)
 {
   if (n < 0 || n > Lp || m > n || -m > n) return mk(0.0, 0.0);
-  return T[(size_t)mp_idx(n, m, Lp) * MP_NDENS + d * 4 + c];
+  return T[(size_t)mp_idx(n, m, Lp) * ldt + boff + d * 4 + c];
 }
 // series coefficient A_nm of potential c (zero beyond the order L)
-__device__ cd mp_A(const cd *T, int L, int Lp, double k, int n, int m, int c)
+__device__ cd mp_A(const cd *T, int L, int Lp, double k, int n, int m, int c, size_t ldt = MP_NDENS, int boff = 0)
 {
   if (n < 0 || n > L || m > n || -m > n) return mk(0.0, 0.0);
-  cd s = mp_T(T, Lp, n, m, 0, c);
-  s = s + (mp_T(T, Lp, n - 1, m, 1, c) * cz1(n, m) + mp_T(T, Lp, n + 1, m, 1, c) * cz2(n, m)) * k;
-  s = s + (mp_T(T, Lp, n - 1, m - 1, 2, c) * cm1(n, m) + mp_T(T, Lp, n + 1, m - 1, 2, c) * cm2(n, m)) * k;
-  s = s + (mp_T(T, Lp, n - 1, m + 1, 3, c) * cp1(n, m) + mp_T(T, Lp, n + 1, m + 1, 3, c) * cp2(n, m)) * k;
+  cd s = mp_T(T, Lp, n, m, 0, c, ldt, boff);
+  s = s + (mp_T(T, Lp, n - 1, m, 1, c, ldt, boff) * cz1(n, m) + mp_T(T, Lp, n + 1, m, 1, c, ldt, boff) * cz2(n, m)) * k;
+  s = s + (mp_T(T, Lp, n - 1, m - 1, 2, c, ldt, boff) * cm1(n, m) +
+           mp_T(T, Lp, n + 1, m - 1, 2, c, ldt, boff) * cm2(n, m)) * k;
+  s = s + (mp_T(T, Lp, n - 1, m + 1, 3, c, ldt, boff) * cp1(n, m) +
+           mp_T(T, Lp, n + 1, m + 1, 3, c, ldt, boff) * cp2(n, m)) * k;
   return s * mk(0.0, 4.0 * M_PI * k);
 }
 
 // coefficient sets of the evaluation, [idx][set][c]: set 0 A (U), 1 the
 // coefficients of d_z U, 2 of d_+ U, 3 of d_- U
-__global__ void k_mp_coef(int L, int Lp, double k, const cd *__restrict__ T, cd *__restrict__ C)
+// nb beams: T and C rows hold 16 values per beam ([idx][b 16 + ...])
+__global__ void k_mp_coef(int L, int Lp, double k, const cd *__restrict__ T, cd *__restrict__ C, int nb = 1)
 {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nidx = mp_count(Lp);
-  if (t >= nidx * 4) return;
-  const int i = t >> 2, c = t & 3;
+  if (t >= (size_t)nidx * 4 * nb) return;
+  const int i = (int)(t / (4 * (size_t)nb)), q = (int)(t % (4 * (size_t)nb)), b = q >> 2, c = q & 3;
+  const size_t ldt = (size_t)MP_NDENS * nb;
+  const int boff = b * MP_NDENS;
   // (n, m) of index i
   int n = 0, m = 0;
   if (i <= Lp) { n = i; m = 0; }
@@ -298,22 +312,114 @@ __global__ void k_mp_coef(int L, int Lp, double k, const cd *__restrict__ T, cd 
     n = am + r / 2;
     m = (r & 1) ? -am : am;
   }
-  cd *o = C + (size_t)i * 16;
-  o[c] = mp_A(T, L, Lp, k, n, m, c);
+  cd *o = C + (size_t)i * ldt + boff;
+  o[c] = mp_A(T, L, Lp, k, n, m, c, ldt, boff);
   cd dz = mk(0.0, 0.0), dp = mk(0.0, 0.0), dm = mk(0.0, 0.0);
   if (n + 1 <= Lp) {
-    if (m <= n + 1 && -m <= n + 1) dz = dz + mp_A(T, L, Lp, k, n + 1, m, c) * cz1(n + 1, m);
-    if (m - 1 <= n + 1 && 1 - m <= n + 1) dp = dp + mp_A(T, L, Lp, k, n + 1, m - 1, c) * cp1(n + 1, m - 1);
-    if (m + 1 <= n + 1 && -m - 1 <= n + 1) dm = dm + mp_A(T, L, Lp, k, n + 1, m + 1, c) * cm1(n + 1, m + 1);
+    if (m <= n + 1 && -m <= n + 1) dz = dz + mp_A(T, L, Lp, k, n + 1, m, c, ldt, boff) * cz1(n + 1, m);
+    if (m - 1 <= n + 1 && 1 - m <= n + 1) dp = dp + mp_A(T, L, Lp, k, n + 1, m - 1, c, ldt, boff) * cp1(n + 1, m - 1);
+    if (m + 1 <= n + 1 && -m - 1 <= n + 1) dm = dm + mp_A(T, L, Lp, k, n + 1, m + 1, c, ldt, boff) * cm1(n + 1, m + 1);
   }
   if (n - 1 >= 0) {
-    if (m <= n - 1 && -m <= n - 1) dz = dz + mp_A(T, L, Lp, k, n - 1, m, c) * cz2(n - 1, m);
-    if (m - 1 <= n - 1 && 1 - m <= n - 1) dp = dp + mp_A(T, L, Lp, k, n - 1, m - 1, c) * cp2(n - 1, m - 1);
-    if (m + 1 <= n - 1 && -m - 1 <= n - 1) dm = dm + mp_A(T, L, Lp, k, n - 1, m + 1, c) * cm2(n - 1, m + 1);
+    if (m <= n - 1 && -m <= n - 1) dz = dz + mp_A(T, L, Lp, k, n - 1, m, c, ldt, boff) * cz2(n - 1, m);
+    if (m - 1 <= n - 1 && 1 - m <= n - 1) dp = dp + mp_A(T, L, Lp, k, n - 1, m - 1, c, ldt, boff) * cp2(n - 1, m - 1);
+    if (m + 1 <= n - 1 && -m - 1 <= n - 1) dm = dm + mp_A(T, L, Lp, k, n - 1, m + 1, c, ldt, boff) * cm2(n - 1, m + 1);
   }
   o[4 + c] = dz * k;
   o[8 + c] = dp * k;
   o[12 + c] = dm * k;
+}
+
+
+// ------------------------------------------------------------------ many beams
+// the 16 densities of every beam at a source point: dens[q][b 16 + d]
+// (beam b's boundary values at U + b ustride)
+__global__ void __launch_bounds__(64) k_mp_dens_multi(MpSrc S, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
+                                                      const QPt *__restrict__ rule, const cd *__restrict__ U,
+                                                      const cd *__restrict__ dU, size_t ustride, int nb, int q0, int nqc,
+                                                      cd *__restrict__ dens)
+{
+  const int ql = blockIdx.x * 64 + threadIdx.x;
+  if (ql >= nqc) return;
+  const int q = q0 + ql;
+  const int s = 1 + q / S.npt, p = q % S.npt;
+  const int sd = sid[s], as = sd < 0 ? -sd : sd;
+  const TriGeo g = geo[as];
+  const QPt qp = rule[S.qoff + p];
+  double n0, n1, n2;
+  tri_N(qp.z, qp.e, n0, n1, n2);
+  const double wt = qp.w * S.scale, sg = sd < 0 ? -1.0 : 1.0;
+  const cd wzp = mk(0.5 * g.W[0], -0.5 * g.W[1]), wzm = mk(0.5 * g.W[0], 0.5 * g.W[1]);
+  cd *dq = dens + (size_t)ql * MP_NDENS * nb;
+  for (int b = 0; b < nb; b++) {
+    const cd *Us = U + (size_t)b * ustride + (size_t)s * 16, *dUs = dU + (size_t)b * ustride + (size_t)s * 16;
+    for (int c = 0; c < 4; c++) {
+      const cd u = Us[c] * n0 + Us[4 + c] * n1 + Us[8 + c] * n2;
+      const cd du = dUs[c] * n0 + dUs[4 + c] * n1 + dUs[8 + c] * n2;
+      const cd bb = u * (-sg * wt);
+      dq[b * MP_NDENS + c] = du * (wt * g.aW);
+      dq[b * MP_NDENS + 4 + c] = bb * g.W[2];
+      dq[b * MP_NDENS + 8 + c] = bb * wzp;
+      dq[b * MP_NDENS + 12 + c] = bb * wzm;
+    }
+  }
+}
+
+// S_n^m of every index at points x_p - c: out[p][idx] (lane per point)
+__global__ void __launch_bounds__(64) k_mp_sbasis(const double *__restrict__ pts, int np, double c0, double c1, double c2,
+                                                  double kr, int Lp, cd *__restrict__ out)
+{
+  __shared__ cd ht[(MP_LPMAX + 1) * 64];
+  const int lane = threadIdx.x, p = blockIdx.x * 64 + lane;
+  if (p >= np) return;
+  const double y[3] = {pts[3 * p] - c0, pts[3 * p + 1] - c1, pts[3 * p + 2] - c2};
+  const double r = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]), ir = 1.0 / r;
+  sph_h(kr * r, Lp, ht + lane, 64);
+  const double u = y[2] * ir;
+  const cd w = mk(y[0] * ir, y[1] * ir);
+  cd *o = out + (size_t)p * mp_count(Lp);
+  double pmm = 0.28209479177387814347;
+  cd wm = mk(1.0, 0.0);
+  for (int m = 0; m <= Lp; m++) {
+    if (m > 0) {
+      pmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m));
+      wm = wm * w;
+    }
+    const cd cwm = conjd(wm);
+    const double sgn = (m & 1) ? -1.0 : 1.0;
+    double p2 = 0.0, p1 = pmm;
+    for (int n = m; n <= Lp; n++) {
+      double pn;
+      if (n == m) pn = pmm;
+      else if (n == m + 1) pn = sqrt(2.0 * m + 3.0) * u * pmm;
+      else pn = pa(n, m) * (u * p1 - pb(n, m) * p2);
+      if (n > m) { p2 = p1; p1 = pn; }
+      const cd hp = ht[n * 64 + lane] * pn;
+      const int i = mp_idx(n, m, Lp);
+      o[i] = hp * wm;
+      if (m > 0) o[i + 1] = hp * cwm * sgn;
+    }
+  }
+}
+
+// the series products C[p][b 16 + set 4 + c] of destination slots s0 + p
+// into the incident data of every beam: U += set 0, dU/dv += v_z set 1 +
+// (v_x - i v_y)/2 set 2 + (v_x + i v_y)/2 set 3
+__global__ void k_mp_combine(const cd *__restrict__ C, int ns, int nb, const double *__restrict__ nrm,
+                             const int *__restrict__ slot, int s0, cd *__restrict__ Ui, cd *__restrict__ dUi,
+                             size_t istride)
+{
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ns * nb) return;
+  const int p = (int)(t / nb), b = (int)(t % nb), sl = s0 + p;
+  const double *v = nrm + 3 * (size_t)sl;
+  const cd vm = mk(0.5 * v[0], -0.5 * v[1]), vp = mk(0.5 * v[0], 0.5 * v[1]);
+  const cd *c = C + ((size_t)p * nb + b) * MP_NDENS;
+  const size_t o = (size_t)b * istride + (size_t)slot[sl] * 4;
+  for (int k = 0; k < 4; k++) {
+    Ui[o + k] = Ui[o + k] + c[k];
+    dUi[o + k] = dUi[o + k] + c[4 + k] * v[2] + c[8 + k] * vm + c[12 + k] * vp;
+  }
 }
 
 // ------------------------------------------------------------------ evaluation
@@ -457,7 +563,7 @@ int g_last_series = 0, g_last_direct = 0, g_last_lmax = -1;   // the last loop's
 
 // grow-only work areas of the loop (bem_mo_release frees them)
 Scratch *g_ws = nullptr;
-enum { W_DID, W_PART, W_CCD, W_CCDSUM, W_B, W_DENS, W_T, W_TSUM, W_COEF, W_TILES, W_PAIRS, W_NWS };
+enum { W_DID, W_PART, W_CCD, W_CCDSUM, W_B, W_DENS, W_T, W_TSUM, W_COEF, W_TILES, W_PAIRS, W_MS, W_MC, W_NWS };
 template <class T> T *ws(int i, size_t n)
 {
   if (!g_ws) g_ws = new Scratch[W_NWS];
@@ -613,8 +719,8 @@ int mp_coefficients(bh_object *o, const Plan &P, int n, cd *coef)
   for (int q0 = 0; q0 < S.nq; q0 += qc) {
     const int nqc = std::min(qc, S.nq - q0);
     bem_prof_begin(BH_K_MP_BASIS, (double)nqc * nidx, g.st);
-    hipLaunchKernelGGL(k_mp_basis, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr, D.d_U, D.d_dU,
-                       q0, nqc, B, ldb, dens);
+    hipLaunchKernelGGL(k_mp_basis<true>, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr, D.d_U,
+                       D.d_dU, q0, nqc, B, ldb, dens);
     bem_prof_end(BH_K_MP_BASIS, g.st);
     const int ns = (nqc + MP_QSPLIT - 1) / MP_QSPLIT;
     bem_prof_begin(BH_K_MP_MOM, 8.0 * nidx * MP_NDENS * (double)nqc, g.st);
@@ -624,7 +730,50 @@ int mp_coefficients(bh_object *o, const Plan &P, int n, cd *coef)
     const size_t nt = (size_t)nidx * MP_NDENS;
     hipLaunchKernelGGL(k_mp_reduce, g1(nt), dim3(256), 0, g.st, (int)nt, ns, T, Ts);
   }
-  hipLaunchKernelGGL(k_mp_coef, g1((size_t)nidx * 4), dim3(256), 0, g.st, S.L, S.Lp, S.kr, Ts, coef);
+  hipLaunchKernelGGL(k_mp_coef, g1((size_t)nidx * 4), dim3(256), 0, g.st, S.L, S.Lp, S.kr, Ts, coef, 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the series coefficients of source object n for all nb beams of the
+// many-beam loop: coef[idx][b 16 + set 4 + c].  The basis of the source
+// points is formed once and shared by every beam; the moments of all beams
+// are one product (idx x points) x (points x 16 nb) per chunk of points.
+int mp_coefficients_multi(bh_object *o, const Plan &P, int n, cd *coef, int nb)
+{
+  const Dom &D = o->dom[0];
+  MpSrc S{};
+  for (int d = 0; d < 3; d++) S.c[d] = P.geo[n].c[d];
+  S.kr = o->kn[0].real();
+  S.scale = g_rscale[2];
+  S.L = P.Lsrc[n];
+  S.Lp = S.L + 1;
+  S.qoff = P.rule[n];
+  S.npt = P.npt[n];
+  S.dNe = D.dNe;
+  S.nq = D.dNe * S.npt;
+  const int nidx = mp_count(S.Lp);
+  const size_t ncol = (size_t)MP_NDENS * nb;
+  // points per chunk: the basis (idx x points) and the densities (points x
+  // 16 nb) each within about 1 GB
+  const size_t cap = std::min<size_t>(((size_t)64 << 20) / nidx, ((size_t)64 << 20) / ncol);
+  const int qc = (int)std::max<size_t>(64, std::min<size_t>((size_t)S.nq, cap));
+  cd *B = ws<cd>(W_B, (size_t)qc * nidx);
+  cd *dens = ws<cd>(W_DENS, (size_t)qc * ncol);
+  cd *T = ws<cd>(W_T, (size_t)nidx * ncol);
+  if (!B || !dens || !T) return fail(BH_EHIP, "series: work area (%d coefficients, %d beams)", nidx, nb);
+  const size_t us = (size_t)(D.dNe + 1) * 16;
+  for (int q0 = 0; q0 < S.nq; q0 += qc) {
+    const int nqc = std::min(qc, S.nq - q0);
+    bem_prof_begin(BH_K_MP_BASIS, (double)nqc * nidx, g.st);
+    hipLaunchKernelGGL(k_mp_basis<false>, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr,
+                       (const cd *)nullptr, (const cd *)nullptr, q0, nqc, B, (size_t)qc, (cd *)nullptr);
+    bem_prof_end(BH_K_MP_BASIS, g.st);
+    hipLaunchKernelGGL(k_mp_dens_multi, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr,
+                       (const cd *)mb_dom_ptr(o, 0, 0), (const cd *)mb_dom_ptr(o, 0, 1), us, nb, q0, nqc, dens);
+    bem_zgemm(nidx, (int)ncol, nqc, 1.0, 0.0, B, (size_t)qc, dens, ncol, q0 ? 1.0 : 0.0, 0.0, T, ncol, g.st);
+  }
+  hipLaunchKernelGGL(k_mp_coef, g1((size_t)nidx * 4 * nb), dim3(256), 0, g.st, S.L, S.Lp, S.kr, T, coef, nb);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -861,6 +1010,87 @@ struct Coupling {
     return 0;
   }
 
+  // the coupling sums of every beam of the many-beam loop (incident data
+  // and boundary values of the beams, nb of them)
+  int sums_multi(bh_object **objs, int N, int nb, bool check)
+  {
+    std::vector<size_t> cofm(N, 0);
+    size_t ncm = 0;
+    for (int n = 0; n < N; n++)
+      if (P.Lsrc[n] >= 0) { cofm[n] = ncm; ncm += (size_t)mp_count(P.Lsrc[n] + 1) * MP_NDENS * nb; }
+    cd *cm = ncm ? ws<cd>(W_COEF, ncm) : nullptr;
+    if (ncm && !cm) return fail(BH_EHIP, "series: coefficients of %d beams", nb);
+    for (int n = 0; n < N; n++)
+      if (P.Lsrc[n] >= 0) { int r = mp_coefficients_multi(objs[n], P, n, cm + cofm[n], nb); if (r) return r; }
+    const size_t ncol = (size_t)MP_NDENS * nb;
+    for (int m = 0; m < N; m++) {
+      bh_object *d = objs[m];
+      const size_t is = ((size_t)d->Ne + 1) * 16;
+      for (int n = 0; n < N; n++) {
+        if (P.Lmn[m][n] < 0) continue;
+        const int Lp = P.Lsrc[n] + 1, nidx = mp_count(Lp);
+        const int sc = (int)std::max<size_t>(64, std::min<size_t>((size_t)d->nslot,
+                                                                   std::min(((size_t)64 << 20) / nidx, ((size_t)64 << 20) / ncol)));
+        cd *Sb = ws<cd>(W_MS, (size_t)sc * nidx), *Cm = ws<cd>(W_MC, (size_t)sc * ncol);
+        if (!Sb || !Cm) return fail(BH_EHIP, "series: evaluation work area");
+        for (int s0 = 0; s0 < d->nslot; s0 += sc) {
+          const int ns = std::min(sc, d->nslot - s0);
+          bem_prof_begin(BH_K_MP_EVAL, (double)ns * nidx * nb, g.st);
+          hipLaunchKernelGGL(k_mp_sbasis, dim3((ns + 63) / 64), dim3(64), 0, g.st, d->d_slotPt + 3 * (size_t)s0, ns,
+                             P.geo[n].c[0], P.geo[n].c[1], P.geo[n].c[2], objs[n]->kn[0].real(), Lp, Sb);
+          bem_zgemm(ns, (int)ncol, nidx, 1.0, 0.0, Sb, (size_t)nidx, cm + cofm[n], ncol, 0.0, 0.0, Cm, ncol, g.st);
+          hipLaunchKernelGGL(k_mp_combine, g1((size_t)ns * nb), dim3(256), 0, g.st, Cm, ns, nb, d->d_slotN, d->d_slotIdx,
+                             s0, mb_ptr(d, 2), mb_ptr(d, 3), is);
+          bem_prof_end(BH_K_MP_EVAL, g.st);
+        }
+      }
+    }
+    // the other pairs: element sums, one beam at a time (the objects'
+    // one-beam state pointers lent to the beam's arrays)
+    for (int m = 0; m < N; m++)
+      for (int d = 0; d < N; d++) {
+        if (d == m || P.Lmn[d][m] != -1) continue;
+        bh_object *dst = objs[d], *src = objs[m];
+        cd *keep[4] = {dst->d_Ui, dst->d_dUi, src->dom[0].d_U, src->dom[0].d_dU};
+        const size_t is = ((size_t)dst->Ne + 1) * 16, us = (size_t)(src->dom[0].dNe + 1) * 16;
+        int r = 0;
+        for (int b = 0; b < nb && !r; b++) {
+          dst->d_Ui = mb_ptr(dst, 2) + b * is;
+          dst->d_dUi = mb_ptr(dst, 3) + b * is;
+          src->dom[0].d_U = mb_dom_ptr(src, 0, 0) + b * us;
+          src->dom[0].d_dU = mb_dom_ptr(src, 0, 1) + b * us;
+          r = add_scattered(dst, src, d_did, check && b == 0, d_part);
+        }
+        dst->d_Ui = keep[0]; dst->d_dUi = keep[1]; src->dom[0].d_U = keep[2]; src->dom[0].d_dU = keep[3];
+        if (r) return r;
+      }
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+
+  // the change measure of every beam; the largest is returned (-1 on a failure)
+  double change_multi(bh_object **objs, int N, int nb)
+  {
+    double *part = ws<double>(W_CCD, 2 * (size_t)totslot * nb), *out = ws<double>(W_CCDSUM, 2 * (size_t)nb);
+    if (!part || !out) return -1.0;
+    for (int b = 0; b < nb; b++) {
+      for (int m = 0; m < N; m++) {
+        bh_object *o = objs[m];
+        const size_t is = ((size_t)o->Ne + 1) * 16;
+        hipLaunchKernelGGL(k_ccd, g1(o->nslot), dim3(256), 0, g.st, mb_ptr(o, 2) + b * is, mb_ptr(o, 4) + b * is,
+                           o->d_slotIdx, o->nslot, part + 2 * ((size_t)b * totslot + slot0[m]));
+      }
+      hipLaunchKernelGGL(k_ccd_sum, dim3(1), dim3(256), 0, g.st, part + 2 * (size_t)b * totslot, totslot, out + 2 * b);
+    }
+    std::vector<double> h(2 * (size_t)nb);
+    if (hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, g.st) != hipSuccess ||
+        hipStreamSynchronize(g.st) != hipSuccess)
+      return -1.0;
+    double cc = 0.0;
+    for (int b = 0; b < nb; b++) cc = std::max(cc, std::max(h[2 * b], h[2 * b + 1]) / (double)totslot);
+    return cc;
+  }
+
   // ccd_f (mo_solve.c:413-455) over every object's slots; -1 on a failure
   double change(bh_object **objs, int N)
   {
@@ -980,6 +1210,65 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     }
   }
   if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return object_solve_EH(o); }))) return rc;
+  if (iters) *iters = iter;
+  return 0;
+}
+
+// Many plane waves as separate right-hand sides (no counterpart in the
+// reference, which sums every beam of a set into one incident field,
+// multi_fbeam.c:764-815, so that per-beam results take one run per beam):
+// nb beams (12 doubles each, as bh_object_set_beams) given to every object,
+// the coupling loop (mo_solve_bv, mo_solve.c:10-88) run for all of them at
+// once -- the series coefficients of all beams from one basis per source
+// object, the block solves of all beams as products with 3 nb to 4 nb
+// columns -- until every beam's change is below ccd after more than
+// iter_min iterations, then the boundary E, H of every beam.  Every object
+// must be in the mixed-precision form and factorised.
+extern "C" int bh_mo_solve_beams(bh_object **objs, int N, int nb, const double *par, double ccd, int iter_min,
+                                 int iter_max, int *iters)
+{
+  API_LOCK;
+  if (N <= 0 || nb <= 0) return fail(BH_EARG, "many beams: %d objects, %d beams", N, nb);
+  if (ensure_rules()) return BH_EHIP;
+  for (int m = 0; m < N; m++)
+    if (objs[m]->fmode != 3) return fail(BH_EUNSUP, "many beams: object %d is not factorised in the mixed-precision form", m);
+  int rc, iter = 0;
+  for (int m = 0; m < N; m++)
+    if ((rc = object_beams_set(objs[m], nb, par))) return rc;
+  const std::vector<double> c3 = cost_of(objs, N, 3), c2 = cost_of(objs, N, 2);
+  if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return form_s(o); }))) return rc;
+  auto solve_all = [&]() { return run_lanes(objs, N, c2, [](bh_object *o) { return object_solve_multi(o); }); };
+  if ((rc = solve_all())) return rc;
+  if (N > 1) {
+    Coupling C;
+    if ((rc = C.init(objs, N))) return rc;
+    bool first = true;
+    auto sweep = [&]() -> int {
+      for (int m = 0; m < N; m++) {
+        bh_object *o = objs[m];
+        const size_t n = ((size_t)o->Ne + 1) * 16 * nb;
+        hipLaunchKernelGGL(k_reset_incident, g1(n), dim3(256), 0, g.st, mb_ptr(o, 2), mb_ptr(o, 3), mb_ptr(o, 0),
+                           mb_ptr(o, 1), mb_ptr(o, 4), n);
+      }
+      int r = C.sums_multi(objs, N, nb, first);
+      if (r) return r;
+      first = false;
+      return solve_all();
+    };
+    const char *pe = getenv("BEM3_HIP_PROGRESS");
+    FILE *pf = nullptr;
+    if (pe && *pe && strcmp(pe, "0") != 0) pf = strcmp(pe, "1") == 0 ? stderr : fopen(pe, "a");
+    struct Closer { FILE *f; ~Closer() { if (f && f != stderr) fclose(f); } } closer{pf};
+    if ((rc = sweep())) return rc;
+    for (iter = 2; iter < iter_max; iter++) {
+      if ((rc = sweep())) return rc;
+      const double cc = C.change_multi(objs, N, nb);
+      if (cc < 0.0) return fail(BH_EHIP, "many beams: change measure");
+      if (pf) { fprintf(pf, "bh_mo_solve_beams: count = %d, largest cc = %g\n", iter, cc); fflush(pf); }
+      if (cc < ccd && iter > iter_min) break;
+    }
+  }
+  if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return object_solve_EH_multi(o); }))) return rc;
   if (iters) *iters = iter;
   return 0;
 }

@@ -59,6 +59,7 @@ struct Scratch {
   size_t n = 0;
   Scratch() = default;
   Scratch(const Scratch &) = delete;
+  Scratch(Scratch &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
   Scratch &operator=(const Scratch &) = delete;
   ~Scratch() { dfree(p); }
   void release() { dfree(p); p = nullptr; n = 0; }
@@ -177,6 +178,17 @@ struct bh_object {
   bem::Scratch ws_part, ws_mask, ws_pairs, ws_start, ws_ghdf, ws_self;   // assembly
   bem::Scratch ws_wt;                  // forming S
   bem::Scratch ws_eh[5];               // boundary E, H: packed rows, products, refinement
+  // many beams as right-hand sides (bh_mo_solve_beams): nbm plane waves;
+  // beam b of each array at + b x (its one-beam size)
+  int nbm = 0;
+  bem::Scratch mb_inc[5];              // Ui0, dUi0, Ui, dUi, Uip: [nb][Ne+1][16]
+  std::vector<bem::Scratch> mb_dom;    // per domain: U, dU ([nb][dNe+1][16]), E, dE, H, dH ([nb][dNe+1][12])
+  bem::Scratch mb_sys[3];              // B, X, R: [nb][8NN]
+  bem::Scratch mb_ws[8];               // solve and boundary-field work areas
+  bem::Scratch mb_beams;               // the beams (Beam records)
+  bool mb_xwarm = false;               // X solves the current factors' system for an earlier B
+  int mb_iters = 0;                    // refinement steps of the last many-beam solve
+  double mb_res = 0.0;                 // its largest final relative residual
 };
 
 namespace bem {
@@ -186,6 +198,14 @@ int object_assemble(bh_object *o, int type);
 int object_factor(bh_object *o, int mode);
 int object_solve(bh_object *o);
 int object_solve_EH(bh_object *o);
+// many beams (bh_mo_solve_beams): nb plane waves (12 doubles each) as the
+// object's right-hand sides and their incident data; the solve and the
+// boundary E, H of all of them (the mixed-precision form with S formed)
+int object_beams_set(bh_object *o, int nb, const double *par);
+int object_solve_multi(bh_object *o);
+int object_solve_EH_multi(bh_object *o);
+cd *mb_ptr(bh_object *o, int which);   // 0 Ui0, 1 dUi0, 2 Ui, 3 dUi, 4 Uip
+cd *mb_dom_ptr(bh_object *o, int d, int which);   // 0 U, 1 dU, 2 E, 3 dE, 4 H, 5 dH
 int prepare_solve(bh_object *o);
 int form_s(bh_object *o);
 int domain_ids(bh_object *o, const double *d_pts, int np, int *d_did);

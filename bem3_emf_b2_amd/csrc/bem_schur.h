@@ -64,3 +64,15 @@ double bem_nrm2(const cd *x, size_t n, double *tmp, hipStream_t st);
 // elementwise conversions: dst = src rounded to FP32; X += d widened to FP64
 void bem_cvt_z2c(const cd *src, cf *dst, size_t n, hipStream_t st);
 void bem_acc_c2z(cd *X, const cf *d, size_t n, hipStream_t st);
+
+// many right-hand sides (nb, beam b at + b 8NN) with S formed: work holds
+// schur_work_multi(NN, nb) values; BH_ESTATE if S is not formed
+static inline size_t schur_work_multi(size_t n, size_t nb) { return 26 * n * nb + 512; }
+int schur_apply_multi(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, int nb, hipStream_t st);
+int schur_apply_multi(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, int nb, hipStream_t st);
+// R_b = B_b - A X_b over the rows of one domain for nb beams; work holds
+// 12 nd nb values
+void bem_resid_domain_multi(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
+                            size_t rowBase, const cd *B, cd *R, cd *work, int nb, hipStream_t st);
+// |x_b|^2 of nb vectors of n values into out[nb] (device)
+void bem_mnrm2(const cd *x, size_t n, int nb, double *out, hipStream_t st);

@@ -326,6 +326,173 @@ static int schur_apply_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, Z
   return 0;
 }
 
+// ---- many right-hand sides at once (nb beams) ----
+// Layouts: row-major n x (K nb) with the column of component c of beam b at
+// b K + c; right-hand sides and solutions in the 8NN layout, beam b at
+// rhs + b 8N.
+template <class Zt>
+__global__ void k_mgather(const Zt *__restrict__ rhs, size_t N, int nb, int K, int c0, Zt *__restrict__ dst, int n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * nb * K) return;
+  const int r = (int)(i / ((size_t)nb * K)), q = (int)(i % ((size_t)nb * K)), b = q / K, c = q % K;
+  dst[i] = rhs[(size_t)b * 8 * N + (size_t)(c0 + c) * N + r];
+}
+template <class Zt>
+__global__ void k_mndot(const Zt *__restrict__ Y, const ColInfo *__restrict__ col, int n, int nb, Zt *__restrict__ v)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * nb) return;
+  const int r = (int)(i / nb), b = (int)(i % nb);
+  const double *nn = col[r].n;
+  const Zt *y = Y + (size_t)r * 3 * nb + 3 * b;
+  v[i] = y[0] * nn[0] + y[1] * nn[1] + y[2] * nn[2];
+}
+template <class Zt>
+__global__ void k_mnscale(const Zt *__restrict__ u, const ColInfo *__restrict__ col, int n, int nb, Zt *__restrict__ Y)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * nb) return;
+  const int r = (int)(i / nb), b = (int)(i % nb);
+  for (int c = 0; c < 3; c++) Y[(size_t)r * 3 * nb + 3 * b + c] = u[i] * col[r].n[c];
+}
+template <class Zt>
+__global__ void k_mrhs_q0(Zt *__restrict__ Y, Zt *__restrict__ b3, const Zt *__restrict__ T, Zt beta, int n, int nb)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * nb) return;
+  const int r = (int)(i / nb), b = (int)(i % nb);
+  const Zt *t = T + (size_t)r * 4 * nb + 4 * b;
+  for (int c = 0; c < 3; c++) Y[(size_t)r * 3 * nb + 3 * b + c] = Y[(size_t)r * 3 * nb + 3 * b + c] + t[c];
+  b3[i] = b3[i] + beta * t[3];
+}
+template <class Zt>
+__global__ void k_mpack_u4(const Zt *__restrict__ Y, const Zt *__restrict__ u3, int n, int nb, Zt *__restrict__ U4)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * nb) return;
+  const int r = (int)(i / nb), b = (int)(i % nb);
+  for (int c = 0; c < 3; c++) U4[(size_t)r * 4 * nb + 4 * b + c] = Y[(size_t)r * 3 * nb + 3 * b + c];
+  U4[(size_t)r * 4 * nb + 4 * b + 3] = u3[i];
+}
+template <class Zt>
+__global__ void k_mfill_x(const Zt *__restrict__ U4, const Zt *__restrict__ Q4, const Zt *__restrict__ q0, int n, int nb,
+                          Zt *__restrict__ X)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * nb) return;
+  const int r = (int)(i / nb), b = (int)(i % nb);
+  Zt *x = X + (size_t)b * 8 * n;
+  for (int c = 0; c < 4; c++) {
+    const size_t o = (size_t)r * 4 * nb + 4 * b + c;
+    x[(size_t)c * n + r] = U4[o];
+    x[(size_t)(c + 4) * n + r] = q0 ? Q4[o] - q0[o] : Q4[o];
+  }
+}
+
+// X_b = A^{-1} rhs_b for nb right-hand sides with S formed (the steps of
+// schur_apply_t, every product and triangular solve on all beams at once);
+// work holds schur_work_multi(n, nb) values
+template <class Zt>
+static int schur_apply_multi_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, Zt *w, int nb, hipStream_t st)
+{
+  if (!f.s_lu) return BH_ESTATE;
+  const int n = f.n;
+  const size_t N = (size_t)n, M = (size_t)nb;
+  Zt *Y = w, *Bc = Y + 3 * N * M, *v = Bc + 3 * N * M, *u3 = v + N * M, *Qn = u3 + N * M, *q0 = Qn + 3 * N * M,
+     *T = q0 + 4 * N * M, *U4 = T + 4 * N * M, *Q4 = U4 + 4 * N * M;
+  hipLaunchKernelGGL(k_mgather<Zt>, grid1(3 * N * M), dim3(256), 0, st, rhs, N, nb, 3, 4, Bc, n);
+  hipLaunchKernelGGL(k_mgather<Zt>, grid1(N * M), dim3(256), 0, st, rhs, N, nb, 1, 7, u3, n);
+  if (r0nz) {
+    hipLaunchKernelGGL(k_mgather<Zt>, grid1(4 * N * M), dim3(256), 0, st, rhs, N, nb, 4, 0, q0, n);
+    xgetrs(n, 4 * nb, f.G0f, f.ld, f.piv0, q0, 4 * M, st);
+    xgemm(n, 4 * nb, n, one<Zt>(), f.G1, f.ld, q0, 4 * M, zero<Zt>(), T, 4 * M, st);
+    hipLaunchKernelGGL(k_mrhs_q0<Zt>, grid1(N * M), dim3(256), 0, st, Bc, u3, T, f.beps, n, nb);
+  }
+  (void)hipMemcpyAsync(Y, Bc, 3 * N * M * sizeof(Zt), hipMemcpyDeviceToDevice, st);
+  xgetrs(n, 3 * nb, f.Mf, f.ld, f.pivM, Y, 3 * M, st);
+  hipLaunchKernelGGL(k_mndot<Zt>, grid1(N * M), dim3(256), 0, st, Y, f.col1, n, nb, v);
+  xgemm(n, nb, n, -f.sigm, f.G1, f.ld, v, M, one<Zt>(), u3, M, st);
+  xgetrs(n, nb, f.Sf, f.ld, f.pivS, u3, M, st);
+  hipLaunchKernelGGL(k_mnscale<Zt>, grid1(N * M), dim3(256), 0, st, u3, f.col1, n, nb, Qn);
+  xgemm(n, 3 * nb, n, -f.delt, f.G1, f.ld, Qn, 3 * M, one<Zt>(), Bc, 3 * M, st);
+  xgetrs(n, 3 * nb, f.Mf, f.ld, f.pivM, Bc, 3 * M, st);
+  hipLaunchKernelGGL(k_mpack_u4<Zt>, grid1(N * M), dim3(256), 0, st, Bc, u3, n, nb, U4);
+  xgemm(n, 4 * nb, n, one<Zt>(), f.Zm, f.ld, U4, 4 * M, zero<Zt>(), Q4, 4 * M, st);
+  hipLaunchKernelGGL(k_mfill_x<Zt>, grid1(N * M), dim3(256), 0, st, U4, Q4, r0nz ? (const Zt *)q0 : nullptr, n, nb, X);
+  return 0;
+}
+
+int schur_apply_multi(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, int nb, hipStream_t st)
+{
+  return schur_apply_multi_t<cf>(f, rhs, r0nz, X, work, nb, st);
+}
+int schur_apply_multi(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, int nb, hipStream_t st)
+{
+  return schur_apply_multi_t<cd>(f, rhs, r0nz, X, work, nb, st);
+}
+
+// R_b = B_b - A X_b over the rows of one domain for nb beams (8NN layout,
+// beam b at + b 8NN): the domain's column vectors of every beam, then two
+// products with 4 nb columns
+__global__ void k_mcolvec(const cd *__restrict__ X, const ColInfo *__restrict__ col, int nd, size_t NN, int nb,
+                          cd *__restrict__ a, cd *__restrict__ b)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nd * nb) return;
+  const int j = (int)(i / nb), q = (int)(i % nb);
+  const ColInfo c = col[j];
+  const cd *x = X + (size_t)q * 8 * NN;
+  const size_t cn = (size_t)c.node, o = ((size_t)j * nb + q) * 4;
+  cd U[4], Q[4];
+  for (int k = 0; k < 4; k++) { U[k] = x[NN * k + cn]; Q[k] = x[NN * (k + 4) + cn]; }
+  for (int k = 0; k < 4; k++) a[o + k] = U[k];
+  if (c.main) {
+    for (int k = 0; k < 4; k++) b[o + k] = -Q[k];
+  } else {
+    for (int k = 0; k < 3; k++) b[o + k] = c.delt * (U[3] * c.n[k]) + Q[k];
+    b[o + 3] = c.sigm * (U[0] * c.n[0] + U[1] * c.n[1] + U[2] * c.n[2]) + c.beps * Q[3];
+  }
+}
+__global__ void k_mresid(const cd *__restrict__ B, const cd *__restrict__ Y, int nd, size_t NN, size_t rowBase, int nb,
+                         cd *__restrict__ R)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)4 * nd * nb) return;
+  const int q = (int)(i / (4 * (size_t)nd)), rem = (int)(i % (4 * (size_t)nd)), c = rem / nd, r = rem % nd;
+  const size_t o = (size_t)q * 8 * NN + rowBase + (size_t)c * nd + r;
+  R[o] = B[o] - Y[((size_t)r * nb + q) * 4 + c];
+}
+
+void bem_resid_domain_multi(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
+                            size_t rowBase, const cd *B, cd *R, cd *w, int nb, hipStream_t st)
+{
+  const size_t N = (size_t)nd, M = (size_t)nb;
+  cd *a = w, *b = w + 4 * N * M, *Y = w + 8 * N * M;
+  hipLaunchKernelGGL(k_mcolvec, grid1(N * M), dim3(256), 0, st, X, col, nd, NN, nb, a, b);
+  bem_zgemm(nd, 4 * nb, nd, 1.0, 0.0, H, ld, a, 4 * M, 0.0, 0.0, Y, 4 * M, st);
+  bem_zgemm(nd, 4 * nb, nd, 1.0, 0.0, G, ld, b, 4 * M, 1.0, 0.0, Y, 4 * M, st);
+  hipLaunchKernelGGL(k_mresid, grid1(4 * N * M), dim3(256), 0, st, B, Y, nd, NN, rowBase, nb, R);
+}
+
+// |x_b|^2 for nb vectors of n values (x_b at x + b n): out[b] (one
+// workgroup per vector, a fixed order)
+__global__ void __launch_bounds__(256) k_mnrm2(const cd *__restrict__ x, size_t n, double *__restrict__ out)
+{
+  __shared__ double s[4];
+  const cd *v = x + (size_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 256) acc += abs2(v[i]);
+  acc = wsum(acc);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+void bem_mnrm2(const cd *x, size_t n, int nb, double *out, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_mnrm2, dim3(nb), dim3(256), 0, st, x, n, out);
+}
+
 int schur_apply(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, hipStream_t st, int *gm_steps)
 {
   return schur_apply_t<cd>(f, rhs, r0nz, X, work, st, gm_steps);

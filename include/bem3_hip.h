@@ -156,6 +156,19 @@ int bh_mo_plan_stats(int *series, int *direct, int *lmax);
  * receives 2 x [Ne+1][4][4] complex (the scattered U, then dU/dn, in the
  * incident-data layout); the objects' incident data are left unchanged.  */
 int bh_mo_coupling_sums(bh_object **objs, int N, int dst, double *out);
+/* Many plane waves as separate right-hand sides (no counterpart in the
+ * reference, which sums the beams of a set into one incident field,
+ * multi_fbeam.c:764-815; per-beam results there take one run per beam):
+ * nb beams of 12 doubles each (as bh_object_set_beams) are given to every
+ * object, the coupling loop runs for all of them at once until every
+ * beam's change is below ccd after more than iter_min iterations, and the
+ * boundary E, H of every beam follow.  Every object must be factorised in
+ * the mixed-precision form.  *iters receives the iteration count.       */
+int bh_mo_solve_beams(bh_object **objs, int N, int nb, const double *par, double ccd, int iter_min, int iter_max,
+                      int *iters);
+/* one beam's values after bh_mo_solve_beams: which 0..5 as
+ * bh_object_state (domain did), 6..9 the incident data Ui, dUi, Ui0, dUi0 */
+int bh_object_beam_state(bh_object *o, int b, int did, int which, double *buf);
 /* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i, mo_EH_mEMP_s/t/i;
  * mo_field.c:11-282): what 0: E,H (6 complex per point), 1: U (4 complex),
  * 2: E,H from the potentials' gradients (EH_mEMP_s, d3b1_field.c:68-138,

@@ -1,0 +1,74 @@
+"""Many plane waves as separate right-hand sides (bh_mo_solve_beams) against
+one coupling loop per beam (bh_mo_solve with that beam alone), both in the
+mixed-precision form: three objects, two of them close enough for element
+sums between them, four beams of seeded random directions and
+polarisations.  Both loops are run to round-off (a change below 1e-24 or
+40 iterations), so the boundary values U, dU, E, H of every domain, object
+and beam agree to 1e-8."""
+import numpy as np
+import pytest
+
+from bem3_emf_b2_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _beams(nb, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nb):
+        th, ph, psi = np.arccos(1 - 2 * rng.uniform()), 2 * np.pi * rng.uniform(), np.pi * rng.uniform()
+        out.append([1.064, 1.33, 0.665, np.cos(psi), 0.0, np.sin(psi), 0.0, 0.0, 0.0, 0.0, th, ph])
+    return np.array(out)
+
+
+def _scene():
+    return [
+        (mg.icosphere(2, 0.35, center=(-1.4, 0.0, 0.0)), 1.5),
+        (mg.box(0.5, 3, center=(1.2, 0.6, 0.2)), 1.6),
+        (mg.icosphere(1, 0.2, center=(1.2, 0.6, 0.75)), 1.45),   # close to the cube: element sums
+    ]
+
+
+def _objects(beam):
+    import bem3_emf_b2_amd as bem
+    G = []
+    for (V, F), n in _scene():
+        rn, ed, md, sd, gd = mg.paired_arrays(V, F)
+        g = bem.Object(rn, ed, md, sd, [1.33, n], 1.064)
+        g.set_precision("mixed")
+        g.set_beams(beam)
+        g.assemble(2)
+        g.factor(0)
+        G.append(g)
+    return G
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_many_beams_match_one_loop_per_beam():
+    import bem3_emf_b2_amd as bem
+    beams = _beams(4, 8)
+    G = _objects(beams[:1])
+    it = bem.mo_solve_beams(G, beams, ccd=1e-24, iter_max=40)
+    series, direct, _ = bem.plan_stats()
+    assert series >= 2 and direct >= 2
+    got = [[{(d, k): g.beam_state(b, d, k) for d in (0, 1) for k in ("U", "dU", "E", "H")} for g in G]
+           for b in range(len(beams))]
+    for g in G:
+        g.close()
+    worst = 0.0
+    for b in range(len(beams)):
+        G1 = _objects(beams[b:b + 1])
+        bem.mo_solve(G1, ccd=1e-24, iter_max=40)
+        for m, g in enumerate(G1):
+            for d in (0, 1):
+                for k in ("U", "dU", "E", "H"):
+                    e = _rel(got[b][m][(d, k)], g.state(d, k))
+                    worst = max(worst, e)
+                    assert e < 1e-8, (b, m, d, k, e)
+        for g in G1:
+            g.close()
+    print("many beams vs one loop per beam: %.2e (%d iterations)" % (worst, it), flush=True)
