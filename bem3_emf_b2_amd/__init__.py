@@ -75,6 +75,9 @@ def lib():
             "bh_mo_coupling_sums": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D]),
             "bh_mo_solve_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, C.c_double, C.c_int, C.c_int, Ip]),
             "bh_object_beam_state": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, D]),
+            "bh_object_select_beam": (C.c_int, [VP, C.c_int]),
+            "bh_mo_force_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, D, D, Ip]),
+            "bh_mo_far_field_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D, C.c_int, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -285,6 +288,15 @@ class Object:
         check(lib().bh_object_beam_state(self.h, b, did, k, dp(out.view(np.float64))))
         return out
 
+    def nbm(self):
+        """beams of the last mo_solve_beams (0: none)"""
+        return getattr(self, "_nbm", 0)
+
+    def select_beam(self, b):
+        """make beam b of mo_solve_beams the object's one-beam state (the
+        field, force and power functions then report beam b)"""
+        check(lib().bh_object_select_beam(self.h, b))
+
     def incident(self, which, value=None):
         k = ["Ui", "dUi", "Ui0", "dUi0"].index(which)
         n = (self.Ne + 1) * 16
@@ -384,7 +396,60 @@ def mo_solve_beams(objs, beams, ccd=1e-6, iter_min=10, iter_max=100):
     par = np.ascontiguousarray(beams, dtype=np.float64).reshape(-1, 12)
     it = np.zeros(1, np.int32)
     check(lib().bh_mo_solve_beams(_handles(objs), len(objs), par.shape[0], dp(par), ccd, iter_min, iter_max, ip(it)))
+    for o in objs:
+        o._nbm = par.shape[0]
     return int(it[0])
+
+
+def select_beam(objs, b):
+    """beam b of mo_solve_beams as every object's one-beam state"""
+    for o in objs:
+        o.select_beam(b)
+
+
+def mo_force_beams(objs, type_, rc=None):
+    """force and torque of every object under every beam of mo_solve_beams:
+    (F, N, flag) of shapes (nb, N, 3), (nb, N, 3), (nb, N); rc: torque
+    centres (N, 3) or None (origin)"""
+    n = len(objs)
+    nb = objs[0].nbm()
+    F = np.zeros((nb, n, 3))
+    Nq = np.zeros((nb, n, 3))
+    flag = np.zeros((nb, n), np.int32)
+    rcp = None
+    if rc is not None:
+        rca = np.ascontiguousarray(rc, dtype=np.float64).reshape(n, 3)
+        rcp = dp(rca)
+    check(lib().bh_mo_force_beams(_handles(objs), n, type_, rcp, dp(F), dp(Nq), ip(flag)))
+    return F, Nq, flag
+
+
+def mo_far_field_beams(objs, pts, type_=1, what="I"):
+    """scattered field of every beam of mo_solve_beams at far points by the
+    objects' series: what "I": |E|^2, |H|^2 of shape (nb, np, 2); "EH":
+    complex E, H of shape (nb, np, 6)"""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    nb = objs[0].nbm()
+    w = {"EH": 0, "I": 1}[what]
+    if w == 0:
+        out = np.zeros((nb, pts.shape[0], 6), np.complex128)
+        buf = out.view(np.float64)
+    else:
+        out = np.zeros((nb, pts.shape[0], 2))
+        buf = out
+    check(lib().bh_mo_far_field_beams(_handles(objs), len(objs), type_, pts.shape[0], dp(pts), w, dp(buf)))
+    return out
+
+
+def far_field_grid(lambda0, mf=2000.0, n_polar=180, n_azimuth=360):
+    """points of a far-field grid at radius mf * lambda0 (example4.c's 3-d
+    map): polar angles (i + 1/2) pi / n_polar, azimuths j 2 pi / n_azimuth;
+    shape (n_polar * n_azimuth, 3), polar-major"""
+    ra = mf * lambda0
+    ph = (np.arange(n_polar) + 0.5) * np.pi / n_polar
+    th = np.arange(n_azimuth) * 2.0 * np.pi / n_azimuth
+    P, T = np.meshgrid(ph, th, indexing="ij")
+    return np.stack([ra * np.sin(P) * np.cos(T), ra * np.sin(P) * np.sin(T), ra * np.cos(P)], -1).reshape(-1, 3)
 
 
 def mo_solve(objs, ccd=1e-6, iter_min=10, iter_max=100):

@@ -1414,10 +1414,16 @@ int bem::object_beams_set(bh_object *o, int nb, const double *par)
   for (int q = 0; q < 3; q++)
     if (!o->mb_sys[q].get<cd>(na * nb)) return fail(BH_EHIP, "beams: systems of %d beams", nb);
   std::vector<Beam> bm(nb);
-  for (int b = 0; b < nb; b++) bm[b] = plane_wave(par + 12 * (size_t)b);
+  for (int b = 0; b < nb; b++) {
+    const double *p = par + 12 * (size_t)b;
+    if (p[0] != o->lambda0 || p[1] != o->n[0].real() || o->n[0].imag() != 0.0)
+      return fail(BH_EARG, "beam %d: wavelength / medium differ from the object's", b);
+    bm[b] = plane_wave(p);
+  }
   Beam *db = o->mb_beams.get<Beam>(nb);
   if (!db) return fail(BH_EHIP, "beams: records");
   HIPCHK(hipMemcpyAsync(db, bm.data(), nb * sizeof(Beam), hipMemcpyHostToDevice, cur_st()));
+  o->mb_hbeams = bm;
   for (int q = 0; q < 5; q++) zero_cd(mb_ptr(o, q), E1 * 16 * nb);
   for (int b = 0; b < nb; b++) {
     const size_t off = (size_t)b * E1 * 16;
@@ -1572,6 +1578,44 @@ extern "C" int bh_object_beam_state(bh_object *o, int b, int did, int which, dou
   const size_t n = (size_t)(o->dom[did].dNe + 1) * (which < 2 ? 16 : 12);
   HIPCHK(hipMemcpy(buf, mb_dom_ptr(o, did, which) + b * n, n * sizeof(cd), hipMemcpyDeviceToHost));
   return 0;
+}
+
+// beam b of the many-beam solve as the object's one-beam state: its
+// boundary values and incident data copied into the object's own arrays and
+// its plane wave made the object's beam set, so that the field, force and
+// power entry points report beam b (what one reference run with that beam
+// alone would hold after mo_solve_iter, mo_solve.c:10-88)
+int bem::object_select_beam(bh_object *o, int b)
+{
+  if (b < 0 || b >= o->nbm || b >= (int)o->mb_hbeams.size()) return fail(BH_EARG, "beam %d of %d", b, o->nbm);
+  if (prepare_solve(o)) return BH_EHIP;
+  const size_t E1 = (size_t)o->Ne + 1, ni = E1 * 16;
+  cd *inc[5] = {o->d_Ui0, o->d_dUi0, o->d_Ui, o->d_dUi, o->d_Uip};
+  for (int q = 0; q < 5; q++)
+    HIPCHK(hipMemcpyAsync(inc[q], mb_ptr(o, q) + b * ni, ni * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    cd *tab[6] = {D.d_U, D.d_dU, D.d_E, D.d_dE, D.d_Hf, D.d_dHf};
+    for (int q = 0; q < 6; q++) {
+      const size_t n = (size_t)(D.dNe + 1) * (q < 2 ? 16 : 12);
+      HIPCHK(hipMemcpyAsync(tab[q], mb_dom_ptr(o, d, q) + b * n, n * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
+    }
+  }
+  o->beams.assign(1, o->mb_hbeams[b]);
+  o->fbeams.clear();
+  dfree(o->d_fb); dfree(o->d_fbt);
+  o->d_fb = nullptr; o->d_fbt = nullptr;
+  if (!o->d_beams && upload(&o->d_beams, &o->beams[0], 1)) return BH_EHIP;
+  HIPCHK(hipMemcpyAsync(o->d_beams, &o->beams[0], sizeof(Beam), hipMemcpyHostToDevice, cur_st()));
+  o->x_warm = false;   // the one-beam solve's X belongs to another right-hand side
+  HIPCHK(hipStreamSynchronize(cur_st()));   // the host record stays alive until here
+  return 0;
+}
+
+extern "C" int bh_object_select_beam(bh_object *o, int b)
+{
+  API_LOCK;
+  return object_select_beam(o, b);
 }
 
 // which: 0 U, 1 dU ([dNe+1][4][4]), 2 E, 3 dE, 4 H, 5 dH ([dNe+1][4][3]);
@@ -1870,9 +1914,8 @@ static int check_ready(bh_object **objs, int N, int oid)
 
 // radiation force and torque about rc (mo_force_FN, mo_force.c:10-240);
 // returns 0, or -1 when the object has no open-domain faces
-extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq)
+static int mo_force(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq)
 {
-  API_LOCK;
   int rc0 = check_ready(objs, N, oid);
   if (rc0) return rc0;
   bh_object *o = objs[oid];
@@ -1895,6 +1938,41 @@ extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const dou
   HIPCHK(hipStreamSynchronize(cur_st()));
   for (int p = 0; p < np; p++)
     for (int j = 0; j < 3; j++) { F[j] += -0.5 * part[6 * p + j]; Nq[j] += -0.5 * part[6 * p + 3 + j]; }
+  return 0;
+}
+
+extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq)
+{
+  API_LOCK;
+  return mo_force(objs, N, oid, type, rc, F, Nq);
+}
+
+// force and torque of every object under every beam of the many-beam solve
+// (mo_force_FN of each object in one reference run per beam): each beam in
+// turn made the objects' one-beam state (bh_object_select_beam), then the
+// force of every object about rc[3 m] (null: the origin).  F, Nq: [nb][N][3];
+// flag[b N + m] = the one-beam return value (-1: no open-domain faces).
+// The objects keep the last beam's state.
+extern "C" int bh_mo_force_beams(bh_object **objs, int N, int type, const double *rc, double *F, double *Nq, int *flag)
+{
+  API_LOCK;
+  if (N <= 0) return fail(BH_EARG, "no objects");
+  const int nb = objs[0]->nbm;
+  for (int m = 0; m < N; m++)
+    if (objs[m]->nbm != nb || nb <= 0) return fail(BH_ESTATE, "object %d: no many-beam solve of %d beams", m, nb);
+  const double zero[3] = {0.0, 0.0, 0.0};
+  for (int b = 0; b < nb; b++) {
+    for (int m = 0; m < N; m++) {
+      int r = object_select_beam(objs[m], b);
+      if (r) return r;
+    }
+    for (int m = 0; m < N; m++) {
+      const size_t o = ((size_t)b * N + m) * 3;
+      int r = mo_force(objs, N, m, type, rc ? rc + 3 * (size_t)m : zero, F + o, Nq + o);
+      if (r > 0) return r;
+      if (flag) flag[(size_t)b * N + m] = r;
+    }
+  }
   return 0;
 }
 

@@ -160,9 +160,18 @@ struct MpSrc {
 // ROWQ: B[q][idx] (rows per point, the moment kernel's layout); otherwise
 // B[idx][q] (row-major idx x points: the A operand of the many-beam moment
 // product).  dens null: no densities (the many-beam path forms its own).
-template <bool ROWQ>
+// shape functions at a rule point: the triangle's interpolation functions
+// (far-rule table), or the values a QPtN table holds (the 4-point node rule
+// carries its own)
+__device__ __forceinline__ void rule_N(const QPt &q, double &n0, double &n1, double &n2) { tri_N(q.z, q.e, n0, n1, n2); }
+__device__ __forceinline__ void rule_N(const QPtN &q, double &n0, double &n1, double &n2)
+{
+  n0 = q.n0; n1 = q.n1; n2 = q.n2;
+}
+
+template <bool ROWQ, class R>
 __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
-                                                 const QPt *__restrict__ rule, const cd *__restrict__ U,
+                                                 const R *__restrict__ rule, const cd *__restrict__ U,
                                                  const cd *__restrict__ dU, int q0, int nqc, cd *__restrict__ B,
                                                  size_t ldb, cd *__restrict__ dens)
 {
@@ -174,14 +183,14 @@ __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restri
   const int s = 1 + q / S.npt, p = q % S.npt;
   const int sd = sid[s], as = sd < 0 ? -sd : sd;
   const TriGeo g = geo[as];
-  const QPt qp = rule[S.qoff + p];
+  const R qp = rule[S.qoff + p];
   double y[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) y[d] = fma(g.c2[d], qp.e, fma(g.c1[d], qp.z, g.c0[d])) - S.c[d];
   // densities: a_c = w scale |W| du_c (monopole), b_c = -sigma w scale u_c
   // (dipole b_c W), split as W_z, (W_x - i W_y)/2, (W_x + i W_y)/2
   double n0, n1, n2;
-  tri_N(qp.z, qp.e, n0, n1, n2);
+  rule_N(qp, n0, n1, n2);
   const double wt = qp.w * S.scale;
   const double sg = sd < 0 ? -1.0 : 1.0;
   const cd *Us = U + (size_t)s * 16, *dUs = dU + (size_t)s * 16;
@@ -334,8 +343,9 @@ __global__ void k_mp_coef(int L, int Lp, double k, const cd *__restrict__ T, cd 
 // ------------------------------------------------------------------ many beams
 // the 16 densities of every beam at a source point: dens[q][b 16 + d]
 // (beam b's boundary values at U + b ustride)
+template <class R>
 __global__ void __launch_bounds__(64) k_mp_dens_multi(MpSrc S, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
-                                                      const QPt *__restrict__ rule, const cd *__restrict__ U,
+                                                      const R *__restrict__ rule, const cd *__restrict__ U,
                                                       const cd *__restrict__ dU, size_t ustride, int nb, int q0, int nqc,
                                                       cd *__restrict__ dens)
 {
@@ -345,9 +355,9 @@ __global__ void __launch_bounds__(64) k_mp_dens_multi(MpSrc S, const TriGeo *__r
   const int s = 1 + q / S.npt, p = q % S.npt;
   const int sd = sid[s], as = sd < 0 ? -sd : sd;
   const TriGeo g = geo[as];
-  const QPt qp = rule[S.qoff + p];
+  const R qp = rule[S.qoff + p];
   double n0, n1, n2;
-  tri_N(qp.z, qp.e, n0, n1, n2);
+  rule_N(qp, n0, n1, n2);
   const double wt = qp.w * S.scale, sg = sd < 0 ? -1.0 : 1.0;
   const cd wzp = mk(0.5 * g.W[0], -0.5 * g.W[1]), wzm = mk(0.5 * g.W[0], 0.5 * g.W[1]);
   cd *dq = dens + (size_t)ql * MP_NDENS * nb;
@@ -420,6 +430,51 @@ __global__ void k_mp_combine(const cd *__restrict__ C, int ns, int nb, const dou
     Ui[o + k] = Ui[o + k] + c[k];
     dUi[o + k] = dUi[o + k] + c[4 + k] * v[2] + c[8 + k] * vm + c[12 + k] * vp;
   }
+}
+
+// E, H of every beam at points p0 + p from one source object's series
+// products C[p][b 16 + set 4 + c] (sets U, d_z U, d_+ U, d_- U):
+// E_i = -d_i U_3 + U_i, H = ch curl (U_0, U_1, U_2) (EH_mEMP_s,
+// d3b1_field.c:131-136), added into EH[b][p][6] (beam b at EH + b ldb)
+__global__ void k_mp_far_eh(const cd *__restrict__ C, int np, int nb, double chr, double chi, int p0, size_t ldb,
+                            cd *__restrict__ EH)
+{
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)np * nb) return;
+  const int p = (int)(t / nb), b = (int)(t % nb);
+  const cd *c = C + ((size_t)p * nb + b) * MP_NDENS;
+  cd dx[4], dy[4], dz[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const cd sp = c[8 + q], sm = c[12 + q], df = sp - sm;
+    dx[q] = (sp + sm) * 0.5;
+    dy[q] = mk(0.5 * df.y, -0.5 * df.x);   // (d_+ - d_-) / 2i
+    dz[q] = c[4 + q];
+  }
+  const cd ch = mk(chr, chi);
+  cd *o = EH + (size_t)b * ldb + (size_t)(p0 + p) * 6;
+  o[0] = o[0] + c[0] - dx[3];
+  o[1] = o[1] + c[1] - dy[3];
+  o[2] = o[2] + c[2] - dz[3];
+  o[3] = o[3] + ch * (dy[2] - dz[1]);
+  o[4] = o[4] + ch * (dz[0] - dx[2]);
+  o[5] = o[5] + ch * (dx[1] - dy[0]);
+}
+
+// |E|^2 and |H|^2 of n (beam, point) rows of 6 (example4.c's ie, ih)
+__global__ void k_eh_intensity(const cd *__restrict__ EH, size_t n, double *__restrict__ I)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cd *e = EH + 6 * i;
+  double a = 0.0, h = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    a += e[k].x * e[k].x + e[k].y * e[k].y;
+    h += e[3 + k].x * e[3 + k].x + e[3 + k].y * e[3 + k].y;
+  }
+  I[2 * i] = a;
+  I[2 * i + 1] = h;
 }
 
 // ------------------------------------------------------------------ evaluation
@@ -563,7 +618,8 @@ int g_last_series = 0, g_last_direct = 0, g_last_lmax = -1;   // the last loop's
 
 // grow-only work areas of the loop (bem_mo_release frees them)
 Scratch *g_ws = nullptr;
-enum { W_DID, W_PART, W_CCD, W_CCDSUM, W_B, W_DENS, W_T, W_TSUM, W_COEF, W_TILES, W_PAIRS, W_MS, W_MC, W_NWS };
+enum { W_DID, W_PART, W_CCD, W_CCDSUM, W_B, W_DENS, W_T, W_TSUM, W_COEF, W_TILES, W_PAIRS, W_MS, W_MC, W_FPTS,
+       W_FEH, W_FI, W_NWS };
 template <class T> T *ws(int i, size_t n)
 {
   if (!g_ws) g_ws = new Scratch[W_NWS];
@@ -719,7 +775,7 @@ int mp_coefficients(bh_object *o, const Plan &P, int n, cd *coef)
   for (int q0 = 0; q0 < S.nq; q0 += qc) {
     const int nqc = std::min(qc, S.nq - q0);
     bem_prof_begin(BH_K_MP_BASIS, (double)nqc * nidx, g.st);
-    hipLaunchKernelGGL(k_mp_basis<true>, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr, D.d_U,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mp_basis<true, QPt>), dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr, D.d_U,
                        D.d_dU, q0, nqc, B, ldb, dens);
     bem_prof_end(BH_K_MP_BASIS, g.st);
     const int ns = (nqc + MP_QSPLIT - 1) / MP_QSPLIT;
@@ -735,23 +791,14 @@ int mp_coefficients(bh_object *o, const Plan &P, int n, cd *coef)
   return 0;
 }
 
-// the series coefficients of source object n for all nb beams of the
-// many-beam loop: coef[idx][b 16 + set 4 + c].  The basis of the source
-// points is formed once and shared by every beam; the moments of all beams
-// are one product (idx x points) x (points x 16 nb) per chunk of points.
-int mp_coefficients_multi(bh_object *o, const Plan &P, int n, cd *coef, int nb)
+// the series coefficients of one source object (centre, order and rule in
+// S; its rule table) for all nb beams: coef[idx][b 16 + set 4 + c].  The
+// basis of the source points is formed once and shared by every beam; the
+// moments of all beams are one product (idx x points) x (points x 16 nb)
+// per chunk of points.
+template <class R> int series_multi(bh_object *o, const MpSrc &S, const R *rule, cd *coef, int nb)
 {
   const Dom &D = o->dom[0];
-  MpSrc S{};
-  for (int d = 0; d < 3; d++) S.c[d] = P.geo[n].c[d];
-  S.kr = o->kn[0].real();
-  S.scale = g_rscale[2];
-  S.L = P.Lsrc[n];
-  S.Lp = S.L + 1;
-  S.qoff = P.rule[n];
-  S.npt = P.npt[n];
-  S.dNe = D.dNe;
-  S.nq = D.dNe * S.npt;
   const int nidx = mp_count(S.Lp);
   const size_t ncol = (size_t)MP_NDENS * nb;
   // points per chunk: the basis (idx x points) and the densities (points x
@@ -766,16 +813,33 @@ int mp_coefficients_multi(bh_object *o, const Plan &P, int n, cd *coef, int nb)
   for (int q0 = 0; q0 < S.nq; q0 += qc) {
     const int nqc = std::min(qc, S.nq - q0);
     bem_prof_begin(BH_K_MP_BASIS, (double)nqc * nidx, g.st);
-    hipLaunchKernelGGL(k_mp_basis<false>, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr,
-                       (const cd *)nullptr, (const cd *)nullptr, q0, nqc, B, (size_t)qc, (cd *)nullptr);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mp_basis<false, R>), dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo,
+                       D.d_sid, rule, (const cd *)nullptr, (const cd *)nullptr, q0, nqc, B, (size_t)qc, (cd *)nullptr);
     bem_prof_end(BH_K_MP_BASIS, g.st);
-    hipLaunchKernelGGL(k_mp_dens_multi, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, g.d_glr,
+    hipLaunchKernelGGL(k_mp_dens_multi<R>, dim3((nqc + 63) / 64), dim3(64), 0, g.st, S, o->d_geo, D.d_sid, rule,
                        (const cd *)mb_dom_ptr(o, 0, 0), (const cd *)mb_dom_ptr(o, 0, 1), us, nb, q0, nqc, dens);
     bem_zgemm(nidx, (int)ncol, nqc, 1.0, 0.0, B, (size_t)qc, dens, ncol, q0 ? 1.0 : 0.0, 0.0, T, ncol, g.st);
   }
   hipLaunchKernelGGL(k_mp_coef, g1((size_t)nidx * 4 * nb), dim3(256), 0, g.st, S.L, S.Lp, S.kr, T, coef, nb);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// the series coefficients of source object n for all nb beams of the
+// many-beam loop (its far rule)
+int mp_coefficients_multi(bh_object *o, const Plan &P, int n, cd *coef, int nb)
+{
+  MpSrc S{};
+  for (int d = 0; d < 3; d++) S.c[d] = P.geo[n].c[d];
+  S.kr = o->kn[0].real();
+  S.scale = g_rscale[2];
+  S.L = P.Lsrc[n];
+  S.Lp = S.L + 1;
+  S.qoff = P.rule[n];
+  S.npt = P.npt[n];
+  S.dNe = o->dom[0].dNe;
+  S.nq = S.dNe * S.npt;
+  return series_multi(o, S, (const QPt *)g.d_glr, coef, nb);
 }
 
 // the scattered field of src at dst's node slots by direct sums, added into
@@ -1270,5 +1334,101 @@ extern "C" int bh_mo_solve_beams(bh_object **objs, int N, int nb, const double *
   }
   if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return object_solve_EH_multi(o); }))) return rc;
   if (iters) *iters = iter;
+  return 0;
+}
+
+// scattered E, H of every beam of the many-beam solve at np points of the
+// open region (mo_EH_mEMP_s summed over the objects, mo_field.c:154-181,
+// d3b1_field.c:68-138; example4.c:51-177 calls it once per point in one
+// run per beam).  Each object's potentials are the series of the points of
+// rule type (0..3, as dcoef_rt: at these distances no element switches
+// rule) about its centre, evaluated for all beams as products
+// (points x coefficients) x (coefficients x 16 nb).  Every point must lie
+// at least 1/0.6 bounding radii from every object's centre (order as the
+// coupling series: truncation about 1e-10); the discrete solid-angle sums
+// F, dF of a closed surface seen from outside are quadrature error there
+// and are taken as zero.  what 0: E, H ([nb][np][6] complex); 1: |E|^2,
+// |H|^2 ([nb][np][2]).
+extern "C" int bh_mo_far_field_beams(bh_object **objs, int N, int type, int np, const double *pts, int what,
+                                     double *out)
+{
+  API_LOCK;
+  if (N <= 0 || np < 0 || type < 0 || type > 3 || what < 0 || what > 1) return fail(BH_EARG, "bad far-field request");
+  const int nb = objs[0]->nbm;
+  for (int m = 0; m < N; m++) {
+    const bh_object *o = objs[m];
+    if (o->nbm != nb || nb <= 0) return fail(BH_ESTATE, "object %d: no many-beam solve of %d beams", m, nb);
+    if (o->nquad != 0 || o->kn[0].imag() != 0.0 || !(o->kn[0].real() > 0.0))
+      return fail(BH_EUNSUP, "far field of many beams: object %d (triangles in a lossless medium only)", m);
+  }
+  if (np == 0) return 0;
+  if (ensure_rules()) return BH_EHIP;
+  // series order of each object from the nearest point
+  std::vector<SrcGeo> geo(N);
+  std::vector<int> Ls(N);
+  for (int m = 0; m < N; m++) {
+    geo[m] = src_geo(objs[m]);
+    double d2 = 1e300;
+    for (int p = 0; p < np; p++) {
+      double q = 0.0;
+      for (int d = 0; d < 3; d++) q += (pts[3 * (size_t)p + d] - geo[m].c[d]) * (pts[3 * (size_t)p + d] - geo[m].c[d]);
+      d2 = std::min(d2, q);
+    }
+    const double rho = geo[m].rad / sqrt(d2), kR = objs[m]->kn[0].real() * geo[m].rad;
+    if (!(rho <= MP_RHO))
+      return fail(BH_EUNSUP, "far field of many beams: a point lies %.4g from object %d's centre (bounding radius %.4g), "
+                  "closer than the series allows (bh_mo_field per beam)", sqrt(d2), m, geo[m].rad);
+    Ls[m] = std::max((int)ceil(log(MP_EPS) / log(rho)), (int)ceil(kR + 6.0 * cbrt(kR)) + 4);
+    if (Ls[m] > MP_LMAX) return fail(BH_EUNSUP, "far field of many beams: object %d needs order %d", m, Ls[m]);
+  }
+  const size_t ncol = (size_t)MP_NDENS * nb, neh = (size_t)nb * np * 6;
+  double *d_pts = ws<double>(W_FPTS, 3 * (size_t)np);
+  cd *EH = ws<cd>(W_FEH, neh);
+  if (!d_pts || !EH) return fail(BH_EHIP, "far field: %d points of %d beams", np, nb);
+  HIPCHK(hipMemcpyAsync(d_pts, pts, 3 * (size_t)np * sizeof(double), hipMemcpyHostToDevice, g.st));
+  hipLaunchKernelGGL(k_zero_cd, g1(neh), dim3(256), 0, g.st, EH, neh);
+  for (int m = 0; m < N; m++) {
+    bh_object *o = objs[m];
+    MpSrc S{};
+    for (int d = 0; d < 3; d++) S.c[d] = geo[m].c[d];
+    S.kr = o->kn[0].real();
+    S.scale = g_rscale[type];
+    S.L = Ls[m];
+    S.Lp = S.L + 1;
+    S.qoff = 0;
+    S.npt = g_nrule[type];
+    S.dNe = o->dom[0].dNe;
+    S.nq = S.dNe * S.npt;
+    const int nidx = mp_count(S.Lp);
+    cd *coef = ws<cd>(W_COEF, (size_t)nidx * ncol);
+    if (!coef) return fail(BH_EHIP, "far field: coefficients of %d beams", nb);
+    int r = series_multi(o, S, (const QPtN *)g_rule[type], coef, nb);
+    if (r) return r;
+    const int pc = (int)std::max<size_t>(64, std::min<size_t>((size_t)np, std::min(((size_t)64 << 20) / nidx,
+                                                                                   ((size_t)64 << 20) / ncol)));
+    cd *Sb = ws<cd>(W_MS, (size_t)pc * nidx), *Cm = ws<cd>(W_MC, (size_t)pc * ncol);
+    if (!Sb || !Cm) return fail(BH_EHIP, "far field: evaluation work area");
+    const double chi = -o->lambda0 / (2.0 * M_PI);   // ch = lambda0 / (2 pi i)
+    for (int p0 = 0; p0 < np; p0 += pc) {
+      const int n = std::min(pc, np - p0);
+      bem_prof_begin(BH_K_MP_EVAL, (double)n * nidx * nb, g.st);
+      hipLaunchKernelGGL(k_mp_sbasis, dim3((n + 63) / 64), dim3(64), 0, g.st, d_pts + 3 * (size_t)p0, n, S.c[0], S.c[1],
+                         S.c[2], S.kr, S.Lp, Sb);
+      bem_zgemm(n, (int)ncol, nidx, 1.0, 0.0, Sb, (size_t)nidx, coef, ncol, 0.0, 0.0, Cm, ncol, g.st);
+      hipLaunchKernelGGL(k_mp_far_eh, g1((size_t)n * nb), dim3(256), 0, g.st, Cm, n, nb, 0.0, chi, p0,
+                         (size_t)np * 6, EH);
+      bem_prof_end(BH_K_MP_EVAL, g.st);
+    }
+  }
+  if (what == 0) {
+    HIPCHK(hipMemcpyAsync(out, EH, neh * sizeof(cd), hipMemcpyDeviceToHost, g.st));
+  } else {
+    double *I = ws<double>(W_FI, 2 * (size_t)nb * np);
+    if (!I) return fail(BH_EHIP, "far field: intensities");
+    hipLaunchKernelGGL(k_eh_intensity, g1((size_t)nb * np), dim3(256), 0, g.st, EH, (size_t)nb * np, I);
+    HIPCHK(hipMemcpyAsync(out, I, 2 * (size_t)nb * np * sizeof(double), hipMemcpyDeviceToHost, g.st));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(g.st));
   return 0;
 }

@@ -186,6 +186,7 @@ struct bh_object {
   bem::Scratch mb_sys[3];              // B, X, R: [nb][8NN]
   bem::Scratch mb_ws[8];               // solve and boundary-field work areas
   bem::Scratch mb_beams;               // the beams (Beam records)
+  std::vector<Beam> mb_hbeams;         // and their host copies
   bool mb_xwarm = false;               // X solves the current factors' system for an earlier B
   int mb_iters = 0;                    // refinement steps of the last many-beam solve
   double mb_res = 0.0;                 // its largest final relative residual
@@ -204,6 +205,7 @@ int object_solve_EH(bh_object *o);
 int object_beams_set(bh_object *o, int nb, const double *par);
 int object_solve_multi(bh_object *o);
 int object_solve_EH_multi(bh_object *o);
+int object_select_beam(bh_object *o, int b);
 cd *mb_ptr(bh_object *o, int which);   // 0 Ui0, 1 dUi0, 2 Ui, 3 dUi, 4 Uip
 cd *mb_dom_ptr(bh_object *o, int d, int which);   // 0 U, 1 dU, 2 E, 3 dE, 4 H, 5 dH
 int prepare_solve(bh_object *o);

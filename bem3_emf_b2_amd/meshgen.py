@@ -232,6 +232,33 @@ def scene16(seed=2024, box_um=10.0, n_obj=16):
     return objs
 
 
+def scene_beams():
+    """Four objects for many incident beams: two spheres of radius 1 um
+    (2028 triangles each), a cube of edge 1.5 um (3072) and a cone of base
+    radius 0.75 um and height 1.5 um (1520), n = 1.59, centres 3 um from
+    the origin along -x, +x, +y and -y.  'V' is centred at the origin;
+    'center' is the object's position."""
+    parts = [("sphere", cubesphere(1.0, 13), (-3.0, 0.0, 0.0)),
+             ("sphere", cubesphere(1.0, 13), (3.0, 0.0, 0.0)),
+             ("cube", box(1.5, 16), (0.0, 3.0, 0.0)),
+             ("cone", cone(0.75, 1.5, 40, 10), (0.0, -3.0, 0.0))]
+    return [dict(kind=k, n=1.59, center=np.array(c), V=VF[0], F=VF[1]) for k, VF, c in parts]
+
+
+def random_plane_waves(nb, seed, lambda0=1.064, n0=1.33, power=1.0):
+    """nb plane waves (ipw.txt rows) with directions uniform on the sphere
+    (theta = arccos(1 - 2u), phi = 2 pi v) and polarisation angle psi
+    uniform in [0, pi): e0 = (cos psi, sin psi) in the wave's own frame"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nb):
+        th = math.acos(1.0 - 2.0 * rng.uniform())
+        ph = 2.0 * math.pi * rng.uniform()
+        psi = math.pi * rng.uniform()
+        out.append([lambda0, n0, power, math.cos(psi), 0.0, math.sin(psi), 0.0, 0.0, 0.0, 0.0, th, ph])
+    return np.array(out)
+
+
 def ensure_dir(p):
     os.makedirs(p, exist_ok=True)
     return p

@@ -169,6 +169,25 @@ int bh_mo_solve_beams(bh_object **objs, int N, int nb, const double *par, double
 /* one beam's values after bh_mo_solve_beams: which 0..5 as
  * bh_object_state (domain did), 6..9 the incident data Ui, dUi, Ui0, dUi0 */
 int bh_object_beam_state(bh_object *o, int b, int did, int which, double *buf);
+/* beam b of the many-beam solve made the object's one-beam state: its
+ * boundary values and incident data copied into the object's arrays and its
+ * plane wave made the object's beam set, so that bh_mo_field, bh_mo_force,
+ * bh_mo_absorb and bh_mo_EH_bd report beam b (what one reference run with
+ * that beam alone holds after mo_solve_iter, mo_solve.c:10-88)          */
+int bh_object_select_beam(bh_object *o, int b);
+/* force and torque of every object under every beam of the many-beam solve
+ * (mo_force_FN, mo_force.c:10-54, of each object in one reference run per
+ * beam): rc[3 m] the torque centre of object m (null: origin); F, Nq
+ * [nb][N][3]; flag[b N + m] as bh_mo_force's return (-1: no open-domain
+ * faces), may be null.  The objects keep the last beam's state.         */
+int bh_mo_force_beams(bh_object **objs, int N, int type, const double *rc, double *F, double *Nq, int *flag);
+/* scattered E, H of every beam of the many-beam solve at np points of the
+ * open region far from the objects (mo_EH_mEMP_s, mo_field.c:154-181, as
+ * example4.c:51-177 calls it per point), by per-object series of rule type
+ * (0..3): every point at least 1/0.6 bounding radii from each object's
+ * centre.  what 0: E, H ([nb][np][6] complex), 1: |E|^2, |H|^2
+ * ([nb][np][2] doubles).                                                */
+int bh_mo_far_field_beams(bh_object **objs, int N, int type, int np, const double *pts, int what, double *out);
 /* fields at np points (mo_EH_s/t/i, mo_mEMP_s/t/i, mo_EH_mEMP_s/t/i;
  * mo_field.c:11-282): what 0: E,H (6 complex per point), 1: U (4 complex),
  * 2: E,H from the potentials' gradients (EH_mEMP_s, d3b1_field.c:68-138,

@@ -4,7 +4,12 @@ mixed-precision form: three objects, two of them close enough for element
 sums between them, four beams of seeded random directions and
 polarisations.  Both loops are run to round-off (a change below 1e-24 or
 40 iterations), so the boundary values U, dU, E, H of every domain, object
-and beam agree to 1e-8."""
+and beam agree to 1e-8, and so do the per-beam force and torque of every
+object (bh_mo_force_beams against bh_mo_force of the one-beam run, both
+rules).  The far field of every beam by the objects' series
+(bh_mo_far_field_beams, 7-point rule) against the element sums of
+EH_mEMP_s (bh_mo_field after bh_object_select_beam) at points 2000
+wavelengths and 12 um away: 1e-8 relative."""
 import numpy as np
 import pytest
 
@@ -48,6 +53,15 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def _far_points():
+    rng = np.random.default_rng(12)
+    u = rng.normal(size=(150, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    u[:100] *= 2000.0 * 1.064
+    u[100:] *= 12.0
+    return u
+
+
 def test_many_beams_match_one_loop_per_beam():
     import bem3_emf_b2_amd as bem
     beams = _beams(4, 8)
@@ -57,9 +71,24 @@ def test_many_beams_match_one_loop_per_beam():
     assert series >= 2 and direct >= 2
     got = [[{(d, k): g.beam_state(b, d, k) for d in (0, 1) for k in ("U", "dU", "E", "H")} for g in G]
            for b in range(len(beams))]
+    rc = np.array([(-1.4, 0.0, 0.0), (1.2, 0.6, 0.2), (1.2, 0.6, 0.75)])   # the objects' centres
+    FN = {t: bem.mo_force_beams(G, t, rc) for t in (0, 1)}
+    pts = _far_points()
+    far = bem.mo_far_field_beams(G, pts, 1, "EH")
+    far_i = bem.mo_far_field_beams(G, pts, 1, "I")
+    assert np.allclose(far_i[..., 0], (np.abs(far[..., :3]) ** 2).sum(-1), rtol=1e-13, atol=0)
+    worst_far = 0.0
+    for b in range(len(beams)):
+        bem.select_beam(G, b)
+        ref, oid = bem.mo_field(G, pts, "EH_mEMP", "s", 1)
+        assert (oid < 0).all()
+        e = max(_rel(far[b, :100], ref[:100]), _rel(far[b, 100:], ref[100:]))   # each radius on its own scale
+        worst_far = max(worst_far, e)
+        assert e < 1e-8, (b, e)
+    print("far field by series vs element sums: %.2e" % worst_far, flush=True)
     for g in G:
         g.close()
-    worst = 0.0
+    worst = worst_f = 0.0
     for b in range(len(beams)):
         G1 = _objects(beams[b:b + 1])
         bem.mo_solve(G1, ccd=1e-24, iter_max=40)
@@ -69,6 +98,15 @@ def test_many_beams_match_one_loop_per_beam():
                     e = _rel(got[b][m][(d, k)], g.state(d, k))
                     worst = max(worst, e)
                     assert e < 1e-8, (b, m, d, k, e)
+            for t in (0, 1):
+                _, F0, N0 = bem.mo_force(G1, m, t, tuple(rc[m]))
+                F1, N1 = FN[t][0][b, m], FN[t][1][b, m]
+                assert FN[t][2][b, m] == 0
+                s = np.abs(F0).max()
+                e = max(np.abs(F1 - F0).max() / s, np.abs(N1 - N0).max() / max(np.abs(N0).max(), s))
+                worst_f = max(worst_f, e)
+                assert e < 1e-8, (b, m, t, e)
         for g in G1:
             g.close()
-    print("many beams vs one loop per beam: %.2e (%d iterations)" % (worst, it), flush=True)
+    print("many beams vs one loop per beam: states %.2e, force/torque %.2e (%d iterations)" % (worst, worst_f, it),
+          flush=True)
