@@ -144,7 +144,7 @@ def plan_stats():
 KERNELS = {"asm_gl": 0, "asm_gh": 1, "asm_self": 2, "zgemm": 3, "lu_panel": 4, "trsm": 5, "field": 6, "scatter": 7,
            "zgemm_small": 8, "field_bd": 9, "cgemm": 10, "cgemm_small": 11, "lu_panel_c": 12, "trsm_c": 13,
            "asm_glr": 14, "field_bd32": 15, "zgemv": 16, "cgemv": 17, "scatter32": 18, "scatter_far": 19,
-           "scatter_far32": 20, "mp_basis": 21, "mp_moments": 22, "mp_eval": 23}
+           "scatter_far32": 20, "mp_basis": 21, "mp_moments": 22, "mp_eval": 23, "field_rows": 24}
 # factorisation forms of bh_object_factor
 FACTOR_MODES = {"auto": 0, "dense": 1, "block": 2, "mixed": 3}
 

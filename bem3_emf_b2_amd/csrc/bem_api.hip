@@ -702,6 +702,12 @@ void bem_launch_fbeam(const double *, const double *, const int *, int, const vo
 void bem_launch_stress(const void *, const double *, const double *, const double *, int, double, const double *,
                        double *, hipStream_t);
 void bem_launch_poynting(const void *, const double *, const double *, int, double *, hipStream_t);
+void bem_launch_field_rows(const double *, const int *, const double *, int, int, const void *, const void *,
+                           const int *, int, double, double, const void *, int, const void *, double, const void *, int,
+                           const double *, const double *, const double *, const double *, size_t, void *, double *,
+                           hipStream_t);
+void bem_launch_pack_ehbv(const void *, const void *, const void *, const void *, size_t, int, int, void *, hipStream_t);
+void bem_launch_rows_add(const void *, int, int, const double *, int, const int *, size_t, void *, hipStream_t);
 void bem_launch_meh_gather(int, int, const void *, const int *, const void *, const void *, size_t, void *, void *,
                            hipStream_t);
 void bem_launch_meh_rows(int, int, const void *, const double *, const void *, const void *, const void *, size_t, double,
@@ -1660,7 +1666,7 @@ int bem::domain_ids(bh_object *o, const double *d_pts, int np, int *d_did)
 // serialised; bh_finalize releases them).  Slots 0-15: bh_mo_field,
 // bh_mo_EH_bd, bh_object_domain and the surface points of the force and
 // power use separate slots where they nest.
-#define BEM_NCWS 20
+#define BEM_NCWS 24
 static Scratch *g_cws = nullptr;
 template <class T> static T *cws(int i, size_t n)
 {
@@ -1947,31 +1953,145 @@ extern "C" int bh_mo_force(bh_object **objs, int N, int oid, int type, const dou
   return mo_force(objs, N, oid, type, rc, F, Nq);
 }
 
+// E, H of nb right-hand sides at n points from one object's domain-0
+// boundary values of the many-beam solve, added into EH[b][p][6] (beam b
+// at + b ldb; point p = idx[i], or i for idx null): the rows of the E, H
+// integral at the points (k_field_rows; on the object itself when d_t is
+// given) times the packed boundary values of every beam, in chunks of
+// points whose rows fit in about 4 GB
+static int rows_field(bh_object *src, const double *d_pts, const int *d_t, const double *d_ze, int n, const int *d_idx,
+                      int nb, cd *EH, size_t ldb)
+{
+  Dom &D = src->dom[0];
+  const size_t K = 8 * ((size_t)D.dNe + 1), ncol = 6 * (size_t)nb;
+  const int pc = (int)std::max<size_t>(4, std::min<size_t>((size_t)n, ((size_t)256 << 20) / K));
+  cd *X = cws<cd>(18, K * ncol), *R = cws<cd>(19, (size_t)pc * K), *Cm = cws<cd>(20, (size_t)pc * ncol);
+  double *Fv = cws<double>(21, pc);
+  CWS_CHECK(X && R && Cm && Fv);
+  const size_t es = (size_t)(D.dNe + 1) * 12;
+  bem_launch_pack_ehbv(mb_dom_ptr(src, 0, 2), mb_dom_ptr(src, 0, 3), mb_dom_ptr(src, 0, 4), mb_dom_ptr(src, 0, 5), es,
+                       D.dNe, nb, X, cur_st());
+  for (int p0 = 0; p0 < n; p0 += pc) {
+    const int m = std::min(pc, n - p0);
+    bem_launch_field_rows(d_pts ? d_pts + 3 * (size_t)p0 : nullptr, d_t ? d_t + p0 : nullptr,
+                       d_ze ? d_ze + 2 * (size_t)p0 : nullptr, m, 0, src->d_geo, src->d_gq, D.d_sid, D.dNe,
+                       src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g.d_glr, g_rscale[2], g_qrule[2],
+                       g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, K, R, Fv, cur_st());
+    bem_zgemm(m, (int)ncol, (int)K, 1.0, 0.0, R, K, X, ncol, 0.0, 0.0, Cm, ncol, cur_st());
+    bem_launch_rows_add(Cm, m, nb, Fv, 1, d_idx ? d_idx + p0 : nullptr, ldb, d_idx ? EH : EH + 6 * (size_t)p0, cur_st());
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// force and torque of object oid under all nb beams (mo_force_FN with the
+// total field of mo_EH_t_bd, mo_force.c:10-240, mo_field.c:227-254, for
+// every beam at once): the stored node values of every beam (rule 0), the
+// self boundary integral at the other surface points and the fields of the
+// other objects as products of integral rows with the boundary values of
+// all beams, each beam's incident wave, then the stress sums per beam
+static int force_beams_one(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq,
+                           int *flag)
+{
+  bh_object *o = objs[oid];
+  const int nb = o->nbm;
+  Dom &D = o->dom[0];
+  SurfPts sp;
+  surface_points(o, type, sp);
+  const int np = (int)sp.t.size();
+  for (int b = 0; b < nb; b++) {
+    for (int j = 0; j < 3; j++) { F[3 * (size_t)b * N + j] = 0.0; Nq[3 * (size_t)b * N + j] = 0.0; }
+    if (flag) flag[(size_t)b * N] = np ? 0 : -1;
+  }
+  if (np == 0) return 0;
+  int *d_t, *d_st, *d_did;
+  double *d_ze, *d_r, *d_w, *d_wt, *d_rc;
+  if (cput(0, &d_t, sp.t.data(), np) || cput(1, &d_st, sp.stored.data(), np) ||
+      cput(2, &d_ze, sp.ze.data(), 2 * (size_t)np) || cput(3, &d_r, sp.r.data(), 3 * (size_t)np) ||
+      cput(4, &d_w, sp.w.data(), 3 * (size_t)np) || cput(5, &d_wt, sp.wt.data(), np) || cput(16, &d_rc, rc, 3))
+    return BH_EHIP;
+  const size_t ldb = (size_t)np * 6;
+  cd *EH = cws<cd>(17, ldb * nb);
+  d_did = cws<int>(9, np);
+  double *part = cws<double>(15, 6 * (size_t)np * nb);
+  CWS_CHECK(EH && d_did && part);
+  HIPCHK(hipMemsetAsync(EH, 0, ldb * nb * sizeof(cd), cur_st()));
+  // stored node values of every beam
+  const size_t es = (size_t)(D.dNe + 1) * 12;
+  for (int b = 0; b < nb; b++)
+    hipLaunchKernelGGL(k_gather_stored, g1(np), dim3(256), 0, cur_st(), d_t, d_st, np,
+                       (const cd *)(mb_dom_ptr(o, 0, 2) + b * es), (const cd *)(mb_dom_ptr(o, 0, 4) + b * es), EH + b * ldb);
+  // self boundary integral at the other points
+  std::vector<int> ci, ct;
+  std::vector<double> cze;
+  for (int p = 0; p < np; p++)
+    if (sp.stored[p] < 0) {
+      ci.push_back(p);
+      ct.push_back(sp.t[p]);
+      cze.push_back(sp.ze[2 * p]);
+      cze.push_back(sp.ze[2 * p + 1]);
+    }
+  const int nc = (int)ci.size();
+  if (nc > 0) {
+    int *d_ci, *d_ct;
+    double *d_cze;
+    if (cput(12, &d_ci, ci.data(), nc) || cput(13, &d_ct, ct.data(), nc) || cput(14, &d_cze, cze.data(), cze.size()))
+      return BH_EHIP;
+    int r = rows_field(o, nullptr, d_ct, d_cze, nc, d_ci, nb, EH, ldb);
+    if (r) return r;
+  }
+  // the other objects (every point in their open domain)
+  std::vector<int> did(np);
+  for (int m = 0; m < N; m++) {
+    if (m == oid) continue;
+    bh_object *q = objs[m];
+    if (domain_ids(q, d_r, np, d_did)) return BH_EHIP;
+    HIPCHK(hipMemcpyAsync(did.data(), d_did, np * sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+    HIPCHK(hipStreamSynchronize(cur_st()));
+    for (int p = 0; p < np; p++)
+      if (did[p] != 0) return fail(BH_EDOM, "force: a surface point of object %d lies inside object %d", oid, m);
+    int r = rows_field(q, d_r, nullptr, nullptr, np, nullptr, nb, EH, ldb);
+    if (r) return r;
+  }
+  // incident wave of each beam, stress sums
+  const Beam *db = (const Beam *)objs[0]->mb_beams.p;
+  const double epd = o->n[0].real() * o->n[0].real();
+  int *d_ones;
+  if (cput(11, &d_ones, std::vector<int>(np, 1).data(), np)) return BH_EHIP;
+  for (int b = 0; b < nb; b++) {
+    bem_launch_add_incident(d_r, d_ones, np, db + b, 1, EH + b * ldb, 0, cur_st());
+    bem_launch_stress(EH + b * ldb, d_r, d_w, d_wt, np, epd, d_rc, part + 6 * (size_t)np * b, cur_st());
+  }
+  std::vector<double> hp(6 * (size_t)np * nb);
+  HIPCHK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
+  for (int b = 0; b < nb; b++) {
+    double *Fb = F + 3 * (size_t)b * N, *Nb = Nq + 3 * (size_t)b * N;
+    const double *pb = hp.data() + 6 * (size_t)np * b;
+    for (int p = 0; p < np; p++)
+      for (int j = 0; j < 3; j++) { Fb[j] += -0.5 * pb[6 * p + j]; Nb[j] += -0.5 * pb[6 * p + 3 + j]; }
+  }
+  return 0;
+}
+
 // force and torque of every object under every beam of the many-beam solve
-// (mo_force_FN of each object in one reference run per beam): each beam in
-// turn made the objects' one-beam state (bh_object_select_beam), then the
-// force of every object about rc[3 m] (null: the origin).  F, Nq: [nb][N][3];
-// flag[b N + m] = the one-beam return value (-1: no open-domain faces).
-// The objects keep the last beam's state.
+// (mo_force_FN of each object in one reference run per beam): rc[3 m] the
+// torque centre of object m (null: the origin); F, Nq: [nb][N][3];
+// flag[b N + m] = -1 when object m has no open-domain faces, else 0
 extern "C" int bh_mo_force_beams(bh_object **objs, int N, int type, const double *rc, double *F, double *Nq, int *flag)
 {
   API_LOCK;
-  if (N <= 0) return fail(BH_EARG, "no objects");
+  if (N <= 0 || type < 0 || type > 1) return fail(BH_EARG, "bad force request");
+  if (ensure_rules()) return BH_EHIP;
   const int nb = objs[0]->nbm;
   for (int m = 0; m < N; m++)
-    if (objs[m]->nbm != nb || nb <= 0) return fail(BH_ESTATE, "object %d: no many-beam solve of %d beams", m, nb);
+    if (objs[m]->nbm != nb || nb <= 0 || objs[m]->mb_dom.empty())
+      return fail(BH_ESTATE, "object %d: no many-beam solve of %d beams", m, nb);
   const double zero[3] = {0.0, 0.0, 0.0};
-  for (int b = 0; b < nb; b++) {
-    for (int m = 0; m < N; m++) {
-      int r = object_select_beam(objs[m], b);
-      if (r) return r;
-    }
-    for (int m = 0; m < N; m++) {
-      const size_t o = ((size_t)b * N + m) * 3;
-      int r = mo_force(objs, N, m, type, rc ? rc + 3 * (size_t)m : zero, F + o, Nq + o);
-      if (r > 0) return r;
-      if (flag) flag[(size_t)b * N + m] = r;
-    }
+  for (int m = 0; m < N; m++) {
+    int r = force_beams_one(objs, N, m, type, rc ? rc + 3 * (size_t)m : zero, F + 3 * (size_t)m, Nq + 3 * (size_t)m,
+                            flag ? flag + m : nullptr);
+    if (r) return r;
   }
   return 0;
 }

@@ -147,6 +147,15 @@ struct QPtN {
   double z, e, w, n0, n1, n2;
 };
 
+// shape functions at a rule point: the triangle's interpolation functions
+// (a QPt table: the far rules), or the values a QPtN table holds (the
+// 4-point node rule carries its own)
+__device__ __forceinline__ void rule_N(const QPt &q, double &n0, double &n1, double &n2) { tri_N(q.z, q.e, n0, n1, n2); }
+__device__ __forceinline__ void rule_N(const QPtN &q, double &n0, double &n1, double &n2)
+{
+  n0 = q.n0; n1 = q.n1; n2 = q.n2;
+}
+
 // point of a rule on the square [-1,1]^2 with its four shape functions
 // (the 4-point node rule carries N = delta and weight 1)
 struct QPtQ {

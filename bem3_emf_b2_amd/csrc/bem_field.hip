@@ -28,9 +28,9 @@ __device__ __forceinline__ void tri_pos(const TriGeo &g, double z, double e, dou
 
 // G[3], H[3], F at rt from one triangle with a tabulated rule; optional
 // derivative along v (dG, dH, dF).  scale = rule scale (aW applied to G).
-template <bool DER>
+template <bool DER, class RT = QPtN>
 __device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, const double *v, double kr, double ki,
-                                         const QPtN *__restrict__ rule, int nq, double scale, bool flip, cd *G, cd *H,
+                                         const RT *__restrict__ rule, int nq, double scale, bool flip, cd *G, cd *H,
                                          double &F, cd *dG, cd *dH, double &dF)
 {
   cd aG[3] = {mk(0, 0), mk(0, 0), mk(0, 0)}, aH[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
@@ -38,7 +38,7 @@ __device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, cons
   double f = 0.0, df = 0.0;
   double Wdv = DER ? g.W[0] * v[0] + g.W[1] * v[1] + g.W[2] * v[2] : 0.0;
   for (int p = 0; p < nq; p++) {
-    const QPtN q = rule[p];
+    const RT q = rule[p];
     double r[3];
     tri_pos(g, q.z, q.e, r);
     double vx = r[0] - rt[0], vy = r[1] - rt[1], vz = r[2] - rt[2];
@@ -48,7 +48,9 @@ __device__ __forceinline__ void tri_coef(const TriGeo &g, const double *rt, cons
     cd cb = mk(-1.0 - ki * R, kr * R);
     cd cbce = cb * ce;
     cd gk = ce * iR, hk = cbce * (rdW * iR2);
-    double wn[3] = {q.w * q.n0, q.w * q.n1, q.w * q.n2};
+    double n0, n1, n2;
+    rule_N(q, n0, n1, n2);
+    double wn[3] = {q.w * n0, q.w * n1, q.w * n2};
 #pragma unroll
     for (int i = 0; i < 3; i++) { fma_cr(aG[i], gk, wn[i]); fma_cr(aH[i], hk, wn[i]); }
     f = fma(q.w, rdW * iR2, f);
@@ -988,6 +990,178 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
   }
 }
 
+// ---------------------------------------------------------------- field rows
+// Rows of the E, H boundary integral of one object's domain did at np
+// points, for fields of many right-hand sides as one product: the element
+// coefficients G_n, H_n of EH_s / EH_s_bd (d3b1_field.c) written out
+// instead of summed, R[p][4 s + n] = G_n and R[p][4 (dNe+1) + 4 s + n] = H_n
+// of element s (n < 3 for a triangle, < 4 for a quadrangle, the other
+// entries and element 0 zero), F[p] = the solid-angle sum.  The field of a
+// right-hand side at p is sum_j R[p][j] X[j] / (did == 0 ? 1 + F : F) with
+// X = [dE | dH ; -E | -H] by (element, node) rows.  SELF: points on the
+// object (element tgt_t[p], local coordinates tgt_ze[p]), the self element
+// by the polar rule of k_field_bd; otherwise points pts off the object.
+// Triangles farther than 4 element radii take the reduced far rules (FP64,
+// the tiers of the mixed-precision form, relative error below 3e-8),
+// nearer ones and quadrangles the Gauss-Legendre rules.
+template <bool SELF>
+__global__ void __launch_bounds__(256) k_field_rows(const double *__restrict__ pts, const int *__restrict__ tgt_t,
+                                                 const double *__restrict__ tgt_ze, int np, int did,
+                                                 const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                 const int *__restrict__ sid, int dNe, double kr, double ki,
+                                                 const QPtN *__restrict__ gl, int ngl, const QPt *__restrict__ glr,
+                                                 double scale, const QPtQ *__restrict__ qgl, int nqgl,
+                                                 const double *__restrict__ xl, const double *__restrict__ wl,
+                                                 const double *__restrict__ xh, const double *__restrict__ wh, size_t ld,
+                                                 cd *__restrict__ R, double *__restrict__ Fo)
+{
+  const int p = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (p >= np) return;
+  double rt[3];
+  int t = 0, td = 0, at = 0;
+  TriGeo gt{};
+  bool tquad = false;
+  double zt = 0.0, et = 0.0;
+  if (SELF) {
+    t = tgt_t[p];
+    zt = tgt_ze[2 * p];
+    et = tgt_ze[2 * p + 1];
+    td = sid[t];
+    at = td < 0 ? -td : td;
+    gt = geo[at];
+    tquad = gt.quad != 0.0;
+    if (tquad) {
+      double Wt[3];
+      quad_geom(gt, gq[at], zt, et, rt, Wt);
+    } else {
+      tri_pos(gt, zt, et, rt);
+    }
+  } else {
+    rt[0] = pts[3 * p]; rt[1] = pts[3 * p + 1]; rt[2] = pts[3 * p + 2];
+  }
+  cd *Rg = R + (size_t)p * ld, *Rh = Rg + 4 * ((size_t)dNe + 1);
+  if (lane < 4) { Rg[lane] = mk(0, 0); Rh[lane] = mk(0, 0); }
+  double F = 0.0;
+  for (int s = 1 + lane; s <= dNe; s += 64) {
+    if (SELF && s == t) continue;   // written below
+    const int sd = sid[s], as = sd < 0 ? -sd : sd;
+    const TriGeo g = geo[as];
+    cd G[4], H[4];
+    double f = 0.0;
+    if (g.quad != 0.0) {
+      quad_coef(g, gq[as], rt, kr, ki, qgl, nqgl, sd < 0, G, H, f);
+    } else {
+      cd x0[3], x1[3];
+      double x2;
+      const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+      const double rad2 = elem_rad2(g), d2 = dx * dx + dy * dy + dz * dz;
+      const bool far = d2 > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
+      int qoff, nq;
+      far_rule(far, sqrt((kr * kr + ki * ki) * rad2), d2 / rad2, qoff, nq);
+      if (far) tri_coef<false, QPt>(g, rt, nullptr, kr, ki, glr + qoff, nq, scale, sd < 0, G, H, f, x0, x1, x2);
+      else tri_coef<false>(g, rt, nullptr, kr, ki, gl, ngl, scale, sd < 0, G, H, f, x0, x1, x2);
+      G[3] = H[3] = mk(0, 0);
+    }
+    F += f;
+#pragma unroll
+    for (int n = 0; n < 4; n++) { Rg[4 * (size_t)s + n] = G[n]; Rh[4 * (size_t)s + n] = H[n]; }
+  }
+  if (SELF) {
+    // self element: polar rule about (zt, et), as k_field_bd
+    cd sg[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, shq[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+    double sfq = 0.0, bdC = 0.0;
+    bool warped = false;
+    if (tquad) {
+      const QuadExt qt = gq[at];
+      bdC = gt.c1[0] * qt.W2[0] + gt.c1[1] * qt.W2[1] + gt.c1[2] * qt.W2[2];
+      const double n1 = sqrt(gt.c1[0] * gt.c1[0] + gt.c1[1] * gt.c1[1] + gt.c1[2] * gt.c1[2]);
+      const double n2 = sqrt(qt.W2[0] * qt.W2[0] + qt.W2[1] * qt.W2[1] + qt.W2[2] * qt.W2[2]);
+      warped = !(fabs(bdC / (n1 * n2)) < 1e-15) && bdC != 0.0;
+      quad_self_part(gt, qt, zt, et, kr, ki, xl, wl, xh, wh, lane, warped, sg, shq, sfq);
+    } else {
+      double zp = 1.0 - zt, zm = -0.5 - zt, ep = 0.5 * BEM_SQ3 - et, em = -0.5 * BEM_SQ3 - et;
+      double r[4] = {sqrt(zm * zm + em * em), sqrt(zp * zp + et * et), sqrt(zm * zm + ep * ep), 0.0};
+      double th[4] = {atan2(em, zm), atan2(-et, zp), atan2(ep, zm), 0.0};
+      r[3] = r[0];
+      th[3] = th[0] + 2.0 * M_PI;
+      for (int ia = lane; ia < 384; ia += 64) {
+        int j = ia / 128, m = (ia % 128) >> 1, half = ia & 1;
+        double dth = th[j + 1] - th[j];
+        double tt = 0.25 * dth * xh[m] + (half ? 0.25 * (3.0 * th[j + 1] + th[j]) : 0.25 * (th[j + 1] + 3.0 * th[j]));
+        double rm = r[j] * r[j + 1] * sin(th[j + 1] - th[j]) / (r[j] * sin(tt - th[j]) - r[j + 1] * sin(tt - th[j + 1]));
+        double c, sn;
+        sincos(tt, &sn, &c);
+        double K[3] = {gt.c1[0] * c + gt.c2[0] * sn, gt.c1[1] * c + gt.c2[1] * sn, gt.c1[2] * c + gt.c2[2] * sn};
+        double aK = sqrt(K[0] * K[0] + K[1] * K[1] + K[2] * K[2]), iaK = 1.0 / aK;
+        double fg = wh[m] * 0.25 * dth * 0.5 * rm;
+        for (int i = 0; i < 12; i++) {
+          double rr = 0.5 * rm * (xl[i] + 1.0);
+          double n0, n1, n2;
+          tri_N(rr * c + zt, rr * sn + et, n0, n1, n2);
+          cd vv = eikr(kr, ki, aK * rr) * (iaK * wl[i] * fg);
+          fma_cr(sg[0], vv, n0); fma_cr(sg[1], vv, n1); fma_cr(sg[2], vv, n2);
+        }
+      }
+    }
+    const int nst = tquad ? 4 : 3;
+    const double sG = tquad ? BEM_I_FP : BEM_I_SPSQ3 * gt.aW;
+    for (int a = 0; a < nst; a++) sg[a] = wsum(sg[a]) * sG;
+    if (warped) {   // wave-uniform
+      const double sH = (td < 0 ? -BEM_I_FP : BEM_I_FP) * bdC;
+      for (int a = 0; a < 4; a++) shq[a] = wsum(shq[a]) * sH;
+      sfq = wsum(sfq) * sH;
+    }
+    if (lane == 0) {
+      for (int n = 0; n < 4; n++) {
+        Rg[4 * (size_t)t + n] = n < nst ? sg[n] : mk(0, 0);
+        Rh[4 * (size_t)t + n] = shq[n];
+      }
+    }
+    F = wsum(F);
+    F += sfq;
+  } else {
+    F = wsum(F);
+  }
+  if (lane == 0) Fo[p] = F;
+}
+
+// X of nb right-hand sides for the rows of k_field_rows: X[j][b 6 + l] with
+// rows j = 4 s + n: dE (l < 3), dH (l >= 3); rows 4 (dNe+1) + 4 s + n: -E,
+// -H (boundary values [s][n][3] of beam b at + b es)
+__global__ void k_pack_ehbv(const cd *__restrict__ E, const cd *__restrict__ dE, const cd *__restrict__ H,
+                            const cd *__restrict__ dH, size_t es, int dNe, int nb, cd *__restrict__ X)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nr = 4 * ((size_t)dNe + 1);
+  if (i >= 2 * nr * nb) return;
+  const size_t j = i / nb;
+  const int b = (int)(i % nb);
+  const bool hpart = j >= nr;
+  const size_t sn = hpart ? j - nr : j;   // 4 s + n
+  const size_t o = (size_t)b * es + sn * 3;
+  cd *x = X + j * 6 * (size_t)nb + 6 * (size_t)b;
+  for (int l = 0; l < 3; l++) {
+    if (hpart) { x[l] = -E[o + l]; x[3 + l] = -H[o + l]; }
+    else { x[l] = dE[o + l]; x[3 + l] = dH[o + l]; }
+  }
+}
+
+// EH[b][idx[i]][l] += C[i][b 6 + l] / den_i, den = 1 + F (open domain) or F
+// (idx null: i; EH of beam b at + b ldb)
+__global__ void k_rows_add(const cd *__restrict__ C, int n, int nb, const double *__restrict__ F, int open,
+                           const int *__restrict__ idx, size_t ldb, cd *__restrict__ EH)
+{
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * nb) return;
+  const int i = (int)(t / nb), b = (int)(t % nb);
+  const double den = open ? 1.0 + F[i] : F[i];
+  const double s = 1.0 / den;
+  const cd *c = C + (size_t)i * 6 * nb + 6 * (size_t)b;
+  cd *o = EH + (size_t)b * ldb + 6 * (size_t)(idx ? idx[i] : i);
+  for (int l = 0; l < 6; l++) o[l] = o[l] + c[l] * s;
+}
+
 // ---------------------------------------------------------------- incident field
 __device__ __forceinline__ void beam_EH(const Beam *__restrict__ bm, int nb, const double *x, const double *v, cd *e,
                                         cd *h, cd *de, cd *dh)
@@ -1395,3 +1569,38 @@ void bem_launch_poynting(const void *EH, const double *wpt, const double *wgt, i
   hipLaunchKernelGGL(k_poynting, dim3((np + 255) / 256), dim3(256), 0, st, (const cd *)EH, wpt, wgt, np, part);
 }
 }
+
+extern "C" {
+void bem_launch_field_rows(const double *pts, const int *tgt_t, const double *tgt_ze, int np, int did, const void *geo,
+                        const void *gq, const int *sid, int dNe, double kr, double ki, const void *gl, int ngl,
+                        const void *glr, double scale, const void *qgl, int nqgl, const double *xl, const double *wl,
+                        const double *xh, const double *wh, size_t ld, void *R, double *F, hipStream_t st)
+{
+  if (np <= 0) return;
+  bem_prof_begin(BH_K_FIELD_ROWS, (double)np * dNe, st);
+  if (tgt_t)
+    hipLaunchKernelGGL(k_field_rows<true>, dim3((np + 3) / 4), dim3(256), 0, st, pts, tgt_t, tgt_ze, np, did,
+                       (const TriGeo *)geo, (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, ngl,
+                       (const QPt *)glr, scale, (const QPtQ *)qgl, nqgl, xl, wl, xh, wh, ld, (cd *)R, F);
+  else
+    hipLaunchKernelGGL(k_field_rows<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, tgt_t, tgt_ze, np, did,
+                       (const TriGeo *)geo, (const QuadExt *)gq, sid, dNe, kr, ki, (const QPtN *)gl, ngl,
+                       (const QPt *)glr, scale, (const QPtQ *)qgl, nqgl, xl, wl, xh, wh, ld, (cd *)R, F);
+  bem_prof_end(BH_K_FIELD_ROWS, st);
+}
+void bem_launch_pack_ehbv(const void *E, const void *dE, const void *H, const void *dH, size_t es, int dNe, int nb,
+                          void *X, hipStream_t st)
+{
+  const size_t n = 8 * ((size_t)dNe + 1) * nb;
+  hipLaunchKernelGGL(k_pack_ehbv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const cd *)E, (const cd *)dE,
+                     (const cd *)H, (const cd *)dH, es, dNe, nb, (cd *)X);
+}
+void bem_launch_rows_add(const void *C, int n, int nb, const double *F, int open, const int *idx, size_t ldb, void *EH,
+                         hipStream_t st)
+{
+  if (n <= 0) return;
+  const size_t m = (size_t)n * nb;
+  hipLaunchKernelGGL(k_rows_add, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, (const cd *)C, n, nb, F, open,
+                     idx, ldb, (cd *)EH);
+}
+}  // extern "C"

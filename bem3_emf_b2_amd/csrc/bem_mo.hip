@@ -160,15 +160,6 @@ struct MpSrc {
 // ROWQ: B[q][idx] (rows per point, the moment kernel's layout); otherwise
 // B[idx][q] (row-major idx x points: the A operand of the many-beam moment
 // product).  dens null: no densities (the many-beam path forms its own).
-// shape functions at a rule point: the triangle's interpolation functions
-// (far-rule table), or the values a QPtN table holds (the 4-point node rule
-// carries its own)
-__device__ __forceinline__ void rule_N(const QPt &q, double &n0, double &n1, double &n2) { tri_N(q.z, q.e, n0, n1, n2); }
-__device__ __forceinline__ void rule_N(const QPtN &q, double &n0, double &n1, double &n2)
-{
-  n0 = q.n0; n1 = q.n1; n2 = q.n2;
-}
-
 template <bool ROWQ, class R>
 __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
                                                  const R *__restrict__ rule, const cd *__restrict__ U,

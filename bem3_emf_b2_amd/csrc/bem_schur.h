@@ -67,7 +67,7 @@ void bem_acc_c2z(cd *X, const cf *d, size_t n, hipStream_t st);
 
 // many right-hand sides (nb, beam b at + b 8NN) with S formed: work holds
 // schur_work_multi(NN, nb) values; BH_ESTATE if S is not formed
-static inline size_t schur_work_multi(size_t n, size_t nb) { return 26 * n * nb + 512; }
+static inline size_t schur_work_multi(size_t n, size_t nb) { return 27 * n * nb + 512; }
 int schur_apply_multi(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, int nb, hipStream_t st);
 int schur_apply_multi(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, int nb, hipStream_t st);
 // R_b = B_b - A X_b over the rows of one domain for nb beams; work holds

@@ -399,6 +399,7 @@ static int schur_apply_multi_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt
   if (!f.s_lu) return BH_ESTATE;
   const int n = f.n;
   const size_t N = (size_t)n, M = (size_t)nb;
+  // 27 N M values: Y 3, Bc 3, v 1, u3 1, Qn 3, q0 4, T 4, U4 4, Q4 4
   Zt *Y = w, *Bc = Y + 3 * N * M, *v = Bc + 3 * N * M, *u3 = v + N * M, *Qn = u3 + N * M, *q0 = Qn + 3 * N * M,
      *T = q0 + 4 * N * M, *U4 = T + 4 * N * M, *Q4 = U4 + 4 * N * M;
   hipLaunchKernelGGL(k_mgather<Zt>, grid1(3 * N * M), dim3(256), 0, st, rhs, N, nb, 3, 4, Bc, n);

@@ -252,7 +252,8 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
  * BH_K_MP_BASIS: the coupling series' basis at the source points
  * ((point, coefficient) pairs); BH_K_MP_MOM: its moments (real flops);
  * BH_K_MP_EVAL: its evaluation at the node slots ((slot, coefficient)
- * pairs).  BH_K_ZGEMV, BH_K_CGEMV: the few-right-hand-side
+ * pairs).  BH_K_FIELD_ROWS: the E, H integral rows of the many-beam force
+ * ((point, source element) pairs).  BH_K_ZGEMV, BH_K_CGEMV: the few-right-hand-side
  * (n <= 8, k >= 256) product kernel, real flops (8 m n k).               */
 #define BH_K_ASM_GL   0
 #define BH_K_ASM_GH   1
@@ -278,7 +279,8 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
 #define BH_K_MP_BASIS 21
 #define BH_K_MP_MOM   22
 #define BH_K_MP_EVAL  23
-#define BH_K_COUNT    24
+#define BH_K_FIELD_ROWS 24
+#define BH_K_COUNT    25
 int bh_prof_enable(int on);
 int bh_prof_read(int kid, double *ms, double *work, int *launches);
 
