@@ -490,6 +490,7 @@ int bem::object_assemble(bh_object *o, int type)
   // every factorisation of earlier matrices is stale from here on
   o->fmode = 0;
   o->x_warm = false;
+  o->mb_xwarm = false;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   o->asm_prec = -1;
   for (int d = 0; d <= o->MN; d++) {
@@ -516,6 +517,7 @@ extern "C" int bh_object_set_precision(bh_object *o, int prec)
   o->prec = prec;
   o->fmode = 0;
   o->x_warm = false;
+  o->mb_xwarm = false;
   o->s_formed = false;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   return 0;
@@ -835,6 +837,8 @@ int bem::prepare_solve(bh_object *o)
       }
     }
     size_t n1 = (size_t)(D.dNe + 1);
+    D.has_inc = false;
+    for (const ColInfo &c : col) D.has_inc = D.has_inc || (!c.main && c.mdid == 0);
     if (upload(&D.d_col, col.data(), col.size()) || upload(&D.d_colS, colS.data(), colS.size()) ||
         upload(&D.d_rowEH, rows.data(), rows.size()) || dalloc(&D.d_U, n1 * 16) || dalloc(&D.d_dU, n1 * 16) ||
         dalloc(&D.d_E, n1 * 12) || dalloc(&D.d_dE, n1 * 12) || dalloc(&D.d_Hf, n1 * 12) || dalloc(&D.d_dHf, n1 * 12))
@@ -1102,6 +1106,7 @@ int bem::form_s(bh_object *o)
   HIPCHK(hipStreamSynchronize(cur_st()));
   if (info) return fail(BH_ESING, "singular Schur complement (zero pivot %d)", info);
   o->s_formed = true;
+  o->fact_gen++;
   return 0;
 }
 
@@ -1173,6 +1178,8 @@ int bem::object_factor(bh_object *o, int mode)
   HIPCHK(hipStreamSynchronize(cur_st()));
   if (info) return fail(BH_ESING, "singular system (zero pivot %d)", info);
   o->fmode = mode;
+  o->fact_gen++;
+  o->mb_xwarm = false;
   return 0;
 }
 
@@ -1398,6 +1405,13 @@ __global__ void k_zero_cd_api(cd *__restrict__ a, size_t n)
 }
 static void zero_cd(cd *a, size_t n) { hipLaunchKernelGGL(k_zero_cd_api, g1(n), dim3(256), 0, cur_st(), a, n); }
 
+// R += B - Bp (the residual of a warm start from the last one)
+__global__ void k_mresid_update(cd *__restrict__ R, const cd *__restrict__ B, const cd *__restrict__ Bp, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) R[i] = R[i] + (B[i] - Bp[i]);
+}
+
 cd *bem::mb_ptr(bh_object *o, int which) { return (cd *)o->mb_inc[which].p; }
 cd *bem::mb_dom_ptr(bh_object *o, int d, int which) { return (cd *)o->mb_dom[(size_t)d * 6 + which].p; }
 
@@ -1417,7 +1431,7 @@ int bem::object_beams_set(bh_object *o, int nb, const double *par)
       if (!o->mb_dom[(size_t)d * 6 + q].get<cd>((size_t)(o->dom[d].dNe + 1) * (q < 2 ? 16 : 12) * nb))
         return fail(BH_EHIP, "beams: boundary values of %d beams", nb);
   const size_t na = 8 * (size_t)o->NN;
-  for (int q = 0; q < 3; q++)
+  for (int q = 0; q < 4; q++)
     if (!o->mb_sys[q].get<cd>(na * nb)) return fail(BH_EHIP, "beams: systems of %d beams", nb);
   std::vector<Beam> bm(nb);
   for (int b = 0; b < nb; b++) {
@@ -1454,7 +1468,7 @@ int bem::object_solve_multi(bh_object *o)
   const int nb = o->nbm;
   if (nb <= 0) return fail(BH_ESTATE, "many beams: no beams set");
   const size_t NN = (size_t)o->NN, na = 8 * NN, E1 = (size_t)o->Ne + 1;
-  cd *B = (cd *)o->mb_sys[0].p, *X = (cd *)o->mb_sys[1].p, *R = (cd *)o->mb_sys[2].p;
+  cd *B = (cd *)o->mb_sys[0].p, *X = (cd *)o->mb_sys[1].p, *R = (cd *)o->mb_sys[2].p, *Bp = (cd *)o->mb_sys[3].p;
   cf *crhs = o->mb_ws[0].get<cf>(na * nb), *cdx = o->mb_ws[1].get<cf>(na * nb);
   cf *cw = o->mb_ws[2].get<cf>(schur_work_multi(NN, nb));
   size_t ndmax = 0;
@@ -1462,13 +1476,14 @@ int bem::object_solve_multi(bh_object *o)
   cd *rw = o->mb_ws[3].get<cd>(12 * ndmax * nb);
   double *nrm = o->mb_ws[4].get<double>(2 * (size_t)nb);
   if (!crhs || !cdx || !cw || !rw || !nrm) return fail(BH_EHIP, "many beams: work areas");
+  // B of every beam: per domain two products with 4 nb columns
   zero_cd(B, na * nb);
-  for (int b = 0; b < nb; b++)
-    for (int d = 0; d <= o->MN; d++) {
-      Dom &D = o->dom[d];
-      bem_launch_build_B(D.d_G, D.d_H, D.nd, D.nd, D.d_col, mb_ptr(o, 2) + (size_t)b * E1 * 16,
-                         mb_ptr(o, 3) + (size_t)b * E1 * 16, row_base(o, d), B + (size_t)b * na, cur_st());
-    }
+  for (int d = 0; d <= o->MN; d++) {
+    Dom &D = o->dom[d];
+    if (D.has_inc)
+      bem_build_B_multi(D.d_G, D.d_H, D.nd, D.nd, D.d_col, mb_ptr(o, 2), mb_ptr(o, 3), E1 * 16, NN, row_base(o, d), B,
+                        rw, nb, cur_st());
+  }
   auto resid = [&]() {
     for (int d = 0; d <= o->MN; d++) {
       Dom &D = o->dom[d];
@@ -1478,13 +1493,24 @@ int bem::object_solve_multi(bh_object *o)
   std::vector<double> hb(nb), hr(nb);
   bem_mnrm2(B, na, nb, nrm, cur_st());
   HIPCHK(hipMemcpyAsync(hb.data(), nrm, nb * sizeof(double), hipMemcpyDeviceToHost, cur_st()));
-  if (o->mb_xwarm) resid();
+  // warm start: R holds Bp - A X of the last solve, so B - A X = R + (B - Bp)
+  if (o->mb_xwarm) hipLaunchKernelGGL(k_mresid_update, g1(na * nb), dim3(256), 0, cur_st(), R, B, Bp, na * nb);
   else {
     zero_cd(X, na * nb);
     HIPCHK(hipMemcpyAsync(R, B, na * nb * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
   }
   o->mb_xwarm = false;
   const SchurF<cf> f = schur32(o);
+  // explicit FP32 inverses of the factors, formed once per factorisation:
+  // every solve of the refinement is then a few products over all beams
+  const size_t n2 = NN * NN;
+  cf *G0i = o->mb_inv[0].get<cf>(n2), *Mi = o->mb_inv[1].get<cf>(n2), *Si = o->mb_inv[2].get<cf>(n2);
+  if (!G0i || !Mi || !Si) return fail(BH_EHIP, "many beams: inverses of order %zu", NN);
+  if (o->mb_inv_gen != o->fact_gen) {
+    if (schur_inverses(f, G0i, Mi, Si, cur_st())) return fail(BH_EHIP, "many beams: inverses");
+    o->mb_inv_gen = o->fact_gen;
+  }
+  const SchurInv<cf> inv{G0i, Mi, Si};
   int it = 0;
   double worst = 0.0;
   for (;;) {
@@ -1495,7 +1521,7 @@ int bem::object_solve_multi(bh_object *o)
     for (int b = 0; b < nb; b++) worst = std::max(worst, hb[b] > 0.0 ? sqrt(hr[b] / hb[b]) : 0.0);
     if (worst <= IR_TOL || it >= IR_MAX) break;
     bem_cvt_z2c(R, crhs, na * nb, cur_st());
-    if (schur_apply_multi(f, crhs, true, cdx, cw, nb, cur_st())) return fail(BH_ESTATE, "many beams: S is not formed");
+    if (schur_apply_multi(f, crhs, true, cdx, cw, nb, cur_st(), &inv)) return fail(BH_ESTATE, "many beams: S is not formed");
     bem_acc_c2z(X, cdx, na * nb, cur_st());
     resid();
     it++;
@@ -1505,6 +1531,7 @@ int bem::object_solve_multi(bh_object *o)
   if (!(worst <= IR_TOL))
     return fail(BH_ECONV, "many beams: relative residual %.3g after %d refinement steps", worst, it);
   o->mb_xwarm = true;
+  HIPCHK(hipMemcpyAsync(Bp, B, na * nb * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
   for (int b = 0; b < nb; b++)
     for (int d = 0; d <= o->MN; d++) {
       Dom &D = o->dom[d];

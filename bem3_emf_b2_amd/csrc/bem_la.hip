@@ -1137,9 +1137,12 @@ static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename 
   if (m <= 0 || n <= 0) return;
   const int tx = (n + ZT - 1) / ZT, ty = (m + ZT - 1) / ZT, T = tx * ty;
   // measured on gfx950: the 128x128 kernel overtakes the 64x64 one from
-  // about 160 tiles (k >= 128) in both precisions
+  // about 160 tiles (k >= 128) in both precisions; with a long k (the
+  // products of many right-hand sides, m x 256 x m) already from 64 tiles,
+  // where its double-buffered 3M slices outrun the 64x64 kernel even with
+  // part of the CUs idle
   const bool skinny = n <= 8 && k >= 256;
-  const bool big = g_big_tiles && !skinny && T >= 160 && k >= 32;
+  const bool big = g_big_tiles && !skinny && ((T >= 160 && k >= 32) || (T >= 64 && k >= 1024));
   const int kid = big ? Cx<R>::KG : skinny ? Cx<R>::KGV : Cx<R>::KGS;
   bem_prof_begin(kid, 8.0 * m * n * k, st);
   if (skinny) {

@@ -101,7 +101,7 @@ struct Dom {
   bool gf_valid = false;               // d_Gf factors the current G (cleared by assembly)
   cf *c_Gf = nullptr;                  // complex-FP32 LU factors of G (mixed-precision form)
   bool gfc_valid = false;
-  // face table for the solid-angle domain test (faces whose main side is d)
+  bool has_inc = false;                // some column takes incident data (open-domain sub side)
 };
 
 
@@ -183,7 +183,10 @@ struct bh_object {
   int nbm = 0;
   bem::Scratch mb_inc[5];              // Ui0, dUi0, Ui, dUi, Uip: [nb][Ne+1][16]
   std::vector<bem::Scratch> mb_dom;    // per domain: U, dU ([nb][dNe+1][16]), E, dE, H, dH ([nb][dNe+1][12])
-  bem::Scratch mb_sys[3];              // B, X, R: [nb][8NN]
+  bem::Scratch mb_sys[4];              // B, X, R, the B of R: [nb][8NN]
+  bem::Scratch mb_inv[3];              // FP32 inverses of LU(G0), LU(M), LU(S) ([NN][NN])
+  unsigned fact_gen = 0;               // factorisations so far (the inverses' validity)
+  unsigned mb_inv_gen = ~0u;           // fact_gen the inverses were formed for
   bem::Scratch mb_ws[8];               // solve and boundary-field work areas
   bem::Scratch mb_beams;               // the beams (Beam records)
   std::vector<Beam> mb_hbeams;         // and their host copies

@@ -68,10 +68,28 @@ void bem_acc_c2z(cd *X, const cf *d, size_t n, hipStream_t st);
 // many right-hand sides (nb, beam b at + b 8NN) with S formed: work holds
 // schur_work_multi(NN, nb) values; BH_ESTATE if S is not formed
 static inline size_t schur_work_multi(size_t n, size_t nb) { return 27 * n * nb + 512; }
-int schur_apply_multi(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, int nb, hipStream_t st);
-int schur_apply_multi(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, int nb, hipStream_t st);
+// explicit inverses of the factors (n x n, row stride n): with many
+// right-hand sides each solve with LU(G0), LU(M), LU(S) becomes one product
+// (one launch in place of a chain of blocked triangular solves); null
+// pointers: solve with the factors
+template <class Zt> struct SchurInv {
+  const Zt *G0i, *Mi, *Si;
+};
+int schur_apply_multi(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, int nb, hipStream_t st,
+                      const SchurInv<cf> *inv = nullptr);
+int schur_apply_multi(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, int nb, hipStream_t st,
+                      const SchurInv<cd> *inv = nullptr);
+// the inverses from the factors (S formed): each LU^{-1} I by the blocked
+// triangular solves with n right-hand sides
+int schur_inverses(const SchurF<cf> &f, cf *G0i, cf *Mi, cf *Si, hipStream_t st);
 // R_b = B_b - A X_b over the rows of one domain for nb beams; work holds
 // 12 nd nb values
+// B rows of one domain for nb beams (k_build_B of every beam at once): the
+// incident data of the open-domain sub-side columns (beam b's at Ui + b
+// istride) as two operands a, c (nd x 4 nb), B = -(H a + G c); work holds
+// 12 nd nb values
+void bem_build_B_multi(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *Ui, const cd *dUi,
+                       size_t istride, size_t NN, size_t rowBase, cd *B, cd *w, int nb, hipStream_t st);
 void bem_resid_domain_multi(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
                             size_t rowBase, const cd *B, cd *R, cd *work, int nb, hipStream_t st);
 // |x_b|^2 of nb vectors of n values into out[nb] (device)

@@ -394,43 +394,91 @@ __global__ void k_mfill_x(const Zt *__restrict__ U4, const Zt *__restrict__ Q4, 
 // schur_apply_t, every product and triangular solve on all beams at once);
 // work holds schur_work_multi(n, nb) values
 template <class Zt>
-static int schur_apply_multi_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, Zt *w, int nb, hipStream_t st)
+static int schur_apply_multi_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, Zt *w, int nb, hipStream_t st,
+                               const SchurInv<Zt> *inv)
 {
   if (!f.s_lu) return BH_ESTATE;
   const int n = f.n;
   const size_t N = (size_t)n, M = (size_t)nb;
-  // 27 N M values: Y 3, Bc 3, v 1, u3 1, Qn 3, q0 4, T 4, U4 4, Q4 4
+  // 27 N M values: Y 3, Bc 3, v 1, u3 1, Qn 3, q0 4, T 4, U4 4, Q4 4 (with
+  // the inverses a product writes into a free area of the same size and
+  // the two names swap)
   Zt *Y = w, *Bc = Y + 3 * N * M, *v = Bc + 3 * N * M, *u3 = v + N * M, *Qn = u3 + N * M, *q0 = Qn + 3 * N * M,
      *T = q0 + 4 * N * M, *U4 = T + 4 * N * M, *Q4 = U4 + 4 * N * M;
   hipLaunchKernelGGL(k_mgather<Zt>, grid1(3 * N * M), dim3(256), 0, st, rhs, N, nb, 3, 4, Bc, n);
   hipLaunchKernelGGL(k_mgather<Zt>, grid1(N * M), dim3(256), 0, st, rhs, N, nb, 1, 7, u3, n);
   if (r0nz) {
     hipLaunchKernelGGL(k_mgather<Zt>, grid1(4 * N * M), dim3(256), 0, st, rhs, N, nb, 4, 0, q0, n);
-    xgetrs(n, 4 * nb, f.G0f, f.ld, f.piv0, q0, 4 * M, st);
+    if (inv) {
+      xgemm(n, 4 * nb, n, one<Zt>(), inv->G0i, N, q0, 4 * M, zero<Zt>(), U4, 4 * M, st);
+      std::swap(q0, U4);
+    } else {
+      xgetrs(n, 4 * nb, f.G0f, f.ld, f.piv0, q0, 4 * M, st);
+    }
     xgemm(n, 4 * nb, n, one<Zt>(), f.G1, f.ld, q0, 4 * M, zero<Zt>(), T, 4 * M, st);
     hipLaunchKernelGGL(k_mrhs_q0<Zt>, grid1(N * M), dim3(256), 0, st, Bc, u3, T, f.beps, n, nb);
   }
-  (void)hipMemcpyAsync(Y, Bc, 3 * N * M * sizeof(Zt), hipMemcpyDeviceToDevice, st);
-  xgetrs(n, 3 * nb, f.Mf, f.ld, f.pivM, Y, 3 * M, st);
+  if (inv) {
+    xgemm(n, 3 * nb, n, one<Zt>(), inv->Mi, N, Bc, 3 * M, zero<Zt>(), Y, 3 * M, st);
+  } else {
+    (void)hipMemcpyAsync(Y, Bc, 3 * N * M * sizeof(Zt), hipMemcpyDeviceToDevice, st);
+    xgetrs(n, 3 * nb, f.Mf, f.ld, f.pivM, Y, 3 * M, st);
+  }
   hipLaunchKernelGGL(k_mndot<Zt>, grid1(N * M), dim3(256), 0, st, Y, f.col1, n, nb, v);
   xgemm(n, nb, n, -f.sigm, f.G1, f.ld, v, M, one<Zt>(), u3, M, st);
-  xgetrs(n, nb, f.Sf, f.ld, f.pivS, u3, M, st);
+  if (inv) {
+    xgemm(n, nb, n, one<Zt>(), inv->Si, N, u3, M, zero<Zt>(), v, M, st);
+    std::swap(u3, v);
+  } else {
+    xgetrs(n, nb, f.Sf, f.ld, f.pivS, u3, M, st);
+  }
   hipLaunchKernelGGL(k_mnscale<Zt>, grid1(N * M), dim3(256), 0, st, u3, f.col1, n, nb, Qn);
   xgemm(n, 3 * nb, n, -f.delt, f.G1, f.ld, Qn, 3 * M, one<Zt>(), Bc, 3 * M, st);
-  xgetrs(n, 3 * nb, f.Mf, f.ld, f.pivM, Bc, 3 * M, st);
+  if (inv) {
+    xgemm(n, 3 * nb, n, one<Zt>(), inv->Mi, N, Bc, 3 * M, zero<Zt>(), Y, 3 * M, st);
+    std::swap(Bc, Y);
+  } else {
+    xgetrs(n, 3 * nb, f.Mf, f.ld, f.pivM, Bc, 3 * M, st);
+  }
   hipLaunchKernelGGL(k_mpack_u4<Zt>, grid1(N * M), dim3(256), 0, st, Bc, u3, n, nb, U4);
   xgemm(n, 4 * nb, n, one<Zt>(), f.Zm, f.ld, U4, 4 * M, zero<Zt>(), Q4, 4 * M, st);
   hipLaunchKernelGGL(k_mfill_x<Zt>, grid1(N * M), dim3(256), 0, st, U4, Q4, r0nz ? (const Zt *)q0 : nullptr, n, nb, X);
   return 0;
 }
 
-int schur_apply_multi(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, int nb, hipStream_t st)
+int schur_apply_multi(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, int nb, hipStream_t st,
+                      const SchurInv<cf> *inv)
 {
-  return schur_apply_multi_t<cf>(f, rhs, r0nz, X, work, nb, st);
+  return schur_apply_multi_t<cf>(f, rhs, r0nz, X, work, nb, st, inv);
 }
-int schur_apply_multi(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, int nb, hipStream_t st)
+int schur_apply_multi(const SchurF<cd> &f, const cd *rhs, bool r0nz, cd *X, cd *work, int nb, hipStream_t st,
+                      const SchurInv<cd> *inv)
 {
-  return schur_apply_multi_t<cd>(f, rhs, r0nz, X, work, nb, st);
+  return schur_apply_multi_t<cd>(f, rhs, r0nz, X, work, nb, st, inv);
+}
+
+template <class Zt> __global__ void k_eye(Zt *__restrict__ A, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * n) return;
+  Zt r;
+  r.x = (i / n == i % n) ? 1 : 0;
+  r.y = 0;
+  A[i] = r;
+}
+
+int schur_inverses(const SchurF<cf> &f, cf *G0i, cf *Mi, cf *Si, hipStream_t st)
+{
+  if (!f.s_lu) return BH_ESTATE;
+  const size_t N = (size_t)f.n;
+  const cf *lu[3] = {f.G0f, f.Mf, f.Sf};
+  const int *pv[3] = {f.piv0, f.pivM, f.pivS};
+  cf *out[3] = {G0i, Mi, Si};
+  for (int q = 0; q < 3; q++) {
+    hipLaunchKernelGGL(k_eye<cf>, grid1(N * N), dim3(256), 0, st, out[q], N);
+    xgetrs(f.n, f.n, lu[q], f.ld, pv[q], out[q], N, st);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : BH_EHIP;
 }
 
 // R_b = B_b - A X_b over the rows of one domain for nb beams (8NN layout,
@@ -474,6 +522,45 @@ void bem_resid_domain_multi(const cd *G, const cd *H, int nd, size_t ld, const C
   bem_zgemm(nd, 4 * nb, nd, 1.0, 0.0, H, ld, a, 4 * M, 0.0, 0.0, Y, 4 * M, st);
   bem_zgemm(nd, 4 * nb, nd, 1.0, 0.0, G, ld, b, 4 * M, 1.0, 0.0, Y, 4 * M, st);
   hipLaunchKernelGGL(k_mresid, grid1(4 * N * M), dim3(256), 0, st, B, Y, nd, NN, rowBase, nb, R);
+}
+
+__global__ void k_mbcol(const ColInfo *__restrict__ col, int nd, int nb, const cd *__restrict__ Ui,
+                        const cd *__restrict__ dUi, size_t istride, cd *__restrict__ a, cd *__restrict__ c)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nd * nb) return;
+  const int j = (int)(i / nb), q = (int)(i % nb);
+  const ColInfo cc = col[j];
+  const size_t o = ((size_t)j * nb + q) * 4;
+  if (cc.main || cc.mdid != 0) {
+    for (int k = 0; k < 4; k++) { a[o + k] = mk(0, 0); c[o + k] = mk(0, 0); }
+    return;
+  }
+  const size_t e = (size_t)q * istride + ((size_t)cc.elem * 4 + cc.slot) * 4;
+  const cd *ui = Ui + e, *dui = dUi + e;
+  for (int k = 0; k < 4; k++) a[o + k] = ui[k];
+  for (int k = 0; k < 3; k++) c[o + k] = dui[k] + ui[3] * (cc.delt * cc.n[k]);
+  const cd nu = ui[0] * cc.n[0] + ui[1] * cc.n[1] + ui[2] * cc.n[2];
+  c[o + 3] = cc.sigm * nu + cc.beps * dui[3];
+}
+__global__ void k_mB(const cd *__restrict__ Y, int nd, size_t NN, size_t rowBase, int nb, cd *__restrict__ B)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)4 * nd * nb) return;
+  const int q = (int)(i / (4 * (size_t)nd)), rem = (int)(i % (4 * (size_t)nd)), cid = rem / nd, r = rem % nd;
+  const cd y = Y[((size_t)r * nb + q) * 4 + cid];
+  B[(size_t)q * 8 * NN + rowBase + (size_t)cid * nd + r] = mk(-y.x, -y.y);
+}
+
+void bem_build_B_multi(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *Ui, const cd *dUi,
+                       size_t istride, size_t NN, size_t rowBase, cd *B, cd *w, int nb, hipStream_t st)
+{
+  const size_t N = (size_t)nd, M = (size_t)nb;
+  cd *a = w, *c = w + 4 * N * M, *Y = w + 8 * N * M;
+  hipLaunchKernelGGL(k_mbcol, grid1(N * M), dim3(256), 0, st, col, nd, nb, Ui, dUi, istride, a, c);
+  bem_zgemm(nd, 4 * nb, nd, 1.0, 0.0, H, ld, a, 4 * M, 0.0, 0.0, Y, 4 * M, st);
+  bem_zgemm(nd, 4 * nb, nd, 1.0, 0.0, G, ld, c, 4 * M, 1.0, 0.0, Y, 4 * M, st);
+  hipLaunchKernelGGL(k_mB, grid1(4 * N * M), dim3(256), 0, st, Y, nd, NN, rowBase, nb, B);
 }
 
 // |x_b|^2 for nb vectors of n values (x_b at x + b n): out[b] (one
