@@ -63,6 +63,7 @@ def lib():
             "bh_mo_EH_bd": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_int, Ip, D, C.c_int, C.c_int, D]),
             "bh_prof_enable": (C.c_int, [C.c_int]),
             "bh_prof_read": (C.c_int, [C.c_int, D, D, Ip]),
+            "bh_prof_read_bytes": (C.c_int, [C.c_int, D, D, D, Ip]),
             "bh_stream": (VP, []),
             "bh_object_solve_stats": (C.c_int, [VP, Ip, D, Ip]),
             "bh_object_set_precision": (C.c_int, [VP, C.c_int]),
@@ -76,6 +77,8 @@ def lib():
             "bh_mo_solve_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, C.c_double, C.c_int, C.c_int, Ip]),
             "bh_object_beam_state": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, D]),
             "bh_object_select_beam": (C.c_int, [VP, C.c_int]),
+            "bh_mo_set_lanes": (C.c_int, [C.c_int]),
+            "bh_elem_coef": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, Ip, Ip, D, D, D, D, D]),
             "bh_mo_force_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, D, D, Ip]),
             "bh_mo_far_field_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D, C.c_int, D]),
         }
@@ -158,6 +161,13 @@ def prof_read(name):
     ms, work, n = np.zeros(1), np.zeros(1), np.zeros(1, np.int32)
     check(lib().bh_prof_read(KERNELS[name], dp(ms), dp(work), ip(n)))
     return float(ms[0]), float(work[0]), int(n[0])
+
+
+def prof_read_bytes(name):
+    """(kernel ms, algorithmic work, algorithmic bytes, launches) summed"""
+    ms, work, by, n = np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1, np.int32)
+    check(lib().bh_prof_read_bytes(KERNELS[name], dp(ms), dp(work), dp(by), ip(n)))
+    return float(ms[0]), float(work[0]), float(by[0]), int(n[0])
 
 
 class Object:
@@ -292,6 +302,25 @@ class Object:
         """beams of the last mo_solve_beams (0: none)"""
         return getattr(self, "_nbm", 0)
 
+    def elem_coef(self, nd, type_, k, s, t=None, ze=None, rt=None, v0=None, v1=None):
+        """element coefficients (bh_elem_coef): n requests of signed source
+        elements s; t == s -> the self element at local coordinates ze,
+        otherwise the points rt; nd derivative directions (v0, v1).  Rows of
+        27 complex values: CC, dC/dv0, dC/dv1 (G 0-3, H 4-7, F 8)"""
+        s = np.ascontiguousarray(s, dtype=np.int32).reshape(-1)
+        n = s.size
+
+        def arr(a, w, dt=np.float64):
+            return None if a is None else np.ascontiguousarray(a, dtype=dt).reshape(n, w)
+        t, ze, rt, v0, v1 = arr(t, 1, np.int32), arr(ze, 2), arr(rt, 3), arr(v0, 3), arr(v1, 3)
+        out = np.zeros((n, 27), np.complex128)
+        k = complex(k)
+        check(lib().bh_elem_coef(self.h, n, nd, type_, k.real, k.imag, ip(s), None if t is None else ip(t),
+                                 None if ze is None else dp(ze), None if rt is None else dp(rt),
+                                 None if v0 is None else dp(v0), None if v1 is None else dp(v1),
+                                 dp(out.view(np.float64))))
+        return out
+
     def select_beam(self, b):
         """make beam b of mo_solve_beams the object's one-beam state (the
         field, force and power functions then report beam b)"""
@@ -399,6 +428,11 @@ def mo_solve_beams(objs, beams, ccd=1e-6, iter_min=10, iter_max=100):
     for o in objs:
         o._nbm = par.shape[0]
     return int(it[0])
+
+
+def set_lanes(n=0):
+    """device streams of the per-object work (0: default)"""
+    check(lib().bh_mo_set_lanes(n))
 
 
 def select_beam(objs, b):

@@ -709,6 +709,12 @@ void bem_launch_field_rows(const double *, const int *, const double *, int, int
                            const double *, const double *, const double *, const double *, size_t, void *, double *,
                            hipStream_t);
 void bem_launch_pack_ehbv(const void *, const void *, const void *, const void *, size_t, int, int, void *, hipStream_t);
+void bem_launch_elem_coef(int, const void *, const void *, int, const int *, const double *, const double *,
+                          const double *, const int *, double, double, const void *, int, double, const void *, int,
+                          const void *, int, double, const void *, int, int, void *, hipStream_t);
+void bem_launch_elem_self(const void *, const void *, const int *, const double *, const int *, int, double, double,
+                          const double *, const double *, const double *, const double *, void *, double *, void *,
+                          hipStream_t);
 void bem_launch_rows_add(const void *, int, int, const double *, int, const int *, size_t, void *, hipStream_t);
 void bem_launch_meh_gather(int, int, const void *, const int *, const void *, const void *, size_t, void *, void *,
                            hipStream_t);
@@ -2120,6 +2126,74 @@ extern "C" int bh_mo_force_beams(bh_object **objs, int N, int type, const double
                             flag ? flag + m : nullptr);
     if (r) return r;
   }
+  return 0;
+}
+
+// element coefficients of the reference's element layer (coef_rt,
+// coef_bd, coef_bd_node, dcoef_rt, dcoef_bd_tz/te, dcoef_bd_t2,
+// dcoef_bd_node_t2; d3b1_elem.c:148-539) for n requests on object o's
+// elements: nd 0 CC only, 1 with dC along v0, 2 along v0 and v1; rule type
+// 0..3; k = kr + i ki.  Request i: signed source element s[i]; t (may be
+// null) the signed target element: t[i] == s[i] takes the self element at
+// local coordinates ze[2i], ze[2i+1] (polar rules, principal-value
+// derivatives along the element's own tangents), otherwise the point
+// rt[3i].  out: 27 complex per request (CC, dC0, dC1; 9 each: G 0-3, H
+// 4-7, F 8).
+extern "C" int bh_elem_coef(bh_object *o, int n, int nd, int type, double kr, double ki, const int *s, const int *t,
+                            const double *ze, const double *rt, const double *v0, const double *v1, double *out)
+{
+  API_LOCK;
+  if (n < 0 || nd < 0 || nd > 2 || type < 0 || type > 3) return fail(BH_EARG, "bad element-coefficient request");
+  if (n == 0) return 0;
+  if (ensure_rules()) return BH_EHIP;
+  std::vector<int> so, io, ss, is;
+  std::vector<double> ro, a0, a1, zs;
+  for (int i = 0; i < n; i++) {
+    const int as = std::abs(s[i]);
+    if (as < 1 || as > o->Ne) return fail(BH_EARG, "element %d of %d", s[i], o->Ne);
+    if (t && t[i] == s[i]) {
+      ss.push_back(s[i]);
+      is.push_back(i);
+      zs.push_back(ze[2 * i]);
+      zs.push_back(ze[2 * i + 1]);
+      continue;
+    }
+    so.push_back(s[i]);
+    io.push_back(i);
+    for (int c = 0; c < 3; c++) {
+      ro.push_back(rt[3 * i + c]);
+      a0.push_back(nd > 0 ? v0[3 * i + c] : 0.0);
+      a1.push_back(nd > 1 ? v1[3 * i + c] : 0.0);
+    }
+  }
+  cd *d_out = cws<cd>(17, 27 * (size_t)n);
+  CWS_CHECK(d_out);
+  const int no = (int)so.size(), ns = (int)ss.size();
+  if (no > 0) {
+    int *d_s, *d_i;
+    double *d_r, *d_a0, *d_a1;
+    if (cput(0, &d_s, so.data(), no) || cput(1, &d_i, io.data(), no) || cput(2, &d_r, ro.data(), ro.size()) ||
+        cput(3, &d_a0, a0.data(), a0.size()) || cput(4, &d_a1, a1.data(), a1.size()))
+      return BH_EHIP;
+    bem_launch_elem_coef(nd, o->d_geo, o->d_gq, no, d_s, d_r, d_a0, d_a1, d_i, kr, ki, g_rule[type], g_nrule[type],
+                         g_rscale[type], g_qrule[type], g_nqrule[type], g_rule[3], g_nrule[3], g_rscale[3], g_qrule[3],
+                         g_nqrule[3], type < 3, d_out, cur_st());
+  }
+  if (ns > 0) {
+    int *d_s, *d_i;
+    double *d_z, *d_tf;
+    cd *d_t;
+    if (cput(5, &d_s, ss.data(), ns) || cput(6, &d_i, is.data(), ns) || cput(7, &d_z, zs.data(), zs.size()))
+      return BH_EHIP;
+    d_t = cws<cd>(8, 24 * (size_t)ns);
+    d_tf = cws<double>(9, 3 * (size_t)ns);
+    CWS_CHECK(d_t && d_tf);
+    bem_launch_elem_self(o->d_geo, o->d_gq, d_s, d_z, d_i, ns, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh, d_t, d_tf, d_out,
+                         cur_st());
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, d_out, 27 * (size_t)n * sizeof(cd), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
   return 0;
 }
 

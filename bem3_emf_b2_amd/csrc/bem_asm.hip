@@ -529,12 +529,11 @@ __device__ __forceinline__ QPol qpol(const TriGeo &g, const QuadExt &qx, const d
 // 1 is t_eta (unnormalised tangents at the node; results divided by their
 // lengths).  H, F and their derivatives carry the factor c1 . W2 and vanish
 // on a plane quadrangle (bil_check_plane).
-__device__ void asm_self_quad(const TriGeo &g, const QuadExt &qx, const RowPt &rp, int td, int row, size_t col,
+__device__ void asm_self_quad(const TriGeo &g, const QuadExt &qx, double zt, double et, int td, int row, size_t col,
                               double kr, double ki, const double *__restrict__ xl, const double *__restrict__ wl,
                               const double *__restrict__ xh, const double *__restrict__ wh, int lane, cd *G, cd *H,
                               cd *dG, cd *dH, size_t ld, double *selfDF)
 {
-  const double zt = c_quad_zt[rp.slot], et = c_quad_et[rp.slot];
   double zp = 1.0 - zt, zm = -1.0 - zt, ep = 1.0 - et, em = -1.0 - et;
   double r[5] = {sqrt(zm * zm + em * em), sqrt(zp * zp + em * em), sqrt(zp * zp + ep * ep), sqrt(zm * zm + ep * ep), 0.0};
   double th[5] = {atan2(em, zm), atan2(em, zp), atan2(ep, zp), atan2(ep, zm), 0.0};
@@ -686,29 +685,17 @@ __device__ void asm_self_quad(const TriGeo &g, const QuadExt &qx, const RowPt &r
   }
 }
 
-__global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
-                                                  const int *__restrict__ sid, const RowPt *__restrict__ rows,
-                                                  const int *__restrict__ colOff, int nrows, double kr, double ki,
-                                                  const double *__restrict__ xl, const double *__restrict__ wl,
-                                                  const double *__restrict__ xh, const double *__restrict__ wh,
-                                                  cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
-                                                  cd *__restrict__ dH, size_t ld, double *__restrict__ selfDF)
+// Self triangle at (zt, et), one wavefront: G by the polar rule of
+// lit_coef_bd (lit_coef.c:290-348) and the principal-value tangential
+// derivatives dG/dt_zeta, dG/dt_eta of lit_dcoef_bd_tz0_PV / te0_PV
+// (lit_dcoef.c:460-635): the complement of the disk D_eps by the polar
+// rule, the disk by its closed-form radial factor.  H, F and their
+// derivatives vanish on a flat triangle.  Every lane returns the wave's
+// scaled sums.
+__device__ void self_tri(const TriGeo &g, double zt, double et, double kr, double ki, const double *__restrict__ xl,
+                         const double *__restrict__ wl, const double *__restrict__ xh, const double *__restrict__ wh,
+                         int lane, cd *Gs, cd *D0s, cd *D1s)
 {
-  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (row >= nrows) return;
-  const RowPt rp = rows[row];
-  int td = sid[rp.t];
-  int at = td < 0 ? -td : td;
-  const TriGeo g = geo[at];
-  if (g.quad != 0.0) {
-    asm_self_quad(g, gq[at], rp, td, row, colOff[rp.t], kr, ki, xl, wl, xh, wh, lane, G, H, dG, dH, ld, selfDF);
-    return;
-  }
-  if (lane == 0) {   // a flat triangle has no self F terms
-    selfDF[(size_t)row * 3 + 0] = 0.0; selfDF[(size_t)row * 3 + 1] = 0.0; selfDF[(size_t)row * 3 + 2] = 0.0;
-  }
-  const double zt = c_tri_zt[rp.slot], et = c_tri_et[rp.slot];
   // polar set-up about (zt, et): vertices (-1/2,-sqrt3/2), (1,0), (-1/2,sqrt3/2)
   double zp = 1.0 - zt, zm = -0.5 - zt, ep = 0.5 * BEM_SQ3 - et, em = -0.5 * BEM_SQ3 - et;
   double r[4] = {sqrt(zm * zm + em * em), sqrt(zp * zp + et * et), sqrt(zm * zm + ep * ep), 0.0};
@@ -789,7 +776,6 @@ __global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo
     }
   }
   const double sG = BEM_I_SPSQ3 * g.aW;
-  size_t col = colOff[rp.t];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     cd ga = wsum(Ga[a]);
@@ -798,10 +784,44 @@ __global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo
     // + i tde (pi/4) / k - tdp (pi/4) / k
     d0 = d0 + cdiv(mk(0, 1) * e0 * (0.25 * M_PI) - p0 * (0.25 * M_PI), kk);
     d1 = d1 + cdiv(mk(0, 1) * e1 * (0.25 * M_PI) - p1 * (0.25 * M_PI), kk);
-    if (lane == 0) {
-      G[(size_t)row * ld + col + a] = ga * sG;
-      dG[((size_t)row * 2 + 0) * ld + col + a] = d0 * (-sG * iaT0);
-      dG[((size_t)row * 2 + 1) * ld + col + a] = d1 * (-sG * iaT1);
+    Gs[a] = ga * sG;
+    D0s[a] = d0 * (-sG * iaT0);
+    D1s[a] = d1 * (-sG * iaT1);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                  const int *__restrict__ sid, const RowPt *__restrict__ rows,
+                                                  const int *__restrict__ colOff, int nrows, double kr, double ki,
+                                                  const double *__restrict__ xl, const double *__restrict__ wl,
+                                                  const double *__restrict__ xh, const double *__restrict__ wh,
+                                                  cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
+                                                  cd *__restrict__ dH, size_t ld, double *__restrict__ selfDF)
+{
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const RowPt rp = rows[row];
+  int td = sid[rp.t];
+  int at = td < 0 ? -td : td;
+  const TriGeo g = geo[at];
+  if (g.quad != 0.0) {
+    asm_self_quad(g, gq[at], c_quad_zt[rp.slot], c_quad_et[rp.slot], td, row, colOff[rp.t], kr, ki, xl, wl, xh, wh,
+                  lane, G, H, dG, dH, ld, selfDF);
+    return;
+  }
+  if (lane == 0) {   // a flat triangle has no self F terms
+    selfDF[(size_t)row * 3 + 0] = 0.0; selfDF[(size_t)row * 3 + 1] = 0.0; selfDF[(size_t)row * 3 + 2] = 0.0;
+  }
+  cd Gs[3], D0s[3], D1s[3];
+  self_tri(g, c_tri_zt[rp.slot], c_tri_et[rp.slot], kr, ki, xl, wl, xh, wh, lane, Gs, D0s, D1s);
+  size_t col = colOff[rp.t];
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      G[(size_t)row * ld + col + a] = Gs[a];
+      dG[((size_t)row * 2 + 0) * ld + col + a] = D0s[a];
+      dG[((size_t)row * 2 + 1) * ld + col + a] = D1s[a];
       H[(size_t)row * ld + col + a] = mk(0, 0);
       dH[((size_t)row * 2 + 0) * ld + col + a] = mk(0, 0);
       dH[((size_t)row * 2 + 1) * ld + col + a] = mk(0, 0);
@@ -834,6 +854,58 @@ __global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPa
     cd *h = H + (size_t)row * ld + row;
     h->x += diag;
     dF[row * 3 + 0] = diag; dF[row * 3 + 1] = f0; dF[row * 3 + 2] = f1;
+  }
+}
+
+// ---- element layer: the self element at a point of its own ----
+// Request i: signed element s[i] at local coordinates (ze[2i], ze[2i+1]);
+// out[27 idx[i] + ...]: CC (G 0-3, H 4-7, F 8), dC/dt_zeta (9-17) and
+// dC/dt_eta (18-26) of lit_coef_bd + lit_dcoef_bd_vt2_PV, or bil_coef_bd +
+// bil_dcoef_bd_vt2_PV for a quadrangle (d3b1_elem.c:187-230, 400-480).  tmp
+// holds 24 complex values and tmpF 3 doubles per request (the quadrangle
+// routine's outputs).
+__global__ void __launch_bounds__(256) k_elem_self(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
+                                                   const int *__restrict__ s, const double *__restrict__ ze,
+                                                   const int *__restrict__ idx, int n, double kr, double ki,
+                                                   const double *__restrict__ xl, const double *__restrict__ wl,
+                                                   const double *__restrict__ xh, const double *__restrict__ wh,
+                                                   cd *__restrict__ tmp, double *__restrict__ tmpF, cd *__restrict__ out)
+{
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int sd = s[i], as = sd < 0 ? -sd : sd;
+  const TriGeo g = geo[as];
+  const double zt = ze[2 * i], et = ze[2 * i + 1];
+  cd *o = out + 27 * (size_t)idx[i];
+  if (g.quad != 0.0) {
+    cd *T = tmp + 24 * (size_t)i;   // G 0-3, H 4-7, dG 8-15, dH 16-23 (t_zeta then t_eta)
+    double *Fq = tmpF + 3 * (size_t)i;
+    asm_self_quad(g, gq[as], zt, et, sd, 0, 0, kr, ki, xl, wl, xh, wh, lane, T, T + 4, T + 8, T + 16, 4, Fq);
+    if (lane == 0) {
+      for (int a = 0; a < 4; a++) {
+        o[a] = T[a];
+        o[4 + a] = T[4 + a];
+        o[9 + a] = T[8 + a];
+        o[13 + a] = T[16 + a];
+        o[18 + a] = T[12 + a];
+        o[22 + a] = T[20 + a];
+      }
+      o[8] = mk(Fq[0], 0.0);
+      o[17] = mk(Fq[1], 0.0);
+      o[26] = mk(Fq[2], 0.0);
+    }
+    return;
+  }
+  cd Gs[3], D0s[3], D1s[3];
+  self_tri(g, zt, et, kr, ki, xl, wl, xh, wh, lane, Gs, D0s, D1s);
+  if (lane == 0) {
+    for (int j = 0; j < 27; j++) o[j] = mk(0, 0);
+    for (int a = 0; a < 3; a++) {
+      o[a] = Gs[a];
+      o[9 + a] = D0s[a];
+      o[18 + a] = D1s[a];
+    }
   }
 }
 
@@ -920,6 +992,14 @@ void bem_launch_asm_self(const void *geo, const void *gq, const int *sid, const 
   bem_prof_end(BH_K_ASM_SELF, st);
 }
 
+void bem_launch_elem_self(const void *geo, const void *gq, const int *s, const double *ze, const int *idx, int n,
+                          double kr, double ki, const double *xl, const double *wl, const double *xh, const double *wh,
+                          void *tmp, double *tmpF, void *out, hipStream_t st)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_elem_self, dim3((n + 3) / 4), dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, s, ze,
+                     idx, n, kr, ki, xl, wl, xh, wh, (cd *)tmp, tmpF, (cd *)out);
+}
 void bem_launch_asm_fin(const double *dfPart, int nwv, const double *ghDF, const int *ghStart, const double *selfDF,
                         int nrows, int open_domain, void *H, size_t ld, double *dF, hipStream_t st)
 {

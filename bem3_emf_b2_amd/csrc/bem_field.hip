@@ -990,6 +990,100 @@ __global__ void __launch_bounds__(256) k_field_bd(const int *__restrict__ tgt_t,
   }
 }
 
+// ---------------------------------------------------------------- element layer
+// Coefficients of one source element at a point off the element, one
+// wavefront per request, lanes over the rule's points: CC (G 0-3, H 4-7,
+// F 8) of coef_rt / coef_bd / coef_bd_node (t != s), and with nd = 1 or 2
+// directions the derivatives dC along v0 (9-17) and v1 (18-26) of dcoef_rt /
+// dcoef_bd_tz / dcoef_bd_te / dcoef_bd_t2 / dcoef_bd_node_t2
+// (d3b1_elem.c:148-539): rule type 0..3 of the element's kind, and for the
+// derivatives with type < 3 the 64x64 rule again when |dF| along a
+// direction exceeds 0.5 (the reference's GHN switch).  out[27 idx[i] + ...].
+template <int ND>
+__device__ __forceinline__ void elem_sums(const TriGeo &g, const QuadExt &qx, const double *rt, const double (*v)[3],
+                                          double kr, double ki, const QPtN *__restrict__ rule, int nq, double scale,
+                                          const QPtQ *__restrict__ qrule, int nqq, bool flip, int lane, cd *G, cd *H,
+                                          double &F, cd (*dG)[4], cd (*dH)[4], double *dF)
+{
+  const bool quad = g.quad != 0.0;
+  const int n = quad ? nqq : nq, per = (n + 63) / 64, p0 = min(n, lane * per), cnt = min(n, p0 + per) - p0;
+  for (int a = 0; a < 4; a++) {
+    G[a] = H[a] = mk(0, 0);
+    for (int d = 0; d < 2; d++) dG[d][a] = dH[d][a] = mk(0, 0);
+  }
+  F = dF[0] = dF[1] = 0.0;
+  if (quad) {
+    if constexpr (ND == 0) quad_coef(g, qx, rt, kr, ki, qrule + p0, cnt, flip, G, H, F);
+    else quad_coef_dir<ND>(g, qx, rt, v, kr, ki, qrule + p0, cnt, flip, G, H, F, dG, dH, dF);
+  } else {
+    cd x0[3], x1[3];
+    double x2;
+    if (ND == 0) tri_coef<false>(g, rt, nullptr, kr, ki, rule + p0, cnt, scale, flip, G, H, F, x0, x1, x2);
+    for (int d = 0; d < ND; d++) {
+      cd tg[3], th[3], dg[3], dh[3];
+      double tf, df;
+      tri_coef<true>(g, rt, v[d], kr, ki, rule + p0, cnt, scale, flip, tg, th, tf, dg, dh, df);
+      if (d == 0) {
+        for (int a = 0; a < 3; a++) { G[a] = tg[a]; H[a] = th[a]; }
+        F = tf;
+      }
+      for (int a = 0; a < 3; a++) { dG[d][a] = dg[a]; dH[d][a] = dh[a]; }
+      dF[d] = df;
+    }
+  }
+  for (int a = 0; a < 4; a++) {
+    G[a] = wsum(G[a]);
+    H[a] = wsum(H[a]);
+    for (int d = 0; d < ND; d++) { dG[d][a] = wsum(dG[d][a]); dH[d][a] = wsum(dH[d][a]); }
+  }
+  F = wsum(F);
+  for (int d = 0; d < ND; d++) dF[d] = wsum(dF[d]);
+}
+
+template <int ND>
+__global__ void __launch_bounds__(256) k_elem_coef(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq, int n,
+                                                   const int *__restrict__ s, const double *__restrict__ rt,
+                                                   const double *__restrict__ v0, const double *__restrict__ v1,
+                                                   const int *__restrict__ idx, double kr, double ki,
+                                                   const QPtN *__restrict__ rule, int nq, double scale,
+                                                   const QPtQ *__restrict__ qrule, int nqq, const QPtN *__restrict__ rgh,
+                                                   int ngh, double sgh, const QPtQ *__restrict__ qgh, int nqgh,
+                                                   int fallback, cd *__restrict__ out)
+{
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int sd = s[i], as = sd < 0 ? -sd : sd;
+  const TriGeo g = geo[as];
+  const QuadExt qx = g.quad != 0.0 ? gq[as] : QuadExt{};
+  const double r[3] = {rt[3 * i], rt[3 * i + 1], rt[3 * i + 2]};
+  double v[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  for (int c = 0; c < 3; c++) {
+    if (ND > 0) v[0][c] = v0[3 * i + c];
+    if (ND > 1) v[1][c] = v1[3 * i + c];
+  }
+  cd G[4], H[4], dG[2][4], dH[2][4];
+  double F, dF[2];
+  elem_sums<ND>(g, qx, r, v, kr, ki, rule, nq, scale, qrule, nqq, sd < 0, lane, G, H, F, dG, dH, dF);
+  if (ND > 0 && fallback) {
+    bool redo = false;
+    for (int d = 0; d < ND; d++) redo = redo || fabs(dF[d]) > BEM_CBD_SDF;
+    if (redo) elem_sums<ND>(g, qx, r, v, kr, ki, rgh, ngh, sgh, qgh, nqgh, sd < 0, lane, G, H, F, dG, dH, dF);
+  }
+  if (lane == 0) {
+    cd *o = out + 27 * (size_t)idx[i];
+    for (int a = 0; a < 4; a++) { o[a] = G[a]; o[4 + a] = H[a]; }
+    o[8] = mk(F, 0.0);
+    for (int d = 0; d < 2; d++) {
+      for (int a = 0; a < 4; a++) {
+        o[9 + 9 * d + a] = d < ND ? dG[d][a] : mk(0, 0);
+        o[13 + 9 * d + a] = d < ND ? dH[d][a] : mk(0, 0);
+      }
+      o[17 + 9 * d] = mk(d < ND ? dF[d] : 0.0, 0.0);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- field rows
 // Rows of the E, H boundary integral of one object's domain did at np
 // points, for fields of many right-hand sides as one product: the element
@@ -1602,5 +1696,25 @@ void bem_launch_rows_add(const void *C, int n, int nb, const double *F, int open
   const size_t m = (size_t)n * nb;
   hipLaunchKernelGGL(k_rows_add, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, (const cd *)C, n, nb, F, open,
                      idx, ldb, (cd *)EH);
+}
+void bem_launch_elem_coef(int nd, const void *geo, const void *gq, int n, const int *s, const double *rt,
+                          const double *v0, const double *v1, const int *idx, double kr, double ki, const void *rule, int nq,
+                          double scale, const void *qrule, int nqq, const void *rgh, int ngh, double sgh, const void *qgh,
+                          int nqgh, int fallback, void *out, hipStream_t st)
+{
+  if (n <= 0) return;
+  const dim3 grid((n + 3) / 4), blk(256);
+  if (nd == 0)
+    hipLaunchKernelGGL(k_elem_coef<0>, grid, blk, 0, st, (const TriGeo *)geo, (const QuadExt *)gq, n, s, rt, v0, v1, idx,
+                       kr, ki, (const QPtN *)rule, nq, scale, (const QPtQ *)qrule, nqq, (const QPtN *)rgh, ngh, sgh,
+                       (const QPtQ *)qgh, nqgh, fallback, (cd *)out);
+  else if (nd == 1)
+    hipLaunchKernelGGL(k_elem_coef<1>, grid, blk, 0, st, (const TriGeo *)geo, (const QuadExt *)gq, n, s, rt, v0, v1, idx,
+                       kr, ki, (const QPtN *)rule, nq, scale, (const QPtQ *)qrule, nqq, (const QPtN *)rgh, ngh, sgh,
+                       (const QPtQ *)qgh, nqgh, fallback, (cd *)out);
+  else
+    hipLaunchKernelGGL(k_elem_coef<2>, grid, blk, 0, st, (const TriGeo *)geo, (const QuadExt *)gq, n, s, rt, v0, v1, idx,
+                       kr, ki, (const QPtN *)rule, nq, scale, (const QPtQ *)qrule, nqq, (const QPtN *)rgh, ngh, sgh,
+                       (const QPtQ *)qgh, nqgh, fallback, (cd *)out);
 }
 }  // extern "C"

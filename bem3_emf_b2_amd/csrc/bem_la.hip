@@ -1144,7 +1144,9 @@ static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename 
   const bool skinny = n <= 8 && k >= 256;
   const bool big = g_big_tiles && !skinny && ((T >= 160 && k >= 32) || (T >= 64 && k >= 1024));
   const int kid = big ? Cx<R>::KG : skinny ? Cx<R>::KGV : Cx<R>::KGS;
-  bem_prof_begin(kid, 8.0 * m * n * k, st);
+  const bool bz = beta.x == 0 && beta.y == 0;
+  bem_prof_begin_b(kid, 8.0 * m * n * k,
+                   ((double)m * k + (double)k * n + (double)m * n * (bz ? 1.0 : 2.0)) * sizeof(typename Cx<R>::Z), st);
   if (skinny) {
     // a few right-hand sides against a long k: stream A once
     dim3 grid((m + 15) / 16);

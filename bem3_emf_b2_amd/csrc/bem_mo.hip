@@ -867,16 +867,21 @@ int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double
 // library's stream (default 2).
 #define MO_LMAX 8
 hipStream_t g_lane_st[MO_LMAX];
-int g_nlanes = 0;
+int g_nlanes = 0;     // streams created so far
+int g_lanes_req = 0;  // bh_mo_set_lanes (0: BEM3_HIP_LANES, else 2)
 
 int lanes()
 {
-  if (g_nlanes) return g_nlanes;
-  int n = 2;   // measured on the 16-object scene: 1 lane 10.8 s, 2 9.1 s, 4 9.4 s, 6 9.9 s
-  const char *e = getenv("BEM3_HIP_LANES");
-  if (e && *e) n = std::max(1, std::min(MO_LMAX, atoi(e)));
-  for (int k = 0; k < n; k++) HIPCHK(hipStreamCreate(&g_lane_st[k]));
-  g_nlanes = n;
+  int n = g_lanes_req;
+  if (n <= 0) {
+    n = 2;   // measured on the 16-object scene: 1 lane 10.8 s, 2 9.1 s, 4 9.4 s, 6 9.9 s
+    const char *e = getenv("BEM3_HIP_LANES");
+    if (e && *e) n = std::max(1, std::min(MO_LMAX, atoi(e)));
+  }
+  while (g_nlanes < n) {
+    HIPCHK(hipStreamCreate(&g_lane_st[g_nlanes]));
+    g_nlanes++;
+  }
   return n;
 }
 
@@ -950,6 +955,17 @@ extern "C" int bh_mo_prepare(bh_object **objs, int N, int type, int mode)
 
 // 0: series where a pair allows it (the mixed-precision form), 1: the
 // direct element sums for every pair
+// device streams of the per-object work (1..8; 0: the default, BEM3_HIP_LANES
+// or 2).  One lane runs every kernel in order on the library's stream, so
+// that per-kernel timings bracket one kernel each.
+extern "C" int bh_mo_set_lanes(int n)
+{
+  API_LOCK;
+  if (n < 0 || n > MO_LMAX) return fail(BH_EARG, "lanes %d (0..%d)", n, MO_LMAX);
+  g_lanes_req = n;
+  return 0;
+}
+
 extern "C" int bh_mo_set_coupling(int mode)
 {
   API_LOCK;

@@ -11,7 +11,7 @@
 namespace {
 struct Rec {
   int kid;
-  double work;
+  double work, bytes;
   hipEvent_t a, b;
   bool closed;
 };
@@ -51,16 +51,20 @@ void bem_prof_release(void)
   P.on = false;
 }
 
-void bem_prof_begin(int kid, double work, hipStream_t st)
+// bytes: the launch's algorithmic HBM traffic (operands read once, results
+// written once), where the caller states it (0 otherwise)
+void bem_prof_begin_b(int kid, double work, double bytes, hipStream_t st)
 {
   if (!P.on || kid < 0 || kid >= BH_K_COUNT) return;
   std::lock_guard<std::mutex> lk(M);
-  Rec r{kid, work, take(), take(), false};
+  Rec r{kid, work, bytes, take(), take(), false};
   if (!r.a || !r.b) return;
   (void)hipEventRecord(r.a, st);
   t_open.push_back(P.recs.size());
   P.recs.push_back(r);
 }
+
+void bem_prof_begin(int kid, double work, hipStream_t st) { bem_prof_begin_b(kid, work, 0.0, st); }
 
 void bem_prof_end(int kid, hipStream_t st)
 {
@@ -84,11 +88,11 @@ int bh_prof_enable(int on)
   return 0;
 }
 
-int bh_prof_read(int kid, double *ms, double *work, int *launches)
+int bh_prof_read_bytes(int kid, double *ms, double *work, double *bytes, int *launches)
 {
   if (kid < 0 || kid >= BH_K_COUNT) return BH_EARG;
   std::lock_guard<std::mutex> lk(M);
-  double t = 0.0, w = 0.0;
+  double t = 0.0, w = 0.0, by = 0.0;
   int n = 0;
   for (const Rec &r : P.recs) {
     if (r.kid != kid || !r.closed) continue;
@@ -97,11 +101,18 @@ int bh_prof_read(int kid, double *ms, double *work, int *launches)
     if (hipEventElapsedTime(&e, r.a, r.b) != hipSuccess) return BH_EHIP;
     t += e;
     w += r.work;
+    by += r.bytes;
     n++;
   }
   if (ms) *ms = t;
   if (work) *work = w;
+  if (bytes) *bytes = by;
   if (launches) *launches = n;
   return 0;
+}
+
+int bh_prof_read(int kid, double *ms, double *work, int *launches)
+{
+  return bh_prof_read_bytes(kid, ms, work, nullptr, launches);
 }
 }

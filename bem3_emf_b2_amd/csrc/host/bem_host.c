@@ -1190,3 +1190,166 @@ int mo_absorb_P(double *P, int oid, int type, MOBJ *mo)
   if (r != 0 && r != -1 && r != -2) bh_or_die(r, "bh_mo_absorb");
   return r;
 }
+
+/* ------------------------------------------------------------ element layer
+ * The public element functions of d3b1_elem.h:101-124.  The geometry
+ * functions read the element constants the host keeps (d3b1_elem.c:10-146);
+ * the coefficient dispatchers (d3b1_elem.c:148-539) run on the device
+ * through bh_elem_coef, one request per call (bh_elem_coef itself takes
+ * any number).  Rule type 4 (the double-exponential rules) is not
+ * provided. */
+int check_element_type(int seid, BOUD *bd) { return bd->ed[abs(seid)][3] == 0 ? ELT3 : ELT4; }
+
+void r_bd(double *r, int seid, double zeta, double eta, BOUD *bd)
+{
+  const int s = abs(seid);
+  const double q = check_element_type(seid, bd) == ELT4 ? zeta * eta : 0.0;
+  for (int i = 0; i < 3; i++) r[i] = bd->cr[s][i][0] + bd->cr[s][i][1] * zeta + bd->cr[s][i][2] * eta + bd->cr[s][i][3] * q;
+}
+
+void r_bd_node(double *r, int seid, int nid, BOUD *bd)
+{
+  for (int i = 0; i < 3; i++) r[i] = bd->ren[abs(seid)][nid][i];
+}
+
+void n_bd_node(double *n, int seid, int nid, BOUD *bd)
+{
+  for (int i = 0; i < 3; i++) n[i] = bd->wen[abs(seid)][nid][i];
+  vunit3(n);
+}
+
+void Tz_bd(double *vTz, int seid, double zeta, double eta, BOUD *bd)
+{
+  const int s = abs(seid), q = check_element_type(seid, bd) == ELT4;
+  (void)zeta;
+  for (int i = 0; i < 3; i++) vTz[i] = bd->cr[s][i][1] + (q ? bd->cr[s][i][3] * eta : 0.0);
+}
+
+void Te_bd(double *vTe, int seid, double zeta, double eta, BOUD *bd)
+{
+  const int s = abs(seid), q = check_element_type(seid, bd) == ELT4;
+  (void)eta;
+  for (int i = 0; i < 3; i++) vTe[i] = bd->cr[s][i][2] + (q ? bd->cr[s][i][3] * zeta : 0.0);
+}
+
+void r_tz_bd(double *r, double *vtz, int seid, double zeta, double eta, BOUD *bd)
+{
+  r_bd(r, seid, zeta, eta, bd);
+  Tz_bd(vtz, seid, zeta, eta, bd);
+  vunit3(vtz);
+}
+
+void r_te_bd(double *r, double *vte, int seid, double zeta, double eta, BOUD *bd)
+{
+  r_bd(r, seid, zeta, eta, bd);
+  Te_bd(vte, seid, zeta, eta, bd);
+  vunit3(vte);
+}
+
+void tz_te_bd_node(double *vtz, double *vte, int seid, int nid, BOUD *bd)
+{
+  const int q = check_element_type(seid, bd) == ELT4;
+  Tz_bd(vtz, seid, q ? bd->zt_44[nid] : 0.0, q ? bd->et_44[nid] : 0.0, bd);
+  Te_bd(vte, seid, q ? bd->zt_44[nid] : 0.0, q ? bd->et_44[nid] : 0.0, bd);
+  vunit3(vtz);
+  vunit3(vte);
+}
+
+/* one request of bh_elem_coef on the object whose boundary data bd is
+ * (BOUD sits inside its DOMD); o27: CC, dC/dv0, dC/dv1 */
+static void elem_request(BOUD *bd, const char *fn, int nd, int type, double complex k, int s, int t, double zt,
+                         double et, const double *rt, const double *v0, const double *v1, double complex *o27)
+{
+  if (type < 0 || type > 3) {
+    printf("b_elem.c, %s(), type number error! type=%d is not defined.Exit...\n", fn, type);
+    exit(1);
+  }
+  DOMD *md = (DOMD *)((char *)bd - offsetof(DOMD, bd));
+  const double zero[3] = {0.0, 0.0, 0.0}, ze[2] = {zt, et};
+  bh_or_die(bh_elem_coef(dev(md), 1, nd, type, creal(k), cimag(k), &s, &t, ze, rt ? rt : zero, v0 ? v0 : zero,
+                         v1 ? v1 : zero, (double *)o27),
+            "bh_elem_coef");
+}
+
+/* local coordinates of node nid of the element that source element s's kind
+ * uses for its self terms (zt_44 / zt_34, d3b1_elem.c:239,262) */
+static void node_ze(BOUD *bd, int s, int nid, double *zt, double *et)
+{
+  const int q = check_element_type(s, bd) == ELT4;
+  *zt = q ? bd->zt_44[nid] : bd->zt_34[nid];
+  *et = q ? bd->et_44[nid] : bd->et_34[nid];
+}
+
+void coef_rt(double complex *CC, double *rt, int s, double complex k, int type, BOUD *bd)
+{
+  double complex o[27];
+  elem_request(bd, "coef_rt", 0, type, k, s, 0, 0.0, 0.0, rt, NULL, NULL, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+}
+
+void coef_bd(double complex *CC, double *rt, int t, double zeta_t, double eta_t, int s, double complex k, int type,
+             BOUD *bd)
+{
+  double complex o[27];
+  elem_request(bd, "coef_bd", 0, type, k, s, t, zeta_t, eta_t, rt, NULL, NULL, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+}
+
+void coef_bd_node(double complex *CC, int t, int nid, int s, double complex k, int type, BOUD *bd)
+{
+  double complex o[27];
+  double zt, et;
+  node_ze(bd, s, nid, &zt, &et);
+  elem_request(bd, "coef_bd_node", 0, type, k, s, t, zt, et, bd->ren[abs(t)][nid], NULL, NULL, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+}
+
+void dcoef_rt(double complex *CC, double complex *dCC, double *rt, double *vt, int s, double complex k, int type,
+              BOUD *bd)
+{
+  double complex o[27];
+  elem_request(bd, "dcoef_rt", 1, type, k, s, 0, 0.0, 0.0, rt, vt, NULL, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+  memcpy(dCC, o + 9, 9 * sizeof(double complex));
+}
+
+void dcoef_bd_tz(double complex *CC, double complex *dCC, double *rt, double *vtz, int t, double zeta_t, double eta_t,
+                 int s, double complex k, int type, BOUD *bd)
+{
+  double complex o[27];
+  elem_request(bd, "dcoef_bd_tz", 1, type, k, s, t, zeta_t, eta_t, rt, vtz, NULL, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+  memcpy(dCC, o + 9, 9 * sizeof(double complex));
+}
+
+void dcoef_bd_te(double complex *CC, double complex *dCC, double *rt, double *vte, int t, double zeta_t, double eta_t,
+                 int s, double complex k, int type, BOUD *bd)
+{
+  double complex o[27];
+  elem_request(bd, "dcoef_bd_te", 1, type, k, s, t, zeta_t, eta_t, rt, vte, NULL, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+  /* the self element's derivatives come along both tangents: t_eta second */
+  memcpy(dCC, o + (t == s ? 18 : 9), 9 * sizeof(double complex));
+}
+
+void dcoef_bd_t2(double complex *CC, double complex *dCdtz, double complex *dCdte, double *rt, double *vtz,
+                 double *vte, int t, double zeta_t, double eta_t, int s, double complex k, int type, BOUD *bd)
+{
+  double complex o[27];
+  elem_request(bd, "dcoef_bd_t2", 2, type, k, s, t, zeta_t, eta_t, rt, vtz, vte, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+  memcpy(dCdtz, o + 9, 9 * sizeof(double complex));
+  memcpy(dCdte, o + 18, 9 * sizeof(double complex));
+}
+
+void dcoef_bd_node_t2(double complex *CC, double complex *dCdtz, double complex *dCdte, int t, int nid, double *vtz,
+                      double *vte, int s, double complex k, int type, BOUD *bd)
+{
+  double complex o[27];
+  double zt, et;
+  node_ze(bd, s, nid, &zt, &et);
+  elem_request(bd, "dcoef_bd_t2", 2, type, k, s, t, zt, et, bd->ren[abs(t)][nid], vtz, vte, o);
+  memcpy(CC, o, 9 * sizeof(double complex));
+  memcpy(dCdtz, o + 9, 9 * sizeof(double complex));
+  memcpy(dCdte, o + 18, 9 * sizeof(double complex));
+}

@@ -17,8 +17,10 @@ residual).  The same line carries the plain FP64 form on the same scene
 (one configuration), the mixed form's relative errors against it (fields at
 200 points outside the objects, force and torque of every object), the
 dominant kernel's roofline, a CPU baseline of the reference algorithm on the
-host cores (run while the FP64 form occupies the GPU), and the single
-level-4 sphere (5120 triangles) of the previous workload in both forms.
+host cores (run while the FP64 form occupies the GPU), the single
+level-4 sphere (5120 triangles) in both forms, and four objects under 64
+plane waves solved together as right-hand sides (force and torque per
+beam, far-field intensity grid per beam).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -102,12 +104,13 @@ def rel_errors(fld, oid, FN, fld0, oid0, FN0):
 
 
 def kernel_table(bem):
+    """name -> (ms, work, launches, algorithmic bytes) summed over launches"""
     kern = {}
     for name in bem.KERNELS:
-        ms, work, n = bem.prof_read(name)
-        kern[name] = (ms, work, n)
+        ms, work, by, n = bem.prof_read_bytes(name)
+        kern[name] = (ms, work, n, by)
     tot = sum(v[0] for v in kern.values())
-    for name, (ms, work, n) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
+    for name, (ms, work, n, _) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
         if n:
             log("  %-13s %9.1f ms total  %6d launches  %5.1f %% of tracked" % (name, ms, n, 100.0 * ms / max(tot, 1e-9)))
     return kern
@@ -256,14 +259,61 @@ def l4_extra(bem, npts=1000):
                                              for a, b in zip(FNmx, FN64))}}
 
 
+def beams_extra(bem, nb=64):
+    """four objects under nb plane waves as right-hand sides (seeded random
+    directions and polarisation angles), end to end: assembly and
+    factorisation of every object, the coupling loop of all beams together
+    to a 1e-8 change with the boundary E/H, force and torque of every object
+    under every beam (both rules), and the far-field intensity grid of
+    180 x 360 points at 2000 wavelengths for every beam"""
+    from bem3_emf_b2_amd import meshgen as mg
+    beams = mg.random_plane_waves(nb, 64)
+    objs = mg.scene_beams()
+    G = []
+    for o in objs:
+        rn, ed, md, sd, gd = mg.paired_arrays(o["V"] + o["center"], o["F"])
+        g = bem.Object(rn, ed, md, sd, [1.33, o["n"]], 1.064)
+        g.set_precision("mixed")
+        g.set_beams(beams[:1])
+        G.append(g)
+    rc = [o["center"] for o in objs]
+    pts = bem.far_field_grid(1.064, 2000.0, 180, 360)
+    bem.sync()
+    t = [time.perf_counter()]
+    bem.mo_prepare(G, 2, "auto")
+    bem.sync()
+    t.append(time.perf_counter())
+    it = bem.mo_solve_beams(G, beams, ccd=1e-8)
+    bem.sync()
+    t.append(time.perf_counter())
+    for typ in (0, 1):
+        bem.mo_force_beams(G, typ, rc)
+    bem.sync()
+    t.append(time.perf_counter())
+    bem.mo_far_field_beams(G, pts, 1, "I")
+    bem.sync()
+    t.append(time.perf_counter())
+    for g in G:
+        g.close()
+    ph = [t[i + 1] - t[i] for i in range(4)]
+    log("%d beams: %.2f s (objects %.2f, coupled solve %.2f in %d iterations, force %.2f, far field %.2f)"
+        % (nb, t[-1] - t[0], *ph[:2], it, *ph[2:]))
+    return {"workload": "4 objects (two 2028-triangle spheres r=1um, 3072-triangle cube 1.5um, 1520-triangle cone; "
+                        "n=1.59 in water) under %d seeded random plane waves 1.064 um as right-hand sides: assembly + "
+                        "factor + coupled solve of all beams to a 1e-8 change + boundary E/H + force/torque of every "
+                        "object and beam (both rules) + 180x360 far-field intensity grid at 2000 wavelengths per beam"
+                        % nb,
+            "s_total": t[-1] - t[0], "s_per_beam": (t[-1] - t[0]) / nb, "iterations": it,
+            "phases_s": {"objects": ph[0], "coupled_solve": ph[1], "force_torque": ph[2], "far_field": ph[3]}}
+
+
 # ------------------------------------------------------------------ roofline
-# k_gemm128<float> at 15360^3: FETCH_SIZE 42,520,650 KiB x 2.0 + WRITE_SIZE 1,843,200 KiB x 1.0
-# (profiles/gemm128_fp32_15360_counters.txt)
-GEMM128_FP32_TRAFFIC = {"bytes": 42520650 * 1024 * 2.0 + 1843200 * 1024 * 1.0, "shape": "15360x15360x15360",
-                        "algorithmic_bytes": 3 * 15360 ** 2 * 8}
-# k_gemm128<double> at 15360^3 (profiles/gemm128_fp64_15360_counters.txt)
-GEMM128_FP64_TRAFFIC = {"bytes": 323492022 * 1024 * 2.0 + 3830400 * 1024 * 1.0, "shape": "15360x15360x15360",
-                        "algorithmic_bytes": 3 * 15360 ** 2 * 16}
+# k_gemm128<float> over the 16-object scene's configuration (3932 launches of
+# every shape the mixed form's factorisations and solves take; one stream):
+# FETCH_SIZE 328,871 KiB x 2.0 + WRITE_SIZE 61,001 KiB x 1.0 per launch on
+# average (profiles/gemm128_fp32_scene16_counters.txt)
+SCENE_GEMM128_FP32_TRAFFIC = {"bytes": (328871.4 * 2.0 + 61000.9 * 1.0) * 1024,
+                              "shape": "16-object scene, mean over 3932 launches"}
 # kernels whose registry work is real flops: (name, bound, peak TFLOP/s)
 FLOP_KERNELS = {"zgemm": ("k_gemm128<double>", PEAK_FP64_TFLOPS), "cgemm": ("k_gemm128<float>", PEAK_FP32_TFLOPS),
                 "zgemm_small": ("k_gemm64<double>", PEAK_FP64_TFLOPS), "cgemm_small": ("k_gemm64<float>", PEAK_FP32_TFLOPS),
@@ -279,16 +329,16 @@ FLOP_PER_POINT = {"asm_gl": 325.0, "field": 80.0, "scatter_far": 160.0}
 
 def roofline(dom, rec):
     """roofline entry of one tracked kernel from its (ms, work, launches)"""
-    ms, work, n = rec
+    ms, work, n, by = rec
     avg_s = ms / 1000.0 / max(n, 1)
     if dom in FLOP_KERNELS:
         name, peak = FLOP_KERNELS[dom]
         ach = work / n / avg_s / 1e12
-        t = GEMM128_FP32_TRAFFIC if dom == "cgemm" else GEMM128_FP64_TRAFFIC if dom == "zgemm" else None
+        t = SCENE_GEMM128_FP32_TRAFFIC if dom == "cgemm" else None
         return {"kernel": name, "bound": "mfma" if "VALU" not in name else "valu-fp64", "achieved": ach, "peak": peak,
                 "unit": "TFLOP/s", "frac": ach / peak, "traffic": t["bytes"] if t else None,
                 "traffic_shape": t["shape"] if t else None,
-                "traffic_algorithmic_bytes": t["algorithmic_bytes"] if t else None,
+                "algorithmic_bytes_per_launch": by / n if by else None,
                 "avg_launch_ms": 1000.0 * avg_s, "work_per_launch": work / n, "work_unit": "real flops"}
     if dom in FLOP_PER_POINT:
         fpp = 432 * FLOP_PER_POINT[dom]
@@ -310,6 +360,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparator (profiling runs)")
     ap.add_argument("--no-l4", action="store_true", help="skip the single L4 sphere")
+    ap.add_argument("--no-beams", action="store_true", help="skip the 64-beam workload")
+    ap.add_argument("--lanes", type=int, default=0, help="device streams of the per-object work (0: default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -335,10 +387,13 @@ def main():
     log("scene: %d objects, %d triangles" % (len(objs), S))
     G = scene_objects(bem, objs, beams, "mixed")
     for w in range(args.warmup):
+        # the first warm-up runs on one stream: the work areas of the
+        # kernel-timing configuration below exist before it
+        bem.set_lanes(1 if w == 0 else args.lanes)
         t0 = time.perf_counter()
         scene_step(bem, G)
         log("warmup %d: %.3f s" % (w, time.perf_counter() - t0))
-    bem.prof_enable(True)
+    bem.set_lanes(args.lanes)
     bem.sync()
     t0 = time.perf_counter()
     it = 0
@@ -349,9 +404,19 @@ def main():
     bem.sync()
     dt = time.perf_counter() - t0
     s_step = dt / args.steps
+    series, direct, lmax = bem.plan_stats()
+    # kernel timings: one more configuration with the per-object work on one
+    # stream, so that the events around each launch bracket that kernel alone
+    # (with several streams they would also count the other streams' kernels)
+    bem.set_lanes(1)
+    bem.prof_enable(True)
+    t1 = time.perf_counter()
+    scene_step(bem, G)
+    s_serial = time.perf_counter() - t1
     kern = kernel_table(bem)
     bem.prof_enable(False)
-    series, direct, lmax = bem.plan_stats()
+    bem.set_lanes(args.lanes)
+    log("kernel-timing configuration (one stream): %.3f s" % s_serial)
     dom = max(kern, key=lambda k: kern[k][0])
     roof = roofline(dom, kern[dom])
     fld, oid, FN = scene_outputs(bem, G, objs, pts)
@@ -375,6 +440,7 @@ def main():
                 "rel_err_of_mixed": rel_errors(fld, oid, FN, f64, oid64, FN64)}
     cpu = finish_cpu_baseline(cpu_proc) if cpu_proc else None
     l4 = None if args.no_l4 else l4_extra(bem)
+    beams64 = None if args.no_beams else beams_extra(bem)
 
     out = {
         "metric": METRIC,
@@ -399,11 +465,16 @@ def main():
                      "coupling_loop": sum(p[1] for p in phases) / len(phases)},
         "fp64_form": fp64,
         "speedup_vs_fp64": (fp64["value"] / s_step) if fp64 else None,
-        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in kern.items() if v[2]},
+        "kernels_ms_per_step": {k: v[0] for k, v in kern.items() if v[2]},
+        "kernel_timing_step": {"lanes": 1, "s": s_serial,
+                               "note": "kernels_ms_per_step and roofline come from one extra configuration with the "
+                                       "per-object work on one stream (events bracket one kernel each); value uses "
+                                       "the default streams"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (cpu["value"] / s_step) if cpu else None,
         "l4_sphere": l4,
+        "beams64": beams64,
     }
     print(json.dumps(out), flush=True)
     bem.finalize()

@@ -310,6 +310,31 @@ void mo_EH_a_bd(double complex *E, double complex *H, int oid, int did, int t, d
 int mo_force_FN(double *F, double *N, double *rc, int oid, int type, MOBJ *mo);
 int mo_absorb_P(double *P, int oid, int type, MOBJ *mo);
 
+/* ---- element layer (d3b1_elem.h:101-124; the constants d3b1_const.h:44-45) ---- */
+#define ELT3 2
+#define ELT4 3
+int check_element_type(int seid, BOUD *bd);
+void r_bd(double *r, int seid, double zeta, double eta, BOUD *bd);
+void r_bd_node(double *r, int seid, int nid, BOUD *bd);
+void n_bd_node(double *n, int seid, int nid, BOUD *bd);
+void r_tz_bd(double *r, double *vtz, int seid, double zeta, double eta, BOUD *bd);
+void r_te_bd(double *r, double *vte, int seid, double zeta, double eta, BOUD *bd);
+void Tz_bd(double *vTz, int seid, double zeta, double eta, BOUD *bd);
+void Te_bd(double *vTe, int seid, double zeta, double eta, BOUD *bd);
+void tz_te_bd_node(double *vtz, double *vte, int seid, int nid, BOUD *bd);
+void coef_rt(double complex *CC, double *rt, int s, double complex k, int type, BOUD *bd);
+void coef_bd(double complex *CC, double *rt, int t, double zeta_t, double eta_t, int s, double complex k, int type, BOUD *bd);
+void coef_bd_node(double complex *CC, int t, int nid, int s, double complex k, int type, BOUD *bd);
+void dcoef_rt(double complex *CC, double complex *dCC, double *rt, double *vt, int s, double complex k, int type, BOUD *bd);
+void dcoef_bd_tz(double complex *CC, double complex *dCC, double *rt, double *vtz, int t, double zeta_t, double eta_t,
+                 int s, double complex k, int type, BOUD *bd);
+void dcoef_bd_te(double complex *CC, double complex *dCC, double *rt, double *vte, int t, double zeta_t, double eta_t,
+                 int s, double complex k, int type, BOUD *bd);
+void dcoef_bd_t2(double complex *CC, double complex *dCdtz, double complex *dCdte, double *rt, double *vtz,
+                 double *vte, int t, double zeta_t, double eta_t, int s, double complex k, int type, BOUD *bd);
+void dcoef_bd_node_t2(double complex *CC, double complex *dCdtz, double complex *dCdte, int t, int nid, double *vtz,
+                      double *vte, int s, double complex k, int type, BOUD *bd);
+
 /* ---- utility the reference's examples call (com_src/my_utils.h:32):
  *      calloc that prints txt and exits on failure ---- */
 void *m_alloc2(size_t num, size_t size, char *txt);

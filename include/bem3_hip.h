@@ -139,6 +139,11 @@ int bh_mo_prepare(bh_object **objs, int N, int type, int mode);
  * environment variable BEM3_HIP_PROGRESS=1 each iteration also prints
  * its change on stderr.                                                  */
 int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters);
+/* device streams (lanes) over which the per-object work of bh_mo_prepare,
+ * bh_mo_solve and bh_mo_solve_beams is spread, one host worker each: 1..8,
+ * 0 for the default (BEM3_HIP_LANES, else 2).  With 1 every kernel runs in
+ * order on the library's stream.  No counterpart in the reference.      */
+int bh_mo_set_lanes(int n);
 /* How the coupling loop forms the field of one object at another's node
  * slots (no counterpart in the reference, which always sums every element
  * pair): 0 (default) a multipole series of the source object where both
@@ -204,6 +209,21 @@ int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, const int *t,
                 double *out);
 /* radiation force and torque about rc (mo_force_FN, mo_force.c:10-240)   */
 int bh_mo_force(bh_object **objs, int N, int oid, int type, const double *rc, double *F, double *Nq);
+
+/* element coefficients of the reference's element layer for n requests on
+ * object o's elements (coef_rt, coef_bd, coef_bd_node, dcoef_rt,
+ * dcoef_bd_tz, dcoef_bd_te, dcoef_bd_t2, dcoef_bd_node_t2;
+ * d3b1_elem.h:112-124, d3b1_elem.c:148-539): nd 0 CC only, 1 and dC along
+ * v0, 2 along v0 and v1; rule type 0..3 (4-point, 7/9-point,
+ * Gauss-Legendre, the 64-point rule; with derivatives and type < 3 the
+ * 64-point rule again where |dF| along a direction exceeds 0.5); k = kr +
+ * i ki.  Request i: signed source element s[i]; t (may be null) the signed
+ * target element: t[i] == s[i] is the self element at local coordinates
+ * ze[2i..2i+1] (polar rules; derivatives are the principal values along
+ * the element's own tangents), otherwise the point rt[3i..3i+2].  out: 27
+ * complex values per request: CC, dC/dv0, dC/dv1, each G 0-3, H 4-7, F 8. */
+int bh_elem_coef(bh_object *o, int n, int nd, int type, double kr, double ki, const int *s, const int *t,
+                 const double *ze, const double *rt, const double *v0, const double *v1, double *out);
 /* absorbed power (mo_absorb_P, mo_absorb.c:9-44); returns -2 when the
  * flux came out negative (its magnitude is returned)                      */
 int bh_mo_absorb(bh_object **objs, int N, int oid, int type, double *P);
@@ -283,6 +303,10 @@ int bh_cgemm_host(int m, int n, int k, const float *alpha, const float *A, const
 #define BH_K_COUNT    25
 int bh_prof_enable(int on);
 int bh_prof_read(int kid, double *ms, double *work, int *launches);
+/* as bh_prof_read, with the summed algorithmic bytes of the launches (the
+ * operands read once and the results written once; stated for the GEMM
+ * kernels, 0 for the others)                                            */
+int bh_prof_read_bytes(int kid, double *ms, double *work, double *bytes, int *launches);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,8 @@ def lib():
             "orc_api_coef_bd": (None, [VP, D, C.c_int, C.c_double, C.c_double, C.c_int, D, C.c_int, D]),
             "orc_api_dcoef_rt": (None, [VP, D, D, C.c_int, D, C.c_int, D, D]),
             "orc_api_dcoef_node_t2": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, D, C.c_int, D, D, D]),
+            "orc_api_dcoef_bd_t2": (C.c_int, [VP, D, D, D, C.c_int, C.c_double, C.c_double, C.c_int, D, C.c_int, D, D,
+                                              D]),
             "orc_api_cmatrix": (C.c_int, [VP, C.c_int, C.c_int, D, D, D, D, D, Ip]),
             "orc_api_cmatrix_rows": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int, D, D, D, D, D, Ip]),
             "orc_api_create_matrix": (VP, [VP, C.c_int, Ip]),
@@ -235,6 +237,21 @@ class Obj:
         kk = np.array([complex(k).real, complex(k).imag])
         rc = lib().orc_api_dcoef_node_t2(self.h, t, nid, s, dp(kk), type_, dp(CC.view(np.float64)),
                                          dp(dz.view(np.float64)), dp(de.view(np.float64)))
+        return rc, CC, dz, de
+
+    def coef_bd(self, rt, t, zt, et, s, k, type_):
+        CC = cbuf(9)
+        kk = np.array([complex(k).real, complex(k).imag])
+        lib().orc_api_coef_bd(self.h, dp(np.asarray(rt, np.float64)), t, zt, et, s, dp(kk), type_,
+                              dp(CC.view(np.float64)))
+        return CC
+
+    def dcoef_bd_t2(self, rt, vtz, vte, t, zt, et, s, k, type_=2):
+        CC, dz, de = cbuf(9), cbuf(9), cbuf(9)
+        kk = np.array([complex(k).real, complex(k).imag])
+        rc = lib().orc_api_dcoef_bd_t2(self.h, dp(np.asarray(rt, np.float64)), dp(np.asarray(vtz, np.float64)),
+                                       dp(np.asarray(vte, np.float64)), t, zt, et, s, dp(kk), type_,
+                                       dp(CC.view(np.float64)), dp(dz.view(np.float64)), dp(de.view(np.float64)))
         return rc, CC, dz, de
 
     def mEMP2_s(self, rt, v, type_=2):

@@ -171,6 +171,8 @@ void orc_bil_dcoef2_GLH(zc *CC, zc *d0, zc *d1, const double *rt, const double *
 void orc_coef_rt(zc *CC, const double *rt, int s, zc k, int type, const orc_obj *o);
 void orc_coef_bd(zc *CC, const double *rt, int t, double zt, double et, int s, zc k, int type, const orc_obj *o);
 void orc_dcoef_rt(zc *CC, zc *dCC, const double *rt, const double *v, int s, zc k, int type, const orc_obj *o);
+int orc_dcoef_bd_t2(zc *CC, zc *dCz, zc *dCe, const double *rt, const double *vtz, const double *vte, int t,
+                    double zt, double et, int s, zc k, int type, const orc_obj *o);
 int orc_dcoef_bd_node_t2(zc *CC, zc *dCz, zc *dCe, int t, int nid, const double *vtz, const double *vte,
                          int s, zc k, int type, const orc_obj *o);
 

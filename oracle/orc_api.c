@@ -136,6 +136,12 @@ int orc_api_dcoef_node_t2(const orc_obj *o, int t, int nid, int s, const double 
   return orc_dcoef_bd_node_t2((zc *)CC, (zc *)dCz, (zc *)dCe, t, nid, vtz, vte, s, k[0] + I * k[1], type, o);
 }
 
+int orc_api_dcoef_bd_t2(const orc_obj *o, const double *rt, const double *vtz, const double *vte, int t, double zt,
+                        double et, int s, const double *k, int type, double *CC, double *dCz, double *dCe)
+{
+  return orc_dcoef_bd_t2((zc *)CC, (zc *)dCz, (zc *)dCe, rt, vtz, vte, t, zt, et, s, k[0] + I * k[1], type, o);
+}
+
 /* padded domain matrices: G,H (N x N), dG,dH (2N x N), dF (3N), N = 4 dNe */
 int orc_api_cmatrix(const orc_obj *o, int did, int type, double *G, double *H, double *dG, double *dH, double *dF, int *ngh)
 {

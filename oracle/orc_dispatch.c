@@ -66,17 +66,17 @@ void orc_dcoef_rt(zc *CC, zc *dCC, const double *rt, const double *v, int s, zc 
   }
 }
 
-/* Coefficients of one (row node, source element) pair with two tangential
- * derivatives, d3b1_elem.c:482-539.  Returns 1 when the GHN fallback was
- * taken, 2 for the self element, 0 otherwise. */
-int orc_dcoef_bd_node_t2(zc *CC, zc *dCz, zc *dCe, int t, int nid, const double *vtz, const double *vte,
-                         int s, zc k, int type, const orc_obj *o)
+/* Coefficients of one source element at a boundary point with two
+ * tangential derivatives, d3b1_elem.c:418-480 (dcoef_bd_t2): t != s the
+ * point rt with the rule of type (GHN again when |Re dF/dv| > CBD_SDF along
+ * either direction); t == s the self element at (zt, et), polar rule and
+ * principal-value derivatives.  Returns 1 when the GHN fallback was taken,
+ * 2 for the self element, 0 otherwise. */
+int orc_dcoef_bd_t2(zc *CC, zc *dCz, zc *dCe, const double *rt, const double *vtz, const double *vte, int t,
+                    double zt, double et, int s, zc k, int type, const orc_obj *o)
 {
   int tri = orc_is_tri(o, s);
-  int at = t < 0 ? -t : t;
   if (t != s) {
-    double rt[3];
-    for (int d = 0; d < 3; d++) rt[d] = ORC_REN(o, at, nid, d);
     if (tri) {
       if (type == 0) orc_lit_dcoef2_4p(CC, dCz, dCe, rt, vtz, vte, s, k, o);
       else if (type == 1) orc_lit_dcoef2_7p(CC, dCz, dCe, rt, vtz, vte, s, k, o);
@@ -94,14 +94,26 @@ int orc_dcoef_bd_node_t2(zc *CC, zc *dCz, zc *dCe, int t, int nid, const double 
     return 0;
   }
   if (tri) {
-    double zt = ORCQ.zt34[nid], et = ORCQ.et34[nid];
     orc_lit_coef_bd(CC, zt, et, s, k, o);
     orc_lit_dcoef_bd_PV(dCz, zt, et, s, k, o, 0);
     orc_lit_dcoef_bd_PV(dCe, zt, et, s, k, o, 1);
     return 2;
   }
-  orc_bil_coef_bd(CC, ORCQ.zt44[nid], ORCQ.et44[nid], s, k, o);
-  orc_bil_dcoef_bd_PV(dCz, ORCQ.zt44[nid], ORCQ.et44[nid], s, k, o, 0);
-  orc_bil_dcoef_bd_PV(dCe, ORCQ.zt44[nid], ORCQ.et44[nid], s, k, o, 1);
+  orc_bil_coef_bd(CC, zt, et, s, k, o);
+  orc_bil_dcoef_bd_PV(dCz, zt, et, s, k, o, 0);
+  orc_bil_dcoef_bd_PV(dCe, zt, et, s, k, o, 1);
   return 2;
+}
+
+/* Coefficients of one (row node, source element) pair with two tangential
+ * derivatives, d3b1_elem.c:482-539: dcoef_bd_t2 at node nid of element t. */
+int orc_dcoef_bd_node_t2(zc *CC, zc *dCz, zc *dCe, int t, int nid, const double *vtz, const double *vte,
+                         int s, zc k, int type, const orc_obj *o)
+{
+  int at = t < 0 ? -t : t;
+  double rt[3];
+  for (int d = 0; d < 3; d++) rt[d] = ORC_REN(o, at, nid, d);
+  const int tri = orc_is_tri(o, s);
+  const double zt = tri ? ORCQ.zt34[nid] : ORCQ.zt44[nid], et = tri ? ORCQ.et34[nid] : ORCQ.et44[nid];
+  return orc_dcoef_bd_t2(CC, dCz, dCe, rt, vtz, vte, t, zt, et, s, k, type, o);
 }
