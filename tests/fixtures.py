@@ -62,3 +62,27 @@ def active_slots(ed, sid):
     act = np.ones((len(sid) - 1, 4), bool)
     act[tri, 3] = False
     return act.reshape(-1)
+
+
+REF_BIN = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref")
+DROPIN_HEADER = os.path.join(os.path.dirname(GOLDEN), "..", "include", "bem3_emf_b2.h")
+
+
+def ref_example(name):
+    """path of the reference's example program relinked against this library
+    (make -C oracle ref, which records the hash of include/bem3_emf_b2.h it
+    compiled against); skips the test when it was never built here (the
+    reference tree is not mounted), and fails it when it was built against
+    another version of the header"""
+    import hashlib
+    import pytest
+    path = os.path.join(REF_BIN, name)
+    if not os.path.exists(path):
+        pytest.skip("reference %s not compiled against the drop-in library (make -C oracle ref)" % name)
+    stamp = os.path.join(REF_BIN, "header.sha256")
+    want = hashlib.sha256(open(DROPIN_HEADER, "rb").read()).hexdigest()
+    got = open(stamp).read().split()[0] if os.path.exists(stamp) else None
+    if got != want:
+        pytest.fail("%s was compiled against another include/bem3_emf_b2.h (%s, now %s): make -C oracle ref"
+                    % (name, got, want))
+    return path

@@ -30,14 +30,13 @@ import pytest
 import oracle as orc
 from bem3_emf_b2_amd import meshgen as mg
 from datfile import parse_example1_log, read_dat
-from fixtures import GOLDEN, ex30_log, ex30_raw
+from fixtures import GOLDEN, ex30_log, ex30_raw, ref_example
 from test_oracle_beams import BEAMS, LAM
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "bem3_emf_b2_amd", "bin")
-EX1 = os.path.join(ROOT, "oracle", "_ref", "example1")
 REF_PT = [0.3, 0.0, 1.1]
 SETS = {k: [v] for k, v in BEAMS.items()}
 SETS["mixed"] = [[0, LAM, 1.33, 0.0, 1e-3, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.3, 0.2, 0, 0], BEAMS["fpw"], BEAMS["bsb"],
@@ -150,10 +149,9 @@ def test_ex30_from_input_files_state(ex30_run):
             assert err < 1e-10, (did, key, err)
 
 
-@pytest.mark.skipif(not os.path.exists(EX1), reason="reference example1 not compiled against the drop-in library")
 def test_ex30_from_input_files_example1(ex30_run):
     d, _ = ex30_run
-    out = _run([EX1, "ex30.dat"], d)
+    out = _run([ref_example("example1"), "ex30.dat"], d)
     with open(os.path.join(d, "ex1.log"), "w") as fp:
         fp.write(out)
     got = parse_example1_log(os.path.join(d, "ex1.log"))
@@ -209,10 +207,9 @@ def test_ex3_convergence_trace(ex3_run):
         assert abs(got[c] - w) <= 2e-5 * w + 1e-10, (c, got[c], w)
 
 
-@pytest.mark.skipif(not os.path.exists(EX1), reason="reference example1 not compiled against the drop-in library")
 def test_ex3_example1(ex3_run):
     d, _ = ex3_run
-    out = _run([EX1, "ex3.dat"], d)
+    out = _run([ref_example("example1"), "ex3.dat"], d)
     with open(os.path.join(d, "ex1.log"), "w") as fp:
         fp.write(out)
     got = parse_example1_log(os.path.join(d, "ex1.log"))

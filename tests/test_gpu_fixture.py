@@ -18,12 +18,11 @@ import numpy as np
 import pytest
 
 from datfile import parse_example1_log
-from fixtures import GOLDEN, v1_log, v1_raw
+from fixtures import GOLDEN, v1_log, v1_raw, ref_example
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EX1 = os.path.join(ROOT, "oracle", "_ref", "example1")
 REF_PT = [0.3, 0.0, 1.1]  # example1.c:14-16
 TOL = 1e-10
 
@@ -85,7 +84,6 @@ def test_absorbed_power_matches_example1_log(v1_device, typ, oid):
     assert abs(P - ref) < TOL * ref
 
 
-@pytest.mark.skipif(not os.path.exists(EX1), reason="reference example1 not compiled against the drop-in library")
 def test_reference_example1_on_v1_dat(tmp_path):
     """example1.c unchanged, relinked against the drop-in library, on the
     reference's v1.dat: every number it prints against the reference's own
@@ -94,7 +92,7 @@ def test_reference_example1_on_v1_dat(tmp_path):
     with gzip.open(os.path.join(GOLDEN, "v1.dat.gz"), "rb") as src, open(dat, "wb") as dst:
         shutil.copyfileobj(src, dst)
     print("run: example1 v1.dat", flush=True)
-    r = subprocess.run([EX1, str(dat)], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
+    r = subprocess.run([ref_example("example1"), str(dat)], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, (r.stdout[-3000:], r.stderr[-3000:])
     (tmp_path / "ex1.log").write_text(r.stdout)
     got = parse_example1_log(str(tmp_path / "ex1.log"))

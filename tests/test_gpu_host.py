@@ -15,11 +15,12 @@ import oracle as orc
 from bem3_emf_b2_amd import meshgen as mg
 from datfile import parse_example1_log, read_dat
 
+from fixtures import ref_example
+
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "bem3_emf_b2_amd", "bin")
-EX1 = os.path.join(ROOT, "oracle", "_ref", "example1")
 
 BEAM = np.array([[1.064, 1.33, 0.665, 1.0, 0.0, 0.2, 0.1, 0.0, 0.0, 0.0, 0.3, 0.1]])
 N_OBJ = 1.59
@@ -85,13 +86,12 @@ def test_solver_log_reports_convergence(solved):
     assert "finished." in log
 
 
-@pytest.mark.skipif(not os.path.exists(EX1), reason="reference example1 not compiled against the drop-in library")
 def test_reference_example1_relinked(solved):
     """example1.c of the reference, compiled unchanged against the drop-in
     header and library, reads the .dat and prints fields, force, torque and
     absorbed power; the oracle computes the same from its own solution."""
     d, O, log = solved
-    out = _run([EX1, "out.dat"], d)
+    out = _run([ref_example("example1"), "out.dat"], d)
     with open(os.path.join(d, "ex1.log"), "w") as fp:
         fp.write(out)
     got = parse_example1_log(os.path.join(d, "ex1.log"))
@@ -110,10 +110,8 @@ def test_reference_example1_relinked(solved):
             assert abs(got["absorb"][typ][m] - P) <= 1e-7 * abs(P) + 1e-14, (typ, m)
 
 
-EX4 = os.path.join(ROOT, "oracle", "_ref", "example4")
 
 
-@pytest.mark.skipif(not os.path.exists(EX4), reason="reference example4 not compiled against the drop-in library")
 def test_reference_example4_far_field_relinked(solved):
     """example4.c (far-field intensity from E, H of the potentials' gradients,
     called from an OpenMP loop) relinked against the drop-in library: the
@@ -121,7 +119,7 @@ def test_reference_example4_far_field_relinked(solved):
     oracle's mo_EH_mEMP_s at the same points."""
     d, O, log = solved
     sn, mf, ty = 12, 2000.0, 1
-    _run([EX4, "out.dat", str(sn), str(mf), str(ty)], d)
+    _run([ref_example("example4"), "out.dat", str(sn), str(mf), str(ty)], d)
     rows = [l.split() for l in open(os.path.join(d, "fsIe_yz.txt")) if not l.startswith("#")]
     assert len(rows) == sn
     ra = mf * 1.064

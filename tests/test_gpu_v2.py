@@ -29,13 +29,12 @@ import numpy as np
 import pytest
 
 from datfile import parse_example1_log
-from fixtures import GOLDEN
+from fixtures import GOLDEN, ref_example
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "bem3_emf_b2_amd", "bin")
-EX1 = os.path.join(ROOT, "oracle", "_ref", "example1")
 OBJ = "sphere_m0_md.obj"
 IPW = "## plane wave\n# number of waves\n1\n#lambda0 ni power real(e0x) imag(e0x) real(e0y) imag(e0y) fx fy fz theta phi\n" \
       "1.064 1.33 1.0e-3 1.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0 0.0\n"
@@ -79,10 +78,9 @@ def test_convergence_trace_matches_reference_log(v2_run):
         assert abs(got[c] - w) <= 2e-5 * w + 1e-10, (c, got[c], w)
 
 
-@pytest.mark.skipif(not os.path.exists(EX1), reason="reference example1 not compiled against the drop-in library")
 def test_reference_example1_matches_reference_log(v2_run):
     d, _ = v2_run
-    out = _run([EX1, "v2.dat"], d)
+    out = _run([ref_example("example1"), "v2.dat"], d)
     with open(os.path.join(d, "ex1.log"), "w") as fp:
         fp.write(out)
     got = parse_example1_log(os.path.join(d, "ex1.log"))
