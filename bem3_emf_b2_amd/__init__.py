@@ -78,6 +78,7 @@ def lib():
             "bh_object_beam_state": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, D]),
             "bh_object_select_beam": (C.c_int, [VP, C.c_int]),
             "bh_mo_set_lanes": (C.c_int, [C.c_int]),
+            "bh_object_set_block_rows": (C.c_int, [VP, C.c_int]),
             "bh_elem_coef": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, Ip, Ip, D, D, D, D, D]),
             "bh_mo_force_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, D, D, Ip]),
             "bh_mo_far_field_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D, C.c_int, D]),
@@ -197,8 +198,15 @@ class Object:
     def set_precision(self, prec):
         """"fp64" (default) or "mixed": FP32 far-element quadrature and
         boundary field sums, factor mode 0 -> the FP32 block elimination
-        refined in FP64"""
-        check(lib().bh_object_set_precision(self.h, {"fp64": 0, "mixed": 1}[prec]))
+        refined in FP64; "mixed_lm": the same with the matrices stored only
+        in FP32 and their FP64 rows assembled again where needed (objects
+        whose FP64 matrices do not fit the device)"""
+        check(lib().bh_object_set_precision(self.h, {"fp64": 0, "mixed": 1, "mixed_lm": 2}[prec]))
+
+    def set_block_rows(self, rows):
+        """rows per block of the "mixed_lm" form's on-demand assembly (0: by
+        the memory budget)"""
+        check(lib().bh_object_set_block_rows(self.h, rows))
 
     def gh_count(self, did):
         return lib().bh_object_gh_count(self.h, did)

@@ -24,7 +24,7 @@ void bem_launch_asm_self(const void *, const void *, const int *, const void *, 
                          const double *, const double *, const double *, const double *, void *, void *, void *, void *,
                          size_t, double *, hipStream_t);
 void bem_launch_asm_fin(const double *, int, const double *, const int *, const double *, int, int, void *, size_t,
-                        double *, hipStream_t);
+                        double *, int, hipStream_t);
 void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
                hipStream_t);
 int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
@@ -418,6 +418,72 @@ int g_nqrule[4] = {0, 0, 0, 0};
 }  // namespace bem
 
 // ------------------------------------------------------------------ assembly
+template <class T> static int dalloc(T **p, size_t n)
+{
+  HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+  HIPCHK(hipMemsetAsync(*p, 0, std::max<size_t>(n, 1) * sizeof(T), cur_st()));
+  return 0;
+}
+
+// rows r0 .. r0+nrows of domain d's matrices into G, H, dG, dH (row stride
+// ld = the domain's node count, the block's first row first) and dF (3 per
+// row); *ngh receives the near pairs redone with the 64-point rule
+static int assemble_rows(bh_object *o, int d, int r0, int nrows, cd *G, cd *H, cd *dG, cd *dH, double *dF, int *ngh)
+{
+  Dom &D = o->dom[d];
+  const size_t nd = (size_t)D.nd, nr = (size_t)nrows;
+  const int nblk = (D.dNe + 255) / 256, nwv = nblk * 4;
+  // work areas: grow-only, owned by the object
+  double *d_part = o->ws_part.get<double>(nr * nwv * 3);
+  unsigned long long *d_mask = o->ws_mask.get<unsigned long long>(nr * nwv);
+  if (!d_part || !d_mask) return fail(BH_EHIP, "assembly: work area of %zu rows", nr);
+  double kr = o->kn[d].real(), ki = o->kn[d].imag();
+  const bool mixed = o->prec != BH_PREC_FP64;
+  const RowPt *rows = D.d_rows + r0;
+  if (mixed)
+    bem_launch_asm_glr(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, D.dNe, nrows, kr, ki, g.d_glr, g_qrule[2],
+                       g_nqrule[2], G, H, dG, dH, nd, d_part, d_mask, nblk, cur_st());
+  else
+    bem_launch_asm_gl(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, D.dNe, nrows, kr, ki, g.d_gl, g.n_gl, g_qrule[2],
+                      g_nqrule[2], G, H, dG, dH, nd, d_part, d_mask, nblk, cur_st());
+  HIPCHK(hipGetLastError());
+  // near pairs in row-major order (deterministic)
+  std::vector<unsigned long long> mask(nr * nwv);
+  HIPCHK(hipMemcpyAsync(mask.data(), d_mask, mask.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, cur_st()));
+  HIPCHK(hipStreamSynchronize(cur_st()));
+  std::vector<GhPair> pairs;
+  std::vector<int> start(nr + 1, 0);
+  for (size_t r = 0; r < nr; r++) {
+    start[r] = (int)pairs.size();
+    for (int w = 0; w < nwv; w++) {
+      unsigned long long m = mask[r * nwv + w];
+      while (m) {
+        int b = __builtin_ctzll(m);
+        m &= m - 1;
+        pairs.push_back({(int)r, w * 64 + b + 1});
+      }
+    }
+  }
+  start[nr] = (int)pairs.size();
+  *ngh = (int)pairs.size();
+  GhPair *d_pairs = o->ws_pairs.get<GhPair>(std::max<size_t>(pairs.size(), 1));
+  int *d_start = o->ws_start.get<int>(start.size());
+  double *d_ghdf = o->ws_ghdf.get<double>(std::max<size_t>(pairs.size(), 1) * 3);
+  double *d_self = o->ws_self.get<double>(nr * 3);   // the self element's F terms per row
+  if (!d_pairs || !d_start || !d_ghdf || !d_self) return fail(BH_EHIP, "assembly: work area of the near pairs");
+  if (!pairs.empty()) HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(GhPair), hipMemcpyHostToDevice, cur_st()));
+  HIPCHK(hipMemcpyAsync(d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, cur_st()));
+  bem_launch_asm_gh(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, d_pairs, (int)pairs.size(), kr, ki, g.d_gh, g.n_gh,
+                    mixed ? (const void *)g.d_gl : nullptr, g.n_gl, g_qrule[3], g_nqrule[3], g_qrule[2], g_nqrule[2],
+                    G, H, dG, dH, nd, d_ghdf, cur_st());
+  bem_launch_asm_self(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, nrows, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh, G,
+                      H, dG, dH, nd, d_self, cur_st());
+  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, d_self, nrows, d == 0, H, nd, dF, r0, cur_st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(cur_st()));   // the host tables above stay alive until here
+  return 0;
+}
+
 static int assemble_domain(bh_object *o, int d)
 {
   Dom &D = o->dom[d];
@@ -429,54 +495,58 @@ static int assemble_domain(bh_object *o, int d)
     HIPCHK(hipMalloc((void **)&D.d_dH, 2 * nd * nd * sizeof(cd)));
     HIPCHK(hipMalloc((void **)&D.d_dF, 3 * nd * sizeof(double)));
   }
-  const int nblk = (D.dNe + 255) / 256, nwv = nblk * 4;
-  // work areas: grow-only, owned by the object
-  double *d_part = o->ws_part.get<double>(nd * nwv * 3);
-  unsigned long long *d_mask = o->ws_mask.get<unsigned long long>(nd * nwv);
-  if (!d_part || !d_mask) return fail(BH_EHIP, "assembly: work area of %zu rows", nd);
-  double kr = o->kn[d].real(), ki = o->kn[d].imag();
-  const bool mixed = o->prec == BH_PREC_MIXED;
-  if (mixed)
-    bem_launch_asm_glr(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_glr, g_qrule[2],
-                       g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, cur_st());
-  else
-    bem_launch_asm_gl(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, D.dNe, (int)nd, kr, ki, g.d_gl, g.n_gl, g_qrule[2],
-                      g_nqrule[2], D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_part, d_mask, nblk, cur_st());
-  HIPCHK(hipGetLastError());
-  // near pairs in row-major order (deterministic)
-  std::vector<unsigned long long> mask(nd * nwv);
-  HIPCHK(hipMemcpyAsync(mask.data(), d_mask, mask.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, cur_st()));
-  HIPCHK(hipStreamSynchronize(cur_st()));
-  std::vector<GhPair> pairs;
-  std::vector<int> start(nd + 1, 0);
-  for (size_t r = 0; r < nd; r++) {
-    start[r] = (int)pairs.size();
-    for (int w = 0; w < nwv; w++) {
-      unsigned long long m = mask[r * nwv + w];
-      while (m) {
-        int b = __builtin_ctzll(m);
-        m &= m - 1;
-        pairs.push_back({(int)r, w * 64 + b + 1});
-      }
-    }
+  return assemble_rows(o, d, 0, (int)nd, D.d_G, D.d_H, D.d_dG, D.d_dH, D.d_dF, &D.n_gh);
+}
+
+// Domain d's FP64 matrices a block of rows at a time: fn(r0, nrows, G, H,
+// dG, dH, dF) with pointers at the block's first row (row stride nd).  The
+// FP64 forms hold them whole (one call); the low-memory form assembles each
+// block again (rows per block within about 16 GB of FP64 rows).
+template <class F> static int domain_blocks(bh_object *o, int d, F fn)
+{
+  Dom &D = o->dom[d];
+  const size_t nd = (size_t)D.nd;
+  if (o->prec != BH_PREC_MIXED_LM) return fn(0, (int)nd, D.d_G, D.d_H, D.d_dG, D.d_dH, D.d_dF);
+  size_t rb = o->lm_rows > 0 ? (size_t)o->lm_rows : ((size_t)16 << 30) / (6 * nd * sizeof(cd));
+  rb = std::max<size_t>(64, std::min(rb, nd));
+  cd *G = o->lm_blk[0].get<cd>(rb * nd), *H = o->lm_blk[1].get<cd>(rb * nd);
+  cd *dG = o->lm_blk[2].get<cd>(2 * rb * nd), *dH = o->lm_blk[3].get<cd>(2 * rb * nd);
+  double *dF = o->lm_blk[4].get<double>(3 * rb);
+  if (!G || !H || !dG || !dH || !dF) return fail(BH_EHIP, "low-memory form: row blocks of %zu rows", rb);
+  int ngh = 0;
+  for (size_t r0 = 0; r0 < nd; r0 += rb) {
+    const int n = (int)std::min(rb, nd - r0);
+    int k = 0, rc = assemble_rows(o, d, (int)r0, n, G, H, dG, dH, dF, &k);
+    if (rc) return rc;
+    ngh += k;
+    if ((rc = fn((int)r0, n, G, H, dG, dH, dF))) return rc;
   }
-  start[nd] = (int)pairs.size();
-  D.n_gh = (int)pairs.size();
-  GhPair *d_pairs = o->ws_pairs.get<GhPair>(std::max<size_t>(pairs.size(), 1));
-  int *d_start = o->ws_start.get<int>(start.size());
-  double *d_ghdf = o->ws_ghdf.get<double>(std::max<size_t>(pairs.size(), 1) * 3);
-  double *d_self = o->ws_self.get<double>(nd * 3);   // the self element's F terms per row
-  if (!d_pairs || !d_start || !d_ghdf || !d_self) return fail(BH_EHIP, "assembly: work area of the near pairs");
-  if (!pairs.empty()) HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(GhPair), hipMemcpyHostToDevice, cur_st()));
-  HIPCHK(hipMemcpyAsync(d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, cur_st()));
-  bem_launch_asm_gh(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, d_pairs, D.n_gh, kr, ki, g.d_gh, g.n_gh,
-                    mixed ? (const void *)g.d_gl : nullptr, g.n_gl, g_qrule[3], g_nqrule[3], g_qrule[2], g_nqrule[2],
-                    D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_ghdf, cur_st());
-  bem_launch_asm_self(o->d_geo, o->d_gq, D.d_sid, D.d_rows, D.d_colOff, (int)nd, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh,
-                      D.d_G, D.d_H, D.d_dG, D.d_dH, nd, d_self, cur_st());
-  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, d_self, (int)nd, d == 0, D.d_H, nd, D.d_dF, cur_st());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(cur_st()));   // the host tables above stay alive until here
+  D.n_gh = ngh;
+  return 0;
+}
+
+// the low-memory form's assembly: every block of G, H rounded into the
+// complex-FP32 storage the factorisation works in (c_G0f: G0, c_Z: H0,
+// c_G1: G1, c_Mf: H1)
+static int assemble_lm(bh_object *o)
+{
+  if (o->MN != 1) return fail(BH_EUNSUP, "the low-memory form needs exactly one interior medium");
+  const size_t NN = (size_t)o->NN, n2 = NN * NN;
+  for (int d = 0; d <= 1; d++)
+    if ((size_t)o->dom[d].nd != NN) return fail(BH_EUNSUP, "low-memory form: unexpected domain sizes");
+  if (!o->c_G0f && (dalloc(&o->c_G0f, n2) || dalloc(&o->c_Z, n2) || dalloc(&o->c_Mf, n2) || dalloc(&o->c_G1, n2)))
+    return BH_EHIP;
+  cf *Gs[2] = {o->c_G0f, o->c_G1}, *Hs[2] = {o->c_Z, o->c_Mf};
+  for (int d = 0; d <= 1; d++) {
+    int rc = domain_blocks(o, d, [&](int r0, int n, const cd *G, const cd *H, const cd *, const cd *, const double *) {
+      bem_cvt_z2c(G, Gs[d] + (size_t)r0 * NN, (size_t)n * NN, cur_st());
+      bem_cvt_z2c(H, Hs[d] + (size_t)r0 * NN, (size_t)n * NN, cur_st());
+      HIPCHK(hipStreamSynchronize(cur_st()));   // the block is assembled again for the next rows
+      return 0;
+    });
+    if (rc) return rc;
+  }
+  o->lm_raw = true;
   return 0;
 }
 
@@ -493,9 +563,15 @@ int bem::object_assemble(bh_object *o, int type)
   o->mb_xwarm = false;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   o->asm_prec = -1;
-  for (int d = 0; d <= o->MN; d++) {
-    int rc = assemble_domain(o, d);
+  o->lm_raw = false;
+  if (o->prec == BH_PREC_MIXED_LM) {
+    int rc = assemble_lm(o);
     if (rc) return rc;
+  } else {
+    for (int d = 0; d <= o->MN; d++) {
+      int rc = assemble_domain(o, d);
+      if (rc) return rc;
+    }
   }
   o->asm_prec = o->prec;
   return 0;
@@ -510,7 +586,7 @@ extern "C" int bh_object_assemble(bh_object *o, int type)
 extern "C" int bh_object_set_precision(bh_object *o, int prec)
 {
   API_LOCK;
-  if (prec != BH_PREC_FP64 && prec != BH_PREC_MIXED) return fail(BH_EARG, "precision %d", prec);
+  if (prec != BH_PREC_FP64 && prec != BH_PREC_MIXED && prec != BH_PREC_MIXED_LM) return fail(BH_EARG, "precision %d", prec);
   if (prec == o->prec) return 0;
   // matrices, factors and warm starts of the other precision are stale:
   // the object has to be assembled (and factorised) again
@@ -525,9 +601,20 @@ extern "C" int bh_object_set_precision(bh_object *o, int prec)
 
 extern "C" int bh_object_gh_count(const bh_object *o, int did) { return o->dom[did].n_gh; }
 
+// rows per block of the low-memory form's on-demand assembly (0: as many as
+// fit about 16 GB of FP64 rows)
+extern "C" int bh_object_set_block_rows(bh_object *o, int rows)
+{
+  API_LOCK;
+  if (rows < 0) return fail(BH_EARG, "block rows %d", rows);
+  o->lm_rows = rows;
+  return 0;
+}
+
 // which: 0 G, 1 H (nd x nd), 2 dG, 3 dH (nd x 2 x nd), 4 dF (nd x 3 doubles)
 extern "C" int bh_object_matrix(const bh_object *o, int did, int which, double *out)
 {
+  if (o->prec == BH_PREC_MIXED_LM) return fail(BH_EUNSUP, "the low-memory form keeps no FP64 matrices");
   API_LOCK;
   if (did < 0 || did > o->MN) return fail(BH_EARG, "bad domain %d", did);
   const Dom &D = o->dom[did];
@@ -544,6 +631,7 @@ extern "C" int bh_object_matrix(const bh_object *o, int did, int which, double *
 // another, each as bh_object_matrix lays a row out
 extern "C" int bh_object_matrix_rows(const bh_object *o, int did, int which, int nrows, const int *rows, double *out)
 {
+  if (o->prec == BH_PREC_MIXED_LM) return fail(BH_EUNSUP, "the low-memory form keeps no FP64 matrices");
   API_LOCK;
   if (did < 0 || did > o->MN || which < 0 || which > 4 || nrows < 0) return fail(BH_EARG, "bad matrix selector");
   const Dom &D = o->dom[did];
@@ -669,12 +757,12 @@ extern "C" int bh_cgemm_host(int m, int n, int k, const float *alpha, const floa
 extern "C" {
 void bem_launch_build_A(const void *, const void *, int, size_t, const void *, size_t, size_t, void *, size_t,
                         hipStream_t);
-void bem_launch_build_B(const void *, const void *, int, size_t, const void *, const void *, const void *, size_t, void *,
-                        hipStream_t);
+void bem_launch_build_B(const void *, const void *, int, int, size_t, const void *, const void *, const void *, size_t,
+                        void *, hipStream_t);
 void bem_launch_store_X(const void *, int, const void *, size_t, const int *, const void *, const void *, void *, void *,
                         hipStream_t);
-void bem_launch_eh_rows(const void *, const void *, int, size_t, const double *, const void *, const int *, const void *,
-                        const void *, const void *, double, double, void *, void *, hipStream_t);
+void bem_launch_eh_rows(const void *, const void *, int, int, size_t, const double *, const void *, const int *,
+                        const void *, const void *, const void *, double, double, void *, void *, hipStream_t);
 void bem_launch_pack_eh(int, const void *, const int *, const void *, const void *, void *, hipStream_t);
 void bem_launch_unpack_eh(int, const void *, const int *, const void *, void *, void *, hipStream_t);
 void bem_launch_hadamard_nn(void *, int, size_t, const void *, hipStream_t);
@@ -791,12 +879,6 @@ int bem::ensure_rules()
   return 0;
 }
 
-template <class T> static int dalloc(T **p, size_t n)
-{
-  HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
-  HIPCHK(hipMemsetAsync(*p, 0, std::max<size_t>(n, 1) * sizeof(T), cur_st()));
-  return 0;
-}
 
 // per-domain column tables, state buffers, slot and face tables
 int bem::prepare_solve(bh_object *o)
@@ -1120,12 +1202,15 @@ int bem::form_s(bh_object *o)
 // 2: Schur, 3: Schur in complex FP32 refined in FP64 (mixed precision)
 int bem::object_factor(bh_object *o, int mode)
 {
-  if (!o->dom[0].d_G || o->asm_prec < 0) return fail(BH_ESTATE, "assemble before factor");
+  const bool lm = o->prec == BH_PREC_MIXED_LM;
+  if ((!lm && !o->dom[0].d_G) || o->asm_prec < 0) return fail(BH_ESTATE, "assemble before factor");
   if (o->asm_prec != o->prec) return fail(BH_ESTATE, "the precision changed since the assembly: assemble again");
   if (mode < 0 || mode > 3) return fail(BH_EARG, "factor mode %d", mode);
+  if (lm && mode != 0 && mode != 3) return fail(BH_EUNSUP, "the low-memory form factorises in mode 3 only");
+  if (lm && !o->lm_raw) return fail(BH_ESTATE, "low-memory form: the matrices were factorised in place: assemble again");
   if (prepare_solve(o)) return BH_EHIP;
   size_t NN = (size_t)o->NN, na = 8 * NN;
-  if (mode == 0) mode = o->MN == 1 ? (o->prec == BH_PREC_MIXED ? 3 : 2) : 1;
+  if (mode == 0) mode = o->MN == 1 ? (o->prec != BH_PREC_FP64 ? 3 : 2) : 1;
   if (mode >= 2 && o->MN != 1) return fail(BH_EUNSUP, "the block elimination needs exactly one interior medium");
   o->fmode = 0;
   o->x_warm = false;
@@ -1154,7 +1239,14 @@ int bem::object_factor(bh_object *o, int mode)
     o->sch_beps = mk(beps.real(), beps.imag());
     o->sch_sigm = mk(sigm.real(), sigm.imag());
     int n = (int)NN;
-    if (mode == 3) {
+    if (mode == 3 && lm) {
+      // the assembled FP32 matrices become the factors in place
+      if ((!o->c_Sf && dalloc(&o->c_Sf, n2)) || (!o->c_w && dalloc(&o->c_w, schur_work(NN) + 16 * NN))) return BH_EHIP;
+      o->lm_raw = false;
+      if (schur_factor_c_inplace(n, NN, o->c_G0f, o->c_Z, o->c_G1, o->c_Mf, o->sch_beps, o->c_Sf, o->d_piv0, o->d_pivM,
+                                 o->d_info, cur_st()))
+        return lu_failed();
+    } else if (mode == 3) {
       if (!o->c_G0f && (dalloc(&o->c_G0f, n2) || dalloc(&o->c_Z, n2) || dalloc(&o->c_Mf, n2) || dalloc(&o->c_Sf, n2) ||
                         dalloc(&o->c_G1, n2) || dalloc(&o->c_w, schur_work(NN) + 16 * NN)))
         return BH_EHIP;
@@ -1202,6 +1294,29 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
 #define IR_MAX 30
 
 // B from the incident data, X = A^{-1} B, boundary values (solve_coefficient_mEMP)
+// B rows of domain d (create_Bmatrix_dac) and the FP64 residual rows of
+// domain d, over the domain's matrices a block of rows at a time
+static int dom_build_B(bh_object *o, int d)
+{
+  Dom &D = o->dom[d];
+  const size_t rb = row_base(o, d);
+  return domain_blocks(o, d, [&](int r0, int n, const cd *G, const cd *H, const cd *, const cd *, const double *) {
+    bem_launch_build_B(G, H, n, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, rb + r0, o->d_B, cur_st());
+    return 0;
+  });
+}
+
+static int dom_resid(bh_object *o, int d)
+{
+  Dom &D = o->dom[d];
+  const size_t rb = row_base(o, d);
+  bem_resid_colvec(D.d_col, D.nd, o->d_X, (size_t)o->NN, o->d_w, cur_st());
+  return domain_blocks(o, d, [&](int r0, int n, const cd *G, const cd *H, const cd *, const cd *, const double *) {
+    bem_resid_rows(G, H, n, D.nd, D.nd, r0, rb, o->d_B, o->d_R, o->d_w, cur_st());
+    return 0;
+  });
+}
+
 static int object_solve_core(bh_object *o)
 {
   if (!o->fmode) return fail(BH_ESTATE, "factor before solve");
@@ -1209,8 +1324,8 @@ static int object_solve_core(bh_object *o)
   size_t NN = (size_t)o->NN, na = 8 * NN;
   HIPCHK(hipMemsetAsync(o->d_B, 0, na * sizeof(cd), cur_st()));
   for (int d = 0; d <= o->MN; d++) {
-    Dom &D = o->dom[d];
-    bem_launch_build_B(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, row_base(o, d), o->d_B, cur_st());
+    int rc = dom_build_B(o, d);
+    if (rc) return rc;
   }
   o->ir_iters = 0;
   o->ir_res = 0.0;
@@ -1230,8 +1345,8 @@ static int object_solve_core(bh_object *o)
     double res = nb > 0.0 ? 1.0 : 0.0;
     if (o->x_warm && nb > 0.0) {
       for (int d = 0; d <= o->MN; d++) {
-        Dom &D = o->dom[d];
-        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, cur_st());
+        int rc = dom_resid(o, d);
+        if (rc) return rc;
       }
       const double nr = bem_nrm2(o->d_R, na, o->d_nrm, cur_st());
       if (nr < 0.0) return fail(BH_EHIP, "norm of the residual");
@@ -1252,8 +1367,8 @@ static int object_solve_core(bh_object *o)
       o->gm_steps += gs;
       bem_acc_c2z(o->d_X, cdx, na, cur_st());
       for (int d = 0; d <= o->MN; d++) {
-        Dom &D = o->dom[d];
-        bem_resid_domain(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_X, NN, row_base(o, d), o->d_B, o->d_R, o->d_w, cur_st());
+        int rc = dom_resid(o, d);
+        if (rc) return rc;
       }
       const double nr = bem_nrm2(o->d_R, na, o->d_nrm, cur_st());
       if (nr < 0.0) return fail(BH_EHIP, "norm of the residual");
@@ -1301,8 +1416,10 @@ extern "C" int bh_object_solve_stats(const bh_object *o, int *iters, double *res
 
 // G x = b for 6 right-hand sides (nd x 6, b in R, x into R) with the
 // complex-FP32 LU of G, refined against the FP64 G
-static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *piv, cd *R, int ncol = 6)
+template <class F> static int domain_blocks(bh_object *o, int d, F fn);
+static int solve_g_mixed(bh_object *o, int d, const cf *Gc, const int *piv, cd *R, int ncol = 6)
 {
+  const Dom &D = o->dom[d];
   const size_t nd = (size_t)D.nd, m = (size_t)ncol * nd;
   cd *X = o->ws_eh[2].get<cd>(m), *Rr = o->ws_eh[3].get<cd>(m);
   cf *C = o->ws_eh[4].get<cf>(m);
@@ -1317,7 +1434,11 @@ static int solve_g_mixed(bh_object *o, const Dom &D, const cf *Gc, const int *pi
     bem_cgetrs(D.nd, ncol, Gc, nd, piv, C, ncol, cur_st());
     bem_acc_c2z(X, C, m, cur_st());
     HIPCHK(hipMemcpyAsync(Rr, R, m * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
-    bem_zgemm(D.nd, ncol, D.nd, -1.0, 0.0, D.d_G, nd, X, ncol, 1.0, 0.0, Rr, ncol, cur_st());
+    rc = domain_blocks(o, d, [&](int r0, int n, const cd *G, const cd *, const cd *, const cd *, const double *) {
+      bem_zgemm(n, ncol, D.nd, -1.0, 0.0, G, nd, X, ncol, 1.0, 0.0, Rr + (size_t)r0 * ncol, ncol, cur_st());
+      return 0;
+    });
+    if (rc) break;
     const double nr = bem_nrm2(Rr, m, o->d_nrm, cur_st());
     if (nr < 0.0) { rc = fail(BH_EHIP, "norm of the residual"); break; }
     res = nr / nb;
@@ -1342,8 +1463,12 @@ int bem::object_solve_EH(bh_object *o)
   for (int d = 0; d <= o->MN; d++) {
     Dom &D = o->dom[d];
     size_t nd = (size_t)D.nd;
-    bem_launch_eh_rows(D.d_dG, D.d_dH, D.nd, nd, D.d_dF, D.d_rowEH, D.d_colS, D.d_col, D.d_U, D.d_dU, ch.real(),
-                       ch.imag(), D.d_E, D.d_Hf, cur_st());
+    int rc = domain_blocks(o, d, [&](int r0, int n, const cd *, const cd *, const cd *dG, const cd *dH, const double *dF) {
+      bem_launch_eh_rows(dG, dH, n, D.nd, nd, dF, D.d_rowEH + r0, D.d_colS, D.d_col, D.d_U, D.d_dU, ch.real(),
+                         ch.imag(), D.d_E, D.d_Hf, cur_st());
+      return 0;
+    });
+    if (rc) return rc;
     // LU of G_d: domain 0 of the block form already factorised G0 for this
     // assembly (in the precision of the form); otherwise factorise G_d once
     // per assembly
@@ -1358,7 +1483,10 @@ int bem::object_solve_EH(bh_object *o)
       if (!D.gfc_valid) {
         if (!D.c_Gf && dalloc(&D.c_Gf, nd * nd)) return BH_EHIP;
         if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
-        bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, cur_st());
+        if (o->prec == BH_PREC_MIXED_LM)   // G1 in complex FP32, kept by the factorisation
+          HIPCHK(hipMemcpyAsync(D.c_Gf, o->c_G1, nd * nd * sizeof(cf), hipMemcpyDeviceToDevice, cur_st()));
+        else
+          bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, cur_st());
         if (bem_cgetrf(D.nd, D.c_Gf, nd, D.d_Gpiv, o->d_info, cur_st())) return lu_failed();
         D.gfc_valid = true;
         D.gf_valid = false;   // the shared pivot vector now belongs to the FP32 factors
@@ -1378,10 +1506,13 @@ int bem::object_solve_EH(bh_object *o)
     cd *P = o->ws_eh[0].get<cd>(nd * 6), *R = o->ws_eh[1].get<cd>(nd * 6);
     if (!P || !R) return fail(BH_EHIP, "boundary fields: work area");
     bem_launch_pack_eh(D.nd, D.d_col, D.d_colS, D.d_E, D.d_Hf, P, cur_st());
-    bem_zgemm(D.nd, 6, D.nd, 1.0, 0.0, D.d_H, nd, P, 6, 0.0, 0.0, R, 6, cur_st());
-    int rc = 0;
+    rc = domain_blocks(o, d, [&](int r0, int n, const cd *, const cd *H, const cd *, const cd *, const double *) {
+      bem_zgemm(n, 6, D.nd, 1.0, 0.0, H, nd, P, 6, 0.0, 0.0, R + (size_t)r0 * 6, 6, cur_st());
+      return 0;
+    });
+    if (rc) return rc;
     if (mixed)
-      rc = solve_g_mixed(o, D, Gc, Gpiv, R);
+      rc = solve_g_mixed(o, d, Gc, Gpiv, R);
     else
       bem_zgetrs(D.nd, 6, Gf, nd, Gpiv, R, 6, 64, cur_st());
     if (!rc) bem_launch_unpack_eh(D.nd, D.d_col, D.d_colS, R, D.d_dE, D.d_dHf, cur_st());
@@ -1427,6 +1558,7 @@ cd *bem::mb_dom_ptr(bh_object *o, int d, int which) { return (cd *)o->mb_dom[(si
 int bem::object_beams_set(bh_object *o, int nb, const double *par)
 {
   if (nb <= 0) return fail(BH_EARG, "no beams");
+  if (o->prec == BH_PREC_MIXED_LM) return fail(BH_EUNSUP, "many beams: not in the low-memory form");
   if (prepare_solve(o)) return BH_EHIP;
   const size_t E1 = (size_t)o->Ne + 1;
   for (int q = 0; q < 5; q++)
@@ -1589,7 +1721,7 @@ int bem::object_solve_EH_multi(bh_object *o)
     cd *P = PU, *Rm = Z;   // reused: [nd x 6nb] each
     bem_launch_mpack_eh(D.nd, nb, D.d_col, D.d_colS, mb_dom_ptr(o, d, 2), mb_dom_ptr(o, d, 4), es, P, cur_st());
     bem_zgemm(D.nd, 6 * nb, D.nd, 1.0, 0.0, D.d_H, nd, P, 6 * nb, 0.0, 0.0, Rm, 6 * nb, cur_st());
-    int rc = solve_g_mixed(o, D, Gc, Gpiv, Rm, 6 * nb);
+    int rc = solve_g_mixed(o, d, Gc, Gpiv, Rm, 6 * nb);
     if (rc) return rc;
     bem_launch_munpack_eh(D.nd, nb, D.d_col, D.d_colS, Rm, mb_dom_ptr(o, d, 3), mb_dom_ptr(o, d, 5), es, cur_st());
   }
@@ -1910,7 +2042,7 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
       if (cput(12, &d_ci, ci.data(), nc) || cput(13, &d_ct, ct.data(), nc) || cput(14, &d_cze, cze.data(), cze.size()))
         return BH_EHIP;
       bem_launch_field_bd(d_ct, d_cze, nc, 0, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[0].real(), o->kn[0].imag(),
-                          g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
+                          g_rule[2], o->prec != BH_PREC_FP64 ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
                           g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf,
                           d_tmp, d_rt, cur_st());
       hipLaunchKernelGGL(k_scatter6, g1(nc), dim3(256), 0, cur_st(), d_tmp, d_ci, nc, d_EH);
@@ -2284,7 +2416,7 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
   // own boundary integral (needs the solved boundary values)
   if (mode != 2) {
     bem_launch_field_bd(d_t, d_ze, np, did, o->d_geo, o->d_gq, D.d_sid, D.dNe, o->kn[did].real(), o->kn[did].imag(),
-                        g_rule[2], o->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
+                        g_rule[2], o->prec != BH_PREC_FP64 ? g.d_gl32 : nullptr, g_nrule[2], g_rscale[2], g_qrule[2],
                         g_nqrule[2], g.d_xl, g.d_wl, g.d_xh, g.d_wh, D.d_E, D.d_dE, D.d_Hf, D.d_dHf, d_out,
                         d_rt2, cur_st());
   }

@@ -833,7 +833,7 @@ __global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo
 __global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPart, int nwv,
                                                  const double *__restrict__ ghDF, const int *__restrict__ ghStart,
                                                  const double *__restrict__ selfDF, int nrows, int open_domain,
-                                                 cd *__restrict__ H, size_t ld, double *__restrict__ dF)
+                                                 cd *__restrict__ H, size_t ld, double *__restrict__ dF, int r0)
 {
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -851,7 +851,7 @@ __global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPa
     // the self element last (a warped quadrangle's F terms)
     f += selfDF[(size_t)row * 3]; f0 += selfDF[(size_t)row * 3 + 1]; f1 += selfDF[(size_t)row * 3 + 2];
     double diag = open_domain ? 1.0 + f : f;
-    cd *h = H + (size_t)row * ld + row;
+    cd *h = H + (size_t)row * ld + row + r0;   // the row's own column (rows r0.. of the domain)
     h->x += diag;
     dF[row * 3 + 0] = diag; dF[row * 3 + 1] = f0; dF[row * 3 + 2] = f1;
   }
@@ -1001,10 +1001,10 @@ void bem_launch_elem_self(const void *geo, const void *gq, const int *s, const d
                      idx, n, kr, ki, xl, wl, xh, wh, (cd *)tmp, tmpF, (cd *)out);
 }
 void bem_launch_asm_fin(const double *dfPart, int nwv, const double *ghDF, const int *ghStart, const double *selfDF,
-                        int nrows, int open_domain, void *H, size_t ld, double *dF, hipStream_t st)
+                        int nrows, int open_domain, void *H, size_t ld, double *dF, int r0, hipStream_t st)
 {
   int nb = (nrows + 3) / 4;
   hipLaunchKernelGGL(k_asm_fin, dim3(nb), dim3(256), 0, st, dfPart, nwv, ghDF, ghStart, selfDF, nrows, open_domain,
-                     (cd *)H, ld, dF);
+                     (cd *)H, ld, dF, r0);
 }
 }

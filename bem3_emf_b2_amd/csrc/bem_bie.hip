@@ -48,13 +48,15 @@ __global__ void k_build_A(const cd *__restrict__ G, const cd *__restrict__ H, in
 
 // B rows of domain d: only columns whose element has this domain as its
 // sub side and the open domain as its main side carry incident data
-__global__ void __launch_bounds__(256) k_build_B(const cd *__restrict__ G, const cd *__restrict__ H, int nd, size_t ld,
-                                                 const ColInfo *__restrict__ col, const cd *__restrict__ Ui,
+// (nrows rows of the domain's nd, from the row at rowBase: a row block of
+// the matrices)
+__global__ void __launch_bounds__(256) k_build_B(const cd *__restrict__ G, const cd *__restrict__ H, int nrows, int nd,
+                                                 size_t ld, const ColInfo *__restrict__ col, const cd *__restrict__ Ui,
                                                  const cd *__restrict__ dUi, size_t rowBase, cd *__restrict__ B)
 {
   const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (r >= nd) return;
+  if (r >= nrows) return;
   cd b[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   for (int j = lane; j < nd; j += 64) {
     const ColInfo c = col[j];
@@ -102,7 +104,9 @@ __global__ void k_store_X(const cd *__restrict__ X, int nd, const ColInfo *__res
 }
 
 
-__global__ void __launch_bounds__(256) k_eh_rows(const cd *__restrict__ dG, const cd *__restrict__ dH, int nd,
+// (nrows rows of a row block: dG, dH, dF and rows start at the block's
+// first row; nd columns)
+__global__ void __launch_bounds__(256) k_eh_rows(const cd *__restrict__ dG, const cd *__restrict__ dH, int nrows, int nd,
                                                  size_t ld, const double *__restrict__ dF,
                                                  const RowEH *__restrict__ rows, const int *__restrict__ colS,
                                                  const ColInfo *__restrict__ col, const cd *__restrict__ U,
@@ -111,7 +115,7 @@ __global__ void __launch_bounds__(256) k_eh_rows(const cd *__restrict__ dG, cons
 {
   const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (r >= nd) return;
+  if (r >= nrows) return;
   cd dz[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)}, de[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   const cd *gz = dG + (size_t)r * 2 * ld, *ge = gz + ld, *hz = dH + (size_t)r * 2 * ld, *he = hz + ld;
   for (int j = lane; j < nd; j += 64) {
@@ -276,10 +280,10 @@ void bem_launch_build_A(const void *G, const void *H, int nd, size_t ld, const v
   hipLaunchKernelGGL(k_build_A, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const cd *)G, (const cd *)H, nd, ld,
                      (const ColInfo *)col, rowBase, NN, (cd *)A, na);
 }
-void bem_launch_build_B(const void *G, const void *H, int nd, size_t ld, const void *col, const void *Ui,
+void bem_launch_build_B(const void *G, const void *H, int nrows, int nd, size_t ld, const void *col, const void *Ui,
                         const void *dUi, size_t rowBase, void *B, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_build_B, dim3((nd + 3) / 4), dim3(256), 0, st, (const cd *)G, (const cd *)H, nd, ld,
+  hipLaunchKernelGGL(k_build_B, dim3((nrows + 3) / 4), dim3(256), 0, st, (const cd *)G, (const cd *)H, nrows, nd, ld,
                      (const ColInfo *)col, (const cd *)Ui, (const cd *)dUi, rowBase, (cd *)B);
 }
 void bem_launch_store_X(const void *X, int nd, const void *col, size_t NN, const int *colS, const void *Ui,
@@ -288,11 +292,11 @@ void bem_launch_store_X(const void *X, int nd, const void *col, size_t NN, const
   hipLaunchKernelGGL(k_store_X, dim3((nd + 255) / 256), dim3(256), 0, st, (const cd *)X, nd, (const ColInfo *)col, NN,
                      colS, (const cd *)Ui, (const cd *)dUi, (cd *)U, (cd *)dU);
 }
-void bem_launch_eh_rows(const void *dG, const void *dH, int nd, size_t ld, const double *dF, const void *rows,
+void bem_launch_eh_rows(const void *dG, const void *dH, int nrows, int nd, size_t ld, const double *dF, const void *rows,
                         const int *colS, const void *col, const void *U, const void *dU, double chr, double chi,
                         void *E, void *H, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_eh_rows, dim3((nd + 3) / 4), dim3(256), 0, st, (const cd *)dG, (const cd *)dH, nd, ld, dF,
+  hipLaunchKernelGGL(k_eh_rows, dim3((nrows + 3) / 4), dim3(256), 0, st, (const cd *)dG, (const cd *)dH, nrows, nd, ld, dF,
                      (const RowEH *)rows, colS, (const ColInfo *)col, (const cd *)U, (const cd *)dU, mk(chr, chi),
                      (cd *)E, (cd *)H);
 }

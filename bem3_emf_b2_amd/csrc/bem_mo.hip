@@ -700,7 +700,7 @@ int make_plan(bh_object **objs, int N, Plan &P)
       const bh_object *s = objs[n];
       const SrcGeo &G = P.geo[n];
       const zd k = s->kn[0];
-      const bool eligible = g_coupling_mode == 0 && s->prec == BH_PREC_MIXED && objs[m]->prec == BH_PREC_MIXED &&
+      const bool eligible = g_coupling_mode == 0 && s->prec != BH_PREC_FP64 && objs[m]->prec != BH_PREC_FP64 &&
                             G.ok && k.imag() == 0.0 && k.real() > 0.0 && G.rad > 0.0;
       if (!eligible) continue;
       double dmin = 1e300;
@@ -849,7 +849,7 @@ int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double
   Dom &S = src->dom[0];
   bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, src->d_gq, S.d_sid, S.dNe,
                      src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
-                     g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], src->prec == BH_PREC_MIXED ? g.d_gl32 : nullptr,
+                     g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], src->prec != BH_PREC_FP64 ? g.d_gl32 : nullptr,
                      S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, src->nquad > 0, d_part, g.st);
   HIPCHK(hipGetLastError());
   return 0;

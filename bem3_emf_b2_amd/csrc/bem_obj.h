@@ -177,6 +177,12 @@ struct bh_object {
   // grow-only device work areas of the entry points (freed with the object)
   bem::Scratch ws_part, ws_mask, ws_pairs, ws_start, ws_ghdf, ws_self;   // assembly
   bem::Scratch ws_wt;                  // forming S
+  // low-memory mixed form: FP64 row blocks of G, H, dG, dH, dF assembled on
+  // demand; lm_raw: the FP32 storage (c_G0f: G0, c_Z: H0, c_G1: G1, c_Mf: H1)
+  // holds the assembled matrices, not yet the factors
+  bem::Scratch lm_blk[5];
+  bool lm_raw = false;
+  int lm_rows = 0;                     // rows per block (0: by the memory budget)
   bem::Scratch ws_eh[5];               // boundary E, H: packed rows, products, refinement
   // many beams as right-hand sides (bh_mo_solve_beams): nbm plane waves;
   // beam b of each array at + b x (its one-beam size)

@@ -47,6 +47,12 @@ int schur_apply(const SchurF<cf> &f, const cf *rhs, bool r0nz, cf *X, cf *work, 
 int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, const cd *H1, cd beps, cf *G0f, cf *Zm,
                    cf *Mf, cf *Sf, cf *G1c, int *piv0, int *pivM, int *info, hipStream_t st);
 
+// the same factors for the low-memory form, in place over the complex-FP32
+// matrices: G0 -> LU(G0), H0 -> Z, H1 -> LU(M); P (NN^2 values) receives
+// Sp; G1 is kept.  Returns 0 or the code of the LU.
+int schur_factor_c_inplace(int n, size_t ld, cf *G0, cf *H0, cf *G1, cf *H1, cd beps, cf *P, int *piv0, int *pivM,
+                           int *info, hipStream_t st);
+
 // S from Sp in place and its LU: W = M^{-1} G1 (Wt: NN^2 scratch),
 // S = Sp - sigma delta G1 (W o n n^T), then LU(S) into Sf / pivS
 int schur_form_s(const SchurF<cd> &f, cd *Sf, int *pivS, cd *Wt, int *info, hipStream_t st);
@@ -56,6 +62,12 @@ int schur_form_s(const SchurF<cf> &f, cf *Sf, int *pivS, cf *Wt, int *info, hipS
 // the domain's FP64 G, H and its column classes; work holds 12 nd values
 void bem_resid_domain(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
                       size_t rowBase, const cd *B, cd *R, cd *work, hipStream_t st);
+// the same in two parts, for row blocks of G, H: the column vectors of X
+// (work[0, 8 nd)), then rows r0 .. r0+nrows from the block (work[8 nd,
+// 8 nd + 4 nrows))
+void bem_resid_colvec(const ColInfo *col, int nd, const cd *X, size_t NN, cd *work, hipStream_t st);
+void bem_resid_rows(const cd *G, const cd *H, int nrows, int nd, size_t ld, int r0, size_t rowBase, const cd *B, cd *R,
+                    cd *work, hipStream_t st);
 
 // Euclidean norm of n complex values (synchronises the stream); tmp holds
 // 256 doubles of device memory

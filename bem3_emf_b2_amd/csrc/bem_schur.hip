@@ -647,6 +647,28 @@ int schur_factor_c(int n, size_t ld, const cd *G0, const cd *H0, const cd *G1, c
   return bem_cgetrf(n, Mf, ld, pivM, info, st);
 }
 
+// y = x + a y in complex FP32
+__global__ void k_cxpay(const cf *__restrict__ x, cf a, cf *__restrict__ y, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] + a * y[i];
+}
+
+int schur_factor_c_inplace(int n, size_t ld, cf *G0, cf *H0, cf *G1, cf *H1, cd beps, cf *P, int *piv0, int *pivM,
+                           int *info, hipStream_t st)
+{
+  const size_t n2 = (size_t)n * ld;
+  int rc;
+  if ((rc = bem_cgetrf(n, G0, ld, piv0, info, st))) return rc;   // G0 -> LU(G0)
+  bem_cgetrs(n, n, G0, ld, piv0, H0, ld, st);                      // H0 -> Z
+  bem_cgemm(n, n, n, 1.0, 0.0, G1, ld, H0, ld, 0.0, 0.0, P, ld, st);   // P = G1 Z
+  hipLaunchKernelGGL(k_cxpay, grid1(n2), dim3(256), 0, st, (const cf *)P, mkf(1.0f, 0.0f), H1, n2);   // H1 -> M
+  // P -> Sp = M + (beta - 1) P = H1 + beta P
+  hipLaunchKernelGGL(k_cxpay, grid1(n2), dim3(256), 0, st, (const cf *)H1, mkf((float)beps.x - 1.0f, (float)beps.y), P,
+                     n2);
+  return bem_cgetrf(n, H1, ld, pivM, info, st);                    // M -> LU(M)
+}
+
 template <class Zt> static int form_s_t(const SchurF<Zt> &f, Zt *Sf, int *pivS, Zt *Wt, int *info, hipStream_t st)
 {
   const int n = f.n;
@@ -699,15 +721,39 @@ __global__ void k_resid(const cd *__restrict__ B, const cd *__restrict__ Y, int 
   R[i] = B[i] - Y[4 * r + c];
 }
 
-void bem_resid_domain(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
-                      size_t rowBase, const cd *B, cd *R, cd *w, hipStream_t st)
+// rows r0 .. r0+nrows of the domain: R[c nd + r] = B[c nd + r] - Y[r - r0][c]
+__global__ void k_resid_rows(const cd *__restrict__ B, const cd *__restrict__ Y, int nd, int r0, int nrows,
+                             cd *__restrict__ R)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * nrows) return;
+  int c = i / nrows, r = i % nrows;
+  const size_t o = (size_t)c * nd + r0 + r;
+  R[o] = B[o] - Y[4 * r + c];
+}
+
+void bem_resid_colvec(const ColInfo *col, int nd, const cd *X, size_t NN, cd *w, hipStream_t st)
+{
+  const size_t N = (size_t)nd;
+  hipLaunchKernelGGL(k_colvec, grid1(N), dim3(256), 0, st, X, col, nd, NN, w, w + 4 * N);
+}
+
+void bem_resid_rows(const cd *G, const cd *H, int nrows, int nd, size_t ld, int r0, size_t rowBase, const cd *B, cd *R,
+                    cd *w, hipStream_t st)
 {
   const size_t N = (size_t)nd;
   cd *a = w, *b = w + 4 * N, *Y = w + 8 * N;
-  hipLaunchKernelGGL(k_colvec, grid1(N), dim3(256), 0, st, X, col, nd, NN, a, b);
-  bem_zgemm(nd, 4, nd, 1.0, 0.0, H, ld, a, 4, 0.0, 0.0, Y, 4, st);
-  bem_zgemm(nd, 4, nd, 1.0, 0.0, G, ld, b, 4, 1.0, 0.0, Y, 4, st);
-  hipLaunchKernelGGL(k_resid, grid1(4 * N), dim3(256), 0, st, B + rowBase, Y, nd, R + rowBase);
+  bem_zgemm(nrows, 4, nd, 1.0, 0.0, H, ld, a, 4, 0.0, 0.0, Y, 4, st);
+  bem_zgemm(nrows, 4, nd, 1.0, 0.0, G, ld, b, 4, 1.0, 0.0, Y, 4, st);
+  hipLaunchKernelGGL(k_resid_rows, grid1(4 * (size_t)nrows), dim3(256), 0, st, B + rowBase, Y, nd, r0, nrows,
+                     R + rowBase);
+}
+
+void bem_resid_domain(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
+                      size_t rowBase, const cd *B, cd *R, cd *w, hipStream_t st)
+{
+  bem_resid_colvec(col, nd, X, NN, w, st);
+  bem_resid_rows(G, H, nd, nd, ld, 0, rowBase, B, R, w, st);
 }
 
 // sum of |x|^2: 256 workgroup partials in a fixed order, then one workgroup

@@ -70,7 +70,19 @@ int bh_object_assemble(bh_object *o, int type);
  * elimination refined in FP64 (mode 3).                                   */
 #define BH_PREC_FP64  0
 #define BH_PREC_MIXED 1
+/* BH_PREC_MIXED_LM: the mixed form for objects whose FP64 matrices do not
+ * fit the device (one interior medium only): G, H are assembled straight
+ * into complex-FP32 storage that the factorisation overwrites in place, and
+ * every FP64 product with them (right-hand side, refinement residual,
+ * boundary E/H rows) assembles their rows again, a block of rows at a time.
+ * Results are those of BH_PREC_MIXED; bh_object_matrix and the many-beam
+ * solve are not available.                                                 */
+#define BH_PREC_MIXED_LM 2
 int bh_object_set_precision(bh_object *o, int prec);
+/* rows per block of BH_PREC_MIXED_LM's on-demand assembly (0, the default:
+ * as many as about 16 GB of FP64 rows hold); no counterpart in the
+ * reference                                                               */
+int bh_object_set_block_rows(bh_object *o, int rows);
 int bh_object_gh_count(const bh_object *o, int did);
 /* download: which 0 G, 1 H (nd x nd), 2 dG, 3 dH (nd x 2 x nd complex),
  * 4 dF (nd x 3 doubles: diagonal 1+F or F, dF/dt_zeta, dF/dt_eta).
