@@ -1299,6 +1299,7 @@ extern "C" int bh_object_factor(bh_object *o, int mode)
 static int dom_build_B(bh_object *o, int d)
 {
   Dom &D = o->dom[d];
+  if (!D.has_inc) return 0;   // no incident data in these rows: they stay zero
   const size_t rb = row_base(o, d);
   return domain_blocks(o, d, [&](int r0, int n, const cd *G, const cd *H, const cd *, const cd *, const double *) {
     bem_launch_build_B(G, H, n, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, rb + r0, o->d_B, cur_st());
