@@ -18,9 +18,10 @@ residual).  The same line carries the plain FP64 form on the same scene
 200 points outside the objects, force and torque of every object), the
 dominant kernel's roofline, a CPU baseline of the reference algorithm on the
 host cores (run while the FP64 form occupies the GPU), the single
-level-4 sphere (5120 triangles) in both forms, and four objects under 64
+level-4 sphere (5120 triangles) in both forms, four objects under 64
 plane waves solved together as right-hand sides (force and torque per
-beam, far-field intensity grid per beam).
+beam, far-field intensity grid per beam), and the level-5 sphere (20480
+triangles) in the low-memory mixed form.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -259,6 +260,52 @@ def l4_extra(bem, npts=1000):
                                              for a, b in zip(FNmx, FN64))}}
 
 
+def l5_extra(bem):
+    """config 3: the level-5 sphere (20480 triangles, NN 61440) in the
+    low-memory mixed form, one configuration end to end: assembly,
+    factorisation, solve with boundary E/H, E/H at 1000 points (7-point
+    rule) and force/torque (rule 0)"""
+    import numpy as np
+
+    from bem3_emf_b2_amd import meshgen as mg
+    V, F = mg.icosphere(5, 0.5)
+    rn, ed, md, sd, gd = mg.paired_arrays(V, F)
+    beams = mg.xpol_plane_wave(lambda0=1.064, n0=1.33, E0=1.0)
+    pts = np.random.default_rng(20240601).uniform(-2.0, 2.0, size=(1000, 3))
+    g = bem.Object(rn, ed, md, sd, [beams[0][1], 1.59], beams[0][0])
+    g.set_precision("mixed_lm")
+    g.set_beams(beams)
+    bem.sync()
+    t = [time.perf_counter()]
+    g.assemble(2)
+    bem.sync()
+    t.append(time.perf_counter())
+    g.factor(0)
+    bem.sync()
+    t.append(time.perf_counter())
+    bem.mo_solve([g])
+    bem.sync()
+    t.append(time.perf_counter())
+    bem.mo_field([g], pts, "EH", "t", 1)
+    bem.mo_force([g], 0, 0)
+    bem.sync()
+    t.append(time.perf_counter())
+    steps, res = g.solve_stats()
+    g.close()
+    ph = [t[i + 1] - t[i] for i in range(4)]
+    log("L5 sphere, low-memory mixed form: %.1f s (assembly %.1f, factorisation %.1f, solve + boundary E/H %.1f, "
+        "fields + force %.1f; %d refinement steps, residual %.1e)" % (t[-1] - t[0], *ph, steps, res))
+    nn2 = 61440.0 ** 2
+    return {"workload": "sphere r=0.5um n=1.59 in water, icosphere L5 (20480 triangles, NN 61440), x-pol plane wave "
+                        "1.064um, low-memory mixed form: assembly + factor + solve + boundary E/H + E/H at 1000 points + "
+                        "force/torque",
+            "s_total": t[-1] - t[0],
+            "phases_s": {"assembly": ph[0], "factorisation": ph[1], "solve_boundary_EH": ph[2], "fields_force": ph[3]},
+            "refinement_steps": steps, "fp64_relative_residual": res,
+            "device_bytes_matrices": {"fp32_storage_and_factors": 6 * nn2 * 8, "fp64_row_blocks": 16 * 2 ** 30,
+                                      "fp64_matrices_not_stored": 12 * nn2 * 16}}
+
+
 def beams_extra(bem, nb=64):
     """four objects under nb plane waves as right-hand sides (seeded random
     directions and polarisation angles), end to end: assembly and
@@ -361,6 +408,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparator (profiling runs)")
     ap.add_argument("--no-l4", action="store_true", help="skip the single L4 sphere")
     ap.add_argument("--no-beams", action="store_true", help="skip the 64-beam workload")
+    ap.add_argument("--no-l5", action="store_true", help="skip the level-5 sphere")
     ap.add_argument("--lanes", type=int, default=0, help="device streams of the per-object work (0: default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 1 GiB (scale of the HBM byte counters)")
@@ -441,6 +489,7 @@ def main():
     cpu = finish_cpu_baseline(cpu_proc) if cpu_proc else None
     l4 = None if args.no_l4 else l4_extra(bem)
     beams64 = None if args.no_beams else beams_extra(bem)
+    l5 = None if args.no_l5 else l5_extra(bem)
 
     out = {
         "metric": METRIC,
@@ -475,6 +524,7 @@ def main():
         "speedup_vs_cpu": (cpu["value"] / s_step) if cpu else None,
         "l4_sphere": l4,
         "beams64": beams64,
+        "l5_sphere": l5,
     }
     print(json.dumps(out), flush=True)
     bem.finalize()
