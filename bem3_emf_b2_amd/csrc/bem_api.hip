@@ -560,6 +560,7 @@ int bem::object_assemble(bh_object *o, int type)
   // every factorisation of earlier matrices is stale from here on
   o->fmode = 0;
   o->x_warm = false;
+  o->r_valid = false;
   o->mb_xwarm = false;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   o->asm_prec = -1;
@@ -593,6 +594,7 @@ extern "C" int bh_object_set_precision(bh_object *o, int prec)
   o->prec = prec;
   o->fmode = 0;
   o->x_warm = false;
+  o->r_valid = false;
   o->mb_xwarm = false;
   o->s_formed = false;
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
@@ -1214,6 +1216,7 @@ int bem::object_factor(bh_object *o, int mode)
   if (mode >= 2 && o->MN != 1) return fail(BH_EUNSUP, "the block elimination needs exactly one interior medium");
   o->fmode = 0;
   o->x_warm = false;
+  o->r_valid = false;
   if (!o->d_info) { HIPCHK(hipMalloc((void **)&o->d_info, sizeof(int))); HIPCHK(hipMalloc((void **)&o->d_work, 256 * sizeof(cd))); }
   if (!o->d_X) { if (dalloc(&o->d_X, na) || dalloc(&o->d_B, na) || dalloc(&o->d_tmp, 8 * NN)) return BH_EHIP; }
   int info = 0;
@@ -1323,6 +1326,7 @@ static int object_solve_core(bh_object *o)
   if (!o->fmode) return fail(BH_ESTATE, "factor before solve");
   if (!o->d_beams) return fail(BH_ESTATE, "set beams before solve");
   size_t NN = (size_t)o->NN, na = 8 * NN;
+  o->r_valid = false;   // B changes here; R is not kept by this path
   HIPCHK(hipMemsetAsync(o->d_B, 0, na * sizeof(cd), cur_st()));
   for (int d = 0; d <= o->MN; d++) {
     int rc = dom_build_B(o, d);
@@ -1357,6 +1361,7 @@ static int object_solve_core(bh_object *o)
       HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, cur_st()));
     }
     o->x_warm = false;
+    o->r_valid = false;
     cf *cw = o->c_w, *crhs = cw + schur_work(NN), *cdx = crhs + 8 * NN;
     o->gm_steps = 0;
     const SchurF<cf> f = schur32(o);
@@ -1780,6 +1785,7 @@ int bem::object_select_beam(bh_object *o, int b)
   if (!o->d_beams && upload(&o->d_beams, &o->beams[0], 1)) return BH_EHIP;
   HIPCHK(hipMemcpyAsync(o->d_beams, &o->beams[0], sizeof(Beam), hipMemcpyHostToDevice, cur_st()));
   o->x_warm = false;   // the one-beam solve's X belongs to another right-hand side
+  o->r_valid = false;
   HIPCHK(hipStreamSynchronize(cur_st()));   // the host record stays alive until here
   return 0;
 }
@@ -1848,6 +1854,115 @@ template <class T> static int cput(int i, T **d, const T *h, size_t n)
   return 0;
 }
 #define CWS_CHECK(p) do { if (!(p)) return fail(BH_EHIP, "device work area"); } while (0)
+
+// The coupling loop's solves of N objects of the mixed-precision form (S
+// formed, matrices stored) in lock step on the library's stream: the
+// right-hand sides of all objects, then refinement steps of every object
+// whose relative residual is above IR_TOL, each step one schur_apply_batch
+// (the triangular solves of all objects together, one launch per factor
+// and triangle) and the objects' FP64 residuals, with one read-back of all
+// norms per step.  A repeated solve starts from the previous solution with
+// the residual updated by the change of the right-hand side (R = B - A X
+// becomes R - B_old + B_new: no matrix pass for the warm start).  Returns
+// BH_EUNSUP (nothing done) if an object does not qualify.
+int bem::objects_solve_batch(bh_object **objs, int N)
+{
+  if (N <= 0) return 0;
+  for (int m = 0; m < N; m++) {
+    const bh_object *o = objs[m];
+    if (o->fmode != 3 || !o->s_formed || o->prec == BH_PREC_MIXED_LM || o->MN != 1 || !o->d_beams)
+      return BH_EUNSUP;
+  }
+  hipStream_t st = cur_st();
+  double *dn = cws<double>(22, 2 * (size_t)N);
+  CWS_CHECK(dn);
+  std::vector<double> hn(2 * (size_t)N), bn(N), res(N);
+  // right-hand sides, and the residuals of the starting solutions
+  for (int m = 0; m < N; m++) {
+    bh_object *o = objs[m];
+    const size_t na = 8 * (size_t)o->NN;
+    if (o->r_valid) bem_launch_axpby(o->d_R, -1.0, 0.0, o->d_B, o->d_R, na, st);   // R = -A X
+    HIPCHK(hipMemsetAsync(o->d_B, 0, na * sizeof(cd), st));
+    for (int d = 0; d <= o->MN; d++) {
+      int rc = dom_build_B(o, d);
+      if (rc) return rc;
+    }
+    if (o->r_valid) {
+      bem_launch_axpby(o->d_R, 1.0, 0.0, o->d_B, o->d_R, na, st);
+    } else {
+      HIPCHK(hipMemsetAsync(o->d_X, 0, na * sizeof(cd), st));
+      HIPCHK(hipMemcpyAsync(o->d_R, o->d_B, na * sizeof(cd), hipMemcpyDeviceToDevice, st));
+    }
+    o->r_valid = false;
+    o->x_warm = false;
+    o->ir_iters = 0;
+    o->gm_steps = 0;
+    bem_nrm2sq_to(o->d_B, na, o->d_nrm, dn + 2 * m, st);
+    bem_nrm2sq_to(o->d_R, na, o->d_nrm, dn + 2 * m + 1, st);
+  }
+  HIPCHK(hipMemcpyAsync(hn.data(), dn, hn.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<int> act;
+  for (int m = 0; m < N; m++) {
+    bn[m] = sqrt(hn[2 * m]);
+    res[m] = bn[m] > 0.0 ? sqrt(hn[2 * m + 1]) / bn[m] : 0.0;
+    if (res[m] > IR_TOL) act.push_back(m);
+  }
+  std::vector<SchurF<cf>> f;
+  std::vector<const cf *> rhs;
+  std::vector<cf *> dx, w;
+  for (int it = 0; !act.empty() && it < IR_MAX; it++) {
+    const int na_ = (int)act.size();
+    f.resize(na_); rhs.resize(na_); dx.resize(na_); w.resize(na_);
+    for (int q = 0; q < na_; q++) {
+      bh_object *o = objs[act[q]];
+      const size_t NN = (size_t)o->NN;
+      cf *cw = o->c_w, *crhs = cw + schur_work(NN), *cdx = crhs + 8 * NN;
+      bem_cvt_z2c(o->d_R, crhs, 8 * NN, st);
+      f[q] = schur32(o);
+      rhs[q] = crhs; dx[q] = cdx; w[q] = cw;
+    }
+    int rc = schur_apply_batch(na_, f.data(), rhs.data(), dx.data(), w.data(), st);
+    if (rc) return fail(rc, "batched block solve of %d objects", na_);
+    for (int q = 0; q < na_; q++) {
+      const int m = act[q];
+      bh_object *o = objs[m];
+      const size_t na = 8 * (size_t)o->NN;
+      bem_acc_c2z(o->d_X, dx[q], na, st);
+      for (int d = 0; d <= o->MN; d++) {
+        rc = dom_resid(o, d);
+        if (rc) return rc;
+      }
+      bem_nrm2sq_to(o->d_R, na, o->d_nrm, dn + 2 * m + 1, st);
+      o->ir_iters++;
+    }
+    HIPCHK(hipMemcpyAsync(hn.data(), dn, hn.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int> next;
+    for (int m : act) {
+      res[m] = sqrt(hn[2 * m + 1]) / bn[m];
+      if (res[m] > IR_TOL) next.push_back(m);
+    }
+    act.swap(next);
+  }
+  for (int m = 0; m < N; m++) {
+    bh_object *o = objs[m];
+    o->ir_res = res[m];
+    if (!(res[m] <= IR_TOL))
+      return fail(BH_ECONV, "mixed-precision refinement (object %d of %d): relative residual %.3g after %d steps", m, N,
+                  res[m], o->ir_iters);
+    const size_t NN = (size_t)o->NN;
+    for (int d = 0; d <= o->MN; d++) {
+      Dom &D = o->dom[d];
+      bem_launch_store_X(o->d_X, D.nd, D.d_col, NN, D.d_colS, o->d_Ui, o->d_dUi, D.d_U, D.d_dU, st);
+    }
+    o->x_warm = true;
+    o->r_valid = true;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return trsv_failed(st);
+}
 
 // fields at points (mo_EH_s/t/i, mo_mEMP_s/t/i; mo_field.c:11-282):
 // what 0 E,H (6 complex per point), 1 U (4); mode 0 s, 1 t, 2 i;

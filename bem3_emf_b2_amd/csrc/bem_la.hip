@@ -17,6 +17,7 @@
 #include "bem_prof.h"
 
 #include <algorithm>
+#include <vector>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -880,6 +881,9 @@ struct LaLane {
   void *coop_buf;        // exchange area of the one-launch panel (uncached)
   void *trsv_buf;        // exchange area of the one-launch triangular solves (uncached)
   int trsv_epoch;
+  void *trsvb_buf;       // exchange area of the batched one-launch solves (uncached)
+  int trsvb_epoch;
+  unsigned trsvb_ticket; // the batch area's ticket counter after the launches enqueued so far
 };
 static LaLane g_lanes[LA_NLANES];
 static std::mutex g_lanes_mu;
@@ -983,16 +987,18 @@ __device__ inline void trsv_invert(Z *D, int lane)
   }
 }
 
+// block b of the one-launch solve of one system (w: its exchange rows and
+// flags)
 template <class R, bool UPPER, int NR>
-__global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__restrict__ A, size_t lda, int n,
-                                                 typename Cx<R>::Z *B, size_t ldb, TrsvWs w, int epoch)
+__device__ __forceinline__ void trsv_body(const typename Cx<R>::Z *__restrict__ A, size_t lda, int n,
+                                          typename Cx<R>::Z *B, size_t ldb, TrsvWs w, int epoch, int b)
 {
   typedef typename Cx<R>::Z Z;
   extern __shared__ double dyn_lds[];
   Z *Dv = (Z *)dyn_lds;                 // [64][65] the diagonal block, then its inverse
   Z *Wn = Dv + 64 * 65;                 // [64][65] the neighbour block, then W = D^{-1} N
   Z *Xw = Wn + 64 * 65;                 // [4][64][TRSV_MAXR] x block, then partial sums, per wave
-  const int nb = (n + 63) / 64, b = blockIdx.x;
+  const int nb = (n + 63) / 64;
   const int i = UPPER ? nb - 1 - b : b;
   const int r0 = 64 * i, rows = min(64, n - r0);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tid = threadIdx.x;
@@ -1093,6 +1099,117 @@ __global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__rest
   }
 }
 
+template <class R, bool UPPER, int NR>
+__global__ void __launch_bounds__(256) k_trsv_sf(const typename Cx<R>::Z *__restrict__ A, size_t lda, int n,
+                                                 typename Cx<R>::Z *B, size_t ldb, TrsvWs w, int epoch)
+{
+  trsv_body<R, UPPER, NR>(A, lda, n, B, ldb, w, epoch, blockIdx.x);
+}
+
+// ---- the one-launch solves of several systems together ----
+// N systems (at most TRSVB_MAXSYS, sorted by block count, largest first)
+// in one launch: the blocks of all systems interleaved round by round
+// (round r: block r of every system that has more than r blocks), so every
+// block comes after the blocks it depends on and all chains advance at
+// once.  A workgroup takes its position from a ticket counter when it
+// starts, not from its workgroup id: a block then waits only for blocks
+// whose workgroups are already running or done, whatever order the
+// hardware dispatches workgroups in, so the launch needs no co-residency.
+// The host passes the counter's value before the launch (base).  System s
+// exchanges its x rows at xoff and its flags at foff of the batch area.
+#define TRSVB_MAXSYS 32
+#define TRSVB_XMAX 262144      // exchange rows of all systems of a launch
+#define TRSVB_NBMAX 8192       // blocks of all systems of a launch
+template <class Z> struct TrsvbSys {
+  const Z *A;
+  Z *B;
+  size_t lda, ldb;
+  int n, nb, xoff, foff;
+};
+template <class Z> struct TrsvbArgs {
+  TrsvbSys<Z> s[TRSVB_MAXSYS];
+  int nsys;
+  unsigned base;
+};
+
+template <class R, bool UPPER, int NR>
+__global__ void __launch_bounds__(256) k_trsv_batch(TrsvbArgs<typename Cx<R>::Z> a, void *xbase, int *flags, int *err,
+                                                    unsigned *ticket, int epoch)
+{
+  typedef typename Cx<R>::Z Z;
+  __shared__ int s_pos;
+  if (threadIdx.x == 0) s_pos = (int)(atomicAdd(ticket, 1u) - a.base);
+  __syncthreads();
+  const int p = s_pos;
+  // the round r of position p: P(r) <= p < P(r + 1), P(r) = sum_s min(nb_s, r)
+  // (the argument table is read at constant indices only)
+  int lo = 0, hi = a.s[0].nb;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    int P = 0;
+#pragma unroll
+    for (int q = 0; q < TRSVB_MAXSYS; q++)
+      if (q < a.nsys) P += min(a.s[q].nb, mid);
+    if (P <= p) lo = mid; else hi = mid;
+  }
+  int P = 0, cnt = 0;
+#pragma unroll
+  for (int q = 0; q < TRSVB_MAXSYS; q++)
+    if (q < a.nsys) {
+      P += min(a.s[q].nb, lo);
+      cnt += a.s[q].nb > lo ? 1 : 0;
+    }
+  const int sy = p - P;   // the systems with more than lo blocks are the first ones
+  if (p < 0 || sy >= cnt) {   // a ticket outside this launch: the counter is off
+    if (threadIdx.x == 0) atomicExch(err, 1);
+    return;
+  }
+  TrsvbSys<Z> y = a.s[0];
+#pragma unroll
+  for (int q = 1; q < TRSVB_MAXSYS; q++)
+    if (q == sy) y = a.s[q];
+  TrsvWs w;
+  w.x = (void *)((Z *)xbase + (size_t)y.xoff * TRSV_MAXR);
+  w.flag = flags + y.foff;
+  w.err = err;
+  trsv_body<R, UPPER, NR>(y.A, y.lda, y.n, y.B, y.ldb, w, epoch, lo);
+}
+
+// row interchanges piv[0..n) of the first nc columns of N right-hand sides
+// (workgroup s: system s), in order, one thread per column
+template <class Z> struct LaswpSys {
+  Z *B;
+  const int *piv;
+  size_t ldb;
+  int n, pad;
+};
+template <class Z> struct LaswpArgs {
+  LaswpSys<Z> s[TRSVB_MAXSYS];
+};
+template <class Z> __global__ void __launch_bounds__(64) k_laswp_batch(LaswpArgs<Z> a, int nc)
+{
+  LaswpSys<Z> q = a.s[0];
+#pragma unroll
+  for (int k = 1; k < TRSVB_MAXSYS; k++)
+    if ((int)blockIdx.x == k) q = a.s[k];
+  const int c = threadIdx.x;
+  __shared__ int sp[256];
+  for (int j0 = 0; j0 < q.n; j0 += 256) {
+    const int len = min(256, q.n - j0);
+    for (int t = threadIdx.x; t < len; t += 64) sp[t] = q.piv[j0 + t];
+    __syncthreads();
+    if (c < nc)
+      for (int t = 0; t < len; t++) {
+        const int j = j0 + t, pv = sp[t];
+        if (pv == j) continue;
+        const Z v = q.B[(size_t)j * q.ldb + c];
+        q.B[(size_t)j * q.ldb + c] = q.B[(size_t)pv * q.ldb + c];
+        q.B[(size_t)pv * q.ldb + c] = v;
+      }
+    __syncthreads();
+  }
+}
+
 __global__ void k_zero_bytes(unsigned int *p, size_t n)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1121,8 +1238,13 @@ template <class R> static int la_init_t()
                        (const void *)k_trsv_sf<R, false, 3>, (const void *)k_trsv_sf<R, false, 4>,
                        (const void *)k_trsv_sf<R, true, 1>,  (const void *)k_trsv_sf<R, true, 2>,
                        (const void *)k_trsv_sf<R, true, 3>,  (const void *)k_trsv_sf<R, true, 4>};
+  const void *sb[8] = {(const void *)k_trsv_batch<R, false, 1>, (const void *)k_trsv_batch<R, false, 2>,
+                       (const void *)k_trsv_batch<R, false, 3>, (const void *)k_trsv_batch<R, false, 4>,
+                       (const void *)k_trsv_batch<R, true, 1>,  (const void *)k_trsv_batch<R, true, 2>,
+                       (const void *)k_trsv_batch<R, true, 3>,  (const void *)k_trsv_batch<R, true, 4>};
   for (int q = 0; q < 8; q++)
-    if (hipFuncSetAttribute(sf[q], hipFuncAttributeMaxDynamicSharedMemorySize, trsv_lds_bytes<R>()) != hipSuccess) {
+    if (hipFuncSetAttribute(sf[q], hipFuncAttributeMaxDynamicSharedMemorySize, trsv_lds_bytes<R>()) != hipSuccess ||
+        hipFuncSetAttribute(sb[q], hipFuncAttributeMaxDynamicSharedMemorySize, trsv_lds_bytes<R>()) != hipSuccess) {
       (void)hipGetLastError();
       g_trsv_ok = false;
     }
@@ -1352,6 +1474,21 @@ template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, i
 #define TRSV_XBYTES ((size_t)TRSV_XMAX * TRSV_MAXR * sizeof(cd))
 #define TRSV_BYTES (TRSV_XBYTES + TRSV_NBMAX * sizeof(int) + 256)
 
+// the lane's exchange area of the one-launch solves (its error flag is
+// shared with the batched solves); false if it cannot be allocated
+static bool trsv_area(LaLane *L, hipStream_t st)
+{
+  if (L->trsv_buf) return true;
+  if (hipExtMallocWithFlags(&L->trsv_buf, TRSV_BYTES, hipDeviceMallocUncached) != hipSuccess) {
+    L->trsv_buf = nullptr;
+    return false;
+  }
+  // flags start below every epoch (a kernel clears them: no memset node)
+  hipLaunchKernelGGL(k_zero_bytes, dim3((TRSV_BYTES / 4 + 255) / 256), dim3(256), 0, st, (unsigned int *)L->trsv_buf,
+                     TRSV_BYTES / 4);
+  return true;
+}
+
 // one-launch solve with the factors for nrhs <= TRSV_MAXR; false if not possible
 // (then the recursive path runs)
 template <class R, bool UPPER>
@@ -1361,16 +1498,7 @@ static bool trsv_sf(const typename Cx<R>::Z *A, size_t lda, int n, typename Cx<R
   const int nb = (n + 63) / 64;
   if (!g_coop || !g_trsv_ok || nrhs > TRSV_MAXR || n > TRSV_XMAX || nb > TRSV_NBMAX) return false;
   LaLane *L = la_lane(st);
-  if (!L) return false;
-  if (!L->trsv_buf) {
-    if (hipExtMallocWithFlags(&L->trsv_buf, TRSV_BYTES, hipDeviceMallocUncached) != hipSuccess) {
-      L->trsv_buf = nullptr;
-      return false;
-    }
-    // flags start below every epoch (a kernel clears them: no memset node)
-    hipLaunchKernelGGL(k_zero_bytes, dim3((TRSV_BYTES / 4 + 255) / 256), dim3(256), 0, st, (unsigned int *)L->trsv_buf,
-                       TRSV_BYTES / 4);
-  }
+  if (!L || !trsv_area(L, st)) return false;
   TrsvWs w;
   char *q = (char *)L->trsv_buf;
   w.x = (void *)q;              q += TRSV_XBYTES;
@@ -1390,6 +1518,104 @@ static bool trsv_sf(const typename Cx<R>::Z *A, size_t lda, int n, typename Cx<R
     return false;
   }
   return true;
+}
+
+#define TRSVB_XBYTES ((size_t)TRSVB_XMAX * TRSV_MAXR * sizeof(cd))
+#define TRSVB_BYTES (TRSVB_XBYTES + TRSVB_NBMAX * sizeof(int) + 256)
+
+// whether N systems of orders n[] fit one batched launch
+static bool trsv_batch_fits(int N, const int *n, int nrhs)
+{
+  if (!g_trsv_ok || N <= 0 || N > TRSVB_MAXSYS || nrhs < 1 || nrhs > TRSV_MAXR) return false;
+  size_t rows = 0, blocks = 0;
+  for (int q = 0; q < N; q++) {
+    if (n[q] <= 0) return false;
+    rows += (size_t)n[q];
+    blocks += (size_t)((n[q] + 63) / 64);
+  }
+  return rows <= TRSVB_XMAX && blocks <= TRSVB_NBMAX;
+}
+
+// X_s := L_s^{-1} X_s (unit lower) or U_s^{-1} X_s of N systems in one
+// launch (k_trsv_batch); the systems must fit (trsv_batch_fits).  False if
+// the exchange area or the launch fails.
+template <class R, bool UPPER>
+static bool trsv_batch(int N, const int *n, int nrhs, const typename Cx<R>::Z *const *A, const size_t *lda,
+                       typename Cx<R>::Z *const *B, const size_t *ldb, hipStream_t st)
+{
+  typedef typename Cx<R>::Z Z;
+  LaLane *L = la_lane(st);
+  if (!L || !trsv_area(L, st)) return false;
+  if (!L->trsvb_buf) {
+    if (hipExtMallocWithFlags(&L->trsvb_buf, TRSVB_BYTES, hipDeviceMallocUncached) != hipSuccess) {
+      L->trsvb_buf = nullptr;
+      return false;
+    }
+    hipLaunchKernelGGL(k_zero_bytes, dim3((TRSVB_BYTES / 4 + 255) / 256), dim3(256), 0, st, (unsigned int *)L->trsvb_buf,
+                       TRSVB_BYTES / 4);
+    L->trsvb_ticket = 0;
+  }
+  std::vector<int> order(N);
+  for (int q = 0; q < N; q++) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+  TrsvbArgs<Z> a{};
+  int xoff = 0, foff = 0;
+  double work = 0.0;
+  for (int q = 0; q < N; q++) {
+    const int k = order[q];
+    TrsvbSys<Z> &y = a.s[q];
+    y.A = A[k]; y.B = B[k]; y.lda = lda[k]; y.ldb = ldb[k];
+    y.n = n[k]; y.nb = (n[k] + 63) / 64; y.xoff = xoff; y.foff = foff;
+    xoff += y.n;
+    foff += y.nb;
+    work += 4.0 * (double)n[k] * n[k] * nrhs;
+  }
+  a.nsys = N;
+  a.base = L->trsvb_ticket;
+  char *p = (char *)L->trsvb_buf;
+  void *xb = (void *)p;
+  int *flags = (int *)(p + TRSVB_XBYTES);
+  unsigned *ticket = (unsigned *)(p + TRSVB_XBYTES + TRSVB_NBMAX * sizeof(int));
+  int *err = (int *)((char *)L->trsv_buf + TRSV_XBYTES + TRSV_NBMAX * sizeof(int));
+  const int epoch = ++L->trsvb_epoch;
+  bem_prof_begin(Cx<R>::KT, work, st);
+  switch (nrhs) {
+  case 1: hipLaunchKernelGGL((k_trsv_batch<R, UPPER, 1>), dim3(foff), dim3(256), trsv_lds_bytes<R>(), st, a, xb, flags, err, ticket, epoch); break;
+  case 2: hipLaunchKernelGGL((k_trsv_batch<R, UPPER, 2>), dim3(foff), dim3(256), trsv_lds_bytes<R>(), st, a, xb, flags, err, ticket, epoch); break;
+  case 3: hipLaunchKernelGGL((k_trsv_batch<R, UPPER, 3>), dim3(foff), dim3(256), trsv_lds_bytes<R>(), st, a, xb, flags, err, ticket, epoch); break;
+  default: hipLaunchKernelGGL((k_trsv_batch<R, UPPER, 4>), dim3(foff), dim3(256), trsv_lds_bytes<R>(), st, a, xb, flags, err, ticket, epoch); break;
+  }
+  bem_prof_end(Cx<R>::KT, st);
+  if (hipGetLastError() != hipSuccess) return false;
+  L->trsvb_ticket += (unsigned)foff;   // every workgroup of the launch takes one ticket
+  return true;
+}
+
+template <class R>
+static void getrs_t(int n, int nrhs, const typename Cx<R>::Z *A, size_t lda, const int *piv, typename Cx<R>::Z *B,
+                    size_t ldb, hipStream_t st);
+
+// solves with the factors of N systems (getrs_t of each): the row
+// interchanges of all systems in one launch, then the unit-lower and the
+// upper one-launch solves of all systems together; systems that do not fit
+// one launch are solved one after another.  Returns 0, or -1 if a batched
+// launch failed after the interchanges.
+template <class R>
+static int getrs_batch_t(int N, const int *n, int nrhs, const typename Cx<R>::Z *const *A, const size_t *lda,
+                         const int *const *piv, typename Cx<R>::Z *const *B, const size_t *ldb, hipStream_t st)
+{
+  typedef typename Cx<R>::Z Z;
+  if (N <= 0 || nrhs <= 0) return 0;
+  if (!trsv_batch_fits(N, n, nrhs)) {
+    for (int q = 0; q < N; q++) getrs_t<R>(n[q], nrhs, A[q], lda[q], piv[q], B[q], ldb[q], st);
+    return 0;
+  }
+  LaswpArgs<Z> la{};
+  for (int q = 0; q < N; q++) la.s[q] = LaswpSys<Z>{B[q], piv[q], ldb[q], n[q], 0};
+  hipLaunchKernelGGL(k_laswp_batch<Z>, dim3(N), dim3(64), 0, st, la, nrhs);
+  if (!trsv_batch<R, false>(N, n, nrhs, A, lda, B, ldb, st)) return -1;
+  if (!trsv_batch<R, true>(N, n, nrhs, A, lda, B, ldb, st)) return -1;
+  return 0;
 }
 
 template <class R>
@@ -1560,6 +1786,19 @@ void bem_cgemm(int m, int n, int k, double ar, double ai, const void *A, size_t 
 {
   gemm_t<float>(m, n, k, mkf((float)ar, (float)ai), (const cf *)A, lda, (const cf *)B, ldb, mkf((float)br, (float)bi),
                 (cf *)C, ldc, st);
+}
+
+// solves with the factors of N systems together (nrhs columns each, row
+// stride ldb[s]); 0, or -1 if a batched launch failed
+int bem_cgetrs_batch(int N, const int *n, int nrhs, const void *const *A, const size_t *lda, const int *const *piv,
+                     void *const *B, const size_t *ldb, hipStream_t st)
+{
+  return getrs_batch_t<float>(N, n, nrhs, (const cf *const *)A, lda, piv, (cf *const *)B, ldb, st);
+}
+int bem_zgetrs_batch(int N, const int *n, int nrhs, const void *const *A, const size_t *lda, const int *const *piv,
+                     void *const *B, const size_t *ldb, hipStream_t st)
+{
+  return getrs_batch_t<double>(N, n, nrhs, (const cd *const *)A, lda, piv, (cd *const *)B, ldb, st);
 }
 
 int bem_cgetrf(int n, void *A, size_t lda, int *piv, int *info, hipStream_t st)

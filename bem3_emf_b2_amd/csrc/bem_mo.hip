@@ -46,6 +46,7 @@
 #include "bem_prof.h"
 
 #include <atomic>
+#include <chrono>
 #include <thread>
 
 using namespace bem;
@@ -1238,16 +1239,42 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     if (!objs[m]->fmode || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d: factor and set beams first", m);
   int rc, iter = 0;
   const std::vector<double> c3 = cost_of(objs, N, 3), c2 = cost_of(objs, N, 2);
-  auto solve_all = [&]() { return run_lanes(objs, N, c2, [](bh_object *o) { return object_solve(o); }); };
+  // the mixed form's objects solve in lock step (objects_solve_batch)
+  // unless BEM3_HIP_BATCH_SOLVE=0; objects it does not take run on the lanes
+  const char *be = getenv("BEM3_HIP_BATCH_SOLVE");
+  const bool batch = !(be && strcmp(be, "0") == 0);
+  auto solve_all = [&]() {
+    if (batch && N > 1) {
+      HIPCHK(hipStreamSynchronize(g.st));
+      const int r = objects_solve_batch(objs, N);
+      if (r != BH_EUNSUP) return r;
+    }
+    return run_lanes(objs, N, c2, [](bh_object *o) { return object_solve(o); });
+  };
+  // every loop starts from the incident field alone (Ui = Ui0), the state
+  // mo_initialize leaves (init_boundary_data_u2, mo_setup.c:704-757): the
+  // incident data of an earlier loop on the same objects hold its coupling
+  // sums, which would start this one at that loop's converged state
+  for (int m = 0; m < N; m++) {
+    bh_object *o = objs[m];
+    const size_t n = (size_t)(o->Ne + 1) * 16;
+    hipLaunchKernelGGL(k_reset_incident, g1(n), dim3(256), 0, g.st, o->d_Ui, o->d_dUi, o->d_Ui0, o->d_dUi0, o->d_Uip, n);
+  }
+  HIPCHK(hipGetLastError());
   if (N == 1) {
     if ((rc = object_solve(objs[0]))) return rc;
   } else {
     // many solves per object: form the Schur complements once
+    const auto tl0 = std::chrono::steady_clock::now();
     if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return form_s(o); }))) return rc;
     Coupling C;
     if ((rc = C.init(objs, N))) return rc;
     bool first = true;
+    // wall seconds of the last sweep's coupling sums (to their completion)
+    // and of its solves, for the progress lines
+    double t_sums = 0.0, t_solves = 0.0;
     auto sweep = [&]() -> int {
+      const auto t0 = std::chrono::steady_clock::now();
       for (int m = 0; m < N; m++) {
         bh_object *o = objs[m];
         size_t n = (size_t)(o->Ne + 1) * 16;
@@ -1256,7 +1283,12 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
       int r = C.sums(objs, N, first);
       if (r) return r;
       first = false;
-      return solve_all();
+      HIPCHK(hipStreamSynchronize(g.st));   // run_lanes waits for it anyway
+      const auto t1 = std::chrono::steady_clock::now();
+      r = solve_all();
+      t_sums = std::chrono::duration<double>(t1 - t0).count();
+      t_solves = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+      return r;
     };
     // BEM3_HIP_PROGRESS=1: one line per iteration on stderr; any other
     // value but 0 names a file the lines are appended to (long solves)
@@ -1265,22 +1297,39 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     if (pe && *pe && strcmp(pe, "0") != 0) pf = strcmp(pe, "1") == 0 ? stderr : fopen(pe, "a");
     struct Closer { FILE *f; ~Closer() { if (f && f != stderr) fclose(f); } } closer{pf};
     if (pf) {
-      fprintf(pf, "bh_mo_solve: %d objects, %d object pairs by series, %d by direct sums\n", N, C.P.npairs_mp,
-              C.P.npairs_direct);
+      fprintf(pf, "bh_mo_solve: %d objects, %d object pairs by series, %d by direct sums; Schur complements %.3f s\n", N,
+              C.P.npairs_mp, C.P.npairs_direct, std::chrono::duration<double>(std::chrono::steady_clock::now() - tl0).count());
       fflush(pf);
     }
-    if ((rc = solve_all())) return rc;
+    {
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((rc = solve_all())) return rc;
+      if (pf) {
+        fprintf(pf, "bh_mo_solve: first solves %.3f s\n", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        fflush(pf);
+      }
+    }
     if ((rc = sweep())) return rc;
+    if (pf) { fprintf(pf, "bh_mo_solve: count = 1, sums %.3f s, solves %.3f s\n", t_sums, t_solves); fflush(pf); }
     for (iter = 2; iter < iter_max; iter++) {
       if ((rc = sweep())) return rc;
       double cc = C.change(objs, N);
       if (cc < 0.0) return fail(BH_EHIP, "coupling loop: change measure");
       if (trace) trace[iter] = cc;
-      if (pf) { fprintf(pf, "bh_mo_solve: count = %d, cc = %g\n", iter, cc); fflush(pf); }
+      if (pf) { fprintf(pf, "bh_mo_solve: count = %d, cc = %g, sums %.3f s, solves %.3f s\n", iter, cc, t_sums, t_solves); fflush(pf); }
       if (cc < ccd && iter > iter_min) break;
     }
   }
+  const auto te0 = std::chrono::steady_clock::now();
   if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return object_solve_EH(o); }))) return rc;
+  if (const char *pe = getenv("BEM3_HIP_PROGRESS"))
+    if (*pe && strcmp(pe, "0") != 0) {
+      FILE *pf = strcmp(pe, "1") == 0 ? stderr : fopen(pe, "a");
+      if (pf) {
+        fprintf(pf, "bh_mo_solve: boundary E/H %.3f s\n", std::chrono::duration<double>(std::chrono::steady_clock::now() - te0).count());
+        if (pf != stderr) fclose(pf); else fflush(pf);
+      }
+    }
   if (iters) *iters = iter;
   return 0;
 }

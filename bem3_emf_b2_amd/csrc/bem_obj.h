@@ -162,6 +162,8 @@ struct bh_object {
   bool s_formed = false;               // Sf holds LU(S) (else Sp = H1 + beta G1 Z, S solved by GMRES)
   bool x_warm = false;                 // d_X solves the current factors' system for an earlier B
                                        // (the refinement of the mixed form starts from it)
+  bool r_valid = false;                // d_R = d_B - A d_X for the current factors' system (kept by
+                                       // objects_solve_batch for its warm start)
   int gm_steps = 0;                    // GMRES steps of the last S solve(s)
   cd *d_X = nullptr, *d_B = nullptr, *d_tmp = nullptr;
   cd *d_w = nullptr;                   // work of the block solve (schur_work(NN))
@@ -207,6 +209,9 @@ namespace bem {
 int object_assemble(bh_object *o, int type);
 int object_factor(bh_object *o, int mode);
 int object_solve(bh_object *o);
+// the mixed-form solves of N objects in lock step (BH_EUNSUP: an object
+// does not qualify, nothing done)
+int objects_solve_batch(bh_object **objs, int N);
 int object_solve_EH(bh_object *o);
 // many beams (bh_mo_solve_beams): nb plane waves (12 doubles each) as the
 // object's right-hand sides and their incident data; the solve and the

@@ -73,6 +73,15 @@ void bem_resid_rows(const cd *G, const cd *H, int nrows, int nd, size_t ld, int 
 // 256 doubles of device memory
 double bem_nrm2(const cd *x, size_t n, double *tmp, hipStream_t st);
 
+// the same, enqueued only: |x|^2 (not its root) into the device double *out
+void bem_nrm2sq_to(const cd *x, size_t n, double *tmp, double *out, hipStream_t st);
+
+// schur_apply (r0nz, S formed) of N objects together: every solve with a
+// factor is one batched launch per triangle over all objects (f[o], rhs[o],
+// X[o], work w[o] of schur_work(n) values per object).  Returns 0,
+// BH_ESTATE if an object's S is not formed, or BH_EHIP.
+int schur_apply_batch(int N, const SchurF<cf> *f, const cf *const *rhs, cf *const *X, cf *const *w, hipStream_t st);
+
 // elementwise conversions: dst = src rounded to FP32; X += d widened to FP64
 void bem_cvt_z2c(const cd *src, cf *dst, size_t n, hipStream_t st);
 void bem_acc_c2z(cd *X, const cf *d, size_t n, hipStream_t st);

@@ -17,6 +17,8 @@ void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *
 void bem_cgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
                hipStream_t);
 int bem_cgetrf(int, void *, size_t, int *, int *, hipStream_t);
+int bem_cgetrs_batch(int, const int *, int, const void *const *, const size_t *, const int *const *, void *const *,
+                     const size_t *, hipStream_t);
 int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
 void bem_zgetrs(int, int, const void *, size_t, const int *, void *, size_t, int, hipStream_t);
 void bem_cgetrs(int, int, const void *, size_t, const int *, void *, size_t, hipStream_t);
@@ -323,6 +325,76 @@ static int schur_apply_t(const SchurF<Zt> &f, const Zt *rhs, bool r0nz, Zt *X, Z
   hipLaunchKernelGGL(k_pack_u4<Zt>, grid1(N), dim3(256), 0, st, Bc, u3, n, U4);
   xgemm(n, 4, n, one<Zt>(), f.Zm, f.ld, U4, 4, zero<Zt>(), Q4, 4, st);
   hipLaunchKernelGGL(k_fill_x<Zt>, grid1(N), dim3(256), 0, st, U4, Q4, r0nz ? (const Zt *)q0 : nullptr, n, X);
+  return 0;
+}
+
+template <class Zt> __global__ void k_copy_v(Zt *__restrict__ dst, const Zt *__restrict__ src, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ---- several objects at once ----
+// The steps of schur_apply_t (r0nz, S formed) for N objects in lock step:
+// each solve with LU(G0), LU(M) or LU(S) is one batched launch per
+// triangle over every object (the chains of all objects advance together),
+// the products and elementwise steps run per object in order on st.
+int schur_apply_batch(int N, const SchurF<cf> *f, const cf *const *rhs, cf *const *X, cf *const *w, hipStream_t st)
+{
+  if (N <= 0) return 0;
+  for (int o = 0; o < N; o++)
+    if (!f[o].s_lu) return BH_ESTATE;
+  std::vector<int> n(N);
+  std::vector<size_t> ld(N), l3(N, 3), l4(N, 4), l1(N, 1);
+  std::vector<const void *> G0f(N), Mf(N), Sf(N);
+  std::vector<const int *> p0(N), pM(N), pS(N);
+  std::vector<void *> q0v(N), Yv(N), u3v(N), Bcv(N);
+  for (int o = 0; o < N; o++) {
+    n[o] = f[o].n;
+    ld[o] = f[o].ld;
+    G0f[o] = f[o].G0f; Mf[o] = f[o].Mf; Sf[o] = f[o].Sf;
+    p0[o] = f[o].piv0; pM[o] = f[o].pivM; pS[o] = f[o].pivS;
+    const size_t M = (size_t)n[o];
+    cf *ww = w[o];
+    Yv[o] = ww; Bcv[o] = ww + 3 * M; u3v[o] = ww + 7 * M; q0v[o] = ww + 11 * M;
+  }
+  auto P = [&](int o, int k) -> cf * { return w[o] + (size_t)k * n[o]; };   // Y 0, Bc 3, v 6, u3 7, Qn 8, q0 11, T 15, U4 19, Q4 23
+  // q0 = G0^{-1} r0 (4 columns), Bc, u3 from the domain-1 rows
+  for (int o = 0; o < N; o++) {
+    const size_t M = (size_t)n[o];
+    const cf *b1 = rhs[o] + 4 * M;
+    hipLaunchKernelGGL(k_gather_cols<cf>, grid1(3 * M), dim3(256), 0, st, b1, M, 3, P(o, 3), n[o]);
+    hipLaunchKernelGGL(k_copy_v<cf>, grid1(M), dim3(256), 0, st, P(o, 7), b1 + 3 * M, M);
+    hipLaunchKernelGGL(k_gather_cols<cf>, grid1(4 * M), dim3(256), 0, st, rhs[o], M, 4, P(o, 11), n[o]);
+  }
+  if (bem_cgetrs_batch(N, n.data(), 4, G0f.data(), ld.data(), p0.data(), q0v.data(), l4.data(), st)) return BH_EHIP;
+  // the domain-1 rows gain G1 q0; Y = M^{-1} [b0 b1 b2]
+  for (int o = 0; o < N; o++) {
+    const size_t M = (size_t)n[o];
+    xgemm(n[o], 4, n[o], one<cf>(), f[o].G1, f[o].ld, P(o, 11), 4, zero<cf>(), P(o, 15), 4, st);
+    hipLaunchKernelGGL(k_rhs_q0<cf>, grid1(M), dim3(256), 0, st, P(o, 3), P(o, 7), P(o, 15), f[o].beps, n[o]);
+    hipLaunchKernelGGL(k_copy_v<cf>, grid1(3 * M), dim3(256), 0, st, P(o, 0), P(o, 3), 3 * M);
+  }
+  if (bem_cgetrs_batch(N, n.data(), 3, Mf.data(), ld.data(), pM.data(), Yv.data(), l3.data(), st)) return BH_EHIP;
+  // u3 = S^{-1} (b3 - sigma G1 sum_c N_c Y_c)
+  for (int o = 0; o < N; o++) {
+    hipLaunchKernelGGL(k_ndot<cf>, grid1((size_t)n[o]), dim3(256), 0, st, P(o, 0), f[o].col1, n[o], P(o, 6));
+    xgemm(n[o], 1, n[o], -f[o].sigm, f[o].G1, f[o].ld, P(o, 6), 1, one<cf>(), P(o, 7), 1, st);
+  }
+  if (bem_cgetrs_batch(N, n.data(), 1, Sf.data(), ld.data(), pS.data(), u3v.data(), l1.data(), st)) return BH_EHIP;
+  // u_c = M^{-1} (b_c - delta G1 N_c u3)
+  for (int o = 0; o < N; o++) {
+    hipLaunchKernelGGL(k_nscale<cf>, grid1((size_t)n[o]), dim3(256), 0, st, P(o, 7), f[o].col1, n[o], P(o, 8));
+    xgemm(n[o], 3, n[o], -f[o].delt, f[o].G1, f[o].ld, P(o, 8), 3, one<cf>(), P(o, 3), 3, st);
+  }
+  if (bem_cgetrs_batch(N, n.data(), 3, Mf.data(), ld.data(), pM.data(), Bcv.data(), l3.data(), st)) return BH_EHIP;
+  // U4 = [u0 u1 u2 u3], Q4 = Z U4 - q0, X = [U4^T ; Q4^T]
+  for (int o = 0; o < N; o++) {
+    hipLaunchKernelGGL(k_pack_u4<cf>, grid1((size_t)n[o]), dim3(256), 0, st, P(o, 3), P(o, 7), n[o], P(o, 19));
+    xgemm(n[o], 4, n[o], one<cf>(), f[o].Zm, f[o].ld, P(o, 19), 4, zero<cf>(), P(o, 23), 4, st);
+    hipLaunchKernelGGL(k_fill_x<cf>, grid1((size_t)n[o]), dim3(256), 0, st, P(o, 19), P(o, 23), (const cf *)P(o, 11), n[o],
+                       X[o]);
+  }
   return 0;
 }
 
@@ -774,6 +846,21 @@ __global__ void __launch_bounds__(256) k_nrm2_fin(double *part)
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) part[0] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+__global__ void __launch_bounds__(256) k_nrm2_fin_to(const double *part, double *out)
+{
+  __shared__ double s[4];
+  double v = wsum(part[threadIdx.x]);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+void bem_nrm2sq_to(const cd *x, size_t n, double *tmp, double *out, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_nrm2_part, dim3(256), dim3(256), 0, st, x, n, tmp);
+  hipLaunchKernelGGL(k_nrm2_fin_to, dim3(1), dim3(256), 0, st, tmp, out);
 }
 
 double bem_nrm2(const cd *x, size_t n, double *tmp, hipStream_t st)
