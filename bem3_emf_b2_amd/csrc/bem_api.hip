@@ -385,7 +385,7 @@ extern "C" void bh_object_destroy(bh_object *o)
   dfree(o->d_piv0); dfree(o->d_pivM); dfree(o->d_pivS);
   dfree(o->d_X); dfree(o->d_B); dfree(o->d_tmp); dfree(o->d_w);
   dfree(o->c_G0f); dfree(o->c_Z); dfree(o->c_Mf); dfree(o->c_Sf); dfree(o->c_G1); dfree(o->c_w);
-  dfree(o->d_R); dfree(o->d_nrm);
+  dfree(o->d_R); dfree(o->d_nrm); dfree(o->d_ginfo);
   delete o;
 }
 
@@ -1456,6 +1456,54 @@ static int solve_g_mixed(bh_object *o, int d, const cf *Gc, const int *piv, cd *
   return rc;
 }
 
+__global__ void k_zero_int(int *__restrict__ a, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// the complex-FP32 LU of G_d (d >= 1) the mixed form's boundary fields
+// solve with (solve_deh_bv, mo_solve.c:663-789), once per assembly; info:
+// a device int for the LU's zero-pivot report
+static int factor_g_mixed(bh_object *o, int d, int *info)
+{
+  Dom &D = o->dom[d];
+  const size_t nd = (size_t)D.nd;
+  if (!D.c_Gf && dalloc(&D.c_Gf, nd * nd)) return BH_EHIP;
+  if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
+  if (o->prec == BH_PREC_MIXED_LM)   // G1 in complex FP32, kept by the factorisation
+    HIPCHK(hipMemcpyAsync(D.c_Gf, o->c_G1, nd * nd * sizeof(cf), hipMemcpyDeviceToDevice, cur_st()));
+  else
+    bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, cur_st());
+  if (bem_cgetrf(D.nd, D.c_Gf, nd, D.d_Gpiv, info, cur_st())) return lu_failed();
+  D.gfc_valid = true;
+  D.gf_valid = false;   // the shared pivot vector now belongs to the FP32 factors
+  return 0;
+}
+
+// factor_g_mixed of every domain d >= 1 of a mixed-form object ahead of its
+// boundary fields (the coupling loop runs it beside the sweeps); its own
+// zero-pivot check, on the thread's current stream
+int bem::object_factor_g(bh_object *o)
+{
+  if (o->fmode != 3) return 0;
+  if (!o->d_ginfo && dalloc(&o->d_ginfo, 1)) return BH_EHIP;
+  for (int d = 1; d <= o->MN; d++) {
+    if (o->dom[d].gfc_valid) continue;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, cur_st(), o->d_ginfo, 1);
+    int rc = factor_g_mixed(o, d, o->d_ginfo);
+    if (rc) return rc;
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, o->d_ginfo, sizeof(int), hipMemcpyDeviceToHost, cur_st()));
+    HIPCHK(hipStreamSynchronize(cur_st()));
+    if (info) {
+      o->dom[d].gfc_valid = false;
+      return fail(BH_ESING, "singular G of domain %d (zero pivot %d)", d, info);
+    }
+  }
+  return 0;
+}
+
 // boundary E, H from the derivative rows, then dE/dn, dH/dn from
 // G [dE|dH] = H [E|H] per domain (solve_eh_bv + solve_deh_bv)
 int bem::object_solve_EH(bh_object *o)
@@ -1487,15 +1535,8 @@ int bem::object_solve_EH(bh_object *o)
       Gpiv = o->d_piv0;
     } else if (mixed) {
       if (!D.gfc_valid) {
-        if (!D.c_Gf && dalloc(&D.c_Gf, nd * nd)) return BH_EHIP;
-        if (!D.d_Gpiv && dalloc(&D.d_Gpiv, nd)) return BH_EHIP;
-        if (o->prec == BH_PREC_MIXED_LM)   // G1 in complex FP32, kept by the factorisation
-          HIPCHK(hipMemcpyAsync(D.c_Gf, o->c_G1, nd * nd * sizeof(cf), hipMemcpyDeviceToDevice, cur_st()));
-        else
-          bem_cvt_z2c(D.d_G, D.c_Gf, nd * nd, cur_st());
-        if (bem_cgetrf(D.nd, D.c_Gf, nd, D.d_Gpiv, o->d_info, cur_st())) return lu_failed();
-        D.gfc_valid = true;
-        D.gf_valid = false;   // the shared pivot vector now belongs to the FP32 factors
+        rc = factor_g_mixed(o, d, o->d_info);
+        if (rc) return rc;
       }
       Gc = D.c_Gf;
       Gpiv = D.d_Gpiv;
@@ -1589,6 +1630,12 @@ int bem::object_beams_set(bh_object *o, int nb, const double *par)
   HIPCHK(hipMemcpyAsync(db, bm.data(), nb * sizeof(Beam), hipMemcpyHostToDevice, cur_st()));
   o->mb_hbeams = bm;
   for (int q = 0; q < 5; q++) zero_cd(mb_ptr(o, q), E1 * 16 * nb);
+  // the boundary values start at zero like the one-beam arrays (dalloc):
+  // the solves write only the slots the elements use, and the areas may
+  // hold an earlier allocation's data
+  for (int d = 0; d <= o->MN; d++)
+    for (int q = 0; q < 6; q++)
+      zero_cd(mb_dom_ptr(o, d, q), (size_t)(o->dom[d].dNe + 1) * (q < 2 ? 16 : 12) * nb);
   for (int b = 0; b < nb; b++) {
     const size_t off = (size_t)b * E1 * 16;
     bem_launch_incident_bd(o->d_slotPt, o->d_slotN, o->d_slotIdx, o->nslot, db + b, 1, mb_ptr(o, 2) + off,

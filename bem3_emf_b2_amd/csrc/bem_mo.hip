@@ -47,6 +47,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <string>
 #include <thread>
 
 using namespace bem;
@@ -65,7 +66,7 @@ void bem_launch_scatter(const double *, const double *, const int *, int, const 
 #define MP_EPS 1e-10            // truncation target (relative)
 #define MP_GAP_RADII 4.0        // gap to the source sphere in element radii
 #define MP_NDENS 16             // point densities: 4 potentials x (monopole, dipole z, +, -)
-#define MP_QSPLIT 2048          // source points per moment workgroup
+#define MP_QSPLIT 256           // source points per moment workgroup
 
 // ------------------------------------------------------------------ series helpers
 __host__ __device__ __forceinline__ int mp_idx(int n, int m, int Lp)
@@ -232,23 +233,38 @@ __global__ void __launch_bounds__(64) k_mp_basis(MpSrc S, const TriGeo *__restri
 }
 
 // T[split][idx][16] = sum over the split's points of B[q][idx] dens[q][:]
-// (thread per idx, densities wave-uniform)
-__global__ void __launch_bounds__(256) k_mp_moments(int nidx, int nqc, int qper, const cd *__restrict__ B, size_t ldb,
-                                                    const cd *__restrict__ dens, cd *__restrict__ T)
+// (one wave per workgroup, lane per idx; the split's densities staged in
+// LDS 64 points at a time and read as broadcasts, the next point's basis
+// value loaded before the current point's 64 multiply-adds)
+#define MP_MOM_CHUNK 64
+__global__ void __launch_bounds__(64) k_mp_moments(int nidx, int nqc, int qper, const cd *__restrict__ B, size_t ldb,
+                                                   const cd *__restrict__ dens, cd *__restrict__ T)
 {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
+  __shared__ cd sd[MP_MOM_CHUNK * MP_NDENS];
+  const int lane = threadIdx.x;
+  const int idx = blockIdx.x * 64 + lane;
   const int split = blockIdx.y;
   const int qa = split * qper, qb = min(nqc, qa + qper);
+  const bool live = idx < nidx;
+  const int ii = live ? idx : 0;
   cd acc[MP_NDENS];
 #pragma unroll
   for (int d = 0; d < MP_NDENS; d++) acc[d] = mk(0.0, 0.0);
-  if (idx < nidx) {
-    for (int q = qa; q < qb; q++) {
-      const cd b = B[(size_t)q * ldb + idx];
-      const cd *dq = dens + (size_t)q * MP_NDENS;
+  for (int q0 = qa; q0 < qb; q0 += MP_MOM_CHUNK) {
+    const int nq = min(MP_MOM_CHUNK, qb - q0);
+    __syncthreads();   // the previous chunk's densities are consumed
+    for (int t = lane; t < nq * MP_NDENS; t += 64) sd[t] = dens[(size_t)q0 * MP_NDENS + t];
+    __syncthreads();
+    cd b = B[(size_t)q0 * ldb + ii];
+    for (int q = 0; q < nq; q++) {
+      const cd bn = q + 1 < nq ? B[(size_t)(q0 + q + 1) * ldb + ii] : b;
+      const cd *dq = sd + q * MP_NDENS;
 #pragma unroll
       for (int d = 0; d < MP_NDENS; d++) acc[d] = acc[d] + b * dq[d];
+      b = bn;
     }
+  }
+  if (live) {
     cd *t = T + ((size_t)split * nidx + idx) * MP_NDENS;
 #pragma unroll
     for (int d = 0; d < MP_NDENS; d++) t[d] = acc[d];
@@ -267,9 +283,16 @@ __global__ void k_mp_reduce(int n, int nsplit, const cd *__restrict__ T, cd *__r
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  cd s = Tsum[i];
-  for (int k = 0; k < nsplit; k++) s = s + T[(size_t)k * n + i];
-  Tsum[i] = s;
+  // eight partial sums (loads of eight splits in flight), added in a fixed order
+  cd p[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) p[u] = mk(0.0, 0.0);
+  int k = 0;
+  for (; k + 8 <= nsplit; k += 8)
+#pragma unroll
+    for (int u = 0; u < 8; u++) p[u] = p[u] + T[(size_t)(k + u) * n + i];
+  for (; k < nsplit; k++) p[0] = p[0] + T[(size_t)k * n + i];
+  Tsum[i] = Tsum[i] + (((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7])));
 }
 
 // moment of density set d (0 monopole, 1 z, 2 +, 3 -) and potential c at (n, m)
@@ -772,7 +795,7 @@ int mp_coefficients(bh_object *o, const Plan &P, int n, cd *coef)
     bem_prof_end(BH_K_MP_BASIS, g.st);
     const int ns = (nqc + MP_QSPLIT - 1) / MP_QSPLIT;
     bem_prof_begin(BH_K_MP_MOM, 8.0 * nidx * MP_NDENS * (double)nqc, g.st);
-    hipLaunchKernelGGL(k_mp_moments, dim3((nidx + 255) / 256, ns), dim3(256), 0, g.st, nidx, nqc, MP_QSPLIT, B, ldb,
+    hipLaunchKernelGGL(k_mp_moments, dim3((nidx + 63) / 64, ns), dim3(64), 0, g.st, nidx, nqc, MP_QSPLIT, B, ldb,
                        dens, T);
     bem_prof_end(BH_K_MP_MOM, g.st);
     const size_t nt = (size_t)nidx * MP_NDENS;
@@ -870,6 +893,7 @@ int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double
 hipStream_t g_lane_st[MO_LMAX];
 int g_nlanes = 0;     // streams created so far
 int g_lanes_req = 0;  // bh_mo_set_lanes (0: BEM3_HIP_LANES, else 2)
+hipStream_t g_gpre_st = nullptr;   // the coupling loop's G1 factorisations (GPrefactor)
 
 int lanes()
 {
@@ -936,6 +960,7 @@ std::vector<double> cost_of(bh_object **objs, int N, int power)
 extern "C" void bem_mo_release(void)
 {
   if (g_ws) { delete[] g_ws; g_ws = nullptr; }
+  if (g_gpre_st) { (void)hipStreamDestroy(g_gpre_st); g_gpre_st = nullptr; }
   for (int k = 0; k < g_nlanes; k++) (void)hipStreamDestroy(g_lane_st[k]);
   g_nlanes = 0;
 }
@@ -1227,6 +1252,44 @@ extern "C" int bh_mo_coupling_sums(bh_object **objs, int N, int dst, double *out
   return rc;
 }
 
+// object_factor_g of N objects by a worker thread on a stream of its own
+// (created once, default priority); finish() joins it and returns its first
+// failure; the destructor joins it too, so no return path leaves it running
+struct GPrefactor {
+  std::thread th;
+  int rc = 0;
+  std::string msg;
+  void start(bh_object **objs, int N, const std::vector<double> &cost)
+  {
+    bool any = false;
+    for (int m = 0; m < N; m++) any = any || objs[m]->fmode == 3;
+    const char *e = getenv("BEM3_HIP_G1_AHEAD");   // 0: the boundary fields factorise G1 themselves
+    if (!any || (e && strcmp(e, "0") == 0)) return;
+    if (!g_gpre_st && hipStreamCreate(&g_gpre_st) != hipSuccess) { g_gpre_st = nullptr; return; }
+    std::vector<int> order(N);
+    for (int i = 0; i < N; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<bh_object *> list;
+    for (int i : order) list.push_back(objs[i]);
+    th = std::thread([this, list]() {
+      set_cur_st(g_gpre_st);
+      for (bh_object *o : list) {
+        const int r = object_factor_g(o);
+        if (r) { rc = r; msg = bh_last_error(); break; }
+      }
+      if (hipStreamSynchronize(g_gpre_st) != hipSuccess && !rc) { rc = BH_EHIP; msg = "G1 factorisation stream"; }
+      set_cur_st(nullptr);
+    });
+  }
+  int finish()
+  {
+    if (th.joinable()) th.join();
+    if (rc) return fail(rc, "%s", msg.c_str());
+    return 0;
+  }
+  ~GPrefactor() { if (th.joinable()) th.join(); }
+};
+
 // block-Jacobi fixed point over objects (mo_solve_bv, mo_solve.c:10-88).
 // trace (optional, iter_max+1 doubles) receives cc per iteration; *iters
 // receives the final iteration count.
@@ -1239,6 +1302,7 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     if (!objs[m]->fmode || !objs[m]->d_beams) return fail(BH_ESTATE, "object %d: factor and set beams first", m);
   int rc, iter = 0;
   const std::vector<double> c3 = cost_of(objs, N, 3), c2 = cost_of(objs, N, 2);
+  GPrefactor gpre;   // joined on every return
   // the mixed form's objects solve in lock step (objects_solve_batch)
   // unless BEM3_HIP_BATCH_SOLVE=0; objects it does not take run on the lanes
   const char *be = getenv("BEM3_HIP_BATCH_SOLVE");
@@ -1267,6 +1331,11 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     // many solves per object: form the Schur complements once
     const auto tl0 = std::chrono::steady_clock::now();
     if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return form_s(o); }))) return rc;
+    // the FP32 LUs of G1 that the boundary fields at the end solve with
+    // (mixed form) are formed beside the sweeps, on a stream of their own
+    // by a worker thread, largest object first: the sweeps' kernels leave
+    // the device room
+    gpre.start(objs, N, c3);
     Coupling C;
     if ((rc = C.init(objs, N))) return rc;
     bool first = true;
@@ -1320,6 +1389,7 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
       if (cc < ccd && iter > iter_min) break;
     }
   }
+  if ((rc = gpre.finish())) return rc;
   const auto te0 = std::chrono::steady_clock::now();
   if ((rc = run_lanes(objs, N, c3, [](bh_object *o) { return object_solve_EH(o); }))) return rc;
   if (const char *pe = getenv("BEM3_HIP_PROGRESS"))

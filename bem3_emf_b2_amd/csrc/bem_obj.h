@@ -173,6 +173,7 @@ struct bh_object {
   cf *c_w = nullptr;                   // schur_work(NN) work, 8 NN right-hand side, 8 NN correction
   cd *d_R = nullptr;                   // 8 NN residual
   double *d_nrm = nullptr;             // 256 partial sums of the norm
+  int *d_ginfo = nullptr;              // zero-pivot report of object_factor_g
   int ir_iters = 0;                    // refinement steps of the last solve
   double ir_res = 0.0;                 // its final relative residual
   int asm_prec = -1;                   // precision the current matrices were assembled in (-1: none)
@@ -213,6 +214,9 @@ int object_solve(bh_object *o);
 // does not qualify, nothing done)
 int objects_solve_batch(bh_object **objs, int N);
 int object_solve_EH(bh_object *o);
+// the FP32 LU of G_d (d >= 1) of a mixed-form object that object_solve_EH
+// needs, ahead of it (once per assembly)
+int object_factor_g(bh_object *o);
 // many beams (bh_mo_solve_beams): nb plane waves (12 doubles each) as the
 // object's right-hand sides and their incident data; the solve and the
 // boundary E, H of all of them (the mixed-precision form with S formed)
