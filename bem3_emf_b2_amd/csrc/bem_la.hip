@@ -17,6 +17,8 @@
 #include "bem_prof.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -1308,6 +1310,25 @@ static inline int half64(int n) { return std::max(64, (n / 2) / 64 * 64); }
 
 // columns [k0, k0+kb) (kb <= 64) of rows [k0, n): pivots into piv[k0..k0+kb)
 // (absolute rows), rows interchanged inside these columns only
+// resident workgroups of k_panel_coop<R> on the device (0: unknown) and
+// whether panels take the ordinary launch (BEM3_HIP_PANEL_COOP=1: always
+// cooperative)
+static int g_panel_cap[2] = {0, 0};
+static bool g_panel_ordinary = true;
+// the streams bound to lanes so far: each may factorise at the same time
+static int la_lanes_bound()
+{
+  std::lock_guard<std::mutex> lk(g_lanes_mu);
+  int k = 0;
+  for (int i = 0; i < LA_NLANES; i++) k += g_lanes[i].used ? 1 : 0;
+  return k;
+}
+template <class R> static bool panel_ordinary(int nbc)
+{
+  const int cap = g_panel_cap[sizeof(R) == sizeof(float) ? 1 : 0];
+  return g_panel_ordinary && cap > 0 && (long)nbc * la_lanes_bound() <= (long)cap;
+}
+
 template <class R>
 static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, int *piv, const LuWs &w, hipStream_t st)
 {
@@ -1320,6 +1341,20 @@ static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, in
     CoopWs cw = w.coop;
     // arrival counters of this launch: the top one and one per group
     (void)hipMemsetAsync(cw.cnt, 0, (size_t)COOP_CSTRIDE * sizeof(unsigned int) * (1 + (nbc + COOP_GS - 1) / COOP_GS), st);
+    // An ordinary launch lets the work of other streams (other objects'
+    // products and panels) run beside the panel; a cooperative launch runs
+    // alone on the device.  The grid barrier needs every workgroup of the
+    // panel resident: that holds once the other kernels finish if the panels
+    // of all streams that factorise at once fit the device together
+    // (lanes x workgroups <= resident capacity), the only other waiters
+    // being the triangular solves' blocks, which always progress.
+    if (panel_ordinary<R>(nbc)) {
+      hipLaunchKernelGGL(k_panel_coop<R>, dim3(nbc), dim3(64), 0, st, A, lda, k0, kb, m, piv, w.info, cw);
+      if (hipGetLastError() == hipSuccess) {
+        bem_prof_end(Cx<R>::KP, st);
+        return;
+      }
+    }
     void *args[] = {&A, &lda, (void *)&k0, (void *)&kb, (void *)&m, &piv, (void *)&w.info, &cw};
     if (hipLaunchCooperativeKernel((const void *)k_panel_coop<R>, dim3(nbc), dim3(64), args, 0, st) == hipSuccess) {
       bem_prof_end(Cx<R>::KP, st);
@@ -1492,32 +1527,20 @@ static bool trsv_area(LaLane *L, hipStream_t st)
 // one-launch solve with the factors for nrhs <= TRSV_MAXR; false if not possible
 // (then the recursive path runs)
 template <class R, bool UPPER>
+static bool trsv_batch(int N, const int *n, int nrhs, const typename Cx<R>::Z *const *A, const size_t *lda,
+                       typename Cx<R>::Z *const *B, const size_t *ldb, hipStream_t st);
+static bool trsv_batch_fits(int N, const int *n, int nrhs);
+
+// one-launch solve with the factors for nrhs <= TRSV_MAXR (the batched
+// launch with one system: ticket-ordered blocks, an ordinary launch that
+// work on other streams can overlap); false if not possible (then the
+// recursive path runs)
+template <class R, bool UPPER>
 static bool trsv_sf(const typename Cx<R>::Z *A, size_t lda, int n, typename Cx<R>::Z *B, size_t ldb, int nrhs,
                     hipStream_t st)
 {
-  const int nb = (n + 63) / 64;
-  if (!g_coop || !g_trsv_ok || nrhs > TRSV_MAXR || n > TRSV_XMAX || nb > TRSV_NBMAX) return false;
-  LaLane *L = la_lane(st);
-  if (!L || !trsv_area(L, st)) return false;
-  TrsvWs w;
-  char *q = (char *)L->trsv_buf;
-  w.x = (void *)q;              q += TRSV_XBYTES;
-  w.flag = (int *)q;            q += TRSV_NBMAX * sizeof(int);
-  w.err = (int *)q;
-  const int epoch = ++L->trsv_epoch;
-  bem_prof_begin(Cx<R>::KT, 4.0 * (double)n * n * nrhs, st);
-  void *args[] = {(void *)&A, &lda, (void *)&n, &B, &ldb, &w, (void *)&epoch};
-  const void *fn = nrhs == 1 ? (const void *)k_trsv_sf<R, UPPER, 1>
-                 : nrhs == 2 ? (const void *)k_trsv_sf<R, UPPER, 2>
-                 : nrhs == 3 ? (const void *)k_trsv_sf<R, UPPER, 3>
-                             : (const void *)k_trsv_sf<R, UPPER, 4>;
-  const bool ok = hipLaunchCooperativeKernel(fn, dim3(nb), dim3(256), args, trsv_lds_bytes<R>(), st) == hipSuccess;
-  bem_prof_end(Cx<R>::KT, st);
-  if (!ok) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return true;
+  if (!trsv_batch_fits(1, &n, nrhs)) return false;
+  return trsv_batch<R, UPPER>(1, &n, nrhs, &A, &lda, &B, &ldb, st);
 }
 
 #define TRSVB_XBYTES ((size_t)TRSVB_XMAX * TRSV_MAXR * sizeof(cd))
@@ -1714,9 +1737,19 @@ int bem_la_init(void)
   if (hipFuncSetAttribute((const void *)k_panel_perm, hipFuncAttributeMaxDynamicSharedMemorySize, PERM_LDS_MAX) != hipSuccess)
     return -1;
   g_big_tiles = true;
-  int dev = 0, coop = 0;
+  int dev = 0, coop = 0, ncu = 0;
   if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess)
     g_coop = coop != 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_panel_coop<double>, 64, 0) == hipSuccess)
+      g_panel_cap[0] = b * ncu;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_panel_coop<float>, 64, 0) == hipSuccess)
+      g_panel_cap[1] = b * ncu;
+  }
+  (void)hipGetLastError();
+  const char *e = getenv("BEM3_HIP_PANEL_COOP");
+  g_panel_ordinary = !(e && strcmp(e, "1") == 0);
   return 0;
 }
 
