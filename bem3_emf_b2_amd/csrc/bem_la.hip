@@ -17,6 +17,7 @@
 #include "bem_prof.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -874,7 +875,7 @@ static bool g_trsv_ok = true;
 // A lane is bound to a stream on first use; its uncached areas are
 // allocated once and kept until the process ends (memory a kernel has read
 // uncached is not returned to the allocator).
-#define LA_NLANES 8
+#define LA_NLANES 16
 struct LaLane {
   hipStream_t st;
   bool used;
@@ -1303,6 +1304,7 @@ struct LuWs {
   int nbmax;
   CoopWs coop;
   bool coop_ok;
+  bool ordinary;   // the panels of this factorisation take ordinary launches
 };
 
 // split point of a recursive step: about half, a multiple of the 64-wide base
@@ -1310,24 +1312,25 @@ static inline int half64(int n) { return std::max(64, (n / 2) / 64 * 64); }
 
 // columns [k0, k0+kb) (kb <= 64) of rows [k0, n): pivots into piv[k0..k0+kb)
 // (absolute rows), rows interchanged inside these columns only
-// resident workgroups of k_panel_coop<R> on the device (0: unknown) and
-// whether panels take the ordinary launch (BEM3_HIP_PANEL_COOP=1: always
-// cooperative)
-static int g_panel_cap[2] = {0, 0};
+// Resident workgroups of k_panel_coop<float> on the device (0: unknown),
+// a k_panel_coop<double> workgroup's share of them in units of the float
+// one, and the units held by factorisations running now.  A factorisation
+// whose panels all fit beside the ones already reserved takes ordinary
+// launches for them (BEM3_HIP_PANEL_COOP=1: never).
+static int g_panel_cap = 0, g_panel_unit_d = 2;
+static std::atomic<long> g_panel_held{0};
 static bool g_panel_ordinary = true;
-// the streams bound to lanes so far: each may factorise at the same time
-static int la_lanes_bound()
+template <class R> static long panel_units(int nbc) { return (long)nbc * (sizeof(R) == sizeof(float) ? 1 : g_panel_unit_d); }
+template <class R> static bool panel_reserve(int nbc)
 {
-  std::lock_guard<std::mutex> lk(g_lanes_mu);
-  int k = 0;
-  for (int i = 0; i < LA_NLANES; i++) k += g_lanes[i].used ? 1 : 0;
-  return k;
+  if (!g_panel_ordinary || g_panel_cap <= 0) return false;
+  const long u = panel_units<R>(nbc);
+  long cur = g_panel_held.load();
+  while (cur + u <= (long)g_panel_cap)
+    if (g_panel_held.compare_exchange_weak(cur, cur + u)) return true;
+  return false;
 }
-template <class R> static bool panel_ordinary(int nbc)
-{
-  const int cap = g_panel_cap[sizeof(R) == sizeof(float) ? 1 : 0];
-  return g_panel_ordinary && cap > 0 && (long)nbc * la_lanes_bound() <= (long)cap;
-}
+template <class R> static void panel_release(int nbc) { g_panel_held -= panel_units<R>(nbc); }
 
 template <class R>
 static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, int *piv, const LuWs &w, hipStream_t st)
@@ -1348,7 +1351,7 @@ static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, in
     // of all streams that factorise at once fit the device together
     // (lanes x workgroups <= resident capacity), the only other waiters
     // being the triangular solves' blocks, which always progress.
-    if (panel_ordinary<R>(nbc)) {
+    if (w.ordinary) {
       hipLaunchKernelGGL(k_panel_coop<R>, dim3(nbc), dim3(64), 0, st, A, lda, k0, kb, m, piv, w.info, cw);
       if (hipGetLastError() == hipSuccess) {
         bem_prof_end(Cx<R>::KP, st);
@@ -1495,12 +1498,16 @@ template <class R> static int getrf_t(int n, typename Cx<R>::Z *A, size_t lda, i
   }
   (void)hipMemsetAsync(info, 0, sizeof(int), st);
   if (w.coop_ok) (void)hipMemsetAsync(w.coop.err, 0, sizeof(int), st);   // error flag of all panels
+  // the largest panel is the first one (nbc workgroups): reserved until the
+  // factorisation has finished (the stream is waited for below)
+  w.ordinary = w.coop_ok && panel_reserve<R>(nbc);
   if (n > 0) lu_rec<R>(A, lda, n, 0, n, piv, w, st);
   if (w.coop_ok) {
     int herr = 0;
-    if (hipMemcpyAsync(&herr, w.coop.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess || herr != 0)
-      return -2;   // a panel barrier timed out
+    const bool ok = hipMemcpyAsync(&herr, w.coop.err, sizeof(int), hipMemcpyDeviceToHost, st) == hipSuccess &&
+                    hipStreamSynchronize(st) == hipSuccess;
+    if (w.ordinary) panel_release<R>(nbc);
+    if (!ok || herr != 0) return -2;   // a panel barrier timed out
   }
   return 0;
 }
@@ -1707,6 +1714,7 @@ static void gesv_noalloc_t(int n, int nrhs, typename Cx<R>::Z *A, typename Cx<R>
   w.nbmax = nblk;
   w.info = info;
   w.coop_ok = false;
+  w.ordinary = false;
   hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(64), 0, st, (unsigned int *)info, (size_t)1);
   lu_rec<R>(A, (size_t)n, n, 0, n, piv, w, st);
   typedef typename Cx<R>::Z Z;
@@ -1741,11 +1749,13 @@ int bem_la_init(void)
   if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess)
     g_coop = coop != 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0) {
-    int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_panel_coop<double>, 64, 0) == hipSuccess)
-      g_panel_cap[0] = b * ncu;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void *)k_panel_coop<float>, 64, 0) == hipSuccess)
-      g_panel_cap[1] = b * ncu;
+    int bd = 0, bf = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bd, (const void *)k_panel_coop<double>, 64, 0) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&bf, (const void *)k_panel_coop<float>, 64, 0) == hipSuccess &&
+        bd > 0 && bf > 0) {
+      g_panel_cap = bf * ncu;
+      g_panel_unit_d = (bf + bd - 1) / bd;
+    }
   }
   (void)hipGetLastError();
   const char *e = getenv("BEM3_HIP_PANEL_COOP");

@@ -888,18 +888,18 @@ int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double
 // products, triangular-solve chains) of different objects then overlap.
 // Streams keep the default priority.  BEM3_HIP_LANES=n (1..8) sets the
 // number of streams; 1 runs everything in the calling thread on the
-// library's stream (default 2).
+// library's stream (default 6).
 #define MO_LMAX 8
 hipStream_t g_lane_st[MO_LMAX];
 int g_nlanes = 0;     // streams created so far
-int g_lanes_req = 0;  // bh_mo_set_lanes (0: BEM3_HIP_LANES, else 2)
+int g_lanes_req = 0;  // bh_mo_set_lanes (0: BEM3_HIP_LANES, else 6)
 hipStream_t g_gpre_st = nullptr;   // the coupling loop's G1 factorisations (GPrefactor)
 
 int lanes()
 {
   int n = g_lanes_req;
   if (n <= 0) {
-    n = 2;   // measured on the 16-object scene: 1 lane 10.8 s, 2 9.1 s, 4 9.4 s, 6 9.9 s
+    n = 6;   // measured on the 16-object scene: 2 lanes 6.9 s, 4 6.35 s, 6 6.1 s, 8 6.5 s
     const char *e = getenv("BEM3_HIP_LANES");
     if (e && *e) n = std::max(1, std::min(MO_LMAX, atoi(e)));
   }

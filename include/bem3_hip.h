@@ -141,7 +141,7 @@ int bh_object_incident(bh_object *o, int which, double *buf, int set);
 /* Assembly (type as bh_object_assemble) and factorisation (mode as
  * bh_object_factor) of N objects -- create_matrix of each object
  * (solve_bieq2.c:10-96) -- with the objects spread over several device
- * streams (BEM3_HIP_LANES, default 2), so that the latency-bound pieces of
+ * streams (BEM3_HIP_LANES, default 6), so that the latency-bound pieces of
  * different objects overlap.                                              */
 int bh_mo_prepare(bh_object **objs, int N, int type, int mode);
 /* Coupling loop over N objects (mo_solve_bv, mo_solve.c:10-88): every
