@@ -1304,6 +1304,15 @@ static int dom_build_B(bh_object *o, int d)
   Dom &D = o->dom[d];
   if (!D.has_inc) return 0;   // no incident data in these rows: they stay zero
   const size_t rb = row_base(o, d);
+  if (o->prec != BH_PREC_MIXED_LM) {
+    // the incident data as two column operands a, c (nd x 4): B = -(H a + G c),
+    // two streaming products over the stored matrices (work: 12 nd values)
+    cd *w = o->ws_b.get<cd>(12 * (size_t)D.nd);
+    if (!w) return fail(BH_EHIP, "right-hand side: work area");
+    bem_build_B_multi(D.d_G, D.d_H, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, ((size_t)o->Ne + 1) * 16, (size_t)o->NN, rb,
+                      o->d_B, w, 1, cur_st());
+    return 0;
+  }
   return domain_blocks(o, d, [&](int r0, int n, const cd *G, const cd *H, const cd *, const cd *, const double *) {
     bem_launch_build_B(G, H, n, D.nd, D.nd, D.d_col, o->d_Ui, o->d_dUi, rb + r0, o->d_B, cur_st());
     return 0;

@@ -180,6 +180,7 @@ struct bh_object {
   // grow-only device work areas of the entry points (freed with the object)
   bem::Scratch ws_part, ws_mask, ws_pairs, ws_start, ws_ghdf, ws_self;   // assembly
   bem::Scratch ws_wt;                  // forming S
+  bem::Scratch ws_b;                   // right-hand side: the incident data as column operands
   // low-memory mixed form: FP64 row blocks of G, H, dG, dH, dF assembled on
   // demand; lm_raw: the FP32 storage (c_G0f: G0, c_Z: H0, c_G1: G1, c_Mf: H1)
   // holds the assembled matrices, not yet the factors

@@ -111,9 +111,10 @@ def kernel_table(bem):
         ms, work, by, n = bem.prof_read_bytes(name)
         kern[name] = (ms, work, n, by)
     tot = sum(v[0] for v in kern.values())
-    for name, (ms, work, n, _) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
+    for name, (ms, work, n, by) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
         if n:
-            log("  %-13s %9.1f ms total  %6d launches  %5.1f %% of tracked" % (name, ms, n, 100.0 * ms / max(tot, 1e-9)))
+            log("  %-13s %9.1f ms total  %6d launches  %5.1f %% of tracked  %8.1f TFLOP/s  %7.0f GB/s"
+                % (name, ms, n, 100.0 * ms / max(tot, 1e-9), work / max(ms, 1e-9) / 1e9, by / max(ms, 1e-9) / 1e6))
     return kern
 
 
