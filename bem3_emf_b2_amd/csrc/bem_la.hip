@@ -324,6 +324,166 @@ __global__ void __launch_bounds__(512) k_gemm128(int m, int n, int k, typename C
       }
 }
 
+// ---- large-tile complex-FP32 GEMM on the bf16 MFMAs ----
+// The 3M products T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi), each real
+// FP32 operand split into three bf16 parts x = x1 + x2 + x3 (x1 = bf16(x),
+// x2 = bf16(x - x1), x3 = bf16(x - x1 - x2): 24 significant bits, all of
+// FP32's) and each real product formed from the six part products whose
+// orders add to at most 4, x1 y1 + x1 y2 + x2 y1 + x2 y2 + x1 y3 + x3 y1,
+// accumulated in FP32 by v_mfma_f32_32x32x16_bf16 (the part products are
+// exact in FP32; the three dropped ones are below 2^-24 of the product).
+// The result carries FP32 rounding error like the FP32-MFMA kernel's, at
+// bf16 matrix rates (16x the FP32 MFMA rate; six products per real product).
+// 128x128 tile per 512-thread workgroup: 8 waves in a 2x4 grid, each a
+// 64x32 block of 2x1 MFMA tiles of 32x32 with three accumulators (96
+// accumulator registers: two waves per SIMD, so that one wave's operand
+// splitting and waits overlap the other's MFMAs).  A K step of 16 stages 18
+// bf16 planes in LDS (A and B, x {r, i, r+i} x 3 parts; 128 rows or columns
+// x 16 k each, the two 8-value halves of a row swapped on every fourth row
+// so that the 16-byte fragment reads of eight lanes hit distinct banks),
+// double-buffered; the next step's operands are loaded into registers
+// before the current step's MFMAs and split into the other buffer after
+// them.  Tiles are ordered per XCD as in k_gemm128.
+// MFMA operand layout (checked on gfx950): lane l holds A[l % 32][8 (l / 32)
+// + j] and B[8 (l / 32) + j][l % 32], j < 8; accumulator r of lane l is
+// C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+#define BF_T 128
+#define BF_K 16
+#define BF_PLANE (BF_T * BF_K)                                  // bf16 values per plane
+#define BF_LDS_BYTES (2 * 18 * BF_PLANE * (int)sizeof(__bf16))  // two stages of 18 planes
+
+// element offset of (row, k) in a plane: 8-value half h = k / 8 stored at
+// half h ^ ((row / 4) & 1)
+__device__ __forceinline__ int bf_off(int row, int k)
+{
+  return row * BF_K + ((((k >> 3) ^ (row >> 2)) & 1) << 3) + (k & 7);
+}
+// x = x1 + x2 + x3 in bf16 parts, into element j of v[q], v[q+1], v[q+2]
+__device__ __forceinline__ void bf_split(float x, bf16x4 *v, int q, int j)
+{
+  const __bf16 h = (__bf16)x;
+  const float r = x - (float)h;
+  const __bf16 m = (__bf16)r;
+  v[q][j] = h;
+  v[q + 1][j] = m;
+  v[q + 2][j] = (__bf16)(r - (float)m);
+}
+
+__global__ void __launch_bounds__(512, 1) k_cgemm_bf6(int m, int n, int k, cf alpha, const cf *__restrict__ A,
+                                                      size_t lda, const cf *__restrict__ B, size_t ldb, cf beta,
+                                                      cf *__restrict__ C, size_t ldc, int tiles_x, int tiles_y)
+{
+  extern __shared__ double dyn_lds[];
+  __bf16 *lds = (__bf16 *)dyn_lds;
+  const int T = tiles_x * tiles_y, per = (T + 7) / 8;
+  const int L = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (L >= T) return;
+  const int GM = 8, gw = GM * tiles_x;
+  const int first = (L / gw) * GM, gsz = min(tiles_y - first, GM);
+  const int ty = first + (L % gw) % gsz, tx = (L % gw) / gsz;
+  const int i0 = ty * BF_T, j0 = tx * BF_T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 2) * 64, wc = (wave & 3) * 32;
+  // staging: thread t holds row (A) / column (B) t % 128 and the k quarter t / 128
+  const int sr = tid & 127, sq = tid >> 7;
+  const int ga = min(i0 + sr, m - 1), gb = min(j0 + sr, n - 1);
+  const bool rowok = i0 + sr < m, colok = j0 + sr < n;
+  const int so = bf_off(sr, 4 * sq);
+  f32x16 acc[3][2];
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[p][a][r] = 0.0f;
+  cf pa[4], pb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int kk = min(k0 + 4 * sq + j, k - 1);
+      pa[j] = A[(size_t)ga * lda + kk];
+      pb[j] = B[(size_t)kk * ldb + gb];
+    }
+  };
+  // the staged operands split into the 18 bf16 planes of stage st
+  auto lstore = [&](int st, int k0) {
+    __bf16 *base = lds + st * 18 * BF_PLANE + so;
+    bf16x4 v[9];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const bool ok = rowok && k0 + 4 * sq + j < k;
+      const float x = ok ? pa[j].x : 0.0f, y = ok ? pa[j].y : 0.0f;
+      bf_split(x, v, 0, j);
+      bf_split(y, v, 3, j);
+      bf_split(x + y, v, 6, j);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) *(bf16x4 *)(base + q * BF_PLANE) = v[q];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const bool ok = colok && k0 + 4 * sq + j < k;
+      const float x = ok ? pb[j].x : 0.0f, y = ok ? pb[j].y : 0.0f;
+      bf_split(x, v, 0, j);
+      bf_split(y, v, 3, j);
+      bf_split(x + y, v, 6, j);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) *(bf16x4 *)(base + (9 + q) * BF_PLANE) = v[q];
+  };
+  gload(0);
+  lstore(0, 0);
+  __syncthreads();
+  int st = 0;
+  const int fr = lane & 31, fk = 8 * (lane >> 5);
+  const int oa0 = bf_off(wr + fr, fk), oa1 = bf_off(wr + 32 + fr, fk), ob = bf_off(wc + fr, fk);
+  for (int k0 = 0; k0 < k; k0 += BF_K) {
+    if (k0 + BF_K < k) gload(k0 + BF_K);
+    const __bf16 *base = lds + st * 18 * BF_PLANE;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      bf16x8 fa[2][3], fb[3];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        fa[0][q] = *(const bf16x8 *)(base + (3 * p + q) * BF_PLANE + oa0);
+        fa[1][q] = *(const bf16x8 *)(base + (3 * p + q) * BF_PLANE + oa1);
+        fb[q] = *(const bf16x8 *)(base + (9 + 3 * p + q) * BF_PLANE + ob);
+      }
+      // part pairs of order <= 4, the small ones first
+#pragma unroll
+      for (int e = 0; e < 6; e++) {
+        const int qa = e == 0 ? 0 : e == 1 ? 2 : e == 2 ? 1 : e == 3 ? 0 : e == 4 ? 1 : 0;
+        const int qb = e == 0 ? 2 : e == 1 ? 0 : e == 2 ? 1 : e == 3 ? 1 : e == 4 ? 0 : 0;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+          acc[p][a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a][qa], fb[qb], acc[p][a], 0, 0, 0);
+      }
+    }
+    // the next step's operands into the other stage (not read in this
+    // step); after the last step they fill the unused stage
+    lstore(st ^ 1, k0 + BF_K);
+    __syncthreads();
+    st ^= 1;
+  }
+  const bool bz = beta.x == 0.0f && beta.y == 0.0f;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int gi = i0 + wr + 32 * a + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int gj = j0 + wc + fr;
+      if (gi < m && gj < n) {
+        cf *c = C + (size_t)gi * ldc + gj;
+        const float t1 = acc[0][a][r], t2 = acc[1][a][r], t3 = acc[2][a][r];
+        cf v = alpha * mkf(t1 - t2, t3 - t1 - t2);
+        if (!bz) v = v + beta * (*c);
+        *c = v;
+      }
+    }
+}
+
 // ---- LU panel, launch-per-column form ----
 // Used where a cooperative launch is not available.  The panel (at most 64
 // columns, rows k0..n-1) is factorised in a column-major copy P
@@ -1254,6 +1414,10 @@ template <class R> static int la_init_t()
   return 0;
 }
 
+// complex-FP32 products on 128x128 tiles take the bf16-split kernel
+// (BEM3_HIP_GEMM_BF6=0: the FP32-MFMA one)
+static bool g_gemm_bf6 = true;
+
 template <class R>
 static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename Cx<R>::Z *A, size_t lda,
                    const typename Cx<R>::Z *B, size_t ldb, typename Cx<R>::Z beta, typename Cx<R>::Z *C, size_t ldc,
@@ -1283,6 +1447,9 @@ static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename 
       hipLaunchKernelGGL((k_gemm_skinny<R, 4>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     else
       hipLaunchKernelGGL((k_gemm_skinny<R, 8>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  } else if (big && sizeof(R) == sizeof(float) && g_gemm_bf6) {
+    hipLaunchKernelGGL(k_cgemm_bf6, dim3(8 * ((T + 7) / 8)), dim3(512), BF_LDS_BYTES, st, m, n, k, *(const cf *)&alpha,
+                       (const cf *)A, lda, (const cf *)B, ldb, *(const cf *)&beta, (cf *)C, ldc, tx, ty);
   } else if (big) {
     // at least one 128x128 tile per CU
     hipLaunchKernelGGL(k_gemm128<R>, dim3(8 * ((T + 7) / 8)), dim3(512), zlds_bytes<R>(), st, m, n, k, alpha, A, lda, B,
@@ -1760,6 +1927,13 @@ int bem_la_init(void)
   (void)hipGetLastError();
   const char *e = getenv("BEM3_HIP_PANEL_COOP");
   g_panel_ordinary = !(e && strcmp(e, "1") == 0);
+  const char *eb = getenv("BEM3_HIP_GEMM_BF6");
+  g_gemm_bf6 = !(eb && strcmp(eb, "0") == 0);
+  if (g_gemm_bf6 &&
+      hipFuncSetAttribute((const void *)k_cgemm_bf6, hipFuncAttributeMaxDynamicSharedMemorySize, BF_LDS_BYTES) != hipSuccess) {
+    (void)hipGetLastError();
+    g_gemm_bf6 = false;
+  }
   return 0;
 }
 

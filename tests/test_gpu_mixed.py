@@ -27,9 +27,12 @@ def _rel(a, b):
 
 
 # (3001, 2999, 77) and (4096, 4096, 64): 128x128-tile kernel; n <= 8 with
-# k >= 256: the streaming kernel; the rest: 64x64 tiles
+# k >= 256: the streaming kernel; 128x128 tiles (the bf16-split kernel) from
+# 160 tiles, or 64 tiles with k >= 1024 (partial tiles and a k that is no
+# multiple of the 16-value step included); the rest: 64x64 tiles
 @pytest.mark.parametrize("m,n,k", [(64, 64, 16), (100, 70, 37), (3001, 2999, 77), (4096, 4096, 64), (3001, 1, 500),
-                                   (1000, 6, 2048), (17, 8, 999), (257, 130, 64)])
+                                   (1000, 6, 2048), (17, 8, 999), (257, 130, 64), (2049, 2050, 1031), (130, 9000, 2000),
+                                   (1537, 1100, 33)])
 def test_cgemm_matches_numpy(m, n, k):
     import bem3_emf_b2_amd as bem
     rng = np.random.default_rng(m * 1000 + n + k + 1)
