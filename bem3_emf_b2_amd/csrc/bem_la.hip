@@ -721,6 +721,23 @@ __device__ __forceinline__ void coop_barrier(unsigned int *cnt, int *err, unsign
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// S[c * 64 + lane] -= l sp[c] for c in [c0, kb): eight columns at a time,
+// their loads issued before the first update (one LDS round trip per eight
+// columns instead of one per column)
+template <class Z>
+__device__ __forceinline__ void panel_update(Z *S, const Z *sp, Z l, int c0, int kb, int lane)
+{
+  int c = c0;
+  for (; c + 8 <= kb; c += 8) {
+    Z a[8], p[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) { a[u] = S[(c + u) * 64 + lane]; p[u] = sp[c + u]; }
+#pragma unroll
+    for (int u = 0; u < 8; u++) S[(c + u) * 64 + lane] = a[u] - l * p[u];
+  }
+  for (; c < kb; c++) S[c * 64 + lane] = S[c * 64 + lane] - l * sp[c];
+}
+
 template <class R>
 __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict__ A, size_t lda, int k0, int kb, int m,
                                                    int *__restrict__ piv, int *__restrict__ info, CoopWs w)
@@ -788,7 +805,7 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
     if (live && !used && (a.x != 0 || a.y != 0)) {
       const Z l = cdiv(S[j * 64 + lane], a);
       S[j * 64 + lane] = l;
-      for (int c = j + 1; c < kb; c++) S[c * 64 + lane] = S[c * 64 + lane] - l * sp[c];
+      panel_update(S, sp, l, j + 1, kb, lane);
     }
     __syncthreads();
   }
@@ -800,6 +817,115 @@ __global__ void __launch_bounds__(64) k_panel_coop(typename Cx<R>::Z *__restrict
   if (blk == 0 && lane < kb) piv[k0 + lane] = k0 + ld_rel(w.ip + lane);
   // every row to its final position (a permutation of 0..m-1; all reads of
   // the panel happened before the first barrier)
+  if (live)
+    for (int c = 0; c < kb; c++) A[(size_t)(k0 + pos) * lda + k0 + c] = S[c * 64 + lane];
+}
+
+// The complex-FP32 panel without a grid barrier: the same steps as
+// k_panel_coop<float> (the same pivots), but each workgroup publishes its
+// candidate for column j as one 8-byte word -- |re| + |im| as a float (the
+// exact value the FP32 entries give), the row (24 bits) and the tag j + 1
+// (8 bits) -- in slot [j & 1][workgroup] (cleared before the launch) after
+// its candidate row's stores have completed, and every workgroup reads the
+// slots of column j until they carry its tag.  The candidates' arrival is
+// then the barrier: a column costs the store of the candidate row, the
+// poll of the slots and the load of the pivot row, where the counter
+// barrier adds two atomic round trips and a separate load of the
+// candidates.  A workgroup runs at most one column ahead of any other (it
+// needs everyone's slot of the column before), so two slot and row buffers
+// suffice.  Step kb (tag kb + 1) only waits for every workgroup: after it
+// nobody reads the panel, and the rows are stored at their final
+// positions.  Every wait is bounded (error flag on timeout).
+__global__ void __launch_bounds__(64) k_panel_slot(cf *__restrict__ A, size_t lda, int k0, int kb, int m,
+                                                   int *__restrict__ piv, int *__restrict__ info, CoopWs w)
+{
+  typedef cf Z;
+  __shared__ Z S[64 * 64];   // S[c * 64 + lane]: column c of this workgroup's 64 rows
+  __shared__ Z sp[64];       // pivot row of the current column
+  Z *candRow = (Z *)w.candRow;
+  unsigned long long *slot = (unsigned long long *)w.candV;   // [2][nbmax]
+  const int lane = threadIdx.x, blk = blockIdx.x, nb = gridDim.x;
+  const int r = blk * 64 + lane;
+  const bool live = r < m;
+  for (int c = 0; c < kb; c++) S[c * 64 + lane] = live ? A[(size_t)(k0 + r) * lda + k0 + c] : zmk<Z>(0, 0);
+  bool used = false;
+  int pos = r;
+  for (int j = 0; j <= kb; j++) {
+    const int buf = j & 1;
+    double v = -1.0;
+    int vi = 0xffffff;
+    if (j < kb && live && !used) {
+      v = zabs1(S[j * 64 + lane]);        // |re|+|im| as LAPACK izamax
+      if (!(v >= 0.0)) v = INFINITY;      // a NaN entry still wins over non-candidates
+      vi = r;
+    }
+    wave_argmax(v, vi);
+    if (j < kb && v >= 0.0 && lane >= j && lane < kb)
+      candRow[((size_t)buf * w.nbmax + blk) * 64 + lane] = S[lane * 64 + (vi - blk * 64)];
+    __builtin_amdgcn_s_waitcnt(0);   // the candidate row (and last column's interchange) stores have completed
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == 0) {
+      const unsigned int fv = (unsigned int)__float_as_int((float)v);
+      const unsigned long long word =
+          (unsigned long long)fv | ((unsigned long long)(((unsigned int)vi & 0xffffffu) | ((unsigned int)(j + 1) << 24)) << 32);
+      __hip_atomic_store(slot + (size_t)buf * w.nbmax + blk, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // every workgroup's candidate for this column: lane q reads slots q, q + 64, ...
+    double b = -1.0;
+    int bi = 0x7fffffff;
+    for (int q0 = 0; q0 < nb; q0 += 64) {
+      const int q = q0 + lane;
+      if (q < nb) {
+        unsigned long long word = 0;
+        int spins = 0;
+        for (;;) {
+          word = __hip_atomic_load(slot + (size_t)buf * w.nbmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((int)(word >> 56) == j + 1) break;
+          if (++spins > COOP_SPIN_LIMIT || __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            atomicExch(w.err, 1);
+            word = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        const double fv = (double)__int_as_float((int)(unsigned int)word);
+        const int fi = (int)((word >> 32) & 0xffffffu);
+        if ((int)(word >> 56) == j + 1 && (fv > b || (fv == b && fi < bi))) { b = fv; bi = fi; }
+      }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (j == kb) break;   // the last step only waited for every workgroup
+    wave_argmax(b, bi);
+    // the interchange of column j-1, published before its owner's slot of this column
+    const int cj = j > 0 ? ld_rel(w.ip + j - 1) : 0;
+    if (j > 0) {
+      if (pos == j - 1) pos = cj; else if (pos == cj) pos = j - 1;
+    }
+    int bw;
+    if (bi < 0 || bi >= m) { bi = m - 1; if (blk == 0 && lane == 0 && *info == 0) *info = k0 + j + 1; }
+    bw = bi / 64;
+    if (blk == 0 && lane == 0 && !(b > 0.0) && *info == 0) *info = k0 + j + 1;
+    if (lane >= j && lane < kb) sp[lane] = ld_x(candRow + ((size_t)buf * w.nbmax + bw) * 64 + lane);
+    __syncthreads();
+    if (r == bi) {
+      used = true;
+      w.ip[j] = pos;   // LAPACK interchange: column j's position <-> the pivot row's current position
+    }
+    const Z a = sp[j];
+    if (live && !used && (a.x != 0 || a.y != 0)) {
+      const Z l = cdiv(S[j * 64 + lane], a);
+      S[j * 64 + lane] = l;
+      panel_update(S, sp, l, j + 1, kb, lane);
+    }
+    __syncthreads();
+  }
+  {
+    const int cj = ld_rel(w.ip + kb - 1);
+    if (pos == kb - 1) pos = cj; else if (pos == cj) pos = kb - 1;
+  }
+  if (blk == 0 && lane < kb) piv[k0 + lane] = k0 + ld_rel(w.ip + lane);
+  // every row to its final position (a permutation of 0..m-1; all reads of
+  // the panel happened before step kb)
   if (live)
     for (int c = 0; c < kb; c++) A[(size_t)(k0 + pos) * lda + k0 + c] = S[c * 64 + lane];
 }
@@ -1487,6 +1613,7 @@ static inline int half64(int n) { return std::max(64, (n / 2) / 64 * 64); }
 static int g_panel_cap = 0, g_panel_unit_d = 2;
 static std::atomic<long> g_panel_held{0};
 static bool g_panel_ordinary = true;
+static bool g_panel_slot = true;   // BEM3_HIP_PANEL_SLOT=0: complex-FP32 panels by the counter barrier
 template <class R> static long panel_units(int nbc) { return (long)nbc * (sizeof(R) == sizeof(float) ? 1 : g_panel_unit_d); }
 template <class R> static bool panel_reserve(int nbc)
 {
@@ -1518,15 +1645,24 @@ static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, in
     // of all streams that factorise at once fit the device together
     // (lanes x workgroups <= resident capacity), the only other waiters
     // being the triangular solves' blocks, which always progress.
+    // complex FP32: the slot form (k_panel_slot), its slots cleared first
+    const void *fn = (const void *)k_panel_coop<R>;
+    if (sizeof(R) == sizeof(float) && g_panel_slot) {
+      fn = (const void *)k_panel_slot;
+      (void)hipMemsetAsync(cw.candV, 0, 2 * (size_t)cw.nbmax * sizeof(double), st);
+    }
     if (w.ordinary) {
-      hipLaunchKernelGGL(k_panel_coop<R>, dim3(nbc), dim3(64), 0, st, A, lda, k0, kb, m, piv, w.info, cw);
+      if (fn == (const void *)k_panel_slot)
+        hipLaunchKernelGGL(k_panel_slot, dim3(nbc), dim3(64), 0, st, (cf *)A, lda, k0, kb, m, piv, w.info, cw);
+      else
+        hipLaunchKernelGGL(k_panel_coop<R>, dim3(nbc), dim3(64), 0, st, A, lda, k0, kb, m, piv, w.info, cw);
       if (hipGetLastError() == hipSuccess) {
         bem_prof_end(Cx<R>::KP, st);
         return;
       }
     }
     void *args[] = {&A, &lda, (void *)&k0, (void *)&kb, (void *)&m, &piv, (void *)&w.info, &cw};
-    if (hipLaunchCooperativeKernel((const void *)k_panel_coop<R>, dim3(nbc), dim3(64), args, 0, st) == hipSuccess) {
+    if (hipLaunchCooperativeKernel(fn, dim3(nbc), dim3(64), args, 0, st) == hipSuccess) {
       bem_prof_end(Cx<R>::KP, st);
       return;
     }
@@ -1927,6 +2063,8 @@ int bem_la_init(void)
   (void)hipGetLastError();
   const char *e = getenv("BEM3_HIP_PANEL_COOP");
   g_panel_ordinary = !(e && strcmp(e, "1") == 0);
+  const char *es = getenv("BEM3_HIP_PANEL_SLOT");
+  g_panel_slot = !(es && strcmp(es, "0") == 0);
   const char *eb = getenv("BEM3_HIP_GEMM_BF6");
   g_gemm_bf6 = !(eb && strcmp(eb, "0") == 0);
   if (g_gemm_bf6 &&
