@@ -963,12 +963,13 @@ __global__ void __launch_bounds__(256) k_panel_perm(const int *__restrict__ pv, 
 // chunk, so the dependent loop over them holds no global loads of piv and
 // no iterations without a swap.
 #define LASWP_CHUNK 2048
+// the interchanges of rows k0..k1-1 in column c (< c1: this thread's
+// column), every thread of the 256-thread workgroup taking part
 template <class Z>
-__global__ void __launch_bounds__(256) k_laswp(Z *A, size_t lda, int c0, int c1, int k0, int k1, const int *piv)
+__device__ __forceinline__ void laswp_body(Z *A, size_t lda, int c, int c1, int k0, int k1, const int *piv)
 {
   __shared__ int sj[LASWP_CHUNK], sp[LASWP_CHUNK];
   __shared__ int cnt, wbase[4];
-  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int j0 = k0; j0 < k1; j0 += LASWP_CHUNK) {
     const int len = min(LASWP_CHUNK, k1 - j0);
@@ -1004,6 +1005,11 @@ __global__ void __launch_bounds__(256) k_laswp(Z *A, size_t lda, int c0, int c1,
       }
     __syncthreads();   // the chunk's tables are reused
   }
+}
+template <class Z>
+__global__ void __launch_bounds__(256) k_laswp(Z *A, size_t lda, int c0, int c1, int k0, int k1, const int *piv)
+{
+  laswp_body<Z>(A, lda, c0 + blockIdx.x * blockDim.x + threadIdx.x, c1, k0, k1, piv);
 }
 
 // Triangular solves with one nb x nb (nb <= 64) diagonal block for ncol
@@ -1475,28 +1481,13 @@ template <class Z> struct LaswpSys {
 template <class Z> struct LaswpArgs {
   LaswpSys<Z> s[TRSVB_MAXSYS];
 };
-template <class Z> __global__ void __launch_bounds__(64) k_laswp_batch(LaswpArgs<Z> a, int nc)
+template <class Z> __global__ void __launch_bounds__(256) k_laswp_batch(LaswpArgs<Z> a, int nc)
 {
   LaswpSys<Z> q = a.s[0];
 #pragma unroll
   for (int k = 1; k < TRSVB_MAXSYS; k++)
     if ((int)blockIdx.x == k) q = a.s[k];
-  const int c = threadIdx.x;
-  __shared__ int sp[256];
-  for (int j0 = 0; j0 < q.n; j0 += 256) {
-    const int len = min(256, q.n - j0);
-    for (int t = threadIdx.x; t < len; t += 64) sp[t] = q.piv[j0 + t];
-    __syncthreads();
-    if (c < nc)
-      for (int t = 0; t < len; t++) {
-        const int j = j0 + t, pv = sp[t];
-        if (pv == j) continue;
-        const Z v = q.B[(size_t)j * q.ldb + c];
-        q.B[(size_t)j * q.ldb + c] = q.B[(size_t)pv * q.ldb + c];
-        q.B[(size_t)pv * q.ldb + c] = v;
-      }
-    __syncthreads();
-  }
+  laswp_body<Z>(q.B, q.ldb, threadIdx.x, nc, 0, q.n, q.piv);
 }
 
 __global__ void k_zero_bytes(unsigned int *p, size_t n)
@@ -1636,8 +1627,10 @@ static void lu_panel(typename Cx<R>::Z *A, size_t lda, int n, int k0, int kb, in
   const int nbc = (m + 63) / 64;
   if (w.coop_ok && nbc <= w.coop.nbmax) {
     CoopWs cw = w.coop;
-    // arrival counters of this launch: the top one and one per group
-    (void)hipMemsetAsync(cw.cnt, 0, (size_t)COOP_CSTRIDE * sizeof(unsigned int) * (1 + (nbc + COOP_GS - 1) / COOP_GS), st);
+    // arrival counters of this launch (the counter barrier): the top one
+    // and one per group
+    if (!(sizeof(R) == sizeof(float) && g_panel_slot))
+      (void)hipMemsetAsync(cw.cnt, 0, (size_t)COOP_CSTRIDE * sizeof(unsigned int) * (1 + (nbc + COOP_GS - 1) / COOP_GS), st);
     // An ordinary launch lets the work of other streams (other objects'
     // products and panels) run beside the panel; a cooperative launch runs
     // alone on the device.  The grid barrier needs every workgroup of the
@@ -1945,7 +1938,7 @@ static int getrs_batch_t(int N, const int *n, int nrhs, const typename Cx<R>::Z 
   }
   LaswpArgs<Z> la{};
   for (int q = 0; q < N; q++) la.s[q] = LaswpSys<Z>{B[q], piv[q], ldb[q], n[q], 0};
-  hipLaunchKernelGGL(k_laswp_batch<Z>, dim3(N), dim3(64), 0, st, la, nrhs);
+  hipLaunchKernelGGL(k_laswp_batch<Z>, dim3(N), dim3(256), 0, st, la, nrhs);
   if (!trsv_batch<R, false>(N, n, nrhs, A, lda, B, ldb, st)) return -1;
   if (!trsv_batch<R, true>(N, n, nrhs, A, lda, B, ldb, st)) return -1;
   return 0;

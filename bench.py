@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 # MI355X published peaks (dense)
 PEAK_FP64_TFLOPS = 78.6     # FP64 vector and FP64 matrix
 PEAK_FP32_TFLOPS = 157.3    # FP32 matrix
+PEAK_BF16_TFLOPS = 2500.0   # bf16 matrix (dense)
 PEAK_HBM_GBS = 8000.0
 
 METRIC = "assembly+solve+coupling s/config vs plain FP64 GPU at 1e-6 field error"
@@ -356,18 +357,19 @@ def beams_extra(bem, nb=64):
 
 
 # ------------------------------------------------------------------ roofline
-# k_gemm128<float> over the 16-object scene's configuration (3932 launches of
-# every shape the mixed form's factorisations and solves take; one stream):
-# FETCH_SIZE 328,871 KiB x 2.0 + WRITE_SIZE 61,001 KiB x 1.0 per launch on
-# average (profiles/gemm128_fp32_scene16_counters.txt)
-SCENE_GEMM128_FP32_TRAFFIC = {"bytes": (328871.4 * 2.0 + 61000.9 * 1.0) * 1024,
-                              "shape": "16-object scene, mean over 3932 launches"}
+# k_cgemm_bf6 over the 16-object scene's configuration (every launch of every
+# shape the mixed form's factorisations and solves take; one stream): HBM
+# bytes per launch on average from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
+# each scaled by the reading of a 1 GiB device copy
+# (profiles/cgemm_bf6_scene16_counters.txt)
+SCENE_GEMM_BF6_TRAFFIC = {"bytes": (327675.9 * 2.0 + 60893.3 * 1.0) * 1024,
+                          "shape": "16-object scene, mean over 3932 launches"}
 # kernels whose registry work is real flops: (name, bound, peak TFLOP/s)
-FLOP_KERNELS = {"zgemm": ("k_gemm128<double>", PEAK_FP64_TFLOPS), "cgemm": ("k_gemm128<float>", PEAK_FP32_TFLOPS),
+FLOP_KERNELS = {"zgemm": ("k_gemm128<double>", PEAK_FP64_TFLOPS), "cgemm": ("k_cgemm_bf6", PEAK_BF16_TFLOPS),
                 "zgemm_small": ("k_gemm64<double>", PEAK_FP64_TFLOPS), "cgemm_small": ("k_gemm64<float>", PEAK_FP32_TFLOPS),
                 "zgemv": ("k_gemm_skinny<double>", PEAK_FP64_TFLOPS), "cgemv": ("k_gemm_skinny<float>", PEAK_FP32_TFLOPS),
                 "trsm": ("triangular solves (FP64)", PEAK_FP64_TFLOPS), "trsm_c": ("triangular solves (FP32)", PEAK_FP32_TFLOPS),
-                "lu_panel": ("k_panel_coop<double>", PEAK_FP64_TFLOPS), "lu_panel_c": ("k_panel_coop<float>", PEAK_FP32_TFLOPS),
+                "lu_panel": ("k_panel_coop<double>", PEAK_FP64_TFLOPS), "lu_panel_c": ("k_panel_slot", PEAK_FP32_TFLOPS),
                 "mp_moments": ("k_mp_moments (FP64 VALU)", PEAK_FP64_TFLOPS)}
 # FP64 operations per quadrature point of the far-pair kernels (counted on
 # the gfx950 ISA of the point loop: v_fma/mul/add_f64 count 2/1/1, the
@@ -379,13 +381,25 @@ def roofline(dom, rec):
     """roofline entry of one tracked kernel from its (ms, work, launches)"""
     ms, work, n, by = rec
     avg_s = ms / 1000.0 / max(n, 1)
+    if dom == "cgemm":
+        # complex FP32 product in the 3M form, each real product as six bf16
+        # part products: the MFMA pipes execute 36 m n k flops per 8 m n k of
+        # the complex product
+        name, peak = FLOP_KERNELS[dom]
+        alg = work / n / avg_s / 1e12
+        ach = 4.5 * alg
+        t = SCENE_GEMM_BF6_TRAFFIC
+        return {"kernel": name, "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                "traffic": t["bytes"] if t else None, "traffic_shape": t["shape"] if t else None,
+                "algorithmic_bytes_per_launch": by / n if by else None,
+                "avg_launch_ms": 1000.0 * avg_s, "work_per_launch": 4.5 * work / n,
+                "work_unit": "bf16 MFMA flops (36 m n k: 3 real products of the 3M form x 6 bf16 part products)",
+                "complex_fp32_tflops_8mnk": alg, "vs_fp32_matrix_peak": alg / PEAK_FP32_TFLOPS}
     if dom in FLOP_KERNELS:
         name, peak = FLOP_KERNELS[dom]
         ach = work / n / avg_s / 1e12
-        t = SCENE_GEMM128_FP32_TRAFFIC if dom == "cgemm" else None
         return {"kernel": name, "bound": "mfma" if "VALU" not in name else "valu-fp64", "achieved": ach, "peak": peak,
-                "unit": "TFLOP/s", "frac": ach / peak, "traffic": t["bytes"] if t else None,
-                "traffic_shape": t["shape"] if t else None,
+                "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
                 "algorithmic_bytes_per_launch": by / n if by else None,
                 "avg_launch_ms": 1000.0 * avg_s, "work_per_launch": work / n, "work_unit": "real flops"}
     if dom in FLOP_PER_POINT:
