@@ -973,9 +973,15 @@ extern "C" int bh_mo_prepare(bh_object **objs, int N, int type, int mode)
   API_LOCK;
   if (N <= 0) return fail(BH_EARG, "no objects");
   if (ensure_rules()) return BH_EHIP;
+  // the mixed form's FP32 LU of G1 (the boundary fields' dE/dH solves,
+  // once per assembly) in the same pass unless BEM3_HIP_G1_AHEAD=0
+  const char *e = getenv("BEM3_HIP_G1_AHEAD");
+  const bool g1 = !(e && strcmp(e, "0") == 0);
   return run_lanes(objs, N, cost_of(objs, N, 3), [&](bh_object *o) {
     int r = object_assemble(o, type);
-    return r ? r : object_factor(o, mode);
+    if (!r) r = object_factor(o, mode);
+    if (!r && g1) r = object_factor_g(o);
+    return r;
   });
 }
 
