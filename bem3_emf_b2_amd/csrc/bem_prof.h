@@ -7,3 +7,4 @@
 extern "C" void bem_prof_begin(int kid, double work, hipStream_t st);
 extern "C" void bem_prof_begin_b(int kid, double work, double bytes, hipStream_t st);
 extern "C" void bem_prof_end(int kid, hipStream_t st);
+extern "C" int bem_prof_on(void);

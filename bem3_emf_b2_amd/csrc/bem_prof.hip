@@ -40,6 +40,10 @@ hipEvent_t take()
 
 extern "C" {
 
+// whether the registry records launches (its events cannot go into a
+// captured graph)
+int bem_prof_on(void) { return P.on ? 1 : 0; }
+
 // destroys the event pool (library shutdown)
 void bem_prof_release(void)
 {
