@@ -82,6 +82,12 @@ def lib():
             "bh_elem_coef": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, Ip, Ip, D, D, D, D, D]),
             "bh_mo_force_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, D, D, Ip]),
             "bh_mo_far_field_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, D, C.c_int, D]),
+            "bh_tune_set": (C.c_int, [C.c_int, C.c_int]),
+            "bh_tune_get": (C.c_int, [C.c_int, Ip]),
+            "bh_tune_load": (C.c_int, [C.c_char_p]),
+            "bh_tune_name": (C.c_char_p, [C.c_int]),
+            "bh_tune_source": (C.c_char_p, []),
+            "bh_gemm_time": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -408,13 +414,50 @@ def zgemm(A, B, Cm=None, alpha=1.0, beta=0.0):
     return Cm
 
 
+# entries of the tuning table (include/bem3_hip.h, BH_TUNE_*)
+TUNE_KEYS = {"lanes": 0, "asm_batch": 1, "gemm_tiles_fp64": 2, "gemm_tiles_fp32": 3, "gemm_longk_tiles": 4,
+             "gemm_longk_k": 5, "gemm_bf6": 6}
+
+
+def tune_get(name=None):
+    """one entry of the tuning table, or all of them as a dict"""
+    if name is None:
+        return {k: tune_get(k) for k in TUNE_KEYS}
+    v = C.c_int(0)
+    check(lib().bh_tune_get(TUNE_KEYS[name], C.byref(v)))
+    return v.value
+
+
+def tune_set(name, value):
+    check(lib().bh_tune_set(TUNE_KEYS[name], int(value)))
+
+
+def tune_load(path):
+    check(lib().bh_tune_load(os.fsencode(path)))
+
+
+def tune_source():
+    return lib().bh_tune_source().decode()
+
+
+GEMM_KERNELS = {"64x64": 0, "128x128": 1, "bf16split": 2}
+
+
+def gemm_time(prec, kernel, m, n, k, reps=5):
+    """mean ms of one product kernel on the device (prec "fp64" or "fp32";
+    kernel "64x64", "128x128" or "bf16split")"""
+    ms = C.c_double(0.0)
+    check(lib().bh_gemm_time({"fp64": 0, "fp32": 1}[prec], GEMM_KERNELS[kernel], m, n, k, reps, C.byref(ms)))
+    return ms.value
+
+
 def _handles(objs):
     return (C.c_void_p * len(objs))(*[o.h.value for o in objs])
 
 
 def mo_prepare(objs, type_=2, mode="auto"):
-    """assembly and factorisation of every object, the objects spread over
-    several device streams"""
+    """assembly of every object as one batch, then the factorisations spread
+    over several device streams"""
     check(lib().bh_mo_prepare(_handles(objs), len(objs), type_, FACTOR_MODES[mode]))
 
 

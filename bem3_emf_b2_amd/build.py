@@ -8,7 +8,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libbem3hip.so")
-SOURCES = ["bem_asm.hip", "bem_la.hip", "bem_field.hip", "bem_bie.hip", "bem_api.hip", "bem_prof.hip", "bem_schur.hip", "bem_mo.hip"]
+SOURCES = ["bem_asm.hip", "bem_la.hip", "bem_field.hip", "bem_bie.hip", "bem_api.hip", "bem_prof.hip", "bem_schur.hip", "bem_mo.hip", "bem_tune.hip"]
 # host side of the drop-in API (include/bem3_emf_b2.h), plain C
 HOST_SOURCES = ["host/bem_host.c"]
 # command-line programs of the reference, built against the drop-in API

@@ -7,24 +7,19 @@
 // (the message is kept for bh_last_error()).
 #include "bem_obj.h"
 
+#include <climits>
+
 using namespace bem;
 
 extern "C" {
-void bem_launch_asm_gl(const void *, const void *, const int *, const void *, const int *, int, int, double, double,
-                       const void *, int, const void *, int, void *, void *, void *, void *, size_t, double *,
-                       unsigned long long *, int, hipStream_t);
-void bem_launch_asm_gh(const void *, const void *, const int *, const void *, const int *, const void *, int, double,
-                       double, const void *, int, const void *, int, const void *, int, const void *, int, void *, void *,
-                       void *, void *, size_t, double *, hipStream_t);
-void bem_launch_asm_glr(const void *, const void *, const int *, const void *, const int *, int, int, double, double,
-                        const void *, const void *, int, void *, void *, void *, void *, size_t, double *,
-                        unsigned long long *, int, hipStream_t);
+void bem_launch_asm_p1(int, const void *, const int *, int, int, double, const void *, int, const void *, int,
+                       hipStream_t);
+void bem_launch_asm_p2(const void *, const int *, int, int, const void *, int, const void *, int, const void *, int,
+                       const void *, int, hipStream_t);
+void bem_launch_asm_p34(const void *, const int *, int, int, const double *, const double *, const double *,
+                        const double *, hipStream_t);
 int bem_rule32(const void *, int, void **);
-void bem_launch_asm_self(const void *, const void *, const int *, const void *, const int *, int, double, double,
-                         const double *, const double *, const double *, const double *, void *, void *, void *, void *,
-                         size_t, double *, hipStream_t);
-void bem_launch_asm_fin(const double *, int, const double *, const int *, const double *, int, int, void *, size_t,
-                        double *, int, hipStream_t);
+int bem_gemm_time(int, int, int, int, int, int, double *, hipStream_t);
 void bem_zgemm(int, int, int, double, double, const void *, size_t, const void *, size_t, double, double, void *, size_t,
                hipStream_t);
 int bem_zgetrf(int, void *, size_t, int *, void *, int *, int, hipStream_t);
@@ -131,6 +126,10 @@ extern "C" int bh_init(int device)
   int n = 0;
   HIPCHK(hipGetDeviceCount(&n));
   if (n <= device) return fail(BH_ENODEV, "no HIP device %d (found %d)", device, n);
+  {
+    int rc = tune_init();
+    if (rc) return rc;
+  }
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreate(&g.st));
   HIPCHK(hipEventCreate(&g.ev[0]));
@@ -162,6 +161,18 @@ extern "C" int bh_init(int device)
     HIPCHK(hipStreamSynchronize(cur_st()));
   }
   g.dev = device;
+  return 0;
+}
+
+extern "C" int bh_gemm_time(int prec, int kernel, int m, int n, int k, int reps, double *ms)
+{
+  API_LOCK;
+  if (ensure_init()) return BH_ENODEV;
+  if (prec < 0 || prec > 1 || kernel < 0 || kernel > 2 || m < 1 || n < 1 || k < 1 || reps < 1 || !ms)
+    return fail(BH_EARG, "bh_gemm_time: prec %d kernel %d shape %d x %d x %d reps %d", prec, kernel, m, n, k, reps);
+  const int rc = bem_gemm_time(prec, kernel, m, n, k, reps, ms, cur_st());
+  if (rc == -2) return fail(BH_EUNSUP, "bh_gemm_time: kernel %d is not available for precision %d", kernel, prec);
+  if (rc) return fail(BH_EHIP, "bh_gemm_time: %s", hipGetErrorString(hipGetLastError()));
   return 0;
 }
 
@@ -425,66 +436,168 @@ template <class T> static int dalloc(T **p, size_t n)
   return 0;
 }
 
-// rows r0 .. r0+nrows of domain d's matrices into G, H, dG, dH (row stride
-// ld = the domain's node count, the block's first row first) and dF (3 per
-// row); *ngh receives the near pairs redone with the 64-point rule
-static int assemble_rows(bh_object *o, int d, int r0, int nrows, cd *G, cd *H, cd *dG, cd *dH, double *dF, int *ngh)
+// one block of rows of one domain in a batched assembly: rows r0 ..
+// r0+nrows of domain d's matrices into G, H, dG, dH (row stride = the
+// domain's node count, the block's first row first) and dF (3 per row);
+// *ngh receives the near pairs redone with the 64-point rule
+struct AsmSpec {
+  bh_object *o;
+  int d, r0, nrows;
+  cd *G, *H, *dG, *dH;
+  double *dF;
+  int *ngh;
+};
+
+// Passes 1-4 of the assembly for every spec of the batch at once: one launch
+// per pass over all of them (the objects and domains of bh_mo_prepare, or
+// the domains of one object), one read-back of the near-pair flags, one
+// upload of the pair lists.  The specs of one object carve its grow-only
+// work areas between them.  All objects of a batch take one precision form.
+static int assemble_batch(const std::vector<AsmSpec> &sp)
 {
-  Dom &D = o->dom[d];
-  const size_t nd = (size_t)D.nd, nr = (size_t)nrows;
-  const int nblk = (D.dNe + 255) / 256, nwv = nblk * 4;
-  // work areas: grow-only, owned by the object
-  double *d_part = o->ws_part.get<double>(nr * nwv * 3);
-  unsigned long long *d_mask = o->ws_mask.get<unsigned long long>(nr * nwv);
-  if (!d_part || !d_mask) return fail(BH_EHIP, "assembly: work area of %zu rows", nr);
-  double kr = o->kn[d].real(), ki = o->kn[d].imag();
-  const bool mixed = o->prec != BH_PREC_FP64;
-  const RowPt *rows = D.d_rows + r0;
+  const int nj = (int)sp.size();
+  if (nj == 0) return 0;
+  const bool mixed = sp[0].o->prec != BH_PREC_FP64;
+  for (const AsmSpec &x : sp)
+    if ((x.o->prec != BH_PREC_FP64) != mixed) return fail(BH_EARG, "batched assembly: objects of both precision forms");
+  // objects of the batch and each spec's offsets into its object's areas
+  std::vector<bh_object *> obs;
+  std::vector<int> oi(nj);
+  for (int j = 0; j < nj; j++) {
+    int k = (int)(std::find(obs.begin(), obs.end(), sp[j].o) - obs.begin());
+    if (k == (int)obs.size()) obs.push_back(sp[j].o);
+    oi[j] = k;
+  }
+  const int no = (int)obs.size();
+  std::vector<int> nblk(nj), nwv(nj);
+  std::vector<size_t> offM(nj), offS(nj), offT(nj), totM(no, 0), totS(no, 0), totT(no, 0);
+  std::vector<int> f1(nj + 1, 0), f2(nj + 1, 0), f3(nj + 1, 0);
+  size_t b1 = 0, r3 = 0;
+  double work = 0.0;
+  for (int j = 0; j < nj; j++) {
+    const Dom &D = sp[j].o->dom[sp[j].d];
+    const size_t nr = (size_t)sp[j].nrows;
+    nblk[j] = (D.dNe + 255) / 256;
+    nwv[j] = nblk[j] * 4;
+    offM[j] = totM[oi[j]]; totM[oi[j]] += nr * nwv[j];   // masks; partial F sums 3 per mask
+    offS[j] = totS[oi[j]]; totS[oi[j]] += nr * 3;        // self F terms
+    offT[j] = totT[oi[j]]; totT[oi[j]] += nr + 1;        // row starts of the pair lists
+    f1[j] = (int)b1; b1 += nr * nblk[j];
+    f3[j] = (int)r3; r3 += nr;
+    work += (double)nr * D.dNe;
+  }
+  if (b1 > (size_t)INT_MAX || r3 > (size_t)INT_MAX / 64) return fail(BH_EARG, "batched assembly: %zu workgroups", b1);
+  f1[nj] = (int)b1; f3[nj] = (int)r3;
+  std::vector<double *> part(no), self(no);
+  std::vector<unsigned long long *> mask(no);
+  std::vector<int *> start(no);
+  for (int k = 0; k < no; k++) {
+    part[k] = obs[k]->ws_part.get<double>(totM[k] * 3);
+    mask[k] = obs[k]->ws_mask.get<unsigned long long>(totM[k]);
+    self[k] = obs[k]->ws_self.get<double>(totS[k]);
+    start[k] = obs[k]->ws_start.get<int>(totT[k]);
+    if (!part[k] || !mask[k] || !self[k] || !start[k]) return fail(BH_EHIP, "assembly: work areas of %zu rows", totS[k] / 3);
+  }
+  std::vector<AsmJob> jobs(nj);
+  for (int j = 0; j < nj; j++) {
+    const AsmSpec &x = sp[j];
+    const Dom &D = x.o->dom[x.d];
+    AsmJob &J = jobs[j];
+    J.geo = (const TriGeo *)x.o->d_geo; J.gq = (const QuadExt *)x.o->d_gq; J.sid = D.d_sid; J.rows = D.d_rows + x.r0;
+    J.colOff = D.d_colOff;
+    J.G = x.G; J.H = x.H; J.dG = x.dG; J.dH = x.dH; J.dF = x.dF;
+    J.dfPart = part[oi[j]] + offM[j] * 3; J.ghMask = mask[oi[j]] + offM[j];
+    J.pairs = nullptr; J.ghStart = start[oi[j]] + offT[j]; J.ghDF = nullptr;
+    J.selfDF = self[oi[j]] + offS[j];
+    J.kr = x.o->kn[x.d].real(); J.ki = x.o->kn[x.d].imag();
+    J.ld = (size_t)D.nd; J.dNe = D.dNe; J.nrows = x.nrows; J.nblk = nblk[j]; J.r0 = x.r0; J.open = x.d == 0; J.pad = 0;
+  }
+  // the device job table: jobs, then the prefix tables of passes 1, 2, 3
+  const size_t jb = ((size_t)nj * sizeof(AsmJob) + 255) / 256 * 256, tb = jb + 3 * (size_t)(nj + 1) * sizeof(int);
+  char *tab = sp[0].o->ws_jobs.get<char>(tb);
+  if (!tab) return fail(BH_EHIP, "assembly: job table");
+  std::vector<char> h1(tb, 0);
+  memcpy(h1.data(), jobs.data(), (size_t)nj * sizeof(AsmJob));
+  int *hf = (int *)(h1.data() + jb);
+  for (int j = 0; j <= nj; j++) { hf[j] = f1[j]; hf[nj + 1 + j] = 0; hf[2 * (nj + 1) + j] = f3[j]; }
+  const int *d_f1 = (const int *)(tab + jb), *d_f2 = d_f1 + (nj + 1), *d_f3 = d_f1 + 2 * (nj + 1);
+  HIPCHK(hipMemcpyAsync(tab, h1.data(), tb, hipMemcpyHostToDevice, cur_st()));
   if (mixed)
-    bem_launch_asm_glr(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, D.dNe, nrows, kr, ki, g.d_glr, g_qrule[2],
-                       g_nqrule[2], G, H, dG, dH, nd, d_part, d_mask, nblk, cur_st());
+    bem_launch_asm_p1(1, tab, d_f1, nj, (int)b1, work, g.d_glr, 0, g_qrule[2], g_nqrule[2], cur_st());
   else
-    bem_launch_asm_gl(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, D.dNe, nrows, kr, ki, g.d_gl, g.n_gl, g_qrule[2],
-                      g_nqrule[2], G, H, dG, dH, nd, d_part, d_mask, nblk, cur_st());
+    bem_launch_asm_p1(0, tab, d_f1, nj, (int)b1, work, g.d_gl, g.n_gl, g_qrule[2], g_nqrule[2], cur_st());
   HIPCHK(hipGetLastError());
-  // near pairs in row-major order (deterministic)
-  std::vector<unsigned long long> mask(nr * nwv);
-  HIPCHK(hipMemcpyAsync(mask.data(), d_mask, mask.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, cur_st()));
+  // near pairs of every job in row-major order (deterministic)
+  std::vector<std::vector<unsigned long long>> hm(nj);
+  for (int j = 0; j < nj; j++) {
+    hm[j].resize((size_t)sp[j].nrows * nwv[j]);
+    HIPCHK(hipMemcpyAsync(hm[j].data(), jobs[j].ghMask, hm[j].size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          cur_st()));
+  }
   HIPCHK(hipStreamSynchronize(cur_st()));
-  std::vector<GhPair> pairs;
-  std::vector<int> start(nr + 1, 0);
-  for (size_t r = 0; r < nr; r++) {
-    start[r] = (int)pairs.size();
-    for (int w = 0; w < nwv; w++) {
-      unsigned long long m = mask[r * nwv + w];
-      while (m) {
-        int b = __builtin_ctzll(m);
-        m &= m - 1;
-        pairs.push_back({(int)r, w * 64 + b + 1});
+  std::vector<std::vector<GhPair>> pairs(nj);
+  std::vector<std::vector<int>> st(nj);
+  std::vector<size_t> offP(nj), totP(no, 0);
+  size_t p2 = 0;
+  for (int j = 0; j < nj; j++) {
+    const size_t nr = (size_t)sp[j].nrows;
+    st[j].assign(nr + 1, 0);
+    for (size_t r = 0; r < nr; r++) {
+      st[j][r] = (int)pairs[j].size();
+      for (int w = 0; w < nwv[j]; w++) {
+        unsigned long long m = hm[j][r * nwv[j] + w];
+        while (m) {
+          const int bt = __builtin_ctzll(m);
+          m &= m - 1;
+          pairs[j].push_back({(int)r, w * 64 + bt + 1});
+        }
       }
     }
+    st[j][nr] = (int)pairs[j].size();
+    *sp[j].ngh = (int)pairs[j].size();
+    offP[j] = totP[oi[j]]; totP[oi[j]] += pairs[j].size();
+    f2[j] = (int)p2; p2 += pairs[j].size();
   }
-  start[nr] = (int)pairs.size();
-  *ngh = (int)pairs.size();
-  GhPair *d_pairs = o->ws_pairs.get<GhPair>(std::max<size_t>(pairs.size(), 1));
-  int *d_start = o->ws_start.get<int>(start.size());
-  double *d_ghdf = o->ws_ghdf.get<double>(std::max<size_t>(pairs.size(), 1) * 3);
-  double *d_self = o->ws_self.get<double>(nr * 3);   // the self element's F terms per row
-  if (!d_pairs || !d_start || !d_ghdf || !d_self) return fail(BH_EHIP, "assembly: work area of the near pairs");
-  if (!pairs.empty()) HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), pairs.size() * sizeof(GhPair), hipMemcpyHostToDevice, cur_st()));
-  HIPCHK(hipMemcpyAsync(d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, cur_st()));
-  bem_launch_asm_gh(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, d_pairs, (int)pairs.size(), kr, ki, g.d_gh, g.n_gh,
-                    mixed ? (const void *)g.d_gl : nullptr, g.n_gl, g_qrule[3], g_nqrule[3], g_qrule[2], g_nqrule[2],
-                    G, H, dG, dH, nd, d_ghdf, cur_st());
-  bem_launch_asm_self(o->d_geo, o->d_gq, D.d_sid, rows, D.d_colOff, nrows, kr, ki, g.d_xl, g.d_wl, g.d_xh, g.d_wh, G,
-                      H, dG, dH, nd, d_self, cur_st());
-  bem_launch_asm_fin(d_part, nwv, d_ghdf, d_start, d_self, nrows, d == 0, H, nd, dF, r0, cur_st());
+  if (p2 > (size_t)INT_MAX / 64) return fail(BH_EARG, "batched assembly: %zu near pairs", p2);
+  f2[nj] = (int)p2;
+  std::vector<GhPair *> dpr(no);
+  std::vector<double *> dgh(no);
+  for (int k = 0; k < no; k++) {
+    dpr[k] = obs[k]->ws_pairs.get<GhPair>(std::max<size_t>(totP[k], 1));
+    dgh[k] = obs[k]->ws_ghdf.get<double>(std::max<size_t>(totP[k], 1) * 3);
+    if (!dpr[k] || !dgh[k]) return fail(BH_EHIP, "assembly: work area of %zu near pairs", totP[k]);
+  }
+  for (int j = 0; j < nj; j++) {
+    jobs[j].pairs = dpr[oi[j]] + offP[j];
+    jobs[j].ghDF = dgh[oi[j]] + offP[j] * 3;
+    if (!pairs[j].empty())
+      HIPCHK(hipMemcpyAsync((void *)jobs[j].pairs, pairs[j].data(), pairs[j].size() * sizeof(GhPair),
+                            hipMemcpyHostToDevice, cur_st()));
+    HIPCHK(hipMemcpyAsync((void *)jobs[j].ghStart, st[j].data(), st[j].size() * sizeof(int), hipMemcpyHostToDevice,
+                          cur_st()));
+  }
+  std::vector<char> h2(tb, 0);
+  memcpy(h2.data(), jobs.data(), (size_t)nj * sizeof(AsmJob));
+  hf = (int *)(h2.data() + jb);
+  for (int j = 0; j <= nj; j++) { hf[j] = f1[j]; hf[nj + 1 + j] = f2[j]; hf[2 * (nj + 1) + j] = f3[j]; }
+  HIPCHK(hipMemcpyAsync(tab, h2.data(), tb, hipMemcpyHostToDevice, cur_st()));
+  bem_launch_asm_p2(tab, d_f2, nj, (int)p2, g.d_gh, g.n_gh, mixed ? (const void *)g.d_gl : nullptr, g.n_gl, g_qrule[3],
+                    g_nqrule[3], g_qrule[2], g_nqrule[2], cur_st());
+  bem_launch_asm_p34(tab, d_f3, nj, (int)r3, g.d_xl, g.d_wl, g.d_xh, g.d_wh, cur_st());
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(cur_st()));   // the host tables above stay alive until here
   return 0;
 }
 
-static int assemble_domain(bh_object *o, int d)
+// rows r0 .. r0+nrows of domain d (a batch of one)
+static int assemble_rows(bh_object *o, int d, int r0, int nrows, cd *G, cd *H, cd *dG, cd *dH, double *dF, int *ngh)
+{
+  return assemble_batch({AsmSpec{o, d, r0, nrows, G, H, dG, dH, dF, ngh}});
+}
+
+// the FP64 matrices of domain d (allocated at the first assembly) as one
+// spec of a batch
+static int domain_spec(bh_object *o, int d, AsmSpec &x)
 {
   Dom &D = o->dom[d];
   size_t nd = (size_t)D.nd;
@@ -495,7 +608,8 @@ static int assemble_domain(bh_object *o, int d)
     HIPCHK(hipMalloc((void **)&D.d_dH, 2 * nd * nd * sizeof(cd)));
     HIPCHK(hipMalloc((void **)&D.d_dF, 3 * nd * sizeof(double)));
   }
-  return assemble_rows(o, d, 0, (int)nd, D.d_G, D.d_H, D.d_dG, D.d_dH, D.d_dF, &D.n_gh);
+  x = AsmSpec{o, d, 0, (int)nd, D.d_G, D.d_H, D.d_dG, D.d_dH, D.d_dF, &D.n_gh};
+  return 0;
 }
 
 // Domain d's FP64 matrices a block of rows at a time: fn(r0, nrows, G, H,
@@ -550,14 +664,9 @@ static int assemble_lm(bh_object *o)
   return 0;
 }
 
-int bem::object_assemble(bh_object *o, int type)
+// every factorisation of earlier matrices is stale from the assembly on
+static void asm_reset(bh_object *o)
 {
-  if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
-  {
-    int rc = ensure_rules();   // the square's rules for quadrangles
-    if (rc) return fail(rc, "rule tables");
-  }
-  // every factorisation of earlier matrices is stale from here on
   o->fmode = 0;
   o->x_warm = false;
   o->r_valid = false;
@@ -565,17 +674,48 @@ int bem::object_assemble(bh_object *o, int type)
   for (Dom &D : o->dom) D.gf_valid = D.gfc_valid = false;
   o->asm_prec = -1;
   o->lm_raw = false;
-  if (o->prec == BH_PREC_MIXED_LM) {
-    int rc = assemble_lm(o);
-    if (rc) return rc;
-  } else {
+}
+
+// Assembly of N objects (create_cmatrix_domain of every domain,
+// solve_bieq2.c:98-184): the FP64 and mixed forms of all objects and
+// domains as one batch per form, one launch per assembly pass; the
+// low-memory form object by object (its row blocks)
+int bem::objects_assemble(bh_object **objs, int N, int type)
+{
+  if (type != 2) return fail(BH_EUNSUP, "assembly rule type %d: only 2 (GLN with GHN fallback) is implemented", type);
+  {
+    int rc = ensure_rules();   // the square's rules for quadrangles
+    if (rc) return fail(rc, "rule tables");
+  }
+  std::vector<AsmSpec> sp[2];   // FP64 form, mixed form
+  for (int i = 0; i < N; i++) {
+    bh_object *o = objs[i];
+    asm_reset(o);
+    if (o->prec == BH_PREC_MIXED_LM) continue;
     for (int d = 0; d <= o->MN; d++) {
-      int rc = assemble_domain(o, d);
+      AsmSpec x;
+      int rc = domain_spec(o, d, x);
       if (rc) return rc;
+      sp[o->prec != BH_PREC_FP64].push_back(x);
     }
   }
-  o->asm_prec = o->prec;
+  for (int f = 0; f < 2; f++) {
+    int rc = assemble_batch(sp[f]);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < N; i++) {
+    if (objs[i]->prec == BH_PREC_MIXED_LM) {
+      int rc = assemble_lm(objs[i]);
+      if (rc) return rc;
+    }
+    objs[i]->asm_prec = objs[i]->prec;
+  }
   return 0;
+}
+
+int bem::object_assemble(bh_object *o, int type)
+{
+  return objects_assemble(&o, 1, type);
 }
 
 extern "C" int bh_object_assemble(bh_object *o, int type)

@@ -242,20 +242,46 @@ __device__ __forceinline__ bool asm_gl_quad(const TriGeo &g, const QuadExt &qx, 
 }
 
 // ---- pass 1 ----
-__global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
-                                                const int *__restrict__ sid, const RowPt *__restrict__ rows,
-                                                const int *__restrict__ colOff, int dNe, double kr, double ki,
-                                                const QPt *__restrict__ rule, int nq, const QPtQ *__restrict__ qrule,
-                                                int nqq, cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
-                                                cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
-                                                unsigned long long *__restrict__ ghMask)
+// The fields of a batch job that passes 1-4 read, in registers.  Pass 1:
+// workgroup b of the launch is workgroup xb of row `row` of its job.
+struct AsmView {
+  const TriGeo *__restrict__ geo;
+  const QuadExt *__restrict__ gq;
+  const int *__restrict__ sid;
+  const int *__restrict__ colOff;
+  cd *__restrict__ G, *__restrict__ H, *__restrict__ dG, *__restrict__ dH;
+  size_t ld;
+  double kr, ki;
+};
+
+__device__ __forceinline__ AsmView asm_view(const AsmJob &J)
 {
-  const int row = blockIdx.y;
-  const RowPt rp = rows[row];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  return {J.geo, J.gq, J.sid, J.colOff, J.G, J.H, J.dG, J.dH, J.ld, J.kr, J.ki};
+}
+
+__global__ void __launch_bounds__(256) k_asm_gl(const AsmJob *__restrict__ jobs, const int *__restrict__ first, int nj,
+                                                const QPt *__restrict__ rule, int nq, const QPtQ *__restrict__ qrule,
+                                                int nqq)
+{
+  const int j = job_of(first, nj, blockIdx.x);
+  const AsmJob &J = jobs[j];
+  const int loc = blockIdx.x - first[j], row = loc / J.nblk, xb = loc - row * J.nblk;
+  const AsmView V = asm_view(J);
+  const TriGeo *__restrict__ geo = V.geo;
+  const QuadExt *__restrict__ gq = V.gq;
+  const int *__restrict__ sid = V.sid;
+  const int *__restrict__ colOff = V.colOff;
+  cd *__restrict__ G = V.G, *__restrict__ H = V.H, *__restrict__ dG = V.dG, *__restrict__ dH = V.dH;
+  const size_t ld = V.ld;
+  const double kr = V.kr, ki = V.ki;
+  const int dNe = J.dNe;
+  double *__restrict__ dfPart = J.dfPart;
+  unsigned long long *__restrict__ ghMask = J.ghMask;
+  const RowPt rp = J.rows[row];
+  const int s = xb * blockDim.x + threadIdx.x + 1;
   const int lane = threadIdx.x & 63;
-  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nwv = (gridDim.x * blockDim.x) >> 6;
+  const int wv = (xb * blockDim.x + threadIdx.x) >> 6;
+  const int nwv = J.nblk * (blockDim.x >> 6);
   bool active = s <= dNe && s != rp.t;
   bool flagged = false;
   double f = 0.0, df0 = 0.0, df1 = 0.0;
@@ -306,20 +332,28 @@ __global__ void __launch_bounds__(256) k_asm_gl(const TriGeo *__restrict__ geo, 
 // where the time goes.  Pairs whose |dF/dt| comes within a relative 1e-4
 // of the GHN switch level are flagged as in the FP32 form of the boundary
 // sums: pass 2 redoes them and takes the reference's decision there.
-__global__ void __launch_bounds__(256) k_asm_glr(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
-                                                 const int *__restrict__ sid, const RowPt *__restrict__ rows,
-                                                 const int *__restrict__ colOff, int dNe, double kr, double ki,
-                                                 const QPt *__restrict__ rule, const QPtQ *__restrict__ qrule, int nqq,
-                                                 cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
-                                                 cd *__restrict__ dH, size_t ld, double *__restrict__ dfPart,
-                                                 unsigned long long *__restrict__ ghMask)
+__global__ void __launch_bounds__(256) k_asm_glr(const AsmJob *__restrict__ jobs, const int *__restrict__ first, int nj,
+                                                 const QPt *__restrict__ rule, const QPtQ *__restrict__ qrule, int nqq)
 {
-  const int row = blockIdx.y;
-  const RowPt rp = rows[row];
-  const int s = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = job_of(first, nj, blockIdx.x);
+  const AsmJob &J = jobs[j];
+  const int loc = blockIdx.x - first[j], row = loc / J.nblk, xb = loc - row * J.nblk;
+  const AsmView V = asm_view(J);
+  const TriGeo *__restrict__ geo = V.geo;
+  const QuadExt *__restrict__ gq = V.gq;
+  const int *__restrict__ sid = V.sid;
+  const int *__restrict__ colOff = V.colOff;
+  cd *__restrict__ G = V.G, *__restrict__ H = V.H, *__restrict__ dG = V.dG, *__restrict__ dH = V.dH;
+  const size_t ld = V.ld;
+  const double kr = V.kr, ki = V.ki;
+  const int dNe = J.dNe;
+  double *__restrict__ dfPart = J.dfPart;
+  unsigned long long *__restrict__ ghMask = J.ghMask;
+  const RowPt rp = J.rows[row];
+  const int s = xb * blockDim.x + threadIdx.x + 1;
   const int lane = threadIdx.x & 63;
-  const int wv = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int nwv = (gridDim.x * blockDim.x) >> 6;
+  const int wv = (xb * blockDim.x + threadIdx.x) >> 6;
+  const int nwv = J.nblk * (blockDim.x >> 6);
   bool active = s <= dNe && s != rp.t;
   bool flagged = false;
   double f = 0.0, df0 = 0.0, df1 = 0.0;
@@ -395,18 +429,29 @@ __device__ __forceinline__ void asm_gh_quad(const TriGeo &g, const QuadExt &qx, 
   }
 }
 
-__global__ void __launch_bounds__(256) k_asm_gh(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
-                                                const int *__restrict__ sid, const RowPt *__restrict__ rows,
-                                                const int *__restrict__ colOff, const GhPair *__restrict__ pairs,
-                                                int npairs, double kr, double ki, const QPt *__restrict__ rule, int nq,
-                                                const QPt *__restrict__ gl, int ngl, const QPtQ *__restrict__ qgh,
-                                                int nqgh, const QPtQ *__restrict__ qgl, int nqgl, cd *__restrict__ G,
-                                                cd *__restrict__ H, cd *__restrict__ dG, cd *__restrict__ dH, size_t ld,
-                                                double *__restrict__ ghDF)
+__global__ void __launch_bounds__(256) k_asm_gh(const AsmJob *__restrict__ jobs, const int *__restrict__ first, int nj,
+                                                const QPt *__restrict__ rule, int nq, const QPt *__restrict__ gl, int ngl,
+                                                const QPtQ *__restrict__ qgh, int nqgh, const QPtQ *__restrict__ qgl,
+                                                int nqgl)
 {
-  const int pid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // one wave per pair; the waves of a workgroup may belong to different jobs
+  const int gp = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (pid >= npairs) return;
+  if (gp >= first[nj]) return;
+  const int j = job_of(first, nj, gp);
+  const AsmJob &J = jobs[j];
+  const int pid = gp - first[j];
+  const AsmView V = asm_view(J);
+  const TriGeo *__restrict__ geo = V.geo;
+  const QuadExt *__restrict__ gq = V.gq;
+  const int *__restrict__ sid = V.sid;
+  const int *__restrict__ colOff = V.colOff;
+  cd *__restrict__ G = V.G, *__restrict__ H = V.H, *__restrict__ dG = V.dG, *__restrict__ dH = V.dH;
+  const size_t ld = V.ld;
+  const double kr = V.kr, ki = V.ki;
+  const GhPair *__restrict__ pairs = J.pairs;
+  const RowPt *__restrict__ rows = J.rows;
+  double *__restrict__ ghDF = J.ghDF;
   const GhPair pr = pairs[pid];
   const RowPt rp = rows[pr.row];
   int sd = sid[pr.s];
@@ -790,17 +835,27 @@ __device__ void self_tri(const TriGeo &g, double zt, double et, double kr, doubl
   }
 }
 
-__global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo, const QuadExt *__restrict__ gq,
-                                                  const int *__restrict__ sid, const RowPt *__restrict__ rows,
-                                                  const int *__restrict__ colOff, int nrows, double kr, double ki,
+__global__ void __launch_bounds__(256) k_asm_self(const AsmJob *__restrict__ jobs, const int *__restrict__ first, int nj,
                                                   const double *__restrict__ xl, const double *__restrict__ wl,
-                                                  const double *__restrict__ xh, const double *__restrict__ wh,
-                                                  cd *__restrict__ G, cd *__restrict__ H, cd *__restrict__ dG,
-                                                  cd *__restrict__ dH, size_t ld, double *__restrict__ selfDF)
+                                                  const double *__restrict__ xh, const double *__restrict__ wh)
 {
-  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // one wave per row; the waves of a workgroup may belong to different jobs
+  const int gr = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (row >= nrows) return;
+  if (gr >= first[nj]) return;
+  const int j = job_of(first, nj, gr);
+  const AsmJob &J = jobs[j];
+  const int row = gr - first[j];
+  const AsmView V = asm_view(J);
+  const TriGeo *__restrict__ geo = V.geo;
+  const QuadExt *__restrict__ gq = V.gq;
+  const int *__restrict__ sid = V.sid;
+  const int *__restrict__ colOff = V.colOff;
+  cd *__restrict__ G = V.G, *__restrict__ H = V.H, *__restrict__ dG = V.dG, *__restrict__ dH = V.dH;
+  const size_t ld = V.ld;
+  const double kr = V.kr, ki = V.ki;
+  const RowPt *__restrict__ rows = J.rows;
+  double *__restrict__ selfDF = J.selfDF;
   const RowPt rp = rows[row];
   int td = sid[rp.t];
   int at = td < 0 ? -td : td;
@@ -830,14 +885,22 @@ __global__ void __launch_bounds__(256) k_asm_self(const TriGeo *__restrict__ geo
 }
 
 // ---- pass 4: per-row sums and the diagonal term ----
-__global__ void __launch_bounds__(256) k_asm_fin(const double *__restrict__ dfPart, int nwv,
-                                                 const double *__restrict__ ghDF, const int *__restrict__ ghStart,
-                                                 const double *__restrict__ selfDF, int nrows, int open_domain,
-                                                 cd *__restrict__ H, size_t ld, double *__restrict__ dF, int r0)
+__global__ void __launch_bounds__(256) k_asm_fin(const AsmJob *__restrict__ jobs, const int *__restrict__ first, int nj)
 {
-  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int gr = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (row >= nrows) return;
+  if (gr >= first[nj]) return;
+  const int j = job_of(first, nj, gr);
+  const AsmJob &J = jobs[j];
+  const int row = gr - first[j];
+  const int nwv = J.nblk * 4, open_domain = J.open, r0 = J.r0;
+  const size_t ld = J.ld;
+  const double *__restrict__ dfPart = J.dfPart;
+  const double *__restrict__ ghDF = J.ghDF;
+  const int *__restrict__ ghStart = J.ghStart;
+  const double *__restrict__ selfDF = J.selfDF;
+  cd *__restrict__ H = J.H;
+  double *__restrict__ dF = J.dF;
   double f = 0.0, f0 = 0.0, f1 = 0.0;
   for (int w = lane; w < nwv; w += 64) {
     const double *p = dfPart + ((size_t)row * nwv + w) * 3;
@@ -911,50 +974,39 @@ __global__ void __launch_bounds__(256) k_elem_self(const TriGeo *__restrict__ ge
 
 extern "C" {
 // launchers used by the C-ABI layer (bem_api.hip)
-// qrule: the square's 12x12 rule (nqq points) for quadrangle source elements
-void bem_launch_asm_gl(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int dNe,
-                       int nrows, double kr, double ki, const void *rule, int nq, const void *qrule, int nqq, void *G,
-                       void *H, void *dG, void *dH, size_t ld, double *dfPart, unsigned long long *ghMask,
-                       int nwv_blocks, hipStream_t st)
+
+// Passes of a batched assembly over nj jobs (AsmJob, device) with their
+// prefix tables (device, nj + 1 ints each).  Pass 1: first1 over the jobs'
+// workgroups (nrows x nblk each), nblocks = first1[nj]; pairs = rows x
+// source elements of the batch (the registry's work).  mixed: the reduced
+// far rules (rule: the far-pair table, qrule: the square's 12x12 rule).
+void bem_launch_asm_p1(int mixed, const void *jobs, const int *first1, int nj, int nblocks, double pairs,
+                       const void *rule, int nq, const void *qrule, int nqq, hipStream_t st)
 {
-  dim3 grid(nwv_blocks, nrows);
-  bem_prof_begin(BH_K_ASM_GL, (double)nrows * dNe, st);
-  hipLaunchKernelGGL(k_asm_gl, grid, dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid, (const RowPt *)rows,
-                     colOff, dNe, kr, ki, (const QPt *)rule, nq, (const QPtQ *)qrule, nqq, (cd *)G, (cd *)H, (cd *)dG,
-                     (cd *)dH, ld, dfPart, ghMask);
-  bem_prof_end(BH_K_ASM_GL, st);
+  if (nblocks <= 0) return;
+  const int kid = mixed ? BH_K_ASM_GLR : BH_K_ASM_GL;
+  bem_prof_begin(kid, pairs, st);
+  if (mixed)
+    hipLaunchKernelGGL(k_asm_glr, dim3(nblocks), dim3(256), 0, st, (const AsmJob *)jobs, first1, nj, (const QPt *)rule,
+                       (const QPtQ *)qrule, nqq);
+  else
+    hipLaunchKernelGGL(k_asm_gl, dim3(nblocks), dim3(256), 0, st, (const AsmJob *)jobs, first1, nj, (const QPt *)rule, nq,
+                       (const QPtQ *)qrule, nqq);
+  bem_prof_end(kid, st);
 }
 
-// gl != nullptr: the pairs come from the FP32 pass and are first redone
-// with the FP64 12x12 rule (gl, ngl points; qgl, nqgl on quadrangles);
-// qgh, nqgh: the square's 64x64 rule
-void bem_launch_asm_gh(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff,
-                       const void *pairs, int npairs, double kr, double ki, const void *rule, int nq, const void *gl,
-                       int ngl, const void *qgh, int nqgh, const void *qgl, int nqgl, void *G, void *H, void *dG,
-                       void *dH, size_t ld, double *ghDF, hipStream_t st)
+// Pass 2: the near pairs of every job (first2 over the pairs, npairs =
+// first2[nj]) with the 64x64 rule (rule, nq; qgh, nqgh on quadrangles).
+// gl != nullptr: the pairs come from the mixed form's pass and are first
+// redone with the FP64 12x12 rule (gl, ngl; qgl, nqgl on quadrangles).
+void bem_launch_asm_p2(const void *jobs, const int *first2, int nj, int npairs, const void *rule, int nq, const void *gl,
+                       int ngl, const void *qgh, int nqgh, const void *qgl, int nqgl, hipStream_t st)
 {
   if (npairs <= 0) return;
-  int nb = (npairs + 3) / 4;
   bem_prof_begin(BH_K_ASM_GH, (double)npairs, st);
-  hipLaunchKernelGGL(k_asm_gh, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
-                     (const RowPt *)rows, colOff, (const GhPair *)pairs, npairs, kr, ki, (const QPt *)rule, nq,
-                     (const QPt *)gl, ngl, (const QPtQ *)qgh, nqgh, (const QPtQ *)qgl, nqgl, (cd *)G, (cd *)H, (cd *)dG,
-                     (cd *)dH, ld, ghDF);
+  hipLaunchKernelGGL(k_asm_gh, dim3((npairs + 3) / 4), dim3(256), 0, st, (const AsmJob *)jobs, first2, nj,
+                     (const QPt *)rule, nq, (const QPt *)gl, ngl, (const QPtQ *)qgh, nqgh, (const QPtQ *)qgl, nqgl);
   bem_prof_end(BH_K_ASM_GH, st);
-}
-
-// rule32: FP32 copy of the 12x12 rule (far pairs), rule: the FP64 one (near pairs)
-void bem_launch_asm_glr(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int dNe,
-                        int nrows, double kr, double ki, const void *rule, const void *qrule, int nqq, void *G, void *H,
-                        void *dG, void *dH, size_t ld, double *dfPart, unsigned long long *ghMask, int nwv_blocks,
-                        hipStream_t st)
-{
-  dim3 grid(nwv_blocks, nrows);
-  bem_prof_begin(BH_K_ASM_GLR, (double)nrows * dNe, st);
-  hipLaunchKernelGGL(k_asm_glr, grid, dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
-                     (const RowPt *)rows, colOff, dNe, kr, ki, (const QPt *)rule, (const QPtQ *)qrule, nqq, (cd *)G,
-                     (cd *)H, (cd *)dG, (cd *)dH, ld, dfPart, ghMask);
-  bem_prof_end(BH_K_ASM_GLR, st);
 }
 
 // FP32 copy (device) of a 3-part rule given on the host: point, weight and
@@ -978,18 +1030,18 @@ int bem_rule32(const void *rule, int nq, void **out)
   return rc;
 }
 
-// selfDF: [nrows][3] the self element's F, dF/dt_z, dF/dt_e (non-zero only
-// on warped quadrangles), written here and added by the final pass
-void bem_launch_asm_self(const void *geo, const void *gq, const int *sid, const void *rows, const int *colOff, int nrows,
-                         double kr, double ki, const double *xl, const double *wl, const double *xh, const double *wh,
-                         void *G, void *H, void *dG, void *dH, size_t ld, double *selfDF, hipStream_t st)
+// Passes 3 and 4: the self element of every row (its F terms into the
+// job's selfDF), then the row sums of F and the diagonal term; first3 over
+// the rows of the jobs, nrows = first3[nj]
+void bem_launch_asm_p34(const void *jobs, const int *first3, int nj, int nrows, const double *xl, const double *wl,
+                        const double *xh, const double *wh, hipStream_t st)
 {
-  int nb = (nrows + 3) / 4;
+  if (nrows <= 0) return;
+  const int nb = (nrows + 3) / 4;
   bem_prof_begin(BH_K_ASM_SELF, (double)nrows, st);
-  hipLaunchKernelGGL(k_asm_self, dim3(nb), dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, sid,
-                     (const RowPt *)rows, colOff, nrows, kr, ki, xl, wl, xh, wh, (cd *)G, (cd *)H, (cd *)dG, (cd *)dH,
-                     ld, selfDF);
+  hipLaunchKernelGGL(k_asm_self, dim3(nb), dim3(256), 0, st, (const AsmJob *)jobs, first3, nj, xl, wl, xh, wh);
   bem_prof_end(BH_K_ASM_SELF, st);
+  hipLaunchKernelGGL(k_asm_fin, dim3(nb), dim3(256), 0, st, (const AsmJob *)jobs, first3, nj);
 }
 
 void bem_launch_elem_self(const void *geo, const void *gq, const int *s, const double *ze, const int *idx, int n,
@@ -999,12 +1051,5 @@ void bem_launch_elem_self(const void *geo, const void *gq, const int *s, const d
   if (n <= 0) return;
   hipLaunchKernelGGL(k_elem_self, dim3((n + 3) / 4), dim3(256), 0, st, (const TriGeo *)geo, (const QuadExt *)gq, s, ze,
                      idx, n, kr, ki, xl, wl, xh, wh, (cd *)tmp, tmpF, (cd *)out);
-}
-void bem_launch_asm_fin(const double *dfPart, int nwv, const double *ghDF, const int *ghStart, const double *selfDF,
-                        int nrows, int open_domain, void *H, size_t ld, double *dF, int r0, hipStream_t st)
-{
-  int nb = (nrows + 3) / 4;
-  hipLaunchKernelGGL(k_asm_fin, dim3(nb), dim3(256), 0, st, dfPart, nwv, ghDF, ghStart, selfDF, nrows, open_domain,
-                     (cd *)H, ld, dF, r0);
 }
 }

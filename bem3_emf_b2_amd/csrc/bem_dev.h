@@ -122,6 +122,47 @@ struct GhPair {
   int row, s;
 };
 
+// One job of a batched assembly: rows [r0, r0 + nrows) of one domain of one
+// object.  Every assembly pass is one launch over the jobs of a batch (all
+// objects and domains of bh_mo_prepare, or the domains of one object); a
+// workgroup finds its job from the launch's prefix tables (job_of).
+struct AsmJob {
+  const TriGeo *geo;
+  const QuadExt *gq;
+  const int *sid;              // signed element of each source slot (1-based)
+  const RowPt *rows;           // the job's rows
+  const int *colOff;           // first column of each source element
+  cd *G, *H, *dG, *dH;         // the job's first row; row stride ld
+  double *dF;                  // 3 per row
+  double *dfPart;              // pass 1: [nrows][nwv][3] per-wave F sums
+  unsigned long long *ghMask;  // pass 1: [nrows][nwv] near-pair flags
+  const GhPair *pairs;         // pass 2: the job's near pairs
+  const int *ghStart;          // [nrows + 1] first pair of each row
+  double *ghDF;                // [npairs][3] their F sums
+  double *selfDF;              // pass 3: [nrows][3] the self element's F terms
+  double kr, ki;               // wavenumber of the domain
+  size_t ld;
+  int dNe;                     // source elements of the domain
+  int nrows, nblk;             // rows; 256-lane workgroups per row in pass 1 (nwv = 4 nblk)
+  int r0;                      // the job's first row in the domain (its diagonal column offset)
+  int open;                    // 1: the open domain 0 (diagonal 1 + F)
+  int pad;
+};
+
+// Prefix tables of a batch of nj jobs: first[j] is job j's first global
+// index (workgroup of pass 1, pair of pass 2, row of passes 3 and 4),
+// first[nj] the total.  The job holding global index i (0 <= i < first[nj]):
+// the last j with first[j] <= i (empty jobs are skipped that way).
+__host__ __device__ inline int job_of(const int *first, int nj, int i)
+{
+  int lo = 0, hi = nj;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 // column of a domain matrix in the 8NN unknown numbering and its boundary
 // condition class (main side of its element or sub side with delta, beta,
 // sigma of the element's media)

@@ -16,6 +16,11 @@
 #include <mutex>
 #include "bem_prof.h"
 
+namespace bem {
+int tune_value(int key);   // the tuning table (bem_tune.hip)
+}
+using bem::tune_value;
+
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
@@ -1531,9 +1536,12 @@ template <class R> static int la_init_t()
   return 0;
 }
 
-// complex-FP32 products on 128x128 tiles take the bf16-split kernel
-// (BEM3_HIP_GEMM_BF6=0: the FP32-MFMA one)
+// the bf16-split kernel's LDS limit is raised (the tuning table's gemm_bf6
+// entry chooses it for complex-FP32 products on 128x128 tiles)
 static bool g_gemm_bf6 = true;
+// the kernel bh_gemm_time forces on the calling thread (-1: the shape and
+// the tuning table decide; 0 64x64 tiles, 1 128x128 tiles, 2 bf16-split)
+static thread_local int g_gemm_force = -1;
 
 template <class R>
 static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename Cx<R>::Z *A, size_t lda,
@@ -1542,13 +1550,19 @@ static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename 
 {
   if (m <= 0 || n <= 0) return;
   const int tx = (n + ZT - 1) / ZT, ty = (m + ZT - 1) / ZT, T = tx * ty;
-  // measured on gfx950: the 128x128 kernel overtakes the 64x64 one from
-  // about 160 tiles (k >= 128) in both precisions; with a long k (the
-  // products of many right-hand sides, m x 256 x m) already from 64 tiles,
-  // where its double-buffered 3M slices outrun the 64x64 kernel even with
-  // part of the CUs idle
-  const bool skinny = n <= 8 && k >= 256;
-  const bool big = g_big_tiles && !skinny && ((T >= 160 && k >= 32) || (T >= 64 && k >= 1024));
+  // the 128x128 kernel takes products of at least the tuning table's tile
+  // count (defaults measured on gfx950: about 160 tiles in both
+  // precisions), and with a long k (the products of many right-hand sides,
+  // m x 256 x m) already from fewer (64), where its double-buffered 3M
+  // slices outrun the 64x64 kernel even with part of the CUs idle
+  const int force = g_gemm_force;
+  const bool skinny = force < 0 && n <= 8 && k >= 256;
+  const int tiles = tune_value(sizeof(R) == sizeof(double) ? BH_TUNE_GEMM_T_D : BH_TUNE_GEMM_T_F);
+  const bool big = g_big_tiles && !skinny &&
+                   (force >= 0 ? force >= 1
+                               : ((T >= tiles && k >= 32) ||
+                                  (T >= tune_value(BH_TUNE_GEMM_LONGK_T) && k >= tune_value(BH_TUNE_GEMM_LONGK_K))));
+  const bool bf6 = g_gemm_bf6 && (force >= 0 ? force == 2 : tune_value(BH_TUNE_GEMM_BF6) != 0);
   const int kid = big ? Cx<R>::KG : skinny ? Cx<R>::KGV : Cx<R>::KGS;
   const bool bz = beta.x == 0 && beta.y == 0;
   bem_prof_begin_b(kid, 8.0 * m * n * k,
@@ -1564,7 +1578,7 @@ static void gemm_t(int m, int n, int k, typename Cx<R>::Z alpha, const typename 
       hipLaunchKernelGGL((k_gemm_skinny<R, 4>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     else
       hipLaunchKernelGGL((k_gemm_skinny<R, 8>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
-  } else if (big && sizeof(R) == sizeof(float) && g_gemm_bf6) {
+  } else if (big && sizeof(R) == sizeof(float) && bf6) {
     hipLaunchKernelGGL(k_cgemm_bf6, dim3(8 * ((T + 7) / 8)), dim3(512), BF_LDS_BYTES, st, m, n, k, *(const cf *)&alpha,
                        (const cf *)A, lda, (const cf *)B, ldb, *(const cf *)&beta, (cf *)C, ldc, tx, ty);
   } else if (big) {
@@ -2058,8 +2072,7 @@ int bem_la_init(void)
   g_panel_ordinary = !(e && strcmp(e, "1") == 0);
   const char *es = getenv("BEM3_HIP_PANEL_SLOT");
   g_panel_slot = !(es && strcmp(es, "0") == 0);
-  const char *eb = getenv("BEM3_HIP_GEMM_BF6");
-  g_gemm_bf6 = !(eb && strcmp(eb, "0") == 0);
+  g_gemm_bf6 = true;
   if (g_gemm_bf6 &&
       hipFuncSetAttribute((const void *)k_cgemm_bf6, hipFuncAttributeMaxDynamicSharedMemorySize, BF_LDS_BYTES) != hipSuccess) {
     (void)hipGetLastError();
@@ -2126,6 +2139,45 @@ void bem_zgetrs(int n, int nrhs, const void *A, size_t lda, const int *piv, void
 {
   (void)nb;
   getrs_t<double>(n, nrhs, (const cd *)A, lda, piv, (cd *)B, ldb, st);
+}
+
+// mean time (ms) of reps launches of one product kernel (kernel 0 64x64
+// tiles, 1 128x128 tiles, 2 bf16-split) on operands of its own, after one
+// warm-up launch: 0, -1 on a HIP error, -2 if the kernel is unavailable
+int bem_gemm_time(int prec, int kernel, int m, int n, int k, int reps, double *ms, hipStream_t st)
+{
+  if (kernel >= 1 && !g_big_tiles) return -2;
+  if (kernel == 2 && (prec != 1 || !g_gemm_bf6)) return -2;
+  const size_t es = prec == 0 ? sizeof(cd) : sizeof(cf);
+  void *A = nullptr, *B = nullptr, *C = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = -1;
+  if (hipMalloc(&A, es * m * k) == hipSuccess && hipMalloc(&B, es * k * n) == hipSuccess &&
+      hipMalloc(&C, es * m * n) == hipSuccess && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+      hipMemsetAsync(A, 0x3c, es * m * k, st) == hipSuccess && hipMemsetAsync(B, 0x3b, es * k * n, st) == hipSuccess) {
+    g_gemm_force = kernel;
+    for (int r = 0; r <= reps; r++) {
+      if (r == 1) (void)hipEventRecord(e0, st);
+      if (prec == 0)
+        gemm_t<double>(m, n, k, mk(1, 0), (const cd *)A, k, (const cd *)B, n, mk(0, 0), (cd *)C, n, st);
+      else
+        gemm_t<float>(m, n, k, mkf(1, 0), (const cf *)A, k, (const cf *)B, n, mkf(0, 0), (cf *)C, n, st);
+    }
+    g_gemm_force = -1;
+    float t = 0.f;
+    if (hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+        hipEventElapsedTime(&t, e0, e1) == hipSuccess && hipGetLastError() == hipSuccess) {
+      *ms = (double)t / reps;
+      rc = 0;
+    }
+  }
+  (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (A) (void)hipFree(A);
+  if (B) (void)hipFree(B);
+  if (C) (void)hipFree(C);
+  return rc;
 }
 
 // complex FP32 (interleaved float pairs)

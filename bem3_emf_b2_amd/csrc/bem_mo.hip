@@ -886,23 +886,19 @@ int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double
 // once, one host worker thread per stream (a worker's entry-point work goes
 // to its stream, set_cur_st).  Latency-bound pieces (LU panels, small
 // products, triangular-solve chains) of different objects then overlap.
-// Streams keep the default priority.  BEM3_HIP_LANES=n (1..8) sets the
-// number of streams; 1 runs everything in the calling thread on the
-// library's stream (default 6).
+// Streams keep the default priority.  The tuning table's lanes entry (or
+// bh_mo_set_lanes) sets the number of streams; 1 runs everything in the
+// calling thread on the library's stream.
 #define MO_LMAX 8
 hipStream_t g_lane_st[MO_LMAX];
 int g_nlanes = 0;     // streams created so far
-int g_lanes_req = 0;  // bh_mo_set_lanes (0: BEM3_HIP_LANES, else 6)
+int g_lanes_req = 0;  // bh_mo_set_lanes (0: the tuning table's lanes entry)
 hipStream_t g_gpre_st = nullptr;   // the coupling loop's G1 factorisations (GPrefactor)
 
 int lanes()
 {
   int n = g_lanes_req;
-  if (n <= 0) {
-    n = 6;   // measured on the 16-object scene: 2 lanes 6.9 s, 4 6.35 s, 6 6.1 s, 8 6.5 s
-    const char *e = getenv("BEM3_HIP_LANES");
-    if (e && *e) n = std::max(1, std::min(MO_LMAX, atoi(e)));
-  }
+  if (n <= 0) n = std::max(1, std::min(MO_LMAX, tune_value(BH_TUNE_LANES)));
   while (g_nlanes < n) {
     HIPCHK(hipStreamCreate(&g_lane_st[g_nlanes]));
     g_nlanes++;
@@ -966,8 +962,11 @@ extern "C" void bem_mo_release(void)
 }
 
 // assembly and factorisation of N objects (create_matrix of each,
-// solve_bieq2.c:10-96), the objects spread over the lanes; type and mode as
-// bh_object_assemble and bh_object_factor
+// solve_bieq2.c:10-96): the assembly of every object and domain as one
+// batch (one launch per assembly pass), then the factorisations spread over
+// the lanes; type and mode as bh_object_assemble and bh_object_factor.
+// BEM3_HIP_ASM_BATCH=0: each object assembled on its lane before its
+// factorisation (assembly and factorisations of different objects overlap).
 extern "C" int bh_mo_prepare(bh_object **objs, int N, int type, int mode)
 {
   API_LOCK;
@@ -977,8 +976,13 @@ extern "C" int bh_mo_prepare(bh_object **objs, int N, int type, int mode)
   // once per assembly) in the same pass unless BEM3_HIP_G1_AHEAD=0
   const char *e = getenv("BEM3_HIP_G1_AHEAD");
   const bool g1 = !(e && strcmp(e, "0") == 0);
+  const bool batch = tune_value(BH_TUNE_ASM_BATCH) != 0;
+  if (batch) {
+    int rc = objects_assemble(objs, N, type);
+    if (rc) return rc;
+  }
   return run_lanes(objs, N, cost_of(objs, N, 3), [&](bh_object *o) {
-    int r = object_assemble(o, type);
+    int r = batch ? 0 : object_assemble(o, type);
     if (!r) r = object_factor(o, mode);
     if (!r && g1) r = object_factor_g(o);
     return r;
@@ -987,8 +991,8 @@ extern "C" int bh_mo_prepare(bh_object **objs, int N, int type, int mode)
 
 // 0: series where a pair allows it (the mixed-precision form), 1: the
 // direct element sums for every pair
-// device streams of the per-object work (1..8; 0: the default, BEM3_HIP_LANES
-// or 2).  One lane runs every kernel in order on the library's stream, so
+// device streams of the per-object work (1..8; 0: the tuning table's lanes
+// entry).  One lane runs every kernel in order on the library's stream, so
 // that per-kernel timings bracket one kernel each.
 extern "C" int bh_mo_set_lanes(int n)
 {

@@ -179,6 +179,7 @@ struct bh_object {
   int asm_prec = -1;                   // precision the current matrices were assembled in (-1: none)
   // grow-only device work areas of the entry points (freed with the object)
   bem::Scratch ws_part, ws_mask, ws_pairs, ws_start, ws_ghdf, ws_self;   // assembly
+  bem::Scratch ws_jobs;                // job table of a batched assembly this object heads
   bem::Scratch ws_wt;                  // forming S
   bem::Scratch ws_b;                   // right-hand side: the incident data as column operands
   // low-memory mixed form: FP64 row blocks of G, H, dG, dH, dF assembled on
@@ -209,6 +210,10 @@ namespace bem {
 // the entry points' work without the entry lock, on the thread's current
 // stream (the coupling loop's workers call these)
 int object_assemble(bh_object *o, int type);
+int objects_assemble(bh_object **objs, int N, int type);
+// the tuning table (bem_tune.hip)
+int tune_value(int key);
+int tune_init();
 int object_factor(bh_object *o, int mode);
 int object_solve(bh_object *o);
 // the mixed-form solves of N objects in lock step (BH_EUNSUP: an object

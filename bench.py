@@ -542,6 +542,8 @@ def main():
         "l4_sphere": l4,
         "beams64": beams64,
         "l5_sphere": l5,
+        "tuning": {"source": os.path.relpath(bem.tune_source(), ROOT) if os.path.isabs(bem.tune_source())
+                   else bem.tune_source(), "entries": bem.tune_get()},
     }
     print(json.dumps(out), flush=True)
     bem.finalize()

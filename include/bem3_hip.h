@@ -140,8 +140,11 @@ int bh_object_incident(bh_object *o, int which, double *buf, int set);
 
 /* Assembly (type as bh_object_assemble) and factorisation (mode as
  * bh_object_factor) of N objects -- create_matrix of each object
- * (solve_bieq2.c:10-96) -- with the objects spread over several device
- * streams (BEM3_HIP_LANES, default 6), so that the latency-bound pieces of
+ * (solve_bieq2.c:10-96).  The assembly of every object and domain runs as
+ * one batch (one launch per assembly pass over all of them; tuning entry
+ * BH_TUNE_ASM_BATCH = 0: each object on its stream before its
+ * factorisation), the factorisations spread over several device streams
+ * (tuning entry BH_TUNE_LANES), so that the latency-bound pieces of
  * different objects overlap.                                              */
 int bh_mo_prepare(bh_object **objs, int N, int type, int mode);
 /* Coupling loop over N objects (mo_solve_bv, mo_solve.c:10-88): every
@@ -153,7 +156,7 @@ int bh_mo_prepare(bh_object **objs, int N, int type, int mode);
 int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, int iter_max, double *trace, int *iters);
 /* device streams (lanes) over which the per-object work of bh_mo_prepare,
  * bh_mo_solve and bh_mo_solve_beams is spread, one host worker each: 1..8,
- * 0 for the default (BEM3_HIP_LANES, else 2).  With 1 every kernel runs in
+ * 0 for the tuning table's entry BH_TUNE_LANES.  With 1 every kernel runs in
  * order on the library's stream.  No counterpart in the reference.      */
 int bh_mo_set_lanes(int n);
 /* How the coupling loop forms the field of one object at another's node
@@ -319,6 +322,41 @@ int bh_prof_read(int kid, double *ms, double *work, int *launches);
  * operands read once and the results written once; stated for the GEMM
  * kernels, 0 for the others)                                            */
 int bh_prof_read_bytes(int kid, double *ms, double *work, double *bytes, int *launches);
+
+/* Tuning table (no counterpart in the reference: the reference's loops
+ * have no schedule to choose).  Entries are schedule and tile choices that
+ * leave results unchanged up to the FP32 rounding of the mixed form's
+ * refined products.  bh_init loads the table named by the environment
+ * variable BEM3_HIP_TUNE, else the file tune_gfx950.txt beside the library
+ * when it exists (lines "name value", '#' comments; written by the
+ * autotuner bem3_emf_b2_amd/autotune.py from measurements on the device),
+ * unless bh_tune_set or bh_tune_load ran first; BEM3_HIP_LANES,
+ * BEM3_HIP_ASM_BATCH and BEM3_HIP_GEMM_BF6 override single entries.     */
+#define BH_TUNE_LANES        0  /* device streams of the per-object work, 1..8                     */
+#define BH_TUNE_ASM_BATCH    1  /* bh_mo_prepare: 1 every object's assembly as one batch, 0 each
+                                   object assembled on its stream before its factorisation        */
+#define BH_TUNE_GEMM_T_D     2  /* FP64 products: tile count (128 x 128) from which the 128x128-tile
+                                   kernel takes a product with k >= 32 (the 64x64 kernel below)   */
+#define BH_TUNE_GEMM_T_F     3  /* the same for complex-FP32 products                             */
+#define BH_TUNE_GEMM_LONGK_T 4  /* ... and from this tile count when k >= BH_TUNE_GEMM_LONGK_K    */
+#define BH_TUNE_GEMM_LONGK_K 5
+#define BH_TUNE_GEMM_BF6     6  /* complex-FP32 128x128 tiles: 1 the bf16-split MFMA kernel, 0 the
+                                   FP32-MFMA kernel                                               */
+#define BH_TUNE_COUNT        7
+int bh_tune_set(int key, int value);
+int bh_tune_get(int key, int *value);
+/* replace the table by a file's entries (entries it does not name take
+ * their defaults); BH_EARG on an unknown name or a value out of range   */
+int bh_tune_load(const char *path);
+const char *bh_tune_name(int key);
+/* the file the table came from ("defaults" when none)                   */
+const char *bh_tune_source(void);
+/* Time one product kernel on device buffers of its own (the autotuner's
+ * probe): C = A B with m x k and k x n operands, prec 0 complex FP64, 1
+ * complex FP32; kernel 0 the 64x64-tile kernel, 1 the 128x128-tile MFMA
+ * kernel, 2 the bf16-split 128x128 kernel (prec 1); *ms is the mean time
+ * of reps launches after one warm-up launch.                            */
+int bh_gemm_time(int prec, int kernel, int m, int n, int k, int reps, double *ms);
 
 #ifdef __cplusplus
 }
