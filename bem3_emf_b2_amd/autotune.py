@@ -47,23 +47,30 @@ def rule_big(T, k, t1, t2, kl):
 
 
 def fit_thresholds(rows, defaults):
-    """rows: (T, k, t_small, t_big); the (t1, t2, kl) that minimise the summed
-    time of the kernels the rule picks (ties keep the defaults)"""
-    Ts = sorted({r[0] for r in rows})
-    cand_t = sorted(set(Ts + [Ts[-1] + 1, defaults[0], defaults[1]]))
-    cand_k = sorted({r[1] for r in rows} | {defaults[2]})
+    """rows: {prec: [(T, k, t_small, t_big)]}; defaults: {prec: t1} and the
+    shared (t2, kl).  The per-precision tile threshold t1 and the shared
+    long-k rule (t2 tiles, k >= kl) that minimise the summed time of the
+    kernels the rule picks over both precisions' grids (ties keep the
+    defaults).  Returns ({prec: t1}, (t2, kl), cost, cost of the defaults)."""
+    Ts = sorted({r[0] for rs in rows.values() for r in rs})
+    cand_t = sorted(set(Ts + [Ts[-1] + 1] + list(defaults[0].values()) + [defaults[1][0]]))
+    cand_k = sorted({r[1] for rs in rows.values() for r in rs} | {defaults[1][1]})
 
-    def cost(t1, t2, kl):
-        return sum(tb if rule_big(T, k, t1, t2, kl) else ts for T, k, ts, tb in rows)
+    def cost(prec, t1, t2, kl):
+        return sum(tb if rule_big(T, k, t1, t2, kl) else ts for T, k, ts, tb in rows[prec])
 
-    best = (cost(*defaults), defaults)
-    for t1, t2, kl in itertools.product(cand_t, cand_t, cand_k):
-        if t2 > t1:
-            continue   # the long-k rule only ever lowers the threshold
-        c = cost(t1, t2, kl)
+    def total(t1s, t2, kl):
+        return sum(cost(p, t1s[p], t2, kl) for p in rows)
+
+    c0 = total(defaults[0], *defaults[1])
+    best = (c0, dict(defaults[0]), tuple(defaults[1]))
+    for t2, kl in itertools.product(cand_t, cand_k):
+        # the long-k rule only lowers a threshold: t1 >= t2
+        t1s = {p: min((t for t in cand_t if t >= t2), key=lambda t: cost(p, t, t2, kl)) for p in rows}
+        c = total(t1s, t2, kl)
         if c < best[0] * (1 - 1e-3):
-            best = (c, (t1, t2, kl))
-    return best[1], best[0], cost(*defaults)
+            best = (c, t1s, (t2, kl))
+    return best[1], best[2], best[0], c0
 
 
 def tune_gemm(quick, notes):
@@ -82,25 +89,23 @@ def tune_gemm(quick, notes):
                  % (3 * len(ks), t_mfma, t_bf6, out["gemm_bf6"]))
     big32 = "bf16split" if out["gemm_bf6"] else "128x128"
     bem.tune_set("gemm_bf6", out["gemm_bf6"])
-    fitted = {}
-    for prec, big, key in (("fp64", "128x128", "gemm_tiles_fp64"), ("fp32", big32, "gemm_tiles_fp32")):
-        rows = []
+    keys = {"fp64": "gemm_tiles_fp64", "fp32": "gemm_tiles_fp32"}
+    rows = {}
+    for prec, big in (("fp64", "128x128"), ("fp32", big32)):
+        rows[prec] = []
         for m in sizes:
             for k in ks:
                 ts = bem.gemm_time(prec, "64x64", m, m, k, reps)
                 tb = bem.gemm_time(prec, big, m, m, k, reps)
-                rows.append((tiles(m, m), k, ts, tb))
+                rows[prec].append((tiles(m, m), k, ts, tb))
                 log("autotune: %s %5d x %5d x %5d  64x64 %.4f ms  %s %.4f ms" % (prec, m, m, k, ts, big, tb))
-        dflt = (bem.tune_get(key), bem.tune_get("gemm_longk_tiles"), bem.tune_get("gemm_longk_k"))
-        (t1, t2, kl), c, c0 = fit_thresholds(rows, dflt)
-        fitted[prec] = (t1, t2, kl, c, c0, len(rows))
-        out[key] = t1
-        notes.append("%s products over %d shapes: 128x128 from %d tiles (long k >= %d from %d): %.2f ms "
-                     "(the defaults %.2f ms)" % (prec, len(rows), t1, kl, t2, c, c0))
-    # one long-k rule for both precisions: the FP64 fit's unless the FP32 fit saves more
-    f64, f32 = fitted["fp64"], fitted["fp32"]
-    pick = f64 if (f64[4] - f64[3]) >= (f32[4] - f32[3]) else f32
-    out["gemm_longk_tiles"], out["gemm_longk_k"] = min(pick[1], out["gemm_tiles_fp64"], out["gemm_tiles_fp32"]), pick[2]
+    dflt = ({p: bem.tune_get(keys[p]) for p in keys}, (bem.tune_get("gemm_longk_tiles"), bem.tune_get("gemm_longk_k")))
+    t1s, (t2, kl), c, c0 = fit_thresholds(rows, dflt)
+    for p in keys:
+        out[keys[p]] = t1s[p]
+    out["gemm_longk_tiles"], out["gemm_longk_k"] = t2, kl
+    notes.append("products over %d shapes per precision: 128x128 tiles from %d (FP64) / %d (FP32) tiles, with k >= %d "
+                 "from %d: %.2f ms (the defaults %.2f ms)" % (len(rows["fp64"]), t1s["fp64"], t1s["fp32"], kl, t2, c, c0))
     for k_, v in out.items():
         bem.tune_set(k_, v)
     return out
@@ -141,7 +146,10 @@ def tune_schedule(quick, notes):
                 t = min(config() for _ in range(1 if quick else 2))
                 log("autotune: scene asm_batch %d lanes %d: %.3f s/config" % (batch, nl, t))
                 notes.append("scene (%d objects) asm_batch %d lanes %d: %.3f s/config" % (len(G), batch, nl, t))
-                if best is None or t < best[0]:
+                # the batched assembly (4 launches and one synchronisation for all
+                # objects) unless the per-object form is faster by more than the
+                # run-to-run spread of a configuration (2 %)
+                if best is None or t < best[0] * (0.98 if (batch == 0 and best[1] == 1) else 1.0):
                     best = (t, batch, nl)
         bem.tune_set("asm_batch", best[1])
         bem.tune_set("lanes", best[2])

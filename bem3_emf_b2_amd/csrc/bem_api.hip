@@ -131,7 +131,10 @@ extern "C" int bh_init(int device)
     if (rc) return rc;
   }
   HIPCHK(hipSetDevice(device));
-  HIPCHK(hipStreamCreate(&g.st));
+  // the library's stream lives as long as the process (bh_finalize keeps it)
+  static hipStream_t s_st = nullptr;
+  if (!s_st) HIPCHK(hipStreamCreate(&s_st));
+  g.st = s_st;
   HIPCHK(hipEventCreate(&g.ev[0]));
   HIPCHK(hipEventCreate(&g.ev[1]));
   if (bem_la_init()) fprintf(stderr, "bem3_hip: could not raise the dynamic LDS limit (%s)\n", hipGetErrorString(hipGetLastError()));
@@ -2758,9 +2761,10 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
 }
 
 // device state of the library released (every object destroyed first):
-// rule tables, work areas, timing events, the stream.  bh_init (or any entry
-// point) sets the library up again.  The uncached exchange areas of the
-// one-launch LU panel and triangular solves stay until the process ends.
+// rule tables, work areas, timing events.  bh_init (or any entry point) sets
+// the library up again.  The uncached exchange areas of the one-launch LU
+// panel and triangular solves and the device streams stay until the
+// process ends.
 extern "C" int bh_finalize(void)
 {
   API_LOCK;
@@ -2780,7 +2784,9 @@ extern "C" int bh_finalize(void)
   bem_prof_release();
   (void)hipEventDestroy(g.ev[0]);
   (void)hipEventDestroy(g.ev[1]);
-  (void)hipStreamDestroy(g.st);
+  // The streams (this one, the lanes', the coupling loop's G1 stream) stay
+  // for the next bh_init: a process whose streams had carried cooperative
+  // launches and were destroyed crashed in the runtime's teardown at exit.
   g = Ctx();
   return 0;
 }

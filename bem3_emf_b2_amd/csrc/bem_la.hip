@@ -23,8 +23,10 @@ using bem::tune_value;
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -1613,8 +1615,12 @@ static inline int half64(int n) { return std::max(64, (n / 2) / 64 * 64); }
 // Resident workgroups of k_panel_coop<float> on the device (0: unknown),
 // a k_panel_coop<double> workgroup's share of them in units of the float
 // one, and the units held by factorisations running now.  A factorisation
-// whose panels all fit beside the ones already reserved takes ordinary
-// launches for them (BEM3_HIP_PANEL_COOP=1: never).
+// reserves the residency of its largest panel and takes ordinary launches
+// for its panels; while the factorisations running on other streams hold
+// too much of the device it waits for them (each releases its share when
+// it returns, without waiting for anyone).  Only a factorisation whose
+// panel alone exceeds the device takes cooperative launches, which run
+// alone on the device (BEM3_HIP_PANEL_COOP=1: always).
 static int g_panel_cap = 0, g_panel_unit_d = 2;
 static std::atomic<long> g_panel_held{0};
 static bool g_panel_ordinary = true;
@@ -1624,10 +1630,13 @@ template <class R> static bool panel_reserve(int nbc)
 {
   if (!g_panel_ordinary || g_panel_cap <= 0) return false;
   const long u = panel_units<R>(nbc);
-  long cur = g_panel_held.load();
-  while (cur + u <= (long)g_panel_cap)
-    if (g_panel_held.compare_exchange_weak(cur, cur + u)) return true;
-  return false;
+  if (u > (long)g_panel_cap) return false;
+  for (int k = 0;; k++) {
+    long cur = g_panel_held.load();
+    while (cur + u <= (long)g_panel_cap)
+      if (g_panel_held.compare_exchange_weak(cur, cur + u)) return true;
+    std::this_thread::sleep_for(std::chrono::microseconds(k < 200 ? 20 : 200));
+  }
 }
 template <class R> static void panel_release(int nbc) { g_panel_held -= panel_units<R>(nbc); }
 

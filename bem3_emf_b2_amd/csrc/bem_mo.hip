@@ -953,12 +953,11 @@ std::vector<double> cost_of(bh_object **objs, int N, int power)
 
 }  // namespace
 
+// the work areas; the streams stay for the life of the process (the
+// library's shutdown keeps every stream it created)
 extern "C" void bem_mo_release(void)
 {
   if (g_ws) { delete[] g_ws; g_ws = nullptr; }
-  if (g_gpre_st) { (void)hipStreamDestroy(g_gpre_st); g_gpre_st = nullptr; }
-  for (int k = 0; k < g_nlanes; k++) (void)hipStreamDestroy(g_lane_st[k]);
-  g_nlanes = 0;
 }
 
 // assembly and factorisation of N objects (create_matrix of each,

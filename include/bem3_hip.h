@@ -36,9 +36,10 @@ const char *bh_last_error(void);
 /* select the device and build the quadrature tables (called implicitly) */
 int bh_init(int device);
 /* Releases the library's device state (rule tables, work areas, timing
- * events, its stream) after the caller destroyed every object; the next
- * entry point sets the library up again.  For programs that must not leave
- * device frees to process exit (no counterpart in the reference).          */
+ * events) after the caller destroyed every object; its device streams stay
+ * until the process ends and are used again.  The next entry point sets
+ * the library up again.  For programs that must not leave device frees to
+ * process exit (no counterpart in the reference).                         */
 int bh_finalize(void);
 int bh_sync(void);
 void *bh_stream(void);
