@@ -68,3 +68,24 @@ def test_committed_table_loads(bem_defaults):
     bem.tune_load(path)
     for name in bem.TUNE_KEYS:
         bem.tune_get(name)
+
+
+def test_autotuner_threshold_fit():
+    """the fit of the dispatch rule on a synthetic grid: the 128x128 kernel
+    wins from 64 tiles in FP64 and from 36 in FP32, and with k >= 1024
+    already from 16 tiles in both; the fit finds exactly those thresholds,
+    and keeps the defaults when they are already optimal"""
+    _lib()
+    from bem3_emf_b2_amd import autotune as at
+    Ts, ks = [9, 16, 25, 36, 64, 100, 144], [64, 256, 1024, 2048]
+
+    def grid(t1):
+        return [(T, k, 1.0, 0.5 if (T >= t1 or (T >= 16 and k >= 1024)) else 2.0) for T in Ts for k in ks]
+
+    rows = {"fp64": grid(64), "fp32": grid(36)}
+    t1s, (t2, kl), c, c0 = at.fit_thresholds(rows, ({"fp64": 160, "fp32": 160}, (64, 2048)))
+    assert t1s == {"fp64": 64, "fp32": 36} and t2 == 16 and kl == 1024
+    assert c < c0
+    t1s, longk, c, c0 = at.fit_thresholds(rows, ({"fp64": 64, "fp32": 36}, (16, 1024)))
+    assert t1s == {"fp64": 64, "fp32": 36} and longk == (16, 1024) and c == c0
+    assert at.rule_big(64, 32, 64, 16, 1024) and not at.rule_big(63, 1023, 64, 16, 1024)
