@@ -362,8 +362,8 @@ def beams_extra(bem, nb=64):
 # bytes per launch on average from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # each scaled by the reading of a 1 GiB device copy
 # (profiles/cgemm_bf6_scene16_counters.txt)
-SCENE_GEMM_BF6_TRAFFIC = {"bytes": (327675.9 * 2.0 + 60893.3 * 1.0) * 1024,
-                          "shape": "16-object scene, mean over 3932 launches"}
+SCENE_GEMM_BF6_TRAFFIC = {"bytes": (173814.5 * 2.0 + 37164.5 * 1.0) * 1024,
+                          "shape": "16-object scene with the shipped tuning table, mean over 7740 launches"}
 # kernels whose registry work is real flops: (name, bound, peak TFLOP/s)
 FLOP_KERNELS = {"zgemm": ("k_gemm128<double>", PEAK_FP64_TFLOPS), "cgemm": ("k_cgemm_bf6", PEAK_BF16_TFLOPS),
                 "zgemm_small": ("k_gemm64<double>", PEAK_FP64_TFLOPS), "cgemm_small": ("k_gemm64<float>", PEAK_FP32_TFLOPS),
