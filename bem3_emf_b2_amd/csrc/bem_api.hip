@@ -916,8 +916,8 @@ void bem_launch_scale_rows(void *, const void *, int, int, double, double, hipSt
 void bem_launch_domain(const double *, int, const double *, const int *, const int *, const int *, int, int *,
                        hipStream_t);
 void bem_launch_field(const double *, const int *, int, int, const void *, const int *, int, double, double, const void *,
-                      int, double, const void *, const void *, int, int, const void *, const void *, const void *,
-                      const void *, void *, int, hipStream_t);
+                      int, const void *, double, const void *, const void *, int, int, const void *, const void *,
+                      const void *, const void *, void *, int, hipStream_t);
 void bem_launch_field_memp(const double *, const int *, int, int, const void *, const void *, const int *, int, double,
                            double, const void *, int, double, int, const void *, int, double, const void *, int,
                            const void *, int, const void *, const void *, double, void *, hipStream_t);
@@ -2216,7 +2216,7 @@ extern "C" int bh_mo_field(bh_object **objs, int N, int what, int mode, int type
         const void *A0 = what == 0 ? (const void *)D.d_E : (const void *)D.d_U;
         const void *dA0 = what == 0 ? (const void *)D.d_dE : (const void *)D.d_dU;
         bem_launch_field(d_pts, d_sel, np, d, o->d_geo, D.d_sid, D.dNe, o->kn[d].real(), o->kn[d].imag(), g_rule[type],
-                         g_nrule[type], g_rscale[type], o->d_gq, g_qrule[type], g_nqrule[type], what, A0, dA0, D.d_Hf,
+                         g_nrule[type], nullptr, g_rscale[type], o->d_gq, g_qrule[type], g_nqrule[type], what, A0, dA0, D.d_Hf,
                          D.d_dHf, d_out, 1, cur_st());
       }
       HIPCHK(hipStreamSynchronize(cur_st()));
@@ -2372,9 +2372,11 @@ static int surface_fields(bh_object **objs, int N, int oid, const SurfPts &sp, c
     if (domain_ids(q, d_r, np, d_did)) return BH_EHIP;
     for (int d = 0; d <= q->MN; d++) {
       Dom &Q = q->dom[d];
+      // the mixed form: far triangles in FP32 with the reduced far rules, as
+      // in the object's own boundary integral
       bem_launch_field(d_r, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
-                       g_nrule[2], g_rscale[2], q->d_gq, g_qrule[2], g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf,
-                       d_EH, 1, cur_st());
+                       g_nrule[2], q->prec != BH_PREC_FP64 ? g.d_gl32 : nullptr, g_rscale[2], q->d_gq, g_qrule[2],
+                       g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf, d_EH, 1, cur_st());
     }
   }
   {
@@ -2743,8 +2745,8 @@ extern "C" int bh_mo_EH_bd(bh_object **objs, int N, int oid, int did, int np, co
       for (int d = 0; d <= q->MN; d++) {
         Dom &Q = q->dom[d];
         bem_launch_field(d_rt, d_did, np, d, q->d_geo, Q.d_sid, Q.dNe, q->kn[d].real(), q->kn[d].imag(), g_rule[2],
-                         g_nrule[2], g_rscale[2], q->d_gq, g_qrule[2], g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf,
-                         d_out, 1, cur_st());
+                         g_nrule[2], q->prec != BH_PREC_FP64 ? g.d_gl32 : nullptr, g_rscale[2], q->d_gq, g_qrule[2],
+                         g_nqrule[2], 0, Q.d_E, Q.d_dE, Q.d_Hf, Q.d_dHf, d_out, 1, cur_st());
       }
     }
     if (mode == 1 || mode == 2) {

@@ -418,12 +418,17 @@ __global__ void __launch_bounds__(256) k_domain(const double *__restrict__ pts, 
 // what: 0 -> E,H from (E,dE,H,dH); 1 -> U from (U,dU).  Output per point:
 // 6 (E,H) or 4 (U) complex values, scaled by 1/(1+F) (open domain) or 1/F.
 // Quadrangles (geo[].quad != 0) take the square's rule qrule (nqq points).
+// F32 (the mixed form's boundary fields of the other objects, as the own
+// boundary integral k_field_bd<true>): triangles farther than 4 element
+// radii from the point take the FP32 reduced far rule of the wave's far
+// lanes (gl32, tri_coef32), nearer ones the FP64 rule.
+template <bool F32>
 __global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, const int *__restrict__ pdid, int np,
                                                int want_did, const TriGeo *__restrict__ geo, const int *__restrict__ sid,
                                                int dNe, double kr, double ki, const QPtN *__restrict__ rule, int nq,
-                                               double scale, const QuadExt *__restrict__ gq,
-                                               const QPtQ *__restrict__ qrule, int nqq, int what,
-                                               const cd *__restrict__ A0, const cd *__restrict__ dA0,
+                                               const QPt32 *__restrict__ gl32, double scale,
+                                               const QuadExt *__restrict__ gq, const QPtQ *__restrict__ qrule, int nqq,
+                                               int what, const cd *__restrict__ A0, const cd *__restrict__ dA0,
                                                const cd *__restrict__ A1, const cd *__restrict__ dA1,
                                                cd *__restrict__ out, int accumulate)
 {
@@ -438,7 +443,7 @@ __global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, c
     int sd = sid[s], as = sd < 0 ? -sd : sd;
     const TriGeo g = geo[as];
     cd G[4], H[4];
-    double f;
+    double f = 0.0;
     int ns = 3;
     if (g.quad != 0.0) {
       quad_coef(g, gq[as], rt, kr, ki, qrule, nqq, sd < 0, G, H, f);
@@ -446,7 +451,16 @@ __global__ void __launch_bounds__(256) k_field(const double *__restrict__ pts, c
     } else {
       cd dGx[3], dHx[3];
       double dfx;
-      tri_coef<false>(g, rt, nullptr, kr, ki, rule, nq, scale, sd < 0, G, H, f, dGx, dHx, dfx);
+      bool far = false;
+      if constexpr (F32) {
+        const double dx = g.c0[0] - rt[0], dy = g.c0[1] - rt[1], dz = g.c0[2] - rt[2];
+        const double rad2 = elem_rad2(g);
+        far = dx * dx + dy * dy + dz * dz > BEM_FAR_RADII * BEM_FAR_RADII * rad2;
+        int qoff, nq32;
+        far_rule(far, sqrt((kr * kr + ki * ki) * rad2), (dx * dx + dy * dy + dz * dz) / rad2, qoff, nq32);
+        if (far) tri_coef32<false>(g, rt, nullptr, kr, ki, gl32 + qoff, nq32, scale, sd < 0, G, H, f, dGx, dHx, dfx);
+      }
+      if (!far) tri_coef<false>(g, rt, nullptr, kr, ki, rule, nq, scale, sd < 0, G, H, f, dGx, dHx, dfx);
       G[3] = H[3] = mk(0, 0);
     }
     F += f;
@@ -1545,16 +1559,24 @@ void bem_launch_domain(const double *pts, int np, const double *fv, const int *f
   hipLaunchKernelGGL(k_domain, dim3((np + 3) / 4), dim3(256), 0, st, pts, np, fv, fsd, fq, segStart, MN, did);
 }
 // gq, qrule, nqq: quadrangle terms and the square's rule of the same type
+// gl32 != nullptr: far triangles by the FP32 reduced far rules (k_field<true>)
 void bem_launch_field(const double *pts, const int *pdid, int np, int want_did, const void *geo, const int *sid,
-                      int dNe, double kr, double ki, const void *rule, int nq, double scale, const void *gq,
-                      const void *qrule, int nqq, int what, const void *A0, const void *dA0, const void *A1,
-                      const void *dA1, void *out, int accumulate, hipStream_t st)
+                      int dNe, double kr, double ki, const void *rule, int nq, const void *gl32, double scale,
+                      const void *gq, const void *qrule, int nqq, int what, const void *A0, const void *dA0,
+                      const void *A1, const void *dA1, void *out, int accumulate, hipStream_t st)
 {
   if (np <= 0) return;
   bem_prof_begin(BH_K_FIELD, (double)np * dNe, st);
-  hipLaunchKernelGGL(k_field, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo, sid,
-                     dNe, kr, ki, (const QPtN *)rule, nq, scale, (const QuadExt *)gq, (const QPtQ *)qrule, nqq, what,
-                     (const cd *)A0, (const cd *)dA0, (const cd *)A1, (const cd *)dA1, (cd *)out, accumulate);
+  if (gl32)
+    hipLaunchKernelGGL(k_field<true>, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did, (const TriGeo *)geo,
+                       sid, dNe, kr, ki, (const QPtN *)rule, nq, (const QPt32 *)gl32, scale, (const QuadExt *)gq,
+                       (const QPtQ *)qrule, nqq, what, (const cd *)A0, (const cd *)dA0, (const cd *)A1, (const cd *)dA1,
+                       (cd *)out, accumulate);
+  else
+    hipLaunchKernelGGL(k_field<false>, dim3((np + 3) / 4), dim3(256), 0, st, pts, pdid, np, want_did,
+                       (const TriGeo *)geo, sid, dNe, kr, ki, (const QPtN *)rule, nq, (const QPt32 *)nullptr, scale,
+                       (const QuadExt *)gq, (const QPtQ *)qrule, nqq, what, (const cd *)A0, (const cd *)dA0,
+                       (const cd *)A1, (const cd *)dA1, (cd *)out, accumulate);
   bem_prof_end(BH_K_FIELD, st);
 }
 // qrule / qgh: the square's rules matching rule / gh, for quadrangles

@@ -484,7 +484,11 @@ def main():
     log("kernel-timing configuration (one stream): %.3f s" % s_serial)
     dom = max(kern, key=lambda k: kern[k][0])
     roof = roofline(dom, kern[dom])
+    t1 = time.perf_counter()
     fld, oid, FN = scene_outputs(bem, G, objs, pts)
+    bem.sync()
+    s_outputs = time.perf_counter() - t1
+    log("outputs (fields at %d points, force and torque of every object): %.3f s" % (len(pts), s_outputs))
     for g in G:
         g.close()
 
@@ -498,10 +502,13 @@ def main():
         it64 = scene_step(bem, G64)
         t64 = time.perf_counter() - t1
         log("fp64 form: %.1f s/config (%d iterations)" % (t64, it64))
+        t1 = time.perf_counter()
         f64, oid64, FN64 = scene_outputs(bem, G64, objs, pts)
+        bem.sync()
+        t64_out = time.perf_counter() - t1
         for g in G64:
             g.close()
-        fp64 = {"value": t64, "unit": "s/config", "iterations": it64,
+        fp64 = {"value": t64, "unit": "s/config", "iterations": it64, "outputs_s": t64_out,
                 "rel_err_of_mixed": rel_errors(fld, oid, FN, f64, oid64, FN64)}
     cpu = finish_cpu_baseline(cpu_proc) if cpu_proc else None
     l4 = None if args.no_l4 else l4_extra(bem)
@@ -529,6 +536,9 @@ def main():
         "triangles_per_s": S / s_step,
         "phases_s": {"objects": sum(p[0] for p in phases) / len(phases),
                      "coupling_loop": sum(p[1] for p in phases) / len(phases)},
+        "outputs_s": {"mixed": s_outputs, "fp64": fp64["outputs_s"] if fp64 else None,
+                      "what": "E, H at 200 points outside the objects and force + torque of every object (rule 0), "
+                              "after the configuration; not part of value"},
         "fp64_form": fp64,
         "speedup_vs_fp64": (fp64["value"] / s_step) if fp64 else None,
         "kernels_ms_per_step": {k: v[0] for k, v in kern.items() if v[2]},
