@@ -173,28 +173,7 @@ __global__ void __launch_bounds__(256) k_gemm_skinny(int m, int n, int k, typena
     for (int j = 0; j < NC; j++) acc[r][j] = zmk<Z>(0, 0);
   const Z *a0 = A + (size_t)min(row0 + 0, m - 1) * lda, *a1 = A + (size_t)min(row0 + 1, m - 1) * lda;
   const Z *a2 = A + (size_t)min(row0 + 2, m - 1) * lda, *a3 = A + (size_t)min(row0 + 3, m - 1) * lda;
-  // U consecutive 64-wide chunks of k per step: their loads are all issued
-  // before the products, so each wave keeps U times the bytes in flight
-  // (the partial sums still take the chunks in order)
-  constexpr int U = NC <= 4 ? 4 : 2;
-  int kk = lane;
-  for (; kk + 64 * (U - 1) < k; kk += 64 * U) {
-    Z b[U][NC], v[U][4];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int q = kk + 64 * u;
-#pragma unroll
-      for (int j = 0; j < NC; j++) b[u][j] = j < n ? B[(size_t)q * ldb + j] : zmk<Z>(0, 0);
-      v[u][0] = a0[q]; v[u][1] = a1[q]; v[u][2] = a2[q]; v[u][3] = a3[q];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int j = 0; j < NC; j++) acc[r][j] = acc[r][j] + v[u][r] * b[u][j];
-  }
-  for (; kk < k; kk += 64) {
+  for (int kk = lane; kk < k; kk += 64) {
     Z b[NC];
 #pragma unroll
     for (int j = 0; j < NC; j++) b[j] = j < n ? B[(size_t)kk * ldb + j] : zmk<Z>(0, 0);
