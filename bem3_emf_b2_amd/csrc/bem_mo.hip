@@ -962,8 +962,9 @@ extern "C" void bem_mo_release(void)
 
 // assembly and factorisation of N objects (create_matrix of each,
 // solve_bieq2.c:10-96): the assembly of every object and domain as one
-// batch (one launch per assembly pass), then the factorisations spread over
-// the lanes; type and mode as bh_object_assemble and bh_object_factor.
+// batch (one launch per assembly pass), then the factorisations (with
+// N > 1 also the Schur complements of the coupling loop) spread over the
+// lanes; type and mode as bh_object_assemble and bh_object_factor.
 // BEM3_HIP_ASM_BATCH=0: each object assembled on its lane before its
 // factorisation (assembly and factorisations of different objects overlap).
 extern "C" int bh_mo_prepare(bh_object **objs, int N, int type, int mode)
@@ -980,10 +981,15 @@ extern "C" int bh_mo_prepare(bh_object **objs, int N, int type, int mode)
     int rc = objects_assemble(objs, N, type);
     if (rc) return rc;
   }
+  // with several objects the coupling loop follows: each object's Schur
+  // complement S and its LU are formed in the same pass (form_s, which the
+  // loop then finds done), so that one pass over the lanes, costliest
+  // object first, covers every per-object factorisation
   return run_lanes(objs, N, cost_of(objs, N, 3), [&](bh_object *o) {
     int r = batch ? 0 : object_assemble(o, type);
     if (!r) r = object_factor(o, mode);
     if (!r && g1) r = object_factor_g(o);
+    if (!r && N > 1) r = form_s(o);
     return r;
   });
 }

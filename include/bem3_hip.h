@@ -144,9 +144,10 @@ int bh_object_incident(bh_object *o, int which, double *buf, int set);
  * (solve_bieq2.c:10-96).  The assembly of every object and domain runs as
  * one batch (one launch per assembly pass over all of them; tuning entry
  * BH_TUNE_ASM_BATCH = 0: each object on its stream before its
- * factorisation), the factorisations spread over several device streams
- * (tuning entry BH_TUNE_LANES), so that the latency-bound pieces of
- * different objects overlap.                                              */
+ * factorisation), the factorisations -- with N > 1 also the Schur
+ * complements the coupling loop solves with -- spread over several device
+ * streams (tuning entry BH_TUNE_LANES), so that the latency-bound pieces
+ * of different objects overlap.                                           */
 int bh_mo_prepare(bh_object **objs, int N, int type, int mode);
 /* Coupling loop over N objects (mo_solve_bv, mo_solve.c:10-88): every
  * object factorised and given the same beams.  Convergence when the
