@@ -176,6 +176,16 @@ def cpu_scene_baseline(target_s=20.0):
     NN = 3.0 * Ne
     t_lu = float((8.0 * na ** 3 + 2 * (8.0 / 3.0) * NN ** 3).sum()) / rate
     t_lu_block = float((43.0 * NN ** 3).sum()) / rate
+    # the same factorisation by numpy's LAPACK (an optimised BLAS: the closest
+    # stand-in here for the reference's MKL zgetrf/zgetri)
+    nl = 3072
+    A = rng.standard_normal((nl, nl)) + 1j * rng.standard_normal((nl, nl)) + nl * np.eye(nl)
+    B = rng.standard_normal((nl, 1)) + 0j
+    t0 = time.perf_counter()
+    np.linalg.solve(A, B)
+    tll = time.perf_counter() - t0
+    del A
+    rate_lapack = (8.0 / 3.0) * nl ** 3 / tll
     # coupling sample: the field of O[1] at the slots of O[0]
     t0 = time.perf_counter()
     orc.add_scattered(O[0], O[1])
@@ -189,8 +199,14 @@ def cpu_scene_baseline(target_s=20.0):
     for ob in O:
         ob.free()
     total = t_asm + t_lu + t_coup + t_solve
+    t_lu_l = float((8.0 * na ** 3 + 2 * (8.0 / 3.0) * NN ** 3).sum()) / rate_lapack
+    t_solve_l = float((8.0 * na ** 2).sum()) * sweeps / rate_lapack
     return {"value": total, "unit": "s/config", "cores": cores, "nproc": os.cpu_count(), "kind": "port",
             "value_block_form": t_asm + t_lu_block + t_coup + float((8.0 * (8 * NN) ** 2).sum()) * sweeps / rate,
+            "value_lapack_lu": t_asm + t_lu_l + t_coup + t_solve_l,
+            "lapack_lu_sample": "numpy.linalg.solve n=%d complex128: %.2f s, %.1f GFLOP/s (the dense solve and the "
+                                "per-sweep mat-vecs of value at this rate: %.0f s + %.0f s)"
+                                % (nl, tll, rate_lapack / 1e9, t_lu_l, t_solve_l),
             "sample": ("oracle on %d host threads (nproc %s): assembly rows of %d/%d elements x 2 domains of one "
                        "object (%.1f s; %.3g s per pair, scaled to the scene's sum 6 Ne^2 = %.3g pairs), LU n=%d "
                        "(%.1f s, %.1f GFLOP/s, scaled to 8 na^3 + 2x8/3 NN^3 per object), add_scattered of one "
