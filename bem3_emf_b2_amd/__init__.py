@@ -72,6 +72,7 @@ def lib():
             "bh_finalize": (C.c_int, []),
             "bh_mo_set_coupling": (C.c_int, [C.c_int]),
             "bh_mo_plan_stats": (C.c_int, [Ip, Ip, Ip]),
+            "bh_mo_plan_split": (C.c_int, [Ip]),
             "bh_mo_prepare": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, C.c_int]),
             "bh_mo_coupling_sums": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D]),
             "bh_mo_solve_beams": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, D, C.c_double, C.c_int, C.c_int, Ip]),
@@ -148,6 +149,14 @@ def plan_stats():
     a = np.zeros(3, np.int32)
     check(lib().bh_mo_plan_stats(ip(a[0:1]), ip(a[1:2]), ip(a[2:3])))
     return int(a[0]), int(a[1]), int(a[2])
+
+
+def plan_split():
+    """object pairs of the last one-beam coupling loop whose series takes only
+    some 64-slot tiles of the destination (the rest by element sums)"""
+    a = np.zeros(1, np.int32)
+    check(lib().bh_mo_plan_split(ip(a)))
+    return int(a[0])
 
 
 # kernel ids of the timing registry (bem3_hip.h)

@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(256) k_ccd_sum(const double *__restrict__ part
 namespace {
 
 int g_coupling_mode = 0;   // 0: series where the pair allows it (mixed form), 1: direct sums only
-int g_last_series = 0, g_last_direct = 0, g_last_lmax = -1;   // the last loop's plan
+int g_last_series = 0, g_last_direct = 0, g_last_split = 0, g_last_lmax = -1;   // the last loop's plan
 
 // grow-only work areas of the loop (bem_mo_release frees them)
 Scratch *g_ws = nullptr;
@@ -700,17 +700,27 @@ void slot_points(const bh_object *o, std::vector<double> &pt)
 struct Plan {
   std::vector<SrcGeo> geo;
   std::vector<std::vector<int>> Lmn;   // [dst][src]: series order, -1 direct, -2 same object
+  // split pairs (Lmn -1, the one-beam loop only): [dst][src] series order of
+  // the 64-slot tiles whose slots all qualify on their own (-1: not split),
+  // and per tile 1 for the series, 0 for the element sums
+  std::vector<std::vector<int>> Ls;
+  std::vector<std::vector<std::vector<char>>> tile_mp;
   std::vector<int> Lsrc;               // per source: largest order over its pairs (-1: none)
   std::vector<int> rule;               // per source: rule-table offset of its far rule
   std::vector<int> npt;                // and points per element
-  int npairs_mp = 0, npairs_direct = 0;
+  int npairs_mp = 0, npairs_direct = 0, npairs_split = 0;
 };
 
-int make_plan(bh_object **objs, int N, Plan &P)
+// split: a pair whose slots do not all qualify takes the series on the
+// 64-slot tiles whose slots do (the one-beam loop; the many-beam loop sums
+// such pairs by elements)
+int make_plan(bh_object **objs, int N, Plan &P, bool split)
 {
   P.geo.resize(N);
   for (int m = 0; m < N; m++) P.geo[m] = src_geo(objs[m]);
   P.Lmn.assign(N, std::vector<int>(N, -1));
+  P.Ls.assign(N, std::vector<int>(N, -1));
+  P.tile_mp.assign(N, std::vector<std::vector<char>>(N));
   P.Lsrc.assign(N, -1);
   P.rule.assign(N, 0);
   P.npt.assign(N, 0);
@@ -727,21 +737,51 @@ int make_plan(bh_object **objs, int N, Plan &P)
       const bool eligible = g_coupling_mode == 0 && s->prec != BH_PREC_FP64 && objs[m]->prec != BH_PREC_FP64 &&
                             G.ok && k.imag() == 0.0 && k.real() > 0.0 && G.rad > 0.0;
       if (!eligible) continue;
-      double dmin = 1e300;
+      // the smallest distance of each 64-slot tile from the source's centre
+      const size_t nt = (ns + 63) / 64;
+      std::vector<double> td(nt, 1e300);
       for (size_t p = 0; p < ns; p++) {
         double q = 0.0;
         for (int d = 0; d < 3; d++) q += (pt[3 * p + d] - G.c[d]) * (pt[3 * p + d] - G.c[d]);
-        dmin = std::min(dmin, q);
+        td[p / 64] = std::min(td[p / 64], q);
       }
+      const double kr = k.real() * G.rad;
+      auto order = [&](double rho) {
+        return std::max((int)ceil(log(MP_EPS) / log(rho)), (int)ceil(kr + 6.0 * cbrt(kr)) + 4);
+      };
+      double dmin = 1e300;
+      for (double q : td) dmin = std::min(dmin, q);
       dmin = sqrt(dmin);
       const double rho = G.rad / dmin, gap = dmin - G.rad;
-      if (!(rho <= MP_RHO) || gap < MP_GAP_RADII * G.erad) continue;
-      const double kr = k.real() * G.rad;
-      const int L = std::max((int)ceil(log(MP_EPS) / log(rho)), (int)ceil(kr + 6.0 * cbrt(kr)) + 4);
+      if (rho <= MP_RHO && gap >= MP_GAP_RADII * G.erad) {
+        const int L = order(rho);
+        if (L > MP_LMAX) continue;
+        P.Lmn[m][n] = L;
+        P.Lsrc[n] = std::max(P.Lsrc[n], L);
+        mingap[n] = std::min(mingap[n], gap);
+        continue;
+      }
+      if (!split) continue;
+      // the tiles that qualify on their own
+      std::vector<char> fl(nt, 0);
+      double rmax = 0.0, gmin = 1e300;
+      int nq = 0;
+      for (size_t t = 0; t < nt; t++) {
+        const double dt = sqrt(td[t]), rt = G.rad / dt, gt = dt - G.rad;
+        if (rt <= MP_RHO && gt >= MP_GAP_RADII * G.erad) {
+          fl[t] = 1;
+          nq++;
+          rmax = std::max(rmax, rt);
+          gmin = std::min(gmin, gt);
+        }
+      }
+      if (nq == 0) continue;
+      const int L = order(rmax);
       if (L > MP_LMAX) continue;
-      P.Lmn[m][n] = L;
+      P.Ls[m][n] = L;
+      P.tile_mp[m][n].swap(fl);
       P.Lsrc[n] = std::max(P.Lsrc[n], L);
-      mingap[n] = std::min(mingap[n], gap);
+      mingap[n] = std::min(mingap[n], gmin);
     }
   }
   // far rule of each source (the mixed form's rules, keyed as far_rule keys
@@ -757,7 +797,7 @@ int make_plan(bh_object **objs, int N, Plan &P)
   }
   for (int m = 0; m < N; m++)
     for (int n = 0; n < N; n++)
-      if (m != n) (P.Lmn[m][n] >= 0 ? P.npairs_mp : P.npairs_direct)++;
+      if (m != n) (P.Lmn[m][n] >= 0 ? P.npairs_mp : P.Ls[m][n] >= 0 ? P.npairs_split : P.npairs_direct)++;
   return 0;
 }
 
@@ -860,18 +900,23 @@ int mp_coefficients_multi(bh_object *o, const Plan &P, int n, cd *coef, int nb)
 // the scattered field of src at dst's node slots by direct sums, added into
 // dst's incident data; check: first verify that no slot lies inside src
 // (the geometry does not change during a solve: checked on the first sweep)
-int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double *d_part)
+// slots s0 .. s0+n of dst (n < 0: every slot)
+int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double *d_part, int s0 = 0, int n = -1)
 {
+  if (n < 0) n = dst->nslot - s0;
+  if (s0 < 0 || n <= 0 || s0 + n > dst->nslot) return n == 0 ? 0 : fail(BH_EARG, "coupling sums: slots %d + %d", s0, n);
+  const double *pts = dst->d_slotPt + 3 * (size_t)s0, *nrm = dst->d_slotN + 3 * (size_t)s0;
+  const int *sidx = dst->d_slotIdx + s0;
   if (check) {
-    if (domain_ids(src, dst->d_slotPt, dst->nslot, d_did)) return BH_EHIP;
-    std::vector<int> did(dst->nslot);
+    if (domain_ids(src, pts, n, d_did)) return BH_EHIP;
+    std::vector<int> did(n);
     HIPCHK(hipMemcpyAsync(did.data(), d_did, did.size() * sizeof(int), hipMemcpyDeviceToHost, g.st));
     HIPCHK(hipStreamSynchronize(g.st));
     for (int v : did)
       if (v != 0) return fail(BH_EDOM, "scattered-field domain error (overlapping objects), domain id %d", v);
   }
   Dom &S = src->dom[0];
-  bem_launch_scatter(dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, src->d_geo, src->d_gq, S.d_sid, S.dNe,
+  bem_launch_scatter(pts, nrm, sidx, n, src->d_geo, src->d_gq, S.d_sid, S.dNe,
                      src->kn[0].real(), src->kn[0].imag(), g_rule[2], g_nrule[2], g_rule[3], g_nrule[3], g_rscale[2],
                      g_qrule[2], g_nqrule[2], g_qrule[3], g_nqrule[3], src->prec != BH_PREC_FP64 ? g.d_gl32 : nullptr,
                      S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, src->nquad > 0, d_part, g.st);
@@ -1026,6 +1071,15 @@ extern "C" int bh_mo_plan_stats(int *series, int *direct, int *lmax)
   return 0;
 }
 
+// object pairs of the last coupling loop whose series takes only some
+// tiles of the destination's slots (the rest by element sums)
+extern "C" int bh_mo_plan_split(int *split)
+{
+  API_LOCK;
+  if (split) *split = g_last_split;
+  return 0;
+}
+
 // block-Jacobi fixed point over objects (mo_solve_bv, mo_solve.c:10-88).
 // trace (optional, iter_max+1 doubles) receives cc per iteration; *iters
 // receives the final iteration count.
@@ -1045,11 +1099,14 @@ struct Coupling {
   MpPair *d_pairs = nullptr;
   double evwork = 0.0;
 
-  int init(bh_object **objs, int N)
+  // split: pairs whose slots do not all qualify for the series take it on
+  // the tiles whose slots do (the one-beam loop)
+  int init(bh_object **objs, int N, bool split = false)
   {
-    make_plan(objs, N, P);
+    make_plan(objs, N, P, split);
     g_last_series = P.npairs_mp;
     g_last_direct = P.npairs_direct;
+    g_last_split = P.npairs_split;
     g_last_lmax = -1;
     for (int n = 0; n < N; n++) g_last_lmax = std::max(g_last_lmax, P.Lsrc[n]);
     slot0.assign(N, 0);
@@ -1065,27 +1122,40 @@ struct Coupling {
     for (int n = 0; n < N; n++)
       if (P.Lsrc[n] >= 0) { coff[n] = ncoef; ncoef += (size_t)mp_count(P.Lsrc[n] + 1) * 16; }
     d_coef = ncoef ? ws<cd>(W_COEF, ncoef) : nullptr;
+    auto mp_pair = [&](int n, int L) {
+      MpPair q{};
+      q.coef = d_coef + coff[n];
+      for (int d = 0; d < 3; d++) q.c[d] = P.geo[n].c[d];
+      q.kr = objs[n]->kn[0].real();
+      q.Lp = P.Lsrc[n] + 1;
+      q.Le = L + 1;
+      return q;
+    };
     for (int m = 0; m < N; m++) {
       bh_object *o = objs[m];
+      bool any_split = false;
+      for (int n = 0; n < N; n++) any_split = any_split || P.Ls[m][n] >= 0;
+      // the object's series sources, shared by its tiles unless a pair is split
       const int p0 = (int)pairs.size();
-      for (int n = 0; n < N; n++) {
-        if (P.Lmn[m][n] < 0) continue;
-        MpPair q{};
-        q.coef = d_coef + coff[n];
-        for (int d = 0; d < 3; d++) q.c[d] = P.geo[n].c[d];
-        q.kr = objs[n]->kn[0].real();
-        q.Lp = P.Lsrc[n] + 1;
-        q.Le = P.Lmn[m][n] + 1;
-        pairs.push_back(q);
-      }
-      const int np = (int)pairs.size() - p0;
-      if (!np) continue;
+      for (int n = 0; n < N; n++)
+        if (P.Lmn[m][n] >= 0) pairs.push_back(mp_pair(n, P.Lmn[m][n]));
+      const int np0 = (int)pairs.size() - p0;
       for (int s0 = 0; s0 < o->nslot; s0 += 64) {
         MpTile t{};
         t.pts = o->d_slotPt; t.nrm = o->d_slotN; t.slot = o->d_slotIdx; t.Ui = o->d_Ui; t.dUi = o->d_dUi;
         t.s0 = s0; t.ns = std::min(64, o->nslot - s0);
-        t.p0 = p0; t.np = np;
-        tiles.push_back(t);
+        t.p0 = p0; t.np = np0;
+        if (any_split) {
+          // this tile's own list, sources in order: the series pairs and the
+          // split pairs whose series takes this tile
+          t.p0 = (int)pairs.size();
+          for (int n = 0; n < N; n++) {
+            if (P.Lmn[m][n] >= 0) pairs.push_back(mp_pair(n, P.Lmn[m][n]));
+            else if (P.Ls[m][n] >= 0 && P.tile_mp[m][n][s0 / 64]) pairs.push_back(mp_pair(n, P.Ls[m][n]));
+          }
+          t.np = (int)pairs.size() - t.p0;
+        }
+        if (t.np > 0) tiles.push_back(t);
       }
     }
     if (!tiles.empty()) {
@@ -1114,10 +1184,29 @@ struct Coupling {
       hipLaunchKernelGGL(k_mp_eval, dim3((unsigned)tiles.size()), dim3(64), 0, g.st, d_tiles, d_pairs);
       bem_prof_end(BH_K_MP_EVAL, g.st);
     }
-    // the other pairs: direct sums
+    // the other pairs: direct sums (of a split pair, over the runs of tiles
+    // the series does not take)
     for (int m = 0; m < N; m++)
-      for (int d = 0; d < N; d++)
-        if (d != m && P.Lmn[d][m] == -1) { int r = add_scattered(objs[d], objs[m], d_did, check, d_part); if (r) return r; }
+      for (int d = 0; d < N; d++) {
+        if (d == m || P.Lmn[d][m] != -1) continue;
+        bh_object *dst = objs[d];
+        if (P.Ls[d][m] < 0) {
+          int r = add_scattered(dst, objs[m], d_did, check, d_part);
+          if (r) return r;
+          continue;
+        }
+        const std::vector<char> &fl = P.tile_mp[d][m];
+        const int nt = (int)fl.size();
+        for (int t0 = 0; t0 < nt;) {
+          if (fl[t0]) { t0++; continue; }
+          int t1 = t0;
+          while (t1 < nt && !fl[t1]) t1++;
+          const int s0 = 64 * t0, n = std::min(64 * t1, dst->nslot) - s0;
+          int r = add_scattered(dst, objs[m], d_did, check, d_part, s0, n);
+          if (r) return r;
+          t0 = t1;
+        }
+      }
     HIPCHK(hipGetLastError());
     return 0;
   }
@@ -1239,7 +1328,7 @@ extern "C" int bh_mo_coupling_sums(bh_object **objs, int N, int dst, double *out
   for (int m = 0; m < N; m++)
     if (!objs[m]->dom[0].d_U || !objs[m]->d_Ui) return fail(BH_ESTATE, "object %d is not solved", m);
   Coupling C;
-  int rc = C.init(objs, N);
+  int rc = C.init(objs, N, true);   // the plan of the one-beam loop
   if (rc) return rc;
   // keep the incident data, clear them, add the sums, read dst's, restore
   std::vector<cd *> keep(2 * (size_t)N);
@@ -1356,7 +1445,7 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     // the device room
     gpre.start(objs, N, c3);
     Coupling C;
-    if ((rc = C.init(objs, N))) return rc;
+    if ((rc = C.init(objs, N, true))) return rc;
     bool first = true;
     // wall seconds of the last sweep's coupling sums (to their completion)
     // and of its solves, for the progress lines
@@ -1417,8 +1506,8 @@ extern "C" int bh_mo_solve(bh_object **objs, int N, double ccd, int iter_min, in
     if (pe && *pe && strcmp(pe, "0") != 0) pf = strcmp(pe, "1") == 0 ? stderr : fopen(pe, "a");
     struct Closer { FILE *f; ~Closer() { if (f && f != stderr) fclose(f); } } closer{pf};
     if (pf) {
-      fprintf(pf, "bh_mo_solve: %d objects, %d object pairs by series, %d by direct sums; Schur complements %.3f s\n", N,
-              C.P.npairs_mp, C.P.npairs_direct, std::chrono::duration<double>(std::chrono::steady_clock::now() - tl0).count());
+      fprintf(pf, "bh_mo_solve: %d objects, %d object pairs by series, %d by direct sums, %d split; Schur complements %.3f s\n",
+              N, C.P.npairs_mp, C.P.npairs_direct, C.P.npairs_split, std::chrono::duration<double>(std::chrono::steady_clock::now() - tl0).count());
       fflush(pf);
     }
     {

@@ -484,6 +484,7 @@ def main():
     dt = time.perf_counter() - t0
     s_step = dt / args.steps
     series, direct, lmax = bem.plan_stats()
+    split = bem.plan_split()
     # kernel timings: one more configuration with the per-object work on one
     # stream, so that the events around each launch bracket that kernel alone
     # (with several streams they would also count the other streams' kernels)
@@ -548,7 +549,8 @@ def main():
                                "n 1.4-1.7) in a 10 um cube of water, x-pol plane wave 1.064 um; assembly + factor of "
                                "every object + mo_solve coupling to a 1e-8 change; mixed-precision form",
                    "objects": len(objs), "triangles": S, "coupling_iterations": it,
-                   "coupling_pairs": {"series": series, "element_sums": direct, "largest_order": lmax}},
+                   "coupling_pairs": {"series": series, "element_sums": direct, "split": split,
+                                      "largest_order": lmax}},
         "triangles_per_s": S / s_step,
         "phases_s": {"objects": sum(p[0] for p in phases) / len(phases),
                      "coupling_loop": sum(p[1] for p in phases) / len(phases)},

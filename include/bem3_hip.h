@@ -167,11 +167,18 @@ int bh_mo_set_lanes(int n);
  * objects are in the mixed-precision form, the source has triangles only,
  * the surrounding medium is lossless and the slots keep a distance (source
  * sphere radius / slot distance <= 0.6, truncation about 1e-10 relative),
- * the reference's element sums elsewhere; 1 the element sums everywhere.  */
+ * decided for all slots of a destination object or, in the one-beam loop,
+ * per tile of 64 of its slots; the reference's element sums elsewhere; 1
+ * the element sums everywhere.                                            */
 int bh_mo_set_coupling(int mode);
 /* ordered object pairs of the last coupling loop of several objects summed
  * by the series and by element sums, and the largest series order used   */
 int bh_mo_plan_stats(int *series, int *direct, int *lmax);
+/* ordered pairs of the last one-beam coupling loop that were split: the
+ * series takes the 64-slot tiles of the destination whose slots all meet
+ * its conditions, the element sums the other tiles (neither counted by
+ * bh_mo_plan_stats); no counterpart in the reference                     */
+int bh_mo_plan_split(int *split);
 /* The coupling sums of the loop (add_scattered_mEMP / mEMP2_s,
  * mo_solve.c:296-396, summed as bh_mo_solve would sum them) from the
  * objects' current boundary values, at object dst's node slots: out

@@ -1,8 +1,9 @@
 """Coupling sums of the mixed-precision form by per-object multipole series
 (bem_mo.hip) against the element sums of the same form (the reduced far
 rules of mEMP2_s, mo_solve.c:296-396): five objects, four of them far
-enough apart for the series, one pair close enough to need the element
-sums.  The series reproduces the discrete sums to its truncation (about
+enough apart for the series, one pair so close that it is split (the series
+on the 64-slot tiles whose slots keep their distance, element sums on the
+others).  The series reproduces the discrete sums to its truncation (about
 1e-10 relative) plus the solid-angle terms it takes as zero (quadrature
 level); the element sums of the mixed form add their pairs in FP32 (about
 1e-8 relative on the scattered field, measured 1.4e-8 on the solved state),
@@ -24,7 +25,8 @@ def _scene():
         (mg.box(0.6, 3, center=(0.2, 1.5, 0.1)), 1.60),
         (mg.cone(0.35, 0.7, 12, 3, center=(0.4, -1.5, 0.3)), 1.55),
         (mg.icosphere(2, 0.3, center=(1.7, 0.2, -0.4)), 1.70),
-        # close to the last sphere: the pair takes element sums
+        # close to the last sphere: the pair is split (series on the tiles of
+        # slots that keep their distance, element sums on the others)
         (mg.icosphere(1, 0.2, center=(1.7, 0.2, 0.25)), 1.50),
     ]
 
@@ -42,7 +44,7 @@ def _solve(mode):
         g.factor(0)
         G.append(g)
     it, tr = bem.mo_solve(G, ccd=1e-8)
-    stats = bem.plan_stats()
+    stats = bem.plan_stats() + (bem.plan_split(),)
     U = [g.state(0, "U") for g in G]
     rng = np.random.default_rng(3)
     pts = rng.uniform(-2.5, 2.5, (64, 3))
@@ -61,10 +63,13 @@ def _rel(a, b):
 def test_series_coupling_matches_element_sums():
     ref = _solve("direct")
     got = _solve("auto")
-    series, direct, lmax = got["stats"]
-    print("pairs by series %d, by element sums %d, largest order %d" % (series, direct, lmax), flush=True)
-    assert series >= 10 and direct >= 2 and lmax > 0
-    assert ref["stats"][0] == 0
+    series, direct, lmax, split = got["stats"]
+    print("pairs by series %d, by element sums %d, split %d, largest order %d" % (series, direct, split, lmax),
+          flush=True)
+    # the close pair is split: the series takes the far tiles of its slots,
+    # the element sums the near ones
+    assert series >= 10 and split >= 1 and direct + split >= 2 and lmax > 0
+    assert ref["stats"][0] == 0 and ref["stats"][3] == 0
     assert got["it"] == ref["it"]
     for a, b in zip(got["U"], ref["U"]):
         assert _rel(a, b) < 1e-7, _rel(a, b)
