@@ -63,6 +63,7 @@ void bem_launch_scatter(const double *, const double *, const int *, int, const 
 #define MP_LMAX 50              // largest series order
 #define MP_LPMAX (MP_LMAX + 1)  // moments and gradient sets run one order further
 #define MP_RHO 0.6              // largest source-sphere radius / slot distance
+#define MP_RHO_SPLIT 0.5        // the same for the tiles of a split pair (keeps its order near the others')
 #define MP_EPS 1e-10            // truncation target (relative)
 #define MP_GAP_RADII 4.0        // gap to the source sphere in element radii
 #define MP_NDENS 16             // point densities: 4 potentials x (monopole, dipole z, +, -)
@@ -634,7 +635,7 @@ int g_last_series = 0, g_last_direct = 0, g_last_split = 0, g_last_lmax = -1;   
 // grow-only work areas of the loop (bem_mo_release frees them)
 Scratch *g_ws = nullptr;
 enum { W_DID, W_PART, W_CCD, W_CCDSUM, W_B, W_DENS, W_T, W_TSUM, W_COEF, W_TILES, W_PAIRS, W_MS, W_MC, W_FPTS,
-       W_FEH, W_FI, W_NWS };
+       W_FEH, W_FI, W_SPT, W_SNM, W_SIDX, W_SLST, W_NWS };
 template <class T> T *ws(int i, size_t n)
 {
   if (!g_ws) g_ws = new Scratch[W_NWS];
@@ -768,7 +769,7 @@ int make_plan(bh_object **objs, int N, Plan &P, bool split)
       int nq = 0;
       for (size_t t = 0; t < nt; t++) {
         const double dt = sqrt(td[t]), rt = G.rad / dt, gt = dt - G.rad;
-        if (rt <= MP_RHO && gt >= MP_GAP_RADII * G.erad) {
+        if (rt <= MP_RHO_SPLIT && gt >= MP_GAP_RADII * G.erad) {
           fl[t] = 1;
           nq++;
           rmax = std::max(rmax, rt);
@@ -900,13 +901,12 @@ int mp_coefficients_multi(bh_object *o, const Plan &P, int n, cd *coef, int nb)
 // the scattered field of src at dst's node slots by direct sums, added into
 // dst's incident data; check: first verify that no slot lies inside src
 // (the geometry does not change during a solve: checked on the first sweep)
-// slots s0 .. s0+n of dst (n < 0: every slot)
-int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double *d_part, int s0 = 0, int n = -1)
+// n slots of dst given by their positions, normals and incident-data
+// indices (device arrays; every slot of dst, or a compact subset)
+int add_scattered_at(bh_object *dst, bh_object *src, const double *pts, const double *nrm, const int *sidx, int n,
+                     int *d_did, bool check, double *d_part)
 {
-  if (n < 0) n = dst->nslot - s0;
-  if (s0 < 0 || n <= 0 || s0 + n > dst->nslot) return n == 0 ? 0 : fail(BH_EARG, "coupling sums: slots %d + %d", s0, n);
-  const double *pts = dst->d_slotPt + 3 * (size_t)s0, *nrm = dst->d_slotN + 3 * (size_t)s0;
-  const int *sidx = dst->d_slotIdx + s0;
+  if (n <= 0) return 0;
   if (check) {
     if (domain_ids(src, pts, n, d_did)) return BH_EHIP;
     std::vector<int> did(n);
@@ -922,6 +922,11 @@ int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double
                      S.d_U, S.d_dU, dst->d_Ui, dst->d_dUi, src->nquad > 0, d_part, g.st);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+int add_scattered(bh_object *dst, bh_object *src, int *d_did, bool check, double *d_part)
+{
+  return add_scattered_at(dst, src, dst->d_slotPt, dst->d_slotN, dst->d_slotIdx, dst->nslot, d_did, check, d_part);
 }
 
 
@@ -1085,8 +1090,27 @@ extern "C" int bh_mo_plan_split(int *split)
 // receives the final iteration count.
 // the coupling sums of one loop: its plan, series coefficients, evaluation
 // tiles and work areas (set up once per loop; the geometry is fixed)
+// compact copies of listed slots: positions, normals, incident-data indices
+__global__ void k_gather_slots(const double *__restrict__ pts, const double *__restrict__ nrm,
+                               const int *__restrict__ sidx, const int *__restrict__ list, int n, double *__restrict__ opt,
+                               double *__restrict__ onm, int *__restrict__ oidx)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = list[i];
+#pragma unroll
+  for (int d = 0; d < 3; d++) { opt[3 * i + d] = pts[3 * p + d]; onm[3 * i + d] = nrm[3 * p + d]; }
+  oidx[i] = sidx[p];
+}
+
 struct Coupling {
   Plan P;
+  // the element-sum slots of every split pair (those of its destination's
+  // tiles that the series does not take), compacted: slots off .. off+n
+  struct Split { int dst, src; size_t off; int n; };
+  std::vector<Split> splits;
+  double *d_spt = nullptr, *d_snm = nullptr;
+  int *d_sidx = nullptr;
   std::vector<int> slot0;
   int maxslot = 0, totslot = 0;
   int *d_did = nullptr;
@@ -1168,6 +1192,34 @@ struct Coupling {
     }
     for (const MpTile &t : tiles)
       for (int i = 0; i < t.np; i++) evwork += (double)t.ns * mp_count(pairs[t.p0 + i].Le);
+    // split pairs: their element-sum slots as one compact list per pair
+    std::vector<int> lst;
+    for (int d = 0; d < N; d++)
+      for (int m = 0; m < N; m++) {
+        if (d == m || P.Lmn[d][m] != -1 || P.Ls[d][m] < 0) continue;
+        const std::vector<char> &fl = P.tile_mp[d][m];
+        Split q{d, m, lst.size(), 0};
+        for (int p = 0; p < objs[d]->nslot; p++)
+          if (!fl[p / 64]) lst.push_back(p);
+        q.n = (int)(lst.size() - q.off);
+        splits.push_back(q);
+      }
+    if (!lst.empty()) {
+      d_spt = ws<double>(W_SPT, 3 * lst.size());
+      d_snm = ws<double>(W_SNM, 3 * lst.size());
+      d_sidx = ws<int>(W_SIDX, lst.size());
+      int *d_lst = ws<int>(W_SLST, lst.size());
+      if (!d_spt || !d_snm || !d_sidx || !d_lst) return fail(BH_EHIP, "coupling loop: slot lists of the split pairs");
+      HIPCHK(hipMemcpyAsync(d_lst, lst.data(), lst.size() * sizeof(int), hipMemcpyHostToDevice, g.st));
+      for (const Split &q : splits) {
+        if (q.n <= 0) continue;
+        const bh_object *o = objs[q.dst];
+        hipLaunchKernelGGL(k_gather_slots, g1((size_t)q.n), dim3(256), 0, g.st, o->d_slotPt, o->d_slotN, o->d_slotIdx,
+                           d_lst + q.off, q.n, d_spt + 3 * q.off, d_snm + 3 * q.off, d_sidx + q.off);
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(g.st));   // the host list is a local
+    }
     return 0;
   }
 
@@ -1184,8 +1236,8 @@ struct Coupling {
       hipLaunchKernelGGL(k_mp_eval, dim3((unsigned)tiles.size()), dim3(64), 0, g.st, d_tiles, d_pairs);
       bem_prof_end(BH_K_MP_EVAL, g.st);
     }
-    // the other pairs: direct sums (of a split pair, over the runs of tiles
-    // the series does not take)
+    // the other pairs: direct sums (of a split pair, over its compact list of
+    // the slots the series does not take: one launch per pair)
     for (int m = 0; m < N; m++)
       for (int d = 0; d < N; d++) {
         if (d == m || P.Lmn[d][m] != -1) continue;
@@ -1195,17 +1247,12 @@ struct Coupling {
           if (r) return r;
           continue;
         }
-        const std::vector<char> &fl = P.tile_mp[d][m];
-        const int nt = (int)fl.size();
-        for (int t0 = 0; t0 < nt;) {
-          if (fl[t0]) { t0++; continue; }
-          int t1 = t0;
-          while (t1 < nt && !fl[t1]) t1++;
-          const int s0 = 64 * t0, n = std::min(64 * t1, dst->nslot) - s0;
-          int r = add_scattered(dst, objs[m], d_did, check, d_part, s0, n);
-          if (r) return r;
-          t0 = t1;
-        }
+        for (const Split &q : splits)
+          if (q.dst == d && q.src == m && q.n > 0) {
+            int r = add_scattered_at(dst, objs[m], d_spt + 3 * q.off, d_snm + 3 * q.off, d_sidx + q.off, q.n, d_did,
+                                     check, d_part);
+            if (r) return r;
+          }
       }
     HIPCHK(hipGetLastError());
     return 0;
