@@ -2064,6 +2064,20 @@ template <class T> static int cput(int i, T **d, const T *h, size_t n)
 // the residual updated by the change of the right-hand side (R = B - A X
 // becomes R - B_old + B_new: no matrix pass for the warm start).  Returns
 // BH_EUNSUP (nothing done) if an object does not qualify.
+// one product of the batched residuals (layout of SkJob, bem_la.hip):
+// C (m x 4) = A (m x k) B (k x 4) + beta C
+struct SkJobH {
+  const cd *A;
+  size_t lda;
+  const cd *B;
+  cd *C;
+  int m, k;
+  double beta;
+};
+// device job tables of the batched residuals (entry points are serialised;
+// released by bh_finalize, never by a destructor at process exit)
+static Scratch *g_skjob = nullptr;
+
 int bem::objects_solve_batch(bh_object **objs, int N)
 {
   if (N <= 0) return 0;
@@ -2123,17 +2137,46 @@ int bem::objects_solve_batch(bh_object **objs, int N)
     }
     int rc = schur_apply_batch(na_, f.data(), rhs.data(), dx.data(), w.data(), st);
     if (rc) return fail(rc, "batched block solve of %d objects", na_);
+    // R = B - A X of every object and domain: the column vectors of X, then
+    // the H and the G products of all of them as one launch each
+    std::vector<SkJobH> jh, jg;
+    std::vector<cd *> Ys;
     for (int q = 0; q < na_; q++) {
-      const int m = act[q];
-      bh_object *o = objs[m];
-      const size_t na = 8 * (size_t)o->NN;
-      bem_acc_c2z(o->d_X, dx[q], na, st);
+      bh_object *o = objs[act[q]];
+      bem_acc_c2z(o->d_X, dx[q], 8 * (size_t)o->NN, st);
+      size_t rows = 0;
+      for (int d = 0; d <= o->MN; d++) rows += (size_t)o->dom[d].nd;
+      cd *wr = o->ws_res.get<cd>(12 * rows);
+      if (!wr) return fail(BH_EHIP, "refinement residual: work area of %zu rows", rows);
+      size_t off = 0;
       for (int d = 0; d <= o->MN; d++) {
-        rc = dom_resid(o, d);
-        if (rc) return rc;
+        Dom &D = o->dom[d];
+        cd *a = wr + off, *c = a + 4 * (size_t)D.nd, *Y = a + 8 * (size_t)D.nd;
+        off += 12 * (size_t)D.nd;
+        bem_resid_colvec(D.d_col, D.nd, o->d_X, (size_t)o->NN, a, st);
+        jh.push_back({D.d_H, (size_t)D.nd, a, Y, D.nd, D.nd, 0.0});
+        jg.push_back({D.d_G, (size_t)D.nd, c, Y, D.nd, D.nd, 1.0});
+        Ys.push_back(Y);
       }
-      bem_nrm2sq_to(o->d_R, na, o->d_nrm, dn + 2 * m + 1, st);
-      o->ir_iters++;
+    }
+    {
+      const size_t tb = (bem_skinny_batch_bytes((int)jh.size()) + 255) / 256 * 256;
+      if (!g_skjob) g_skjob = new Scratch;
+      char *dtab = g_skjob->get<char>(2 * tb);
+      std::vector<char> h1(tb), h2(tb);
+      if (!dtab || bem_zgemm_skinny4_batch(jh.data(), (int)jh.size(), dtab, h1.data(), st) ||
+          bem_zgemm_skinny4_batch(jg.data(), (int)jg.size(), dtab + tb, h2.data(), st))
+        return fail(BH_EHIP, "batched refinement residuals of %d objects", na_);
+      size_t k = 0;
+      for (int q = 0; q < na_; q++) {
+        const int m = act[q];
+        bh_object *o = objs[m];
+        for (int d = 0; d <= o->MN; d++, k++) bem_resid_fin(o->d_B, Ys[k], o->dom[d].nd, row_base(o, d), o->d_R, st);
+        bem_nrm2sq_to(o->d_R, 8 * (size_t)o->NN, o->d_nrm, dn + 2 * m + 1, st);
+        o->ir_iters++;
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));   // the host job tables stay alive until here
     }
     HIPCHK(hipMemcpyAsync(hn.data(), dn, hn.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -2774,6 +2817,7 @@ extern "C" int bh_finalize(void)
   (void)hipStreamSynchronize(cur_st());
   bem_mo_release();
   if (g_cws) { delete[] g_cws; g_cws = nullptr; }
+  if (g_skjob) { delete g_skjob; g_skjob = nullptr; }
   for (int i = 0; i < 4; i++) {
     dfree(g_rule[i]); g_rule[i] = nullptr; g_nrule[i] = 0;
     dfree(g_qrule[i]); g_qrule[i] = nullptr; g_nqrule[i] = 0;

@@ -155,16 +155,15 @@ __global__ void __launch_bounds__(256) k_gemm64(int m, int n, int k, typename Cx
 // streams 4 rows of A along k (lanes over k, coalesced), the matching rows of
 // B (NC contiguous values per k) are reused across the 4 rows, and the
 // 4 x NC partial sums are reduced across the wave at the end.
+// rows row0 .. row0+3 of C = alpha A B + beta C, one wave
 template <class R, int NC>
-__global__ void __launch_bounds__(256) k_gemm_skinny(int m, int n, int k, typename Cx<R>::Z alpha,
-                                                     const typename Cx<R>::Z *__restrict__ A, size_t lda,
-                                                     const typename Cx<R>::Z *__restrict__ B, size_t ldb,
-                                                     typename Cx<R>::Z beta, typename Cx<R>::Z *__restrict__ C,
-                                                     size_t ldc)
+__device__ __forceinline__ void skinny_rows(int row0, int m, int n, int k, typename Cx<R>::Z alpha,
+                                            const typename Cx<R>::Z *__restrict__ A, size_t lda,
+                                            const typename Cx<R>::Z *__restrict__ B, size_t ldb,
+                                            typename Cx<R>::Z beta, typename Cx<R>::Z *__restrict__ C, size_t ldc)
 {
   typedef typename Cx<R>::Z Z;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row0 = (blockIdx.x * 4 + wave) * 4;
+  const int lane = threadIdx.x & 63;
   if (row0 >= m) return;
   Z acc[4][NC];
 #pragma unroll
@@ -198,6 +197,38 @@ __global__ void __launch_bounds__(256) k_gemm_skinny(int m, int n, int k, typena
         if (!bz) v = v + beta * (*c);
         *c = v;
       }
+}
+
+template <class R, int NC>
+__global__ void __launch_bounds__(256) k_gemm_skinny(int m, int n, int k, typename Cx<R>::Z alpha,
+                                                     const typename Cx<R>::Z *__restrict__ A, size_t lda,
+                                                     const typename Cx<R>::Z *__restrict__ B, size_t ldb,
+                                                     typename Cx<R>::Z beta, typename Cx<R>::Z *__restrict__ C,
+                                                     size_t ldc)
+{
+  skinny_rows<R, NC>((blockIdx.x * 4 + (threadIdx.x >> 6)) * 4, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+// many complex-FP64 products with 4 right-hand sides in one launch (the
+// refinement residuals of all objects and domains): C = A B + beta C with
+// B, C row-major k x 4 / m x 4; workgroup b belongs to the job whose range
+// of the prefix table `first` holds it (16 rows per workgroup), and each
+// job's rows are those of k_gemm_skinny on it alone
+struct SkJob {
+  const cd *A;
+  size_t lda;
+  const cd *B;
+  cd *C;
+  int m, k;
+  double beta;
+};
+__global__ void __launch_bounds__(256) k_gemm_skinny4_batch(const SkJob *__restrict__ jobs, const int *__restrict__ first,
+                                                            int nj)
+{
+  const int j = job_of(first, nj, blockIdx.x);
+  const SkJob J = jobs[j];
+  skinny_rows<double, 4>(((blockIdx.x - first[j]) * 4 + (threadIdx.x >> 6)) * 4, J.m, 4, J.k, mk(1.0, 0.0), J.A, J.lda,
+                         J.B, 4, mk(J.beta, 0.0), J.C, 4);
 }
 
 // ---- large-tile GEMM ----
@@ -2128,6 +2159,36 @@ void bem_la_release(void)
     g_lanes[i].lu_ws = nullptr;
     g_lanes[i].lu_bytes = 0;
   }
+}
+
+// nj products of k_gemm_skinny4_batch: jobs (host, nj entries of
+// {A, lda, B, C, m, k, beta}) copied with their prefix table into dev (a
+// device area of at least bem_skinny_batch_bytes(nj) bytes), one launch
+size_t bem_skinny_batch_bytes(int nj) { return ((size_t)nj * sizeof(SkJob) + 255) / 256 * 256 + (size_t)(nj + 1) * sizeof(int); }
+int bem_zgemm_skinny4_batch(const void *jobs, int nj, void *dev, void *host_tab, hipStream_t st)
+{
+  if (nj <= 0) return 0;
+  const SkJob *J = (const SkJob *)jobs;
+  const size_t jb = ((size_t)nj * sizeof(SkJob) + 255) / 256 * 256;
+  char *h = (char *)host_tab;   // bem_skinny_batch_bytes(nj) bytes, alive until the copy completes
+  memcpy(h, J, (size_t)nj * sizeof(SkJob));
+  int *f = (int *)(h + jb);
+  long tot = 0;
+  double flops = 0.0, bytes = 0.0;
+  for (int i = 0; i < nj; i++) {
+    f[i] = (int)tot;
+    tot += (J[i].m + 15) / 16;
+    flops += 8.0 * J[i].m * 4.0 * J[i].k;
+    bytes += ((double)J[i].m * J[i].k + 4.0 * J[i].k + 4.0 * J[i].m * (J[i].beta != 0.0 ? 2.0 : 1.0)) * sizeof(cd);
+  }
+  if (tot > 0x7fffffffL) return -1;
+  f[nj] = (int)tot;
+  if (hipMemcpyAsync(dev, h, bem_skinny_batch_bytes(nj), hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+  bem_prof_begin_b(Cx<double>::KGV, flops, bytes, st);
+  hipLaunchKernelGGL(k_gemm_skinny4_batch, dim3((unsigned)tot), dim3(256), 0, st, (const SkJob *)dev,
+                     (const int *)((char *)dev + jb), nj);
+  bem_prof_end(Cx<double>::KGV, st);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // complex FP64 (interleaved double pairs)

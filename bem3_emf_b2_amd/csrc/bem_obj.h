@@ -180,6 +180,7 @@ struct bh_object {
   // grow-only device work areas of the entry points (freed with the object)
   bem::Scratch ws_part, ws_mask, ws_pairs, ws_start, ws_ghdf, ws_self;   // assembly
   bem::Scratch ws_jobs;                // job table of a batched assembly this object heads
+  bem::Scratch ws_res;                 // batched refinement residuals: per domain a, c, Y (12 nd)
   bem::Scratch ws_wt;                  // forming S
   bem::Scratch ws_b;                   // right-hand side: the incident data as column operands
   // low-memory mixed form: FP64 row blocks of G, H, dG, dH, dF assembled on

@@ -68,6 +68,9 @@ void bem_resid_domain(const cd *G, const cd *H, int nd, size_t ld, const ColInfo
 void bem_resid_colvec(const ColInfo *col, int nd, const cd *X, size_t NN, cd *work, hipStream_t st);
 void bem_resid_rows(const cd *G, const cd *H, int nrows, int nd, size_t ld, int r0, size_t rowBase, const cd *B, cd *R,
                     cd *work, hipStream_t st);
+void bem_resid_fin(const cd *B, const cd *Y, int nd, size_t rowBase, cd *R, hipStream_t st);
+extern "C" size_t bem_skinny_batch_bytes(int nj);
+extern "C" int bem_zgemm_skinny4_batch(const void *jobs, int nj, void *dev, void *host_tab, hipStream_t st);
 
 // Euclidean norm of n complex values (synchronises the stream); tmp holds
 // 256 doubles of device memory

@@ -821,6 +821,13 @@ void bem_resid_rows(const cd *G, const cd *H, int nrows, int nd, size_t ld, int 
                      R + rowBase);
 }
 
+// R = B - Y on a domain's nd rows (rows rowBase.. of the 8NN system), Y the
+// products of bem_resid_rows formed elsewhere (the batched residuals)
+void bem_resid_fin(const cd *B, const cd *Y, int nd, size_t rowBase, cd *R, hipStream_t st)
+{
+  hipLaunchKernelGGL(k_resid_rows, grid1(4 * (size_t)nd), dim3(256), 0, st, B + rowBase, Y, nd, 0, nd, R + rowBase);
+}
+
 void bem_resid_domain(const cd *G, const cd *H, int nd, size_t ld, const ColInfo *col, const cd *X, size_t NN,
                       size_t rowBase, const cd *B, cd *R, cd *w, hipStream_t st)
 {
